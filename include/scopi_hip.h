@@ -1,0 +1,162 @@
+/*
+ * scopi_hip.h -- C ABI of the MI355X (gfx950) contact-step library libscopi_hip.so.
+ *
+ * The reference (hpc-maths/scopi v0.1.0) has no FFI: its per-step policies are
+ * C++ template slots of ScopiSolver<dim, problem, optim, contact, vap>
+ * (include/scopi/solver.hpp:91-107).  These entry points are what the
+ * project's C++ slot policies (scopi_amd/host/scopi_hip_policies.hpp) bind; each
+ * one names the reference member it replaces.  Plain pointers and sizes only:
+ * every entry returns an int status (SCOPI_HIP_OK or a negative error code)
+ * and never throws; device memory belongs to the context, host buffers to the
+ * caller.
+ *
+ * Array conventions:
+ *   - per-particle vectors: n x 3 row-major doubles, 3 slots even in 2D;
+ *     2D rotation velocity and moment of inertia sit in slot 2 (the z axis),
+ *     as in the reference's 6-dof velocity layout
+ *     (include/scopi/solvers/minimization_problem.hpp:15-29);
+ *   - quaternions: n x 4, scalar first (include/scopi/quaternion.hpp:86-124);
+ *   - size: n x 3 (sphere radius / segment length in column 0,
+ *     superellipsoid semi-axes in columns 0..2); squareness: n x 2 (e, n);
+ *   - indices are global container indices, obstacles first
+ *     (include/scopi/container.hpp:338-384).
+ */
+#ifndef SCOPI_HIP_H
+#define SCOPI_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ------------------------------------------------------------------ status */
+#define SCOPI_HIP_OK 0
+#define SCOPI_HIP_ERR_ARG (-1)         /* bad argument / NULL pointer            */
+#define SCOPI_HIP_ERR_HIP (-2)         /* a HIP runtime call failed              */
+#define SCOPI_HIP_ERR_CAPACITY (-3)    /* size beyond the context's capacity     */
+#define SCOPI_HIP_ERR_STATE (-4)       /* call out of order (e.g. no contacts)   */
+#define SCOPI_HIP_ERR_UNSUPPORTED (-5) /* shape pair / option not implemented    */
+#define SCOPI_HIP_NOT_CONVERGED 1      /* solve returned the last iterate        */
+
+/* ------------------------------------------------------------------ enums */
+enum { SCOPI_SHAPE_SPHERE = 0, SCOPI_SHAPE_PLANE = 1, SCOPI_SHAPE_SEGMENT = 2, SCOPI_SHAPE_SUPERELLIPSOID = 3 };
+/* problem tags NoFriction / Friction (include/scopi/contact/property.hpp:14-20) */
+enum { SCOPI_PROBLEM_NOFRICTION = 0, SCOPI_PROBLEM_FRICTION = 1 };
+/* contact slot: contact_kdtree (radius prefilter) / contact_brute_force */
+enum { SCOPI_CONTACT_KDTREE = 0, SCOPI_CONTACT_BRUTE_FORCE = 1 };
+/* vap slot: vap_fixed / vap_fpd */
+enum { SCOPI_VAP_FIXED = 0, SCOPI_VAP_FPD = 1 };
+/* optim method: apgd / pgd (include/scopi/solvers/apgd.hpp:39-227) */
+enum { SCOPI_METHOD_APGD = 0, SCOPI_METHOD_PGD = 1 };
+/* arithmetic of the projected-gradient loop */
+enum { SCOPI_PRECISION_FP64 = 0, SCOPI_PRECISION_FP32_REFINE = 1 };
+
+/* Every tunable of the slots, with the reference's defaults filled in by
+ * scopi_hip_default_params: ContactsParams (contact_kdtree.hpp:27-61),
+ * apgd_params (apgd.hpp:105-124), contact_property<Friction>::mu
+ * (contact/property.hpp:72). */
+typedef struct scopi_hip_params {
+    int32_t problem;
+    int32_t contact_method;
+    int32_t vap;
+    int32_t method;
+    int32_t precision;
+    int32_t dynamic_descent;
+    double dmax;
+    double kd_tree_radius; /* squared length, strict < (nanoflann L2 adaptor) */
+    double alpha;
+    double tolerance;
+    double mu;
+    int64_t max_ite;
+    double fp32_tolerance; /* FP32 phase stop, precision == FP32_REFINE (0: automatic) */
+    int32_t warm_start;    /* 1: start the solve from the previous step's lambda of the same (i,j) */
+    int32_t reserved;
+} scopi_hip_params;
+
+typedef struct scopi_hip_solve_info {
+    int64_t n_contacts;
+    int64_t rows;          /* nc (NoFriction) or 3 nc (Friction) */
+    int64_t iterations;    /* accepted projected-gradient iterations (all phases) */
+    int64_t iterations_fp32;
+    int64_t backtracks;    /* rejected line-search trials */
+    int64_t q_applies;     /* products with Q = dt^2 B M^-1 B^T */
+    int32_t converged;
+    int32_t status;
+    double residual;       /* last ||lambda_{k+1} - lambda_k||_2 */
+} scopi_hip_solve_info;
+
+typedef struct scopi_hip_ctx scopi_hip_ctx;
+
+int scopi_hip_default_params(scopi_hip_params* params);
+
+/* Context = one scene on one GPU (the current HIP device), with its own stream.
+ * Capacities grow on demand; the initial values only avoid reallocation.
+ * Replaces the storage of ScopiSolver/scopi_container (solver.hpp:194-206). */
+int scopi_hip_create(int32_t dim, int64_t capacity_particles, int64_t capacity_contacts, scopi_hip_ctx** ctx);
+int scopi_hip_destroy(scopi_hip_ctx* ctx);
+/* Text of the last error on this context (never NULL). */
+const char* scopi_hip_last_error(const scopi_hip_ctx* ctx);
+
+/* Host -> device particle state, obstacles in [0, n_inactive).  Replaces the
+ * container fields read by every slot (container.hpp:248-323, 475-597).
+ * squareness may be NULL when no superellipsoid is present. */
+int scopi_hip_upload_state(scopi_hip_ctx* ctx, int64_t n, int64_t n_inactive, const int32_t* shape, const double* pos,
+                           const double* q, const double* size, const double* squareness, const double* v,
+                           const double* vd, const double* omega, const double* domega, const double* f,
+                           const double* m, const double* J);
+/* Device -> host dynamic state (any pointer may be NULL). */
+int scopi_hip_download_state(scopi_hip_ctx* ctx, double* pos, double* q, double* v, double* omega);
+/* Host -> device velocities only (the vap slot run on the host, or a user edit). */
+int scopi_hip_upload_velocities(scopi_hip_ctx* ctx, const double* v, const double* omega);
+
+/* Contact slot: contact_kdtree<problem>::run_impl (contact/contact_kdtree.hpp:200-268)
+ * or contact_brute_force<problem>::run_impl (contact_brute_force.hpp:102-137) on the
+ * current device state.  Result stays on the device, sorted by (i, j). */
+int scopi_hip_compute_contacts(scopi_hip_ctx* ctx, const scopi_hip_params* params, int64_t* n_contacts);
+/* Copy the device contact list to host (std::vector<neighbor<dim, problem>>,
+ * objects/neighbor.hpp:23-70).  nij/pi/pj are nc x 3.  Any pointer may be NULL. */
+int scopi_hip_download_contacts(scopi_hip_ctx* ctx, int64_t* i, int64_t* j, double* dij, double* nij, double* pi,
+                                double* pj);
+/* Install a host contact list (any contact policy feeding the GPU optim slot).
+ * mu may be NULL (every contact then takes params->mu at solve time). */
+int scopi_hip_upload_contacts(scopi_hip_ctx* ctx, int64_t nc, const int64_t* i, const int64_t* j, const double* dij,
+                              const double* nij, const double* pi, const double* pj, const double* mu);
+
+/* Vap slot: vap_fixed (vap/vap_fixed.hpp:46-55) or vap_fpd (vap/vap_fpd.hpp:70-82,
+ * src/vap/vap_fpd.cpp:6-26) applied to the device velocities. */
+int scopi_hip_set_a_priori_velocity(scopi_hip_ctx* ctx, const scopi_hip_params* params, double dt);
+
+/* Optim slot: OptimGradient<apgd|pgd>::run (solvers/OptimGradient.hpp:252-306):
+ * assemble B from the device contacts, solve the dual problem, compute
+ * u = v + dt M^-1 B^T lambda (kept on the device). */
+int scopi_hip_solve(scopi_hip_ctx* ctx, const scopi_hip_params* params, double dt, scopi_hip_solve_info* info);
+/* get_uadapt / get_wadapt / lagrange_multiplier (OptimGradient.hpp:317-335):
+ * u, omega are Na x 3; lambda has nc (NoFriction) or 3 nc (Friction) entries. */
+int scopi_hip_download_solution(scopi_hip_ctx* ctx, double* u, double* omega, double* lambda);
+
+/* ScopiSolver::update_velocity + move_active_particles (solver.hpp:440-546). */
+int scopi_hip_update_and_move(scopi_hip_ctx* ctx, double dt);
+/* ScopiSolver::displacement_obstacles (solver.hpp:329-355). */
+int scopi_hip_displacement_obstacles(scopi_hip_ctx* ctx, double dt);
+
+/* ScopiSolver::run(dt, total_it, initial_iter) (solver.hpp:266-303) with every
+ * slot on the device and no output files: obstacles, contacts, vap, solve,
+ * update, move per step.  info (may be NULL) accumulates over the steps. */
+int scopi_hip_run(scopi_hip_ctx* ctx, const scopi_hip_params* params, double dt, int64_t total_it,
+                  int64_t initial_iter, scopi_hip_solve_info* info);
+
+/* Device time of the dominant projected-gradient kernels over the last
+ * scopi_hip_run / scopi_hip_solve, measured with HIP events on the context's
+ * stream: out[0] = summed milliseconds of the B^T-gather kernel, out[1] =
+ * launches, out[2] = ms of the row kernel (B w), out[3] = launches, out[4] =
+ * ms of the trial kernel, out[5] = launches.  Timing is enabled by
+ * scopi_hip_set_kernel_timing(ctx, 1); it serialises the solve loop. */
+int scopi_hip_set_kernel_timing(scopi_hip_ctx* ctx, int32_t enable);
+int scopi_hip_kernel_timing(scopi_hip_ctx* ctx, double* out);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* SCOPI_HIP_H */

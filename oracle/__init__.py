@@ -1,0 +1,181 @@
+"""ctypes wrapper of the FP64 CPU restatement (liboracle.so).
+
+TEST INFRASTRUCTURE ONLY: tests/, __graft_entry__.smoke() and bench.py's
+cpu_baseline leg are the only callers.  The product path never imports this.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB_PATH = os.path.join(_HERE, "liboracle.so")
+_lib = None
+
+NOFRICTION, FRICTION = 0, 1
+KDTREE, BRUTE_FORCE = 0, 1
+VAP_FIXED, VAP_FPD = 0, 1
+APGD, PGD = 0, 1
+
+
+class Params(C.Structure):
+    _fields_ = [
+        ("problem", C.c_int32), ("contact_method", C.c_int32), ("vap", C.c_int32), ("method", C.c_int32),
+        ("dmax", C.c_double), ("kd_tree_radius", C.c_double), ("alpha", C.c_double), ("max_ite", C.c_int64),
+        ("tolerance", C.c_double), ("dynamic_descent", C.c_int32), ("pad", C.c_int32), ("mu", C.c_double),
+    ]
+
+
+class Stats(C.Structure):
+    _fields_ = [
+        ("n_contacts", C.c_int64), ("iterations", C.c_int64), ("backtracks", C.c_int64), ("q_applies", C.c_int64),
+        ("converged", C.c_int64), ("total_iterations", C.c_int64), ("total_q_applies", C.c_int64),
+        ("t_contacts", C.c_double), ("t_vap", C.c_double), ("t_solve", C.c_double), ("t_move", C.c_double),
+    ]
+
+
+def build():
+    subprocess.run(["make", "-s", "-C", _HERE], check=True)
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(_LIB_PATH):
+            build()
+        L = C.CDLL(_LIB_PATH)
+        P = C.c_void_p
+        D = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
+        I32 = np.ctypeslib.ndpointer(dtype=np.int32, flags="C_CONTIGUOUS")
+        I64 = np.ctypeslib.ndpointer(dtype=np.int64, flags="C_CONTIGUOUS")
+        L.oracle_scene_new.restype = P
+        L.oracle_scene_free.argtypes = [P]
+        L.oracle_default_params.argtypes = [C.POINTER(Params)]
+        L.oracle_set_params.argtypes = [P, C.POINTER(Params)]
+        L.oracle_scene_set.argtypes = [P, C.c_int32, C.c_int64, C.c_int64, I32] + [D] * 10
+        L.oracle_get_state.argtypes = [P, D, D, D, D]
+        L.oracle_run.argtypes = [P, C.c_double, C.c_int64, C.c_int64]
+        L.oracle_compute_contacts.argtypes = [P]
+        L.oracle_compute_contacts.restype = C.c_int64
+        L.oracle_n_contacts.argtypes = [P]
+        L.oracle_n_contacts.restype = C.c_int64
+        L.oracle_get_contacts.argtypes = [P, I64, D, D, D, D]
+        L.oracle_amat.argtypes = [P, D, D]
+        L.oracle_atmat.argtypes = [P, D, D]
+        L.oracle_solve.argtypes = [P, C.c_double, D, D, D]
+        L.oracle_set_a_priori_velocity.argtypes = [P, C.c_double]
+        L.oracle_get_stats.argtypes = [P, C.POINTER(Stats)]
+        L.oracle_closest_points.argtypes = [C.c_int32, C.c_int32, D, D, D, C.c_int32, D, D, D, D, D, D, D]
+        L.oracle_quaternion.argtypes = [C.c_double, D, D]
+        L.oracle_gen_2d_case_spheres.argtypes = [C.c_int32, D, D, D, D]
+        _lib = L
+    return _lib
+
+
+def default_params(**kw) -> Params:
+    p = Params()
+    lib().oracle_default_params(C.byref(p))
+    for k, v in kw.items():
+        setattr(p, k, v)
+    return p
+
+
+class Oracle:
+    """One scene of the CPU restatement (ScopiSolver state + slot parameters)."""
+
+    def __init__(self, scene, **params):
+        self._L = lib()
+        self._h = self._L.oracle_scene_new()
+        self.dim = scene.dim
+        self.n = scene.n
+        self.n_inactive = scene.n_inactive
+        self.problem = params.get("problem", NOFRICTION)
+        self.set_params(**params)
+        rc = self._L.oracle_scene_set(
+            self._h, scene.dim, scene.n, scene.n_inactive, np.ascontiguousarray(scene.shape, np.int32),
+            *(np.ascontiguousarray(getattr(scene, k), np.float64) for k in (
+                "pos", "q", "size", "v", "vd", "omega", "domega", "f", "m", "J")))
+        if rc != 0:
+            raise ValueError("oracle_scene_set failed")
+
+    def __del__(self):
+        if getattr(self, "_h", None):
+            self._L.oracle_scene_free(self._h)
+            self._h = None
+
+    def set_params(self, **kw):
+        self.params = default_params(**kw)
+        self.problem = self.params.problem
+        self._L.oracle_set_params(self._h, C.byref(self.params))
+
+    def run(self, dt, total_it, initial_iter=0):
+        self._L.oracle_run(self._h, float(dt), int(total_it), int(initial_iter))
+
+    def state(self):
+        pos = np.zeros((self.n, 3)); q = np.zeros((self.n, 4)); v = np.zeros((self.n, 3)); om = np.zeros((self.n, 3))
+        self._L.oracle_get_state(self._h, pos, q, v, om)
+        return dict(pos=pos, q=q, v=v, omega=om)
+
+    def compute_contacts(self):
+        return int(self._L.oracle_compute_contacts(self._h))
+
+    def contacts(self):
+        nc = int(self._L.oracle_n_contacts(self._h))
+        ij = np.zeros((nc, 2), np.int64); dij = np.zeros(nc); nij = np.zeros((nc, 3))
+        pi = np.zeros((nc, 3)); pj = np.zeros((nc, 3))
+        if nc:
+            self._L.oracle_get_contacts(self._h, ij, dij, nij, pi, pj)
+        return dict(ij=ij, dij=dij, nij=nij, pi=pi, pj=pj)
+
+    def amat(self, u):
+        nc = int(self._L.oracle_n_contacts(self._h))
+        out = np.zeros(3 * nc)
+        self._L.oracle_amat(self._h, np.ascontiguousarray(u, np.float64), out)
+        return out
+
+    def atmat(self, f):
+        out = np.zeros(6 * (self.n - self.n_inactive))
+        self._L.oracle_atmat(self._h, np.ascontiguousarray(f, np.float64), out)
+        return out
+
+    def set_a_priori_velocity(self, dt):
+        self._L.oracle_set_a_priori_velocity(self._h, float(dt))
+
+    def solve(self, dt):
+        nc = int(self._L.oracle_n_contacts(self._h))
+        na = self.n - self.n_inactive
+        lam = np.zeros(max(1, nc * (3 if self.problem == FRICTION else 1)))
+        u = np.zeros(max(1, 3 * na)); w = np.zeros(max(1, 3 * na))
+        self._L.oracle_solve(self._h, float(dt), lam, u, w)
+        return dict(lam=lam[: nc * (3 if self.problem == FRICTION else 1)], u=u[: 3 * na].reshape(na, 3),
+                    w=w[: 3 * na].reshape(na, 3))
+
+    def stats(self):
+        s = Stats()
+        self._L.oracle_get_stats(self._h, C.byref(s))
+        return {k: getattr(s, k) for k, _ in Stats._fields_}
+
+
+def closest_points(dim, shape_i, pos_i, q_i, size_i, shape_j, pos_j, q_j, size_j):
+    L = lib()
+
+    def v3(x):
+        o = np.zeros(3); x = np.atleast_1d(np.asarray(x, np.float64)); o[: x.size] = x; return o
+    dij = np.zeros(1); nij = np.zeros(3); pi = np.zeros(3); pj = np.zeros(3)
+    rc = L.oracle_closest_points(dim, shape_i, v3(pos_i), np.ascontiguousarray(q_i, np.float64), v3(size_i),
+                                 shape_j, v3(pos_j), np.ascontiguousarray(q_j, np.float64), v3(size_j),
+                                 dij, nij, pi, pj)
+    if rc != 0:
+        raise ValueError("no closest-point form for this pair")
+    return dict(dij=float(dij[0]), nij=nij[:dim], pi=pi[:dim], pj=pj[:dim])
+
+
+def gen_2d_case_spheres(n=3):
+    L = lib()
+    k = 2 * n * n
+    x = np.zeros(k); y = np.zeros(k); r = np.zeros(k); vel = np.zeros(k)
+    L.oracle_gen_2d_case_spheres(n, x, y, r, vel)
+    return x, y, r, vel

@@ -1,0 +1,996 @@
+// scopi_oracle.cpp -- FP64 CPU restatement of SCoPI's per-time-step contact
+// pipeline (reference: hpc-maths/scopi v0.1.0).
+//
+// TEST INFRASTRUCTURE ONLY.  This is the checker the HIP path is compared
+// against; only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline
+// leg may load it.  It follows the reference's arithmetic order where that
+// order can move a result by more than an ulp, and cites the reference file
+// and line for every function.  Built with -ffp-contract=off so that no FMA
+// contraction changes a predicate.
+//
+// What pins each function (see tests/test_oracle_*.py):
+//   closest points (sphere/plane/segment)  -> test/test_closest_points.cpp known answers,
+//                                             contact entries of test/references/*.json
+//   contact set (kd-tree / brute force)    -> test/test_contacts_kdtree.cpp known answers,
+//                                             brute-force enumeration with the same predicates
+//   A / A^T                                -> test/test_matrices.cpp (adjointness, known values),
+//                                             finite differences of the closed-form distance
+//   APGD, NoFriction and Friction          -> KKT conditions, exhaustive active-set optimum,
+//                                             analytic sphere/plane velocity (test_gradient.cpp:23-84)
+//   full step                              -> test/references/{two_spheres_*, 2d_case_spheres,
+//                                             obstacles_sphere_sphere_moving, 3spheres_nofriction,
+//                                             sphere_plane_friction_mu01}.json, inclined-plane
+//                                             expected errors (test_obstacles.cpp:256-313)
+#include "scopi_oracle.h"
+
+#include <algorithm>
+#include <array>
+#include <chrono>
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <random>
+#include <vector>
+
+namespace {
+
+using vec3 = std::array<double, 3>;
+using quat = std::array<double, 4>;
+using mat3 = std::array<std::array<double, 3>, 3>;
+
+struct Contact {
+    int64_t i, j;
+    double dij;
+    vec3 nij, pi, pj;
+    double mu;
+};
+
+struct Scene {
+    int dim = 2;
+    int64_t n = 0, n_inactive = 0;
+    std::vector<int32_t> shape;
+    std::vector<vec3> pos, size, v, vd, omega, domega, f, J;
+    std::vector<quat> q;
+    std::vector<double> m;
+    oracle_params prm{};
+    std::vector<Contact> contacts;
+    std::vector<double> lambda;
+    // optim slot outputs (OptimGradient::m_u / m_omega), Na x 3
+    std::vector<double> u, w;
+    oracle_stats st{};
+    int64_t na() const { return n - n_inactive; }
+};
+
+double seconds_since(std::chrono::steady_clock::time_point t0) {
+    return std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+}
+
+// xt::sign (include/scopi/objects/methods/closest_points.hpp:777), also src/utils.cpp:47-50
+inline double sign(double x) { return (x > 0) ? 1.0 : ((x < 0) ? -1.0 : 0.0); }
+
+// BLAS-style dot / nrm2 over the first n entries (closest_points.hpp:42-48 use xt::linalg).
+inline double dot(const double* a, const double* b, int n) {
+    double s = 0.0;
+    for (int d = 0; d < n; ++d) s += a[d] * b[d];
+    return s;
+}
+inline double nrm2(const double* a, int n) { return std::sqrt(dot(a, a, n)); }
+
+// rotation_matrix_impl, 3D: include/scopi/quaternion.hpp:86-124 (q scalar-first)
+mat3 rotation3(const quat& q) {
+    const double x = q[0], y = q[1], z = q[2], w = q[3];
+    mat3 r;
+    r[0] = {1 - 2 * z * z - 2 * w * w, 2 * y * z - 2 * x * w, 2 * y * w + 2 * x * z};
+    r[1] = {2 * y * z + 2 * x * w, 1 - 2 * y * y - 2 * w * w, 2 * z * w - 2 * x * y};
+    r[2] = {2 * y * w - 2 * x * z, 2 * z * w + 2 * x * y, 1 - 2 * y * y - 2 * z * z};
+    return r;
+}
+
+// Column 0 of rotation_matrix<dim>(q): plane::normal (plane.hpp:142-147),
+// segment::normal (segment.hpp:154-158).  2D form: quaternion.hpp:65-84.
+vec3 first_column(int dim, const quat& q) {
+    if (dim == 2) {
+        const double x = q[0], w = q[3];
+        return {1 - 2 * w * w, 2 * x * w, 0.0};
+    }
+    mat3 r = rotation3(q);
+    return {r[0][0], r[1][0], r[2][0]};
+}
+
+// mult_quaternion: src/quaternion.cpp:92-120
+quat mult_quaternion(const quat& q1, const quat& q2) {
+    quat r;
+    r[0] = q1[0] * q2[0];
+    for (int d = 0; d < 3; ++d) {
+        r[0] -= q1[d + 1] * q2[d + 1];
+        r[d + 1] = q1[0] * q2[d + 1] + q2[0] * q1[d + 1];
+    }
+    r[1] += (q1[2] * q2[3] - q1[3] * q2[2]);
+    r[2] += (q1[3] * q2[1] - q1[1] * q2[3]);
+    r[3] += (q1[1] * q2[2] - q1[2] * q2[1]);
+    return r;
+}
+
+// normalize: src/quaternion.cpp:6-10
+void normalize(quat& q) {
+    double n = nrm2(q.data(), 4);
+    for (auto& c : q) c /= n;
+}
+
+// ------------------------------------------------------------------ closest points
+// closest_points(sphere, sphere): closest_points.hpp:36-50
+Contact cp_sphere_sphere(int dim, const vec3& ci, double ri, const vec3& cj, double rj) {
+    vec3 d{}, u{};
+    for (int k = 0; k < dim; ++k) d[k] = cj[k] - ci[k];
+    double nd = nrm2(d.data(), dim);
+    for (int k = 0; k < dim; ++k) u[k] = d[k] / nd;
+    Contact c{};
+    for (int k = 0; k < dim; ++k) {
+        c.pi[k] = ci[k] + ri * u[k];
+        c.pj[k] = cj[k] - rj * u[k];
+    }
+    vec3 e{};
+    for (int k = 0; k < dim; ++k) e[k] = c.pj[k] - cj[k];
+    double ne = nrm2(e.data(), dim);
+    for (int k = 0; k < dim; ++k) c.nij[k] = e[k] / ne;
+    vec3 pd{};
+    for (int k = 0; k < dim; ++k) pd[k] = c.pi[k] - c.pj[k];
+    c.dij = dot(pd.data(), c.nij.data(), dim);
+    return c;
+}
+
+// closest_points(sphere, plane): closest_points.hpp:768-787
+Contact cp_sphere_plane(int dim, const vec3& cs, double r, const vec3& pp, const quat& qp) {
+    vec3 nrm = first_column(dim, qp);
+    vec3 d{};
+    for (int k = 0; k < dim; ++k) d[k] = cs[k] - pp[k];
+    double p2s = dot(d.data(), nrm.data(), dim);
+    double sg = sign(p2s);
+    Contact c{};
+    for (int k = 0; k < dim; ++k) {
+        c.pi[k] = cs[k] - sg * r * nrm[k];
+        c.pj[k] = cs[k] - p2s * nrm[k];
+        c.nij[k] = sg * nrm[k];
+    }
+    vec3 pd{};
+    for (int k = 0; k < dim; ++k) pd[k] = c.pi[k] - c.pj[k];
+    c.dij = dot(pd.data(), c.nij.data(), dim);
+    return c;
+}
+
+// closest_points(plane, sphere): closest_points.hpp:802-809
+Contact swap_sides(int dim, Contact c) {
+    for (int k = 0; k < dim; ++k) c.nij[k] *= -1.0;
+    std::swap(c.pi, c.pj);
+    return c;
+}
+
+// closest_points(sphere, segment): closest_points.hpp:824-859 (2D only; segment.hpp:154-177)
+Contact cp_sphere_segment(int dim, const vec3& cs, double r, const vec3& sp, const quat& sq, double length) {
+    vec3 nrm = first_column(dim, sq);
+    vec3 tangent{-nrm[1], nrm[0], 0.0};
+    // extrema: R * (0, -+L/2) + pos, with the 2D rotation of quaternion.hpp:65-84
+    const double x = sq[0], w = sq[3];
+    const double r00 = 1 - 2 * w * w, r01 = -2 * x * w, r10 = 2 * x * w, r11 = 1 - 2 * w * w;
+    vec3 e0{r00 * 0 + r01 * (-0.5 * length) + sp[0], r10 * 0 + r11 * (-0.5 * length) + sp[1], 0.0};
+    vec3 e1{r00 * 0 + r01 * (0.5 * length) + sp[0], r10 * 0 + r11 * (0.5 * length) + sp[1], 0.0};
+    Contact c{};
+    vec3 d0{cs[0] - e0[0], cs[1] - e0[1], 0.0};
+    vec3 d1{cs[0] - e1[0], cs[1] - e1[1], 0.0};
+    auto endpoint = [&](const vec3& dd, const vec3& e) {
+        double nd = nrm2(dd.data(), 2);
+        for (int k = 0; k < 2; ++k) c.nij[k] = dd[k] / nd;
+        for (int k = 0; k < 2; ++k) c.pi[k] = cs[k] - c.nij[k] * r;
+        c.pj = e;
+        vec3 pd{c.pi[0] - c.pj[0], c.pi[1] - c.pj[1], 0.0};
+        c.dij = dot(pd.data(), c.nij.data(), 2);
+    };
+    if (dot(d0.data(), tangent.data(), 2) < 0) {
+        endpoint(d0, e0);
+    } else if (dot(d1.data(), tangent.data(), 2) > 0) {
+        endpoint(d1, e1);
+    } else {
+        vec3 d{cs[0] - sp[0], cs[1] - sp[1], 0.0};
+        double s2s = dot(d.data(), nrm.data(), 2);
+        double sg = sign(s2s);
+        for (int k = 0; k < 2; ++k) {
+            c.pi[k] = cs[k] - sg * r * nrm[k];
+            c.pj[k] = cs[k] - s2s * nrm[k];
+            c.nij[k] = sg * nrm[k];
+        }
+        vec3 pd{c.pi[0] - c.pj[0], c.pi[1] - c.pj[1], 0.0};
+        c.dij = dot(pd.data(), c.nij.data(), 2);
+    }
+    (void)dim;
+    return c;
+}
+
+// closest_points_dispatcher (closest_points.hpp:2037-2043, antisymmetric dispatch):
+// returns false for pairs the reference has no closed form for.
+bool closest_points(int dim, int si, const vec3& pi, const quat& qi, const vec3& szi, int sj, const vec3& pj,
+                    const quat& qj, const vec3& szj, Contact& out) {
+    if (si == ORACLE_SPHERE && sj == ORACLE_SPHERE) {
+        out = cp_sphere_sphere(dim, pi, szi[0], pj, szj[0]);
+        return true;
+    }
+    if (si == ORACLE_SPHERE && sj == ORACLE_PLANE) {
+        out = cp_sphere_plane(dim, pi, szi[0], pj, qj);
+        return true;
+    }
+    if (si == ORACLE_PLANE && sj == ORACLE_SPHERE) {
+        out = swap_sides(dim, cp_sphere_plane(dim, pj, szj[0], pi, qi));
+        return true;
+    }
+    if (dim == 2 && si == ORACLE_SPHERE && sj == ORACLE_SEGMENT) {
+        out = cp_sphere_segment(dim, pi, szi[0], pj, qj, szj[0]);
+        return true;
+    }
+    if (dim == 2 && si == ORACLE_SEGMENT && sj == ORACLE_SPHERE) {
+        out = swap_sides(dim, cp_sphere_segment(dim, pj, szj[0], pi, qi, szi[0]));
+        return true;
+    }
+    // plane/plane, segment/segment, plane/segment: the reference returns a
+    // default-constructed neighbor (closest_points.hpp:898-917); such pairs are
+    // never produced by the configurations exercised here.
+    return false;
+}
+
+// compute_exact_distance: contact/base.hpp:103-141 (non-periodic box)
+void exact_distance(const Scene& s, int64_t i, int64_t j, std::vector<Contact>& out) {
+    Contact c{};
+    if (!closest_points(s.dim, s.shape[i], s.pos[i], s.q[i], s.size[i], s.shape[j], s.pos[j], s.q[j], s.size[j], c))
+        return;
+    if (c.dij < s.prm.dmax) {
+        c.i = i;
+        c.j = j;
+        c.mu = s.prm.mu;
+        out.push_back(c);
+    }
+}
+
+// sort_contacts: contact/base.hpp:156-175
+void sort_contacts(std::vector<Contact>& c) {
+    std::sort(c.begin(), c.end(), [](const Contact& a, const Contact& b) {
+        if (a.i < b.i) return true;
+        if (a.i == b.i) return a.j < b.j;
+        return false;
+    });
+}
+
+// nanoflann L2_Simple_Adaptor::evalMetric: sum of diff*diff in coordinate order.
+inline double kd_dist2(const Scene& s, int64_t a, int64_t b) {
+    double r = 0.0;
+    for (int d = 0; d < s.dim; ++d) {
+        double diff = s.pos[a][d] - s.pos[b][d];
+        r += diff * diff;
+    }
+    return r;
+}
+
+// contact_kdtree::run_impl: contact/contact_kdtree.hpp:200-268.  The set of
+// (i, j) candidates is every active pair with i < j and squared centre
+// distance < kd_tree_radius (nanoflann radiusSearch, strict <).  The oracle
+// enumerates that exact set through a uniform grid whose cell is at least the
+// search radius, then applies the same predicate to every candidate.
+std::vector<Contact> contacts_kdtree(const Scene& s) {
+    std::vector<Contact> out;
+    const int64_t a0 = s.n_inactive, na = s.na();
+    if (na > 1) {
+        const double h = std::sqrt(s.prm.kd_tree_radius) * (1.0 + 1e-12) + 1e-300;
+        vec3 lo{1e300, 1e300, 1e300}, hi{-1e300, -1e300, -1e300};
+        for (int64_t p = a0; p < s.n; ++p)
+            for (int d = 0; d < s.dim; ++d) {
+                lo[d] = std::min(lo[d], s.pos[p][d]);
+                hi[d] = std::max(hi[d], s.pos[p][d]);
+            }
+        std::array<int64_t, 3> nc{1, 1, 1};
+        for (int d = 0; d < s.dim; ++d) {
+            double cells = std::floor((hi[d] - lo[d]) / h) + 1.0;
+            nc[d] = (int64_t)std::min(cells, 2048.0);
+        }
+        std::array<double, 3> cellw{h, h, h};
+        for (int d = 0; d < s.dim; ++d)
+            cellw[d] = std::max(h, (hi[d] - lo[d]) / (double)nc[d] * (1.0 + 1e-12));
+        auto cell_of = [&](int64_t p, int d) {
+            int64_t c = (int64_t)std::floor((s.pos[p][d] - lo[d]) / cellw[d]);
+            return std::min(std::max<int64_t>(c, 0), nc[d] - 1);
+        };
+        std::vector<std::pair<int64_t, int64_t>> key(na);
+        for (int64_t p = 0; p < na; ++p) {
+            int64_t k = 0;
+            for (int d = s.dim - 1; d >= 0; --d) k = k * nc[d] + cell_of(a0 + p, d);
+            key[p] = {k, a0 + p};
+        }
+        std::sort(key.begin(), key.end());
+        std::vector<int64_t> keys(na);
+        for (int64_t p = 0; p < na; ++p) keys[p] = key[p].first;
+        std::vector<int64_t> cand;
+        for (int64_t i = a0; i < s.n - 1; ++i) {
+            std::array<int64_t, 3> c{0, 0, 0};
+            for (int d = 0; d < s.dim; ++d) c[d] = cell_of(i, d);
+            cand.clear();
+            const int zr = (s.dim == 3) ? 1 : 0;
+            for (int dz = -zr; dz <= zr; ++dz)
+                for (int dy = -1; dy <= 1; ++dy)
+                    for (int dx = -1; dx <= 1; ++dx) {
+                        int64_t cx = c[0] + dx, cy = c[1] + dy, cz = c[2] + dz;
+                        if (cx < 0 || cy < 0 || cz < 0 || cx >= nc[0] || cy >= nc[1] || cz >= nc[2]) continue;
+                        int64_t k = (cz * nc[1] + cy) * nc[0] + cx;
+                        if (s.dim == 2) k = cy * nc[0] + cx;
+                        auto lo_it = std::lower_bound(keys.begin(), keys.end(), k);
+                        for (auto it = lo_it; it != keys.end() && *it == k; ++it) {
+                            int64_t j = key[it - keys.begin()].second;
+                            if (i < j && kd_dist2(s, i, j) < s.prm.kd_tree_radius) cand.push_back(j);
+                        }
+                    }
+            for (int64_t j : cand) exact_distance(s, i, j, out);
+        }
+    }
+    // obstacles: contact_kdtree.hpp:249-256
+    for (int64_t i = 0; i < s.n_inactive; ++i)
+        for (int64_t j = s.n_inactive; j < s.n; ++j) exact_distance(s, i, j, out);
+    sort_contacts(out);
+    return out;
+}
+
+// contact_brute_force::run_impl: contact/contact_brute_force.hpp:102-137
+std::vector<Contact> contacts_brute_force(const Scene& s) {
+    std::vector<Contact> out;
+    for (int64_t i = s.n_inactive; i < s.n - 1; ++i)
+        for (int64_t j = i + 1; j < s.n; ++j) exact_distance(s, i, j, out);
+    for (int64_t i = 0; i < s.n_inactive; ++i)
+        for (int64_t j = s.n_inactive; j < s.n; ++j) exact_distance(s, i, j, out);
+    sort_contacts(out);
+    return out;
+}
+
+// ------------------------------------------------------------------ matrices
+// detail::cross<dim>: matrix/velocities.hpp:26-44
+inline vec3 cross(int dim, const vec3& x, const vec3& y) {
+    vec3 o;
+    if (dim == 2) {
+        o[0] = x[1] * y[2];
+        o[1] = -x[0] * y[2];
+        o[2] = x[0] * y[1] - x[1] * y[0];
+    } else {
+        o[0] = x[1] * y[2] - x[2] * y[1];
+        o[1] = x[2] * y[0] - x[0] * y[2];
+        o[2] = x[0] * y[1] - x[1] * y[0];
+    }
+    return o;
+}
+// detail::mat_mult: matrix/velocities.hpp:16-24
+inline vec3 mat_mult(const mat3& a, const vec3& x) {
+    return {a[0][0] * x[0] + a[0][1] * x[1] + a[0][2] * x[2], a[1][0] * x[0] + a[1][1] * x[1] + a[1][2] * x[2],
+            a[2][0] * x[0] + a[2][1] * x[1] + a[2][2] * x[2]};
+}
+inline vec3 mat_mult_t(const mat3& a, const vec3& x) {
+    return {a[0][0] * x[0] + a[1][0] * x[1] + a[2][0] * x[2], a[0][1] * x[0] + a[1][1] * x[1] + a[2][1] * x[2],
+            a[0][2] * x[0] + a[1][2] * x[1] + a[2][2] * x[2]};
+}
+
+// rij = c.pi - pos(c.i); in 2D the third slot is never read by cross<2>.
+inline vec3 arm(const Scene& s, const vec3& p, int64_t k) {
+    vec3 r{0, 0, 0};
+    for (int d = 0; d < s.dim; ++d) r[d] = p[d] - s.pos[k][d];
+    return r;
+}
+
+// AMatrix::mat_mult: matrix/velocities.hpp:67-144 (u: 6*Na, out: 3*nc)
+void amat(const Scene& s, const std::vector<Contact>& cs, const double* u, double* out) {
+    const int64_t off = s.n_inactive, rot = 3 * s.na();
+    for (size_t c = 0; c < cs.size(); ++c) {
+        const Contact& ct = cs[c];
+        double* o = out + 3 * c;
+        o[0] = o[1] = o[2] = 0.0;
+        if (ct.i >= off) {
+            int64_t st = (ct.i - off) * 3;
+            vec3 v{u[st], u[st + 1], u[st + 2]}, om{u[rot + st], u[rot + st + 1], u[rot + st + 2]};
+            vec3 cr = cross(s.dim, arm(s, ct.pi, ct.i), mat_mult(rotation3(s.q[ct.i]), om));
+            for (int d = 0; d < 3; ++d) o[d] += v[d] - cr[d];
+        }
+        if (ct.j >= off) {
+            int64_t st = (ct.j - off) * 3;
+            vec3 v{u[st], u[st + 1], u[st + 2]}, om{u[rot + st], u[rot + st + 1], u[rot + st + 2]};
+            vec3 cr = cross(s.dim, arm(s, ct.pj, ct.j), mat_mult(rotation3(s.q[ct.j]), om));
+            for (int d = 0; d < 3; ++d) o[d] -= v[d] - cr[d];
+        }
+    }
+}
+
+// ATMatrix::mat_mult: matrix/velocities.hpp:167-254 (f: 3*nc, out: 6*Na)
+void atmat(const Scene& s, const std::vector<Contact>& cs, const double* f, double* out) {
+    const int64_t off = s.n_inactive, rot = 3 * s.na();
+    std::fill(out, out + 6 * s.na(), 0.0);
+    for (size_t c = 0; c < cs.size(); ++c) {
+        const Contact& ct = cs[c];
+        vec3 fv{f[3 * c], f[3 * c + 1], f[3 * c + 2]};
+        if (ct.i >= off) {
+            int64_t st = (ct.i - off) * 3;
+            vec3 res = mat_mult_t(rotation3(s.q[ct.i]), cross(s.dim, arm(s, ct.pi, ct.i), fv));
+            for (int d = 0; d < 3; ++d) {
+                out[st + d] += fv[d];
+                out[rot + st + d] += res[d];
+            }
+        }
+        if (ct.j >= off) {
+            int64_t st = (ct.j - off) * 3;
+            vec3 res = mat_mult_t(rotation3(s.q[ct.j]), cross(s.dim, arm(s, ct.pj, ct.j), fv));
+            for (int d = 0; d < 3; ++d) {
+                out[st + d] -= fv[d];
+                out[rot + st + d] -= res[d];
+            }
+        }
+    }
+}
+
+// ------------------------------------------------------------------ minimization problem
+// minimization_problem<problem, ...>: solvers/minimization_problem.hpp:10-162 with
+// LagrangeMultiplier<NoFriction> (lagrange_multiplier.hpp:44-114) and
+// LagrangeMultiplier<Friction> (lagrange_multiplier.hpp:208-273).
+struct MinProblem {
+    const Scene& s;
+    const std::vector<Contact>& cs;
+    double dt;
+    bool friction;
+    std::vector<double> invM, C;
+    mutable std::vector<double> g3, w6, a3;
+    mutable int64_t q_applies = 0;
+
+    MinProblem(const Scene& sc, const std::vector<Contact>& c, double dt_) : s(sc), cs(c), dt(dt_) {
+        friction = (s.prm.problem == ORACLE_FRICTION);
+        const int64_t na = s.na(), off = s.n_inactive, nc = (int64_t)cs.size();
+        // M_inverse: minimization_problem.hpp:10-31
+        invM.assign(6 * na, 0.0);
+        for (int64_t i = 0; i < na; ++i) {
+            for (int d = 0; d < s.dim; ++d) invM[3 * i + d] = 1. / s.m[off + i];
+            if (s.dim == 2)
+                invM[3 * na + 3 * i + 2] = 1. / s.J[off + i][2];
+            else
+                for (int d = 0; d < 3; ++d) invM[3 * na + 3 * i + d] = 1. / s.J[off + i][d];
+        }
+        // CVector: minimization_problem.hpp:33-70 (D: velocities.hpp:274-289)
+        std::vector<double> U(6 * na, 0.0);
+        for (int64_t i = 0; i < na; ++i) {
+            for (int d = 0; d < s.dim; ++d) U[3 * i + d] = s.v[off + i][d];
+            if (s.dim == 2)
+                U[3 * na + 3 * i + 2] = s.omega[off + i][2];
+            else
+                for (int d = 0; d < 3; ++d) U[3 * na + 3 * i + d] = s.omega[off + i][d];
+        }
+        std::vector<double> AU(3 * nc);
+        amat(s, cs, U.data(), AU.data());
+        C.assign(3 * nc, 0.0);
+        for (int64_t k = 0; k < nc; ++k)
+            for (int d = 0; d < 3; ++d) {
+                double dn = (d < s.dim) ? cs[k].dij * cs[k].nij[d] : 0.0;
+                C[3 * k + d] = dn + dt * AU[3 * k + d];
+            }
+        g3.resize(3 * nc);
+        w6.resize(6 * na);
+        a3.resize(3 * nc);
+    }
+    int64_t size() const { return friction ? 3 * (int64_t)cs.size() : (int64_t)cs.size(); }
+
+    // local2global (lagrange_multiplier.hpp:78-92); the unset 2D slot is 0.
+    void l2g(const double* x, double* out) const {
+        const int64_t nc = cs.size();
+        if (friction) {
+            std::copy(x, x + 3 * nc, out);
+            return;
+        }
+        for (int64_t k = 0; k < nc; ++k) {
+            out[3 * k + 2] = 0.0;
+            for (int d = 0; d < s.dim; ++d) out[3 * k + d] = x[k] * cs[k].nij[d];
+        }
+    }
+    // global2local (lagrange_multiplier.hpp:62-76)
+    void g2l(const double* x, double* out) const {
+        const int64_t nc = cs.size();
+        if (friction) {
+            std::copy(x, x + 3 * nc, out);
+            return;
+        }
+        for (int64_t k = 0; k < nc; ++k) {
+            double acc = 0;
+            for (int d = 0; d < s.dim; ++d) acc += x[3 * k + d] * cs[k].nij[d];
+            out[k] = acc;
+        }
+    }
+    // QMatrix::operator(): minimization_problem.hpp:85-92  Q x = dt^2 A(M^-1 A^T x)
+    void Q(const double* xg, double* out) const {
+        ++q_applies;
+        atmat(s, cs, xg, w6.data());
+        for (size_t k = 0; k < w6.size(); ++k) w6[k] = invM[k] * w6[k];
+        amat(s, cs, w6.data(), out);
+        for (size_t k = 0; k < 3 * cs.size(); ++k) out[k] = dt * dt * out[k];
+    }
+    // gradient: minimization_problem.hpp:122-128 (S = 0 for both problems)
+    void gradient(const double* lam, double* out) const {
+        l2g(lam, g3.data());
+        Q(g3.data(), a3.data());
+        for (size_t k = 0; k < a3.size(); ++k) a3[k] = a3[k] + C[k];
+        g2l(a3.data(), out);
+    }
+    // operator(): minimization_problem.hpp:130-134
+    double value(const double* lam) const {
+        l2g(lam, g3.data());
+        Q(g3.data(), a3.data());
+        double r = 0.0;
+        for (size_t k = 0; k < a3.size(); ++k) r += g3[k] * (0.5 * a3[k] + C[k]);
+        return r;  // + <lambda, S> with S = 0
+    }
+    // velocities: minimization_problem.hpp:136-139 -> M^-1 A^T l2g(lambda)
+    void velocities(const double* lam, double* out) const {
+        l2g(lam, g3.data());
+        atmat(s, cs, g3.data(), out);
+        for (int64_t k = 0; k < 6 * s.na(); ++k) out[k] = invM[k] * out[k];
+    }
+    // projection: NoFriction lagrange_multiplier.hpp:104-107; Friction :243-268
+    void projection(double* lam) const {
+        const int64_t nc = cs.size();
+        if (!friction) {
+            for (int64_t k = 0; k < nc; ++k) lam[k] = std::max(lam[k], 0.);
+            return;
+        }
+        for (int64_t k = 0; k < nc; ++k) {
+            double* li = lam + 3 * k;
+            const double* n = cs[k].nij.data();
+            double ln = dot(li, n, s.dim);
+            vec3 lt{0, 0, 0};
+            for (int d = 0; d < s.dim; ++d) lt[d] = li[d] - ln * n[d];
+            double norm = nrm2(lt.data(), s.dim);
+            double mu = cs[k].mu;
+            if (norm > mu * ln) {
+                if (ln <= -mu * norm) {
+                    for (int d = 0; d < s.dim; ++d) li[d] = 0;
+                } else {
+                    double new_norm = (mu * mu) * (norm + ln / mu) / (mu * mu + 1);
+                    double new_ln = new_norm / mu;
+                    for (int d = 0; d < s.dim; ++d) lt[d] /= norm;
+                    for (int d = 0; d < s.dim; ++d) li[d] = new_norm * lt[d] + new_ln * n[d];
+                }
+            }
+        }
+    }
+};
+
+struct SolveResult {
+    std::vector<double> lambda;
+    int64_t iterations = 0, backtracks = 0, converged = 0;
+};
+
+double norm_diff(const std::vector<double>& a, const std::vector<double>& b) {
+    double r = 0.0;
+    for (size_t k = 0; k < a.size(); ++k) {
+        double d = a[k] - b[k];
+        r += d * d;
+    }
+    return std::sqrt(r);
+}
+
+// apgd::operator(): solvers/apgd.hpp:147-222 (theta is uniform, kept scalar)
+SolveResult apgd(const MinProblem& mp, const oracle_params& prm) {
+    const int64_t n = mp.size();
+    SolveResult res;
+    std::vector<double> lam_n(n, 0.0), lam_np1(n, 0.0), y_n(n, 0.0), y_np1(n, 0.0), dG(n), tmp(n);
+    double theta_n = 1.0, theta_np1 = 1.0;
+    double alpha = prm.alpha;
+    double lipsch = 1. / alpha;
+    int64_t ite = 0;
+    bool conv = false;
+    while (ite < prm.max_ite) {
+        ++ite;
+        mp.gradient(y_n.data(), dG.data());
+        for (int64_t k = 0; k < n; ++k) lam_np1[k] = y_n[k] - alpha * dG[k];
+        mp.projection(lam_np1.data());
+        if (prm.dynamic_descent) {
+            while (true) {
+                double f_new = mp.value(lam_np1.data());
+                double f_y = mp.value(y_n.data());
+                double dt_ = 0.0, nr = 0.0;
+                for (int64_t k = 0; k < n; ++k) {
+                    double d = lam_np1[k] - y_n[k];
+                    dt_ += dG[k] * d;
+                    nr += d * d;
+                }
+                nr = std::sqrt(nr);
+                if (!(f_new >= f_y + dt_ + 0.5 * lipsch * std::pow(nr, 2))) break;
+                lipsch *= 2;
+                alpha = 1. / lipsch;
+                for (int64_t k = 0; k < n; ++k) lam_np1[k] = y_n[k] - alpha * dG[k];
+                mp.projection(lam_np1.data());
+                ++res.backtracks;
+            }
+        }
+        if (norm_diff(lam_np1, lam_n) < prm.tolerance) {
+            std::swap(lam_n, lam_np1);
+            conv = true;
+            break;
+        }
+        theta_np1 = 0.5 * (theta_n * std::sqrt(4 + theta_n * theta_n) - theta_n * theta_n);
+        double beta = theta_n * (1 - theta_n) / (theta_n * theta_n + theta_np1);
+        for (int64_t k = 0; k < n; ++k) y_np1[k] = lam_np1[k] + beta * (lam_np1[k] - lam_n[k]);
+        if (prm.dynamic_descent) {
+            double d = 0.0;
+            for (int64_t k = 0; k < n; ++k) d += dG[k] * (lam_np1[k] - lam_n[k]);
+            if (d > 0) {
+                y_np1 = lam_np1;
+                theta_np1 = 1.;
+            }
+            lipsch *= 0.97;
+            alpha = 1. / lipsch;
+        }
+        std::swap(lam_n, lam_np1);
+        std::swap(theta_n, theta_np1);
+        std::swap(y_n, y_np1);
+    }
+    res.iterations = ite;
+    res.converged = conv ? 1 : 0;
+    res.lambda = lam_n;
+    return res;
+}
+
+// pgd::operator(): solvers/apgd.hpp:60-98
+SolveResult pgd(const MinProblem& mp, const oracle_params& prm) {
+    const int64_t n = mp.size();
+    SolveResult res;
+    std::vector<double> lam_n(n, 0.0), lam_np1(n, 0.0), dG(n);
+    int64_t ite = 0;
+    bool conv = false;
+    while (ite < prm.max_ite) {
+        ++ite;
+        mp.gradient(lam_n.data(), dG.data());
+        for (int64_t k = 0; k < n; ++k) lam_np1[k] = lam_n[k] - prm.alpha * dG[k];
+        mp.projection(lam_np1.data());
+        if (norm_diff(lam_np1, lam_n) < prm.tolerance) {
+            std::swap(lam_n, lam_np1);
+            conv = true;
+            break;
+        }
+        std::swap(lam_n, lam_np1);
+    }
+    res.iterations = ite;
+    res.converged = conv ? 1 : 0;
+    res.lambda = lam_n;
+    return res;
+}
+
+// OptimGradient::run: solvers/OptimGradient.hpp:252-306
+void optim_run(Scene& s, double dt) {
+    const int64_t na = s.na(), off = s.n_inactive;
+    s.u.assign(3 * na, 0.0);
+    s.w.assign(3 * na, 0.0);
+    if (!s.contacts.empty()) {
+        MinProblem mp(s, s.contacts, dt);
+        SolveResult r = (s.prm.method == ORACLE_METHOD_PGD) ? pgd(mp, s.prm) : apgd(mp, s.prm);
+        s.lambda = r.lambda;
+        std::vector<double> vel(6 * na);
+        mp.velocities(s.lambda.data(), vel.data());
+        for (int64_t i = 0; i < na; ++i) {
+            for (int d = 0; d < s.dim; ++d) {
+                s.u[3 * i + d] = s.v[i + off][d] + dt * vel[i * 3 + d];
+                if (s.dim == 3) s.w[3 * i + d] = s.omega[i + off][d] + dt * vel[(i + na) * 3 + d];
+            }
+            if (s.dim == 2) s.w[3 * i + 2] = s.omega[i + off][2] + dt * vel[(i + na) * 3 + 2];
+        }
+        s.st.iterations = r.iterations;
+        s.st.backtracks = r.backtracks;
+        s.st.q_applies = mp.q_applies;
+        s.st.converged = r.converged;
+        s.st.total_iterations += r.iterations;
+        s.st.total_q_applies += mp.q_applies;
+    } else {
+        s.lambda.clear();
+        for (int64_t i = 0; i < na; ++i) {
+            for (int d = 0; d < s.dim; ++d) {
+                s.u[3 * i + d] = s.v[i + off][d];
+                if (s.dim == 3) s.w[3 * i + d] = s.omega[i + off][d];
+            }
+            if (s.dim == 2) s.w[3 * i + 2] = s.omega[i + off][2];
+        }
+        s.st.iterations = 0;
+        s.st.backtracks = 0;
+        s.st.q_applies = 0;
+        s.st.converged = 1;
+    }
+}
+
+// vap_fixed (vap/vap_fixed.hpp:46-55) / vap_fpd (vap/vap_fpd.hpp:70-82, src/vap/vap_fpd.cpp:6-26)
+void set_a_priori_velocity(Scene& s, double dt) {
+    for (int64_t i = s.n_inactive; i < s.n; ++i) {
+        if (s.prm.vap == ORACLE_VAP_FIXED) {
+            s.v[i] = s.vd[i];
+            s.omega[i] = s.domega[i];
+        } else {
+            for (int d = 0; d < s.dim; ++d) s.v[i][d] += dt * s.f[i][d] / s.m[i];
+            if (s.dim == 3) {
+                const double o1 = s.omega[i][0], o2 = s.omega[i][1], o3 = s.omega[i][2];
+                const double j1 = s.J[i][0], j2 = s.J[i][1], j3 = s.J[i][2];
+                vec3 res{o2 * o3 * (j3 - j2), o1 * o3 * (j1 - j3), o1 * o2 * (j2 - j1)};
+                for (int d = 0; d < 3; ++d) s.omega[i][d] += res[d];
+            }
+        }
+    }
+}
+
+// q <- q * exp(omega dt / 2), normalised: solver.hpp:329-355 and 440-480
+void rotate(int dim, quat& q, const vec3& om, double dt) {
+    vec3 w{0, 0, 0};
+    double normw;
+    if (dim == 2) {
+        w = {0, 0, om[2]};
+        normw = std::abs(om[2]);
+    } else {
+        w = om;
+        normw = nrm2(w.data(), 3);
+    }
+    if (normw == 0) normw = 1;
+    quat expw;
+    expw[0] = std::cos(0.5 * normw * dt);
+    for (int d = 0; d < 3; ++d) expw[d + 1] = std::sin(0.5 * normw * dt) / normw * w[d];
+    q = mult_quaternion(q, expw);
+    normalize(q);
+}
+
+// displacement_obstacles: solver.hpp:329-355 (get_omega: 2D (0,0,w), 3D w)
+void displacement_obstacles(Scene& s, double dt) {
+    for (int64_t i = 0; i < s.n_inactive; ++i) {
+        vec3 w = s.domega[i];
+        if (s.dim == 2) w = {0, 0, s.domega[i][2]};
+        double normw = nrm2(w.data(), 3);
+        if (normw == 0) normw = 1;
+        quat expw;
+        expw[0] = std::cos(0.5 * normw * dt);
+        for (int d = 0; d < 3; ++d) expw[d + 1] = std::sin(0.5 * normw * dt) / normw * w[d];
+        for (int d = 0; d < s.dim; ++d) s.pos[i][d] += dt * s.vd[i][d];
+        s.q[i] = mult_quaternion(s.q[i], expw);
+        normalize(s.q[i]);
+    }
+}
+
+// update_velocity (solver.hpp:527-546) + move_active_particles (solver.hpp:440-480)
+void update_and_move(Scene& s, double dt) {
+    const int64_t off = s.n_inactive;
+    for (int64_t i = 0; i < s.na(); ++i) {
+        for (int d = 0; d < s.dim; ++d) s.v[i + off][d] = s.u[3 * i + d];
+        if (s.dim == 2)
+            s.omega[i + off][2] = s.w[3 * i + 2];
+        else
+            for (int d = 0; d < 3; ++d) s.omega[i + off][d] = s.w[3 * i + d];
+    }
+    for (int64_t i = off; i < s.n; ++i) {
+        rotate(s.dim, s.q[i], s.omega[i], dt);
+        for (int d = 0; d < s.dim; ++d) s.pos[i][d] += dt * s.v[i][d];
+    }
+}
+
+std::vector<Contact> compute_contacts(const Scene& s) {
+    return (s.prm.contact_method == ORACLE_CONTACT_BRUTE_FORCE) ? contacts_brute_force(s) : contacts_kdtree(s);
+}
+
+}  // namespace
+
+extern "C" {
+
+void* oracle_scene_new(void) {
+    Scene* s = new Scene();
+    oracle_default_params(&s->prm);
+    return s;
+}
+void oracle_scene_free(void* scene) { delete static_cast<Scene*>(scene); }
+
+// Defaults: ContactsParams (contact_kdtree.hpp:52,60), apgd_params (apgd.hpp:120-123),
+// contact_property<Friction>::mu (contact/property.hpp:72).
+int oracle_default_params(oracle_params* p) {
+    if (!p) return -1;
+    std::memset(p, 0, sizeof(*p));
+    p->problem = ORACLE_NOFRICTION;
+    p->contact_method = ORACLE_CONTACT_KDTREE;
+    p->vap = ORACLE_VAP_FIXED;
+    p->method = ORACLE_METHOD_APGD;
+    p->dmax = 2.0;
+    p->kd_tree_radius = 17.0;
+    p->alpha = 0.05;
+    p->max_ite = 10000;
+    p->tolerance = 1e-7;
+    p->dynamic_descent = 1;
+    p->mu = 0.1;
+    return 0;
+}
+
+int oracle_set_params(void* scene, const oracle_params* p) {
+    if (!scene || !p) return -1;
+    static_cast<Scene*>(scene)->prm = *p;
+    return 0;
+}
+
+int oracle_scene_set(void* scene, int32_t dim, int64_t n, int64_t n_inactive, const int32_t* shape, const double* pos,
+                     const double* q, const double* size, const double* v, const double* vd, const double* omega,
+                     const double* domega, const double* f, const double* m, const double* J) {
+    if (!scene || (dim != 2 && dim != 3) || n < 0 || n_inactive < 0 || n_inactive > n) return -1;
+    Scene& s = *static_cast<Scene*>(scene);
+    s.dim = dim;
+    s.n = n;
+    s.n_inactive = n_inactive;
+    s.shape.assign(shape, shape + n);
+    auto load3 = [&](std::vector<vec3>& dst, const double* src) {
+        dst.resize(n);
+        for (int64_t i = 0; i < n; ++i) dst[i] = {src[3 * i], src[3 * i + 1], src[3 * i + 2]};
+    };
+    load3(s.pos, pos);
+    load3(s.size, size);
+    load3(s.v, v);
+    load3(s.vd, vd);
+    load3(s.omega, omega);
+    load3(s.domega, domega);
+    load3(s.f, f);
+    load3(s.J, J);
+    s.q.resize(n);
+    for (int64_t i = 0; i < n; ++i) s.q[i] = {q[4 * i], q[4 * i + 1], q[4 * i + 2], q[4 * i + 3]};
+    s.m.assign(m, m + n);
+    s.contacts.clear();
+    s.lambda.clear();
+    s.st = oracle_stats{};
+    return 0;
+}
+
+int oracle_get_state(void* scene, double* pos, double* q, double* v, double* omega) {
+    if (!scene) return -1;
+    Scene& s = *static_cast<Scene*>(scene);
+    for (int64_t i = 0; i < s.n; ++i) {
+        for (int d = 0; d < 3; ++d) {
+            if (pos) pos[3 * i + d] = s.pos[i][d];
+            if (v) v[3 * i + d] = s.v[i][d];
+            if (omega) omega[3 * i + d] = s.omega[i][d];
+        }
+        for (int d = 0; d < 4; ++d)
+            if (q) q[4 * i + d] = s.q[i][d];
+    }
+    return 0;
+}
+
+// ScopiSolver::run: solver.hpp:266-303 (no output; transfer() is inert for
+// NoFriction/Friction because every contact carries the default property).
+int oracle_run(void* scene, double dt, int64_t total_it, int64_t initial_iter) {
+    if (!scene) return -1;
+    Scene& s = *static_cast<Scene*>(scene);
+    for (int64_t nite = initial_iter; nite < total_it; ++nite) {
+        auto t0 = std::chrono::steady_clock::now();
+        displacement_obstacles(s, dt);
+        s.contacts = compute_contacts(s);
+        s.st.n_contacts = (int64_t)s.contacts.size();
+        s.st.t_contacts += seconds_since(t0);
+        t0 = std::chrono::steady_clock::now();
+        set_a_priori_velocity(s, dt);
+        s.st.t_vap += seconds_since(t0);
+        t0 = std::chrono::steady_clock::now();
+        optim_run(s, dt);
+        s.st.t_solve += seconds_since(t0);
+        t0 = std::chrono::steady_clock::now();
+        update_and_move(s, dt);
+        s.st.t_move += seconds_since(t0);
+    }
+    return 0;
+}
+
+int64_t oracle_compute_contacts(void* scene) {
+    if (!scene) return -1;
+    Scene& s = *static_cast<Scene*>(scene);
+    s.contacts = compute_contacts(s);
+    s.st.n_contacts = (int64_t)s.contacts.size();
+    return (int64_t)s.contacts.size();
+}
+
+int64_t oracle_n_contacts(void* scene) {
+    if (!scene) return -1;
+    return (int64_t) static_cast<Scene*>(scene)->contacts.size();
+}
+
+int oracle_get_contacts(void* scene, int64_t* ij, double* dij, double* nij, double* pi, double* pj) {
+    if (!scene) return -1;
+    Scene& s = *static_cast<Scene*>(scene);
+    for (size_t c = 0; c < s.contacts.size(); ++c) {
+        const Contact& ct = s.contacts[c];
+        if (ij) {
+            ij[2 * c] = ct.i;
+            ij[2 * c + 1] = ct.j;
+        }
+        if (dij) dij[c] = ct.dij;
+        for (int d = 0; d < 3; ++d) {
+            if (nij) nij[3 * c + d] = ct.nij[d];
+            if (pi) pi[3 * c + d] = ct.pi[d];
+            if (pj) pj[3 * c + d] = ct.pj[d];
+        }
+    }
+    return 0;
+}
+
+int oracle_amat(void* scene, const double* u, double* out) {
+    if (!scene) return -1;
+    Scene& s = *static_cast<Scene*>(scene);
+    amat(s, s.contacts, u, out);
+    return 0;
+}
+
+int oracle_atmat(void* scene, const double* f, double* out) {
+    if (!scene) return -1;
+    Scene& s = *static_cast<Scene*>(scene);
+    atmat(s, s.contacts, f, out);
+    return 0;
+}
+
+int oracle_solve(void* scene, double dt, double* lambda, double* u, double* w) {
+    if (!scene) return -1;
+    Scene& s = *static_cast<Scene*>(scene);
+    optim_run(s, dt);
+    if (lambda) std::copy(s.lambda.begin(), s.lambda.end(), lambda);
+    if (u) std::copy(s.u.begin(), s.u.end(), u);
+    if (w) std::copy(s.w.begin(), s.w.end(), w);
+    return 0;
+}
+
+int oracle_set_a_priori_velocity(void* scene, double dt) {
+    if (!scene) return -1;
+    set_a_priori_velocity(*static_cast<Scene*>(scene), dt);
+    return 0;
+}
+
+int oracle_get_stats(void* scene, oracle_stats* st) {
+    if (!scene || !st) return -1;
+    *st = static_cast<Scene*>(scene)->st;
+    return 0;
+}
+
+int oracle_closest_points(int32_t dim, int32_t shape_i, const double* pos_i, const double* q_i, const double* size_i,
+                          int32_t shape_j, const double* pos_j, const double* q_j, const double* size_j, double* dij,
+                          double* nij, double* pi, double* pj) {
+    vec3 a{pos_i[0], pos_i[1], pos_i[2]}, b{pos_j[0], pos_j[1], pos_j[2]};
+    vec3 sa{size_i[0], size_i[1], size_i[2]}, sb{size_j[0], size_j[1], size_j[2]};
+    quat qa{q_i[0], q_i[1], q_i[2], q_i[3]}, qb{q_j[0], q_j[1], q_j[2], q_j[3]};
+    Contact c{};
+    if (!closest_points(dim, shape_i, a, qa, sa, shape_j, b, qb, sb, c)) return -2;
+    *dij = c.dij;
+    for (int d = 0; d < 3; ++d) {
+        nij[d] = c.nij[d];
+        pi[d] = c.pi[d];
+        pj[d] = c.pj[d];
+    }
+    return 0;
+}
+
+int oracle_quaternion(double angle, const double* axis, double* q) {
+    quat o{std::cos(angle / 2), std::sin(angle / 2) * axis[0], std::sin(angle / 2) * axis[1],
+           std::sin(angle / 2) * axis[2]};
+    normalize(o);
+    for (int d = 0; d < 4; ++d) q[d] = o[d];
+    return 0;
+}
+
+// test/test_sphere.cpp:430-470: 2*n*n spheres from std::minstd_rand0(123).
+int oracle_gen_2d_case_spheres(int32_t n, double* x, double* y, double* r, double* velocity) {
+    std::minstd_rand0 generator(123);
+    std::uniform_real_distribution<double> distrib_r(0.2, 0.4);
+    std::uniform_real_distribution<double> distrib_move_x(-0.1, 0.1);
+    std::uniform_real_distribution<double> distrib_move_y(-0.1, 0.1);
+    std::uniform_real_distribution<double> distrib_velocity(2., 5.);
+    int k = 0;
+    for (int i = 0; i < n; ++i)
+        for (int j = 0; j < n; ++j) {
+            double rr = distrib_r(generator);
+            double xx = (i + 0.5) + distrib_move_x(generator);
+            double yy = (j + 0.5) + distrib_move_y(generator);
+            double vel = distrib_velocity(generator);
+            x[k] = xx, y[k] = yy, r[k] = rr, velocity[k] = vel;
+            ++k;
+            rr = distrib_r(generator);
+            xx = (n + i + 0.5) + distrib_move_x(generator);
+            yy = (j + 0.5) + distrib_move_y(generator);
+            vel = distrib_velocity(generator);
+            x[k] = xx, y[k] = yy, r[k] = rr, velocity[k] = -vel;
+            ++k;
+        }
+    return 0;
+}
+
+}  // extern "C"

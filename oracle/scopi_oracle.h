@@ -1,0 +1,109 @@
+/*
+ * scopi_oracle.h -- C interface of the FP64 CPU restatement of SCoPI's
+ * per-time-step contact pipeline.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This library is the checker: only tests/,
+ * __graft_entry__.smoke() and bench.py's cpu_baseline leg may load it.  The
+ * product path (scopi_amd/, libscopi_hip.so) never links or calls it.
+ *
+ * Array conventions (shared with include/scopi_hip.h):
+ *   - every per-particle vector is stored with 3 slots (x, y, z) even in 2D;
+ *   - 2D rotation velocity / moment of inertia live in slot 2 (the z axis),
+ *     which is the slot the reference's 6-dof u-vector uses in 2D
+ *     (include/scopi/solvers/minimization_problem.hpp:15-29);
+ *   - quaternions are scalar-first (w, x, y, z) (include/scopi/quaternion.hpp:86-124);
+ *   - size[3]: sphere radius in size[0]; segment length in size[0];
+ *     superellipsoid semi-axes in size[0..2]; planes ignore it.
+ */
+#ifndef SCOPI_ORACLE_H
+#define SCOPI_ORACLE_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum { ORACLE_SPHERE = 0, ORACLE_PLANE = 1, ORACLE_SEGMENT = 2 };
+enum { ORACLE_NOFRICTION = 0, ORACLE_FRICTION = 1 };
+enum { ORACLE_CONTACT_KDTREE = 0, ORACLE_CONTACT_BRUTE_FORCE = 1 };
+enum { ORACLE_VAP_FIXED = 0, ORACLE_VAP_FPD = 1 };
+enum { ORACLE_METHOD_APGD = 0, ORACLE_METHOD_PGD = 1 };
+
+typedef struct oracle_params {
+    int32_t problem;         /* ORACLE_NOFRICTION / ORACLE_FRICTION            */
+    int32_t contact_method;  /* kd-tree predicate set / brute force            */
+    int32_t vap;             /* vap_fixed / vap_fpd                            */
+    int32_t method;          /* apgd / pgd                                     */
+    double dmax;             /* ContactsParams::dmax (default 2)               */
+    double kd_tree_radius;   /* squared search radius (default 17)            */
+    double alpha;            /* apgd_params::alpha (default 0.05)              */
+    int64_t max_ite;         /* apgd_params::max_ite (default 10000)           */
+    double tolerance;        /* apgd_params::tolerance (default 1e-7)          */
+    int32_t dynamic_descent; /* apgd_params::dynamic_descent (default true)    */
+    int32_t pad;
+    double mu;               /* contact_property<Friction>::mu (default 0.1)   */
+} oracle_params;
+
+typedef struct oracle_stats {
+    int64_t n_contacts;   /* contacts of the last step                        */
+    int64_t iterations;   /* APGD outer iterations of the last solve          */
+    int64_t backtracks;   /* line-search rejections of the last solve         */
+    int64_t q_applies;    /* Q-matrix applications of the last solve          */
+    int64_t converged;    /* 1 if the last solve met the tolerance            */
+    int64_t total_iterations; /* summed over every solve since the scene was set */
+    int64_t total_q_applies;
+    double t_contacts, t_vap, t_solve, t_move; /* seconds, summed over steps  */
+} oracle_stats;
+
+void* oracle_scene_new(void);
+void oracle_scene_free(void* scene);
+
+int oracle_default_params(oracle_params* p);
+int oracle_set_params(void* scene, const oracle_params* p);
+
+/* Replace the whole particle set (obstacles first: indices [0, n_inactive)). */
+int oracle_scene_set(void* scene, int32_t dim, int64_t n, int64_t n_inactive, const int32_t* shape,
+                     const double* pos, const double* q, const double* size, const double* v,
+                     const double* vd, const double* omega, const double* domega, const double* f,
+                     const double* m, const double* J);
+int oracle_get_state(void* scene, double* pos, double* q, double* v, double* omega);
+
+/* ScopiSolver::run(dt, total_it, initial_iter) without output files. */
+int oracle_run(void* scene, double dt, int64_t total_it, int64_t initial_iter);
+
+/* The contact slot alone, on the current state; returns nc or <0. */
+int64_t oracle_compute_contacts(void* scene);
+/* Copy the current contact list (last step's, or the last compute). */
+int oracle_get_contacts(void* scene, int64_t* ij, double* dij, double* nij, double* pi, double* pj);
+int64_t oracle_n_contacts(void* scene);
+
+/* Matrix-free operators on the current contact list. */
+int oracle_amat(void* scene, const double* u, double* out);  /* 6*Na -> 3*nc */
+int oracle_atmat(void* scene, const double* f, double* out); /* 3*nc -> 6*Na */
+
+/* The optim slot alone: solve on the current contacts with the current
+ * velocities taken as a-priori velocities.  lambda (size nc or 3nc), u (Na*3),
+ * w (Na*3) may be NULL. */
+int oracle_solve(void* scene, double dt, double* lambda, double* u, double* w);
+/* The vap slot alone. */
+int oracle_set_a_priori_velocity(void* scene, double dt);
+
+int oracle_get_stats(void* scene, oracle_stats* st);
+
+/* closest_points(object_i, object_j) for one pair, no dmax filter. */
+int oracle_closest_points(int32_t dim, int32_t shape_i, const double* pos_i, const double* q_i,
+                          const double* size_i, int32_t shape_j, const double* pos_j, const double* q_j,
+                          const double* size_j, double* dij, double* nij, double* pi, double* pj);
+
+/* quaternion(angle, axis) of src/quaternion.cpp:12-26. */
+int oracle_quaternion(double angle, const double* axis, double* q);
+
+/* Scene generator of test/test_sphere.cpp:423-470 (std::minstd_rand0(123)). */
+int oracle_gen_2d_case_spheres(int32_t n, double* x, double* y, double* r, double* velocity);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

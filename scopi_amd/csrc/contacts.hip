@@ -1,0 +1,433 @@
+// contacts.hip -- device contact search for the contact slot.
+//
+// Replaces contact_kdtree::run_impl (include/scopi/contact/contact_kdtree.hpp:200-268)
+// and contact_brute_force::run_impl (contact_brute_force.hpp:102-137).  The pair set is
+// the reference's exactly:
+//   active pairs  : i < j, both active, |c_i - c_j|^2 < kd_tree_radius (kd-tree only,
+//                   nanoflann L2 adaptor, strict <) and dij < dmax (strict);
+//   obstacle pairs: every (obstacle o, active j) with dij < dmax;
+//   order         : sorted by (i, j) (contact/base.hpp:156-175).
+// Candidates come from a uniform grid whose cell width is at least the
+// candidate cut-off, so every pair that can satisfy the predicates lies in the
+// 3^dim neighbouring cells; the predicates themselves are evaluated with the
+// reference's arithmetic (common.hpp).  Cells are ordered by Morton code and
+// particles are bucketed by a stable radix sort of those codes.  Each active
+// particle i counts, then emits, its partners j > i into its own segment of
+// the output, which it sorts by j, so the result is deterministic and needs no
+// global sort.
+#include "contacts.hpp"
+
+#include <algorithm>
+#include <cmath>
+
+namespace scopi_hip {
+
+constexpr int kMaxBits3 = 7;   // 128 cells per axis in 3D -> 2^21 cells
+constexpr int kMaxBits2 = 10;  // 1024 cells per axis in 2D -> 2^20 cells
+constexpr int64_t kMaxCells = int64_t(1) << 21;
+
+__device__ inline uint32_t spread3(uint32_t v) {
+    v &= 0x3ffu;
+    v = (v | (v << 16)) & 0x030000FFu;
+    v = (v | (v << 8)) & 0x0300F00Fu;
+    v = (v | (v << 4)) & 0x030C30C3u;
+    v = (v | (v << 2)) & 0x09249249u;
+    return v;
+}
+__device__ inline uint32_t spread2(uint32_t v) {
+    v &= 0xffffu;
+    v = (v | (v << 8)) & 0x00FF00FFu;
+    v = (v | (v << 4)) & 0x0F0F0F0Fu;
+    v = (v | (v << 2)) & 0x33333333u;
+    v = (v | (v << 1)) & 0x55555555u;
+    return v;
+}
+template <int DIM>
+__device__ inline uint32_t morton(int cx, int cy, int cz) {
+    if (DIM == 3) return spread3((uint32_t)cx) | (spread3((uint32_t)cy) << 1) | (spread3((uint32_t)cz) << 2);
+    return spread2((uint32_t)cx) | (spread2((uint32_t)cy) << 1);
+}
+
+template <int DIM>
+__device__ inline int cell_coord(const GridInfo& g, double x, int d) {
+    int c = (int)floor((x - g.lo[d]) / g.w[d]);
+    return c < 0 ? 0 : (c >= g.nc[d] ? g.nc[d] - 1 : c);
+}
+
+// Bounding radius of an active particle (sphere radius, largest SE semi-axis).
+__device__ inline double bounding_radius(const Particles& P, int64_t k) {
+    const int s = P.shape[k];
+    if (s == SCOPI_SHAPE_SUPERELLIPSOID) {
+        double a = P.size[k], b = P.size[P.cap + k], c = P.size[2 * P.cap + k];
+        return sqrt(a * a + b * b + c * c);
+    }
+    return P.size[k];
+}
+
+// ---------------------------------------------------------------- grid set-up
+// partial[b*7 + {0..2}] = min, [3..5] = max, [6] = max bounding radius
+__global__ void __launch_bounds__(kBlock) k_bbox_partial(Particles P, double* __restrict__ partial) {
+    __shared__ double red[7][kBlock];
+    const int64_t na = P.n - P.n_inactive;
+    double lo[3] = {1e300, 1e300, 1e300}, hi[3] = {-1e300, -1e300, -1e300}, rm = 0.0;
+    for (int64_t t = (int64_t)blockIdx.x * kBlock + threadIdx.x; t < na; t += (int64_t)gridDim.x * kBlock) {
+        const int64_t k = P.n_inactive + t;
+        for (int d = 0; d < 3; ++d) {
+            double x = P.pos[d * P.cap + k];
+            lo[d] = fmin(lo[d], x);
+            hi[d] = fmax(hi[d], x);
+        }
+        rm = fmax(rm, bounding_radius(P, k));
+    }
+    for (int d = 0; d < 3; ++d) {
+        red[d][threadIdx.x] = lo[d];
+        red[3 + d][threadIdx.x] = hi[d];
+    }
+    red[6][threadIdx.x] = rm;
+    __syncthreads();
+    for (int o = kBlock / 2; o > 0; o >>= 1) {
+        if ((int)threadIdx.x < o) {
+            for (int d = 0; d < 3; ++d) red[d][threadIdx.x] = fmin(red[d][threadIdx.x], red[d][threadIdx.x + o]);
+            for (int d = 3; d < 7; ++d) red[d][threadIdx.x] = fmax(red[d][threadIdx.x], red[d][threadIdx.x + o]);
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x < 7) partial[blockIdx.x * 7 + threadIdx.x] = red[threadIdx.x][0];
+}
+
+__global__ void k_grid_setup(const double* __restrict__ partial, int nparts, int dim, int kd, double kd_radius,
+                             double dmax, int shapes_closed_form, GridInfo* __restrict__ g) {
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    double lo[3] = {1e300, 1e300, 1e300}, hi[3] = {-1e300, -1e300, -1e300}, rm = 0.0;
+    for (int b = 0; b < nparts; ++b) {
+        for (int d = 0; d < 3; ++d) {
+            lo[d] = fmin(lo[d], partial[b * 7 + d]);
+            hi[d] = fmax(hi[d], partial[b * 7 + 3 + d]);
+        }
+        rm = fmax(rm, partial[b * 7 + 6]);
+    }
+    // Any pair with dij < dmax has |c_i - c_j| < r_i + r_j + dmax for closed-form
+    // shapes; the margin covers the roundings of the computed dij.
+    double geo = (2.0 * rm + dmax) * (1.0 + 1e-9) + 1e-12;
+    double cut = geo;
+    if (kd) {
+        double kdc = sqrt(kd_radius) * (1.0 + 1e-12) + 1e-300;
+        cut = shapes_closed_form ? fmin(geo, kdc) : kdc;
+    }
+    if (!(cut > 0)) cut = 1e-300;
+    const int bits_max = (dim == 3) ? kMaxBits3 : kMaxBits2;
+    int maxc = 1;
+    for (int d = 0; d < 3; ++d) {
+        g->lo[d] = (d < dim) ? lo[d] : 0.0;
+        double ext = (d < dim) ? fmax(hi[d] - lo[d], 0.0) : 0.0;
+        double cells = floor(ext / cut) + 1.0;
+        int nc = (int)fmin(cells, (double)(1 << bits_max));
+        if (d >= dim) nc = 1;
+        g->nc[d] = nc;
+        double w = cut;
+        if (nc == (1 << bits_max)) w = fmax(cut, ext / (double)nc * (1.0 + 1e-12));
+        g->w[d] = w;
+        maxc = nc > maxc ? nc : maxc;
+    }
+    int bits = 0;
+    while ((1 << bits) < maxc) ++bits;
+    g->bits = bits;
+    g->cutoff = cut;
+    g->rmax = rm;
+}
+
+template <int DIM>
+__global__ void __launch_bounds__(kBlock) k_cell_keys(Particles P, const GridInfo* __restrict__ gp,
+                                                     uint32_t* __restrict__ keys, int32_t* __restrict__ vals) {
+    const int64_t na = P.n - P.n_inactive;
+    const int64_t t = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (t >= na) return;
+    const GridInfo g = *gp;
+    const int64_t k = P.n_inactive + t;
+    int cx = cell_coord<DIM>(g, P.pos[k], 0);
+    int cy = cell_coord<DIM>(g, P.pos[P.cap + k], 1);
+    int cz = (DIM == 3) ? cell_coord<DIM>(g, P.pos[2 * P.cap + k], 2) : 0;
+    keys[t] = morton<DIM>(cx, cy, cz);
+    vals[t] = (int32_t)k;
+}
+
+__global__ void __launch_bounds__(kBlock) k_fill_int(int* __restrict__ a, int64_t n, int v) {
+    for (int64_t t = (int64_t)blockIdx.x * kBlock + threadIdx.x; t < n; t += (int64_t)gridDim.x * kBlock) a[t] = v;
+}
+
+__global__ void __launch_bounds__(kBlock) k_cell_range(const uint32_t* __restrict__ keys, int64_t na,
+                                                      int* __restrict__ start, int* __restrict__ end) {
+    const int64_t s = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (s >= na) return;
+    const uint32_t k = keys[s];
+    if (s == 0 || keys[s - 1] != k) start[k] = (int)s;
+    if (s == na - 1 || keys[s + 1] != k) end[k] = (int)(s + 1);
+}
+
+// Visits every j > i in the neighbouring cells that satisfies both predicates.
+template <int DIM, bool KD, class F>
+__device__ inline void for_partners(const Particles& P, const GridInfo& g, const int* __restrict__ start,
+                                    const int* __restrict__ end, const int32_t* __restrict__ sorted, int64_t i,
+                                    double kd_radius, double dmax, F&& visit) {
+    const d3 ci = load3(P.pos, P.cap, i);
+    const int cx = cell_coord<DIM>(g, ci.x, 0), cy = cell_coord<DIM>(g, ci.y, 1);
+    const int cz = (DIM == 3) ? cell_coord<DIM>(g, ci.z, 2) : 0;
+    const int zr = (DIM == 3) ? 1 : 0;
+    for (int dz = -zr; dz <= zr; ++dz) {
+        const int z = cz + dz;
+        if (z < 0 || z >= g.nc[2]) continue;
+        for (int dy = -1; dy <= 1; ++dy) {
+            const int y = cy + dy;
+            if (y < 0 || y >= g.nc[1]) continue;
+            for (int dx = -1; dx <= 1; ++dx) {
+                const int x = cx + dx;
+                if (x < 0 || x >= g.nc[0]) continue;
+                const uint32_t key = morton<DIM>(x, y, z);
+                const int s1 = end[key];
+                for (int s = start[key]; s < s1; ++s) {
+                    const int64_t j = sorted[s];
+                    if (j <= i) continue;
+                    if (KD) {
+                        const d3 cj = load3(P.pos, P.cap, j);
+                        if (!(kd_dist2<DIM>(ci, cj) < kd_radius)) continue;
+                    }
+                    CP cp;
+                    if (!closest_points<DIM>(P, i, j, cp)) continue;
+                    if (cp.dij < dmax) visit(j);
+                }
+            }
+        }
+    }
+}
+
+template <int DIM, bool KD>
+__global__ void __launch_bounds__(kBlock) k_pair_count(Particles P, const GridInfo* __restrict__ gp,
+                                                      const int* __restrict__ start, const int* __restrict__ end,
+                                                      const int32_t* __restrict__ sorted, double kd_radius, double dmax,
+                                                      int* __restrict__ counts) {
+    const int64_t na = P.n - P.n_inactive;
+    const int64_t t = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (t >= na) return;
+    const GridInfo g = *gp;
+    int cnt = 0;
+    for_partners<DIM, KD>(P, g, start, end, sorted, P.n_inactive + t, kd_radius, dmax, [&](int64_t) { ++cnt; });
+    counts[t] = cnt;
+}
+
+template <int DIM, bool KD>
+__global__ void __launch_bounds__(kBlock) k_pair_emit(Particles P, const GridInfo* __restrict__ gp,
+                                                     const int* __restrict__ start, const int* __restrict__ end,
+                                                     const int32_t* __restrict__ sorted, double kd_radius, double dmax,
+                                                     const int* __restrict__ offsets, int32_t* __restrict__ pair_j) {
+    const int64_t na = P.n - P.n_inactive;
+    const int64_t t = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (t >= na) return;
+    const GridInfo g = *gp;
+    const int64_t base = offsets[t];
+    int64_t cnt = 0;
+    for_partners<DIM, KD>(P, g, start, end, sorted, P.n_inactive + t, kd_radius, dmax,
+                          [&](int64_t j) { pair_j[base + cnt++] = (int32_t)j; });
+    // insertion sort of this particle's partners by j
+    for (int64_t a = 1; a < cnt; ++a) {
+        const int32_t key = pair_j[base + a];
+        int64_t b = a - 1;
+        while (b >= 0 && pair_j[base + b] > key) {
+            pair_j[base + b + 1] = pair_j[base + b];
+            --b;
+        }
+        pair_j[base + b + 1] = key;
+    }
+}
+
+template <int DIM>
+__global__ void __launch_bounds__(kBlock) k_obstacle_flags(Particles P, double dmax, int* __restrict__ flags) {
+    const int64_t na = P.n - P.n_inactive;
+    const int64_t t = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (t >= na * P.n_inactive) return;
+    const int64_t o = t / na, j = P.n_inactive + (t - o * na);
+    CP cp;
+    flags[t] = (closest_points<DIM>(P, o, j, cp) && cp.dij < dmax) ? 1 : 0;
+}
+
+__global__ void __launch_bounds__(kBlock) k_obstacle_emit(int64_t n_inactive, int64_t na, const int* __restrict__ flags,
+                                                         const int* __restrict__ offs, Contacts C) {
+    const int64_t t = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (t >= na * n_inactive || !flags[t]) return;
+    const int64_t o = t / na;
+    const int64_t pos = offs[t];
+    C.i[pos] = (int32_t)o;
+    C.j[pos] = (int32_t)(n_inactive + (t - o * na));
+}
+
+__global__ void __launch_bounds__(kBlock) k_pair_scatter_i(int64_t n_inactive, int64_t na, int64_t nc_obs,
+                                                          const int* __restrict__ counts,
+                                                          const int* __restrict__ offsets,
+                                                          const int32_t* __restrict__ pair_j, Contacts C) {
+    const int64_t t = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (t >= na) return;
+    const int64_t b = offsets[t], cnt = counts[t];
+    for (int64_t k = 0; k < cnt; ++k) {
+        C.i[nc_obs + b + k] = (int32_t)(n_inactive + t);
+        C.j[nc_obs + b + k] = pair_j[b + k];
+    }
+}
+
+// compute_exact_distance (contact/base.hpp:103-141): geometry of every emitted pair.
+template <int DIM>
+__global__ void __launch_bounds__(kBlock) k_contact_geometry(Particles P, Contacts C, int64_t nc, double mu) {
+    const int64_t c = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (c >= nc) return;
+    CP cp;
+    closest_points<DIM>(P, C.i[c], C.j[c], cp);
+    C.dij[c] = cp.dij;
+    store3(C.nij, C.cap, c, cp.nij);
+    store3(C.pi, C.cap, c, cp.pi);
+    store3(C.pj, C.cap, c, cp.pj);
+    C.mu[c] = mu;
+}
+
+// ---------------------------------------------------------------- host
+void contacts_free(Contacts& C) {
+    for (void* p : {(void*)C.i, (void*)C.j, (void*)C.dij, (void*)C.nij, (void*)C.pi, (void*)C.pj, (void*)C.mu})
+        if (p) (void)hipFree(p);
+    C = Contacts{};
+}
+
+void contacts_reserve(Contacts& C, int64_t n) {
+    if (n <= C.cap) return;
+    contacts_free(C);
+    const int64_t cap = std::max<int64_t>(n + n / 4, 1024);
+    C.cap = cap;
+    SCOPI_HIP_CHECK(hipMalloc(&C.i, cap * sizeof(int32_t)));
+    SCOPI_HIP_CHECK(hipMalloc(&C.j, cap * sizeof(int32_t)));
+    SCOPI_HIP_CHECK(hipMalloc(&C.dij, cap * sizeof(double)));
+    SCOPI_HIP_CHECK(hipMalloc(&C.nij, 3 * cap * sizeof(double)));
+    SCOPI_HIP_CHECK(hipMalloc(&C.pi, 3 * cap * sizeof(double)));
+    SCOPI_HIP_CHECK(hipMalloc(&C.pj, 3 * cap * sizeof(double)));
+    SCOPI_HIP_CHECK(hipMalloc(&C.mu, cap * sizeof(double)));
+}
+
+ContactSearch::~ContactSearch() {
+    for (void* p : {(void*)d_grid_, (void*)d_partial_, (void*)keys_, (void*)vals_, (void*)counts_, (void*)offsets_,
+                    (void*)obst_flags_, (void*)obst_offsets_, (void*)cell_start_, (void*)cell_end_, (void*)pair_j_,
+                    (void*)d_totals_})
+        if (p) (void)hipFree(p);
+    if (h_totals_) (void)hipHostFree(h_totals_);
+}
+
+void ContactSearch::reserve_particles(int64_t na, int64_t n_inactive) {
+    if (!d_grid_) {
+        SCOPI_HIP_CHECK(hipMalloc(&d_grid_, sizeof(GridInfo)));
+        SCOPI_HIP_CHECK(hipMalloc(&d_totals_, 2 * sizeof(int)));
+        SCOPI_HIP_CHECK(hipHostMalloc(&h_totals_, 2 * sizeof(int)));
+        SCOPI_HIP_CHECK(hipMalloc(&cell_start_, kMaxCells * sizeof(int)));
+        SCOPI_HIP_CHECK(hipMalloc(&cell_end_, kMaxCells * sizeof(int)));
+        cells_cap_ = kMaxCells;
+    }
+    if (na + 1 > pcap_) {
+        for (void* p : {(void*)keys_, (void*)vals_, (void*)counts_, (void*)offsets_})
+            if (p) SCOPI_HIP_CHECK(hipFree(p));
+        pcap_ = na + na / 4 + 1024;
+        SCOPI_HIP_CHECK(hipMalloc(&keys_, pcap_ * sizeof(uint32_t)));
+        SCOPI_HIP_CHECK(hipMalloc(&vals_, pcap_ * sizeof(int32_t)));
+        SCOPI_HIP_CHECK(hipMalloc(&counts_, pcap_ * sizeof(int)));
+        SCOPI_HIP_CHECK(hipMalloc(&offsets_, pcap_ * sizeof(int)));
+        const int64_t nparts = std::min<int64_t>(grid_for(pcap_), 1024);
+        if (nparts * 7 > partial_cap_) {
+            if (d_partial_) SCOPI_HIP_CHECK(hipFree(d_partial_));
+            partial_cap_ = nparts * 7;
+            SCOPI_HIP_CHECK(hipMalloc(&d_partial_, partial_cap_ * sizeof(double)));
+        }
+    }
+    const int64_t no = na * n_inactive + 1;
+    if (no > ocap_) {
+        if (obst_flags_) SCOPI_HIP_CHECK(hipFree(obst_flags_));
+        if (obst_offsets_) SCOPI_HIP_CHECK(hipFree(obst_offsets_));
+        ocap_ = no + no / 4 + 1024;
+        SCOPI_HIP_CHECK(hipMalloc(&obst_flags_, ocap_ * sizeof(int)));
+        SCOPI_HIP_CHECK(hipMalloc(&obst_offsets_, ocap_ * sizeof(int)));
+    }
+}
+
+void ContactSearch::reserve_pairs(int64_t n) {
+    if (n <= pair_cap_) return;
+    if (pair_j_) SCOPI_HIP_CHECK(hipFree(pair_j_));
+    pair_cap_ = n + n / 4 + 1024;
+    SCOPI_HIP_CHECK(hipMalloc(&pair_j_, pair_cap_ * sizeof(int32_t)));
+}
+
+int64_t ContactSearch::run(const Particles& P, Contacts& C, const scopi_hip_params& prm, hipStream_t s) {
+    const int64_t na = P.n - P.n_inactive, ni = P.n_inactive;
+    const bool kd = prm.contact_method == SCOPI_CONTACT_KDTREE;
+    reserve_particles(na, ni);
+    int64_t nc_aa = 0;
+    if (na > 1) {
+        const unsigned nparts = (unsigned)std::min<int64_t>(grid_for(na), 1024);
+        hipLaunchKernelGGL(k_bbox_partial, dim3(nparts), dim3(kBlock), 0, s, P, d_partial_);
+        hipLaunchKernelGGL(k_grid_setup, dim3(1), dim3(64), 0, s, d_partial_, (int)nparts, P.dim, kd ? 1 : 0,
+                           prm.kd_tree_radius, prm.dmax, 1, d_grid_);
+        if (P.dim == 3)
+            hipLaunchKernelGGL(k_cell_keys<3>, dim3(grid_for(na)), dim3(kBlock), 0, s, P, d_grid_, keys_, vals_);
+        else
+            hipLaunchKernelGGL(k_cell_keys<2>, dim3(grid_for(na)), dim3(kBlock), 0, s, P, d_grid_, keys_, vals_);
+        const int key_bits = (P.dim == 3) ? 3 * kMaxBits3 : 2 * kMaxBits2;
+        sorter_.sort(keys_, vals_, na, key_bits, s, scanner_);
+        const int64_t ncells = (P.dim == 3) ? (int64_t(1) << (3 * kMaxBits3)) : (int64_t(1) << (2 * kMaxBits2));
+        hipLaunchKernelGGL(k_fill_int, dim3(1024), dim3(kBlock), 0, s, cell_start_, ncells, 0);
+        hipLaunchKernelGGL(k_fill_int, dim3(1024), dim3(kBlock), 0, s, cell_end_, ncells, 0);
+        hipLaunchKernelGGL(k_cell_range, dim3(grid_for(na)), dim3(kBlock), 0, s, keys_, na, cell_start_, cell_end_);
+#define SCOPI_PAIR_LAUNCH(KERNEL, ...)                                                                         \
+    do {                                                                                                       \
+        if (P.dim == 3 && kd)                                                                                  \
+            hipLaunchKernelGGL((KERNEL<3, true>), dim3(grid_for(na)), dim3(kBlock), 0, s, __VA_ARGS__);       \
+        else if (P.dim == 3)                                                                                   \
+            hipLaunchKernelGGL((KERNEL<3, false>), dim3(grid_for(na)), dim3(kBlock), 0, s, __VA_ARGS__);      \
+        else if (kd)                                                                                           \
+            hipLaunchKernelGGL((KERNEL<2, true>), dim3(grid_for(na)), dim3(kBlock), 0, s, __VA_ARGS__);       \
+        else                                                                                                   \
+            hipLaunchKernelGGL((KERNEL<2, false>), dim3(grid_for(na)), dim3(kBlock), 0, s, __VA_ARGS__);      \
+    } while (0)
+        SCOPI_PAIR_LAUNCH(k_pair_count, P, d_grid_, cell_start_, cell_end_, vals_, prm.kd_tree_radius, prm.dmax,
+                          counts_);
+        scanner_.exclusive(counts_, offsets_, na, d_totals_, s);
+    } else {
+        hipLaunchKernelGGL(k_fill_int, dim3(1), dim3(64), 0, s, d_totals_, 1, 0);
+    }
+    const int64_t no = na * ni;
+    if (no > 0) {
+        if (P.dim == 3)
+            hipLaunchKernelGGL(k_obstacle_flags<3>, dim3(grid_for(no)), dim3(kBlock), 0, s, P, prm.dmax, obst_flags_);
+        else
+            hipLaunchKernelGGL(k_obstacle_flags<2>, dim3(grid_for(no)), dim3(kBlock), 0, s, P, prm.dmax, obst_flags_);
+        scanner_.exclusive(obst_flags_, obst_offsets_, no, d_totals_ + 1, s);
+    } else {
+        hipLaunchKernelGGL(k_fill_int, dim3(1), dim3(64), 0, s, d_totals_ + 1, 1, 0);
+    }
+    SCOPI_HIP_CHECK(hipMemcpyAsync(h_totals_, d_totals_, 2 * sizeof(int), hipMemcpyDeviceToHost, s));
+    SCOPI_HIP_CHECK(hipStreamSynchronize(s));
+    nc_aa = (na > 1) ? h_totals_[0] : 0;
+    nc_obs_ = (no > 0) ? h_totals_[1] : 0;
+    const int64_t nc = nc_aa + nc_obs_;
+    contacts_reserve(C, nc);
+    if (no > 0 && nc_obs_ > 0)
+        hipLaunchKernelGGL(k_obstacle_emit, dim3(grid_for(no)), dim3(kBlock), 0, s, ni, na, obst_flags_,
+                           obst_offsets_, C);
+    if (nc_aa > 0) {
+        reserve_pairs(nc_aa);
+        SCOPI_PAIR_LAUNCH(k_pair_emit, P, d_grid_, cell_start_, cell_end_, vals_, prm.kd_tree_radius, prm.dmax,
+                          offsets_, pair_j_);
+        hipLaunchKernelGGL(k_pair_scatter_i, dim3(grid_for(na)), dim3(kBlock), 0, s, ni, na, nc_obs_, counts_,
+                           offsets_, pair_j_, C);
+    }
+#undef SCOPI_PAIR_LAUNCH
+    if (nc > 0) {
+        if (P.dim == 3)
+            hipLaunchKernelGGL(k_contact_geometry<3>, dim3(grid_for(nc)), dim3(kBlock), 0, s, P, C, nc, prm.mu);
+        else
+            hipLaunchKernelGGL(k_contact_geometry<2>, dim3(grid_for(nc)), dim3(kBlock), 0, s, P, C, nc, prm.mu);
+    }
+    SCOPI_HIP_CHECK(hipGetLastError());
+    return nc;
+}
+
+}  // namespace scopi_hip

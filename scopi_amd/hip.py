@@ -1,0 +1,223 @@
+"""ctypes binding of libscopi_hip.so (include/scopi_hip.h).
+
+This is the product path's Python face: it loads the in-tree HIP library and
+raises if it is missing or if a call fails -- there is no CPU fallback.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libscopi_hip.so")
+
+SHAPE_SPHERE, SHAPE_PLANE, SHAPE_SEGMENT, SHAPE_SUPERELLIPSOID = 0, 1, 2, 3
+NOFRICTION, FRICTION = 0, 1
+KDTREE, BRUTE_FORCE = 0, 1
+VAP_FIXED, VAP_FPD = 0, 1
+APGD, PGD = 0, 1
+FP64, FP32_REFINE = 0, 1
+
+# every symbol include/scopi_hip.h declares
+EXPORTS = [
+    "scopi_hip_default_params", "scopi_hip_create", "scopi_hip_destroy", "scopi_hip_last_error",
+    "scopi_hip_upload_state", "scopi_hip_download_state", "scopi_hip_upload_velocities",
+    "scopi_hip_compute_contacts", "scopi_hip_download_contacts", "scopi_hip_upload_contacts",
+    "scopi_hip_set_a_priori_velocity", "scopi_hip_solve", "scopi_hip_download_solution",
+    "scopi_hip_update_and_move", "scopi_hip_displacement_obstacles", "scopi_hip_run",
+    "scopi_hip_set_kernel_timing", "scopi_hip_kernel_timing",
+]
+
+
+class Params(C.Structure):
+    _fields_ = [
+        ("problem", C.c_int32), ("contact_method", C.c_int32), ("vap", C.c_int32), ("method", C.c_int32),
+        ("precision", C.c_int32), ("dynamic_descent", C.c_int32), ("dmax", C.c_double),
+        ("kd_tree_radius", C.c_double), ("alpha", C.c_double), ("tolerance", C.c_double), ("mu", C.c_double),
+        ("max_ite", C.c_int64), ("fp32_tolerance", C.c_double), ("warm_start", C.c_int32), ("reserved", C.c_int32),
+    ]
+
+
+class SolveInfo(C.Structure):
+    _fields_ = [
+        ("n_contacts", C.c_int64), ("rows", C.c_int64), ("iterations", C.c_int64), ("iterations_fp32", C.c_int64),
+        ("backtracks", C.c_int64), ("q_applies", C.c_int64), ("converged", C.c_int32), ("status", C.c_int32),
+        ("residual", C.c_double),
+    ]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+_lib = None
+
+
+class ScopiHipError(RuntimeError):
+    pass
+
+
+def lib():
+    """Load libscopi_hip.so; raises if it has not been built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise ScopiHipError(f"{LIB_PATH} is missing: build it with `python -m scopi_amd.build`")
+    L = C.CDLL(LIB_PATH)
+    P = C.c_void_p
+    D = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
+    I32 = np.ctypeslib.ndpointer(dtype=np.int32, flags="C_CONTIGUOUS")
+    I64 = np.ctypeslib.ndpointer(dtype=np.int64, flags="C_CONTIGUOUS")
+    Dn = C.c_void_p  # nullable double*
+    L.scopi_hip_default_params.argtypes = [C.POINTER(Params)]
+    L.scopi_hip_create.argtypes = [C.c_int32, C.c_int64, C.c_int64, C.POINTER(P)]
+    L.scopi_hip_destroy.argtypes = [P]
+    L.scopi_hip_last_error.argtypes = [P]
+    L.scopi_hip_last_error.restype = C.c_char_p
+    L.scopi_hip_upload_state.argtypes = [P, C.c_int64, C.c_int64, I32, D, D, D, D, D, D, D, D, D, D, D]
+    L.scopi_hip_download_state.argtypes = [P, Dn, Dn, Dn, Dn]
+    L.scopi_hip_upload_velocities.argtypes = [P, Dn, Dn]
+    L.scopi_hip_compute_contacts.argtypes = [P, C.POINTER(Params), C.POINTER(C.c_int64)]
+    L.scopi_hip_download_contacts.argtypes = [P, I64, I64, D, D, D, D]
+    L.scopi_hip_upload_contacts.argtypes = [P, C.c_int64, I64, I64, D, D, D, D, Dn]
+    L.scopi_hip_set_a_priori_velocity.argtypes = [P, C.POINTER(Params), C.c_double]
+    L.scopi_hip_solve.argtypes = [P, C.POINTER(Params), C.c_double, C.POINTER(SolveInfo)]
+    L.scopi_hip_download_solution.argtypes = [P, Dn, Dn, Dn]
+    L.scopi_hip_update_and_move.argtypes = [P, C.c_double]
+    L.scopi_hip_displacement_obstacles.argtypes = [P, C.c_double]
+    L.scopi_hip_run.argtypes = [P, C.POINTER(Params), C.c_double, C.c_int64, C.c_int64, C.POINTER(SolveInfo)]
+    L.scopi_hip_set_kernel_timing.argtypes = [P, C.c_int32]
+    L.scopi_hip_kernel_timing.argtypes = [P, D]
+    for name in [n for n in dir(L) if n.startswith("scopi_hip_") and n != "scopi_hip_last_error"]:
+        getattr(L, name).restype = C.c_int
+    _lib = L
+    return L
+
+
+def params(**kw) -> Params:
+    p = Params()
+    lib().scopi_hip_default_params(C.byref(p))
+    for k, v in kw.items():
+        if not hasattr(p, k):
+            raise AttributeError(k)
+        setattr(p, k, v)
+    return p
+
+
+def _ptr(a):
+    return None if a is None else a.ctypes.data_as(C.c_void_p)
+
+
+class HipScene:
+    """One scene resident on the current GPU (a scopi_hip_ctx)."""
+
+    def __init__(self, scene, capacity_contacts=0):
+        self._L = lib()
+        self._h = C.c_void_p()
+        rc = self._L.scopi_hip_create(scene.dim, scene.n, capacity_contacts, C.byref(self._h))
+        if rc != 0:
+            raise ScopiHipError(f"scopi_hip_create failed ({rc})")
+        self.dim = scene.dim
+        self.n = scene.n
+        self.n_inactive = scene.n_inactive
+        self.upload(scene)
+
+    def close(self):
+        if getattr(self, "_h", None) and self._h.value:
+            self._L.scopi_hip_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        self.close()
+
+    def _check(self, rc, what):
+        if rc < 0:
+            msg = self._L.scopi_hip_last_error(self._h).decode()
+            raise ScopiHipError(f"{what} failed ({rc}): {msg}")
+        return rc
+
+    def upload(self, scene):
+        f = lambda a: np.ascontiguousarray(a, np.float64)
+        self._check(self._L.scopi_hip_upload_state(
+            self._h, scene.n, scene.n_inactive, np.ascontiguousarray(scene.shape, np.int32), f(scene.pos), f(scene.q),
+            f(scene.size), f(scene.squareness), f(scene.v), f(scene.vd), f(scene.omega), f(scene.domega), f(scene.f),
+            f(scene.m), f(scene.J)), "upload_state")
+        self.n = scene.n
+        self.n_inactive = scene.n_inactive
+
+    def state(self):
+        n = self.n
+        pos, q, v, om = np.zeros((n, 3)), np.zeros((n, 4)), np.zeros((n, 3)), np.zeros((n, 3))
+        self._check(self._L.scopi_hip_download_state(self._h, _ptr(pos), _ptr(q), _ptr(v), _ptr(om)), "download_state")
+        return dict(pos=pos, q=q, v=v, omega=om)
+
+    def upload_velocities(self, v=None, omega=None):
+        v = None if v is None else np.ascontiguousarray(v, np.float64)
+        omega = None if omega is None else np.ascontiguousarray(omega, np.float64)
+        self._check(self._L.scopi_hip_upload_velocities(self._h, _ptr(v), _ptr(omega)), "upload_velocities")
+
+    def compute_contacts(self, prm: Params) -> int:
+        nc = C.c_int64()
+        self._check(self._L.scopi_hip_compute_contacts(self._h, C.byref(prm), C.byref(nc)), "compute_contacts")
+        self.nc = nc.value
+        return nc.value
+
+    def contacts(self):
+        nc = self.nc
+        ci = np.zeros(max(nc, 1), np.int64); cj = np.zeros(max(nc, 1), np.int64); dij = np.zeros(max(nc, 1))
+        nij = np.zeros((max(nc, 1), 3)); pi = np.zeros((max(nc, 1), 3)); pj = np.zeros((max(nc, 1), 3))
+        self._check(self._L.scopi_hip_download_contacts(self._h, ci, cj, dij, nij, pi, pj), "download_contacts")
+        return dict(ij=np.stack([ci[:nc], cj[:nc]], axis=1), dij=dij[:nc], nij=nij[:nc], pi=pi[:nc], pj=pj[:nc])
+
+    def upload_contacts(self, ij, dij, nij, pi, pj, mu=None):
+        nc = int(len(dij))
+        ij = np.asarray(ij, np.int64).reshape(nc, 2)
+        f = lambda a, w: np.ascontiguousarray(np.asarray(a, np.float64).reshape(max(nc, 1) if nc else 1, w)
+                                               if nc else np.zeros((1, w)))
+        ci = np.ascontiguousarray(ij[:, 0]) if nc else np.zeros(1, np.int64)
+        cj = np.ascontiguousarray(ij[:, 1]) if nc else np.zeros(1, np.int64)
+        mu_a = None if mu is None else np.ascontiguousarray(mu, np.float64)
+        self._check(self._L.scopi_hip_upload_contacts(self._h, nc, ci, cj, np.ascontiguousarray(dij, np.float64)
+                                                      if nc else np.zeros(1), f(nij, 3), f(pi, 3), f(pj, 3),
+                                                      _ptr(mu_a)), "upload_contacts")
+        self.nc = nc
+
+    def set_a_priori_velocity(self, prm: Params, dt):
+        self._check(self._L.scopi_hip_set_a_priori_velocity(self._h, C.byref(prm), float(dt)), "set_a_priori_velocity")
+
+    def solve(self, prm: Params, dt):
+        info = SolveInfo()
+        self._check(self._L.scopi_hip_solve(self._h, C.byref(prm), float(dt), C.byref(info)), "solve")
+        self.last_rows = info.rows
+        return info.as_dict()
+
+    def solution(self):
+        na = self.n - self.n_inactive
+        u = np.zeros((max(na, 1), 3)); w = np.zeros((max(na, 1), 3))
+        rows = getattr(self, "last_rows", 0)
+        lam = np.zeros(max(rows, 1))
+        self._check(self._L.scopi_hip_download_solution(self._h, _ptr(u), _ptr(w), _ptr(lam)), "download_solution")
+        return dict(u=u[:na], w=w[:na], lam=lam[:rows])
+
+    def update_and_move(self, dt):
+        self._check(self._L.scopi_hip_update_and_move(self._h, float(dt)), "update_and_move")
+
+    def displacement_obstacles(self, dt):
+        self._check(self._L.scopi_hip_displacement_obstacles(self._h, float(dt)), "displacement_obstacles")
+
+    def run(self, prm: Params, dt, total_it, initial_iter=0):
+        info = SolveInfo()
+        self._check(self._L.scopi_hip_run(self._h, C.byref(prm), float(dt), int(total_it), int(initial_iter),
+                                          C.byref(info)), "run")
+        return info.as_dict()
+
+    def set_kernel_timing(self, on=True):
+        self._check(self._L.scopi_hip_set_kernel_timing(self._h, 1 if on else 0), "set_kernel_timing")
+
+    def kernel_timing(self):
+        out = np.zeros(6)
+        self._check(self._L.scopi_hip_kernel_timing(self._h, out), "kernel_timing")
+        return dict(bt_ms=out[0], bt_launches=int(out[1]), row_ms=out[2], row_launches=int(out[3]),
+                    trial_ms=out[4], trial_launches=int(out[5]))

@@ -1,0 +1,241 @@
+"""Particle containers and the synthetic workloads of the contact-step benchmarks.
+
+`Container` mirrors `scopi_container<dim>::push_back(object, property)`
+(include/scopi/container.hpp:338-384): obstacles (inactive objects) must be
+pushed before active particles, and every per-particle field is stored as a
+structure of arrays.  Conventions shared with the C-ABI (include/scopi_hip.h):
+
+* vectors use 3 slots even in 2D; the 2D rotation velocity and moment of
+  inertia sit in slot 2, the slot the reference's 6-dof velocity vector uses
+  (include/scopi/solvers/minimization_problem.hpp:15-29);
+* quaternions are scalar-first (include/scopi/quaternion.hpp:86-124);
+* `size[:, 0]` is a sphere radius or a segment length; superellipsoids use
+  `size[:, 0:3]` (semi-axes) and `squareness[:, 0:2]` = (e, n).
+"""
+from __future__ import annotations
+
+import math
+from dataclasses import dataclass, field
+
+import numpy as np
+
+SPHERE, PLANE, SEGMENT, SUPERELLIPSOID = 0, 1, 2, 3
+
+
+def quaternion(angle: float, axis=(0.0, 0.0, 1.0)) -> np.ndarray:
+    """quaternion(angle, axis) of src/quaternion.cpp:12-26 (normalised, scalar first)."""
+    h = angle / 2.0
+    q = np.array([math.cos(h), math.sin(h) * axis[0], math.sin(h) * axis[1], math.sin(h) * axis[2]])
+    return q / math.sqrt(float(np.dot(q, q)))
+
+
+@dataclass
+class Property:
+    """property<dim> (include/scopi/property.hpp:160-199); all fields default to zero, mass to 1."""
+
+    velocity: tuple = (0.0, 0.0, 0.0)
+    desired_velocity: tuple = (0.0, 0.0, 0.0)
+    omega: tuple = (0.0, 0.0, 0.0)
+    desired_omega: tuple = (0.0, 0.0, 0.0)
+    force: tuple = (0.0, 0.0, 0.0)
+    mass: float = 1.0
+    moment_inertia: tuple = (0.0, 0.0, 0.0)
+    active: bool = True
+
+
+def _vec3(x, dim, rot=False):
+    """Place a dim-vector (or a 2D scalar rotation quantity) into 3 slots."""
+    if np.isscalar(x):
+        out = np.zeros(3)
+        out[2 if rot else 0] = float(x)
+        return out
+    x = np.asarray(x, dtype=np.float64).ravel()
+    out = np.zeros(3)
+    out[: x.size] = x
+    return out
+
+
+@dataclass
+class Container:
+    dim: int
+    shape: list = field(default_factory=list)
+    pos: list = field(default_factory=list)
+    q: list = field(default_factory=list)
+    size: list = field(default_factory=list)
+    squareness: list = field(default_factory=list)
+    v: list = field(default_factory=list)
+    vd: list = field(default_factory=list)
+    omega: list = field(default_factory=list)
+    domega: list = field(default_factory=list)
+    f: list = field(default_factory=list)
+    m: list = field(default_factory=list)
+    J: list = field(default_factory=list)
+    n_inactive: int = 0
+
+    def _push(self, shape, pos, q, size, sq, prop: Property):
+        if not prop.active and self.n_inactive != len(self.shape):
+            raise RuntimeError("All the obstacles must be pushed before the active particles.")
+        self.shape.append(shape)
+        self.pos.append(_vec3(pos, self.dim))
+        self.q.append(np.asarray(q, dtype=np.float64))
+        self.size.append(_vec3(size, 3))
+        self.squareness.append(np.asarray(sq, dtype=np.float64))
+        self.v.append(_vec3(prop.velocity, self.dim))
+        self.vd.append(_vec3(prop.desired_velocity, self.dim))
+        self.omega.append(_vec3(prop.omega, self.dim, rot=True))
+        self.domega.append(_vec3(prop.desired_omega, self.dim, rot=True))
+        self.f.append(_vec3(prop.force, self.dim))
+        self.m.append(float(prop.mass))
+        self.J.append(_vec3(prop.moment_inertia, self.dim, rot=True))
+        if not prop.active:
+            self.n_inactive += 1
+
+    def sphere(self, pos, radius, prop=None, q=None):
+        """sphere<dim>(pos, radius) (include/scopi/objects/types/sphere.hpp:139-151)."""
+        self._push(SPHERE, pos, quaternion(0.0) if q is None else q, (radius, 0, 0), (1.0, 1.0), prop or Property())
+
+    def plane(self, pos, angle=0.0, prop=None, q=None):
+        """plane<dim>(pos, angle) / plane<dim>(pos, q) (include/scopi/objects/types/plane.hpp:122-140)."""
+        self._push(PLANE, pos, quaternion(angle) if q is None else q, (0, 0, 0), (1.0, 1.0),
+                   prop or Property(active=False))
+
+    def segment(self, pos, length, angle=0.0, prop=None, q=None):
+        """segment<2>(pos, q, length) (include/scopi/objects/types/segment.hpp:135-141)."""
+        self._push(SEGMENT, pos, quaternion(angle) if q is None else q, (length, 0, 0), (1.0, 1.0),
+                   prop or Property(active=False))
+
+    def superellipsoid(self, pos, q, radius, squareness, prop=None):
+        """superellipsoid<3>(pos, q, radius(3), squareness(e, n)) (superellipsoid.hpp)."""
+        self._push(SUPERELLIPSOID, pos, q, radius, squareness, prop or Property())
+
+    def arrays(self) -> "Scene":
+        n = len(self.shape)
+        def a(x, w):
+            return np.ascontiguousarray(np.asarray(x, dtype=np.float64).reshape(n, w))
+        return Scene(
+            dim=self.dim, n_inactive=self.n_inactive,
+            shape=np.asarray(self.shape, dtype=np.int32).reshape(n),
+            pos=a(self.pos, 3), q=a(self.q, 4), size=a(self.size, 3), squareness=a(self.squareness, 2),
+            v=a(self.v, 3), vd=a(self.vd, 3), omega=a(self.omega, 3), domega=a(self.domega, 3),
+            f=a(self.f, 3), m=np.asarray(self.m, dtype=np.float64).reshape(n), J=a(self.J, 3))
+
+
+@dataclass
+class Scene:
+    """Structure-of-arrays particle state (obstacles first)."""
+
+    dim: int
+    n_inactive: int
+    shape: np.ndarray
+    pos: np.ndarray
+    q: np.ndarray
+    size: np.ndarray
+    squareness: np.ndarray
+    v: np.ndarray
+    vd: np.ndarray
+    omega: np.ndarray
+    domega: np.ndarray
+    f: np.ndarray
+    m: np.ndarray
+    J: np.ndarray
+
+    @property
+    def n(self) -> int:
+        return int(self.shape.shape[0])
+
+    @property
+    def n_active(self) -> int:
+        return self.n - self.n_inactive
+
+    def copy(self) -> "Scene":
+        return Scene(self.dim, self.n_inactive, *(getattr(self, k).copy() for k in (
+            "shape", "pos", "q", "size", "squareness", "v", "vd", "omega", "domega", "f", "m", "J")))
+
+
+# ----------------------------------------------------------------------------- workloads
+
+def box_planes_3d(lo, hi):
+    """Floor z=lo[2] and four walls x=lo/hi, y=lo/hi as planes (normal = first column of R(q)).
+
+    Plane orientations follow demos/box_spheres_3d.cpp:34-63 (quaternion(angle, axis)).
+    """
+    ez, ey = (0.0, 0.0, 1.0), (0.0, 1.0, 0.0)
+    return [
+        ((0.5 * (lo[0] + hi[0]), 0.5 * (lo[1] + hi[1]), lo[2]), quaternion(-math.pi / 2, ey)),  # floor, n=(0,0,1)
+        ((lo[0], 0.0, 0.0), quaternion(0.0, ez)),                                               # x = lo
+        ((hi[0], 0.0, 0.0), quaternion(0.0, ez)),                                               # x = hi
+        ((0.0, lo[1], 0.0), quaternion(math.pi / 2, ez)),                                       # y = lo
+        ((0.0, hi[1], 0.0), quaternion(math.pi / 2, ez)),                                       # y = hi
+    ]
+
+
+def disks_2d(n_side=32, spacing=0.025, jitter=0.002, rmin=0.008, rmax=0.012, seed=1234):
+    """1,024 disks on a jittered 32x32 lattice above the plane y=0 (gravity (0,-9.81) via vap_fpd).
+
+    The first lattice row's centre height is `spacing` (0.025) above the plane.
+    """
+    rng = np.random.default_rng(seed)
+    c = Container(2)
+    c.plane((0.0, 0.0), math.pi / 2)  # normal (0, 1)
+    for i in range(n_side):
+        for j in range(n_side):
+            r = rng.uniform(rmin, rmax)
+            x = (i + 0.5) * spacing + rng.uniform(-jitter, jitter)
+            y = (j + 1.0) * spacing + rng.uniform(-jitter, jitter)
+            c.sphere((x, y), r, Property(mass=1.0, moment_inertia=0.1, force=(0.0, -9.81)))
+    return c.arrays()
+
+
+def spheres_3d(nx=64, ny=64, nz=32, spacing=0.025, jitter=0.003, rmin=0.009, rmax=0.011, seed=4321,
+               velocity_scale=0.0):
+    """Jittered nx*ny*nz lattice of spheres in a box of 5 planes (floor + 4 walls).
+
+    Lattice site (i, j, k) sits at ((i+0.5)s, (j+0.5)s, (k+0.5)s) + jitter with the floor at z=0
+    and the walls half a spacing outside the lattice.  Gravity (0,0,-9.81)*m via vap_fpd, m=1,
+    J=(0.1,0.1,0.1).  `velocity_scale` > 0 adds a seeded uniform initial velocity in
+    [-velocity_scale, velocity_scale]^3 per sphere (0 for the configuration as specified).
+    """
+    rng = np.random.default_rng(seed)
+    n = nx * ny * nz
+    lo = (0.0, 0.0, 0.0)
+    hi = (nx * spacing, ny * spacing, nz * spacing + 1.0)
+    planes = box_planes_3d(lo, hi)
+    ni = len(planes)
+    N = ni + n
+    shape = np.empty(N, np.int32)
+    pos = np.zeros((N, 3))
+    q = np.zeros((N, 4))
+    size = np.zeros((N, 3))
+    shape[:ni] = PLANE
+    for k, (p, qq) in enumerate(planes):
+        pos[k] = p
+        q[k] = qq
+    ii, jj, kk = np.meshgrid(np.arange(nx), np.arange(ny), np.arange(nz), indexing="ij")
+    base = np.stack([(ii.ravel() + 0.5), (jj.ravel() + 0.5), (kk.ravel() + 0.5)], axis=1) * spacing
+    pos[ni:] = base + rng.uniform(-jitter, jitter, size=(n, 3))
+    size[ni:, 0] = rng.uniform(rmin, rmax, size=n)
+    shape[ni:] = SPHERE
+    q[ni:, 0] = 1.0
+    v = np.zeros((N, 3))
+    if velocity_scale > 0:
+        v[ni:] = rng.uniform(-velocity_scale, velocity_scale, size=(n, 3))
+    f = np.zeros((N, 3))
+    f[ni:, 2] = -9.81
+    m = np.ones(N)
+    J = np.zeros((N, 3))
+    J[ni:] = 0.1
+    return Scene(dim=3, n_inactive=ni, shape=shape, pos=pos, q=q, size=size, squareness=np.ones((N, 2)),
+                 v=v, vd=np.zeros((N, 3)), omega=np.zeros((N, 3)), domega=np.zeros((N, 3)), f=f, m=m, J=J)
+
+
+def spheres_3d_count(n_particles, seed=0, **kw):
+    """A C2-recipe scene with about `n_particles` spheres (lattice nx=ny=2*nz, scaled to the count)."""
+    nz = max(1, int(round((n_particles / 4.0) ** (1.0 / 3.0))))
+    nx = max(1, int(round(math.sqrt(n_particles / nz))))
+    ny = max(1, int(math.ceil(n_particles / (nx * nz))))
+    return spheres_3d(nx=nx, ny=ny, nz=nz, seed=seed, **kw)
+
+
+# Default solver parameters of the workloads (BASELINE configurations).
+CONFIG1 = dict(dim=2, dt=0.005, dmax=0.025, kd_tree_radius=(0.024 + 0.025) ** 2, tolerance=1e-6, steps=200)
+CONFIG2 = dict(dim=3, dt=1e-3, dmax=0.02, kd_tree_radius=0.042 ** 2, tolerance=1e-6)

@@ -1,0 +1,94 @@
+"""Pins the CPU restatement to the reference's own golden outputs.
+
+test/references/*.json are compared on "objects" with the tolerances of the
+reference tests (diffFile, test/utils.cpp:14-75: absolute, 1e-6 in test_sphere /
+test_obstacles via test/utils.hpp:15, 1e-5 in test_gradient.cpp:19).
+"""
+import math
+
+import numpy as np
+import pytest
+
+from tests import fixture_scenes as fs
+
+CASES = {
+    "two_spheres_asymmetrical": (lambda o: fs.two_spheres(True), 1e-6),
+    "two_spheres_symmetrical": (lambda o: fs.two_spheres(False), 1e-6),
+    "2d_case_spheres": (lambda o: fs.case_2d_spheres(o.gen_2d_case_spheres), 1e-6),
+    "obstacles_sphere_sphere_moving": (lambda o: fs.obstacles_sphere_sphere_moving(), 1e-6),
+    "3spheres_nofriction": (lambda o: fs.three_spheres_nofriction(), 1e-5),
+    "sphere_plane_friction_mu01": (lambda o: fs.sphere_plane_friction_mu01(), 1e-5),
+}
+
+
+@pytest.mark.parametrize("contact_method", [0, 1], ids=["kdtree", "brute_force"])
+@pytest.mark.parametrize("name", list(CASES))
+def test_fixture_objects(oracle_lib, name, contact_method):
+    build, tol = CASES[name]
+    case = build(oracle_lib)
+    o = oracle_lib.Oracle(case["scene"], vap=case["vap"], contact_method=contact_method, **case["params"])
+    o.run(case["dt"], case["total_it"])
+    err = fs.compare_objects(fs.load_fixture(case["fixture"]), 2, o.state(), case["scene"].n, tol)
+    assert err <= tol, f"{name}: max |objects - reference| = {err}"
+
+
+def test_fixture_contacts_2d_case(oracle_lib):
+    """The "contacts" of 2d_case_spheres.json are the last step's contacts, computed
+    before the last move: run 19 steps, then compute contacts on that state."""
+    case = fs.case_2d_spheres(oracle_lib.gen_2d_case_spheres)
+    o = oracle_lib.Oracle(case["scene"], vap=case["vap"])
+    o.run(case["dt"], case["total_it"] - 1)
+    o.compute_contacts()
+    mine = o.contacts()
+    ref = fs.load_fixture("2d_case_spheres")["contacts"]
+    assert len(ref) == len(mine["dij"]) == 117
+    for k, c in enumerate(ref):
+        assert (c["i"], c["j"]) == tuple(mine["ij"][k])
+        assert abs(c["distance"] - mine["dij"][k]) <= 1e-6
+        np.testing.assert_allclose(mine["nij"][k, :2], c["normal"], atol=1e-6)
+        np.testing.assert_allclose(mine["pi"][k, :2], c["pi"], atol=1e-6)
+        np.testing.assert_allclose(mine["pj"][k, :2], c["pj"], atol=1e-6)
+
+
+@pytest.mark.parametrize("alpha,e_pos,e_q,e_v", fs.INCLINED_EXPECTED)
+def test_inclined_plane_expected_errors(oracle_lib, alpha, e_pos, e_q, e_v):
+    """test_obstacles.cpp:256-313 with doctest::Approx semantics (scale 1)."""
+    case = fs.inclined_plane(alpha)
+    o = oracle_lib.Oracle(case["scene"], vap=case["vap"])
+    o.run(case["dt"], case["total_it"])
+    st = o.state()
+    t = 0.01 * 1001
+    pa, th = fs.analytical_solution_sphere_plane(alpha, 0.0, t, 1.0, 1.0, 2.0)
+    va, _ = fs.analytical_solution_sphere_plane_velocity(alpha, 0.0, t, 1.0, 1.0, 2.0)
+    qa = np.array([math.cos(th / 2), 0.0, 0.0, math.sin(th / 2)])
+    err_pos = np.linalg.norm(st["pos"][1, :2] - pa) / np.linalg.norm(pa)
+    err_q = np.linalg.norm(st["q"][1] - qa) / np.linalg.norm(qa)
+    err_v = np.linalg.norm(st["v"][1, :2] - va) / np.linalg.norm(va)
+
+    def approx(x, ref, eps=np.finfo(np.float32).eps * 100):
+        return abs(x - ref) < eps * (1.0 + max(abs(x), abs(ref)))
+
+    assert approx(err_pos, e_pos)
+    assert approx(err_q, e_q)
+    assert approx(err_v, e_v, 1e-4)
+    assert abs(st["omega"][1, 2]) < 1e-5
+
+
+def test_sphere_plane_without_friction_analytic(oracle_lib):
+    """test_gradient.cpp:23-84: the velocity matches the analytic one to 1e-12."""
+    from scopi_amd.scenes import Container, Property
+    radius, g, dt, total_it, h, alpha = 1.0, 1.0, 0.01, 1000, 2.0, math.pi / 4
+    c = Container(2)
+    c.plane((0.0, 0.0), math.pi / 2)
+    c.sphere((0.0, h), radius, Property(mass=1, moment_inertia=radius * radius / 2.0,
+                                        force=(g * math.sin(alpha), -g * math.cos(alpha))))
+    o = oracle_lib.Oracle(c.arrays(), vap=1, tolerance=1e-7, max_ite=10000, alpha=0.01, dynamic_descent=1)
+    o.run(dt, total_it)
+    st = o.state()
+    t = dt * total_it
+    t_impact = math.sqrt(2 * (h - radius) / (g * math.cos(alpha)))
+    v_an = np.array([g * t * math.sin(alpha), 0.0 if t >= t_impact else -g * t * math.cos(alpha)])
+    x_an = np.array([g * t * t * math.sin(alpha) / 2, radius])
+    assert np.linalg.norm(st["v"][1, :2] - v_an) / np.linalg.norm(v_an) <= 1e-12
+    assert np.linalg.norm(st["pos"][1, :2] - x_an) / np.linalg.norm(x_an) <= 1e-3
+    assert abs(st["omega"][1, 2]) < 1e-12
