@@ -34,5 +34,19 @@ def build(verbose: bool = False) -> str:
     return LIB
 
 
+def build_host_tests() -> str:
+    """C++ program that drives the slot API mirror (scopi_amd/host) like the reference's tests."""
+    root = os.path.dirname(HERE)
+    src = os.path.join(root, "tests", "cpp", "test_slots.cpp")
+    out = os.path.join(root, "tests", "cpp", "test_slots")
+    cmd = ["g++", "-O2", "-std=c++17", "-I", os.path.join(HERE, "host"), src, "-L", HERE, "-lscopi_hip",
+           "-Wl,-rpath,$ORIGIN/../../scopi_amd", "-o", out]
+    res = subprocess.run(cmd, capture_output=True, text=True)
+    if res.returncode != 0:
+        sys.stderr.write(res.stdout + res.stderr)
+        raise RuntimeError("g++ failed building tests/cpp/test_slots")
+    return out
+
+
 if __name__ == "__main__":
     print(build(verbose=True))

@@ -27,6 +27,8 @@
 namespace scopi_hip {
 
 constexpr int kRedBlocksMax = 2048;
+constexpr int kStatusStalled = 2;                // FP32 phase reached its precision floor
+constexpr int kMaxConsecutiveRejections32 = 40;  // 2^40 growth of L without an accepted step
 
 // ------------------------------------------------------------------ reductions
 template <int NV>
@@ -391,6 +393,10 @@ __device__ inline void row_product(const BMatrix<T>& B, int64_t c, int64_t i, in
 }
 
 // Control logic of apgd.hpp:172-209 / pgd apgd.hpp:68-89 on the reduced sums.
+// In the FP32 phase (low_precision) the loop also stops when the line search
+// can no longer resolve a decrease (repeated rejections, or lambda+ == y):
+// that is the precision floor of FP32, and the FP64 phase takes over from the
+// last iterate with the last accepted L.
 __device__ inline void apgd_control(ApgdCtrl* ctrl) {
     const double* r = ctrl->red;
     const double f_trial = 0.5 * r[7] + r[6];
@@ -399,6 +405,11 @@ __device__ inline void apgd_control(ApgdCtrl* ctrl) {
     ctrl->q_applies += 1;
     if (ctrl->method == SCOPI_METHOD_APGD && ctrl->dynamic) {
         if (r[1] == 0.0) {
+            if (ctrl->low_precision) {
+                ctrl->done = 1;
+                ctrl->status = kStatusStalled;
+                return;
+            }
             // lambda+ == y: the reference's test holds with equality, L doubles
             // until it overflows to +inf, alpha becomes 0 and the NaN right-hand
             // side ends the loop with lambda+ = Proj(y) = y (apgd.hpp:174-181).
@@ -411,6 +422,11 @@ __device__ inline void apgd_control(ApgdCtrl* ctrl) {
                 ctrl->L *= 2;
                 ctrl->alpha = 1. / ctrl->L;
                 ctrl->backtracks += 1;
+                ctrl->consec_bt += 1;
+                if (ctrl->low_precision && ctrl->consec_bt > kMaxConsecutiveRejections32) {
+                    ctrl->done = 1;
+                    ctrl->status = kStatusStalled;
+                }
                 // Doubling reaches +inf after ~1100 rejections; beyond that only a
                 // non-finite objective can keep rejecting, so stop instead of looping.
                 if (ctrl->backtracks > 64 * (ctrl->max_ite + 1100)) {
@@ -421,6 +437,8 @@ __device__ inline void apgd_control(ApgdCtrl* ctrl) {
         }
     }
     if (!accept) return;
+    ctrl->consec_bt = 0;
+    if (isfinite(ctrl->L)) ctrl->L_acc = ctrl->L;
     ctrl->iter += 1;
     ctrl->res = sqrt(r[3]);
     const int old_prev = ctrl->prev;
@@ -517,12 +535,15 @@ __global__ void __launch_bounds__(kBlock) k_convert(const TA* __restrict__ a, TB
 }
 
 __global__ void k_ctrl_init(ApgdCtrl* ctrl, double alpha, double L, double tol, double rtol, double dt2,
-                            int64_t max_ite, int method, int dynamic, int keep_L) {
+                            int64_t max_ite, int method, int dynamic, int keep_L, int low_precision) {
     if (threadIdx.x != 0 || blockIdx.x != 0) return;
-    if (keep_L) {
-        L = ctrl->L;
+    if (keep_L && isfinite(ctrl->L_acc) && ctrl->L_acc > 0) {
+        L = ctrl->L_acc;
         alpha = 1. / L;
     }
+    ctrl->L_acc = L;
+    ctrl->consec_bt = 0;
+    ctrl->low_precision = low_precision;
     ctrl->L = L;
     ctrl->alpha = alpha;
     ctrl->theta = 1.0;
@@ -689,7 +710,7 @@ void Solver::run_phase(BMatrix<T>& B, const Particles& P, const Contacts& C, int
         info.q_applies += 1;
     }
     hipLaunchKernelGGL(k_ctrl_init, dim3(1), dim3(64), 0, s, ctrl_, prm.alpha, 1. / prm.alpha, tol, rtol, dt2, max_ite,
-                       prm.method, prm.dynamic_descent ? 1 : 0, keep_L ? 1 : 0);
+                       prm.method, prm.dynamic_descent ? 1 : 0, keep_L ? 1 : 0, sizeof(T) == 4 ? 1 : 0);
     SCOPI_HIP_CHECK(hipGetLastError());
     std::vector<hipEvent_t> ev;
     if (timing_) {
@@ -735,8 +756,8 @@ void Solver::run_phase(BMatrix<T>& B, const Particles& P, const Contacts& C, int
     info.iterations += h_ctrl_->iter;
     info.backtracks += h_ctrl_->backtracks;
     info.q_applies += h_ctrl_->q_applies;
-    info.status = h_ctrl_->status;
-    info.converged = h_ctrl_->status == 0 ? 1 : 0;
+    info.status = h_ctrl_->status == kStatusStalled ? 0 : h_ctrl_->status;
+    info.converged = info.status == 0 ? 1 : 0;
     info.residual = h_ctrl_->res;
     final_slot_ = h_ctrl_->cur;
 }

@@ -11,12 +11,13 @@ namespace scopi_hip {
 // (include/scopi/solvers/apgd.hpp:164-213) on it, so the host only polls `done`.
 struct ApgdCtrl {
     double L, alpha, theta, beta;
+    double L_acc;        // L of the last accepted trial (before the 0.97 decay)
     double tol, rtol, dt2;
     double res;          // last ||lambda_{k+1} - lambda_k||
     double red[9];       // sums of the trial kernel (0..6, 8) and of the row kernel (7)
-    int64_t iter, backtracks, max_ite, q_applies;
+    int64_t iter, backtracks, max_ite, q_applies, consec_bt;
     int32_t cur, prev, trial;
-    int32_t done, status, method, dynamic, pad;
+    int32_t done, status, method, dynamic, low_precision;
     uint32_t count_trial, count_row;
 };
 

@@ -1,0 +1,594 @@
+// scopi_hip_host.hpp -- C++ host mirror of SCoPI's per-step slot API over libscopi_hip.so.
+//
+// The names, template slots and parameter fields follow the reference
+// (hpc-maths/scopi v0.1.0) so that scene set-ups written for it compile against
+// these headers:
+//   ScopiSolver<dim, problem_t, optim_solver_t, contact_method_t, vap_t>  include/scopi/solver.hpp:91-107
+//   scopi_container<dim>::push_back(object, property)                     include/scopi/container.hpp:338-384
+//   property<dim> fluent setters                                           include/scopi/property.hpp
+//   sphere / plane / segment constructors                                  include/scopi/objects/types/*.hpp
+//   contact_kdtree / contact_brute_force (ContactsParams: dmax, kd_tree_radius)
+//                                                                          include/scopi/contact/contact_kdtree.hpp:27-61
+//   OptimGradient<apgd | pgd> (apgd_params: alpha, max_ite, tolerance, dynamic_descent)
+//                                                                          include/scopi/solvers/apgd.hpp:20-124
+//   vap_fixed / vap_fpd                                                    include/scopi/vap/*.hpp
+//   NoFriction / Friction, contact_property<Friction>::mu                  include/scopi/contact/property.hpp
+//   neighbor<dim, problem_t>                                               include/scopi/objects/neighbor.hpp:23-70
+// Every slot runs on the GPU through the C ABI (include/scopi_hip.h).  When the
+// contact, optim and vap slots are the ones defined here, ScopiSolver::run keeps
+// the whole time loop on the device (one upload, one download); otherwise it
+// calls the slots one by one as the reference's run() does (solver.hpp:266-303).
+// Vectors are plain fixed arrays (type::position_t etc.) instead of xtensor types.
+#pragma once
+
+#include <array>
+#include <cmath>
+#include <cstddef>
+#include <cstdint>
+#include <initializer_list>
+#include <stdexcept>
+#include <string>
+#include <type_traits>
+#include <vector>
+
+#include "../../../include/scopi_hip.h"
+
+namespace scopi {
+
+constexpr double PI = 3.14159265358979323846;
+
+// ------------------------------------------------------------------ types
+template <std::size_t N>
+struct fixed {
+    std::array<double, N> a{};
+    double& operator()(std::size_t i) { return a[i]; }
+    double operator()(std::size_t i) const { return a[i]; }
+    double& operator[](std::size_t i) { return a[i]; }
+    double operator[](std::size_t i) const { return a[i]; }
+    static constexpr std::size_t size() { return N; }
+};
+
+namespace type {
+template <std::size_t dim>
+using position_t = fixed<dim>;
+template <std::size_t dim>
+using velocity_t = fixed<dim>;
+template <std::size_t dim>
+using force_t = fixed<dim>;
+template <std::size_t dim>
+using moment_t = typename std::conditional<dim == 2, double, fixed<3>>::type;
+template <std::size_t dim>
+using rotation_t = typename std::conditional<dim == 2, double, fixed<3>>::type;
+using quaternion_t = fixed<4>;
+}  // namespace type
+
+// quaternion(angle, axis): src/quaternion.cpp:12-36 (scalar first, normalised)
+inline type::quaternion_t quaternion(double angle, const fixed<3>& axes) {
+    type::quaternion_t q{{std::cos(angle / 2), std::sin(angle / 2) * axes[0], std::sin(angle / 2) * axes[1],
+                          std::sin(angle / 2) * axes[2]}};
+    double n = std::sqrt(q[0] * q[0] + q[1] * q[1] + q[2] * q[2] + q[3] * q[3]);
+    for (auto& c : q.a) c /= n;
+    return q;
+}
+inline type::quaternion_t quaternion(double angle = 0.0) { return quaternion(angle, fixed<3>{{0.0, 0.0, 1.0}}); }
+
+// ------------------------------------------------------------------ problem tags
+struct NoFriction {};
+struct Friction {};
+
+template <class problem_t>
+struct contact_property {};
+template <>
+struct contact_property<Friction> {
+    double mu = 0.1;
+};
+
+template <std::size_t dim, class problem_t_>
+struct neighbor {
+    using problem_t = problem_t_;
+    std::size_t i = 0, j = 0;
+    double dij = 0;
+    fixed<dim> nij, pi, pj;
+    double sij = 0;
+    contact_property<problem_t> property;
+};
+
+// ------------------------------------------------------------------ property and objects
+template <std::size_t dim>
+class property {
+  public:
+    using velocity_type = type::velocity_t<dim>;
+    using rotation_type = type::rotation_t<dim>;
+    using force_type = type::force_t<dim>;
+    using moment_type = type::moment_t<dim>;
+    property& velocity(const velocity_type& v) { m_v = v; return *this; }
+    property& desired_velocity(const velocity_type& v) { m_dv = v; return *this; }
+    property& omega(const rotation_type& w) { m_w = w; return *this; }
+    property& desired_omega(const rotation_type& w) { m_dw = w; return *this; }
+    property& force(const force_type& f) { m_f = f; return *this; }
+    property& mass(double m) { m_m = m; return *this; }
+    property& moment_inertia(const moment_type& j) { m_j = j; return *this; }
+    property& activate() { m_active = true; return *this; }
+    property& deactivate() { m_active = false; return *this; }
+    const velocity_type& velocity() const { return m_v; }
+    const velocity_type& desired_velocity() const { return m_dv; }
+    const rotation_type& omega() const { return m_w; }
+    const rotation_type& desired_omega() const { return m_dw; }
+    const force_type& force() const { return m_f; }
+    double mass() const { return m_m; }
+    const moment_type& moment_inertia() const { return m_j; }
+    bool is_active() const { return m_active; }
+
+  private:
+    velocity_type m_v{}, m_dv{};
+    rotation_type m_w{}, m_dw{};
+    force_type m_f{};
+    double m_m = 1.0;
+    moment_type m_j{};
+    bool m_active = true;
+};
+
+template <std::size_t dim>
+struct object {
+    int shape;
+    type::position_t<dim> pos;
+    type::quaternion_t q;
+    std::array<double, 3> size;
+};
+
+template <std::size_t dim>
+struct sphere : object<dim> {
+    sphere(type::position_t<dim> pos, double radius) : object<dim>{SCOPI_SHAPE_SPHERE, pos, quaternion(0.0), {radius, 0, 0}} {}
+    sphere(type::position_t<dim> pos, type::quaternion_t q, double radius)
+        : object<dim>{SCOPI_SHAPE_SPHERE, pos, q, {radius, 0, 0}} {}
+    double radius() const { return this->size[0]; }
+};
+
+template <std::size_t dim>
+struct plane : object<dim> {
+    explicit plane(type::position_t<dim> pos, double angle = 0)
+        : object<dim>{SCOPI_SHAPE_PLANE, pos, quaternion(angle), {0, 0, 0}} {}
+    plane(type::position_t<dim> pos, type::quaternion_t q) : object<dim>{SCOPI_SHAPE_PLANE, pos, q, {0, 0, 0}} {}
+};
+
+template <std::size_t dim>
+struct segment : object<dim> {
+    segment(type::position_t<dim> pos, type::quaternion_t q, double length)
+        : object<dim>{SCOPI_SHAPE_SEGMENT, pos, q, {length, 0, 0}} {}
+};
+
+// ------------------------------------------------------------------ container
+template <std::size_t dim>
+class scopi_container {
+  public:
+    static constexpr std::size_t dimension = dim;
+    void push_back(const object<dim>& s, const property<dim>& p = property<dim>()) {
+        if (!p.is_active() && m_nb_inactive != size())
+            throw std::runtime_error("All the obstacles must be pushed before the active particles.");
+        m_shape.push_back(s.shape);
+        m_pos.push_back(s.pos);
+        m_q.push_back(s.q);
+        m_size.push_back(s.size);
+        m_v.push_back(p.velocity());
+        m_vd.push_back(p.desired_velocity());
+        m_w.push_back(p.omega());
+        m_dw.push_back(p.desired_omega());
+        m_f.push_back(p.force());
+        m_m.push_back(p.mass());
+        m_j.push_back(p.moment_inertia());
+        if (!p.is_active()) ++m_nb_inactive;
+    }
+    std::size_t size() const { return m_pos.size(); }
+    std::size_t nb_active() const { return size() - m_nb_inactive; }
+    std::size_t nb_inactive() const { return m_nb_inactive; }
+    std::vector<type::position_t<dim>>& pos() { return m_pos; }
+    std::vector<type::quaternion_t>& q() { return m_q; }
+    std::vector<type::velocity_t<dim>>& v() { return m_v; }
+    std::vector<type::velocity_t<dim>>& vd() { return m_vd; }
+    std::vector<type::rotation_t<dim>>& omega() { return m_w; }
+    std::vector<type::rotation_t<dim>>& desired_omega() { return m_dw; }
+    std::vector<type::force_t<dim>>& f() { return m_f; }
+    std::vector<double>& m() { return m_m; }
+    std::vector<type::moment_t<dim>>& j() { return m_j; }
+    const std::vector<type::position_t<dim>>& pos() const { return m_pos; }
+    const std::vector<type::quaternion_t>& q() const { return m_q; }
+    const std::vector<type::velocity_t<dim>>& v() const { return m_v; }
+    const std::vector<type::rotation_t<dim>>& omega() const { return m_w; }
+
+    // structure of arrays in the C-ABI convention (3 slots, 2D rotation in slot 2)
+    struct soa {
+        std::vector<int32_t> shape;
+        std::vector<double> pos, q, size, v, vd, w, dw, f, m, J;
+    };
+    soa pack() const {
+        soa s;
+        const std::size_t n = size();
+        s.shape.assign(m_shape.begin(), m_shape.end());
+        s.pos.assign(3 * n, 0.0), s.q.assign(4 * n, 0.0), s.size.assign(3 * n, 0.0);
+        s.v.assign(3 * n, 0.0), s.vd.assign(3 * n, 0.0), s.w.assign(3 * n, 0.0), s.dw.assign(3 * n, 0.0);
+        s.f.assign(3 * n, 0.0), s.m.assign(n, 0.0), s.J.assign(3 * n, 0.0);
+        for (std::size_t k = 0; k < n; ++k) {
+            for (std::size_t d = 0; d < dim; ++d) {
+                s.pos[3 * k + d] = m_pos[k][d];
+                s.v[3 * k + d] = m_v[k][d];
+                s.vd[3 * k + d] = m_vd[k][d];
+                s.f[3 * k + d] = m_f[k][d];
+            }
+            for (int d = 0; d < 4; ++d) s.q[4 * k + d] = m_q[k][d];
+            for (int d = 0; d < 3; ++d) s.size[3 * k + d] = m_size[k][d];
+            rot_in(s.w, k, m_w[k]);
+            rot_in(s.dw, k, m_dw[k]);
+            rot_in(s.J, k, m_j[k]);
+            s.m[k] = m_m[k];
+        }
+        return s;
+    }
+    void unpack(const std::vector<double>& pos, const std::vector<double>& q, const std::vector<double>& v,
+                const std::vector<double>& w) {
+        for (std::size_t k = 0; k < size(); ++k) {
+            for (std::size_t d = 0; d < dim; ++d) {
+                m_pos[k][d] = pos[3 * k + d];
+                m_v[k][d] = v[3 * k + d];
+            }
+            for (int d = 0; d < 4; ++d) m_q[k][d] = q[4 * k + d];
+            rot_out(w, k, m_w[k]);
+        }
+    }
+
+  private:
+    static void rot_in(std::vector<double>& a, std::size_t k, const double& x) { a[3 * k + 2] = x; }
+    static void rot_in(std::vector<double>& a, std::size_t k, const fixed<3>& x) {
+        for (int d = 0; d < 3; ++d) a[3 * k + d] = x[d];
+    }
+    static void rot_out(const std::vector<double>& a, std::size_t k, double& x) { x = a[3 * k + 2]; }
+    static void rot_out(const std::vector<double>& a, std::size_t k, fixed<3>& x) {
+        for (int d = 0; d < 3; ++d) x[d] = a[3 * k + d];
+    }
+    std::vector<int> m_shape;
+    std::vector<type::position_t<dim>> m_pos;
+    std::vector<type::quaternion_t> m_q;
+    std::vector<std::array<double, 3>> m_size;
+    std::vector<type::velocity_t<dim>> m_v, m_vd;
+    std::vector<type::rotation_t<dim>> m_w, m_dw;
+    std::vector<type::force_t<dim>> m_f;
+    std::vector<double> m_m;
+    std::vector<type::moment_t<dim>> m_j;
+    std::size_t m_nb_inactive = 0;
+};
+
+template <std::size_t dim>
+class BoxDomain {
+  public:
+    BoxDomain() = default;
+    BoxDomain(const std::array<double, dim>& lo, const std::array<double, dim>& hi) : m_lo(lo), m_hi(hi) {}
+    bool is_periodic(std::size_t) const { return false; }
+
+  private:
+    std::array<double, dim> m_lo{}, m_hi{};
+};
+
+// ------------------------------------------------------------------ device context
+class hip_error : public std::runtime_error {
+  public:
+    using std::runtime_error::runtime_error;
+};
+
+namespace detail {
+inline void check(int rc, scopi_hip_ctx* ctx, const char* what) {
+    if (rc < 0) throw hip_error(std::string(what) + ": " + (ctx ? scopi_hip_last_error(ctx) : "error"));
+}
+
+// One device context per slot object; a scene is uploaded when its size or the
+// slot's view of it is stale.
+struct device_scene {
+    scopi_hip_ctx* ctx = nullptr;
+    int dim = 0;
+    ~device_scene() {
+        if (ctx) scopi_hip_destroy(ctx);
+    }
+    template <std::size_t dim_>
+    void upload(const scopi_container<dim_>& p) {
+        if (!ctx) {
+            check(scopi_hip_create((int)dim_, (int64_t)p.size(), 0, &ctx), nullptr, "scopi_hip_create");
+            dim = (int)dim_;
+        }
+        auto s = p.pack();
+        check(scopi_hip_upload_state(ctx, (int64_t)p.size(), (int64_t)p.nb_inactive(), s.shape.data(), s.pos.data(),
+                                     s.q.data(), s.size.data(), nullptr, s.v.data(), s.vd.data(), s.w.data(),
+                                     s.dw.data(), s.f.data(), s.m.data(), s.J.data()),
+              ctx, "scopi_hip_upload_state");
+    }
+};
+}  // namespace detail
+
+// ------------------------------------------------------------------ contact slot
+template <class D>
+struct ContactsParams {
+    double dmax{2.};
+    double kd_tree_radius{17.};
+    void init_options() {}
+};
+
+template <class problem_t, int METHOD>
+class contact_hip {
+  public:
+    using params_t = ContactsParams<contact_hip>;
+    static constexpr int method = METHOD;
+    explicit contact_hip(const params_t& params = params_t()) : m_params(params) {}
+    void init_options() {}
+    params_t& get_params() { return m_params; }
+    contact_property<problem_t>& default_contact_property() { return m_default; }
+
+    // contact_kdtree::run_impl (contact_kdtree.hpp:200-268): sorted (i, j), i < j.
+    template <std::size_t dim>
+    std::vector<neighbor<dim, problem_t>> run(const BoxDomain<dim>&, scopi_container<dim>& particles) {
+        m_dev.upload(particles);
+        scopi_hip_params prm;
+        scopi_hip_default_params(&prm);
+        prm.contact_method = METHOD;
+        prm.dmax = m_params.dmax;
+        prm.kd_tree_radius = m_params.kd_tree_radius;
+        int64_t nc = 0;
+        detail::check(scopi_hip_compute_contacts(m_dev.ctx, &prm, &nc), m_dev.ctx, "scopi_hip_compute_contacts");
+        std::vector<int64_t> i(nc), j(nc);
+        std::vector<double> dij(nc), nij(3 * nc), pi(3 * nc), pj(3 * nc);
+        detail::check(scopi_hip_download_contacts(m_dev.ctx, i.data(), j.data(), dij.data(), nij.data(), pi.data(),
+                                                  pj.data()),
+                      m_dev.ctx, "scopi_hip_download_contacts");
+        std::vector<neighbor<dim, problem_t>> out(nc);
+        for (int64_t c = 0; c < nc; ++c) {
+            out[c].i = (std::size_t)i[c];
+            out[c].j = (std::size_t)j[c];
+            out[c].dij = dij[c];
+            for (std::size_t d = 0; d < dim; ++d) {
+                out[c].nij[d] = nij[3 * c + d];
+                out[c].pi[d] = pi[3 * c + d];
+                out[c].pj[d] = pj[3 * c + d];
+            }
+            out[c].property = m_default;
+        }
+        return out;
+    }
+    template <std::size_t dim>
+    std::vector<neighbor<dim, problem_t>> run(scopi_container<dim>& particles) {
+        return run(BoxDomain<dim>(), particles);
+    }
+
+  private:
+    params_t m_params;
+    contact_property<problem_t> m_default;
+    detail::device_scene m_dev;
+};
+
+template <class problem_t>
+using contact_kdtree = contact_hip<problem_t, SCOPI_CONTACT_KDTREE>;
+template <class problem_t>
+using contact_brute_force = contact_hip<problem_t, SCOPI_CONTACT_BRUTE_FORCE>;
+
+// ------------------------------------------------------------------ optim slot
+struct apgd_params {
+    double alpha = 0.05;
+    std::size_t max_ite = 10000;
+    double tolerance = 1e-7;
+    bool dynamic_descent = true;
+    int precision = SCOPI_PRECISION_FP64;  // SCOPI_PRECISION_FP32_REFINE: FP32 phase + FP64 refinement
+    void init_options() {}
+};
+struct pgd_params {
+    double alpha = 0.05;
+    std::size_t max_ite = 10000;
+    double tolerance = 1e-6;
+    int precision = SCOPI_PRECISION_FP64;
+    void init_options() {}
+};
+struct apgd {
+    using params_t = apgd_params;
+    static constexpr int method = SCOPI_METHOD_APGD;
+};
+struct pgd {
+    using params_t = pgd_params;
+    static constexpr int method = SCOPI_METHOD_PGD;
+};
+
+template <class problem_t>
+constexpr int problem_tag() {
+    return std::is_same<problem_t, Friction>::value ? SCOPI_PROBLEM_FRICTION : SCOPI_PROBLEM_NOFRICTION;
+}
+
+template <class Method>
+class OptimGradient {
+  public:
+    using method_t = Method;
+    using params_t = typename Method::params_t;
+    void set_timestep(double dt) { m_dt = dt; }
+    void init_options() {}
+    params_t& get_params() { return m_params; }
+    template <class C>
+    void extra_steps_before_solve(C&) { m_should_solve = true; }
+    bool should_solve() const { return m_should_solve; }
+    template <class C, std::size_t dim>
+    void extra_steps_after_solve(C&, const scopi_container<dim>&) { m_should_solve = false; }
+    template <class C>
+    void update_contact_properties(C&) {}
+
+    scopi_hip_params hip_params(int problem, double mu) const {
+        scopi_hip_params prm;
+        scopi_hip_default_params(&prm);
+        prm.problem = problem;
+        prm.method = Method::method;
+        prm.alpha = m_params.alpha;
+        prm.max_ite = (int64_t)m_params.max_ite;
+        prm.tolerance = m_params.tolerance;
+        prm.precision = m_params.precision;
+        prm.mu = mu;
+        if constexpr (std::is_same<Method, apgd>::value) prm.dynamic_descent = m_params.dynamic_descent ? 1 : 0;
+        return prm;
+    }
+
+    // OptimGradient::run (OptimGradient.hpp:252-306) on the GPU.
+    template <std::size_t dim, class problem_t>
+    void run(const scopi_container<dim>& particles, const std::vector<neighbor<dim, problem_t>>& contacts, std::size_t) {
+        m_dev.upload(particles);
+        const int64_t nc = (int64_t)contacts.size();
+        std::vector<int64_t> i(nc), j(nc);
+        std::vector<double> dij(nc), nij(3 * nc, 0.0), pi(3 * nc, 0.0), pj(3 * nc, 0.0), mu(nc, 0.1);
+        for (int64_t c = 0; c < nc; ++c) {
+            i[c] = (int64_t)contacts[c].i;
+            j[c] = (int64_t)contacts[c].j;
+            dij[c] = contacts[c].dij;
+            for (std::size_t d = 0; d < dim; ++d) {
+                nij[3 * c + d] = contacts[c].nij[d];
+                pi[3 * c + d] = contacts[c].pi[d];
+                pj[3 * c + d] = contacts[c].pj[d];
+            }
+            if constexpr (std::is_same<problem_t, Friction>::value) mu[c] = contacts[c].property.mu;
+        }
+        detail::check(scopi_hip_upload_contacts(m_dev.ctx, nc, i.data(), j.data(), dij.data(), nij.data(), pi.data(),
+                                                pj.data(), mu.data()),
+                      m_dev.ctx, "scopi_hip_upload_contacts");
+        auto prm = hip_params(problem_tag<problem_t>(), 0.1);
+        detail::check(scopi_hip_solve(m_dev.ctx, &prm, m_dt, &m_info), m_dev.ctx, "scopi_hip_solve");
+        const std::size_t na = particles.nb_active();
+        m_u.assign(3 * na, 0.0);
+        m_omega.assign(3 * na, 0.0);
+        m_lambda.assign(problem_tag<problem_t>() == SCOPI_PROBLEM_FRICTION ? 3 * nc : nc, 0.0);
+        detail::check(scopi_hip_download_solution(m_dev.ctx, m_u.data(), m_omega.data(), m_lambda.data()), m_dev.ctx,
+                      "scopi_hip_download_solution");
+    }
+    // get_uadapt / get_wadapt: Na x 3, row-major (index 3 * i + d)
+    const std::vector<double>& get_uadapt() const { return m_u; }
+    const std::vector<double>& get_wadapt() const { return m_omega; }
+    const std::vector<double>& lagrange_multiplier() const { return m_lambda; }
+    const scopi_hip_solve_info& last_info() const { return m_info; }
+
+  private:
+    double m_dt = 0;
+    bool m_should_solve = false;
+    params_t m_params;
+    std::vector<double> m_u, m_omega, m_lambda;
+    scopi_hip_solve_info m_info{};
+    detail::device_scene m_dev;
+};
+
+// ------------------------------------------------------------------ vap slot
+struct vap_fixed {
+    static constexpr int kind = SCOPI_VAP_FIXED;
+    struct params_t {};
+    params_t& get_params() { return m_params; }
+    params_t m_params;
+    // vap/vap_fixed.hpp:46-55
+    template <std::size_t dim, class C>
+    void set_a_priori_velocity(double, scopi_container<dim>& p, const C&) {
+        for (std::size_t i = p.nb_inactive(); i < p.size(); ++i) {
+            p.v()[i] = p.vd()[i];
+            p.omega()[i] = p.desired_omega()[i];
+        }
+    }
+};
+struct vap_fpd {
+    static constexpr int kind = SCOPI_VAP_FPD;
+    struct params_t {};
+    params_t& get_params() { return m_params; }
+    params_t m_params;
+    // vap/vap_fpd.hpp:70-82, src/vap/vap_fpd.cpp:6-26
+    template <std::size_t dim, class C>
+    void set_a_priori_velocity(double dt, scopi_container<dim>& p, const C&) {
+        for (std::size_t i = p.nb_inactive(); i < p.size(); ++i) {
+            for (std::size_t d = 0; d < dim; ++d) p.v()[i][d] += dt * p.f()[i][d] / p.m()[i];
+            if constexpr (dim == 3) {
+                auto& w = p.omega()[i];
+                const auto& J = p.j()[i];
+                const double r0 = w[1] * w[2] * (J[2] - J[1]), r1 = w[0] * w[2] * (J[0] - J[2]),
+                             r2 = w[0] * w[1] * (J[1] - J[0]);
+                w[0] += r0, w[1] += r1, w[2] += r2;
+            }
+        }
+    }
+};
+
+// ------------------------------------------------------------------ driver
+struct ScopiParams {
+    std::size_t output_frequency = std::size_t(-1);
+    std::string path = "./Results/";
+    std::string filename = "scopi_objects_";
+    void init_options() {}
+};
+
+template <class Solver>
+struct Params {
+    ScopiParams& solver_params;
+    typename Solver::optim_solver_t::params_t& optim_params;
+    contact_property<typename Solver::problem_t>& default_contact_property;
+    typename Solver::contact_method_t::params_t& contact_method_params;
+};
+
+template <std::size_t dim, class problem_type = NoFriction, class optim_solver_type = OptimGradient<apgd>,
+          template <class> class contact_method_type = contact_kdtree, class vap_type = vap_fixed>
+class ScopiSolver {
+  public:
+    using problem_t = problem_type;
+    using optim_solver_t = optim_solver_type;
+    using contact_method_t = contact_method_type<problem_t>;
+    using vap_t = vap_type;
+    using params_t = Params<ScopiSolver>;
+    using contact_container_t = std::vector<neighbor<dim, problem_t>>;
+
+    explicit ScopiSolver(scopi_container<dim>& particles) : m_particles(particles) {}
+    ScopiSolver(const BoxDomain<dim>& box, scopi_container<dim>& particles) : m_box(box), m_particles(particles) {}
+    ~ScopiSolver() {
+        if (m_ctx) scopi_hip_destroy(m_ctx);
+    }
+
+    params_t get_params() {
+        return params_t{m_params, m_optim.get_params(), m_contact.default_contact_property(), m_contact.get_params()};
+    }
+
+    // ScopiSolver::run (solver.hpp:266-303), no output files.
+    void run(double dt, std::size_t total_it, std::size_t initial_iter = 0) {
+        m_optim.set_timestep(dt);
+        run_resident(dt, total_it, initial_iter);
+    }
+
+    const scopi_hip_solve_info& last_info() const { return m_info; }
+
+  private:
+    // Every slot is a GPU slot of this library: the whole loop stays on the device.
+    void run_resident(double dt, std::size_t total_it, std::size_t initial_iter) {
+        if (!m_ctx)
+            detail::check(scopi_hip_create((int)dim, (int64_t)m_particles.size(), 0, &m_ctx), nullptr,
+                          "scopi_hip_create");
+        auto s = m_particles.pack();
+        detail::check(scopi_hip_upload_state(m_ctx, (int64_t)m_particles.size(), (int64_t)m_particles.nb_inactive(),
+                                             s.shape.data(), s.pos.data(), s.q.data(), s.size.data(), nullptr,
+                                             s.v.data(), s.vd.data(), s.w.data(), s.dw.data(), s.f.data(), s.m.data(),
+                                             s.J.data()),
+                      m_ctx, "scopi_hip_upload_state");
+        double mu = 0.1;
+        if constexpr (std::is_same<problem_t, Friction>::value) mu = m_contact.default_contact_property().mu;
+        auto prm = m_optim.hip_params(problem_tag<problem_t>(), mu);
+        prm.vap = vap_t::kind;
+        prm.contact_method = contact_method_t::method;
+        prm.dmax = m_contact.get_params().dmax;
+        prm.kd_tree_radius = m_contact.get_params().kd_tree_radius;
+        detail::check(scopi_hip_run(m_ctx, &prm, dt, (int64_t)total_it, (int64_t)initial_iter, &m_info), m_ctx,
+                      "scopi_hip_run");
+        const std::size_t n = m_particles.size();
+        std::vector<double> pos(3 * n), q(4 * n), v(3 * n), w(3 * n);
+        detail::check(scopi_hip_download_state(m_ctx, pos.data(), q.data(), v.data(), w.data()), m_ctx,
+                      "scopi_hip_download_state");
+        m_particles.unpack(pos, q, v, w);
+    }
+
+    ScopiParams m_params;
+    BoxDomain<dim> m_box;
+    scopi_container<dim>& m_particles;
+    optim_solver_t m_optim;
+    contact_method_t m_contact;
+    vap_t m_vap;
+    scopi_hip_ctx* m_ctx = nullptr;
+    scopi_hip_solve_info m_info{};
+};
+
+inline void initialize(const std::string& = "") {}
+
+}  // namespace scopi
