@@ -70,8 +70,9 @@ typedef struct scopi_hip_params {
     double mu;
     int64_t max_ite;
     double fp32_tolerance; /* FP32 phase stop, precision == FP32_REFINE (0: automatic) */
-    int32_t warm_start;    /* 1: start the solve from the previous step's lambda of the same (i,j) */
-    int32_t reserved;
+    int32_t warm_start;       /* reserved, must be 0 (lambda_0 = 0 every step, apgd.hpp:152-159) */
+    int32_t general_jacobian; /* 1: keep the rotational blocks of B even when every active particle is a
+                                 sphere (they vanish for spheres; this only changes speed) */
 } scopi_hip_params;
 
 typedef struct scopi_hip_solve_info {

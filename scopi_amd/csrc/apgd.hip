@@ -5,19 +5,21 @@
 // The reference applies the matrix-free A / A^T (matrix/velocities.hpp:67-254)
 // and recomputes two rotation matrices per contact side on every product.
 // Here B is assembled once per step as a block-row sparse matrix (one block
-// row of 2 particle blocks per contact, column index = particle, values as
-// structure of arrays), and products with Q = dt^2 B M^-1 B^T are two
-// bandwidth-bound passes:
-//   k_bt  : w = M^-1 B^T lambda, one thread per particle, gathering its
-//           contacts through the particle-major transpose (deterministic:
-//           no atomics);
-//   k_row : q = dt^2 B w, one thread per contact, plus the reductions.
+// row per contact with two particle blocks; the column index is the particle,
+// so no column array is stored; layouts in apgd.hpp), plus a particle-major
+// copy of its blocks, and every product with Q = dt^2 B M^-1 B^T is two
+// streaming passes:
+//   k_bt  : w = M^-1 B^T lambda, one thread per particle over its contiguous
+//           entries of the particle-major copy (a gather: deterministic, no atomics);
+//   k_row : q = dt^2 B w, one thread per contact.
 // k_trial forms y, Qy and the gradient lazily from the two previous iterates
 // (Q y = (1+beta) Q lambda_k - beta Q lambda_{k-1}), projects, and reduces the
-// scalars of the line search, so each accepted iteration costs one Q product
-// instead of the reference's three (gradient + two objective evaluations,
-// apgd.hpp:168-181).  The accept / backtrack / restart logic of
-// apgd.hpp:172-209 runs on the device in the last block of k_row.
+// scalars of the line search, so each trial costs one Q product instead of the
+// reference's three (gradient + two objective evaluations, apgd.hpp:168-181).
+// Block sums are plain stores reduced by the next kernel (block 0 of k_bt, and
+// the one-block k_ctrl that also runs the accept / backtrack / restart logic of
+// apgd.hpp:172-209): kernel boundaries order them, so no block ever issues a
+// device-scope fence.
 #include "apgd.hpp"
 
 #include <algorithm>
@@ -27,10 +29,12 @@
 namespace scopi_hip {
 
 constexpr int kRedBlocksMax = 2048;
+constexpr int kPartialStride = 16;
 constexpr int kStatusStalled = 2;                // FP32 phase reached its precision floor
 constexpr int kMaxConsecutiveRejections32 = 40;  // 2^40 growth of L without an accepted step
 
 // ------------------------------------------------------------------ reductions
+// Sum of NV values over the block; the totals land in thread 0.
 template <int NV>
 __device__ inline void block_sum(double (&v)[NV], double (*lds)[kBlock / 64]) {
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
@@ -53,40 +57,19 @@ __device__ inline void block_sum(double (&v)[NV], double (*lds)[kBlock / 64]) {
     __syncthreads();
 }
 
-// Writes this block's NV sums to partial[block*16 + k]; the last block to
-// arrive sums every block's partials in block order (deterministic) and
-// returns true in all its threads with the totals in `tot` (thread 0).
+// Sum of the first NV partials of `nblocks` blocks, in block order (deterministic).
 template <int NV>
-__device__ inline bool last_block_reduce(double (&v)[NV], double* __restrict__ partial, uint32_t* counter,
-                                         double (*lds)[kBlock / 64], double (&tot)[NV]) {
-    __shared__ uint32_t ticket;
-    block_sum<NV>(v, lds);
-    if (threadIdx.x == 0) {
+__device__ inline void reduce_partials(const double* __restrict__ partial, int nblocks, double (*lds)[kBlock / 64],
+                                       double (&tot)[NV]) {
 #pragma unroll
-        for (int k = 0; k < NV; ++k) partial[blockIdx.x * 16 + k] = v[k];
-        __threadfence();
-        ticket = atomicAdd(counter, 1u);
-    }
-    __syncthreads();
-    if (ticket != gridDim.x - 1) return false;
-    __threadfence();
-    double acc[NV];
+    for (int k = 0; k < NV; ++k) tot[k] = 0.0;
+    for (int b = threadIdx.x; b < nblocks; b += kBlock)
 #pragma unroll
-    for (int k = 0; k < NV; ++k) acc[k] = 0.0;
-    for (unsigned b = threadIdx.x; b < gridDim.x; b += kBlock)
-#pragma unroll
-        for (int k = 0; k < NV; ++k) acc[k] += __builtin_nontemporal_load(&partial[b * 16 + k]);
-    block_sum<NV>(acc, lds);
-    if (threadIdx.x == 0) {
-#pragma unroll
-        for (int k = 0; k < NV; ++k) tot[k] = acc[k];
-        *counter = 0u;
-    }
-    return true;
+        for (int k = 0; k < NV; ++k) tot[k] += partial[b * kPartialStride + k];
+    block_sum<NV>(tot, lds);
 }
 
-// ------------------------------------------------------------------ assembly
-template <int DIM>
+// ------------------------------------------------------------------ geometry helpers
 __device__ inline d3 cross3(d3 x, d3 y) {
 #pragma clang fp contract(off)
     return d3{x.y * y.z - x.z * y.y, x.z * y.x - x.x * y.z, x.x * y.y - x.y * y.x};
@@ -102,31 +85,35 @@ __device__ inline d3 mat_mult_t(const m3& a, d3 x) {
               a.a[0][2] * x.x + a.a[1][2] * x.y + a.a[2][2] * x.z};
 }
 
-// M_inverse (solvers/minimization_problem.hpp:10-31), 6 x pcap, slot-major.
-template <class T>
-__global__ void __launch_bounds__(kBlock) k_minv(Particles P, int64_t pcap, T* __restrict__ minv) {
+// ------------------------------------------------------------------ assembly
+// M_inverse (solvers/minimization_problem.hpp:10-31), NW per particle.
+template <class T, int LAY>
+__global__ void __launch_bounds__(kBlock) k_minv(Particles P, T* __restrict__ minv) {
+    constexpr int NW = lay<LAY>::NW;
     const int64_t na = P.n - P.n_inactive;
     const int64_t p = (int64_t)blockIdx.x * kBlock + threadIdx.x;
     if (p >= na) return;
     const int64_t k = P.n_inactive + p;
     const double im = 1. / P.m[k];
-    for (int d = 0; d < 3; ++d) minv[d * pcap + p] = (T)((d < P.dim) ? im : 0.0);
-    for (int d = 0; d < 3; ++d) {
-        double ij = 0.0;
-        if (P.dim == 3 || d == 2) ij = 1. / P.J[d * P.cap + k];
-        minv[(3 + d) * pcap + p] = (T)ij;
-    }
+    for (int d = 0; d < 3; ++d) minv[p * NW + d] = (T)((d < P.dim) ? im : 0.0);
+    if (NW == 6)
+        for (int d = 0; d < 3; ++d) {
+            double ij = 0.0;
+            if (P.dim == 3 || d == 2) ij = 1. / P.J[d * P.cap + k];
+            minv[p * NW + 3 + d] = (T)ij;
+        }
 }
 
-// Coefficients of B and the vector c = g2l(D n + dt A U) (minimization_problem.hpp:33-70,
-// velocities.hpp:67-144, lagrange_multiplier.hpp:62-76).
-template <int DIM, bool FRICTION>
+// Coefficients of B and c = g2l(D n + dt A U) (minimization_problem.hpp:33-70,
+// velocities.hpp:67-144, lagrange_multiplier.hpp:62-76), with U the a-priori velocities.
+template <int DIM, int LAY>
 __global__ void __launch_bounds__(kBlock) k_assemble(Particles P, Contacts C, int64_t nc, double dt,
                                                     BMatrix<double> B64, BMatrix<float> B32, int with32) {
 #pragma clang fp contract(off)
+    constexpr int NC = lay<LAY>::NC, NR = lay<LAY>::NR;
     const int64_t c = (int64_t)blockIdx.x * kBlock + threadIdx.x;
     if (c >= nc) return;
-    const int64_t cap = B64.cap, pc = P.cap;
+    const int64_t pc = P.cap;
     const int64_t i = C.i[c], j = C.j[c];
     const bool ai = i >= P.n_inactive, aj = j >= P.n_inactive;
     const d3 n = load3(C.nij, C.cap, c);
@@ -142,7 +129,7 @@ __global__ void __launch_bounds__(kBlock) k_assemble(Particles P, Contacts C, in
         if (DIM == 2) v.z = 0.0;
         d3 om = load3(P.omega, pc, k);
         if (DIM == 2) om = mk(0.0, 0.0, om.z);
-        d3 cr = cross3<DIM>(r, mat_mult(R, om));
+        d3 cr = cross3(r, mat_mult(R, om));
         return mk(v.x - cr.x, v.y - cr.y, v.z - cr.z);
     };
     if (ai) {
@@ -157,36 +144,36 @@ __global__ void __launch_bounds__(kBlock) k_assemble(Particles P, Contacts C, in
     B64.mu[c] = C.mu[c];
     if (with32) B32.mu[c] = (float)C.mu[c];
     const double Cv[3] = {dij * n.x + dt * au.x, dij * n.y + dt * au.y, (DIM == 3 ? dij * n.z : 0.0) + dt * au.z};
-    if (!FRICTION) {
-        d3 a_i = ai ? mat_mult_t(Ri, cross3<DIM>(ri, n)) : mk(0, 0, 0);
-        d3 a_j = aj ? mat_mult_t(Rj, cross3<DIM>(rj, n)) : mk(0, 0, 0);
-        const double co[9] = {n.x, n.y, n.z, a_i.x, a_i.y, a_i.z, a_j.x, a_j.y, a_j.z};
+    double co[NC];
+    double cr[NR];
+    co[0] = n.x, co[1] = n.y, co[2] = n.z;
+    if (LAY == kRigid) {
+        d3 a_i = ai ? mat_mult_t(Ri, cross3(ri, n)) : mk(0, 0, 0);
+        d3 a_j = aj ? mat_mult_t(Rj, cross3(rj, n)) : mk(0, 0, 0);
+        co[3] = a_i.x, co[4] = a_i.y, co[5] = a_i.z;
+        co[6] = a_j.x, co[7] = a_j.y, co[8] = a_j.z;
+    }
+    if (LAY == kCone) {
+        for (int d = 0; d < 3; ++d) {
+            const d3 e = mk(d == 0, d == 1, d == 2);
+            d3 ki = (ai && d < DIM) ? mat_mult_t(Ri, cross3(ri, e)) : mk(0, 0, 0);
+            d3 kj = (aj && d < DIM) ? mat_mult_t(Rj, cross3(rj, e)) : mk(0, 0, 0);
+            co[3 + 3 * d] = ki.x, co[4 + 3 * d] = ki.y, co[5 + 3 * d] = ki.z;
+            co[12 + 3 * d] = kj.x, co[13 + 3 * d] = kj.y, co[14 + 3 * d] = kj.z;
+        }
+        for (int d = 0; d < 3; ++d) cr[d] = (d < DIM) ? Cv[d] : 0.0;
+    } else {
         double cl = 0.0;
         cl += Cv[0] * n.x;
         cl += Cv[1] * n.y;
         if (DIM == 3) cl += Cv[2] * n.z;
-        for (int k = 0; k < 9; ++k) B64.coef[k * cap + c] = co[k];
-        B64.c[c] = cl;
-        if (with32) {
-            for (int k = 0; k < 9; ++k) B32.coef[k * cap + c] = (float)co[k];
-            B32.c[c] = (float)cl;
-        }
-    } else {
-        double co[21];
-        co[0] = n.x, co[1] = n.y, co[2] = n.z;
-        for (int d = 0; d < 3; ++d) {
-            const d3 e = mk(d == 0, d == 1, d == 2);
-            d3 ki = (ai && d < DIM) ? mat_mult_t(Ri, cross3<DIM>(ri, e)) : mk(0, 0, 0);
-            d3 kj = (aj && d < DIM) ? mat_mult_t(Rj, cross3<DIM>(rj, e)) : mk(0, 0, 0);
-            co[3 + 3 * d] = ki.x, co[4 + 3 * d] = ki.y, co[5 + 3 * d] = ki.z;
-            co[12 + 3 * d] = kj.x, co[13 + 3 * d] = kj.y, co[14 + 3 * d] = kj.z;
-        }
-        for (int k = 0; k < 21; ++k) B64.coef[k * cap + c] = co[k];
-        for (int d = 0; d < 3; ++d) B64.c[d * cap + c] = (d < DIM) ? Cv[d] : 0.0;
-        if (with32) {
-            for (int k = 0; k < 21; ++k) B32.coef[k * cap + c] = (float)co[k];
-            for (int d = 0; d < 3; ++d) B32.c[d * cap + c] = (float)((d < DIM) ? Cv[d] : 0.0);
-        }
+        cr[0] = cl;
+    }
+    for (int k = 0; k < NC; ++k) B64.coef[c * NC + k] = co[k];
+    for (int d = 0; d < NR; ++d) B64.c[c * NR + d] = cr[d];
+    if (with32) {
+        for (int k = 0; k < NC; ++k) B32.coef[c * NC + k] = (float)co[k];
+        for (int d = 0; d < NR; ++d) B32.c[c * NR + d] = (float)cr[d];
     }
 }
 
@@ -222,6 +209,48 @@ __global__ void __launch_bounds__(kBlock) k_jrange(const uint32_t* __restrict__ 
     if (e == nc - 1 || keys[e + 1] != k) end[k] = (int)(e + 1);
 }
 
+__global__ void __launch_bounds__(kBlock) k_tcount(const int* __restrict__ is, const int* __restrict__ ie,
+                                                  const int* __restrict__ js, const int* __restrict__ je, int64_t na,
+                                                  int* __restrict__ cnt) {
+    const int64_t p = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (p > na) return;
+    cnt[p] = (p < na) ? (ie[p] - is[p]) + (je[p] - js[p]) : 0;
+}
+
+// Particle-major copy of B's blocks: for particle p, entries [tstart[p], tstart[p+1]):
+// its contacts as i (ascending c), then as j (ascending c), sign folded into the values.
+template <class T, int LAY>
+__global__ void __launch_bounds__(kBlock) k_transpose(BMatrix<T> B, int64_t na, const int* __restrict__ is,
+                                                     const int* __restrict__ ie, const int* __restrict__ js,
+                                                     const int* __restrict__ je, const int32_t* __restrict__ jlist,
+                                                     const int* __restrict__ tstart, int32_t* __restrict__ tc) {
+    constexpr int NC = lay<LAY>::NC, NT = lay<LAY>::NT;
+    const int64_t p = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (p >= na) return;
+    int64_t e = tstart[p];
+    for (int c = is[p]; c < ie[p]; ++c, ++e) {
+        tc[e] = c;
+        if (LAY == kRotFree)
+            for (int k = 0; k < 3; ++k) B.tcoef[e * NT + k] = B.coef[(int64_t)c * NC + k];
+        if (LAY == kRigid)
+            for (int k = 0; k < 6; ++k) B.tcoef[e * NT + k] = B.coef[(int64_t)c * NC + k];
+        if (LAY == kCone)
+            for (int k = 0; k < 9; ++k) B.tcoef[e * NT + k] = B.coef[(int64_t)c * NC + 3 + k];
+    }
+    for (int m = js[p]; m < je[p]; ++m, ++e) {
+        const int c = jlist[m];
+        tc[e] = ~c;
+        if (LAY == kRotFree)
+            for (int k = 0; k < 3; ++k) B.tcoef[e * NT + k] = -B.coef[(int64_t)c * NC + k];
+        if (LAY == kRigid) {
+            for (int k = 0; k < 3; ++k) B.tcoef[e * NT + k] = -B.coef[(int64_t)c * NC + k];
+            for (int k = 0; k < 3; ++k) B.tcoef[e * NT + 3 + k] = -B.coef[(int64_t)c * NC + 6 + k];
+        }
+        if (LAY == kCone)
+            for (int k = 0; k < 9; ++k) B.tcoef[e * NT + k] = -B.coef[(int64_t)c * NC + 12 + k];
+    }
+}
+
 // ------------------------------------------------------------------ the loop
 template <class T>
 __device__ inline T* slot(T* base, int s, int64_t stride) {
@@ -229,28 +258,28 @@ __device__ inline T* slot(T* base, int s, int64_t stride) {
 }
 
 // Trial point: y, Qy and grad f(y) from (lambda_k, lambda_{k-1}) and their Q
-// products, lambda+ = Proj(y - alpha grad), and the line-search sums:
-// red = {<g, l+ - y>, |l+ - y|^2, <g, l+ - l_k>, |l+ - l_k|^2, <y, Qy>, <c, y>, <c, l+>, -, |l+|^2}.
-template <class T, bool FRICTION>
-__global__ void __launch_bounds__(kBlock) k_trial(ApgdCtrl* __restrict__ ctrl, BMatrix<T> B, int64_t nc,
+// products, lambda+ = Proj(y - alpha grad), and the block sums of
+// {<g, l+ - y>, |l+ - y|^2, <g, l+ - l_k>, |l+ - l_k|^2, <y, Qy>, <c, y>, <c, l+>, |l+|^2}.
+template <class T, int LAY>
+__global__ void __launch_bounds__(kBlock) k_trial(const ApgdCtrl* __restrict__ ctrl, BMatrix<T> B, int64_t nc,
                                                  double* __restrict__ partial) {
+    constexpr int NR = lay<LAY>::NR, NC = lay<LAY>::NC;
     __shared__ double lds[8][kBlock / 64];
     if (ctrl->done) return;
     const int cur = ctrl->cur, prev = ctrl->prev, trial = ctrl->trial;
-    const int64_t rows_stride = FRICTION ? 3 * B.cap : B.cap;
+    const int64_t stride = NR * B.cap;
     const T beta = (T)ctrl->beta, alpha = (T)ctrl->alpha;
-    const T* lc = slot<T>(B.lam, cur, rows_stride);
-    const T* lp = slot<T>(B.lam, prev, rows_stride);
-    const T* qc = slot<T>(B.qlam, cur, rows_stride);
-    const T* qp = slot<T>(B.qlam, prev, rows_stride);
-    T* lt = slot<T>(B.lam, trial, rows_stride);
+    const T* lc = slot<T>(B.lam, cur, stride);
+    const T* lp = slot<T>(B.lam, prev, stride);
+    const T* qc = slot<T>(B.qlam, cur, stride);
+    const T* qp = slot<T>(B.qlam, prev, stride);
+    T* lt = slot<T>(B.lam, trial, stride);
     double s[8] = {0, 0, 0, 0, 0, 0, 0, 0};
     for (int64_t c = (int64_t)blockIdx.x * kBlock + threadIdx.x; c < nc; c += (int64_t)gridDim.x * kBlock) {
-        constexpr int NR = FRICTION ? 3 : 1;
         T y[NR], g[NR], z[NR], l0[NR], cc[NR], qy[NR];
 #pragma unroll
         for (int d = 0; d < NR; ++d) {
-            const int64_t r = d * B.cap + c;
+            const int64_t r = c * NR + d;
             l0[d] = lc[r];
             const T l1 = lp[r];
             const T q0 = qc[r], q1 = qp[r];
@@ -260,11 +289,11 @@ __global__ void __launch_bounds__(kBlock) k_trial(ApgdCtrl* __restrict__ ctrl, B
             g[d] = qy[d] + cc[d];
             z[d] = y[d] - alpha * g[d];
         }
-        if (!FRICTION) {
+        if (LAY != kCone) {
             z[0] = z[0] > (T)0 ? z[0] : (T)0;  // max(lambda, 0): lagrange_multiplier.hpp:104-107
         } else {
             // Coulomb-cone projection: lagrange_multiplier.hpp:243-268
-            const T nx = B.coef[0 * B.cap + c], ny = B.coef[1 * B.cap + c], nz = B.coef[2 * B.cap + c];
+            const T nx = B.coef[c * NC + 0], ny = B.coef[c * NC + 1], nz = B.coef[c * NC + 2];
             const T mu = B.mu[c];
             const T ln = z[0] * nx + z[1] * ny + z[2] * nz;
             T tx = z[0] - ln * nx, ty = z[1] - ln * ny, tz = z[2] - ln * nz;
@@ -284,8 +313,7 @@ __global__ void __launch_bounds__(kBlock) k_trial(ApgdCtrl* __restrict__ ctrl, B
         }
 #pragma unroll
         for (int d = 0; d < NR; ++d) {
-            const int64_t r = d * B.cap + c;
-            lt[r] = z[d];
+            lt[c * NR + d] = z[d];
             const double dy = (double)z[d] - (double)y[d], dl = (double)z[d] - (double)l0[d];
             s[0] += (double)g[d] * dy;
             s[1] += dy * dy;
@@ -297,98 +325,129 @@ __global__ void __launch_bounds__(kBlock) k_trial(ApgdCtrl* __restrict__ ctrl, B
             s[7] += (double)z[d] * (double)z[d];
         }
     }
-    double tot[8];
-    if (last_block_reduce<8>(s, partial, &ctrl->count_trial, lds, tot) && threadIdx.x == 0) {
-        for (int k = 0; k < 7; ++k) ctrl->red[k] = tot[k];
-        ctrl->red[8] = tot[7];
-    }
+    block_sum<8>(s, lds);
+    if (threadIdx.x == 0)
+#pragma unroll
+        for (int k = 0; k < 8; ++k) partial[blockIdx.x * kPartialStride + k] = s[k];
 }
 
-// w = M^-1 B^T lambda (ATMatrix::mat_mult, velocities.hpp:167-254, then M^-1),
-// one thread per active particle.
-template <class T, bool FRICTION>
-__global__ void __launch_bounds__(kBlock) k_bt(const ApgdCtrl* __restrict__ ctrl, int use_slot, BMatrix<T> B,
-                                              int64_t na, const int* __restrict__ istart, const int* __restrict__ iend,
-                                              const int* __restrict__ jstart, const int* __restrict__ jend,
-                                              const int32_t* __restrict__ jlist, int64_t pcap) {
+// w = M^-1 B^T lambda (ATMatrix::mat_mult, velocities.hpp:167-254, then M^-1), one
+// thread per active particle over its entries of the particle-major copy.  In the
+// loop, block 0 also reduces the trial kernel's block sums into ctrl->red.
+template <class T, int LAY>
+__global__ void __launch_bounds__(kBlock) k_bt(ApgdCtrl* __restrict__ ctrl, int use_slot, BMatrix<T> B, int64_t na,
+                                              const int* __restrict__ tstart, const int32_t* __restrict__ tc,
+                                              const double* __restrict__ partial_trial, int ntrial) {
+    constexpr int NR = lay<LAY>::NR, NT = lay<LAY>::NT, NW = lay<LAY>::NW;
+    __shared__ double lds[8][kBlock / 64];
     if (ctrl && ctrl->done) return;
+    if (ctrl && partial_trial && blockIdx.x == 0) {
+        double tot[8];
+        reduce_partials<8>(partial_trial, ntrial, lds, tot);
+        if (threadIdx.x == 0) {
+            for (int k = 0; k < 7; ++k) ctrl->red[k] = tot[k];
+            ctrl->red[8] = tot[7];
+        }
+    }
     const int64_t p = (int64_t)blockIdx.x * kBlock + threadIdx.x;
     if (p >= na) return;
     const int s = (use_slot >= 0) ? use_slot : ctrl->trial;
-    const int64_t rows_stride = FRICTION ? 3 * B.cap : B.cap;
-    const T* lam = slot<T>(B.lam, s, rows_stride);
-    const int64_t cap = B.cap;
-    T v0 = 0, v1 = 0, v2 = 0, w0 = 0, w1 = 0, w2 = 0;
-    const int i0 = istart[p], i1 = iend[p];
-    for (int c = i0; c < i1; ++c) {
-        if (!FRICTION) {
+    const T* lam = slot<T>(B.lam, s, NR * B.cap);
+    T acc[NW];
+#pragma unroll
+    for (int k = 0; k < NW; ++k) acc[k] = (T)0;
+    const int e0 = tstart[p], e1 = tstart[p + 1];
+    for (int e = e0; e < e1; ++e) {
+        const int32_t cs = tc[e];
+        const int64_t c = cs < 0 ? ~cs : cs;
+        const T* tv = B.tcoef + (int64_t)e * NT;
+        if (LAY != kCone) {
             const T l = lam[c];
-            v0 += l * B.coef[0 * cap + c];
-            v1 += l * B.coef[1 * cap + c];
-            v2 += l * B.coef[2 * cap + c];
-            w0 += l * B.coef[3 * cap + c];
-            w1 += l * B.coef[4 * cap + c];
-            w2 += l * B.coef[5 * cap + c];
+#pragma unroll
+            for (int k = 0; k < NT; ++k) acc[k] += l * tv[k];
         } else {
-            const T l0 = lam[c], l1 = lam[cap + c], l2 = lam[2 * cap + c];
-            v0 += l0, v1 += l1, v2 += l2;
-            w0 += l0 * B.coef[3 * cap + c] + l1 * B.coef[6 * cap + c] + l2 * B.coef[9 * cap + c];
-            w1 += l0 * B.coef[4 * cap + c] + l1 * B.coef[7 * cap + c] + l2 * B.coef[10 * cap + c];
-            w2 += l0 * B.coef[5 * cap + c] + l1 * B.coef[8 * cap + c] + l2 * B.coef[11 * cap + c];
+            const T l0 = lam[c * 3], l1 = lam[c * 3 + 1], l2 = lam[c * 3 + 2];
+            if (cs < 0) {
+                acc[0] -= l0, acc[1] -= l1, acc[2] -= l2;
+            } else {
+                acc[0] += l0, acc[1] += l1, acc[2] += l2;
+            }
+#pragma unroll
+            for (int k = 0; k < 3; ++k) acc[3 + k] += l0 * tv[k] + l1 * tv[3 + k] + l2 * tv[6 + k];
         }
     }
-    const int j0 = jstart[p], j1 = jend[p];
-    for (int e = j0; e < j1; ++e) {
-        const int c = jlist[e];
-        if (!FRICTION) {
-            const T l = lam[c];
-            v0 -= l * B.coef[0 * cap + c];
-            v1 -= l * B.coef[1 * cap + c];
-            v2 -= l * B.coef[2 * cap + c];
-            w0 -= l * B.coef[6 * cap + c];
-            w1 -= l * B.coef[7 * cap + c];
-            w2 -= l * B.coef[8 * cap + c];
-        } else {
-            const T l0 = lam[c], l1 = lam[cap + c], l2 = lam[2 * cap + c];
-            v0 -= l0, v1 -= l1, v2 -= l2;
-            w0 -= l0 * B.coef[12 * cap + c] + l1 * B.coef[15 * cap + c] + l2 * B.coef[18 * cap + c];
-            w1 -= l0 * B.coef[13 * cap + c] + l1 * B.coef[16 * cap + c] + l2 * B.coef[19 * cap + c];
-            w2 -= l0 * B.coef[14 * cap + c] + l1 * B.coef[17 * cap + c] + l2 * B.coef[20 * cap + c];
-        }
-    }
-    B.w[0 * pcap + p] = B.minv[0 * pcap + p] * v0;
-    B.w[1 * pcap + p] = B.minv[1 * pcap + p] * v1;
-    B.w[2 * pcap + p] = B.minv[2 * pcap + p] * v2;
-    B.w[3 * pcap + p] = B.minv[3 * pcap + p] * w0;
-    B.w[4 * pcap + p] = B.minv[4 * pcap + p] * w1;
-    B.w[5 * pcap + p] = B.minv[5 * pcap + p] * w2;
+#pragma unroll
+    for (int k = 0; k < NW; ++k) B.w[p * NW + k] = B.minv[p * NW + k] * acc[k];
 }
 
-template <class T, bool FRICTION>
-__device__ inline void row_product(const BMatrix<T>& B, int64_t c, int64_t i, int64_t j, int64_t pcap, T dt2, T (&q)[3]) {
-    const int64_t cap = B.cap;
-    T wi[6] = {0, 0, 0, 0, 0, 0}, wj[6] = {0, 0, 0, 0, 0, 0};
-    if (i >= 0)
+template <class T, int LAY>
+__device__ inline void row_product(const BMatrix<T>& B, int64_t c, int64_t i, int64_t j, T dt2, T (&q)[3]) {
+    constexpr int NC = lay<LAY>::NC, NW = lay<LAY>::NW;
+    T wi[NW], wj[NW];
 #pragma unroll
-        for (int k = 0; k < 6; ++k) wi[k] = B.w[k * pcap + i];
+    for (int k = 0; k < NW; ++k) wi[k] = (i >= 0) ? B.w[i * NW + k] : (T)0;
 #pragma unroll
-    for (int k = 0; k < 6; ++k) wj[k] = B.w[k * pcap + j];
-    if (!FRICTION) {
-        const T nx = B.coef[0 * cap + c], ny = B.coef[1 * cap + c], nz = B.coef[2 * cap + c];
-        T acc = nx * (wi[0] - wj[0]) + ny * (wi[1] - wj[1]) + nz * (wi[2] - wj[2]);
-        acc += B.coef[3 * cap + c] * wi[3] + B.coef[4 * cap + c] * wi[4] + B.coef[5 * cap + c] * wi[5];
-        acc -= B.coef[6 * cap + c] * wj[3] + B.coef[7 * cap + c] * wj[4] + B.coef[8 * cap + c] * wj[5];
+    for (int k = 0; k < NW; ++k) wj[k] = B.w[j * NW + k];
+    const T* co = B.coef + c * NC;
+    if (LAY == kRotFree) {
+        q[0] = dt2 * (co[0] * (wi[0] - wj[0]) + co[1] * (wi[1] - wj[1]) + co[2] * (wi[2] - wj[2]));
+    } else if (LAY == kRigid) {
+        T acc = co[0] * (wi[0] - wj[0]) + co[1] * (wi[1] - wj[1]) + co[2] * (wi[2] - wj[2]);
+        acc += co[3] * wi[3] + co[4] * wi[4] + co[5] * wi[5];
+        acc -= co[6] * wj[3] + co[7] * wj[4] + co[8] * wj[5];
         q[0] = dt2 * acc;
     } else {
 #pragma unroll
         for (int d = 0; d < 3; ++d) {
             T acc = wi[d] - wj[d];
-            acc += B.coef[(3 + 3 * d) * cap + c] * wi[3] + B.coef[(4 + 3 * d) * cap + c] * wi[4] +
-                   B.coef[(5 + 3 * d) * cap + c] * wi[5];
-            acc -= B.coef[(12 + 3 * d) * cap + c] * wj[3] + B.coef[(13 + 3 * d) * cap + c] * wj[4] +
-                   B.coef[(14 + 3 * d) * cap + c] * wj[5];
+            acc += co[3 + 3 * d] * wi[3] + co[4 + 3 * d] * wi[4] + co[5 + 3 * d] * wi[5];
+            acc -= co[12 + 3 * d] * wj[3] + co[13 + 3 * d] * wj[4] + co[14 + 3 * d] * wj[5];
             q[d] = dt2 * acc;
         }
+    }
+}
+
+// q = dt^2 B w for the trial slot and the block sums of <lambda+, Q lambda+>.
+template <class T, int LAY>
+__global__ void __launch_bounds__(kBlock) k_row(const ApgdCtrl* __restrict__ ctrl, BMatrix<T> B, int64_t nc,
+                                               const int32_t* __restrict__ ci, const int32_t* __restrict__ cj,
+                                               int64_t ni, double* __restrict__ partial) {
+    constexpr int NR = lay<LAY>::NR;
+    __shared__ double lds[1][kBlock / 64];
+    if (ctrl->done) return;
+    const int trial = ctrl->trial;
+    const T dt2 = (T)ctrl->dt2;
+    const int64_t stride = NR * B.cap;
+    const T* lt = slot<T>(B.lam, trial, stride);
+    T* qt = slot<T>(B.qlam, trial, stride);
+    double s[1] = {0.0};
+    for (int64_t c = (int64_t)blockIdx.x * kBlock + threadIdx.x; c < nc; c += (int64_t)gridDim.x * kBlock) {
+        const int64_t i = (int64_t)ci[c] - ni, j = (int64_t)cj[c] - ni;
+        T q[3];
+        row_product<T, LAY>(B, c, i, j, dt2, q);
+#pragma unroll
+        for (int d = 0; d < NR; ++d) {
+            qt[c * NR + d] = q[d];
+            s[0] += (double)lt[c * NR + d] * (double)q[d];
+        }
+    }
+    block_sum<1>(s, lds);
+    if (threadIdx.x == 0) partial[blockIdx.x * kPartialStride] = s[0];
+}
+
+// Q lambda for a given slot (warm start), no reduction.
+template <class T, int LAY>
+__global__ void __launch_bounds__(kBlock) k_row_plain(BMatrix<T> B, int use_slot, int64_t nc,
+                                                     const int32_t* __restrict__ ci, const int32_t* __restrict__ cj,
+                                                     int64_t ni, double dt2) {
+    constexpr int NR = lay<LAY>::NR;
+    T* qs = slot<T>(B.qlam, use_slot, NR * B.cap);
+    for (int64_t c = (int64_t)blockIdx.x * kBlock + threadIdx.x; c < nc; c += (int64_t)gridDim.x * kBlock) {
+        const int64_t i = (int64_t)ci[c] - ni, j = (int64_t)cj[c] - ni;
+        T q[3];
+        row_product<T, LAY>(B, c, i, j, (T)dt2, q);
+#pragma unroll
+        for (int d = 0; d < NR; ++d) qs[c * NR + d] = q[d];
     }
 }
 
@@ -474,51 +533,16 @@ __device__ inline void apgd_control(ApgdCtrl* ctrl) {
     }
 }
 
-// q = dt^2 B w for the trial slot, <lambda+, Q lambda+>, then the control step.
-template <class T, bool FRICTION>
-__global__ void __launch_bounds__(kBlock) k_row(ApgdCtrl* __restrict__ ctrl, BMatrix<T> B, int64_t nc,
-                                               const int32_t* __restrict__ ci, const int32_t* __restrict__ cj,
-                                               int64_t ni, int64_t pcap, double* __restrict__ partial) {
-    __shared__ double lds[8][kBlock / 64];
+// One block: <lambda+, Q lambda+> from the row kernel's block sums, then the control step.
+__global__ void __launch_bounds__(kBlock) k_ctrl(ApgdCtrl* __restrict__ ctrl, const double* __restrict__ partial_row,
+                                                int nrow) {
+    __shared__ double lds[1][kBlock / 64];
     if (ctrl->done) return;
-    const int trial = ctrl->trial;
-    const T dt2 = (T)ctrl->dt2;
-    const int64_t rows_stride = FRICTION ? 3 * B.cap : B.cap;
-    const T* lt = slot<T>(B.lam, trial, rows_stride);
-    T* qt = slot<T>(B.qlam, trial, rows_stride);
-    double s[1] = {0.0};
-    for (int64_t c = (int64_t)blockIdx.x * kBlock + threadIdx.x; c < nc; c += (int64_t)gridDim.x * kBlock) {
-        const int64_t i = (int64_t)ci[c] - ni, j = (int64_t)cj[c] - ni;
-        T q[3];
-        row_product<T, FRICTION>(B, c, i, j, pcap, dt2, q);
-        constexpr int NR = FRICTION ? 3 : 1;
-#pragma unroll
-        for (int d = 0; d < NR; ++d) {
-            qt[d * B.cap + c] = q[d];
-            s[0] += (double)lt[d * B.cap + c] * (double)q[d];
-        }
-    }
     double tot[1];
-    if (last_block_reduce<1>(s, partial, &ctrl->count_row, lds, tot) && threadIdx.x == 0) {
+    reduce_partials<1>(partial_row, nrow, lds, tot);
+    if (threadIdx.x == 0) {
         ctrl->red[7] = tot[0];
         apgd_control(ctrl);
-    }
-}
-
-// Q lambda for a given slot (warm start), no reduction.
-template <class T, bool FRICTION>
-__global__ void __launch_bounds__(kBlock) k_row_plain(BMatrix<T> B, int use_slot, int64_t nc,
-                                                     const int32_t* __restrict__ ci, const int32_t* __restrict__ cj,
-                                                     int64_t ni, int64_t pcap, double dt2) {
-    const int64_t rows_stride = FRICTION ? 3 * B.cap : B.cap;
-    T* qs = slot<T>(B.qlam, use_slot, rows_stride);
-    for (int64_t c = (int64_t)blockIdx.x * kBlock + threadIdx.x; c < nc; c += (int64_t)gridDim.x * kBlock) {
-        const int64_t i = (int64_t)ci[c] - ni, j = (int64_t)cj[c] - ni;
-        T q[3];
-        row_product<T, FRICTION>(B, c, i, j, pcap, (T)dt2, q);
-        constexpr int NR = FRICTION ? 3 : 1;
-#pragma unroll
-        for (int d = 0; d < NR; ++d) qs[d * B.cap + c] = q[d];
     }
 }
 
@@ -528,10 +552,9 @@ __global__ void __launch_bounds__(kBlock) k_zero(T* __restrict__ a, int64_t n) {
 }
 
 template <class TA, class TB>
-__global__ void __launch_bounds__(kBlock) k_convert(const TA* __restrict__ a, TB* __restrict__ b, int64_t n,
-                                                   int64_t stride_a, int64_t stride_b, int ncomp) {
+__global__ void __launch_bounds__(kBlock) k_convert(const TA* __restrict__ a, TB* __restrict__ b, int64_t n) {
     for (int64_t t = (int64_t)blockIdx.x * kBlock + threadIdx.x; t < n; t += (int64_t)gridDim.x * kBlock)
-        for (int d = 0; d < ncomp; ++d) b[d * stride_b + t] = (TB)a[d * stride_a + t];
+        b[t] = (TB)a[t];
 }
 
 __global__ void k_ctrl_init(ApgdCtrl* ctrl, double alpha, double L, double tol, double rtol, double dt2,
@@ -564,12 +587,11 @@ __global__ void k_ctrl_init(ApgdCtrl* ctrl, double alpha, double L, double tol, 
     ctrl->status = (max_ite <= 0) ? SCOPI_HIP_NOT_CONVERGED : 0;
     ctrl->method = method;
     ctrl->dynamic = dynamic;
-    ctrl->count_trial = 0;
-    ctrl->count_row = 0;
 }
 
-// u = v + dt M^-1 B^T lambda, omega likewise (OptimGradient.hpp:267-284); with
-// no contact u = v (OptimGradient.hpp:286-303).
+// u = v + dt M^-1 B^T lambda, omega likewise (OptimGradient.hpp:267-284); with no
+// contact, or no rotational block (kRotFree), the corresponding part is v / omega.
+template <int NW>
 __global__ void __launch_bounds__(kBlock) k_velocities(Particles P, const double* __restrict__ w, int64_t pcap,
                                                       double dt, int has_w, double* __restrict__ u,
                                                       double* __restrict__ om) {
@@ -581,15 +603,17 @@ __global__ void __launch_bounds__(kBlock) k_velocities(Particles P, const double
     for (int d = 0; d < 3; ++d) {
         const bool tr = d < P.dim;
         const bool rot = (P.dim == 3) || d == 2;
-        u[d * pcap + p] = tr ? P.v[d * P.cap + k] + (has_w ? dt * w[d * pcap + p] : 0.0) : 0.0;
-        om[d * pcap + p] = rot ? P.omega[d * P.cap + k] + (has_w ? dt * w[(3 + d) * pcap + p] : 0.0) : 0.0;
+        u[d * pcap + p] = tr ? P.v[d * P.cap + k] + (has_w ? dt * w[p * NW + d] : 0.0) : 0.0;
+        om[d * pcap + p] =
+            rot ? P.omega[d * P.cap + k] + ((has_w && NW == 6) ? dt * w[p * NW + 3 + d] : 0.0) : 0.0;
     }
 }
 
 // ------------------------------------------------------------------ host
 template <class T>
 static void bfree(BMatrix<T>& B) {
-    for (void* p : {(void*)B.coef, (void*)B.c, (void*)B.mu, (void*)B.minv, (void*)B.lam, (void*)B.qlam, (void*)B.w})
+    for (void* p : {(void*)B.coef, (void*)B.tcoef, (void*)B.c, (void*)B.mu, (void*)B.minv, (void*)B.lam,
+                    (void*)B.qlam, (void*)B.w})
         if (p) (void)hipFree(p);
     B = BMatrix<T>{};
 }
@@ -597,7 +621,9 @@ static void bfree(BMatrix<T>& B) {
 template <class T>
 static void balloc(BMatrix<T>& B, int64_t ccap, int64_t pcap) {
     B.cap = ccap;
+    B.pcap = pcap;
     SCOPI_HIP_CHECK(hipMalloc(&B.coef, 21 * ccap * sizeof(T)));
+    SCOPI_HIP_CHECK(hipMalloc(&B.tcoef, 9 * 2 * ccap * sizeof(T)));
     SCOPI_HIP_CHECK(hipMalloc(&B.c, 3 * ccap * sizeof(T)));
     SCOPI_HIP_CHECK(hipMalloc(&B.mu, ccap * sizeof(T)));
     SCOPI_HIP_CHECK(hipMalloc(&B.minv, 6 * pcap * sizeof(T)));
@@ -609,38 +635,38 @@ static void balloc(BMatrix<T>& B, int64_t ccap, int64_t pcap) {
 Solver::~Solver() {
     bfree(b64_);
     bfree(b32_);
-    for (void* p : {(void*)istart_, (void*)iend_, (void*)jstart_, (void*)jend_, (void*)jkeys_, (void*)jlist_, (void*)u_,
-                    (void*)w_, (void*)ctrl_, (void*)partial_})
+    for (void* p : {(void*)istart_, (void*)iend_, (void*)jstart_, (void*)jend_, (void*)tcount_, (void*)tstart_,
+                    (void*)tc_, (void*)jkeys_, (void*)jlist_, (void*)u_, (void*)w_, (void*)ctrl_,
+                    (void*)partial_trial_, (void*)partial_row_})
         if (p) (void)hipFree(p);
     if (h_ctrl_) (void)hipHostFree(h_ctrl_);
 }
 
-void Solver::reserve(int64_t na, int64_t nc, bool friction, bool fp32) {
-    (void)friction;
+void Solver::reserve(int64_t na, int64_t nc, bool fp32) {
     if (!ctrl_) {
         SCOPI_HIP_CHECK(hipMalloc(&ctrl_, sizeof(ApgdCtrl)));
         SCOPI_HIP_CHECK(hipHostMalloc(&h_ctrl_, sizeof(ApgdCtrl)));
-        partial_cap_ = kRedBlocksMax * 16;
-        SCOPI_HIP_CHECK(hipMalloc(&partial_, partial_cap_ * sizeof(double)));
+        SCOPI_HIP_CHECK(hipMalloc(&partial_trial_, kRedBlocksMax * kPartialStride * sizeof(double)));
+        SCOPI_HIP_CHECK(hipMalloc(&partial_row_, kRedBlocksMax * kPartialStride * sizeof(double)));
     }
-    const bool grow_p = na + 1 > pcap_, grow_c = nc + 1 > ccap_;
+    const bool grow_p = na + 2 > pcap_, grow_c = nc + 1 > ccap_;
     if (grow_p) {
-        for (void* p : {(void*)istart_, (void*)iend_, (void*)jstart_, (void*)jend_, (void*)u_, (void*)w_})
+        for (void* p : {(void*)istart_, (void*)iend_, (void*)jstart_, (void*)jend_, (void*)tcount_, (void*)tstart_,
+                        (void*)u_, (void*)w_})
             if (p) SCOPI_HIP_CHECK(hipFree(p));
         pcap_ = na + na / 4 + 1024;
-        SCOPI_HIP_CHECK(hipMalloc(&istart_, pcap_ * sizeof(int)));
-        SCOPI_HIP_CHECK(hipMalloc(&iend_, pcap_ * sizeof(int)));
-        SCOPI_HIP_CHECK(hipMalloc(&jstart_, pcap_ * sizeof(int)));
-        SCOPI_HIP_CHECK(hipMalloc(&jend_, pcap_ * sizeof(int)));
+        for (int** a : {&istart_, &iend_, &jstart_, &jend_, &tcount_, &tstart_})
+            SCOPI_HIP_CHECK(hipMalloc(a, pcap_ * sizeof(int)));
         SCOPI_HIP_CHECK(hipMalloc(&u_, 3 * pcap_ * sizeof(double)));
         SCOPI_HIP_CHECK(hipMalloc(&w_, 3 * pcap_ * sizeof(double)));
     }
     if (grow_c) {
-        if (jkeys_) SCOPI_HIP_CHECK(hipFree(jkeys_));
-        if (jlist_) SCOPI_HIP_CHECK(hipFree(jlist_));
+        for (void* p : {(void*)jkeys_, (void*)jlist_, (void*)tc_})
+            if (p) SCOPI_HIP_CHECK(hipFree(p));
         ccap_ = nc + nc / 4 + 1024;
         SCOPI_HIP_CHECK(hipMalloc(&jkeys_, ccap_ * sizeof(uint32_t)));
         SCOPI_HIP_CHECK(hipMalloc(&jlist_, ccap_ * sizeof(int32_t)));
+        SCOPI_HIP_CHECK(hipMalloc(&tc_, 2 * ccap_ * sizeof(int32_t)));
     }
     if (grow_p || grow_c || !b64_.coef) {
         bfree(b64_);
@@ -665,48 +691,27 @@ void Solver::timing(double* out) const {
     out[5] = (double)n_trial_;
 }
 
-template <class T, bool FR>
-static void launch_round(hipStream_t s, unsigned gred, unsigned gpart, ApgdCtrl* ctrl, BMatrix<T>& B, const Contacts& C,
-                         int64_t na, int64_t nc, int64_t ni, int64_t pcap, const int* is, const int* ie, const int* js,
-                         const int* je, const int32_t* jl, double* partial, hipEvent_t* ev) {
-    if (ev) SCOPI_HIP_CHECK(hipEventRecord(ev[0], s));
-    hipLaunchKernelGGL((k_trial<T, FR>), dim3(gred), dim3(kBlock), 0, s, ctrl, B, nc, partial);
-    if (ev) SCOPI_HIP_CHECK(hipEventRecord(ev[1], s));
-    hipLaunchKernelGGL((k_bt<T, FR>), dim3(gpart), dim3(kBlock), 0, s, (const ApgdCtrl*)ctrl, -1, B, na, is, ie, js, je,
-                       jl, pcap);
-    if (ev) SCOPI_HIP_CHECK(hipEventRecord(ev[2], s));
-    hipLaunchKernelGGL((k_row<T, FR>), dim3(gred), dim3(kBlock), 0, s, ctrl, B, nc, (const int32_t*)C.i,
-                       (const int32_t*)C.j, ni, pcap, partial);
-    if (ev) SCOPI_HIP_CHECK(hipEventRecord(ev[3], s));
-}
-
-template <class T>
-void Solver::run_phase(BMatrix<T>& B, const Particles& P, const Contacts& C, int64_t na, int64_t nc, bool friction,
+template <class T, int LAY>
+void Solver::run_phase(BMatrix<T>& B, const Particles& P, const Contacts& C, int64_t na, int64_t nc,
                        const scopi_hip_params& prm, double dt, double tol, double rtol, int64_t max_ite, bool warm,
                        bool keep_L, hipStream_t s, scopi_hip_solve_info& info) {
+    constexpr int NR = lay<LAY>::NR;
     const unsigned gred = (unsigned)std::min<int64_t>(grid_for(nc), kRedBlocksMax);
     const unsigned gpart = grid_for(na);
     const int64_t ni = P.n_inactive;
-    const int64_t rstride = friction ? 3 * B.cap : B.cap;
+    const int64_t stride = NR * B.cap;
     const double dt2 = dt * dt;
     if (!warm) {
-        hipLaunchKernelGGL(k_zero<T>, dim3(1024), dim3(kBlock), 0, s, B.lam, 3 * rstride);
-        hipLaunchKernelGGL(k_zero<T>, dim3(1024), dim3(kBlock), 0, s, B.qlam, 3 * rstride);
+        hipLaunchKernelGGL(k_zero<T>, dim3(1024), dim3(kBlock), 0, s, B.lam, 3 * stride);
+        hipLaunchKernelGGL(k_zero<T>, dim3(1024), dim3(kBlock), 0, s, B.qlam, 3 * stride);
     } else {
         // lambda_0 is in slot 0: lambda_{-1} = lambda_0 and both Q products
-        SCOPI_HIP_CHECK(hipMemcpyAsync(B.lam + rstride, B.lam, rstride * sizeof(T), hipMemcpyDeviceToDevice, s));
-        if (friction) {
-            hipLaunchKernelGGL((k_bt<T, true>), dim3(gpart), dim3(kBlock), 0, s, (const ApgdCtrl*)nullptr, 0, B, na,
-                               istart_, iend_, jstart_, jend_, jlist_, pcap_);
-            hipLaunchKernelGGL((k_row_plain<T, true>), dim3(gred), dim3(kBlock), 0, s, B, 0, nc, (const int32_t*)C.i,
-                               (const int32_t*)C.j, ni, pcap_, dt2);
-        } else {
-            hipLaunchKernelGGL((k_bt<T, false>), dim3(gpart), dim3(kBlock), 0, s, (const ApgdCtrl*)nullptr, 0, B, na,
-                               istart_, iend_, jstart_, jend_, jlist_, pcap_);
-            hipLaunchKernelGGL((k_row_plain<T, false>), dim3(gred), dim3(kBlock), 0, s, B, 0, nc, (const int32_t*)C.i,
-                               (const int32_t*)C.j, ni, pcap_, dt2);
-        }
-        SCOPI_HIP_CHECK(hipMemcpyAsync(B.qlam + rstride, B.qlam, rstride * sizeof(T), hipMemcpyDeviceToDevice, s));
+        SCOPI_HIP_CHECK(hipMemcpyAsync(B.lam + stride, B.lam, stride * sizeof(T), hipMemcpyDeviceToDevice, s));
+        hipLaunchKernelGGL((k_bt<T, LAY>), dim3(gpart), dim3(kBlock), 0, s, (ApgdCtrl*)nullptr, 0, B, na, tstart_, tc_,
+                           (const double*)nullptr, 0);
+        hipLaunchKernelGGL((k_row_plain<T, LAY>), dim3(gred), dim3(kBlock), 0, s, B, 0, nc, (const int32_t*)C.i,
+                           (const int32_t*)C.j, ni, dt2);
+        SCOPI_HIP_CHECK(hipMemcpyAsync(B.qlam + stride, B.qlam, stride * sizeof(T), hipMemcpyDeviceToDevice, s));
         info.q_applies += 1;
     }
     hipLaunchKernelGGL(k_ctrl_init, dim3(1), dim3(64), 0, s, ctrl_, prm.alpha, 1. / prm.alpha, tol, rtol, dt2, max_ite,
@@ -722,18 +727,24 @@ void Solver::run_phase(BMatrix<T>& B, const Particles& P, const Contacts& C, int
     for (;;) {
         for (int r = 0; r < batch; ++r) {
             hipEvent_t* e = timing_ ? &ev[4 * r] : nullptr;
-            if (friction)
-                launch_round<T, true>(s, gred, gpart, ctrl_, B, C, na, nc, ni, pcap_, istart_, iend_, jstart_, jend_,
-                                      jlist_, partial_, e);
-            else
-                launch_round<T, false>(s, gred, gpart, ctrl_, B, C, na, nc, ni, pcap_, istart_, iend_, jstart_, jend_,
-                                       jlist_, partial_, e);
+            if (e) SCOPI_HIP_CHECK(hipEventRecord(e[0], s));
+            hipLaunchKernelGGL((k_trial<T, LAY>), dim3(gred), dim3(kBlock), 0, s, (const ApgdCtrl*)ctrl_, B, nc,
+                               partial_trial_);
+            if (e) SCOPI_HIP_CHECK(hipEventRecord(e[1], s));
+            hipLaunchKernelGGL((k_bt<T, LAY>), dim3(gpart), dim3(kBlock), 0, s, ctrl_, -1, B, na, tstart_, tc_,
+                               (const double*)partial_trial_, (int)gred);
+            if (e) SCOPI_HIP_CHECK(hipEventRecord(e[2], s));
+            hipLaunchKernelGGL((k_row<T, LAY>), dim3(gred), dim3(kBlock), 0, s, (const ApgdCtrl*)ctrl_, B, nc,
+                               (const int32_t*)C.i, (const int32_t*)C.j, ni, partial_row_);
+            if (e) SCOPI_HIP_CHECK(hipEventRecord(e[3], s));
+            hipLaunchKernelGGL(k_ctrl, dim3(1), dim3(kBlock), 0, s, ctrl_, (const double*)partial_row_, (int)gred);
         }
         SCOPI_HIP_CHECK(hipGetLastError());
         SCOPI_HIP_CHECK(hipMemcpyAsync(h_ctrl_, ctrl_, sizeof(ApgdCtrl), hipMemcpyDeviceToHost, s));
         SCOPI_HIP_CHECK(hipStreamSynchronize(s));
-        const int64_t rounds = h_ctrl_->iter + h_ctrl_->backtracks;
+        const int64_t rounds = h_ctrl_->q_applies;  // one per executed round
         if (timing_) {
+            // rounds after `done` exit at their first instruction: only executed ones count
             const int64_t executed = std::min<int64_t>(rounds - before, batch);
             for (int64_t r = 0; r < executed; ++r) {
                 float ms;
@@ -762,81 +773,93 @@ void Solver::run_phase(BMatrix<T>& B, const Particles& P, const Contacts& C, int
     final_slot_ = h_ctrl_->cur;
 }
 
-int Solver::solve(const Particles& P, const Contacts& C, int64_t nc, const scopi_hip_params& prm, double dt,
-                  hipStream_t s, scopi_hip_solve_info* info) {
+template <int LAY>
+void Solver::solve_layout(const Particles& P, const Contacts& C, int64_t nc, const scopi_hip_params& prm, double dt,
+                          hipStream_t s, scopi_hip_solve_info& info) {
+    constexpr int NR = lay<LAY>::NR, NW = lay<LAY>::NW;
     const int64_t na = P.n - P.n_inactive, ni = P.n_inactive;
-    const bool friction = prm.problem == SCOPI_PROBLEM_FRICTION;
     const bool fp32 = prm.precision == SCOPI_PRECISION_FP32_REFINE;
-    if (prm.problem != SCOPI_PROBLEM_NOFRICTION && prm.problem != SCOPI_PROBLEM_FRICTION) return SCOPI_HIP_ERR_UNSUPPORTED;
-    if (prm.method != SCOPI_METHOD_APGD && prm.method != SCOPI_METHOD_PGD) return SCOPI_HIP_ERR_UNSUPPORTED;
-    if (!(prm.alpha > 0)) return SCOPI_HIP_ERR_ARG;
-    reserve(na, nc, friction, fp32);
-    nc_ = nc;
-    rows_ = friction ? 3 * nc : nc;
-    last_friction_ = friction;
-    scopi_hip_solve_info loc{};
-    loc.n_contacts = nc;
-    loc.rows = rows_;
-    loc.converged = 1;
-    if (nc == 0 || na == 0) {
-        hipLaunchKernelGGL(k_velocities, dim3(grid_for(na)), dim3(kBlock), 0, s, P, (const double*)nullptr, pcap_, dt, 0,
-                           u_, w_);
-        SCOPI_HIP_CHECK(hipGetLastError());
-        has_solution_ = true;
-        if (info) *info = loc;
-        return SCOPI_HIP_OK;
-    }
     // assembly: M^-1, B, c
-    hipLaunchKernelGGL(k_minv<double>, dim3(grid_for(na)), dim3(kBlock), 0, s, P, pcap_, b64_.minv);
-    if (fp32) hipLaunchKernelGGL(k_minv<float>, dim3(grid_for(na)), dim3(kBlock), 0, s, P, pcap_, b32_.minv);
+    hipLaunchKernelGGL((k_minv<double, LAY>), dim3(grid_for(na)), dim3(kBlock), 0, s, P, b64_.minv);
+    if (fp32) hipLaunchKernelGGL((k_minv<float, LAY>), dim3(grid_for(na)), dim3(kBlock), 0, s, P, b32_.minv);
     BMatrix<float> b32 = fp32 ? b32_ : BMatrix<float>{};
     const int w32 = fp32 ? 1 : 0;
-    if (P.dim == 3 && !friction)
-        hipLaunchKernelGGL((k_assemble<3, false>), dim3(grid_for(nc)), dim3(kBlock), 0, s, P, C, nc, dt, b64_, b32, w32);
-    else if (P.dim == 3)
-        hipLaunchKernelGGL((k_assemble<3, true>), dim3(grid_for(nc)), dim3(kBlock), 0, s, P, C, nc, dt, b64_, b32, w32);
-    else if (!friction)
-        hipLaunchKernelGGL((k_assemble<2, false>), dim3(grid_for(nc)), dim3(kBlock), 0, s, P, C, nc, dt, b64_, b32, w32);
+    if (P.dim == 3)
+        hipLaunchKernelGGL((k_assemble<3, LAY>), dim3(grid_for(nc)), dim3(kBlock), 0, s, P, C, nc, dt, b64_, b32, w32);
     else
-        hipLaunchKernelGGL((k_assemble<2, true>), dim3(grid_for(nc)), dim3(kBlock), 0, s, P, C, nc, dt, b64_, b32, w32);
+        hipLaunchKernelGGL((k_assemble<2, LAY>), dim3(grid_for(nc)), dim3(kBlock), 0, s, P, C, nc, dt, b64_, b32, w32);
     // particle-major transpose
-    hipLaunchKernelGGL(k_fill_i, dim3(256), dim3(kBlock), 0, s, istart_, na, 0);
-    hipLaunchKernelGGL(k_fill_i, dim3(256), dim3(kBlock), 0, s, iend_, na, 0);
-    hipLaunchKernelGGL(k_fill_i, dim3(256), dim3(kBlock), 0, s, jstart_, na, 0);
-    hipLaunchKernelGGL(k_fill_i, dim3(256), dim3(kBlock), 0, s, jend_, na, 0);
+    for (int* a : {istart_, iend_, jstart_, jend_})
+        hipLaunchKernelGGL(k_fill_i, dim3(256), dim3(kBlock), 0, s, a, na, 0);
     hipLaunchKernelGGL(k_irange, dim3(grid_for(nc)), dim3(kBlock), 0, s, C.i, nc, ni, istart_, iend_);
     hipLaunchKernelGGL(k_jkeys, dim3(grid_for(nc)), dim3(kBlock), 0, s, C.j, nc, ni, jkeys_, jlist_);
     int bits = 1;
     while ((int64_t(1) << bits) < na) ++bits;
     sorter_.sort(jkeys_, jlist_, nc, bits, s, scanner_);
     hipLaunchKernelGGL(k_jrange, dim3(grid_for(nc)), dim3(kBlock), 0, s, jkeys_, nc, jstart_, jend_);
+    hipLaunchKernelGGL(k_tcount, dim3(grid_for(na + 1)), dim3(kBlock), 0, s, istart_, iend_, jstart_, jend_, na,
+                       tcount_);
+    scanner_.exclusive(tcount_, tstart_, na + 1, nullptr, s);
+    hipLaunchKernelGGL((k_transpose<double, LAY>), dim3(grid_for(na)), dim3(kBlock), 0, s, b64_, na, istart_, iend_,
+                       jstart_, jend_, jlist_, tstart_, tc_);
+    if (fp32)
+        hipLaunchKernelGGL((k_transpose<float, LAY>), dim3(grid_for(na)), dim3(kBlock), 0, s, b32_, na, istart_, iend_,
+                           jstart_, jend_, jlist_, tstart_, tc_);
     SCOPI_HIP_CHECK(hipGetLastError());
-    const int64_t rs64 = friction ? 3 * b64_.cap : b64_.cap;
     if (fp32) {
-        // FP32 phase to a relative stop, then FP64 refinement warm-started from it
+        // FP32 phase to a relative stop or its precision floor, then FP64 refinement
+        // warm-started from it with the last accepted L
         const double rtol32 = prm.fp32_tolerance > 0 ? 0.0 : 1e-5;
         const double tol32 = prm.fp32_tolerance > 0 ? prm.fp32_tolerance : prm.tolerance;
-        run_phase<float>(b32_, P, C, na, nc, friction, prm, dt, tol32, rtol32, prm.max_ite, false, false, s, loc);
-        loc.iterations_fp32 = loc.iterations;
-        const int64_t rs32 = friction ? 3 * b32_.cap : b32_.cap;
+        run_phase<float, LAY>(b32_, P, C, na, nc, prm, dt, tol32, rtol32, prm.max_ite, false, false, s, info);
+        info.iterations_fp32 = info.iterations;
+        const int64_t stride32 = NR * b32_.cap;
         hipLaunchKernelGGL((k_convert<float, double>), dim3(1024), dim3(kBlock), 0, s,
-                           b32_.lam + (int64_t)final_slot_ * rs32, b64_.lam, nc, b32_.cap, b64_.cap, friction ? 3 : 1);
-        const int64_t left = std::max<int64_t>(prm.max_ite - loc.iterations, 1);
-        run_phase<double>(b64_, P, C, na, nc, friction, prm, dt, prm.tolerance, 0.0, left, true, true, s, loc);
+                           b32_.lam + (int64_t)final_slot_ * stride32, b64_.lam, NR * nc);
+        const int64_t left = std::max<int64_t>(prm.max_ite - info.iterations, 1);
+        run_phase<double, LAY>(b64_, P, C, na, nc, prm, dt, prm.tolerance, 0.0, left, true, true, s, info);
     } else {
-        run_phase<double>(b64_, P, C, na, nc, friction, prm, dt, prm.tolerance, 0.0, prm.max_ite, false, false, s, loc);
+        run_phase<double, LAY>(b64_, P, C, na, nc, prm, dt, prm.tolerance, 0.0, prm.max_ite, false, false, s, info);
     }
     // velocities from the final lambda (FP64)
-    if (friction)
-        hipLaunchKernelGGL((k_bt<double, true>), dim3(grid_for(na)), dim3(kBlock), 0, s, (const ApgdCtrl*)nullptr,
-                           final_slot_, b64_, na, istart_, iend_, jstart_, jend_, jlist_, pcap_);
-    else
-        hipLaunchKernelGGL((k_bt<double, false>), dim3(grid_for(na)), dim3(kBlock), 0, s, (const ApgdCtrl*)nullptr,
-                           final_slot_, b64_, na, istart_, iend_, jstart_, jend_, jlist_, pcap_);
-    hipLaunchKernelGGL(k_velocities, dim3(grid_for(na)), dim3(kBlock), 0, s, P, (const double*)b64_.w, pcap_, dt, 1, u_,
-                       w_);
+    hipLaunchKernelGGL((k_bt<double, LAY>), dim3(grid_for(na)), dim3(kBlock), 0, s, (ApgdCtrl*)nullptr, final_slot_,
+                       b64_, na, tstart_, tc_, (const double*)nullptr, 0);
+    hipLaunchKernelGGL(k_velocities<NW>, dim3(grid_for(na)), dim3(kBlock), 0, s, P, (const double*)b64_.w, pcap_, dt, 1,
+                       u_, w_);
     SCOPI_HIP_CHECK(hipGetLastError());
-    (void)rs64;
+}
+
+int Solver::solve(const Particles& P, const Contacts& C, int64_t nc, const scopi_hip_params& prm, double dt,
+                  bool rot_free, hipStream_t s, scopi_hip_solve_info* info) {
+    const int64_t na = P.n - P.n_inactive;
+    const bool friction = prm.problem == SCOPI_PROBLEM_FRICTION;
+    const bool fp32 = prm.precision == SCOPI_PRECISION_FP32_REFINE;
+    if (prm.problem != SCOPI_PROBLEM_NOFRICTION && prm.problem != SCOPI_PROBLEM_FRICTION) return SCOPI_HIP_ERR_UNSUPPORTED;
+    if (prm.method != SCOPI_METHOD_APGD && prm.method != SCOPI_METHOD_PGD) return SCOPI_HIP_ERR_UNSUPPORTED;
+    if (!(prm.alpha > 0)) return SCOPI_HIP_ERR_ARG;
+    reserve(na, nc, fp32);
+    layout_ = friction ? kCone : ((rot_free && !prm.general_jacobian) ? kRotFree : kRigid);
+    nr_ = friction ? 3 : 1;
+    nc_ = nc;
+    rows_ = nr_ * nc;
+    scopi_hip_solve_info loc{};
+    loc.n_contacts = nc;
+    loc.rows = rows_;
+    loc.converged = 1;
+    if (nc == 0 || na == 0) {
+        hipLaunchKernelGGL(k_velocities<6>, dim3(grid_for(na)), dim3(kBlock), 0, s, P, (const double*)nullptr, pcap_, dt,
+                           0, u_, w_);
+        SCOPI_HIP_CHECK(hipGetLastError());
+        has_solution_ = true;
+        if (info) *info = loc;
+        return SCOPI_HIP_OK;
+    }
+    if (layout_ == kCone)
+        solve_layout<kCone>(P, C, nc, prm, dt, s, loc);
+    else if (layout_ == kRigid)
+        solve_layout<kRigid>(P, C, nc, prm, dt, s, loc);
+    else
+        solve_layout<kRotFree>(P, C, nc, prm, dt, s, loc);
     has_solution_ = true;
     if (info) *info = loc;
     return SCOPI_HIP_OK;
@@ -844,22 +867,10 @@ int Solver::solve(const Particles& P, const Contacts& C, int64_t nc, const scopi
 
 void Solver::download_lambda(double* host, hipStream_t s) const {
     if (!has_solution_ || nc_ == 0) return;
-    const int64_t rs = last_friction_ ? 3 * b64_.cap : b64_.cap;
-    const double* lam = b64_.lam + (int64_t)final_slot_ * rs;
-    std::vector<double> tmp(last_friction_ ? 3 * nc_ : nc_);
-    if (!last_friction_) {
-        SCOPI_HIP_CHECK(hipMemcpyAsync(tmp.data(), lam, nc_ * sizeof(double), hipMemcpyDeviceToHost, s));
-        SCOPI_HIP_CHECK(hipStreamSynchronize(s));
-        std::copy(tmp.begin(), tmp.end(), host);
-        return;
-    }
-    // device rows are component-major (d * cap + c); the reference's order is 3c + d
-    for (int d = 0; d < 3; ++d)
-        SCOPI_HIP_CHECK(hipMemcpyAsync(tmp.data() + d * nc_, lam + d * b64_.cap, nc_ * sizeof(double),
-                                       hipMemcpyDeviceToHost, s));
+    const double* lam = b64_.lam + (int64_t)final_slot_ * nr_ * b64_.cap;
+    // rows are stored per contact (c * NR + d): the reference's order
+    SCOPI_HIP_CHECK(hipMemcpyAsync(host, lam, nr_ * nc_ * sizeof(double), hipMemcpyDeviceToHost, s));
     SCOPI_HIP_CHECK(hipStreamSynchronize(s));
-    for (int64_t c = 0; c < nc_; ++c)
-        for (int d = 0; d < 3; ++d) host[3 * c + d] = tmp[d * nc_ + c];
 }
 
 }  // namespace scopi_hip

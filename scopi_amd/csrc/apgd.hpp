@@ -6,9 +6,34 @@
 
 namespace scopi_hip {
 
-// Scalar state of the projected-gradient loop, owned by the device.  The last
-// block of the row kernel runs the reference's control logic
-// (include/scopi/solvers/apgd.hpp:164-213) on it, so the host only polls `done`.
+// Layouts of the contact Jacobian B (one block row per contact, two particle blocks).
+//   kRotFree : NoFriction with spherical active particles.  The contact normal passes
+//              through the sphere centre, so the rotational blocks R^T (r x n) vanish:
+//              row c = [ n ] on particle i, -[ n ] on j (3 values per contact).
+//   kRigid   : NoFriction, general shapes: row c = [ n | a_i ] on i, -[ n | a_j ] on j,
+//              a_k = R_k^T (r_k x n), r_k = p_k - x_k (9 values per contact).
+//   kCone    : Friction: rows (c, d), d = 0..2, = [ e_d | K_i[d] ] on i, -[ e_d | K_j[d] ] on j,
+//              K_k[d] = R_k^T (r_k x e_d); n is kept for the cone projection (21 values).
+enum Layout : int { kRotFree = 0, kRigid = 1, kCone = 2 };
+
+template <int LAY>
+struct lay;
+template <>
+struct lay<kRotFree> {
+    static constexpr int NR = 1, NC = 3, NT = 3, NW = 3;
+};
+template <>
+struct lay<kRigid> {
+    static constexpr int NR = 1, NC = 9, NT = 6, NW = 6;
+};
+template <>
+struct lay<kCone> {
+    static constexpr int NR = 3, NC = 21, NT = 9, NW = 6;
+};
+
+// Scalar state of the projected-gradient loop, owned by the device.  A one-block
+// control kernel runs the reference's logic (include/scopi/solvers/apgd.hpp:164-213)
+// on it after every product, so the host only polls `done`.
 struct ApgdCtrl {
     double L, alpha, theta, beta;
     double L_acc;        // L of the last accepted trial (before the 0.97 decay)
@@ -18,51 +43,58 @@ struct ApgdCtrl {
     int64_t iter, backtracks, max_ite, q_applies, consec_bt;
     int32_t cur, prev, trial;
     int32_t done, status, method, dynamic, low_precision;
-    uint32_t count_trial, count_row;
 };
 
-// Coefficients of the contact Jacobian B (one block row per contact).
-// NoFriction: row c = [ n_c | a_i,c ] on particle i and -[ n_c | a_j,c ] on j,
-//   a_k,c = R_k^T (r_k x n_c), r_k = p_k - x_k.
-// Friction: rows (c, d), d = 0..2 = [ e_d | K_i,c[d] ] on i, -[ e_d | K_j,c[d] ] on j,
-//   K_k,c[d] = R_k^T (r_k x e_d).
+// Device arrays of one precision.  Per-contact and per-particle data are
+// arrays of small structures (AoS): coef[c*NC + k], vectors[c*NR + d], w[p*NW + k].
 template <class T>
 struct BMatrix {
-    int64_t cap = 0;
-    T* coef = nullptr;   // NoFriction: 9 x cap (n, a_i, a_j); Friction: 21 x cap (n, K_i, K_j)
-    T* c = nullptr;      // rows: cap (NoFriction) or 3 cap (Friction), component-major
-    T* mu = nullptr;     // friction coefficient per contact (Friction)
-    T* minv = nullptr;   // 6 x pcap
-    T* lam = nullptr;    // 3 slots x rows
-    T* qlam = nullptr;   // 3 slots x rows
-    T* w = nullptr;      // 6 x pcap
+    int64_t cap = 0;     // contacts
+    int64_t pcap = 0;    // particles
+    T* coef = nullptr;   // row-major coefficients, NC per contact
+    T* tcoef = nullptr;  // particle-major copy, NT per entry (2 entries per contact), sign folded in
+    T* c = nullptr;      // g2l(D n + dt A U), NR per contact
+    T* mu = nullptr;     // friction coefficient per contact
+    T* minv = nullptr;   // diagonal of M^-1, NW per particle
+    T* lam = nullptr;    // 3 slots of NR*cap
+    T* qlam = nullptr;   // 3 slots of NR*cap
+    T* w = nullptr;      // M^-1 B^T lambda, NW per particle
 };
 
 class Solver {
   public:
     ~Solver();
     // Assembles B and c from the device contacts and the a-priori velocities,
-    // solves, and leaves u = v + dt M^-1 B^T lambda in `u_out` / `w_out` (3 x pcap each).
+    // solves, and leaves u = v + dt M^-1 B^T lambda in u() / w() (3 x pcap each).
+    // rot_free: every active particle is a sphere (enables the kRotFree layout).
     int solve(const Particles& P, const Contacts& C, int64_t nc, const scopi_hip_params& prm, double dt,
-              hipStream_t s, scopi_hip_solve_info* info);
-    // lambda of the last solve (FP64, row layout of the reference: nc or 3 nc).
+              bool rot_free, hipStream_t s, scopi_hip_solve_info* info);
+    // lambda of the last solve (FP64) in the reference's order (nc, or 3 nc with index 3c + d).
     void download_lambda(double* host, hipStream_t s) const;
     const double* u() const { return u_; }
     const double* w() const { return w_; }
     int64_t pcap() const { return pcap_; }
     int64_t last_rows() const { return rows_; }
-    int64_t last_nc() const { return nc_; }
     bool has_solution() const { return has_solution_; }
 
-    void set_timing(bool on) { timing_ = on; }
+    void set_timing(bool on) {
+        timing_ = on;
+        t_bt_ms_ = t_row_ms_ = t_trial_ms_ = 0.0;
+        n_bt_ = n_row_ = n_trial_ = 0;
+    }
+    // [ms_bt, n_bt, ms_row, n_row, ms_trial, n_trial]
     void timing(double* out) const;
+    int layout() const { return layout_; }
 
   private:
-    void reserve(int64_t na, int64_t nc, bool friction, bool fp32);
-    template <class T>
-    void run_phase(BMatrix<T>& B, const Particles& P, const Contacts& C, int64_t na, int64_t nc, bool friction,
+    void reserve(int64_t na, int64_t nc, bool fp32);
+    template <class T, int LAY>
+    void run_phase(BMatrix<T>& B, const Particles& P, const Contacts& C, int64_t na, int64_t nc,
                    const scopi_hip_params& prm, double dt, double tol, double rtol, int64_t max_ite, bool warm,
                    bool keep_L, hipStream_t s, scopi_hip_solve_info& info);
+    template <int LAY>
+    void solve_layout(const Particles& P, const Contacts& C, int64_t nc, const scopi_hip_params& prm, double dt,
+                      hipStream_t s, scopi_hip_solve_info& info);
 
     RadixSorter sorter_;
     Scanner scanner_;
@@ -70,24 +102,28 @@ class Solver {
     BMatrix<double> b64_;
     BMatrix<float> b32_;
     bool b32_alloc_ = false;
-    // particle-major transpose: contacts where the particle is i (contiguous)
-    // and where it is j (stable-sorted list of contact indices)
+    // particle-major transpose: contacts where the particle is i (contiguous, sorted
+    // by (i, j)) and where it is j (stable radix sort), merged into one entry list
     int* istart_ = nullptr;
     int* iend_ = nullptr;
     int* jstart_ = nullptr;
     int* jend_ = nullptr;
+    int* tcount_ = nullptr;
+    int* tstart_ = nullptr;   // na + 1
+    int32_t* tc_ = nullptr;   // 2 cap: contact index, ~c for the j side
     uint32_t* jkeys_ = nullptr;
     int32_t* jlist_ = nullptr;
     double* u_ = nullptr;  // 3 x pcap
     double* w_ = nullptr;  // 3 x pcap
     ApgdCtrl* ctrl_ = nullptr;
     ApgdCtrl* h_ctrl_ = nullptr;  // pinned
-    double* partial_ = nullptr;
-    int64_t partial_cap_ = 0;
+    double* partial_trial_ = nullptr;
+    double* partial_row_ = nullptr;
     int64_t rows_ = 0, nc_ = 0;
+    int nr_ = 1;
     int final_slot_ = 0;
+    int layout_ = kRotFree;
     bool has_solution_ = false;
-    bool last_friction_ = false;
     // kernel timing (HIP events on the solve stream)
     bool timing_ = false;
     double t_bt_ms_ = 0, t_row_ms_ = 0, t_trial_ms_ = 0;

@@ -20,6 +20,7 @@ struct scopi_hip_ctx {
     Contacts C;
     int64_t nc = 0;
     bool contacts_valid = false;
+    bool all_spheres = true;  // every active particle is a sphere (rotation-free B for NoFriction)
     ContactSearch search;
     Solver solver;
     std::string err;
@@ -188,7 +189,8 @@ bool params_ok(const scopi_hip_params* p) {
            (p->contact_method == SCOPI_CONTACT_KDTREE || p->contact_method == SCOPI_CONTACT_BRUTE_FORCE) &&
            (p->vap == SCOPI_VAP_FIXED || p->vap == SCOPI_VAP_FPD) &&
            (p->method == SCOPI_METHOD_APGD || p->method == SCOPI_METHOD_PGD) &&
-           (p->precision == SCOPI_PRECISION_FP64 || p->precision == SCOPI_PRECISION_FP32_REFINE);
+           (p->precision == SCOPI_PRECISION_FP64 || p->precision == SCOPI_PRECISION_FP32_REFINE) &&
+           p->warm_start == 0;
 }
 
 }  // namespace
@@ -212,6 +214,7 @@ int scopi_hip_default_params(scopi_hip_params* p) {
     p->max_ite = 10000;
     p->fp32_tolerance = 0.0;
     p->warm_start = 0;
+    p->general_jacobian = 0;
     return SCOPI_HIP_OK;
 }
 
@@ -292,6 +295,8 @@ int scopi_hip_upload_state(scopi_hip_ctx* ctx, int64_t n, int64_t n_inactive, co
         }
         ctx->contacts_valid = false;
         ctx->nc = 0;
+        ctx->all_spheres = true;
+        for (int64_t k = n_inactive; k < n; ++k) ctx->all_spheres = ctx->all_spheres && shape[k] == SCOPI_SHAPE_SPHERE;
         return SCOPI_HIP_OK;
     });
 }
@@ -414,7 +419,7 @@ int scopi_hip_solve(scopi_hip_ctx* ctx, const scopi_hip_params* params, double d
         if (ctx->nc > 0)
             hipLaunchKernelGGL(k_fill_mu, dim3(grid_for(ctx->nc)), dim3(kBlock), 0, ctx->stream, ctx->C.mu, ctx->nc,
                                params->mu);
-        int rc = ctx->solver.solve(ctx->P, ctx->C, ctx->nc, *params, dt, ctx->stream, info);
+        int rc = ctx->solver.solve(ctx->P, ctx->C, ctx->nc, *params, dt, ctx->all_spheres, ctx->stream, info);
         if (rc != SCOPI_HIP_OK) return fail(ctx, rc, "solve: unsupported option");
         SCOPI_HIP_CHECK(hipStreamSynchronize(ctx->stream));
         return SCOPI_HIP_OK;
@@ -470,7 +475,7 @@ int scopi_hip_run(scopi_hip_ctx* ctx, const scopi_hip_params* params, double dt,
             ctx->contacts_valid = true;
             if (na > 0) hipLaunchKernelGGL(k_vap, dim3(grid_for(na)), dim3(kBlock), 0, s, P, params->vap, dt);
             scopi_hip_solve_info one{};
-            int rc = ctx->solver.solve(P, ctx->C, ctx->nc, *params, dt, s, &one);
+            int rc = ctx->solver.solve(P, ctx->C, ctx->nc, *params, dt, ctx->all_spheres, s, &one);
             if (rc != SCOPI_HIP_OK) return fail(ctx, rc, "run: solve failed");
             if (na > 0)
                 hipLaunchKernelGGL(k_update_move, dim3(grid_for(na)), dim3(kBlock), 0, s, P, ctx->solver.u(),

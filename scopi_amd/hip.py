@@ -36,7 +36,8 @@ class Params(C.Structure):
         ("problem", C.c_int32), ("contact_method", C.c_int32), ("vap", C.c_int32), ("method", C.c_int32),
         ("precision", C.c_int32), ("dynamic_descent", C.c_int32), ("dmax", C.c_double),
         ("kd_tree_radius", C.c_double), ("alpha", C.c_double), ("tolerance", C.c_double), ("mu", C.c_double),
-        ("max_ite", C.c_int64), ("fp32_tolerance", C.c_double), ("warm_start", C.c_int32), ("reserved", C.c_int32),
+        ("max_ite", C.c_int64), ("fp32_tolerance", C.c_double), ("warm_start", C.c_int32),
+        ("general_jacobian", C.c_int32),
     ]
 
 
