@@ -1,0 +1,679 @@
+// superellipsoid.hpp -- FP64 restatement of the superellipsoid closest points of
+// SCoPI (include/scopi/objects/methods/closest_points.hpp:263-733) for the CPU oracle.
+//
+// TEST INFRASTRUCTURE ONLY (included by scopi_oracle.cpp).
+//
+//   geometry        superellipsoid::point / normal (objects/types/superellipsoid.hpp:281-325),
+//                   binit_xy / ainit_yz (superellipsoid.hpp:359-466)
+//   SE3-SE3         576 x 576 penalised initial grid (closest_points.hpp:604-678), residual
+//                   F(a1, b1, a2, b2) = [(p2 - p1) x n1 ; n1.n2 + |n1||n2|] (:284-382),
+//                   hybrd1 (minpack.hpp:851-923: mode 2, unit diagonal, factor 100,
+//                   xtol 1e-10, maxfev 200 (n + 1)), result (:724-728)
+//   Powell hybrid   MINPACK hybrd / enorm / qrfac / qform / dogleg / r1updt / r1mpyq,
+//                   restated from the published algorithm (More, Garbow, Hillstrom,
+//                   ANL-80-74, 1980).  The Jacobian is evaluated exactly by forward-mode
+//                   automatic differentiation of F (the reference hand-codes the same
+//                   derivative, closest_points.hpp:385-603).
+#pragma once
+
+#include <algorithm>
+#include <array>
+#include <cmath>
+#include <vector>
+
+namespace se_oracle {
+
+using v3 = std::array<double, 3>;
+using m33 = std::array<std::array<double, 3>, 3>;
+
+// A superellipsoid: centre, semi-axes, squareness (e = squareness(0), n = squareness(1)), rotation.
+struct SE {
+    v3 c, r;
+    double e, n;
+    m33 R;
+};
+
+inline double sgn(double x) { return (x > 0) ? 1.0 : ((x < 0) ? -1.0 : 0.0); }
+
+inline v3 rot(const m33& R, const v3& x) {
+    return {R[0][0] * x[0] + R[0][1] * x[1] + R[0][2] * x[2], R[1][0] * x[0] + R[1][1] * x[1] + R[1][2] * x[2],
+            R[2][0] * x[0] + R[2][1] * x[1] + R[2][2] * x[2]};
+}
+
+// superellipsoid::point(a, b): superellipsoid.hpp:292-300
+inline v3 se_point(const SE& s, double a, double b) {
+    v3 pt;
+    pt[0] = s.r[0] * sgn(std::cos(a)) * std::pow(std::abs(std::cos(a)), s.n) * sgn(std::cos(b)) *
+            std::pow(std::abs(std::cos(b)), s.e);
+    pt[1] = s.r[1] * sgn(std::cos(a)) * std::pow(std::abs(std::cos(a)), s.n) * sgn(std::sin(b)) *
+            std::pow(std::abs(std::sin(b)), s.e);
+    pt[2] = s.r[2] * sgn(std::sin(a)) * std::pow(std::abs(std::sin(a)), s.n);
+    v3 w = rot(s.R, pt);
+    return {w[0] + s.c[0], w[1] + s.c[1], w[2] + s.c[2]};
+}
+
+// superellipsoid::normal(a, b): superellipsoid.hpp:314-325 (unit outward normal)
+inline v3 se_normal(const SE& s, double a, double b) {
+    v3 nn;
+    nn[0] = s.r[1] * s.r[2] * sgn(std::cos(a)) * std::pow(std::abs(std::cos(a)), 2 - s.n) * sgn(std::cos(b)) *
+            std::pow(std::abs(std::cos(b)), 2 - s.e);
+    nn[1] = s.r[0] * s.r[2] * sgn(std::cos(a)) * std::pow(std::abs(std::cos(a)), 2 - s.n) * sgn(std::sin(b)) *
+            std::pow(std::abs(std::sin(b)), 2 - s.e);
+    nn[2] = s.r[0] * s.r[1] * sgn(std::sin(a)) * std::pow(std::abs(std::sin(a)), 2 - s.n);
+    v3 w = rot(s.R, nn);
+    double l = std::sqrt(w[0] * w[0] + w[1] * w[1] + w[2] * w[2]);
+    return {w[0] / l, w[1] / l, w[2] / l};
+}
+
+// xt::linspace<double>(lo, hi, n, endpoint = true)
+inline double linspace(double lo, double hi, int n, int i) { return lo + (hi - lo) / (n - 1) * i; }
+
+// binit_xy(n): superellipsoid.hpp:359-412
+inline std::vector<double> binit_xy(const SE& s, int n) {
+    const double pi = 4 * std::atan(1);
+    std::vector<double> bs;
+    for (int i = 0; i < n; ++i) {
+        const double b_ell = linspace(0, 0.5 * pi, n, i);
+        const double x_ell = s.r[0] * std::cos(b_ell);
+        const double y_ell = s.r[1] * std::sin(b_ell);
+        const double d = std::pow(std::pow(s.r[1] * x_ell, 2.0 / s.e) + std::pow(s.r[0] * y_ell, 2.0 / s.e), s.e / 2.0);
+        const double y_supell = y_ell * s.r[0] * s.r[1] / d;
+        const double sinb = std::max(-1.0, std::min(1.0, std::sqrt(std::pow(std::pow(y_supell / s.r[1], 2), 1.0 / s.e))));
+        double b = std::asin(sinb);
+        if (b > pi) b -= 2 * pi;
+        if (b < -pi) b += 2 * pi;
+        bs.push_back(b);
+        bs.push_back(b + pi / 2);
+        bs.push_back(b - pi);
+        bs.push_back(b - pi / 2);
+    }
+    for (auto& b : bs)
+        if (std::abs(b) < 1.0e-6) b = 0.01;
+    return bs;
+}
+
+// ainit_yz(n): superellipsoid.hpp:415-466
+inline std::vector<double> ainit_yz(const SE& s, int n) {
+    const double pi = 4 * std::atan(1);
+    std::vector<double> as;
+    for (int i = 0; i < n; ++i) {
+        const double a_ell = linspace(0, 0.5 * pi, n, i);
+        const double y_ell = s.r[1] * std::cos(a_ell);
+        const double z_ell = s.r[2] * std::sin(a_ell);
+        const double d = std::pow(std::pow(s.r[2] * y_ell, 2 / s.n) + std::pow(s.r[1] * z_ell, 2 / s.n), s.n / 2);
+        const double z_supell = z_ell * s.r[1] * s.r[2] / d;
+        const double sina = std::max(-1.0, std::min(1.0, std::sqrt(std::pow(std::pow(z_supell / s.r[2], 2), 1 / s.n))));
+        double a = std::asin(sina);
+        if (a > pi) a -= 2 * pi;
+        if (a < -pi) a += 2 * pi;
+        as.push_back(a);
+        as.push_back(a + pi / 2);
+        as.push_back(a - pi);
+        as.push_back(a - pi / 2);
+        for (auto& x : as)
+            if (std::abs(x) < 1.0e-6) x = 0.01;
+    }
+    return as;
+}
+
+// xt::unique: sorted distinct values
+inline std::vector<double> unique_sorted(std::vector<double> v) {
+    std::sort(v.begin(), v.end());
+    v.erase(std::unique(v.begin(), v.end()), v.end());
+    return v;
+}
+
+// ------------------------------------------------------------------ forward-mode AD
+struct D4 {
+    double v;
+    double d[4];
+};
+inline D4 cst(double x) { return D4{x, {0, 0, 0, 0}}; }
+inline D4 var(double x, int k) {
+    D4 r = cst(x);
+    r.d[k] = 1.0;
+    return r;
+}
+inline D4 operator+(const D4& a, const D4& b) {
+    D4 r{a.v + b.v, {}};
+    for (int k = 0; k < 4; ++k) r.d[k] = a.d[k] + b.d[k];
+    return r;
+}
+inline D4 operator-(const D4& a, const D4& b) {
+    D4 r{a.v - b.v, {}};
+    for (int k = 0; k < 4; ++k) r.d[k] = a.d[k] - b.d[k];
+    return r;
+}
+inline D4 operator-(const D4& a) {
+    D4 r{-a.v, {}};
+    for (int k = 0; k < 4; ++k) r.d[k] = -a.d[k];
+    return r;
+}
+inline D4 operator*(const D4& a, const D4& b) {
+    D4 r{a.v * b.v, {}};
+    for (int k = 0; k < 4; ++k) r.d[k] = a.d[k] * b.v + a.v * b.d[k];
+    return r;
+}
+inline D4 operator*(double s, const D4& a) {
+    D4 r{s * a.v, {}};
+    for (int k = 0; k < 4; ++k) r.d[k] = s * a.d[k];
+    return r;
+}
+inline D4 dsqrt(const D4& a) {
+    const double s = std::sqrt(a.v);
+    D4 r{s, {}};
+    for (int k = 0; k < 4; ++k) r.d[k] = a.d[k] / (2 * s);
+    return r;
+}
+inline D4 dcos(const D4& a) {
+    D4 r{std::cos(a.v), {}};
+    for (int k = 0; k < 4; ++k) r.d[k] = -std::sin(a.v) * a.d[k];
+    return r;
+}
+inline D4 dsin(const D4& a) {
+    D4 r{std::sin(a.v), {}};
+    for (int k = 0; k < 4; ++k) r.d[k] = std::cos(a.v) * a.d[k];
+    return r;
+}
+// sign(x) |x|^p and its derivative p |x|^(p-1) x'
+inline D4 spow(const D4& a, double p) {
+    const double ax = std::abs(a.v);
+    D4 r{sgn(a.v) * std::pow(ax, p), {}};
+    const double dv = p * std::pow(ax, p - 1);
+    for (int k = 0; k < 4; ++k) r.d[k] = dv * a.d[k];
+    return r;
+}
+
+// Residual of closest_points.hpp:284-382 with its Jacobian.
+struct SEPair {
+    SE s1, s2;
+};
+
+inline void se_se_residual(const SEPair& P, const double* u, double* F, double* J /* 4x4 row-major or null */) {
+    D4 a1 = var(u[0], 0), b1 = var(u[1], 1), a2 = var(u[2], 2), b2 = var(u[3], 3);
+    const SE &s1 = P.s1, &s2 = P.s2;
+    D4 ca1 = dcos(a1), sa1 = dsin(a1), cb1 = dcos(b1), sb1 = dsin(b1);
+    D4 ca2 = dcos(a2), sa2 = dsin(a2), cb2 = dcos(b2), sb2 = dsin(b2);
+    D4 ca1n = spow(ca1, s1.n), ca1n2 = spow(ca1, 2 - s1.n), sa1n = spow(sa1, s1.n), sa1n2 = spow(sa1, 2 - s1.n);
+    D4 cb1e = spow(cb1, s1.e), cb1e2 = spow(cb1, 2 - s1.e), sb1e = spow(sb1, s1.e), sb1e2 = spow(sb1, 2 - s1.e);
+    D4 ca2n = spow(ca2, s2.n), ca2n2 = spow(ca2, 2 - s2.n), sa2n = spow(sa2, s2.n), sa2n2 = spow(sa2, 2 - s2.n);
+    D4 cb2e = spow(cb2, s2.e), cb2e2 = spow(cb2, 2 - s2.e), sb2e = spow(sb2, s2.e), sb2e2 = spow(sb2, 2 - s2.e);
+    // body-frame normal (A1..A3) and point (A4..A6) of s1; point (B1..B3) and normal (B4..B6) of s2
+    D4 A[3] = {(s1.r[1] * s1.r[2]) * (ca1n2 * cb1e2), (s1.r[0] * s1.r[2]) * (sb1e2 * ca1n2), (s1.r[0] * s1.r[1]) * sa1n2};
+    D4 Ap[3] = {s1.r[0] * (ca1n * cb1e), s1.r[1] * (sb1e * ca1n), s1.r[2] * sa1n};
+    D4 Bp[3] = {s2.r[0] * (ca2n * cb2e), s2.r[1] * (sb2e * ca2n), s2.r[2] * sa2n};
+    D4 Bn[3] = {(s2.r[1] * s2.r[2]) * (ca2n2 * cb2e2), (s2.r[0] * s2.r[2]) * (sb2e2 * ca2n2), (s2.r[0] * s2.r[1]) * sa2n2};
+    D4 n1[3], n2[3], d[3];
+    for (int k = 0; k < 3; ++k) {
+        n1[k] = s1.R[k][0] * A[0] + s1.R[k][1] * A[1] + s1.R[k][2] * A[2];
+        n2[k] = s2.R[k][0] * Bn[0] + s2.R[k][1] * Bn[1] + s2.R[k][2] * Bn[2];
+        D4 p1 = s1.R[k][0] * Ap[0] + s1.R[k][1] * Ap[1] + s1.R[k][2] * Ap[2];
+        D4 p2 = s2.R[k][0] * Bp[0] + s2.R[k][1] * Bp[1] + s2.R[k][2] * Bp[2];
+        d[k] = (p2 - p1) + cst(s2.c[k] - s1.c[k]);
+    }
+    D4 f[4];
+    f[0] = -(n1[1] * d[2]) + n1[2] * d[1];
+    f[1] = n1[0] * d[2] - n1[2] * d[0];
+    f[2] = -(n1[0] * d[1]) + n1[1] * d[0];
+    D4 dotn = n1[0] * n2[0] + n1[1] * n2[1] + n1[2] * n2[2];
+    D4 l1 = dsqrt(n1[0] * n1[0] + n1[1] * n1[1] + n1[2] * n1[2]);
+    D4 l2 = dsqrt(n2[0] * n2[0] + n2[1] * n2[1] + n2[2] * n2[2]);
+    f[3] = dotn + l1 * l2;
+    for (int i = 0; i < 4; ++i) {
+        F[i] = f[i].v;
+        if (J)
+            for (int k = 0; k < 4; ++k) J[4 * i + k] = f[i].d[k];
+    }
+}
+
+// ------------------------------------------------------------------ MINPACK hybrd (n = 4)
+constexpr int N = 4;
+constexpr int LR = N * (N + 1) / 2;
+
+inline double enorm(int n, const double* x) {
+    const double rdwarf = 3.834e-20, rgiant = 1.304e19;
+    double s1 = 0, s2 = 0, s3 = 0, x1max = 0, x3max = 0;
+    const double agiant = rgiant / n;
+    for (int i = 0; i < n; ++i) {
+        const double xabs = std::abs(x[i]);
+        if (xabs > rdwarf && xabs < agiant) {
+            s2 += xabs * xabs;
+        } else if (xabs <= rdwarf) {
+            if (xabs > x3max) {
+                s3 = 1.0 + s3 * (x3max / xabs) * (x3max / xabs);
+                x3max = xabs;
+            } else if (xabs != 0.0) {
+                s3 += (xabs / x3max) * (xabs / x3max);
+            }
+        } else {
+            if (xabs > x1max) {
+                s1 = 1.0 + s1 * (x1max / xabs) * (x1max / xabs);
+                x1max = xabs;
+            } else {
+                s1 += (xabs / x1max) * (xabs / x1max);
+            }
+        }
+    }
+    if (s1 != 0.0) return x1max * std::sqrt(s1 + (s2 / x1max) / x1max);
+    if (s2 != 0.0) {
+        if (s2 >= x3max) return std::sqrt(s2 * (1.0 + (x3max / s2) * (x3max * s3)));
+        return std::sqrt(x3max * ((s2 / x3max) + (x3max * s3)));
+    }
+    return x3max * std::sqrt(s3);
+}
+
+// a is column-major n x n (a[i + n*j])
+inline void qrfac(double* a, double* rdiag, double* acnorm) {
+    for (int j = 0; j < N; ++j) {
+        acnorm[j] = enorm(N, a + N * j);
+        rdiag[j] = acnorm[j];
+    }
+    for (int j = 0; j < N; ++j) {
+        double ajnorm = enorm(N - j, a + j + N * j);
+        if (ajnorm != 0.0) {
+            if (a[j + N * j] < 0.0) ajnorm = -ajnorm;
+            for (int i = j; i < N; ++i) a[i + N * j] /= ajnorm;
+            a[j + N * j] += 1.0;
+            for (int k = j + 1; k < N; ++k) {
+                double sum = 0.0;
+                for (int i = j; i < N; ++i) sum += a[i + N * j] * a[i + N * k];
+                const double temp = sum / a[j + N * j];
+                for (int i = j; i < N; ++i) a[i + N * k] -= temp * a[i + N * j];
+            }
+        }
+        rdiag[j] = -ajnorm;
+    }
+}
+
+inline void qform(double* q, double* wa) {
+    for (int j = 1; j < N; ++j)
+        for (int i = 0; i < j; ++i) q[i + N * j] = 0.0;
+    for (int l = 0; l < N; ++l) {
+        const int k = N - 1 - l;
+        for (int i = k; i < N; ++i) {
+            wa[i] = q[i + N * k];
+            q[i + N * k] = 0.0;
+        }
+        q[k + N * k] = 1.0;
+        if (wa[k] != 0.0) {
+            for (int j = k; j < N; ++j) {
+                double sum = 0.0;
+                for (int i = k; i < N; ++i) sum += q[i + N * j] * wa[i];
+                const double temp = sum / wa[k];
+                for (int i = k; i < N; ++i) q[i + N * j] -= temp * wa[i];
+            }
+        }
+    }
+}
+
+// r: upper triangle packed by rows (r[l] with row-major order); 1-based index arithmetic kept.
+inline void dogleg(const double* r, const double* diag, const double* qtb, double delta, double* x, double* wa1,
+                   double* wa2) {
+    const double epsmch = 2.220446049250313e-16;
+    int jj = (N * (N + 1)) / 2 + 1;
+    for (int k = 1; k <= N; ++k) {
+        const int j = N - k + 1;
+        const int jp1 = j + 1;
+        jj -= k;
+        int l = jj + 1;
+        double sum = 0.0;
+        for (int i = jp1; i <= N; ++i) {
+            sum += r[l - 1] * x[i - 1];
+            ++l;
+        }
+        double temp = r[jj - 1];
+        if (temp == 0.0) {
+            l = j;
+            for (int i = 1; i <= j; ++i) {
+                temp = std::max(temp, std::abs(r[l - 1]));
+                l += N - i;
+            }
+            temp = epsmch * temp;
+            if (temp == 0.0) temp = epsmch;
+        }
+        x[j - 1] = (qtb[j - 1] - sum) / temp;
+    }
+    for (int j = 0; j < N; ++j) {
+        wa1[j] = 0.0;
+        wa2[j] = diag[j] * x[j];
+    }
+    const double qnorm = enorm(N, wa2);
+    if (qnorm <= delta) return;
+    int l = 1;
+    for (int j = 1; j <= N; ++j) {
+        const double temp = qtb[j - 1];
+        for (int i = j; i <= N; ++i) {
+            wa1[i - 1] += r[l - 1] * temp;
+            ++l;
+        }
+        wa1[j - 1] /= diag[j - 1];
+    }
+    const double gnorm = enorm(N, wa1);
+    double sgnorm = 0.0;
+    double alpha = delta / qnorm;
+    if (gnorm != 0.0) {
+        for (int j = 0; j < N; ++j) wa1[j] = (wa1[j] / gnorm) / diag[j];
+        l = 1;
+        for (int j = 1; j <= N; ++j) {
+            double sum = 0.0;
+            for (int i = j; i <= N; ++i) {
+                sum += r[l - 1] * wa1[i - 1];
+                ++l;
+            }
+            wa2[j - 1] = sum;
+        }
+        const double temp = enorm(N, wa2);
+        sgnorm = (gnorm / temp) / temp;
+        alpha = 0.0;
+        if (sgnorm < delta) {
+            const double bnorm = enorm(N, qtb);
+            double t = (bnorm / gnorm) * (bnorm / qnorm) * (sgnorm / delta);
+            t = t - (delta / qnorm) * (sgnorm / delta) * (sgnorm / delta) +
+                std::sqrt((t - (delta / qnorm)) * (t - (delta / qnorm)) +
+                          (1.0 - (delta / qnorm) * (delta / qnorm)) * (1.0 - (sgnorm / delta) * (sgnorm / delta)));
+            alpha = ((delta / qnorm) * (1.0 - (sgnorm / delta) * (sgnorm / delta))) / t;
+        }
+    }
+    const double temp = (1.0 - alpha) * std::min(sgnorm, delta);
+    for (int j = 0; j < N; ++j) x[j] = temp * wa1[j] + alpha * x[j];
+}
+
+// Rank-1 update of the packed triangular s (m = n = N); 1-based index arithmetic kept.
+inline void r1updt(double* s, const double* u, double* v, double* w, bool& sing) {
+    const double giant = 1.79769313486231570e308;
+    const int m = N, n = N;
+    int jj = (n * (2 * m - n + 1)) / 2 - (m - n);
+    int l = jj;
+    for (int i = n; i <= m; ++i) {
+        w[i - 1] = s[l - 1];
+        ++l;
+    }
+    for (int nmj = 1; nmj <= n - 1; ++nmj) {
+        const int j = n - nmj;
+        jj -= m - j + 1;
+        w[j - 1] = 0.0;
+        if (v[j - 1] != 0.0) {
+            double cs, sn, tau;
+            if (std::abs(v[n - 1]) < std::abs(v[j - 1])) {
+                const double cotan = v[n - 1] / v[j - 1];
+                sn = 0.5 / std::sqrt(0.25 + 0.25 * cotan * cotan);
+                cs = sn * cotan;
+                tau = 1.0;
+                if (std::abs(cs) * giant > 1.0) tau = 1.0 / cs;
+            } else {
+                const double tn = v[j - 1] / v[n - 1];
+                cs = 0.5 / std::sqrt(0.25 + 0.25 * tn * tn);
+                sn = cs * tn;
+                tau = sn;
+            }
+            v[n - 1] = sn * v[j - 1] + cs * v[n - 1];
+            v[j - 1] = tau;
+            l = jj;
+            for (int i = j; i <= m; ++i) {
+                const double temp = cs * s[l - 1] - sn * w[i - 1];
+                w[i - 1] = sn * s[l - 1] + cs * w[i - 1];
+                s[l - 1] = temp;
+                ++l;
+            }
+        }
+    }
+    for (int i = 1; i <= m; ++i) w[i - 1] += v[n - 1] * u[i - 1];
+    sing = false;
+    for (int j = 1; j <= n - 1; ++j) {
+        if (w[j - 1] != 0.0) {
+            double cs, sn, tau;
+            if (std::abs(s[jj - 1]) < std::abs(w[j - 1])) {
+                const double cotan = s[jj - 1] / w[j - 1];
+                sn = 0.5 / std::sqrt(0.25 + 0.25 * cotan * cotan);
+                cs = sn * cotan;
+                tau = 1.0;
+                if (std::abs(cs) * giant > 1.0) tau = 1.0 / cs;
+            } else {
+                const double tn = w[j - 1] / s[jj - 1];
+                cs = 0.5 / std::sqrt(0.25 + 0.25 * tn * tn);
+                sn = cs * tn;
+                tau = sn;
+            }
+            l = jj;
+            for (int i = j; i <= m; ++i) {
+                const double temp = cs * s[l - 1] + sn * w[i - 1];
+                w[i - 1] = -sn * s[l - 1] + cs * w[i - 1];
+                s[l - 1] = temp;
+                ++l;
+            }
+            w[j - 1] = tau;
+        }
+        if (s[jj - 1] == 0.0) sing = true;
+        jj += m - j + 1;
+    }
+    l = jj;
+    for (int i = n; i <= m; ++i) {
+        s[l - 1] = w[i - 1];
+        ++l;
+    }
+    if (s[jj - 1] == 0.0) sing = true;
+}
+
+// a is column-major m x n with leading dimension lda
+inline void r1mpyq(int m, int n, double* a, int lda, const double* v, const double* w) {
+    for (int nmj = 1; nmj <= n - 1; ++nmj) {
+        const int j = n - nmj;
+        double cs, sn;
+        if (std::abs(v[j - 1]) > 1.0) {
+            cs = 1.0 / v[j - 1];
+            sn = std::sqrt(1.0 - cs * cs);
+        } else {
+            sn = v[j - 1];
+            cs = std::sqrt(1.0 - sn * sn);
+        }
+        for (int i = 1; i <= m; ++i) {
+            const double temp = cs * a[(i - 1) + lda * (j - 1)] - sn * a[(i - 1) + lda * (n - 1)];
+            a[(i - 1) + lda * (n - 1)] = sn * a[(i - 1) + lda * (j - 1)] + cs * a[(i - 1) + lda * (n - 1)];
+            a[(i - 1) + lda * (j - 1)] = temp;
+        }
+    }
+    for (int j = 1; j <= n - 1; ++j) {
+        double cs, sn;
+        if (std::abs(w[j - 1]) > 1.0) {
+            cs = 1.0 / w[j - 1];
+            sn = std::sqrt(1.0 - cs * cs);
+        } else {
+            sn = w[j - 1];
+            cs = std::sqrt(1.0 - sn * sn);
+        }
+        for (int i = 1; i <= m; ++i) {
+            const double temp = cs * a[(i - 1) + lda * (j - 1)] + sn * a[(i - 1) + lda * (n - 1)];
+            a[(i - 1) + lda * (n - 1)] = -sn * a[(i - 1) + lda * (j - 1)] + cs * a[(i - 1) + lda * (n - 1)];
+            a[(i - 1) + lda * (j - 1)] = temp;
+        }
+    }
+}
+
+// hybrd1 (minpack.hpp:851-923) with the analytic Jacobian: returns info (1: converged).
+template <class Fn>
+inline int hybrd1(Fn fcn, double* x, int* nfev_out = nullptr) {
+    const double xtol = 1.0e-10, factor = 100.0, epsmch = 2.220446049250313e-16;
+    const int maxfev = 200 * (N + 1);
+    const double p1 = 0.1, p5 = 0.5, p001 = 0.001, p0001 = 0.0001;
+    double diag[N] = {1.0, 1.0, 1.0, 1.0};  // mode 2: wa[0..n) = 1 (minpack.hpp:881-884)
+    double fvec[N], fjac[N * N], r[LR], qtf[N], wa1[N], wa2[N], wa3[N], wa4[N];
+    int info = 0, nfev = 0;
+    fcn(x, fvec, nullptr);
+    nfev = 1;
+    double fnorm = enorm(N, fvec);
+    const int msum = N;
+    int iter = 1, ncsuc = 0, ncfail = 0, nslow1 = 0, nslow2 = 0;
+    double delta = 0.0, xnorm = 0.0;
+    for (;;) {
+        bool jeval = true;
+        double Jrow[N * N];
+        double ftmp[N];
+        fcn(x, ftmp, Jrow);
+        for (int i = 0; i < N; ++i)
+            for (int j = 0; j < N; ++j) fjac[i + N * j] = Jrow[N * i + j];
+        nfev += msum;
+        qrfac(fjac, wa1, wa2);
+        if (iter == 1) {
+            for (int j = 0; j < N; ++j) wa3[j] = diag[j] * x[j];
+            xnorm = enorm(N, wa3);
+            delta = factor * xnorm;
+            if (delta == 0.0) delta = factor;
+        }
+        for (int i = 0; i < N; ++i) qtf[i] = fvec[i];
+        for (int j = 0; j < N; ++j) {
+            if (fjac[j + N * j] != 0.0) {
+                double sum = 0.0;
+                for (int i = j; i < N; ++i) sum += fjac[i + N * j] * qtf[i];
+                const double temp = -sum / fjac[j + N * j];
+                for (int i = j; i < N; ++i) qtf[i] += fjac[i + N * j] * temp;
+            }
+        }
+        bool sing = false;
+        for (int j = 0; j < N; ++j) {
+            int l = j;
+            for (int i = 0; i < j; ++i) {
+                r[l] = fjac[i + N * j];
+                l += N - i - 1;
+            }
+            r[l] = wa1[j];
+            if (wa1[j] == 0.0) sing = true;
+        }
+        (void)sing;
+        qform(fjac, wa1);
+        for (;;) {
+            dogleg(r, diag, qtf, delta, wa1, wa2, wa3);
+            for (int j = 0; j < N; ++j) {
+                wa1[j] = -wa1[j];
+                wa2[j] = x[j] + wa1[j];
+                wa3[j] = diag[j] * wa1[j];
+            }
+            const double pnorm = enorm(N, wa3);
+            if (iter == 1) delta = std::min(delta, pnorm);
+            fcn(wa2, wa4, nullptr);
+            ++nfev;
+            const double fnorm1 = enorm(N, wa4);
+            double actred = -1.0;
+            if (fnorm1 < fnorm) actred = 1.0 - (fnorm1 / fnorm) * (fnorm1 / fnorm);
+            int l = 0;
+            for (int i = 0; i < N; ++i) {
+                double sum = 0.0;
+                for (int j = i; j < N; ++j) {
+                    sum += r[l] * wa1[j];
+                    ++l;
+                }
+                wa3[i] = qtf[i] + sum;
+            }
+            const double temp = enorm(N, wa3);
+            double prered = 0.0;
+            if (temp < fnorm) prered = 1.0 - (temp / fnorm) * (temp / fnorm);
+            double ratio = 0.0;
+            if (prered > 0.0) ratio = actred / prered;
+            if (ratio < p1) {
+                ncsuc = 0;
+                ++ncfail;
+                delta = p5 * delta;
+            } else {
+                ncfail = 0;
+                ++ncsuc;
+                if (ratio >= p5 || ncsuc > 1) delta = std::max(delta, pnorm / p5);
+                if (std::abs(ratio - 1.0) <= p1) delta = pnorm / p5;
+            }
+            if (ratio >= p0001) {
+                for (int j = 0; j < N; ++j) {
+                    x[j] = wa2[j];
+                    wa2[j] = diag[j] * x[j];
+                    fvec[j] = wa4[j];
+                }
+                xnorm = enorm(N, wa2);
+                fnorm = fnorm1;
+                ++iter;
+            }
+            ++nslow1;
+            if (actred >= p001) nslow1 = 0;
+            if (jeval) ++nslow2;
+            if (actred >= p1) nslow2 = 0;
+            if (delta <= xtol * xnorm || fnorm == 0.0) info = 1;
+            if (info != 0) break;
+            if (maxfev <= nfev) info = 2;
+            if (p1 * std::max(p1 * delta, pnorm) <= epsmch * xnorm) info = 3;
+            if (nslow2 == 5) info = 4;
+            if (nslow1 == 10) info = 5;
+            if (info != 0) break;
+            if (ncfail == 2) break;
+            for (int j = 0; j < N; ++j) {
+                double sum = 0.0;
+                for (int i = 0; i < N; ++i) sum += fjac[i + N * j] * wa4[i];
+                wa2[j] = (sum - wa3[j]) / pnorm;
+                wa1[j] = diag[j] * ((diag[j] * wa1[j]) / pnorm);
+                if (ratio >= p0001) qtf[j] = sum;
+            }
+            bool sg = false;
+            r1updt(r, wa1, wa2, wa3, sg);
+            r1mpyq(N, N, fjac, N, wa2, wa3);
+            r1mpyq(1, N, qtf, 1, wa2, wa3);
+            jeval = false;
+        }
+        if (info != 0) break;
+    }
+    if (nfev_out) *nfev_out = nfev;
+    if (info == 5) info = 4;  // hybrd1 (minpack.hpp:918-921)
+    return info;
+}
+
+// closest_points(superellipsoid<3>, superellipsoid<3>): closest_points.hpp:263-733.
+// Returns the reference's (pi, pj, nij, dij); u holds the root and info hybrd's code.
+struct SEResult {
+    v3 pi, pj, nij;
+    double dij;
+    double u[4];
+    int info;
+    int nfev;
+};
+
+inline SEResult se_se(const SE& s1, const SE& s2) {
+    const int num = 6;
+    std::vector<double> b1 = unique_sorted(binit_xy(s1, num)), a1 = unique_sorted(ainit_yz(s1, num));
+    std::vector<double> b2 = unique_sorted(binit_xy(s2, num)), a2 = unique_sorted(ainit_yz(s2, num));
+    // meshgrid(ainit, binit) flattened row-major: k -> (a[k / nb], b[k % nb])
+    const int n1 = (int)(a1.size() * b1.size()), n2 = (int)(a2.size() * b2.size());
+    std::vector<v3> P1(n1), N1(n1), P2(n2), N2(n2);
+    for (int k = 0; k < n1; ++k) {
+        const double a = a1[k / b1.size()], b = b1[k % b1.size()];
+        P1[k] = se_point(s1, a, b);
+        N1[k] = se_normal(s1, a, b);
+    }
+    for (int k = 0; k < n2; ++k) {
+        const double a = a2[k / b2.size()], b = b2[k % b2.size()];
+        P2[k] = se_point(s2, a, b);
+        N2[k] = se_normal(s2, a, b);
+    }
+    double dmin = INFINITY;
+    int imin = 0, jmin = 0;
+    for (int i = 0; i < n1; ++i)
+        for (int j = 0; j < n2; ++j) {
+            const double dx = P1[i][0] - P2[j][0], dy = P1[i][1] - P2[j][1], dz = P1[i][2] - P2[j][2];
+            const double dist = std::sqrt(dx * dx + dy * dy + dz * dz) +
+                                2 * (1 + (N1[i][0] * N2[j][0] + N1[i][1] * N2[j][1] + N1[i][2] * N2[j][2]));
+            if (dist < dmin) {  // first minimum in row-major order (xt::where)
+                dmin = dist;
+                imin = i;
+                jmin = j;
+            }
+        }
+    SEResult res;
+    res.u[0] = a1[imin / b1.size()];
+    res.u[1] = b1[imin % b1.size()];
+    res.u[2] = a2[jmin / b2.size()];
+    res.u[3] = b2[jmin % b2.size()];
+    SEPair pair{s1, s2};
+    res.info = hybrd1([&](const double* u, double* F, double* J) { se_se_residual(pair, u, F, J); }, res.u, &res.nfev);
+    res.pi = se_point(s1, res.u[0], res.u[1]);
+    res.pj = se_point(s2, res.u[2], res.u[3]);
+    res.nij = se_normal(s2, res.u[2], res.u[3]);
+    const v3 d = {res.pi[0] - res.pj[0], res.pi[1] - res.pj[1], res.pi[2] - res.pj[2]};
+    const double sd = sgn(d[0] * res.nij[0] + d[1] * res.nij[1] + d[2] * res.nij[2]);
+    res.dij = sd * std::sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
+    return res;
+}
+
+}  // namespace se_oracle
