@@ -106,6 +106,7 @@ def main():
     ap.add_argument("--nx", type=int, default=64)
     ap.add_argument("--ny", type=int, default=64)
     ap.add_argument("--nz", type=int, default=32)
+    ap.add_argument("--tune", action="store_true", help="sweep the solver's launch shapes before timing")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py runs on one GPU only: the contact step of one scene is not sharded across GPUs "
@@ -125,6 +126,12 @@ def main():
     log(f"[bench] warmup {args.warmup} steps")
     if args.warmup:
         g.run(prm, dt, args.warmup)
+    if args.tune:
+        from scopi_amd import tune
+        g.compute_contacts(prm)
+        g.set_a_priori_velocity(prm, dt)
+        tuned = tune.tune(g, prm, dt)
+        log(f"[bench] tuned launch shapes: {tuned}")
     g.set_kernel_timing(True)
     torch.cuda.synchronize()
     log(f"[bench] timing {args.steps} steps")

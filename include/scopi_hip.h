@@ -154,6 +154,13 @@ int scopi_hip_run(scopi_hip_ctx* ctx, const scopi_hip_params* params, double dt,
  * ms of the trial kernel, out[5] = launches.  Timing is enabled by
  * scopi_hip_set_kernel_timing(ctx, 1); it serialises the solve loop. */
 int scopi_hip_set_kernel_timing(scopi_hip_ctx* ctx, int32_t enable);
+
+/* Launch shapes of the projected-gradient kernels (the hook the autotuner in
+ * scopi_amd.tune drives; no counterpart in the reference):
+ *   SCOPI_TUNE_REDUCE_BLOCKS: blocks of 256 threads of the per-contact kernels (1..2048, default 2048);
+ *   SCOPI_TUNE_BT_GROUP: lanes sharing one particle in the B^T gather (1, 2, 4, 8 or 16, default 8). */
+enum { SCOPI_TUNE_REDUCE_BLOCKS = 1, SCOPI_TUNE_BT_GROUP = 2 };
+int scopi_hip_set_tuning(scopi_hip_ctx* ctx, int32_t key, int32_t value);
 int scopi_hip_kernel_timing(scopi_hip_ctx* ctx, double* out);
 
 #ifdef __cplusplus

@@ -331,10 +331,13 @@ __global__ void __launch_bounds__(kBlock) k_trial(const ApgdCtrl* __restrict__ c
         for (int k = 0; k < 8; ++k) partial[blockIdx.x * kPartialStride + k] = s[k];
 }
 
-// w = M^-1 B^T lambda (ATMatrix::mat_mult, velocities.hpp:167-254, then M^-1), one
-// thread per active particle over its entries of the particle-major copy.  In the
-// loop, block 0 also reduces the trial kernel's block sums into ctrl->red.
-template <class T, int LAY>
+// w = M^-1 B^T lambda (ATMatrix::mat_mult, velocities.hpp:167-254, then M^-1).
+// G consecutive lanes share one active particle: lane k of the group takes its
+// entries k, k + G, ... of the particle-major copy (so a wavefront reads 64/G
+// contiguous runs of G entries per load), then the group sums its partial
+// results with lane shuffles.  In the loop, block 0 also reduces the trial
+// kernel's block sums into ctrl->red.
+template <class T, int LAY, int G>
 __global__ void __launch_bounds__(kBlock) k_bt(ApgdCtrl* __restrict__ ctrl, int use_slot, BMatrix<T> B, int64_t na,
                                               const int* __restrict__ tstart, const int32_t* __restrict__ tc,
                                               const double* __restrict__ partial_trial, int ntrial) {
@@ -349,15 +352,16 @@ __global__ void __launch_bounds__(kBlock) k_bt(ApgdCtrl* __restrict__ ctrl, int 
             ctrl->red[8] = tot[7];
         }
     }
-    const int64_t p = (int64_t)blockIdx.x * kBlock + threadIdx.x;
-    if (p >= na) return;
+    const int64_t p = ((int64_t)blockIdx.x * kBlock + threadIdx.x) / G;
+    const int sub = (int)(threadIdx.x % G);
+    const bool valid = p < na;
     const int s = (use_slot >= 0) ? use_slot : ctrl->trial;
     const T* lam = slot<T>(B.lam, s, NR * B.cap);
     T acc[NW];
 #pragma unroll
     for (int k = 0; k < NW; ++k) acc[k] = (T)0;
-    const int e0 = tstart[p], e1 = tstart[p + 1];
-    for (int e = e0; e < e1; ++e) {
+    const int e0 = valid ? tstart[p] : 0, e1 = valid ? tstart[p + 1] : 0;
+    for (int e = e0 + sub; e < e1; e += G) {
         const int32_t cs = tc[e];
         const int64_t c = cs < 0 ? ~cs : cs;
         const T* tv = B.tcoef + (int64_t)e * NT;
@@ -377,7 +381,35 @@ __global__ void __launch_bounds__(kBlock) k_bt(ApgdCtrl* __restrict__ ctrl, int 
         }
     }
 #pragma unroll
-    for (int k = 0; k < NW; ++k) B.w[p * NW + k] = B.minv[p * NW + k] * acc[k];
+    for (int o = G / 2; o > 0; o >>= 1)
+#pragma unroll
+        for (int k = 0; k < NW; ++k) acc[k] += __shfl_xor(acc[k], o, G);
+    if (valid && sub == 0)
+#pragma unroll
+        for (int k = 0; k < NW; ++k) B.w[p * NW + k] = B.minv[p * NW + k] * acc[k];
+}
+
+template <class T, int LAY>
+static void launch_bt(int G, hipStream_t s, ApgdCtrl* ctrl, int use_slot, const BMatrix<T>& B, int64_t na,
+                      const int* tstart, const int32_t* tc, const double* partial, int ntrial) {
+    const unsigned grid = grid_for(na * (int64_t)G);
+    switch (G) {
+        case 1:
+            hipLaunchKernelGGL((k_bt<T, LAY, 1>), dim3(grid), dim3(kBlock), 0, s, ctrl, use_slot, B, na, tstart, tc, partial, ntrial);
+            break;
+        case 2:
+            hipLaunchKernelGGL((k_bt<T, LAY, 2>), dim3(grid), dim3(kBlock), 0, s, ctrl, use_slot, B, na, tstart, tc, partial, ntrial);
+            break;
+        case 4:
+            hipLaunchKernelGGL((k_bt<T, LAY, 4>), dim3(grid), dim3(kBlock), 0, s, ctrl, use_slot, B, na, tstart, tc, partial, ntrial);
+            break;
+        case 16:
+            hipLaunchKernelGGL((k_bt<T, LAY, 16>), dim3(grid), dim3(kBlock), 0, s, ctrl, use_slot, B, na, tstart, tc, partial, ntrial);
+            break;
+        default:
+            hipLaunchKernelGGL((k_bt<T, LAY, 8>), dim3(grid), dim3(kBlock), 0, s, ctrl, use_slot, B, na, tstart, tc, partial, ntrial);
+            break;
+    }
 }
 
 template <class T, int LAY>
@@ -696,8 +728,7 @@ void Solver::run_phase(BMatrix<T>& B, const Particles& P, const Contacts& C, int
                        const scopi_hip_params& prm, double dt, double tol, double rtol, int64_t max_ite, bool warm,
                        bool keep_L, hipStream_t s, scopi_hip_solve_info& info) {
     constexpr int NR = lay<LAY>::NR;
-    const unsigned gred = (unsigned)std::min<int64_t>(grid_for(nc), kRedBlocksMax);
-    const unsigned gpart = grid_for(na);
+    const unsigned gred = (unsigned)std::min<int64_t>(grid_for(nc), reduce_blocks_);
     const int64_t ni = P.n_inactive;
     const int64_t stride = NR * B.cap;
     const double dt2 = dt * dt;
@@ -707,8 +738,7 @@ void Solver::run_phase(BMatrix<T>& B, const Particles& P, const Contacts& C, int
     } else {
         // lambda_0 is in slot 0: lambda_{-1} = lambda_0 and both Q products
         SCOPI_HIP_CHECK(hipMemcpyAsync(B.lam + stride, B.lam, stride * sizeof(T), hipMemcpyDeviceToDevice, s));
-        hipLaunchKernelGGL((k_bt<T, LAY>), dim3(gpart), dim3(kBlock), 0, s, (ApgdCtrl*)nullptr, 0, B, na, tstart_, tc_,
-                           (const double*)nullptr, 0);
+        launch_bt<T, LAY>(bt_group_, s, (ApgdCtrl*)nullptr, 0, B, na, tstart_, tc_, nullptr, 0);
         hipLaunchKernelGGL((k_row_plain<T, LAY>), dim3(gred), dim3(kBlock), 0, s, B, 0, nc, (const int32_t*)C.i,
                            (const int32_t*)C.j, ni, dt2);
         SCOPI_HIP_CHECK(hipMemcpyAsync(B.qlam + stride, B.qlam, stride * sizeof(T), hipMemcpyDeviceToDevice, s));
@@ -731,8 +761,7 @@ void Solver::run_phase(BMatrix<T>& B, const Particles& P, const Contacts& C, int
             hipLaunchKernelGGL((k_trial<T, LAY>), dim3(gred), dim3(kBlock), 0, s, (const ApgdCtrl*)ctrl_, B, nc,
                                partial_trial_);
             if (e) SCOPI_HIP_CHECK(hipEventRecord(e[1], s));
-            hipLaunchKernelGGL((k_bt<T, LAY>), dim3(gpart), dim3(kBlock), 0, s, ctrl_, -1, B, na, tstart_, tc_,
-                               (const double*)partial_trial_, (int)gred);
+            launch_bt<T, LAY>(bt_group_, s, ctrl_, -1, B, na, tstart_, tc_, partial_trial_, (int)gred);
             if (e) SCOPI_HIP_CHECK(hipEventRecord(e[2], s));
             hipLaunchKernelGGL((k_row<T, LAY>), dim3(gred), dim3(kBlock), 0, s, (const ApgdCtrl*)ctrl_, B, nc,
                                (const int32_t*)C.i, (const int32_t*)C.j, ni, partial_row_);
@@ -822,11 +851,24 @@ void Solver::solve_layout(const Particles& P, const Contacts& C, int64_t nc, con
         run_phase<double, LAY>(b64_, P, C, na, nc, prm, dt, prm.tolerance, 0.0, prm.max_ite, false, false, s, info);
     }
     // velocities from the final lambda (FP64)
-    hipLaunchKernelGGL((k_bt<double, LAY>), dim3(grid_for(na)), dim3(kBlock), 0, s, (ApgdCtrl*)nullptr, final_slot_,
-                       b64_, na, tstart_, tc_, (const double*)nullptr, 0);
+    launch_bt<double, LAY>(bt_group_, s, (ApgdCtrl*)nullptr, final_slot_, b64_, na, tstart_, tc_, nullptr, 0);
     hipLaunchKernelGGL(k_velocities<NW>, dim3(grid_for(na)), dim3(kBlock), 0, s, P, (const double*)b64_.w, pcap_, dt, 1,
                        u_, w_);
     SCOPI_HIP_CHECK(hipGetLastError());
+}
+
+int Solver::set_tuning(int key, int value) {
+    if (key == SCOPI_TUNE_REDUCE_BLOCKS) {
+        if (value < 1 || value > kRedBlocksMax) return SCOPI_HIP_ERR_ARG;
+        reduce_blocks_ = value;
+        return SCOPI_HIP_OK;
+    }
+    if (key == SCOPI_TUNE_BT_GROUP) {
+        if (value != 1 && value != 2 && value != 4 && value != 8 && value != 16) return SCOPI_HIP_ERR_ARG;
+        bt_group_ = value;
+        return SCOPI_HIP_OK;
+    }
+    return SCOPI_HIP_ERR_ARG;
 }
 
 int Solver::solve(const Particles& P, const Contacts& C, int64_t nc, const scopi_hip_params& prm, double dt,

@@ -85,6 +85,8 @@ class Solver {
     // [ms_bt, n_bt, ms_row, n_row, ms_trial, n_trial]
     void timing(double* out) const;
     int layout() const { return layout_; }
+    // launch-shape tuning (SCOPI_TUNE_* keys of scopi_hip.h)
+    int set_tuning(int key, int value);
 
   private:
     void reserve(int64_t na, int64_t nc, bool fp32);
@@ -123,6 +125,8 @@ class Solver {
     int nr_ = 1;
     int final_slot_ = 0;
     int layout_ = kRotFree;
+    int reduce_blocks_ = 2048;  // grid of the per-contact kernels (grid-stride)
+    int bt_group_ = 8;          // lanes per particle in k_bt
     bool has_solution_ = false;
     // kernel timing (HIP events on the solve stream)
     bool timing_ = false;

@@ -503,6 +503,13 @@ int scopi_hip_set_kernel_timing(scopi_hip_ctx* ctx, int32_t enable) {
     return SCOPI_HIP_OK;
 }
 
+int scopi_hip_set_tuning(scopi_hip_ctx* ctx, int32_t key, int32_t value) {
+    if (!ctx) return SCOPI_HIP_ERR_ARG;
+    const int rc = ctx->solver.set_tuning(key, value);
+    if (rc != SCOPI_HIP_OK) return fail(ctx, rc, "set_tuning: unknown key or value out of range");
+    return SCOPI_HIP_OK;
+}
+
 int scopi_hip_kernel_timing(scopi_hip_ctx* ctx, double* out) {
     if (!ctx || !out) return SCOPI_HIP_ERR_ARG;
     ctx->solver.timing(out);

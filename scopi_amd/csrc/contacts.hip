@@ -95,17 +95,37 @@ __global__ void __launch_bounds__(kBlock) k_bbox_partial(Particles P, double* __
     if (threadIdx.x < 7) partial[blockIdx.x * 7 + threadIdx.x] = red[threadIdx.x][0];
 }
 
-__global__ void k_grid_setup(const double* __restrict__ partial, int nparts, int dim, int kd, double kd_radius,
-                             double dmax, int shapes_closed_form, GridInfo* __restrict__ g) {
-    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+__global__ void __launch_bounds__(kBlock) k_grid_setup(const double* __restrict__ partial, int nparts, int dim, int kd,
+                                                      double kd_radius, double dmax, int shapes_closed_form,
+                                                      GridInfo* __restrict__ g) {
+    __shared__ double red[7][kBlock];
     double lo[3] = {1e300, 1e300, 1e300}, hi[3] = {-1e300, -1e300, -1e300}, rm = 0.0;
-    for (int b = 0; b < nparts; ++b) {
+    for (int b = threadIdx.x; b < nparts; b += kBlock) {
         for (int d = 0; d < 3; ++d) {
             lo[d] = fmin(lo[d], partial[b * 7 + d]);
             hi[d] = fmax(hi[d], partial[b * 7 + 3 + d]);
         }
         rm = fmax(rm, partial[b * 7 + 6]);
     }
+    for (int d = 0; d < 3; ++d) {
+        red[d][threadIdx.x] = lo[d];
+        red[3 + d][threadIdx.x] = hi[d];
+    }
+    red[6][threadIdx.x] = rm;
+    __syncthreads();
+    for (int o = kBlock / 2; o > 0; o >>= 1) {
+        if ((int)threadIdx.x < o) {
+            for (int d = 0; d < 3; ++d) red[d][threadIdx.x] = fmin(red[d][threadIdx.x], red[d][threadIdx.x + o]);
+            for (int d = 3; d < 7; ++d) red[d][threadIdx.x] = fmax(red[d][threadIdx.x], red[d][threadIdx.x + o]);
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x != 0) return;
+    for (int d = 0; d < 3; ++d) {
+        lo[d] = red[d][0];
+        hi[d] = red[3 + d][0];
+    }
+    rm = red[6][0];
     // Any pair with dij < dmax has |c_i - c_j| < r_i + r_j + dmax for closed-form
     // shapes; the margin covers the roundings of the computed dij.
     double geo = (2.0 * rm + dmax) * (1.0 + 1e-9) + 1e-12;
@@ -364,7 +384,7 @@ int64_t ContactSearch::run(const Particles& P, Contacts& C, const scopi_hip_para
     if (na > 1) {
         const unsigned nparts = (unsigned)std::min<int64_t>(grid_for(na), 1024);
         hipLaunchKernelGGL(k_bbox_partial, dim3(nparts), dim3(kBlock), 0, s, P, d_partial_);
-        hipLaunchKernelGGL(k_grid_setup, dim3(1), dim3(64), 0, s, d_partial_, (int)nparts, P.dim, kd ? 1 : 0,
+        hipLaunchKernelGGL(k_grid_setup, dim3(1), dim3(kBlock), 0, s, d_partial_, (int)nparts, P.dim, kd ? 1 : 0,
                            prm.kd_tree_radius, prm.dmax, 1, d_grid_);
         if (P.dim == 3)
             hipLaunchKernelGGL(k_cell_keys<3>, dim3(grid_for(na)), dim3(kBlock), 0, s, P, d_grid_, keys_, vals_);

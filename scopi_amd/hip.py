@@ -27,8 +27,9 @@ EXPORTS = [
     "scopi_hip_compute_contacts", "scopi_hip_download_contacts", "scopi_hip_upload_contacts",
     "scopi_hip_set_a_priori_velocity", "scopi_hip_solve", "scopi_hip_download_solution",
     "scopi_hip_update_and_move", "scopi_hip_displacement_obstacles", "scopi_hip_run",
-    "scopi_hip_set_kernel_timing", "scopi_hip_kernel_timing",
+    "scopi_hip_set_kernel_timing", "scopi_hip_kernel_timing", "scopi_hip_set_tuning",
 ]
+TUNE_REDUCE_BLOCKS, TUNE_BT_GROUP = 1, 2
 
 
 class Params(C.Structure):
@@ -91,6 +92,7 @@ def lib():
     L.scopi_hip_run.argtypes = [P, C.POINTER(Params), C.c_double, C.c_int64, C.c_int64, C.POINTER(SolveInfo)]
     L.scopi_hip_set_kernel_timing.argtypes = [P, C.c_int32]
     L.scopi_hip_kernel_timing.argtypes = [P, D]
+    L.scopi_hip_set_tuning.argtypes = [P, C.c_int32, C.c_int32]
     for name in [n for n in dir(L) if n.startswith("scopi_hip_") and n != "scopi_hip_last_error"]:
         getattr(L, name).restype = C.c_int
     _lib = L
@@ -216,6 +218,9 @@ class HipScene:
 
     def set_kernel_timing(self, on=True):
         self._check(self._L.scopi_hip_set_kernel_timing(self._h, 1 if on else 0), "set_kernel_timing")
+
+    def set_tuning(self, key, value):
+        self._check(self._L.scopi_hip_set_tuning(self._h, int(key), int(value)), "set_tuning")
 
     def kernel_timing(self):
         out = np.zeros(6)
