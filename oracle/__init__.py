@@ -55,7 +55,7 @@ def lib():
         L.oracle_scene_free.argtypes = [P]
         L.oracle_default_params.argtypes = [C.POINTER(Params)]
         L.oracle_set_params.argtypes = [P, C.POINTER(Params)]
-        L.oracle_scene_set.argtypes = [P, C.c_int32, C.c_int64, C.c_int64, I32] + [D] * 10
+        L.oracle_scene_set.argtypes = [P, C.c_int32, C.c_int64, C.c_int64, I32] + [D] * 11
         L.oracle_get_state.argtypes = [P, D, D, D, D]
         L.oracle_run.argtypes = [P, C.c_double, C.c_int64, C.c_int64]
         L.oracle_compute_contacts.argtypes = [P]
@@ -68,7 +68,12 @@ def lib():
         L.oracle_solve.argtypes = [P, C.c_double, D, D, D]
         L.oracle_set_a_priori_velocity.argtypes = [P, C.c_double]
         L.oracle_get_stats.argtypes = [P, C.POINTER(Stats)]
-        L.oracle_closest_points.argtypes = [C.c_int32, C.c_int32, D, D, D, C.c_int32, D, D, D, D, D, D, D]
+        L.oracle_closest_points.argtypes = [C.c_int32, C.c_int32, D, D, D, D, C.c_int32, D, D, D, D, D, D, D, D]
+        L.oracle_se_closest_points.argtypes = [D] * 8 + [D, D, D, D, D, D, I32, I32, I32]
+        L.oracle_se_residual.argtypes = [D] * 8 + [D, D, C.c_void_p]
+        L.oracle_se_point_normal.argtypes = [D, D, D, D, C.c_double, C.c_double, D, D]
+        L.oracle_set_se_math.argtypes = [C.c_int32]
+        L.oracle_math.argtypes = [C.c_int32, C.c_double, C.c_double, C.POINTER(C.c_double)]
         L.oracle_quaternion.argtypes = [C.c_double, D, D]
         L.oracle_gen_2d_case_spheres.argtypes = [C.c_int32, D, D, D, D]
         _lib = L
@@ -97,7 +102,7 @@ class Oracle:
         rc = self._L.oracle_scene_set(
             self._h, scene.dim, scene.n, scene.n_inactive, np.ascontiguousarray(scene.shape, np.int32),
             *(np.ascontiguousarray(getattr(scene, k), np.float64) for k in (
-                "pos", "q", "size", "v", "vd", "omega", "domega", "f", "m", "J")))
+                "pos", "q", "size", "squareness", "v", "vd", "omega", "domega", "f", "m", "J")))
         if rc != 0:
             raise ValueError("oracle_scene_set failed")
 
@@ -159,18 +164,61 @@ class Oracle:
         return {k: getattr(s, k) for k, _ in Stats._fields_}
 
 
-def closest_points(dim, shape_i, pos_i, q_i, size_i, shape_j, pos_j, q_j, size_j):
-    L = lib()
+def _v(x, n):
+    o = np.zeros(n); x = np.atleast_1d(np.asarray(x, np.float64)); o[: x.size] = x
+    return o
 
-    def v3(x):
-        o = np.zeros(3); x = np.atleast_1d(np.asarray(x, np.float64)); o[: x.size] = x; return o
+
+def closest_points(dim, shape_i, pos_i, q_i, size_i, shape_j, pos_j, q_j, size_j, sq_i=(1.0, 1.0),
+                   sq_j=(1.0, 1.0)):
+    L = lib()
     dij = np.zeros(1); nij = np.zeros(3); pi = np.zeros(3); pj = np.zeros(3)
-    rc = L.oracle_closest_points(dim, shape_i, v3(pos_i), np.ascontiguousarray(q_i, np.float64), v3(size_i),
-                                 shape_j, v3(pos_j), np.ascontiguousarray(q_j, np.float64), v3(size_j),
-                                 dij, nij, pi, pj)
+    rc = L.oracle_closest_points(dim, shape_i, _v(pos_i, 3), np.ascontiguousarray(q_i, np.float64), _v(size_i, 3),
+                                 _v(sq_i, 2), shape_j, _v(pos_j, 3), np.ascontiguousarray(q_j, np.float64),
+                                 _v(size_j, 3), _v(sq_j, 2), dij, nij, pi, pj)
     if rc != 0:
         raise ValueError("no closest-point form for this pair")
     return dict(dij=float(dij[0]), nij=nij[:dim], pi=pi[:dim], pj=pj[:dim])
+
+
+def se_closest_points(pos_i, q_i, size_i, sq_i, pos_j, q_j, size_j, sq_j):
+    """closest_points(superellipsoid<3>, superellipsoid<3>) with hybrd1 diagnostics."""
+    L = lib()
+    dij = np.zeros(1); nij = np.zeros(3); pi = np.zeros(3); pj = np.zeros(3); u0 = np.zeros(4); u = np.zeros(4)
+    info = np.zeros(1, np.int32); nfev = np.zeros(1, np.int32); njev = np.zeros(1, np.int32)
+    L.oracle_se_closest_points(_v(pos_i, 3), _v(q_i, 4), _v(size_i, 3), _v(sq_i, 2), _v(pos_j, 3), _v(q_j, 4),
+                               _v(size_j, 3), _v(sq_j, 2), dij, nij, pi, pj, u0, u, info, nfev, njev)
+    return dict(dij=float(dij[0]), nij=nij, pi=pi, pj=pj, u0=u0, u=u, info=int(info[0]), nfev=int(nfev[0]),
+                njev=int(njev[0]))
+
+
+def se_residual(pos_i, q_i, size_i, sq_i, pos_j, q_j, size_j, sq_j, u, jacobian=False):
+    L = lib()
+    F = np.zeros(4); J = np.zeros(16)
+    L.oracle_se_residual(_v(pos_i, 3), _v(q_i, 4), _v(size_i, 3), _v(sq_i, 2), _v(pos_j, 3), _v(q_j, 4),
+                         _v(size_j, 3), _v(sq_j, 2), _v(u, 4), F, J.ctypes.data_as(C.c_void_p) if jacobian else None)
+    return (F, J.reshape(4, 4)) if jacobian else F
+
+
+def set_se_math(portable: bool):
+    """Select the elementary functions of the superellipsoid solve (C library or portable)."""
+    lib().oracle_set_se_math(1 if portable else 0)
+
+
+MATH_FN = {"cos": 0, "sin": 1, "asin": 2, "log": 3, "exp": 4, "pow": 5}
+
+
+def portable_math(fn, x, y=0.0):
+    out = C.c_double()
+    lib().oracle_math(MATH_FN[fn], float(x), float(y), C.byref(out))
+    return out.value
+
+
+def se_point_normal(pos, q, size, sq, a, b):
+    L = lib()
+    p = np.zeros(3); n = np.zeros(3)
+    L.oracle_se_point_normal(_v(pos, 3), _v(q, 4), _v(size, 3), _v(sq, 2), float(a), float(b), p, n)
+    return p, n
 
 
 def gen_2d_case_spheres(n=3):

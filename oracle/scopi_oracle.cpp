@@ -22,6 +22,7 @@
 //                                             sphere_plane_friction_mu01}.json, inclined-plane
 //                                             expected errors (test_obstacles.cpp:256-313)
 #include "scopi_oracle.h"
+#include "superellipsoid.hpp"
 
 #include <algorithm>
 #include <array>
@@ -50,6 +51,7 @@ struct Scene {
     int64_t n = 0, n_inactive = 0;
     std::vector<int32_t> shape;
     std::vector<vec3> pos, size, v, vd, omega, domega, f, J;
+    std::vector<std::array<double, 2>> sq;  // superellipsoid squareness (e, n)
     std::vector<quat> q;
     std::vector<double> m;
     oracle_params prm{};
@@ -207,8 +209,34 @@ Contact cp_sphere_segment(int dim, const vec3& cs, double r, const vec3& sp, con
 
 // closest_points_dispatcher (closest_points.hpp:2037-2043, antisymmetric dispatch):
 // returns false for pairs the reference has no closed form for.
-bool closest_points(int dim, int si, const vec3& pi, const quat& qi, const vec3& szi, int sj, const vec3& pj,
-                    const quat& qj, const vec3& szj, Contact& out) {
+// Elementary functions of the superellipsoid solve: 0 = C library, 1 = portable (se_math.hpp).
+int g_se_math = 0;
+
+se_oracle::SEResult se_solve(const se_oracle::SE& s1, const se_oracle::SE& s2) {
+    return g_se_math ? se_oracle::se_se<se_oracle::PortableMath>(s1, s2) : se_oracle::se_se<se_oracle::LibMath>(s1, s2);
+}
+
+se_oracle::SE make_se(const vec3& c, const quat& q, const vec3& r, const std::array<double, 2>& sq) {
+    se_oracle::SE s;
+    s.c = c;
+    s.r = r;
+    s.e = sq[0];
+    s.n = sq[1];
+    s.R = rotation3(q);
+    return s;
+}
+
+bool closest_points(int dim, int si, const vec3& pi, const quat& qi, const vec3& szi, const std::array<double, 2>& sqi,
+                    int sj, const vec3& pj, const quat& qj, const vec3& szj, const std::array<double, 2>& sqj,
+                    Contact& out) {
+    if (dim == 3 && si == ORACLE_SUPERELLIPSOID && sj == ORACLE_SUPERELLIPSOID) {
+        const se_oracle::SEResult r = se_solve(make_se(pi, qi, szi, sqi), make_se(pj, qj, szj, sqj));
+        out.pi = r.pi;
+        out.pj = r.pj;
+        out.nij = r.nij;
+        out.dij = r.dij;
+        return true;
+    }
     if (si == ORACLE_SPHERE && sj == ORACLE_SPHERE) {
         out = cp_sphere_sphere(dim, pi, szi[0], pj, szj[0]);
         return true;
@@ -238,7 +266,8 @@ bool closest_points(int dim, int si, const vec3& pi, const quat& qi, const vec3&
 // compute_exact_distance: contact/base.hpp:103-141 (non-periodic box)
 void exact_distance(const Scene& s, int64_t i, int64_t j, std::vector<Contact>& out) {
     Contact c{};
-    if (!closest_points(s.dim, s.shape[i], s.pos[i], s.q[i], s.size[i], s.shape[j], s.pos[j], s.q[j], s.size[j], c))
+    if (!closest_points(s.dim, s.shape[i], s.pos[i], s.q[i], s.size[i], s.sq[i], s.shape[j], s.pos[j], s.q[j],
+                        s.size[j], s.sq[j], c))
         return;
     if (c.dij < s.prm.dmax) {
         c.i = i;
@@ -806,7 +835,7 @@ int oracle_set_params(void* scene, const oracle_params* p) {
 }
 
 int oracle_scene_set(void* scene, int32_t dim, int64_t n, int64_t n_inactive, const int32_t* shape, const double* pos,
-                     const double* q, const double* size, const double* v, const double* vd, const double* omega,
+                     const double* q, const double* size, const double* squareness, const double* v, const double* vd, const double* omega,
                      const double* domega, const double* f, const double* m, const double* J) {
     if (!scene || (dim != 2 && dim != 3) || n < 0 || n_inactive < 0 || n_inactive > n) return -1;
     Scene& s = *static_cast<Scene*>(scene);
@@ -828,6 +857,9 @@ int oracle_scene_set(void* scene, int32_t dim, int64_t n, int64_t n_inactive, co
     load3(s.J, J);
     s.q.resize(n);
     for (int64_t i = 0; i < n; ++i) s.q[i] = {q[4 * i], q[4 * i + 1], q[4 * i + 2], q[4 * i + 3]};
+    s.sq.assign(n, {1.0, 1.0});
+    if (squareness)
+        for (int64_t i = 0; i < n; ++i) s.sq[i] = {squareness[2 * i], squareness[2 * i + 1]};
     s.m.assign(m, m + n);
     s.contacts.clear();
     s.lambda.clear();
@@ -943,18 +975,98 @@ int oracle_get_stats(void* scene, oracle_stats* st) {
 }
 
 int oracle_closest_points(int32_t dim, int32_t shape_i, const double* pos_i, const double* q_i, const double* size_i,
-                          int32_t shape_j, const double* pos_j, const double* q_j, const double* size_j, double* dij,
-                          double* nij, double* pi, double* pj) {
+                          const double* sq_i, int32_t shape_j, const double* pos_j, const double* q_j,
+                          const double* size_j, const double* sq_j, double* dij, double* nij, double* pi, double* pj) {
     vec3 a{pos_i[0], pos_i[1], pos_i[2]}, b{pos_j[0], pos_j[1], pos_j[2]};
     vec3 sa{size_i[0], size_i[1], size_i[2]}, sb{size_j[0], size_j[1], size_j[2]};
     quat qa{q_i[0], q_i[1], q_i[2], q_i[3]}, qb{q_j[0], q_j[1], q_j[2], q_j[3]};
+    std::array<double, 2> ea{1.0, 1.0}, eb{1.0, 1.0};
+    if (sq_i) ea = {sq_i[0], sq_i[1]};
+    if (sq_j) eb = {sq_j[0], sq_j[1]};
     Contact c{};
-    if (!closest_points(dim, shape_i, a, qa, sa, shape_j, b, qb, sb, c)) return -2;
+    if (!closest_points(dim, shape_i, a, qa, sa, ea, shape_j, b, qb, sb, eb, c)) return -2;
     *dij = c.dij;
     for (int d = 0; d < 3; ++d) {
         nij[d] = c.nij[d];
         pi[d] = c.pi[d];
         pj[d] = c.pj[d];
+    }
+    return 0;
+}
+
+namespace {
+se_oracle::SE se_from(const double* pos, const double* q, const double* size, const double* sq) {
+    return make_se({pos[0], pos[1], pos[2]}, {q[0], q[1], q[2], q[3]}, {size[0], size[1], size[2]},
+                   {sq ? sq[0] : 1.0, sq ? sq[1] : 1.0});
+}
+}  // namespace
+
+int oracle_se_closest_points(const double* pos_i, const double* q_i, const double* size_i, const double* sq_i,
+                             const double* pos_j, const double* q_j, const double* size_j, const double* sq_j,
+                             double* dij, double* nij, double* pi, double* pj, double* u0, double* u, int32_t* info,
+                             int32_t* nfev, int32_t* njev) {
+    const se_oracle::SE s1 = se_from(pos_i, q_i, size_i, sq_i), s2 = se_from(pos_j, q_j, size_j, sq_j);
+    double start[4];
+    if (g_se_math)
+        se_oracle::se_grid_start<se_oracle::PortableMath>(s1, s2, start);
+    else
+        se_oracle::se_grid_start<se_oracle::LibMath>(s1, s2, start);
+    const se_oracle::SEResult r = se_solve(s1, s2);
+    *dij = r.dij;
+    for (int d = 0; d < 3; ++d) {
+        nij[d] = r.nij[d];
+        pi[d] = r.pi[d];
+        pj[d] = r.pj[d];
+    }
+    for (int k = 0; k < 4; ++k) {
+        if (u0) u0[k] = start[k];
+        if (u) u[k] = r.u[k];
+    }
+    if (info) *info = r.info;
+    if (nfev) *nfev = r.nfev;
+    if (njev) *njev = r.njev;
+    return 0;
+}
+
+int oracle_se_residual(const double* pos_i, const double* q_i, const double* size_i, const double* sq_i,
+                       const double* pos_j, const double* q_j, const double* size_j, const double* sq_j,
+                       const double* u, double* F, double* J) {
+    se_oracle::SEPair P{se_from(pos_i, q_i, size_i, sq_i), se_from(pos_j, q_j, size_j, sq_j)};
+    if (g_se_math)
+        se_oracle::se_se_residual<se_oracle::PortableMath>(P, u, F, J);
+    else
+        se_oracle::se_se_residual<se_oracle::LibMath>(P, u, F, J);
+    return 0;
+}
+
+int oracle_set_se_math(int32_t portable) {
+    g_se_math = portable ? 1 : 0;
+    return 0;
+}
+
+int oracle_math(int32_t fn, double x, double y, double* out) {
+    switch (fn) {
+        case 0: *out = se_math::cos(x); break;
+        case 1: *out = se_math::sin(x); break;
+        case 2: *out = se_math::asin(x); break;
+        case 3: *out = se_math::log(x); break;
+        case 4: *out = se_math::exp(x); break;
+        case 5: *out = se_math::pow(x, y); break;
+        default: return -1;
+    }
+    return 0;
+}
+
+int oracle_se_point_normal(const double* pos, const double* q, const double* size, const double* sq, double a,
+                           double b, double* point, double* normal) {
+    const se_oracle::SE s = se_from(pos, q, size, sq);
+    const se_oracle::v3 p = g_se_math ? se_oracle::se_point<se_oracle::PortableMath>(s, a, b)
+                                      : se_oracle::se_point<se_oracle::LibMath>(s, a, b);
+    const se_oracle::v3 n = g_se_math ? se_oracle::se_normal<se_oracle::PortableMath>(s, a, b)
+                                      : se_oracle::se_normal<se_oracle::LibMath>(s, a, b);
+    for (int d = 0; d < 3; ++d) {
+        point[d] = p[d];
+        normal[d] = n[d];
     }
     return 0;
 }

@@ -24,7 +24,7 @@
 extern "C" {
 #endif
 
-enum { ORACLE_SPHERE = 0, ORACLE_PLANE = 1, ORACLE_SEGMENT = 2 };
+enum { ORACLE_SPHERE = 0, ORACLE_PLANE = 1, ORACLE_SEGMENT = 2, ORACLE_SUPERELLIPSOID = 3 };
 enum { ORACLE_NOFRICTION = 0, ORACLE_FRICTION = 1 };
 enum { ORACLE_CONTACT_KDTREE = 0, ORACLE_CONTACT_BRUTE_FORCE = 1 };
 enum { ORACLE_VAP_FIXED = 0, ORACLE_VAP_FPD = 1 };
@@ -62,9 +62,10 @@ void oracle_scene_free(void* scene);
 int oracle_default_params(oracle_params* p);
 int oracle_set_params(void* scene, const oracle_params* p);
 
-/* Replace the whole particle set (obstacles first: indices [0, n_inactive)). */
+/* Replace the whole particle set (obstacles first: indices [0, n_inactive)).
+ * squareness: n x 2 (e, n) of superellipsoids, may be NULL (then 1, 1). */
 int oracle_scene_set(void* scene, int32_t dim, int64_t n, int64_t n_inactive, const int32_t* shape,
-                     const double* pos, const double* q, const double* size, const double* v,
+                     const double* pos, const double* q, const double* size, const double* squareness, const double* v,
                      const double* vd, const double* omega, const double* domega, const double* f,
                      const double* m, const double* J);
 int oracle_get_state(void* scene, double* pos, double* q, double* v, double* omega);
@@ -91,10 +92,32 @@ int oracle_set_a_priori_velocity(void* scene, double dt);
 
 int oracle_get_stats(void* scene, oracle_stats* st);
 
-/* closest_points(object_i, object_j) for one pair, no dmax filter. */
+/* closest_points(object_i, object_j) for one pair, no dmax filter.
+ * sq_i / sq_j: superellipsoid squareness (e, n), may be NULL for other shapes. */
 int oracle_closest_points(int32_t dim, int32_t shape_i, const double* pos_i, const double* q_i,
-                          const double* size_i, int32_t shape_j, const double* pos_j, const double* q_j,
-                          const double* size_j, double* dij, double* nij, double* pi, double* pj);
+                          const double* size_i, const double* sq_i, int32_t shape_j, const double* pos_j,
+                          const double* q_j, const double* size_j, const double* sq_j, double* dij, double* nij,
+                          double* pi, double* pj);
+
+/* closest_points(superellipsoid<3>, superellipsoid<3>) with the solver's
+ * diagnostics: u0 = grid start (a1, b1, a2, b2), u = end point, info / nfev of hybrd1
+ * (nfev counts n per Jacobian, as minpack.hpp:497-499 does) and the Jacobian count. */
+int oracle_se_closest_points(const double* pos_i, const double* q_i, const double* size_i, const double* sq_i,
+                             const double* pos_j, const double* q_j, const double* size_j, const double* sq_j,
+                             double* dij, double* nij, double* pi, double* pj, double* u0, double* u, int32_t* info,
+                             int32_t* nfev, int32_t* njev);
+/* Residual F(u) (4) and Jacobian (4x4 row-major, may be NULL) of the SE3-SE3 system. */
+int oracle_se_residual(const double* pos_i, const double* q_i, const double* size_i, const double* sq_i,
+                       const double* pos_j, const double* q_j, const double* size_j, const double* sq_j,
+                       const double* u, double* F, double* J);
+/* Elementary functions of the superellipsoid solve: 0 = C library (default),
+ * 1 = the portable functions of se_math.hpp, which the device evaluates identically. */
+int oracle_set_se_math(int32_t portable);
+/* One portable elementary function: fn 0 cos, 1 sin, 2 asin, 3 log, 4 exp, 5 pow(x, y). */
+int oracle_math(int32_t fn, double x, double y, double* out);
+/* superellipsoid::point / normal (a, b). */
+int oracle_se_point_normal(const double* pos, const double* q, const double* size, const double* sq, double a,
+                           double b, double* point, double* normal);
 
 /* quaternion(angle, axis) of src/quaternion.cpp:12-26. */
 int oracle_quaternion(double angle, const double* axis, double* q);

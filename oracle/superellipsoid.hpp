@@ -21,6 +21,8 @@
 #include <cmath>
 #include <vector>
 
+#include "se_math.hpp"
+
 namespace se_oracle {
 
 using v3 = std::array<double, 3>;
@@ -35,31 +37,48 @@ struct SE {
 
 inline double sgn(double x) { return (x > 0) ? 1.0 : ((x < 0) ? -1.0 : 0.0); }
 
+// Elementary-function policies: the C library (the reference's arithmetic) or the
+// portable functions of se_math.hpp (bit-reproducible on the device).
+struct LibMath {
+    static double cos(double x) { return std::cos(x); }
+    static double sin(double x) { return std::sin(x); }
+    static double asin(double x) { return std::asin(x); }
+    static double pow(double x, double p) { return std::pow(x, p); }
+};
+struct PortableMath {
+    static double cos(double x) { return se_math::cos(x); }
+    static double sin(double x) { return se_math::sin(x); }
+    static double asin(double x) { return se_math::asin(x); }
+    static double pow(double x, double p) { return se_math::pow(x, p); }
+};
+
 inline v3 rot(const m33& R, const v3& x) {
     return {R[0][0] * x[0] + R[0][1] * x[1] + R[0][2] * x[2], R[1][0] * x[0] + R[1][1] * x[1] + R[1][2] * x[2],
             R[2][0] * x[0] + R[2][1] * x[1] + R[2][2] * x[2]};
 }
 
 // superellipsoid::point(a, b): superellipsoid.hpp:292-300
+template <class M>
 inline v3 se_point(const SE& s, double a, double b) {
     v3 pt;
-    pt[0] = s.r[0] * sgn(std::cos(a)) * std::pow(std::abs(std::cos(a)), s.n) * sgn(std::cos(b)) *
-            std::pow(std::abs(std::cos(b)), s.e);
-    pt[1] = s.r[1] * sgn(std::cos(a)) * std::pow(std::abs(std::cos(a)), s.n) * sgn(std::sin(b)) *
-            std::pow(std::abs(std::sin(b)), s.e);
-    pt[2] = s.r[2] * sgn(std::sin(a)) * std::pow(std::abs(std::sin(a)), s.n);
+    pt[0] = s.r[0] * sgn(M::cos(a)) * M::pow(std::abs(M::cos(a)), s.n) * sgn(M::cos(b)) *
+            M::pow(std::abs(M::cos(b)), s.e);
+    pt[1] = s.r[1] * sgn(M::cos(a)) * M::pow(std::abs(M::cos(a)), s.n) * sgn(M::sin(b)) *
+            M::pow(std::abs(M::sin(b)), s.e);
+    pt[2] = s.r[2] * sgn(M::sin(a)) * M::pow(std::abs(M::sin(a)), s.n);
     v3 w = rot(s.R, pt);
     return {w[0] + s.c[0], w[1] + s.c[1], w[2] + s.c[2]};
 }
 
 // superellipsoid::normal(a, b): superellipsoid.hpp:314-325 (unit outward normal)
+template <class M>
 inline v3 se_normal(const SE& s, double a, double b) {
     v3 nn;
-    nn[0] = s.r[1] * s.r[2] * sgn(std::cos(a)) * std::pow(std::abs(std::cos(a)), 2 - s.n) * sgn(std::cos(b)) *
-            std::pow(std::abs(std::cos(b)), 2 - s.e);
-    nn[1] = s.r[0] * s.r[2] * sgn(std::cos(a)) * std::pow(std::abs(std::cos(a)), 2 - s.n) * sgn(std::sin(b)) *
-            std::pow(std::abs(std::sin(b)), 2 - s.e);
-    nn[2] = s.r[0] * s.r[1] * sgn(std::sin(a)) * std::pow(std::abs(std::sin(a)), 2 - s.n);
+    nn[0] = s.r[1] * s.r[2] * sgn(M::cos(a)) * M::pow(std::abs(M::cos(a)), 2 - s.n) * sgn(M::cos(b)) *
+            M::pow(std::abs(M::cos(b)), 2 - s.e);
+    nn[1] = s.r[0] * s.r[2] * sgn(M::cos(a)) * M::pow(std::abs(M::cos(a)), 2 - s.n) * sgn(M::sin(b)) *
+            M::pow(std::abs(M::sin(b)), 2 - s.e);
+    nn[2] = s.r[0] * s.r[1] * sgn(M::sin(a)) * M::pow(std::abs(M::sin(a)), 2 - s.n);
     v3 w = rot(s.R, nn);
     double l = std::sqrt(w[0] * w[0] + w[1] * w[1] + w[2] * w[2]);
     return {w[0] / l, w[1] / l, w[2] / l};
@@ -69,17 +88,18 @@ inline v3 se_normal(const SE& s, double a, double b) {
 inline double linspace(double lo, double hi, int n, int i) { return lo + (hi - lo) / (n - 1) * i; }
 
 // binit_xy(n): superellipsoid.hpp:359-412
+template <class M>
 inline std::vector<double> binit_xy(const SE& s, int n) {
     const double pi = 4 * std::atan(1);
     std::vector<double> bs;
     for (int i = 0; i < n; ++i) {
         const double b_ell = linspace(0, 0.5 * pi, n, i);
-        const double x_ell = s.r[0] * std::cos(b_ell);
-        const double y_ell = s.r[1] * std::sin(b_ell);
-        const double d = std::pow(std::pow(s.r[1] * x_ell, 2.0 / s.e) + std::pow(s.r[0] * y_ell, 2.0 / s.e), s.e / 2.0);
+        const double x_ell = s.r[0] * M::cos(b_ell);
+        const double y_ell = s.r[1] * M::sin(b_ell);
+        const double d = M::pow(M::pow(s.r[1] * x_ell, 2.0 / s.e) + M::pow(s.r[0] * y_ell, 2.0 / s.e), s.e / 2.0);
         const double y_supell = y_ell * s.r[0] * s.r[1] / d;
-        const double sinb = std::max(-1.0, std::min(1.0, std::sqrt(std::pow(std::pow(y_supell / s.r[1], 2), 1.0 / s.e))));
-        double b = std::asin(sinb);
+        const double sinb = std::max(-1.0, std::min(1.0, std::sqrt(M::pow(M::pow(y_supell / s.r[1], 2), 1.0 / s.e))));
+        double b = M::asin(sinb);
         if (b > pi) b -= 2 * pi;
         if (b < -pi) b += 2 * pi;
         bs.push_back(b);
@@ -93,17 +113,18 @@ inline std::vector<double> binit_xy(const SE& s, int n) {
 }
 
 // ainit_yz(n): superellipsoid.hpp:415-466
+template <class M>
 inline std::vector<double> ainit_yz(const SE& s, int n) {
     const double pi = 4 * std::atan(1);
     std::vector<double> as;
     for (int i = 0; i < n; ++i) {
         const double a_ell = linspace(0, 0.5 * pi, n, i);
-        const double y_ell = s.r[1] * std::cos(a_ell);
-        const double z_ell = s.r[2] * std::sin(a_ell);
-        const double d = std::pow(std::pow(s.r[2] * y_ell, 2 / s.n) + std::pow(s.r[1] * z_ell, 2 / s.n), s.n / 2);
+        const double y_ell = s.r[1] * M::cos(a_ell);
+        const double z_ell = s.r[2] * M::sin(a_ell);
+        const double d = M::pow(M::pow(s.r[2] * y_ell, 2 / s.n) + M::pow(s.r[1] * z_ell, 2 / s.n), s.n / 2);
         const double z_supell = z_ell * s.r[1] * s.r[2] / d;
-        const double sina = std::max(-1.0, std::min(1.0, std::sqrt(std::pow(std::pow(z_supell / s.r[2], 2), 1 / s.n))));
-        double a = std::asin(sina);
+        const double sina = std::max(-1.0, std::min(1.0, std::sqrt(M::pow(M::pow(z_supell / s.r[2], 2), 1 / s.n))));
+        double a = M::asin(sina);
         if (a > pi) a -= 2 * pi;
         if (a < -pi) a += 2 * pi;
         as.push_back(a);
@@ -165,21 +186,24 @@ inline D4 dsqrt(const D4& a) {
     for (int k = 0; k < 4; ++k) r.d[k] = a.d[k] / (2 * s);
     return r;
 }
+template <class M>
 inline D4 dcos(const D4& a) {
-    D4 r{std::cos(a.v), {}};
-    for (int k = 0; k < 4; ++k) r.d[k] = -std::sin(a.v) * a.d[k];
+    D4 r{M::cos(a.v), {}};
+    for (int k = 0; k < 4; ++k) r.d[k] = -M::sin(a.v) * a.d[k];
     return r;
 }
+template <class M>
 inline D4 dsin(const D4& a) {
-    D4 r{std::sin(a.v), {}};
-    for (int k = 0; k < 4; ++k) r.d[k] = std::cos(a.v) * a.d[k];
+    D4 r{M::sin(a.v), {}};
+    for (int k = 0; k < 4; ++k) r.d[k] = M::cos(a.v) * a.d[k];
     return r;
 }
 // sign(x) |x|^p and its derivative p |x|^(p-1) x'
+template <class M>
 inline D4 spow(const D4& a, double p) {
     const double ax = std::abs(a.v);
-    D4 r{sgn(a.v) * std::pow(ax, p), {}};
-    const double dv = p * std::pow(ax, p - 1);
+    D4 r{sgn(a.v) * M::pow(ax, p), {}};
+    const double dv = p * M::pow(ax, p - 1);
     for (int k = 0; k < 4; ++k) r.d[k] = dv * a.d[k];
     return r;
 }
@@ -189,35 +213,42 @@ struct SEPair {
     SE s1, s2;
 };
 
+template <class M>
 inline void se_se_residual(const SEPair& P, const double* u, double* F, double* J /* 4x4 row-major or null */) {
+    // Values follow the reference's expression order (left-to-right products and
+    // sums, closest_points.hpp:323-381); derivatives ride along by forward-mode AD.
     D4 a1 = var(u[0], 0), b1 = var(u[1], 1), a2 = var(u[2], 2), b2 = var(u[3], 3);
     const SE &s1 = P.s1, &s2 = P.s2;
-    D4 ca1 = dcos(a1), sa1 = dsin(a1), cb1 = dcos(b1), sb1 = dsin(b1);
-    D4 ca2 = dcos(a2), sa2 = dsin(a2), cb2 = dcos(b2), sb2 = dsin(b2);
-    D4 ca1n = spow(ca1, s1.n), ca1n2 = spow(ca1, 2 - s1.n), sa1n = spow(sa1, s1.n), sa1n2 = spow(sa1, 2 - s1.n);
-    D4 cb1e = spow(cb1, s1.e), cb1e2 = spow(cb1, 2 - s1.e), sb1e = spow(sb1, s1.e), sb1e2 = spow(sb1, 2 - s1.e);
-    D4 ca2n = spow(ca2, s2.n), ca2n2 = spow(ca2, 2 - s2.n), sa2n = spow(sa2, s2.n), sa2n2 = spow(sa2, 2 - s2.n);
-    D4 cb2e = spow(cb2, s2.e), cb2e2 = spow(cb2, 2 - s2.e), sb2e = spow(sb2, s2.e), sb2e2 = spow(sb2, 2 - s2.e);
+    D4 ca1 = dcos<M>(a1), sa1 = dsin<M>(a1), cb1 = dcos<M>(b1), sb1 = dsin<M>(b1);
+    D4 ca2 = dcos<M>(a2), sa2 = dsin<M>(a2), cb2 = dcos<M>(b2), sb2 = dsin<M>(b2);
+    D4 ca1n = spow<M>(ca1, s1.n), ca1n2 = spow<M>(ca1, 2 - s1.n), sa1n = spow<M>(sa1, s1.n), sa1n2 = spow<M>(sa1, 2 - s1.n);
+    D4 cb1e = spow<M>(cb1, s1.e), cb1e2 = spow<M>(cb1, 2 - s1.e), sb1e = spow<M>(sb1, s1.e), sb1e2 = spow<M>(sb1, 2 - s1.e);
+    D4 ca2n = spow<M>(ca2, s2.n), ca2n2 = spow<M>(ca2, 2 - s2.n), sa2n = spow<M>(sa2, s2.n), sa2n2 = spow<M>(sa2, 2 - s2.n);
+    D4 cb2e = spow<M>(cb2, s2.e), cb2e2 = spow<M>(cb2, 2 - s2.e), sb2e = spow<M>(sb2, s2.e), sb2e2 = spow<M>(sb2, 2 - s2.e);
     // body-frame normal (A1..A3) and point (A4..A6) of s1; point (B1..B3) and normal (B4..B6) of s2
-    D4 A[3] = {(s1.r[1] * s1.r[2]) * (ca1n2 * cb1e2), (s1.r[0] * s1.r[2]) * (sb1e2 * ca1n2), (s1.r[0] * s1.r[1]) * sa1n2};
-    D4 Ap[3] = {s1.r[0] * (ca1n * cb1e), s1.r[1] * (sb1e * ca1n), s1.r[2] * sa1n};
-    D4 Bp[3] = {s2.r[0] * (ca2n * cb2e), s2.r[1] * (sb2e * ca2n), s2.r[2] * sa2n};
-    D4 Bn[3] = {(s2.r[1] * s2.r[2]) * (ca2n2 * cb2e2), (s2.r[0] * s2.r[2]) * (sb2e2 * ca2n2), (s2.r[0] * s2.r[1]) * sa2n2};
+    const D4 A1 = ((s1.r[1] * s1.r[2]) * ca1n2) * cb1e2, A2 = ((s1.r[0] * s1.r[2]) * sb1e2) * ca1n2,
+             A3 = (s1.r[0] * s1.r[1]) * sa1n2;
+    const D4 A4 = (s1.r[0] * ca1n) * cb1e, A5 = (s1.r[1] * sb1e) * ca1n, A6 = s1.r[2] * sa1n;
+    const D4 B1 = (s2.r[0] * ca2n) * cb2e, B2 = (s2.r[1] * sb2e) * ca2n, B3 = s2.r[2] * sa2n;
+    const D4 B4 = ((s2.r[1] * s2.r[2]) * ca2n2) * cb2e2, B5 = ((s2.r[0] * s2.r[2]) * sb2e2) * ca2n2,
+             B6 = (s2.r[0] * s2.r[1]) * sa2n2;
+    const m33 &R1 = s1.R, &R2 = s2.R;
     D4 n1[3], n2[3], d[3];
     for (int k = 0; k < 3; ++k) {
-        n1[k] = s1.R[k][0] * A[0] + s1.R[k][1] * A[1] + s1.R[k][2] * A[2];
-        n2[k] = s2.R[k][0] * Bn[0] + s2.R[k][1] * Bn[1] + s2.R[k][2] * Bn[2];
-        D4 p1 = s1.R[k][0] * Ap[0] + s1.R[k][1] * Ap[1] + s1.R[k][2] * Ap[2];
-        D4 p2 = s2.R[k][0] * Bp[0] + s2.R[k][1] * Bp[1] + s2.R[k][2] * Bp[2];
-        d[k] = (p2 - p1) + cst(s2.c[k] - s1.c[k]);
+        n1[k] = (R1[k][0] * A1 + R1[k][1] * A2) + R1[k][2] * A3;
+        n2[k] = (R2[k][0] * B4 + R2[k][1] * B5) + R2[k][2] * B6;
+        // -Mk0 A4 - Mk1 A5 - Mk2 A6 + Nk0 B1 + Nk1 B2 + Nk2 B3 - c1k + c2k
+        d[k] = (((((((-(R1[k][0] * A4)) - R1[k][1] * A5) - R1[k][2] * A6) + R2[k][0] * B1) + R2[k][1] * B2) + R2[k][2] * B3) -
+                cst(s1.c[k])) +
+               cst(s2.c[k]);
     }
     D4 f[4];
-    f[0] = -(n1[1] * d[2]) + n1[2] * d[1];
+    f[0] = (-(n1[1] * d[2])) + n1[2] * d[1];
     f[1] = n1[0] * d[2] - n1[2] * d[0];
-    f[2] = -(n1[0] * d[1]) + n1[1] * d[0];
-    D4 dotn = n1[0] * n2[0] + n1[1] * n2[1] + n1[2] * n2[2];
-    D4 l1 = dsqrt(n1[0] * n1[0] + n1[1] * n1[1] + n1[2] * n1[2]);
-    D4 l2 = dsqrt(n2[0] * n2[0] + n2[1] * n2[1] + n2[2] * n2[2]);
+    f[2] = (-(n1[0] * d[1])) + n1[1] * d[0];
+    const D4 dotn = (n1[0] * n2[0] + n1[1] * n2[1]) + n1[2] * n2[2];
+    const D4 l1 = dsqrt((n1[0] * n1[0] + n1[1] * n1[1]) + n1[2] * n1[2]);
+    const D4 l2 = dsqrt((n2[0] * n2[0] + n2[1] * n2[1]) + n2[2] * n2[2]);
     f[3] = dotn + l1 * l2;
     for (int i = 0; i < 4; ++i) {
         F[i] = f[i].v;
@@ -491,13 +522,13 @@ inline void r1mpyq(int m, int n, double* a, int lda, const double* v, const doub
 
 // hybrd1 (minpack.hpp:851-923) with the analytic Jacobian: returns info (1: converged).
 template <class Fn>
-inline int hybrd1(Fn fcn, double* x, int* nfev_out = nullptr) {
+inline int hybrd1(Fn fcn, double* x, int* nfev_out = nullptr, int* njev_out = nullptr) {
     const double xtol = 1.0e-10, factor = 100.0, epsmch = 2.220446049250313e-16;
     const int maxfev = 200 * (N + 1);
     const double p1 = 0.1, p5 = 0.5, p001 = 0.001, p0001 = 0.0001;
     double diag[N] = {1.0, 1.0, 1.0, 1.0};  // mode 2: wa[0..n) = 1 (minpack.hpp:881-884)
     double fvec[N], fjac[N * N], r[LR], qtf[N], wa1[N], wa2[N], wa3[N], wa4[N];
-    int info = 0, nfev = 0;
+    int info = 0, nfev = 0, njev = 0;
     fcn(x, fvec, nullptr);
     nfev = 1;
     double fnorm = enorm(N, fvec);
@@ -509,6 +540,7 @@ inline int hybrd1(Fn fcn, double* x, int* nfev_out = nullptr) {
         double Jrow[N * N];
         double ftmp[N];
         fcn(x, ftmp, Jrow);
+        ++njev;
         for (int i = 0; i < N; ++i)
             for (int j = 0; j < N; ++j) fjac[i + N * j] = Jrow[N * i + j];
         nfev += msum;
@@ -616,6 +648,7 @@ inline int hybrd1(Fn fcn, double* x, int* nfev_out = nullptr) {
         if (info != 0) break;
     }
     if (nfev_out) *nfev_out = nfev;
+    if (njev_out) *njev_out = njev;
     if (info == 5) info = 4;  // hybrd1 (minpack.hpp:918-921)
     return info;
 }
@@ -628,24 +661,28 @@ struct SEResult {
     double u[4];
     int info;
     int nfev;
+    int njev;
 };
 
-inline SEResult se_se(const SE& s1, const SE& s2) {
+// First minimum, in row-major order (xt::where), of the penalised distance
+// |p1 - p2| + 2 (1 + n1.n2) over the two 576-point grids: closest_points.hpp:604-690.
+template <class M>
+inline void se_grid_start(const SE& s1, const SE& s2, double* u0) {
     const int num = 6;
-    std::vector<double> b1 = unique_sorted(binit_xy(s1, num)), a1 = unique_sorted(ainit_yz(s1, num));
-    std::vector<double> b2 = unique_sorted(binit_xy(s2, num)), a2 = unique_sorted(ainit_yz(s2, num));
+    std::vector<double> b1 = unique_sorted(binit_xy<M>(s1, num)), a1 = unique_sorted(ainit_yz<M>(s1, num));
+    std::vector<double> b2 = unique_sorted(binit_xy<M>(s2, num)), a2 = unique_sorted(ainit_yz<M>(s2, num));
     // meshgrid(ainit, binit) flattened row-major: k -> (a[k / nb], b[k % nb])
     const int n1 = (int)(a1.size() * b1.size()), n2 = (int)(a2.size() * b2.size());
     std::vector<v3> P1(n1), N1(n1), P2(n2), N2(n2);
     for (int k = 0; k < n1; ++k) {
         const double a = a1[k / b1.size()], b = b1[k % b1.size()];
-        P1[k] = se_point(s1, a, b);
-        N1[k] = se_normal(s1, a, b);
+        P1[k] = se_point<M>(s1, a, b);
+        N1[k] = se_normal<M>(s1, a, b);
     }
     for (int k = 0; k < n2; ++k) {
         const double a = a2[k / b2.size()], b = b2[k % b2.size()];
-        P2[k] = se_point(s2, a, b);
-        N2[k] = se_normal(s2, a, b);
+        P2[k] = se_point<M>(s2, a, b);
+        N2[k] = se_normal<M>(s2, a, b);
     }
     double dmin = INFINITY;
     int imin = 0, jmin = 0;
@@ -654,22 +691,27 @@ inline SEResult se_se(const SE& s1, const SE& s2) {
             const double dx = P1[i][0] - P2[j][0], dy = P1[i][1] - P2[j][1], dz = P1[i][2] - P2[j][2];
             const double dist = std::sqrt(dx * dx + dy * dy + dz * dz) +
                                 2 * (1 + (N1[i][0] * N2[j][0] + N1[i][1] * N2[j][1] + N1[i][2] * N2[j][2]));
-            if (dist < dmin) {  // first minimum in row-major order (xt::where)
+            if (dist < dmin) {  // first minimum in row-major order
                 dmin = dist;
                 imin = i;
                 jmin = j;
             }
         }
+    u0[0] = a1[imin / b1.size()];
+    u0[1] = b1[imin % b1.size()];
+    u0[2] = a2[jmin / b2.size()];
+    u0[3] = b2[jmin % b2.size()];
+}
+
+template <class M>
+inline SEResult se_se(const SE& s1, const SE& s2) {
     SEResult res;
-    res.u[0] = a1[imin / b1.size()];
-    res.u[1] = b1[imin % b1.size()];
-    res.u[2] = a2[jmin / b2.size()];
-    res.u[3] = b2[jmin % b2.size()];
+    se_grid_start<M>(s1, s2, res.u);
     SEPair pair{s1, s2};
-    res.info = hybrd1([&](const double* u, double* F, double* J) { se_se_residual(pair, u, F, J); }, res.u, &res.nfev);
-    res.pi = se_point(s1, res.u[0], res.u[1]);
-    res.pj = se_point(s2, res.u[2], res.u[3]);
-    res.nij = se_normal(s2, res.u[2], res.u[3]);
+    res.info = hybrd1([&](const double* u, double* F, double* J) { se_se_residual<M>(pair, u, F, J); }, res.u, &res.nfev, &res.njev);
+    res.pi = se_point<M>(s1, res.u[0], res.u[1]);
+    res.pj = se_point<M>(s2, res.u[2], res.u[3]);
+    res.nij = se_normal<M>(s2, res.u[2], res.u[3]);
     const v3 d = {res.pi[0] - res.pj[0], res.pi[1] - res.pj[1], res.pi[2] - res.pj[2]};
     const double sd = sgn(d[0] * res.nij[0] + d[1] * res.nij[1] + d[2] * res.nij[2]);
     res.dij = sd * std::sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
