@@ -101,7 +101,10 @@ const char* scopi_hip_last_error(const scopi_hip_ctx* ctx);
 
 /* Host -> device particle state, obstacles in [0, n_inactive).  Replaces the
  * container fields read by every slot (container.hpp:248-323, 475-597).
- * squareness may be NULL when no superellipsoid is present. */
+ * squareness may be NULL when no superellipsoid is present.  Active particles
+ * are spheres, or superellipsoids in a 3D scene without obstacles or spheres
+ * (SCOPI_HIP_ERR_UNSUPPORTED otherwise: the sphere-superellipsoid and
+ * plane-superellipsoid closest points are not implemented). */
 int scopi_hip_upload_state(scopi_hip_ctx* ctx, int64_t n, int64_t n_inactive, const int32_t* shape, const double* pos,
                            const double* q, const double* size, const double* squareness, const double* v,
                            const double* vd, const double* omega, const double* domega, const double* f,
@@ -155,9 +158,22 @@ int scopi_hip_run(scopi_hip_ctx* ctx, const scopi_hip_params* params, double dt,
  * scopi_hip_set_kernel_timing(ctx, 1); it serialises the solve loop. */
 int scopi_hip_set_kernel_timing(scopi_hip_ctx* ctx, int32_t enable);
 
+/* closest_points(superellipsoid<3>, superellipsoid<3>) for n independent pairs
+ * (replaces include/scopi/objects/methods/closest_points.hpp:263-733 with
+ * minpack.hpp:851-923): pair k is the body (pos_i[3k..], q_i[4k..],
+ * size_i[3k..] semi-axes, sq_i[2k..] squareness (e, n)) and the body at the same
+ * offsets of the _j arrays.  Outputs per pair: dij[k], nij / pi / pj[3k..] as the
+ * reference's neighbor record; optional (may be NULL): u0[4k..] the grid start,
+ * u[4k..] the hybrd1 end point (a1, b1, a2, b2), info[k] and nfev[k] of hybrd1.
+ * Runs on the context's device and stream; uses scratch memory of its own. */
+int scopi_hip_se_closest_points(scopi_hip_ctx* ctx, int64_t n, const double* pos_i, const double* q_i,
+                                const double* size_i, const double* sq_i, const double* pos_j, const double* q_j,
+                                const double* size_j, const double* sq_j, double* dij, double* nij, double* pi,
+                                double* pj, double* u0, double* u, int32_t* info, int32_t* nfev);
+
 /* Launch shapes of the projected-gradient kernels (the hook the autotuner in
  * scopi_amd.tune drives; no counterpart in the reference):
- *   SCOPI_TUNE_REDUCE_BLOCKS: blocks of 256 threads of the per-contact kernels (1..2048, default 2048);
+ *   SCOPI_TUNE_REDUCE_BLOCKS: blocks of 256 threads of the per-contact kernels (1..2048, default 1024);
  *   SCOPI_TUNE_BT_GROUP: lanes sharing one particle in the B^T gather (1, 2, 4, 8 or 16, default 8). */
 enum { SCOPI_TUNE_REDUCE_BLOCKS = 1, SCOPI_TUNE_BT_GROUP = 2 };
 int scopi_hip_set_tuning(scopi_hip_ctx* ctx, int32_t key, int32_t value);

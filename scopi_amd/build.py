@@ -11,7 +11,7 @@ import sys
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libscopi_hip.so")
-SOURCES = ["api.hip", "contacts.hip", "apgd.hip", "sort_scan.hip"]
+SOURCES = ["api.hip", "contacts.hip", "apgd.hip", "sort_scan.hip", "superellipsoid.hip"]
 
 # Packed FP32 instructions are disabled for every device pass.
 HIPCC_FLAGS = [

@@ -125,7 +125,7 @@ class Solver {
     int nr_ = 1;
     int final_slot_ = 0;
     int layout_ = kRotFree;
-    int reduce_blocks_ = 2048;  // grid of the per-contact kernels (grid-stride)
+    int reduce_blocks_ = 1024;  // grid of the per-contact kernels (grid-stride)
     int bt_group_ = 8;          // lanes per particle in k_bt
     bool has_solution_ = false;
     // kernel timing (HIP events on the solve stream)

@@ -133,8 +133,21 @@ __global__ void __launch_bounds__(kBlock) k_grid_setup(const double* __restrict_
     if (kd) {
         double kdc = sqrt(kd_radius) * (1.0 + 1e-12) + 1e-300;
         cut = shapes_closed_form ? fmin(geo, kdc) : kdc;
+    } else if (!shapes_closed_form) {
+        cut = INFINITY;  // brute force over shapes without a distance bound: every pair
     }
     if (!(cut > 0)) cut = 1e-300;
+    if (cut == INFINITY) {
+        for (int d = 0; d < 3; ++d) {
+            g->lo[d] = (d < dim) ? lo[d] : 0.0;
+            g->nc[d] = 1;
+            g->w[d] = INFINITY;
+        }
+        g->bits = 0;
+        g->cutoff = cut;
+        g->rmax = rm;
+        return;
+    }
     const int bits_max = (dim == 3) ? kMaxBits3 : kMaxBits2;
     int maxc = 1;
     for (int d = 0; d < 3; ++d) {
@@ -184,8 +197,9 @@ __global__ void __launch_bounds__(kBlock) k_cell_range(const uint32_t* __restric
     if (s == na - 1 || keys[s + 1] != k) end[k] = (int)(s + 1);
 }
 
-// Visits every j > i in the neighbouring cells that satisfies both predicates.
-template <int DIM, bool KD, class F>
+// Visits every j > i in the neighbouring cells that satisfies both predicates
+// (CAND: the kd-tree predicate only; the closest points come later).
+template <int DIM, bool KD, bool CAND, class F>
 __device__ inline void for_partners(const Particles& P, const GridInfo& g, const int* __restrict__ start,
                                     const int* __restrict__ end, const int32_t* __restrict__ sorted, int64_t i,
                                     double kd_radius, double dmax, F&& visit) {
@@ -211,6 +225,10 @@ __device__ inline void for_partners(const Particles& P, const GridInfo& g, const
                         const d3 cj = load3(P.pos, P.cap, j);
                         if (!(kd_dist2<DIM>(ci, cj) < kd_radius)) continue;
                     }
+                    if (CAND) {
+                        visit(j);
+                        continue;
+                    }
                     CP cp;
                     if (!closest_points<DIM>(P, i, j, cp)) continue;
                     if (cp.dij < dmax) visit(j);
@@ -220,7 +238,7 @@ __device__ inline void for_partners(const Particles& P, const GridInfo& g, const
     }
 }
 
-template <int DIM, bool KD>
+template <int DIM, bool KD, bool CAND = false>
 __global__ void __launch_bounds__(kBlock) k_pair_count(Particles P, const GridInfo* __restrict__ gp,
                                                       const int* __restrict__ start, const int* __restrict__ end,
                                                       const int32_t* __restrict__ sorted, double kd_radius, double dmax,
@@ -230,11 +248,11 @@ __global__ void __launch_bounds__(kBlock) k_pair_count(Particles P, const GridIn
     if (t >= na) return;
     const GridInfo g = *gp;
     int cnt = 0;
-    for_partners<DIM, KD>(P, g, start, end, sorted, P.n_inactive + t, kd_radius, dmax, [&](int64_t) { ++cnt; });
+    for_partners<DIM, KD, CAND>(P, g, start, end, sorted, P.n_inactive + t, kd_radius, dmax, [&](int64_t) { ++cnt; });
     counts[t] = cnt;
 }
 
-template <int DIM, bool KD>
+template <int DIM, bool KD, bool CAND = false>
 __global__ void __launch_bounds__(kBlock) k_pair_emit(Particles P, const GridInfo* __restrict__ gp,
                                                      const int* __restrict__ start, const int* __restrict__ end,
                                                      const int32_t* __restrict__ sorted, double kd_radius, double dmax,
@@ -245,8 +263,8 @@ __global__ void __launch_bounds__(kBlock) k_pair_emit(Particles P, const GridInf
     const GridInfo g = *gp;
     const int64_t base = offsets[t];
     int64_t cnt = 0;
-    for_partners<DIM, KD>(P, g, start, end, sorted, P.n_inactive + t, kd_radius, dmax,
-                          [&](int64_t j) { pair_j[base + cnt++] = (int32_t)j; });
+    for_partners<DIM, KD, CAND>(P, g, start, end, sorted, P.n_inactive + t, kd_radius, dmax,
+                                [&](int64_t j) { pair_j[base + cnt++] = (int32_t)j; });
     // insertion sort of this particle's partners by j
     for (int64_t a = 1; a < cnt; ++a) {
         const int32_t key = pair_j[base + a];
@@ -306,6 +324,43 @@ __global__ void __launch_bounds__(kBlock) k_contact_geometry(Particles P, Contac
     C.mu[c] = mu;
 }
 
+// candidate (i, j) of the superellipsoid path
+__global__ void __launch_bounds__(kBlock) k_cand_scatter(int64_t n_inactive, int64_t na, const int* __restrict__ counts,
+                                                        const int* __restrict__ offsets,
+                                                        const int32_t* __restrict__ pair_j, SeCandidates S) {
+    const int64_t t = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (t >= na) return;
+    const int64_t b = offsets[t], cnt = counts[t];
+    for (int64_t k = 0; k < cnt; ++k) {
+        S.i[b + k] = (int32_t)(n_inactive + t);
+        S.j[b + k] = pair_j[b + k];
+    }
+}
+
+__global__ void __launch_bounds__(kBlock) k_cand_flags(SeCandidates S, int64_t n, double dmax, int* __restrict__ flags) {
+    const int64_t c = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (c >= n) return;
+    flags[c] = (S.dij[c] < dmax) ? 1 : 0;
+}
+
+// compute_exact_distance's dmax filter (contact/base.hpp:118): keeps the candidates in order
+__global__ void __launch_bounds__(kBlock) k_cand_emit(SeCandidates S, int64_t n, const int* __restrict__ flags,
+                                                     const int* __restrict__ offs, int64_t base, Contacts C,
+                                                     double mu) {
+    const int64_t c = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (c >= n || !flags[c]) return;
+    const int64_t o = base + offs[c];
+    C.i[o] = S.i[c];
+    C.j[o] = S.j[c];
+    C.dij[o] = S.dij[c];
+    for (int d = 0; d < 3; ++d) {
+        C.nij[d * C.cap + o] = S.nij[d * S.cap + c];
+        C.pi[d * C.cap + o] = S.pi[d * S.cap + c];
+        C.pj[d * C.cap + o] = S.pj[d * S.cap + c];
+    }
+    C.mu[o] = mu;
+}
+
 // ---------------------------------------------------------------- host
 void contacts_free(Contacts& C) {
     for (void* p : {(void*)C.i, (void*)C.j, (void*)C.dij, (void*)C.nij, (void*)C.pi, (void*)C.pj, (void*)C.mu})
@@ -330,9 +385,11 @@ void contacts_reserve(Contacts& C, int64_t n) {
 ContactSearch::~ContactSearch() {
     for (void* p : {(void*)d_grid_, (void*)d_partial_, (void*)keys_, (void*)vals_, (void*)counts_, (void*)offsets_,
                     (void*)obst_flags_, (void*)obst_offsets_, (void*)cell_start_, (void*)cell_end_, (void*)pair_j_,
-                    (void*)d_totals_})
+                    (void*)d_totals_, (void*)cand_flags_, (void*)cand_offsets_})
         if (p) (void)hipFree(p);
     if (h_totals_) (void)hipHostFree(h_totals_);
+    se_grids_free(se_grids_);
+    se_candidates_free(se_cand_);
 }
 
 void ContactSearch::reserve_particles(int64_t na, int64_t n_inactive) {
@@ -376,7 +433,9 @@ void ContactSearch::reserve_pairs(int64_t n) {
     SCOPI_HIP_CHECK(hipMalloc(&pair_j_, pair_cap_ * sizeof(int32_t)));
 }
 
-int64_t ContactSearch::run(const Particles& P, Contacts& C, const scopi_hip_params& prm, hipStream_t s) {
+int64_t ContactSearch::run(const Particles& P, Contacts& C, const scopi_hip_params& prm, hipStream_t s,
+                           bool closed_form) {
+    if (!closed_form) return run_candidates(P, C, prm, s);
     const int64_t na = P.n - P.n_inactive, ni = P.n_inactive;
     const bool kd = prm.contact_method == SCOPI_CONTACT_KDTREE;
     reserve_particles(na, ni);
@@ -446,6 +505,69 @@ int64_t ContactSearch::run(const Particles& P, Contacts& C, const scopi_hip_para
         else
             hipLaunchKernelGGL(k_contact_geometry<2>, dim3(grid_for(nc)), dim3(kBlock), 0, s, P, C, nc, prm.mu);
     }
+    SCOPI_HIP_CHECK(hipGetLastError());
+    return nc;
+}
+
+// Scenes with superellipsoids (no obstacles, 3D): candidates by the kd-tree predicate
+// (every pair for brute force), closest points of every candidate on the device,
+// then the dmax filter in candidate order.
+int64_t ContactSearch::run_candidates(const Particles& P, Contacts& C, const scopi_hip_params& prm, hipStream_t s) {
+    const int64_t na = P.n - P.n_inactive, ni = P.n_inactive;
+    const bool kd = prm.contact_method == SCOPI_CONTACT_KDTREE;
+    nc_obs_ = 0;
+    if (na < 2) return 0;
+    reserve_particles(na, ni);
+    const unsigned nparts = (unsigned)std::min<int64_t>(grid_for(na), 1024);
+    hipLaunchKernelGGL(k_bbox_partial, dim3(nparts), dim3(kBlock), 0, s, P, d_partial_);
+    hipLaunchKernelGGL(k_grid_setup, dim3(1), dim3(kBlock), 0, s, d_partial_, (int)nparts, P.dim, kd ? 1 : 0,
+                       prm.kd_tree_radius, prm.dmax, 0, d_grid_);
+    hipLaunchKernelGGL(k_cell_keys<3>, dim3(grid_for(na)), dim3(kBlock), 0, s, P, d_grid_, keys_, vals_);
+    sorter_.sort(keys_, vals_, na, 3 * kMaxBits3, s, scanner_);
+    const int64_t ncells = int64_t(1) << (3 * kMaxBits3);
+    hipLaunchKernelGGL(k_fill_int, dim3(1024), dim3(kBlock), 0, s, cell_start_, ncells, 0);
+    hipLaunchKernelGGL(k_fill_int, dim3(1024), dim3(kBlock), 0, s, cell_end_, ncells, 0);
+    hipLaunchKernelGGL(k_cell_range, dim3(grid_for(na)), dim3(kBlock), 0, s, keys_, na, cell_start_, cell_end_);
+    if (kd)
+        hipLaunchKernelGGL((k_pair_count<3, true, true>), dim3(grid_for(na)), dim3(kBlock), 0, s, P, d_grid_,
+                           cell_start_, cell_end_, vals_, prm.kd_tree_radius, prm.dmax, counts_);
+    else
+        hipLaunchKernelGGL((k_pair_count<3, false, true>), dim3(grid_for(na)), dim3(kBlock), 0, s, P, d_grid_,
+                           cell_start_, cell_end_, vals_, prm.kd_tree_radius, prm.dmax, counts_);
+    scanner_.exclusive(counts_, offsets_, na, d_totals_, s);
+    SCOPI_HIP_CHECK(hipMemcpyAsync(h_totals_, d_totals_, sizeof(int), hipMemcpyDeviceToHost, s));
+    SCOPI_HIP_CHECK(hipStreamSynchronize(s));
+    const int64_t ncand = h_totals_[0];
+    if (ncand == 0) return 0;
+    reserve_pairs(ncand);
+    se_candidates_reserve(se_cand_, ncand);
+    if (ncand + 1 > cand_cap_) {
+        if (cand_flags_) SCOPI_HIP_CHECK(hipFree(cand_flags_));
+        if (cand_offsets_) SCOPI_HIP_CHECK(hipFree(cand_offsets_));
+        cand_cap_ = ncand + ncand / 4 + 1024;
+        SCOPI_HIP_CHECK(hipMalloc(&cand_flags_, cand_cap_ * sizeof(int)));
+        SCOPI_HIP_CHECK(hipMalloc(&cand_offsets_, cand_cap_ * sizeof(int)));
+    }
+    if (kd)
+        hipLaunchKernelGGL((k_pair_emit<3, true, true>), dim3(grid_for(na)), dim3(kBlock), 0, s, P, d_grid_,
+                           cell_start_, cell_end_, vals_, prm.kd_tree_radius, prm.dmax, offsets_, pair_j_);
+    else
+        hipLaunchKernelGGL((k_pair_emit<3, false, true>), dim3(grid_for(na)), dim3(kBlock), 0, s, P, d_grid_,
+                           cell_start_, cell_end_, vals_, prm.kd_tree_radius, prm.dmax, offsets_, pair_j_);
+    hipLaunchKernelGGL(k_cand_scatter, dim3(grid_for(na)), dim3(kBlock), 0, s, ni, na, counts_, offsets_, pair_j_,
+                       se_cand_);
+    se_closest_points(P, se_grids_, se_cand_, ncand, s);
+    hipLaunchKernelGGL(k_cand_flags, dim3(grid_for(ncand)), dim3(kBlock), 0, s, se_cand_, ncand, prm.dmax,
+                       cand_flags_);
+    scanner_.exclusive(cand_flags_, cand_offsets_, ncand, d_totals_ + 1, s);
+    SCOPI_HIP_CHECK(hipMemcpyAsync(h_totals_ + 1, d_totals_ + 1, sizeof(int), hipMemcpyDeviceToHost, s));
+    SCOPI_HIP_CHECK(hipStreamSynchronize(s));
+    const int64_t nc = h_totals_[1];
+    ncand_ = ncand;
+    contacts_reserve(C, nc);
+    if (nc > 0)
+        hipLaunchKernelGGL(k_cand_emit, dim3(grid_for(ncand)), dim3(kBlock), 0, s, se_cand_, ncand, cand_flags_,
+                           cand_offsets_, (int64_t)0, C, prm.mu);
     SCOPI_HIP_CHECK(hipGetLastError());
     return nc;
 }

@@ -3,6 +3,7 @@
 
 #include "common.hpp"
 #include "sort_scan.hpp"
+#include "superellipsoid.hpp"
 
 namespace scopi_hip {
 
@@ -21,10 +22,15 @@ class ContactSearch {
   public:
     ~ContactSearch();
     // Runs on `s`; grows `C` when needed; returns nc (host sync inside).
-    int64_t run(const Particles& P, Contacts& C, const scopi_hip_params& prm, hipStream_t s);
+    // closed_form: every shape pair has a closed-form distance (no superellipsoid).
+    int64_t run(const Particles& P, Contacts& C, const scopi_hip_params& prm, hipStream_t s, bool closed_form = true);
     int64_t last_obstacle_contacts() const { return nc_obs_; }
+    // superellipsoid path: candidates of the last run and their closest points
+    int64_t last_candidates() const { return ncand_; }
+    const SeCandidates& candidates() const { return se_cand_; }
 
   private:
+    int64_t run_candidates(const Particles& P, Contacts& C, const scopi_hip_params& prm, hipStream_t s);
     void reserve_particles(int64_t na, int64_t n_inactive);
     void reserve_pairs(int64_t n);
     RadixSorter sorter_;
@@ -47,6 +53,11 @@ class ContactSearch {
     int* d_totals_ = nullptr;   // [0] active pairs, [1] obstacle pairs
     int* h_totals_ = nullptr;   // pinned
     int64_t nc_obs_ = 0;
+    SeGrids se_grids_;
+    SeCandidates se_cand_;
+    int* cand_flags_ = nullptr;
+    int* cand_offsets_ = nullptr;
+    int64_t cand_cap_ = 0, ncand_ = 0;
 };
 
 // Allocates / grows the structure-of-arrays contact list (keeps nothing).

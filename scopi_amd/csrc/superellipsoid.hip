@@ -1,0 +1,992 @@
+// superellipsoid.hip -- device closest points of superellipsoid pairs (see superellipsoid.hpp).
+#include "superellipsoid.hpp"
+
+#include <algorithm>
+
+// Every floating-point operation of this file is a separately rounded IEEE
+// operation: no contraction into FMA, so that the device reproduces the CPU
+// restatement bit for bit.
+#pragma clang fp contract(off)
+
+namespace scopi_hip {
+namespace {
+
+// ------------------------------------------------------------------ portable elementary functions
+// Textbook methods, identical operation order on every platform:
+//   cos / sin: Cody-Waite reduction by pi/2 (33 + 33 + 53 bits), Taylor to degree 17 / 18;
+//   asin: Maclaurin series on [0, 1/2], asin x = pi/2 - 2 asin(sqrt((1 - x) / 2)) above;
+//   log: 2 atanh((m - 1) / (m + 1)) series on m in [sqrt(2)/2, sqrt(2)), ln2 in 32 + 53 bits;
+//   exp: k ln2 + r reduction, Taylor to degree 14, 2^k through the exponent bits;
+//   pow(x >= 0, p) = exp(p log x).
+constexpr double kPio2_1 = 0x1.921fb54400000p+0;
+constexpr double kPio2_2 = 0x1.0b4611a600000p-34;
+constexpr double kPio2_3 = 0x1.3198a2e037073p-69;
+constexpr double kTwoOverPi = 0x1.45f306dc9c883p-1;
+constexpr double kPio2 = 0x1.921fb54442d18p+0;
+constexpr double kPi = 0x1.921fb54442d18p+1;
+constexpr double kLn2Hi = 0x1.62e42ff000000p-1;
+constexpr double kLn2Lo = -0x1.718432a1b0e26p-35;
+constexpr double kInvLn2 = 0x1.71547652b82fep+0;
+constexpr double kSqrt2 = 0x1.6a09e667f3bcdp+0;
+constexpr double kF2 = 0.5, kF3 = 1.0 / 6.0, kF4 = 1.0 / 24.0, kF5 = 1.0 / 120.0, kF6 = 1.0 / 720.0,
+                 kF7 = 1.0 / 5040.0, kF8 = 1.0 / 40320.0, kF9 = 1.0 / 362880.0, kF10 = 1.0 / 3628800.0,
+                 kF11 = 1.0 / 39916800.0, kF12 = 1.0 / 479001600.0, kF13 = 1.0 / 6227020800.0,
+                 kF14 = 1.0 / 87178291200.0, kF15 = 1.0 / 1307674368000.0, kF16 = 1.0 / 20922789888000.0,
+                 kF17 = 1.0 / 355687428096000.0, kF18 = 1.0 / 6402373705728000.0;
+
+__device__ inline double sin_poly(double r) {
+    const double s = r * r;
+    double p = kF17;
+    p = p * s - kF15;
+    p = p * s + kF13;
+    p = p * s - kF11;
+    p = p * s + kF9;
+    p = p * s - kF7;
+    p = p * s + kF5;
+    p = p * s - kF3;
+    return r + (r * s) * p;
+}
+__device__ inline double cos_poly(double r) {
+    const double s = r * r;
+    double p = kF18;
+    p = p * s - kF16;
+    p = p * s + kF14;
+    p = p * s - kF12;
+    p = p * s + kF10;
+    p = p * s - kF8;
+    p = p * s + kF6;
+    p = p * s - kF4;
+    p = p * s + kF2;
+    return 1.0 - s * p;
+}
+__device__ inline double reduce_pio2(double x, int& quadrant) {
+    const double k = rint(x * kTwoOverPi);
+    const double r = ((x - k * kPio2_1) - k * kPio2_2) - k * kPio2_3;
+    quadrant = (int)(((int64_t)k) & 3);
+    return r;
+}
+__device__ inline double pm_cos(double x) {
+    if (!(fabs(x) < 1e15)) return x - x;
+    int q;
+    const double r = reduce_pio2(x, q);
+    if (q == 0) return cos_poly(r);
+    if (q == 1) return -sin_poly(r);
+    if (q == 2) return -cos_poly(r);
+    return sin_poly(r);
+}
+__device__ inline double pm_sin(double x) {
+    if (!(fabs(x) < 1e15)) return x - x;
+    int q;
+    const double r = reduce_pio2(x, q);
+    if (q == 0) return sin_poly(r);
+    if (q == 1) return cos_poly(r);
+    if (q == 2) return -sin_poly(r);
+    return -cos_poly(r);
+}
+__device__ inline double asin_small(double x) {
+    double c[30];
+    double t = 1.0;
+    c[0] = 1.0;
+    for (int n = 1; n < 30; ++n) {
+        t = t * (double)(2 * n - 1) / (double)(2 * n);
+        c[n] = t / (double)(2 * n + 1);
+    }
+    const double z = x * x;
+    double p = c[29];
+    for (int n = 28; n >= 1; --n) p = p * z + c[n];
+    return x + (x * z) * p;
+}
+__device__ inline double pm_asin(double x) {
+    if (!(x >= -1.0 && x <= 1.0)) return (x - x) / (x - x);
+    const double a = fabs(x);
+    double r;
+    if (a <= 0.5)
+        r = asin_small(a);
+    else
+        r = kPio2 - 2.0 * asin_small(sqrt((1.0 - a) * 0.5));
+    return x < 0 ? -r : r;
+}
+__device__ inline double pm_log(double x) {
+    if (x != x || x < 0) return (x - x) / (x - x);
+    if (x == 0) return -INFINITY;
+    if (x == INFINITY) return x;
+    int k = 0;
+    if (x < 0x1p-1022) {
+        x = x * 0x1p54;
+        k = -54;
+    }
+    const uint64_t u = (uint64_t)__double_as_longlong(x);
+    k += (int)((u >> 52) & 0x7ff) - 1023;
+    double m = __longlong_as_double((long long)((u & 0x000fffffffffffffULL) | 0x3ff0000000000000ULL));
+    if (m > kSqrt2) {
+        m = m * 0.5;
+        k += 1;
+    }
+    const double f = m - 1.0;
+    const double s = f / (2.0 + f);
+    const double z = s * s;
+    double p = 1.0 / 23.0;
+    p = p * z + 1.0 / 21.0;
+    p = p * z + 1.0 / 19.0;
+    p = p * z + 1.0 / 17.0;
+    p = p * z + 1.0 / 15.0;
+    p = p * z + 1.0 / 13.0;
+    p = p * z + 1.0 / 11.0;
+    p = p * z + 1.0 / 9.0;
+    p = p * z + 1.0 / 7.0;
+    p = p * z + 1.0 / 5.0;
+    p = p * z + 1.0 / 3.0;
+    const double lm = 2.0 * s + (2.0 * s) * (z * p);
+    const double dk = (double)k;
+    return dk * kLn2Hi + (lm + dk * kLn2Lo);
+}
+__device__ inline double pm_exp(double y) {
+    if (y != y) return y;
+    if (y > 709.782712893384) return INFINITY;
+    if (y < -745.2) return 0.0;
+    const double k = rint(y * kInvLn2);
+    const double r = (y - k * kLn2Hi) - k * kLn2Lo;
+    double p = kF14;
+    p = p * r + kF13;
+    p = p * r + kF12;
+    p = p * r + kF11;
+    p = p * r + kF10;
+    p = p * r + kF9;
+    p = p * r + kF8;
+    p = p * r + kF7;
+    p = p * r + kF6;
+    p = p * r + kF5;
+    p = p * r + kF4;
+    p = p * r + kF3;
+    p = p * r + kF2;
+    p = p * r + 1.0;
+    p = p * r + 1.0;
+    int ki = (int)k;
+    if (ki > 1023) {
+        p = p * 0x1p1023;
+        ki -= 1023;
+    } else if (ki < -1022) {
+        p = p * 0x1p-1000;
+        ki += 1000;
+    }
+    return p * __longlong_as_double((long long)((uint64_t)(ki + 1023) << 52));
+}
+__device__ inline double pm_pow(double x, double p) {  // x >= 0
+    if (x != x || p != p) return x + p;
+    if (x == 0) return p > 0 ? 0.0 : (p == 0 ? 1.0 : INFINITY);
+    if (x == INFINITY) return p > 0 ? INFINITY : (p == 0 ? 1.0 : 0.0);
+    return pm_exp(p * pm_log(x));
+}
+// std::min / std::max semantics (first argument on ties and NaN)
+__device__ inline double smin(double a, double b) { return (b < a) ? b : a; }
+__device__ inline double smax(double a, double b) { return (a < b) ? b : a; }
+
+// ------------------------------------------------------------------ shape geometry
+struct SE {
+    double c[3], r[3];
+    double e, n;
+    double R[3][3];
+};
+
+__device__ inline SE load_se(const Particles& P, int64_t k) {
+    SE s;
+    for (int d = 0; d < 3; ++d) {
+        s.c[d] = P.pos[d * P.cap + k];
+        s.r[d] = P.size[d * P.cap + k];
+    }
+    s.e = P.sq[k];
+    s.n = P.sq[P.cap + k];
+    const m3 R = rotation3(P.q[k], P.q[P.cap + k], P.q[2 * P.cap + k], P.q[3 * P.cap + k]);
+    for (int a = 0; a < 3; ++a)
+        for (int b = 0; b < 3; ++b) s.R[a][b] = R.a[a][b];
+    return s;
+}
+
+// superellipsoid::point(a, b): superellipsoid.hpp:292-300
+__device__ inline void se_point(const SE& s, double a, double b, double* out) {
+    const double ca = pm_cos(a), sa = pm_sin(a), cb = pm_cos(b), sb = pm_sin(b);
+    double pt[3];
+    pt[0] = s.r[0] * sgn(ca) * pm_pow(fabs(ca), s.n) * sgn(cb) * pm_pow(fabs(cb), s.e);
+    pt[1] = s.r[1] * sgn(ca) * pm_pow(fabs(ca), s.n) * sgn(sb) * pm_pow(fabs(sb), s.e);
+    pt[2] = s.r[2] * sgn(sa) * pm_pow(fabs(sa), s.n);
+    for (int d = 0; d < 3; ++d) out[d] = (s.R[d][0] * pt[0] + s.R[d][1] * pt[1] + s.R[d][2] * pt[2]) + s.c[d];
+}
+
+// superellipsoid::normal(a, b): superellipsoid.hpp:314-325
+__device__ inline void se_normal(const SE& s, double a, double b, double* out) {
+    const double ca = pm_cos(a), sa = pm_sin(a), cb = pm_cos(b), sb = pm_sin(b);
+    double nn[3];
+    nn[0] = s.r[1] * s.r[2] * sgn(ca) * pm_pow(fabs(ca), 2 - s.n) * sgn(cb) * pm_pow(fabs(cb), 2 - s.e);
+    nn[1] = s.r[0] * s.r[2] * sgn(ca) * pm_pow(fabs(ca), 2 - s.n) * sgn(sb) * pm_pow(fabs(sb), 2 - s.e);
+    nn[2] = s.r[0] * s.r[1] * sgn(sa) * pm_pow(fabs(sa), 2 - s.n);
+    double w[3];
+    for (int d = 0; d < 3; ++d) w[d] = s.R[d][0] * nn[0] + s.R[d][1] * nn[1] + s.R[d][2] * nn[2];
+    const double l = sqrt(w[0] * w[0] + w[1] * w[1] + w[2] * w[2]);
+    for (int d = 0; d < 3; ++d) out[d] = w[d] / l;
+}
+
+// binit_xy(6) (superellipsoid.hpp:359-412) when xy, ainit_yz(6) (:415-466) otherwise;
+// sorted distinct values (xt::unique).  Returns the count.
+__device__ inline int init_angles(const SE& s, bool xy, double* out) {
+    constexpr int num = 6;
+    double v[kSeGridAngles];
+    int m = 0;
+    for (int i = 0; i < num; ++i) {
+        const double t_ell = 0.0 + (0.5 * kPi - 0.0) / (num - 1) * i;
+        double ang;
+        if (xy) {
+            const double x_ell = s.r[0] * pm_cos(t_ell);
+            const double y_ell = s.r[1] * pm_sin(t_ell);
+            const double d = pm_pow(pm_pow(s.r[1] * x_ell, 2.0 / s.e) + pm_pow(s.r[0] * y_ell, 2.0 / s.e), s.e / 2.0);
+            const double y_supell = y_ell * s.r[0] * s.r[1] / d;
+            const double sinb = smax(-1.0, smin(1.0, sqrt(pm_pow(pm_pow(y_supell / s.r[1], 2), 1.0 / s.e))));
+            ang = pm_asin(sinb);
+        } else {
+            const double y_ell = s.r[1] * pm_cos(t_ell);
+            const double z_ell = s.r[2] * pm_sin(t_ell);
+            const double d = pm_pow(pm_pow(s.r[2] * y_ell, 2 / s.n) + pm_pow(s.r[1] * z_ell, 2 / s.n), s.n / 2);
+            const double z_supell = z_ell * s.r[1] * s.r[2] / d;
+            const double sina = smax(-1.0, smin(1.0, sqrt(pm_pow(pm_pow(z_supell / s.r[2], 2), 1 / s.n))));
+            ang = pm_asin(sina);
+        }
+        if (ang > kPi) ang -= 2 * kPi;
+        if (ang < -kPi) ang += 2 * kPi;
+        v[m++] = ang;
+        v[m++] = ang + kPi / 2;
+        v[m++] = ang - kPi;
+        v[m++] = ang - kPi / 2;
+    }
+    for (int k = 0; k < m; ++k)
+        if (fabs(v[k]) < 1.0e-6) v[k] = 0.01;
+    for (int a = 1; a < m; ++a) {  // insertion sort
+        const double key = v[a];
+        int b = a - 1;
+        while (b >= 0 && v[b] > key) {
+            v[b + 1] = v[b];
+            --b;
+        }
+        v[b + 1] = key;
+    }
+    int cnt = 0;
+    for (int k = 0; k < m; ++k)
+        if (cnt == 0 || !(v[k] == out[cnt - 1])) out[cnt++] = v[k];
+    return cnt;
+}
+
+// ------------------------------------------------------------------ residual with forward-mode AD
+struct D4 {
+    double v;
+    double d[4];
+};
+__device__ inline D4 cst(double x) { return D4{x, {0.0, 0.0, 0.0, 0.0}}; }
+__device__ inline D4 var(double x, int k) {
+    D4 r = cst(x);
+    r.d[k] = 1.0;
+    return r;
+}
+__device__ inline D4 operator+(const D4& a, const D4& b) {
+    D4 r{a.v + b.v, {}};
+    for (int k = 0; k < 4; ++k) r.d[k] = a.d[k] + b.d[k];
+    return r;
+}
+__device__ inline D4 operator-(const D4& a, const D4& b) {
+    D4 r{a.v - b.v, {}};
+    for (int k = 0; k < 4; ++k) r.d[k] = a.d[k] - b.d[k];
+    return r;
+}
+__device__ inline D4 operator-(const D4& a) {
+    D4 r{-a.v, {}};
+    for (int k = 0; k < 4; ++k) r.d[k] = -a.d[k];
+    return r;
+}
+__device__ inline D4 operator*(const D4& a, const D4& b) {
+    D4 r{a.v * b.v, {}};
+    for (int k = 0; k < 4; ++k) r.d[k] = a.d[k] * b.v + a.v * b.d[k];
+    return r;
+}
+__device__ inline D4 operator*(double s, const D4& a) {
+    D4 r{s * a.v, {}};
+    for (int k = 0; k < 4; ++k) r.d[k] = s * a.d[k];
+    return r;
+}
+__device__ inline D4 dsqrt(const D4& a) {
+    const double s = sqrt(a.v);
+    D4 r{s, {}};
+    for (int k = 0; k < 4; ++k) r.d[k] = a.d[k] / (2 * s);
+    return r;
+}
+__device__ inline D4 dcos(const D4& a) {
+    D4 r{pm_cos(a.v), {}};
+    const double sn = pm_sin(a.v);
+    for (int k = 0; k < 4; ++k) r.d[k] = -sn * a.d[k];
+    return r;
+}
+__device__ inline D4 dsin(const D4& a) {
+    D4 r{pm_sin(a.v), {}};
+    const double cs = pm_cos(a.v);
+    for (int k = 0; k < 4; ++k) r.d[k] = cs * a.d[k];
+    return r;
+}
+__device__ inline D4 spow(const D4& a, double p) {
+    const double ax = fabs(a.v);
+    D4 r{sgn(a.v) * pm_pow(ax, p), {}};
+    const double dv = p * pm_pow(ax, p - 1);
+    for (int k = 0; k < 4; ++k) r.d[k] = dv * a.d[k];
+    return r;
+}
+
+// F(a1, b1, a2, b2) of closest_points.hpp:282-382 (values in the reference's order)
+// and its Jacobian J (row-major, or not at all).
+__device__ inline void se_residual(const SE& s1, const SE& s2, const double* u, double* F, double* J) {
+    D4 a1 = var(u[0], 0), b1 = var(u[1], 1), a2 = var(u[2], 2), b2 = var(u[3], 3);
+    D4 ca1 = dcos(a1), sa1 = dsin(a1), cb1 = dcos(b1), sb1 = dsin(b1);
+    D4 ca2 = dcos(a2), sa2 = dsin(a2), cb2 = dcos(b2), sb2 = dsin(b2);
+    D4 ca1n = spow(ca1, s1.n), ca1n2 = spow(ca1, 2 - s1.n), sa1n = spow(sa1, s1.n), sa1n2 = spow(sa1, 2 - s1.n);
+    D4 cb1e = spow(cb1, s1.e), cb1e2 = spow(cb1, 2 - s1.e), sb1e = spow(sb1, s1.e), sb1e2 = spow(sb1, 2 - s1.e);
+    D4 ca2n = spow(ca2, s2.n), ca2n2 = spow(ca2, 2 - s2.n), sa2n = spow(sa2, s2.n), sa2n2 = spow(sa2, 2 - s2.n);
+    D4 cb2e = spow(cb2, s2.e), cb2e2 = spow(cb2, 2 - s2.e), sb2e = spow(sb2, s2.e), sb2e2 = spow(sb2, 2 - s2.e);
+    const D4 A1 = ((s1.r[1] * s1.r[2]) * ca1n2) * cb1e2, A2 = ((s1.r[0] * s1.r[2]) * sb1e2) * ca1n2,
+             A3 = (s1.r[0] * s1.r[1]) * sa1n2;
+    const D4 A4 = (s1.r[0] * ca1n) * cb1e, A5 = (s1.r[1] * sb1e) * ca1n, A6 = s1.r[2] * sa1n;
+    const D4 B1 = (s2.r[0] * ca2n) * cb2e, B2 = (s2.r[1] * sb2e) * ca2n, B3 = s2.r[2] * sa2n;
+    const D4 B4 = ((s2.r[1] * s2.r[2]) * ca2n2) * cb2e2, B5 = ((s2.r[0] * s2.r[2]) * sb2e2) * ca2n2,
+             B6 = (s2.r[0] * s2.r[1]) * sa2n2;
+    D4 n1[3], n2[3], d[3];
+    for (int k = 0; k < 3; ++k) {
+        n1[k] = (s1.R[k][0] * A1 + s1.R[k][1] * A2) + s1.R[k][2] * A3;
+        n2[k] = (s2.R[k][0] * B4 + s2.R[k][1] * B5) + s2.R[k][2] * B6;
+        d[k] = (((((((-(s1.R[k][0] * A4)) - s1.R[k][1] * A5) - s1.R[k][2] * A6) + s2.R[k][0] * B1) +
+                  s2.R[k][1] * B2) +
+                 s2.R[k][2] * B3) -
+                cst(s1.c[k])) +
+               cst(s2.c[k]);
+    }
+    D4 f[4];
+    f[0] = (-(n1[1] * d[2])) + n1[2] * d[1];
+    f[1] = n1[0] * d[2] - n1[2] * d[0];
+    f[2] = (-(n1[0] * d[1])) + n1[1] * d[0];
+    const D4 dotn = (n1[0] * n2[0] + n1[1] * n2[1]) + n1[2] * n2[2];
+    const D4 l1 = dsqrt((n1[0] * n1[0] + n1[1] * n1[1]) + n1[2] * n1[2]);
+    const D4 l2 = dsqrt((n2[0] * n2[0] + n2[1] * n2[1]) + n2[2] * n2[2]);
+    f[3] = dotn + l1 * l2;
+    for (int i = 0; i < 4; ++i) {
+        F[i] = f[i].v;
+        if (J)
+            for (int k = 0; k < 4; ++k) J[4 * i + k] = f[i].d[k];
+    }
+}
+
+// ------------------------------------------------------------------ MINPACK hybrd1, n = 4
+// Powell's hybrid method (More, Garbow, Hillstrom, ANL-80-74): enorm, qrfac, qform,
+// dogleg, r1updt, r1mpyq with the analytic Jacobian, as minpack.hpp:337-923 runs it.
+constexpr int N = 4;
+constexpr int LR = N * (N + 1) / 2;
+
+__device__ inline double enorm(int n, const double* x) {
+    const double rdwarf = 3.834e-20, rgiant = 1.304e19;
+    double s1 = 0, s2 = 0, s3 = 0, x1max = 0, x3max = 0;
+    const double agiant = rgiant / n;
+    for (int i = 0; i < n; ++i) {
+        const double xabs = fabs(x[i]);
+        if (xabs > rdwarf && xabs < agiant) {
+            s2 += xabs * xabs;
+        } else if (xabs <= rdwarf) {
+            if (xabs > x3max) {
+                s3 = 1.0 + s3 * (x3max / xabs) * (x3max / xabs);
+                x3max = xabs;
+            } else if (xabs != 0.0) {
+                s3 += (xabs / x3max) * (xabs / x3max);
+            }
+        } else {
+            if (xabs > x1max) {
+                s1 = 1.0 + s1 * (x1max / xabs) * (x1max / xabs);
+                x1max = xabs;
+            } else {
+                s1 += (xabs / x1max) * (xabs / x1max);
+            }
+        }
+    }
+    if (s1 != 0.0) return x1max * sqrt(s1 + (s2 / x1max) / x1max);
+    if (s2 != 0.0) {
+        if (s2 >= x3max) return sqrt(s2 * (1.0 + (x3max / s2) * (x3max * s3)));
+        return sqrt(x3max * ((s2 / x3max) + (x3max * s3)));
+    }
+    return x3max * sqrt(s3);
+}
+
+__device__ inline void qrfac(double* a, double* rdiag, double* acnorm) {
+    for (int j = 0; j < N; ++j) {
+        acnorm[j] = enorm(N, a + N * j);
+        rdiag[j] = acnorm[j];
+    }
+    for (int j = 0; j < N; ++j) {
+        double ajnorm = enorm(N - j, a + j + N * j);
+        if (ajnorm != 0.0) {
+            if (a[j + N * j] < 0.0) ajnorm = -ajnorm;
+            for (int i = j; i < N; ++i) a[i + N * j] /= ajnorm;
+            a[j + N * j] += 1.0;
+            for (int k = j + 1; k < N; ++k) {
+                double sum = 0.0;
+                for (int i = j; i < N; ++i) sum += a[i + N * j] * a[i + N * k];
+                const double temp = sum / a[j + N * j];
+                for (int i = j; i < N; ++i) a[i + N * k] -= temp * a[i + N * j];
+            }
+        }
+        rdiag[j] = -ajnorm;
+    }
+}
+
+__device__ inline void qform(double* q, double* wa) {
+    for (int j = 1; j < N; ++j)
+        for (int i = 0; i < j; ++i) q[i + N * j] = 0.0;
+    for (int l = 0; l < N; ++l) {
+        const int k = N - 1 - l;
+        for (int i = k; i < N; ++i) {
+            wa[i] = q[i + N * k];
+            q[i + N * k] = 0.0;
+        }
+        q[k + N * k] = 1.0;
+        if (wa[k] != 0.0) {
+            for (int j = k; j < N; ++j) {
+                double sum = 0.0;
+                for (int i = k; i < N; ++i) sum += q[i + N * j] * wa[i];
+                const double temp = sum / wa[k];
+                for (int i = k; i < N; ++i) q[i + N * j] -= temp * wa[i];
+            }
+        }
+    }
+}
+
+__device__ inline void dogleg(const double* r, const double* diag, const double* qtb, double delta, double* x,
+                              double* wa1, double* wa2) {
+    const double epsmch = 2.220446049250313e-16;
+    int jj = (N * (N + 1)) / 2 + 1;
+    for (int k = 1; k <= N; ++k) {
+        const int j = N - k + 1;
+        const int jp1 = j + 1;
+        jj -= k;
+        int l = jj + 1;
+        double sum = 0.0;
+        for (int i = jp1; i <= N; ++i) {
+            sum += r[l - 1] * x[i - 1];
+            ++l;
+        }
+        double temp = r[jj - 1];
+        if (temp == 0.0) {
+            l = j;
+            for (int i = 1; i <= j; ++i) {
+                temp = smax(temp, fabs(r[l - 1]));
+                l += N - i;
+            }
+            temp = epsmch * temp;
+            if (temp == 0.0) temp = epsmch;
+        }
+        x[j - 1] = (qtb[j - 1] - sum) / temp;
+    }
+    for (int j = 0; j < N; ++j) {
+        wa1[j] = 0.0;
+        wa2[j] = diag[j] * x[j];
+    }
+    const double qnorm = enorm(N, wa2);
+    if (qnorm <= delta) return;
+    int l = 1;
+    for (int j = 1; j <= N; ++j) {
+        const double temp = qtb[j - 1];
+        for (int i = j; i <= N; ++i) {
+            wa1[i - 1] += r[l - 1] * temp;
+            ++l;
+        }
+        wa1[j - 1] /= diag[j - 1];
+    }
+    const double gnorm = enorm(N, wa1);
+    double sgnorm = 0.0;
+    double alpha = delta / qnorm;
+    if (gnorm != 0.0) {
+        for (int j = 0; j < N; ++j) wa1[j] = (wa1[j] / gnorm) / diag[j];
+        l = 1;
+        for (int j = 1; j <= N; ++j) {
+            double sum = 0.0;
+            for (int i = j; i <= N; ++i) {
+                sum += r[l - 1] * wa1[i - 1];
+                ++l;
+            }
+            wa2[j - 1] = sum;
+        }
+        const double temp = enorm(N, wa2);
+        sgnorm = (gnorm / temp) / temp;
+        alpha = 0.0;
+        if (sgnorm < delta) {
+            const double bnorm = enorm(N, qtb);
+            double t = (bnorm / gnorm) * (bnorm / qnorm) * (sgnorm / delta);
+            t = t - (delta / qnorm) * (sgnorm / delta) * (sgnorm / delta) +
+                sqrt((t - (delta / qnorm)) * (t - (delta / qnorm)) +
+                     (1.0 - (delta / qnorm) * (delta / qnorm)) * (1.0 - (sgnorm / delta) * (sgnorm / delta)));
+            alpha = ((delta / qnorm) * (1.0 - (sgnorm / delta) * (sgnorm / delta))) / t;
+        }
+    }
+    const double temp = (1.0 - alpha) * smin(sgnorm, delta);
+    for (int j = 0; j < N; ++j) x[j] = temp * wa1[j] + alpha * x[j];
+}
+
+__device__ inline void r1updt(double* s, const double* u, double* v, double* w, bool& sing) {
+    const double giant = 1.79769313486231570e308;
+    const int m = N, n = N;
+    int jj = (n * (2 * m - n + 1)) / 2 - (m - n);
+    int l = jj;
+    for (int i = n; i <= m; ++i) {
+        w[i - 1] = s[l - 1];
+        ++l;
+    }
+    for (int nmj = 1; nmj <= n - 1; ++nmj) {
+        const int j = n - nmj;
+        jj -= m - j + 1;
+        w[j - 1] = 0.0;
+        if (v[j - 1] != 0.0) {
+            double cs, sn, tau;
+            if (fabs(v[n - 1]) < fabs(v[j - 1])) {
+                const double cotan = v[n - 1] / v[j - 1];
+                sn = 0.5 / sqrt(0.25 + 0.25 * cotan * cotan);
+                cs = sn * cotan;
+                tau = 1.0;
+                if (fabs(cs) * giant > 1.0) tau = 1.0 / cs;
+            } else {
+                const double tn = v[j - 1] / v[n - 1];
+                cs = 0.5 / sqrt(0.25 + 0.25 * tn * tn);
+                sn = cs * tn;
+                tau = sn;
+            }
+            v[n - 1] = sn * v[j - 1] + cs * v[n - 1];
+            v[j - 1] = tau;
+            l = jj;
+            for (int i = j; i <= m; ++i) {
+                const double temp = cs * s[l - 1] - sn * w[i - 1];
+                w[i - 1] = sn * s[l - 1] + cs * w[i - 1];
+                s[l - 1] = temp;
+                ++l;
+            }
+        }
+    }
+    for (int i = 1; i <= m; ++i) w[i - 1] += v[n - 1] * u[i - 1];
+    sing = false;
+    for (int j = 1; j <= n - 1; ++j) {
+        if (w[j - 1] != 0.0) {
+            double cs, sn, tau;
+            if (fabs(s[jj - 1]) < fabs(w[j - 1])) {
+                const double cotan = s[jj - 1] / w[j - 1];
+                sn = 0.5 / sqrt(0.25 + 0.25 * cotan * cotan);
+                cs = sn * cotan;
+                tau = 1.0;
+                if (fabs(cs) * giant > 1.0) tau = 1.0 / cs;
+            } else {
+                const double tn = w[j - 1] / s[jj - 1];
+                cs = 0.5 / sqrt(0.25 + 0.25 * tn * tn);
+                sn = cs * tn;
+                tau = sn;
+            }
+            l = jj;
+            for (int i = j; i <= m; ++i) {
+                const double temp = cs * s[l - 1] + sn * w[i - 1];
+                w[i - 1] = -sn * s[l - 1] + cs * w[i - 1];
+                s[l - 1] = temp;
+                ++l;
+            }
+            w[j - 1] = tau;
+        }
+        if (s[jj - 1] == 0.0) sing = true;
+        jj += m - j + 1;
+    }
+    l = jj;
+    for (int i = n; i <= m; ++i) {
+        s[l - 1] = w[i - 1];
+        ++l;
+    }
+    if (s[jj - 1] == 0.0) sing = true;
+}
+
+__device__ inline void r1mpyq(int m, int n, double* a, int lda, const double* v, const double* w) {
+    for (int nmj = 1; nmj <= n - 1; ++nmj) {
+        const int j = n - nmj;
+        double cs, sn;
+        if (fabs(v[j - 1]) > 1.0) {
+            cs = 1.0 / v[j - 1];
+            sn = sqrt(1.0 - cs * cs);
+        } else {
+            sn = v[j - 1];
+            cs = sqrt(1.0 - sn * sn);
+        }
+        for (int i = 1; i <= m; ++i) {
+            const double temp = cs * a[(i - 1) + lda * (j - 1)] - sn * a[(i - 1) + lda * (n - 1)];
+            a[(i - 1) + lda * (n - 1)] = sn * a[(i - 1) + lda * (j - 1)] + cs * a[(i - 1) + lda * (n - 1)];
+            a[(i - 1) + lda * (j - 1)] = temp;
+        }
+    }
+    for (int j = 1; j <= n - 1; ++j) {
+        double cs, sn;
+        if (fabs(w[j - 1]) > 1.0) {
+            cs = 1.0 / w[j - 1];
+            sn = sqrt(1.0 - cs * cs);
+        } else {
+            sn = w[j - 1];
+            cs = sqrt(1.0 - sn * sn);
+        }
+        for (int i = 1; i <= m; ++i) {
+            const double temp = cs * a[(i - 1) + lda * (j - 1)] + sn * a[(i - 1) + lda * (n - 1)];
+            a[(i - 1) + lda * (n - 1)] = -sn * a[(i - 1) + lda * (j - 1)] + cs * a[(i - 1) + lda * (n - 1)];
+            a[(i - 1) + lda * (j - 1)] = temp;
+        }
+    }
+}
+
+// hybrd1 with the analytic Jacobian; returns info (1 converged, 2 maxfev, 3 xtol too
+// small, 4 no progress; 5 is folded into 4 as minpack.hpp:918-921 does).
+__device__ inline int hybrd1(const SE& s1, const SE& s2, double* x, int* nfev_out) {
+    const double xtol = 1.0e-10, factor = 100.0, epsmch = 2.220446049250313e-16;
+    const int maxfev = 200 * (N + 1);
+    const double p1 = 0.1, p5 = 0.5, p001 = 0.001, p0001 = 0.0001;
+    double diag[N] = {1.0, 1.0, 1.0, 1.0};
+    double fvec[N], fjac[N * N], r[LR], qtf[N], wa1[N], wa2[N], wa3[N], wa4[N];
+    int info = 0, nfev = 0;
+    se_residual(s1, s2, x, fvec, nullptr);
+    nfev = 1;
+    double fnorm = enorm(N, fvec);
+    const int msum = N;
+    int iter = 1, ncsuc = 0, ncfail = 0, nslow1 = 0, nslow2 = 0;
+    double delta = 0.0, xnorm = 0.0;
+    for (;;) {
+        bool jeval = true;
+        double Jrow[N * N];
+        double ftmp[N];
+        se_residual(s1, s2, x, ftmp, Jrow);
+        for (int i = 0; i < N; ++i)
+            for (int j = 0; j < N; ++j) fjac[i + N * j] = Jrow[N * i + j];
+        nfev += msum;
+        qrfac(fjac, wa1, wa2);
+        if (iter == 1) {
+            for (int j = 0; j < N; ++j) wa3[j] = diag[j] * x[j];
+            xnorm = enorm(N, wa3);
+            delta = factor * xnorm;
+            if (delta == 0.0) delta = factor;
+        }
+        for (int i = 0; i < N; ++i) qtf[i] = fvec[i];
+        for (int j = 0; j < N; ++j) {
+            if (fjac[j + N * j] != 0.0) {
+                double sum = 0.0;
+                for (int i = j; i < N; ++i) sum += fjac[i + N * j] * qtf[i];
+                const double temp = -sum / fjac[j + N * j];
+                for (int i = j; i < N; ++i) qtf[i] += fjac[i + N * j] * temp;
+            }
+        }
+        for (int j = 0; j < N; ++j) {
+            int l = j;
+            for (int i = 0; i < j; ++i) {
+                r[l] = fjac[i + N * j];
+                l += N - i - 1;
+            }
+            r[l] = wa1[j];
+        }
+        qform(fjac, wa1);
+        for (;;) {
+            dogleg(r, diag, qtf, delta, wa1, wa2, wa3);
+            for (int j = 0; j < N; ++j) {
+                wa1[j] = -wa1[j];
+                wa2[j] = x[j] + wa1[j];
+                wa3[j] = diag[j] * wa1[j];
+            }
+            const double pnorm = enorm(N, wa3);
+            if (iter == 1) delta = smin(delta, pnorm);
+            se_residual(s1, s2, wa2, wa4, nullptr);
+            ++nfev;
+            const double fnorm1 = enorm(N, wa4);
+            double actred = -1.0;
+            if (fnorm1 < fnorm) actred = 1.0 - (fnorm1 / fnorm) * (fnorm1 / fnorm);
+            int l = 0;
+            for (int i = 0; i < N; ++i) {
+                double sum = 0.0;
+                for (int j = i; j < N; ++j) {
+                    sum += r[l] * wa1[j];
+                    ++l;
+                }
+                wa3[i] = qtf[i] + sum;
+            }
+            const double temp = enorm(N, wa3);
+            double prered = 0.0;
+            if (temp < fnorm) prered = 1.0 - (temp / fnorm) * (temp / fnorm);
+            double ratio = 0.0;
+            if (prered > 0.0) ratio = actred / prered;
+            if (ratio < p1) {
+                ncsuc = 0;
+                ++ncfail;
+                delta = p5 * delta;
+            } else {
+                ncfail = 0;
+                ++ncsuc;
+                if (ratio >= p5 || ncsuc > 1) delta = smax(delta, pnorm / p5);
+                if (fabs(ratio - 1.0) <= p1) delta = pnorm / p5;
+            }
+            if (ratio >= p0001) {
+                for (int j = 0; j < N; ++j) {
+                    x[j] = wa2[j];
+                    wa2[j] = diag[j] * x[j];
+                    fvec[j] = wa4[j];
+                }
+                xnorm = enorm(N, wa2);
+                fnorm = fnorm1;
+                ++iter;
+            }
+            ++nslow1;
+            if (actred >= p001) nslow1 = 0;
+            if (jeval) ++nslow2;
+            if (actred >= p1) nslow2 = 0;
+            if (delta <= xtol * xnorm || fnorm == 0.0) info = 1;
+            if (info != 0) break;
+            if (maxfev <= nfev) info = 2;
+            if (p1 * smax(p1 * delta, pnorm) <= epsmch * xnorm) info = 3;
+            if (nslow2 == 5) info = 4;
+            if (nslow1 == 10) info = 5;
+            if (info != 0) break;
+            if (ncfail == 2) break;
+            for (int j = 0; j < N; ++j) {
+                double sum = 0.0;
+                for (int i = 0; i < N; ++i) sum += fjac[i + N * j] * wa4[i];
+                wa2[j] = (sum - wa3[j]) / pnorm;
+                wa1[j] = diag[j] * ((diag[j] * wa1[j]) / pnorm);
+                if (ratio >= p0001) qtf[j] = sum;
+            }
+            bool sg = false;
+            r1updt(r, wa1, wa2, wa3, sg);
+            r1mpyq(N, N, fjac, N, wa2, wa3);
+            r1mpyq(1, N, qtf, 1, wa2, wa3);
+            jeval = false;
+        }
+        if (info != 0) break;
+    }
+    *nfev_out = nfev;
+    if (info == 5) info = 4;
+    return info;
+}
+
+// ------------------------------------------------------------------ kernels
+__device__ inline bool is_se(const Particles& P, int64_t k) { return P.shape[k] == SCOPI_SHAPE_SUPERELLIPSOID; }
+
+// initial angles: one lane per active particle
+__global__ void __launch_bounds__(kBlock) k_se_angles(Particles P, SeGrids G) {
+    const int64_t t = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    const int64_t na = P.n - P.n_inactive;
+    if (t >= na) return;
+    const int64_t k = P.n_inactive + t;
+    if (!is_se(P, k)) {
+        G.count[2 * t] = G.count[2 * t + 1] = 0;
+        return;
+    }
+    const SE s = load_se(P, k);
+    double* ang = G.ang + 2 * kSeGridAngles * t;
+    G.count[2 * t] = init_angles(s, false, ang);                 // a: ainit_yz
+    G.count[2 * t + 1] = init_angles(s, true, ang + kSeGridAngles);  // b: binit_xy
+}
+
+// grid points and normals: one lane per (active particle, grid point), meshgrid(a, b)
+// flattened row-major: point k = (a[k / nb], b[k % nb])
+__global__ void __launch_bounds__(kBlock) k_se_points(Particles P, SeGrids G) {
+    const int64_t gid = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    const int64_t na = P.n - P.n_inactive;
+    const int64_t t = gid / kSeGridPoints;
+    const int kp = (int)(gid - t * kSeGridPoints);
+    if (t >= na) return;
+    const int ca = G.count[2 * t], cb = G.count[2 * t + 1];
+    if (kp >= ca * cb) return;
+    const int64_t k = P.n_inactive + t;
+    const SE s = load_se(P, k);
+    const double* ang = G.ang + 2 * kSeGridAngles * t;
+    const double a = ang[kp / cb], b = ang[kSeGridAngles + kp % cb];
+    double p[3], nn[3];
+    se_point(s, a, b, p);
+    se_normal(s, a, b, nn);
+    double* out = G.pts + 6 * kSeGridPoints * t;
+    for (int d = 0; d < 3; ++d) {
+        out[d * kSeGridPoints + kp] = p[d];
+        out[(3 + d) * kSeGridPoints + kp] = nn[d];
+    }
+}
+
+// Penalised grid minimum: one workgroup of kSeGridBlock lanes per candidate pair.
+// Lane l owns grid points i = l, l + B, l + 2B of body 1 and scans body 2's grid
+// (staged in LDS) in order; its first strict minimum per i, then the (value, index)
+// minimum over lanes, is the first minimum of the row-major scan.  A term whose
+// penalty 2 (1 + n1.n2) already reaches the lane's best cannot win (the distance
+// term is >= 0), so its square root is skipped.
+constexpr int kSeGridBlock = 192;  // 3 x 192 = 576
+__global__ void __launch_bounds__(kSeGridBlock) k_se_grid(Particles P, SeGrids G, SeCandidates S, int64_t n) {
+    __shared__ double g2[6][kSeGridPoints];
+    __shared__ double red_v[kSeGridBlock];
+    __shared__ int red_i[kSeGridBlock];
+    const int64_t c = blockIdx.x;
+    if (c >= n) return;
+    const int64_t i = S.i[c], j = S.j[c];
+    if (!is_se(P, i) || !is_se(P, j)) return;
+    const int64_t t1 = i - P.n_inactive, t2 = j - P.n_inactive;
+    const int na1 = G.count[2 * t1], nb1 = G.count[2 * t1 + 1];
+    const int na2 = G.count[2 * t2], nb2 = G.count[2 * t2 + 1];
+    const int n1 = na1 * nb1, n2 = na2 * nb2;
+    const double* src2 = G.pts + 6 * kSeGridPoints * t2;
+    for (int k = threadIdx.x; k < n2; k += kSeGridBlock)
+        for (int d = 0; d < 6; ++d) g2[d][k] = src2[d * kSeGridPoints + k];
+    const double* src1 = G.pts + 6 * kSeGridPoints * t1;
+    double p1[3][3], m1[3][3], best[3];
+    int bj[3];
+    bool ok[3];
+    for (int r = 0; r < 3; ++r) {
+        const int ii = threadIdx.x + r * kSeGridBlock;
+        ok[r] = ii < n1;
+        best[r] = INFINITY;
+        bj[r] = -1;
+        for (int d = 0; d < 3; ++d) {
+            p1[r][d] = ok[r] ? src1[d * kSeGridPoints + ii] : 0.0;
+            m1[r][d] = ok[r] ? src1[(3 + d) * kSeGridPoints + ii] : 0.0;
+        }
+    }
+    __syncthreads();
+    for (int jj = 0; jj < n2; ++jj) {
+        const double nx = g2[3][jj], ny = g2[4][jj], nz = g2[5][jj];
+#pragma unroll
+        for (int r = 0; r < 3; ++r) {
+            const double pen = 2 * (1 + (m1[r][0] * nx + m1[r][1] * ny + m1[r][2] * nz));
+            if (ok[r] && pen < best[r]) {
+                const double dx = p1[r][0] - g2[0][jj], dy = p1[r][1] - g2[1][jj], dz = p1[r][2] - g2[2][jj];
+                const double dist = sqrt(dx * dx + dy * dy + dz * dz) + pen;
+                if (dist < best[r]) {
+                    best[r] = dist;
+                    bj[r] = jj;
+                }
+            }
+        }
+    }
+    // lane minimum in row-major order (i ascending over r), then across lanes
+    double bv = INFINITY;
+    int bidx = 0x7fffffff;
+    for (int r = 0; r < 3; ++r) {
+        if (bj[r] < 0) continue;
+        const int idx = (threadIdx.x + r * kSeGridBlock) * n2 + bj[r];
+        if (best[r] < bv || (best[r] == bv && idx < bidx)) {
+            bv = best[r];
+            bidx = idx;
+        }
+    }
+    red_v[threadIdx.x] = bv;
+    red_i[threadIdx.x] = bidx;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double v = red_v[0];
+        int idx = red_i[0];
+        for (int l = 1; l < kSeGridBlock; ++l)
+            if (red_v[l] < v || (red_v[l] == v && red_i[l] < idx)) {
+                v = red_v[l];
+                idx = red_i[l];
+            }
+        int im = 0, jm = 0;
+        if (idx != 0x7fffffff) {
+            im = idx / n2;
+            jm = idx - im * n2;
+        }
+        const double* ang1 = G.ang + 2 * kSeGridAngles * t1;
+        const double* ang2 = G.ang + 2 * kSeGridAngles * t2;
+        S.u0[4 * c + 0] = ang1[im / nb1];
+        S.u0[4 * c + 1] = ang1[kSeGridAngles + im % nb1];
+        S.u0[4 * c + 2] = ang2[jm / nb2];
+        S.u0[4 * c + 3] = ang2[kSeGridAngles + jm % nb2];
+    }
+}
+
+// hybrd1 from the grid start and the contact record: one lane per candidate pair.
+__global__ void __launch_bounds__(kBlock) k_se_solve(Particles P, SeCandidates S, int64_t n) {
+    const int64_t c = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (c >= n) return;
+    const int64_t i = S.i[c], j = S.j[c];
+    if (!is_se(P, i) || !is_se(P, j)) {
+        CP cp;
+        const bool ok = closest_points<3>(P, i, j, cp);
+        S.dij[c] = ok ? cp.dij : INFINITY;
+        const d3 z = mk(0.0, 0.0, 0.0);
+        store3(S.nij, S.cap, c, ok ? cp.nij : z);
+        store3(S.pi, S.cap, c, ok ? cp.pi : z);
+        store3(S.pj, S.cap, c, ok ? cp.pj : z);
+        S.info[c] = -1;
+        S.nfev[c] = 0;
+        return;
+    }
+    const SE s1 = load_se(P, i), s2 = load_se(P, j);
+    double x[4];
+    for (int k = 0; k < 4; ++k) x[k] = S.u0[4 * c + k];
+    int nfev = 0;
+    const int info = hybrd1(s1, s2, x, &nfev);
+    double pi[3], pj[3], nij[3];
+    se_point(s1, x[0], x[1], pi);
+    se_point(s2, x[2], x[3], pj);
+    se_normal(s2, x[2], x[3], nij);
+    const double dx = pi[0] - pj[0], dy = pi[1] - pj[1], dz = pi[2] - pj[2];
+    const double sd = sgn(dx * nij[0] + dy * nij[1] + dz * nij[2]);
+    S.dij[c] = sd * sqrt(dx * dx + dy * dy + dz * dz);
+    for (int d = 0; d < 3; ++d) {
+        S.nij[d * S.cap + c] = nij[d];
+        S.pi[d * S.cap + c] = pi[d];
+        S.pj[d * S.cap + c] = pj[d];
+    }
+    for (int k = 0; k < 4; ++k) S.u[4 * c + k] = x[k];
+    S.info[c] = info;
+    S.nfev[c] = nfev;
+}
+
+}  // namespace
+
+// ------------------------------------------------------------------ host
+void se_grids_free(SeGrids& G) {
+    for (void* p : {(void*)G.ang, (void*)G.count, (void*)G.pts})
+        if (p) (void)hipFree(p);
+    G = SeGrids{};
+}
+
+void se_grids_reserve(SeGrids& G, int64_t n) {
+    if (n <= G.cap) return;
+    se_grids_free(G);
+    const int64_t cap = std::max<int64_t>(n, 64);
+    G.cap = cap;
+    SCOPI_HIP_CHECK(hipMalloc(&G.ang, 2 * kSeGridAngles * cap * sizeof(double)));
+    SCOPI_HIP_CHECK(hipMalloc(&G.count, 2 * cap * sizeof(int32_t)));
+    SCOPI_HIP_CHECK(hipMalloc(&G.pts, (size_t)6 * kSeGridPoints * cap * sizeof(double)));
+}
+
+void se_candidates_free(SeCandidates& S) {
+    for (void* p : {(void*)S.i, (void*)S.j, (void*)S.u0, (void*)S.u, (void*)S.info, (void*)S.nfev, (void*)S.dij,
+                    (void*)S.nij, (void*)S.pi, (void*)S.pj})
+        if (p) (void)hipFree(p);
+    S = SeCandidates{};
+}
+
+void se_candidates_reserve(SeCandidates& S, int64_t n) {
+    if (n <= S.cap) return;
+    se_candidates_free(S);
+    const int64_t cap = std::max<int64_t>(n + n / 4, 256);
+    S.cap = cap;
+    SCOPI_HIP_CHECK(hipMalloc(&S.i, cap * sizeof(int32_t)));
+    SCOPI_HIP_CHECK(hipMalloc(&S.j, cap * sizeof(int32_t)));
+    SCOPI_HIP_CHECK(hipMalloc(&S.u0, 4 * cap * sizeof(double)));
+    SCOPI_HIP_CHECK(hipMalloc(&S.u, 4 * cap * sizeof(double)));
+    SCOPI_HIP_CHECK(hipMalloc(&S.info, cap * sizeof(int32_t)));
+    SCOPI_HIP_CHECK(hipMalloc(&S.nfev, cap * sizeof(int32_t)));
+    SCOPI_HIP_CHECK(hipMalloc(&S.dij, cap * sizeof(double)));
+    SCOPI_HIP_CHECK(hipMalloc(&S.nij, 3 * cap * sizeof(double)));
+    SCOPI_HIP_CHECK(hipMalloc(&S.pi, 3 * cap * sizeof(double)));
+    SCOPI_HIP_CHECK(hipMalloc(&S.pj, 3 * cap * sizeof(double)));
+}
+
+void se_closest_points(const Particles& P, SeGrids& G, SeCandidates& S, int64_t n, hipStream_t s) {
+    const int64_t na = P.n - P.n_inactive;
+    if (n <= 0 || na <= 0) return;
+    se_grids_reserve(G, na);
+    hipLaunchKernelGGL(k_se_angles, dim3(grid_for(na)), dim3(kBlock), 0, s, P, G);
+    hipLaunchKernelGGL(k_se_points, dim3(grid_for(na * kSeGridPoints)), dim3(kBlock), 0, s, P, G);
+    hipLaunchKernelGGL(k_se_grid, dim3((unsigned)n), dim3(kSeGridBlock), 0, s, P, G, S, n);
+    hipLaunchKernelGGL(k_se_solve, dim3(grid_for(n)), dim3(kBlock), 0, s, P, S, n);
+    SCOPI_HIP_CHECK(hipGetLastError());
+}
+
+}  // namespace scopi_hip

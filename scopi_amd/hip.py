@@ -27,7 +27,7 @@ EXPORTS = [
     "scopi_hip_compute_contacts", "scopi_hip_download_contacts", "scopi_hip_upload_contacts",
     "scopi_hip_set_a_priori_velocity", "scopi_hip_solve", "scopi_hip_download_solution",
     "scopi_hip_update_and_move", "scopi_hip_displacement_obstacles", "scopi_hip_run",
-    "scopi_hip_set_kernel_timing", "scopi_hip_kernel_timing", "scopi_hip_set_tuning",
+    "scopi_hip_set_kernel_timing", "scopi_hip_kernel_timing", "scopi_hip_set_tuning", "scopi_hip_se_closest_points",
 ]
 TUNE_REDUCE_BLOCKS, TUNE_BT_GROUP = 1, 2
 
@@ -93,6 +93,7 @@ def lib():
     L.scopi_hip_set_kernel_timing.argtypes = [P, C.c_int32]
     L.scopi_hip_kernel_timing.argtypes = [P, D]
     L.scopi_hip_set_tuning.argtypes = [P, C.c_int32, C.c_int32]
+    L.scopi_hip_se_closest_points.argtypes = [P, C.c_int64] + [D] * 12 + [C.c_void_p] * 4
     for name in [n for n in dir(L) if n.startswith("scopi_hip_") and n != "scopi_hip_last_error"]:
         getattr(L, name).restype = C.c_int
     _lib = L
@@ -221,6 +222,21 @@ class HipScene:
 
     def set_tuning(self, key, value):
         self._check(self._L.scopi_hip_set_tuning(self._h, int(key), int(value)), "set_tuning")
+
+    def se_closest_points(self, pos_i, q_i, size_i, sq_i, pos_j, q_j, size_j, sq_j):
+        """closest_points(superellipsoid<3>, superellipsoid<3>) of n independent pairs on the device."""
+        f = lambda a, w: np.ascontiguousarray(np.asarray(a, np.float64).reshape(-1, w))
+        args = [f(pos_i, 3), f(q_i, 4), f(size_i, 3), f(sq_i, 2), f(pos_j, 3), f(q_j, 4), f(size_j, 3), f(sq_j, 2)]
+        n = args[0].shape[0]
+        if any(a.shape[0] != n for a in args):
+            raise ValueError("se_closest_points: every array needs one row per pair")
+        m = max(n, 1)
+        dij = np.zeros(m); nij = np.zeros((m, 3)); pi = np.zeros((m, 3)); pj = np.zeros((m, 3))
+        u0 = np.zeros((m, 4)); u = np.zeros((m, 4)); info = np.zeros(m, np.int32); nfev = np.zeros(m, np.int32)
+        self._check(self._L.scopi_hip_se_closest_points(self._h, n, *args, dij, nij, pi, pj, _ptr(u0), _ptr(u),
+                                                        _ptr(info), _ptr(nfev)), "se_closest_points")
+        return dict(dij=dij[:n], nij=nij[:n], pi=pi[:n], pj=pj[:n], u0=u0[:n], u=u[:n], info=info[:n],
+                    nfev=nfev[:n])
 
     def kernel_timing(self):
         out = np.zeros(6)

@@ -236,6 +236,35 @@ def spheres_3d_count(n_particles, seed=0, **kw):
     return spheres_3d(nx=nx, ny=ny, nz=nz, seed=seed, **kw)
 
 
+def superellipsoids_3d(nx=32, ny=32, nz=32, spacing=0.035, jitter=0.003, amin=0.008, amax=0.015, emin=0.4,
+                       emax=1.6, velocity_scale=0.5, seed=2024):
+    """Jittered nx*ny*nz lattice of superellipsoids, no obstacles, no gravity.
+
+    Semi-axes U[amin, amax]^3, squareness (e, n) U[emin, emax]^2, orientations uniform
+    random unit quaternions (normalised 4-Gaussians), m = 1, J = (0.1, 0.1, 0.1).  The
+    lattice spacing leaves most neighbours separated, so the scene is driven by seeded
+    uniform initial velocities in [-velocity_scale, velocity_scale]^3 (vap_fpd with zero
+    force keeps them): approaching pairs come into contact within a few steps.
+    """
+    rng = np.random.default_rng(seed)
+    n = nx * ny * nz
+    ii, jj, kk = np.meshgrid(np.arange(nx), np.arange(ny), np.arange(nz), indexing="ij")
+    base = np.stack([(ii.ravel() + 0.5), (jj.ravel() + 0.5), (kk.ravel() + 0.5)], axis=1) * spacing
+    pos = base + rng.uniform(-jitter, jitter, size=(n, 3))
+    size = rng.uniform(amin, amax, size=(n, 3))
+    sq = rng.uniform(emin, emax, size=(n, 2))
+    q = rng.normal(size=(n, 4))
+    q /= np.linalg.norm(q, axis=1, keepdims=True)
+    v = rng.uniform(-velocity_scale, velocity_scale, size=(n, 3)) if velocity_scale > 0 else np.zeros((n, 3))
+    J = np.full((n, 3), 0.1)
+    return Scene(dim=3, n_inactive=0, shape=np.full(n, SUPERELLIPSOID, np.int32), pos=pos, q=q, size=size,
+                 squareness=sq, v=v, vd=np.zeros((n, 3)), omega=np.zeros((n, 3)), domega=np.zeros((n, 3)),
+                 f=np.zeros((n, 3)), m=np.ones(n), J=J)
+
+
 # Default solver parameters of the workloads (BASELINE configurations).
 CONFIG1 = dict(dim=2, dt=0.005, dmax=0.025, kd_tree_radius=(0.024 + 0.025) ** 2, tolerance=1e-6, steps=200)
 CONFIG2 = dict(dim=3, dt=1e-3, dmax=0.02, kd_tree_radius=0.042 ** 2, tolerance=1e-6)
+# kd_tree_radius = (2 |a|_max + dmax)^2 with |a|_max = sqrt(3) 0.015: never binding for these shapes
+CONFIG3 = dict(dim=3, dt=1e-3, dmax=0.02, kd_tree_radius=(2 * math.sqrt(3.0) * 0.015 + 0.02) ** 2, mu=0.3,
+               tolerance=1e-6, steps=100)

@@ -1,0 +1,198 @@
+"""GPU parity of the superellipsoid path (closest points, contact set, friction solve, steps).
+
+The superellipsoid closest-point solve (grid start + MINPACK hybrd1) is chaotic at
+the rounding level: one ulp in an input can move the end point by ~1e-6 in the
+distance.  The oracle therefore runs with its portable elementary functions
+(oracle.set_se_math(True)), which the device evaluates with the same operations, and
+the bar for closest points and contact sets is bit-for-bit equality.  Solves and
+steps are held to the solver bars of the sphere tests.
+"""
+import math
+import sys
+
+import numpy as np
+import pytest
+
+from scopi_amd import hip, scenes
+from scopi_amd.scenes import quaternion
+from tests import oracle_problem as op
+
+pytestmark = pytest.mark.gpu
+
+KD3 = scenes.CONFIG3["kd_tree_radius"]
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+@pytest.fixture
+def se_oracle(oracle_lib):
+    oracle_lib.set_se_math(True)
+    yield oracle_lib
+    oracle_lib.set_se_math(False)
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    g = hip.HipScene(scenes.superellipsoids_3d(2, 2, 2, seed=1))
+    yield g
+    g.close()
+
+
+def random_pairs(n, seed, gap_lo=-0.01, gap_hi=0.003):
+    rng = np.random.default_rng(seed)
+    qa = rng.normal(size=(n, 4)); qa /= np.linalg.norm(qa, axis=1, keepdims=True)
+    qb = rng.normal(size=(n, 4)); qb /= np.linalg.norm(qb, axis=1, keepdims=True)
+    ra = rng.uniform(0.008, 0.015, (n, 3)); rb = rng.uniform(0.008, 0.015, (n, 3))
+    ea = rng.uniform(0.4, 1.6, (n, 2)); eb = rng.uniform(0.4, 1.6, (n, 2))
+    ca = rng.uniform(-1, 1, (n, 3))
+    cb = ca + np.array([0.035, 0.0, 0.0]) + rng.uniform(gap_lo, gap_hi, (n, 3))
+    return ca, qa, ra, ea, cb, qb, rb, eb
+
+
+def oracle_pairs(se, args):
+    n = len(args[0])
+    out = {k: [] for k in ("dij", "nij", "pi", "pj", "u0", "u", "info", "nfev")}
+    for k in range(n):
+        r = se.se_closest_points(*(a[k] for a in args))
+        for key in out:
+            out[key].append(r[key])
+    return {k: np.array(v) for k, v in out.items()}
+
+
+def test_se_pairs_bit_exact(se_oracle, ctx):
+    """Random pairs (separated and overlapping) and the reference's known answers."""
+    args = random_pairs(48, 5)
+    c30, s30 = math.sqrt(3.0) / 2.0, 0.5
+    known = [((-0.2, 0, 0), quaternion(0.0), (0.1, 0.2, 0.3), (1, 1), (0.2, 0, 0), quaternion(0.0), (0.1, 0.2, 0.3),
+              (1, 1)),
+             ((-0.4 * c30, 0.4 * s30, 0), quaternion(-math.pi / 6), (0.1, 0.4, 0.5), (1, 1),
+              (0.4 * c30, -0.4 * s30, 0), quaternion(math.pi - math.pi / 6), (0.2, 0.3, 0.6), (1, 1))]
+    args = tuple(np.concatenate([a, np.array([kk[m] for kk in known], np.float64)]) for m, a in enumerate(args))
+    say(f"[se-pairs] n={len(args[0])}")
+    g = ctx.se_closest_points(*args)
+    o = oracle_pairs(se_oracle, args)
+    say(f"  info counts gpu={np.bincount(g['info'])} nfev max={g['nfev'].max()}")
+    for k in ("u0", "u", "info", "nfev", "dij", "nij", "pi", "pj"):
+        assert np.array_equal(g[k], o[k]), (k, np.abs(np.asarray(g[k], float) - np.asarray(o[k], float)).max())
+    assert abs(g["dij"][-2] - 0.2) < 1e-6 and abs(g["dij"][-1] - 0.5) < 1e-6
+
+
+def hip_contacts(sc, prm):
+    g = hip.HipScene(sc)
+    nc = g.compute_contacts(prm)
+    c = g.contacts()
+    g.close()
+    return nc, c
+
+
+@pytest.mark.parametrize("method", [hip.KDTREE, hip.BRUTE_FORCE], ids=["kdtree", "brute_force"])
+def test_se_contact_set_bit_exact(se_oracle, method):
+    sc = scenes.superellipsoids_3d(5, 5, 4, seed=7) if method == hip.KDTREE else scenes.superellipsoids_3d(3, 3, 3,
+                                                                                                           seed=8)
+    prm = hip.params(problem=hip.FRICTION, mu=0.3, dmax=0.02, kd_tree_radius=KD3, contact_method=method)
+    say(f"[se-contacts] method={method} n={sc.n}")
+    nc, g = hip_contacts(sc, prm)
+    o = se_oracle.Oracle(sc, problem=1, mu=0.3, dmax=0.02, kd_tree_radius=KD3, contact_method=method)
+    assert o.compute_contacts() == nc and nc > 0
+    oc = o.contacts()
+    assert np.array_equal(g["ij"], oc["ij"])
+    for k in ("dij", "nij", "pi", "pj"):
+        assert np.array_equal(g[k], oc[k]), k
+    say(f"  nc={nc}")
+
+
+def se_case(seed=11):
+    return scenes.superellipsoids_3d(4, 4, 4, seed=seed, velocity_scale=3.0)
+
+
+def se_first_step(sc, prm, dt):
+    g = hip.HipScene(sc)
+    nc = g.compute_contacts(prm)
+    g.set_a_priori_velocity(prm, dt)
+    info = g.solve(prm, dt)
+    sol = g.solution()
+    g.close()
+    return np.concatenate([sol["u"].ravel(), sol["w"].ravel()]), info, nc
+
+
+def exact_optimum(se, sc, dt):
+    o = se.Oracle(sc, problem=1, mu=0.3, dmax=0.02, kd_tree_radius=KD3, vap=1)
+    o.compute_contacts()
+    o.set_a_priori_velocity(dt)
+    A, invM, c, U, nij = op.sparse_problem(o, sc, dt, friction=True)
+    # a-priori velocities: vap_fpd with zero force keeps v, so U is the scene's velocity
+    lam, u, L = op.exact_solution(A, invM, c, dt, U, friction=True, nij=nij, mu=0.3, dim=3)
+    na = sc.n_active
+    u_exact = np.concatenate([u[:3 * na], u[3 * na:]])
+    return o, u_exact, L
+
+
+@pytest.mark.parametrize("precision", [hip.FP64, hip.FP32_REFINE], ids=["fp64", "fp32_refine"])
+def test_se_friction_solve(se_oracle, precision):
+    """Friction mu 0.3 on superellipsoid contacts: the fixed-step path agrees with the oracle and the
+    exact optimum to 1e-8; the adaptive path is within max(3 e_oracle, 1e-6) of the optimum."""
+    dt = 1e-3
+    sc = se_case()
+    o, u_exact, L = exact_optimum(se_oracle, sc, dt)
+    na = sc.n_active
+    prm = hip.params(problem=hip.FRICTION, mu=0.3, dmax=0.02, kd_tree_radius=KD3, vap=hip.VAP_FPD,
+                     dynamic_descent=0, alpha=1.0 / L, tolerance=1e-10, max_ite=200000, precision=precision)
+    u, info, nc = se_first_step(sc, prm, dt)
+    say(f"[se-solve] precision={precision} nc={nc} iterations={info['iterations']}")
+    assert info["converged"] == 1 and nc == len(o.contacts()["dij"])
+    np.testing.assert_allclose(u, u_exact, atol=1e-8)
+    o.set_params(problem=1, mu=0.3, dmax=0.02, kd_tree_radius=KD3, vap=1, dynamic_descent=0, alpha=1.0 / L,
+                 tolerance=1e-10, max_ite=200000)
+    r = o.solve(dt)
+    np.testing.assert_allclose(u, np.concatenate([r["u"].ravel(), r["w"].ravel()]), atol=1e-8)
+    # reference defaults (dynamic descent), tol 1e-9
+    prm = hip.params(problem=hip.FRICTION, mu=0.3, dmax=0.02, kd_tree_radius=KD3, vap=hip.VAP_FPD, tolerance=1e-9,
+                     max_ite=200000, precision=precision)
+    u, info, _ = se_first_step(sc, prm, dt)
+    o.set_params(problem=1, mu=0.3, dmax=0.02, kd_tree_radius=KD3, vap=1, tolerance=1e-9, max_ite=200000)
+    r = o.solve(dt)
+    e_orc = float(np.abs(np.concatenate([r["u"].ravel(), r["w"].ravel()]) - u_exact).max())
+    e_gpu = float(np.abs(u - u_exact).max())
+    say(f"  adaptive e_gpu={e_gpu:.3e} e_oracle={e_orc:.3e} iterations={info['iterations']}")
+    assert info["converged"] == 1 and e_gpu <= max(3 * e_orc, 1e-6)
+    assert na == 64
+
+
+def test_se_steps_against_oracle(se_oracle):
+    """Four full steps (Friction mu 0.3, fixed step so both sides iterate the same map).
+
+    After the first step the two states differ by the solver's rounding, and the
+    superellipsoid solve amplifies differences of one ulp into ~1e-6 in the
+    distance (it stops on "no progress" wherever the roundings leave it).  The bar is
+    therefore calibrated on the oracle itself: the GPU trajectory must be as close to
+    the oracle's as the oracle's is to its own run from a state perturbed by one ulp
+    (x20), or within 1e-9."""
+    dt = 1e-3
+    steps = 4
+    sc = scenes.superellipsoids_3d(3, 3, 4, seed=13, velocity_scale=3.0)
+    _, _, L = exact_optimum(se_oracle, sc, dt)
+    kw = dict(problem=1, mu=0.3, dmax=0.02, kd_tree_radius=KD3, vap=1, dynamic_descent=0, alpha=0.5 / L,
+              tolerance=1e-11, max_ite=400000)
+
+    def oracle_run(scene):
+        o = se_oracle.Oracle(scene, **kw)
+        o.run(dt, steps)
+        return o.state()
+    ref = oracle_run(sc)
+    spread = 0.0
+    for sign in (1.0, -1.0):
+        pert = scenes.superellipsoids_3d(3, 3, 4, seed=13, velocity_scale=3.0)
+        pert.pos[:] = np.nextafter(pert.pos, pert.pos + sign)
+        spread = max(spread, float(np.abs(oracle_run(pert)["pos"] - ref["pos"]).max()))
+    g = hip.HipScene(sc)
+    info = g.run(hip.params(problem=hip.FRICTION, mu=0.3, dmax=0.02, kd_tree_radius=KD3, vap=hip.VAP_FPD,
+                            dynamic_descent=0, alpha=0.5 / L, tolerance=1e-11, max_ite=400000), dt, steps)
+    sg = g.state()
+    g.close()
+    dpos = float(np.abs(sg["pos"] - ref["pos"]).max())
+    say(f"[se-steps] max|dpos|={dpos:.3e} oracle 1-ulp spread={spread:.3e} contacts(last)={info['n_contacts']}")
+    assert info["n_contacts"] > 0
+    assert dpos <= max(20.0 * spread, 1e-9)
