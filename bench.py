@@ -1,6 +1,7 @@
 """Benchmark: time-steps/s of the contact step at 131,072 spheres (APGD tol 1e-6), one MI355X.
 
 python bench.py [--gpus 1] [--steps K] [--warmup W] [--precision fp64|fp32_refine] [--no-cpu]
+                [--kernel-steps N] [--calibrate-copy MIB] [--tune]
 
 Workload: 131,072 spheres, radii U[0.009, 0.011], on a
 jittered 64x64x32 lattice (spacing 0.025, jitter +-0.003) in a box of 5 planes,
@@ -11,9 +12,12 @@ loop (obstacles, contact search, vap, B assembly, APGD, velocity update,
 integration).  W untimed steps, then K timed steps from the state they left.
 
 Prints one JSON line (rank 0) with the metric, a `roofline` object for the
-dominant solver kernel (algorithmic bytes per launch / average launch time from
-HIP events on the library's stream) and a `cpu_baseline` object (the CPU
-restatement of the reference, timed on a bounded sample of the same workload).
+dominant solver kernel and a `cpu_baseline` object (the CPU restatement of the
+reference, timed on a bounded sample of the same workload).  The K timed steps run
+uninstrumented; right after them, --kernel-steps more steps run with HIP events on
+the library's stream around every solver kernel (events serialise the host loop,
+so that pass only yields the per-kernel average launch times): roofline.achieved
+= algorithmic bytes per launch / that average.
 """
 from __future__ import annotations
 
@@ -33,27 +37,28 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def kernel_bytes(nc, na, s, friction=False):
-    """Algorithmic (unique) HBM bytes per launch of each projected-gradient kernel.
+# (rows per contact NR, B coefficients per contact NC, per transposed entry NT, per particle NW)
+LAYOUTS = {"rotation_free": (1, 3, 3, 3), "rigid": (1, 9, 6, 6), "cone": (3, 21, 9, 6)}
 
-    s = value bytes (8 FP64, 4 FP32).  NoFriction: 9 coefficients per contact
-    (n, a_i, a_j), int32 particle indices, one row per contact.
-      k_bt    : lambda, 9 coefficients and the j-list entry per contact, 4 range
-                ints + M^-1 (6) + w (6) per particle;
-      k_row   : i, j, 9 coefficients, lambda+ read, Q lambda+ write per contact,
-                w (6) per particle;
-      k_trial : lambda_k, lambda_{k-1}, Q lambda_k, Q lambda_{k-1}, c read and
-                lambda+ write per row.
+
+def kernel_bytes(nc, nc_obs, na, s, layout="rotation_free"):
+    """Algorithmic (unique) HBM bytes of one launch of each projected-gradient kernel.
+
+    s = value bytes (8 FP64, 4 FP32); entries of the particle-major copy of B:
+    2 per particle pair, 1 per obstacle contact.
+      k_trial : lambda_k, lambda_{k-1}, Q lambda_k, Q lambda_{k-1}, c read and lambda+
+                written per row (+ the normal and mu per contact for the cone);
+      k_bt    : per entry its contact index (int32) and NT coefficients; lambda+ once
+                per row; per particle its entry offset (int32), M^-1 and w (NW each);
+      k_row   : per contact i, j (int32), NC coefficients, lambda+ read and Q lambda+
+                written per row; w (NW) per particle.
     """
-    if friction:
-        rows = 3 * nc
-        bt = nc * (3 * s + 18 * s + 4) + na * (16 + 12 * s)
-        row = nc * (8 + 18 * s + 6 * s) + na * 6 * s
-        trial = rows * 6 * s + nc * (4 * s)
-    else:
-        bt = nc * (s + 9 * s + 4) + na * (16 + 12 * s)
-        row = nc * (8 + 9 * s + 2 * s) + na * 6 * s
-        trial = nc * 6 * s
+    NR, NC, NT, NW = LAYOUTS[layout]
+    rows = NR * nc
+    entries = 2 * nc - nc_obs
+    trial = rows * 6 * s + (nc * 4 * s if layout == "cone" else 0)
+    bt = entries * (4 + NT * s) + rows * s + na * (4 + 2 * NW * s)
+    row = nc * 8 + nc * NC * s + 2 * rows * s + na * NW * s
     return dict(bt=bt, row=row, trial=trial)
 
 
@@ -96,6 +101,19 @@ def cpu_baseline(sc, prm_kw, per_step_counts, budget_s=20.0):
                 seconds_per_step=t_step, q_product_ms=t_q * 1e3, contact_search_s=t_contacts)
 
 
+def copy_calibration(mb):
+    """A plain device copy of `mb` MiB (read + write) so that a counter run (rocprofv3 --pmc
+    FETCH_SIZE / WRITE_SIZE) sees a kernel of known traffic beside the solver kernels."""
+    import torch
+    n = mb * (1 << 20) // 4
+    x = torch.ones(n, dtype=torch.float32, device="cuda")
+    y = torch.empty_like(x)
+    torch.cuda.synchronize()
+    y.copy_(x)
+    torch.cuda.synchronize()
+    del x, y
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -107,12 +125,17 @@ def main():
     ap.add_argument("--ny", type=int, default=64)
     ap.add_argument("--nz", type=int, default=32)
     ap.add_argument("--tune", action="store_true", help="sweep the solver's launch shapes before timing")
+    ap.add_argument("--kernel-steps", type=int, default=3,
+                    help="steps of the instrumented pass (HIP events around every solver kernel)")
+    ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
+                    help="run one plain device copy of MIB MiB first (traffic calibration for counter runs)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py runs on one GPU only: the contact step of one scene is not sharded across GPUs "
               "in this build (see DESIGN.md, 'More than one GPU').", file=sys.stderr)
         sys.exit(2)
 
+    import numpy as np
     import torch
     from scopi_amd import hip, scenes
 
@@ -122,6 +145,8 @@ def main():
     prm = hip.params(vap=hip.VAP_FPD, dmax=0.02, kd_tree_radius=0.042 ** 2, tolerance=1e-6, precision=precision)
     dt = 1e-3
     torch.cuda.init()
+    if args.calibrate_copy:
+        copy_calibration(args.calibrate_copy)
     g = hip.HipScene(sc, capacity_contacts=20 * sc.n)
     log(f"[bench] warmup {args.warmup} steps")
     if args.warmup:
@@ -132,30 +157,56 @@ def main():
         g.set_a_priori_velocity(prm, dt)
         tuned = tune.tune(g, prm, dt)
         log(f"[bench] tuned launch shapes: {tuned}")
-    g.set_kernel_timing(True)
     torch.cuda.synchronize()
     log(f"[bench] timing {args.steps} steps")
     t0 = time.perf_counter()
     info = g.run(prm, dt, args.steps)
     torch.cuda.synchronize()
     elapsed = time.perf_counter() - t0
-    kt = g.kernel_timing()
     steps_per_s = args.steps / elapsed
     nc = info["n_contacts"]
     na = sc.n_active
     it_per_step = info["iterations"] / args.steps
     bt_per_step = info["backtracks"] / args.steps
-    s = 8 if precision == hip.FP64 else 4
-    kb = kernel_bytes(nc, na, 8)
-    per = {}
-    for name, ms_key, n_key in (("bt", "bt_ms", "bt_launches"), ("row", "row_ms", "row_launches"),
-                                ("trial", "trial_ms", "trial_launches")):
-        n = kt[n_key]
-        per[name] = dict(avg_us=(kt[ms_key] / n * 1e3) if n else None, launches=n, total_ms=kt[ms_key])
-    dom = max(per, key=lambda k: per[k]["total_ms"])
-    kname = {"bt": "k_bt", "row": "k_row", "trial": "k_trial"}[dom]
-    avg_s = per[dom]["avg_us"] * 1e-6 if per[dom]["avg_us"] else None
-    achieved = (kb[dom] / avg_s / 1e9) if avg_s else None
+    it32 = info["iterations_fp32"] / max(1, info["iterations"])
+
+    # Instrumented pass: HIP events on the library's stream around every solver kernel
+    # (they serialise the host loop, so this pass is not the throughput figure).
+    per, kt_steps_per_s = {}, None
+    if args.kernel_steps > 0:
+        log(f"[bench] instrumented pass, {args.kernel_steps} steps")
+        g.set_kernel_timing(True)
+        t1 = time.perf_counter()
+        info_k = g.run(prm, dt, args.kernel_steps)
+        torch.cuda.synchronize()
+        kt_steps_per_s = args.kernel_steps / (time.perf_counter() - t1)
+        kt = g.kernel_timing()
+        g.set_kernel_timing(False)
+        nc = info_k["n_contacts"]
+        it32 = info_k["iterations_fp32"] / max(1, info_k["iterations"])
+        for name, ms_key, n_key in (("bt", "bt_ms", "bt_launches"), ("row", "row_ms", "row_launches"),
+                                    ("trial", "trial_ms", "trial_launches")):
+            n = kt[n_key]
+            per[name] = dict(avg_us=(kt[ms_key] / n * 1e3) if n else None, launches=n, total_ms=kt[ms_key])
+    ct = g.contacts() if nc else dict(ij=np.zeros((0, 2), np.int64))
+    nc_obs = int(np.count_nonzero(ct["ij"][:, 0] < sc.n_inactive))
+    # FP32 phase launches move s = 4 bytes per value, FP64 ones s = 8 (weighted by iterations)
+    kb64 = kernel_bytes(nc, nc_obs, na, 8)
+    kb32 = kernel_bytes(nc, nc_obs, na, 4)
+    kb = {k: it32 * kb32[k] + (1 - it32) * kb64[k] for k in kb64}
+    roofline = None
+    if per:
+        dom = max(per, key=lambda k: per[k]["total_ms"])
+        avg_s = per[dom]["avg_us"] * 1e-6 if per[dom]["avg_us"] else None
+        achieved = (kb[dom] / avg_s / 1e9) if avg_s else None
+        for k in per:
+            a = per[k]["avg_us"]
+            per[k]["algorithmic_bytes"] = kb[k]
+            per[k]["achieved_gbs"] = kb[k] / (a * 1e-6) / 1e9 if a else None
+        roofline = {"bound": "hbm", "kernel": {"bt": "k_bt", "row": "k_row", "trial": "k_trial"}[dom],
+                    "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                    "frac": (achieved / HBM_PEAK_GBS) if achieved else None, "traffic": None,
+                    "algorithmic_bytes_per_launch": kb[dom], "avg_launch_us": per[dom]["avg_us"]}
     log(f"[bench] {steps_per_s:.3f} steps/s, nc={nc}, iterations/step={it_per_step:.1f}, "
         f"backtracks/step={bt_per_step:.1f}, kernels={per}")
     result = {
@@ -178,12 +229,9 @@ def main():
         "apgd_iterations_per_step": it_per_step,
         "line_search_rejections_per_step": bt_per_step,
         "kernels": per,
-        "roofline": {"bound": "hbm", "kernel": kname, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": (achieved / HBM_PEAK_GBS) if achieved else None, "traffic": None,
-                     "algorithmic_bytes_per_launch": kb[dom]},
+        "instrumented_steps_per_s": kt_steps_per_s,
+        "roofline": roofline,
     }
-    if precision != hip.FP64:
-        result["roofline"]["note"] = "bytes counted at FP64 values; FP32-phase launches move half of it"
     if not args.no_cpu:
         log("[bench] CPU baseline (bounded sample)")
         prm_kw = dict(vap=1, dmax=0.02, kd_tree_radius=0.042 ** 2, tolerance=1e-6)

@@ -16,10 +16,12 @@
 // (Q y = (1+beta) Q lambda_k - beta Q lambda_{k-1}), projects, and reduces the
 // scalars of the line search, so each trial costs one Q product instead of the
 // reference's three (gradient + two objective evaluations, apgd.hpp:168-181).
-// Block sums are plain stores reduced by the next kernel (block 0 of k_bt, and
-// the one-block k_ctrl that also runs the accept / backtrack / restart logic of
-// apgd.hpp:172-209): kernel boundaries order them, so no block ever issues a
-// device-scope fence.
+// Block sums are plain stores reduced by the next kernel: block 0 of k_bt sums the
+// trial kernel's, and every block of the next round's k_trial sums k_row's and
+// runs the accept / backtrack / restart logic of apgd.hpp:172-209 on its own
+// copy of the state (identical in every block), block 0 publishing it.  A round is
+// three launches, kernel boundaries order every exchange, and no block ever
+// issues a device-scope fence.
 #include "apgd.hpp"
 
 #include <algorithm>
@@ -260,15 +262,22 @@ __device__ inline T* slot(T* base, int s, int64_t stride) {
 // Trial point: y, Qy and grad f(y) from (lambda_k, lambda_{k-1}) and their Q
 // products, lambda+ = Proj(y - alpha grad), and the block sums of
 // {<g, l+ - y>, |l+ - y|^2, <g, l+ - l_k>, |l+ - l_k|^2, <y, Qy>, <c, y>, <c, l+>, |l+|^2}.
+__device__ inline void control_step(const ApgdCtrl* __restrict__ in, ApgdCtrl* __restrict__ out,
+                                    const double* __restrict__ partial_row, int nrow, ApgdCtrl& st,
+                                    double (*lds)[kBlock / 64]);
+
 template <class T, int LAY>
-__global__ void __launch_bounds__(kBlock) k_trial(const ApgdCtrl* __restrict__ ctrl, BMatrix<T> B, int64_t nc,
-                                                 double* __restrict__ partial) {
+__global__ void __launch_bounds__(kBlock) k_trial(const ApgdCtrl* __restrict__ in, ApgdCtrl* __restrict__ out,
+                                                 BMatrix<T> B, int64_t nc, double* __restrict__ partial,
+                                                 const double* __restrict__ partial_row, int nrow) {
     constexpr int NR = lay<LAY>::NR, NC = lay<LAY>::NC;
     __shared__ double lds[8][kBlock / 64];
-    if (ctrl->done) return;
-    const int cur = ctrl->cur, prev = ctrl->prev, trial = ctrl->trial;
+    __shared__ ApgdCtrl st;
+    control_step(in, out, partial_row, nrow, st, lds);
+    if (st.done) return;
+    const int cur = st.cur, prev = st.prev, trial = st.trial;
     const int64_t stride = NR * B.cap;
-    const T beta = (T)ctrl->beta, alpha = (T)ctrl->alpha;
+    const T beta = (T)st.beta, alpha = (T)st.alpha;
     const T* lc = slot<T>(B.lam, cur, stride);
     const T* lp = slot<T>(B.lam, prev, stride);
     const T* qc = slot<T>(B.qlam, cur, stride);
@@ -441,7 +450,7 @@ __device__ inline void row_product(const BMatrix<T>& B, int64_t c, int64_t i, in
 
 // q = dt^2 B w for the trial slot and the block sums of <lambda+, Q lambda+>.
 template <class T, int LAY>
-__global__ void __launch_bounds__(kBlock) k_row(const ApgdCtrl* __restrict__ ctrl, BMatrix<T> B, int64_t nc,
+__global__ void __launch_bounds__(kBlock) k_row(ApgdCtrl* __restrict__ ctrl, BMatrix<T> B, int64_t nc,
                                                const int32_t* __restrict__ ci, const int32_t* __restrict__ cj,
                                                int64_t ni, double* __restrict__ partial) {
     constexpr int NR = lay<LAY>::NR;
@@ -452,6 +461,7 @@ __global__ void __launch_bounds__(kBlock) k_row(const ApgdCtrl* __restrict__ ctr
     const int64_t stride = NR * B.cap;
     const T* lt = slot<T>(B.lam, trial, stride);
     T* qt = slot<T>(B.qlam, trial, stride);
+    if (blockIdx.x == 0 && threadIdx.x == 0) ctrl->pending = 1;
     double s[1] = {0.0};
     for (int64_t c = (int64_t)blockIdx.x * kBlock + threadIdx.x; c < nc; c += (int64_t)gridDim.x * kBlock) {
         const int64_t i = (int64_t)ci[c] - ni, j = (int64_t)cj[c] - ni;
@@ -566,16 +576,33 @@ __device__ inline void apgd_control(ApgdCtrl* ctrl) {
 }
 
 // One block: <lambda+, Q lambda+> from the row kernel's block sums, then the control step.
-__global__ void __launch_bounds__(kBlock) k_ctrl(ApgdCtrl* __restrict__ ctrl, const double* __restrict__ partial_row,
-                                                int nrow) {
-    __shared__ double lds[1][kBlock / 64];
-    if (ctrl->done) return;
-    double tot[1];
-    reduce_partials<1>(partial_row, nrow, lds, tot);
-    if (threadIdx.x == 0) {
-        ctrl->red[7] = tot[0];
-        apgd_control(ctrl);
+// Publishes the pending control step of the previous product to the block's copy
+// `st` (every block computes the same state: same inputs, same reduction order)
+// and block 0 writes it to `out`.
+__device__ inline void control_step(const ApgdCtrl* __restrict__ in, ApgdCtrl* __restrict__ out,
+                                    const double* __restrict__ partial_row, int nrow, ApgdCtrl& st,
+                                    double (*lds)[kBlock / 64]) {
+    if (threadIdx.x == 0) st = *in;
+    __syncthreads();
+    if (st.pending && !st.done) {
+        double tot[1];
+        reduce_partials<1>(partial_row, nrow, lds, tot);
+        if (threadIdx.x == 0) {
+            st.red[7] = tot[0];
+            apgd_control(&st);
+        }
     }
+    if (threadIdx.x == 0) st.pending = 0;
+    __syncthreads();
+    if (blockIdx.x == 0 && threadIdx.x == 0) *out = st;
+}
+
+// The control step alone (end of a batch of rounds, before the host reads the state).
+__global__ void __launch_bounds__(kBlock) k_ctrl(const ApgdCtrl* __restrict__ in, ApgdCtrl* __restrict__ out,
+                                                const double* __restrict__ partial_row, int nrow) {
+    __shared__ double lds[1][kBlock / 64];
+    __shared__ ApgdCtrl st;
+    control_step(in, out, partial_row, nrow, st, lds);
 }
 
 template <class T>
@@ -589,11 +616,14 @@ __global__ void __launch_bounds__(kBlock) k_convert(const TA* __restrict__ a, TB
         b[t] = (TB)a[t];
 }
 
-__global__ void k_ctrl_init(ApgdCtrl* ctrl, double alpha, double L, double tol, double rtol, double dt2,
-                            int64_t max_ite, int method, int dynamic, int keep_L, int low_precision) {
+// Initial state in `ctrl`; with keep_L, L starts from the last accepted L of the
+// previous phase's final state `last`.
+__global__ void k_ctrl_init(ApgdCtrl* ctrl, const ApgdCtrl* last, double alpha, double L, double tol, double rtol,
+                            double dt2, int64_t max_ite, int method, int dynamic, int keep_L, int low_precision) {
     if (threadIdx.x != 0 || blockIdx.x != 0) return;
-    if (keep_L && isfinite(ctrl->L_acc) && ctrl->L_acc > 0) {
-        L = ctrl->L_acc;
+    const double L_last = last->L_acc;
+    if (keep_L && isfinite(L_last) && L_last > 0) {
+        L = L_last;
         alpha = 1. / L;
     }
     ctrl->L_acc = L;
@@ -619,6 +649,7 @@ __global__ void k_ctrl_init(ApgdCtrl* ctrl, double alpha, double L, double tol, 
     ctrl->status = (max_ite <= 0) ? SCOPI_HIP_NOT_CONVERGED : 0;
     ctrl->method = method;
     ctrl->dynamic = dynamic;
+    ctrl->pending = 0;
 }
 
 // u = v + dt M^-1 B^T lambda, omega likewise (OptimGradient.hpp:267-284); with no
@@ -676,7 +707,8 @@ Solver::~Solver() {
 
 void Solver::reserve(int64_t na, int64_t nc, bool fp32) {
     if (!ctrl_) {
-        SCOPI_HIP_CHECK(hipMalloc(&ctrl_, sizeof(ApgdCtrl)));
+        SCOPI_HIP_CHECK(hipMalloc(&ctrl_, 2 * sizeof(ApgdCtrl)));
+        SCOPI_HIP_CHECK(hipMemset(ctrl_, 0, 2 * sizeof(ApgdCtrl)));
         SCOPI_HIP_CHECK(hipHostMalloc(&h_ctrl_, sizeof(ApgdCtrl)));
         SCOPI_HIP_CHECK(hipMalloc(&partial_trial_, kRedBlocksMax * kPartialStride * sizeof(double)));
         SCOPI_HIP_CHECK(hipMalloc(&partial_row_, kRedBlocksMax * kPartialStride * sizeof(double)));
@@ -744,8 +776,9 @@ void Solver::run_phase(BMatrix<T>& B, const Particles& P, const Contacts& C, int
         SCOPI_HIP_CHECK(hipMemcpyAsync(B.qlam + stride, B.qlam, stride * sizeof(T), hipMemcpyDeviceToDevice, s));
         info.q_applies += 1;
     }
-    hipLaunchKernelGGL(k_ctrl_init, dim3(1), dim3(64), 0, s, ctrl_, prm.alpha, 1. / prm.alpha, tol, rtol, dt2, max_ite,
-                       prm.method, prm.dynamic_descent ? 1 : 0, keep_L ? 1 : 0, sizeof(T) == 4 ? 1 : 0);
+    hipLaunchKernelGGL(k_ctrl_init, dim3(1), dim3(64), 0, s, ctrl_, (const ApgdCtrl*)(ctrl_ + ctrl_cur_), prm.alpha,
+                       1. / prm.alpha, tol, rtol, dt2, max_ite, prm.method, prm.dynamic_descent ? 1 : 0,
+                       keep_L ? 1 : 0, sizeof(T) == 4 ? 1 : 0);
     SCOPI_HIP_CHECK(hipGetLastError());
     std::vector<hipEvent_t> ev;
     if (timing_) {
@@ -754,22 +787,26 @@ void Solver::run_phase(BMatrix<T>& B, const Particles& P, const Contacts& C, int
     }
     int64_t before = 0;
     int batch = 2;
+    int cur = 0;  // copy of ApgdCtrl holding the newest state
     for (;;) {
         for (int r = 0; r < batch; ++r) {
             hipEvent_t* e = timing_ ? &ev[4 * r] : nullptr;
             if (e) SCOPI_HIP_CHECK(hipEventRecord(e[0], s));
-            hipLaunchKernelGGL((k_trial<T, LAY>), dim3(gred), dim3(kBlock), 0, s, (const ApgdCtrl*)ctrl_, B, nc,
-                               partial_trial_);
+            hipLaunchKernelGGL((k_trial<T, LAY>), dim3(gred), dim3(kBlock), 0, s, (const ApgdCtrl*)(ctrl_ + cur),
+                               ctrl_ + (1 - cur), B, nc, partial_trial_, (const double*)partial_row_, (int)gred);
+            cur = 1 - cur;
             if (e) SCOPI_HIP_CHECK(hipEventRecord(e[1], s));
-            launch_bt<T, LAY>(bt_group_, s, ctrl_, -1, B, na, tstart_, tc_, partial_trial_, (int)gred);
+            launch_bt<T, LAY>(bt_group_, s, ctrl_ + cur, -1, B, na, tstart_, tc_, partial_trial_, (int)gred);
             if (e) SCOPI_HIP_CHECK(hipEventRecord(e[2], s));
-            hipLaunchKernelGGL((k_row<T, LAY>), dim3(gred), dim3(kBlock), 0, s, (const ApgdCtrl*)ctrl_, B, nc,
+            hipLaunchKernelGGL((k_row<T, LAY>), dim3(gred), dim3(kBlock), 0, s, ctrl_ + cur, B, nc,
                                (const int32_t*)C.i, (const int32_t*)C.j, ni, partial_row_);
             if (e) SCOPI_HIP_CHECK(hipEventRecord(e[3], s));
-            hipLaunchKernelGGL(k_ctrl, dim3(1), dim3(kBlock), 0, s, ctrl_, (const double*)partial_row_, (int)gred);
         }
+        hipLaunchKernelGGL(k_ctrl, dim3(1), dim3(kBlock), 0, s, (const ApgdCtrl*)(ctrl_ + cur), ctrl_ + (1 - cur),
+                           (const double*)partial_row_, (int)gred);
+        cur = 1 - cur;
         SCOPI_HIP_CHECK(hipGetLastError());
-        SCOPI_HIP_CHECK(hipMemcpyAsync(h_ctrl_, ctrl_, sizeof(ApgdCtrl), hipMemcpyDeviceToHost, s));
+        SCOPI_HIP_CHECK(hipMemcpyAsync(h_ctrl_, ctrl_ + cur, sizeof(ApgdCtrl), hipMemcpyDeviceToHost, s));
         SCOPI_HIP_CHECK(hipStreamSynchronize(s));
         const int64_t rounds = h_ctrl_->q_applies;  // one per executed round
         if (timing_) {
@@ -792,6 +829,7 @@ void Solver::run_phase(BMatrix<T>& B, const Particles& P, const Contacts& C, int
         if (h_ctrl_->done) break;
         batch = std::min(batch * 2, 64);
     }
+    ctrl_cur_ = cur;
     for (auto& e : ev) (void)hipEventDestroy(e);
     info.iterations += h_ctrl_->iter;
     info.backtracks += h_ctrl_->backtracks;

@@ -31,9 +31,11 @@ struct lay<kCone> {
     static constexpr int NR = 3, NC = 21, NT = 9, NW = 6;
 };
 
-// Scalar state of the projected-gradient loop, owned by the device.  A one-block
-// control kernel runs the reference's logic (include/scopi/solvers/apgd.hpp:164-213)
-// on it after every product, so the host only polls `done`.
+// Scalar state of the projected-gradient loop, owned by the device.  Every block of
+// the next round's trial kernel runs the reference's control logic
+// (include/scopi/solvers/apgd.hpp:164-213) on its own copy after each product, so
+// the host only polls `done`.  Two copies alternate: a round reads one and its
+// trial kernel's block 0 publishes the updated state in the other.
 struct ApgdCtrl {
     double L, alpha, theta, beta;
     double L_acc;        // L of the last accepted trial (before the 0.97 decay)
@@ -43,6 +45,7 @@ struct ApgdCtrl {
     int64_t iter, backtracks, max_ite, q_applies, consec_bt;
     int32_t cur, prev, trial;
     int32_t done, status, method, dynamic, low_precision;
+    int32_t pending;     // a product waits for its control step (set by the row kernel)
 };
 
 // Device arrays of one precision.  Per-contact and per-particle data are
@@ -117,13 +120,14 @@ class Solver {
     int32_t* jlist_ = nullptr;
     double* u_ = nullptr;  // 3 x pcap
     double* w_ = nullptr;  // 3 x pcap
-    ApgdCtrl* ctrl_ = nullptr;
+    ApgdCtrl* ctrl_ = nullptr;    // 2 alternating copies
     ApgdCtrl* h_ctrl_ = nullptr;  // pinned
     double* partial_trial_ = nullptr;
     double* partial_row_ = nullptr;
     int64_t rows_ = 0, nc_ = 0;
     int nr_ = 1;
     int final_slot_ = 0;
+    int ctrl_cur_ = 0;  // copy of ApgdCtrl with the last phase's final state
     int layout_ = kRotFree;
     int reduce_blocks_ = 1024;  // grid of the per-contact kernels (grid-stride)
     int bt_group_ = 8;          // lanes per particle in k_bt

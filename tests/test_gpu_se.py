@@ -148,51 +148,66 @@ def test_se_friction_solve(se_oracle, precision):
                  tolerance=1e-10, max_ite=200000)
     r = o.solve(dt)
     np.testing.assert_allclose(u, np.concatenate([r["u"].ravel(), r["w"].ravel()]), atol=1e-8)
-    # reference defaults (dynamic descent), tol 1e-9
+    # reference defaults (dynamic descent), tol 1e-9.  Where the reference's stopping test
+    # |l+ - l| < tol fires depends on the line-search path; its distance to the optimum over
+    # initial steps alpha in [0.03, 0.07] calibrates the bar.
     prm = hip.params(problem=hip.FRICTION, mu=0.3, dmax=0.02, kd_tree_radius=KD3, vap=hip.VAP_FPD, tolerance=1e-9,
                      max_ite=200000, precision=precision)
     u, info, _ = se_first_step(sc, prm, dt)
-    o.set_params(problem=1, mu=0.3, dmax=0.02, kd_tree_radius=KD3, vap=1, tolerance=1e-9, max_ite=200000)
-    r = o.solve(dt)
-    e_orc = float(np.abs(np.concatenate([r["u"].ravel(), r["w"].ravel()]) - u_exact).max())
+    e_orc = []
+    for a in (0.03, 0.04, 0.045, 0.05, 0.055, 0.06, 0.07):
+        o.set_params(problem=1, mu=0.3, dmax=0.02, kd_tree_radius=KD3, vap=1, tolerance=1e-9, max_ite=200000, alpha=a)
+        r = o.solve(dt)
+        e_orc.append(float(np.abs(np.concatenate([r["u"].ravel(), r["w"].ravel()]) - u_exact).max()))
     e_gpu = float(np.abs(u - u_exact).max())
-    say(f"  adaptive e_gpu={e_gpu:.3e} e_oracle={e_orc:.3e} iterations={info['iterations']}")
-    assert info["converged"] == 1 and e_gpu <= max(3 * e_orc, 1e-6)
+    say(f"  adaptive e_gpu={e_gpu:.3e} e_oracle(alpha 0.05)={e_orc[3]:.3e} max over alpha={max(e_orc):.3e} "
+        f"iterations={info['iterations']}")
+    assert info["converged"] == 1 and e_gpu <= max(3 * max(e_orc), 1e-6)
     assert na == 64
 
 
 def test_se_steps_against_oracle(se_oracle):
-    """Four full steps (Friction mu 0.3, fixed step so both sides iterate the same map).
+    """One and four full steps (Friction mu 0.3, fixed step so both sides iterate the same map).
 
-    After the first step the two states differ by the solver's rounding, and the
-    superellipsoid solve amplifies differences of one ulp into ~1e-6 in the
-    distance (it stops on "no progress" wherever the roundings leave it).  The bar is
-    therefore calibrated on the oracle itself: the GPU trajectory must be as close to
-    the oracle's as the oracle's is to its own run from a state perturbed by one ulp
-    (x20), or within 1e-9."""
+    One step starts from the same state, so the contacts are bit-identical and the positions
+    agree to 1e-10 (dt x 1e-7 in velocity).  Later steps start from states that differ by the
+    solver's rounding, and the superellipsoid solve can turn an ulp into a different end point
+    (rarely a spurious root and a large impulse, as in the reference).  Per particle, the bar is
+    calibrated on the oracle itself: |gpu - oracle| <= 20 x the oracle's own change under a
+    one-ulp perturbation of the initial positions, + 1e-9."""
     dt = 1e-3
-    steps = 4
     sc = scenes.superellipsoids_3d(3, 3, 4, seed=13, velocity_scale=3.0)
     _, _, L = exact_optimum(se_oracle, sc, dt)
     kw = dict(problem=1, mu=0.3, dmax=0.02, kd_tree_radius=KD3, vap=1, dynamic_descent=0, alpha=0.5 / L,
               tolerance=1e-11, max_ite=400000)
+    hprm = hip.params(problem=hip.FRICTION, mu=0.3, dmax=0.02, kd_tree_radius=KD3, vap=hip.VAP_FPD,
+                      dynamic_descent=0, alpha=0.5 / L, tolerance=1e-11, max_ite=400000)
 
-    def oracle_run(scene):
+    def oracle_run(scene, steps):
         o = se_oracle.Oracle(scene, **kw)
         o.run(dt, steps)
         return o.state()
-    ref = oracle_run(sc)
-    spread = 0.0
+
+    def gpu_run(steps):
+        g = hip.HipScene(sc)
+        info = g.run(hprm, dt, steps)
+        st = g.state()
+        g.close()
+        return st, info
+    sg, _ = gpu_run(1)
+    d1 = float(np.abs(sg["pos"] - oracle_run(sc, 1)["pos"]).max())
+    say(f"[se-steps] 1 step max|dpos|={d1:.3e}")
+    assert d1 < 1e-10  # dt x 1e-7 in velocity
+    steps = 4
+    ref = oracle_run(sc, steps)["pos"]
+    spread = np.zeros(sc.n)
     for sign in (1.0, -1.0):
         pert = scenes.superellipsoids_3d(3, 3, 4, seed=13, velocity_scale=3.0)
         pert.pos[:] = np.nextafter(pert.pos, pert.pos + sign)
-        spread = max(spread, float(np.abs(oracle_run(pert)["pos"] - ref["pos"]).max()))
-    g = hip.HipScene(sc)
-    info = g.run(hip.params(problem=hip.FRICTION, mu=0.3, dmax=0.02, kd_tree_radius=KD3, vap=hip.VAP_FPD,
-                            dynamic_descent=0, alpha=0.5 / L, tolerance=1e-11, max_ite=400000), dt, steps)
-    sg = g.state()
-    g.close()
-    dpos = float(np.abs(sg["pos"] - ref["pos"]).max())
-    say(f"[se-steps] max|dpos|={dpos:.3e} oracle 1-ulp spread={spread:.3e} contacts(last)={info['n_contacts']}")
+        spread = np.maximum(spread, np.abs(oracle_run(pert, steps)["pos"] - ref).max(axis=1))
+    sg, info = gpu_run(steps)
+    dpos = np.abs(sg["pos"] - ref).max(axis=1)
+    say(f"[se-steps] {steps} steps max|dpos|={dpos.max():.3e} oracle 1-ulp spread max={spread.max():.3e}, "
+        f"{int(np.count_nonzero(spread > 1e-9))}/{sc.n} particles with spread > 1e-9, contacts(last)={info['n_contacts']}")
     assert info["n_contacts"] > 0
-    assert dpos <= max(20.0 * spread, 1e-9)
+    assert np.all(dpos <= 20.0 * spread + 1e-9)
