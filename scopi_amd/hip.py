@@ -215,6 +215,9 @@ class HipScene:
         info = SolveInfo()
         self._check(self._L.scopi_hip_run(self._h, C.byref(prm), float(dt), int(total_it), int(initial_iter),
                                           C.byref(info)), "run")
+        if total_it > initial_iter:
+            self.nc = info.n_contacts  # contacts of the last step stay on the device
+            self.last_rows = info.rows
         return info.as_dict()
 
     def set_kernel_timing(self, on=True):
