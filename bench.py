@@ -114,6 +114,46 @@ def copy_calibration(mb):
     del x, y
 
 
+def bench_superellipsoids(args):
+    """32,768 superellipsoids (32^3 jittered lattice, spacing 0.035, semi-axes U[0.008, 0.015],
+    squareness U[0.4, 1.6], random orientations, seeded velocities U[-0.5, 0.5]), Friction mu 0.3,
+    APGD tol 1e-6, dt 1e-3, dmax 0.02, kd_tree_radius (2 sqrt(3) 0.015 + 0.02)^2."""
+    import torch
+    from scopi_amd import hip, scenes
+    cfg = scenes.CONFIG3
+    log("[bench] building the 32k-superellipsoid scene")
+    sc = scenes.superellipsoids_3d()
+    precision = hip.FP64 if args.precision == "fp64" else hip.FP32_REFINE
+    prm = hip.params(problem=hip.FRICTION, mu=cfg["mu"], vap=hip.VAP_FPD, dmax=cfg["dmax"],
+                     kd_tree_radius=cfg["kd_tree_radius"], tolerance=cfg["tolerance"], precision=precision)
+    dt = cfg["dt"]
+    torch.cuda.init()
+    g = hip.HipScene(sc, capacity_contacts=16 * sc.n)
+    log(f"[bench] warmup {args.warmup} steps")
+    if args.warmup:
+        g.run(prm, dt, args.warmup)
+    torch.cuda.synchronize()
+    log(f"[bench] timing {args.steps} steps")
+    t0 = time.perf_counter()
+    info = g.run(prm, dt, args.steps)
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t0
+    result = {
+        "metric": "time-steps/s at 32,768 superellipsoids, Friction mu 0.3, APGD tol 1e-6",
+        "value": args.steps / elapsed, "unit": "time-steps/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": 1e3 * elapsed / args.steps, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f64" if precision == hip.FP64 else "f32+f64",
+        "data": "synthetic",
+        "config": {"workload": "32,768 superellipsoids (32^3 jittered lattice, spacing 0.035), Friction mu 0.3, "
+                               "APGD tol 1e-6, dt 1e-3, dmax 0.02", "precision": args.precision},
+        "n_contacts_last_step": info["n_contacts"],
+        "apgd_iterations_per_step": info["iterations"] / args.steps,
+        "line_search_rejections_per_step": info["backtracks"] / args.steps,
+    }
+    print(json.dumps(result), flush=True)
+    g.close()
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -129,6 +169,9 @@ def main():
                     help="steps of the instrumented pass (HIP events around every solver kernel)")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="run one plain device copy of MIB MiB first (traffic calibration for counter runs)")
+    ap.add_argument("--workload", choices=["spheres", "superellipsoids"], default="spheres",
+                    help="spheres: the headline 131k-sphere step; superellipsoids: 32,768 superellipsoids, "
+                         "Friction mu 0.3 (a secondary line)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py runs on one GPU only: the contact step of one scene is not sharded across GPUs "
@@ -139,6 +182,8 @@ def main():
     import torch
     from scopi_amd import hip, scenes
 
+    if args.workload == "superellipsoids":
+        return bench_superellipsoids(args)
     log("[bench] building the 131k-sphere scene")
     sc = scenes.spheres_3d(args.nx, args.ny, args.nz)
     precision = hip.FP64 if args.precision == "fp64" else hip.FP32_REFINE

@@ -24,8 +24,11 @@
 // issues a device-scope fence.
 #include "apgd.hpp"
 
+#include <hip/hip_ext.h>
+
 #include <algorithm>
 #include <cmath>
+#include <type_traits>
 #include <vector>
 
 namespace scopi_hip {
@@ -284,54 +287,91 @@ __global__ void __launch_bounds__(kBlock) k_trial(const ApgdCtrl* __restrict__ i
     const T* qp = slot<T>(B.qlam, prev, stride);
     T* lt = slot<T>(B.lam, trial, stride);
     double s[8] = {0, 0, 0, 0, 0, 0, 0, 0};
-    for (int64_t c = (int64_t)blockIdx.x * kBlock + threadIdx.x; c < nc; c += (int64_t)gridDim.x * kBlock) {
-        T y[NR], g[NR], z[NR], l0[NR], cc[NR], qy[NR];
+    // two consecutive contacts per lane and iteration: their 2 NR rows are loaded as
+    // NR aligned pairs (slots are even-sized, c0 is even)
+    using V2 = typename std::conditional<sizeof(T) == 8, double2, float2>::type;
+    for (int64_t c0 = 2 * ((int64_t)blockIdx.x * kBlock + threadIdx.x); c0 < nc;
+         c0 += 2 * (int64_t)gridDim.x * kBlock) {
+        T l0[2 * NR], l1[2 * NR], q0[2 * NR], q1[2 * NR], cc[2 * NR];
+        if (c0 + 1 < nc) {
 #pragma unroll
-        for (int d = 0; d < NR; ++d) {
-            const int64_t r = c * NR + d;
-            l0[d] = lc[r];
-            const T l1 = lp[r];
-            const T q0 = qc[r], q1 = qp[r];
-            cc[d] = B.c[r];
-            y[d] = l0[d] + beta * (l0[d] - l1);
-            qy[d] = q0 + beta * (q0 - q1);
-            g[d] = qy[d] + cc[d];
-            z[d] = y[d] - alpha * g[d];
-        }
-        if (LAY != kCone) {
-            z[0] = z[0] > (T)0 ? z[0] : (T)0;  // max(lambda, 0): lagrange_multiplier.hpp:104-107
+            for (int h = 0; h < NR; ++h) {
+                const int64_t r = c0 * NR + 2 * h;
+                const V2 a = *reinterpret_cast<const V2*>(lc + r), b = *reinterpret_cast<const V2*>(lp + r);
+                const V2 e = *reinterpret_cast<const V2*>(qc + r), f = *reinterpret_cast<const V2*>(qp + r);
+                const V2 g = *reinterpret_cast<const V2*>(B.c + r);
+                l0[2 * h] = a.x, l0[2 * h + 1] = a.y, l1[2 * h] = b.x, l1[2 * h + 1] = b.y;
+                q0[2 * h] = e.x, q0[2 * h + 1] = e.y, q1[2 * h] = f.x, q1[2 * h + 1] = f.y;
+                cc[2 * h] = g.x, cc[2 * h + 1] = g.y;
+            }
         } else {
-            // Coulomb-cone projection: lagrange_multiplier.hpp:243-268
-            const T nx = B.coef[c * NC + 0], ny = B.coef[c * NC + 1], nz = B.coef[c * NC + 2];
-            const T mu = B.mu[c];
-            const T ln = z[0] * nx + z[1] * ny + z[2] * nz;
-            T tx = z[0] - ln * nx, ty = z[1] - ln * ny, tz = z[2] - ln * nz;
-            const T norm = sqrt(tx * tx + ty * ty + tz * tz);
-            if (norm > mu * ln) {
-                if (ln <= -mu * norm) {
-                    z[0] = z[1] = z[2] = (T)0;
-                } else {
-                    const T nn = (mu * mu) * (norm + ln / mu) / (mu * mu + (T)1);
-                    const T nl = nn / mu;
-                    tx /= norm, ty /= norm, tz /= norm;
-                    z[0] = nn * tx + nl * nx;
-                    z[1] = nn * ty + nl * ny;
-                    z[2] = nn * tz + nl * nz;
-                }
+#pragma unroll
+            for (int d = 0; d < NR; ++d) {
+                const int64_t r = c0 * NR + d;
+                l0[d] = lc[r], l1[d] = lp[r], q0[d] = qc[r], q1[d] = qp[r], cc[d] = B.c[r];
             }
         }
+        const int m = (c0 + 1 < nc) ? 2 : 1;
+        T zz[2 * NR];
+        for (int k = 0; k < m; ++k) {
+            const int64_t c = c0 + k;
+            T y[NR], g[NR], z[NR], qy[NR];
 #pragma unroll
-        for (int d = 0; d < NR; ++d) {
-            lt[c * NR + d] = z[d];
-            const double dy = (double)z[d] - (double)y[d], dl = (double)z[d] - (double)l0[d];
-            s[0] += (double)g[d] * dy;
-            s[1] += dy * dy;
-            s[2] += (double)g[d] * dl;
-            s[3] += dl * dl;
-            s[4] += (double)y[d] * (double)qy[d];
-            s[5] += (double)cc[d] * (double)y[d];
-            s[6] += (double)cc[d] * (double)z[d];
-            s[7] += (double)z[d] * (double)z[d];
+            for (int d = 0; d < NR; ++d) {
+                const int o = k * NR + d;
+                y[d] = l0[o] + beta * (l0[o] - l1[o]);
+                qy[d] = q0[o] + beta * (q0[o] - q1[o]);
+                g[d] = qy[d] + cc[o];
+                z[d] = y[d] - alpha * g[d];
+            }
+            if (LAY != kCone) {
+                z[0] = z[0] > (T)0 ? z[0] : (T)0;  // max(lambda, 0): lagrange_multiplier.hpp:104-107
+            } else {
+                // Coulomb-cone projection: lagrange_multiplier.hpp:243-268
+                const T nx = B.coef[c * NC + 0], ny = B.coef[c * NC + 1], nz = B.coef[c * NC + 2];
+                const T mu = B.mu[c];
+                const T ln = z[0] * nx + z[1] * ny + z[2] * nz;
+                T tx = z[0] - ln * nx, ty = z[1] - ln * ny, tz = z[2] - ln * nz;
+                const T norm = sqrt(tx * tx + ty * ty + tz * tz);
+                if (norm > mu * ln) {
+                    if (ln <= -mu * norm) {
+                        z[0] = z[1] = z[2] = (T)0;
+                    } else {
+                        const T nn = (mu * mu) * (norm + ln / mu) / (mu * mu + (T)1);
+                        const T nl = nn / mu;
+                        tx /= norm, ty /= norm, tz /= norm;
+                        z[0] = nn * tx + nl * nx;
+                        z[1] = nn * ty + nl * ny;
+                        z[2] = nn * tz + nl * nz;
+                    }
+                }
+            }
+#pragma unroll
+            for (int d = 0; d < NR; ++d) {
+                const int o = k * NR + d;
+                zz[o] = z[d];
+                const double dy = (double)z[d] - (double)y[d], dl = (double)z[d] - (double)l0[o];
+                s[0] += (double)g[d] * dy;
+                s[1] += dy * dy;
+                s[2] += (double)g[d] * dl;
+                s[3] += dl * dl;
+                s[4] += (double)y[d] * (double)qy[d];
+                s[5] += (double)cc[o] * (double)y[d];
+                s[6] += (double)cc[o] * (double)z[d];
+                s[7] += (double)z[d] * (double)z[d];
+            }
+        }
+        if (m == 2) {
+#pragma unroll
+            for (int h = 0; h < NR; ++h) {
+                V2 o;
+                o.x = zz[2 * h];
+                o.y = zz[2 * h + 1];
+                *reinterpret_cast<V2*>(lt + c0 * NR + 2 * h) = o;
+            }
+        } else {
+#pragma unroll
+            for (int d = 0; d < NR; ++d) lt[c0 * NR + d] = zz[d];
         }
     }
     block_sum<8>(s, lds);
@@ -400,24 +440,23 @@ __global__ void __launch_bounds__(kBlock) k_bt(ApgdCtrl* __restrict__ ctrl, int 
 
 template <class T, int LAY>
 static void launch_bt(int G, hipStream_t s, ApgdCtrl* ctrl, int use_slot, const BMatrix<T>& B, int64_t na,
-                      const int* tstart, const int32_t* tc, const double* partial, int ntrial) {
+                      const int* tstart, const int32_t* tc, const double* partial, int ntrial,
+                      hipEvent_t e0 = nullptr, hipEvent_t e1 = nullptr) {
     const unsigned grid = grid_for(na * (int64_t)G);
+    auto go = [&](auto kernel) {
+        if (e0)  // start / stop events taken from the kernel's own dispatch
+            hipExtLaunchKernelGGL(kernel, dim3(grid), dim3(kBlock), 0, s, e0, e1, 0, ctrl, use_slot, B, na, tstart, tc,
+                                  partial, ntrial);
+        else
+            hipLaunchKernelGGL(kernel, dim3(grid), dim3(kBlock), 0, s, ctrl, use_slot, B, na, tstart, tc, partial,
+                               ntrial);
+    };
     switch (G) {
-        case 1:
-            hipLaunchKernelGGL((k_bt<T, LAY, 1>), dim3(grid), dim3(kBlock), 0, s, ctrl, use_slot, B, na, tstart, tc, partial, ntrial);
-            break;
-        case 2:
-            hipLaunchKernelGGL((k_bt<T, LAY, 2>), dim3(grid), dim3(kBlock), 0, s, ctrl, use_slot, B, na, tstart, tc, partial, ntrial);
-            break;
-        case 4:
-            hipLaunchKernelGGL((k_bt<T, LAY, 4>), dim3(grid), dim3(kBlock), 0, s, ctrl, use_slot, B, na, tstart, tc, partial, ntrial);
-            break;
-        case 16:
-            hipLaunchKernelGGL((k_bt<T, LAY, 16>), dim3(grid), dim3(kBlock), 0, s, ctrl, use_slot, B, na, tstart, tc, partial, ntrial);
-            break;
-        default:
-            hipLaunchKernelGGL((k_bt<T, LAY, 8>), dim3(grid), dim3(kBlock), 0, s, ctrl, use_slot, B, na, tstart, tc, partial, ntrial);
-            break;
+        case 1: go(k_bt<T, LAY, 1>); break;
+        case 2: go(k_bt<T, LAY, 2>); break;
+        case 4: go(k_bt<T, LAY, 4>); break;
+        case 16: go(k_bt<T, LAY, 16>); break;
+        default: go(k_bt<T, LAY, 8>); break;
     }
 }
 
@@ -727,7 +766,7 @@ void Solver::reserve(int64_t na, int64_t nc, bool fp32) {
     if (grow_c) {
         for (void* p : {(void*)jkeys_, (void*)jlist_, (void*)tc_})
             if (p) SCOPI_HIP_CHECK(hipFree(p));
-        ccap_ = nc + nc / 4 + 1024;
+        ccap_ = (nc + nc / 4 + 1024 + 63) / 64 * 64;  // even: slots stay aligned for paired loads
         SCOPI_HIP_CHECK(hipMalloc(&jkeys_, ccap_ * sizeof(uint32_t)));
         SCOPI_HIP_CHECK(hipMalloc(&jlist_, ccap_ * sizeof(int32_t)));
         SCOPI_HIP_CHECK(hipMalloc(&tc_, 2 * ccap_ * sizeof(int32_t)));
@@ -782,7 +821,7 @@ void Solver::run_phase(BMatrix<T>& B, const Particles& P, const Contacts& C, int
     SCOPI_HIP_CHECK(hipGetLastError());
     std::vector<hipEvent_t> ev;
     if (timing_) {
-        ev.resize(4 * 64);
+        ev.resize(6 * 64);
         for (auto& e : ev) SCOPI_HIP_CHECK(hipEventCreate(&e));
     }
     int64_t before = 0;
@@ -790,17 +829,23 @@ void Solver::run_phase(BMatrix<T>& B, const Particles& P, const Contacts& C, int
     int cur = 0;  // copy of ApgdCtrl holding the newest state
     for (;;) {
         for (int r = 0; r < batch; ++r) {
-            hipEvent_t* e = timing_ ? &ev[4 * r] : nullptr;
-            if (e) SCOPI_HIP_CHECK(hipEventRecord(e[0], s));
-            hipLaunchKernelGGL((k_trial<T, LAY>), dim3(gred), dim3(kBlock), 0, s, (const ApgdCtrl*)(ctrl_ + cur),
-                               ctrl_ + (1 - cur), B, nc, partial_trial_, (const double*)partial_row_, (int)gred);
+            hipEvent_t* e = timing_ ? &ev[6 * r] : nullptr;
+            if (e)
+                hipExtLaunchKernelGGL(k_trial<T, LAY>, dim3(gred), dim3(kBlock), 0, s, e[0], e[1], 0,
+                                      (const ApgdCtrl*)(ctrl_ + cur), ctrl_ + (1 - cur), B, nc, partial_trial_,
+                                      (const double*)partial_row_, (int)gred);
+            else
+                hipLaunchKernelGGL((k_trial<T, LAY>), dim3(gred), dim3(kBlock), 0, s, (const ApgdCtrl*)(ctrl_ + cur),
+                                   ctrl_ + (1 - cur), B, nc, partial_trial_, (const double*)partial_row_, (int)gred);
             cur = 1 - cur;
-            if (e) SCOPI_HIP_CHECK(hipEventRecord(e[1], s));
-            launch_bt<T, LAY>(bt_group_, s, ctrl_ + cur, -1, B, na, tstart_, tc_, partial_trial_, (int)gred);
-            if (e) SCOPI_HIP_CHECK(hipEventRecord(e[2], s));
-            hipLaunchKernelGGL((k_row<T, LAY>), dim3(gred), dim3(kBlock), 0, s, ctrl_ + cur, B, nc,
-                               (const int32_t*)C.i, (const int32_t*)C.j, ni, partial_row_);
-            if (e) SCOPI_HIP_CHECK(hipEventRecord(e[3], s));
+            launch_bt<T, LAY>(bt_group_, s, ctrl_ + cur, -1, B, na, tstart_, tc_, partial_trial_, (int)gred,
+                              e ? e[2] : nullptr, e ? e[3] : nullptr);
+            if (e)
+                hipExtLaunchKernelGGL(k_row<T, LAY>, dim3(gred), dim3(kBlock), 0, s, e[4], e[5], 0, ctrl_ + cur, B,
+                                      nc, (const int32_t*)C.i, (const int32_t*)C.j, ni, partial_row_);
+            else
+                hipLaunchKernelGGL((k_row<T, LAY>), dim3(gred), dim3(kBlock), 0, s, ctrl_ + cur, B, nc,
+                                   (const int32_t*)C.i, (const int32_t*)C.j, ni, partial_row_);
         }
         hipLaunchKernelGGL(k_ctrl, dim3(1), dim3(kBlock), 0, s, (const ApgdCtrl*)(ctrl_ + cur), ctrl_ + (1 - cur),
                            (const double*)partial_row_, (int)gred);
@@ -814,11 +859,11 @@ void Solver::run_phase(BMatrix<T>& B, const Particles& P, const Contacts& C, int
             const int64_t executed = std::min<int64_t>(rounds - before, batch);
             for (int64_t r = 0; r < executed; ++r) {
                 float ms;
-                SCOPI_HIP_CHECK(hipEventElapsedTime(&ms, ev[4 * r + 0], ev[4 * r + 1]));
+                SCOPI_HIP_CHECK(hipEventElapsedTime(&ms, ev[6 * r + 0], ev[6 * r + 1]));
                 t_trial_ms_ += ms;
-                SCOPI_HIP_CHECK(hipEventElapsedTime(&ms, ev[4 * r + 1], ev[4 * r + 2]));
+                SCOPI_HIP_CHECK(hipEventElapsedTime(&ms, ev[6 * r + 2], ev[6 * r + 3]));
                 t_bt_ms_ += ms;
-                SCOPI_HIP_CHECK(hipEventElapsedTime(&ms, ev[4 * r + 2], ev[4 * r + 3]));
+                SCOPI_HIP_CHECK(hipEventElapsedTime(&ms, ev[6 * r + 4], ev[6 * r + 5]));
                 t_row_ms_ += ms;
             }
             n_trial_ += executed;
