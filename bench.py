@@ -101,6 +101,21 @@ def cpu_baseline(sc, prm_kw, per_step_counts, budget_s=20.0):
                 seconds_per_step=t_step, q_product_ms=t_q * 1e3, contact_search_s=t_contacts)
 
 
+def measured_traffic(kernel, precision):
+    """HBM bytes per launch of `kernel` from the committed counter summary of this workload
+    (profiles/apgd_spheres131k_fp64_hbm_traffic.md: two rocprofv3 --pmc runs, FETCH_SIZE and
+    WRITE_SIZE, calibrated on a plain copy), or None."""
+    import re
+    path = os.path.join(ROOT, "profiles", "apgd_spheres131k_fp64_hbm_traffic.md")
+    if precision != "fp64" or not os.path.exists(path):
+        return None
+    for line in open(path):
+        m = re.match(r"\|\s*`" + re.escape(kernel) + r"<[^`]*`\s*\|(?:[^|]*\|){5}\s*([0-9.]+)\s*\|", line)
+        if m:
+            return float(m.group(1)) * 1e6
+    return None
+
+
 def copy_calibration(mb):
     """A plain device copy of `mb` MiB (read + write) so that a counter run (rocprofv3 --pmc
     FETCH_SIZE / WRITE_SIZE) sees a kernel of known traffic beside the solver kernels."""
@@ -250,7 +265,9 @@ def main():
             per[k]["achieved_gbs"] = kb[k] / (a * 1e-6) / 1e9 if a else None
         roofline = {"bound": "hbm", "kernel": {"bt": "k_bt", "row": "k_row", "trial": "k_trial"}[dom],
                     "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                    "frac": (achieved / HBM_PEAK_GBS) if achieved else None, "traffic": None,
+                    "frac": (achieved / HBM_PEAK_GBS) if achieved else None,
+                    "traffic": measured_traffic({"bt": "k_bt", "row": "k_row", "trial": "k_trial"}[dom],
+                                                args.precision),
                     "algorithmic_bytes_per_launch": kb[dom], "avg_launch_us": per[dom]["avg_us"]}
     log(f"[bench] {steps_per_s:.3f} steps/s, nc={nc}, iterations/step={it_per_step:.1f}, "
         f"backtracks/step={bt_per_step:.1f}, kernels={per}")

@@ -134,6 +134,7 @@ struct object {
     type::position_t<dim> pos;
     type::quaternion_t q;
     std::array<double, 3> size;
+    std::array<double, 2> sq{1.0, 1.0};  // superellipsoid squareness (e, n)
 };
 
 template <std::size_t dim>
@@ -149,6 +150,18 @@ struct plane : object<dim> {
     explicit plane(type::position_t<dim> pos, double angle = 0)
         : object<dim>{SCOPI_SHAPE_PLANE, pos, quaternion(angle), {0, 0, 0}} {}
     plane(type::position_t<dim> pos, type::quaternion_t q) : object<dim>{SCOPI_SHAPE_PLANE, pos, q, {0, 0, 0}} {}
+};
+
+// superellipsoid<3>(pos, q, radius (semi-axes), squareness (e, n)):
+// include/scopi/objects/types/superellipsoid.hpp:111-139 (3D only here).
+template <std::size_t dim>
+struct superellipsoid : object<dim> {
+    static_assert(dim == 3, "superellipsoids are supported in 3D");
+    superellipsoid(type::position_t<dim> pos, type::quaternion_t q, fixed<3> radius, fixed<2> squareness)
+        : object<dim>{SCOPI_SHAPE_SUPERELLIPSOID, pos, q, {radius[0], radius[1], radius[2]},
+                      {squareness[0], squareness[1]}} {}
+    fixed<3> radius() const { return fixed<3>{{this->size[0], this->size[1], this->size[2]}}; }
+    fixed<2> squareness() const { return fixed<2>{{this->sq[0], this->sq[1]}}; }
 };
 
 template <std::size_t dim>
@@ -169,6 +182,7 @@ class scopi_container {
         m_pos.push_back(s.pos);
         m_q.push_back(s.q);
         m_size.push_back(s.size);
+        m_sq.push_back(s.sq);
         m_v.push_back(p.velocity());
         m_vd.push_back(p.desired_velocity());
         m_w.push_back(p.omega());
@@ -198,13 +212,13 @@ class scopi_container {
     // structure of arrays in the C-ABI convention (3 slots, 2D rotation in slot 2)
     struct soa {
         std::vector<int32_t> shape;
-        std::vector<double> pos, q, size, v, vd, w, dw, f, m, J;
+        std::vector<double> pos, q, size, sq, v, vd, w, dw, f, m, J;
     };
     soa pack() const {
         soa s;
         const std::size_t n = size();
         s.shape.assign(m_shape.begin(), m_shape.end());
-        s.pos.assign(3 * n, 0.0), s.q.assign(4 * n, 0.0), s.size.assign(3 * n, 0.0);
+        s.pos.assign(3 * n, 0.0), s.q.assign(4 * n, 0.0), s.size.assign(3 * n, 0.0), s.sq.assign(2 * n, 1.0);
         s.v.assign(3 * n, 0.0), s.vd.assign(3 * n, 0.0), s.w.assign(3 * n, 0.0), s.dw.assign(3 * n, 0.0);
         s.f.assign(3 * n, 0.0), s.m.assign(n, 0.0), s.J.assign(3 * n, 0.0);
         for (std::size_t k = 0; k < n; ++k) {
@@ -216,6 +230,7 @@ class scopi_container {
             }
             for (int d = 0; d < 4; ++d) s.q[4 * k + d] = m_q[k][d];
             for (int d = 0; d < 3; ++d) s.size[3 * k + d] = m_size[k][d];
+            for (int d = 0; d < 2; ++d) s.sq[2 * k + d] = m_sq[k][d];
             rot_in(s.w, k, m_w[k]);
             rot_in(s.dw, k, m_dw[k]);
             rot_in(s.J, k, m_j[k]);
@@ -248,6 +263,7 @@ class scopi_container {
     std::vector<type::position_t<dim>> m_pos;
     std::vector<type::quaternion_t> m_q;
     std::vector<std::array<double, 3>> m_size;
+    std::vector<std::array<double, 2>> m_sq;
     std::vector<type::velocity_t<dim>> m_v, m_vd;
     std::vector<type::rotation_t<dim>> m_w, m_dw;
     std::vector<type::force_t<dim>> m_f;
@@ -294,7 +310,7 @@ struct device_scene {
         }
         auto s = p.pack();
         check(scopi_hip_upload_state(ctx, (int64_t)p.size(), (int64_t)p.nb_inactive(), s.shape.data(), s.pos.data(),
-                                     s.q.data(), s.size.data(), nullptr, s.v.data(), s.vd.data(), s.w.data(),
+                                     s.q.data(), s.size.data(), s.sq.data(), s.v.data(), s.vd.data(), s.w.data(),
                                      s.dw.data(), s.f.data(), s.m.data(), s.J.data()),
               ctx, "scopi_hip_upload_state");
     }
@@ -559,7 +575,7 @@ class ScopiSolver {
                           "scopi_hip_create");
         auto s = m_particles.pack();
         detail::check(scopi_hip_upload_state(m_ctx, (int64_t)m_particles.size(), (int64_t)m_particles.nb_inactive(),
-                                             s.shape.data(), s.pos.data(), s.q.data(), s.size.data(), nullptr,
+                                             s.shape.data(), s.pos.data(), s.q.data(), s.size.data(), s.sq.data(),
                                              s.v.data(), s.vd.data(), s.w.data(), s.dw.data(), s.f.data(), s.m.data(),
                                              s.J.data()),
                       m_ctx, "scopi_hip_upload_state");

@@ -4,12 +4,14 @@
 // test/references/*.json.  Scene set-ups follow test/test_sphere.cpp:341-484,
 // test/test_obstacles.cpp:216-254, test/test_gradient.cpp:156-213, 377-439 and
 // test/test_contacts_kdtree.cpp:14-72.
+#include <cmath>
 #include <cstdio>
 #include <random>
 #include <string>
 
 #include <scopi/objects/types/plane.hpp>
 #include <scopi/objects/types/sphere.hpp>
+#include <scopi/objects/types/superellipsoid.hpp>
 #include <scopi/solver.hpp>
 #include <scopi/vap/vap_fpd.hpp>
 
@@ -164,6 +166,36 @@ void slots_one_by_one() {
     std::fflush(stdout);
 }
 
+// test_closest_points.cpp:1632-1700 (superellipsoid_superellipsoid_3d and its 30-degree variant)
+// through the contact slot.
+void superellipsoids_3d() {
+    constexpr std::size_t dim = 3;
+    const double PI = 3.14159265358979323846, c30 = std::sqrt(3.) / 2., s30 = 0.5, dist = 0.4;
+    for (int rot = 0; rot < 2; ++rot) {
+        scopi_container<dim> particles;
+        auto p = property<dim>().mass(1.).moment_inertia({{0.1, 0.1, 0.1}});
+        if (rot == 0) {
+            particles.push_back(superellipsoid<dim>({{-0.2, 0.0, 0.0}}, {quaternion(0.)}, {{0.1, 0.2, 0.3}}, {{1, 1}}), p);
+            particles.push_back(superellipsoid<dim>({{0.2, 0.0, 0.0}}, {quaternion(0.)}, {{0.1, 0.2, 0.3}}, {{1, 1}}), p);
+        } else {
+            particles.push_back(superellipsoid<dim>({{-dist * c30, dist * s30, 0.}}, {quaternion(-PI / 6.)},
+                                                    {{0.1, 0.4, 0.5}}, {{1, 1}}), p);
+            particles.push_back(superellipsoid<dim>({{dist * c30, -dist * s30, 0.}}, {quaternion(PI - PI / 6.)},
+                                                    {{0.2, 0.3, 0.6}}, {{1, 1}}), p);
+        }
+        ContactsParams<contact_kdtree<NoFriction>> params;
+        params.dmax = 1.0;
+        contact_kdtree<NoFriction> cont(params);
+        auto contacts = cont.run(particles);
+        std::printf("{\"case\": \"superellipsoids_3d_%d\", \"n\": %zu, \"dij\": %.17g, \"nij\": [%.17g, %.17g, %.17g], "
+                    "\"pi\": [%.17g, %.17g, %.17g], \"pj\": [%.17g, %.17g, %.17g]}\n",
+                    rot, contacts.size(), contacts[0].dij, contacts[0].nij[0], contacts[0].nij[1], contacts[0].nij[2],
+                    contacts[0].pi[0], contacts[0].pi[1], contacts[0].pi[2], contacts[0].pj[0], contacts[0].pj[1],
+                    contacts[0].pj[2]);
+    }
+    std::fflush(stdout);
+}
+
 int main(int argc, char** argv) {
     std::string only = argc > 1 ? argv[1] : "";
     using kd_fixed = ScopiSolver<2, NoFriction, OptimGradient<apgd>, contact_kdtree, vap_fixed>;
@@ -178,6 +210,7 @@ int main(int argc, char** argv) {
         if (only.empty() || only == "three") three_spheres();
         if (only.empty() || only == "friction") sphere_plane_friction();
         if (only.empty() || only == "slots") slots_one_by_one();
+        if (only.empty() || only == "superellipsoids") superellipsoids_3d();
     } catch (const std::exception& e) {
         std::fprintf(stderr, "test_slots: %s\n", e.what());
         return 1;

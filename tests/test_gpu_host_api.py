@@ -61,3 +61,20 @@ def test_contact_slot_then_optim_slot(output):
     assert abs(o["lambda"] - lam) < 1e-8
     np.testing.assert_allclose(o["u0"], [1 - lam * r2, 1 - lam * r2], atol=1e-8)
     np.testing.assert_allclose(o["u1"], [lam * r2, lam * r2], atol=1e-8)
+
+
+def test_superellipsoid_contact_slot(output):
+    """The reference's SE3-SE3 known answers (test_closest_points.cpp:1632-1700) through the C++
+    contact slot: superellipsoid<3> objects in a scopi_container, contact_kdtree<NoFriction>::run."""
+    c30, s30 = math.sqrt(3.0) / 2.0, 0.5
+    known = {"superellipsoids_3d_0": ((-0.1, 0, 0), (0.1, 0, 0), (-1.0, 0, 0), 0.2),
+             "superellipsoids_3d_1": ((-0.3 * c30, 0.3 * s30, 0), (0.2 * c30, -0.2 * s30, 0), (-c30, s30, 0), 0.5)}
+    for name, (pi, pj, nij, dij) in known.items():
+        c = output[name]
+        assert c["n"] == 1
+        # doctest::Approx default tolerance of the reference's test
+        tol = lambda ref: np.finfo(np.float32).eps * 100 * (1.0 + abs(ref))
+        assert abs(c["dij"] - dij) < tol(dij)
+        for got, ref in ((c["pi"], pi), (c["pj"], pj), (c["nij"], nij)):
+            for g, r in zip(got, ref):
+                assert abs(g - r) < tol(r), (name, got, ref)
