@@ -273,106 +273,143 @@ __device__ inline int init_angles(const SE& s, bool xy, double* out) {
     return cnt;
 }
 
-// ------------------------------------------------------------------ residual with forward-mode AD
-struct D4 {
-    double v;
-    double d[4];
+// ------------------------------------------------------------------ residual and Jacobian
+// F(a1, b1, a2, b2) of closest_points.hpp:282-382, values in the reference's order.  The
+// Jacobian is the forward-mode derivative of the same expression, evaluated one
+// direction at a time (D1: value + one derivative): every derivative component is
+// computed by the same operations, in the same order, as in a 4-component forward
+// mode, so the result is identical to it, with a quarter of the live registers.
+// The transcendental part (8 cos/sin, 16 signed powers and their derivative
+// factors) is evaluated once per point; the two powers of each |cos| / |sin| share
+// one logarithm, which gives the same bits as two separate pow calls.
+struct D1 {
+    double v, d;
 };
-__device__ inline D4 cst(double x) { return D4{x, {0.0, 0.0, 0.0, 0.0}}; }
-__device__ inline D4 var(double x, int k) {
-    D4 r = cst(x);
-    r.d[k] = 1.0;
-    return r;
+__device__ inline D1 operator+(const D1& a, const D1& b) { return D1{a.v + b.v, a.d + b.d}; }
+__device__ inline D1 operator-(const D1& a, const D1& b) { return D1{a.v - b.v, a.d - b.d}; }
+__device__ inline D1 operator-(const D1& a) { return D1{-a.v, -a.d}; }
+__device__ inline D1 operator*(const D1& a, const D1& b) { return D1{a.v * b.v, a.d * b.v + a.v * b.d}; }
+__device__ inline D1 operator*(double s, const D1& a) { return D1{s * a.v, s * a.d}; }
+__device__ inline D1 dsqrt(const D1& a) {
+    const double r = sqrt(a.v);
+    return D1{r, a.d / (2 * r)};
 }
-__device__ inline D4 operator+(const D4& a, const D4& b) {
-    D4 r{a.v + b.v, {}};
-    for (int k = 0; k < 4; ++k) r.d[k] = a.d[k] + b.d[k];
-    return r;
-}
-__device__ inline D4 operator-(const D4& a, const D4& b) {
-    D4 r{a.v - b.v, {}};
-    for (int k = 0; k < 4; ++k) r.d[k] = a.d[k] - b.d[k];
-    return r;
-}
-__device__ inline D4 operator-(const D4& a) {
-    D4 r{-a.v, {}};
-    for (int k = 0; k < 4; ++k) r.d[k] = -a.d[k];
-    return r;
-}
-__device__ inline D4 operator*(const D4& a, const D4& b) {
-    D4 r{a.v * b.v, {}};
-    for (int k = 0; k < 4; ++k) r.d[k] = a.d[k] * b.v + a.v * b.d[k];
-    return r;
-}
-__device__ inline D4 operator*(double s, const D4& a) {
-    D4 r{s * a.v, {}};
-    for (int k = 0; k < 4; ++k) r.d[k] = s * a.d[k];
-    return r;
-}
-__device__ inline D4 dsqrt(const D4& a) {
-    const double s = sqrt(a.v);
-    D4 r{s, {}};
-    for (int k = 0; k < 4; ++k) r.d[k] = a.d[k] / (2 * s);
-    return r;
-}
-__device__ inline D4 dcos(const D4& a) {
-    D4 r{pm_cos(a.v), {}};
-    const double sn = pm_sin(a.v);
-    for (int k = 0; k < 4; ++k) r.d[k] = -sn * a.d[k];
-    return r;
-}
-__device__ inline D4 dsin(const D4& a) {
-    D4 r{pm_sin(a.v), {}};
-    const double cs = pm_cos(a.v);
-    for (int k = 0; k < 4; ++k) r.d[k] = cs * a.d[k];
-    return r;
-}
-__device__ inline D4 spow(const D4& a, double p) {
-    const double ax = fabs(a.v);
-    D4 r{sgn(a.v) * pm_pow(ax, p), {}};
-    const double dv = p * pm_pow(ax, p - 1);
-    for (int k = 0; k < 4; ++k) r.d[k] = dv * a.d[k];
-    return r;
+__device__ inline double dsqrt(double a) { return sqrt(a); }
+template <class V>
+__device__ inline V cst(double x);
+template <>
+__device__ inline double cst<double>(double x) { return x; }
+template <>
+__device__ inline D1 cst<D1>(double x) { return D1{x, 0.0}; }
+
+// |x|^p and p |x|^(p-1) for the exponent pair (p, 2 - p) of one angle function, sharing log|x|
+__device__ inline void pow_pairs(double x, double p, double* v, double* dv) {
+    const double ax = fabs(x);
+    const double q[2] = {p, 2 - p};
+    if (ax != ax || ax == 0 || ax == INFINITY) {
+        for (int k = 0; k < 2; ++k) {
+            v[k] = pm_pow(ax, q[k]);
+            if (dv) dv[k] = q[k] * pm_pow(ax, q[k] - 1);
+        }
+        return;
+    }
+    const double L = pm_log(ax);
+    for (int k = 0; k < 2; ++k) {
+        v[k] = pm_exp(q[k] * L);
+        if (dv) dv[k] = q[k] * pm_exp((q[k] - 1) * L);
+    }
 }
 
-// F(a1, b1, a2, b2) of closest_points.hpp:282-382 (values in the reference's order)
-// and its Jacobian J (row-major, or not at all).
-__device__ inline void se_residual(const SE& s1, const SE& s2, const double* u, double* F, double* J) {
-    D4 a1 = var(u[0], 0), b1 = var(u[1], 1), a2 = var(u[2], 2), b2 = var(u[3], 3);
-    D4 ca1 = dcos(a1), sa1 = dsin(a1), cb1 = dcos(b1), sb1 = dsin(b1);
-    D4 ca2 = dcos(a2), sa2 = dsin(a2), cb2 = dcos(b2), sb2 = dsin(b2);
-    D4 ca1n = spow(ca1, s1.n), ca1n2 = spow(ca1, 2 - s1.n), sa1n = spow(sa1, s1.n), sa1n2 = spow(sa1, 2 - s1.n);
-    D4 cb1e = spow(cb1, s1.e), cb1e2 = spow(cb1, 2 - s1.e), sb1e = spow(sb1, s1.e), sb1e2 = spow(sb1, 2 - s1.e);
-    D4 ca2n = spow(ca2, s2.n), ca2n2 = spow(ca2, 2 - s2.n), sa2n = spow(sa2, s2.n), sa2n2 = spow(sa2, 2 - s2.n);
-    D4 cb2e = spow(cb2, s2.e), cb2e2 = spow(cb2, 2 - s2.e), sb2e = spow(sb2, s2.e), sb2e2 = spow(sb2, 2 - s2.e);
-    const D4 A1 = ((s1.r[1] * s1.r[2]) * ca1n2) * cb1e2, A2 = ((s1.r[0] * s1.r[2]) * sb1e2) * ca1n2,
-             A3 = (s1.r[0] * s1.r[1]) * sa1n2;
-    const D4 A4 = (s1.r[0] * ca1n) * cb1e, A5 = (s1.r[1] * sb1e) * ca1n, A6 = s1.r[2] * sa1n;
-    const D4 B1 = (s2.r[0] * ca2n) * cb2e, B2 = (s2.r[1] * sb2e) * ca2n, B3 = s2.r[2] * sa2n;
-    const D4 B4 = ((s2.r[1] * s2.r[2]) * ca2n2) * cb2e2, B5 = ((s2.r[0] * s2.r[2]) * sb2e2) * ca2n2,
-             B6 = (s2.r[0] * s2.r[1]) * sa2n2;
-    D4 n1[3], n2[3], d[3];
-    for (int k = 0; k < 3; ++k) {
-        n1[k] = (s1.R[k][0] * A1 + s1.R[k][1] * A2) + s1.R[k][2] * A3;
-        n2[k] = (s2.R[k][0] * B4 + s2.R[k][1] * B5) + s2.R[k][2] * B6;
-        d[k] = (((((((-(s1.R[k][0] * A4)) - s1.R[k][1] * A5) - s1.R[k][2] * A6) + s2.R[k][0] * B1) +
-                  s2.R[k][1] * B2) +
-                 s2.R[k][2] * B3) -
-                cst(s1.c[k])) +
-               cst(s2.c[k]);
+// Transcendental terms at u: cs[2m] = cos u_m, cs[2m+1] = sin u_m; powers pw[2f + k] of the
+// angle function f (0 cos a1, 1 sin a1, 2 cos b1, 3 sin b1, 4..7 the same for body 2) with
+// exponent k (0: the shape exponent n or e, 1: 2 minus it), signed by the function.
+struct SeTerms {
+    double cs[8];
+    double pw[16], dp[16];
+};
+__device__ inline void se_terms(const SE& s1, const SE& s2, const double* u, SeTerms& t, bool deriv) {
+    for (int m = 0; m < 4; ++m) {
+        t.cs[2 * m] = pm_cos(u[m]);
+        t.cs[2 * m + 1] = pm_sin(u[m]);
     }
-    D4 f[4];
-    f[0] = (-(n1[1] * d[2])) + n1[2] * d[1];
-    f[1] = n1[0] * d[2] - n1[2] * d[0];
-    f[2] = (-(n1[0] * d[1])) + n1[1] * d[0];
-    const D4 dotn = (n1[0] * n2[0] + n1[1] * n2[1]) + n1[2] * n2[2];
-    const D4 l1 = dsqrt((n1[0] * n1[0] + n1[1] * n1[1]) + n1[2] * n1[2]);
-    const D4 l2 = dsqrt((n2[0] * n2[0] + n2[1] * n2[1]) + n2[2] * n2[2]);
-    f[3] = dotn + l1 * l2;
-    for (int i = 0; i < 4; ++i) {
-        F[i] = f[i].v;
-        if (J)
-            for (int k = 0; k < 4; ++k) J[4 * i + k] = f[i].d[k];
+    // function f: (angle m, cos or sin, exponent): a -> n, b -> e
+    const double ex[8] = {s1.n, s1.n, s1.e, s1.e, s2.n, s2.n, s2.e, s2.e};
+    const int src[8] = {0, 1, 2, 3, 4, 5, 6, 7};  // cs index of function f: cos a1, sin a1, cos b1, ...
+    for (int f = 0; f < 8; ++f) {
+        const double x = t.cs[src[f]];
+        pow_pairs(x, ex[f], &t.pw[2 * f], deriv ? &t.dp[2 * f] : nullptr);
+        t.pw[2 * f] = sgn(x) * t.pw[2 * f];
+        t.pw[2 * f + 1] = sgn(x) * t.pw[2 * f + 1];
+    }
+}
+
+// the algebraic part; V = double (values) or D1 (direction k of the derivative)
+template <class V>
+__device__ inline void se_algebra(const SE& s1, const SE& s2, const SeTerms& t, int k, V* F) {
+    // angle functions with their derivative along direction k (cos' = -sin, sin' = cos)
+    V fn[8];
+    for (int m = 0; m < 4; ++m) {
+        if constexpr (sizeof(V) == sizeof(D1)) {
+            const double seed = (m == k) ? 1.0 : 0.0;
+            fn[2 * m] = D1{t.cs[2 * m], -t.cs[2 * m + 1] * seed};
+            fn[2 * m + 1] = D1{t.cs[2 * m + 1], t.cs[2 * m] * seed};
+        } else {
+            fn[2 * m] = cst<V>(t.cs[2 * m]);
+            fn[2 * m + 1] = cst<V>(t.cs[2 * m + 1]);
+        }
+    }
+    // signed powers: value sgn(x)|x|^p, derivative p|x|^(p-1) x'
+    auto P = [&](int f, int e) -> V {
+        if constexpr (sizeof(V) == sizeof(D1))
+            return D1{t.pw[2 * f + e], t.dp[2 * f + e] * fn[f].d};
+        else
+            return t.pw[2 * f + e];
+    };
+    const V ca1n = P(0, 0), ca1n2 = P(0, 1), sa1n = P(1, 0), sa1n2 = P(1, 1);
+    const V cb1e = P(2, 0), cb1e2 = P(2, 1), sb1e = P(3, 0), sb1e2 = P(3, 1);
+    const V ca2n = P(4, 0), ca2n2 = P(4, 1), sa2n = P(5, 0), sa2n2 = P(5, 1);
+    const V cb2e = P(6, 0), cb2e2 = P(6, 1), sb2e = P(7, 0), sb2e2 = P(7, 1);
+    const V A1 = ((s1.r[1] * s1.r[2]) * ca1n2) * cb1e2, A2 = ((s1.r[0] * s1.r[2]) * sb1e2) * ca1n2,
+            A3 = (s1.r[0] * s1.r[1]) * sa1n2;
+    const V A4 = (s1.r[0] * ca1n) * cb1e, A5 = (s1.r[1] * sb1e) * ca1n, A6 = s1.r[2] * sa1n;
+    const V B1 = (s2.r[0] * ca2n) * cb2e, B2 = (s2.r[1] * sb2e) * ca2n, B3 = s2.r[2] * sa2n;
+    const V B4 = ((s2.r[1] * s2.r[2]) * ca2n2) * cb2e2, B5 = ((s2.r[0] * s2.r[2]) * sb2e2) * ca2n2,
+            B6 = (s2.r[0] * s2.r[1]) * sa2n2;
+    V n1[3], n2[3], d[3];
+    for (int q = 0; q < 3; ++q) {
+        n1[q] = (s1.R[q][0] * A1 + s1.R[q][1] * A2) + s1.R[q][2] * A3;
+        n2[q] = (s2.R[q][0] * B4 + s2.R[q][1] * B5) + s2.R[q][2] * B6;
+        d[q] = (((((((-(s1.R[q][0] * A4)) - s1.R[q][1] * A5) - s1.R[q][2] * A6) + s2.R[q][0] * B1) +
+                  s2.R[q][1] * B2) +
+                 s2.R[q][2] * B3) -
+                cst<V>(s1.c[q])) +
+               cst<V>(s2.c[q]);
+    }
+    F[0] = (-(n1[1] * d[2])) + n1[2] * d[1];
+    F[1] = n1[0] * d[2] - n1[2] * d[0];
+    F[2] = (-(n1[0] * d[1])) + n1[1] * d[0];
+    const V dotn = (n1[0] * n2[0] + n1[1] * n2[1]) + n1[2] * n2[2];
+    const V l1 = dsqrt((n1[0] * n1[0] + n1[1] * n1[1]) + n1[2] * n1[2]);
+    const V l2 = dsqrt((n2[0] * n2[0] + n2[1] * n2[1]) + n2[2] * n2[2]);
+    F[3] = dotn + l1 * l2;
+}
+
+// F at u, and the Jacobian (row-major) when J is not null
+__device__ inline void se_residual(const SE& s1, const SE& s2, const double* u, double* F, double* J) {
+    SeTerms t;
+    se_terms(s1, s2, u, t, J != nullptr);
+    if (!J) {
+        se_algebra<double>(s1, s2, t, -1, F);
+        return;
+    }
+#pragma unroll 1
+    for (int k = 0; k < 4; ++k) {
+        D1 f[4];
+        se_algebra<D1>(s1, s2, t, k, f);
+        for (int i = 0; i < 4; ++i) {
+            J[4 * i + k] = f[i].d;
+            if (k == 0) F[i] = f[i].v;
+        }
     }
 }
 
