@@ -410,23 +410,45 @@ __global__ void __launch_bounds__(kBlock) k_bt(ApgdCtrl* __restrict__ ctrl, int 
 #pragma unroll
     for (int k = 0; k < NW; ++k) acc[k] = (T)0;
     const int e0 = valid ? tstart[p] : 0, e1 = valid ? tstart[p + 1] : 0;
-    for (int e = e0 + sub; e < e1; e += G) {
-        const int32_t cs = tc[e];
-        const int64_t c = cs < 0 ? ~cs : cs;
-        const T* tv = B.tcoef + (int64_t)e * NT;
-        if (LAY != kCone) {
-            const T l = lam[c];
+    // two entries per iteration, every load issued before the first use (the same
+    // accumulation order as one entry at a time)
+    for (int e = e0 + sub; e < e1; e += 2 * G) {
+        const int eb = e + G;
+        const bool hb = eb < e1;
+        const int32_t csa = tc[e];
+        const int32_t csb = hb ? tc[eb] : csa;
+        const int64_t ca = csa < 0 ? ~csa : csa, cb = csb < 0 ? ~csb : csb;
+        const T* tva = B.tcoef + (int64_t)e * NT;
+        const T* tvb = B.tcoef + (int64_t)(hb ? eb : e) * NT;
+        T ta[NT], tb[NT];
 #pragma unroll
-            for (int k = 0; k < NT; ++k) acc[k] += l * tv[k];
+        for (int k = 0; k < NT; ++k) ta[k] = tva[k], tb[k] = tvb[k];
+        if (LAY != kCone) {
+            const T la = lam[ca], lb = lam[cb];
+#pragma unroll
+            for (int k = 0; k < NT; ++k) acc[k] += la * ta[k];
+            if (hb)
+#pragma unroll
+                for (int k = 0; k < NT; ++k) acc[k] += lb * tb[k];
         } else {
-            const T l0 = lam[c * 3], l1 = lam[c * 3 + 1], l2 = lam[c * 3 + 2];
-            if (cs < 0) {
-                acc[0] -= l0, acc[1] -= l1, acc[2] -= l2;
+            const T a0 = lam[ca * 3], a1 = lam[ca * 3 + 1], a2 = lam[ca * 3 + 2];
+            const T b0 = lam[cb * 3], b1 = lam[cb * 3 + 1], b2 = lam[cb * 3 + 2];
+            if (csa < 0) {
+                acc[0] -= a0, acc[1] -= a1, acc[2] -= a2;
             } else {
-                acc[0] += l0, acc[1] += l1, acc[2] += l2;
+                acc[0] += a0, acc[1] += a1, acc[2] += a2;
             }
 #pragma unroll
-            for (int k = 0; k < 3; ++k) acc[3 + k] += l0 * tv[k] + l1 * tv[3 + k] + l2 * tv[6 + k];
+            for (int k = 0; k < 3; ++k) acc[3 + k] += a0 * ta[k] + a1 * ta[3 + k] + a2 * ta[6 + k];
+            if (hb) {
+                if (csb < 0) {
+                    acc[0] -= b0, acc[1] -= b1, acc[2] -= b2;
+                } else {
+                    acc[0] += b0, acc[1] += b1, acc[2] += b2;
+                }
+#pragma unroll
+                for (int k = 0; k < 3; ++k) acc[3 + k] += b0 * tb[k] + b1 * tb[3 + k] + b2 * tb[6 + k];
+            }
         }
     }
 #pragma unroll

@@ -65,7 +65,7 @@ __device__ inline double reduce_pio2(double x, int& quadrant) {
     quadrant = (int)(((int64_t)k) & 3);
     return r;
 }
-__device__ inline double pm_cos(double x) {
+__device__ __attribute__((noinline)) double pm_cos(double x) {
     if (!(fabs(x) < 1e15)) return x - x;
     int q;
     const double r = reduce_pio2(x, q);
@@ -74,7 +74,7 @@ __device__ inline double pm_cos(double x) {
     if (q == 2) return -cos_poly(r);
     return sin_poly(r);
 }
-__device__ inline double pm_sin(double x) {
+__device__ __attribute__((noinline)) double pm_sin(double x) {
     if (!(fabs(x) < 1e15)) return x - x;
     int q;
     const double r = reduce_pio2(x, q);
@@ -106,7 +106,7 @@ __device__ inline double pm_asin(double x) {
         r = kPio2 - 2.0 * asin_small(sqrt((1.0 - a) * 0.5));
     return x < 0 ? -r : r;
 }
-__device__ inline double pm_log(double x) {
+__device__ __attribute__((noinline)) double pm_log(double x) {
     if (x != x || x < 0) return (x - x) / (x - x);
     if (x == 0) return -INFINITY;
     if (x == INFINITY) return x;
@@ -140,7 +140,7 @@ __device__ inline double pm_log(double x) {
     const double dk = (double)k;
     return dk * kLn2Hi + (lm + dk * kLn2Lo);
 }
-__device__ inline double pm_exp(double y) {
+__device__ __attribute__((noinline)) double pm_exp(double y) {
     if (y != y) return y;
     if (y > 709.782712893384) return INFINITY;
     if (y < -745.2) return 0.0;
@@ -203,7 +203,7 @@ __device__ inline SE load_se(const Particles& P, int64_t k) {
 }
 
 // superellipsoid::point(a, b): superellipsoid.hpp:292-300
-__device__ inline void se_point(const SE& s, double a, double b, double* out) {
+__device__ __attribute__((noinline)) void se_point(const SE& s, double a, double b, double* out) {
     const double ca = pm_cos(a), sa = pm_sin(a), cb = pm_cos(b), sb = pm_sin(b);
     double pt[3];
     pt[0] = s.r[0] * sgn(ca) * pm_pow(fabs(ca), s.n) * sgn(cb) * pm_pow(fabs(cb), s.e);
@@ -213,7 +213,7 @@ __device__ inline void se_point(const SE& s, double a, double b, double* out) {
 }
 
 // superellipsoid::normal(a, b): superellipsoid.hpp:314-325
-__device__ inline void se_normal(const SE& s, double a, double b, double* out) {
+__device__ __attribute__((noinline)) void se_normal(const SE& s, double a, double b, double* out) {
     const double ca = pm_cos(a), sa = pm_sin(a), cb = pm_cos(b), sb = pm_sin(b);
     double nn[3];
     nn[0] = s.r[1] * s.r[2] * sgn(ca) * pm_pow(fabs(ca), 2 - s.n) * sgn(cb) * pm_pow(fabs(cb), 2 - s.e);
@@ -419,7 +419,7 @@ __device__ inline void se_residual(const SE& s1, const SE& s2, const double* u, 
 constexpr int N = 4;
 constexpr int LR = N * (N + 1) / 2;
 
-__device__ inline double enorm(int n, const double* x) {
+__device__ __attribute__((noinline)) double enorm(int n, const double* x) {
     const double rdwarf = 3.834e-20, rgiant = 1.304e19;
     double s1 = 0, s2 = 0, s3 = 0, x1max = 0, x3max = 0;
     const double agiant = rgiant / n;
@@ -451,7 +451,7 @@ __device__ inline double enorm(int n, const double* x) {
     return x3max * sqrt(s3);
 }
 
-__device__ inline void qrfac(double* a, double* rdiag, double* acnorm) {
+__device__ __attribute__((noinline)) void qrfac(double* a, double* rdiag, double* acnorm) {
     for (int j = 0; j < N; ++j) {
         acnorm[j] = enorm(N, a + N * j);
         rdiag[j] = acnorm[j];
@@ -473,7 +473,7 @@ __device__ inline void qrfac(double* a, double* rdiag, double* acnorm) {
     }
 }
 
-__device__ inline void qform(double* q, double* wa) {
+__device__ __attribute__((noinline)) void qform(double* q, double* wa) {
     for (int j = 1; j < N; ++j)
         for (int i = 0; i < j; ++i) q[i + N * j] = 0.0;
     for (int l = 0; l < N; ++l) {
@@ -494,7 +494,7 @@ __device__ inline void qform(double* q, double* wa) {
     }
 }
 
-__device__ inline void dogleg(const double* r, const double* diag, const double* qtb, double delta, double* x,
+__device__ __attribute__((noinline)) void dogleg(const double* r, const double* diag, const double* qtb, double delta, double* x,
                               double* wa1, double* wa2) {
     const double epsmch = 2.220446049250313e-16;
     int jj = (N * (N + 1)) / 2 + 1;
@@ -565,7 +565,7 @@ __device__ inline void dogleg(const double* r, const double* diag, const double*
     for (int j = 0; j < N; ++j) x[j] = temp * wa1[j] + alpha * x[j];
 }
 
-__device__ inline void r1updt(double* s, const double* u, double* v, double* w, bool& sing) {
+__device__ __attribute__((noinline)) void r1updt(double* s, const double* u, double* v, double* w, bool& sing) {
     const double giant = 1.79769313486231570e308;
     const int m = N, n = N;
     int jj = (n * (2 * m - n + 1)) / 2 - (m - n);
@@ -640,7 +640,7 @@ __device__ inline void r1updt(double* s, const double* u, double* v, double* w, 
     if (s[jj - 1] == 0.0) sing = true;
 }
 
-__device__ inline void r1mpyq(int m, int n, double* a, int lda, const double* v, const double* w) {
+__device__ __attribute__((noinline)) void r1mpyq(int m, int n, double* a, int lda, const double* v, const double* w) {
     for (int nmj = 1; nmj <= n - 1; ++nmj) {
         const int j = n - nmj;
         double cs, sn;
@@ -675,63 +675,66 @@ __device__ inline void r1mpyq(int m, int n, double* a, int lda, const double* v,
 }
 
 // hybrd1 with the analytic Jacobian; returns info (1 converged, 2 maxfev, 3 xtol too
-// small, 4 no progress; 5 is folded into 4 as minpack.hpp:918-921 does).
+// small, 4 no progress; 5 is folded into 4 as minpack.hpp:918-921 does).  The outer
+// (Jacobian) and inner (trial step) loops of hybrd are one loop here with a single
+// residual evaluation per pass (phase 0: F at the start, 1: F and J at x, 2: F at the
+// trial point), which keeps one copy of the residual code in the kernel; the
+// arithmetic is the loops' own, operation for operation.
 __device__ inline int hybrd1(const SE& s1, const SE& s2, double* x, int* nfev_out) {
     const double xtol = 1.0e-10, factor = 100.0, epsmch = 2.220446049250313e-16;
     const int maxfev = 200 * (N + 1);
     const double p1 = 0.1, p5 = 0.5, p001 = 0.001, p0001 = 0.0001;
     double diag[N] = {1.0, 1.0, 1.0, 1.0};
     double fvec[N], fjac[N * N], r[LR], qtf[N], wa1[N], wa2[N], wa3[N], wa4[N];
+    double Jrow[N * N], ftmp[N];
     int info = 0, nfev = 0;
-    se_residual(s1, s2, x, fvec, nullptr);
-    nfev = 1;
-    double fnorm = enorm(N, fvec);
     const int msum = N;
     int iter = 1, ncsuc = 0, ncfail = 0, nslow1 = 0, nslow2 = 0;
-    double delta = 0.0, xnorm = 0.0;
+    double delta = 0.0, xnorm = 0.0, fnorm = 0.0, pnorm = 0.0;
+    bool jeval = true;
+    int phase = 0;
     for (;;) {
-        bool jeval = true;
-        double Jrow[N * N];
-        double ftmp[N];
-        se_residual(s1, s2, x, ftmp, Jrow);
-        for (int i = 0; i < N; ++i)
-            for (int j = 0; j < N; ++j) fjac[i + N * j] = Jrow[N * i + j];
-        nfev += msum;
-        qrfac(fjac, wa1, wa2);
-        if (iter == 1) {
-            for (int j = 0; j < N; ++j) wa3[j] = diag[j] * x[j];
-            xnorm = enorm(N, wa3);
-            delta = factor * xnorm;
-            if (delta == 0.0) delta = factor;
+        se_residual(s1, s2, phase == 2 ? wa2 : x, phase == 0 ? fvec : (phase == 1 ? ftmp : wa4),
+                    phase == 1 ? Jrow : nullptr);
+        if (phase == 0) {
+            nfev = 1;
+            fnorm = enorm(N, fvec);
+            phase = 1;
+            continue;
         }
-        for (int i = 0; i < N; ++i) qtf[i] = fvec[i];
-        for (int j = 0; j < N; ++j) {
-            if (fjac[j + N * j] != 0.0) {
-                double sum = 0.0;
-                for (int i = j; i < N; ++i) sum += fjac[i + N * j] * qtf[i];
-                const double temp = -sum / fjac[j + N * j];
-                for (int i = j; i < N; ++i) qtf[i] += fjac[i + N * j] * temp;
+        if (phase == 1) {
+            // top of hybrd's outer loop: factor the Jacobian at x
+            jeval = true;
+            for (int i = 0; i < N; ++i)
+                for (int j = 0; j < N; ++j) fjac[i + N * j] = Jrow[N * i + j];
+            nfev += msum;
+            qrfac(fjac, wa1, wa2);
+            if (iter == 1) {
+                for (int j = 0; j < N; ++j) wa3[j] = diag[j] * x[j];
+                xnorm = enorm(N, wa3);
+                delta = factor * xnorm;
+                if (delta == 0.0) delta = factor;
             }
-        }
-        for (int j = 0; j < N; ++j) {
-            int l = j;
-            for (int i = 0; i < j; ++i) {
-                r[l] = fjac[i + N * j];
-                l += N - i - 1;
-            }
-            r[l] = wa1[j];
-        }
-        qform(fjac, wa1);
-        for (;;) {
-            dogleg(r, diag, qtf, delta, wa1, wa2, wa3);
+            for (int i = 0; i < N; ++i) qtf[i] = fvec[i];
             for (int j = 0; j < N; ++j) {
-                wa1[j] = -wa1[j];
-                wa2[j] = x[j] + wa1[j];
-                wa3[j] = diag[j] * wa1[j];
+                if (fjac[j + N * j] != 0.0) {
+                    double sum = 0.0;
+                    for (int i = j; i < N; ++i) sum += fjac[i + N * j] * qtf[i];
+                    const double temp = -sum / fjac[j + N * j];
+                    for (int i = j; i < N; ++i) qtf[i] += fjac[i + N * j] * temp;
+                }
             }
-            const double pnorm = enorm(N, wa3);
-            if (iter == 1) delta = smin(delta, pnorm);
-            se_residual(s1, s2, wa2, wa4, nullptr);
+            for (int j = 0; j < N; ++j) {
+                int l = j;
+                for (int i = 0; i < j; ++i) {
+                    r[l] = fjac[i + N * j];
+                    l += N - i - 1;
+                }
+                r[l] = wa1[j];
+            }
+            qform(fjac, wa1);
+        } else {
+            // inner loop, after the residual at the trial point
             ++nfev;
             const double fnorm1 = enorm(N, wa4);
             double actred = -1.0;
@@ -781,7 +784,10 @@ __device__ inline int hybrd1(const SE& s1, const SE& s2, double* x, int* nfev_ou
             if (nslow2 == 5) info = 4;
             if (nslow1 == 10) info = 5;
             if (info != 0) break;
-            if (ncfail == 2) break;
+            if (ncfail == 2) {
+                phase = 1;  // back to the outer loop: a new Jacobian at x
+                continue;
+            }
             for (int j = 0; j < N; ++j) {
                 double sum = 0.0;
                 for (int i = 0; i < N; ++i) sum += fjac[i + N * j] * wa4[i];
@@ -795,7 +801,16 @@ __device__ inline int hybrd1(const SE& s1, const SE& s2, double* x, int* nfev_ou
             r1mpyq(1, N, qtf, 1, wa2, wa3);
             jeval = false;
         }
-        if (info != 0) break;
+        // propose the next trial point (head of hybrd's inner loop)
+        dogleg(r, diag, qtf, delta, wa1, wa2, wa3);
+        for (int j = 0; j < N; ++j) {
+            wa1[j] = -wa1[j];
+            wa2[j] = x[j] + wa1[j];
+            wa3[j] = diag[j] * wa1[j];
+        }
+        pnorm = enorm(N, wa3);
+        if (iter == 1) delta = smin(delta, pnorm);
+        phase = 2;
     }
     *nfev_out = nfev;
     if (info == 5) info = 4;
