@@ -102,9 +102,8 @@ const char* scopi_hip_last_error(const scopi_hip_ctx* ctx);
 /* Host -> device particle state, obstacles in [0, n_inactive).  Replaces the
  * container fields read by every slot (container.hpp:248-323, 475-597).
  * squareness may be NULL when no superellipsoid is present.  Active particles
- * are spheres, or superellipsoids in a 3D scene without obstacles or spheres
- * (SCOPI_HIP_ERR_UNSUPPORTED otherwise: the sphere-superellipsoid and
- * plane-superellipsoid closest points are not implemented). */
+ * are spheres or (3D) superellipsoids; obstacles are planes, spheres or (2D)
+ * segments. */
 int scopi_hip_upload_state(scopi_hip_ctx* ctx, int64_t n, int64_t n_inactive, const int32_t* shape, const double* pos,
                            const double* q, const double* size, const double* squareness, const double* v,
                            const double* vd, const double* omega, const double* domega, const double* f,
@@ -170,6 +169,20 @@ int scopi_hip_se_closest_points(scopi_hip_ctx* ctx, int64_t n, const double* pos
                                 const double* size_i, const double* sq_i, const double* pos_j, const double* q_j,
                                 const double* size_j, const double* sq_j, double* dij, double* nij, double* pi,
                                 double* pj, double* u0, double* u, int32_t* info, int32_t* nfev);
+
+/* closest_points_dispatcher::dispatch (closest_points.hpp:2037-2043) for n independent
+ * 3D pairs of spheres, planes and superellipsoids, in the given (i, j) order: closed
+ * forms, the superellipsoid pair's grid start + hybrd1, or the sphere / plane against
+ * superellipsoid Newton solve (:944-1302, :1478-1768, utils.hpp:81-118).  Arrays as for
+ * scopi_hip_se_closest_points plus the shape tags; sq_i / sq_j may be NULL when no
+ * superellipsoid is involved.  info[k]: hybrd1's info (superellipsoid pairs), Newton's
+ * iteration count or -1 (sphere / plane pairs), -1 for closed forms; nfev[k]: hybrd1's
+ * evaluations (0 otherwise). */
+int scopi_hip_closest_points(scopi_hip_ctx* ctx, int64_t n, const int32_t* shape_i, const double* pos_i,
+                             const double* q_i, const double* size_i, const double* sq_i, const int32_t* shape_j,
+                             const double* pos_j, const double* q_j, const double* size_j, const double* sq_j,
+                             double* dij, double* nij, double* pi, double* pj, double* u0, double* u, int32_t* info,
+                             int32_t* nfev);
 
 /* Launch shapes of the projected-gradient kernels (the hook the autotuner in
  * scopi_amd.tune drives; no counterpart in the reference):

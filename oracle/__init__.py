@@ -73,6 +73,9 @@ def lib():
         L.oracle_se_residual.argtypes = [D] * 8 + [D, D, C.c_void_p]
         L.oracle_se_point_normal.argtypes = [D, D, D, D, C.c_double, C.c_double, D, D]
         L.oracle_set_se_math.argtypes = [C.c_int32]
+        L.oracle_newton_closest_points.argtypes = [C.c_int32, D, D, D, D, C.c_int32, D, D, D, D, D, D, D, D, D, D, I32]
+        L.oracle_newton_residual.argtypes = [C.c_int32] + [D] * 8 + [D, C.c_void_p]
+        L.oracle_lu_solve4.argtypes = [D, D, D]
         L.oracle_math.argtypes = [C.c_int32, C.c_double, C.c_double, C.POINTER(C.c_double)]
         L.oracle_quaternion.argtypes = [C.c_double, D, D]
         L.oracle_gen_2d_case_spheres.argtypes = [C.c_int32, D, D, D, D]
@@ -198,6 +201,33 @@ def se_residual(pos_i, q_i, size_i, sq_i, pos_j, q_j, size_j, sq_j, u, jacobian=
     L.oracle_se_residual(_v(pos_i, 3), _v(q_i, 4), _v(size_i, 3), _v(sq_i, 2), _v(pos_j, 3), _v(q_j, 4),
                          _v(size_j, 3), _v(sq_j, 2), _v(u, 4), F, J.ctypes.data_as(C.c_void_p) if jacobian else None)
     return (F, J.reshape(4, 4)) if jacobian else F
+
+
+def newton_closest_points(shape_i, pos_i, q_i, size_i, sq_i, shape_j, pos_j, q_j, size_j, sq_j):
+    """closest_points of a sphere or plane against a superellipsoid (dispatcher order) + diagnostics."""
+    L = lib()
+    dij = np.zeros(1); nij = np.zeros(3); pi = np.zeros(3); pj = np.zeros(3); u0 = np.zeros(4); u = np.zeros(4)
+    it = np.zeros(1, np.int32)
+    rc = L.oracle_newton_closest_points(shape_i, _v(pos_i, 3), _v(q_i, 4), _v(size_i, 3), _v(sq_i, 2), shape_j,
+                                        _v(pos_j, 3), _v(q_j, 4), _v(size_j, 3), _v(sq_j, 2), dij, nij, pi, pj, u0, u,
+                                        it)
+    if rc != 0:
+        raise ValueError("not a sphere/plane - superellipsoid pair")
+    return dict(dij=float(dij[0]), nij=nij, pi=pi, pj=pj, u0=u0, u=u, iterations=int(it[0]))
+
+
+def newton_residual(kind, pos_i, q_i, size_i, sq_i, pos_j, q_j, size_j, u, jacobian=False):
+    L = lib()
+    F = np.zeros(4); J = np.zeros(16)
+    L.oracle_newton_residual(kind, _v(pos_i, 3), _v(q_i, 4), _v(size_i, 3), _v(sq_i, 2), _v(pos_j, 3), _v(q_j, 4),
+                             _v(size_j, 3), _v(u, 4), F, J.ctypes.data_as(C.c_void_p) if jacobian else None)
+    return (F, J.reshape(4, 4)) if jacobian else F
+
+
+def lu_solve4(A, b):
+    x = np.zeros(4)
+    lib().oracle_lu_solve4(np.ascontiguousarray(A, np.float64).ravel(), np.ascontiguousarray(b, np.float64), x)
+    return x
 
 
 def set_se_math(portable: bool):

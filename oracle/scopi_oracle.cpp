@@ -226,11 +226,66 @@ se_oracle::SE make_se(const vec3& c, const quat& q, const vec3& r, const std::ar
     return s;
 }
 
+se_oracle::Sph make_sph(const vec3& c, const quat& q, const vec3& r) {
+    se_oracle::Sph s;
+    s.c = c;
+    s.r = r[0];
+    s.R = rotation3(q);
+    return s;
+}
+se_oracle::Pln make_pln(const vec3& c, const quat& q) {
+    se_oracle::Pln p;
+    p.c = c;
+    p.R = rotation3(q);
+    return p;
+}
+
+// sphere / plane against a superellipsoid (3D), in the reference's dispatch order:
+// (sphere i, SE j) and (SE i, plane j) directly; (SE i, sphere j) and (plane i, SE j) through
+// the other order with nij negated and pi / pj swapped (closest_points.hpp:1295-1302, 1761-1768).
+bool newton_pair(int si, const vec3& pi, const quat& qi, const vec3& szi, const std::array<double, 2>& sqi, int sj,
+                 const vec3& pj, const quat& qj, const vec3& szj, const std::array<double, 2>& sqj,
+                 se_oracle::NewtonResult& r) {
+    const bool port = g_se_math != 0;
+    bool flip = false;
+    if (si == ORACLE_SPHERE && sj == ORACLE_SUPERELLIPSOID) {
+        r = port ? se_oracle::sph_se<se_oracle::PortableMath>(make_sph(pi, qi, szi), make_se(pj, qj, szj, sqj))
+                 : se_oracle::sph_se<se_oracle::LibMath>(make_sph(pi, qi, szi), make_se(pj, qj, szj, sqj));
+    } else if (si == ORACLE_SUPERELLIPSOID && sj == ORACLE_SPHERE) {
+        r = port ? se_oracle::sph_se<se_oracle::PortableMath>(make_sph(pj, qj, szj), make_se(pi, qi, szi, sqi))
+                 : se_oracle::sph_se<se_oracle::LibMath>(make_sph(pj, qj, szj), make_se(pi, qi, szi, sqi));
+        flip = true;
+    } else if (si == ORACLE_SUPERELLIPSOID && sj == ORACLE_PLANE) {
+        r = port ? se_oracle::se_pln<se_oracle::PortableMath>(make_se(pi, qi, szi, sqi), make_pln(pj, qj))
+                 : se_oracle::se_pln<se_oracle::LibMath>(make_se(pi, qi, szi, sqi), make_pln(pj, qj));
+    } else if (si == ORACLE_PLANE && sj == ORACLE_SUPERELLIPSOID) {
+        r = port ? se_oracle::se_pln<se_oracle::PortableMath>(make_se(pj, qj, szj, sqj), make_pln(pi, qi))
+                 : se_oracle::se_pln<se_oracle::LibMath>(make_se(pj, qj, szj, sqj), make_pln(pi, qi));
+        flip = true;
+    } else {
+        return false;
+    }
+    if (flip) {
+        for (int d = 0; d < 3; ++d) r.nij[d] *= -1.;
+        std::swap(r.pi, r.pj);
+    }
+    return true;
+}
+
 bool closest_points(int dim, int si, const vec3& pi, const quat& qi, const vec3& szi, const std::array<double, 2>& sqi,
                     int sj, const vec3& pj, const quat& qj, const vec3& szj, const std::array<double, 2>& sqj,
                     Contact& out) {
     if (dim == 3 && si == ORACLE_SUPERELLIPSOID && sj == ORACLE_SUPERELLIPSOID) {
         const se_oracle::SEResult r = se_solve(make_se(pi, qi, szi, sqi), make_se(pj, qj, szj, sqj));
+        out.pi = r.pi;
+        out.pj = r.pj;
+        out.nij = r.nij;
+        out.dij = r.dij;
+        return true;
+    }
+    if (dim == 3 && (si == ORACLE_SUPERELLIPSOID || sj == ORACLE_SUPERELLIPSOID)) {
+        se_oracle::NewtonResult r;
+        if (!newton_pair(si, pi, qi, szi, sqi, sj, pj, qj, szj, sqj, r)) return false;
         out.pi = r.pi;
         out.pj = r.pj;
         out.nij = r.nij;
@@ -1025,6 +1080,61 @@ int oracle_se_closest_points(const double* pos_i, const double* q_i, const doubl
     if (info) *info = r.info;
     if (nfev) *nfev = r.nfev;
     if (njev) *njev = r.njev;
+    return 0;
+}
+
+int oracle_newton_closest_points(int32_t shape_i, const double* pos_i, const double* q_i, const double* size_i,
+                                 const double* sq_i, int32_t shape_j, const double* pos_j, const double* q_j,
+                                 const double* size_j, const double* sq_j, double* dij, double* nij, double* pi,
+                                 double* pj, double* u0, double* u, int32_t* iterations) {
+    se_oracle::NewtonResult r;
+    const vec3 a{pos_i[0], pos_i[1], pos_i[2]}, b{pos_j[0], pos_j[1], pos_j[2]};
+    const vec3 sa{size_i[0], size_i[1], size_i[2]}, sb{size_j[0], size_j[1], size_j[2]};
+    const quat qa{q_i[0], q_i[1], q_i[2], q_i[3]}, qb{q_j[0], q_j[1], q_j[2], q_j[3]};
+    const std::array<double, 2> ea{sq_i ? sq_i[0] : 1.0, sq_i ? sq_i[1] : 1.0}, eb{sq_j ? sq_j[0] : 1.0, sq_j ? sq_j[1] : 1.0};
+    if (!newton_pair(shape_i, a, qa, sa, ea, shape_j, b, qb, sb, eb, r)) return -2;
+    *dij = r.dij;
+    for (int d = 0; d < 3; ++d) {
+        nij[d] = r.nij[d];
+        pi[d] = r.pi[d];
+        pj[d] = r.pj[d];
+    }
+    for (int k = 0; k < 4; ++k) {
+        if (u0) u0[k] = r.u0[k];
+        if (u) u[k] = r.u[k];
+    }
+    if (iterations) *iterations = r.iterations;
+    return 0;
+}
+
+int oracle_newton_residual(int32_t kind, const double* pos_i, const double* q_i, const double* size_i,
+                           const double* sq_i, const double* pos_j, const double* q_j, const double* size_j,
+                           const double* u, double* F, double* J) {
+    const se_oracle::SE s1 = se_from(pos_i, q_i, size_i, sq_i);
+    if (kind == 0) {
+        const se_oracle::Sph s2 = make_sph({pos_j[0], pos_j[1], pos_j[2]}, {q_j[0], q_j[1], q_j[2], q_j[3]},
+                                           {size_j[0], size_j[1], size_j[2]});
+        if (g_se_math)
+            se_oracle::sph_se_residual<se_oracle::PortableMath>(s1, s2, u, F, J);
+        else
+            se_oracle::sph_se_residual<se_oracle::LibMath>(s1, s2, u, F, J);
+    } else {
+        const se_oracle::Pln p2 = make_pln({pos_j[0], pos_j[1], pos_j[2]}, {q_j[0], q_j[1], q_j[2], q_j[3]});
+        const se_oracle::v3 np = se_oracle::pln_normal(p2);
+        const double sp = (s1.c[0] - p2.c[0]) * np[0] + (s1.c[1] - p2.c[1]) * np[1] + (s1.c[2] - p2.c[2]) * np[2];
+        if (g_se_math)
+            se_oracle::se_pln_residual<se_oracle::PortableMath>(s1, p2, sp, u, F, J);
+        else
+            se_oracle::se_pln_residual<se_oracle::LibMath>(s1, p2, sp, u, F, J);
+    }
+    return 0;
+}
+
+int oracle_lu_solve4(const double* A, const double* b, double* x) {
+    double M[16];
+    for (int k = 0; k < 16; ++k) M[k] = A[k];
+    for (int k = 0; k < 4; ++k) x[k] = b[k];
+    se_oracle::lu_solve4(M, x);
     return 0;
 }
 

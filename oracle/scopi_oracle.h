@@ -106,6 +106,20 @@ int oracle_se_closest_points(const double* pos_i, const double* q_i, const doubl
                              const double* pos_j, const double* q_j, const double* size_j, const double* sq_j,
                              double* dij, double* nij, double* pi, double* pj, double* u0, double* u, int32_t* info,
                              int32_t* nfev, int32_t* njev);
+/* closest_points of a sphere or a plane against a superellipsoid (3D), in the dispatcher's
+ * order (shape tags as above), with newton_method's start u0, end point u and iteration
+ * count (-1: not converged). */
+int oracle_newton_closest_points(int32_t shape_i, const double* pos_i, const double* q_i, const double* size_i,
+                                 const double* sq_i, int32_t shape_j, const double* pos_j, const double* q_j,
+                                 const double* size_j, const double* sq_j, double* dij, double* nij, double* pi,
+                                 double* pj, double* u0, double* u, int32_t* iterations);
+/* Residual and Jacobian of the Newton pairs: kind 0 = (superellipsoid i, sphere j) of
+ * closest_points(sphere, superellipsoid), kind 1 = (superellipsoid i, plane j). */
+int oracle_newton_residual(int32_t kind, const double* pos_i, const double* q_i, const double* size_i,
+                           const double* sq_i, const double* pos_j, const double* q_j, const double* size_j,
+                           const double* u, double* F, double* J);
+/* The 4x4 linear solve of newton_method (LU with partial pivoting), A row-major. */
+int oracle_lu_solve4(const double* A, const double* b, double* x);
 /* Residual F(u) (4) and Jacobian (4x4 row-major, may be NULL) of the SE3-SE3 system. */
 int oracle_se_residual(const double* pos_i, const double* q_i, const double* size_i, const double* sq_i,
                        const double* pos_j, const double* q_j, const double* size_j, const double* sq_j,

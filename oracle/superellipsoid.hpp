@@ -718,4 +718,284 @@ inline SEResult se_se(const SE& s1, const SE& s2) {
     return res;
 }
 
+// ================================================================== sphere / plane against a superellipsoid
+// closest_points(sphere<3>, superellipsoid<3>) (closest_points.hpp:944-1281) and
+// closest_points(superellipsoid<3>, plane<3>) (:1478-1746): damped Newton
+// (utils.hpp:81-118: newton_method(u0, F, GradF, args, 200, ftol 1e-10, xtol 1e-7))
+// from the first minimum of a 10 x 10 table of point distances over the diagonal
+// (a_i, b_i) of linspace(-pi/2, pi/2, 10) x linspace(-pi, pi, 10).
+//
+// Residuals, with s1 the superellipsoid (the reference names it s1 in both):
+//   F = [(p2 - p1) x n1 ; n1 . n2 + |n1| |n2|], p2 / n2 the sphere's point / normal, or the
+//   plane's point R (0, a2, b2) + c and ((c_s1 - c_p) . n_p) n_p.
+// Both write the superellipsoid normal as n1 = R (|ca1|^(2-n) cb1e2 ry rz,
+// |ca1|^(2-n) sb1e2 rx rz, sign(ca1) sa1n2 rx ry) (:1018-1020, :1571-1573), i.e. sign(cos a1)
+// times superellipsoid::normal.  GradF (:1053-1252, :1577-1711) is the chain rule of F
+// with the reference's partial derivatives of these terms; for the sphere pair the
+// derivative of |cos a1|^(2-n) is written (2-n)|cos a1|^(1-n) (-sin a1), without the
+// sign(cos a1) factor of the true derivative (:1088, ca1n3), which the plane pair has
+// (:1617).  The restatement evaluates the Jacobian by forward-mode differentiation
+// with that rule.
+struct Sph {
+    v3 c;
+    double r;
+    m33 R;
+};
+struct Pln {
+    v3 c;
+    m33 R;
+};
+
+// sphere::point / normal (a, b): sphere.hpp:203-234
+template <class M>
+inline v3 sph_point(const Sph& s, double a, double b) {
+    const v3 pt = {s.r * M::cos(a) * M::cos(b), s.r * M::cos(a) * M::sin(b), s.r * M::sin(a)};
+    const v3 w = rot(s.R, pt);
+    return {w[0] + s.c[0], w[1] + s.c[1], w[2] + s.c[2]};
+}
+template <class M>
+inline v3 sph_normal(const Sph& s, double a, double b) {
+    const v3 nn = {M::cos(a) * M::cos(b), M::cos(a) * M::sin(b), M::sin(a)};
+    const v3 w = rot(s.R, nn);
+    const double l = std::sqrt(w[0] * w[0] + w[1] * w[1] + w[2] * w[2]);
+    return {w[0] / l, w[1] / l, w[2] / l};
+}
+// plane::point(a, b) = R (0, a, b) + c (plane.hpp:184-192); plane::normal = column 0 of R
+inline v3 pln_point(const Pln& p, double a, double b) {
+    const v3 w = rot(p.R, v3{0.0, a, b});
+    return {w[0] + p.c[0], w[1] + p.c[1], w[2] + p.c[2]};
+}
+inline v3 pln_normal(const Pln& p) { return {p.R[0][0], p.R[1][0], p.R[2][0]}; }
+
+// |x|^p (no sign) with derivative p |x|^(p-1) sign(x) x' (exact) or p |x|^(p-1) x' (the
+// sphere pair's Jacobian)
+template <class M>
+inline D4 upow(const D4& a, double p, bool exact) {
+    const double ax = std::abs(a.v);
+    D4 r{M::pow(ax, p), {}};
+    const double dv = exact ? (p * sgn(a.v)) * M::pow(ax, p - 1) : p * M::pow(ax, p - 1);
+    for (int k = 0; k < 4; ++k) r.d[k] = dv * a.d[k];
+    return r;
+}
+inline D4 dabs(const D4& a) {
+    D4 r{std::abs(a.v), {}};
+    const double sg = sgn(a.v);
+    for (int k = 0; k < 4; ++k) r.d[k] = sg * a.d[k];
+    return r;
+}
+
+// the superellipsoid terms n1 (A1..A3) and p1 - c1 (A4..A6) of both residuals
+template <class M>
+inline void se_side_terms(const SE& s1, const D4& a1, const D4& b1, bool exact, D4* A) {
+    const D4 ca1 = dcos<M>(a1), sa1 = dsin<M>(a1), cb1 = dcos<M>(b1), sb1 = dsin<M>(b1);
+    const D4 ca1n = spow<M>(ca1, s1.n), ca1n2 = upow<M>(ca1, 2 - s1.n, exact);
+    const D4 sa1n = spow<M>(sa1, s1.n), sa1n2 = spow<M>(sa1, 2 - s1.n);
+    const D4 cb1e = spow<M>(cb1, s1.e), cb1e2 = spow<M>(cb1, 2 - s1.e);
+    const D4 sb1e = spow<M>(sb1, s1.e), sb1e2 = spow<M>(sb1, 2 - s1.e);
+    A[0] = ((s1.r[1] * s1.r[2]) * ca1n2) * cb1e2;
+    A[1] = ((s1.r[0] * s1.r[2]) * sb1e2) * ca1n2;
+    A[2] = sgn(ca1.v) * ((s1.r[0] * s1.r[1]) * sa1n2);
+    A[3] = (s1.r[0] * ca1n) * cb1e;
+    A[4] = (s1.r[1] * sb1e) * ca1n;
+    A[5] = s1.r[2] * sa1n;
+}
+
+// F and J (row-major or null) of the sphere pair, closest_points.hpp:958-1252
+template <class M>
+inline void sph_se_residual(const SE& s1, const Sph& s2, const double* u, double* F, double* J) {
+    D4 a1 = var(u[0], 0), b1 = var(u[1], 1), a2 = var(u[2], 2), b2 = var(u[3], 3);
+    D4 A[6];
+    se_side_terms<M>(s1, a1, b1, false, A);
+    const D4 ca2 = dcos<M>(a2), sa2 = dsin<M>(a2), cb2 = dcos<M>(b2), sb2 = dsin<M>(b2);
+    const D4 ca2n2 = dabs(ca2);
+    const double r = s2.r;
+    const D4 B1 = (r * ca2) * cb2, B2 = (r * sb2) * ca2, B3 = r * sa2;
+    const D4 B4 = ((r * r) * ca2n2) * cb2, B5 = ((r * r) * sb2) * ca2n2, B6 = sgn(ca2.v) * ((r * r) * sa2);
+    const m33 &R1 = s1.R, &R2 = s2.R;
+    D4 n1[3], n2[3], d[3];
+    for (int k = 0; k < 3; ++k) {
+        n1[k] = (R1[k][0] * A[0] + R1[k][1] * A[1]) + R1[k][2] * A[2];
+        n2[k] = (R2[k][0] * B4 + R2[k][1] * B5) + R2[k][2] * B6;
+        d[k] = (((((((-(R1[k][0] * A[3])) - R1[k][1] * A[4]) - R1[k][2] * A[5]) + R2[k][0] * B1) + R2[k][1] * B2) +
+                 R2[k][2] * B3) -
+                cst(s1.c[k])) +
+               cst(s2.c[k]);
+    }
+    D4 f[4];
+    f[0] = (-(n1[1] * d[2])) + n1[2] * d[1];
+    f[1] = n1[0] * d[2] - n1[2] * d[0];
+    f[2] = (-(n1[0] * d[1])) + n1[1] * d[0];
+    const D4 dotn = (n1[0] * n2[0] + n1[1] * n2[1]) + n1[2] * n2[2];
+    const D4 l1 = dsqrt((n1[0] * n1[0] + n1[1] * n1[1]) + n1[2] * n1[2]);
+    const D4 l2 = dsqrt((n2[0] * n2[0] + n2[1] * n2[1]) + n2[2] * n2[2]);
+    f[3] = dotn + l1 * l2;
+    for (int i = 0; i < 4; ++i) {
+        F[i] = f[i].v;
+        if (J)
+            for (int k = 0; k < 4; ++k) J[4 * i + k] = f[i].d[k];
+    }
+}
+
+// F and J of the plane pair, closest_points.hpp:1492-1711.  sp is args(29) of the reference,
+// which is the dot product (c_s1 - c_p) . n_p itself (:1484-1493), not its sign as the
+// name `sign_p2s` suggests; F3 = sp n_p . n1 + |n_p| |n1| then has a root only when |sp| >= 1.
+template <class M>
+inline void se_pln_residual(const SE& s1, const Pln& p2, double sp, const double* u, double* F, double* J) {
+    D4 a1 = var(u[0], 0), b1 = var(u[1], 1), a2 = var(u[2], 2), b2 = var(u[3], 3);
+    D4 A[6];
+    se_side_terms<M>(s1, a1, b1, true, A);
+    const m33 &R1 = s1.R, &N = p2.R;
+    D4 n1[3], d[3];
+    for (int k = 0; k < 3; ++k) {
+        n1[k] = (R1[k][0] * A[0] + R1[k][1] * A[1]) + R1[k][2] * A[2];
+        d[k] = ((((((-(R1[k][0] * A[3])) - R1[k][1] * A[4]) - R1[k][2] * A[5]) + N[k][1] * a2) + N[k][2] * b2) +
+                cst(p2.c[k])) -
+               cst(s1.c[k]);
+    }
+    D4 f[4];
+    f[0] = (-(n1[1] * d[2])) + n1[2] * d[1];
+    f[1] = n1[0] * d[2] - n1[2] * d[0];
+    f[2] = (-(n1[0] * d[1])) + n1[1] * d[0];
+    const D4 dotn = ((N[0][0] * sp) * n1[0] + (N[1][0] * sp) * n1[1]) + (N[2][0] * sp) * n1[2];
+    const double lp = std::sqrt((N[0][0] * N[0][0] + N[1][0] * N[1][0]) + N[2][0] * N[2][0]);
+    const D4 l1 = dsqrt((n1[0] * n1[0] + n1[1] * n1[1]) + n1[2] * n1[2]);
+    f[3] = dotn + lp * l1;
+    for (int i = 0; i < 4; ++i) {
+        F[i] = f[i].v;
+        if (J)
+            for (int k = 0; k < 4; ++k) J[4 * i + k] = f[i].d[k];
+    }
+}
+
+// xt::linalg::solve (LAPACK gesv): LU with partial pivoting (first largest |pivot|),
+// forward and back substitution.  A is row-major 4 x 4, b is overwritten with x.
+inline void lu_solve4(double* A, double* b) {
+    int piv[4];
+    for (int k = 0; k < 4; ++k) {
+        int p = k;
+        double best = std::abs(A[4 * k + k]);
+        for (int i = k + 1; i < 4; ++i)
+            if (std::abs(A[4 * i + k]) > best) {
+                best = std::abs(A[4 * i + k]);
+                p = i;
+            }
+        piv[k] = p;
+        if (p != k)
+            for (int j = 0; j < 4; ++j) std::swap(A[4 * k + j], A[4 * p + j]);
+        if (A[4 * k + k] != 0.0) {
+            const double inv = 1.0 / A[4 * k + k];
+            for (int i = k + 1; i < 4; ++i) A[4 * i + k] *= inv;
+        }
+        for (int i = k + 1; i < 4; ++i)
+            for (int j = k + 1; j < 4; ++j) A[4 * i + j] -= A[4 * i + k] * A[4 * k + j];
+    }
+    for (int k = 0; k < 4; ++k)
+        if (piv[k] != k) std::swap(b[k], b[piv[k]]);
+    for (int i = 1; i < 4; ++i)
+        for (int j = 0; j < i; ++j) b[i] -= A[4 * i + j] * b[j];
+    for (int i = 3; i >= 0; --i) {
+        for (int j = i + 1; j < 4; ++j) b[i] -= A[4 * i + j] * b[j];
+        b[i] /= A[4 * i + i];
+    }
+}
+
+inline double norm4(const double* x) { return std::sqrt(((x[0] * x[0] + x[1] * x[1]) + x[2] * x[2]) + x[3] * x[3]); }
+
+// newton_method (utils.hpp:81-118); returns the iteration count, -1 when itermax is reached
+template <class Fn>
+inline int newton(Fn fcn, double* u, int itermax = 200, double ftol = 1.0e-10, double xtol = 1.0e-7) {
+    int cc = 0;
+    while (cc < itermax) {
+        double F[4], J[16], d[4];
+        fcn(u, F, J);
+        for (int k = 0; k < 4; ++k) d[k] = -F[k];
+        lu_solve4(J, d);
+        if (norm4(d) < xtol) return cc;
+        double t = 1;
+        const double ferr = norm4(F);
+        if (ferr < ftol) return cc;
+        for (;;) {
+            double ut[4], Ft[4];
+            for (int k = 0; k < 4; ++k) ut[k] = u[k] + t * d[k];
+            fcn(ut, Ft, nullptr);
+            if (!(norm4(Ft) > ferr && t > 0.01)) break;
+            t -= 0.01;
+        }
+        for (int k = 0; k < 4; ++k) u[k] += t * d[k];
+        cc += 1;
+    }
+    return -1;
+}
+
+// first minimum of the 10 x 10 diagonal-grid distance table (closest_points.hpp:1255-1271, 1720-1736)
+template <class P1, class P2>
+inline void diag_grid_start(P1 point1, P2 point2, double* u0) {
+    const double pi = 4 * std::atan(1);
+    const int num = 10;
+    double dmin = INFINITY;
+    int imin = 0, jmin = 0;
+    for (int i = 0; i < num; ++i) {
+        const v3 p = point1(linspace(-pi / 2, pi / 2, num, i), linspace(-pi, pi, num, i));
+        for (int j = 0; j < num; ++j) {
+            const v3 q = point2(linspace(-pi / 2, pi / 2, num, j), linspace(-pi, pi, num, j));
+            const double dx = p[0] - q[0], dy = p[1] - q[1], dz = p[2] - q[2];
+            const double dist = std::sqrt(dx * dx + dy * dy + dz * dz);
+            if (dist < dmin) {
+                dmin = dist;
+                imin = i;
+                jmin = j;
+            }
+        }
+    }
+    u0[0] = linspace(-pi / 2, pi / 2, num, imin);
+    u0[1] = linspace(-pi, pi, num, imin);
+    u0[2] = linspace(-pi / 2, pi / 2, num, jmin);
+    u0[3] = linspace(-pi, pi, num, jmin);
+}
+
+struct NewtonResult {
+    v3 pi, pj, nij;
+    double dij;
+    double u0[4], u[4];
+    int iterations;  // newton_method's count, -1 when it did not converge
+};
+
+// closest_points(sphere s2, superellipsoid s1): pi on the superellipsoid, pj on the sphere,
+// nij the sphere's normal, dij = (pi - pj) . nij (closest_points.hpp:1275-1279)
+template <class M>
+inline NewtonResult sph_se(const Sph& s2, const SE& s1) {
+    NewtonResult res;
+    diag_grid_start([&](double a, double b) { return se_point<M>(s1, a, b); },
+                    [&](double a, double b) { return sph_point<M>(s2, a, b); }, res.u0);
+    for (int k = 0; k < 4; ++k) res.u[k] = res.u0[k];
+    res.iterations =
+        newton([&](const double* u, double* F, double* J) { sph_se_residual<M>(s1, s2, u, F, J); }, res.u);
+    res.pi = se_point<M>(s1, res.u[0], res.u[1]);
+    res.pj = sph_point<M>(s2, res.u[2], res.u[3]);
+    res.nij = sph_normal<M>(s2, res.u[2], res.u[3]);
+    res.dij = (res.pi[0] - res.pj[0]) * res.nij[0] + (res.pi[1] - res.pj[1]) * res.nij[1] +
+              (res.pi[2] - res.pj[2]) * res.nij[2];
+    return res;
+}
+
+// closest_points(superellipsoid s1, plane p2): pi on the superellipsoid, pj on the plane,
+// nij the plane's normal (not oriented), dij = (pi - pj) . nij (closest_points.hpp:1740-1744)
+template <class M>
+inline NewtonResult se_pln(const SE& s1, const Pln& p2) {
+    NewtonResult res;
+    const v3 np = pln_normal(p2);
+    const double sp = (s1.c[0] - p2.c[0]) * np[0] + (s1.c[1] - p2.c[1]) * np[1] + (s1.c[2] - p2.c[2]) * np[2];
+    diag_grid_start([&](double a, double b) { return se_point<M>(s1, a, b); },
+                    [&](double a, double b) { return pln_point(p2, a, b); }, res.u0);
+    for (int k = 0; k < 4; ++k) res.u[k] = res.u0[k];
+    res.iterations =
+        newton([&](const double* u, double* F, double* J) { se_pln_residual<M>(s1, p2, sp, u, F, J); }, res.u);
+    res.pi = se_point<M>(s1, res.u[0], res.u[1]);
+    res.pj = pln_point(p2, res.u[2], res.u[3]);
+    res.nij = np;
+    res.dij = (res.pi[0] - res.pj[0]) * res.nij[0] + (res.pi[1] - res.pj[1]) * res.nij[1] +
+              (res.pi[2] - res.pj[2]) * res.nij[2];
+    return res;
+}
+
 }  // namespace se_oracle

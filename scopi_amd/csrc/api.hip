@@ -261,7 +261,7 @@ int scopi_hip_upload_state(scopi_hip_ctx* ctx, int64_t n, int64_t n_inactive, co
             !domega || !f || !m || !J)
             return fail(ctx, SCOPI_HIP_ERR_ARG, "upload_state: bad argument");
         if (n > INT32_MAX / 2) return fail(ctx, SCOPI_HIP_ERR_CAPACITY, "upload_state: too many particles");
-        bool has_se = false, has_sphere = false;
+        bool has_se = false;
         for (int64_t k = 0; k < n; ++k) {
             const int s = shape[k];
             const bool active = k >= n_inactive;
@@ -273,12 +273,9 @@ int scopi_hip_upload_state(scopi_hip_ctx* ctx, int64_t n, int64_t n_inactive, co
             if (s == SCOPI_SHAPE_SEGMENT && ctx->dim != 2)
                 return fail(ctx, SCOPI_HIP_ERR_UNSUPPORTED, "upload_state: segments are 2D objects");
             has_se = has_se || s == SCOPI_SHAPE_SUPERELLIPSOID;
-            has_sphere = has_sphere || (active && s == SCOPI_SHAPE_SPHERE);
         }
-        if (has_se && (ctx->dim != 3 || n_inactive > 0 || has_sphere))
-            return fail(ctx, SCOPI_HIP_ERR_UNSUPPORTED,
-                        "upload_state: superellipsoids are supported in 3D scenes of superellipsoids only "
-                        "(no obstacles, no spheres)");
+        if (has_se && ctx->dim != 3)
+            return fail(ctx, SCOPI_HIP_ERR_UNSUPPORTED, "upload_state: superellipsoids are 3D objects here");
         ctx->has_se = has_se;
         particles_reserve(ctx->P, n);
         Particles& P = ctx->P;
@@ -331,63 +328,92 @@ int scopi_hip_upload_velocities(scopi_hip_ctx* ctx, const double* v, const doubl
     });
 }
 
+namespace {
+// n independent pairs (particles k and n + k of a scratch set) through the candidate kernels
+int pairs_batch(scopi_hip_ctx* ctx, int64_t n, const int32_t* shape_i, const double* pos_i, const double* q_i,
+                const double* size_i, const double* sq_i, const int32_t* shape_j, const double* pos_j,
+                const double* q_j, const double* size_j, const double* sq_j, double* dij, double* nij, double* pi,
+                double* pj, double* u0, double* u, int32_t* info, int32_t* nfev) {
+    if (n < 0 || n > INT32_MAX / 4 || (n > 0 && (!pos_i || !q_i || !size_i || !pos_j || !q_j || !size_j || !dij ||
+                                                 !nij || !pi || !pj)))
+        return fail(ctx, SCOPI_HIP_ERR_ARG, "closest_points: bad argument");
+    if (n == 0) return SCOPI_HIP_OK;
+    std::vector<int32_t> shape(2 * n, SCOPI_SHAPE_SUPERELLIPSOID), ci(n), cj(n);
+    for (int64_t k = 0; k < n; ++k) {
+        if (shape_i) shape[k] = shape_i[k];
+        if (shape_j) shape[n + k] = shape_j[k];
+        for (int64_t m : {k, n + k})
+            if (shape[m] != SCOPI_SHAPE_SPHERE && shape[m] != SCOPI_SHAPE_PLANE &&
+                shape[m] != SCOPI_SHAPE_SUPERELLIPSOID)
+                return fail(ctx, SCOPI_HIP_ERR_UNSUPPORTED, "closest_points: 3D spheres, planes, superellipsoids");
+        ci[k] = (int32_t)k;
+        cj[k] = (int32_t)(n + k);
+    }
+    hipStream_t s = ctx->stream;
+    Particles T{};
+    T.dim = 3;
+    SeCandidates S{};
+    SeGrids G{};
+    struct Cleanup {
+        Particles& T;
+        SeCandidates& S;
+        SeGrids& G;
+        ~Cleanup() {
+            particles_free(T);
+            se_candidates_free(S);
+            se_grids_free(G);
+        }
+    } cleanup{T, S, G};
+    particles_reserve(T, 2 * n);
+    T.n = 2 * n;
+    T.n_inactive = 0;
+    auto both = [&](const double* a, const double* b, int w, double fill) {
+        std::vector<double> t((size_t)2 * n * w, fill);
+        if (a) std::memcpy(t.data(), a, (size_t)n * w * sizeof(double));
+        if (b) std::memcpy(t.data() + (size_t)n * w, b, (size_t)n * w * sizeof(double));
+        return t;
+    };
+    upload_rows(T.pos, T.cap, both(pos_i, pos_j, 3, 0.0).data(), 2 * n, 3, s);
+    upload_rows(T.q, T.cap, both(q_i, q_j, 4, 0.0).data(), 2 * n, 4, s);
+    upload_rows(T.size, T.cap, both(size_i, size_j, 3, 0.0).data(), 2 * n, 3, s);
+    upload_rows(T.sq, T.cap, both(sq_i, sq_j, 2, 1.0).data(), 2 * n, 2, s);
+    SCOPI_HIP_CHECK(hipMemcpyAsync(T.shape, shape.data(), 2 * n * sizeof(int32_t), hipMemcpyHostToDevice, s));
+    se_candidates_reserve(S, n);
+    SCOPI_HIP_CHECK(hipMemcpyAsync(S.i, ci.data(), n * sizeof(int32_t), hipMemcpyHostToDevice, s));
+    SCOPI_HIP_CHECK(hipMemcpyAsync(S.j, cj.data(), n * sizeof(int32_t), hipMemcpyHostToDevice, s));
+    se_closest_points(T, G, S, n, s);
+    SCOPI_HIP_CHECK(hipMemcpyAsync(dij, S.dij, n * sizeof(double), hipMemcpyDeviceToHost, s));
+    if (u0) SCOPI_HIP_CHECK(hipMemcpyAsync(u0, S.u0, 4 * n * sizeof(double), hipMemcpyDeviceToHost, s));
+    if (u) SCOPI_HIP_CHECK(hipMemcpyAsync(u, S.u, 4 * n * sizeof(double), hipMemcpyDeviceToHost, s));
+    if (info) SCOPI_HIP_CHECK(hipMemcpyAsync(info, S.info, n * sizeof(int32_t), hipMemcpyDeviceToHost, s));
+    if (nfev) SCOPI_HIP_CHECK(hipMemcpyAsync(nfev, S.nfev, n * sizeof(int32_t), hipMemcpyDeviceToHost, s));
+    download_rows(S.nij, S.cap, nij, n, 3, s);
+    download_rows(S.pi, S.cap, pi, n, 3, s);
+    download_rows(S.pj, S.cap, pj, n, 3, s);
+    return SCOPI_HIP_OK;
+}
+}  // namespace
+
 int scopi_hip_se_closest_points(scopi_hip_ctx* ctx, int64_t n, const double* pos_i, const double* q_i,
                                 const double* size_i, const double* sq_i, const double* pos_j, const double* q_j,
                                 const double* size_j, const double* sq_j, double* dij, double* nij, double* pi,
                                 double* pj, double* u0, double* u, int32_t* info, int32_t* nfev) {
     return guarded(ctx, [&]() -> int {
-        if (n < 0 || n > INT32_MAX / 4 || (n > 0 && (!pos_i || !q_i || !size_i || !sq_i || !pos_j || !q_j ||
-                                                     !size_j || !sq_j || !dij || !nij || !pi || !pj)))
-            return fail(ctx, SCOPI_HIP_ERR_ARG, "se_closest_points: bad argument");
-        if (n == 0) return SCOPI_HIP_OK;
-        hipStream_t s = ctx->stream;
-        // the two bodies of pair k are particles k and n + k of a scratch particle set
-        Particles T{};
-        T.dim = 3;
-        SeCandidates S{};
-        SeGrids G{};
-        struct Cleanup {
-            Particles& T;
-            SeCandidates& S;
-            SeGrids& G;
-            ~Cleanup() {
-                particles_free(T);
-                se_candidates_free(S);
-                se_grids_free(G);
-            }
-        } cleanup{T, S, G};
-        particles_reserve(T, 2 * n);
-        T.n = 2 * n;
-        T.n_inactive = 0;
-        auto both = [&](const double* a, const double* b, int w) {
-            std::vector<double> t((size_t)2 * n * w);
-            std::memcpy(t.data(), a, (size_t)n * w * sizeof(double));
-            std::memcpy(t.data() + (size_t)n * w, b, (size_t)n * w * sizeof(double));
-            return t;
-        };
-        upload_rows(T.pos, T.cap, both(pos_i, pos_j, 3).data(), 2 * n, 3, s);
-        upload_rows(T.q, T.cap, both(q_i, q_j, 4).data(), 2 * n, 4, s);
-        upload_rows(T.size, T.cap, both(size_i, size_j, 3).data(), 2 * n, 3, s);
-        upload_rows(T.sq, T.cap, both(sq_i, sq_j, 2).data(), 2 * n, 2, s);
-        std::vector<int32_t> shape(2 * n, SCOPI_SHAPE_SUPERELLIPSOID), ci(n), cj(n);
-        for (int64_t k = 0; k < n; ++k) {
-            ci[k] = (int32_t)k;
-            cj[k] = (int32_t)(n + k);
-        }
-        SCOPI_HIP_CHECK(hipMemcpyAsync(T.shape, shape.data(), 2 * n * sizeof(int32_t), hipMemcpyHostToDevice, s));
-        se_candidates_reserve(S, n);
-        SCOPI_HIP_CHECK(hipMemcpyAsync(S.i, ci.data(), n * sizeof(int32_t), hipMemcpyHostToDevice, s));
-        SCOPI_HIP_CHECK(hipMemcpyAsync(S.j, cj.data(), n * sizeof(int32_t), hipMemcpyHostToDevice, s));
-        se_closest_points(T, G, S, n, s);
-        SCOPI_HIP_CHECK(hipMemcpyAsync(dij, S.dij, n * sizeof(double), hipMemcpyDeviceToHost, s));
-        if (u0) SCOPI_HIP_CHECK(hipMemcpyAsync(u0, S.u0, 4 * n * sizeof(double), hipMemcpyDeviceToHost, s));
-        if (u) SCOPI_HIP_CHECK(hipMemcpyAsync(u, S.u, 4 * n * sizeof(double), hipMemcpyDeviceToHost, s));
-        if (info) SCOPI_HIP_CHECK(hipMemcpyAsync(info, S.info, n * sizeof(int32_t), hipMemcpyDeviceToHost, s));
-        if (nfev) SCOPI_HIP_CHECK(hipMemcpyAsync(nfev, S.nfev, n * sizeof(int32_t), hipMemcpyDeviceToHost, s));
-        download_rows(S.nij, S.cap, nij, n, 3, s);
-        download_rows(S.pi, S.cap, pi, n, 3, s);
-        download_rows(S.pj, S.cap, pj, n, 3, s);
-        return SCOPI_HIP_OK;
+        if (n > 0 && (!sq_i || !sq_j)) return fail(ctx, SCOPI_HIP_ERR_ARG, "se_closest_points: bad argument");
+        return pairs_batch(ctx, n, nullptr, pos_i, q_i, size_i, sq_i, nullptr, pos_j, q_j, size_j, sq_j, dij, nij, pi,
+                           pj, u0, u, info, nfev);
+    });
+}
+
+int scopi_hip_closest_points(scopi_hip_ctx* ctx, int64_t n, const int32_t* shape_i, const double* pos_i,
+                             const double* q_i, const double* size_i, const double* sq_i, const int32_t* shape_j,
+                             const double* pos_j, const double* q_j, const double* size_j, const double* sq_j,
+                             double* dij, double* nij, double* pi, double* pj, double* u0, double* u, int32_t* info,
+                             int32_t* nfev) {
+    return guarded(ctx, [&]() -> int {
+        if (n > 0 && (!shape_i || !shape_j)) return fail(ctx, SCOPI_HIP_ERR_ARG, "closest_points: bad argument");
+        return pairs_batch(ctx, n, shape_i, pos_i, q_i, size_i, sq_i, shape_j, pos_j, q_j, size_j, sq_j, dij, nij, pi,
+                           pj, u0, u, info, nfev);
     });
 }
 

@@ -324,17 +324,27 @@ __global__ void __launch_bounds__(kBlock) k_contact_geometry(Particles P, Contac
     C.mu[c] = mu;
 }
 
-// candidate (i, j) of the superellipsoid path
+// candidate (i, j) of the superellipsoid path: active pairs after the obstacle pairs
 __global__ void __launch_bounds__(kBlock) k_cand_scatter(int64_t n_inactive, int64_t na, const int* __restrict__ counts,
                                                         const int* __restrict__ offsets,
-                                                        const int32_t* __restrict__ pair_j, SeCandidates S) {
+                                                        const int32_t* __restrict__ pair_j, int64_t base,
+                                                        SeCandidates S) {
     const int64_t t = (int64_t)blockIdx.x * kBlock + threadIdx.x;
     if (t >= na) return;
     const int64_t b = offsets[t], cnt = counts[t];
     for (int64_t k = 0; k < cnt; ++k) {
-        S.i[b + k] = (int32_t)(n_inactive + t);
-        S.j[b + k] = pair_j[b + k];
+        S.i[base + b + k] = (int32_t)(n_inactive + t);
+        S.j[base + b + k] = pair_j[b + k];
     }
+}
+
+// every (obstacle o, active j) pair is a candidate (contact_kdtree.hpp:249-256 brute-forces them)
+__global__ void __launch_bounds__(kBlock) k_cand_obstacles(int64_t n_inactive, int64_t na, SeCandidates S) {
+    const int64_t k = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (k >= n_inactive * na) return;
+    const int64_t o = k / na;
+    S.i[k] = (int32_t)o;
+    S.j[k] = (int32_t)(n_inactive + (k - o * na));
 }
 
 __global__ void __launch_bounds__(kBlock) k_cand_flags(SeCandidates S, int64_t n, double dmax, int* __restrict__ flags) {
@@ -509,37 +519,42 @@ int64_t ContactSearch::run(const Particles& P, Contacts& C, const scopi_hip_para
     return nc;
 }
 
-// Scenes with superellipsoids (no obstacles, 3D): candidates by the kd-tree predicate
-// (every pair for brute force), closest points of every candidate on the device,
-// then the dmax filter in candidate order.
+// Scenes with superellipsoids (3D): candidates are every (obstacle, active) pair and the active
+// pairs passing the kd-tree predicate (every pair for brute force); closest points of every
+// candidate on the device, then the dmax filter in candidate order, which is (i, j) order.
 int64_t ContactSearch::run_candidates(const Particles& P, Contacts& C, const scopi_hip_params& prm, hipStream_t s) {
     const int64_t na = P.n - P.n_inactive, ni = P.n_inactive;
     const bool kd = prm.contact_method == SCOPI_CONTACT_KDTREE;
     nc_obs_ = 0;
-    if (na < 2) return 0;
+    if (na < 1) return 0;
     reserve_particles(na, ni);
-    const unsigned nparts = (unsigned)std::min<int64_t>(grid_for(na), 1024);
-    hipLaunchKernelGGL(k_bbox_partial, dim3(nparts), dim3(kBlock), 0, s, P, d_partial_);
-    hipLaunchKernelGGL(k_grid_setup, dim3(1), dim3(kBlock), 0, s, d_partial_, (int)nparts, P.dim, kd ? 1 : 0,
-                       prm.kd_tree_radius, prm.dmax, 0, d_grid_);
-    hipLaunchKernelGGL(k_cell_keys<3>, dim3(grid_for(na)), dim3(kBlock), 0, s, P, d_grid_, keys_, vals_);
-    sorter_.sort(keys_, vals_, na, 3 * kMaxBits3, s, scanner_);
-    const int64_t ncells = int64_t(1) << (3 * kMaxBits3);
-    hipLaunchKernelGGL(k_fill_int, dim3(1024), dim3(kBlock), 0, s, cell_start_, ncells, 0);
-    hipLaunchKernelGGL(k_fill_int, dim3(1024), dim3(kBlock), 0, s, cell_end_, ncells, 0);
-    hipLaunchKernelGGL(k_cell_range, dim3(grid_for(na)), dim3(kBlock), 0, s, keys_, na, cell_start_, cell_end_);
-    if (kd)
-        hipLaunchKernelGGL((k_pair_count<3, true, true>), dim3(grid_for(na)), dim3(kBlock), 0, s, P, d_grid_,
-                           cell_start_, cell_end_, vals_, prm.kd_tree_radius, prm.dmax, counts_);
-    else
-        hipLaunchKernelGGL((k_pair_count<3, false, true>), dim3(grid_for(na)), dim3(kBlock), 0, s, P, d_grid_,
-                           cell_start_, cell_end_, vals_, prm.kd_tree_radius, prm.dmax, counts_);
-    scanner_.exclusive(counts_, offsets_, na, d_totals_, s);
-    SCOPI_HIP_CHECK(hipMemcpyAsync(h_totals_, d_totals_, sizeof(int), hipMemcpyDeviceToHost, s));
-    SCOPI_HIP_CHECK(hipStreamSynchronize(s));
-    const int64_t ncand = h_totals_[0];
+    int64_t nact = 0;
+    if (na > 1) {
+        const unsigned nparts = (unsigned)std::min<int64_t>(grid_for(na), 1024);
+        hipLaunchKernelGGL(k_bbox_partial, dim3(nparts), dim3(kBlock), 0, s, P, d_partial_);
+        hipLaunchKernelGGL(k_grid_setup, dim3(1), dim3(kBlock), 0, s, d_partial_, (int)nparts, P.dim, kd ? 1 : 0,
+                           prm.kd_tree_radius, prm.dmax, 0, d_grid_);
+        hipLaunchKernelGGL(k_cell_keys<3>, dim3(grid_for(na)), dim3(kBlock), 0, s, P, d_grid_, keys_, vals_);
+        sorter_.sort(keys_, vals_, na, 3 * kMaxBits3, s, scanner_);
+        const int64_t ncells = int64_t(1) << (3 * kMaxBits3);
+        hipLaunchKernelGGL(k_fill_int, dim3(1024), dim3(kBlock), 0, s, cell_start_, ncells, 0);
+        hipLaunchKernelGGL(k_fill_int, dim3(1024), dim3(kBlock), 0, s, cell_end_, ncells, 0);
+        hipLaunchKernelGGL(k_cell_range, dim3(grid_for(na)), dim3(kBlock), 0, s, keys_, na, cell_start_, cell_end_);
+        if (kd)
+            hipLaunchKernelGGL((k_pair_count<3, true, true>), dim3(grid_for(na)), dim3(kBlock), 0, s, P, d_grid_,
+                               cell_start_, cell_end_, vals_, prm.kd_tree_radius, prm.dmax, counts_);
+        else
+            hipLaunchKernelGGL((k_pair_count<3, false, true>), dim3(grid_for(na)), dim3(kBlock), 0, s, P, d_grid_,
+                               cell_start_, cell_end_, vals_, prm.kd_tree_radius, prm.dmax, counts_);
+        scanner_.exclusive(counts_, offsets_, na, d_totals_, s);
+        SCOPI_HIP_CHECK(hipMemcpyAsync(h_totals_, d_totals_, sizeof(int), hipMemcpyDeviceToHost, s));
+        SCOPI_HIP_CHECK(hipStreamSynchronize(s));
+        nact = h_totals_[0];
+    }
+    const int64_t nobs = ni * na;
+    const int64_t ncand = nobs + nact;
     if (ncand == 0) return 0;
-    reserve_pairs(ncand);
+    reserve_pairs(std::max<int64_t>(nact, 1));
     se_candidates_reserve(se_cand_, ncand);
     if (ncand + 1 > cand_cap_) {
         if (cand_flags_) SCOPI_HIP_CHECK(hipFree(cand_flags_));
@@ -548,14 +563,18 @@ int64_t ContactSearch::run_candidates(const Particles& P, Contacts& C, const sco
         SCOPI_HIP_CHECK(hipMalloc(&cand_flags_, cand_cap_ * sizeof(int)));
         SCOPI_HIP_CHECK(hipMalloc(&cand_offsets_, cand_cap_ * sizeof(int)));
     }
-    if (kd)
-        hipLaunchKernelGGL((k_pair_emit<3, true, true>), dim3(grid_for(na)), dim3(kBlock), 0, s, P, d_grid_,
-                           cell_start_, cell_end_, vals_, prm.kd_tree_radius, prm.dmax, offsets_, pair_j_);
-    else
-        hipLaunchKernelGGL((k_pair_emit<3, false, true>), dim3(grid_for(na)), dim3(kBlock), 0, s, P, d_grid_,
-                           cell_start_, cell_end_, vals_, prm.kd_tree_radius, prm.dmax, offsets_, pair_j_);
-    hipLaunchKernelGGL(k_cand_scatter, dim3(grid_for(na)), dim3(kBlock), 0, s, ni, na, counts_, offsets_, pair_j_,
-                       se_cand_);
+    if (nobs > 0)
+        hipLaunchKernelGGL(k_cand_obstacles, dim3(grid_for(nobs)), dim3(kBlock), 0, s, ni, na, se_cand_);
+    if (nact > 0) {
+        if (kd)
+            hipLaunchKernelGGL((k_pair_emit<3, true, true>), dim3(grid_for(na)), dim3(kBlock), 0, s, P, d_grid_,
+                               cell_start_, cell_end_, vals_, prm.kd_tree_radius, prm.dmax, offsets_, pair_j_);
+        else
+            hipLaunchKernelGGL((k_pair_emit<3, false, true>), dim3(grid_for(na)), dim3(kBlock), 0, s, P, d_grid_,
+                               cell_start_, cell_end_, vals_, prm.kd_tree_radius, prm.dmax, offsets_, pair_j_);
+        hipLaunchKernelGGL(k_cand_scatter, dim3(grid_for(na)), dim3(kBlock), 0, s, ni, na, counts_, offsets_,
+                           pair_j_, nobs, se_cand_);
+    }
     se_closest_points(P, se_grids_, se_cand_, ncand, s);
     hipLaunchKernelGGL(k_cand_flags, dim3(grid_for(ncand)), dim3(kBlock), 0, s, se_cand_, ncand, prm.dmax,
                        cand_flags_);

@@ -203,7 +203,7 @@ __device__ inline SE load_se(const Particles& P, int64_t k) {
 }
 
 // superellipsoid::point(a, b): superellipsoid.hpp:292-300
-__device__ __attribute__((noinline)) void se_point(const SE& s, double a, double b, double* out) {
+__device__ inline void se_point(const SE& s, double a, double b, double* out) {
     const double ca = pm_cos(a), sa = pm_sin(a), cb = pm_cos(b), sb = pm_sin(b);
     double pt[3];
     pt[0] = s.r[0] * sgn(ca) * pm_pow(fabs(ca), s.n) * sgn(cb) * pm_pow(fabs(cb), s.e);
@@ -213,7 +213,7 @@ __device__ __attribute__((noinline)) void se_point(const SE& s, double a, double
 }
 
 // superellipsoid::normal(a, b): superellipsoid.hpp:314-325
-__device__ __attribute__((noinline)) void se_normal(const SE& s, double a, double b, double* out) {
+__device__ inline void se_normal(const SE& s, double a, double b, double* out) {
     const double ca = pm_cos(a), sa = pm_sin(a), cb = pm_cos(b), sb = pm_sin(b);
     double nn[3];
     nn[0] = s.r[1] * s.r[2] * sgn(ca) * pm_pow(fabs(ca), 2 - s.n) * sgn(cb) * pm_pow(fabs(cb), 2 - s.e);
@@ -419,7 +419,7 @@ __device__ inline void se_residual(const SE& s1, const SE& s2, const double* u, 
 constexpr int N = 4;
 constexpr int LR = N * (N + 1) / 2;
 
-__device__ __attribute__((noinline)) double enorm(int n, const double* x) {
+__device__ inline double enorm(int n, const double* x) {
     const double rdwarf = 3.834e-20, rgiant = 1.304e19;
     double s1 = 0, s2 = 0, s3 = 0, x1max = 0, x3max = 0;
     const double agiant = rgiant / n;
@@ -451,7 +451,7 @@ __device__ __attribute__((noinline)) double enorm(int n, const double* x) {
     return x3max * sqrt(s3);
 }
 
-__device__ __attribute__((noinline)) void qrfac(double* a, double* rdiag, double* acnorm) {
+__device__ inline void qrfac(double* a, double* rdiag, double* acnorm) {
     for (int j = 0; j < N; ++j) {
         acnorm[j] = enorm(N, a + N * j);
         rdiag[j] = acnorm[j];
@@ -473,7 +473,7 @@ __device__ __attribute__((noinline)) void qrfac(double* a, double* rdiag, double
     }
 }
 
-__device__ __attribute__((noinline)) void qform(double* q, double* wa) {
+__device__ inline void qform(double* q, double* wa) {
     for (int j = 1; j < N; ++j)
         for (int i = 0; i < j; ++i) q[i + N * j] = 0.0;
     for (int l = 0; l < N; ++l) {
@@ -494,7 +494,7 @@ __device__ __attribute__((noinline)) void qform(double* q, double* wa) {
     }
 }
 
-__device__ __attribute__((noinline)) void dogleg(const double* r, const double* diag, const double* qtb, double delta, double* x,
+__device__ inline void dogleg(const double* r, const double* diag, const double* qtb, double delta, double* x,
                               double* wa1, double* wa2) {
     const double epsmch = 2.220446049250313e-16;
     int jj = (N * (N + 1)) / 2 + 1;
@@ -565,7 +565,7 @@ __device__ __attribute__((noinline)) void dogleg(const double* r, const double* 
     for (int j = 0; j < N; ++j) x[j] = temp * wa1[j] + alpha * x[j];
 }
 
-__device__ __attribute__((noinline)) void r1updt(double* s, const double* u, double* v, double* w, bool& sing) {
+__device__ inline void r1updt(double* s, const double* u, double* v, double* w, bool& sing) {
     const double giant = 1.79769313486231570e308;
     const int m = N, n = N;
     int jj = (n * (2 * m - n + 1)) / 2 - (m - n);
@@ -640,7 +640,7 @@ __device__ __attribute__((noinline)) void r1updt(double* s, const double* u, dou
     if (s[jj - 1] == 0.0) sing = true;
 }
 
-__device__ __attribute__((noinline)) void r1mpyq(int m, int n, double* a, int lda, const double* v, const double* w) {
+__device__ inline void r1mpyq(int m, int n, double* a, int lda, const double* v, const double* w) {
     for (int nmj = 1; nmj <= n - 1; ++nmj) {
         const int j = n - nmj;
         double cs, sn;
@@ -674,147 +674,488 @@ __device__ __attribute__((noinline)) void r1mpyq(int m, int n, double* a, int ld
     }
 }
 
-// hybrd1 with the analytic Jacobian; returns info (1 converged, 2 maxfev, 3 xtol too
-// small, 4 no progress; 5 is folded into 4 as minpack.hpp:918-921 does).  The outer
-// (Jacobian) and inner (trial step) loops of hybrd are one loop here with a single
-// residual evaluation per pass (phase 0: F at the start, 1: F and J at x, 2: F at the
-// trial point), which keeps one copy of the residual code in the kernel; the
-// arithmetic is the loops' own, operation for operation.
-__device__ inline int hybrd1(const SE& s1, const SE& s2, double* x, int* nfev_out) {
+// hybrd1 with the analytic Jacobian (info 1 converged, 2 maxfev, 3 xtol too small, 4 no
+// progress; 5 is folded into 4 as minpack.hpp:918-921 does), as a resumable per-lane state:
+// hybrd's outer (Jacobian) and inner (trial step) loops become passes with one residual
+// evaluation each (phase 0: F at the start, 1: F and J at x, 2: F at the trial point), so a
+// lane can stop after any pass and take another pair; the arithmetic is the loops' own,
+// operation for operation.
+struct Hybrd {
+    double x[N], fvec[N], fjac[N * N], r[LR], qtf[N], wa1[N], wa2[N], wa3[N], wa4[N], Jrow[N * N], ftmp[N];
+    double delta, xnorm, fnorm, pnorm;
+    int info, nfev, iter, ncsuc, ncfail, nslow1, nslow2, phase;
+    bool jeval;
+};
+
+__device__ inline void hybrd_start(Hybrd& h, const double* u0) {
+    for (int k = 0; k < N; ++k) h.x[k] = u0[k];
+    h.delta = h.xnorm = h.fnorm = h.pnorm = 0.0;
+    h.info = h.nfev = 0;
+    h.iter = 1;
+    h.ncsuc = h.ncfail = h.nslow1 = h.nslow2 = 0;
+    h.phase = 0;
+    h.jeval = true;
+}
+
+// one pass; returns true when hybrd1 has ended (h.info, h.nfev, h.x hold its results)
+__device__ inline bool hybrd_pass(const SE& s1, const SE& s2, Hybrd& h) {
     const double xtol = 1.0e-10, factor = 100.0, epsmch = 2.220446049250313e-16;
     const int maxfev = 200 * (N + 1);
     const double p1 = 0.1, p5 = 0.5, p001 = 0.001, p0001 = 0.0001;
-    double diag[N] = {1.0, 1.0, 1.0, 1.0};
-    double fvec[N], fjac[N * N], r[LR], qtf[N], wa1[N], wa2[N], wa3[N], wa4[N];
-    double Jrow[N * N], ftmp[N];
-    int info = 0, nfev = 0;
+    const double diag[N] = {1.0, 1.0, 1.0, 1.0};  // mode 2: wa[0..n) = 1 (minpack.hpp:881-884)
     const int msum = N;
-    int iter = 1, ncsuc = 0, ncfail = 0, nslow1 = 0, nslow2 = 0;
-    double delta = 0.0, xnorm = 0.0, fnorm = 0.0, pnorm = 0.0;
-    bool jeval = true;
+    se_residual(s1, s2, h.phase == 2 ? h.wa2 : h.x, h.phase == 0 ? h.fvec : (h.phase == 1 ? h.ftmp : h.wa4),
+                h.phase == 1 ? h.Jrow : nullptr);
+    if (h.phase == 0) {
+        h.nfev = 1;
+        h.fnorm = enorm(N, h.fvec);
+        h.phase = 1;
+        return false;
+    }
+    if (h.phase == 1) {
+        // top of hybrd's outer loop: factor the Jacobian at x
+        h.jeval = true;
+        for (int i = 0; i < N; ++i)
+            for (int j = 0; j < N; ++j) h.fjac[i + N * j] = h.Jrow[N * i + j];
+        h.nfev += msum;
+        qrfac(h.fjac, h.wa1, h.wa2);
+        if (h.iter == 1) {
+            for (int j = 0; j < N; ++j) h.wa3[j] = diag[j] * h.x[j];
+            h.xnorm = enorm(N, h.wa3);
+            h.delta = factor * h.xnorm;
+            if (h.delta == 0.0) h.delta = factor;
+        }
+        for (int i = 0; i < N; ++i) h.qtf[i] = h.fvec[i];
+        for (int j = 0; j < N; ++j) {
+            if (h.fjac[j + N * j] != 0.0) {
+                double sum = 0.0;
+                for (int i = j; i < N; ++i) sum += h.fjac[i + N * j] * h.qtf[i];
+                const double temp = -sum / h.fjac[j + N * j];
+                for (int i = j; i < N; ++i) h.qtf[i] += h.fjac[i + N * j] * temp;
+            }
+        }
+        for (int j = 0; j < N; ++j) {
+            int l = j;
+            for (int i = 0; i < j; ++i) {
+                h.r[l] = h.fjac[i + N * j];
+                l += N - i - 1;
+            }
+            h.r[l] = h.wa1[j];
+        }
+        qform(h.fjac, h.wa1);
+    } else {
+        // inner loop, after the residual at the trial point
+        ++h.nfev;
+        const double fnorm1 = enorm(N, h.wa4);
+        double actred = -1.0;
+        if (fnorm1 < h.fnorm) actred = 1.0 - (fnorm1 / h.fnorm) * (fnorm1 / h.fnorm);
+        int l = 0;
+        for (int i = 0; i < N; ++i) {
+            double sum = 0.0;
+            for (int j = i; j < N; ++j) {
+                sum += h.r[l] * h.wa1[j];
+                ++l;
+            }
+            h.wa3[i] = h.qtf[i] + sum;
+        }
+        const double temp = enorm(N, h.wa3);
+        double prered = 0.0;
+        if (temp < h.fnorm) prered = 1.0 - (temp / h.fnorm) * (temp / h.fnorm);
+        double ratio = 0.0;
+        if (prered > 0.0) ratio = actred / prered;
+        if (ratio < p1) {
+            h.ncsuc = 0;
+            ++h.ncfail;
+            h.delta = p5 * h.delta;
+        } else {
+            h.ncfail = 0;
+            ++h.ncsuc;
+            if (ratio >= p5 || h.ncsuc > 1) h.delta = smax(h.delta, h.pnorm / p5);
+            if (fabs(ratio - 1.0) <= p1) h.delta = h.pnorm / p5;
+        }
+        if (ratio >= p0001) {
+            for (int j = 0; j < N; ++j) {
+                h.x[j] = h.wa2[j];
+                h.wa2[j] = diag[j] * h.x[j];
+                h.fvec[j] = h.wa4[j];
+            }
+            h.xnorm = enorm(N, h.wa2);
+            h.fnorm = fnorm1;
+            ++h.iter;
+        }
+        ++h.nslow1;
+        if (actred >= p001) h.nslow1 = 0;
+        if (h.jeval) ++h.nslow2;
+        if (actred >= p1) h.nslow2 = 0;
+        if (h.delta <= xtol * h.xnorm || h.fnorm == 0.0) h.info = 1;
+        if (h.info == 0) {
+            if (maxfev <= h.nfev) h.info = 2;
+            if (p1 * smax(p1 * h.delta, h.pnorm) <= epsmch * h.xnorm) h.info = 3;
+            if (h.nslow2 == 5) h.info = 4;
+            if (h.nslow1 == 10) h.info = 5;
+        }
+        if (h.info != 0) {
+            if (h.info == 5) h.info = 4;
+            return true;
+        }
+        if (h.ncfail == 2) {
+            h.phase = 1;  // back to the outer loop: a new Jacobian at x
+            return false;
+        }
+        for (int j = 0; j < N; ++j) {
+            double sum = 0.0;
+            for (int i = 0; i < N; ++i) sum += h.fjac[i + N * j] * h.wa4[i];
+            h.wa2[j] = (sum - h.wa3[j]) / h.pnorm;
+            h.wa1[j] = diag[j] * ((diag[j] * h.wa1[j]) / h.pnorm);
+            if (ratio >= p0001) h.qtf[j] = sum;
+        }
+        bool sg = false;
+        r1updt(h.r, h.wa1, h.wa2, h.wa3, sg);
+        r1mpyq(N, N, h.fjac, N, h.wa2, h.wa3);
+        r1mpyq(1, N, h.qtf, 1, h.wa2, h.wa3);
+        h.jeval = false;
+    }
+    // propose the next trial point (head of hybrd's inner loop)
+    dogleg(h.r, diag, h.qtf, h.delta, h.wa1, h.wa2, h.wa3);
+    for (int j = 0; j < N; ++j) {
+        h.wa1[j] = -h.wa1[j];
+        h.wa2[j] = h.x[j] + h.wa1[j];
+        h.wa3[j] = diag[j] * h.wa1[j];
+    }
+    h.pnorm = enorm(N, h.wa3);
+    if (h.iter == 1) h.delta = smin(h.delta, h.pnorm);
+    h.phase = 2;
+    return false;
+}
+
+// ------------------------------------------------------------------ sphere / plane against a superellipsoid
+// closest_points(sphere<3>, superellipsoid<3>) (closest_points.hpp:944-1281) and
+// closest_points(superellipsoid<3>, plane<3>) (:1478-1746): damped Newton (utils.hpp:81-118,
+// 200 iterations, ftol 1e-10, xtol 1e-7) from the first minimum of a 10 x 10 distance table over
+// the diagonal (a_i, b_i) of linspace(-pi/2, pi/2, 10) x linspace(-pi, pi, 10).  s1 is the
+// superellipsoid; its normal is written sign(cos a1) superellipsoid::normal, and GradF is the chain
+// rule of F with the reference's partial derivatives: for the sphere pair the derivative of
+// |cos a1|^(2-n) lacks the sign(cos a1) factor (ca1n3, :1088), for the plane pair it has it (:1617).
+// The plane pair's `sign_p2s` is the dot product (c_s1 - c_p) . n_p itself (:1484-1493).
+struct Sph {
+    double c[3], r;
+    double R[3][3];
+};
+struct Pln {
+    double c[3];
+    double R[3][3];
+};
+
+__device__ inline void load_rot(const Particles& P, int64_t k, double (&R)[3][3]) {
+    const m3 M = rotation3(P.q[k], P.q[P.cap + k], P.q[2 * P.cap + k], P.q[3 * P.cap + k]);
+    for (int a = 0; a < 3; ++a)
+        for (int b = 0; b < 3; ++b) R[a][b] = M.a[a][b];
+}
+__device__ inline Sph load_sph(const Particles& P, int64_t k) {
+    Sph s;
+    for (int d = 0; d < 3; ++d) s.c[d] = P.pos[d * P.cap + k];
+    s.r = P.size[k];
+    load_rot(P, k, s.R);
+    return s;
+}
+__device__ inline Pln load_pln(const Particles& P, int64_t k) {
+    Pln p;
+    for (int d = 0; d < 3; ++d) p.c[d] = P.pos[d * P.cap + k];
+    load_rot(P, k, p.R);
+    return p;
+}
+
+// sphere::point / normal (a, b): sphere.hpp:203-234; plane::point(a, b) = R (0, a, b) + c
+__device__ inline void sph_point(const Sph& s, double a, double b, double* out) {
+    const double pt[3] = {s.r * pm_cos(a) * pm_cos(b), s.r * pm_cos(a) * pm_sin(b), s.r * pm_sin(a)};
+    for (int d = 0; d < 3; ++d) out[d] = (s.R[d][0] * pt[0] + s.R[d][1] * pt[1] + s.R[d][2] * pt[2]) + s.c[d];
+}
+__device__ inline void sph_normal(const Sph& s, double a, double b, double* out) {
+    const double nn[3] = {pm_cos(a) * pm_cos(b), pm_cos(a) * pm_sin(b), pm_sin(a)};
+    double w[3];
+    for (int d = 0; d < 3; ++d) w[d] = s.R[d][0] * nn[0] + s.R[d][1] * nn[1] + s.R[d][2] * nn[2];
+    const double l = sqrt(w[0] * w[0] + w[1] * w[1] + w[2] * w[2]);
+    for (int d = 0; d < 3; ++d) out[d] = w[d] / l;
+}
+__device__ inline void pln_point(const Pln& p, double a, double b, double* out) {
+    const double pt[3] = {0.0, a, b};
+    for (int d = 0; d < 3; ++d) out[d] = (p.R[d][0] * pt[0] + p.R[d][1] * pt[1] + p.R[d][2] * pt[2]) + p.c[d];
+}
+
+// transcendental terms of the superellipsoid side at (a1, b1): cos/sin of the four angles, and
+// the powers of cos a1, sin a1, cos b1, sin b1 with their derivative factors; the second power of
+// cos a1 is unsigned, its derivative factor with sign(cos a1) when `exact`.
+struct NtTerms {
+    double cs[8];
+    double pw[8], dp[8];
+};
+__device__ inline void nt_terms(const SE& s1, const double* u, bool exact, NtTerms& t, bool deriv) {
+    for (int m = 0; m < 4; ++m) {
+        t.cs[2 * m] = pm_cos(u[m]);
+        t.cs[2 * m + 1] = pm_sin(u[m]);
+    }
+    const double ex[4] = {s1.n, s1.n, s1.e, s1.e};
+    for (int f = 0; f < 4; ++f) {
+        const double x = t.cs[f];
+        double v[2], dv[2];
+        pow_pairs(x, ex[f], v, deriv ? dv : nullptr);
+        t.pw[2 * f] = sgn(x) * v[0];
+        t.pw[2 * f + 1] = (f == 0) ? v[1] : sgn(x) * v[1];
+        if (deriv) {
+            t.dp[2 * f] = dv[0];
+            t.dp[2 * f + 1] = dv[1];
+            if (f == 0 && exact) {  // (p sign(x)) |x|^(p-1) instead of p |x|^(p-1)
+                const double p = 2 - s1.n, ax = fabs(x);
+                t.dp[1] = (ax != ax || ax == 0 || ax == INFINITY) ? (p * sgn(x)) * pm_pow(ax, p - 1)
+                                                                 : (p * sgn(x)) * pm_exp((p - 1) * pm_log(ax));
+            }
+        }
+    }
+}
+
+// F of the sphere pair (kind 0) or the plane pair (kind 1); V = double (values) or D1 (direction k)
+template <class V>
+__device__ inline void nt_algebra(int kind, const SE& s1, const Sph& s2, const Pln& p2, double sp, const NtTerms& t,
+                                  const double* u, int k, V* F) {
+    V fn[8];
+    for (int m = 0; m < 4; ++m) {
+        if constexpr (sizeof(V) == sizeof(D1)) {
+            const double seed = (m == k) ? 1.0 : 0.0;
+            fn[2 * m] = D1{t.cs[2 * m], -t.cs[2 * m + 1] * seed};
+            fn[2 * m + 1] = D1{t.cs[2 * m + 1], t.cs[2 * m] * seed};
+        } else {
+            fn[2 * m] = cst<V>(t.cs[2 * m]);
+            fn[2 * m + 1] = cst<V>(t.cs[2 * m + 1]);
+        }
+    }
+    auto P = [&](int f, int e) -> V {
+        if constexpr (sizeof(V) == sizeof(D1))
+            return D1{t.pw[2 * f + e], t.dp[2 * f + e] * fn[f].d};
+        else
+            return t.pw[2 * f + e];
+    };
+    const V ca1n = P(0, 0), ca1n2 = P(0, 1), sa1n = P(1, 0), sa1n2 = P(1, 1);
+    const V cb1e = P(2, 0), cb1e2 = P(2, 1), sb1e = P(3, 0), sb1e2 = P(3, 1);
+    const double sgc = sgn(t.cs[0]);
+    const V A1 = ((s1.r[1] * s1.r[2]) * ca1n2) * cb1e2, A2 = ((s1.r[0] * s1.r[2]) * sb1e2) * ca1n2,
+            A3 = sgc * ((s1.r[0] * s1.r[1]) * sa1n2);
+    const V A4 = (s1.r[0] * ca1n) * cb1e, A5 = (s1.r[1] * sb1e) * ca1n, A6 = s1.r[2] * sa1n;
+    V n1[3], d[3];
+    for (int q = 0; q < 3; ++q) n1[q] = (s1.R[q][0] * A1 + s1.R[q][1] * A2) + s1.R[q][2] * A3;
+    V f3;
+    if (kind == 0) {
+        const V ca2 = fn[4], sa2 = fn[5], cb2 = fn[6], sb2 = fn[7];
+        V ca2n2;
+        if constexpr (sizeof(V) == sizeof(D1))
+            ca2n2 = D1{fabs(ca2.v), sgn(ca2.v) * ca2.d};
+        else
+            ca2n2 = fabs(ca2);
+        const double r = s2.r, sg2 = sgn(t.cs[4]);
+        const V B1 = (r * ca2) * cb2, B2 = (r * sb2) * ca2, B3 = r * sa2;
+        const V B4 = ((r * r) * ca2n2) * cb2, B5 = ((r * r) * sb2) * ca2n2, B6 = sg2 * ((r * r) * sa2);
+        V n2[3];
+        for (int q = 0; q < 3; ++q) {
+            n2[q] = (s2.R[q][0] * B4 + s2.R[q][1] * B5) + s2.R[q][2] * B6;
+            d[q] = (((((((-(s1.R[q][0] * A4)) - s1.R[q][1] * A5) - s1.R[q][2] * A6) + s2.R[q][0] * B1) +
+                      s2.R[q][1] * B2) +
+                     s2.R[q][2] * B3) -
+                    cst<V>(s1.c[q])) +
+                   cst<V>(s2.c[q]);
+        }
+        const V dotn = (n1[0] * n2[0] + n1[1] * n2[1]) + n1[2] * n2[2];
+        const V l1 = dsqrt((n1[0] * n1[0] + n1[1] * n1[1]) + n1[2] * n1[2]);
+        const V l2 = dsqrt((n2[0] * n2[0] + n2[1] * n2[1]) + n2[2] * n2[2]);
+        f3 = dotn + l1 * l2;
+    } else {
+        V a2, b2;
+        if constexpr (sizeof(V) == sizeof(D1)) {
+            a2 = D1{u[2], k == 2 ? 1.0 : 0.0};
+            b2 = D1{u[3], k == 3 ? 1.0 : 0.0};
+        } else {
+            a2 = u[2];
+            b2 = u[3];
+        }
+        for (int q = 0; q < 3; ++q)
+            d[q] = ((((((-(s1.R[q][0] * A4)) - s1.R[q][1] * A5) - s1.R[q][2] * A6) + p2.R[q][1] * a2) +
+                     p2.R[q][2] * b2) +
+                    cst<V>(p2.c[q])) -
+                   cst<V>(s1.c[q]);
+        const V dotn = ((p2.R[0][0] * sp) * n1[0] + (p2.R[1][0] * sp) * n1[1]) + (p2.R[2][0] * sp) * n1[2];
+        const double lp = sqrt((p2.R[0][0] * p2.R[0][0] + p2.R[1][0] * p2.R[1][0]) + p2.R[2][0] * p2.R[2][0]);
+        const V l1 = dsqrt((n1[0] * n1[0] + n1[1] * n1[1]) + n1[2] * n1[2]);
+        f3 = dotn + lp * l1;
+    }
+    F[0] = (-(n1[1] * d[2])) + n1[2] * d[1];
+    F[1] = n1[0] * d[2] - n1[2] * d[0];
+    F[2] = (-(n1[0] * d[1])) + n1[1] * d[0];
+    F[3] = f3;
+}
+
+__device__ inline void nt_residual(int kind, const SE& s1, const Sph& s2, const Pln& p2, double sp, const double* u,
+                                   double* F, double* J) {
+    NtTerms t;
+    nt_terms(s1, u, kind == 1, t, J != nullptr);
+    if (!J) {
+        nt_algebra<double>(kind, s1, s2, p2, sp, t, u, -1, F);
+        return;
+    }
+#pragma unroll 1
+    for (int k = 0; k < 4; ++k) {
+        D1 f[4];
+        nt_algebra<D1>(kind, s1, s2, p2, sp, t, u, k, f);
+        for (int i = 0; i < 4; ++i) {
+            J[4 * i + k] = f[i].d;
+            if (k == 0) F[i] = f[i].v;
+        }
+    }
+}
+
+// xt::linalg::solve: LU with partial pivoting (first largest |pivot|), A row-major, b -> x
+__device__ __attribute__((noinline)) void lu_solve4(double* A, double* b) {
+    int piv[4];
+    for (int k = 0; k < 4; ++k) {
+        int p = k;
+        double best = fabs(A[4 * k + k]);
+        for (int i = k + 1; i < 4; ++i)
+            if (fabs(A[4 * i + k]) > best) {
+                best = fabs(A[4 * i + k]);
+                p = i;
+            }
+        piv[k] = p;
+        if (p != k)
+            for (int j = 0; j < 4; ++j) {
+                const double tmp = A[4 * k + j];
+                A[4 * k + j] = A[4 * p + j];
+                A[4 * p + j] = tmp;
+            }
+        if (A[4 * k + k] != 0.0) {
+            const double inv = 1.0 / A[4 * k + k];
+            for (int i = k + 1; i < 4; ++i) A[4 * i + k] *= inv;
+        }
+        for (int i = k + 1; i < 4; ++i)
+            for (int j = k + 1; j < 4; ++j) A[4 * i + j] -= A[4 * i + k] * A[4 * k + j];
+    }
+    for (int k = 0; k < 4; ++k)
+        if (piv[k] != k) {
+            const double tmp = b[k];
+            b[k] = b[piv[k]];
+            b[piv[k]] = tmp;
+        }
+    for (int i = 1; i < 4; ++i)
+        for (int j = 0; j < i; ++j) b[i] -= A[4 * i + j] * b[j];
+    for (int i = 3; i >= 0; --i) {
+        for (int j = i + 1; j < 4; ++j) b[i] -= A[4 * i + j] * b[j];
+        b[i] /= A[4 * i + i];
+    }
+}
+
+__device__ inline double norm4(const double* x) {
+    return sqrt(((x[0] * x[0] + x[1] * x[1]) + x[2] * x[2]) + x[3] * x[3]);
+}
+
+// newton_method (utils.hpp:81-118): iteration count, -1 at itermax.  One residual call
+// site per pass (phase 0: F and J at u, 1: F at the line-search point).
+__device__ inline int newton(int kind, const SE& s1, const Sph& s2, const Pln& p2, double sp, double* u) {
+    const int itermax = 200;
+    const double ftol = 1.0e-10, xtol = 1.0e-7;
+    int cc = 0;
+    double F[4], J[16], dd[4], ut[4], Ft[4], ferr = 0.0, t = 1.0;
     int phase = 0;
-    for (;;) {
-        se_residual(s1, s2, phase == 2 ? wa2 : x, phase == 0 ? fvec : (phase == 1 ? ftmp : wa4),
-                    phase == 1 ? Jrow : nullptr);
+    while (cc < itermax) {
+        nt_residual(kind, s1, s2, p2, sp, phase == 0 ? u : ut, phase == 0 ? F : Ft, phase == 0 ? J : nullptr);
         if (phase == 0) {
-            nfev = 1;
-            fnorm = enorm(N, fvec);
-            phase = 1;
+            for (int q = 0; q < 4; ++q) dd[q] = -F[q];
+            lu_solve4(J, dd);
+            if (norm4(dd) < xtol) return cc;
+            t = 1;
+            ferr = norm4(F);
+            if (ferr < ftol) return cc;
+        } else if (norm4(Ft) > ferr && t > 0.01) {
+            t -= 0.01;
+        } else {
+            for (int q = 0; q < 4; ++q) u[q] += t * dd[q];
+            cc += 1;
+            phase = 0;
             continue;
         }
-        if (phase == 1) {
-            // top of hybrd's outer loop: factor the Jacobian at x
-            jeval = true;
-            for (int i = 0; i < N; ++i)
-                for (int j = 0; j < N; ++j) fjac[i + N * j] = Jrow[N * i + j];
-            nfev += msum;
-            qrfac(fjac, wa1, wa2);
-            if (iter == 1) {
-                for (int j = 0; j < N; ++j) wa3[j] = diag[j] * x[j];
-                xnorm = enorm(N, wa3);
-                delta = factor * xnorm;
-                if (delta == 0.0) delta = factor;
-            }
-            for (int i = 0; i < N; ++i) qtf[i] = fvec[i];
-            for (int j = 0; j < N; ++j) {
-                if (fjac[j + N * j] != 0.0) {
-                    double sum = 0.0;
-                    for (int i = j; i < N; ++i) sum += fjac[i + N * j] * qtf[i];
-                    const double temp = -sum / fjac[j + N * j];
-                    for (int i = j; i < N; ++i) qtf[i] += fjac[i + N * j] * temp;
-                }
-            }
-            for (int j = 0; j < N; ++j) {
-                int l = j;
-                for (int i = 0; i < j; ++i) {
-                    r[l] = fjac[i + N * j];
-                    l += N - i - 1;
-                }
-                r[l] = wa1[j];
-            }
-            qform(fjac, wa1);
-        } else {
-            // inner loop, after the residual at the trial point
-            ++nfev;
-            const double fnorm1 = enorm(N, wa4);
-            double actred = -1.0;
-            if (fnorm1 < fnorm) actred = 1.0 - (fnorm1 / fnorm) * (fnorm1 / fnorm);
-            int l = 0;
-            for (int i = 0; i < N; ++i) {
-                double sum = 0.0;
-                for (int j = i; j < N; ++j) {
-                    sum += r[l] * wa1[j];
-                    ++l;
-                }
-                wa3[i] = qtf[i] + sum;
-            }
-            const double temp = enorm(N, wa3);
-            double prered = 0.0;
-            if (temp < fnorm) prered = 1.0 - (temp / fnorm) * (temp / fnorm);
-            double ratio = 0.0;
-            if (prered > 0.0) ratio = actred / prered;
-            if (ratio < p1) {
-                ncsuc = 0;
-                ++ncfail;
-                delta = p5 * delta;
-            } else {
-                ncfail = 0;
-                ++ncsuc;
-                if (ratio >= p5 || ncsuc > 1) delta = smax(delta, pnorm / p5);
-                if (fabs(ratio - 1.0) <= p1) delta = pnorm / p5;
-            }
-            if (ratio >= p0001) {
-                for (int j = 0; j < N; ++j) {
-                    x[j] = wa2[j];
-                    wa2[j] = diag[j] * x[j];
-                    fvec[j] = wa4[j];
-                }
-                xnorm = enorm(N, wa2);
-                fnorm = fnorm1;
-                ++iter;
-            }
-            ++nslow1;
-            if (actred >= p001) nslow1 = 0;
-            if (jeval) ++nslow2;
-            if (actred >= p1) nslow2 = 0;
-            if (delta <= xtol * xnorm || fnorm == 0.0) info = 1;
-            if (info != 0) break;
-            if (maxfev <= nfev) info = 2;
-            if (p1 * smax(p1 * delta, pnorm) <= epsmch * xnorm) info = 3;
-            if (nslow2 == 5) info = 4;
-            if (nslow1 == 10) info = 5;
-            if (info != 0) break;
-            if (ncfail == 2) {
-                phase = 1;  // back to the outer loop: a new Jacobian at x
-                continue;
-            }
-            for (int j = 0; j < N; ++j) {
-                double sum = 0.0;
-                for (int i = 0; i < N; ++i) sum += fjac[i + N * j] * wa4[i];
-                wa2[j] = (sum - wa3[j]) / pnorm;
-                wa1[j] = diag[j] * ((diag[j] * wa1[j]) / pnorm);
-                if (ratio >= p0001) qtf[j] = sum;
-            }
-            bool sg = false;
-            r1updt(r, wa1, wa2, wa3, sg);
-            r1mpyq(N, N, fjac, N, wa2, wa3);
-            r1mpyq(1, N, qtf, 1, wa2, wa3);
-            jeval = false;
-        }
-        // propose the next trial point (head of hybrd's inner loop)
-        dogleg(r, diag, qtf, delta, wa1, wa2, wa3);
-        for (int j = 0; j < N; ++j) {
-            wa1[j] = -wa1[j];
-            wa2[j] = x[j] + wa1[j];
-            wa3[j] = diag[j] * wa1[j];
-        }
-        pnorm = enorm(N, wa3);
-        if (iter == 1) delta = smin(delta, pnorm);
-        phase = 2;
+        for (int q = 0; q < 4; ++q) ut[q] = u[q] + t * dd[q];
+        phase = 1;
     }
-    *nfev_out = nfev;
-    if (info == 5) info = 4;
-    return info;
+    return -1;
+}
+
+__device__ inline double lin(double lo, double hi, int n, int i) { return lo + (hi - lo) / (n - 1) * i; }
+
+// first minimum of the 10 x 10 diagonal distance table; kind 0: sphere s2, 1: plane p2
+__device__ inline void diag_grid_start(int kind, const SE& s1, const Sph& s2, const Pln& p2, double* u0) {
+    const int num = 10;
+    double dmin = INFINITY;
+    int imin = 0, jmin = 0;
+    for (int i = 0; i < num; ++i) {
+        double p[3];
+        se_point(s1, lin(-kPi / 2, kPi / 2, num, i), lin(-kPi, kPi, num, i), p);
+        for (int j = 0; j < num; ++j) {
+            double q[3];
+            if (kind == 0)
+                sph_point(s2, lin(-kPi / 2, kPi / 2, num, j), lin(-kPi, kPi, num, j), q);
+            else
+                pln_point(p2, lin(-kPi / 2, kPi / 2, num, j), lin(-kPi, kPi, num, j), q);
+            const double dx = p[0] - q[0], dy = p[1] - q[1], dz = p[2] - q[2];
+            const double dist = sqrt(dx * dx + dy * dy + dz * dz);
+            if (dist < dmin) {
+                dmin = dist;
+                imin = i;
+                jmin = j;
+            }
+        }
+    }
+    u0[0] = lin(-kPi / 2, kPi / 2, num, imin);
+    u0[1] = lin(-kPi, kPi, num, imin);
+    u0[2] = lin(-kPi / 2, kPi / 2, num, jmin);
+    u0[3] = lin(-kPi, kPi, num, jmin);
+}
+
+// One sphere / plane - superellipsoid pair in dispatcher order: writes the record (pi, pj, nij,
+// dij) as the reference returns it for (object i, object j), the start and end points and the
+// Newton count.
+__device__ __attribute__((noinline)) void newton_pair(const Particles& P, int64_t i, int64_t j, double* pi, double* pj, double* nij,
+                                   double& dij, double* u0, double* u, int& its) {
+    const int si = P.shape[i], sj = P.shape[j];
+    const bool se_i = si == SCOPI_SHAPE_SUPERELLIPSOID;
+    const int64_t kse = se_i ? j : i;  // the other object
+    const int64_t ks = se_i ? i : j;
+    const int other = se_i ? sj : si;
+    const int kind = (other == SCOPI_SHAPE_SPHERE) ? 0 : 1;
+    // reference order: (sphere, SE) and (SE, plane) directly, the others swapped afterwards
+    const bool flip = (kind == 0) ? se_i : !se_i;
+    const SE s1 = load_se(P, ks);
+    Sph s2{};
+    Pln p2{};
+    double sp = 0.0;
+    if (kind == 0) {
+        s2 = load_sph(P, kse);
+    } else {
+        p2 = load_pln(P, kse);
+        sp = (s1.c[0] - p2.c[0]) * p2.R[0][0] + (s1.c[1] - p2.c[1]) * p2.R[1][0] + (s1.c[2] - p2.c[2]) * p2.R[2][0];
+    }
+    diag_grid_start(kind, s1, s2, p2, u0);
+    for (int q = 0; q < 4; ++q) u[q] = u0[q];
+    its = newton(kind, s1, s2, p2, sp, u);
+    double a[3], b[3], n[3];
+    se_point(s1, u[0], u[1], a);
+    if (kind == 0) {
+        sph_point(s2, u[2], u[3], b);
+        sph_normal(s2, u[2], u[3], n);
+    } else {
+        pln_point(p2, u[2], u[3], b);
+        for (int q = 0; q < 3; ++q) n[q] = p2.R[q][0];
+    }
+    dij = (a[0] - b[0]) * n[0] + (a[1] - b[1]) * n[1] + (a[2] - b[2]) * n[2];
+    for (int q = 0; q < 3; ++q) {
+        pi[q] = flip ? b[q] : a[q];
+        pj[q] = flip ? a[q] : b[q];
+        nij[q] = flip ? n[q] * -1. : n[q];
+    }
 }
 
 // ------------------------------------------------------------------ kernels
@@ -948,44 +1289,80 @@ __global__ void __launch_bounds__(kSeGridBlock) k_se_grid(Particles P, SeGrids G
     }
 }
 
-// hybrd1 from the grid start and the contact record: one lane per candidate pair.
-__global__ void __launch_bounds__(kBlock) k_se_solve(Particles P, SeCandidates S, int64_t n) {
-    const int64_t c = (int64_t)blockIdx.x * kBlock + threadIdx.x;
-    if (c >= n) return;
-    const int64_t i = S.i[c], j = S.j[c];
-    if (!is_se(P, i) || !is_se(P, j)) {
-        CP cp;
-        const bool ok = closest_points<3>(P, i, j, cp);
-        S.dij[c] = ok ? cp.dij : INFINITY;
-        const d3 z = mk(0.0, 0.0, 0.0);
-        store3(S.nij, S.cap, c, ok ? cp.nij : z);
-        store3(S.pi, S.cap, c, ok ? cp.pi : z);
-        store3(S.pj, S.cap, c, ok ? cp.pj : z);
-        S.info[c] = -1;
-        S.nfev[c] = 0;
-        return;
-    }
-    const SE s1 = load_se(P, i), s2 = load_se(P, j);
-    double x[4];
-    for (int k = 0; k < 4; ++k) x[k] = S.u0[4 * c + k];
-    int nfev = 0;
-    const int info = hybrd1(s1, s2, x, &nfev);
-    double pi[3], pj[3], nij[3];
-    se_point(s1, x[0], x[1], pi);
-    se_point(s2, x[2], x[3], pj);
-    se_normal(s2, x[2], x[3], nij);
-    const double dx = pi[0] - pj[0], dy = pi[1] - pj[1], dz = pi[2] - pj[2];
-    const double sd = sgn(dx * nij[0] + dy * nij[1] + dz * nij[2]);
-    S.dij[c] = sd * sqrt(dx * dx + dy * dy + dz * dz);
+// Closest points of every candidate: a work queue.  Each lane takes the next pair index from a
+// device counter; superellipsoid pairs run hybrd1 pass by pass, and a lane whose pair ends takes
+// another at once, so lanes of a wavefront are not held by the pair with the most evaluations.
+// Sphere / plane pairs with a superellipsoid run their Newton solve, other pairs their closed form.
+__device__ inline void write_record(SeCandidates& S, int64_t c, double dij, const double* nij, const double* pi,
+                                    const double* pj) {
+    S.dij[c] = dij;
     for (int d = 0; d < 3; ++d) {
         S.nij[d * S.cap + c] = nij[d];
         S.pi[d * S.cap + c] = pi[d];
         S.pj[d * S.cap + c] = pj[d];
     }
-    for (int k = 0; k < 4; ++k) S.u[4 * c + k] = x[k];
-    S.info[c] = info;
-    S.nfev[c] = nfev;
 }
+
+constexpr int kSolveBlock = 64;
+__global__ void __launch_bounds__(kSolveBlock) k_se_solve(Particles P, SeCandidates S, int64_t n) {
+    SE s1, s2;
+    Hybrd h;
+    int64_t c = atomicAdd(S.counter, 1);
+    bool busy = false;
+    for (;;) {
+        if (!busy) {
+            if (c >= n) break;
+            const int64_t i = S.i[c], j = S.j[c];
+            const bool se_i = is_se(P, i), se_j = is_se(P, j);
+            if (se_i && se_j) {
+                s1 = load_se(P, i);
+                s2 = load_se(P, j);
+                hybrd_start(h, S.u0 + 4 * c);
+                busy = true;
+            } else {
+                double pi[3], pj[3], nij[3];
+                if (se_i != se_j) {
+                    double u0[4], u[4], dij;
+                    int its;
+                    newton_pair(P, i, j, pi, pj, nij, dij, u0, u, its);
+                    write_record(S, c, dij, nij, pi, pj);
+                    for (int k = 0; k < 4; ++k) {
+                        S.u0[4 * c + k] = u0[k];
+                        S.u[4 * c + k] = u[k];
+                    }
+                    S.info[c] = its;
+                } else {
+                    CP cp;
+                    const bool ok = closest_points<3>(P, i, j, cp);
+                    const d3 z = mk(0.0, 0.0, 0.0);
+                    const d3 a = ok ? cp.nij : z, b = ok ? cp.pi : z, e = ok ? cp.pj : z;
+                    const double na[3] = {a.x, a.y, a.z}, pa[3] = {b.x, b.y, b.z}, pb[3] = {e.x, e.y, e.z};
+                    write_record(S, c, ok ? cp.dij : INFINITY, na, pa, pb);
+                    S.info[c] = -1;
+                }
+                S.nfev[c] = 0;
+                c = atomicAdd(S.counter, 1);
+                continue;
+            }
+        }
+        if (!hybrd_pass(s1, s2, h)) continue;
+        // hybrd1 ended: the contact record of closest_points.hpp:724-728
+        double pi[3], pj[3], nij[3];
+        se_point(s1, h.x[0], h.x[1], pi);
+        se_point(s2, h.x[2], h.x[3], pj);
+        se_normal(s2, h.x[2], h.x[3], nij);
+        const double dx = pi[0] - pj[0], dy = pi[1] - pj[1], dz = pi[2] - pj[2];
+        const double sd = sgn(dx * nij[0] + dy * nij[1] + dz * nij[2]);
+        write_record(S, c, sd * sqrt(dx * dx + dy * dy + dz * dz), nij, pi, pj);
+        for (int k = 0; k < 4; ++k) S.u[4 * c + k] = h.x[k];
+        S.info[c] = h.info;
+        S.nfev[c] = h.nfev;
+        busy = false;
+        c = atomicAdd(S.counter, 1);
+    }
+}
+
+__global__ void k_zero_counter(int* counter) { *counter = 0; }
 
 }  // namespace
 
@@ -1008,7 +1385,7 @@ void se_grids_reserve(SeGrids& G, int64_t n) {
 
 void se_candidates_free(SeCandidates& S) {
     for (void* p : {(void*)S.i, (void*)S.j, (void*)S.u0, (void*)S.u, (void*)S.info, (void*)S.nfev, (void*)S.dij,
-                    (void*)S.nij, (void*)S.pi, (void*)S.pj})
+                    (void*)S.nij, (void*)S.pi, (void*)S.pj, (void*)S.counter})
         if (p) (void)hipFree(p);
     S = SeCandidates{};
 }
@@ -1028,6 +1405,7 @@ void se_candidates_reserve(SeCandidates& S, int64_t n) {
     SCOPI_HIP_CHECK(hipMalloc(&S.nij, 3 * cap * sizeof(double)));
     SCOPI_HIP_CHECK(hipMalloc(&S.pi, 3 * cap * sizeof(double)));
     SCOPI_HIP_CHECK(hipMalloc(&S.pj, 3 * cap * sizeof(double)));
+    SCOPI_HIP_CHECK(hipMalloc(&S.counter, sizeof(int)));
 }
 
 void se_closest_points(const Particles& P, SeGrids& G, SeCandidates& S, int64_t n, hipStream_t s) {
@@ -1037,7 +1415,11 @@ void se_closest_points(const Particles& P, SeGrids& G, SeCandidates& S, int64_t 
     hipLaunchKernelGGL(k_se_angles, dim3(grid_for(na)), dim3(kBlock), 0, s, P, G);
     hipLaunchKernelGGL(k_se_points, dim3(grid_for(na * kSeGridPoints)), dim3(kBlock), 0, s, P, G);
     hipLaunchKernelGGL(k_se_grid, dim3((unsigned)n), dim3(kSeGridBlock), 0, s, P, G, S, n);
-    hipLaunchKernelGGL(k_se_solve, dim3(grid_for(n)), dim3(kBlock), 0, s, P, S, n);
+    hipLaunchKernelGGL(k_zero_counter, dim3(1), dim3(1), 0, s, S.counter);
+    // enough lanes to fill the device at the solve kernel's occupancy; the queue balances them
+    const int64_t lanes = std::min<int64_t>(n, 2048 * kSolveBlock);
+    hipLaunchKernelGGL(k_se_solve, dim3((unsigned)((lanes + kSolveBlock - 1) / kSolveBlock)), dim3(kSolveBlock), 0, s,
+                       P, S, n);
     SCOPI_HIP_CHECK(hipGetLastError());
 }
 

@@ -9,9 +9,11 @@
 //   k_se_grid     one workgroup per candidate pair: first minimum, in row-major order,
 //                 of |p1 - p2| + 2 (1 + n1.n2) over the 576 x 576 point pairs
 //                 (closest_points.hpp:604-690), body 2's grid staged in LDS;
-//   k_se_solve    one lane per candidate pair: MINPACK hybrd1 (minpack.hpp:851-923,
-//                 Powell hybrid, mode 2, factor 100, xtol 1e-10, maxfev 1000) on the 4x4
-//                 system of closest_points.hpp:282-382, then the contact record (:724-728).
+//   k_se_solve    a work queue over the candidate pairs, one pair per lane at a time:
+//                 MINPACK hybrd1 (minpack.hpp:851-923, Powell hybrid, mode 2, factor 100,
+//                 xtol 1e-10, maxfev 1000) on the 4x4 system of closest_points.hpp:282-382,
+//                 then the contact record (:724-728); sphere / plane pairs with a
+//                 superellipsoid run the reference's damped Newton (utils.hpp:81-118).
 // All arithmetic is FP64 without contraction, and cos/sin/asin/pow are the
 // portable functions of this file, so that the result is reproducible bit for bit
 // (the solve is chaotic at the rounding level: hybrd1 usually stops on "no
@@ -46,6 +48,7 @@ struct SeCandidates {
     double* nij = nullptr;    // 3 * cap, component-major
     double* pi = nullptr;
     double* pj = nullptr;
+    int* counter = nullptr;   // work queue of the solve kernel
 };
 
 void se_grids_reserve(SeGrids& G, int64_t n);

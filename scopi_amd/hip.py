@@ -28,6 +28,7 @@ EXPORTS = [
     "scopi_hip_set_a_priori_velocity", "scopi_hip_solve", "scopi_hip_download_solution",
     "scopi_hip_update_and_move", "scopi_hip_displacement_obstacles", "scopi_hip_run",
     "scopi_hip_set_kernel_timing", "scopi_hip_kernel_timing", "scopi_hip_set_tuning", "scopi_hip_se_closest_points",
+    "scopi_hip_closest_points",
 ]
 TUNE_REDUCE_BLOCKS, TUNE_BT_GROUP = 1, 2
 
@@ -94,6 +95,8 @@ def lib():
     L.scopi_hip_kernel_timing.argtypes = [P, D]
     L.scopi_hip_set_tuning.argtypes = [P, C.c_int32, C.c_int32]
     L.scopi_hip_se_closest_points.argtypes = [P, C.c_int64] + [D] * 12 + [C.c_void_p] * 4
+    L.scopi_hip_closest_points.argtypes = ([P, C.c_int64, I32, D, D, D, C.c_void_p, I32, D, D, D, C.c_void_p] +
+                                           [D] * 4 + [C.c_void_p] * 4)
     for name in [n for n in dir(L) if n.startswith("scopi_hip_") and n != "scopi_hip_last_error"]:
         getattr(L, name).restype = C.c_int
     _lib = L
@@ -238,6 +241,21 @@ class HipScene:
         u0 = np.zeros((m, 4)); u = np.zeros((m, 4)); info = np.zeros(m, np.int32); nfev = np.zeros(m, np.int32)
         self._check(self._L.scopi_hip_se_closest_points(self._h, n, *args, dij, nij, pi, pj, _ptr(u0), _ptr(u),
                                                         _ptr(info), _ptr(nfev)), "se_closest_points")
+        return dict(dij=dij[:n], nij=nij[:n], pi=pi[:n], pj=pj[:n], u0=u0[:n], u=u[:n], info=info[:n],
+                    nfev=nfev[:n])
+
+    def closest_points(self, shape_i, pos_i, q_i, size_i, sq_i, shape_j, pos_j, q_j, size_j, sq_j):
+        """closest_points_dispatcher::dispatch of n independent 3D pairs on the device."""
+        f = lambda a, w: np.ascontiguousarray(np.asarray(a, np.float64).reshape(-1, w))
+        si = np.ascontiguousarray(shape_i, np.int32); sj = np.ascontiguousarray(shape_j, np.int32)
+        n = len(si)
+        args = [f(pos_i, 3), f(q_i, 4), f(size_i, 3), f(sq_i, 2), f(pos_j, 3), f(q_j, 4), f(size_j, 3), f(sq_j, 2)]
+        m = max(n, 1)
+        dij = np.zeros(m); nij = np.zeros((m, 3)); pi = np.zeros((m, 3)); pj = np.zeros((m, 3))
+        u0 = np.zeros((m, 4)); u = np.zeros((m, 4)); info = np.zeros(m, np.int32); nfev = np.zeros(m, np.int32)
+        self._check(self._L.scopi_hip_closest_points(
+            self._h, n, si, args[0], args[1], args[2], _ptr(args[3]), sj, args[4], args[5], args[6], _ptr(args[7]),
+            dij, nij, pi, pj, _ptr(u0), _ptr(u), _ptr(info), _ptr(nfev)), "closest_points")
         return dict(dij=dij[:n], nij=nij[:n], pi=pi[:n], pj=pj[:n], u0=u0[:n], u=u[:n], info=info[:n],
                     nfev=nfev[:n])
 

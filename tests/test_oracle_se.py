@@ -238,3 +238,117 @@ def test_portable_log_pow(oracle_lib):
     assert pm("pow", 0.0, 0.5) == 0.0 and pm("pow", 0.0, 0.0) == 1.0 and pm("pow", 0.0, -0.5) == math.inf
     assert pm("pow", 1.0, 0.7) == 1.0 and pm("exp", 0.0) == 1.0 and pm("asin", 1.0) == math.pi / 2
     assert pm("cos", 0.0) == 1.0 and pm("sin", 0.0) == 0.0
+
+
+# ------------------------------------------------------------------ sphere / plane against a superellipsoid
+SPHERE, PLANE, SUPERELLIPSOID = 0, 1, 3
+
+
+def test_lu_solve_matches_lapack(oracle_lib):
+    rng = np.random.default_rng(2)
+    for _ in range(50):
+        A = rng.normal(size=(4, 4)); b = rng.normal(size=4)
+        x = oracle_lib.lu_solve4(A, b)
+        np.testing.assert_allclose(x, np.linalg.solve(A, b), rtol=1e-10, atol=1e-12)
+
+
+def random_se_sphere(rng):
+    q = rng.normal(size=4); q /= np.linalg.norm(q)
+    return (np.zeros(3), q, rng.uniform(0.008, 0.015, 3), rng.uniform(0.4, 1.6, 2),
+            np.array([0.03, 0.0, 0.0]) + rng.uniform(-0.005, 0.005, 3), quaternion(0.0), (0.01, 0, 0))
+
+
+def test_newton_jacobians(se):
+    """GradF of both Newton pairs = the derivative of F, except for the sphere pair's
+    |cos a1|^(2-n) terms, whose derivative the reference writes without sign(cos a1)
+    (closest_points.hpp:1088 ca1n3; the plane pair has it, :1617)."""
+    rng = np.random.default_rng(4)
+    h = 1e-7
+    for kind in (0, 1):
+        for _ in range(6):
+            ci, qi, ri, ei, cj, qj, rj = random_se_sphere(rng)
+            if kind == 1:
+                qj = quaternion(-math.pi / 2, (0.0, 1.0, 0.0))  # plane z = c, normal (0, 0, 1)
+                cj = np.array([0.0, 0.0, -0.03])
+            for cos_sign in (1, -1):
+                u = rng.uniform(-1.2, 1.2, 4)
+                u[0] = rng.uniform(0.2, 1.3) if cos_sign > 0 else rng.uniform(1.9, 2.9)  # sign of cos a1
+                F, J = se.newton_residual(kind, ci, qi, ri, ei, cj, qj, rj, u, jacobian=True)
+                fd = np.zeros((4, 4))
+                for k in range(4):
+                    e = np.zeros(4); e[k] = h
+                    fd[:, k] = (se.newton_residual(kind, ci, qi, ri, ei, cj, qj, rj, u + e) -
+                                se.newton_residual(kind, ci, qi, ri, ei, cj, qj, rj, u - e)) / (2 * h)
+                tol = 1e-5 * np.abs(fd).max()
+                # columns of (a2, b2) and of b1 are always exact
+                assert np.abs(J[:, 1:] - fd[:, 1:]).max() < tol
+                if kind == 1 or cos_sign > 0:
+                    assert np.abs(J[:, 0] - fd[:, 0]).max() < tol
+                else:
+                    assert np.abs(J[:, 0] - fd[:, 0]).max() > tol  # the reference's sign-less derivative
+
+
+def test_sphere_superellipsoid_known_answer(se):
+    """test_closest_points.cpp:1098-1127 (marked may_fail in the reference: Newton stops on
+    its 1e-7 step test short of the degenerate root): sphere (0.2, 0, 0) r 0.1 against the
+    ellipsoid (-0.2, 0, 0), axes (0.1, 0.2, 0.3): pi on the superellipsoid, pj on the sphere."""
+    q0 = quaternion(0.0)
+    r = se.newton_closest_points(SPHERE, (0.2, 0, 0), q0, (0.1, 0, 0), (1, 1), SUPERELLIPSOID, (-0.2, 0, 0), q0,
+                                 (0.1, 0.2, 0.3), (1, 1))
+    if r["iterations"] < 0:
+        # with the portable functions this degenerate start does not meet the step test in 200
+        # iterations; the end point still lies on both surfaces
+        assert abs(np.linalg.norm(r["pj"] - np.array([0.2, 0, 0])) - 0.1) < 1e-12
+        return
+    assert abs(r["dij"] - 0.2) < 1e-6
+    np.testing.assert_allclose(r["pi"], (-0.1, 0, 0), atol=1e-4)
+    np.testing.assert_allclose(r["pj"], (0.1, 0, 0), atol=1e-4)
+    np.testing.assert_allclose(r["nij"], (-1, 0, 0), atol=1e-3)
+    # the other order negates nij and swaps the points (closest_points.hpp:1295-1302)
+    s = se.newton_closest_points(SUPERELLIPSOID, (-0.2, 0, 0), q0, (0.1, 0.2, 0.3), (1, 1), SPHERE, (0.2, 0, 0), q0,
+                                 (0.1, 0, 0), (1, 1))
+    assert s["dij"] == r["dij"] and np.array_equal(s["nij"], -r["nij"])
+    assert np.array_equal(s["pi"], r["pj"]) and np.array_equal(s["pj"], r["pi"])
+
+
+def test_sphere_superellipsoid_sphere_case(oracle_lib):
+    """e = n = 1 with equal semi-axes: a sphere pair.  The reference's Newton (damped, 1e-7
+    step test, start on the 10-point diagonal) ends on both surfaces always; most converged
+    ends agree with the closed form to 2e-6 in the distance (1e-3 in the points, the step
+    test's reach on this degenerate root), and some land on another root of F (other
+    point pairs with anti-parallel normals), as the reference's may_fail tests record.
+    (C library functions: the reference's arithmetic.)"""
+    se = oracle_lib
+    se.set_se_math(False)
+    rng = np.random.default_rng(9)
+    conv = agree = 0
+    for _ in range(12):
+        ra, rb = rng.uniform(0.01, 0.02, 2)
+        d = rng.normal(size=3); d /= np.linalg.norm(d)
+        ca = np.zeros(3); cb = d * (ra + rb + rng.uniform(0.002, 0.02))
+        r = se.newton_closest_points(SPHERE, ca, quaternion(0.0), (ra, 0, 0), (1, 1), SUPERELLIPSOID, cb,
+                                     quaternion(0.0), (rb, rb, rb), (1, 1))
+        ref = se.closest_points(3, 0, ca, quaternion(0.0), (ra, 0, 0), 0, cb, quaternion(0.0), (rb, 0, 0))
+        assert abs(np.linalg.norm(r["pj"] - ca) - ra) < 1e-12 and abs(np.linalg.norm(r["pi"] - cb) - rb) < 1e-12
+        if r["iterations"] >= 0:
+            conv += 1
+            agree += (abs(r["dij"] - ref["dij"]) < 2e-6 and np.abs(r["pi"] - ref["pj"]).max() < 1e-3 and
+                      np.abs(r["pj"] - ref["pi"]).max() < 1e-3)
+    assert conv >= 6 and agree >= conv // 2 + 1
+
+
+def test_superellipsoid_plane_record(se):
+    """The plane pair's record: pj on the plane, nij the plane's normal (not oriented), dij = (pi - pj) . nij;
+    the other order negates nij and swaps the points (closest_points.hpp:1740-1768)."""
+    rng = np.random.default_rng(10)
+    qp = quaternion(-math.pi / 2, (0.0, 1.0, 0.0))
+    n = np.array([0.0, 0.0, 1.0])
+    for _ in range(5):
+        ci, qi, ri, ei, _, _, _ = random_se_sphere(rng)
+        cp = np.array([0.0, 0.0, -0.02])
+        r = se.newton_closest_points(SUPERELLIPSOID, ci, qi, ri, ei, PLANE, cp, qp, (0, 0, 0), (1, 1))
+        assert abs((r["pj"] - cp) @ n) < 1e-12 * max(1.0, np.abs(r["pj"]).max())
+        np.testing.assert_allclose(np.abs(r["nij"]), n, atol=1e-15)
+        assert abs(r["dij"] - (r["pi"] - r["pj"]) @ r["nij"]) <= 1e-12 * max(1.0, abs(r["dij"]))
+        s = se.newton_closest_points(PLANE, cp, qp, (0, 0, 0), (1, 1), SUPERELLIPSOID, ci, qi, ri, ei)
+        assert np.array_equal(s["nij"], -r["nij"]) and np.array_equal(s["pi"], r["pj"])
