@@ -184,6 +184,19 @@ int scopi_hip_closest_points(scopi_hip_ctx* ctx, int64_t n, const int32_t* shape
                              double* dij, double* nij, double* pi, double* pj, double* u0, double* u, int32_t* info,
                              int32_t* nfev);
 
+/* The device's portable elementary functions of the superellipsoid solves, elementwise
+ * on host arrays (tests): fn 0 cos, 1 sin, 2 asin, 3 log, 4 exp, 5 pow(x, y), x >= 0.
+ * No counterpart in the reference. */
+int scopi_hip_portable_math(scopi_hip_ctx* ctx, int32_t fn, int64_t n, const double* x, const double* y,
+                            double* out);
+
+/* Diagnostics of the sphere (kind 0) / plane (kind 1) - superellipsoid Newton solve on the
+ * device (tests): in[0..11] the superellipsoid (centre, q, semi-axes, e, n), in[12..19]
+ * the other object (centre, q, sphere radius), in[20..23] u, in[24..39] a row-major 4x4
+ * matrix A, in[40..43] b; out[0..3] F(u), out[4..19] the Jacobian (row-major),
+ * out[20..23] the solve of A x = b.  No counterpart in the reference. */
+int scopi_hip_newton_probe(scopi_hip_ctx* ctx, int32_t kind, const double* in, double* out);
+
 /* Launch shapes of the projected-gradient kernels (the hook the autotuner in
  * scopi_amd.tune drives; no counterpart in the reference):
  *   SCOPI_TUNE_REDUCE_BLOCKS: blocks of 256 threads of the per-contact kernels (1..2048, default 1024);

@@ -417,6 +417,46 @@ int scopi_hip_closest_points(scopi_hip_ctx* ctx, int64_t n, const int32_t* shape
     });
 }
 
+int scopi_hip_newton_probe(scopi_hip_ctx* ctx, int32_t kind, const double* in, double* out) {
+    return guarded(ctx, [&]() -> int {
+        if ((kind != 0 && kind != 1) || !in || !out) return fail(ctx, SCOPI_HIP_ERR_ARG, "newton_probe: bad argument");
+        double* d = nullptr;
+        SCOPI_HIP_CHECK(hipMalloc(&d, 72 * sizeof(double)));
+        struct Free {
+            double* p;
+            ~Free() { (void)hipFree(p); }
+        } f{d};
+        hipStream_t s = ctx->stream;
+        SCOPI_HIP_CHECK(hipMemcpyAsync(d, in, 44 * sizeof(double), hipMemcpyHostToDevice, s));
+        newton_probe(kind, d, d + 44, s);
+        SCOPI_HIP_CHECK(hipMemcpyAsync(out, d + 44, 24 * sizeof(double), hipMemcpyDeviceToHost, s));
+        SCOPI_HIP_CHECK(hipStreamSynchronize(s));
+        return SCOPI_HIP_OK;
+    });
+}
+
+int scopi_hip_portable_math(scopi_hip_ctx* ctx, int32_t fn, int64_t n, const double* x, const double* y,
+                            double* out) {
+    return guarded(ctx, [&]() -> int {
+        if (fn < 0 || fn > 5 || n < 0 || (n > 0 && (!x || !out || (fn == 5 && !y))))
+            return fail(ctx, SCOPI_HIP_ERR_ARG, "portable_math: bad argument");
+        if (n == 0) return SCOPI_HIP_OK;
+        double* d = nullptr;
+        SCOPI_HIP_CHECK(hipMalloc(&d, 3 * n * sizeof(double)));
+        struct Free {
+            double* p;
+            ~Free() { (void)hipFree(p); }
+        } f{d};
+        hipStream_t s = ctx->stream;
+        SCOPI_HIP_CHECK(hipMemcpyAsync(d, x, n * sizeof(double), hipMemcpyHostToDevice, s));
+        if (y) SCOPI_HIP_CHECK(hipMemcpyAsync(d + n, y, n * sizeof(double), hipMemcpyHostToDevice, s));
+        portable_math(fn, n, d, d + n, d + 2 * n, s);
+        SCOPI_HIP_CHECK(hipMemcpyAsync(out, d + 2 * n, n * sizeof(double), hipMemcpyDeviceToHost, s));
+        SCOPI_HIP_CHECK(hipStreamSynchronize(s));
+        return SCOPI_HIP_OK;
+    });
+}
+
 int scopi_hip_compute_contacts(scopi_hip_ctx* ctx, const scopi_hip_params* params, int64_t* n_contacts) {
     return guarded(ctx, [&]() -> int {
         if (!params_ok(params)) return fail(ctx, SCOPI_HIP_ERR_ARG, "compute_contacts: bad params");

@@ -1118,7 +1118,7 @@ __device__ inline void diag_grid_start(int kind, const SE& s1, const Sph& s2, co
 // One sphere / plane - superellipsoid pair in dispatcher order: writes the record (pi, pj, nij,
 // dij) as the reference returns it for (object i, object j), the start and end points and the
 // Newton count.
-__device__ __attribute__((noinline)) void newton_pair(const Particles& P, int64_t i, int64_t j, double* pi, double* pj, double* nij,
+__device__ inline void newton_pair(const Particles& P, int64_t i, int64_t j, double* pi, double* pj, double* nij,
                                    double& dij, double* u0, double* u, int& its) {
     const int si = P.shape[i], sj = P.shape[j];
     const bool se_i = si == SCOPI_SHAPE_SUPERELLIPSOID;
@@ -1320,18 +1320,12 @@ __global__ void __launch_bounds__(kSolveBlock) k_se_solve(Particles P, SeCandida
                 hybrd_start(h, S.u0 + 4 * c);
                 busy = true;
             } else {
+                if (se_i != se_j) {  // sphere / plane - superellipsoid: k_newton_pairs
+                    c = atomicAdd(S.counter, 1);
+                    continue;
+                }
                 double pi[3], pj[3], nij[3];
-                if (se_i != se_j) {
-                    double u0[4], u[4], dij;
-                    int its;
-                    newton_pair(P, i, j, pi, pj, nij, dij, u0, u, its);
-                    write_record(S, c, dij, nij, pi, pj);
-                    for (int k = 0; k < 4; ++k) {
-                        S.u0[4 * c + k] = u0[k];
-                        S.u[4 * c + k] = u[k];
-                    }
-                    S.info[c] = its;
-                } else {
+                {
                     CP cp;
                     const bool ok = closest_points<3>(P, i, j, cp);
                     const d3 z = mk(0.0, 0.0, 0.0);
@@ -1364,7 +1358,87 @@ __global__ void __launch_bounds__(kSolveBlock) k_se_solve(Particles P, SeCandida
 
 __global__ void k_zero_counter(int* counter) { *counter = 0; }
 
+// sphere / plane - superellipsoid candidates: the reference's damped Newton, one lane per pair
+__global__ void __launch_bounds__(kSolveBlock) k_newton_pairs(Particles P, SeCandidates S, int64_t n) {
+    const int64_t c = (int64_t)blockIdx.x * kSolveBlock + threadIdx.x;
+    if (c >= n) return;
+    const int64_t i = S.i[c], j = S.j[c];
+    if (is_se(P, i) == is_se(P, j)) return;
+    double pi[3], pj[3], nij[3], u0[4], u[4], dij;
+    int its;
+    newton_pair(P, i, j, pi, pj, nij, dij, u0, u, its);
+    write_record(S, c, dij, nij, pi, pj);
+    for (int k = 0; k < 4; ++k) {
+        S.u0[4 * c + k] = u0[k];
+        S.u[4 * c + k] = u[k];
+    }
+    S.info[c] = its;
+    S.nfev[c] = 0;
+}
+
+// the portable elementary functions, elementwise (tests)
+__global__ void __launch_bounds__(kBlock) k_portable_math(int fn, int64_t n, const double* __restrict__ x,
+                                                         const double* __restrict__ y, double* __restrict__ out) {
+    const int64_t k = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (k >= n) return;
+    const double a = x[k];
+    double r;
+    switch (fn) {
+        case 0: r = pm_cos(a); break;
+        case 1: r = pm_sin(a); break;
+        case 2: r = pm_asin(a); break;
+        case 3: r = pm_log(a); break;
+        case 4: r = pm_exp(a); break;
+        default: r = pm_pow(a, y[k]); break;
+    }
+    out[k] = r;
+}
+
+// residual / Jacobian of a Newton pair and the 4x4 solve, one thread (tests): kind 0 sphere, 1 plane;
+// in: SE (c, q, r, sq: 12 doubles), other (c, q, r: 8 doubles), u (4), A (16), b (4);
+// out: F (4), J (16), x (4)
+__global__ void k_newton_probe(int kind, const double* __restrict__ in, double* __restrict__ out) {
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    SE s1;
+    for (int d = 0; d < 3; ++d) {
+        s1.c[d] = in[d];
+        s1.r[d] = in[7 + d];
+    }
+    s1.e = in[10];
+    s1.n = in[11];
+    const m3 R1 = rotation3(in[3], in[4], in[5], in[6]);
+    const m3 R2 = rotation3(in[15], in[16], in[17], in[18]);
+    Sph s2{};
+    Pln p2{};
+    for (int a = 0; a < 3; ++a)
+        for (int b = 0; b < 3; ++b) {
+            s1.R[a][b] = R1.a[a][b];
+            s2.R[a][b] = p2.R[a][b] = R2.a[a][b];
+        }
+    for (int d = 0; d < 3; ++d) s2.c[d] = p2.c[d] = in[12 + d];
+    s2.r = in[19];
+    double sp = 0.0;
+    if (kind == 1)
+        sp = (s1.c[0] - p2.c[0]) * p2.R[0][0] + (s1.c[1] - p2.c[1]) * p2.R[1][0] + (s1.c[2] - p2.c[2]) * p2.R[2][0];
+    nt_residual(kind, s1, s2, p2, sp, in + 20, out, out + 4);
+    double A[16], b[4];
+    for (int k = 0; k < 16; ++k) A[k] = in[24 + k];
+    for (int k = 0; k < 4; ++k) b[k] = in[40 + k];
+    lu_solve4(A, b);
+    for (int k = 0; k < 4; ++k) out[20 + k] = b[k];
+}
+
 }  // namespace
+
+void newton_probe(int kind, const double* in, double* out, hipStream_t s) {
+    hipLaunchKernelGGL(k_newton_probe, dim3(1), dim3(64), 0, s, kind, in, out);
+    SCOPI_HIP_CHECK(hipGetLastError());
+}
+
+void portable_math(int fn, int64_t n, const double* x, const double* y, double* out, hipStream_t s) {
+    if (n > 0) hipLaunchKernelGGL(k_portable_math, dim3(grid_for(n)), dim3(kBlock), 0, s, fn, n, x, y, out);
+    SCOPI_HIP_CHECK(hipGetLastError());
+}
 
 // ------------------------------------------------------------------ host
 void se_grids_free(SeGrids& G) {
@@ -1415,6 +1489,8 @@ void se_closest_points(const Particles& P, SeGrids& G, SeCandidates& S, int64_t 
     hipLaunchKernelGGL(k_se_angles, dim3(grid_for(na)), dim3(kBlock), 0, s, P, G);
     hipLaunchKernelGGL(k_se_points, dim3(grid_for(na * kSeGridPoints)), dim3(kBlock), 0, s, P, G);
     hipLaunchKernelGGL(k_se_grid, dim3((unsigned)n), dim3(kSeGridBlock), 0, s, P, G, S, n);
+    hipLaunchKernelGGL(k_newton_pairs, dim3((unsigned)((n + kSolveBlock - 1) / kSolveBlock)), dim3(kSolveBlock), 0, s,
+                       P, S, n);
     hipLaunchKernelGGL(k_zero_counter, dim3(1), dim3(1), 0, s, S.counter);
     // enough lanes to fill the device at the solve kernel's occupancy; the queue balances them
     const int64_t lanes = std::min<int64_t>(n, 2048 * kSolveBlock);

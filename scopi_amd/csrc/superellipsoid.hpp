@@ -61,4 +61,12 @@ void se_candidates_free(SeCandidates& S);
 void se_closest_points(const Particles& P, SeGrids& G, SeCandidates& S, int64_t n, hipStream_t s);
 
 
+// The portable elementary functions on the device, elementwise (device arrays):
+// fn 0 cos, 1 sin, 2 asin, 3 log, 4 exp, 5 pow(x, y) (x >= 0).
+void portable_math(int fn, int64_t n, const double* x, const double* y, double* out, hipStream_t s);
+
+// Residual / Jacobian of a sphere (kind 0) or plane (kind 1) - superellipsoid pair and the
+// 4x4 solve of its Newton step, on the device (tests); layouts in superellipsoid.hip.
+void newton_probe(int kind, const double* in, double* out, hipStream_t s);
+
 }  // namespace scopi_hip

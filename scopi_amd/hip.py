@@ -28,7 +28,7 @@ EXPORTS = [
     "scopi_hip_set_a_priori_velocity", "scopi_hip_solve", "scopi_hip_download_solution",
     "scopi_hip_update_and_move", "scopi_hip_displacement_obstacles", "scopi_hip_run",
     "scopi_hip_set_kernel_timing", "scopi_hip_kernel_timing", "scopi_hip_set_tuning", "scopi_hip_se_closest_points",
-    "scopi_hip_closest_points",
+    "scopi_hip_closest_points", "scopi_hip_portable_math", "scopi_hip_newton_probe",
 ]
 TUNE_REDUCE_BLOCKS, TUNE_BT_GROUP = 1, 2
 
@@ -95,6 +95,8 @@ def lib():
     L.scopi_hip_kernel_timing.argtypes = [P, D]
     L.scopi_hip_set_tuning.argtypes = [P, C.c_int32, C.c_int32]
     L.scopi_hip_se_closest_points.argtypes = [P, C.c_int64] + [D] * 12 + [C.c_void_p] * 4
+    L.scopi_hip_portable_math.argtypes = [P, C.c_int32, C.c_int64, D, C.c_void_p, D]
+    L.scopi_hip_newton_probe.argtypes = [P, C.c_int32, D, D]
     L.scopi_hip_closest_points.argtypes = ([P, C.c_int64, I32, D, D, D, C.c_void_p, I32, D, D, D, C.c_void_p] +
                                            [D] * 4 + [C.c_void_p] * 4)
     for name in [n for n in dir(L) if n.startswith("scopi_hip_") and n != "scopi_hip_last_error"]:
@@ -258,6 +260,25 @@ class HipScene:
             dij, nij, pi, pj, _ptr(u0), _ptr(u), _ptr(info), _ptr(nfev)), "closest_points")
         return dict(dij=dij[:n], nij=nij[:n], pi=pi[:n], pj=pj[:n], u0=u0[:n], u=u[:n], info=info[:n],
                     nfev=nfev[:n])
+
+    def portable_math(self, fn, x, y=None):
+        """The device's portable elementary function `fn` (cos, sin, asin, log, exp, pow) elementwise."""
+        code = {"cos": 0, "sin": 1, "asin": 2, "log": 3, "exp": 4, "pow": 5}[fn]
+        x = np.ascontiguousarray(x, np.float64)
+        yy = None if y is None else np.ascontiguousarray(y, np.float64)
+        out = np.zeros(max(len(x), 1))
+        self._check(self._L.scopi_hip_portable_math(self._h, code, len(x), x if len(x) else np.zeros(1), _ptr(yy),
+                                                    out), "portable_math")
+        return out[:len(x)]
+
+    def newton_probe(self, kind, se, other, u, A, b):
+        """F, J at u of a sphere (0) / plane (1) - superellipsoid pair and the solve of A x = b, on the device.
+        se = (centre, q, semi-axes, (e, n)); other = (centre, q, radius)."""
+        v = np.concatenate([np.ravel(a) for a in (se[0], se[1], se[2], se[3], other[0], other[1], [other[2]], u,
+                                                  np.ravel(A), b)]).astype(np.float64)
+        out = np.zeros(24)
+        self._check(self._L.scopi_hip_newton_probe(self._h, kind, np.ascontiguousarray(v), out), "newton_probe")
+        return out[:4], out[4:20].reshape(4, 4), out[20:]
 
     def kernel_timing(self):
         out = np.zeros(6)

@@ -244,6 +244,9 @@ def test_newton_pairs_bit_exact(se_oracle, ctx):
     g = ctx.closest_points(*cols)
     for k in range(len(cols[0])):
         o = se_oracle.newton_closest_points(*(c[k] for c in cols))
+        if not np.array_equal(g["u"][k], o["u"], equal_nan=True):
+            say(f"  pair {k} shapes {cols[0][k]},{cols[5][k]}: u0 gpu {g['u0'][k]} oracle {o['u0']}; "
+                f"info gpu {g['info'][k]} oracle {o['iterations']}; u gpu {g['u'][k]} oracle {o['u']}")
         for key, gk in (("dij", g["dij"][k]), ("nij", g["nij"][k]), ("pi", g["pi"][k]), ("pj", g["pj"][k]),
                         ("u0", g["u0"][k]), ("u", g["u"][k])):
             assert np.array_equal(np.asarray(gk), np.asarray(o[key]), equal_nan=True), (k, key, gk, o[key])
@@ -287,3 +290,59 @@ def test_mixed_scene_contacts_bit_exact(se_oracle, method):
         assert np.array_equal(g[k], oc[k]), k
     kinds = {(int(sc.shape[i]), int(sc.shape[j])) for i, j in g["ij"]}
     say(f"  nc={nc} pair kinds={sorted(kinds)}")
+
+
+@pytest.mark.parametrize("fn", ["cos", "sin", "asin", "log", "exp", "pow"])
+def test_portable_math_bit_exact(se_oracle, ctx, fn):
+    """The device's elementary functions equal the oracle's portable ones bit for bit, over
+    ordinary, large, tiny and special arguments."""
+    rng = np.random.default_rng(6)
+    if fn in ("cos", "sin"):
+        x = np.concatenate([rng.uniform(-10, 10, 2000), rng.uniform(-1e13, 1e13, 500),
+                            rng.choice([-1, 1], 500) * 10.0 ** rng.uniform(13, 15, 500), 10.0 ** rng.uniform(-320, -1, 200),
+                            [0.0, -0.0, np.pi / 2, 1e15, -8.13671627e+14, 7.58242260e+12, np.inf, np.nan]])
+    elif fn == "asin":
+        x = np.concatenate([rng.uniform(-1, 1, 2000), [0.5, -0.5, 1.0, -1.0, 0.0, 1.5, np.nan]])
+    elif fn == "log":
+        x = np.concatenate([10.0 ** rng.uniform(-320, 308, 2500), [0.0, 1.0, np.inf, -1.0, np.nan, 5e-324]])
+    elif fn == "exp":
+        x = np.concatenate([rng.uniform(-750, 715, 2500), [0.0, -0.0, 709.78, 709.79, -745.1, -746.0, np.nan]])
+    else:
+        x = np.concatenate([10.0 ** rng.uniform(-320, 300, 2500), [0.0, 0.0, 0.0, 1.0, np.inf, np.nan]])
+    y = rng.uniform(-1.5, 2.6, len(x)) if fn == "pow" else None
+    if fn == "pow":
+        y[-6:-3] = [0.5, 0.0, -0.5]
+    g = ctx.portable_math(fn, x, y)
+    o = np.array([se_oracle.portable_math(fn, a, 0.0 if y is None else b) for a, b in
+                  zip(x, y if y is not None else np.zeros(len(x)))])
+    bad = np.flatnonzero(~((g == o) | (np.isnan(g) & np.isnan(o))))
+    assert bad.size == 0, [(x[k], None if y is None else y[k], g[k], o[k]) for k in bad[:5]]
+
+
+def test_newton_residual_and_solve_bit_exact(se_oracle, ctx):
+    """The device's residual, Jacobian and 4x4 solve of the Newton pairs equal the oracle's bit for bit
+    (ordinary and diverged, very large arguments)."""
+    rng = np.random.default_rng(8)
+    qp = quaternion(-math.pi / 2, (0.0, 1.0, 0.0))
+    bad = []
+    for t in range(60):
+        kind = t % 2
+        q = rng.normal(size=4); q /= np.linalg.norm(q)
+        se = (rng.uniform(-0.5, 0.5, 3), q, rng.uniform(0.008, 0.015, 3), rng.uniform(0.4, 1.6, 2))
+        if kind == 0:
+            qs = rng.normal(size=4); qs /= np.linalg.norm(qs)
+            other = (se[0] + np.array([0.03, 0, 0]), qs, rng.uniform(0.008, 0.015))
+        else:
+            other = (se[0] - np.array([0, 0, 0.03]), qp, 0.0)
+        u = rng.uniform(-3, 3, 4) * (10.0 ** rng.uniform(0, 14, 4) if t % 3 == 0 else 1.0)
+        A = rng.normal(size=(4, 4)); b = rng.normal(size=4)
+        F, J, x = ctx.newton_probe(kind, se, other, u, A, b)
+        Fo, Jo = se_oracle.newton_residual(kind, se[0], se[1], se[2], se[3], other[0], other[1], (other[2], 0, 0), u,
+                                           jacobian=True)
+        xo = se_oracle.lu_solve4(A, b)
+        for name, g, o in (("F", F, Fo), ("J", J, Jo), ("x", x, xo)):
+            if not np.array_equal(g, o, equal_nan=True):
+                bad.append((t, kind, name, u, g, o))
+    for b_ in bad[:4]:
+        say(f"  mismatch t={b_[0]} kind={b_[1]} {b_[2]} u={b_[3]}\n   gpu {b_[4]}\n   orc {b_[5]}")
+    assert not bad
