@@ -205,6 +205,47 @@ enum { SCOPI_TUNE_REDUCE_BLOCKS = 1, SCOPI_TUNE_BT_GROUP = 2 };
 int scopi_hip_set_tuning(scopi_hip_ctx* ctx, int32_t key, int32_t value);
 int scopi_hip_kernel_timing(scopi_hip_ctx* ctx, double* out);
 
+/* ---- Batches of independent scenes ------------------------------------------
+ * No counterpart in the reference, which steps one scene per ScopiSolver: a batch
+ * advances many ScopiSolver::run loops (solver.hpp:266-303) together, each with one
+ * scene's semantics (obstacles, contacts, vap, solve, update, move per step), sharing
+ * one set of launches per kernel.  Scenes are 3D, share the params and dt given at
+ * creation, have sphere active particles and sphere / plane obstacles.  A scene's
+ * results do not depend on the other scenes of the batch (bitwise: the same alone
+ * and in any mix).  Scene ids are returned by add_scene and stay valid until
+ * remove_scene; scenes added or removed take effect at the next call that reads or
+ * steps the batch.  The params must not ask for warm_start. */
+typedef struct scopi_hip_batch scopi_hip_batch;
+int scopi_hip_batch_create(const scopi_hip_params* params, double dt, scopi_hip_batch** out);
+int scopi_hip_batch_destroy(scopi_hip_batch* b);
+const char* scopi_hip_batch_last_error(const scopi_hip_batch* b);
+/* Arrays as for scopi_hip_upload_state (the scene's obstacles in [0, n_inactive)). */
+int scopi_hip_batch_add_scene(scopi_hip_batch* b, int64_t n, int64_t n_inactive, const int32_t* shape,
+                              const double* pos, const double* q, const double* size, const double* squareness,
+                              const double* v, const double* vd, const double* omega, const double* domega,
+                              const double* f, const double* m, const double* J, int64_t* scene_id);
+int scopi_hip_batch_remove_scene(scopi_hip_batch* b, int64_t scene_id);
+/* n_steps time steps of every scene of the batch. */
+int scopi_hip_batch_step(scopi_hip_batch* b, int64_t n_steps);
+/* Scenes, particles and contacts (last step) of the batch (any pointer may be NULL). */
+int scopi_hip_batch_size(scopi_hip_batch* b, int64_t* n_scenes, int64_t* n_particles, int64_t* n_contacts);
+/* One scene's state, in its own particle order (as scopi_hip_download_state). */
+int scopi_hip_batch_download_scene(scopi_hip_batch* b, int64_t scene_id, double* pos, double* q, double* v,
+                                   double* omega);
+/* Steps done by the scene and the solve info of its last step (either may be NULL). */
+int scopi_hip_batch_scene_info(scopi_hip_batch* b, int64_t scene_id, int64_t* steps, scopi_hip_solve_info* info);
+/* The scene's contacts of the last step, with the scene's own particle indices
+ * (as scopi_hip_download_contacts); *n_contacts is always written, arrays may be NULL. */
+int scopi_hip_batch_download_contacts(scopi_hip_batch* b, int64_t scene_id, int64_t* n_contacts, int64_t* i,
+                                      int64_t* j, double* dij, double* nij, double* pi, double* pj);
+/* The scene's u / omega / lambda of the last step (as scopi_hip_download_solution). */
+int scopi_hip_batch_download_solution(scopi_hip_batch* b, int64_t scene_id, double* u, double* omega,
+                                      double* lambda);
+/* Device bytes a scene of n particles (n_inactive obstacles) may hold in a batch
+ * with these params: the admission estimate of a serving loop (about 12 contacts
+ * per active particle, growth margins included). */
+int64_t scopi_hip_scene_footprint(int64_t n, int64_t n_inactive, const scopi_hip_params* params);
+
 #ifdef __cplusplus
 }
 #endif

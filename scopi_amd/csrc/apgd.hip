@@ -35,6 +35,13 @@ namespace scopi_hip {
 
 constexpr int kRedBlocksMax = 2048;
 constexpr int kPartialStride = 16;
+// Blocks of the per-contact kernels for nc contacts: one per kRowsPerBlock contacts, at most
+// `cap` (grid-stride beyond).  The same rule sizes a scene's share of a batch's grid, so a
+// scene solved in a batch runs the one-scene arithmetic exactly.
+constexpr int64_t kRowsPerBlock = 1024;
+inline int64_t row_blocks(int64_t nc, int cap) {
+    return std::min<int64_t>(std::max<int64_t>((nc + kRowsPerBlock - 1) / kRowsPerBlock, 1), cap);
+}
 constexpr int kStatusStalled = 2;                // FP32 phase reached its precision floor
 constexpr int kMaxConsecutiveRejections32 = 40;  // 2^40 growth of L without an accepted step
 
@@ -267,16 +274,40 @@ __device__ inline T* slot(T* base, int s, int64_t stride) {
 // {<g, l+ - y>, |l+ - y|^2, <g, l+ - l_k>, |l+ - l_k|^2, <y, Qy>, <c, y>, <c, l+>, |l+|^2}.
 __device__ inline void control_step(const ApgdCtrl* __restrict__ in, ApgdCtrl* __restrict__ out,
                                     const double* __restrict__ partial_row, int nrow, ApgdCtrl& st,
-                                    double (*lds)[kBlock / 64]);
+                                    double (*lds)[kBlock / 64], bool leader, int32_t* done_flag);
+
+// The block's scene and its share of the per-contact grid.
+struct SceneBlock {
+    int sc, lb, nb;      // scene, block index within the scene, blocks of the scene
+    int64_t lo, hi;      // the scene's contacts
+    int64_t pbase;       // first block sum of the scene
+};
+__device__ inline SceneBlock scene_block(const SceneMap& M, int64_t nc) {
+    SceneBlock b;
+    if (!M.rblk) {
+        b.sc = 0, b.lb = blockIdx.x, b.nb = gridDim.x, b.lo = 0, b.hi = nc, b.pbase = 0;
+        return b;
+    }
+    b.sc = M.rblk[blockIdx.x];
+    const int f = M.rfirst[b.sc];
+    b.lb = blockIdx.x - f;
+    b.nb = M.rcount[b.sc];
+    b.lo = M.crange[2 * b.sc];
+    b.hi = M.crange[2 * b.sc + 1];
+    b.pbase = f;
+    return b;
+}
 
 template <class T, int LAY>
 __global__ void __launch_bounds__(kBlock) k_trial(const ApgdCtrl* __restrict__ in, ApgdCtrl* __restrict__ out,
                                                  BMatrix<T> B, int64_t nc, double* __restrict__ partial,
-                                                 const double* __restrict__ partial_row, int nrow) {
+                                                 const double* __restrict__ partial_row, int nrow, SceneMap M) {
     constexpr int NR = lay<LAY>::NR, NC = lay<LAY>::NC;
     __shared__ double lds[8][kBlock / 64];
     __shared__ ApgdCtrl st;
-    control_step(in, out, partial_row, nrow, st, lds);
+    const SceneBlock sb = scene_block(M, nc);
+    control_step(in + sb.sc, out + sb.sc, partial_row + sb.pbase * kPartialStride, M.rblk ? sb.nb : nrow, st, lds,
+                 sb.lb == 0, M.done ? M.done + sb.sc : nullptr);
     if (st.done) return;
     const int cur = st.cur, prev = st.prev, trial = st.trial;
     const int64_t stride = NR * B.cap;
@@ -288,12 +319,14 @@ __global__ void __launch_bounds__(kBlock) k_trial(const ApgdCtrl* __restrict__ i
     T* lt = slot<T>(B.lam, trial, stride);
     double s[8] = {0, 0, 0, 0, 0, 0, 0, 0};
     // two consecutive contacts per lane and iteration: their 2 NR rows are loaded as
-    // NR aligned pairs (slots are even-sized, c0 is even)
+    // NR aligned pairs when the scene starts on an even contact (slots are even-sized)
     using V2 = typename std::conditional<sizeof(T) == 8, double2, float2>::type;
-    for (int64_t c0 = 2 * ((int64_t)blockIdx.x * kBlock + threadIdx.x); c0 < nc;
-         c0 += 2 * (int64_t)gridDim.x * kBlock) {
+    const int64_t hi = sb.hi;
+    const bool aligned = (sb.lo & 1) == 0;
+    for (int64_t c0 = sb.lo + 2 * ((int64_t)sb.lb * kBlock + threadIdx.x); c0 < hi; c0 += 2 * (int64_t)sb.nb * kBlock) {
         T l0[2 * NR], l1[2 * NR], q0[2 * NR], q1[2 * NR], cc[2 * NR];
-        if (c0 + 1 < nc) {
+        const int m = (c0 + 1 < hi) ? 2 : 1;
+        if (m == 2 && aligned) {
 #pragma unroll
             for (int h = 0; h < NR; ++h) {
                 const int64_t r = c0 * NR + 2 * h;
@@ -305,13 +338,11 @@ __global__ void __launch_bounds__(kBlock) k_trial(const ApgdCtrl* __restrict__ i
                 cc[2 * h] = g.x, cc[2 * h + 1] = g.y;
             }
         } else {
-#pragma unroll
-            for (int d = 0; d < NR; ++d) {
-                const int64_t r = c0 * NR + d;
-                l0[d] = lc[r], l1[d] = lp[r], q0[d] = qc[r], q1[d] = qp[r], cc[d] = B.c[r];
+            for (int o = 0; o < m * NR; ++o) {
+                const int64_t r = c0 * NR + o;
+                l0[o] = lc[r], l1[o] = lp[r], q0[o] = qc[r], q1[o] = qp[r], cc[o] = B.c[r];
             }
         }
-        const int m = (c0 + 1 < nc) ? 2 : 1;
         T zz[2 * NR];
         for (int k = 0; k < m; ++k) {
             const int64_t c = c0 + k;
@@ -361,7 +392,7 @@ __global__ void __launch_bounds__(kBlock) k_trial(const ApgdCtrl* __restrict__ i
                 s[7] += (double)z[d] * (double)z[d];
             }
         }
-        if (m == 2) {
+        if (m == 2 && aligned) {
 #pragma unroll
             for (int h = 0; h < NR; ++h) {
                 V2 o;
@@ -370,8 +401,7 @@ __global__ void __launch_bounds__(kBlock) k_trial(const ApgdCtrl* __restrict__ i
                 *reinterpret_cast<V2*>(lt + c0 * NR + 2 * h) = o;
             }
         } else {
-#pragma unroll
-            for (int d = 0; d < NR; ++d) lt[c0 * NR + d] = zz[d];
+            for (int o = 0; o < m * NR; ++o) lt[c0 * NR + o] = zz[o];
         }
     }
     block_sum<8>(s, lds);
@@ -389,21 +419,29 @@ __global__ void __launch_bounds__(kBlock) k_trial(const ApgdCtrl* __restrict__ i
 template <class T, int LAY, int G>
 __global__ void __launch_bounds__(kBlock) k_bt(ApgdCtrl* __restrict__ ctrl, int use_slot, BMatrix<T> B, int64_t na,
                                               const int* __restrict__ tstart, const int32_t* __restrict__ tc,
-                                              const double* __restrict__ partial_trial, int ntrial) {
+                                              const double* __restrict__ partial_trial, int ntrial, SceneMap M) {
     constexpr int NR = lay<LAY>::NR, NT = lay<LAY>::NT, NW = lay<LAY>::NW;
     __shared__ double lds[8][kBlock / 64];
+    const bool multi = M.bblk != nullptr;
+    const int sc = multi ? M.bblk[blockIdx.x] : 0;
+    const int lb = multi ? (int)blockIdx.x - M.bfirst[sc] : (int)blockIdx.x;
+    if (ctrl) ctrl += sc;
     if (ctrl && ctrl->done) return;
-    if (ctrl && partial_trial && blockIdx.x == 0) {
+    if (ctrl && partial_trial && lb == 0) {
         double tot[8];
-        reduce_partials<8>(partial_trial, ntrial, lds, tot);
+        if (multi)
+            reduce_partials<8>(partial_trial + (int64_t)M.rfirst[sc] * kPartialStride, M.rcount[sc], lds, tot);
+        else
+            reduce_partials<8>(partial_trial, ntrial, lds, tot);
         if (threadIdx.x == 0) {
             for (int k = 0; k < 7; ++k) ctrl->red[k] = tot[k];
             ctrl->red[8] = tot[7];
         }
     }
-    const int64_t p = ((int64_t)blockIdx.x * kBlock + threadIdx.x) / G;
+    const int64_t p0 = multi ? M.arange[2 * sc] : 0, p1 = multi ? M.arange[2 * sc + 1] : na;
+    const int64_t p = p0 + ((int64_t)lb * kBlock + threadIdx.x) / G;
     const int sub = (int)(threadIdx.x % G);
-    const bool valid = p < na;
+    const bool valid = p < p1;
     const int s = (use_slot >= 0) ? use_slot : ctrl->trial;
     const T* lam = slot<T>(B.lam, s, NR * B.cap);
     T acc[NW];
@@ -463,15 +501,16 @@ __global__ void __launch_bounds__(kBlock) k_bt(ApgdCtrl* __restrict__ ctrl, int 
 template <class T, int LAY>
 static void launch_bt(int G, hipStream_t s, ApgdCtrl* ctrl, int use_slot, const BMatrix<T>& B, int64_t na,
                       const int* tstart, const int32_t* tc, const double* partial, int ntrial,
-                      hipEvent_t e0 = nullptr, hipEvent_t e1 = nullptr) {
-    const unsigned grid = grid_for(na * (int64_t)G);
+                      hipEvent_t e0 = nullptr, hipEvent_t e1 = nullptr, const SceneMap& M = SceneMap{},
+                      int grid_blocks = 0) {
+    const unsigned grid = M.bblk ? (unsigned)grid_blocks : grid_for(na * (int64_t)G);
     auto go = [&](auto kernel) {
         if (e0)  // start / stop events taken from the kernel's own dispatch
             hipExtLaunchKernelGGL(kernel, dim3(grid), dim3(kBlock), 0, s, e0, e1, 0, ctrl, use_slot, B, na, tstart, tc,
-                                  partial, ntrial);
+                                  partial, ntrial, M);
         else
             hipLaunchKernelGGL(kernel, dim3(grid), dim3(kBlock), 0, s, ctrl, use_slot, B, na, tstart, tc, partial,
-                               ntrial);
+                               ntrial, M);
     };
     switch (G) {
         case 1: go(k_bt<T, LAY, 1>); break;
@@ -513,18 +552,20 @@ __device__ inline void row_product(const BMatrix<T>& B, int64_t c, int64_t i, in
 template <class T, int LAY>
 __global__ void __launch_bounds__(kBlock) k_row(ApgdCtrl* __restrict__ ctrl, BMatrix<T> B, int64_t nc,
                                                const int32_t* __restrict__ ci, const int32_t* __restrict__ cj,
-                                               int64_t ni, double* __restrict__ partial) {
+                                               int64_t ni, double* __restrict__ partial, SceneMap M) {
     constexpr int NR = lay<LAY>::NR;
     __shared__ double lds[1][kBlock / 64];
+    const SceneBlock sb = scene_block(M, nc);
+    ctrl += sb.sc;
     if (ctrl->done) return;
     const int trial = ctrl->trial;
     const T dt2 = (T)ctrl->dt2;
     const int64_t stride = NR * B.cap;
     const T* lt = slot<T>(B.lam, trial, stride);
     T* qt = slot<T>(B.qlam, trial, stride);
-    if (blockIdx.x == 0 && threadIdx.x == 0) ctrl->pending = 1;
+    if (sb.lb == 0 && threadIdx.x == 0) ctrl->pending = 1;
     double s[1] = {0.0};
-    for (int64_t c = (int64_t)blockIdx.x * kBlock + threadIdx.x; c < nc; c += (int64_t)gridDim.x * kBlock) {
+    for (int64_t c = sb.lo + (int64_t)sb.lb * kBlock + threadIdx.x; c < sb.hi; c += (int64_t)sb.nb * kBlock) {
         const int64_t i = (int64_t)ci[c] - ni, j = (int64_t)cj[c] - ni;
         T q[3];
         row_product<T, LAY>(B, c, i, j, dt2, q);
@@ -642,7 +683,7 @@ __device__ inline void apgd_control(ApgdCtrl* ctrl) {
 // and block 0 writes it to `out`.
 __device__ inline void control_step(const ApgdCtrl* __restrict__ in, ApgdCtrl* __restrict__ out,
                                     const double* __restrict__ partial_row, int nrow, ApgdCtrl& st,
-                                    double (*lds)[kBlock / 64]) {
+                                    double (*lds)[kBlock / 64], bool leader, int32_t* done_flag) {
     if (threadIdx.x == 0) st = *in;
     __syncthreads();
     if (st.pending && !st.done) {
@@ -655,15 +696,25 @@ __device__ inline void control_step(const ApgdCtrl* __restrict__ in, ApgdCtrl* _
     }
     if (threadIdx.x == 0) st.pending = 0;
     __syncthreads();
-    if (blockIdx.x == 0 && threadIdx.x == 0) *out = st;
+    if (leader && threadIdx.x == 0) {
+        *out = st;
+        if (done_flag) *done_flag = st.done;
+    }
 }
 
-// The control step alone (end of a batch of rounds, before the host reads the state).
+// The control step alone (end of a batch of rounds, before the host reads the state);
+// with scenes, one block per scene of the grid.
 __global__ void __launch_bounds__(kBlock) k_ctrl(const ApgdCtrl* __restrict__ in, ApgdCtrl* __restrict__ out,
-                                                const double* __restrict__ partial_row, int nrow) {
+                                                const double* __restrict__ partial_row, int nrow, SceneMap M) {
     __shared__ double lds[1][kBlock / 64];
     __shared__ ApgdCtrl st;
-    control_step(in, out, partial_row, nrow, st, lds);
+    if (!M.rblk) {
+        control_step(in, out, partial_row, nrow, st, lds, true, nullptr);
+        return;
+    }
+    const int sc = M.slist[blockIdx.x];
+    control_step(in + sc, out + sc, partial_row + (int64_t)M.rfirst[sc] * kPartialStride, M.rcount[sc], st, lds, true,
+                 M.done + sc);
 }
 
 template <class T>
@@ -711,6 +762,71 @@ __global__ void k_ctrl_init(ApgdCtrl* ctrl, const ApgdCtrl* last, double alpha, 
     ctrl->method = method;
     ctrl->dynamic = dynamic;
     ctrl->pending = 0;
+}
+
+// Initial state of every scene in ctrl[0 .. K); with keep_L, from the scene's final
+// state of the previous phase, ctrl[last_buf[sc] * K + sc] (L, and the iteration
+// budget left).  A scene without contacts is done at once.
+__global__ void k_ctrl_init_scenes(ApgdCtrl* ctrl, const int32_t* __restrict__ last_buf, int K, double alpha0,
+                                   double L0, double tol, double rtol, double dt2, int64_t max_ite0, int method,
+                                   int dynamic, int keep_L, int low_precision, const int64_t* __restrict__ crange,
+                                   int32_t* __restrict__ done) {
+    const int sc = blockIdx.x * blockDim.x + threadIdx.x;
+    if (sc >= K) return;
+    double L = L0, alpha = alpha0;
+    int64_t max_ite = max_ite0;
+    if (keep_L) {
+        const ApgdCtrl& last = ctrl[(int64_t)last_buf[sc] * K + sc];
+        const double L_last = last.L_acc;
+        if (isfinite(L_last) && L_last > 0) {
+            L = L_last;
+            alpha = 1. / L;
+        }
+        max_ite = max_ite0 - last.iter;
+        if (max_ite < 1) max_ite = 1;
+    }
+    ApgdCtrl c{};
+    c.L_acc = L;
+    c.consec_bt = 0;
+    c.low_precision = low_precision;
+    c.L = L;
+    c.alpha = alpha;
+    c.theta = 1.0;
+    c.beta = 0.0;
+    c.tol = tol;
+    c.rtol = rtol;
+    c.dt2 = dt2;
+    c.res = 0.0;
+    for (int k = 0; k < 9; ++k) c.red[k] = 0.0;
+    c.iter = 0;
+    c.backtracks = 0;
+    c.max_ite = max_ite;
+    c.q_applies = 0;
+    c.cur = 0;
+    c.prev = 1;
+    c.trial = 2;
+    const bool empty = crange[2 * sc + 1] <= crange[2 * sc];
+    c.done = (empty || max_ite <= 0) ? 1 : 0;
+    c.status = (!empty && max_ite <= 0) ? SCOPI_HIP_NOT_CONVERGED : 0;
+    if (!c.done) c.res = INFINITY;
+    c.method = method;
+    c.dynamic = dynamic;
+    c.pending = 0;
+    ctrl[sc] = c;
+    done[sc] = c.done;
+}
+
+// dst slot 0 <- src slot src_slot[scene], row by row over every scene's contacts
+// (per-contact grid of the scene map); a copy onto itself is skipped.
+template <class TA, class TB, int NR>
+__global__ void __launch_bounds__(kBlock) k_scene_slot(const TA* __restrict__ src, int64_t src_stride, TB* dst,
+                                                      const int32_t* __restrict__ src_slot, SceneMap M) {
+    const SceneBlock sb = scene_block(M, 0);
+    const int sl = src_slot[sb.sc];
+    if (std::is_same<TA, TB>::value && sl == 0 && (const void*)src == (const void*)dst) return;
+    const TA* a = src + (int64_t)sl * src_stride;
+    for (int64_t r = NR * sb.lo + (int64_t)sb.lb * kBlock + threadIdx.x; r < NR * sb.hi; r += (int64_t)sb.nb * kBlock)
+        dst[r] = (TB)a[r];
 }
 
 // u = v + dt M^-1 B^T lambda, omega likewise (OptimGradient.hpp:267-284); with no
@@ -764,6 +880,10 @@ Solver::~Solver() {
                     (void*)partial_trial_, (void*)partial_row_})
         if (p) (void)hipFree(p);
     if (h_ctrl_) (void)hipHostFree(h_ctrl_);
+    for (void* p : {(void*)d_arange_, (void*)d_done_, (void*)d_buf_, (void*)d_tab_, (void*)mctrl_})
+        if (p) (void)hipFree(p);
+    for (void* p : {(void*)h_done_, (void*)h_tab_, (void*)h_ctrls_})
+        if (p) (void)hipHostFree(p);
 }
 
 void Solver::reserve(int64_t na, int64_t nc, bool fp32) {
@@ -773,6 +893,7 @@ void Solver::reserve(int64_t na, int64_t nc, bool fp32) {
         SCOPI_HIP_CHECK(hipHostMalloc(&h_ctrl_, sizeof(ApgdCtrl)));
         SCOPI_HIP_CHECK(hipMalloc(&partial_trial_, kRedBlocksMax * kPartialStride * sizeof(double)));
         SCOPI_HIP_CHECK(hipMalloc(&partial_row_, kRedBlocksMax * kPartialStride * sizeof(double)));
+        partial_cap_ = kRedBlocksMax;
     }
     const bool grow_p = na + 2 > pcap_, grow_c = nc + 1 > ccap_;
     if (grow_p) {
@@ -821,7 +942,7 @@ void Solver::run_phase(BMatrix<T>& B, const Particles& P, const Contacts& C, int
                        const scopi_hip_params& prm, double dt, double tol, double rtol, int64_t max_ite, bool warm,
                        bool keep_L, hipStream_t s, scopi_hip_solve_info& info) {
     constexpr int NR = lay<LAY>::NR;
-    const unsigned gred = (unsigned)std::min<int64_t>(grid_for(nc), reduce_blocks_);
+    const unsigned gred = (unsigned)row_blocks(nc, reduce_blocks_);
     const int64_t ni = P.n_inactive;
     const int64_t stride = NR * B.cap;
     const double dt2 = dt * dt;
@@ -855,22 +976,23 @@ void Solver::run_phase(BMatrix<T>& B, const Particles& P, const Contacts& C, int
             if (e)
                 hipExtLaunchKernelGGL(k_trial<T, LAY>, dim3(gred), dim3(kBlock), 0, s, e[0], e[1], 0,
                                       (const ApgdCtrl*)(ctrl_ + cur), ctrl_ + (1 - cur), B, nc, partial_trial_,
-                                      (const double*)partial_row_, (int)gred);
+                                      (const double*)partial_row_, (int)gred, SceneMap{});
             else
                 hipLaunchKernelGGL((k_trial<T, LAY>), dim3(gred), dim3(kBlock), 0, s, (const ApgdCtrl*)(ctrl_ + cur),
-                                   ctrl_ + (1 - cur), B, nc, partial_trial_, (const double*)partial_row_, (int)gred);
+                                   ctrl_ + (1 - cur), B, nc, partial_trial_, (const double*)partial_row_, (int)gred,
+                                   SceneMap{});
             cur = 1 - cur;
             launch_bt<T, LAY>(bt_group_, s, ctrl_ + cur, -1, B, na, tstart_, tc_, partial_trial_, (int)gred,
                               e ? e[2] : nullptr, e ? e[3] : nullptr);
             if (e)
                 hipExtLaunchKernelGGL(k_row<T, LAY>, dim3(gred), dim3(kBlock), 0, s, e[4], e[5], 0, ctrl_ + cur, B,
-                                      nc, (const int32_t*)C.i, (const int32_t*)C.j, ni, partial_row_);
+                                      nc, (const int32_t*)C.i, (const int32_t*)C.j, ni, partial_row_, SceneMap{});
             else
                 hipLaunchKernelGGL((k_row<T, LAY>), dim3(gred), dim3(kBlock), 0, s, ctrl_ + cur, B, nc,
-                                   (const int32_t*)C.i, (const int32_t*)C.j, ni, partial_row_);
+                                   (const int32_t*)C.i, (const int32_t*)C.j, ni, partial_row_, SceneMap{});
         }
         hipLaunchKernelGGL(k_ctrl, dim3(1), dim3(kBlock), 0, s, (const ApgdCtrl*)(ctrl_ + cur), ctrl_ + (1 - cur),
-                           (const double*)partial_row_, (int)gred);
+                           (const double*)partial_row_, (int)gred, SceneMap{});
         cur = 1 - cur;
         SCOPI_HIP_CHECK(hipGetLastError());
         SCOPI_HIP_CHECK(hipMemcpyAsync(h_ctrl_, ctrl_ + cur, sizeof(ApgdCtrl), hipMemcpyDeviceToHost, s));
@@ -908,11 +1030,8 @@ void Solver::run_phase(BMatrix<T>& B, const Particles& P, const Contacts& C, int
 }
 
 template <int LAY>
-void Solver::solve_layout(const Particles& P, const Contacts& C, int64_t nc, const scopi_hip_params& prm, double dt,
-                          hipStream_t s, scopi_hip_solve_info& info) {
-    constexpr int NR = lay<LAY>::NR, NW = lay<LAY>::NW;
+void Solver::assemble(const Particles& P, const Contacts& C, int64_t nc, double dt, bool fp32, hipStream_t s) {
     const int64_t na = P.n - P.n_inactive, ni = P.n_inactive;
-    const bool fp32 = prm.precision == SCOPI_PRECISION_FP32_REFINE;
     // assembly: M^-1, B, c
     hipLaunchKernelGGL((k_minv<double, LAY>), dim3(grid_for(na)), dim3(kBlock), 0, s, P, b64_.minv);
     if (fp32) hipLaunchKernelGGL((k_minv<float, LAY>), dim3(grid_for(na)), dim3(kBlock), 0, s, P, b32_.minv);
@@ -940,6 +1059,15 @@ void Solver::solve_layout(const Particles& P, const Contacts& C, int64_t nc, con
         hipLaunchKernelGGL((k_transpose<float, LAY>), dim3(grid_for(na)), dim3(kBlock), 0, s, b32_, na, istart_, iend_,
                            jstart_, jend_, jlist_, tstart_, tc_);
     SCOPI_HIP_CHECK(hipGetLastError());
+}
+
+template <int LAY>
+void Solver::solve_layout(const Particles& P, const Contacts& C, int64_t nc, const scopi_hip_params& prm, double dt,
+                          hipStream_t s, scopi_hip_solve_info& info) {
+    constexpr int NR = lay<LAY>::NR, NW = lay<LAY>::NW;
+    const int64_t na = P.n - P.n_inactive;
+    const bool fp32 = prm.precision == SCOPI_PRECISION_FP32_REFINE;
+    assemble<LAY>(P, C, nc, dt, fp32, s);
     if (fp32) {
         // FP32 phase to a relative stop or its precision floor, then FP64 refinement
         // warm-started from it with the last accepted L
@@ -1010,6 +1138,268 @@ int Solver::solve(const Particles& P, const Contacts& C, int64_t nc, const scopi
     has_solution_ = true;
     if (info) *info = loc;
     return SCOPI_HIP_OK;
+}
+
+// ------------------------------------------------------------------ scenes
+void Solver::scene_tables(const std::vector<int>& scenes, hipStream_t s) {
+    const int K = scenes_->nscene;
+    SCOPI_HIP_CHECK(hipStreamSynchronize(s));  // the staging buffer may still feed a queued copy
+    int64_t R = 0, Bn = 0;
+    for (int sc : scenes) {
+        R += scene_rblocks_[sc];
+        Bn += scene_bblocks_[sc];
+    }
+    const int64_t need = R + Bn + 3 * (int64_t)K + (int64_t)scenes.size();
+    if (need > tab_cap_) {
+        if (d_tab_) SCOPI_HIP_CHECK(hipFree(d_tab_));
+        if (h_tab_) SCOPI_HIP_CHECK(hipHostFree(h_tab_));
+        tab_cap_ = need + need / 2 + 1024;
+        SCOPI_HIP_CHECK(hipMalloc(&d_tab_, tab_cap_ * sizeof(int32_t)));
+        SCOPI_HIP_CHECK(hipHostMalloc(&h_tab_, tab_cap_ * sizeof(int32_t)));
+    }
+    int32_t* rblk = h_tab_;
+    int32_t* bblk = rblk + R;
+    int32_t* rfirst = bblk + Bn;
+    int32_t* rcount = rfirst + K;
+    int32_t* bfirst = rcount + K;
+    int32_t* slist = bfirst + K;
+    for (int k = 0; k < K; ++k) rfirst[k] = rcount[k] = bfirst[k] = 0;
+    int64_t r = 0, b = 0;
+    for (size_t k = 0; k < scenes.size(); ++k) {
+        const int sc = scenes[k];
+        rfirst[sc] = (int32_t)r;
+        rcount[sc] = scene_rblocks_[sc];
+        for (int m = 0; m < scene_rblocks_[sc]; ++m) rblk[r++] = sc;
+        bfirst[sc] = (int32_t)b;
+        for (int m = 0; m < scene_bblocks_[sc]; ++m) bblk[b++] = sc;
+        slist[k] = sc;
+    }
+    SCOPI_HIP_CHECK(hipMemcpyAsync(d_tab_, h_tab_, need * sizeof(int32_t), hipMemcpyHostToDevice, s));
+    map_.rblk = d_tab_;
+    map_.bblk = d_tab_ + R;
+    map_.rfirst = d_tab_ + R + Bn;
+    map_.rcount = map_.rfirst + K;
+    map_.bfirst = map_.rcount + K;
+    map_.slist = map_.bfirst + K;
+    rgrid_ = (int)R;
+    bgrid_ = (int)Bn;
+    nlist_ = (int)scenes.size();
+}
+
+template <class T, int LAY>
+void Solver::run_phase_scenes(BMatrix<T>& B, const Contacts& C, int64_t na, int64_t nc, const scopi_hip_params& prm,
+                              double dt, double tol, double rtol, int64_t max_ite, bool warm, bool keep_L,
+                              hipStream_t s, std::vector<scopi_hip_solve_info>& info) {
+    constexpr int NR = lay<LAY>::NR;
+    const int K = scenes_->nscene;
+    const int64_t stride = NR * B.cap;
+    const double dt2 = dt * dt;
+    const unsigned gnc = (unsigned)std::min<int64_t>(grid_for(nc), reduce_blocks_);
+    if (!warm) {
+        hipLaunchKernelGGL(k_zero<T>, dim3(1024), dim3(kBlock), 0, s, B.lam, 3 * stride);
+        hipLaunchKernelGGL(k_zero<T>, dim3(1024), dim3(kBlock), 0, s, B.qlam, 3 * stride);
+    } else {
+        // every scene's lambda_0 is in slot 0: lambda_{-1} = lambda_0 and both Q products
+        SCOPI_HIP_CHECK(hipMemcpyAsync(B.lam + stride, B.lam, stride * sizeof(T), hipMemcpyDeviceToDevice, s));
+        launch_bt<T, LAY>(bt_group_, s, (ApgdCtrl*)nullptr, 0, B, na, tstart_, tc_, nullptr, 0);
+        hipLaunchKernelGGL((k_row_plain<T, LAY>), dim3(gnc), dim3(kBlock), 0, s, B, 0, nc, (const int32_t*)C.i,
+                           (const int32_t*)C.j, p_inactive_, dt2);
+        SCOPI_HIP_CHECK(hipMemcpyAsync(B.qlam + stride, B.qlam, stride * sizeof(T), hipMemcpyDeviceToDevice, s));
+        for (auto& in : info) in.q_applies += 1;
+    }
+    hipLaunchKernelGGL(k_ctrl_init_scenes, dim3(grid_for(K, 64)), dim3(64), 0, s, mctrl_, (const int32_t*)d_buf_, K,
+                       prm.alpha, 1. / prm.alpha, tol, rtol, dt2, max_ite, prm.method, prm.dynamic_descent ? 1 : 0,
+                       keep_L ? 1 : 0, sizeof(T) == 4 ? 1 : 0, scenes_->d_crange, d_done_);
+    SCOPI_HIP_CHECK(hipGetLastError());
+    std::vector<int> live(K), buf(K, 0);
+    for (int k = 0; k < K; ++k) live[k] = k;
+    scene_tables(live, s);
+    int batch = 2;
+    int cur = 0;  // ctrl copy (K scenes) holding the newest state of the scenes in the grid
+    for (;;) {
+        for (int r = 0; r < batch; ++r) {
+            hipLaunchKernelGGL((k_trial<T, LAY>), dim3(rgrid_), dim3(kBlock), 0, s,
+                               (const ApgdCtrl*)(mctrl_ + (int64_t)cur * K), mctrl_ + (int64_t)(1 - cur) * K, B, nc,
+                               partial_trial_, (const double*)partial_row_, 0, map_);
+            cur = 1 - cur;
+            launch_bt<T, LAY>(bt_group_, s, mctrl_ + (int64_t)cur * K, -1, B, na, tstart_, tc_, partial_trial_, 0,
+                              nullptr, nullptr, map_, bgrid_);
+            hipLaunchKernelGGL((k_row<T, LAY>), dim3(rgrid_), dim3(kBlock), 0, s, mctrl_ + (int64_t)cur * K, B, nc,
+                               (const int32_t*)C.i, (const int32_t*)C.j, p_inactive_, partial_row_, map_);
+        }
+        hipLaunchKernelGGL(k_ctrl, dim3(nlist_), dim3(kBlock), 0, s, (const ApgdCtrl*)(mctrl_ + (int64_t)cur * K),
+                           mctrl_ + (int64_t)(1 - cur) * K, (const double*)partial_row_, 0, map_);
+        cur = 1 - cur;
+        SCOPI_HIP_CHECK(hipGetLastError());
+        SCOPI_HIP_CHECK(hipMemcpyAsync(h_done_, d_done_, K * sizeof(int32_t), hipMemcpyDeviceToHost, s));
+        SCOPI_HIP_CHECK(hipStreamSynchronize(s));
+        std::vector<int> next;
+        next.reserve(live.size());
+        for (int sc : live) {
+            if (h_done_[sc])
+                buf[sc] = cur;
+            else
+                next.push_back(sc);
+        }
+        if (next.empty()) break;
+        if (next.size() != live.size()) {
+            live.swap(next);
+            scene_tables(live, s);
+        }
+        batch = std::min(batch * 2, 64);
+    }
+    // final states: scene sc in copy buf[sc]
+    scene_buf_ = buf;
+    SCOPI_HIP_CHECK(hipMemcpyAsync(d_buf_, scene_buf_.data(), K * sizeof(int32_t), hipMemcpyHostToDevice, s));
+    SCOPI_HIP_CHECK(hipMemcpyAsync(h_ctrls_, mctrl_, 2 * (int64_t)K * sizeof(ApgdCtrl), hipMemcpyDeviceToHost, s));
+    SCOPI_HIP_CHECK(hipStreamSynchronize(s));
+    for (int sc = 0; sc < K; ++sc) {
+        const ApgdCtrl& c = h_ctrls_[(int64_t)buf[sc] * K + sc];
+        scopi_hip_solve_info& in = info[sc];
+        in.iterations += c.iter;
+        in.backtracks += c.backtracks;
+        in.q_applies += c.q_applies;
+        in.status = c.status == kStatusStalled ? 0 : c.status;
+        in.converged = in.status == 0 ? 1 : 0;
+        in.residual = c.res;
+    }
+}
+
+template <int LAY>
+void Solver::solve_layout_scenes(const Particles& P, const Contacts& C, int64_t nc, const scopi_hip_params& prm,
+                                 double dt, hipStream_t s, std::vector<scopi_hip_solve_info>& info) {
+    constexpr int NR = lay<LAY>::NR, NW = lay<LAY>::NW;
+    const int K = scenes_->nscene;
+    const int64_t na = P.n - P.n_inactive;
+    const bool fp32 = prm.precision == SCOPI_PRECISION_FP32_REFINE;
+    assemble<LAY>(P, C, nc, dt, fp32, s);
+    std::vector<int> all(K);
+    for (int k = 0; k < K; ++k) all[k] = k;
+    // per scene, the slot holding the final lambda of the last phase
+    std::vector<int32_t> sl(K);
+    auto slots_to_device = [&](int32_t* dev) {
+        for (int k = 0; k < K; ++k) sl[k] = h_ctrls_[(int64_t)scene_buf_[k] * K + k].cur;
+        SCOPI_HIP_CHECK(hipMemcpyAsync(dev, sl.data(), K * sizeof(int32_t), hipMemcpyHostToDevice, s));
+        SCOPI_HIP_CHECK(hipStreamSynchronize(s));
+    };
+    int32_t* d_slot = d_done_;  // reused as the per-scene slot list between phases
+    if (fp32) {
+        const double rtol32 = prm.fp32_tolerance > 0 ? 0.0 : 1e-5;
+        const double tol32 = prm.fp32_tolerance > 0 ? prm.fp32_tolerance : prm.tolerance;
+        run_phase_scenes<float, LAY>(b32_, C, na, nc, prm, dt, tol32, rtol32, prm.max_ite, false, false, s, info);
+        for (auto& in : info) in.iterations_fp32 = in.iterations;
+        slots_to_device(d_slot);
+        scene_tables(all, s);
+        hipLaunchKernelGGL((k_scene_slot<float, double, NR>), dim3(rgrid_), dim3(kBlock), 0, s,
+                           (const float*)b32_.lam, NR * b32_.cap, b64_.lam, (const int32_t*)d_slot, map_);
+        run_phase_scenes<double, LAY>(b64_, C, na, nc, prm, dt, prm.tolerance, 0.0, prm.max_ite, true, true, s, info);
+    } else {
+        run_phase_scenes<double, LAY>(b64_, C, na, nc, prm, dt, prm.tolerance, 0.0, prm.max_ite, false, false, s,
+                                      info);
+    }
+    // every scene's final lambda to slot 0, then the velocities (FP64)
+    slots_to_device(d_slot);
+    scene_tables(all, s);
+    hipLaunchKernelGGL((k_scene_slot<double, double, NR>), dim3(rgrid_), dim3(kBlock), 0, s, (const double*)b64_.lam,
+                       NR * b64_.cap, b64_.lam, (const int32_t*)d_slot, map_);
+    final_slot_ = 0;
+    launch_bt<double, LAY>(bt_group_, s, (ApgdCtrl*)nullptr, 0, b64_, na, tstart_, tc_, nullptr, 0);
+    hipLaunchKernelGGL(k_velocities<NW>, dim3(grid_for(na)), dim3(kBlock), 0, s, P, (const double*)b64_.w, pcap_, dt, 1,
+                       u_, w_);
+    SCOPI_HIP_CHECK(hipGetLastError());
+}
+
+int Solver::solve_scenes(const Particles& P, const Contacts& C, int64_t nc, const scopi_hip_params& prm, double dt,
+                         bool rot_free, hipStream_t s, const SceneSet& S, std::vector<scopi_hip_solve_info>& info) {
+    const int64_t na = P.n - P.n_inactive;
+    const int K = S.nscene;
+    const bool friction = prm.problem == SCOPI_PROBLEM_FRICTION;
+    const bool fp32 = prm.precision == SCOPI_PRECISION_FP32_REFINE;
+    if (prm.problem != SCOPI_PROBLEM_NOFRICTION && prm.problem != SCOPI_PROBLEM_FRICTION) return SCOPI_HIP_ERR_UNSUPPORTED;
+    if (prm.method != SCOPI_METHOD_APGD && prm.method != SCOPI_METHOD_PGD) return SCOPI_HIP_ERR_UNSUPPORTED;
+    if (!(prm.alpha > 0) || K < 1 || (int)S.alo.size() != K || (int)S.ahi.size() != K) return SCOPI_HIP_ERR_ARG;
+    reserve(na, nc, fp32);
+    layout_ = friction ? kCone : ((rot_free && !prm.general_jacobian) ? kRotFree : kRigid);
+    nr_ = friction ? 3 : 1;
+    nc_ = nc;
+    rows_ = nr_ * nc;
+    p_inactive_ = P.n_inactive;
+    scenes_ = &S;
+    if (K > nscene_cap_) {
+        for (void* p : {(void*)d_arange_, (void*)d_done_, (void*)d_buf_, (void*)mctrl_})
+            if (p) SCOPI_HIP_CHECK(hipFree(p));
+        for (void* p : {(void*)h_done_, (void*)h_ctrls_})
+            if (p) SCOPI_HIP_CHECK(hipHostFree(p));
+        nscene_cap_ = K + K / 2 + 16;
+        SCOPI_HIP_CHECK(hipMalloc(&d_arange_, 2 * (int64_t)nscene_cap_ * sizeof(int64_t)));
+        SCOPI_HIP_CHECK(hipMalloc(&d_done_, nscene_cap_ * sizeof(int32_t)));
+        SCOPI_HIP_CHECK(hipMalloc(&d_buf_, nscene_cap_ * sizeof(int32_t)));
+        SCOPI_HIP_CHECK(hipMalloc(&mctrl_, 2 * (int64_t)nscene_cap_ * sizeof(ApgdCtrl)));
+        SCOPI_HIP_CHECK(hipMemset(mctrl_, 0, 2 * (int64_t)nscene_cap_ * sizeof(ApgdCtrl)));
+        SCOPI_HIP_CHECK(hipHostMalloc(&h_done_, nscene_cap_ * sizeof(int32_t)));
+        SCOPI_HIP_CHECK(hipHostMalloc(&h_ctrls_, 2 * (int64_t)nscene_cap_ * sizeof(ApgdCtrl)));
+    }
+    // per-scene shares of the grids: the per-contact grid from the scene's contacts,
+    // the k_bt grid from its active particles (neither depends on the other scenes)
+    std::vector<int64_t> ar(2 * K), hc(2 * K);
+    SCOPI_HIP_CHECK(hipMemcpyAsync(hc.data(), S.d_crange, 2 * K * sizeof(int64_t), hipMemcpyDeviceToHost, s));
+    SCOPI_HIP_CHECK(hipStreamSynchronize(s));
+    scene_rblocks_.assign(K, 1);
+    scene_bblocks_.assign(K, 1);
+    int64_t R = 0;
+    for (int k = 0; k < K; ++k) {
+        ar[2 * k] = S.alo[k];
+        ar[2 * k + 1] = S.ahi[k];
+        const int64_t nck = hc[2 * k + 1] - hc[2 * k];
+        scene_rblocks_[k] = (int)row_blocks(nck, reduce_blocks_);
+        scene_bblocks_[k] = (int)std::max<int64_t>(grid_for((S.ahi[k] - S.alo[k]) * (int64_t)bt_group_), 1);
+        R += scene_rblocks_[k];
+    }
+    if (R > partial_cap_) {
+        for (double** a : {&partial_trial_, &partial_row_}) {
+            SCOPI_HIP_CHECK(hipFree(*a));
+            SCOPI_HIP_CHECK(hipMalloc(a, (R + 1024) * kPartialStride * sizeof(double)));
+        }
+        partial_cap_ = R + 1024;
+    }
+    SCOPI_HIP_CHECK(hipMemcpyAsync(d_arange_, ar.data(), 2 * K * sizeof(int64_t), hipMemcpyHostToDevice, s));
+    map_ = SceneMap{};
+    map_.crange = S.d_crange;
+    map_.arange = d_arange_;
+    map_.done = d_done_;
+    hipLaunchKernelGGL(k_fill_i, dim3(grid_for(K)), dim3(kBlock), 0, s, d_buf_, (int64_t)K, 0);
+    info.assign(K, scopi_hip_solve_info{});
+    for (int k = 0; k < K; ++k) {
+        info[k].n_contacts = hc[2 * k + 1] - hc[2 * k];
+        info[k].rows = nr_ * info[k].n_contacts;
+        info[k].converged = 1;
+    }
+    if (nc == 0 || na == 0) {
+        hipLaunchKernelGGL(k_velocities<6>, dim3(grid_for(na)), dim3(kBlock), 0, s, P, (const double*)nullptr, pcap_, dt,
+                           0, u_, w_);
+        SCOPI_HIP_CHECK(hipGetLastError());
+        SCOPI_HIP_CHECK(hipStreamSynchronize(s));
+        has_solution_ = true;
+        scenes_ = nullptr;
+        return SCOPI_HIP_OK;
+    }
+    if (layout_ == kCone)
+        solve_layout_scenes<kCone>(P, C, nc, prm, dt, s, info);
+    else if (layout_ == kRigid)
+        solve_layout_scenes<kRigid>(P, C, nc, prm, dt, s, info);
+    else
+        solve_layout_scenes<kRotFree>(P, C, nc, prm, dt, s, info);
+    SCOPI_HIP_CHECK(hipStreamSynchronize(s));
+    has_solution_ = true;
+    scenes_ = nullptr;
+    return SCOPI_HIP_OK;
+}
+
+void Solver::download_lambda_range(double* host, int64_t lo, int64_t hi, hipStream_t s) const {
+    if (!has_solution_ || hi <= lo) return;
+    const double* lam = b64_.lam + (int64_t)final_slot_ * nr_ * b64_.cap;
+    SCOPI_HIP_CHECK(hipMemcpyAsync(host, lam + nr_ * lo, nr_ * (hi - lo) * sizeof(double), hipMemcpyDeviceToHost, s));
+    SCOPI_HIP_CHECK(hipStreamSynchronize(s));
 }
 
 void Solver::download_lambda(double* host, hipStream_t s) const {

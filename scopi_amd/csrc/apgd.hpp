@@ -4,6 +4,8 @@
 #include "common.hpp"
 #include "sort_scan.hpp"
 
+#include <vector>
+
 namespace scopi_hip {
 
 // Layouts of the contact Jacobian B (one block row per contact, two particle blocks).
@@ -48,6 +50,32 @@ struct ApgdCtrl {
     int32_t pending;     // a product waits for its control step (set by the row kernel)
 };
 
+// Independent scenes solved together.  The scenes' active particles and contacts
+// occupy consecutive ranges of one packed problem; the per-contact kernels
+// (k_trial, k_row) and k_bt run one grid over all scenes in which every block
+// belongs to one scene.  Each scene keeps its own control state (ApgdCtrl) and
+// block sums, and its blocks split its rows the same way whichever other scenes
+// share the launch, so a scene's iterates do not depend on the others.
+// A null `rblk` means one scene spanning the whole grid.
+struct SceneMap {
+    const int32_t* rblk = nullptr;     // per-contact grid: scene of each block
+    const int32_t* rfirst = nullptr;   // per scene: its first block in the per-contact grid
+    const int32_t* rcount = nullptr;   // per scene: its blocks in the per-contact grid
+    const int32_t* bblk = nullptr;     // k_bt grid: scene of each block
+    const int32_t* bfirst = nullptr;   // per scene: its first block in the k_bt grid
+    const int32_t* slist = nullptr;    // scenes of the grid, in order (one control block each)
+    const int64_t* crange = nullptr;   // per scene: contacts [lo, hi)
+    const int64_t* arange = nullptr;   // per scene: active particles [lo, hi)
+    int32_t* done = nullptr;           // per scene: done flag, written by the control steps
+};
+
+// Host description of a scene set (scenes in order, consecutive ranges).
+struct SceneSet {
+    int nscene = 0;
+    const int64_t* d_crange = nullptr;   // device, 2 per scene: contacts [lo, hi)
+    std::vector<int64_t> alo, ahi;       // active particles of each scene (active index)
+};
+
 // Device arrays of one precision.  Per-contact and per-particle data are
 // arrays of small structures (AoS): coef[c*NC + k], vectors[c*NR + d], w[p*NW + k].
 template <class T>
@@ -72,8 +100,14 @@ class Solver {
     // rot_free: every active particle is a sphere (enables the kRotFree layout).
     int solve(const Particles& P, const Contacts& C, int64_t nc, const scopi_hip_params& prm, double dt,
               bool rot_free, hipStream_t s, scopi_hip_solve_info* info);
+    // The same for a set of independent scenes packed in P / C (scene ranges in `S`);
+    // one solve info per scene.  Afterwards every scene's final lambda is in slot 0.
+    int solve_scenes(const Particles& P, const Contacts& C, int64_t nc, const scopi_hip_params& prm, double dt,
+                     bool rot_free, hipStream_t s, const SceneSet& S, std::vector<scopi_hip_solve_info>& info);
     // lambda of the last solve (FP64) in the reference's order (nc, or 3 nc with index 3c + d).
     void download_lambda(double* host, hipStream_t s) const;
+    // rows [NR lo, NR hi) of the last solve's lambda (FP64)
+    void download_lambda_range(double* host, int64_t lo, int64_t hi, hipStream_t s) const;
     const double* u() const { return u_; }
     const double* w() const { return w_; }
     int64_t pcap() const { return pcap_; }
@@ -100,6 +134,17 @@ class Solver {
     template <int LAY>
     void solve_layout(const Particles& P, const Contacts& C, int64_t nc, const scopi_hip_params& prm, double dt,
                       hipStream_t s, scopi_hip_solve_info& info);
+    template <int LAY>
+    void assemble(const Particles& P, const Contacts& C, int64_t nc, double dt, bool fp32, hipStream_t s);
+    // multi-scene loop: tables of the scenes still running, phase loop, per-scene results
+    void scene_tables(const std::vector<int>& scenes, hipStream_t s);
+    template <class T, int LAY>
+    void run_phase_scenes(BMatrix<T>& B, const Contacts& C, int64_t na, int64_t nc, const scopi_hip_params& prm,
+                          double dt, double tol, double rtol, int64_t max_ite, bool warm, bool keep_L, hipStream_t s,
+                          std::vector<scopi_hip_solve_info>& info);
+    template <int LAY>
+    void solve_layout_scenes(const Particles& P, const Contacts& C, int64_t nc, const scopi_hip_params& prm, double dt,
+                             hipStream_t s, std::vector<scopi_hip_solve_info>& info);
 
     RadixSorter sorter_;
     Scanner scanner_;
@@ -132,6 +177,24 @@ class Solver {
     int reduce_blocks_ = 1024;  // grid of the per-contact kernels (grid-stride)
     int bt_group_ = 8;          // lanes per particle in k_bt
     bool has_solution_ = false;
+    // multi-scene state
+    const SceneSet* scenes_ = nullptr;
+    int nscene_cap_ = 0;
+    int64_t* d_arange_ = nullptr;    // 2 per scene
+    int32_t* d_done_ = nullptr;      // per scene
+    int32_t* h_done_ = nullptr;      // pinned, per scene
+    int32_t* d_buf_ = nullptr;       // per scene: ctrl copy holding its state (phase end)
+    int32_t* d_tab_ = nullptr;       // block tables (SceneMap arrays)
+    int32_t* h_tab_ = nullptr;       // pinned staging of the tables
+    int64_t tab_cap_ = 0;
+    ApgdCtrl* h_ctrls_ = nullptr;    // pinned, per scene
+    ApgdCtrl* mctrl_ = nullptr;      // 2 copies x scenes
+    int64_t partial_cap_ = 0;        // blocks
+    int64_t p_inactive_ = 0;
+    SceneMap map_;
+    int rgrid_ = 0, bgrid_ = 0, nlist_ = 0;
+    std::vector<int> scene_rblocks_, scene_bblocks_;
+    std::vector<int32_t> scene_buf_;  // per scene: ctrl copy holding its final state
     // kernel timing (HIP events on the solve stream)
     bool timing_ = false;
     double t_bt_ms_ = 0, t_row_ms_ = 0, t_trial_ms_ = 0;

@@ -1,6 +1,7 @@
 // api.hip -- the extern "C" layer (include/scopi_hip.h): one context per scene,
 // the per-particle kernels of the vap slot and of the integration, and the
 // device-resident time loop (ScopiSolver::run, include/scopi/solver.hpp:266-303).
+#include <algorithm>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -648,6 +649,439 @@ int scopi_hip_kernel_timing(scopi_hip_ctx* ctx, double* out) {
     if (!ctx || !out) return SCOPI_HIP_ERR_ARG;
     ctx->solver.timing(out);
     return SCOPI_HIP_OK;
+}
+
+}  // extern "C"
+
+// ============================================================== batches of scenes
+struct scopi_hip_batch {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    scopi_hip_params prm{};
+    double dt = 0.0;
+    Particles P;
+    Contacts C;
+    int64_t nc = 0;
+    BatchSearch search;
+    Solver solver;
+    struct Scene {
+        int64_t id = 0, n = 0, n_inactive = 0;
+        int64_t ob_lo = 0, a_lo = 0;  // packed positions of its obstacles and active particles
+        int64_t steps = 0;
+        scopi_hip_solve_info info{};
+        bool has_contacts = false;
+    };
+    std::vector<Scene> scenes;  // packed order
+    struct Staged {             // added, not packed yet (host copies)
+        Scene sc;
+        std::vector<int32_t> shape;
+        std::vector<double> pos, q, size, sq, v, vd, omega, domega, f, m, J;
+    };
+    std::vector<Staged> staged;
+    std::vector<int64_t> removed;
+    int64_t next_id = 1;
+    bool solved = false;
+    std::string err;
+};
+
+namespace {
+
+int bfail(scopi_hip_batch* b, int code, const std::string& msg) {
+    if (b) b->err = msg;
+    return code;
+}
+
+template <class F>
+int bguarded(scopi_hip_batch* b, F&& f) {
+    if (!b) return SCOPI_HIP_ERR_ARG;
+    try {
+        SCOPI_HIP_CHECK(hipSetDevice(b->device));
+        return f();
+    } catch (const hip_error& e) {
+        return bfail(b, SCOPI_HIP_ERR_HIP, std::string(e.what) + ": " + hipGetErrorString(e.code));
+    } catch (const std::exception& e) {
+        return bfail(b, SCOPI_HIP_ERR_HIP, e.what());
+    }
+}
+
+// every field of particle map[k] of S into particle k of D (map[k] < 0: left as is)
+__global__ void __launch_bounds__(kBlock) k_gather_particles(Particles D, Particles S, const int32_t* __restrict__ map,
+                                                            int64_t n) {
+    const int64_t k = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (k >= n || map[k] < 0) return;
+    const int64_t o = map[k];
+    D.shape[k] = S.shape[o];
+    D.m[k] = S.m[o];
+    for (int d = 0; d < 3; ++d) {
+        D.pos[d * D.cap + k] = S.pos[d * S.cap + o];
+        D.size[d * D.cap + k] = S.size[d * S.cap + o];
+        D.v[d * D.cap + k] = S.v[d * S.cap + o];
+        D.vd[d * D.cap + k] = S.vd[d * S.cap + o];
+        D.omega[d * D.cap + k] = S.omega[d * S.cap + o];
+        D.domega[d * D.cap + k] = S.domega[d * S.cap + o];
+        D.f[d * D.cap + k] = S.f[d * S.cap + o];
+        D.J[d * D.cap + k] = S.J[d * S.cap + o];
+    }
+    for (int d = 0; d < 4; ++d) D.q[d * D.cap + k] = S.q[d * S.cap + o];
+    for (int d = 0; d < 2; ++d) D.sq[d * D.cap + k] = S.sq[d * S.cap + o];
+}
+
+// host rows [r0, r0 + n) of an (rows x w) array -> device component-major at k0
+void upload_rows_at(double* dev, int64_t cap, int64_t k0, const double* host, int64_t r0, int64_t n, int w,
+                    hipStream_t s) {
+    std::vector<double> t((size_t)w * n);
+    for (int64_t i = 0; i < n; ++i)
+        for (int d = 0; d < w; ++d) t[(size_t)d * n + i] = host[(r0 + i) * w + d];
+    for (int d = 0; d < w; ++d)
+        SCOPI_HIP_CHECK(hipMemcpyAsync(dev + d * cap + k0, t.data() + (size_t)d * n, n * sizeof(double),
+                                       hipMemcpyHostToDevice, s));
+    SCOPI_HIP_CHECK(hipStreamSynchronize(s));
+}
+
+// Packs the kept scenes (in order) and the staged ones: obstacles of every scene
+// first, then the active particles of every scene.
+void batch_pack(scopi_hip_batch* b) {
+    if (b->staged.empty() && b->removed.empty()) return;
+    std::vector<scopi_hip_batch::Scene> next;
+    for (const auto& sc : b->scenes)
+        if (std::find(b->removed.begin(), b->removed.end(), sc.id) == b->removed.end()) next.push_back(sc);
+    const size_t kept = next.size();
+    for (const auto& st : b->staged) next.push_back(st.sc);
+    int64_t NI = 0, N = 0;
+    for (const auto& sc : next) {
+        NI += sc.n_inactive;
+        N += sc.n;
+    }
+    if (N > INT32_MAX / 2) throw hip_error(hipErrorInvalidValue, "batch: too many particles");
+    std::vector<int32_t> map((size_t)std::max<int64_t>(N, 1), -1);
+    int64_t ob = 0, ac = NI;
+    for (size_t k = 0; k < next.size(); ++k) {
+        auto& sc = next[k];
+        const int64_t na = sc.n - sc.n_inactive;
+        if (k < kept) {
+            for (int64_t t = 0; t < sc.n_inactive; ++t) map[ob + t] = (int32_t)(sc.ob_lo + t);
+            for (int64_t t = 0; t < na; ++t) map[ac + t] = (int32_t)(sc.a_lo + t);
+        }
+        sc.ob_lo = ob;
+        sc.a_lo = ac;
+        ob += sc.n_inactive;
+        ac += na;
+    }
+    hipStream_t s = b->stream;
+    Particles Q{};
+    Q.dim = 3;
+    particles_reserve(Q, std::max<int64_t>(N, 1));
+    Q.n = N;
+    Q.n_inactive = NI;
+    if (N > 0 && b->P.n > 0) {
+        int32_t* dmap = nullptr;
+        SCOPI_HIP_CHECK(hipMalloc(&dmap, N * sizeof(int32_t)));
+        SCOPI_HIP_CHECK(hipMemcpyAsync(dmap, map.data(), N * sizeof(int32_t), hipMemcpyHostToDevice, s));
+        hipLaunchKernelGGL(k_gather_particles, dim3(grid_for(N)), dim3(kBlock), 0, s, Q, b->P, (const int32_t*)dmap, N);
+        SCOPI_HIP_CHECK(hipGetLastError());
+        SCOPI_HIP_CHECK(hipStreamSynchronize(s));
+        SCOPI_HIP_CHECK(hipFree(dmap));
+    }
+    for (size_t k = kept; k < next.size(); ++k) {
+        const auto& st = b->staged[k - kept];
+        const auto& sc = next[k];
+        const int64_t ni = sc.n_inactive, na = sc.n - ni;
+        for (int part = 0; part < 2; ++part) {
+            const int64_t r0 = part ? ni : 0, cnt = part ? na : ni, k0 = part ? sc.a_lo : sc.ob_lo;
+            if (cnt == 0) continue;
+            SCOPI_HIP_CHECK(hipMemcpyAsync(Q.shape + k0, st.shape.data() + r0, cnt * sizeof(int32_t),
+                                           hipMemcpyHostToDevice, s));
+            SCOPI_HIP_CHECK(hipMemcpyAsync(Q.m + k0, st.m.data() + r0, cnt * sizeof(double), hipMemcpyHostToDevice, s));
+            upload_rows_at(Q.pos, Q.cap, k0, st.pos.data(), r0, cnt, 3, s);
+            upload_rows_at(Q.q, Q.cap, k0, st.q.data(), r0, cnt, 4, s);
+            upload_rows_at(Q.size, Q.cap, k0, st.size.data(), r0, cnt, 3, s);
+            upload_rows_at(Q.sq, Q.cap, k0, st.sq.data(), r0, cnt, 2, s);
+            upload_rows_at(Q.v, Q.cap, k0, st.v.data(), r0, cnt, 3, s);
+            upload_rows_at(Q.vd, Q.cap, k0, st.vd.data(), r0, cnt, 3, s);
+            upload_rows_at(Q.omega, Q.cap, k0, st.omega.data(), r0, cnt, 3, s);
+            upload_rows_at(Q.domega, Q.cap, k0, st.domega.data(), r0, cnt, 3, s);
+            upload_rows_at(Q.f, Q.cap, k0, st.f.data(), r0, cnt, 3, s);
+            upload_rows_at(Q.J, Q.cap, k0, st.J.data(), r0, cnt, 3, s);
+        }
+    }
+    particles_free(b->P);
+    b->P = Q;
+    b->scenes.swap(next);
+    b->staged.clear();
+    b->removed.clear();
+    // scene table of the search: work units and cell keys in scene order
+    std::vector<SceneRanges> R(b->scenes.size());
+    int64_t u = 0, cell = 0;
+    for (size_t k = 0; k < b->scenes.size(); ++k) {
+        const auto& sc = b->scenes[k];
+        const int64_t na = sc.n - sc.n_inactive;
+        R[k].ob_lo = sc.ob_lo;
+        R[k].ob_hi = sc.ob_lo + sc.n_inactive;
+        R[k].a_lo = sc.a_lo;
+        R[k].a_hi = sc.a_lo + na;
+        R[k].u_lo = u;
+        R[k].cell0 = cell;
+        R[k].cells = 2 * na + 64;
+        u += (sc.n_inactive + 1) * na;
+        cell += R[k].cells;
+        b->scenes[k].has_contacts = false;
+    }
+    b->search.set_scenes(R, NI, N - NI, s);
+    b->nc = 0;
+    b->solved = false;
+}
+
+scopi_hip_batch::Scene* batch_find(scopi_hip_batch* b, int64_t id) {
+    for (auto& sc : b->scenes)
+        if (sc.id == id) return &sc;
+    return nullptr;
+}
+
+}  // namespace
+
+extern "C" {
+
+int scopi_hip_batch_create(const scopi_hip_params* params, double dt, scopi_hip_batch** out) {
+    if (!out || !params_ok(params) || !(dt > 0)) return SCOPI_HIP_ERR_ARG;
+    *out = nullptr;
+    auto* b = new scopi_hip_batch();
+    b->prm = *params;
+    b->dt = dt;
+    b->P.dim = 3;
+    try {
+        SCOPI_HIP_CHECK(hipGetDevice(&b->device));
+        SCOPI_HIP_CHECK(hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking));
+        contacts_reserve(b->C, 1024);
+    } catch (const hip_error& e) {
+        std::fprintf(stderr, "scopi_hip_batch_create: %s: %s\n", e.what, hipGetErrorString(e.code));
+        if (b->stream) (void)hipStreamDestroy(b->stream);
+        delete b;
+        return SCOPI_HIP_ERR_HIP;
+    }
+    *out = b;
+    return SCOPI_HIP_OK;
+}
+
+int scopi_hip_batch_destroy(scopi_hip_batch* b) {
+    if (!b) return SCOPI_HIP_ERR_ARG;
+    (void)hipSetDevice(b->device);
+    if (b->stream) (void)hipStreamSynchronize(b->stream);
+    particles_free(b->P);
+    contacts_free(b->C);
+    if (b->stream) (void)hipStreamDestroy(b->stream);
+    delete b;
+    return SCOPI_HIP_OK;
+}
+
+const char* scopi_hip_batch_last_error(const scopi_hip_batch* b) { return b ? b->err.c_str() : "null batch"; }
+
+int scopi_hip_batch_add_scene(scopi_hip_batch* b, int64_t n, int64_t n_inactive, const int32_t* shape,
+                              const double* pos, const double* q, const double* size, const double* squareness,
+                              const double* v, const double* vd, const double* omega, const double* domega,
+                              const double* f, const double* m, const double* J, int64_t* scene_id) {
+    return bguarded(b, [&]() -> int {
+        if (n < 0 || n_inactive < 0 || n_inactive > n || !shape || !pos || !q || !size || !v || !vd || !omega ||
+            !domega || !f || !m || !J || !scene_id)
+            return bfail(b, SCOPI_HIP_ERR_ARG, "batch_add_scene: bad argument");
+        for (int64_t k = 0; k < n; ++k) {
+            const int s = shape[k];
+            const bool active = k >= n_inactive;
+            if (active && s != SCOPI_SHAPE_SPHERE)
+                return bfail(b, SCOPI_HIP_ERR_UNSUPPORTED, "batch_add_scene: active particles must be spheres");
+            if (!active && s != SCOPI_SHAPE_SPHERE && s != SCOPI_SHAPE_PLANE)
+                return bfail(b, SCOPI_HIP_ERR_UNSUPPORTED, "batch_add_scene: obstacles must be spheres or planes");
+        }
+        scopi_hip_batch::Staged st;
+        st.sc.id = b->next_id++;
+        st.sc.n = n;
+        st.sc.n_inactive = n_inactive;
+        st.shape.assign(shape, shape + n);
+        auto cp = [&](std::vector<double>& dst, const double* src, int w, double fill) {
+            if (src)
+                dst.assign(src, src + n * w);
+            else
+                dst.assign((size_t)n * w, fill);
+        };
+        cp(st.pos, pos, 3, 0.0);
+        cp(st.q, q, 4, 0.0);
+        cp(st.size, size, 3, 0.0);
+        cp(st.sq, squareness, 2, 1.0);
+        cp(st.v, v, 3, 0.0);
+        cp(st.vd, vd, 3, 0.0);
+        cp(st.omega, omega, 3, 0.0);
+        cp(st.domega, domega, 3, 0.0);
+        cp(st.f, f, 3, 0.0);
+        cp(st.m, m, 1, 0.0);
+        cp(st.J, J, 3, 0.0);
+        *scene_id = st.sc.id;
+        b->staged.push_back(std::move(st));
+        return SCOPI_HIP_OK;
+    });
+}
+
+int scopi_hip_batch_remove_scene(scopi_hip_batch* b, int64_t scene_id) {
+    return bguarded(b, [&]() -> int {
+        for (size_t k = 0; k < b->staged.size(); ++k)
+            if (b->staged[k].sc.id == scene_id) {
+                b->staged.erase(b->staged.begin() + k);
+                return SCOPI_HIP_OK;
+            }
+        if (!batch_find(b, scene_id) ||
+            std::find(b->removed.begin(), b->removed.end(), scene_id) != b->removed.end())
+            return bfail(b, SCOPI_HIP_ERR_ARG, "batch_remove_scene: unknown scene id");
+        b->removed.push_back(scene_id);
+        return SCOPI_HIP_OK;
+    });
+}
+
+int scopi_hip_batch_step(scopi_hip_batch* b, int64_t n_steps) {
+    return bguarded(b, [&]() -> int {
+        if (n_steps < 0) return bfail(b, SCOPI_HIP_ERR_ARG, "batch_step: n_steps < 0");
+        batch_pack(b);
+        const int K = (int)b->scenes.size();
+        if (K == 0 || n_steps == 0) return SCOPI_HIP_OK;
+        hipStream_t s = b->stream;
+        Particles& P = b->P;
+        const int64_t na = P.n - P.n_inactive;
+        SceneSet S;
+        S.nscene = K;
+        S.alo.resize(K);
+        S.ahi.resize(K);
+        for (int k = 0; k < K; ++k) {
+            S.alo[k] = b->scenes[k].a_lo - P.n_inactive;
+            S.ahi[k] = S.alo[k] + (b->scenes[k].n - b->scenes[k].n_inactive);
+        }
+        std::vector<scopi_hip_solve_info> infos;
+        for (int64_t it = 0; it < n_steps; ++it) {
+            if (P.n_inactive > 0)
+                hipLaunchKernelGGL(k_obstacles, dim3(grid_for(P.n_inactive)), dim3(kBlock), 0, s, P, b->dt);
+            b->nc = b->search.run(P, b->C, b->prm, s);
+            if (na > 0) hipLaunchKernelGGL(k_vap, dim3(grid_for(na)), dim3(kBlock), 0, s, P, b->prm.vap, b->dt);
+            S.d_crange = b->search.d_crange();
+            int rc = b->solver.solve_scenes(P, b->C, b->nc, b->prm, b->dt, true, s, S, infos);
+            if (rc != SCOPI_HIP_OK) return bfail(b, rc, "batch_step: solve failed");
+            if (na > 0)
+                hipLaunchKernelGGL(k_update_move, dim3(grid_for(na)), dim3(kBlock), 0, s, P, b->solver.u(),
+                                   b->solver.w(), b->solver.pcap(), b->dt);
+            SCOPI_HIP_CHECK(hipGetLastError());
+            for (int k = 0; k < K; ++k) {
+                b->scenes[k].info = infos[k];
+                b->scenes[k].steps += 1;
+                b->scenes[k].has_contacts = true;
+            }
+            b->solved = true;
+        }
+        SCOPI_HIP_CHECK(hipStreamSynchronize(s));
+        return SCOPI_HIP_OK;
+    });
+}
+
+int scopi_hip_batch_size(scopi_hip_batch* b, int64_t* n_scenes, int64_t* n_particles, int64_t* n_contacts) {
+    return bguarded(b, [&]() -> int {
+        batch_pack(b);
+        if (n_scenes) *n_scenes = (int64_t)b->scenes.size();
+        if (n_particles) *n_particles = b->P.n;
+        if (n_contacts) *n_contacts = b->nc;
+        return SCOPI_HIP_OK;
+    });
+}
+
+int scopi_hip_batch_download_scene(scopi_hip_batch* b, int64_t scene_id, double* pos, double* q, double* v,
+                                   double* omega) {
+    return bguarded(b, [&]() -> int {
+        batch_pack(b);
+        const auto* sc = batch_find(b, scene_id);
+        if (!sc) return bfail(b, SCOPI_HIP_ERR_ARG, "batch_download_scene: unknown scene id");
+        const Particles& P = b->P;
+        const int64_t ni = sc->n_inactive, na = sc->n - ni;
+        struct F {
+            double* h;
+            const double* d;
+            int w;
+        };
+        for (const F& f : {F{pos, P.pos, 3}, F{q, P.q, 4}, F{v, P.v, 3}, F{omega, P.omega, 3}}) {
+            if (!f.h) continue;
+            if (ni) download_rows(f.d + sc->ob_lo, P.cap, f.h, ni, f.w, b->stream, 0);
+            if (na) download_rows(f.d + sc->a_lo, P.cap, f.h, na, f.w, b->stream, ni);
+        }
+        return SCOPI_HIP_OK;
+    });
+}
+
+int scopi_hip_batch_scene_info(scopi_hip_batch* b, int64_t scene_id, int64_t* steps, scopi_hip_solve_info* info) {
+    return bguarded(b, [&]() -> int {
+        batch_pack(b);
+        const auto* sc = batch_find(b, scene_id);
+        if (!sc) return bfail(b, SCOPI_HIP_ERR_ARG, "batch_scene_info: unknown scene id");
+        if (steps) *steps = sc->steps;
+        if (info) *info = sc->info;
+        return SCOPI_HIP_OK;
+    });
+}
+
+int scopi_hip_batch_download_contacts(scopi_hip_batch* b, int64_t scene_id, int64_t* n_contacts, int64_t* i,
+                                      int64_t* j, double* dij, double* nij, double* pi, double* pj) {
+    return bguarded(b, [&]() -> int {
+        batch_pack(b);
+        const auto* sc = batch_find(b, scene_id);
+        if (!sc || !n_contacts) return bfail(b, SCOPI_HIP_ERR_ARG, "batch_download_contacts: bad argument");
+        if (!sc->has_contacts) return bfail(b, SCOPI_HIP_ERR_STATE, "batch_download_contacts: no step since packing");
+        const int k = (int)(sc - b->scenes.data());
+        const int64_t lo = b->search.crange()[2 * k], hi = b->search.crange()[2 * k + 1], nc = hi - lo;
+        *n_contacts = nc;
+        if (nc == 0) return SCOPI_HIP_OK;
+        hipStream_t s = b->stream;
+        const int64_t ni = sc->n_inactive;
+        auto local = [&](int32_t g) -> int64_t { return g < b->P.n_inactive ? g - sc->ob_lo : ni + (g - sc->a_lo); };
+        std::vector<int32_t> t(nc);
+        for (auto pr : {std::make_pair(i, b->C.i), std::make_pair(j, b->C.j)}) {
+            if (!pr.first) continue;
+            SCOPI_HIP_CHECK(hipMemcpyAsync(t.data(), pr.second + lo, nc * sizeof(int32_t), hipMemcpyDeviceToHost, s));
+            SCOPI_HIP_CHECK(hipStreamSynchronize(s));
+            for (int64_t c = 0; c < nc; ++c) pr.first[c] = local(t[c]);
+        }
+        if (dij) {
+            SCOPI_HIP_CHECK(hipMemcpyAsync(dij, b->C.dij + lo, nc * sizeof(double), hipMemcpyDeviceToHost, s));
+            SCOPI_HIP_CHECK(hipStreamSynchronize(s));
+        }
+        if (nij) download_rows(b->C.nij + lo, b->C.cap, nij, nc, 3, s);
+        if (pi) download_rows(b->C.pi + lo, b->C.cap, pi, nc, 3, s);
+        if (pj) download_rows(b->C.pj + lo, b->C.cap, pj, nc, 3, s);
+        return SCOPI_HIP_OK;
+    });
+}
+
+int scopi_hip_batch_download_solution(scopi_hip_batch* b, int64_t scene_id, double* u, double* omega,
+                                      double* lambda) {
+    return bguarded(b, [&]() -> int {
+        batch_pack(b);
+        const auto* sc = batch_find(b, scene_id);
+        if (!sc) return bfail(b, SCOPI_HIP_ERR_ARG, "batch_download_solution: unknown scene id");
+        if (!b->solved || !sc->has_contacts) return bfail(b, SCOPI_HIP_ERR_STATE, "batch_download_solution: no solve");
+        const int64_t na = sc->n - sc->n_inactive, a0 = sc->a_lo - b->P.n_inactive;
+        if (u && na) download_rows(b->solver.u() + a0, b->solver.pcap(), u, na, 3, b->stream);
+        if (omega && na) download_rows(b->solver.w() + a0, b->solver.pcap(), omega, na, 3, b->stream);
+        if (lambda) {
+            const int k = (int)(sc - b->scenes.data());
+            b->solver.download_lambda_range(lambda, b->search.crange()[2 * k], b->search.crange()[2 * k + 1],
+                                            b->stream);
+        }
+        return SCOPI_HIP_OK;
+    });
+}
+
+int64_t scopi_hip_scene_footprint(int64_t n, int64_t n_inactive, const scopi_hip_params* params) {
+    if (n < 0 || n_inactive < 0 || n_inactive > n) return -1;
+    const int64_t na = n - n_inactive;
+    const bool fp32 = params && params->precision == SCOPI_PRECISION_FP32_REFINE;
+    const int64_t nc = 12 * na;
+    // particles: state (31 doubles + tag), solver per particle, search (keys, units, cells)
+    const int64_t per_particle = 31 * 8 + 4;
+    const int64_t per_active = (6 + 6 + 6) * 4 + 6 * 8 + 2 * 6 * 8 * (fp32 ? 2 : 1) + 3 * 4 + 8 * (n_inactive + 1) + 2 * 4 * 3;
+    // contacts: record (96 B), B and its vectors (488 B FP64, 244 B more for FP32), transposes (24 B)
+    const int64_t per_contact = 96 + 488 + (fp32 ? 244 : 0) + 24;
+    const int64_t bytes = n * per_particle + na * per_active + nc * per_contact;
+    return bytes + bytes / 4;
 }
 
 }  // extern "C"

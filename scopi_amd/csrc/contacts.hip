@@ -371,6 +371,208 @@ __global__ void __launch_bounds__(kBlock) k_cand_emit(SeCandidates S, int64_t n,
     C.mu[o] = mu;
 }
 
+// ---------------------------------------------------------------- batches of scenes
+// One block per scene: bounding box of its active particles, then its grid (cell
+// width >= the candidate cut-off, as k_grid_setup; dense cells, at most `cells`).
+__global__ void __launch_bounds__(kBlock) k_scene_grid(Particles P, const SceneRanges* __restrict__ scenes, int kd,
+                                                      double kd_radius, double dmax, GridInfo* __restrict__ grids) {
+    __shared__ double red[7][kBlock];
+    const SceneRanges sc = scenes[blockIdx.x];
+    double lo[3] = {1e300, 1e300, 1e300}, hi[3] = {-1e300, -1e300, -1e300}, rm = 0.0;
+    for (int64_t k = sc.a_lo + threadIdx.x; k < sc.a_hi; k += kBlock) {
+        for (int d = 0; d < 3; ++d) {
+            const double x = P.pos[d * P.cap + k];
+            lo[d] = fmin(lo[d], x);
+            hi[d] = fmax(hi[d], x);
+        }
+        rm = fmax(rm, bounding_radius(P, k));
+    }
+    for (int d = 0; d < 3; ++d) {
+        red[d][threadIdx.x] = lo[d];
+        red[3 + d][threadIdx.x] = hi[d];
+    }
+    red[6][threadIdx.x] = rm;
+    __syncthreads();
+    for (int o = kBlock / 2; o > 0; o >>= 1) {
+        if ((int)threadIdx.x < o) {
+            for (int d = 0; d < 3; ++d) red[d][threadIdx.x] = fmin(red[d][threadIdx.x], red[d][threadIdx.x + o]);
+            for (int d = 3; d < 7; ++d) red[d][threadIdx.x] = fmax(red[d][threadIdx.x], red[d][threadIdx.x + o]);
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x != 0) return;
+    GridInfo g;
+    rm = red[6][0];
+    double cut = (2.0 * rm + dmax) * (1.0 + 1e-9) + 1e-12;
+    if (kd) cut = fmin(cut, sqrt(kd_radius) * (1.0 + 1e-12) + 1e-300);
+    if (!(cut > 0)) cut = 1e-300;
+    double ext[3];
+    int64_t nc[3];
+    for (int d = 0; d < 3; ++d) {
+        g.lo[d] = sc.a_hi > sc.a_lo ? red[d][0] : 0.0;
+        ext[d] = sc.a_hi > sc.a_lo ? fmax(red[3 + d][0] - red[d][0], 0.0) : 0.0;
+        nc[d] = (int64_t)fmin(floor(ext[d] / cut) + 1.0, 1048576.0);
+    }
+    // at most `cells` cells: coarser cells along the longest axes (any width >= cut finds every pair)
+    const int64_t budget = sc.cells > 1 ? sc.cells : 1;
+    while (nc[0] * nc[1] * nc[2] > budget) {
+        int a = 0;
+        for (int d = 1; d < 3; ++d)
+            if (nc[d] > nc[a]) a = d;
+        nc[a] = (nc[a] + 1) / 2 < nc[a] ? (nc[a] + 1) / 2 : nc[a] - 1;
+        if (nc[a] < 1) nc[a] = 1;
+    }
+    for (int d = 0; d < 3; ++d) {
+        g.nc[d] = (int)nc[d];
+        const bool coarse = (double)nc[d] < floor(ext[d] / cut) + 1.0;
+        g.w[d] = coarse ? fmax(cut, ext[d] / (double)nc[d] * (1.0 + 1e-12)) : cut;
+    }
+    g.bits = 0;
+    g.cutoff = cut;
+    g.rmax = rm;
+    grids[blockIdx.x] = g;
+}
+
+__device__ inline int64_t dense_key(const GridInfo& g, int64_t cell0, int cx, int cy, int cz) {
+    return cell0 + cx + (int64_t)g.nc[0] * (cy + (int64_t)g.nc[1] * cz);
+}
+
+__global__ void __launch_bounds__(kBlock) k_scene_keys(Particles P, const SceneRanges* __restrict__ scenes,
+                                                      const GridInfo* __restrict__ grids,
+                                                      const int32_t* __restrict__ scene_of, uint32_t* __restrict__ keys,
+                                                      int32_t* __restrict__ vals) {
+    const int64_t na = P.n - P.n_inactive;
+    const int64_t t = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (t >= na) return;
+    const int sc = scene_of[t];
+    const GridInfo g = grids[sc];
+    const int64_t k = P.n_inactive + t;
+    const int cx = cell_coord<3>(g, P.pos[k], 0), cy = cell_coord<3>(g, P.pos[P.cap + k], 1);
+    const int cz = cell_coord<3>(g, P.pos[2 * P.cap + k], 2);
+    keys[t] = (uint32_t)dense_key(g, scenes[sc].cell0, cx, cy, cz);
+    vals[t] = (int32_t)k;
+}
+
+// for_partners on a scene's dense grid (same predicates and visiting rule)
+template <bool KD, class F>
+__device__ inline void for_partners_dense(const Particles& P, const GridInfo& g, int64_t cell0,
+                                          const int* __restrict__ start, const int* __restrict__ end,
+                                          const int32_t* __restrict__ sorted, int64_t i, double kd_radius, double dmax,
+                                          F&& visit) {
+    const d3 ci = load3(P.pos, P.cap, i);
+    const int cx = cell_coord<3>(g, ci.x, 0), cy = cell_coord<3>(g, ci.y, 1), cz = cell_coord<3>(g, ci.z, 2);
+    for (int dz = -1; dz <= 1; ++dz) {
+        const int z = cz + dz;
+        if (z < 0 || z >= g.nc[2]) continue;
+        for (int dy = -1; dy <= 1; ++dy) {
+            const int y = cy + dy;
+            if (y < 0 || y >= g.nc[1]) continue;
+            for (int dx = -1; dx <= 1; ++dx) {
+                const int x = cx + dx;
+                if (x < 0 || x >= g.nc[0]) continue;
+                const int64_t key = dense_key(g, cell0, x, y, z);
+                const int s1 = end[key];
+                for (int s = start[key]; s < s1; ++s) {
+                    const int64_t j = sorted[s];
+                    if (j <= i) continue;
+                    if (KD) {
+                        const d3 cj = load3(P.pos, P.cap, j);
+                        if (!(kd_dist2<3>(ci, cj) < kd_radius)) continue;
+                    }
+                    CP cp;
+                    if (!closest_points<3>(P, i, j, cp)) continue;
+                    if (cp.dij < dmax) visit(j);
+                }
+            }
+        }
+    }
+}
+
+__device__ inline int unit_scene(const int64_t* __restrict__ ulo, int K, int64_t u) {
+    int a = 0, b = K - 1;  // last scene with ulo <= u
+    while (a < b) {
+        const int m = (a + b + 1) >> 1;
+        if (ulo[m] <= u)
+            a = m;
+        else
+            b = m - 1;
+    }
+    return a;
+}
+
+// Work units of every scene: obstacle x active pairs (flag), then one unit per
+// active particle (its partners j > i).  EMIT writes the contacts at `offsets`.
+template <bool KD, bool EMIT>
+__global__ void __launch_bounds__(kBlock) k_scene_units(Particles P, const SceneRanges* __restrict__ scenes,
+                                                       const int64_t* __restrict__ ulo, int K,
+                                                       const GridInfo* __restrict__ grids,
+                                                       const int* __restrict__ start, const int* __restrict__ end,
+                                                       const int32_t* __restrict__ sorted, double kd_radius,
+                                                       double dmax, int* __restrict__ counts,
+                                                       const int* __restrict__ offsets, Contacts C) {
+    const int64_t u = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (u >= ulo[K]) return;
+    const int sc = unit_scene(ulo, K, u);
+    const SceneRanges r = scenes[sc];
+    const int64_t na = r.a_hi - r.a_lo, no = (r.ob_hi - r.ob_lo) * na;
+    const int64_t local = u - r.u_lo;
+    if (local < no) {
+        const int64_t o = r.ob_lo + local / na, j = r.a_lo + local % na;
+        if (!EMIT) {
+            CP cp;
+            counts[u] = (closest_points<3>(P, o, j, cp) && cp.dij < dmax) ? 1 : 0;
+        } else if (counts[u]) {
+            const int64_t pos = offsets[u];
+            C.i[pos] = (int32_t)o;
+            C.j[pos] = (int32_t)j;
+        }
+        return;
+    }
+    const int64_t i = r.a_lo + (local - no);
+    const GridInfo g = grids[sc];
+    if (!EMIT) {
+        int cnt = 0;
+        if (na > 1) for_partners_dense<KD>(P, g, r.cell0, start, end, sorted, i, kd_radius, dmax, [&](int64_t) { ++cnt; });
+        counts[u] = cnt;
+        return;
+    }
+    const int64_t cnt = counts[u];
+    if (cnt == 0) return;
+    const int64_t base = offsets[u];
+    int64_t m = 0;
+    for_partners_dense<KD>(P, g, r.cell0, start, end, sorted, i, kd_radius, dmax,
+                           [&](int64_t j) {
+                               if (m < cnt) C.j[base + m] = (int32_t)j;  // the count pass found cnt partners
+                               ++m;
+                           });
+    for (int64_t a = 1; a < cnt; ++a) {  // insertion sort by j
+        const int32_t key = C.j[base + a];
+        int64_t b = a - 1;
+        while (b >= 0 && C.j[base + b] > key) {
+            C.j[base + b + 1] = C.j[base + b];
+            --b;
+        }
+        C.j[base + b + 1] = key;
+    }
+    for (int64_t a = 0; a < cnt; ++a) C.i[base + a] = (int32_t)i;
+}
+
+__global__ void k_scene_crange(const int64_t* __restrict__ ulo, int K, const int* __restrict__ offsets,
+                               const int* __restrict__ total, int64_t* __restrict__ crange, int64_t* __restrict__ io) {
+    const int sc = blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t U = ulo[K];
+    auto at = [&](int64_t u) -> int64_t { return u < U ? (int64_t)offsets[u] : (int64_t)*total; };
+    if (sc == 0 && io) io[0] = (U > 0) ? *total : 0;
+    if (sc >= K) return;
+    const int64_t a = (U > 0) ? at(ulo[sc]) : 0, b = (U > 0) ? at(ulo[sc + 1]) : 0;
+    crange[2 * sc] = a;
+    crange[2 * sc + 1] = b;
+    if (io) {
+        io[1 + 2 * sc] = a;
+        io[2 + 2 * sc] = b;
+    }
+}
+
 // ---------------------------------------------------------------- host
 void contacts_free(Contacts& C) {
     for (void* p : {(void*)C.i, (void*)C.j, (void*)C.dij, (void*)C.nij, (void*)C.pi, (void*)C.pj, (void*)C.mu})
@@ -587,6 +789,133 @@ int64_t ContactSearch::run_candidates(const Particles& P, Contacts& C, const sco
     if (nc > 0)
         hipLaunchKernelGGL(k_cand_emit, dim3(grid_for(ncand)), dim3(kBlock), 0, s, se_cand_, ncand, cand_flags_,
                            cand_offsets_, (int64_t)0, C, prm.mu);
+    SCOPI_HIP_CHECK(hipGetLastError());
+    return nc;
+}
+
+// ---------------------------------------------------------------- batches of scenes (host)
+BatchSearch::~BatchSearch() {
+    for (void* p : {(void*)d_scenes_, (void*)d_ulo_, (void*)d_scene_of_, (void*)d_grids_, (void*)d_crange_,
+                    (void*)keys_, (void*)vals_, (void*)counts_, (void*)offsets_, (void*)cell_start_,
+                    (void*)cell_end_, (void*)d_total_})
+        if (p) (void)hipFree(p);
+    if (h_io_) (void)hipHostFree(h_io_);
+}
+
+void BatchSearch::set_scenes(const std::vector<SceneRanges>& scenes, int64_t n_inactive, int64_t na, hipStream_t s) {
+    const int K = (int)scenes.size();
+    K_ = K;
+    na_ = na;
+    ni_ = n_inactive;
+    if (K > kcap_) {
+        for (void* p : {(void*)d_scenes_, (void*)d_ulo_, (void*)d_grids_, (void*)d_crange_})
+            if (p) SCOPI_HIP_CHECK(hipFree(p));
+        if (h_io_) SCOPI_HIP_CHECK(hipHostFree(h_io_));
+        kcap_ = K + K / 2 + 16;
+        SCOPI_HIP_CHECK(hipMalloc(&d_scenes_, kcap_ * sizeof(SceneRanges)));
+        SCOPI_HIP_CHECK(hipMalloc(&d_ulo_, (kcap_ + 1) * sizeof(int64_t)));
+        SCOPI_HIP_CHECK(hipMalloc(&d_grids_, kcap_ * sizeof(GridInfo)));
+        SCOPI_HIP_CHECK(hipMalloc(&d_crange_, 2 * kcap_ * sizeof(int64_t)));
+        SCOPI_HIP_CHECK(hipHostMalloc(&h_io_, (2 * kcap_ + 1) * sizeof(int64_t)));
+    }
+    if (!d_total_) SCOPI_HIP_CHECK(hipMalloc(&d_total_, sizeof(int)));
+    std::vector<int64_t> ulo(K + 1);
+    std::vector<int32_t> scene_of(std::max<int64_t>(na, 1));
+    int64_t cells = 0;
+    for (int k = 0; k < K; ++k) {
+        ulo[k] = scenes[k].u_lo;
+        cells = std::max(cells, scenes[k].cell0 + scenes[k].cells);
+        for (int64_t a = scenes[k].a_lo; a < scenes[k].a_hi; ++a) scene_of[a - n_inactive] = k;
+    }
+    units_ = K ? scenes[K - 1].u_lo + (scenes[K - 1].ob_hi - scenes[K - 1].ob_lo + 1) *
+                                          (scenes[K - 1].a_hi - scenes[K - 1].a_lo)
+               : 0;
+    ulo[K] = units_;
+    cells_ = cells;
+    if (units_ >= INT32_MAX || cells_ >= (int64_t(1) << 31)) throw hip_error(hipErrorInvalidValue, "batch search: too many work units");
+    if (na + 1 > pcap_) {
+        for (void* p : {(void*)keys_, (void*)vals_, (void*)d_scene_of_})
+            if (p) SCOPI_HIP_CHECK(hipFree(p));
+        pcap_ = na + na / 4 + 1024;
+        SCOPI_HIP_CHECK(hipMalloc(&keys_, pcap_ * sizeof(uint32_t)));
+        SCOPI_HIP_CHECK(hipMalloc(&vals_, pcap_ * sizeof(int32_t)));
+        SCOPI_HIP_CHECK(hipMalloc(&d_scene_of_, pcap_ * sizeof(int32_t)));
+    }
+    if (units_ + 1 > ucap_) {
+        for (void* p : {(void*)counts_, (void*)offsets_})
+            if (p) SCOPI_HIP_CHECK(hipFree(p));
+        ucap_ = units_ + units_ / 4 + 1024;
+        SCOPI_HIP_CHECK(hipMalloc(&counts_, ucap_ * sizeof(int)));
+        SCOPI_HIP_CHECK(hipMalloc(&offsets_, ucap_ * sizeof(int)));
+    }
+    if (cells_ + 1 > ccap_) {
+        for (void* p : {(void*)cell_start_, (void*)cell_end_})
+            if (p) SCOPI_HIP_CHECK(hipFree(p));
+        ccap_ = cells_ + cells_ / 4 + 1024;
+        SCOPI_HIP_CHECK(hipMalloc(&cell_start_, ccap_ * sizeof(int)));
+        SCOPI_HIP_CHECK(hipMalloc(&cell_end_, ccap_ * sizeof(int)));
+    }
+    if (K > 0) {
+        SCOPI_HIP_CHECK(hipMemcpyAsync(d_scenes_, scenes.data(), K * sizeof(SceneRanges), hipMemcpyHostToDevice, s));
+        SCOPI_HIP_CHECK(hipMemcpyAsync(d_ulo_, ulo.data(), (K + 1) * sizeof(int64_t), hipMemcpyHostToDevice, s));
+    }
+    if (na > 0)
+        SCOPI_HIP_CHECK(hipMemcpyAsync(d_scene_of_, scene_of.data(), na * sizeof(int32_t), hipMemcpyHostToDevice, s));
+    SCOPI_HIP_CHECK(hipStreamSynchronize(s));
+    h_crange_.assign(2 * K, 0);
+}
+
+int64_t BatchSearch::run(const Particles& P, Contacts& C, const scopi_hip_params& prm, hipStream_t s) {
+    const int K = K_;
+    if (K == 0) return 0;
+    const bool kd = prm.contact_method == SCOPI_CONTACT_KDTREE;
+    const int64_t na = na_;
+    if (na > 0) {
+        hipLaunchKernelGGL(k_scene_grid, dim3(K), dim3(kBlock), 0, s, P, (const SceneRanges*)d_scenes_, kd ? 1 : 0,
+                           prm.kd_tree_radius, prm.dmax, d_grids_);
+        hipLaunchKernelGGL(k_scene_keys, dim3(grid_for(na)), dim3(kBlock), 0, s, P, (const SceneRanges*)d_scenes_,
+                           (const GridInfo*)d_grids_, (const int32_t*)d_scene_of_, keys_, vals_);
+        int bits = 1;
+        while ((int64_t(1) << bits) < cells_) ++bits;
+        sorter_.sort(keys_, vals_, na, bits, s, scanner_);
+        hipLaunchKernelGGL(k_fill_int, dim3(1024), dim3(kBlock), 0, s, cell_start_, cells_, 0);
+        hipLaunchKernelGGL(k_fill_int, dim3(1024), dim3(kBlock), 0, s, cell_end_, cells_, 0);
+        hipLaunchKernelGGL(k_cell_range, dim3(grid_for(na)), dim3(kBlock), 0, s, keys_, na, cell_start_, cell_end_);
+    }
+    const unsigned gu = grid_for(std::max<int64_t>(units_, 1));
+    if (units_ > 0) {
+        if (kd)
+            hipLaunchKernelGGL((k_scene_units<true, false>), dim3(gu), dim3(kBlock), 0, s, P,
+                               (const SceneRanges*)d_scenes_, (const int64_t*)d_ulo_, K, (const GridInfo*)d_grids_,
+                               (const int*)cell_start_, (const int*)cell_end_, (const int32_t*)vals_,
+                               prm.kd_tree_radius, prm.dmax, counts_, (const int*)offsets_, C);
+        else
+            hipLaunchKernelGGL((k_scene_units<false, false>), dim3(gu), dim3(kBlock), 0, s, P,
+                               (const SceneRanges*)d_scenes_, (const int64_t*)d_ulo_, K, (const GridInfo*)d_grids_,
+                               (const int*)cell_start_, (const int*)cell_end_, (const int32_t*)vals_,
+                               prm.kd_tree_radius, prm.dmax, counts_, (const int*)offsets_, C);
+        scanner_.exclusive(counts_, offsets_, units_, d_total_, s);
+    }
+    hipLaunchKernelGGL(k_scene_crange, dim3(grid_for(K, 64)), dim3(64), 0, s, (const int64_t*)d_ulo_, K,
+                       (const int*)offsets_, (const int*)d_total_, d_crange_, (int64_t*)nullptr);
+    SCOPI_HIP_CHECK(hipMemcpyAsync(h_io_ + 1, d_crange_, 2 * K * sizeof(int64_t), hipMemcpyDeviceToHost, s));
+    SCOPI_HIP_CHECK(hipStreamSynchronize(s));
+    for (int k = 0; k < 2 * K; ++k) h_crange_[k] = h_io_[1 + k];
+    const int64_t nc = h_crange_[2 * K - 1];
+    contacts_reserve(C, std::max<int64_t>(nc, 1));
+    if (nc > 0) {
+        if (kd)
+            hipLaunchKernelGGL((k_scene_units<true, true>), dim3(gu), dim3(kBlock), 0, s, P,
+                               (const SceneRanges*)d_scenes_, (const int64_t*)d_ulo_, K, (const GridInfo*)d_grids_,
+                               (const int*)cell_start_, (const int*)cell_end_, (const int32_t*)vals_,
+                               prm.kd_tree_radius, prm.dmax, counts_, (const int*)offsets_, C);
+        else
+            hipLaunchKernelGGL((k_scene_units<false, true>), dim3(gu), dim3(kBlock), 0, s, P,
+                               (const SceneRanges*)d_scenes_, (const int64_t*)d_ulo_, K, (const GridInfo*)d_grids_,
+                               (const int*)cell_start_, (const int*)cell_end_, (const int32_t*)vals_,
+                               prm.kd_tree_radius, prm.dmax, counts_, (const int*)offsets_, C);
+        hipLaunchKernelGGL(k_contact_geometry<3>, dim3(grid_for(nc)), dim3(kBlock), 0, s, P, C, nc, prm.mu);
+    }
     SCOPI_HIP_CHECK(hipGetLastError());
     return nc;
 }

@@ -5,6 +5,8 @@
 #include "sort_scan.hpp"
 #include "superellipsoid.hpp"
 
+#include <vector>
+
 namespace scopi_hip {
 
 struct GridInfo {
@@ -58,6 +60,57 @@ class ContactSearch {
     int* cand_flags_ = nullptr;
     int* cand_offsets_ = nullptr;
     int64_t cand_cap_ = 0, ncand_ = 0;
+};
+
+// Ranges of one scene of a batch.  A batch packs the obstacles of all scenes
+// first, then the active particles of all scenes, each scene in a consecutive
+// range of both; its contacts come out in one consecutive range, obstacle pairs
+// first, each part in the reference's (i, j) order.
+struct SceneRanges {
+    int64_t ob_lo, ob_hi;   // obstacles (particle indices)
+    int64_t a_lo, a_hi;     // active particles (particle indices)
+    int64_t u_lo;           // first work unit: obstacle x active units, then one unit per active particle
+    int64_t cell0, cells;   // the scene's range of cell keys
+};
+
+// Contact search of every scene of a batch in one pass: per-scene uniform
+// grids (dense cell keys in the scene's own key range, at most `cells` cells),
+// one radix sort over all active particles, one count / scan / emit over the
+// work units of all scenes.  Each scene's pair set is exactly the one
+// ContactSearch::run finds for that scene alone (closed-form shapes).
+class BatchSearch {
+  public:
+    ~BatchSearch();
+    // Uploads the scene table (host order = packed order); call after repacking.
+    void set_scenes(const std::vector<SceneRanges>& scenes, int64_t n_inactive, int64_t na, hipStream_t s);
+    // Returns nc (host sync inside); per-scene contact ranges in d_crange() / crange().
+    int64_t run(const Particles& P, Contacts& C, const scopi_hip_params& prm, hipStream_t s);
+    const int64_t* d_crange() const { return d_crange_; }
+    const std::vector<int64_t>& crange() const { return h_crange_; }
+
+  private:
+    RadixSorter sorter_;
+    Scanner scanner_;
+    int K_ = 0;
+    int64_t na_ = 0, ni_ = 0, units_ = 0, cells_ = 0;
+    SceneRanges* d_scenes_ = nullptr;
+    int64_t* d_ulo_ = nullptr;       // K + 1 unit offsets
+    int32_t* d_scene_of_ = nullptr;  // scene of each active particle
+    GridInfo* d_grids_ = nullptr;
+    int64_t* d_crange_ = nullptr;    // 2 K
+    int64_t* h_io_ = nullptr;        // pinned: total + 2 K
+    std::vector<int64_t> h_crange_;
+    int kcap_ = 0;
+    uint32_t* keys_ = nullptr;
+    int32_t* vals_ = nullptr;
+    int64_t pcap_ = 0;
+    int* counts_ = nullptr;
+    int* offsets_ = nullptr;
+    int64_t ucap_ = 0;
+    int* cell_start_ = nullptr;
+    int* cell_end_ = nullptr;
+    int64_t ccap_ = 0;
+    int* d_total_ = nullptr;
 };
 
 // Allocates / grows the structure-of-arrays contact list (keeps nothing).

@@ -29,6 +29,10 @@ EXPORTS = [
     "scopi_hip_update_and_move", "scopi_hip_displacement_obstacles", "scopi_hip_run",
     "scopi_hip_set_kernel_timing", "scopi_hip_kernel_timing", "scopi_hip_set_tuning", "scopi_hip_se_closest_points",
     "scopi_hip_closest_points", "scopi_hip_portable_math", "scopi_hip_newton_probe",
+    "scopi_hip_batch_create", "scopi_hip_batch_destroy", "scopi_hip_batch_last_error", "scopi_hip_batch_add_scene",
+    "scopi_hip_batch_remove_scene", "scopi_hip_batch_step", "scopi_hip_batch_size", "scopi_hip_batch_download_scene",
+    "scopi_hip_batch_scene_info", "scopi_hip_batch_download_contacts", "scopi_hip_batch_download_solution",
+    "scopi_hip_scene_footprint",
 ]
 TUNE_REDUCE_BLOCKS, TUNE_BT_GROUP = 1, 2
 
@@ -99,7 +103,23 @@ def lib():
     L.scopi_hip_newton_probe.argtypes = [P, C.c_int32, D, D]
     L.scopi_hip_closest_points.argtypes = ([P, C.c_int64, I32, D, D, D, C.c_void_p, I32, D, D, D, C.c_void_p] +
                                            [D] * 4 + [C.c_void_p] * 4)
-    for name in [n for n in dir(L) if n.startswith("scopi_hip_") and n != "scopi_hip_last_error"]:
+    L.scopi_hip_batch_create.argtypes = [C.POINTER(Params), C.c_double, C.POINTER(P)]
+    L.scopi_hip_batch_destroy.argtypes = [P]
+    L.scopi_hip_batch_last_error.argtypes = [P]
+    L.scopi_hip_batch_last_error.restype = C.c_char_p
+    L.scopi_hip_batch_add_scene.argtypes = [P, C.c_int64, C.c_int64, I32, D, D, D, D, D, D, D, D, D, D, D,
+                                            C.POINTER(C.c_int64)]
+    L.scopi_hip_batch_remove_scene.argtypes = [P, C.c_int64]
+    L.scopi_hip_batch_step.argtypes = [P, C.c_int64]
+    L.scopi_hip_batch_size.argtypes = [P, C.c_void_p, C.c_void_p, C.c_void_p]
+    L.scopi_hip_batch_download_scene.argtypes = [P, C.c_int64, Dn, Dn, Dn, Dn]
+    L.scopi_hip_batch_scene_info.argtypes = [P, C.c_int64, C.c_void_p, C.c_void_p]
+    L.scopi_hip_batch_download_contacts.argtypes = [P, C.c_int64, C.POINTER(C.c_int64)] + [C.c_void_p] * 6
+    L.scopi_hip_batch_download_solution.argtypes = [P, C.c_int64, Dn, Dn, Dn]
+    L.scopi_hip_scene_footprint.argtypes = [C.c_int64, C.c_int64, C.POINTER(Params)]
+    L.scopi_hip_scene_footprint.restype = C.c_int64
+    for name in [n for n in dir(L) if n.startswith("scopi_hip_") and n not in (
+            "scopi_hip_last_error", "scopi_hip_batch_last_error", "scopi_hip_scene_footprint")]:
         getattr(L, name).restype = C.c_int
     _lib = L
     return L
@@ -285,3 +305,100 @@ class HipScene:
         self._check(self._L.scopi_hip_kernel_timing(self._h, out), "kernel_timing")
         return dict(bt_ms=out[0], bt_launches=int(out[1]), row_ms=out[2], row_launches=int(out[3]),
                     trial_ms=out[4], trial_launches=int(out[5]))
+
+
+def scene_footprint(n, n_inactive, prm: Params) -> int:
+    """Device bytes a scene may hold in a batch (the admission estimate of a serving loop)."""
+    return int(lib().scopi_hip_scene_footprint(int(n), int(n_inactive), C.byref(prm)))
+
+
+class HipBatch:
+    """Independent 3D sphere scenes stepped together on the current GPU (a scopi_hip_batch).
+
+    Each scene follows ScopiSolver::run's semantics on its own; the batch shares one set of
+    kernel launches per step among all of them, and a scene's results do not depend on which
+    other scenes share the batch.
+    """
+
+    def __init__(self, prm: Params, dt):
+        self._L = lib()
+        self._h = C.c_void_p()
+        self._prm = prm
+        rc = self._L.scopi_hip_batch_create(C.byref(prm), float(dt), C.byref(self._h))
+        if rc != 0:
+            raise ScopiHipError(f"scopi_hip_batch_create failed ({rc})")
+        self._scenes = {}
+
+    def close(self):
+        if getattr(self, "_h", None) and self._h.value:
+            self._L.scopi_hip_batch_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        self.close()
+
+    def _check(self, rc, what):
+        if rc < 0:
+            msg = self._L.scopi_hip_batch_last_error(self._h).decode()
+            raise ScopiHipError(f"{what} failed ({rc}): {msg}")
+        return rc
+
+    def add(self, scene) -> int:
+        f = lambda a: np.ascontiguousarray(a, np.float64)
+        sid = C.c_int64()
+        self._check(self._L.scopi_hip_batch_add_scene(
+            self._h, scene.n, scene.n_inactive, np.ascontiguousarray(scene.shape, np.int32), f(scene.pos), f(scene.q),
+            f(scene.size), f(scene.squareness), f(scene.v), f(scene.vd), f(scene.omega), f(scene.domega), f(scene.f),
+            f(scene.m), f(scene.J), C.byref(sid)), "batch_add_scene")
+        self._scenes[sid.value] = (scene.n, scene.n_inactive)
+        return sid.value
+
+    def remove(self, sid):
+        self._check(self._L.scopi_hip_batch_remove_scene(self._h, int(sid)), "batch_remove_scene")
+        self._scenes.pop(sid, None)
+
+    def step(self, n_steps=1):
+        self._check(self._L.scopi_hip_batch_step(self._h, int(n_steps)), "batch_step")
+
+    def size(self):
+        a, b, c = C.c_int64(), C.c_int64(), C.c_int64()
+        self._check(self._L.scopi_hip_batch_size(self._h, C.byref(a), C.byref(b), C.byref(c)), "batch_size")
+        return dict(scenes=a.value, particles=b.value, contacts=c.value)
+
+    def state(self, sid):
+        n, _ = self._scenes[sid]
+        pos, q, v, om = np.zeros((n, 3)), np.zeros((n, 4)), np.zeros((n, 3)), np.zeros((n, 3))
+        self._check(self._L.scopi_hip_batch_download_scene(self._h, int(sid), _ptr(pos), _ptr(q), _ptr(v), _ptr(om)),
+                    "batch_download_scene")
+        return dict(pos=pos, q=q, v=v, omega=om)
+
+    def info(self, sid):
+        steps = C.c_int64()
+        info = SolveInfo()
+        self._check(self._L.scopi_hip_batch_scene_info(self._h, int(sid), C.byref(steps), C.byref(info)),
+                    "batch_scene_info")
+        d = info.as_dict()
+        d["steps"] = steps.value
+        return d
+
+    def contacts(self, sid):
+        nc = C.c_int64()
+        self._check(self._L.scopi_hip_batch_download_contacts(self._h, int(sid), C.byref(nc), None, None, None, None,
+                                                              None, None), "batch_download_contacts")
+        m = max(nc.value, 1)
+        ci = np.zeros(m, np.int64); cj = np.zeros(m, np.int64); dij = np.zeros(m)
+        nij = np.zeros((m, 3)); pi = np.zeros((m, 3)); pj = np.zeros((m, 3))
+        self._check(self._L.scopi_hip_batch_download_contacts(self._h, int(sid), C.byref(nc), _ptr(ci), _ptr(cj),
+                                                              _ptr(dij), _ptr(nij), _ptr(pi), _ptr(pj)),
+                    "batch_download_contacts")
+        k = nc.value
+        return dict(ij=np.stack([ci[:k], cj[:k]], axis=1), dij=dij[:k], nij=nij[:k], pi=pi[:k], pj=pj[:k])
+
+    def solution(self, sid):
+        n, ni = self._scenes[sid]
+        na = n - ni
+        rows = self.info(sid)["rows"]
+        u = np.zeros((max(na, 1), 3)); w = np.zeros((max(na, 1), 3)); lam = np.zeros(max(rows, 1))
+        self._check(self._L.scopi_hip_batch_download_solution(self._h, int(sid), _ptr(u), _ptr(w), _ptr(lam)),
+                    "batch_download_solution")
+        return dict(u=u[:na], w=w[:na], lam=lam[:rows])

@@ -187,16 +187,17 @@ def disks_2d(n_side=32, spacing=0.025, jitter=0.002, rmin=0.008, rmax=0.012, see
 
 
 def spheres_3d(nx=64, ny=64, nz=32, spacing=0.025, jitter=0.003, rmin=0.009, rmax=0.011, seed=4321,
-               velocity_scale=0.0):
+               velocity_scale=0.0, count=None):
     """Jittered nx*ny*nz lattice of spheres in a box of 5 planes (floor + 4 walls).
 
     Lattice site (i, j, k) sits at ((i+0.5)s, (j+0.5)s, (k+0.5)s) + jitter with the floor at z=0
     and the walls half a spacing outside the lattice.  Gravity (0,0,-9.81)*m via vap_fpd, m=1,
     J=(0.1,0.1,0.1).  `velocity_scale` > 0 adds a seeded uniform initial velocity in
     [-velocity_scale, velocity_scale]^3 per sphere (0 for the configuration as specified).
+    `count` keeps the first `count` lattice sites, layer by layer from the floor up.
     """
     rng = np.random.default_rng(seed)
-    n = nx * ny * nz
+    n = nx * ny * nz if count is None else int(count)
     lo = (0.0, 0.0, 0.0)
     hi = (nx * spacing, ny * spacing, nz * spacing + 1.0)
     planes = box_planes_3d(lo, hi)
@@ -212,6 +213,10 @@ def spheres_3d(nx=64, ny=64, nz=32, spacing=0.025, jitter=0.003, rmin=0.009, rma
         q[k] = qq
     ii, jj, kk = np.meshgrid(np.arange(nx), np.arange(ny), np.arange(nz), indexing="ij")
     base = np.stack([(ii.ravel() + 0.5), (jj.ravel() + 0.5), (kk.ravel() + 0.5)], axis=1) * spacing
+    if count is not None:
+        if n > len(base):
+            raise ValueError("spheres_3d: count exceeds the lattice")
+        base = base[np.lexsort((jj.ravel(), ii.ravel(), kk.ravel()))][:n]
     pos[ni:] = base + rng.uniform(-jitter, jitter, size=(n, 3))
     size[ni:, 0] = rng.uniform(rmin, rmax, size=n)
     shape[ni:] = SPHERE
@@ -228,12 +233,25 @@ def spheres_3d(nx=64, ny=64, nz=32, spacing=0.025, jitter=0.003, rmin=0.009, rma
                  v=v, vd=np.zeros((N, 3)), omega=np.zeros((N, 3)), domega=np.zeros((N, 3)), f=f, m=m, J=J)
 
 
-def spheres_3d_count(n_particles, seed=0, **kw):
-    """A C2-recipe scene with about `n_particles` spheres (lattice nx=ny=2*nz, scaled to the count)."""
+def spheres_3d_count(n_particles, seed=0, exact=False, **kw):
+    """A scene of the 131k-sphere recipe scaled to `n_particles` spheres (lattice nx = ny = 2 nz).
+
+    exact=True keeps exactly `n_particles` sites (the top layer partly filled).
+    """
     nz = max(1, int(round((n_particles / 4.0) ** (1.0 / 3.0))))
     nx = max(1, int(round(math.sqrt(n_particles / nz))))
     ny = max(1, int(math.ceil(n_particles / (nx * nz))))
+    if exact:
+        return spheres_3d(nx=nx, ny=ny, nz=nz, seed=seed, count=n_particles, **kw)
     return spheres_3d(nx=nx, ny=ny, nz=nz, seed=seed, **kw)
+
+
+def batched_sphere_scenes(n_scenes=512, nmin=256, nmax=32768, seed=77):
+    """Independent sphere scenes with log-uniform sizes in [nmin, nmax] (seeded), each built by
+    the 131k-sphere recipe scaled to its count (box of 5 planes, gravity, jittered lattice)."""
+    rng = np.random.default_rng(seed)
+    sizes = np.exp(rng.uniform(math.log(nmin), math.log(nmax), size=n_scenes)).round().astype(np.int64)
+    return [spheres_3d_count(int(n), seed=seed * 100003 + k, exact=True) for k, n in enumerate(sizes)]
 
 
 def superellipsoids_3d(nx=32, ny=32, nz=32, spacing=0.035, jitter=0.003, amin=0.008, amax=0.015, emin=0.4,
@@ -268,3 +286,6 @@ CONFIG2 = dict(dim=3, dt=1e-3, dmax=0.02, kd_tree_radius=0.042 ** 2, tolerance=1
 # kd_tree_radius = (2 |a|_max + dmax)^2 with |a|_max = sqrt(3) 0.015: never binding for these shapes
 CONFIG3 = dict(dim=3, dt=1e-3, dmax=0.02, kd_tree_radius=(2 * math.sqrt(3.0) * 0.015 + 0.02) ** 2, mu=0.3,
                tolerance=1e-6, steps=100)
+# 512 independent sphere scenes, log-uniform [256, 32768] particles, FP32 + FP64 refinement
+CONFIG4 = dict(dim=3, dt=1e-3, dmax=0.02, kd_tree_radius=0.042 ** 2, tolerance=1e-6, steps=50, n_scenes=512,
+               nmin=256, nmax=32768)

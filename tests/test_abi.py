@@ -13,7 +13,7 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 def declared():
     with open(os.path.join(ROOT, "include", "scopi_hip.h")) as fh:
         text = fh.read()
-    return sorted(set(re.findall(r"^\s*(?:int|const char\*)\s+(scopi_hip_\w+)\s*\(", text, re.M)))
+    return sorted(set(re.findall(r"^\s*(?:int|int64_t|const char\*)\s+(scopi_hip_\w+)\s*\(", text, re.M)))
 
 
 def test_header_lists_entry_points():
