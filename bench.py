@@ -237,6 +237,60 @@ def bench_batched(args):
     print(json.dumps(result), flush=True)
 
 
+def bench_served(args):
+    """Served scenes: Poisson arrivals (rate --rate per s) of --requests scene requests, 80% sphere
+    scenes (the 131k-sphere recipe scaled, NoFriction) and 20% superellipsoid scenes (the 32k recipe
+    scaled, Friction mu 0.3), log-uniform 512..65,536 particles, --request-steps steps each, admitted
+    by estimated HBM footprint into one resident batch per kind (scopi_amd.serve).  The one-at-a-time
+    line steps every request's scene alone (its own context) on a sample of steps after one warm-up
+    step and scales each to its step count."""
+    import torch
+    from scopi_amd import hip, serve
+    kinds, footprint, prm = serve.hip_kinds()
+    reqs = serve.poisson_requests(args.requests, args.rate, [("spheres", 0.8), ("superellipsoids", 0.2)],
+                                  (512, 65536), args.request_steps, seed=2025)
+    torch.cuda.init()
+    log(f"[bench] serving {len(reqs)} requests ({sum(r.kind == 'superellipsoids' for r in reqs)} superellipsoid "
+        f"scenes), {args.request_steps} steps each, rate {args.rate}/s")
+    done = lambda r: log(f"[bench] request {r.rid} ({r.kind}, {r.n} particles) done at {r.finished:.2f} s")
+    srv = serve.Server(kinds, serve.served_scene, footprint, hbm_budget=int(args.hbm_budget_gib * (1 << 30)),
+                       on_finish=done)
+    st = srv.run(reqs)
+    log(f"[bench] served: {json.dumps(st)}")
+    one = None
+    if args.one_at_a_time:
+        log("[bench] one-at-a-time sample")
+        t_total = 0.0
+        for r in reqs:
+            g = hip.HipScene(serve.served_scene(r))
+            g.run(prm[r.kind], 1e-3, 1)
+            torch.cuda.synchronize()
+            t1 = time.perf_counter()
+            k = min(args.sample_steps, r.steps)
+            g.run(prm[r.kind], 1e-3, k)
+            torch.cuda.synchronize()
+            t_total += (time.perf_counter() - t1) / k * r.steps
+            g.close()
+        one = {"scene_steps_per_s": st["scene_steps"] / t_total, "seconds_for_all_requests": t_total,
+               "sample": f"each request alone: {args.sample_steps} timed steps after 1 warm-up step, scaled to its "
+                         f"{args.request_steps} steps"}
+    result = {
+        "metric": "served scene time-steps/s (Poisson arrivals, 80% sphere / 20% superellipsoid scenes, "
+                  "512..65,536 particles) with p50/p99 step latency",
+        "value": st["scene_steps_per_s"], "unit": "scene-steps/s", "n_gpus": 1, "steps": st["scene_steps"],
+        "warmup": 0, "ms_per_step": st["step_latency_ms"]["p50"], "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+        "config": {"workload": f"{len(reqs)} requests, Poisson rate {args.rate}/s, {args.request_steps} steps each, "
+                               "sizes log-uniform 512..65,536, spheres NoFriction / superellipsoids Friction mu 0.3, "
+                               f"HBM budget {args.hbm_budget_gib} GiB"},
+        "step_latency_ms": st["step_latency_ms"], "wait_ms": st["wait_ms"], "completion_s": st["completion_s"],
+        "peak_resident_scenes": st["peak_resident_scenes"],
+        "one_at_a_time": one,
+        "speedup_vs_one_at_a_time": (st["scene_steps_per_s"] / one["scene_steps_per_s"]) if one else None,
+    }
+    print(json.dumps(result), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -252,11 +306,16 @@ def main():
                     help="steps of the instrumented pass (HIP events around every solver kernel)")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="run one plain device copy of MIB MiB first (traffic calibration for counter runs)")
-    ap.add_argument("--workload", choices=["spheres", "superellipsoids", "batched"], default="spheres",
+    ap.add_argument("--workload", choices=["spheres", "superellipsoids", "batched", "served"], default="spheres",
                     help="spheres: the headline 131k-sphere step; superellipsoids: 32,768 superellipsoids, "
                          "Friction mu 0.3; batched: independent sphere scenes advanced together (secondary lines)")
     ap.add_argument("--scenes", type=int, default=512, help="batched: number of scenes")
-    ap.add_argument("--one-at-a-time", action="store_true", help="batched: also step every scene alone")
+    ap.add_argument("--one-at-a-time", action="store_true", help="batched / served: also step every scene alone")
+    ap.add_argument("--requests", type=int, default=24, help="served: number of requests")
+    ap.add_argument("--rate", type=float, default=4.0, help="served: mean arrivals per second")
+    ap.add_argument("--request-steps", type=int, default=100, help="served: steps asked by each request")
+    ap.add_argument("--sample-steps", type=int, default=2, help="served: timed steps per request alone")
+    ap.add_argument("--hbm-budget-gib", type=float, default=96.0, help="served: device memory for resident scenes")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU: the contact step is not sharded across GPUs.", file=sys.stderr)
@@ -270,6 +329,8 @@ def main():
         return bench_superellipsoids(args)
     if args.workload == "batched":
         return bench_batched(args)
+    if args.workload == "served":
+        return bench_served(args)
     log("[bench] building the 131k-sphere scene")
     sc = scenes.spheres_3d(args.nx, args.ny, args.nz)
     precision = hip.FP64 if args.precision == "fp64" else hip.FP32_REFINE

@@ -209,8 +209,10 @@ int scopi_hip_kernel_timing(scopi_hip_ctx* ctx, double* out);
  * No counterpart in the reference, which steps one scene per ScopiSolver: a batch
  * advances many ScopiSolver::run loops (solver.hpp:266-303) together, each with one
  * scene's semantics (obstacles, contacts, vap, solve, update, move per step), sharing
- * one set of launches per kernel.  Scenes are 3D, share the params and dt given at
- * creation, have sphere active particles and sphere / plane obstacles.  A scene's
+ * one set of launches per kernel.  Scenes are 3D and share the params and dt given
+ * at creation; active particles are spheres or superellipsoids, obstacles spheres,
+ * planes or superellipsoids (a batch holding a superellipsoid searches every scene
+ * through the candidate path with the device closest points).  A scene's
  * results do not depend on the other scenes of the batch (bitwise: the same alone
  * and in any mix).  Scene ids are returned by add_scene and stay valid until
  * remove_scene; scenes added or removed take effect at the next call that reads or

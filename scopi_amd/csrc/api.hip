@@ -670,6 +670,8 @@ struct scopi_hip_batch {
         int64_t steps = 0;
         scopi_hip_solve_info info{};
         bool has_contacts = false;
+        bool has_se = false;       // superellipsoids among its particles
+        bool all_spheres = true;   // every active particle is a sphere
     };
     std::vector<Scene> scenes;  // packed order
     struct Staged {             // added, not packed yet (host copies)
@@ -681,6 +683,8 @@ struct scopi_hip_batch {
     std::vector<int64_t> removed;
     int64_t next_id = 1;
     bool solved = false;
+    bool has_se = false;       // any scene with superellipsoids: candidate search for the batch
+    bool all_spheres = true;   // every active particle of the batch is a sphere
     std::string err;
 };
 
@@ -829,6 +833,12 @@ void batch_pack(scopi_hip_batch* b) {
     b->search.set_scenes(R, NI, N - NI, s);
     b->nc = 0;
     b->solved = false;
+    b->has_se = false;
+    b->all_spheres = true;
+    for (const auto& sc : b->scenes) {
+        b->has_se = b->has_se || sc.has_se;
+        b->all_spheres = b->all_spheres && sc.all_spheres;
+    }
 }
 
 scopi_hip_batch::Scene* batch_find(scopi_hip_batch* b, int64_t id) {
@@ -883,15 +893,19 @@ int scopi_hip_batch_add_scene(scopi_hip_batch* b, int64_t n, int64_t n_inactive,
         if (n < 0 || n_inactive < 0 || n_inactive > n || !shape || !pos || !q || !size || !v || !vd || !omega ||
             !domega || !f || !m || !J || !scene_id)
             return bfail(b, SCOPI_HIP_ERR_ARG, "batch_add_scene: bad argument");
+        scopi_hip_batch::Staged st;
         for (int64_t k = 0; k < n; ++k) {
             const int s = shape[k];
             const bool active = k >= n_inactive;
-            if (active && s != SCOPI_SHAPE_SPHERE)
-                return bfail(b, SCOPI_HIP_ERR_UNSUPPORTED, "batch_add_scene: active particles must be spheres");
-            if (!active && s != SCOPI_SHAPE_SPHERE && s != SCOPI_SHAPE_PLANE)
-                return bfail(b, SCOPI_HIP_ERR_UNSUPPORTED, "batch_add_scene: obstacles must be spheres or planes");
+            if (s != SCOPI_SHAPE_SPHERE && s != SCOPI_SHAPE_PLANE && s != SCOPI_SHAPE_SUPERELLIPSOID)
+                return bfail(b, SCOPI_HIP_ERR_UNSUPPORTED, "batch_add_scene: 3D spheres, planes, superellipsoids");
+            if (active && s == SCOPI_SHAPE_PLANE)
+                return bfail(b, SCOPI_HIP_ERR_UNSUPPORTED, "batch_add_scene: planes must be obstacles");
+            st.sc.has_se = st.sc.has_se || s == SCOPI_SHAPE_SUPERELLIPSOID;
+            if (active) st.sc.all_spheres = st.sc.all_spheres && s == SCOPI_SHAPE_SPHERE;
         }
-        scopi_hip_batch::Staged st;
+        if (st.sc.has_se && !squareness)
+            return bfail(b, SCOPI_HIP_ERR_ARG, "batch_add_scene: superellipsoids need squareness");
         st.sc.id = b->next_id++;
         st.sc.n = n;
         st.sc.n_inactive = n_inactive;
@@ -955,10 +969,10 @@ int scopi_hip_batch_step(scopi_hip_batch* b, int64_t n_steps) {
         for (int64_t it = 0; it < n_steps; ++it) {
             if (P.n_inactive > 0)
                 hipLaunchKernelGGL(k_obstacles, dim3(grid_for(P.n_inactive)), dim3(kBlock), 0, s, P, b->dt);
-            b->nc = b->search.run(P, b->C, b->prm, s);
+            b->nc = b->search.run(P, b->C, b->prm, s, !b->has_se);
             if (na > 0) hipLaunchKernelGGL(k_vap, dim3(grid_for(na)), dim3(kBlock), 0, s, P, b->prm.vap, b->dt);
             S.d_crange = b->search.d_crange();
-            int rc = b->solver.solve_scenes(P, b->C, b->nc, b->prm, b->dt, true, s, S, infos);
+            int rc = b->solver.solve_scenes(P, b->C, b->nc, b->prm, b->dt, b->all_spheres, s, S, infos);
             if (rc != SCOPI_HIP_OK) return bfail(b, rc, "batch_step: solve failed");
             if (na > 0)
                 hipLaunchKernelGGL(k_update_move, dim3(grid_for(na)), dim3(kBlock), 0, s, P, b->solver.u(),

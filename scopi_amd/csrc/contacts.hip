@@ -375,7 +375,8 @@ __global__ void __launch_bounds__(kBlock) k_cand_emit(SeCandidates S, int64_t n,
 // One block per scene: bounding box of its active particles, then its grid (cell
 // width >= the candidate cut-off, as k_grid_setup; dense cells, at most `cells`).
 __global__ void __launch_bounds__(kBlock) k_scene_grid(Particles P, const SceneRanges* __restrict__ scenes, int kd,
-                                                      double kd_radius, double dmax, GridInfo* __restrict__ grids) {
+                                                      double kd_radius, double dmax, int shapes_closed_form,
+                                                      GridInfo* __restrict__ grids) {
     __shared__ double red[7][kBlock];
     const SceneRanges sc = scenes[blockIdx.x];
     double lo[3] = {1e300, 1e300, 1e300}, hi[3] = {-1e300, -1e300, -1e300}, rm = 0.0;
@@ -404,14 +405,19 @@ __global__ void __launch_bounds__(kBlock) k_scene_grid(Particles P, const SceneR
     GridInfo g;
     rm = red[6][0];
     double cut = (2.0 * rm + dmax) * (1.0 + 1e-9) + 1e-12;
-    if (kd) cut = fmin(cut, sqrt(kd_radius) * (1.0 + 1e-12) + 1e-300);
+    if (kd) {
+        const double kdc = sqrt(kd_radius) * (1.0 + 1e-12) + 1e-300;
+        cut = shapes_closed_form ? fmin(cut, kdc) : kdc;
+    } else if (!shapes_closed_form) {
+        cut = INFINITY;  // brute force over shapes without a distance bound: every pair
+    }
     if (!(cut > 0)) cut = 1e-300;
     double ext[3];
     int64_t nc[3];
     for (int d = 0; d < 3; ++d) {
         g.lo[d] = sc.a_hi > sc.a_lo ? red[d][0] : 0.0;
         ext[d] = sc.a_hi > sc.a_lo ? fmax(red[3 + d][0] - red[d][0], 0.0) : 0.0;
-        nc[d] = (int64_t)fmin(floor(ext[d] / cut) + 1.0, 1048576.0);
+        nc[d] = (cut == INFINITY) ? 1 : (int64_t)fmin(floor(ext[d] / cut) + 1.0, 1048576.0);
     }
     // at most `cells` cells: coarser cells along the longest axes (any width >= cut finds every pair)
     const int64_t budget = sc.cells > 1 ? sc.cells : 1;
@@ -425,7 +431,7 @@ __global__ void __launch_bounds__(kBlock) k_scene_grid(Particles P, const SceneR
     for (int d = 0; d < 3; ++d) {
         g.nc[d] = (int)nc[d];
         const bool coarse = (double)nc[d] < floor(ext[d] / cut) + 1.0;
-        g.w[d] = coarse ? fmax(cut, ext[d] / (double)nc[d] * (1.0 + 1e-12)) : cut;
+        g.w[d] = (cut == INFINITY) ? INFINITY : (coarse ? fmax(cut, ext[d] / (double)nc[d] * (1.0 + 1e-12)) : cut);
     }
     g.bits = 0;
     g.cutoff = cut;
@@ -454,7 +460,7 @@ __global__ void __launch_bounds__(kBlock) k_scene_keys(Particles P, const SceneR
 }
 
 // for_partners on a scene's dense grid (same predicates and visiting rule)
-template <bool KD, class F>
+template <bool KD, bool CAND, class F>
 __device__ inline void for_partners_dense(const Particles& P, const GridInfo& g, int64_t cell0,
                                           const int* __restrict__ start, const int* __restrict__ end,
                                           const int32_t* __restrict__ sorted, int64_t i, double kd_radius, double dmax,
@@ -479,6 +485,10 @@ __device__ inline void for_partners_dense(const Particles& P, const GridInfo& g,
                         const d3 cj = load3(P.pos, P.cap, j);
                         if (!(kd_dist2<3>(ci, cj) < kd_radius)) continue;
                     }
+                    if (CAND) {
+                        visit(j);
+                        continue;
+                    }
                     CP cp;
                     if (!closest_points<3>(P, i, j, cp)) continue;
                     if (cp.dij < dmax) visit(j);
@@ -501,15 +511,18 @@ __device__ inline int unit_scene(const int64_t* __restrict__ ulo, int K, int64_t
 }
 
 // Work units of every scene: obstacle x active pairs (flag), then one unit per
-// active particle (its partners j > i).  EMIT writes the contacts at `offsets`.
-template <bool KD, bool EMIT>
+// active particle (its partners j > i).  EMIT writes the pairs at `offsets` into
+// (out_i, out_j).  CAND: candidates of the superellipsoid path (every obstacle pair,
+// the kd-tree predicate alone for active pairs; closest points come later).
+template <bool KD, bool EMIT, bool CAND>
 __global__ void __launch_bounds__(kBlock) k_scene_units(Particles P, const SceneRanges* __restrict__ scenes,
                                                        const int64_t* __restrict__ ulo, int K,
                                                        const GridInfo* __restrict__ grids,
                                                        const int* __restrict__ start, const int* __restrict__ end,
                                                        const int32_t* __restrict__ sorted, double kd_radius,
                                                        double dmax, int* __restrict__ counts,
-                                                       const int* __restrict__ offsets, Contacts C) {
+                                                       const int* __restrict__ offsets, int32_t* __restrict__ out_i,
+                                                       int32_t* __restrict__ out_j) {
     const int64_t u = (int64_t)blockIdx.x * kBlock + threadIdx.x;
     if (u >= ulo[K]) return;
     const int sc = unit_scene(ulo, K, u);
@@ -520,11 +533,11 @@ __global__ void __launch_bounds__(kBlock) k_scene_units(Particles P, const Scene
         const int64_t o = r.ob_lo + local / na, j = r.a_lo + local % na;
         if (!EMIT) {
             CP cp;
-            counts[u] = (closest_points<3>(P, o, j, cp) && cp.dij < dmax) ? 1 : 0;
+            counts[u] = CAND ? 1 : ((closest_points<3>(P, o, j, cp) && cp.dij < dmax) ? 1 : 0);
         } else if (counts[u]) {
             const int64_t pos = offsets[u];
-            C.i[pos] = (int32_t)o;
-            C.j[pos] = (int32_t)j;
+            out_i[pos] = (int32_t)o;
+            out_j[pos] = (int32_t)j;
         }
         return;
     }
@@ -532,7 +545,9 @@ __global__ void __launch_bounds__(kBlock) k_scene_units(Particles P, const Scene
     const GridInfo g = grids[sc];
     if (!EMIT) {
         int cnt = 0;
-        if (na > 1) for_partners_dense<KD>(P, g, r.cell0, start, end, sorted, i, kd_radius, dmax, [&](int64_t) { ++cnt; });
+        if (na > 1)
+            for_partners_dense<KD, CAND>(P, g, r.cell0, start, end, sorted, i, kd_radius, dmax,
+                                         [&](int64_t) { ++cnt; });
         counts[u] = cnt;
         return;
     }
@@ -540,21 +555,29 @@ __global__ void __launch_bounds__(kBlock) k_scene_units(Particles P, const Scene
     if (cnt == 0) return;
     const int64_t base = offsets[u];
     int64_t m = 0;
-    for_partners_dense<KD>(P, g, r.cell0, start, end, sorted, i, kd_radius, dmax,
-                           [&](int64_t j) {
-                               if (m < cnt) C.j[base + m] = (int32_t)j;  // the count pass found cnt partners
-                               ++m;
-                           });
+    for_partners_dense<KD, CAND>(P, g, r.cell0, start, end, sorted, i, kd_radius, dmax, [&](int64_t j) {
+        if (m < cnt) out_j[base + m] = (int32_t)j;  // the count pass found cnt partners
+        ++m;
+    });
     for (int64_t a = 1; a < cnt; ++a) {  // insertion sort by j
-        const int32_t key = C.j[base + a];
+        const int32_t key = out_j[base + a];
         int64_t b = a - 1;
-        while (b >= 0 && C.j[base + b] > key) {
-            C.j[base + b + 1] = C.j[base + b];
+        while (b >= 0 && out_j[base + b] > key) {
+            out_j[base + b + 1] = out_j[base + b];
             --b;
         }
-        C.j[base + b + 1] = key;
+        out_j[base + b + 1] = key;
     }
-    for (int64_t a = 0; a < cnt; ++a) C.i[base + a] = (int32_t)i;
+    for (int64_t a = 0; a < cnt; ++a) out_i[base + a] = (int32_t)i;
+}
+
+// range[k] (a position in an array of n entries scanned into offs) -> its scanned offset
+__global__ void k_remap_ranges(int64_t* __restrict__ range, int n2, const int* __restrict__ offs, int64_t n,
+                               const int* __restrict__ total) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n2) return;
+    const int64_t x = range[k];
+    range[k] = (n > 0) ? (x < n ? (int64_t)offs[x] : (int64_t)*total) : 0;
 }
 
 __global__ void k_scene_crange(const int64_t* __restrict__ ulo, int K, const int* __restrict__ offsets,
@@ -800,6 +823,10 @@ BatchSearch::~BatchSearch() {
                     (void*)cell_end_, (void*)d_total_})
         if (p) (void)hipFree(p);
     if (h_io_) (void)hipHostFree(h_io_);
+    for (void* p : {(void*)cand_flags_, (void*)cand_offsets_})
+        if (p) (void)hipFree(p);
+    se_grids_free(se_grids_);
+    se_candidates_free(se_cand_);
 }
 
 void BatchSearch::set_scenes(const std::vector<SceneRanges>& scenes, int64_t n_inactive, int64_t na, hipStream_t s) {
@@ -865,14 +892,15 @@ void BatchSearch::set_scenes(const std::vector<SceneRanges>& scenes, int64_t n_i
     h_crange_.assign(2 * K, 0);
 }
 
-int64_t BatchSearch::run(const Particles& P, Contacts& C, const scopi_hip_params& prm, hipStream_t s) {
+int64_t BatchSearch::run(const Particles& P, Contacts& C, const scopi_hip_params& prm, hipStream_t s,
+                         bool closed_form) {
     const int K = K_;
     if (K == 0) return 0;
     const bool kd = prm.contact_method == SCOPI_CONTACT_KDTREE;
     const int64_t na = na_;
     if (na > 0) {
         hipLaunchKernelGGL(k_scene_grid, dim3(K), dim3(kBlock), 0, s, P, (const SceneRanges*)d_scenes_, kd ? 1 : 0,
-                           prm.kd_tree_radius, prm.dmax, d_grids_);
+                           prm.kd_tree_radius, prm.dmax, closed_form ? 1 : 0, d_grids_);
         hipLaunchKernelGGL(k_scene_keys, dim3(grid_for(na)), dim3(kBlock), 0, s, P, (const SceneRanges*)d_scenes_,
                            (const GridInfo*)d_grids_, (const int32_t*)d_scene_of_, keys_, vals_);
         int bits = 1;
@@ -883,39 +911,73 @@ int64_t BatchSearch::run(const Particles& P, Contacts& C, const scopi_hip_params
         hipLaunchKernelGGL(k_cell_range, dim3(grid_for(na)), dim3(kBlock), 0, s, keys_, na, cell_start_, cell_end_);
     }
     const unsigned gu = grid_for(std::max<int64_t>(units_, 1));
+    auto units = [&](bool emit, int32_t* oi, int32_t* oj) {
+        auto go = [&](auto kernel) {
+            hipLaunchKernelGGL(kernel, dim3(gu), dim3(kBlock), 0, s, P, (const SceneRanges*)d_scenes_,
+                               (const int64_t*)d_ulo_, K, (const GridInfo*)d_grids_, (const int*)cell_start_,
+                               (const int*)cell_end_, (const int32_t*)vals_, prm.kd_tree_radius, prm.dmax, counts_,
+                               (const int*)offsets_, oi, oj);
+        };
+        const int sel = (kd ? 1 : 0) + (emit ? 2 : 0) + (closed_form ? 0 : 4);
+        switch (sel) {
+            case 0: go(k_scene_units<false, false, false>); break;
+            case 1: go(k_scene_units<true, false, false>); break;
+            case 2: go(k_scene_units<false, true, false>); break;
+            case 3: go(k_scene_units<true, true, false>); break;
+            case 4: go(k_scene_units<false, false, true>); break;
+            case 5: go(k_scene_units<true, false, true>); break;
+            case 6: go(k_scene_units<false, true, true>); break;
+            default: go(k_scene_units<true, true, true>); break;
+        }
+    };
     if (units_ > 0) {
-        if (kd)
-            hipLaunchKernelGGL((k_scene_units<true, false>), dim3(gu), dim3(kBlock), 0, s, P,
-                               (const SceneRanges*)d_scenes_, (const int64_t*)d_ulo_, K, (const GridInfo*)d_grids_,
-                               (const int*)cell_start_, (const int*)cell_end_, (const int32_t*)vals_,
-                               prm.kd_tree_radius, prm.dmax, counts_, (const int*)offsets_, C);
-        else
-            hipLaunchKernelGGL((k_scene_units<false, false>), dim3(gu), dim3(kBlock), 0, s, P,
-                               (const SceneRanges*)d_scenes_, (const int64_t*)d_ulo_, K, (const GridInfo*)d_grids_,
-                               (const int*)cell_start_, (const int*)cell_end_, (const int32_t*)vals_,
-                               prm.kd_tree_radius, prm.dmax, counts_, (const int*)offsets_, C);
+        units(false, nullptr, nullptr);
         scanner_.exclusive(counts_, offsets_, units_, d_total_, s);
     }
+    // pair (or candidate) ranges of the scenes
     hipLaunchKernelGGL(k_scene_crange, dim3(grid_for(K, 64)), dim3(64), 0, s, (const int64_t*)d_ulo_, K,
                        (const int*)offsets_, (const int*)d_total_, d_crange_, (int64_t*)nullptr);
+    SCOPI_HIP_CHECK(hipMemcpyAsync(h_io_ + 1, d_crange_, 2 * K * sizeof(int64_t), hipMemcpyDeviceToHost, s));
+    SCOPI_HIP_CHECK(hipStreamSynchronize(s));
+    const int64_t npairs = (units_ > 0) ? h_io_[2 * K] : 0;
+    if (closed_form) {
+        for (int k = 0; k < 2 * K; ++k) h_crange_[k] = h_io_[1 + k];
+        contacts_reserve(C, std::max<int64_t>(npairs, 1));
+        if (npairs > 0) {
+            units(true, C.i, C.j);
+            hipLaunchKernelGGL(k_contact_geometry<3>, dim3(grid_for(npairs)), dim3(kBlock), 0, s, P, C, npairs,
+                               prm.mu);
+        }
+        SCOPI_HIP_CHECK(hipGetLastError());
+        return npairs;
+    }
+    // superellipsoid path: every candidate's closest points, then the dmax filter in candidate order
+    const int64_t ncand = npairs;
+    if (ncand > 0) {
+        se_candidates_reserve(se_cand_, ncand);
+        if (ncand + 1 > cand_cap_) {
+            if (cand_flags_) SCOPI_HIP_CHECK(hipFree(cand_flags_));
+            if (cand_offsets_) SCOPI_HIP_CHECK(hipFree(cand_offsets_));
+            cand_cap_ = ncand + ncand / 4 + 1024;
+            SCOPI_HIP_CHECK(hipMalloc(&cand_flags_, cand_cap_ * sizeof(int)));
+            SCOPI_HIP_CHECK(hipMalloc(&cand_offsets_, cand_cap_ * sizeof(int)));
+        }
+        units(true, se_cand_.i, se_cand_.j);
+        se_closest_points(P, se_grids_, se_cand_, ncand, s);
+        hipLaunchKernelGGL(k_cand_flags, dim3(grid_for(ncand)), dim3(kBlock), 0, s, se_cand_, ncand, prm.dmax,
+                           cand_flags_);
+        scanner_.exclusive(cand_flags_, cand_offsets_, ncand, d_total_, s);
+    }
+    hipLaunchKernelGGL(k_remap_ranges, dim3(grid_for(2 * K, 64)), dim3(64), 0, s, d_crange_, 2 * K,
+                       (const int*)cand_offsets_, ncand, (const int*)d_total_);
     SCOPI_HIP_CHECK(hipMemcpyAsync(h_io_ + 1, d_crange_, 2 * K * sizeof(int64_t), hipMemcpyDeviceToHost, s));
     SCOPI_HIP_CHECK(hipStreamSynchronize(s));
     for (int k = 0; k < 2 * K; ++k) h_crange_[k] = h_io_[1 + k];
     const int64_t nc = h_crange_[2 * K - 1];
     contacts_reserve(C, std::max<int64_t>(nc, 1));
-    if (nc > 0) {
-        if (kd)
-            hipLaunchKernelGGL((k_scene_units<true, true>), dim3(gu), dim3(kBlock), 0, s, P,
-                               (const SceneRanges*)d_scenes_, (const int64_t*)d_ulo_, K, (const GridInfo*)d_grids_,
-                               (const int*)cell_start_, (const int*)cell_end_, (const int32_t*)vals_,
-                               prm.kd_tree_radius, prm.dmax, counts_, (const int*)offsets_, C);
-        else
-            hipLaunchKernelGGL((k_scene_units<false, true>), dim3(gu), dim3(kBlock), 0, s, P,
-                               (const SceneRanges*)d_scenes_, (const int64_t*)d_ulo_, K, (const GridInfo*)d_grids_,
-                               (const int*)cell_start_, (const int*)cell_end_, (const int32_t*)vals_,
-                               prm.kd_tree_radius, prm.dmax, counts_, (const int*)offsets_, C);
-        hipLaunchKernelGGL(k_contact_geometry<3>, dim3(grid_for(nc)), dim3(kBlock), 0, s, P, C, nc, prm.mu);
-    }
+    if (nc > 0)
+        hipLaunchKernelGGL(k_cand_emit, dim3(grid_for(ncand)), dim3(kBlock), 0, s, se_cand_, ncand, cand_flags_,
+                           cand_offsets_, (int64_t)0, C, prm.mu);
     SCOPI_HIP_CHECK(hipGetLastError());
     return nc;
 }

@@ -83,8 +83,9 @@ class BatchSearch {
     ~BatchSearch();
     // Uploads the scene table (host order = packed order); call after repacking.
     void set_scenes(const std::vector<SceneRanges>& scenes, int64_t n_inactive, int64_t na, hipStream_t s);
-    // Returns nc (host sync inside); per-scene contact ranges in d_crange() / crange().
-    int64_t run(const Particles& P, Contacts& C, const scopi_hip_params& prm, hipStream_t s);
+    // Returns nc (host syncs inside); per-scene contact ranges in d_crange() / crange().
+    // closed_form false: superellipsoids present, candidate path with the device closest points.
+    int64_t run(const Particles& P, Contacts& C, const scopi_hip_params& prm, hipStream_t s, bool closed_form = true);
     const int64_t* d_crange() const { return d_crange_; }
     const std::vector<int64_t>& crange() const { return h_crange_; }
 
@@ -111,6 +112,11 @@ class BatchSearch {
     int* cell_end_ = nullptr;
     int64_t ccap_ = 0;
     int* d_total_ = nullptr;
+    SeGrids se_grids_;
+    SeCandidates se_cand_;
+    int* cand_flags_ = nullptr;
+    int* cand_offsets_ = nullptr;
+    int64_t cand_cap_ = 0;
 };
 
 // Allocates / grows the structure-of-arrays contact list (keeps nothing).

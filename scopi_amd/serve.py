@@ -1,0 +1,209 @@
+"""Serving loop over batches of scenes (continuous batching).
+
+Scene requests arrive over time (e.g. a Poisson process); each asks for a number of time
+steps of one scene.  Requests of one kind (a parameter set: sphere scenes NoFriction,
+superellipsoid scenes Friction, ...) share one resident batch (scopi_hip_batch): an
+arrived request joins its kind's batch as soon as the device memory budget allows
+(admission by each scene's estimated HBM footprint, first come first served), advances by
+one time step per step of its batch, and leaves the batch when its steps are done, which
+frees its share of the budget for the requests still waiting.  Particle state stays on the
+device between steps.  The batches of different kinds step concurrently (one host thread
+per kind, one HIP stream per batch), so their kernels overlap on the GPU.
+
+The reference has no serving layer: every scene follows ScopiSolver::run
+(include/scopi/solver.hpp:266-303) step by step, and a scene's results do not depend on
+the other scenes of its batch (bitwise), so a served scene ends exactly where it ends when
+it runs alone.
+
+Reported per request: the latency of each of its steps (the duration of the batch step
+that advanced it, packing of arrivals and departures included), its wait before admission
+and its completion time; over the run: p50 / p99 step latency and scene-steps/s.
+"""
+from __future__ import annotations
+
+import time
+from collections import deque
+from concurrent.futures import ThreadPoolExecutor
+from dataclasses import dataclass, field
+
+import numpy as np
+
+
+@dataclass
+class Request:
+    rid: int
+    kind: str
+    n: int              # particles asked for (the scene factory's size argument)
+    seed: int
+    steps: int
+    arrival: float      # seconds after the start of the run
+    # filled by the server
+    admitted: float | None = None
+    finished: float | None = None
+    done_steps: int = 0
+    step_latencies: list = field(default_factory=list)
+    result: dict | None = None
+
+
+class KindGroup:
+    """The resident batch of one kind of request."""
+
+    def __init__(self, kind, batch):
+        self.kind = kind
+        self.batch = batch
+        self.active = {}       # scene id -> (request, footprint)
+        self.last_step_s = 0.0
+
+
+class Server:
+    """Continuous-batching request loop.
+
+    kinds: {kind: make_batch()} -- a factory of the batch of each kind (HipBatch-like:
+    add(scene) -> id, remove(id), step(n), state(id));
+    make_scene(request) -> scene; footprint(request, scene) -> bytes;
+    hbm_budget: bytes the resident scenes may hold together.
+    """
+
+    def __init__(self, kinds, make_scene, footprint, hbm_budget, keep_results=False, clock=time.perf_counter,
+                 sleep=time.sleep, on_finish=None):
+        self.groups = {k: KindGroup(k, f()) for k, f in kinds.items()}
+        self.make_scene = make_scene
+        self.footprint = footprint
+        self.budget = int(hbm_budget)
+        self.keep_results = keep_results
+        self.clock = clock
+        self.sleep = sleep
+        self.used = 0
+        self.peak_used = 0
+        self.peak_active = 0
+        self.ticks = 0
+        self._built = {}
+        self.on_finish = on_finish  # called with each request when its steps are done
+
+    def _admit(self, pending, now):
+        while pending and pending[0].arrival <= now:
+            r = pending[0]
+            if r.rid not in self._built:  # the head may wait: build its scene once
+                sc = self.make_scene(r)
+                self._built[r.rid] = (sc, int(self.footprint(r, sc)))
+            scene, fp = self._built[r.rid]
+            busy = any(g.active for g in self.groups.values())
+            if self.used + fp > self.budget and busy:
+                return  # head of the queue waits for memory (first come, first served)
+            if fp > self.budget:
+                raise RuntimeError(f"request {r.rid}: footprint {fp} B exceeds the budget {self.budget} B")
+            g = self.groups[r.kind]
+            sid = g.batch.add(scene)
+            g.active[sid] = (r, fp)
+            r.admitted = now
+            self.used += fp
+            self.peak_used = max(self.peak_used, self.used)
+            pending.popleft()
+            del self._built[r.rid]
+        self.peak_active = max(self.peak_active, sum(len(g.active) for g in self.groups.values()))
+
+    def _step_group(self, g):
+        t = self.clock()
+        g.batch.step(1)
+        g.last_step_s = self.clock() - t
+        return g
+
+    def run(self, requests):
+        pending = deque(sorted(requests, key=lambda r: (r.arrival, r.rid)))
+        t0 = self.clock()
+        pool = ThreadPoolExecutor(max_workers=max(1, len(self.groups)))
+        try:
+            while pending or any(g.active for g in self.groups.values()):
+                now = self.clock() - t0
+                self._admit(pending, now)
+                live = [g for g in self.groups.values() if g.active]
+                if not live:
+                    if pending:
+                        self.sleep(max(0.0, pending[0].arrival - (self.clock() - t0)))
+                    continue
+                for g in pool.map(self._step_group, live):
+                    pass
+                self.ticks += 1
+                now = self.clock() - t0
+                for g in live:
+                    for sid, (r, fp) in list(g.active.items()):
+                        r.step_latencies.append(g.last_step_s)
+                        r.done_steps += 1
+                        if r.done_steps >= r.steps:
+                            if self.keep_results:
+                                r.result = g.batch.state(sid)
+                            g.batch.remove(sid)
+                            del g.active[sid]
+                            self.used -= fp
+                            r.finished = now
+                            if self.on_finish:
+                                self.on_finish(r)
+        finally:
+            pool.shutdown(wait=True)
+        return stats(requests, self)
+
+
+def stats(requests, server=None):
+    lat = np.concatenate([np.asarray(r.step_latencies, float) for r in requests]) if requests else np.zeros(0)
+    first = min(r.arrival for r in requests)
+    last = max(r.finished for r in requests)
+    steps = sum(r.done_steps for r in requests)
+    out = {
+        "requests": len(requests),
+        "scene_steps": steps,
+        "wall_s": last - first,
+        "scene_steps_per_s": steps / (last - first) if last > first else float("nan"),
+        "step_latency_ms": {"p50": 1e3 * float(np.percentile(lat, 50)), "p99": 1e3 * float(np.percentile(lat, 99)),
+                            "max": 1e3 * float(lat.max())},
+        "wait_ms": {"p50": 1e3 * float(np.median([r.admitted - r.arrival for r in requests])),
+                    "max": 1e3 * float(max(r.admitted - r.arrival for r in requests))},
+        "completion_s": {"p50": float(np.median([r.finished - r.arrival for r in requests])),
+                         "max": float(max(r.finished - r.arrival for r in requests))},
+    }
+    if server is not None:
+        out.update(ticks=server.ticks, peak_resident_scenes=server.peak_active, peak_budget_bytes=server.peak_used,
+                   budget_bytes=server.budget)
+    return out
+
+
+def poisson_requests(n, rate, kinds, sizes, steps, seed=0):
+    """n requests with exponential inter-arrival times (mean 1/rate s); kinds = [(kind, probability)],
+    sizes = (lo, hi): log-uniform particle counts."""
+    rng = np.random.default_rng(seed)
+    t = 0.0
+    names = [k for k, _ in kinds]
+    prob = np.array([p for _, p in kinds], float)
+    prob /= prob.sum()
+    out = []
+    for i in range(n):
+        t += rng.exponential(1.0 / rate)
+        kind = names[int(rng.choice(len(names), p=prob))]
+        size = int(round(float(np.exp(rng.uniform(np.log(sizes[0]), np.log(sizes[1]))))))
+        out.append(Request(rid=i, kind=kind, n=size, seed=int(rng.integers(1 << 30)), steps=steps, arrival=t))
+    return out
+
+
+def served_scene(r: Request):
+    """The scene of a request: the 131k-sphere recipe scaled to r.n spheres (kind "spheres"), or the
+    32k-superellipsoid recipe on a lattice of about r.n sites (kind "superellipsoids")."""
+    from scopi_amd import scenes
+    if r.kind == "spheres":
+        return scenes.spheres_3d_count(r.n, seed=r.seed, exact=True)
+    side = max(2, int(round(r.n ** (1.0 / 3.0))))
+    return scenes.superellipsoids_3d(nx=side, ny=side, nz=max(1, int(round(r.n / (side * side)))), seed=r.seed)
+
+
+def hip_kinds(dt=1e-3, sphere_precision=None):
+    """Batch factories of the two request kinds of the served workload: sphere scenes
+    (NoFriction, vap_fpd, dmax 0.02, kd_tree_radius 0.042^2) and superellipsoid scenes
+    (Friction mu 0.3, dmax 0.02, kd_tree_radius (2 sqrt(3) 0.015 + 0.02)^2), APGD tol 1e-6."""
+    from scopi_amd import hip, scenes
+    c3 = scenes.CONFIG3
+    sp = hip.params(vap=hip.VAP_FPD, dmax=0.02, kd_tree_radius=0.042 ** 2, tolerance=1e-6,
+                    precision=hip.FP64 if sphere_precision is None else sphere_precision)
+    se = hip.params(problem=hip.FRICTION, mu=c3["mu"], vap=hip.VAP_FPD, dmax=c3["dmax"],
+                    kd_tree_radius=c3["kd_tree_radius"], tolerance=c3["tolerance"])
+    prm = {"spheres": sp, "superellipsoids": se}
+    kinds = {k: (lambda p=p: hip.HipBatch(p, dt)) for k, p in prm.items()}
+    footprint = lambda r, sc: hip.scene_footprint(sc.n, sc.n_inactive, prm[r.kind])
+    return kinds, footprint, prm
