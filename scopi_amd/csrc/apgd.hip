@@ -684,11 +684,14 @@ __device__ inline void apgd_control(ApgdCtrl* ctrl) {
 __device__ inline void control_step(const ApgdCtrl* __restrict__ in, ApgdCtrl* __restrict__ out,
                                     const double* __restrict__ partial_row, int nrow, ApgdCtrl& st,
                                     double (*lds)[kBlock / 64], bool leader, int32_t* done_flag) {
+    // the row kernel's block sums are read before the state is known to need them, so
+    // their loads overlap the state's (same summation order as reduce_partials)
+    double tot[1] = {0.0};
+    for (int b = threadIdx.x; b < nrow; b += kBlock) tot[0] += partial_row[b * kPartialStride];
     if (threadIdx.x == 0) st = *in;
     __syncthreads();
     if (st.pending && !st.done) {
-        double tot[1];
-        reduce_partials<1>(partial_row, nrow, lds, tot);
+        block_sum<1>(tot, lds);
         if (threadIdx.x == 0) {
             st.red[7] = tot[0];
             apgd_control(&st);

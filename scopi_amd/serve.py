@@ -21,9 +21,9 @@ and its completion time; over the run: p50 / p99 step latency and scene-steps/s.
 """
 from __future__ import annotations
 
+import threading
 import time
 from collections import deque
-from concurrent.futures import ThreadPoolExecutor
 from dataclasses import dataclass, field
 
 import numpy as np
@@ -46,13 +46,14 @@ class Request:
 
 
 class KindGroup:
-    """The resident batch of one kind of request."""
+    """The resident batch of one kind of request and its step loop."""
 
     def __init__(self, kind, batch):
         self.kind = kind
         self.batch = batch
         self.active = {}       # scene id -> (request, footprint)
-        self.last_step_s = 0.0
+        self.inbox = []        # admitted (request, scene, footprint), not in the batch yet
+        self.steps = 0
 
 
 class Server:
@@ -62,6 +63,10 @@ class Server:
     add(scene) -> id, remove(id), step(n), state(id));
     make_scene(request) -> scene; footprint(request, scene) -> bytes;
     hbm_budget: bytes the resident scenes may hold together.
+
+    The calling thread admits arrived requests (first come, first served, while the budget
+    allows); every kind's batch steps in a thread of its own, taking in its admitted
+    requests between two steps and releasing the budget of the finished ones.
     """
 
     def __init__(self, kinds, make_scene, footprint, hbm_budget, keep_results=False, clock=time.perf_counter,
@@ -73,73 +78,98 @@ class Server:
         self.keep_results = keep_results
         self.clock = clock
         self.sleep = sleep
+        self.on_finish = on_finish  # called with each request when its steps are done
         self.used = 0
         self.peak_used = 0
         self.peak_active = 0
         self.ticks = 0
-        self._built = {}
-        self.on_finish = on_finish  # called with each request when its steps are done
+        self._cv = threading.Condition()
+        self._closed = False
+        self._error = None
+        self._t0 = 0.0
 
-    def _admit(self, pending, now):
-        while pending and pending[0].arrival <= now:
-            r = pending[0]
-            if r.rid not in self._built:  # the head may wait: build its scene once
-                sc = self.make_scene(r)
-                self._built[r.rid] = (sc, int(self.footprint(r, sc)))
-            scene, fp = self._built[r.rid]
-            busy = any(g.active for g in self.groups.values())
-            if self.used + fp > self.budget and busy:
-                return  # head of the queue waits for memory (first come, first served)
-            if fp > self.budget:
-                raise RuntimeError(f"request {r.rid}: footprint {fp} B exceeds the budget {self.budget} B")
-            g = self.groups[r.kind]
-            sid = g.batch.add(scene)
-            g.active[sid] = (r, fp)
-            r.admitted = now
-            self.used += fp
-            self.peak_used = max(self.peak_used, self.used)
-            pending.popleft()
-            del self._built[r.rid]
-        self.peak_active = max(self.peak_active, sum(len(g.active) for g in self.groups.values()))
+    def _now(self):
+        return self.clock() - self._t0
 
-    def _step_group(self, g):
-        t = self.clock()
-        g.batch.step(1)
-        g.last_step_s = self.clock() - t
-        return g
+    def _group_loop(self, g):
+        try:
+            while True:
+                with self._cv:
+                    while not g.inbox and not g.active and not self._closed:
+                        self._cv.wait(0.05)
+                    if self._closed and not g.inbox and not g.active:
+                        return
+                    incoming, g.inbox = g.inbox, []
+                for r, scene, fp in incoming:
+                    g.active[g.batch.add(scene)] = (r, fp)
+                if not g.active:
+                    continue
+                t = self.clock()
+                g.batch.step(1)
+                dt = self.clock() - t
+                g.steps += 1
+                now = self._now()
+                freed = 0
+                for sid, (r, fp) in list(g.active.items()):
+                    r.step_latencies.append(dt)
+                    r.done_steps += 1
+                    if r.done_steps >= r.steps:
+                        if self.keep_results:
+                            r.result = g.batch.state(sid)
+                        g.batch.remove(sid)
+                        del g.active[sid]
+                        freed += fp
+                        r.finished = now
+                        if self.on_finish:
+                            self.on_finish(r)
+                with self._cv:
+                    self.used -= freed
+                    self.ticks += 1
+                    self._cv.notify_all()
+        except BaseException as e:  # surfaced by run()
+            with self._cv:
+                self._error = e
+                self._closed = True
+                self._cv.notify_all()
 
     def run(self, requests):
         pending = deque(sorted(requests, key=lambda r: (r.arrival, r.rid)))
-        t0 = self.clock()
-        pool = ThreadPoolExecutor(max_workers=max(1, len(self.groups)))
+        self._t0 = self.clock()
+        threads = [threading.Thread(target=self._group_loop, args=(g,), daemon=True) for g in self.groups.values()]
+        for t in threads:
+            t.start()
         try:
-            while pending or any(g.active for g in self.groups.values()):
-                now = self.clock() - t0
-                self._admit(pending, now)
-                live = [g for g in self.groups.values() if g.active]
-                if not live:
-                    if pending:
-                        self.sleep(max(0.0, pending[0].arrival - (self.clock() - t0)))
+            while pending:
+                r = pending[0]
+                wait = r.arrival - self._now()
+                if wait > 0:
+                    self.sleep(min(wait, 0.05))
                     continue
-                for g in pool.map(self._step_group, live):
-                    pass
-                self.ticks += 1
-                now = self.clock() - t0
-                for g in live:
-                    for sid, (r, fp) in list(g.active.items()):
-                        r.step_latencies.append(g.last_step_s)
-                        r.done_steps += 1
-                        if r.done_steps >= r.steps:
-                            if self.keep_results:
-                                r.result = g.batch.state(sid)
-                            g.batch.remove(sid)
-                            del g.active[sid]
-                            self.used -= fp
-                            r.finished = now
-                            if self.on_finish:
-                                self.on_finish(r)
+                scene = self.make_scene(r)
+                fp = int(self.footprint(r, scene))
+                if fp > self.budget:
+                    raise RuntimeError(f"request {r.rid}: footprint {fp} B exceeds the budget {self.budget} B")
+                with self._cv:
+                    while self.used + fp > self.budget and self._error is None:
+                        self._cv.wait(0.05)  # head of the queue waits for memory
+                    if self._error is not None:
+                        break
+                    self.used += fp
+                    self.peak_used = max(self.peak_used, self.used)
+                    r.admitted = self._now()
+                    self.groups[r.kind].inbox.append((r, scene, fp))
+                    resident = sum(len(g.active) + len(g.inbox) for g in self.groups.values())
+                    self.peak_active = max(self.peak_active, resident)
+                    self._cv.notify_all()
+                pending.popleft()
         finally:
-            pool.shutdown(wait=True)
+            with self._cv:
+                self._closed = True
+                self._cv.notify_all()
+            for t in threads:
+                t.join()
+        if self._error is not None:
+            raise self._error
         return stats(requests, self)
 
 
