@@ -44,6 +44,7 @@ inline int64_t row_blocks(int64_t nc, int cap) {
 }
 constexpr int kStatusStalled = 2;                // FP32 phase reached its precision floor
 constexpr int kMaxConsecutiveRejections32 = 40;  // 2^40 growth of L without an accepted step
+constexpr int64_t kMaxStagnation32 = 300;        // FP32 accepted iterations without halving the residual
 
 // ------------------------------------------------------------------ reductions
 // Sum of NV values over the block; the totals land in thread 0.
@@ -323,85 +324,147 @@ __global__ void __launch_bounds__(kBlock) k_trial(const ApgdCtrl* __restrict__ i
     using V2 = typename std::conditional<sizeof(T) == 8, double2, float2>::type;
     const int64_t hi = sb.hi;
     const bool aligned = (sb.lo & 1) == 0;
-    for (int64_t c0 = sb.lo + 2 * ((int64_t)sb.lb * kBlock + threadIdx.x); c0 < hi; c0 += 2 * (int64_t)sb.nb * kBlock) {
-        T l0[2 * NR], l1[2 * NR], q0[2 * NR], q1[2 * NR], cc[2 * NR];
-        const int m = (c0 + 1 < hi) ? 2 : 1;
-        if (m == 2 && aligned) {
+    if constexpr (NR == 1) {
+        // one row per contact: four consecutive contacts per lane and iteration, loaded as two
+        // aligned pairs when the scene starts on an even contact; all loads issued together
+        const int64_t step4 = 4 * (int64_t)sb.nb * kBlock;
+        for (int64_t c0 = sb.lo + 4 * ((int64_t)sb.lb * kBlock + threadIdx.x); c0 < hi; c0 += step4) {
+            T l0[4], l1[4], q0[4], q1[4], cc[4];
+            const int m = (hi - c0) < 4 ? (int)(hi - c0) : 4;
+            if (m == 4 && aligned) {
 #pragma unroll
-            for (int h = 0; h < NR; ++h) {
-                const int64_t r = c0 * NR + 2 * h;
-                const V2 a = *reinterpret_cast<const V2*>(lc + r), b = *reinterpret_cast<const V2*>(lp + r);
-                const V2 e = *reinterpret_cast<const V2*>(qc + r), f = *reinterpret_cast<const V2*>(qp + r);
-                const V2 g = *reinterpret_cast<const V2*>(B.c + r);
-                l0[2 * h] = a.x, l0[2 * h + 1] = a.y, l1[2 * h] = b.x, l1[2 * h + 1] = b.y;
-                q0[2 * h] = e.x, q0[2 * h + 1] = e.y, q1[2 * h] = f.x, q1[2 * h + 1] = f.y;
-                cc[2 * h] = g.x, cc[2 * h + 1] = g.y;
-            }
-        } else {
-            for (int o = 0; o < m * NR; ++o) {
-                const int64_t r = c0 * NR + o;
-                l0[o] = lc[r], l1[o] = lp[r], q0[o] = qc[r], q1[o] = qp[r], cc[o] = B.c[r];
-            }
-        }
-        T zz[2 * NR];
-        for (int k = 0; k < m; ++k) {
-            const int64_t c = c0 + k;
-            T y[NR], g[NR], z[NR], qy[NR];
-#pragma unroll
-            for (int d = 0; d < NR; ++d) {
-                const int o = k * NR + d;
-                y[d] = l0[o] + beta * (l0[o] - l1[o]);
-                qy[d] = q0[o] + beta * (q0[o] - q1[o]);
-                g[d] = qy[d] + cc[o];
-                z[d] = y[d] - alpha * g[d];
-            }
-            if (LAY != kCone) {
-                z[0] = z[0] > (T)0 ? z[0] : (T)0;  // max(lambda, 0): lagrange_multiplier.hpp:104-107
+                for (int h = 0; h < 2; ++h) {
+                    const int64_t r = c0 + 2 * h;
+                    const V2 a = *reinterpret_cast<const V2*>(lc + r), b = *reinterpret_cast<const V2*>(lp + r);
+                    const V2 e = *reinterpret_cast<const V2*>(qc + r), f = *reinterpret_cast<const V2*>(qp + r);
+                    const V2 g = *reinterpret_cast<const V2*>(B.c + r);
+                    l0[2 * h] = a.x, l0[2 * h + 1] = a.y, l1[2 * h] = b.x, l1[2 * h + 1] = b.y;
+                    q0[2 * h] = e.x, q0[2 * h + 1] = e.y, q1[2 * h] = f.x, q1[2 * h + 1] = f.y;
+                    cc[2 * h] = g.x, cc[2 * h + 1] = g.y;
+                }
             } else {
-                // Coulomb-cone projection: lagrange_multiplier.hpp:243-268
-                const T nx = B.coef[c * NC + 0], ny = B.coef[c * NC + 1], nz = B.coef[c * NC + 2];
-                const T mu = B.mu[c];
-                const T ln = z[0] * nx + z[1] * ny + z[2] * nz;
-                T tx = z[0] - ln * nx, ty = z[1] - ln * ny, tz = z[2] - ln * nz;
-                const T norm = sqrt(tx * tx + ty * ty + tz * tz);
-                if (norm > mu * ln) {
-                    if (ln <= -mu * norm) {
-                        z[0] = z[1] = z[2] = (T)0;
-                    } else {
-                        const T nn = (mu * mu) * (norm + ln / mu) / (mu * mu + (T)1);
-                        const T nl = nn / mu;
-                        tx /= norm, ty /= norm, tz /= norm;
-                        z[0] = nn * tx + nl * nx;
-                        z[1] = nn * ty + nl * ny;
-                        z[2] = nn * tz + nl * nz;
-                    }
+#pragma unroll
+                for (int o = 0; o < 4; ++o) {
+                    const int64_t r = c0 + (o < m ? o : 0);
+                    l0[o] = lc[r], l1[o] = lp[r], q0[o] = qc[r], q1[o] = qp[r], cc[o] = B.c[r];
                 }
             }
+            T zz[4];
 #pragma unroll
-            for (int d = 0; d < NR; ++d) {
-                const int o = k * NR + d;
-                zz[o] = z[d];
-                const double dy = (double)z[d] - (double)y[d], dl = (double)z[d] - (double)l0[o];
-                s[0] += (double)g[d] * dy;
-                s[1] += dy * dy;
-                s[2] += (double)g[d] * dl;
-                s[3] += dl * dl;
-                s[4] += (double)y[d] * (double)qy[d];
-                s[5] += (double)cc[o] * (double)y[d];
-                s[6] += (double)cc[o] * (double)z[d];
-                s[7] += (double)z[d] * (double)z[d];
+            for (int k = 0; k < 4; ++k) {
+                const T y = l0[k] + beta * (l0[k] - l1[k]);
+                const T qy = q0[k] + beta * (q0[k] - q1[k]);
+                const T g = qy + cc[k];
+                T z = y - alpha * g;
+                z = z > (T)0 ? z : (T)0;  // max(lambda, 0): lagrange_multiplier.hpp:104-107
+                zz[k] = z;
+                if (k < m) {
+                    const double dy = (double)z - (double)y, dl = (double)z - (double)l0[k];
+                    s[0] += (double)g * dy;
+                    s[1] += dy * dy;
+                    s[2] += (double)g * dl;
+                    s[3] += dl * dl;
+                    s[4] += (double)y * (double)qy;
+                    s[5] += (double)cc[k] * (double)y;
+                    s[6] += (double)cc[k] * (double)z;
+                    s[7] += (double)z * (double)z;
+                }
+            }
+            if (m == 4 && aligned) {
+#pragma unroll
+                for (int h = 0; h < 2; ++h) {
+                    V2 o;
+                    o.x = zz[2 * h];
+                    o.y = zz[2 * h + 1];
+                    *reinterpret_cast<V2*>(lt + c0 + 2 * h) = o;
+                }
+            } else {
+#pragma unroll
+                for (int o = 0; o < 4; ++o)
+                    if (o < m) lt[c0 + o] = zz[o];
             }
         }
-        if (m == 2 && aligned) {
-#pragma unroll
-            for (int h = 0; h < NR; ++h) {
-                V2 o;
-                o.x = zz[2 * h];
-                o.y = zz[2 * h + 1];
-                *reinterpret_cast<V2*>(lt + c0 * NR + 2 * h) = o;
+    } else {
+        for (int64_t c0 = sb.lo + 2 * ((int64_t)sb.lb * kBlock + threadIdx.x); c0 < hi; c0 += 2 * (int64_t)sb.nb * kBlock) {
+            T l0[2 * NR], l1[2 * NR], q0[2 * NR], q1[2 * NR], cc[2 * NR];
+            const int m = (c0 + 1 < hi) ? 2 : 1;
+            if (m == 2 && aligned) {
+    #pragma unroll
+                for (int h = 0; h < NR; ++h) {
+                    const int64_t r = c0 * NR + 2 * h;
+                    const V2 a = *reinterpret_cast<const V2*>(lc + r), b = *reinterpret_cast<const V2*>(lp + r);
+                    const V2 e = *reinterpret_cast<const V2*>(qc + r), f = *reinterpret_cast<const V2*>(qp + r);
+                    const V2 g = *reinterpret_cast<const V2*>(B.c + r);
+                    l0[2 * h] = a.x, l0[2 * h + 1] = a.y, l1[2 * h] = b.x, l1[2 * h + 1] = b.y;
+                    q0[2 * h] = e.x, q0[2 * h + 1] = e.y, q1[2 * h] = f.x, q1[2 * h + 1] = f.y;
+                    cc[2 * h] = g.x, cc[2 * h + 1] = g.y;
+                }
+            } else {
+                for (int o = 0; o < m * NR; ++o) {
+                    const int64_t r = c0 * NR + o;
+                    l0[o] = lc[r], l1[o] = lp[r], q0[o] = qc[r], q1[o] = qp[r], cc[o] = B.c[r];
+                }
             }
-        } else {
-            for (int o = 0; o < m * NR; ++o) lt[c0 * NR + o] = zz[o];
+            T zz[2 * NR];
+            for (int k = 0; k < m; ++k) {
+                const int64_t c = c0 + k;
+                T y[NR], g[NR], z[NR], qy[NR];
+    #pragma unroll
+                for (int d = 0; d < NR; ++d) {
+                    const int o = k * NR + d;
+                    y[d] = l0[o] + beta * (l0[o] - l1[o]);
+                    qy[d] = q0[o] + beta * (q0[o] - q1[o]);
+                    g[d] = qy[d] + cc[o];
+                    z[d] = y[d] - alpha * g[d];
+                }
+                if (LAY != kCone) {
+                    z[0] = z[0] > (T)0 ? z[0] : (T)0;  // max(lambda, 0): lagrange_multiplier.hpp:104-107
+                } else {
+                    // Coulomb-cone projection: lagrange_multiplier.hpp:243-268
+                    const T nx = B.coef[c * NC + 0], ny = B.coef[c * NC + 1], nz = B.coef[c * NC + 2];
+                    const T mu = B.mu[c];
+                    const T ln = z[0] * nx + z[1] * ny + z[2] * nz;
+                    T tx = z[0] - ln * nx, ty = z[1] - ln * ny, tz = z[2] - ln * nz;
+                    const T norm = sqrt(tx * tx + ty * ty + tz * tz);
+                    if (norm > mu * ln) {
+                        if (ln <= -mu * norm) {
+                            z[0] = z[1] = z[2] = (T)0;
+                        } else {
+                            const T nn = (mu * mu) * (norm + ln / mu) / (mu * mu + (T)1);
+                            const T nl = nn / mu;
+                            tx /= norm, ty /= norm, tz /= norm;
+                            z[0] = nn * tx + nl * nx;
+                            z[1] = nn * ty + nl * ny;
+                            z[2] = nn * tz + nl * nz;
+                        }
+                    }
+                }
+    #pragma unroll
+                for (int d = 0; d < NR; ++d) {
+                    const int o = k * NR + d;
+                    zz[o] = z[d];
+                    const double dy = (double)z[d] - (double)y[d], dl = (double)z[d] - (double)l0[o];
+                    s[0] += (double)g[d] * dy;
+                    s[1] += dy * dy;
+                    s[2] += (double)g[d] * dl;
+                    s[3] += dl * dl;
+                    s[4] += (double)y[d] * (double)qy[d];
+                    s[5] += (double)cc[o] * (double)y[d];
+                    s[6] += (double)cc[o] * (double)z[d];
+                    s[7] += (double)z[d] * (double)z[d];
+                }
+            }
+            if (m == 2 && aligned) {
+    #pragma unroll
+                for (int h = 0; h < NR; ++h) {
+                    V2 o;
+                    o.x = zz[2 * h];
+                    o.y = zz[2 * h + 1];
+                    *reinterpret_cast<V2*>(lt + c0 * NR + 2 * h) = o;
+                }
+            } else {
+                for (int o = 0; o < m * NR; ++o) lt[c0 * NR + o] = zz[o];
+            }
         }
     }
     block_sum<8>(s, lds);
@@ -425,8 +488,16 @@ __global__ void __launch_bounds__(kBlock) k_bt(ApgdCtrl* __restrict__ ctrl, int 
     const bool multi = M.bblk != nullptr;
     const int sc = multi ? M.bblk[blockIdx.x] : 0;
     const int lb = multi ? (int)blockIdx.x - M.bfirst[sc] : (int)blockIdx.x;
+    const int64_t p0 = multi ? M.arange[2 * sc] : 0, p1 = multi ? M.arange[2 * sc + 1] : na;
+    const int64_t p = p0 + ((int64_t)lb * kBlock + threadIdx.x) / G;
+    const int sub = (int)(threadIdx.x % G);
+    const bool valid = p < p1;
+    // the state and the particle's entry range are loaded together: neither waits for the other
     if (ctrl) ctrl += sc;
-    if (ctrl && ctrl->done) return;
+    const int done = ctrl ? ctrl->done : 0;
+    const int s = (use_slot >= 0) ? use_slot : (ctrl ? ctrl->trial : 0);
+    const int e0 = valid ? tstart[p] : 0, e1 = valid ? tstart[p + 1] : 0;
+    if (done) return;
     if (ctrl && partial_trial && lb == 0) {
         double tot[8];
         if (multi)
@@ -438,16 +509,10 @@ __global__ void __launch_bounds__(kBlock) k_bt(ApgdCtrl* __restrict__ ctrl, int 
             ctrl->red[8] = tot[7];
         }
     }
-    const int64_t p0 = multi ? M.arange[2 * sc] : 0, p1 = multi ? M.arange[2 * sc + 1] : na;
-    const int64_t p = p0 + ((int64_t)lb * kBlock + threadIdx.x) / G;
-    const int sub = (int)(threadIdx.x % G);
-    const bool valid = p < p1;
-    const int s = (use_slot >= 0) ? use_slot : ctrl->trial;
     const T* lam = slot<T>(B.lam, s, NR * B.cap);
     T acc[NW];
 #pragma unroll
     for (int k = 0; k < NW; ++k) acc[k] = (T)0;
-    const int e0 = valid ? tstart[p] : 0, e1 = valid ? tstart[p + 1] : 0;
     // two entries per iteration, every load issued before the first use (the same
     // accumulation order as one entry at a time)
     for (int e = e0 + sub; e < e1; e += 2 * G) {
@@ -557,23 +622,51 @@ __global__ void __launch_bounds__(kBlock) k_row(ApgdCtrl* __restrict__ ctrl, BMa
     __shared__ double lds[1][kBlock / 64];
     const SceneBlock sb = scene_block(M, nc);
     ctrl += sb.sc;
-    if (ctrl->done) return;
+    // the state and the first contact's particle indices are loaded together; the loop
+    // then loads each next contact's indices one iteration ahead
+    const int done = ctrl->done;
     const int trial = ctrl->trial;
-    const T dt2 = (T)ctrl->dt2;
+    const double dt2d = ctrl->dt2;
+    const int64_t cstep = (int64_t)sb.nb * kBlock;
+    int64_t c = sb.lo + (int64_t)sb.lb * kBlock + threadIdx.x;
+    int32_t in_ = 0, jn_ = 0;
+    if (c < sb.hi) in_ = ci[c], jn_ = cj[c];
+    if (done) return;
+    const T dt2 = (T)dt2d;
     const int64_t stride = NR * B.cap;
     const T* lt = slot<T>(B.lam, trial, stride);
     T* qt = slot<T>(B.qlam, trial, stride);
     if (sb.lb == 0 && threadIdx.x == 0) ctrl->pending = 1;
     double s[1] = {0.0};
-    for (int64_t c = sb.lo + (int64_t)sb.lb * kBlock + threadIdx.x; c < sb.hi; c += (int64_t)sb.nb * kBlock) {
-        const int64_t i = (int64_t)ci[c] - ni, j = (int64_t)cj[c] - ni;
-        T q[3];
-        row_product<T, LAY>(B, c, i, j, dt2, q);
+    // two contacts (c, c + cstep) per iteration: both products' loads are issued before
+    // either result is stored (the second is a duplicate of the first past the end)
+    int32_t in1 = in_, jn1 = jn_;
+    if (c + cstep < sb.hi) in1 = ci[c + cstep], jn1 = cj[c + cstep];
+    for (; c < sb.hi; c += 2 * cstep) {
+        const int64_t c1 = c + cstep;
+        const bool h1 = c1 < sb.hi;
+        const int64_t i0 = (int64_t)in_ - ni, j0 = (int64_t)jn_ - ni;
+        const int64_t i1 = (int64_t)in1 - ni, j1 = (int64_t)jn1 - ni;
+        const int64_t cn = c + 2 * cstep;
+        if (cn < sb.hi) in_ = ci[cn], jn_ = cj[cn];
+        if (cn + cstep < sb.hi) in1 = ci[cn + cstep], jn1 = cj[cn + cstep];
+        T q0[3], q1[3];
+        row_product<T, LAY>(B, c, i0, j0, dt2, q0);
+        row_product<T, LAY>(B, h1 ? c1 : c, i1, j1, dt2, q1);
+        T l0[NR], l1[NR];
+#pragma unroll
+        for (int d = 0; d < NR; ++d) l0[d] = lt[c * NR + d], l1[d] = lt[(h1 ? c1 : c) * NR + d];
 #pragma unroll
         for (int d = 0; d < NR; ++d) {
-            qt[c * NR + d] = q[d];
-            s[0] += (double)lt[c * NR + d] * (double)q[d];
+            qt[c * NR + d] = q0[d];
+            s[0] += (double)l0[d] * (double)q0[d];
         }
+        if (h1)
+#pragma unroll
+            for (int d = 0; d < NR; ++d) {
+                qt[c1 * NR + d] = q1[d];
+                s[0] += (double)l1[d] * (double)q1[d];
+            }
     }
     block_sum<1>(s, lds);
     if (threadIdx.x == 0) partial[blockIdx.x * kPartialStride] = s[0];
@@ -645,12 +738,23 @@ __device__ inline void apgd_control(ApgdCtrl* ctrl) {
     ctrl->iter += 1;
     ctrl->res = sqrt(r[3]);
     const int old_prev = ctrl->prev;
-    if (ctrl->res < ctrl->tol || ctrl->res < ctrl->rtol * sqrt(r[8])) {
+    // the FP32 phase also ends when the residual has stopped decreasing (its precision floor)
+    bool stagnant = false;
+    if (ctrl->low_precision) {
+        if (!(ctrl->res >= 0.5 * ctrl->best_res)) {
+            ctrl->best_res = ctrl->res;
+            ctrl->since_best = 0;
+        } else {
+            ctrl->since_best += 1;
+            stagnant = ctrl->since_best > kMaxStagnation32;
+        }
+    }
+    if (ctrl->res < ctrl->tol || ctrl->res < ctrl->rtol * sqrt(r[8]) || stagnant) {
         ctrl->prev = ctrl->cur;
         ctrl->cur = ctrl->trial;
         ctrl->trial = old_prev;
         ctrl->done = 1;
-        ctrl->status = 0;
+        ctrl->status = stagnant ? kStatusStalled : 0;
         return;
     }
     double theta = ctrl->theta, theta1 = 1.0, beta = 0.0;
@@ -752,6 +856,8 @@ __global__ void k_ctrl_init(ApgdCtrl* ctrl, const ApgdCtrl* last, double alpha, 
     ctrl->rtol = rtol;
     ctrl->dt2 = dt2;
     ctrl->res = INFINITY;
+    ctrl->best_res = INFINITY;
+    ctrl->since_best = 0;
     for (int k = 0; k < 9; ++k) ctrl->red[k] = 0.0;
     ctrl->iter = 0;
     ctrl->backtracks = 0;
@@ -800,6 +906,8 @@ __global__ void k_ctrl_init_scenes(ApgdCtrl* ctrl, const int32_t* __restrict__ l
     c.rtol = rtol;
     c.dt2 = dt2;
     c.res = 0.0;
+    c.best_res = INFINITY;
+    c.since_best = 0;
     for (int k = 0; k < 9; ++k) c.red[k] = 0.0;
     c.iter = 0;
     c.backtracks = 0;

@@ -43,6 +43,8 @@ struct ApgdCtrl {
     double L_acc;        // L of the last accepted trial (before the 0.97 decay)
     double tol, rtol, dt2;
     double res;          // last ||lambda_{k+1} - lambda_k||
+    double best_res;     // FP32 phase: smallest res so far, to within a factor 2
+    int64_t since_best;  // FP32 phase: accepted iterations since best_res last halved
     double red[9];       // sums of the trial kernel (0..6, 8) and of the row kernel (7)
     int64_t iter, backtracks, max_ite, q_applies, consec_bt;
     int32_t cur, prev, trial;

@@ -145,7 +145,7 @@ def test_solve_adaptive_vs_exact_optimum(name, precision):
     prm = case_params(case, precision=precision)
     say(f"[solve-adaptive] {name} precision={precision}")
     u, info, nc = gpu_first_step(case, prm)
-    assert nc == int(GOLDEN[name + "/nc"]) and info["converged"] == 1
+    assert nc == int(GOLDEN[name + "/nc"]) and info["converged"] == 1, info
     e_gpu = float(np.abs(u - GOLDEN[name + "/u_exact"]).max())
     e_orc = float(GOLDEN[name + "/e_oracle"])
     say(f"  e_gpu={e_gpu:.3e} e_oracle={e_orc:.3e} iterations={info['iterations']} fp32={info['iterations_fp32']}")
