@@ -302,6 +302,8 @@ def main():
     ap.add_argument("--ny", type=int, default=64)
     ap.add_argument("--nz", type=int, default=32)
     ap.add_argument("--tune", action="store_true", help="sweep the solver's launch shapes before timing")
+    ap.add_argument("--graph-rounds", type=int, default=0,
+                    help="replay the projected-gradient rounds as a hipGraph of this many rounds (odd; 0: eager)")
     ap.add_argument("--kernel-steps", type=int, default=3,
                     help="steps of the instrumented pass (HIP events around every solver kernel)")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
@@ -340,6 +342,8 @@ def main():
     if args.calibrate_copy:
         copy_calibration(args.calibrate_copy)
     g = hip.HipScene(sc, capacity_contacts=20 * sc.n)
+    if args.graph_rounds:
+        g.set_tuning(hip.TUNE_GRAPH_ROUNDS, args.graph_rounds)
     log(f"[bench] warmup {args.warmup} steps")
     if args.warmup:
         g.run(prm, dt, args.warmup)
@@ -418,7 +422,8 @@ def main():
         "data": "synthetic",
         "config": {"workload": f"{na} spheres ({args.nx}x{args.ny}x{args.nz} jittered lattice) + 5 planes, "
                                "NoFriction APGD tol 1e-6, dt 1e-3, dmax 0.02, kd_tree_radius 0.042^2",
-                   "n_spheres": na, "n_contacts_last_step": nc, "precision": args.precision},
+                   "n_spheres": na, "n_contacts_last_step": nc, "precision": args.precision,
+                   "graph_rounds": args.graph_rounds},
         "contact_iterations_per_s": nc * info["iterations"] / elapsed,
         "apgd_iterations_per_step": it_per_step,
         "line_search_rejections_per_step": bt_per_step,

@@ -200,8 +200,11 @@ int scopi_hip_newton_probe(scopi_hip_ctx* ctx, int32_t kind, const double* in, d
 /* Launch shapes of the projected-gradient kernels (the hook the autotuner in
  * scopi_amd.tune drives; no counterpart in the reference):
  *   SCOPI_TUNE_REDUCE_BLOCKS: blocks of 256 threads of the per-contact kernels (1..2048, default 1024);
- *   SCOPI_TUNE_BT_GROUP: lanes sharing one particle in the B^T gather (1, 2, 4, 8 or 16, default 8). */
-enum { SCOPI_TUNE_REDUCE_BLOCKS = 1, SCOPI_TUNE_BT_GROUP = 2 };
+ *   SCOPI_TUNE_BT_GROUP: lanes sharing one particle in the B^T gather (1, 2, 4, 8 or 16, default 8);
+ *   SCOPI_TUNE_GRAPH_ROUNDS: 0 (default) launches the projected-gradient rounds one by one; an odd
+ *     value R captures R rounds, the control step and the state's copy to the host in a hipGraph
+ *     once per solve phase and replays it until the loop is done (same kernels, same results). */
+enum { SCOPI_TUNE_REDUCE_BLOCKS = 1, SCOPI_TUNE_BT_GROUP = 2, SCOPI_TUNE_GRAPH_ROUNDS = 3 };
 int scopi_hip_set_tuning(scopi_hip_ctx* ctx, int32_t key, int32_t value);
 int scopi_hip_kernel_timing(scopi_hip_ctx* ctx, double* out);
 

@@ -1081,7 +1081,40 @@ void Solver::run_phase(BMatrix<T>& B, const Particles& P, const Contacts& C, int
     int64_t before = 0;
     int batch = 2;
     int cur = 0;  // copy of ApgdCtrl holding the newest state
-    for (;;) {
+    if (graph_rounds_ > 0 && !timing_) {
+        // hipGraph of graph_rounds_ rounds, the control step and the state's copy to the host,
+        // replayed until the loop is done (rounds after `done` exit at their first instruction).
+        // graph_rounds_ is odd: the rounds and the control step swap the state copies an even
+        // number of times, so every replay starts from copy 0 like the first.
+        hipGraph_t graph = nullptr;
+        hipGraphExec_t exec = nullptr;
+        SCOPI_HIP_CHECK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+        int c = 0;
+        for (int r = 0; r < graph_rounds_; ++r) {
+            hipLaunchKernelGGL((k_trial<T, LAY>), dim3(gred), dim3(kBlock), 0, s, (const ApgdCtrl*)(ctrl_ + c),
+                               ctrl_ + (1 - c), B, nc, partial_trial_, (const double*)partial_row_, (int)gred,
+                               SceneMap{});
+            c = 1 - c;
+            launch_bt<T, LAY>(bt_group_, s, ctrl_ + c, -1, B, na, tstart_, tc_, partial_trial_, (int)gred);
+            hipLaunchKernelGGL((k_row<T, LAY>), dim3(gred), dim3(kBlock), 0, s, ctrl_ + c, B, nc,
+                               (const int32_t*)C.i, (const int32_t*)C.j, ni, partial_row_, SceneMap{});
+        }
+        hipLaunchKernelGGL(k_ctrl, dim3(1), dim3(kBlock), 0, s, (const ApgdCtrl*)(ctrl_ + c), ctrl_ + (1 - c),
+                           (const double*)partial_row_, (int)gred, SceneMap{});
+        c = 1 - c;
+        SCOPI_HIP_CHECK(hipMemcpyAsync(h_ctrl_, ctrl_ + c, sizeof(ApgdCtrl), hipMemcpyDeviceToHost, s));
+        SCOPI_HIP_CHECK(hipStreamEndCapture(s, &graph));
+        SCOPI_HIP_CHECK(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
+        for (;;) {
+            SCOPI_HIP_CHECK(hipGraphLaunch(exec, s));
+            SCOPI_HIP_CHECK(hipStreamSynchronize(s));
+            if (h_ctrl_->done) break;
+        }
+        SCOPI_HIP_CHECK(hipGraphExecDestroy(exec));
+        SCOPI_HIP_CHECK(hipGraphDestroy(graph));
+        cur = c;
+    }
+    for (; !(graph_rounds_ > 0 && !timing_);) {
         for (int r = 0; r < batch; ++r) {
             hipEvent_t* e = timing_ ? &ev[6 * r] : nullptr;
             if (e)
@@ -1205,6 +1238,11 @@ int Solver::set_tuning(int key, int value) {
     if (key == SCOPI_TUNE_REDUCE_BLOCKS) {
         if (value < 1 || value > kRedBlocksMax) return SCOPI_HIP_ERR_ARG;
         reduce_blocks_ = value;
+        return SCOPI_HIP_OK;
+    }
+    if (key == SCOPI_TUNE_GRAPH_ROUNDS) {
+        if (value < 0 || value > 255 || (value > 0 && value % 2 == 0)) return SCOPI_HIP_ERR_ARG;
+        graph_rounds_ = value;
         return SCOPI_HIP_OK;
     }
     if (key == SCOPI_TUNE_BT_GROUP) {

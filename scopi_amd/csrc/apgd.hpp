@@ -178,6 +178,7 @@ class Solver {
     int layout_ = kRotFree;
     int reduce_blocks_ = 1024;  // grid of the per-contact kernels (grid-stride)
     int bt_group_ = 8;          // lanes per particle in k_bt
+    int graph_rounds_ = 0;      // > 0: the one-scene loop replays a hipGraph of this many rounds
     bool has_solution_ = false;
     // multi-scene state
     const SceneSet* scenes_ = nullptr;

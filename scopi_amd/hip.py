@@ -34,7 +34,7 @@ EXPORTS = [
     "scopi_hip_batch_scene_info", "scopi_hip_batch_download_contacts", "scopi_hip_batch_download_solution",
     "scopi_hip_scene_footprint",
 ]
-TUNE_REDUCE_BLOCKS, TUNE_BT_GROUP = 1, 2
+TUNE_REDUCE_BLOCKS, TUNE_BT_GROUP, TUNE_GRAPH_ROUNDS = 1, 2, 3
 
 
 class Params(C.Structure):
