@@ -25,11 +25,17 @@
 #include <cmath>
 #include <cstddef>
 #include <cstdint>
+#include <cstdio>
+#include <filesystem>
+#include <fstream>
+#include <future>
 #include <initializer_list>
 #include <stdexcept>
 #include <string>
 #include <type_traits>
 #include <vector>
+
+#include "json_output.hpp"
 
 #include "../../../include/scopi_hip.h"
 
@@ -207,7 +213,16 @@ class scopi_container {
     const std::vector<type::position_t<dim>>& pos() const { return m_pos; }
     const std::vector<type::quaternion_t>& q() const { return m_q; }
     const std::vector<type::velocity_t<dim>>& v() const { return m_v; }
+    const std::vector<type::velocity_t<dim>>& vd() const { return m_vd; }
     const std::vector<type::rotation_t<dim>>& omega() const { return m_w; }
+    const std::vector<type::rotation_t<dim>>& desired_omega() const { return m_dw; }
+    const std::vector<type::force_t<dim>>& f() const { return m_f; }
+    const std::vector<double>& m() const { return m_m; }
+    const std::vector<type::moment_t<dim>>& j() const { return m_j; }
+    // shape tag, size (radius / semi-axes / length) and squareness of object k
+    int shape(std::size_t k) const { return m_shape[k]; }
+    const std::array<double, 3>& object_size(std::size_t k) const { return m_size[k]; }
+    const std::array<double, 2>& object_squareness(std::size_t k) const { return m_sq[k]; }
 
     // structure of arrays in the C-ABI convention (3 slots, 2D rotation in slot 2)
     struct soa {
@@ -282,6 +297,171 @@ class BoxDomain {
   private:
     std::array<double, dim> m_lo{}, m_hi{};
 };
+
+// ------------------------------------------------------------------ output files
+namespace detail {
+// rotation_matrix<dim>(q), row-major (include/scopi/quaternion.hpp:65-124), evaluated as the
+// reference's build that wrote test/references/*.json evaluates it: with its products
+// contracted into fused multiply-adds (a - b * c -> fma(-b, c, a), a * b + c * d ->
+// fma(c, d, a * b)); its 2D diagonal 1 - 2 w w is fma(-2 w, w, 1) in those files.
+inline std::array<double, 4> rotation_matrix2(const type::quaternion_t& q) {
+    const double x = q(0), w = q(3);
+    const double d = std::fma(-(2 * w), w, 1.0);
+    return {{d, -2 * x * w, 2 * x * w, d}};
+}
+inline std::array<double, 9> rotation_matrix3(const type::quaternion_t& q) {
+    const double x = q(0), y = q(1), z = q(2), w = q(3);
+    auto diag = [](double a, double b) { return std::fma(-(2 * b), b, std::fma(-(2 * a), a, 1.0)); };
+    auto sub = [](double a, double b, double c, double d) { return std::fma(2 * a, b, -((2 * c) * d)); };
+    auto add = [](double a, double b, double c, double d) { return std::fma(2 * c, d, (2 * a) * b); };
+    return {{diag(z, w), sub(y, z, x, w), add(y, w, x, z), add(y, z, x, w), diag(y, w), sub(z, w, x, y),
+             sub(y, w, x, z), add(z, w, x, y), diag(y, z)}};
+}
+template <std::size_t dim>
+output::value vec(const fixed<dim>& v) {
+    output::value a = output::value::array();
+    for (std::size_t d = 0; d < dim; ++d) a.push_back(v[d]);
+    return a;
+}
+inline output::value rot_value(double x) { return x; }
+inline output::value rot_value(const fixed<3>& x) { return vec<3>(x); }
+
+template <class problem_t>
+output::value property_json(const contact_property<problem_t>&) {  // contact/property.hpp:44-49
+    output::value p;
+    p["type"] = "no friction";
+    return p;
+}
+inline output::value property_json(const contact_property<Friction>& c) {  // contact/property.hpp:64-70
+    output::value p;
+    p["type"] = "friction";
+    p["mu"] = c.mu;
+    return p;
+}
+}  // namespace detail
+
+// The document ScopiSolver::write_output_files writes (include/scopi/solver.hpp:369-421):
+// every object as write_objects (objects/methods/write_objects.hpp:45-160) with its
+// properties, then every contact as neighbor::to_json (objects/neighbor.hpp:35-46), with
+// "lagrange multiplier" = lambda[ic] when `lambda` is given (the reference indexes its flat
+// multiplier vector by the contact's position, also for Friction's three rows per contact).
+template <std::size_t dim, class problem_t>
+output::value output_document(const scopi_container<dim>& p, const std::vector<neighbor<dim, problem_t>>& contacts,
+                              const std::vector<double>* lambda = nullptr) {
+    output::value doc;
+    doc["objects"] = output::value();
+    for (std::size_t k = 0; k < p.size(); ++k) {
+        output::value o;
+        const type::quaternion_t& q = p.q()[k];
+        output::value rot = output::value::array();
+        fixed<dim> col0;  // first column of the rotation: plane / segment normal
+        if constexpr (dim == 2) {
+            const auto R = detail::rotation_matrix2(q);
+            for (double x : R) rot.push_back(x);
+            col0[0] = R[0], col0[1] = R[2];
+        } else {
+            const auto R = detail::rotation_matrix3(q);
+            for (double x : R) rot.push_back(x);
+            col0[0] = R[0], col0[1] = R[3], col0[2] = R[6];
+        }
+        const auto& sz = p.object_size(k);
+        switch (p.shape(k)) {
+            case SCOPI_SHAPE_SPHERE:
+                o["type"] = "sphere";
+                o["id"] = k;
+                o["position"] = detail::vec<dim>(p.pos()[k]);
+                o["radius"] = sz[0];
+                o["rotation"] = rot;
+                o["quaternion"] = detail::vec<4>(q);
+                break;
+            case SCOPI_SHAPE_SUPERELLIPSOID: {
+                o["type"] = "superellipsoid";
+                o["id"] = k;
+                o["position"] = detail::vec<dim>(p.pos()[k]);
+                fixed<dim> ax;
+                for (std::size_t d = 0; d < dim; ++d) ax[d] = sz[d];
+                o["radius"] = detail::vec<dim>(ax);
+                output::value e = output::value::array();
+                for (std::size_t d = 0; d + 1 < dim; ++d) e.push_back(p.object_squareness(k)[d]);
+                o["squareness"] = e;
+                o["rotation"] = rot;
+                o["quaternion"] = detail::vec<4>(q);
+                break;
+            }
+            case SCOPI_SHAPE_PLANE:
+                o["type"] = "plane";
+                o["id"] = k;
+                o["position"] = detail::vec<dim>(p.pos()[k]);
+                o["normal"] = detail::vec<dim>(col0);
+                o["rotation"] = rot;
+                o["quaternion"] = detail::vec<4>(q);
+                break;
+            default: {  // segment (2D): extrema R (0, -+L/2) + x, tangent (-n1, n0)
+                o["type"] = "segment";
+                o["id"] = k;
+                fixed<dim> p1, p2, tangent;
+                const double L = sz[0];
+                if constexpr (dim == 2) {
+                    const auto R = detail::rotation_matrix2(q);
+                    for (int d = 0; d < 2; ++d) {
+                        p1[d] = (R[2 * d] * 0 + R[2 * d + 1] * (-0.5 * L)) + p.pos()[k][d];
+                        p2[d] = (R[2 * d] * 0 + R[2 * d + 1] * (0.5 * L)) + p.pos()[k][d];
+                    }
+                    tangent[0] = -col0[1], tangent[1] = col0[0];
+                }
+                o["p1"] = detail::vec<dim>(p1);
+                o["p2"] = detail::vec<dim>(p2);
+                o["normal"] = detail::vec<dim>(col0);
+                o["tangent"] = detail::vec<dim>(tangent);
+                o["quaternion"] = detail::vec<4>(q);
+                break;
+            }
+        }
+        output::value& prop = o["properties"];
+        prop["velocity"] = detail::vec<dim>(p.v()[k]);
+        prop["desired_velocity"] = detail::vec<dim>(p.vd()[k]);
+        prop["omega"] = detail::rot_value(p.omega()[k]);
+        prop["desired_omega"] = detail::rot_value(p.desired_omega()[k]);
+        prop["force"] = detail::vec<dim>(p.f()[k]);
+        prop["mass"] = p.m()[k];
+        prop["moment_inertia"] = detail::rot_value(p.j()[k]);
+        prop["active"] = k >= p.nb_inactive();
+        doc["objects"].push_back(o);
+    }
+    doc["contacts"] = output::value();
+    std::size_t ic = 0;
+    for (const auto& c : contacts) {
+        output::value cj;
+        cj["i"] = c.i;
+        cj["j"] = c.j;
+        cj["pi"] = detail::vec<dim>(c.pi);
+        cj["pj"] = detail::vec<dim>(c.pj);
+        cj["normal"] = detail::vec<dim>(c.nij);
+        cj["distance"] = c.dij;
+        cj["property"] = detail::property_json(c.property);
+        if (lambda) cj["lagrange multiplier"] = ic < lambda->size() ? (*lambda)[ic] : 0.0;
+        ++ic;
+        doc["contacts"].push_back(cj);
+    }
+    return doc;
+}
+
+// {path}/{filename}_{nite:04d}.json (nlohmann's dump with an indent of 4) or .bson
+inline std::string write_output_document(const output::value& doc, const std::filesystem::path& path,
+                                         const std::string& filename, std::size_t nite, bool binary) {
+    if (!std::filesystem::exists(path)) std::filesystem::create_directories(path);
+    char tail[32];
+    std::snprintf(tail, sizeof(tail), "_%04zu.%s", nite, binary ? "bson" : "json");
+    const std::string name = (path / filename).string() + tail;
+    std::ofstream file(name, std::ios::out | std::ios::binary);
+    if (binary) {
+        const auto b = output::to_bson(doc);
+        file.write(reinterpret_cast<const char*>(b.data()), (std::streamsize)b.size());
+    } else {
+        file << output::dump(doc, 4);
+    }
+    return name;
+}
 
 // ------------------------------------------------------------------ device context
 class hip_error : public std::runtime_error {
@@ -523,10 +703,15 @@ struct vap_fpd {
 };
 
 // ------------------------------------------------------------------ driver
+// ScopiParams (include/scopi/params.hpp:89-130, src/params.cpp:6-14): output every
+// `output_frequency` steps into {path}/{filename}_{step:04d}.json (or .bson).
 struct ScopiParams {
-    std::size_t output_frequency = std::size_t(-1);
-    std::string path = "./Results/";
-    std::string filename = "scopi_objects_";
+    std::size_t output_frequency = 1;
+    std::filesystem::path path = std::filesystem::current_path() / "Results";
+    std::string filename = "scopi_objects";
+    bool write_velocity = false;
+    bool write_lagrange_multiplier = false;
+    bool binary_output = false;
     void init_options() {}
 };
 
@@ -552,6 +737,7 @@ class ScopiSolver {
     explicit ScopiSolver(scopi_container<dim>& particles) : m_particles(particles) {}
     ScopiSolver(const BoxDomain<dim>& box, scopi_container<dim>& particles) : m_box(box), m_particles(particles) {}
     ~ScopiSolver() {
+        if (m_pending.valid()) m_pending.wait();
         if (m_ctx) scopi_hip_destroy(m_ctx);
     }
 
@@ -559,17 +745,43 @@ class ScopiSolver {
         return params_t{m_params, m_optim.get_params(), m_contact.default_contact_property(), m_contact.get_params()};
     }
 
-    // ScopiSolver::run (solver.hpp:266-303), no output files.
+    // ScopiSolver::run (solver.hpp:266-303): the output file of the initial state, then the
+    // steps on the device in runs between two output steps; at each output step the state,
+    // the last step's contacts (and multipliers) come back to the host and the file is
+    // formatted and written by a background task while the device goes on.
     void run(double dt, std::size_t total_it, std::size_t initial_iter = 0) {
         m_optim.set_timestep(dt);
-        run_resident(dt, total_it, initial_iter);
+        upload();
+        write_output_files(m_old_contacts, initial_iter);
+        auto prm = hip_params();
+        const std::size_t freq = m_params.output_frequency;
+        const bool periodic = freq != std::size_t(-1) && freq > 0;
+        std::size_t nite = initial_iter;
+        scopi_hip_solve_info info{};
+        while (nite < total_it) {
+            std::size_t stop = total_it;
+            if (periodic) stop = std::min(total_it, (nite / freq + 1) * freq);
+            detail::check(scopi_hip_run(m_ctx, &prm, dt, (int64_t)stop, (int64_t)nite, &info), m_ctx,
+                          "scopi_hip_run");
+            accumulate(info);
+            nite = stop;
+            if (periodic && nite % freq == 0) {
+                download_state();
+                download_contacts(info, m_old_contacts, m_params.write_lagrange_multiplier ? &m_lambda : nullptr);
+                write_output_files(m_old_contacts, nite);
+            }
+        }
+        download_state();
+        if (total_it > initial_iter) download_contacts(info, m_old_contacts, nullptr);
+        wait_output();
     }
 
     const scopi_hip_solve_info& last_info() const { return m_info; }
+    const contact_container_t& last_contacts() const { return m_old_contacts; }
 
   private:
     // Every slot is a GPU slot of this library: the whole loop stays on the device.
-    void run_resident(double dt, std::size_t total_it, std::size_t initial_iter) {
+    void upload() {
         if (!m_ctx)
             detail::check(scopi_hip_create((int)dim, (int64_t)m_particles.size(), 0, &m_ctx), nullptr,
                           "scopi_hip_create");
@@ -579,20 +791,84 @@ class ScopiSolver {
                                              s.v.data(), s.vd.data(), s.w.data(), s.dw.data(), s.f.data(), s.m.data(),
                                              s.J.data()),
                       m_ctx, "scopi_hip_upload_state");
+        m_info = scopi_hip_solve_info{};
+        m_info.converged = 1;
+    }
+    double mu() {
         double mu = 0.1;
         if constexpr (std::is_same<problem_t, Friction>::value) mu = m_contact.default_contact_property().mu;
-        auto prm = m_optim.hip_params(problem_tag<problem_t>(), mu);
+        return mu;
+    }
+    scopi_hip_params hip_params() {
+        auto prm = m_optim.hip_params(problem_tag<problem_t>(), mu());
         prm.vap = vap_t::kind;
         prm.contact_method = contact_method_t::method;
         prm.dmax = m_contact.get_params().dmax;
         prm.kd_tree_radius = m_contact.get_params().kd_tree_radius;
-        detail::check(scopi_hip_run(m_ctx, &prm, dt, (int64_t)total_it, (int64_t)initial_iter, &m_info), m_ctx,
-                      "scopi_hip_run");
+        return prm;
+    }
+    void accumulate(const scopi_hip_solve_info& one) {
+        m_info.n_contacts = one.n_contacts;
+        m_info.rows = one.rows;
+        m_info.iterations += one.iterations;
+        m_info.iterations_fp32 += one.iterations_fp32;
+        m_info.backtracks += one.backtracks;
+        m_info.q_applies += one.q_applies;
+        m_info.converged = m_info.converged && one.converged;
+        if (one.status != 0) m_info.status = one.status;
+        m_info.residual = one.residual;
+    }
+    void download_state() {
         const std::size_t n = m_particles.size();
         std::vector<double> pos(3 * n), q(4 * n), v(3 * n), w(3 * n);
         detail::check(scopi_hip_download_state(m_ctx, pos.data(), q.data(), v.data(), w.data()), m_ctx,
                       "scopi_hip_download_state");
         m_particles.unpack(pos, q, v, w);
+    }
+    // the last step's contacts (as compute_contacts returned them) and multipliers
+    void download_contacts(const scopi_hip_solve_info& info, contact_container_t& out, std::vector<double>* lambda) {
+        const std::size_t nc = (std::size_t)info.n_contacts;
+        out.assign(nc, typename contact_container_t::value_type{});
+        if (nc == 0) {
+            if (lambda) lambda->clear();
+            return;
+        }
+        std::vector<int64_t> ci(nc), cj(nc);
+        std::vector<double> dij(nc), nij(3 * nc), pi(3 * nc), pj(3 * nc);
+        detail::check(scopi_hip_download_contacts(m_ctx, ci.data(), cj.data(), dij.data(), nij.data(), pi.data(),
+                                                  pj.data()),
+                      m_ctx, "scopi_hip_download_contacts");
+        for (std::size_t c = 0; c < nc; ++c) {
+            auto& nb = out[c];
+            nb.i = (std::size_t)ci[c];
+            nb.j = (std::size_t)cj[c];
+            nb.dij = dij[c];
+            for (std::size_t d = 0; d < dim; ++d) {
+                nb.nij[d] = nij[3 * c + d];
+                nb.pi[d] = pi[3 * c + d];
+                nb.pj[d] = pj[3 * c + d];
+            }
+            if constexpr (std::is_same<problem_t, Friction>::value) nb.property.mu = mu();
+        }
+        if (lambda) {
+            lambda->assign((std::size_t)info.rows, 0.0);
+            detail::check(scopi_hip_download_solution(m_ctx, nullptr, nullptr, lambda->data()), m_ctx,
+                          "scopi_hip_download_solution");
+        }
+    }
+    // write_output_files (solver.hpp:369-438): the document is built here, formatting and
+    // writing run in a background task (one at a time)
+    void write_output_files(const contact_container_t& contacts, std::size_t nite) {
+        auto doc = output_document(m_particles, contacts,
+                                   m_params.write_lagrange_multiplier && !contacts.empty() ? &m_lambda : nullptr);
+        wait_output();
+        m_pending = std::async(std::launch::async, [doc = std::move(doc), path = m_params.path,
+                                                    name = m_params.filename, nite, bin = m_params.binary_output]() {
+            return write_output_document(doc, path, name, nite, bin);
+        });
+    }
+    void wait_output() {
+        if (m_pending.valid()) m_pending.get();
     }
 
     ScopiParams m_params;
@@ -603,6 +879,9 @@ class ScopiSolver {
     vap_t m_vap;
     scopi_hip_ctx* m_ctx = nullptr;
     scopi_hip_solve_info m_info{};
+    contact_container_t m_old_contacts;
+    std::vector<double> m_lambda;
+    std::future<std::string> m_pending;
 };
 
 inline void initialize(const std::string& = "") {}

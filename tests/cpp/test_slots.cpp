@@ -17,9 +17,23 @@
 
 using namespace scopi;
 
+// Output files as the reference's tests ask for them (test_sphere.cpp:369-378): one at the
+// start and one after the last step, {path}/{filename}_{step:04d}.json; the printed line
+// names the last one.
+template <class S>
+std::string reference_output(S& solver, const char* path, const char* filename, std::size_t total_it) {
+    auto params = solver.get_params();
+    params.solver_params.output_frequency = total_it;
+    params.solver_params.filename = filename;
+    params.solver_params.path = path;
+    char name[256];
+    std::snprintf(name, sizeof(name), "%s/%s_%04zu.json", path, filename, total_it);
+    return name;
+}
+
 template <std::size_t dim>
-void dump(const char* name, scopi_container<dim>& p) {
-    std::printf("{\"case\": \"%s\", \"objects\": [", name);
+void dump(const char* name, scopi_container<dim>& p, const std::string& file) {
+    std::printf("{\"case\": \"%s\", \"output\": \"%s\", \"objects\": [", name, file.c_str());
     for (std::size_t k = 0; k < p.size(); ++k) {
         std::printf("%s{\"position\": [%.17g, %.17g], \"quaternion\": [%.17g, %.17g, %.17g, %.17g], "
                     "\"velocity\": [%.17g, %.17g], \"omega\": %.17g}",
@@ -43,8 +57,10 @@ void two_spheres(bool asym) {
     particles.push_back(s1, p.desired_velocity({{0.25, 0}}));
     particles.push_back(s2, p.desired_velocity({{-0.25, 0}}));
     SolverType solver(particles);
+    const char* name = asym ? "two_spheres_asymmetrical" : "two_spheres_symmetrical";
+    const auto file = reference_output(solver, "two_spheres", name, total_it);
     solver.run(dt, total_it);
-    dump(asym ? "two_spheres_asymmetrical" : "two_spheres_symmetrical", particles);
+    dump(name, particles, file);
 }
 
 template <class SolverType>
@@ -77,8 +93,9 @@ void critical_2d_spheres() {
         }
     }
     SolverType solver(particles);
+    const auto file = reference_output(solver, "critical_2d_spheres", "2d_case_spheres", total_it);
     solver.run(dt, total_it);
-    dump("2d_case_spheres", particles);
+    dump("2d_case_spheres", particles, file);
 }
 
 void sphere_sphere_moving() {
@@ -92,8 +109,9 @@ void sphere_sphere_moving() {
     particles.push_back(obstacle, property<dim>().deactivate());
     particles.push_back(s, prop.force({{0., -10.}}));
     ScopiSolver<dim, NoFriction, OptimGradient<apgd>, contact_kdtree, vap_fpd> solver(particles);
+    const auto file = reference_output(solver, "obstacles", "obstacles_sphere_sphere_moving", total_it);
     solver.run(dt, total_it);
-    dump("obstacles_sphere_sphere_moving", particles);
+    dump("obstacles_sphere_sphere_moving", particles, file);
 }
 
 void three_spheres() {
@@ -113,8 +131,9 @@ void three_spheres() {
     params.optim_params.alpha = 0.1;
     params.optim_params.dynamic_descent = true;
     std::size_t total_it = Tf / dt;
+    const auto file = reference_output(solver, "three_spheres", "3spheres_nofriction", total_it);
     solver.run(dt, total_it);
-    dump("3spheres_nofriction", particles);
+    dump("3spheres_nofriction", particles, file);
 }
 
 void sphere_plane_friction() {
@@ -134,8 +153,9 @@ void sphere_plane_friction() {
     params.optim_params.dynamic_descent = true;
     params.default_contact_property.mu = 0.1;
     std::size_t total_it = Tf / dt;
+    const auto file = reference_output(solver, "friction", "sphere_plane_friction_mu01", total_it);
     solver.run(dt, total_it);
-    dump("sphere_plane_friction_mu01", particles);
+    dump("sphere_plane_friction_mu01", particles, file);
 }
 
 // test_contacts_kdtree.cpp:14-72 through the contact slot alone, then the optim slot alone.

@@ -24,12 +24,51 @@ TOL = {"two_spheres_asymmetrical": 1e-6, "two_spheres_symmetrical": 1e-6, "2d_ca
 
 
 @pytest.fixture(scope="module")
-def output():
+def output(tmp_path_factory):
     if not os.path.exists(BIN):
         pytest.fail(f"{BIN} not built (run __graft_entry__.build())")
-    res = subprocess.run([BIN], capture_output=True, text=True, timeout=240)
+    cwd = tmp_path_factory.mktemp("slots")  # the output files land here, as in the reference's test runs
+    res = subprocess.run([BIN], capture_output=True, text=True, timeout=240, cwd=str(cwd))
     assert res.returncode == 0, res.stderr
-    return {d["case"]: d for d in (json.loads(l) for l in res.stdout.splitlines() if l.startswith("{"))}
+    out = {d["case"]: d for d in (json.loads(l) for l in res.stdout.splitlines() if l.startswith("{"))}
+    for d in out.values():
+        if "output" in d:
+            d["output"] = os.path.join(str(cwd), d["output"])
+    return out
+
+
+def same_within(ref, got, tol, where="objects"):
+    """diffFile (test/utils.cpp:14-75): the same structure, floats within tol, everything else equal."""
+    if isinstance(ref, float) and isinstance(got, float):
+        assert abs(ref - got) <= tol, f"{where}: {got} vs {ref}"
+    elif isinstance(ref, dict):
+        assert isinstance(got, dict) and sorted(ref) == sorted(got), f"{where}: keys {sorted(got)} vs {sorted(ref)}"
+        for k in ref:
+            same_within(ref[k], got[k], tol, f"{where}/{k}")
+    elif isinstance(ref, list):
+        assert isinstance(got, list) and len(ref) == len(got), f"{where}: length"
+        for k, (a, b) in enumerate(zip(ref, got)):
+            same_within(a, b, tol, f"{where}/{k}")
+    else:
+        assert type(ref) is type(got) and ref == got, f"{where}: {got!r} vs {ref!r}"
+
+
+@pytest.mark.parametrize("name", list(TOL))
+def test_output_file_against_reference_file(output, name):
+    """The file the mirror's ScopiSolver writes after the last step (its output path, set up as the
+    reference's tests set it) against the reference's file: check_reference_file's comparison of
+    "objects", plus the same contact records (keys and pairs)."""
+    path = output[name]["output"]
+    assert os.path.exists(path), path
+    first = os.path.join(os.path.dirname(path), os.path.basename(path).rsplit("_", 1)[0] + "_0000.json")
+    assert os.path.exists(first), first
+    got = json.load(open(path))
+    ref = fs.load_fixture(name)
+    same_within(ref["objects"], got["objects"], TOL[name])
+    rc, gc = ref["contacts"] or [], got["contacts"] or []
+    assert [(c["i"], c["j"]) for c in rc] == [(c["i"], c["j"]) for c in gc]
+    for a, b in zip(rc, gc):
+        assert sorted(a) == sorted(b) and a["property"] == b["property"]
 
 
 @pytest.mark.parametrize("name", list(TOL))
