@@ -139,6 +139,12 @@ int scopi_hip_solve(scopi_hip_ctx* ctx, const scopi_hip_params* params, double d
 int scopi_hip_download_solution(scopi_hip_ctx* ctx, double* u, double* omega, double* lambda);
 
 /* ScopiSolver::update_velocity + move_active_particles (solver.hpp:440-546). */
+/* BoxDomain of the scene (include/scopi/box.hpp; ScopiSolver(box, particles)): dim entries
+ * each, lower corner, upper corner, periodic[d] != 0 makes axis d periodic.  With periodic
+ * axes the contact slot adds periodic copies of the objects within dmax of a lower bound
+ * (utils.hpp:228-257, contact/base.hpp:70-139) and the move wraps objects that leave the box
+ * (solver.hpp:482-522).  Not supported with superellipsoids (SCOPI_HIP_ERR_UNSUPPORTED). */
+int scopi_hip_set_box(scopi_hip_ctx* ctx, const double* lo, const double* hi, const int32_t* periodic);
 int scopi_hip_update_and_move(scopi_hip_ctx* ctx, double dt);
 /* ScopiSolver::displacement_obstacles (solver.hpp:329-355). */
 int scopi_hip_displacement_obstacles(scopi_hip_ctx* ctx, double dt);

@@ -57,6 +57,7 @@ def lib():
         L.oracle_set_params.argtypes = [P, C.POINTER(Params)]
         L.oracle_scene_set.argtypes = [P, C.c_int32, C.c_int64, C.c_int64, I32] + [D] * 11
         L.oracle_get_state.argtypes = [P, D, D, D, D]
+        L.oracle_set_box.argtypes = [P, D, D, I32]
         L.oracle_run.argtypes = [P, C.c_double, C.c_int64, C.c_int64]
         L.oracle_compute_contacts.argtypes = [P]
         L.oracle_compute_contacts.restype = C.c_int64
@@ -118,6 +119,13 @@ class Oracle:
         self.params = default_params(**kw)
         self.problem = self.params.problem
         self._L.oracle_set_params(self._h, C.byref(self.params))
+
+    def set_box(self, lo, hi, periodic):
+        """BoxDomain(lo, hi).with_periodicity(axes with periodic[d] true)."""
+        lo = np.ascontiguousarray(lo, np.float64)
+        hi = np.ascontiguousarray(hi, np.float64)
+        per = np.ascontiguousarray([1 if p else 0 for p in periodic], np.int32)
+        assert lib().oracle_set_box(self._h, lo, hi, per) == 0
 
     def run(self, dt, total_it, initial_iter=0):
         self._L.oracle_run(self._h, float(dt), int(total_it), int(initial_iter))

@@ -46,9 +46,16 @@ struct Contact {
     double mu;
 };
 
+struct Box {  // BoxDomain (include/scopi/box.hpp)
+    vec3 lo{0, 0, 0}, hi{0, 0, 0};
+    std::array<bool, 3> periodic{false, false, false};
+    bool any() const { return periodic[0] || periodic[1] || periodic[2]; }
+};
+
 struct Scene {
     int dim = 2;
     int64_t n = 0, n_inactive = 0;
+    Box box;
     std::vector<int32_t> shape;
     std::vector<vec3> pos, size, v, vd, omega, domega, f, J;
     std::vector<std::array<double, 2>> sq;  // superellipsoid squareness (e, n)
@@ -312,9 +319,13 @@ bool closest_points(int dim, int si, const vec3& pi, const quat& qi, const vec3&
         out = swap_sides(dim, cp_sphere_segment(dim, pj, szj[0], pi, qi, szi[0]));
         return true;
     }
-    // plane/plane, segment/segment, plane/segment: the reference returns a
-    // default-constructed neighbor (closest_points.hpp:898-917); such pairs are
-    // never produced by the configurations exercised here.
+    // plane/plane, segment/segment, plane/segment: the reference returns a value-initialised
+    // neighbor (closest_points.hpp:750-753, 898-917): dij = 0, zero vectors.  Such pairs only
+    // arise between an obstacle and a periodic copy of an obstacle.
+    if ((si == ORACLE_PLANE || si == ORACLE_SEGMENT) && (sj == ORACLE_PLANE || sj == ORACLE_SEGMENT)) {
+        out = Contact{};
+        return true;
+    }
     return false;
 }
 
@@ -849,8 +860,78 @@ void update_and_move(Scene& s, double dt) {
     }
 }
 
-std::vector<Contact> compute_contacts(const Scene& s) {
+std::vector<Contact> search(const Scene& s) {
     return (s.prm.contact_method == ORACLE_CONTACT_BRUTE_FORCE) ? contacts_brute_force(s) : contacts_kdtree(s);
+}
+
+// contact_base::run with a periodic box (contact/base.hpp:70-77): periodic copies of the
+// objects near a lower bound (add_objects_from_periodicity, utils.hpp:228-257: per periodic
+// axis d in order, every object io with pos[d] - dmax < lower(d), copied at pos[d] + L(d),
+// appended after all objects and searched as active particles), the search on the extended
+// set, then compute_exact_distance's periodic rule (base.hpp:116-139): a pair is kept when
+// one of its particles is an original, a copy's index maps back to its original, and a
+// copy's contact point moves back by L along every periodic axis.  The kept pairs are
+// sorted by (i, j) (sort_contacts); pairs that meet twice (directly and through a copy)
+// keep the search's order, the direct one first.
+std::vector<Contact> compute_contacts(const Scene& s) {
+    if (!s.box.any()) return search(s);
+    Scene e = s;
+    std::vector<int64_t> orig;
+    for (int d = 0; d < s.dim; ++d) {
+        if (!s.box.periodic[d]) continue;
+        const double L = s.box.hi[d] - s.box.lo[d];
+        for (int64_t io = 0; io < s.n; ++io) {
+            if (!(s.pos[io][d] - s.prm.dmax < s.box.lo[d])) continue;
+            vec3 p = s.pos[io];
+            p[d] += L;
+            e.shape.push_back(s.shape[io]);
+            e.pos.push_back(p);
+            e.size.push_back(s.size[io]);
+            e.sq.push_back(s.sq[io]);
+            e.q.push_back(s.q[io]);
+            e.v.push_back(s.v[io]);
+            e.vd.push_back(s.vd[io]);
+            e.omega.push_back(s.omega[io]);
+            e.domega.push_back(s.domega[io]);
+            e.f.push_back(s.f[io]);
+            e.J.push_back(s.J[io]);
+            e.m.push_back(s.m[io]);
+            orig.push_back(io);
+        }
+    }
+    e.n = (int64_t)e.pos.size();
+    std::vector<Contact> ext = search(e), out;
+    for (Contact c : ext) {
+        if (!(c.i < s.n || c.j < s.n)) continue;
+        const bool gi = c.i >= s.n, gj = c.j >= s.n;
+        if (gi) c.i = orig[c.i - s.n];
+        if (gj) c.j = orig[c.j - s.n];
+        for (int d = 0; d < s.dim; ++d) {
+            if (!s.box.periodic[d]) continue;
+            const double L = s.box.hi[d] - s.box.lo[d];
+            if (gi) c.pi[d] -= L;
+            if (gj) c.pj[d] -= L;
+        }
+        out.push_back(c);
+    }
+    std::stable_sort(out.begin(), out.end(), [](const Contact& a, const Contact& b) {
+        return a.i < b.i || (a.i == b.i && a.j < b.j);
+    });
+    return out;
+}
+
+// the end of move_active_particles (solver.hpp:482-522): along a periodic axis, an object
+// past the upper bound moves back by L, one past the lower bound forward by L (obstacles too)
+void periodic_wrap(Scene& s) {
+    if (!s.box.any()) return;
+    for (int64_t io = 0; io < s.n; ++io)
+        for (int d = 0; d < s.dim; ++d) {
+            if (!s.box.periodic[d]) continue;
+            const double L = s.box.hi[d] - s.box.lo[d];
+            const bool plus = s.pos[io][d] > s.box.hi[d], minus = s.pos[io][d] < s.box.lo[d];
+            if (plus) s.pos[io][d] -= L;
+            if (minus) s.pos[io][d] += L;
+        }
 }
 
 }  // namespace
@@ -922,6 +1003,18 @@ int oracle_scene_set(void* scene, int32_t dim, int64_t n, int64_t n_inactive, co
     return 0;
 }
 
+int oracle_set_box(void* scene, const double* lo, const double* hi, const int32_t* periodic) {
+    if (!scene || !lo || !hi || !periodic) return -1;
+    Scene& s = *static_cast<Scene*>(scene);
+    for (int d = 0; d < s.dim; ++d) {
+        s.box.lo[d] = lo[d];
+        s.box.hi[d] = hi[d];
+        s.box.periodic[d] = periodic[d] != 0;
+    }
+    for (int d = s.dim; d < 3; ++d) s.box.periodic[d] = false;
+    return 0;
+}
+
 int oracle_get_state(void* scene, double* pos, double* q, double* v, double* omega) {
     if (!scene) return -1;
     Scene& s = *static_cast<Scene*>(scene);
@@ -956,6 +1049,7 @@ int oracle_run(void* scene, double dt, int64_t total_it, int64_t initial_iter) {
         s.st.t_solve += seconds_since(t0);
         t0 = std::chrono::steady_clock::now();
         update_and_move(s, dt);
+        periodic_wrap(s);
         s.st.t_move += seconds_since(t0);
     }
     return 0;

@@ -68,6 +68,8 @@ int oracle_scene_set(void* scene, int32_t dim, int64_t n, int64_t n_inactive, co
                      const double* pos, const double* q, const double* size, const double* squareness, const double* v,
                      const double* vd, const double* omega, const double* domega, const double* f,
                      const double* m, const double* J);
+/* BoxDomain of the scene (dim entries each): lower / upper corner, periodic axes. */
+int oracle_set_box(void* scene, const double* lo, const double* hi, const int32_t* periodic);
 int oracle_get_state(void* scene, double* pos, double* q, double* v, double* omega);
 
 /* ScopiSolver::run(dt, total_it, initial_iter) without output files. */

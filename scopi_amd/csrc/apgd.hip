@@ -205,11 +205,12 @@ __global__ void __launch_bounds__(kBlock) k_irange(const int32_t* __restrict__ c
     if (c == nc - 1 || ci[c + 1] != i) end[i - ni] = (int)(c + 1);
 }
 
-__global__ void __launch_bounds__(kBlock) k_jkeys(const int32_t* __restrict__ cj, int64_t nc, int64_t ni,
+// j-side entries: key = j's active index; an obstacle j (periodic boxes) gets key na, past every particle
+__global__ void __launch_bounds__(kBlock) k_jkeys(const int32_t* __restrict__ cj, int64_t nc, int64_t ni, int64_t na,
                                                  uint32_t* __restrict__ keys, int32_t* __restrict__ vals) {
     const int64_t c = (int64_t)blockIdx.x * kBlock + threadIdx.x;
     if (c >= nc) return;
-    keys[c] = (uint32_t)(cj[c] - ni);
+    keys[c] = (uint32_t)(cj[c] >= ni ? cj[c] - ni : na);
     vals[c] = (int32_t)c;
 }
 
@@ -593,7 +594,7 @@ __device__ inline void row_product(const BMatrix<T>& B, int64_t c, int64_t i, in
 #pragma unroll
     for (int k = 0; k < NW; ++k) wi[k] = (i >= 0) ? B.w[i * NW + k] : (T)0;
 #pragma unroll
-    for (int k = 0; k < NW; ++k) wj[k] = B.w[j * NW + k];
+    for (int k = 0; k < NW; ++k) wj[k] = (j >= 0) ? B.w[j * NW + k] : (T)0;  // j is an obstacle only in periodic boxes
     const T* co = B.coef + c * NC;
     if (LAY == kRotFree) {
         q[0] = dt2 * (co[0] * (wi[0] - wj[0]) + co[1] * (wi[1] - wj[1]) + co[2] * (wi[2] - wj[2]));
@@ -1189,9 +1190,9 @@ void Solver::assemble(const Particles& P, const Contacts& C, int64_t nc, double 
     for (int* a : {istart_, iend_, jstart_, jend_})
         hipLaunchKernelGGL(k_fill_i, dim3(256), dim3(kBlock), 0, s, a, na, 0);
     hipLaunchKernelGGL(k_irange, dim3(grid_for(nc)), dim3(kBlock), 0, s, C.i, nc, ni, istart_, iend_);
-    hipLaunchKernelGGL(k_jkeys, dim3(grid_for(nc)), dim3(kBlock), 0, s, C.j, nc, ni, jkeys_, jlist_);
+    hipLaunchKernelGGL(k_jkeys, dim3(grid_for(nc)), dim3(kBlock), 0, s, C.j, nc, ni, na, jkeys_, jlist_);
     int bits = 1;
-    while ((int64_t(1) << bits) < na) ++bits;
+    while ((int64_t(1) << bits) <= na) ++bits;  // keys 0 .. na
     sorter_.sort(jkeys_, jlist_, nc, bits, s, scanner_);
     hipLaunchKernelGGL(k_jrange, dim3(grid_for(nc)), dim3(kBlock), 0, s, jkeys_, nc, jstart_, jend_);
     hipLaunchKernelGGL(k_tcount, dim3(grid_for(na + 1)), dim3(kBlock), 0, s, istart_, iend_, jstart_, jend_, na,

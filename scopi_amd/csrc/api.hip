@@ -26,6 +26,22 @@ struct scopi_hip_ctx {
     ContactSearch search;
     Solver solver;
     std::string err;
+    // BoxDomain (include/scopi/box.hpp): periodic axes get periodic copies in the search
+    double box_lo[3] = {0, 0, 0}, box_hi[3] = {0, 0, 0};
+    int periodic[3] = {0, 0, 0};
+    bool any_periodic() const { return periodic[0] || periodic[1] || periodic[2]; }
+    Contacts C2;                   // the search's pairs over the particles and their copies
+    int* pflags = nullptr;         // periodic copies: flags, offsets, originals, sort buffers
+    int* poffs = nullptr;
+    int32_t* porig = nullptr;
+    int* ptotal = nullptr;
+    int* h_ptotal = nullptr;
+    uint32_t* pkeys = nullptr;
+    int32_t* pvals = nullptr;
+    int32_t* pi_of = nullptr;
+    int64_t cap_flags = 0, cap_offs = 0, cap_orig = 0, cap_keys = 0, cap_vals = 0, cap_iof = 0;
+    RadixSorter psort;
+    Scanner pscan;
 };
 
 namespace {
@@ -124,6 +140,123 @@ __global__ void k_obstacles(Particles P, double dt) {
     rotate_q(P.q, c, k, wx, wy, wz, normw, dt);
 }
 
+// ------------------------------------------------------------------ periodic boxes
+// add_objects_from_periodicity (utils.hpp:228-257): along periodic axis d (in order), object k
+// gets a copy when pos[d] - dmax < lower(d); flags in [axis slot][k] order, as the copies are made.
+struct BoxDev {
+    double lo[3], hi[3];
+    int axes[3];  // the periodic axes
+    int naxes;
+};
+
+__global__ void __launch_bounds__(kBlock) k_copy_flags(Particles P, int64_t n, BoxDev b, double dmax,
+                                                      int* __restrict__ flags) {
+    const int64_t t = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (t >= n * b.naxes) return;
+    const int d = b.axes[t / n];
+    const int64_t k = t % n;
+    flags[t] = (P.pos[d * P.cap + k] - dmax < b.lo[d]) ? 1 : 0;
+}
+
+// copy g = offs[t] of object k, moved by L along its axis, at particle n + g
+__global__ void __launch_bounds__(kBlock) k_make_copies(Particles P, int64_t n, BoxDev b, const int* __restrict__ flags,
+                                                       const int* __restrict__ offs, int32_t* __restrict__ orig) {
+    const int64_t t = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (t >= n * b.naxes || !flags[t]) return;
+    const int d = b.axes[t / n];
+    const int64_t k = t % n, g = n + offs[t];
+    const int64_t c = P.cap;
+    P.shape[g] = P.shape[k];
+    P.m[g] = P.m[k];
+    for (int e = 0; e < 3; ++e) {
+        P.pos[e * c + g] = P.pos[e * c + k];
+        P.size[e * c + g] = P.size[e * c + k];
+        P.v[e * c + g] = P.v[e * c + k];
+        P.vd[e * c + g] = P.vd[e * c + k];
+        P.omega[e * c + g] = P.omega[e * c + k];
+        P.domega[e * c + g] = P.domega[e * c + k];
+        P.f[e * c + g] = P.f[e * c + k];
+        P.J[e * c + g] = P.J[e * c + k];
+    }
+    for (int e = 0; e < 4; ++e) P.q[e * c + g] = P.q[e * c + k];
+    for (int e = 0; e < 2; ++e) P.sq[e * c + g] = P.sq[e * c + k];
+    P.pos[d * c + g] += b.hi[d] - b.lo[d];
+    orig[g - n] = (int32_t)k;
+}
+
+// compute_exact_distance's periodic rule (contact/base.hpp:116-139): keep a pair when one
+// particle is an original (i < j, so i < n); a copy's index maps to its original and its
+// contact point moves back by L along every periodic axis
+__global__ void __launch_bounds__(kBlock) k_keep_pairs(const int32_t* __restrict__ ci, int64_t nc, int64_t n,
+                                                      int* __restrict__ keep) {
+    const int64_t c = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (c < nc) keep[c] = ci[c] < n ? 1 : 0;
+}
+
+__global__ void __launch_bounds__(kBlock) k_map_pairs(Contacts E, int64_t nc, int64_t n, BoxDev b,
+                                                     const int32_t* __restrict__ orig, const int* __restrict__ keep,
+                                                     const int* __restrict__ pos, Contacts T, uint32_t* __restrict__ jkey,
+                                                     int32_t* __restrict__ ikey, int32_t* __restrict__ vals) {
+    const int64_t c = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (c >= nc || !keep[c]) return;
+    const int64_t p = pos[c];
+    const int32_t i = E.i[c], je = E.j[c];
+    const bool copy = je >= n;
+    const int32_t j = copy ? orig[je - n] : je;
+    T.i[p] = i;
+    T.j[p] = j;
+    T.dij[p] = E.dij[c];
+    T.mu[p] = E.mu[c];
+    for (int e = 0; e < 3; ++e) {
+        T.nij[e * T.cap + p] = E.nij[e * E.cap + c];
+        T.pi[e * T.cap + p] = E.pi[e * E.cap + c];
+        double pj = E.pj[e * E.cap + c];
+        for (int a = 0; a < b.naxes; ++a)
+            if (copy && b.axes[a] == e) pj -= b.hi[e] - b.lo[e];
+        T.pj[e * T.cap + p] = pj;
+    }
+    jkey[p] = (uint32_t)j;
+    ikey[p] = i;
+    vals[p] = (int32_t)p;
+}
+
+__global__ void __launch_bounds__(kBlock) k_permute_keys(const int32_t* __restrict__ ikey, const int32_t* __restrict__ vals,
+                                                        int64_t nc, uint32_t* __restrict__ keys) {
+    const int64_t k = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (k < nc) keys[k] = (uint32_t)ikey[vals[k]];
+}
+
+__global__ void __launch_bounds__(kBlock) k_gather_pairs(Contacts T, const int32_t* __restrict__ vals, int64_t nc,
+                                                        Contacts C) {
+    const int64_t k = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (k >= nc) return;
+    const int64_t p = vals[k];
+    C.i[k] = T.i[p];
+    C.j[k] = T.j[p];
+    C.dij[k] = T.dij[p];
+    C.mu[k] = T.mu[p];
+    for (int e = 0; e < 3; ++e) {
+        C.nij[e * C.cap + k] = T.nij[e * T.cap + p];
+        C.pi[e * C.cap + k] = T.pi[e * T.cap + p];
+        C.pj[e * C.cap + k] = T.pj[e * T.cap + p];
+    }
+}
+
+// the end of move_active_particles (solver.hpp:482-522): along a periodic axis an object past
+// the upper bound moves back by L, one past the lower bound forward by L (obstacles too)
+__global__ void __launch_bounds__(kBlock) k_periodic_wrap(Particles P, BoxDev b) {
+#pragma clang fp contract(off)
+    const int64_t k = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (k >= P.n) return;
+    for (int a = 0; a < b.naxes; ++a) {
+        const int d = b.axes[a];
+        double& x = P.pos[d * P.cap + k];
+        const bool plus = x > b.hi[d], minus = x < b.lo[d];
+        if (plus) x -= b.hi[d] - b.lo[d];
+        if (minus) x += b.hi[d] - b.lo[d];
+    }
+}
+
 void particles_free(Particles& P) {
     for (void* p : {(void*)P.shape, (void*)P.pos, (void*)P.q, (void*)P.size, (void*)P.sq, (void*)P.v, (void*)P.vd,
                     (void*)P.omega, (void*)P.domega, (void*)P.f, (void*)P.m, (void*)P.J})
@@ -168,6 +301,35 @@ void download_rows(const double* dev, int64_t cap, double* host, int64_t n, int 
         for (int d = 0; d < w; ++d) host[(host_rows_offset + i) * w + d] = t[(size_t)d * n + i];
 }
 
+// grows P to `cap` particles keeping its contents
+void particles_grow(Particles& P, int64_t cap, hipStream_t s) {
+    if (cap <= P.cap) return;
+    Particles Q{};
+    Q.dim = P.dim;
+    particles_reserve(Q, cap);
+    Q.n = P.n;
+    Q.n_inactive = P.n_inactive;
+    const int64_t n = P.n;
+    if (n > 0) {
+        SCOPI_HIP_CHECK(hipMemcpyAsync(Q.shape, P.shape, n * sizeof(int32_t), hipMemcpyDeviceToDevice, s));
+        SCOPI_HIP_CHECK(hipMemcpyAsync(Q.m, P.m, n * sizeof(double), hipMemcpyDeviceToDevice, s));
+        struct F {
+            double* dst;
+            const double* src;
+            int w;
+        };
+        for (const F& f : {F{Q.pos, P.pos, 3}, F{Q.q, P.q, 4}, F{Q.size, P.size, 3}, F{Q.sq, P.sq, 2}, F{Q.v, P.v, 3},
+                           F{Q.vd, P.vd, 3}, F{Q.omega, P.omega, 3}, F{Q.domega, P.domega, 3}, F{Q.f, P.f, 3},
+                           F{Q.J, P.J, 3}})
+            for (int d = 0; d < f.w; ++d)
+                SCOPI_HIP_CHECK(hipMemcpyAsync(f.dst + d * Q.cap, f.src + d * P.cap, n * sizeof(double),
+                                               hipMemcpyDeviceToDevice, s));
+        SCOPI_HIP_CHECK(hipStreamSynchronize(s));
+    }
+    particles_free(P);
+    P = Q;
+}
+
 int fail(scopi_hip_ctx* ctx, int code, const std::string& msg) {
     if (ctx) ctx->err = msg;
     return code;
@@ -193,6 +355,102 @@ bool params_ok(const scopi_hip_params* p) {
            (p->method == SCOPI_METHOD_APGD || p->method == SCOPI_METHOD_PGD) &&
            (p->precision == SCOPI_PRECISION_FP64 || p->precision == SCOPI_PRECISION_FP32_REFINE) &&
            p->warm_start == 0;
+}
+
+BoxDev box_dev(const scopi_hip_ctx* ctx) {
+    BoxDev b{};
+    for (int d = 0; d < 3; ++d) {
+        b.lo[d] = ctx->box_lo[d];
+        b.hi[d] = ctx->box_hi[d];
+        if (d < ctx->dim && ctx->periodic[d]) b.axes[b.naxes++] = d;
+    }
+    return b;
+}
+
+template <class T>
+void grow_buf(T*& p, int64_t& cap, int64_t need) {
+    if (need <= cap) return;
+    if (p) SCOPI_HIP_CHECK(hipFree(p));
+    cap = need + need / 4 + 1024;
+    SCOPI_HIP_CHECK(hipMalloc(&p, cap * sizeof(T)));
+}
+
+// contact_base::run (contact/base.hpp:70-77): with periodic axes, the search runs over the
+// particles and their periodic copies (appended after them), and its pairs are kept, mapped
+// back and sorted by (i, j) (stable: a pair met directly and through a copy keeps the
+// direct one first)
+int64_t contacts_step(scopi_hip_ctx* ctx, const scopi_hip_params& prm) {
+    hipStream_t s = ctx->stream;
+    Particles& P = ctx->P;
+    const BoxDev b = box_dev(ctx);
+    if (b.naxes == 0) return ctx->search.run(P, ctx->C, prm, s, !ctx->has_se);
+    const int64_t n = P.n, nt = n * b.naxes;
+    if (!ctx->ptotal) {
+        SCOPI_HIP_CHECK(hipMalloc(&ctx->ptotal, sizeof(int)));
+        SCOPI_HIP_CHECK(hipHostMalloc(&ctx->h_ptotal, sizeof(int)));
+    }
+    grow_buf(ctx->pflags, ctx->cap_flags, nt + 1);
+    grow_buf(ctx->poffs, ctx->cap_offs, nt + 1);
+    hipLaunchKernelGGL(k_copy_flags, dim3(grid_for(std::max<int64_t>(nt, 1))), dim3(kBlock), 0, s, P, n, b, prm.dmax,
+                       ctx->pflags);
+    ctx->pscan.exclusive(ctx->pflags, ctx->poffs, nt, ctx->ptotal, s);
+    SCOPI_HIP_CHECK(hipMemcpyAsync(ctx->h_ptotal, ctx->ptotal, sizeof(int), hipMemcpyDeviceToHost, s));
+    SCOPI_HIP_CHECK(hipStreamSynchronize(s));
+    const int64_t G = nt > 0 ? *ctx->h_ptotal : 0;
+    if (n + G > P.cap) particles_grow(P, n + G + (n + G) / 4, s);
+    if (G > 0) {
+        grow_buf(ctx->porig, ctx->cap_orig, G);
+        hipLaunchKernelGGL(k_make_copies, dim3(grid_for(nt)), dim3(kBlock), 0, s, P, n, b, (const int*)ctx->pflags,
+                           (const int*)ctx->poffs, ctx->porig);
+    }
+    P.n = n + G;
+    int64_t next = 0;
+    try {
+        // copies of obstacles (planes) are searched as particles: no bounding radius for the cut-off
+        next = ctx->search.run(P, ctx->C2, prm, s, !ctx->has_se, P.n_inactive == 0);
+    } catch (...) {
+        P.n = n;
+        throw;
+    }
+    P.n = n;
+    if (next == 0) {
+        contacts_reserve(ctx->C, 1);
+        return 0;
+    }
+    grow_buf(ctx->pflags, ctx->cap_flags, next + 1);
+    grow_buf(ctx->poffs, ctx->cap_offs, next + 1);
+    hipLaunchKernelGGL(k_keep_pairs, dim3(grid_for(next)), dim3(kBlock), 0, s, (const int32_t*)ctx->C2.i, next, n,
+                       ctx->pflags);
+    ctx->pscan.exclusive(ctx->pflags, ctx->poffs, next, ctx->ptotal, s);
+    SCOPI_HIP_CHECK(hipMemcpyAsync(ctx->h_ptotal, ctx->ptotal, sizeof(int), hipMemcpyDeviceToHost, s));
+    SCOPI_HIP_CHECK(hipStreamSynchronize(s));
+    const int64_t nc = *ctx->h_ptotal;
+    contacts_reserve(ctx->C, std::max<int64_t>(nc, 1));
+    if (nc == 0) return 0;
+    grow_buf(ctx->pkeys, ctx->cap_keys, nc);
+    grow_buf(ctx->pvals, ctx->cap_vals, nc);
+    grow_buf(ctx->pi_of, ctx->cap_iof, nc);
+    hipLaunchKernelGGL(k_map_pairs, dim3(grid_for(next)), dim3(kBlock), 0, s, ctx->C2, next, n, b,
+                       (const int32_t*)ctx->porig, (const int*)ctx->pflags, (const int*)ctx->poffs, ctx->C, ctx->pkeys,
+                       ctx->pi_of, ctx->pvals);
+    int bits = 1;
+    while ((int64_t(1) << bits) <= n) ++bits;
+    ctx->psort.sort(ctx->pkeys, ctx->pvals, nc, bits, s, ctx->pscan);  // by j ...
+    hipLaunchKernelGGL(k_permute_keys, dim3(grid_for(nc)), dim3(kBlock), 0, s, (const int32_t*)ctx->pi_of,
+                       (const int32_t*)ctx->pvals, nc, ctx->pkeys);
+    ctx->psort.sort(ctx->pkeys, ctx->pvals, nc, bits, s, ctx->pscan);  // ... then stably by i
+    hipLaunchKernelGGL(k_gather_pairs, dim3(grid_for(nc)), dim3(kBlock), 0, s, ctx->C, (const int32_t*)ctx->pvals, nc,
+                       ctx->C2);
+    SCOPI_HIP_CHECK(hipGetLastError());
+    std::swap(ctx->C, ctx->C2);
+    return nc;
+}
+
+void periodic_wrap(scopi_hip_ctx* ctx) {
+    const BoxDev b = box_dev(ctx);
+    if (b.naxes == 0 || ctx->P.n == 0) return;
+    hipLaunchKernelGGL(k_periodic_wrap, dim3(grid_for(ctx->P.n)), dim3(kBlock), 0, ctx->stream, ctx->P, b);
+    SCOPI_HIP_CHECK(hipGetLastError());
 }
 
 }  // namespace
@@ -246,6 +504,11 @@ int scopi_hip_destroy(scopi_hip_ctx* ctx) {
     if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
     particles_free(ctx->P);
     contacts_free(ctx->C);
+    contacts_free(ctx->C2);
+    for (void* p : {(void*)ctx->pflags, (void*)ctx->poffs, (void*)ctx->porig, (void*)ctx->ptotal, (void*)ctx->pkeys,
+                    (void*)ctx->pvals, (void*)ctx->pi_of})
+        if (p) (void)hipFree(p);
+    if (ctx->h_ptotal) (void)hipHostFree(ctx->h_ptotal);
     if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
     delete ctx;
     return SCOPI_HIP_OK;
@@ -277,6 +540,8 @@ int scopi_hip_upload_state(scopi_hip_ctx* ctx, int64_t n, int64_t n_inactive, co
         }
         if (has_se && ctx->dim != 3)
             return fail(ctx, SCOPI_HIP_ERR_UNSUPPORTED, "upload_state: superellipsoids are 3D objects here");
+        if (has_se && ctx->any_periodic())
+            return fail(ctx, SCOPI_HIP_ERR_UNSUPPORTED, "upload_state: periodic axes with superellipsoids are not supported");
         ctx->has_se = has_se;
         particles_reserve(ctx->P, n);
         Particles& P = ctx->P;
@@ -461,7 +726,7 @@ int scopi_hip_portable_math(scopi_hip_ctx* ctx, int32_t fn, int64_t n, const dou
 int scopi_hip_compute_contacts(scopi_hip_ctx* ctx, const scopi_hip_params* params, int64_t* n_contacts) {
     return guarded(ctx, [&]() -> int {
         if (!params_ok(params)) return fail(ctx, SCOPI_HIP_ERR_ARG, "compute_contacts: bad params");
-        ctx->nc = ctx->search.run(ctx->P, ctx->C, *params, ctx->stream, !ctx->has_se);
+        ctx->nc = contacts_step(ctx, *params);
         ctx->contacts_valid = true;
         if (n_contacts) *n_contacts = ctx->nc;
         return SCOPI_HIP_OK;
@@ -554,7 +819,7 @@ int scopi_hip_solve(scopi_hip_ctx* ctx, const scopi_hip_params* params, double d
         if (ctx->nc > 0)
             hipLaunchKernelGGL(k_fill_mu, dim3(grid_for(ctx->nc)), dim3(kBlock), 0, ctx->stream, ctx->C.mu, ctx->nc,
                                params->mu);
-        int rc = ctx->solver.solve(ctx->P, ctx->C, ctx->nc, *params, dt, ctx->all_spheres, ctx->stream, info);
+        int rc = ctx->solver.solve(ctx->P, ctx->C, ctx->nc, *params, dt, ctx->all_spheres && !ctx->any_periodic(), ctx->stream, info);
         if (rc != SCOPI_HIP_OK) return fail(ctx, rc, "solve: unsupported option");
         SCOPI_HIP_CHECK(hipStreamSynchronize(ctx->stream));
         return SCOPI_HIP_OK;
@@ -579,6 +844,7 @@ int scopi_hip_update_and_move(scopi_hip_ctx* ctx, double dt) {
         if (na > 0)
             hipLaunchKernelGGL(k_update_move, dim3(grid_for(na)), dim3(kBlock), 0, ctx->stream, ctx->P, ctx->solver.u(),
                                ctx->solver.w(), ctx->solver.pcap(), dt);
+        periodic_wrap(ctx);
         SCOPI_HIP_CHECK(hipGetLastError());
         SCOPI_HIP_CHECK(hipStreamSynchronize(ctx->stream));
         return SCOPI_HIP_OK;
@@ -606,15 +872,16 @@ int scopi_hip_run(scopi_hip_ctx* ctx, const scopi_hip_params* params, double dt,
         for (int64_t nite = initial_iter; nite < total_it; ++nite) {
             if (P.n_inactive > 0)
                 hipLaunchKernelGGL(k_obstacles, dim3(grid_for(P.n_inactive)), dim3(kBlock), 0, s, P, dt);
-            ctx->nc = ctx->search.run(P, ctx->C, *params, s, !ctx->has_se);
+            ctx->nc = contacts_step(ctx, *params);
             ctx->contacts_valid = true;
             if (na > 0) hipLaunchKernelGGL(k_vap, dim3(grid_for(na)), dim3(kBlock), 0, s, P, params->vap, dt);
             scopi_hip_solve_info one{};
-            int rc = ctx->solver.solve(P, ctx->C, ctx->nc, *params, dt, ctx->all_spheres, s, &one);
+            int rc = ctx->solver.solve(P, ctx->C, ctx->nc, *params, dt, ctx->all_spheres && !ctx->any_periodic(), s, &one);
             if (rc != SCOPI_HIP_OK) return fail(ctx, rc, "run: solve failed");
             if (na > 0)
                 hipLaunchKernelGGL(k_update_move, dim3(grid_for(na)), dim3(kBlock), 0, s, P, ctx->solver.u(),
                                    ctx->solver.w(), ctx->solver.pcap(), dt);
+            periodic_wrap(ctx);
             SCOPI_HIP_CHECK(hipGetLastError());
             acc.n_contacts = one.n_contacts;
             acc.rows = one.rows;
@@ -628,6 +895,26 @@ int scopi_hip_run(scopi_hip_ctx* ctx, const scopi_hip_params* params, double dt,
         }
         SCOPI_HIP_CHECK(hipStreamSynchronize(s));
         if (info) *info = acc;
+        return SCOPI_HIP_OK;
+    });
+}
+
+int scopi_hip_set_box(scopi_hip_ctx* ctx, const double* lo, const double* hi, const int32_t* periodic) {
+    return guarded(ctx, [&]() -> int {
+        if (!lo || !hi || !periodic) return fail(ctx, SCOPI_HIP_ERR_ARG, "set_box: bad argument");
+        bool any = false;
+        for (int d = 0; d < ctx->dim; ++d) {
+            if (periodic[d] && !(hi[d] > lo[d])) return fail(ctx, SCOPI_HIP_ERR_ARG, "set_box: periodic axis with hi <= lo");
+            any = any || periodic[d];
+        }
+        if (any && ctx->has_se)
+            return fail(ctx, SCOPI_HIP_ERR_UNSUPPORTED, "set_box: periodic axes with superellipsoids are not supported");
+        for (int d = 0; d < 3; ++d) {
+            ctx->box_lo[d] = d < ctx->dim ? lo[d] : 0.0;
+            ctx->box_hi[d] = d < ctx->dim ? hi[d] : 0.0;
+            ctx->periodic[d] = (d < ctx->dim && periodic[d]) ? 1 : 0;
+        }
+        ctx->contacts_valid = false;
         return SCOPI_HIP_OK;
     });
 }

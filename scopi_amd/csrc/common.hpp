@@ -219,6 +219,13 @@ __device__ inline bool closest_points(const Particles& P, int64_t a, int64_t b, 
         out = swap_sides(cp_sphere_segment2(cb, P.size[b], ca, P.q[a], P.q[3 * cap + a], P.size[a]));
         return true;
     }
+    // plane / segment pairs: the reference's value-initialised neighbor (closest_points.hpp:750-753,
+    // 898-917), dij = 0 and zero vectors; only an obstacle and a periodic copy of one meet this way
+    if ((sa == SCOPI_SHAPE_PLANE || sa == SCOPI_SHAPE_SEGMENT) && (sb == SCOPI_SHAPE_PLANE || sb == SCOPI_SHAPE_SEGMENT)) {
+        out.dij = 0.0;
+        out.nij = out.pi = out.pj = mk(0.0, 0.0, 0.0);
+        return true;
+    }
     return false;
 }
 

@@ -669,7 +669,7 @@ void ContactSearch::reserve_pairs(int64_t n) {
 }
 
 int64_t ContactSearch::run(const Particles& P, Contacts& C, const scopi_hip_params& prm, hipStream_t s,
-                           bool closed_form) {
+                           bool closed_form, bool bounded) {
     if (!closed_form) return run_candidates(P, C, prm, s);
     const int64_t na = P.n - P.n_inactive, ni = P.n_inactive;
     const bool kd = prm.contact_method == SCOPI_CONTACT_KDTREE;
@@ -679,7 +679,7 @@ int64_t ContactSearch::run(const Particles& P, Contacts& C, const scopi_hip_para
         const unsigned nparts = (unsigned)std::min<int64_t>(grid_for(na), 1024);
         hipLaunchKernelGGL(k_bbox_partial, dim3(nparts), dim3(kBlock), 0, s, P, d_partial_);
         hipLaunchKernelGGL(k_grid_setup, dim3(1), dim3(kBlock), 0, s, d_partial_, (int)nparts, P.dim, kd ? 1 : 0,
-                           prm.kd_tree_radius, prm.dmax, 1, d_grid_);
+                           prm.kd_tree_radius, prm.dmax, bounded ? 1 : 0, d_grid_);
         if (P.dim == 3)
             hipLaunchKernelGGL(k_cell_keys<3>, dim3(grid_for(na)), dim3(kBlock), 0, s, P, d_grid_, keys_, vals_);
         else

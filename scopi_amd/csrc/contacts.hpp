@@ -25,7 +25,11 @@ class ContactSearch {
     ~ContactSearch();
     // Runs on `s`; grows `C` when needed; returns nc (host sync inside).
     // closed_form: every shape pair has a closed-form distance (no superellipsoid).
-    int64_t run(const Particles& P, Contacts& C, const scopi_hip_params& prm, hipStream_t s, bool closed_form = true);
+    // bounded: every searched particle has a finite bounding radius (false when periodic copies
+    // of planes are searched as particles: the candidate cut-off is then the kd-tree radius, or
+    // every pair for brute force).
+    int64_t run(const Particles& P, Contacts& C, const scopi_hip_params& prm, hipStream_t s, bool closed_form = true,
+                bool bounded = true);
     int64_t last_obstacle_contacts() const { return nc_obs_; }
     // superellipsoid path: candidates of the last run and their closest points
     int64_t last_candidates() const { return ncand_; }

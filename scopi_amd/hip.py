@@ -32,7 +32,7 @@ EXPORTS = [
     "scopi_hip_batch_create", "scopi_hip_batch_destroy", "scopi_hip_batch_last_error", "scopi_hip_batch_add_scene",
     "scopi_hip_batch_remove_scene", "scopi_hip_batch_step", "scopi_hip_batch_size", "scopi_hip_batch_download_scene",
     "scopi_hip_batch_scene_info", "scopi_hip_batch_download_contacts", "scopi_hip_batch_download_solution",
-    "scopi_hip_scene_footprint",
+    "scopi_hip_scene_footprint", "scopi_hip_set_box",
 ]
 TUNE_REDUCE_BLOCKS, TUNE_BT_GROUP, TUNE_GRAPH_ROUNDS = 1, 2, 3
 
@@ -103,6 +103,7 @@ def lib():
     L.scopi_hip_newton_probe.argtypes = [P, C.c_int32, D, D]
     L.scopi_hip_closest_points.argtypes = ([P, C.c_int64, I32, D, D, D, C.c_void_p, I32, D, D, D, C.c_void_p] +
                                            [D] * 4 + [C.c_void_p] * 4)
+    L.scopi_hip_set_box.argtypes = [P, D, D, I32]
     L.scopi_hip_batch_create.argtypes = [C.POINTER(Params), C.c_double, C.POINTER(P)]
     L.scopi_hip_batch_destroy.argtypes = [P]
     L.scopi_hip_batch_last_error.argtypes = [P]
@@ -244,6 +245,13 @@ class HipScene:
             self.nc = info.n_contacts  # contacts of the last step stay on the device
             self.last_rows = info.rows
         return info.as_dict()
+
+    def set_box(self, lo, hi, periodic):
+        """BoxDomain(lo, hi).with_periodicity(axes with periodic[d] true) of the scene."""
+        lo = np.ascontiguousarray(lo, np.float64)
+        hi = np.ascontiguousarray(hi, np.float64)
+        per = np.ascontiguousarray([1 if p else 0 for p in periodic], np.int32)
+        self._check(self._L.scopi_hip_set_box(self._h, lo, hi, per), "set_box")
 
     def set_kernel_timing(self, on=True):
         self._check(self._L.scopi_hip_set_kernel_timing(self._h, 1 if on else 0), "set_kernel_timing")

@@ -287,15 +287,34 @@ class scopi_container {
     std::size_t m_nb_inactive = 0;
 };
 
+// BoxDomain (include/scopi/box.hpp): corners and periodic axes.
 template <std::size_t dim>
 class BoxDomain {
   public:
     BoxDomain() = default;
     BoxDomain(const std::array<double, dim>& lo, const std::array<double, dim>& hi) : m_lo(lo), m_hi(hi) {}
-    bool is_periodic(std::size_t) const { return false; }
+    BoxDomain& with_periodicity() {
+        m_periodic.fill(true);
+        return *this;
+    }
+    BoxDomain& with_periodicity(std::size_t axis) {
+        m_periodic[axis] = true;
+        return *this;
+    }
+    BoxDomain& with_periodicity(const std::initializer_list<std::size_t>& axes) {
+        for (auto a : axes) m_periodic[a] = true;
+        return *this;
+    }
+    const std::array<bool, dim>& is_periodic() const { return m_periodic; }
+    bool is_periodic(std::size_t axis) const { return m_periodic[axis]; }
+    const std::array<double, dim>& min_corner() const { return m_lo; }
+    const std::array<double, dim>& max_corner() const { return m_hi; }
+    double lower_bound(std::size_t axis) const { return m_lo[axis]; }
+    double upper_bound(std::size_t axis) const { return m_hi[axis]; }
 
   private:
     std::array<double, dim> m_lo{}, m_hi{};
+    std::array<bool, dim> m_periodic{};
 };
 
 // ------------------------------------------------------------------ output files
@@ -791,6 +810,14 @@ class ScopiSolver {
                                              s.v.data(), s.vd.data(), s.w.data(), s.dw.data(), s.f.data(), s.m.data(),
                                              s.J.data()),
                       m_ctx, "scopi_hip_upload_state");
+        double lo[3] = {0, 0, 0}, hi[3] = {0, 0, 0};
+        int32_t per[3] = {0, 0, 0};
+        for (std::size_t d = 0; d < dim; ++d) {
+            lo[d] = m_box.lower_bound(d);
+            hi[d] = m_box.upper_bound(d);
+            per[d] = m_box.is_periodic(d) ? 1 : 0;
+        }
+        detail::check(scopi_hip_set_box(m_ctx, lo, hi, per), m_ctx, "scopi_hip_set_box");
         m_info = scopi_hip_solve_info{};
         m_info.converged = 1;
     }

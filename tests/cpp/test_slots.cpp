@@ -158,6 +158,26 @@ void sphere_plane_friction() {
     dump("sphere_plane_friction_mu01", particles, file);
 }
 
+// A periodic box through ScopiSolver(box, particles): two spheres moving towards each other
+// across the periodic x bound of [0, 1] x [0, 1] (BoxDomain::with_periodicity(0)).
+void periodic_pair() {
+    constexpr std::size_t dim = 2;
+    scopi_container<dim> particles;
+    sphere<dim> s1({{0.012, 0.5}}, 0.01);
+    sphere<dim> s2({{0.985, 0.503}}, 0.012);
+    particles.push_back(s1, property<dim>().mass(1.).moment_inertia(0.1).desired_velocity({{-0.5, 0.}}));
+    particles.push_back(s2, property<dim>().mass(1.).moment_inertia(0.1).desired_velocity({{0.5, 0.}}));
+    BoxDomain<dim> box({{0., 0.}}, {{1., 1.}});
+    box.with_periodicity(0);
+    ScopiSolver<dim, NoFriction, OptimGradient<apgd>, contact_kdtree, vap_fixed> solver(box, particles);
+    auto params = solver.get_params();
+    params.contact_method_params.dmax = 0.02;
+    params.contact_method_params.kd_tree_radius = 0.1 * 0.1;
+    const auto file = reference_output(solver, "periodic", "periodic_pair", 40);
+    solver.run(0.005, 40);
+    dump("periodic_pair", particles, file);
+}
+
 // test_contacts_kdtree.cpp:14-72 through the contact slot alone, then the optim slot alone.
 void slots_one_by_one() {
     constexpr std::size_t dim = 2;
@@ -231,6 +251,7 @@ int main(int argc, char** argv) {
         if (only.empty() || only == "friction") sphere_plane_friction();
         if (only.empty() || only == "slots") slots_one_by_one();
         if (only.empty() || only == "superellipsoids") superellipsoids_3d();
+        if (only.empty() || only == "periodic") periodic_pair();
     } catch (const std::exception& e) {
         std::fprintf(stderr, "test_slots: %s\n", e.what());
         return 1;

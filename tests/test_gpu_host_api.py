@@ -117,3 +117,23 @@ def test_superellipsoid_contact_slot(output):
         for got, ref in ((c["pi"], pi), (c["pj"], pj), (c["nij"], nij)):
             for g, r in zip(got, ref):
                 assert abs(g - r) < tol(r), (name, got, ref)
+
+
+def test_periodic_box_through_slot_api(output, oracle_lib):
+    """ScopiSolver(box, particles) with BoxDomain::with_periodicity(0): the two spheres meet across the
+    periodic bound; the mirror's end state equals the oracle's (same box, same slots) to 1e-9 modulo L."""
+    from scopi_amd.scenes import Container, Property
+    got = output["periodic_pair"]["objects"]
+    c = Container(2)
+    c.sphere((0.012, 0.5), 0.01, Property(mass=1.0, moment_inertia=0.1, desired_velocity=(-0.5, 0.0)))
+    c.sphere((0.985, 0.503), 0.012, Property(mass=1.0, moment_inertia=0.1, desired_velocity=(0.5, 0.0)))
+    o = oracle_lib.Oracle(c.arrays(), dmax=0.02, kd_tree_radius=0.1 ** 2)
+    o.set_box([0.0, 0.0], [1.0, 1.0], [True, False])
+    o.run(0.005, 40)
+    so = o.state()
+    for k, ob in enumerate(got):
+        d = np.array(ob["position"]) - so["pos"][k, :2]
+        d[0] -= np.round(d[0])
+        assert np.abs(d).max() <= 1e-9, (k, ob["position"], so["pos"][k])
+    # the spheres met across the bound: the contact pushed them apart in y
+    assert abs(got[0]["position"][1] - 0.5) > 1e-3 and abs(got[1]["position"][1] - 0.503) > 1e-3
