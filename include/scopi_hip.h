@@ -42,11 +42,14 @@ extern "C" {
 /* ------------------------------------------------------------------ enums */
 enum { SCOPI_SHAPE_SPHERE = 0, SCOPI_SHAPE_PLANE = 1, SCOPI_SHAPE_SEGMENT = 2, SCOPI_SHAPE_SUPERELLIPSOID = 3 };
 /* problem tags NoFriction / Friction (include/scopi/contact/property.hpp:14-20) */
-enum { SCOPI_PROBLEM_NOFRICTION = 0, SCOPI_PROBLEM_FRICTION = 1 };
+/* problem tags: NoFriction, Friction, FrictionFixedPoint (Friction with the outer fixed point
+ * on s_ij, OptimGradient.hpp:88-166) */
+enum { SCOPI_PROBLEM_NOFRICTION = 0, SCOPI_PROBLEM_FRICTION = 1, SCOPI_PROBLEM_FRICTION_FIXED_POINT = 2 };
 /* contact slot: contact_kdtree (radius prefilter) / contact_brute_force */
 enum { SCOPI_CONTACT_KDTREE = 0, SCOPI_CONTACT_BRUTE_FORCE = 1 };
 /* vap slot: vap_fixed / vap_fpd */
-enum { SCOPI_VAP_FIXED = 0, SCOPI_VAP_FPD = 1 };
+/* vap slot: vap_fixed, vap_fpd, vap_projection (velocities given by scopi_hip_set_projection_velocity) */
+enum { SCOPI_VAP_FIXED = 0, SCOPI_VAP_FPD = 1, SCOPI_VAP_PROJECTION = 2 };
 /* optim method: apgd / pgd (include/scopi/solvers/apgd.hpp:39-227) */
 enum { SCOPI_METHOD_APGD = 0, SCOPI_METHOD_PGD = 1 };
 /* arithmetic of the projected-gradient loop */
@@ -73,6 +76,8 @@ typedef struct scopi_hip_params {
     int32_t warm_start;       /* reserved, must be 0 (lambda_0 = 0 every step, apgd.hpp:152-159) */
     int32_t general_jacobian; /* 1: keep the rotational blocks of B even when every active particle is a
                                  sphere (they vanish for spheres; this only changes speed) */
+    double fixed_point_tol;       /* FrictionFixedPoint: contact_property<FixedPoint> (property.hpp:97-98) */
+    int64_t fixed_point_max_iter;
 } scopi_hip_params;
 
 typedef struct scopi_hip_solve_info {
@@ -85,6 +90,8 @@ typedef struct scopi_hip_solve_info {
     int32_t converged;
     int32_t status;
     double residual;       /* last ||lambda_{k+1} - lambda_k||_2 */
+    int64_t outer_iterations;  /* FrictionFixedPoint: solves of the fixed point (0 otherwise) */
+    double fixed_point_error;  /* FrictionFixedPoint: the last relative change of s */
 } scopi_hip_solve_info;
 
 typedef struct scopi_hip_ctx scopi_hip_ctx;
@@ -126,9 +133,15 @@ int scopi_hip_download_contacts(scopi_hip_ctx* ctx, int64_t* i, int64_t* j, doub
 int scopi_hip_upload_contacts(scopi_hip_ctx* ctx, int64_t nc, const int64_t* i, const int64_t* j, const double* dij,
                               const double* nij, const double* pi, const double* pj, const double* mu);
 
-/* Vap slot: vap_fixed (vap/vap_fixed.hpp:46-55) or vap_fpd (vap/vap_fpd.hpp:70-82,
- * src/vap/vap_fpd.cpp:6-26) applied to the device velocities. */
+/* Vap slot: vap_fixed (vap/vap_fixed.hpp:46-55), vap_fpd (vap/vap_fpd.hpp:70-82,
+ * src/vap/vap_fpd.cpp:6-26) or vap_projection (vap/vap_projection.hpp:34-49) applied to the
+ * device velocities. */
 int scopi_hip_set_a_priori_velocity(scopi_hip_ctx* ctx, const scopi_hip_params* params, double dt);
+/* vap_projection::set_u_w (src/vap/vap_projection.cpp:6-10): u, w are Na x 3 (active particles).
+ * The slot then sets every active particle's desired velocity to u and its rotation velocity to
+ * w(i, 2) (in 3D the scalar is assigned to the three components, as the reference's assignment
+ * of a scalar to a 3-vector does). */
+int scopi_hip_set_projection_velocity(scopi_hip_ctx* ctx, const double* u, const double* w);
 
 /* Optim slot: OptimGradient<apgd|pgd>::run (solvers/OptimGradient.hpp:252-306):
  * assemble B from the device contacts, solve the dual problem, compute
@@ -225,7 +238,9 @@ int scopi_hip_kernel_timing(scopi_hip_ctx* ctx, double* out);
  * results do not depend on the other scenes of the batch (bitwise: the same alone
  * and in any mix).  Scene ids are returned by add_scene and stay valid until
  * remove_scene; scenes added or removed take effect at the next call that reads or
- * steps the batch.  The params must not ask for warm_start. */
+ * steps the batch.  The params must not ask for warm_start or vap_projection
+ * (SCOPI_HIP_ERR_ARG); FrictionFixedPoint is refused with SCOPI_HIP_ERR_UNSUPPORTED
+ * (its outer loop is per scene; step such scenes through scopi_hip_run). */
 typedef struct scopi_hip_batch scopi_hip_batch;
 int scopi_hip_batch_create(const scopi_hip_params* params, double dt, scopi_hip_batch** out);
 int scopi_hip_batch_destroy(scopi_hip_batch* b);

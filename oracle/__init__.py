@@ -15,9 +15,9 @@ _HERE = os.path.dirname(os.path.abspath(__file__))
 _LIB_PATH = os.path.join(_HERE, "liboracle.so")
 _lib = None
 
-NOFRICTION, FRICTION = 0, 1
+NOFRICTION, FRICTION, FRICTION_FIXED_POINT = 0, 1, 2
 KDTREE, BRUTE_FORCE = 0, 1
-VAP_FIXED, VAP_FPD = 0, 1
+VAP_FIXED, VAP_FPD, VAP_PROJECTION = 0, 1, 2
 APGD, PGD = 0, 1
 
 
@@ -26,6 +26,7 @@ class Params(C.Structure):
         ("problem", C.c_int32), ("contact_method", C.c_int32), ("vap", C.c_int32), ("method", C.c_int32),
         ("dmax", C.c_double), ("kd_tree_radius", C.c_double), ("alpha", C.c_double), ("max_ite", C.c_int64),
         ("tolerance", C.c_double), ("dynamic_descent", C.c_int32), ("pad", C.c_int32), ("mu", C.c_double),
+        ("fixed_point_tol", C.c_double), ("fixed_point_max_iter", C.c_double),
     ]
 
 
@@ -33,7 +34,7 @@ class Stats(C.Structure):
     _fields_ = [
         ("n_contacts", C.c_int64), ("iterations", C.c_int64), ("backtracks", C.c_int64), ("q_applies", C.c_int64),
         ("converged", C.c_int64), ("total_iterations", C.c_int64), ("total_q_applies", C.c_int64),
-        ("t_contacts", C.c_double), ("t_vap", C.c_double), ("t_solve", C.c_double), ("t_move", C.c_double),
+        ("fixed_point_iterations", C.c_int64), ("t_contacts", C.c_double), ("t_vap", C.c_double), ("t_solve", C.c_double), ("t_move", C.c_double),
     ]
 
 
@@ -58,6 +59,7 @@ def lib():
         L.oracle_scene_set.argtypes = [P, C.c_int32, C.c_int64, C.c_int64, I32] + [D] * 11
         L.oracle_get_state.argtypes = [P, D, D, D, D]
         L.oracle_set_box.argtypes = [P, D, D, I32]
+        L.oracle_set_projection_velocity.argtypes = [P, D, D]
         L.oracle_run.argtypes = [P, C.c_double, C.c_int64, C.c_int64]
         L.oracle_compute_contacts.argtypes = [P]
         L.oracle_compute_contacts.restype = C.c_int64
@@ -127,6 +129,12 @@ class Oracle:
         per = np.ascontiguousarray([1 if p else 0 for p in periodic], np.int32)
         assert lib().oracle_set_box(self._h, lo, hi, per) == 0
 
+    def set_projection_velocity(self, u, w):
+        """vap_projection::set_u_w (Na x 3 each)."""
+        na = self.n - self.n_inactive
+        assert lib().oracle_set_projection_velocity(self._h, np.ascontiguousarray(np.reshape(u, (na, 3)), np.float64),
+                                                    np.ascontiguousarray(np.reshape(w, (na, 3)), np.float64)) == 0
+
     def run(self, dt, total_it, initial_iter=0):
         self._L.oracle_run(self._h, float(dt), int(total_it), int(initial_iter))
 
@@ -163,10 +171,10 @@ class Oracle:
     def solve(self, dt):
         nc = int(self._L.oracle_n_contacts(self._h))
         na = self.n - self.n_inactive
-        lam = np.zeros(max(1, nc * (3 if self.problem == FRICTION else 1)))
+        lam = np.zeros(max(1, nc * (1 if self.problem == NOFRICTION else 3)))
         u = np.zeros(max(1, 3 * na)); w = np.zeros(max(1, 3 * na))
         self._L.oracle_solve(self._h, float(dt), lam, u, w)
-        return dict(lam=lam[: nc * (3 if self.problem == FRICTION else 1)], u=u[: 3 * na].reshape(na, 3),
+        return dict(lam=lam[: nc * (1 if self.problem == NOFRICTION else 3)], u=u[: 3 * na].reshape(na, 3),
                     w=w[: 3 * na].reshape(na, 3))
 
     def stats(self):

@@ -56,6 +56,7 @@ struct Scene {
     int dim = 2;
     int64_t n = 0, n_inactive = 0;
     Box box;
+    std::vector<double> proj_u, proj_w;  // vap_projection::set_u_w, Na x 3
     std::vector<int32_t> shape;
     std::vector<vec3> pos, size, v, vd, omega, domega, f, J;
     std::vector<std::array<double, 2>> sq;  // superellipsoid squareness (e, n)
@@ -532,8 +533,15 @@ struct MinProblem {
     mutable std::vector<double> g3, w6, a3;
     mutable int64_t q_applies = 0;
 
-    MinProblem(const Scene& sc, const std::vector<Contact>& c, double dt_) : s(sc), cs(c), dt(dt_) {
-        friction = (s.prm.problem == ORACLE_FRICTION);
+    std::vector<double> S;  // S_Vector: s_ij n_ij per contact for FrictionFixedPoint (lagrange_multiplier.hpp:290-297)
+
+    MinProblem(const Scene& sc, const std::vector<Contact>& c, double dt_, const std::vector<double>* sij = nullptr)
+        : s(sc), cs(c), dt(dt_) {
+        friction = (s.prm.problem == ORACLE_FRICTION || s.prm.problem == ORACLE_FRICTION_FIXED_POINT);
+        S.assign(friction ? 3 * cs.size() : cs.size(), 0.0);
+        if (sij)
+            for (size_t k = 0; k < cs.size(); ++k)
+                for (int d = 0; d < s.dim; ++d) S[3 * k + d] = (*sij)[k] * cs[k].nij[d];
         const int64_t na = s.na(), off = s.n_inactive, nc = (int64_t)cs.size();
         // M_inverse: minimization_problem.hpp:10-31
         invM.assign(6 * na, 0.0);
@@ -600,20 +608,23 @@ struct MinProblem {
         amat(s, cs, w6.data(), out);
         for (size_t k = 0; k < 3 * cs.size(); ++k) out[k] = dt * dt * out[k];
     }
-    // gradient: minimization_problem.hpp:122-128 (S = 0 for both problems)
+    // gradient: minimization_problem.hpp:122-128, g2l(Q l2g(lambda) + C) + S
     void gradient(const double* lam, double* out) const {
         l2g(lam, g3.data());
         Q(g3.data(), a3.data());
         for (size_t k = 0; k < a3.size(); ++k) a3[k] = a3[k] + C[k];
         g2l(a3.data(), out);
+        for (int64_t k = 0; k < size(); ++k) out[k] = out[k] + S[k];
     }
-    // operator(): minimization_problem.hpp:130-134
+    // operator(): minimization_problem.hpp:130-134, <l2g(lambda), Q l2g(lambda) / 2 + C> + <lambda, S>
     double value(const double* lam) const {
         l2g(lam, g3.data());
         Q(g3.data(), a3.data());
         double r = 0.0;
         for (size_t k = 0; k < a3.size(); ++k) r += g3[k] * (0.5 * a3[k] + C[k]);
-        return r;  // + <lambda, S> with S = 0
+        double rs = 0.0;
+        for (int64_t k = 0; k < size(); ++k) rs += lam[k] * S[k];
+        return r + rs;
     }
     // velocities: minimization_problem.hpp:136-139 -> M^-1 A^T l2g(lambda)
     void velocities(const double* lam, double* out) const {
@@ -752,12 +763,62 @@ SolveResult pgd(const MinProblem& mp, const oracle_params& prm) {
 }
 
 // OptimGradient::run: solvers/OptimGradient.hpp:252-306
+void optim_solve(Scene& s, double dt, const std::vector<double>* sij);
+
+// OptimGradient::run inside ScopiSolver::run's should_solve loop (solver.hpp:287-292): one solve,
+// or for FrictionFixedPoint the outer fixed point of OptimGradient.hpp:88-166: s_ij = mu dt,
+// solve, s_ij = mu dt |T(A U)_ij| (tangential part of the contact's relative velocity under the
+// solve's velocities), until (1/(mu dt)) |s_new - s_old| / (1 + (1/(mu dt)) |s_new|) < tol or
+// max_iter solves (mu, tol, max_iter of the first contact).
 void optim_run(Scene& s, double dt) {
+    s.st.fixed_point_iterations = 0;
+    if (s.prm.problem != ORACLE_FRICTION_FIXED_POINT || s.contacts.empty()) {
+        optim_solve(s, dt, nullptr);
+        return;
+    }
+    const double mu = s.contacts[0].mu, mudt = mu * dt;
+    const int64_t nc = (int64_t)s.contacts.size(), na = s.na();
+    std::vector<double> sij(nc);
+    for (int64_t k = 0; k < nc; ++k) sij[k] = s.contacts[k].mu * dt;
+    int64_t it = 0, total = 0;
+    for (;;) {
+        optim_solve(s, dt, &sij);
+        total += s.st.iterations;
+        std::vector<double> U(6 * na, 0.0), AU(3 * nc);
+        for (int64_t i = 0; i < na; ++i) {
+            for (int d = 0; d < s.dim; ++d) U[3 * i + d] = s.u[3 * i + d];
+            if (s.dim == 2)
+                U[3 * na + 3 * i + 2] = s.w[3 * i + 2];
+            else
+                for (int d = 0; d < 3; ++d) U[3 * na + 3 * i + d] = s.w[3 * i + d];
+        }
+        amat(s, s.contacts, U.data(), AU.data());
+        double ds2 = 0.0, s2 = 0.0;
+        for (int64_t k = 0; k < nc; ++k) {
+            const double* a = AU.data() + 3 * k;
+            const double* n = s.contacts[k].nij.data();
+            const double an = dot(a, n, s.dim);
+            vec3 t{0, 0, 0};
+            for (int d = 0; d < s.dim; ++d) t[d] = a[d] - n[d] * an;
+            const double old = sij[k];
+            sij[k] = s.contacts[k].mu * dt * nrm2(t.data(), s.dim);
+            ds2 += (sij[k] - old) * (sij[k] - old);
+            s2 += sij[k] * sij[k];
+        }
+        const double err = (1 / mudt) * std::sqrt(ds2) / (1 + (1 / mudt) * std::sqrt(s2));
+        ++it;
+        if (err < s.prm.fixed_point_tol || (double)it >= s.prm.fixed_point_max_iter) break;
+    }
+    s.st.fixed_point_iterations = it;
+    s.st.iterations = total;
+}
+
+void optim_solve(Scene& s, double dt, const std::vector<double>* sij) {
     const int64_t na = s.na(), off = s.n_inactive;
     s.u.assign(3 * na, 0.0);
     s.w.assign(3 * na, 0.0);
     if (!s.contacts.empty()) {
-        MinProblem mp(s, s.contacts, dt);
+        MinProblem mp(s, s.contacts, dt, sij);
         SolveResult r = (s.prm.method == ORACLE_METHOD_PGD) ? pgd(mp, s.prm) : apgd(mp, s.prm);
         s.lambda = r.lambda;
         std::vector<double> vel(6 * na);
@@ -794,7 +855,16 @@ void optim_run(Scene& s, double dt) {
 // vap_fixed (vap/vap_fixed.hpp:46-55) / vap_fpd (vap/vap_fpd.hpp:70-82, src/vap/vap_fpd.cpp:6-26)
 void set_a_priori_velocity(Scene& s, double dt) {
     for (int64_t i = s.n_inactive; i < s.n; ++i) {
-        if (s.prm.vap == ORACLE_VAP_FIXED) {
+        if (s.prm.vap == ORACLE_VAP_PROJECTION) {
+            // vap_projection::set_a_priori_velocity_impl (vap/vap_projection.hpp:34-49): the desired
+            // velocity takes u, the rotation velocity w(i, 2) (a scalar assigned to a 3-vector in 3D)
+            const int64_t a = i - s.n_inactive;
+            for (int d = 0; d < s.dim; ++d) s.vd[i][d] = s.proj_u[3 * a + d];
+            if (s.dim == 2)
+                s.omega[i][2] = s.proj_w[3 * a + 2];
+            else
+                for (int d = 0; d < 3; ++d) s.omega[i][d] = s.proj_w[3 * a + 2];
+        } else if (s.prm.vap == ORACLE_VAP_FIXED) {
             s.v[i] = s.vd[i];
             s.omega[i] = s.domega[i];
         } else {
@@ -961,6 +1031,8 @@ int oracle_default_params(oracle_params* p) {
     p->tolerance = 1e-7;
     p->dynamic_descent = 1;
     p->mu = 0.1;
+    p->fixed_point_tol = 1e-6;
+    p->fixed_point_max_iter = 1000;
     return 0;
 }
 
@@ -1012,6 +1084,14 @@ int oracle_set_box(void* scene, const double* lo, const double* hi, const int32_
         s.box.periodic[d] = periodic[d] != 0;
     }
     for (int d = s.dim; d < 3; ++d) s.box.periodic[d] = false;
+    return 0;
+}
+
+int oracle_set_projection_velocity(void* scene, const double* u, const double* w) {
+    if (!scene || !u || !w) return -1;
+    Scene& s = *static_cast<Scene*>(scene);
+    s.proj_u.assign(u, u + 3 * s.na());
+    s.proj_w.assign(w, w + 3 * s.na());
     return 0;
 }
 

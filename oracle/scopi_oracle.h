@@ -25,9 +25,9 @@ extern "C" {
 #endif
 
 enum { ORACLE_SPHERE = 0, ORACLE_PLANE = 1, ORACLE_SEGMENT = 2, ORACLE_SUPERELLIPSOID = 3 };
-enum { ORACLE_NOFRICTION = 0, ORACLE_FRICTION = 1 };
+enum { ORACLE_NOFRICTION = 0, ORACLE_FRICTION = 1, ORACLE_FRICTION_FIXED_POINT = 2 };
 enum { ORACLE_CONTACT_KDTREE = 0, ORACLE_CONTACT_BRUTE_FORCE = 1 };
-enum { ORACLE_VAP_FIXED = 0, ORACLE_VAP_FPD = 1 };
+enum { ORACLE_VAP_FIXED = 0, ORACLE_VAP_FPD = 1, ORACLE_VAP_PROJECTION = 2 };
 enum { ORACLE_METHOD_APGD = 0, ORACLE_METHOD_PGD = 1 };
 
 typedef struct oracle_params {
@@ -43,6 +43,8 @@ typedef struct oracle_params {
     int32_t dynamic_descent; /* apgd_params::dynamic_descent (default true)    */
     int32_t pad;
     double mu;               /* contact_property<Friction>::mu (default 0.1)   */
+    double fixed_point_tol;  /* contact_property<FixedPoint> (default 1e-6)    */
+    double fixed_point_max_iter; /* (default 1000)                            */
 } oracle_params;
 
 typedef struct oracle_stats {
@@ -53,6 +55,7 @@ typedef struct oracle_stats {
     int64_t converged;    /* 1 if the last solve met the tolerance            */
     int64_t total_iterations; /* summed over every solve since the scene was set */
     int64_t total_q_applies;
+    int64_t fixed_point_iterations; /* FrictionFixedPoint: solves of the last step */
     double t_contacts, t_vap, t_solve, t_move; /* seconds, summed over steps  */
 } oracle_stats;
 
@@ -70,6 +73,8 @@ int oracle_scene_set(void* scene, int32_t dim, int64_t n, int64_t n_inactive, co
                      const double* m, const double* J);
 /* BoxDomain of the scene (dim entries each): lower / upper corner, periodic axes. */
 int oracle_set_box(void* scene, const double* lo, const double* hi, const int32_t* periodic);
+/* vap_projection::set_u_w: u, w are Na x 3. */
+int oracle_set_projection_velocity(void* scene, const double* u, const double* w);
 int oracle_get_state(void* scene, double* pos, double* q, double* v, double* omega);
 
 /* ScopiSolver::run(dt, total_it, initial_iter) without output files. */

@@ -992,8 +992,10 @@ Solver::~Solver() {
                     (void*)partial_trial_, (void*)partial_row_})
         if (p) (void)hipFree(p);
     if (h_ctrl_) (void)hipHostFree(h_ctrl_);
-    for (void* p : {(void*)d_arange_, (void*)d_done_, (void*)d_buf_, (void*)d_tab_, (void*)mctrl_})
+    for (void* p : {(void*)d_arange_, (void*)d_done_, (void*)d_buf_, (void*)d_tab_, (void*)mctrl_, (void*)fp_c0_,
+                    (void*)fp_s_, (void*)fp_partial_})
         if (p) (void)hipFree(p);
+    if (h_fp_partial_) (void)hipHostFree(h_fp_partial_);
     for (void* p : {(void*)h_done_, (void*)h_tab_, (void*)h_ctrls_})
         if (p) (void)hipHostFree(p);
 }
@@ -1213,17 +1215,29 @@ void Solver::solve_layout(const Particles& P, const Contacts& C, int64_t nc, con
     const int64_t na = P.n - P.n_inactive;
     const bool fp32 = prm.precision == SCOPI_PRECISION_FP32_REFINE;
     assemble<LAY>(P, C, nc, dt, fp32, s);
+    solve_phases<LAY>(P, C, nc, prm, dt, s, info);
+}
+
+// The projected-gradient solve on the assembled B and c, then u = v + dt M^-1 B^T lambda.
+template <int LAY>
+void Solver::solve_phases(const Particles& P, const Contacts& C, int64_t nc, const scopi_hip_params& prm, double dt,
+                          hipStream_t s, scopi_hip_solve_info& info) {
+    constexpr int NR = lay<LAY>::NR, NW = lay<LAY>::NW;
+    const int64_t na = P.n - P.n_inactive;
+    const bool fp32 = prm.precision == SCOPI_PRECISION_FP32_REFINE;
     if (fp32) {
         // FP32 phase to a relative stop or its precision floor, then FP64 refinement
         // warm-started from it with the last accepted L
         const double rtol32 = prm.fp32_tolerance > 0 ? 0.0 : 1e-5;
         const double tol32 = prm.fp32_tolerance > 0 ? prm.fp32_tolerance : prm.tolerance;
+        const int64_t before = info.iterations;
         run_phase<float, LAY>(b32_, P, C, na, nc, prm, dt, tol32, rtol32, prm.max_ite, false, false, s, info);
-        info.iterations_fp32 = info.iterations;
+        const int64_t it32 = info.iterations - before;
+        info.iterations_fp32 += it32;
         const int64_t stride32 = NR * b32_.cap;
         hipLaunchKernelGGL((k_convert<float, double>), dim3(1024), dim3(kBlock), 0, s,
                            b32_.lam + (int64_t)final_slot_ * stride32, b64_.lam, NR * nc);
-        const int64_t left = std::max<int64_t>(prm.max_ite - info.iterations, 1);
+        const int64_t left = std::max<int64_t>(prm.max_ite - it32, 1);
         run_phase<double, LAY>(b64_, P, C, na, nc, prm, dt, prm.tolerance, 0.0, left, true, true, s, info);
     } else {
         run_phase<double, LAY>(b64_, P, C, na, nc, prm, dt, prm.tolerance, 0.0, prm.max_ite, false, false, s, info);
@@ -1233,6 +1247,127 @@ void Solver::solve_layout(const Particles& P, const Contacts& C, int64_t nc, con
     hipLaunchKernelGGL(k_velocities<NW>, dim3(grid_for(na)), dim3(kBlock), 0, s, P, (const double*)b64_.w, pcap_, dt, 1,
                        u_, w_);
     SCOPI_HIP_CHECK(hipGetLastError());
+}
+
+// ------------------------------------------------------------------ FrictionFixedPoint
+// c = c0 + S with S = s_ij n_ij on the contact's first dim rows (LagrangeMultiplier<dim,
+// FrictionFixedPoint> S_Vector, lagrange_multiplier.hpp:290-297; the gradient and the objective
+// of minimization_problem.hpp:130-141 add S to Q lambda + c and lambda . S to the objective, the
+// same as solving with c + S)
+__global__ void __launch_bounds__(kBlock) k_apply_s(BMatrix<double> B64, BMatrix<float> B32, int with32,
+                                                   const double* __restrict__ c0, const double* __restrict__ sij,
+                                                   int64_t nc, int dim) {
+#pragma clang fp contract(off)
+    const int64_t c = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (c >= nc) return;
+    for (int d = 0; d < 3; ++d) {
+        const double v = c0[3 * c + d] + (d < dim ? sij[c] * B64.coef[c * 21 + d] : 0.0);
+        B64.c[3 * c + d] = v;
+        if (with32) B32.c[3 * c + d] = (float)v;
+    }
+}
+
+// s_ij = mu_ij dt (extra_steps_before_solve<FrictionFixedPoint>, OptimGradient.hpp:86-94)
+__global__ void __launch_bounds__(kBlock) k_init_s(double* __restrict__ sij, const double* __restrict__ mu, int64_t nc,
+                                                  double dt) {
+#pragma clang fp contract(off)
+    for (int64_t c = (int64_t)blockIdx.x * kBlock + threadIdx.x; c < nc; c += (int64_t)gridDim.x * kBlock)
+        sij[c] = mu[c] * dt;
+}
+
+// extra_steps_after_solve<FrictionFixedPoint> (OptimGradient.hpp:116-166): A U at every contact
+// from the solve's velocities (the rows of B: e_d . (u_i - u_j) + K_i[d] . w_i - K_j[d] . w_j),
+// its tangential part T, s_ij = mu_ij dt |T| over the first dim components; block sums of
+// |s_new - s_old|^2 and |s_new|^2, and mu of contact 0 (the error's scale) after the sums.
+__global__ void __launch_bounds__(kBlock) k_fixed_point_s(BMatrix<double> B, const int32_t* __restrict__ ci,
+                                                         const int32_t* __restrict__ cj, int64_t ni, int64_t nc,
+                                                         const double* __restrict__ u, const double* __restrict__ w,
+                                                         int64_t pcap, double dt, int dim, double* __restrict__ sij,
+                                                         double* __restrict__ partial) {
+#pragma clang fp contract(off)
+    __shared__ double lds[2][kBlock / 64];
+    double acc[2] = {0.0, 0.0};
+    for (int64_t c = (int64_t)blockIdx.x * kBlock + threadIdx.x; c < nc; c += (int64_t)gridDim.x * kBlock) {
+        const int64_t i = (int64_t)ci[c] - ni, j = (int64_t)cj[c] - ni;
+        const double* co = B.coef + c * 21;
+        double au[3];
+        for (int d = 0; d < 3; ++d) {
+            double a = 0.0;
+            if (i >= 0) {
+                a += u[d * pcap + i];
+                a += co[3 + 3 * d] * w[i] + co[4 + 3 * d] * w[pcap + i] + co[5 + 3 * d] * w[2 * pcap + i];
+            }
+            if (j >= 0) {
+                a -= u[d * pcap + j];
+                a -= co[12 + 3 * d] * w[j] + co[13 + 3 * d] * w[pcap + j] + co[14 + 3 * d] * w[2 * pcap + j];
+            }
+            au[d] = a;
+        }
+        double an = 0.0;
+        for (int d = 0; d < dim; ++d) an += au[d] * co[d];
+        double t2 = 0.0;
+        for (int d = 0; d < dim; ++d) {
+            const double t = au[d] - co[d] * an;
+            t2 += t * t;
+        }
+        const double s_new = B.mu[c] * dt * sqrt(t2);
+        const double ds = s_new - sij[c];
+        sij[c] = s_new;
+        acc[0] += ds * ds;
+        acc[1] += s_new * s_new;
+    }
+    block_sum<2>(acc, lds);
+    if (threadIdx.x == 0) {
+        partial[2 * blockIdx.x] = acc[0];
+        partial[2 * blockIdx.x + 1] = acc[1];
+        if (blockIdx.x == 0) partial[2 * gridDim.x] = B.mu[0];
+    }
+}
+
+template <int LAY>
+void Solver::solve_fixed_point(const Particles& P, const Contacts& C, int64_t nc, const scopi_hip_params& prm,
+                               double dt, hipStream_t s, scopi_hip_solve_info& info) {
+    const int64_t ni = P.n_inactive;
+    const bool fp32 = prm.precision == SCOPI_PRECISION_FP32_REFINE;
+    assemble<LAY>(P, C, nc, dt, fp32, s);
+    if (nc > fp_cap_) {
+        for (void* p : {(void*)fp_c0_, (void*)fp_s_})
+            if (p) SCOPI_HIP_CHECK(hipFree(p));
+        fp_cap_ = nc + nc / 4 + 1024;
+        SCOPI_HIP_CHECK(hipMalloc(&fp_c0_, 3 * fp_cap_ * sizeof(double)));
+        SCOPI_HIP_CHECK(hipMalloc(&fp_s_, fp_cap_ * sizeof(double)));
+    }
+    if (!fp_partial_) {
+        SCOPI_HIP_CHECK(hipMalloc(&fp_partial_, (2 * 1024 + 1) * sizeof(double)));
+        SCOPI_HIP_CHECK(hipHostMalloc(&h_fp_partial_, (2 * 1024 + 1) * sizeof(double)));
+    }
+    SCOPI_HIP_CHECK(hipMemcpyAsync(fp_c0_, b64_.c, 3 * nc * sizeof(double), hipMemcpyDeviceToDevice, s));
+    hipLaunchKernelGGL(k_init_s, dim3(256), dim3(kBlock), 0, s, fp_s_, (const double*)b64_.mu, nc, dt);
+    const unsigned gp = (unsigned)std::min<int64_t>(grid_for(nc), 1024);
+    BMatrix<float> b32 = fp32 ? b32_ : BMatrix<float>{};
+    info.outer_iterations = 0;
+    for (;;) {
+        hipLaunchKernelGGL(k_apply_s, dim3(grid_for(nc)), dim3(kBlock), 0, s, b64_, b32, fp32 ? 1 : 0,
+                           (const double*)fp_c0_, (const double*)fp_s_, nc, P.dim);
+        solve_phases<LAY>(P, C, nc, prm, dt, s, info);
+        info.outer_iterations += 1;
+        hipLaunchKernelGGL(k_fixed_point_s, dim3(gp), dim3(kBlock), 0, s, b64_, (const int32_t*)C.i,
+                           (const int32_t*)C.j, ni, nc, (const double*)u_, (const double*)w_, pcap_, dt, P.dim,
+                           fp_s_, fp_partial_);
+        SCOPI_HIP_CHECK(
+            hipMemcpyAsync(h_fp_partial_, fp_partial_, (2 * gp + 1) * sizeof(double), hipMemcpyDeviceToHost, s));
+        SCOPI_HIP_CHECK(hipStreamSynchronize(s));
+        double ds2 = 0.0, s2 = 0.0;
+        for (unsigned b = 0; b < gp; ++b) {
+            ds2 += h_fp_partial_[2 * b];
+            s2 += h_fp_partial_[2 * b + 1];
+        }
+        // err = (1 / (mu_0 dt)) |s_new - s_old| / (1 + (1 / (mu_0 dt)) |s_new|)  (OptimGradient.hpp:157-158)
+        const double mudt = h_fp_partial_[2 * gp] * dt;
+        const double err = (1 / mudt) * std::sqrt(ds2) / (1 + (1 / mudt) * std::sqrt(s2));
+        info.fixed_point_error = err;
+        if (err < prm.fixed_point_tol || info.outer_iterations >= prm.fixed_point_max_iter) break;
+    }
 }
 
 int Solver::set_tuning(int key, int value) {
@@ -1257,11 +1392,13 @@ int Solver::set_tuning(int key, int value) {
 int Solver::solve(const Particles& P, const Contacts& C, int64_t nc, const scopi_hip_params& prm, double dt,
                   bool rot_free, hipStream_t s, scopi_hip_solve_info* info) {
     const int64_t na = P.n - P.n_inactive;
-    const bool friction = prm.problem == SCOPI_PROBLEM_FRICTION;
+    const bool fixed_point = prm.problem == SCOPI_PROBLEM_FRICTION_FIXED_POINT;
+    const bool friction = prm.problem == SCOPI_PROBLEM_FRICTION || fixed_point;
     const bool fp32 = prm.precision == SCOPI_PRECISION_FP32_REFINE;
-    if (prm.problem != SCOPI_PROBLEM_NOFRICTION && prm.problem != SCOPI_PROBLEM_FRICTION) return SCOPI_HIP_ERR_UNSUPPORTED;
+    if (prm.problem != SCOPI_PROBLEM_NOFRICTION && !friction) return SCOPI_HIP_ERR_UNSUPPORTED;
     if (prm.method != SCOPI_METHOD_APGD && prm.method != SCOPI_METHOD_PGD) return SCOPI_HIP_ERR_UNSUPPORTED;
     if (!(prm.alpha > 0)) return SCOPI_HIP_ERR_ARG;
+    if (fixed_point && !(dt > 0)) return SCOPI_HIP_ERR_ARG;
     reserve(na, nc, fp32);
     layout_ = friction ? kCone : ((rot_free && !prm.general_jacobian) ? kRotFree : kRigid);
     nr_ = friction ? 3 : 1;
@@ -1279,7 +1416,9 @@ int Solver::solve(const Particles& P, const Contacts& C, int64_t nc, const scopi
         if (info) *info = loc;
         return SCOPI_HIP_OK;
     }
-    if (layout_ == kCone)
+    if (fixed_point)
+        solve_fixed_point<kCone>(P, C, nc, prm, dt, s, loc);
+    else if (layout_ == kCone)
         solve_layout<kCone>(P, C, nc, prm, dt, s, loc);
     else if (layout_ == kRigid)
         solve_layout<kRigid>(P, C, nc, prm, dt, s, loc);

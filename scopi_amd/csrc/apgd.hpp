@@ -138,6 +138,12 @@ class Solver {
                       hipStream_t s, scopi_hip_solve_info& info);
     template <int LAY>
     void assemble(const Particles& P, const Contacts& C, int64_t nc, double dt, bool fp32, hipStream_t s);
+    template <int LAY>
+    void solve_phases(const Particles& P, const Contacts& C, int64_t nc, const scopi_hip_params& prm, double dt,
+                      hipStream_t s, scopi_hip_solve_info& info);
+    template <int LAY>
+    void solve_fixed_point(const Particles& P, const Contacts& C, int64_t nc, const scopi_hip_params& prm, double dt,
+                           hipStream_t s, scopi_hip_solve_info& info);
     // multi-scene loop: tables of the scenes still running, phase loop, per-scene results
     void scene_tables(const std::vector<int>& scenes, hipStream_t s);
     template <class T, int LAY>
@@ -180,6 +186,12 @@ class Solver {
     int bt_group_ = 8;          // lanes per particle in k_bt
     int graph_rounds_ = 0;      // > 0: the one-scene loop replays a hipGraph of this many rounds
     bool has_solution_ = false;
+    // FrictionFixedPoint: c without S, s_ij, block sums of the error
+    double* fp_c0_ = nullptr;
+    double* fp_s_ = nullptr;
+    double* fp_partial_ = nullptr;
+    double* h_fp_partial_ = nullptr;
+    int64_t fp_cap_ = 0;
     // multi-scene state
     const SceneSet* scenes_ = nullptr;
     int nscene_cap_ = 0;

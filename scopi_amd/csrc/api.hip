@@ -30,6 +30,8 @@ struct scopi_hip_ctx {
     double box_lo[3] = {0, 0, 0}, box_hi[3] = {0, 0, 0};
     int periodic[3] = {0, 0, 0};
     bool any_periodic() const { return periodic[0] || periodic[1] || periodic[2]; }
+    double* proj = nullptr;        // vap_projection: u then w, Na x 3 each
+    int64_t proj_n = -1;           // active particles the projection velocities were given for
     Contacts C2;                   // the search's pairs over the particles and their copies
     int* pflags = nullptr;         // periodic copies: flags, offsets, originals, sort buffers
     int* poffs = nullptr;
@@ -46,13 +48,23 @@ struct scopi_hip_ctx {
 
 namespace {
 
-// vap_fixed (vap/vap_fixed.hpp:46-55) / vap_fpd (vap/vap_fpd.hpp:70-82, src/vap/vap_fpd.cpp:6-26)
-__global__ void __launch_bounds__(kBlock) k_vap(Particles P, int kind, double dt) {
+// vap_fixed (vap/vap_fixed.hpp:46-55) / vap_fpd (vap/vap_fpd.hpp:70-82, src/vap/vap_fpd.cpp:6-26) /
+// vap_projection (vap/vap_projection.hpp:34-49: vd = u, omega = w(i, 2) broadcast; pu, pw: Na x 3)
+__global__ void __launch_bounds__(kBlock) k_vap(Particles P, int kind, double dt, const double* __restrict__ pu,
+                                               const double* __restrict__ pw) {
 #pragma clang fp contract(off)
     const int64_t t = (int64_t)blockIdx.x * kBlock + threadIdx.x;
     const int64_t k = P.n_inactive + t;
     if (k >= P.n) return;
     const int64_t c = P.cap;
+    if (kind == SCOPI_VAP_PROJECTION) {
+        for (int d = 0; d < P.dim; ++d) P.vd[d * c + k] = pu[3 * t + d];
+        if (P.dim == 2)
+            P.omega[2 * c + k] = pw[3 * t + 2];
+        else
+            for (int d = 0; d < 3; ++d) P.omega[d * c + k] = pw[3 * t + 2];
+        return;
+    }
     if (kind == SCOPI_VAP_FIXED) {
         for (int d = 0; d < 3; ++d) {
             P.v[d * c + k] = P.vd[d * c + k];
@@ -349,9 +361,11 @@ int guarded(scopi_hip_ctx* ctx, F&& f) {
 }
 
 bool params_ok(const scopi_hip_params* p) {
-    return p && (p->problem == SCOPI_PROBLEM_NOFRICTION || p->problem == SCOPI_PROBLEM_FRICTION) &&
+    return p &&
+           (p->problem == SCOPI_PROBLEM_NOFRICTION || p->problem == SCOPI_PROBLEM_FRICTION ||
+            p->problem == SCOPI_PROBLEM_FRICTION_FIXED_POINT) &&
            (p->contact_method == SCOPI_CONTACT_KDTREE || p->contact_method == SCOPI_CONTACT_BRUTE_FORCE) &&
-           (p->vap == SCOPI_VAP_FIXED || p->vap == SCOPI_VAP_FPD) &&
+           (p->vap == SCOPI_VAP_FIXED || p->vap == SCOPI_VAP_FPD || p->vap == SCOPI_VAP_PROJECTION) &&
            (p->method == SCOPI_METHOD_APGD || p->method == SCOPI_METHOD_PGD) &&
            (p->precision == SCOPI_PRECISION_FP64 || p->precision == SCOPI_PRECISION_FP32_REFINE) &&
            p->warm_start == 0;
@@ -475,6 +489,8 @@ int scopi_hip_default_params(scopi_hip_params* p) {
     p->fp32_tolerance = 0.0;
     p->warm_start = 0;
     p->general_jacobian = 0;
+    p->fixed_point_tol = 1e-6;
+    p->fixed_point_max_iter = 1000;
     return SCOPI_HIP_OK;
 }
 
@@ -505,6 +521,7 @@ int scopi_hip_destroy(scopi_hip_ctx* ctx) {
     particles_free(ctx->P);
     contacts_free(ctx->C);
     contacts_free(ctx->C2);
+    if (ctx->proj) (void)hipFree(ctx->proj);
     for (void* p : {(void*)ctx->pflags, (void*)ctx->poffs, (void*)ctx->porig, (void*)ctx->ptotal, (void*)ctx->pkeys,
                     (void*)ctx->pvals, (void*)ctx->pi_of})
         if (p) (void)hipFree(p);
@@ -796,11 +813,32 @@ int scopi_hip_upload_contacts(scopi_hip_ctx* ctx, int64_t nc, const int64_t* i, 
     });
 }
 
+int scopi_hip_set_projection_velocity(scopi_hip_ctx* ctx, const double* u, const double* w) {
+    return guarded(ctx, [&]() -> int {
+        if (!u || !w) return fail(ctx, SCOPI_HIP_ERR_ARG, "set_projection_velocity: bad argument");
+        const int64_t na = ctx->P.n - ctx->P.n_inactive;
+        if (ctx->proj) SCOPI_HIP_CHECK(hipFree(ctx->proj));
+        ctx->proj = nullptr;
+        SCOPI_HIP_CHECK(hipMalloc(&ctx->proj, 6 * std::max<int64_t>(na, 1) * sizeof(double)));
+        if (na > 0) {
+            SCOPI_HIP_CHECK(hipMemcpyAsync(ctx->proj, u, 3 * na * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
+            SCOPI_HIP_CHECK(
+                hipMemcpyAsync(ctx->proj + 3 * na, w, 3 * na * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
+        }
+        SCOPI_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+        ctx->proj_n = na;
+        return SCOPI_HIP_OK;
+    });
+}
+
 int scopi_hip_set_a_priori_velocity(scopi_hip_ctx* ctx, const scopi_hip_params* params, double dt) {
     return guarded(ctx, [&]() -> int {
         if (!params_ok(params)) return fail(ctx, SCOPI_HIP_ERR_ARG, "set_a_priori_velocity: bad params");
+        if (params->vap == SCOPI_VAP_PROJECTION && ctx->proj_n != ctx->P.n - ctx->P.n_inactive)
+            return fail(ctx, SCOPI_HIP_ERR_STATE, "set_a_priori_velocity: no projection velocities for these particles");
         const int64_t na = ctx->P.n - ctx->P.n_inactive;
-        if (na > 0) hipLaunchKernelGGL(k_vap, dim3(grid_for(na)), dim3(kBlock), 0, ctx->stream, ctx->P, params->vap, dt);
+        if (na > 0) hipLaunchKernelGGL(k_vap, dim3(grid_for(na)), dim3(kBlock), 0, ctx->stream, ctx->P, params->vap, dt,
+                                      (const double*)ctx->proj, (const double*)(ctx->proj ? ctx->proj + 3 * na : nullptr));
         SCOPI_HIP_CHECK(hipGetLastError());
         return SCOPI_HIP_OK;
     });
@@ -864,6 +902,8 @@ int scopi_hip_run(scopi_hip_ctx* ctx, const scopi_hip_params* params, double dt,
                   scopi_hip_solve_info* info) {
     return guarded(ctx, [&]() -> int {
         if (!params_ok(params)) return fail(ctx, SCOPI_HIP_ERR_ARG, "run: bad params");
+        if (params->vap == SCOPI_VAP_PROJECTION && ctx->proj_n != ctx->P.n - ctx->P.n_inactive)
+            return fail(ctx, SCOPI_HIP_ERR_STATE, "run: no projection velocities for these particles");
         scopi_hip_solve_info acc{};
         acc.converged = 1;
         hipStream_t s = ctx->stream;
@@ -874,7 +914,9 @@ int scopi_hip_run(scopi_hip_ctx* ctx, const scopi_hip_params* params, double dt,
                 hipLaunchKernelGGL(k_obstacles, dim3(grid_for(P.n_inactive)), dim3(kBlock), 0, s, P, dt);
             ctx->nc = contacts_step(ctx, *params);
             ctx->contacts_valid = true;
-            if (na > 0) hipLaunchKernelGGL(k_vap, dim3(grid_for(na)), dim3(kBlock), 0, s, P, params->vap, dt);
+            if (na > 0)
+                hipLaunchKernelGGL(k_vap, dim3(grid_for(na)), dim3(kBlock), 0, s, P, params->vap, dt,
+                                   (const double*)ctx->proj, (const double*)(ctx->proj ? ctx->proj + 3 * na : nullptr));
             scopi_hip_solve_info one{};
             int rc = ctx->solver.solve(P, ctx->C, ctx->nc, *params, dt, ctx->all_spheres && !ctx->any_periodic(), s, &one);
             if (rc != SCOPI_HIP_OK) return fail(ctx, rc, "run: solve failed");
@@ -892,6 +934,8 @@ int scopi_hip_run(scopi_hip_ctx* ctx, const scopi_hip_params* params, double dt,
             acc.converged = acc.converged && one.converged;
             if (one.status != 0) acc.status = one.status;
             acc.residual = one.residual;
+            acc.outer_iterations += one.outer_iterations;
+            acc.fixed_point_error = one.fixed_point_error;
         }
         SCOPI_HIP_CHECK(hipStreamSynchronize(s));
         if (info) *info = acc;
@@ -1139,7 +1183,8 @@ scopi_hip_batch::Scene* batch_find(scopi_hip_batch* b, int64_t id) {
 extern "C" {
 
 int scopi_hip_batch_create(const scopi_hip_params* params, double dt, scopi_hip_batch** out) {
-    if (!out || !params_ok(params) || !(dt > 0)) return SCOPI_HIP_ERR_ARG;
+    if (!out || !params_ok(params) || !(dt > 0) || params->vap == SCOPI_VAP_PROJECTION) return SCOPI_HIP_ERR_ARG;
+    if (params->problem == SCOPI_PROBLEM_FRICTION_FIXED_POINT) return SCOPI_HIP_ERR_UNSUPPORTED;
     *out = nullptr;
     auto* b = new scopi_hip_batch();
     b->prm = *params;
@@ -1257,7 +1302,9 @@ int scopi_hip_batch_step(scopi_hip_batch* b, int64_t n_steps) {
             if (P.n_inactive > 0)
                 hipLaunchKernelGGL(k_obstacles, dim3(grid_for(P.n_inactive)), dim3(kBlock), 0, s, P, b->dt);
             b->nc = b->search.run(P, b->C, b->prm, s, !b->has_se);
-            if (na > 0) hipLaunchKernelGGL(k_vap, dim3(grid_for(na)), dim3(kBlock), 0, s, P, b->prm.vap, b->dt);
+            if (na > 0)
+                hipLaunchKernelGGL(k_vap, dim3(grid_for(na)), dim3(kBlock), 0, s, P, b->prm.vap, b->dt,
+                                   (const double*)nullptr, (const double*)nullptr);
             S.d_crange = b->search.d_crange();
             int rc = b->solver.solve_scenes(P, b->C, b->nc, b->prm, b->dt, b->all_spheres, s, S, infos);
             if (rc != SCOPI_HIP_OK) return bfail(b, rc, "batch_step: solve failed");

@@ -14,9 +14,9 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libscopi_hip.so")
 
 SHAPE_SPHERE, SHAPE_PLANE, SHAPE_SEGMENT, SHAPE_SUPERELLIPSOID = 0, 1, 2, 3
-NOFRICTION, FRICTION = 0, 1
+NOFRICTION, FRICTION, FRICTION_FIXED_POINT = 0, 1, 2
 KDTREE, BRUTE_FORCE = 0, 1
-VAP_FIXED, VAP_FPD = 0, 1
+VAP_FIXED, VAP_FPD, VAP_PROJECTION = 0, 1, 2
 APGD, PGD = 0, 1
 FP64, FP32_REFINE = 0, 1
 
@@ -32,7 +32,7 @@ EXPORTS = [
     "scopi_hip_batch_create", "scopi_hip_batch_destroy", "scopi_hip_batch_last_error", "scopi_hip_batch_add_scene",
     "scopi_hip_batch_remove_scene", "scopi_hip_batch_step", "scopi_hip_batch_size", "scopi_hip_batch_download_scene",
     "scopi_hip_batch_scene_info", "scopi_hip_batch_download_contacts", "scopi_hip_batch_download_solution",
-    "scopi_hip_scene_footprint", "scopi_hip_set_box",
+    "scopi_hip_scene_footprint", "scopi_hip_set_box", "scopi_hip_set_projection_velocity",
 ]
 TUNE_REDUCE_BLOCKS, TUNE_BT_GROUP, TUNE_GRAPH_ROUNDS = 1, 2, 3
 
@@ -43,7 +43,7 @@ class Params(C.Structure):
         ("precision", C.c_int32), ("dynamic_descent", C.c_int32), ("dmax", C.c_double),
         ("kd_tree_radius", C.c_double), ("alpha", C.c_double), ("tolerance", C.c_double), ("mu", C.c_double),
         ("max_ite", C.c_int64), ("fp32_tolerance", C.c_double), ("warm_start", C.c_int32),
-        ("general_jacobian", C.c_int32),
+        ("general_jacobian", C.c_int32), ("fixed_point_tol", C.c_double), ("fixed_point_max_iter", C.c_int64),
     ]
 
 
@@ -51,7 +51,7 @@ class SolveInfo(C.Structure):
     _fields_ = [
         ("n_contacts", C.c_int64), ("rows", C.c_int64), ("iterations", C.c_int64), ("iterations_fp32", C.c_int64),
         ("backtracks", C.c_int64), ("q_applies", C.c_int64), ("converged", C.c_int32), ("status", C.c_int32),
-        ("residual", C.c_double),
+        ("residual", C.c_double), ("outer_iterations", C.c_int64), ("fixed_point_error", C.c_double),
     ]
 
     def as_dict(self):
@@ -104,6 +104,7 @@ def lib():
     L.scopi_hip_closest_points.argtypes = ([P, C.c_int64, I32, D, D, D, C.c_void_p, I32, D, D, D, C.c_void_p] +
                                            [D] * 4 + [C.c_void_p] * 4)
     L.scopi_hip_set_box.argtypes = [P, D, D, I32]
+    L.scopi_hip_set_projection_velocity.argtypes = [P, D, D]
     L.scopi_hip_batch_create.argtypes = [C.POINTER(Params), C.c_double, C.POINTER(P)]
     L.scopi_hip_batch_destroy.argtypes = [P]
     L.scopi_hip_batch_last_error.argtypes = [P]
@@ -245,6 +246,13 @@ class HipScene:
             self.nc = info.n_contacts  # contacts of the last step stay on the device
             self.last_rows = info.rows
         return info.as_dict()
+
+    def set_projection_velocity(self, u, w):
+        """vap_projection::set_u_w: u, w are Na x 3."""
+        na = self.n - self.n_inactive
+        u = np.ascontiguousarray(np.reshape(u, (max(na, 1) if na else 1, 3)) if na else np.zeros((1, 3)), np.float64)
+        w = np.ascontiguousarray(np.reshape(w, (max(na, 1) if na else 1, 3)) if na else np.zeros((1, 3)), np.float64)
+        self._check(self._L.scopi_hip_set_projection_velocity(self._h, u, w), "set_projection_velocity")
 
     def set_box(self, lo, hi, periodic):
         """BoxDomain(lo, hi).with_periodicity(axes with periodic[d] true) of the scene."""

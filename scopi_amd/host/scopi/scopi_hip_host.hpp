@@ -12,7 +12,8 @@
 //   OptimGradient<apgd | pgd> (apgd_params: alpha, max_ite, tolerance, dynamic_descent)
 //                                                                          include/scopi/solvers/apgd.hpp:20-124
 //   vap_fixed / vap_fpd                                                    include/scopi/vap/*.hpp
-//   NoFriction / Friction, contact_property<Friction>::mu                  include/scopi/contact/property.hpp
+//   NoFriction / Friction / FrictionFixedPoint, contact_property<...>       include/scopi/contact/property.hpp
+//     (mu; fixed_point_tol, fixed_point_max_iter)
 //   neighbor<dim, problem_t>                                               include/scopi/objects/neighbor.hpp:23-70
 // Every slot runs on the GPU through the C ABI (include/scopi_hip.h).  When the
 // contact, optim and vap slots are the ones defined here, ScopiSolver::run keeps
@@ -81,6 +82,7 @@ inline type::quaternion_t quaternion(double angle = 0.0) { return quaternion(ang
 // ------------------------------------------------------------------ problem tags
 struct NoFriction {};
 struct Friction {};
+struct FrictionFixedPoint {};
 
 template <class problem_t>
 struct contact_property {};
@@ -88,6 +90,20 @@ template <>
 struct contact_property<Friction> {
     double mu = 0.1;
 };
+// contact_property<FrictionFixedPoint> (contact/property.hpp:75-149): Friction's mu and
+// FixedPoint's outer-loop controls (fixed_point_max_iter is a double there too)
+template <>
+struct contact_property<FrictionFixedPoint> {
+    double mu = 0.1;
+    double fixed_point_tol = 1e-6;
+    double fixed_point_max_iter = 1000;
+};
+
+// problems whose contacts carry a friction coefficient
+template <class problem_t>
+constexpr bool has_mu() {
+    return std::is_same<problem_t, Friction>::value || std::is_same<problem_t, FrictionFixedPoint>::value;
+}
 
 template <std::size_t dim, class problem_t_>
 struct neighbor {
@@ -357,6 +373,21 @@ inline output::value property_json(const contact_property<Friction>& c) {  // co
     p["mu"] = c.mu;
     return p;
 }
+// The reference stops the outer loop when its count (1, 2, ...) reaches the double
+// fixed_point_max_iter; the device takes the same bound as the smallest such count.
+inline void set_fixed_point(scopi_hip_params& prm, const contact_property<FrictionFixedPoint>& c) {
+    prm.fixed_point_tol = c.fixed_point_tol;
+    const double m = std::ceil(c.fixed_point_max_iter);
+    prm.fixed_point_max_iter = !(m > 1.0) ? 1 : m > 1e15 ? (int64_t)1e15 : (int64_t)m;
+}
+inline output::value property_json(const contact_property<FrictionFixedPoint>& c) {  // contact/property.hpp:140-148
+    output::value p;
+    p["type"] = "friction fixed point";
+    p["mu"] = c.mu;
+    p["fixed_point_tol"] = c.fixed_point_tol;
+    p["fixed_point_max_iter"] = c.fixed_point_max_iter;
+    return p;
+}
 }  // namespace detail
 
 // The document ScopiSolver::write_output_files writes (include/scopi/solver.hpp:369-421):
@@ -607,7 +638,9 @@ struct pgd {
 
 template <class problem_t>
 constexpr int problem_tag() {
-    return std::is_same<problem_t, Friction>::value ? SCOPI_PROBLEM_FRICTION : SCOPI_PROBLEM_NOFRICTION;
+    return std::is_same<problem_t, Friction>::value             ? SCOPI_PROBLEM_FRICTION
+           : std::is_same<problem_t, FrictionFixedPoint>::value ? SCOPI_PROBLEM_FRICTION_FIXED_POINT
+                                                                 : SCOPI_PROBLEM_NOFRICTION;
 }
 
 template <class Method>
@@ -656,17 +689,20 @@ class OptimGradient {
                 pi[3 * c + d] = contacts[c].pi[d];
                 pj[3 * c + d] = contacts[c].pj[d];
             }
-            if constexpr (std::is_same<problem_t, Friction>::value) mu[c] = contacts[c].property.mu;
+            if constexpr (has_mu<problem_t>()) mu[c] = contacts[c].property.mu;
         }
         detail::check(scopi_hip_upload_contacts(m_dev.ctx, nc, i.data(), j.data(), dij.data(), nij.data(), pi.data(),
                                                 pj.data(), mu.data()),
                       m_dev.ctx, "scopi_hip_upload_contacts");
         auto prm = hip_params(problem_tag<problem_t>(), 0.1);
+        // the outer loop reads its controls from the first contact (OptimGradient.hpp:145-146)
+        if constexpr (std::is_same<problem_t, FrictionFixedPoint>::value)
+            if (nc > 0) detail::set_fixed_point(prm, contacts[0].property);
         detail::check(scopi_hip_solve(m_dev.ctx, &prm, m_dt, &m_info), m_dev.ctx, "scopi_hip_solve");
         const std::size_t na = particles.nb_active();
         m_u.assign(3 * na, 0.0);
         m_omega.assign(3 * na, 0.0);
-        m_lambda.assign(problem_tag<problem_t>() == SCOPI_PROBLEM_FRICTION ? 3 * nc : nc, 0.0);
+        m_lambda.assign(has_mu<problem_t>() ? 3 * nc : nc, 0.0);
         detail::check(scopi_hip_download_solution(m_dev.ctx, m_u.data(), m_omega.data(), m_lambda.data()), m_dev.ctx,
                       "scopi_hip_download_solution");
     }
@@ -719,6 +755,43 @@ struct vap_fpd {
             }
         }
     }
+};
+
+// vap_projection (vap/vap_projection.hpp:34-49, src/vap/vap_projection.cpp:6-10): the desired
+// velocity of every active particle takes u(i, .), its rotation velocity w(i, 2) (in 3D the scalar
+// goes to all three components, as the reference assigns a scalar to a 3-vector).  u, w: Na x 3,
+// given by set_u_w; ScopiSolver::vap() reaches the slot object.
+struct vap_projection {
+    static constexpr int kind = SCOPI_VAP_PROJECTION;
+    struct params_t {};
+    params_t& get_params() { return m_params; }
+    params_t m_params;
+    template <class U, class W>
+    void set_u_w(const U& u, const W& w) {  // u(i, d), w(i, d) indexable, Na rows
+        m_u.clear();
+        m_w.clear();
+        for (std::size_t i = 0; i < u.shape()[0]; ++i)
+            for (std::size_t d = 0; d < 3; ++d) {
+                m_u.push_back(u(i, d));
+                m_w.push_back(w(i, d));
+            }
+    }
+    void set_u_w(const std::vector<double>& u, const std::vector<double>& w) {  // flat Na x 3
+        m_u = u;
+        m_w = w;
+    }
+    template <std::size_t dim, class C>
+    void set_a_priori_velocity(double, scopi_container<dim>& p, const C&) {
+        for (std::size_t i = p.nb_inactive(); i < p.size(); ++i) {
+            const std::size_t a = i - p.nb_inactive();
+            for (std::size_t d = 0; d < dim; ++d) p.vd()[i][d] = m_u[3 * a + d];
+            if constexpr (dim == 2)
+                p.omega()[i] = m_w[3 * a + 2];
+            else
+                for (int d = 0; d < 3; ++d) p.omega()[i][d] = m_w[3 * a + 2];
+        }
+    }
+    std::vector<double> m_u, m_w;
 };
 
 // ------------------------------------------------------------------ driver
@@ -797,6 +870,7 @@ class ScopiSolver {
 
     const scopi_hip_solve_info& last_info() const { return m_info; }
     const contact_container_t& last_contacts() const { return m_old_contacts; }
+    vap_t& vap() { return m_vap; }
 
   private:
     // Every slot is a GPU slot of this library: the whole loop stays on the device.
@@ -818,16 +892,25 @@ class ScopiSolver {
             per[d] = m_box.is_periodic(d) ? 1 : 0;
         }
         detail::check(scopi_hip_set_box(m_ctx, lo, hi, per), m_ctx, "scopi_hip_set_box");
+        if constexpr (std::is_same<vap_t, vap_projection>::value) {
+            const std::size_t na = m_particles.nb_active();
+            if (m_vap.m_u.size() != 3 * na || m_vap.m_w.size() != 3 * na)
+                throw std::runtime_error("vap_projection: set_u_w needs Na x 3 velocities");
+            detail::check(scopi_hip_set_projection_velocity(m_ctx, m_vap.m_u.data(), m_vap.m_w.data()), m_ctx,
+                          "scopi_hip_set_projection_velocity");
+        }
         m_info = scopi_hip_solve_info{};
         m_info.converged = 1;
     }
     double mu() {
         double mu = 0.1;
-        if constexpr (std::is_same<problem_t, Friction>::value) mu = m_contact.default_contact_property().mu;
+        if constexpr (has_mu<problem_t>()) mu = m_contact.default_contact_property().mu;
         return mu;
     }
     scopi_hip_params hip_params() {
         auto prm = m_optim.hip_params(problem_tag<problem_t>(), mu());
+        if constexpr (std::is_same<problem_t, FrictionFixedPoint>::value)
+            detail::set_fixed_point(prm, m_contact.default_contact_property());
         prm.vap = vap_t::kind;
         prm.contact_method = contact_method_t::method;
         prm.dmax = m_contact.get_params().dmax;
@@ -840,6 +923,8 @@ class ScopiSolver {
         m_info.iterations += one.iterations;
         m_info.iterations_fp32 += one.iterations_fp32;
         m_info.backtracks += one.backtracks;
+        m_info.outer_iterations += one.outer_iterations;
+        m_info.fixed_point_error = one.fixed_point_error;
         m_info.q_applies += one.q_applies;
         m_info.converged = m_info.converged && one.converged;
         if (one.status != 0) m_info.status = one.status;
@@ -875,7 +960,7 @@ class ScopiSolver {
                 nb.pi[d] = pi[3 * c + d];
                 nb.pj[d] = pj[3 * c + d];
             }
-            if constexpr (std::is_same<problem_t, Friction>::value) nb.property.mu = mu();
+            if constexpr (has_mu<problem_t>()) nb.property = m_contact.default_contact_property();
         }
         if (lambda) {
             lambda->assign((std::size_t)info.rows, 0.0);

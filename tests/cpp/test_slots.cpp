@@ -136,6 +136,97 @@ void three_spheres() {
     dump("3spheres_nofriction", particles, file);
 }
 
+// test/test_gradient.cpp:505-572: the three spheres under FrictionFixedPoint, mu 0.5
+void three_spheres_friction_fp() {
+    constexpr std::size_t dim = 2;
+    scopi_container<dim> particles;
+    sphere<dim> s1({{-1.7, 1.3}}, 0.5);
+    sphere<dim> s2({{0.5, 1.7}}, 0.5);
+    sphere<dim> s3({{4.5, 1.3}}, 0.5);
+    particles.push_back(s1, property<dim>().mass(1).moment_inertia(1).force({{1.7, -1.3}}));
+    particles.push_back(s2, property<dim>().mass(1).moment_inertia(1).force({{-0.5, -1.7}}));
+    particles.push_back(s3, property<dim>().mass(1).moment_inertia(1).force({{-4.5, -1.3}}));
+    double Tf = 7, dt = 0.1;
+    ScopiSolver<dim, FrictionFixedPoint, OptimGradient<apgd>, contact_kdtree, vap_fpd> solver(particles);
+    auto params = solver.get_params();
+    params.optim_params.tolerance = 1e-7;
+    params.optim_params.max_ite = 10000;
+    params.optim_params.alpha = 0.1;
+    params.optim_params.dynamic_descent = true;
+    params.default_contact_property.mu = 0.5;
+    params.default_contact_property.fixed_point_tol = 1e-6;
+    params.default_contact_property.fixed_point_max_iter = 1000;
+    std::size_t total_it = Tf / dt;
+    const auto file = reference_output(solver, "three_spheres_fp", "3spheres_friction_fp", total_it);
+    solver.run(dt, total_it);
+    dump("3spheres_friction_fp", particles, file);
+}
+
+// test/test_gradient.cpp:215-282 with mu = `mu`: a sphere on a horizontal plane under gravity
+// tilted by pi/4, FrictionFixedPoint on the device time loop
+void sphere_plane_fixed_point(double mu) {
+    constexpr std::size_t dim = 2;
+    scopi_container<dim> particles;
+    const double radius = 1., g = 1., dt = 0.01, h = 2. * radius, alpha = PI / 4;
+    const std::size_t total_it = 500;
+    plane<dim> pl({{0., 0.}}, PI / 2);
+    sphere<dim> s({{0, h}}, radius);
+    particles.push_back(pl, property<dim>().deactivate());
+    particles.push_back(s, property<dim>().mass(1).moment_inertia(1. * radius * radius / 2.).force(
+                               {{g * std::sin(alpha), -g * std::cos(alpha)}}));
+    ScopiSolver<dim, FrictionFixedPoint, OptimGradient<apgd>, contact_kdtree, vap_fpd> solver(particles);
+    auto params = solver.get_params();
+    params.optim_params.tolerance = 1e-7;
+    params.optim_params.max_ite = 10000;
+    params.optim_params.alpha = 0.01;
+    params.optim_params.dynamic_descent = true;
+    params.default_contact_property.mu = mu;
+    params.default_contact_property.fixed_point_tol = 1e-6;
+    params.default_contact_property.fixed_point_max_iter = 1000;
+    params.solver_params.output_frequency = std::size_t(-1);
+    solver.run(dt, total_it);
+    char name[64];
+    std::snprintf(name, sizeof(name), "sphere_plane_fp_mu%g", mu);
+    std::printf("{\"case\": \"%s\", \"outer_iterations\": %lld, \"objects\": [", name,
+                (long long)solver.last_info().outer_iterations);
+    std::printf("{\"position\": [%.17g, %.17g], \"quaternion\": [%.17g, %.17g, %.17g, %.17g], "
+                "\"velocity\": [%.17g, %.17g], \"omega\": %.17g}]}\n",
+                particles.pos()[1][0], particles.pos()[1][1], particles.q()[1][0], particles.q()[1][1],
+                particles.q()[1][2], particles.q()[1][3], particles.v()[1][0], particles.v()[1][1], particles.omega()[1]);
+    std::fflush(stdout);
+}
+
+// The contact slot and the optim slot called one by one with FrictionFixedPoint: a unit sphere
+// (m 1, J 1/2) touching a horizontal plane with velocity (1, -1); the contact's own mu (0.2) and
+// fixed-point controls drive the solve.  Coulomb's answer for one step: the normal impulse
+// cancels the approach, the friction impulse is mu times it (the contact point still slides),
+// so u = (1 - mu, 0), omega = -mu r / J = -0.4.
+void fixed_point_slot() {
+    constexpr std::size_t dim = 2;
+    scopi_container<dim> particles;
+    particles.push_back(plane<dim>({{0., 0.}}, PI / 2), property<dim>().deactivate());
+    particles.push_back(sphere<dim>({{0., 1.}}, 1.), property<dim>().mass(1).moment_inertia(0.5).velocity({{1., -1.}}));
+    ContactsParams<contact_kdtree<FrictionFixedPoint>> cp;
+    contact_kdtree<FrictionFixedPoint> cont(cp);
+    auto contacts = cont.run(particles);
+    for (auto& c : contacts) {
+        c.property.mu = 0.2;
+        c.property.fixed_point_tol = 1e-10;
+        c.property.fixed_point_max_iter = 1000;
+    }
+    OptimGradient<apgd> optim;
+    optim.set_timestep(0.01);
+    optim.get_params().tolerance = 1e-11;
+    optim.get_params().max_ite = 100000;
+    optim.run(particles, contacts, 0);
+    const auto& u = optim.get_uadapt();
+    const auto& w = optim.get_wadapt();
+    std::printf("{\"case\": \"fixed_point_slot\", \"n\": %zu, \"u\": [%.17g, %.17g], \"omega\": %.17g, "
+                "\"outer_iterations\": %lld}\n",
+                contacts.size(), u[0], u[1], w[2], (long long)optim.last_info().outer_iterations);
+    std::fflush(stdout);
+}
+
 void sphere_plane_friction() {
     constexpr std::size_t dim = 2;
     scopi_container<dim> particles;
@@ -249,6 +340,12 @@ int main(int argc, char** argv) {
         if (only.empty() || only == "moving") sphere_sphere_moving();
         if (only.empty() || only == "three") three_spheres();
         if (only.empty() || only == "friction") sphere_plane_friction();
+        if (only.empty() || only == "fixed_point") {
+            three_spheres_friction_fp();
+            sphere_plane_fixed_point(0.5);
+            sphere_plane_fixed_point(0.1);
+            fixed_point_slot();
+        }
         if (only.empty() || only == "slots") slots_one_by_one();
         if (only.empty() || only == "superellipsoids") superellipsoids_3d();
         if (only.empty() || only == "periodic") periodic_pair();

@@ -66,6 +66,16 @@ def three_spheres_nofriction():
                 fixture="3spheres_nofriction")
 
 
+def three_spheres_friction_fp():
+    """test/test_gradient.cpp:505-572: the 3 spheres of 3spheres_nofriction under FrictionFixedPoint
+    (mu 0.5, fixed_point_tol 1e-6, fixed_point_max_iter 1000), apgd tol 1e-7, alpha 0.1, 70 steps."""
+    case = three_spheres_nofriction()
+    case.update(total_it=70, fixture="3spheres_friction_fp",
+                params=dict(tolerance=1e-7, max_ite=10000, alpha=0.1, dynamic_descent=1, problem=2, mu=0.5,
+                            fixed_point_tol=1e-6, fixed_point_max_iter=1000))
+    return case
+
+
 def sphere_plane_friction_mu01():
     """test/test_gradient.cpp:156-213: Friction, plane at pi/2-pi/6, mu 0.1, dt 0.1, 200 steps."""
     c = Container(2)
@@ -149,3 +159,59 @@ def compare_objects(fixture, dim, state, n, tol):
         mine = state["omega"][k, 2:3] if dim == 2 else state["omega"][k]
         worst = max(worst, float(np.max(np.abs(om - mine))))
     return worst
+
+
+def sphere_plane_tilted_gravity(mu, problem, total_it=500, dt=0.01):
+    """test/test_gradient.cpp:215-282: a horizontal plane (angle pi/2) and a sphere of radius 1 at
+    height 2 under the force (g sin a, -g cos a), a = pi/4, g = 1; apgd tol 1e-7, alpha 0.01,
+    fixed_point_tol 1e-6, fixed_point_max_iter 1000, vap_fpd."""
+    radius, g, h, alpha = 1.0, 1.0, 2.0, PI / 4
+    c = Container(2)
+    c.plane((0.0, 0.0), PI / 2)
+    c.sphere((0.0, h), radius, Property(mass=1.0, moment_inertia=radius * radius / 2.0,
+                                        force=(g * math.sin(alpha), -g * math.cos(alpha))))
+    return dict(scene=c.arrays(), dt=dt, total_it=total_it, vap=FPD, alpha=alpha, radius=radius, g=g, h=h, mu=mu,
+                params=dict(tolerance=1e-7, max_ite=10000, alpha=0.01, dynamic_descent=1, problem=problem, mu=mu,
+                            fixed_point_tol=1e-6, fixed_point_max_iter=1000))
+
+
+def analytical_sphere_plane_friction(alpha, mu, t, r, g, y0):
+    """test/analytical_solution.cpp:130-191: position, rotation angle, velocity and rotation velocity
+    of a sphere dropped on a horizontal plane under tilted gravity, rolling without slipping when
+    tan(alpha) <= 3 mu, else sliding with Coulomb friction."""
+    t_impact = math.sqrt(2 * (y0 - r) / (g * math.cos(alpha)))
+    if t <= t_impact:
+        return (np.array([g * t * t * math.sin(alpha) / 2, -g * t * t * math.cos(alpha) / 2 + y0]), 0.0,
+                np.array([g * t * math.sin(alpha), -g * t * math.cos(alpha)]), 0.0)
+    v_t_m = g * t_impact * math.sin(alpha)
+    v_n_m = -g * t_impact * math.cos(alpha)
+    t2 = t - t_impact
+    xt_impact = g * t_impact * t_impact * math.sin(alpha) / 2
+    if math.tan(alpha) <= 3 * mu:
+        x_t = g * math.sin(alpha) * t2 * t2 / 3.0 + 2.0 * v_t_m * t2 / 3.0 + xt_impact
+        theta = -g * math.sin(alpha) * t2 * t2 / (3.0 * r) - 2 * v_t_m * t2 / (3.0 * r)
+        v_t = 2.0 * g * math.sin(alpha) * t2 / 3.0 + 2.0 * v_t_m / 3.0
+        omega = -2.0 * g * math.sin(alpha) * t2 / (3.0 * r) - 2 * v_t_m / (3.0 * r)
+    else:
+        x_t = g * (math.sin(alpha) - mu * math.cos(alpha)) * t2 * t2 / 2.0 + (v_t_m + mu * v_n_m) * t2 + xt_impact
+        theta = -mu * g * math.cos(alpha) * t2 * t2 / r + 2 * mu * v_n_m * t2 / r
+        v_t = g * (math.sin(alpha) - mu * math.cos(alpha)) * t2 + (v_t_m + mu * v_n_m)
+        omega = -2.0 * mu * g * math.cos(alpha) * t2 / r + 2 * mu * v_n_m / r
+    return np.array([x_t, r]), theta, np.array([v_t, 0.0]), omega
+
+
+def sphere_plane_errors(case, state):
+    """The four errors of test/test_gradient.cpp:264-277 (position, quaternion, velocity: relative
+    norms; omega: std::norm of a double is its square, so (dw)^2 / w^2)."""
+    t = case["dt"] * case["total_it"]
+    pa, th, va, wa = analytical_sphere_plane_friction(case["alpha"], case["mu"], t, case["radius"], case["g"], case["h"])
+    qa = np.array([math.cos(th / 2), 0.0, 0.0, math.sin(th / 2)])
+    return dict(pos=np.linalg.norm(state["pos"][1, :2] - pa) / np.linalg.norm(pa),
+                q=np.linalg.norm(state["q"][1] - qa) / np.linalg.norm(qa),
+                v=np.linalg.norm(state["v"][1, :2] - va) / np.linalg.norm(va),
+                omega=(state["omega"][1, 2] - wa) ** 2 / wa ** 2)
+
+
+def doctest_le(x, bound, eps=float(np.finfo(np.float32).eps) * 100):
+    """doctest's `x <= Approx(bound)` (epsilon 100 float eps, scale 1)."""
+    return x < bound or abs(x - bound) < eps * (1.0 + max(abs(x), abs(bound)))

@@ -20,7 +20,8 @@ pytestmark = pytest.mark.gpu
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 BIN = os.path.join(ROOT, "tests", "cpp", "test_slots")
 TOL = {"two_spheres_asymmetrical": 1e-6, "two_spheres_symmetrical": 1e-6, "2d_case_spheres": 1e-6,
-       "obstacles_sphere_sphere_moving": 1e-6, "3spheres_nofriction": 1e-5, "sphere_plane_friction_mu01": 1e-5}
+       "obstacles_sphere_sphere_moving": 1e-6, "3spheres_nofriction": 1e-5, "sphere_plane_friction_mu01": 1e-5,
+       "3spheres_friction_fp": 1e-5}
 
 
 @pytest.fixture(scope="module")
@@ -137,3 +138,28 @@ def test_periodic_box_through_slot_api(output, oracle_lib):
         assert np.abs(d).max() <= 1e-9, (k, ob["position"], so["pos"][k])
     # the spheres met across the bound: the contact pushed them apart in y
     assert abs(got[0]["position"][1] - 0.5) > 1e-3 and abs(got[1]["position"][1] - 0.503) > 1e-3
+
+
+@pytest.mark.parametrize("mu", [0.5, 0.1])
+def test_fixed_point_tilted_gravity_through_solver(output, mu):
+    """ScopiSolver<2, FrictionFixedPoint, ...> on test_gradient.cpp:215-282's scene: rolling (mu 0.5,
+    the reference's bounds) and sliding (mu 0.1, Coulomb's closed form to 1e-8)."""
+    d = output[f"sphere_plane_fp_mu{mu:g}"]
+    o = d["objects"][0]
+    st = dict(pos=np.zeros((2, 3)), q=np.zeros((2, 4)), v=np.zeros((2, 3)), omega=np.zeros((2, 3)))
+    st["pos"][1, :2], st["q"][1], st["v"][1, :2], st["omega"][1, 2] = o["position"], o["quaternion"], o["velocity"], o["omega"]
+    case = fs.sphere_plane_tilted_gravity(mu, 2)
+    e = fs.sphere_plane_errors(case, st)
+    assert fs.doctest_le(e["pos"], 1e-2) and fs.doctest_le(e["q"], 1e-2)
+    if mu == 0.5:
+        assert fs.doctest_le(e["v"], 1e-12) and fs.doctest_le(e["omega"], 1e-12)
+    else:
+        assert e["v"] < 1e-8 and abs(o["position"][1] - 1.0) < 1e-8
+    assert d["outer_iterations"] > 300  # one or more outer passes per step in contact (from step 169 on)
+
+
+def test_fixed_point_optim_slot_closed_form(output):
+    d = output["fixed_point_slot"]
+    assert d["n"] == 1 and d["outer_iterations"] > 1
+    np.testing.assert_allclose(d["u"], [0.8, 0.0], atol=1e-7)
+    assert abs(d["omega"] + 0.4) < 1e-7

@@ -203,6 +203,7 @@ FIXTURES = {
     "obstacles_sphere_sphere_moving": (lambda o: fs.obstacles_sphere_sphere_moving(), 1e-6),
     "3spheres_nofriction": (lambda o: fs.three_spheres_nofriction(), 1e-5),
     "sphere_plane_friction_mu01": (lambda o: fs.sphere_plane_friction_mu01(), 1e-5),
+    "3spheres_friction_fp": (lambda o: fs.three_spheres_friction_fp(), 1e-5),
 }
 
 
