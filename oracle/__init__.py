@@ -15,7 +15,7 @@ _HERE = os.path.dirname(os.path.abspath(__file__))
 _LIB_PATH = os.path.join(_HERE, "liboracle.so")
 _lib = None
 
-NOFRICTION, FRICTION, FRICTION_FIXED_POINT = 0, 1, 2
+NOFRICTION, FRICTION, FRICTION_FIXED_POINT, VISCOUS, VISCOUS_FRICTION = 0, 1, 2, 3, 4
 KDTREE, BRUTE_FORCE = 0, 1
 VAP_FIXED, VAP_FPD, VAP_PROJECTION = 0, 1, 2
 APGD, PGD = 0, 1
@@ -27,6 +27,7 @@ class Params(C.Structure):
         ("dmax", C.c_double), ("kd_tree_radius", C.c_double), ("alpha", C.c_double), ("max_ite", C.c_int64),
         ("tolerance", C.c_double), ("dynamic_descent", C.c_int32), ("pad", C.c_int32), ("mu", C.c_double),
         ("fixed_point_tol", C.c_double), ("fixed_point_max_iter", C.c_double),
+        ("gamma", C.c_double), ("gamma_min", C.c_double), ("gamma_tol", C.c_double),
     ]
 
 
@@ -70,6 +71,11 @@ def lib():
         L.oracle_atmat.argtypes = [P, D, D]
         L.oracle_solve.argtypes = [P, C.c_double, D, D, D]
         L.oracle_set_a_priori_velocity.argtypes = [P, C.c_double]
+        L.oracle_set_forces.argtypes = [P, D]
+        L.oracle_lambda_size.argtypes = [P]
+        L.oracle_lambda_size.restype = C.c_int64
+        L.oracle_get_contact_gamma.argtypes = [P, D]
+        L.oracle_set_contact_gamma.argtypes = [P, D]
         L.oracle_get_stats.argtypes = [P, C.POINTER(Stats)]
         L.oracle_closest_points.argtypes = [C.c_int32, C.c_int32, D, D, D, D, C.c_int32, D, D, D, D, D, D, D, D]
         L.oracle_se_closest_points.argtypes = [D] * 8 + [D, D, D, D, D, D, I32, I32, I32]
@@ -171,11 +177,25 @@ class Oracle:
     def solve(self, dt):
         nc = int(self._L.oracle_n_contacts(self._h))
         na = self.n - self.n_inactive
-        lam = np.zeros(max(1, nc * (1 if self.problem == NOFRICTION else 3)))
+        lam = np.zeros(max(1, 4 * nc))  # the largest layout: four rows per contact
         u = np.zeros(max(1, 3 * na)); w = np.zeros(max(1, 3 * na))
         self._L.oracle_solve(self._h, float(dt), lam, u, w)
-        return dict(lam=lam[: nc * (1 if self.problem == NOFRICTION else 3)], u=u[: 3 * na].reshape(na, 3),
-                    w=w[: 3 * na].reshape(na, 3))
+        nl = int(self._L.oracle_lambda_size(self._h))
+        return dict(lam=lam[:nl], u=u[: 3 * na].reshape(na, 3), w=w[: 3 * na].reshape(na, 3))
+
+    def set_forces(self, f):
+        """Replace the external forces (n x 3); the last step's contacts and properties stay."""
+        assert self._L.oracle_set_forces(self._h, np.ascontiguousarray(np.reshape(f, (self.n, 3)), np.float64)) == 0
+
+    def contact_gamma(self):
+        g = np.zeros(max(1, int(self._L.oracle_n_contacts(self._h))))
+        assert self._L.oracle_get_contact_gamma(self._h, g) == 0
+        return g[: int(self._L.oracle_n_contacts(self._h))]
+
+    def set_contact_gamma(self, gamma):
+        nc = int(self._L.oracle_n_contacts(self._h))
+        g = np.ascontiguousarray(np.broadcast_to(np.asarray(gamma, np.float64), (nc,)))
+        assert self._L.oracle_set_contact_gamma(self._h, g if nc else np.zeros(1)) == 0
 
     def stats(self):
         s = Stats()

@@ -30,6 +30,7 @@
 #include <cmath>
 #include <cstdint>
 #include <cstring>
+#include <map>
 #include <random>
 #include <vector>
 
@@ -43,7 +44,10 @@ struct Contact {
     int64_t i, j;
     double dij;
     vec3 nij, pi, pj;
+    // contact_property<problem> (contact/property.hpp): mu, and for the Viscous problems the
+    // adhesion potential gamma with its threshold gamma_min and tolerance gamma_tol
     double mu;
+    double gamma, gamma_min, gamma_tol;
 };
 
 struct Box {  // BoxDomain (include/scopi/box.hpp)
@@ -64,6 +68,7 @@ struct Scene {
     std::vector<double> m;
     oracle_params prm{};
     std::vector<Contact> contacts;
+    std::vector<Contact> old_contacts;  // ScopiSolver::m_old_contacts: the last step's contacts and properties
     std::vector<double> lambda;
     // optim slot outputs (OptimGradient::m_u / m_omega), Na x 3
     std::vector<double> u, w;
@@ -340,6 +345,9 @@ void exact_distance(const Scene& s, int64_t i, int64_t j, std::vector<Contact>& 
         c.i = i;
         c.j = j;
         c.mu = s.prm.mu;
+        c.gamma = s.prm.gamma;
+        c.gamma_min = s.prm.gamma_min;
+        c.gamma_tol = s.prm.gamma_tol;
         out.push_back(c);
     }
 }
@@ -521,28 +529,83 @@ void atmat(const Scene& s, const std::vector<Contact>& cs, const double* f, doub
 }
 
 // ------------------------------------------------------------------ minimization problem
-// minimization_problem<problem, ...>: solvers/minimization_problem.hpp:10-162 with
-// LagrangeMultiplier<NoFriction> (lagrange_multiplier.hpp:44-114) and
-// LagrangeMultiplier<Friction> (lagrange_multiplier.hpp:208-273).
+// minimization_problem<problem, ...>: solvers/minimization_problem.hpp:10-162 with the
+// LagrangeMultiplier of each problem (lagrange_multiplier.hpp):
+//   NoFriction (:44-114): one row per contact, lambda n;
+//   Friction / FrictionFixedPoint (:208-273, :275-345): three rows per contact, the Coulomb cone;
+//   Viscous (:116-206): one row per contact, and for a contact with gamma < -gamma_tol a second
+//     row appended after all first rows (in contact order): lambda = (x+ - x-) n, both >= 0;
+//   ViscousFriction (:347-553): rows in contact order, 1 (gamma >= -gamma_tol), 2 (x+, x-,
+//     gamma > gamma_min) or 4 (gamma == gamma_min: x-, then a cone multiplier f of dim rows,
+//     lambda = -x- n + f, projected on the nearer of the cone and the half-line x- >= 0).
+enum { kModeNormal = 0, kModeCone = 1, kModeViscous = 2, kModeViscousFriction = 3 };
+
+int problem_mode(int problem) {
+    switch (problem) {
+        case ORACLE_FRICTION:
+        case ORACLE_FRICTION_FIXED_POINT: return kModeCone;
+        case ORACLE_VISCOUS: return kModeViscous;
+        case ORACLE_VISCOUS_FRICTION: return kModeViscousFriction;
+        default: return kModeNormal;
+    }
+}
+
 struct MinProblem {
     const Scene& s;
     const std::vector<Contact>& cs;
     double dt;
-    bool friction;
+    int mode;
     std::vector<double> invM, C;
     mutable std::vector<double> g3, w6, a3;
     mutable int64_t q_applies = 0;
+    // Viscous layouts: the kind of every contact (0: one row, 1: two rows, 2: four rows), its first
+    // row and, for Viscous, the row of its second multiplier; nloc rows in all
+    std::vector<int> kind;
+    std::vector<int64_t> row, row2;
+    int64_t nloc = 0;
 
-    std::vector<double> S;  // S_Vector: s_ij n_ij per contact for FrictionFixedPoint (lagrange_multiplier.hpp:290-297)
+    // S_Vector: s_ij n_ij per contact for FrictionFixedPoint (lagrange_multiplier.hpp:290-297),
+    // dt mu s_ij n_ij on the cone rows of a four-row ViscousFriction contact (:379-400)
+    std::vector<double> S;
 
     MinProblem(const Scene& sc, const std::vector<Contact>& c, double dt_, const std::vector<double>* sij = nullptr)
         : s(sc), cs(c), dt(dt_) {
-        friction = (s.prm.problem == ORACLE_FRICTION || s.prm.problem == ORACLE_FRICTION_FIXED_POINT);
-        S.assign(friction ? 3 * cs.size() : cs.size(), 0.0);
-        if (sij)
-            for (size_t k = 0; k < cs.size(); ++k)
-                for (int d = 0; d < s.dim; ++d) S[3 * k + d] = (*sij)[k] * cs[k].nij[d];
+        mode = problem_mode(s.prm.problem);
         const int64_t na = s.na(), off = s.n_inactive, nc = (int64_t)cs.size();
+        kind.assign(nc, 0);
+        row.assign(nc, 0);
+        row2.assign(nc, -1);
+        if (mode == kModeNormal) nloc = nc;
+        if (mode == kModeCone) nloc = 3 * nc;
+        if (mode == kModeViscous) {
+            nloc = nc;
+            for (int64_t k = 0; k < nc; ++k) {
+                row[k] = k;
+                if (cs[k].gamma < -cs[k].gamma_tol) {
+                    kind[k] = 1;
+                    row2[k] = nloc++;
+                }
+            }
+        }
+        if (mode == kModeViscousFriction) {
+            for (int64_t k = 0; k < nc; ++k) {
+                row[k] = nloc;
+                if (cs[k].gamma < -cs[k].gamma_tol) {
+                    kind[k] = (cs[k].gamma != cs[k].gamma_min) ? 1 : 2;
+                    nloc += (kind[k] == 1) ? 2 : 4;
+                } else {
+                    nloc += 1;
+                }
+            }
+        }
+        S.assign(nloc, 0.0);
+        if (sij && mode == kModeCone)
+            for (int64_t k = 0; k < nc; ++k)
+                for (int d = 0; d < s.dim; ++d) S[3 * k + d] = (*sij)[k] * cs[k].nij[d];
+        if (sij && mode == kModeViscousFriction)
+            for (int64_t k = 0; k < nc; ++k)
+                if (kind[k] == 2)
+                    for (int d = 0; d < s.dim; ++d) S[row[k] + 1 + d] = dt * cs[k].mu * (*sij)[k] * cs[k].nij[d];
         // M_inverse: minimization_problem.hpp:10-31
         invM.assign(6 * na, 0.0);
         for (int64_t i = 0; i < na; ++i) {
@@ -573,31 +636,62 @@ struct MinProblem {
         w6.resize(6 * na);
         a3.resize(3 * nc);
     }
-    int64_t size() const { return friction ? 3 * (int64_t)cs.size() : (int64_t)cs.size(); }
+    int64_t size() const { return nloc; }
 
-    // local2global (lagrange_multiplier.hpp:78-92); the unset 2D slot is 0.
+    // local2global (NoFriction :78-92, Viscous :165-184, ViscousFriction :436-464); the unset
+    // 2D slot is 0.
     void l2g(const double* x, double* out) const {
         const int64_t nc = cs.size();
-        if (friction) {
+        if (mode == kModeCone) {
             std::copy(x, x + 3 * nc, out);
             return;
         }
         for (int64_t k = 0; k < nc; ++k) {
-            out[3 * k + 2] = 0.0;
-            for (int d = 0; d < s.dim; ++d) out[3 * k + d] = x[k] * cs[k].nij[d];
+            const double* n = cs[k].nij.data();
+            out[3 * k] = out[3 * k + 1] = out[3 * k + 2] = 0.0;
+            if (mode == kModeNormal) {
+                for (int d = 0; d < s.dim; ++d) out[3 * k + d] = x[k] * n[d];
+            } else if (mode == kModeViscous) {
+                const double a = kind[k] ? x[k] - x[row2[k]] : x[k];
+                for (int d = 0; d < s.dim; ++d) out[3 * k + d] = a * n[d];
+            } else {
+                const int64_t r = row[k];
+                if (kind[k] == 0)
+                    for (int d = 0; d < s.dim; ++d) out[3 * k + d] = x[r] * n[d];
+                else if (kind[k] == 1)
+                    for (int d = 0; d < s.dim; ++d) out[3 * k + d] = (x[r] - x[r + 1]) * n[d];
+                else
+                    for (int d = 0; d < s.dim; ++d) out[3 * k + d] = (-x[r]) * n[d] + x[r + 1 + d];
+            }
         }
     }
-    // global2local (lagrange_multiplier.hpp:62-76)
+    // global2local (NoFriction :62-76, Viscous :143-163, ViscousFriction :403-434)
     void g2l(const double* x, double* out) const {
         const int64_t nc = cs.size();
-        if (friction) {
+        if (mode == kModeCone) {
             std::copy(x, x + 3 * nc, out);
             return;
         }
+        if (mode == kModeViscousFriction) std::fill(out, out + nloc, 0.0);
         for (int64_t k = 0; k < nc; ++k) {
-            double acc = 0;
-            for (int d = 0; d < s.dim; ++d) acc += x[3 * k + d] * cs[k].nij[d];
-            out[k] = acc;
+            const double dn = dot(x + 3 * k, cs[k].nij.data(), s.dim);
+            if (mode == kModeNormal) {
+                out[k] = dn;
+            } else if (mode == kModeViscous) {
+                out[k] = dn;
+                if (kind[k]) out[row2[k]] = -dn;
+            } else {
+                const int64_t r = row[k];
+                if (kind[k] == 0) {
+                    out[r] = dn;
+                } else if (kind[k] == 1) {
+                    out[r] = dn;
+                    out[r + 1] = -dn;
+                } else {
+                    out[r] = -dn;
+                    for (int d = 0; d < s.dim; ++d) out[r + 1 + d] = x[3 * k + d];
+                }
+            }
         }
     }
     // QMatrix::operator(): minimization_problem.hpp:85-92  Q x = dt^2 A(M^-1 A^T x)
@@ -632,30 +726,56 @@ struct MinProblem {
         atmat(s, cs, g3.data(), out);
         for (int64_t k = 0; k < 6 * s.na(); ++k) out[k] = invM[k] * out[k];
     }
-    // projection: NoFriction lagrange_multiplier.hpp:104-107; Friction :243-268
+    // the Coulomb-cone projection of a dim-vector (lagrange_multiplier.hpp:243-268, :500-519)
+    void cone(double* li, const double* n, double mu) const {
+        double ln = dot(li, n, s.dim);
+        vec3 lt{0, 0, 0};
+        for (int d = 0; d < s.dim; ++d) lt[d] = li[d] - ln * n[d];
+        double norm = nrm2(lt.data(), s.dim);
+        if (norm > mu * ln) {
+            if (ln <= -mu * norm) {
+                for (int d = 0; d < s.dim; ++d) li[d] = 0;
+            } else {
+                double new_norm = (mu * mu) * (norm + ln / mu) / (mu * mu + 1);
+                double new_ln = new_norm / mu;
+                for (int d = 0; d < s.dim; ++d) lt[d] /= norm;
+                for (int d = 0; d < s.dim; ++d) li[d] = new_norm * lt[d] + new_ln * n[d];
+            }
+        }
+    }
+    // projection: NoFriction / Viscous max(lambda, 0) (:104-107, :197-201); Friction :243-268;
+    // ViscousFriction :487-547
     void projection(double* lam) const {
         const int64_t nc = cs.size();
-        if (!friction) {
-            for (int64_t k = 0; k < nc; ++k) lam[k] = std::max(lam[k], 0.);
+        if (mode == kModeNormal || mode == kModeViscous) {
+            for (int64_t k = 0; k < nloc; ++k) lam[k] = std::max(lam[k], 0.);
+            return;
+        }
+        if (mode == kModeCone) {
+            for (int64_t k = 0; k < nc; ++k) cone(lam + 3 * k, cs[k].nij.data(), cs[k].mu);
             return;
         }
         for (int64_t k = 0; k < nc; ++k) {
-            double* li = lam + 3 * k;
-            const double* n = cs[k].nij.data();
-            double ln = dot(li, n, s.dim);
-            vec3 lt{0, 0, 0};
-            for (int d = 0; d < s.dim; ++d) lt[d] = li[d] - ln * n[d];
-            double norm = nrm2(lt.data(), s.dim);
-            double mu = cs[k].mu;
-            if (norm > mu * ln) {
-                if (ln <= -mu * norm) {
-                    for (int d = 0; d < s.dim; ++d) li[d] = 0;
-                } else {
-                    double new_norm = (mu * mu) * (norm + ln / mu) / (mu * mu + 1);
-                    double new_ln = new_norm / mu;
-                    for (int d = 0; d < s.dim; ++d) lt[d] /= norm;
-                    for (int d = 0; d < s.dim; ++d) li[d] = new_norm * lt[d] + new_ln * n[d];
+            double* li = lam + row[k];
+            if (kind[k] == 0) {
+                li[0] = std::max(li[0], 0.);
+            } else if (kind[k] == 1) {
+                li[0] = std::max(li[0], 0.);
+                li[1] = std::max(li[1], 0.);
+            } else {
+                // the nearer (Euclidean, over the 1 + dim rows) of (0, Proj_cone(f)) and (max(x-, 0), 0)
+                double fric[4] = {0.0, li[1], li[2], li[3]};
+                cone(fric + 1, cs[k].nij.data(), cs[k].mu);
+                double moins[4] = {std::max(li[0], 0.), 0.0, 0.0, 0.0};
+                double df[4], dm[4];
+                for (int d = 0; d < 1 + s.dim; ++d) {
+                    df[d] = fric[d] - li[d];
+                    dm[d] = moins[d] - li[d];
                 }
+                if (nrm2(df, 1 + s.dim) < nrm2(dm, 1 + s.dim))
+                    for (int d = 0; d < 1 + s.dim; ++d) li[d] = fric[d];
+                else
+                    for (int d = 0; d < 1 + s.dim; ++d) li[d] = moins[d];
             }
         }
     }
@@ -765,52 +885,106 @@ SolveResult pgd(const MinProblem& mp, const oracle_params& prm) {
 // OptimGradient::run: solvers/OptimGradient.hpp:252-306
 void optim_solve(Scene& s, double dt, const std::vector<double>* sij);
 
+// A U for the optim slot's velocities (the relative velocity at every contact), 3 rows per contact.
+std::vector<double> contact_velocities(const Scene& s) {
+    const int64_t na = s.na(), nc = (int64_t)s.contacts.size();
+    std::vector<double> U(6 * na, 0.0), AU(3 * nc);
+    for (int64_t i = 0; i < na; ++i) {
+        for (int d = 0; d < s.dim; ++d) U[3 * i + d] = s.u[3 * i + d];
+        if (s.dim == 2)
+            U[3 * na + 3 * i + 2] = s.w[3 * i + 2];
+        else
+            for (int d = 0; d < 3; ++d) U[3 * na + 3 * i + d] = s.w[3 * i + d];
+    }
+    amat(s, s.contacts, U.data(), AU.data());
+    return AU;
+}
+
+// |T (A U)_k|: the norm of the tangential part of contact k's relative velocity
+double tangential_norm(const Scene& s, const std::vector<double>& AU, int64_t k) {
+    const double* a = AU.data() + 3 * k;
+    const double* n = s.contacts[k].nij.data();
+    const double an = dot(a, n, s.dim);
+    vec3 t{0, 0, 0};
+    for (int d = 0; d < s.dim; ++d) t[d] = a[d] - n[d] * an;
+    return nrm2(t.data(), s.dim);
+}
+
 // OptimGradient::run inside ScopiSolver::run's should_solve loop (solver.hpp:287-292): one solve,
-// or for FrictionFixedPoint the outer fixed point of OptimGradient.hpp:88-166: s_ij = mu dt,
-// solve, s_ij = mu dt |T(A U)_ij| (tangential part of the contact's relative velocity under the
-// solve's velocities), until (1/(mu dt)) |s_new - s_old| / (1 + (1/(mu dt)) |s_new|) < tol or
-// max_iter solves (mu, tol, max_iter of the first contact).
+// or an outer fixed point on s_ij (OptimGradient.hpp:86-240), controlled by the first contact's
+// fixed_point_tol / fixed_point_max_iter:
+//   FrictionFixedPoint: s_ij = mu dt, solve, s_ij = mu dt |T(A U)_ij| (tangential part of the
+//     contact's relative velocity under the solve's velocities), until
+//     (1/(mu dt)) |s_new - s_old| / (1 + (1/(mu dt)) |s_new|) < tol or max_iter solves;
+//   ViscousFriction: s_ij = 1 on the contacts at gamma == gamma_min, solve, s_ij = |T(A U)_ij| on
+//     those contacts, until |s_new - s_old| / (1 + |s_new|) < tol (both vectors 0 on the other
+//     contacts) or max_iter solves.
 void optim_run(Scene& s, double dt) {
     s.st.fixed_point_iterations = 0;
-    if (s.prm.problem != ORACLE_FRICTION_FIXED_POINT || s.contacts.empty()) {
+    const bool ffp = s.prm.problem == ORACLE_FRICTION_FIXED_POINT;
+    const bool vf = s.prm.problem == ORACLE_VISCOUS_FRICTION;
+    if (!(ffp || vf) || s.contacts.empty()) {
         optim_solve(s, dt, nullptr);
         return;
     }
-    const double mu = s.contacts[0].mu, mudt = mu * dt;
-    const int64_t nc = (int64_t)s.contacts.size(), na = s.na();
-    std::vector<double> sij(nc);
-    for (int64_t k = 0; k < nc; ++k) sij[k] = s.contacts[k].mu * dt;
+    const Contact& c0 = s.contacts[0];
+    const double mudt = c0.mu * dt;
+    const int64_t nc = (int64_t)s.contacts.size();
+    std::vector<double> sij(nc, 0.0);
+    for (int64_t k = 0; k < nc; ++k) {
+        if (ffp) sij[k] = s.contacts[k].mu * dt;
+        if (vf && s.contacts[k].gamma == s.contacts[k].gamma_min) sij[k] = 1.;
+    }
     int64_t it = 0, total = 0;
     for (;;) {
         optim_solve(s, dt, &sij);
         total += s.st.iterations;
-        std::vector<double> U(6 * na, 0.0), AU(3 * nc);
-        for (int64_t i = 0; i < na; ++i) {
-            for (int d = 0; d < s.dim; ++d) U[3 * i + d] = s.u[3 * i + d];
-            if (s.dim == 2)
-                U[3 * na + 3 * i + 2] = s.w[3 * i + 2];
-            else
-                for (int d = 0; d < 3; ++d) U[3 * na + 3 * i + d] = s.w[3 * i + d];
-        }
-        amat(s, s.contacts, U.data(), AU.data());
+        const std::vector<double> AU = contact_velocities(s);
         double ds2 = 0.0, s2 = 0.0;
         for (int64_t k = 0; k < nc; ++k) {
-            const double* a = AU.data() + 3 * k;
-            const double* n = s.contacts[k].nij.data();
-            const double an = dot(a, n, s.dim);
-            vec3 t{0, 0, 0};
-            for (int d = 0; d < s.dim; ++d) t[d] = a[d] - n[d] * an;
+            if (vf && s.contacts[k].gamma != s.contacts[k].gamma_min) continue;
             const double old = sij[k];
-            sij[k] = s.contacts[k].mu * dt * nrm2(t.data(), s.dim);
+            sij[k] = ffp ? s.contacts[k].mu * dt * tangential_norm(s, AU, k) : tangential_norm(s, AU, k);
             ds2 += (sij[k] - old) * (sij[k] - old);
             s2 += sij[k] * sij[k];
         }
-        const double err = (1 / mudt) * std::sqrt(ds2) / (1 + (1 / mudt) * std::sqrt(s2));
+        const double err = ffp ? (1 / mudt) * std::sqrt(ds2) / (1 + (1 / mudt) * std::sqrt(s2))
+                               : std::sqrt(ds2) / (1 + std::sqrt(s2));
         ++it;
         if (err < s.prm.fixed_point_tol || (double)it >= s.prm.fixed_point_max_iter) break;
     }
     s.st.fixed_point_iterations = it;
     s.st.iterations = total;
+}
+
+// update_contact_properties (OptimGradient.hpp:19-54), Viscous / ViscousFriction only:
+// gamma = min(max(gamma - dt n . lambda_global, gamma_min), 0) with the last solve's lambda.
+void update_contact_properties(Scene& s, double dt) {
+    const int mode = problem_mode(s.prm.problem);
+    if ((mode != kModeViscous && mode != kModeViscousFriction) || s.contacts.empty()) return;
+    MinProblem mp(s, s.contacts, dt);
+    std::vector<double> lg(3 * s.contacts.size());
+    mp.l2g(s.lambda.data(), lg.data());
+    for (size_t k = 0; k < s.contacts.size(); ++k) {
+        Contact& c = s.contacts[k];
+        c.gamma = std::min(std::max(c.gamma - dt * dot(c.nij.data(), lg.data() + 3 * k, s.dim), c.gamma_min), 0.);
+    }
+}
+
+// ScopiSolver::transfer (solver.hpp:241-259): a new contact whose (i, j) was a contact of the last
+// step takes that contact's property (the last one of that pair when the pair occurs twice).
+void transfer(const std::vector<Contact>& old, std::vector<Contact>& cs) {
+    std::map<std::pair<int64_t, int64_t>, size_t> index;
+    for (size_t k = 0; k < old.size(); ++k) index[{old[k].i, old[k].j}] = k;
+    for (Contact& c : cs) {
+        auto it = index.find({c.i, c.j});
+        if (it == index.end()) continue;
+        const Contact& o = old[it->second];
+        c.mu = o.mu;
+        c.gamma = o.gamma;
+        c.gamma_min = o.gamma_min;
+        c.gamma_tol = o.gamma_tol;
+    }
 }
 
 void optim_solve(Scene& s, double dt, const std::vector<double>* sij) {
@@ -1033,6 +1207,9 @@ int oracle_default_params(oracle_params* p) {
     p->mu = 0.1;
     p->fixed_point_tol = 1e-6;
     p->fixed_point_max_iter = 1000;
+    p->gamma = 0.0;  // contact_property<Viscous> (contact/property.hpp:125-127)
+    p->gamma_min = -2.;
+    p->gamma_tol = 1e-6;
     return 0;
 }
 
@@ -1070,6 +1247,7 @@ int oracle_scene_set(void* scene, int32_t dim, int64_t n, int64_t n_inactive, co
         for (int64_t i = 0; i < n; ++i) s.sq[i] = {squareness[2 * i], squareness[2 * i + 1]};
     s.m.assign(m, m + n);
     s.contacts.clear();
+    s.old_contacts.clear();
     s.lambda.clear();
     s.st = oracle_stats{};
     return 0;
@@ -1110,8 +1288,8 @@ int oracle_get_state(void* scene, double* pos, double* q, double* v, double* ome
     return 0;
 }
 
-// ScopiSolver::run: solver.hpp:266-303 (no output; transfer() is inert for
-// NoFriction/Friction because every contact carries the default property).
+// ScopiSolver::run: solver.hpp:266-303 (no output).  The last step's contacts stay with the
+// scene between calls, as m_old_contacts stays with the solver.
 int oracle_run(void* scene, double dt, int64_t total_it, int64_t initial_iter) {
     if (!scene) return -1;
     Scene& s = *static_cast<Scene*>(scene);
@@ -1119,6 +1297,7 @@ int oracle_run(void* scene, double dt, int64_t total_it, int64_t initial_iter) {
         auto t0 = std::chrono::steady_clock::now();
         displacement_obstacles(s, dt);
         s.contacts = compute_contacts(s);
+        if (!s.old_contacts.empty()) transfer(s.old_contacts, s.contacts);
         s.st.n_contacts = (int64_t)s.contacts.size();
         s.st.t_contacts += seconds_since(t0);
         t0 = std::chrono::steady_clock::now();
@@ -1126,6 +1305,8 @@ int oracle_run(void* scene, double dt, int64_t total_it, int64_t initial_iter) {
         s.st.t_vap += seconds_since(t0);
         t0 = std::chrono::steady_clock::now();
         optim_run(s, dt);
+        update_contact_properties(s, dt);
+        s.old_contacts = s.contacts;
         s.st.t_solve += seconds_since(t0);
         t0 = std::chrono::steady_clock::now();
         update_and_move(s, dt);
@@ -1188,6 +1369,32 @@ int oracle_solve(void* scene, double dt, double* lambda, double* u, double* w) {
     if (lambda) std::copy(s.lambda.begin(), s.lambda.end(), lambda);
     if (u) std::copy(s.u.begin(), s.u.end(), u);
     if (w) std::copy(s.w.begin(), s.w.end(), w);
+    return 0;
+}
+
+int oracle_set_forces(void* scene, const double* f) {
+    if (!scene || !f) return -1;
+    Scene& s = *static_cast<Scene*>(scene);
+    for (int64_t i = 0; i < s.n; ++i) s.f[i] = {f[3 * i], f[3 * i + 1], f[3 * i + 2]};
+    return 0;
+}
+
+int64_t oracle_lambda_size(void* scene) {
+    if (!scene) return -1;
+    return (int64_t) static_cast<Scene*>(scene)->lambda.size();
+}
+
+int oracle_get_contact_gamma(void* scene, double* gamma) {
+    if (!scene || !gamma) return -1;
+    Scene& s = *static_cast<Scene*>(scene);
+    for (size_t k = 0; k < s.contacts.size(); ++k) gamma[k] = s.contacts[k].gamma;
+    return 0;
+}
+
+int oracle_set_contact_gamma(void* scene, const double* gamma) {
+    if (!scene || !gamma) return -1;
+    Scene& s = *static_cast<Scene*>(scene);
+    for (size_t k = 0; k < s.contacts.size(); ++k) s.contacts[k].gamma = gamma[k];
     return 0;
 }
 

@@ -25,7 +25,8 @@ extern "C" {
 #endif
 
 enum { ORACLE_SPHERE = 0, ORACLE_PLANE = 1, ORACLE_SEGMENT = 2, ORACLE_SUPERELLIPSOID = 3 };
-enum { ORACLE_NOFRICTION = 0, ORACLE_FRICTION = 1, ORACLE_FRICTION_FIXED_POINT = 2 };
+enum { ORACLE_NOFRICTION = 0, ORACLE_FRICTION = 1, ORACLE_FRICTION_FIXED_POINT = 2, ORACLE_VISCOUS = 3,
+       ORACLE_VISCOUS_FRICTION = 4 };
 enum { ORACLE_CONTACT_KDTREE = 0, ORACLE_CONTACT_BRUTE_FORCE = 1 };
 enum { ORACLE_VAP_FIXED = 0, ORACLE_VAP_FPD = 1, ORACLE_VAP_PROJECTION = 2 };
 enum { ORACLE_METHOD_APGD = 0, ORACLE_METHOD_PGD = 1 };
@@ -45,6 +46,9 @@ typedef struct oracle_params {
     double mu;               /* contact_property<Friction>::mu (default 0.1)   */
     double fixed_point_tol;  /* contact_property<FixedPoint> (default 1e-6)    */
     double fixed_point_max_iter; /* (default 1000)                            */
+    double gamma;            /* contact_property<Viscous>::gamma (default 0)   */
+    double gamma_min;        /* (default -2)                                   */
+    double gamma_tol;        /* (default 1e-6)                                 */
 } oracle_params;
 
 typedef struct oracle_stats {
@@ -55,7 +59,7 @@ typedef struct oracle_stats {
     int64_t converged;    /* 1 if the last solve met the tolerance            */
     int64_t total_iterations; /* summed over every solve since the scene was set */
     int64_t total_q_applies;
-    int64_t fixed_point_iterations; /* FrictionFixedPoint: solves of the last step */
+    int64_t fixed_point_iterations; /* FrictionFixedPoint / ViscousFriction: solves of the last step */
     double t_contacts, t_vap, t_solve, t_move; /* seconds, summed over steps  */
 } oracle_stats;
 
@@ -94,6 +98,14 @@ int oracle_atmat(void* scene, const double* f, double* out); /* 3*nc -> 6*Na */
  * velocities taken as a-priori velocities.  lambda (size nc or 3nc), u (Na*3),
  * w (Na*3) may be NULL. */
 int oracle_solve(void* scene, double dt, double* lambda, double* u, double* w);
+/* Replace the external forces (n x 3); the contacts of the last step and their properties stay,
+ * as they do when a reference test edits particles.f() between two runs. */
+int oracle_set_forces(void* scene, const double* f);
+/* Size of the last solve's multiplier vector (nc, 3nc, or the Viscous layouts' variable size). */
+int64_t oracle_lambda_size(void* scene);
+/* The current contacts' gamma (contact_property<Viscous>::gamma): get, or set before a solve. */
+int oracle_get_contact_gamma(void* scene, double* gamma);
+int oracle_set_contact_gamma(void* scene, const double* gamma);
 /* The vap slot alone. */
 int oracle_set_a_priori_velocity(void* scene, double dt);
 

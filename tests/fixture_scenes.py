@@ -76,6 +76,26 @@ def three_spheres_friction_fp():
     return case
 
 
+def three_spheres_viscous():
+    """test/test_gradient.cpp:441-503: the 3 spheres of 3spheres_nofriction under Viscous (default
+    gamma 0, gamma_min -2, gamma_tol 1e-6), apgd tol 1e-7, alpha 0.1, 70 steps."""
+    case = three_spheres_nofriction()
+    case.update(total_it=70, fixture="3spheres_viscous",
+                params=dict(tolerance=1e-7, max_ite=10000, alpha=0.1, dynamic_descent=1, problem=3))
+    return case
+
+
+def three_spheres_viscous_friction():
+    """test/test_gradient.cpp:574-640: ViscousFriction, mu 0.5, fixed_point_tol 1e-3, max_iter 1000,
+    gamma 0, gamma_min -1.4, gamma_tol 1e-6, apgd tol 1e-7, alpha 0.1, 70 steps."""
+    case = three_spheres_nofriction()
+    case.update(total_it=70, fixture="3spheres_viscous_friction",
+                params=dict(tolerance=1e-7, max_ite=10000, alpha=0.1, dynamic_descent=1, problem=4, mu=0.5,
+                            fixed_point_tol=1e-3, fixed_point_max_iter=1000, gamma=0.0, gamma_min=-1.4,
+                            gamma_tol=1e-6))
+    return case
+
+
 def sphere_plane_friction_mu01():
     """test/test_gradient.cpp:156-213: Friction, plane at pi/2-pi/6, mu 0.1, dt 0.1, 200 steps."""
     c = Container(2)
@@ -215,3 +235,80 @@ def sphere_plane_errors(case, state):
 def doctest_le(x, bound, eps=float(np.finfo(np.float32).eps) * 100):
     """doctest's `x <= Approx(bound)` (epsilon 100 float eps, scale 1)."""
     return x < bound or abs(x - bound) < eps * (1.0 + max(abs(x), abs(bound)))
+
+
+def analytical_sphere_plane_viscous(alpha, t, r, g, y0, gamma_min, t_inv):
+    """test/analytical_solution.cpp:39-129: a sphere dropped on a horizontal plane under gravity
+    tilted by alpha, Viscous, the gravity reversed at t_inv; position and velocity (no rotation)."""
+    t_impact = math.sqrt(2 * (y0 - r) / (g * math.cos(alpha)))
+    t_gamma_min = -gamma_min / (g * math.cos(alpha))
+    gamma_inv = gamma_min if t_inv > t_gamma_min else -g * t_inv * math.cos(alpha)
+    t_unstick = t_inv - gamma_inv / (g * math.cos(alpha))
+    sa, ca = math.sin(alpha), math.cos(alpha)
+    if t < t_impact:
+        return np.array([g * t * t * sa / 2, -g * t * t * ca / 2 + y0]), np.array([g * t * sa, -g * t * ca])
+    if t < t_inv:
+        return np.array([g * t * t * sa / 2, r]), np.array([g * t * sa, 0.0])
+    x0 = g * t_inv ** 2 * sa / 2 + g * t_inv * sa * (t - t_inv) - g * (t - t_inv) ** 2 * sa / 2
+    v0 = g * t_inv * sa - g * (t - t_inv) * sa
+    if t < t_unstick:
+        return np.array([x0, r]), np.array([v0, 0.0])
+    return np.array([x0, r + g * (t - t_unstick) ** 2 * ca / 2]), np.array([v0, g * (t - t_unstick) * ca])
+
+
+def analytical_sphere_plane_viscous_friction(alpha, mu, t, r, g, y0, gamma_min, t_inv):
+    """test/analytical_solution.cpp:193-419: the same drop under ViscousFriction; position, rotation
+    angle, velocity and rotation velocity."""
+    sa, ca, ta = math.sin(alpha), math.cos(alpha), math.tan(alpha)
+    t_impact = math.sqrt(2 * (y0 - r) / (g * ca))
+    t_gamma_min = -gamma_min / (g * ca)
+    gamma_inv = gamma_min if t_inv > t_gamma_min else -g * t_inv * ca
+    if t < t_impact:
+        return (np.array([g * t * t * sa / 2, -g * t * t * ca / 2 + y0]), 0.0, np.array([g * t * sa, -g * t * ca]), 0.0)
+    if t < t_gamma_min:
+        return np.array([g * t * t * sa / 2, r]), 0.0, np.array([g * t * sa, 0.0]), 0.0
+    v_t_gm = -gamma_min * ta
+    x_t_gm = g * t_gamma_min ** 2 * sa / 2
+
+    def sliding_then_rolling(tt):
+        """(x_t, angle, v_t, omega) at tt >= t_gamma_min before the reversal."""
+        t2 = tt - t_gamma_min
+        slide = (g * (sa - mu * ca) * t2 ** 2 / 2 + v_t_gm * t2 + x_t_gm, -2. * mu * g * ca * t2 ** 2 / (2 * r),
+                 g * (sa - mu * ca) * t2 + v_t_gm, -2. * mu * g * ca * t2 / r)
+        if ta <= 3 * mu:
+            t_ns = -v_t_gm / (g * ca * (ta - 3 * mu)) + t_gamma_min
+            if tt > t_ns:
+                vt_ns = g * (sa - mu * ca) * (t_ns - t_gamma_min) + v_t_gm
+                om_ns = -2. * mu * g * ca * (t_ns - t_gamma_min) / r
+                xt_ns = g * (sa - mu * ca) * (t_ns - t_gamma_min) ** 2 / 2 + v_t_gm * (t_ns - t_gamma_min) + x_t_gm
+                an_ns = -2. * mu * g * ca * (t_ns - t_gamma_min) ** 2 / (2 * r)
+                return (2. * g * sa * (tt - t_ns) ** 2 / 6. + vt_ns * (tt - t_ns) + xt_ns,
+                        -2. * g * sa * (tt - t_ns) ** 2 / (6. * r) + om_ns * (tt - t_ns) + an_ns,
+                        2. * g * sa * (tt - t_ns) / 3. + vt_ns, -2. * g * sa * (tt - t_ns) / (3. * r) + om_ns)
+        return slide
+
+    if t < t_inv:
+        x, ang, v, om = sliding_then_rolling(t)
+        return np.array([x, r]), ang, np.array([v, 0.0]), om
+    x_i, ang_i, v_i, om_i = sliding_then_rolling(t_inv)
+    t_unstick = t_inv - gamma_inv / (g * ca)
+    x = np.array([v_i * (t - t_inv) - g * sa * (t - t_inv) ** 2 / 2 + x_i, r])
+    v = np.array([v_i - g * sa * (t - t_inv), 0.0])
+    if t >= t_unstick:
+        x[1] = g * ca * (t - t_unstick) ** 2 / 2 + r
+        v[1] = g * ca * (t - t_unstick)
+    return x, om_i * (t - t_inv) + ang_i, v, om_i
+
+
+def sphere_plane_reversed_gravity(problem, steps1, steps2, **prm):
+    """test/test_gradient.cpp:86-155 / 284-373: a sphere of radius 1 at height 2 over a horizontal plane,
+    force (g sin a, -g cos a), a = pi/4, g = 1, dt 0.01, steps1 steps, then the force reversed for
+    steps2 steps; apgd tol 1e-7, alpha 0.1, vap_fpd."""
+    radius, g, h, alpha = 1.0, 1.0, 2.0, PI / 4
+    c = Container(2)
+    c.plane((0.0, 0.0), PI / 2)
+    c.sphere((0.0, h), radius, Property(mass=1.0, moment_inertia=radius * radius / 2.0,
+                                        force=(g * math.sin(alpha), -g * math.cos(alpha))))
+    return dict(scene=c.arrays(), dt=0.01, steps=(steps1, steps2), vap=FPD, alpha=alpha, radius=radius, g=g, h=h,
+                reversed_force=(-g * math.sin(alpha), g * math.cos(alpha)),
+                params=dict(tolerance=1e-7, max_ite=10000, alpha=0.1, dynamic_descent=1, problem=problem, **prm))

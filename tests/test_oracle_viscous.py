@@ -1,0 +1,100 @@
+"""Viscous and ViscousFriction in the CPU restatement, pinned by the reference's own tests and by
+closed forms of one time step.
+
+* test/test_gradient.cpp:86-155 ("sphere - plane viscous without friction") and 284-373 ("sphere -
+  plane viscous friction"): a sphere dropped on a plane under tilted gravity, the gravity reversed
+  after 500 steps; the reference's analytical solutions (test/analytical_solution.cpp:39-419) with
+  the reference's bounds.  The adhesion that holds the sphere after the reversal lives in gamma,
+  carried from step to step by transfer: NoFriction, or a lost gamma, lets go at once and fails.
+* 3spheres_viscous.json and 3spheres_viscous_friction.json are covered with the other fixtures.
+* One step of a unit sphere (m 1, J 1/2) touching a plane, dt 0.1, per contact kind: the
+  multiplier's impulse is m |v_n| = 1, so gamma moves by exactly 1 (then clamped to
+  [gamma_min, 0]); a two-row contact holds under tension, a one-row contact does not; a
+  four-row ViscousFriction contact (gamma == gamma_min) is Coulomb in compression (u = (1 - mu,
+  0), omega = -mu r / J) and frictionless in tension; friction acts on no other kind.
+"""
+import math
+
+import numpy as np
+import pytest
+
+from scopi_amd.scenes import Container, Property
+from tests import fixture_scenes as fs
+
+
+def run_reversed(oracle_lib, case):
+    o = oracle_lib.Oracle(case["scene"], vap=case["vap"], **case["params"])
+    s1, s2 = case["steps"]
+    o.run(case["dt"], s1)
+    f = case["scene"].f.copy()
+    f[1, :2] = case["reversed_force"]
+    o.set_forces(f)
+    o.run(case["dt"], s2)
+    return o.state()
+
+
+def test_sphere_plane_viscous_analytic(oracle_lib):
+    case = fs.sphere_plane_reversed_gravity(oracle_lib.VISCOUS, 500, 500)
+    st = run_reversed(oracle_lib, case)
+    pa, va = fs.analytical_sphere_plane_viscous(case["alpha"], 10.0, 1.0, 1.0, 2.0, -2.0, 5.0)
+    e_pos = np.linalg.norm(st["pos"][1, :2] - pa) / np.linalg.norm(pa)
+    e_q = np.linalg.norm(st["q"][1] - [1.0, 0.0, 0.0, 0.0])
+    e_v = np.linalg.norm(st["v"][1, :2] - va) / np.linalg.norm(va)
+    assert fs.doctest_le(e_pos, 1e-3) and fs.doctest_le(e_q, 1e-3) and fs.doctest_le(e_v, 1e-3)
+    assert abs(st["omega"][1, 2]) < 1e-5
+
+
+def test_reversed_gravity_needs_adhesion(oracle_lib):
+    """The same run without adhesion (NoFriction) leaves the plane at the reversal."""
+    case = fs.sphere_plane_reversed_gravity(oracle_lib.NOFRICTION, 500, 500)
+    st = run_reversed(oracle_lib, case)
+    pa, _ = fs.analytical_sphere_plane_viscous(case["alpha"], 10.0, 1.0, 1.0, 2.0, -2.0, 5.0)
+    assert np.linalg.norm(st["pos"][1, :2] - pa) / np.linalg.norm(pa) > 1e-2
+
+
+def test_sphere_plane_viscous_friction_analytic(oracle_lib):
+    case = fs.sphere_plane_reversed_gravity(oracle_lib.VISCOUS_FRICTION, 500, 100, mu=0.5, fixed_point_tol=1e-3,
+                                            fixed_point_max_iter=1000, gamma=0.0, gamma_min=-1.4, gamma_tol=1e-6)
+    st = run_reversed(oracle_lib, case)
+    pa, th, va, wa = fs.analytical_sphere_plane_viscous_friction(case["alpha"], 0.5, 6.0, 1.0, 1.0, 2.0, -1.4, 5.0)
+    qa = np.array([math.cos(th / 2), 0.0, 0.0, math.sin(th / 2)])
+    e_pos = np.linalg.norm(st["pos"][1, :2] - pa) / np.linalg.norm(pa)
+    e_q = np.linalg.norm(st["q"][1] - qa) / np.linalg.norm(qa)
+    e_v = np.linalg.norm(st["v"][1, :2] - va) / np.linalg.norm(va)
+    e_w = (st["omega"][1, 2] - wa) ** 2 / wa ** 2
+    assert fs.doctest_le(e_pos, 1e-1) and fs.doctest_le(e_q, 1e-1)
+    assert fs.doctest_le(e_v, 1e-3) and fs.doctest_le(e_w, 1e-3)
+
+
+def one_step(oracle_lib, problem, vd, gamma, **kw):
+    c = Container(2)
+    c.plane((0.0, 0.0), math.pi / 2)
+    c.sphere((0.0, 1.0), 1.0, Property(mass=1, moment_inertia=0.5, desired_velocity=vd))
+    o = oracle_lib.Oracle(c.arrays(), vap=0, problem=problem, gamma=gamma, tolerance=1e-12, max_ite=100000, **kw)
+    o.run(0.1, 1)
+    st = o.state()
+    return st["v"][1, :2], st["omega"][1, 2], o.contact_gamma()
+
+
+VF = dict(mu=0.2, gamma_min=-1.4, fixed_point_tol=1e-12)
+# (problem, desired velocity, gamma, expected velocity, expected omega, expected gamma after the step)
+ONE_STEP = [
+    ("viscous", (0, 1), 0.0, (0, 1), 0.0, 0.0),          # one row, tension: lets go
+    ("viscous", (0, 1), -2.0, (0, 0), 0.0, -1.0),        # two rows, tension: holds, gamma rises by 1
+    ("viscous", (0, 1), -0.5, (0, 0), 0.0, 0.0),         # holds, gamma clamped at 0
+    ("viscous", (0, -1), 0.0, (0, 0), 0.0, -1.0),        # compression: gamma falls by 1
+    ("viscous_friction", (1, -1), -1.4, (0.8, 0), -0.4, -1.4),  # four rows, compression: Coulomb
+    ("viscous_friction", (1, 1), -1.4, (1, 0), 0.0, -0.4),      # four rows, tension: holds, no friction
+    ("viscous_friction", (1, -1), -0.7, (1, 0), 0.0, -1.4),     # two rows: no friction, clamped at gamma_min
+    ("viscous_friction", (1, 1), -0.7, (1, 0), 0.0, 0.0),
+    ("viscous_friction", (1, -1), 0.0, (1, 0), 0.0, -1.0),      # one row: no friction
+]
+
+
+@pytest.mark.parametrize("problem,vd,gamma,u,w,g_after", ONE_STEP)
+def test_one_step_per_contact_kind(oracle_lib, problem, vd, gamma, u, w, g_after):
+    pb = oracle_lib.VISCOUS if problem == "viscous" else oracle_lib.VISCOUS_FRICTION
+    v, om, g = one_step(oracle_lib, pb, vd, gamma, **(VF if pb == oracle_lib.VISCOUS_FRICTION else {}))
+    np.testing.assert_allclose(v, u, atol=1e-6)
+    assert abs(om - w) < 1e-6
+    assert g.shape == (1,) and abs(g[0] - g_after) < 1e-6
