@@ -41,13 +41,19 @@ extern "C" {
 
 /* ------------------------------------------------------------------ enums */
 enum { SCOPI_SHAPE_SPHERE = 0, SCOPI_SHAPE_PLANE = 1, SCOPI_SHAPE_SEGMENT = 2, SCOPI_SHAPE_SUPERELLIPSOID = 3 };
-/* problem tags NoFriction / Friction (include/scopi/contact/property.hpp:14-20) */
-/* problem tags: NoFriction, Friction, FrictionFixedPoint (Friction with the outer fixed point
- * on s_ij, OptimGradient.hpp:88-166) */
-enum { SCOPI_PROBLEM_NOFRICTION = 0, SCOPI_PROBLEM_FRICTION = 1, SCOPI_PROBLEM_FRICTION_FIXED_POINT = 2 };
+/* problem tags (include/scopi/contact/property.hpp:14-37): NoFriction, Friction,
+ * FrictionFixedPoint (Friction with the outer fixed point on s_ij, OptimGradient.hpp:86-166),
+ * Viscous and ViscousFriction (adhesion potential gamma per contact, carried from step to step,
+ * lagrange_multiplier.hpp:116-206 and 347-553, OptimGradient.hpp:19-54 and 96-240) */
+enum {
+    SCOPI_PROBLEM_NOFRICTION = 0,
+    SCOPI_PROBLEM_FRICTION = 1,
+    SCOPI_PROBLEM_FRICTION_FIXED_POINT = 2,
+    SCOPI_PROBLEM_VISCOUS = 3,
+    SCOPI_PROBLEM_VISCOUS_FRICTION = 4
+};
 /* contact slot: contact_kdtree (radius prefilter) / contact_brute_force */
 enum { SCOPI_CONTACT_KDTREE = 0, SCOPI_CONTACT_BRUTE_FORCE = 1 };
-/* vap slot: vap_fixed / vap_fpd */
 /* vap slot: vap_fixed, vap_fpd, vap_projection (velocities given by scopi_hip_set_projection_velocity) */
 enum { SCOPI_VAP_FIXED = 0, SCOPI_VAP_FPD = 1, SCOPI_VAP_PROJECTION = 2 };
 /* optim method: apgd / pgd (include/scopi/solvers/apgd.hpp:39-227) */
@@ -76,13 +82,19 @@ typedef struct scopi_hip_params {
     int32_t warm_start;       /* reserved, must be 0 (lambda_0 = 0 every step, apgd.hpp:152-159) */
     int32_t general_jacobian; /* 1: keep the rotational blocks of B even when every active particle is a
                                  sphere (they vanish for spheres; this only changes speed) */
-    double fixed_point_tol;       /* FrictionFixedPoint: contact_property<FixedPoint> (property.hpp:97-98) */
+    double fixed_point_tol;       /* FrictionFixedPoint / ViscousFriction: contact_property<FixedPoint>
+                                     (property.hpp:97-98) */
     int64_t fixed_point_max_iter;
+    double gamma;     /* Viscous / ViscousFriction: contact_property<Viscous> (property.hpp:125-127): the  */
+    double gamma_min; /* gamma of a new contact (default 0), its lower bound (-2) and the tolerance of */
+    double gamma_tol; /* the test gamma < -gamma_tol (1e-6) */
 } scopi_hip_params;
 
 typedef struct scopi_hip_solve_info {
     int64_t n_contacts;
-    int64_t rows;          /* nc (NoFriction) or 3 nc (Friction) */
+    int64_t rows;          /* multiplier rows: nc (NoFriction), 3 nc (Friction, FrictionFixedPoint), nc plus
+                              the contacts with gamma < -gamma_tol (Viscous), 1, 2 or 4 per contact
+                              (ViscousFriction) */
     int64_t iterations;    /* accepted projected-gradient iterations (all phases) */
     int64_t iterations_fp32;
     int64_t backtracks;    /* rejected line-search trials */
@@ -90,8 +102,8 @@ typedef struct scopi_hip_solve_info {
     int32_t converged;
     int32_t status;
     double residual;       /* last ||lambda_{k+1} - lambda_k||_2 */
-    int64_t outer_iterations;  /* FrictionFixedPoint: solves of the fixed point (0 otherwise) */
-    double fixed_point_error;  /* FrictionFixedPoint: the last relative change of s */
+    int64_t outer_iterations;  /* FrictionFixedPoint / ViscousFriction: solves of the fixed point (0 otherwise) */
+    double fixed_point_error;  /* the fixed point's last relative change of s */
 } scopi_hip_solve_info;
 
 typedef struct scopi_hip_ctx scopi_hip_ctx;
@@ -145,10 +157,22 @@ int scopi_hip_set_projection_velocity(scopi_hip_ctx* ctx, const double* u, const
 
 /* Optim slot: OptimGradient<apgd|pgd>::run (solvers/OptimGradient.hpp:252-306):
  * assemble B from the device contacts, solve the dual problem, compute
- * u = v + dt M^-1 B^T lambda (kept on the device). */
+ * u = v + dt M^-1 B^T lambda (kept on the device).  For FrictionFixedPoint and
+ * ViscousFriction the call runs the whole should_solve loop (solver.hpp:287-292); for
+ * Viscous and ViscousFriction it then applies update_contact_properties
+ * (OptimGradient.hpp:19-54) to the contacts' gamma. */
 int scopi_hip_solve(scopi_hip_ctx* ctx, const scopi_hip_params* params, double dt, scopi_hip_solve_info* info);
+/* Viscous problems: the gamma of every current contact (nc values, in the contact list's
+ * order).  Contacts found by scopi_hip_compute_contacts or installed by
+ * scopi_hip_upload_contacts start at params->gamma when solved; set_contact_gamma gives
+ * them other values (the properties a contact policy carries, e.g. after transfer).
+ * download_contact_gamma reads them back, after a solve the updated values. */
+int scopi_hip_set_contact_gamma(scopi_hip_ctx* ctx, const double* gamma);
+int scopi_hip_download_contact_gamma(scopi_hip_ctx* ctx, double* gamma);
 /* get_uadapt / get_wadapt / lagrange_multiplier (OptimGradient.hpp:317-335):
- * u, omega are Na x 3; lambda has nc (NoFriction) or 3 nc (Friction) entries. */
+ * u, omega are Na x 3; lambda has info.rows entries in the reference's order (for
+ * Viscous: the first row of every contact, then the second rows in contact order; for
+ * ViscousFriction: each contact's rows together, in contact order). */
 int scopi_hip_download_solution(scopi_hip_ctx* ctx, double* u, double* omega, double* lambda);
 
 /* ScopiSolver::update_velocity + move_active_particles (solver.hpp:440-546). */
@@ -163,8 +187,11 @@ int scopi_hip_update_and_move(scopi_hip_ctx* ctx, double dt);
 int scopi_hip_displacement_obstacles(scopi_hip_ctx* ctx, double dt);
 
 /* ScopiSolver::run(dt, total_it, initial_iter) (solver.hpp:266-303) with every
- * slot on the device and no output files: obstacles, contacts, vap, solve,
- * update, move per step.  info (may be NULL) accumulates over the steps. */
+ * slot on the device and no output files: obstacles, contacts, transfer, vap, solve,
+ * update, move per step.  info (may be NULL) accumulates over the steps.  The last
+ * step's contacts and their gamma stay with the context for the next step's transfer
+ * (solver.hpp:241-259), also across calls, as m_old_contacts stays with the solver;
+ * scopi_hip_upload_state with another particle count, or another obstacle count, drops them. */
 int scopi_hip_run(scopi_hip_ctx* ctx, const scopi_hip_params* params, double dt, int64_t total_it,
                   int64_t initial_iter, scopi_hip_solve_info* info);
 
@@ -239,8 +266,9 @@ int scopi_hip_kernel_timing(scopi_hip_ctx* ctx, double* out);
  * and in any mix).  Scene ids are returned by add_scene and stay valid until
  * remove_scene; scenes added or removed take effect at the next call that reads or
  * steps the batch.  The params must not ask for warm_start or vap_projection
- * (SCOPI_HIP_ERR_ARG); FrictionFixedPoint is refused with SCOPI_HIP_ERR_UNSUPPORTED
- * (its outer loop is per scene; step such scenes through scopi_hip_run). */
+ * (SCOPI_HIP_ERR_ARG); FrictionFixedPoint, Viscous and ViscousFriction are refused with
+ * SCOPI_HIP_ERR_UNSUPPORTED (their outer loop and contact history are per scene; step such
+ * scenes through scopi_hip_run). */
 typedef struct scopi_hip_batch scopi_hip_batch;
 int scopi_hip_batch_create(const scopi_hip_params* params, double dt, scopi_hip_batch** out);
 int scopi_hip_batch_destroy(scopi_hip_batch* b);

@@ -72,6 +72,7 @@ def lib():
         L.oracle_solve.argtypes = [P, C.c_double, D, D, D]
         L.oracle_set_a_priori_velocity.argtypes = [P, C.c_double]
         L.oracle_set_forces.argtypes = [P, D]
+        L.oracle_update_contact_properties.argtypes = [P, C.c_double]
         L.oracle_lambda_size.argtypes = [P]
         L.oracle_lambda_size.restype = C.c_int64
         L.oracle_get_contact_gamma.argtypes = [P, D]
@@ -182,6 +183,10 @@ class Oracle:
         self._L.oracle_solve(self._h, float(dt), lam, u, w)
         nl = int(self._L.oracle_lambda_size(self._h))
         return dict(lam=lam[:nl], u=u[: 3 * na].reshape(na, 3), w=w[: 3 * na].reshape(na, 3))
+
+    def update_contact_properties(self, dt):
+        """The contacts' gamma from the last solve's multipliers (Viscous problems)."""
+        assert self._L.oracle_update_contact_properties(self._h, float(dt)) == 0
 
     def set_forces(self, f):
         """Replace the external forces (n x 3); the last step's contacts and properties stay."""

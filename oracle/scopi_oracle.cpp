@@ -1372,6 +1372,12 @@ int oracle_solve(void* scene, double dt, double* lambda, double* u, double* w) {
     return 0;
 }
 
+int oracle_update_contact_properties(void* scene, double dt) {
+    if (!scene) return -1;
+    update_contact_properties(*static_cast<Scene*>(scene), dt);
+    return 0;
+}
+
 int oracle_set_forces(void* scene, const double* f) {
     if (!scene || !f) return -1;
     Scene& s = *static_cast<Scene*>(scene);

@@ -98,6 +98,9 @@ int oracle_atmat(void* scene, const double* f, double* out); /* 3*nc -> 6*Na */
  * velocities taken as a-priori velocities.  lambda (size nc or 3nc), u (Na*3),
  * w (Na*3) may be NULL. */
 int oracle_solve(void* scene, double dt, double* lambda, double* u, double* w);
+/* OptimGradient::update_contact_properties after oracle_solve (Viscous problems: the contacts'
+ * gamma from the last solve's multipliers). */
+int oracle_update_contact_properties(void* scene, double dt);
 /* Replace the external forces (n x 3); the contacts of the last step and their properties stay,
  * as they do when a reference test edits particles.f() between two runs. */
 int oracle_set_forces(void* scene, const double* f);

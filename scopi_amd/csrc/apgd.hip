@@ -166,7 +166,7 @@ __global__ void __launch_bounds__(kBlock) k_assemble(Particles P, Contacts C, in
         co[3] = a_i.x, co[4] = a_i.y, co[5] = a_i.z;
         co[6] = a_j.x, co[7] = a_j.y, co[8] = a_j.z;
     }
-    if (LAY == kCone) {
+    if (LAY == kCone || LAY == kVisc) {
         for (int d = 0; d < 3; ++d) {
             const d3 e = mk(d == 0, d == 1, d == 2);
             d3 ki = (ai && d < DIM) ? mat_mult_t(Ri, cross3(ri, e)) : mk(0, 0, 0);
@@ -174,7 +174,26 @@ __global__ void __launch_bounds__(kBlock) k_assemble(Particles P, Contacts C, in
             co[3 + 3 * d] = ki.x, co[4 + 3 * d] = ki.y, co[5 + 3 * d] = ki.z;
             co[12 + 3 * d] = kj.x, co[13 + 3 * d] = kj.y, co[14 + 3 * d] = kj.z;
         }
+    }
+    if (LAY == kCone) {
         for (int d = 0; d < 3; ++d) cr[d] = (d < DIM) ? Cv[d] : 0.0;
+    } else if (LAY == kVisc) {
+        // global2local of C (lagrange_multiplier.hpp:143-163, 403-434) for the contact's kind
+        double cl = 0.0;
+        cl += Cv[0] * n.x;
+        cl += Cv[1] * n.y;
+        if (DIM == 3) cl += Cv[2] * n.z;
+        const int kd = B64.kind[c] & kKindMask;
+        for (int d = 0; d < NR; ++d) cr[d] = 0.0;
+        if (kd == 0) {
+            cr[0] = cl;
+        } else if (kd == 1) {
+            cr[0] = cl;
+            cr[1] = -cl;
+        } else {
+            cr[0] = -cl;
+            for (int d = 0; d < DIM; ++d) cr[1 + d] = Cv[d];
+        }
     } else {
         double cl = 0.0;
         cl += Cv[0] * n.x;
@@ -248,7 +267,7 @@ __global__ void __launch_bounds__(kBlock) k_transpose(BMatrix<T> B, int64_t na, 
             for (int k = 0; k < 3; ++k) B.tcoef[e * NT + k] = B.coef[(int64_t)c * NC + k];
         if (LAY == kRigid)
             for (int k = 0; k < 6; ++k) B.tcoef[e * NT + k] = B.coef[(int64_t)c * NC + k];
-        if (LAY == kCone)
+        if (LAY == kCone || LAY == kVisc)
             for (int k = 0; k < 9; ++k) B.tcoef[e * NT + k] = B.coef[(int64_t)c * NC + 3 + k];
     }
     for (int m = js[p]; m < je[p]; ++m, ++e) {
@@ -260,7 +279,7 @@ __global__ void __launch_bounds__(kBlock) k_transpose(BMatrix<T> B, int64_t na, 
             for (int k = 0; k < 3; ++k) B.tcoef[e * NT + k] = -B.coef[(int64_t)c * NC + k];
             for (int k = 0; k < 3; ++k) B.tcoef[e * NT + 3 + k] = -B.coef[(int64_t)c * NC + 6 + k];
         }
-        if (LAY == kCone)
+        if (LAY == kCone || LAY == kVisc)
             for (int k = 0; k < 9; ++k) B.tcoef[e * NT + k] = -B.coef[(int64_t)c * NC + 12 + k];
     }
 }
@@ -269,6 +288,68 @@ __global__ void __launch_bounds__(kBlock) k_transpose(BMatrix<T> B, int64_t na, 
 template <class T>
 __device__ inline T* slot(T* base, int s, int64_t stride) {
     return base + (int64_t)s * stride;
+}
+
+// kVisc: the global multiplier (3 rows) of contact c from its 4 local slots (local2global,
+// lagrange_multiplier.hpp:165-184, 436-464)
+template <class T>
+__device__ inline void visc_global(const BMatrix<T>& B, const T* __restrict__ lam, int64_t c, T& g0, T& g1, T& g2) {
+    constexpr int NC = lay<kVisc>::NC;
+    const int kd = B.kind[c] & kKindMask;
+    const T* x = lam + 4 * c;
+    const T nx = B.coef[c * NC], ny = B.coef[c * NC + 1], nz = B.coef[c * NC + 2];
+    if (kd == 2) {
+        const T m = -x[0];
+        g0 = m * nx + x[1];
+        g1 = m * ny + x[2];
+        g2 = m * nz + x[3];
+    } else {
+        const T a = (kd == 1) ? x[0] - x[1] : x[0];
+        g0 = a * nx;
+        g1 = a * ny;
+        g2 = a * nz;
+    }
+}
+
+// kVisc: the 4 local rows of a global vector q at contact c (global2local,
+// lagrange_multiplier.hpp:143-163, 403-434)
+template <class T>
+__device__ inline void visc_local(const BMatrix<T>& B, int64_t c, const T (&q)[3], T (&out)[4]) {
+    constexpr int NC = lay<kVisc>::NC;
+    const int kd = B.kind[c] & kKindMask;
+    T dn = q[0] * B.coef[c * NC];
+    dn += q[1] * B.coef[c * NC + 1];
+    dn += q[2] * B.coef[c * NC + 2];
+    out[0] = out[1] = out[2] = out[3] = (T)0;
+    if (kd == 0) {
+        out[0] = dn;
+    } else if (kd == 1) {
+        out[0] = dn;
+        out[1] = -dn;
+    } else {
+        out[0] = -dn;
+        out[1] = q[0], out[2] = q[1], out[3] = q[2];
+    }
+}
+
+// Coulomb-cone projection of f (lagrange_multiplier.hpp:243-268, 500-519)
+template <class T>
+__device__ inline void cone_project(T (&f)[3], T nx, T ny, T nz, T mu) {
+    const T ln = f[0] * nx + f[1] * ny + f[2] * nz;
+    T tx = f[0] - ln * nx, ty = f[1] - ln * ny, tz = f[2] - ln * nz;
+    const T norm = sqrt(tx * tx + ty * ty + tz * tz);
+    if (norm > mu * ln) {
+        if (ln <= -mu * norm) {
+            f[0] = f[1] = f[2] = (T)0;
+        } else {
+            const T nn = (mu * mu) * (norm + ln / mu) / (mu * mu + (T)1);
+            const T nl = nn / mu;
+            tx /= norm, ty /= norm, tz /= norm;
+            f[0] = nn * tx + nl * nx;
+            f[1] = nn * ty + nl * ny;
+            f[2] = nn * tz + nl * nz;
+        }
+    }
 }
 
 // Trial point: y, Qy and grad f(y) from (lambda_k, lambda_{k-1}) and their Q
@@ -418,7 +499,31 @@ __global__ void __launch_bounds__(kBlock) k_trial(const ApgdCtrl* __restrict__ i
                     g[d] = qy[d] + cc[o];
                     z[d] = y[d] - alpha * g[d];
                 }
-                if (LAY != kCone) {
+                if (LAY == kVisc) {
+                    const int kd = B.kind[c] & kKindMask;
+                    if (kd == 2) {
+                        // ViscousFriction at gamma_min (lagrange_multiplier.hpp:495-539): the nearer of
+                        // (0, Proj_cone(f)) and (max(x-, 0), 0)
+                        T f[3] = {z[1], z[2], z[3]};
+                        cone_project<T>(f, B.coef[c * NC], B.coef[c * NC + 1], B.coef[c * NC + 2], B.mu[c]);
+                        const T zm = z[0] > (T)0 ? z[0] : (T)0;
+                        T df = z[0] * z[0], dm = (zm - z[0]) * (zm - z[0]);
+                        for (int d = 0; d < 3; ++d) {
+                            df += (f[d] - z[1 + d]) * (f[d] - z[1 + d]);
+                            dm += z[1 + d] * z[1 + d];
+                        }
+                        if (sqrt(df) < sqrt(dm)) {
+                            z[0] = (T)0, z[1] = f[0], z[2] = f[1], z[3] = f[2];
+                        } else {
+                            z[0] = zm, z[1] = z[2] = z[3] = (T)0;
+                        }
+                    } else {
+                        // max(lambda, 0) on the contact's 1 or 2 rows (lagrange_multiplier.hpp:197-201, 487-493)
+                        z[0] = z[0] > (T)0 ? z[0] : (T)0;
+                        z[1] = (kd == 1 && z[1] > (T)0) ? z[1] : (T)0;
+                        z[2] = z[3] = (T)0;
+                    }
+                } else if (LAY != kCone) {
                     z[0] = z[0] > (T)0 ? z[0] : (T)0;  // max(lambda, 0): lagrange_multiplier.hpp:104-107
                 } else {
                     // Coulomb-cone projection: lagrange_multiplier.hpp:243-268
@@ -527,7 +632,7 @@ __global__ void __launch_bounds__(kBlock) k_bt(ApgdCtrl* __restrict__ ctrl, int 
         T ta[NT], tb[NT];
 #pragma unroll
         for (int k = 0; k < NT; ++k) ta[k] = tva[k], tb[k] = tvb[k];
-        if (LAY != kCone) {
+        if (LAY == kRotFree || LAY == kRigid) {
             const T la = lam[ca], lb = lam[cb];
 #pragma unroll
             for (int k = 0; k < NT; ++k) acc[k] += la * ta[k];
@@ -535,8 +640,14 @@ __global__ void __launch_bounds__(kBlock) k_bt(ApgdCtrl* __restrict__ ctrl, int 
 #pragma unroll
                 for (int k = 0; k < NT; ++k) acc[k] += lb * tb[k];
         } else {
-            const T a0 = lam[ca * 3], a1 = lam[ca * 3 + 1], a2 = lam[ca * 3 + 2];
-            const T b0 = lam[cb * 3], b1 = lam[cb * 3 + 1], b2 = lam[cb * 3 + 2];
+            T a0, a1, a2, b0, b1, b2;
+            if (LAY == kVisc) {
+                visc_global<T>(B, lam, ca, a0, a1, a2);
+                visc_global<T>(B, lam, cb, b0, b1, b2);
+            } else {
+                a0 = lam[ca * 3], a1 = lam[ca * 3 + 1], a2 = lam[ca * 3 + 2];
+                b0 = lam[cb * 3], b1 = lam[cb * 3 + 1], b2 = lam[cb * 3 + 2];
+            }
             if (csa < 0) {
                 acc[0] -= a0, acc[1] -= a1, acc[2] -= a2;
             } else {
@@ -651,9 +762,17 @@ __global__ void __launch_bounds__(kBlock) k_row(ApgdCtrl* __restrict__ ctrl, BMa
         const int64_t cn = c + 2 * cstep;
         if (cn < sb.hi) in_ = ci[cn], jn_ = cj[cn];
         if (cn + cstep < sb.hi) in1 = ci[cn + cstep], jn1 = cj[cn + cstep];
-        T q0[3], q1[3];
-        row_product<T, LAY>(B, c, i0, j0, dt2, q0);
-        row_product<T, LAY>(B, h1 ? c1 : c, i1, j1, dt2, q1);
+        T g0[3], g1[3];
+        row_product<T, LAY>(B, c, i0, j0, dt2, g0);
+        row_product<T, LAY>(B, h1 ? c1 : c, i1, j1, dt2, g1);
+        T q0[NR], q1[NR];
+        if constexpr (LAY == kVisc) {
+            visc_local<T>(B, c, g0, q0);
+            visc_local<T>(B, h1 ? c1 : c, g1, q1);
+        } else {
+#pragma unroll
+            for (int d = 0; d < NR; ++d) q0[d] = g0[d], q1[d] = g1[d];
+        }
         T l0[NR], l1[NR];
 #pragma unroll
         for (int d = 0; d < NR; ++d) l0[d] = lt[c * NR + d], l1[d] = lt[(h1 ? c1 : c) * NR + d];
@@ -682,8 +801,15 @@ __global__ void __launch_bounds__(kBlock) k_row_plain(BMatrix<T> B, int use_slot
     T* qs = slot<T>(B.qlam, use_slot, NR * B.cap);
     for (int64_t c = (int64_t)blockIdx.x * kBlock + threadIdx.x; c < nc; c += (int64_t)gridDim.x * kBlock) {
         const int64_t i = (int64_t)ci[c] - ni, j = (int64_t)cj[c] - ni;
-        T q[3];
-        row_product<T, LAY>(B, c, i, j, (T)dt2, q);
+        T g[3];
+        row_product<T, LAY>(B, c, i, j, (T)dt2, g);
+        T q[NR];
+        if constexpr (LAY == kVisc) {
+            visc_local<T>(B, c, g, q);
+        } else {
+#pragma unroll
+            for (int d = 0; d < NR; ++d) q[d] = g[d];
+        }
 #pragma unroll
         for (int d = 0; d < NR; ++d) qs[c * NR + d] = q[d];
     }
@@ -976,11 +1102,11 @@ static void balloc(BMatrix<T>& B, int64_t ccap, int64_t pcap) {
     B.pcap = pcap;
     SCOPI_HIP_CHECK(hipMalloc(&B.coef, 21 * ccap * sizeof(T)));
     SCOPI_HIP_CHECK(hipMalloc(&B.tcoef, 9 * 2 * ccap * sizeof(T)));
-    SCOPI_HIP_CHECK(hipMalloc(&B.c, 3 * ccap * sizeof(T)));
+    SCOPI_HIP_CHECK(hipMalloc(&B.c, 4 * ccap * sizeof(T)));  // NR <= 4 rows per contact
     SCOPI_HIP_CHECK(hipMalloc(&B.mu, ccap * sizeof(T)));
     SCOPI_HIP_CHECK(hipMalloc(&B.minv, 6 * pcap * sizeof(T)));
-    SCOPI_HIP_CHECK(hipMalloc(&B.lam, 3 * 3 * ccap * sizeof(T)));
-    SCOPI_HIP_CHECK(hipMalloc(&B.qlam, 3 * 3 * ccap * sizeof(T)));
+    SCOPI_HIP_CHECK(hipMalloc(&B.lam, 3 * 4 * ccap * sizeof(T)));
+    SCOPI_HIP_CHECK(hipMalloc(&B.qlam, 3 * 4 * ccap * sizeof(T)));
     SCOPI_HIP_CHECK(hipMalloc(&B.w, 6 * pcap * sizeof(T)));
 }
 
@@ -993,9 +1119,10 @@ Solver::~Solver() {
         if (p) (void)hipFree(p);
     if (h_ctrl_) (void)hipHostFree(h_ctrl_);
     for (void* p : {(void*)d_arange_, (void*)d_done_, (void*)d_buf_, (void*)d_tab_, (void*)mctrl_, (void*)fp_c0_,
-                    (void*)fp_s_, (void*)fp_partial_})
+                    (void*)fp_s_, (void*)fp_partial_, (void*)vkind_, (void*)vrows_})
         if (p) (void)hipFree(p);
     if (h_fp_partial_) (void)hipHostFree(h_fp_partial_);
+    if (h_vrows_) (void)hipHostFree(h_vrows_);
     for (void* p : {(void*)h_done_, (void*)h_tab_, (void*)h_ctrls_})
         if (p) (void)hipHostFree(p);
 }
@@ -1249,45 +1376,57 @@ void Solver::solve_phases(const Particles& P, const Contacts& C, int64_t nc, con
     SCOPI_HIP_CHECK(hipGetLastError());
 }
 
-// ------------------------------------------------------------------ FrictionFixedPoint
-// c = c0 + S with S = s_ij n_ij on the contact's first dim rows (LagrangeMultiplier<dim,
-// FrictionFixedPoint> S_Vector, lagrange_multiplier.hpp:290-297; the gradient and the objective
-// of minimization_problem.hpp:130-141 add S to Q lambda + c and lambda . S to the objective, the
-// same as solving with c + S)
+// ------------------------------------------------------------------ fixed points on s_ij
+// FrictionFixedPoint: c = c0 + S with S = s_ij n_ij on the contact's first dim rows
+// (LagrangeMultiplier<dim, FrictionFixedPoint> S_Vector, lagrange_multiplier.hpp:290-297; the
+// gradient and the objective of minimization_problem.hpp:130-141 add S to Q lambda + c and
+// lambda . S to the objective, the same as solving with c + S).
+// ViscousFriction (kVisc): S = dt mu_ij s_ij n_ij on the cone rows of the contacts of kind 2
+// (lagrange_multiplier.hpp:379-400).
+template <int LAY>
 __global__ void __launch_bounds__(kBlock) k_apply_s(BMatrix<double> B64, BMatrix<float> B32, int with32,
                                                    const double* __restrict__ c0, const double* __restrict__ sij,
-                                                   int64_t nc, int dim) {
+                                                   int64_t nc, int dim, double dt) {
 #pragma clang fp contract(off)
+    constexpr int NR = lay<LAY>::NR, NC = lay<LAY>::NC;
     const int64_t c = (int64_t)blockIdx.x * kBlock + threadIdx.x;
     if (c >= nc) return;
-    for (int d = 0; d < 3; ++d) {
-        const double v = c0[3 * c + d] + (d < dim ? sij[c] * B64.coef[c * 21 + d] : 0.0);
-        B64.c[3 * c + d] = v;
-        if (with32) B32.c[3 * c + d] = (float)v;
+    const bool k2 = LAY == kVisc && (B64.kind[c] & kKindMask) == 2;
+    for (int d = 0; d < NR; ++d) {
+        double v = c0[NR * c + d];
+        if (LAY == kCone && d < dim) v = c0[NR * c + d] + sij[c] * B64.coef[c * NC + d];
+        if (k2 && d >= 1 && d <= dim) v = c0[NR * c + d] + dt * B64.mu[c] * sij[c] * B64.coef[c * NC + d - 1];
+        B64.c[NR * c + d] = v;
+        if (with32) B32.c[NR * c + d] = (float)v;
     }
 }
 
-// s_ij = mu_ij dt (extra_steps_before_solve<FrictionFixedPoint>, OptimGradient.hpp:86-94)
-__global__ void __launch_bounds__(kBlock) k_init_s(double* __restrict__ sij, const double* __restrict__ mu, int64_t nc,
-                                                  double dt) {
+// extra_steps_before_solve (OptimGradient.hpp:86-106): s_ij = mu_ij dt (FrictionFixedPoint), or
+// 1 on the contacts at gamma == gamma_min (ViscousFriction; the others take no part)
+__global__ void __launch_bounds__(kBlock) k_init_s(double* __restrict__ sij, const double* __restrict__ mu,
+                                                  const int32_t* __restrict__ kind, int64_t nc, double dt) {
 #pragma clang fp contract(off)
     for (int64_t c = (int64_t)blockIdx.x * kBlock + threadIdx.x; c < nc; c += (int64_t)gridDim.x * kBlock)
-        sij[c] = mu[c] * dt;
+        sij[c] = kind ? ((kind[c] & kAtGammaMin) ? 1.0 : 0.0) : mu[c] * dt;
 }
 
-// extra_steps_after_solve<FrictionFixedPoint> (OptimGradient.hpp:116-166): A U at every contact
-// from the solve's velocities (the rows of B: e_d . (u_i - u_j) + K_i[d] . w_i - K_j[d] . w_j),
-// its tangential part T, s_ij = mu_ij dt |T| over the first dim components; block sums of
-// |s_new - s_old|^2 and |s_new|^2, and mu of contact 0 (the error's scale) after the sums.
+// extra_steps_after_solve (OptimGradient.hpp:116-240): A U at every contact from the solve's
+// velocities (the rows of B: e_d . (u_i - u_j) + K_i[d] . w_i - K_j[d] . w_j), its tangential
+// part T, s_ij = mu_ij dt |T| (FrictionFixedPoint) or |T| on the contacts at gamma == gamma_min
+// (ViscousFriction, `kind` given), over the first dim components; block sums of
+// |s_new - s_old|^2 and |s_new|^2 (the other ViscousFriction contacts add 0), and mu of
+// contact 0 (the FrictionFixedPoint error's scale) after the sums.
 __global__ void __launch_bounds__(kBlock) k_fixed_point_s(BMatrix<double> B, const int32_t* __restrict__ ci,
                                                          const int32_t* __restrict__ cj, int64_t ni, int64_t nc,
                                                          const double* __restrict__ u, const double* __restrict__ w,
-                                                         int64_t pcap, double dt, int dim, double* __restrict__ sij,
+                                                         int64_t pcap, double dt, int dim,
+                                                         const int32_t* __restrict__ kind, double* __restrict__ sij,
                                                          double* __restrict__ partial) {
 #pragma clang fp contract(off)
     __shared__ double lds[2][kBlock / 64];
     double acc[2] = {0.0, 0.0};
     for (int64_t c = (int64_t)blockIdx.x * kBlock + threadIdx.x; c < nc; c += (int64_t)gridDim.x * kBlock) {
+        if (kind && !(kind[c] & kAtGammaMin)) continue;
         const int64_t i = (int64_t)ci[c] - ni, j = (int64_t)cj[c] - ni;
         const double* co = B.coef + c * 21;
         double au[3];
@@ -1310,7 +1449,7 @@ __global__ void __launch_bounds__(kBlock) k_fixed_point_s(BMatrix<double> B, con
             const double t = au[d] - co[d] * an;
             t2 += t * t;
         }
-        const double s_new = B.mu[c] * dt * sqrt(t2);
+        const double s_new = kind ? sqrt(t2) : B.mu[c] * dt * sqrt(t2);
         const double ds = s_new - sij[c];
         sij[c] = s_new;
         acc[0] += ds * ds;
@@ -1324,36 +1463,76 @@ __global__ void __launch_bounds__(kBlock) k_fixed_point_s(BMatrix<double> B, con
     }
 }
 
+// Viscous problems: the kind word of every contact (kVisc) and the number of multiplier rows
+// (LagrangeMultiplier<Viscous> :125-141, <ViscousFriction> :357-378)
+__global__ void __launch_bounds__(kBlock) k_visc_kinds(const double* __restrict__ gamma, int64_t nc, double gmin,
+                                                      double gtol, int with_friction, int32_t* __restrict__ kind,
+                                                      unsigned long long* __restrict__ rows) {
+    const int64_t c = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    unsigned long long r = 0;
+    if (c < nc) {
+        const double g = gamma[c];
+        int kd = 0;
+        r = 1;
+        if (g < -gtol) {
+            kd = (with_friction && g == gmin) ? 2 : 1;
+            r = (kd == 2) ? 4 : 2;
+        }
+        kind[c] = kd | ((g == gmin) ? kAtGammaMin : 0);
+    }
+    for (int o = 32; o > 0; o >>= 1) r += __shfl_xor(r, o);
+    if ((threadIdx.x & 63) == 0 && r) atomicAdd(rows, r);
+}
+
+// update_contact_properties (OptimGradient.hpp:19-54): gamma = min(max(gamma - dt n . lambda_global,
+// gamma_min), 0), lambda_global = local2global of the final multipliers
+__global__ void __launch_bounds__(kBlock) k_visc_gamma(BMatrix<double> B, const double* __restrict__ lam, int64_t nc,
+                                                      int dim, double dt, double gmin, double* __restrict__ gamma) {
+#pragma clang fp contract(off)
+    const int64_t c = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (c >= nc) return;
+    double g[3];
+    visc_global<double>(B, lam, c, g[0], g[1], g[2]);
+    double dn = 0.0;
+    for (int d = 0; d < dim; ++d) dn += B.coef[c * 21 + d] * g[d];
+    const double v = gamma[c] - dt * dn;
+    const double m = (v < gmin) ? gmin : v;  // std::max(v, gamma_min)
+    gamma[c] = (0.0 < m) ? 0.0 : m;          // std::min(., 0.)
+}
+
 template <int LAY>
 void Solver::solve_fixed_point(const Particles& P, const Contacts& C, int64_t nc, const scopi_hip_params& prm,
                                double dt, hipStream_t s, scopi_hip_solve_info& info) {
+    constexpr int NR = lay<LAY>::NR;
     const int64_t ni = P.n_inactive;
     const bool fp32 = prm.precision == SCOPI_PRECISION_FP32_REFINE;
+    const bool visc = LAY == kVisc;  // ViscousFriction, else FrictionFixedPoint
     assemble<LAY>(P, C, nc, dt, fp32, s);
     if (nc > fp_cap_) {
         for (void* p : {(void*)fp_c0_, (void*)fp_s_})
             if (p) SCOPI_HIP_CHECK(hipFree(p));
         fp_cap_ = nc + nc / 4 + 1024;
-        SCOPI_HIP_CHECK(hipMalloc(&fp_c0_, 3 * fp_cap_ * sizeof(double)));
+        SCOPI_HIP_CHECK(hipMalloc(&fp_c0_, 4 * fp_cap_ * sizeof(double)));
         SCOPI_HIP_CHECK(hipMalloc(&fp_s_, fp_cap_ * sizeof(double)));
     }
     if (!fp_partial_) {
         SCOPI_HIP_CHECK(hipMalloc(&fp_partial_, (2 * 1024 + 1) * sizeof(double)));
         SCOPI_HIP_CHECK(hipHostMalloc(&h_fp_partial_, (2 * 1024 + 1) * sizeof(double)));
     }
-    SCOPI_HIP_CHECK(hipMemcpyAsync(fp_c0_, b64_.c, 3 * nc * sizeof(double), hipMemcpyDeviceToDevice, s));
-    hipLaunchKernelGGL(k_init_s, dim3(256), dim3(kBlock), 0, s, fp_s_, (const double*)b64_.mu, nc, dt);
+    SCOPI_HIP_CHECK(hipMemcpyAsync(fp_c0_, b64_.c, NR * nc * sizeof(double), hipMemcpyDeviceToDevice, s));
+    const int32_t* kind = visc ? (const int32_t*)vkind_ : nullptr;
+    hipLaunchKernelGGL(k_init_s, dim3(256), dim3(kBlock), 0, s, fp_s_, (const double*)b64_.mu, kind, nc, dt);
     const unsigned gp = (unsigned)std::min<int64_t>(grid_for(nc), 1024);
     BMatrix<float> b32 = fp32 ? b32_ : BMatrix<float>{};
     info.outer_iterations = 0;
     for (;;) {
-        hipLaunchKernelGGL(k_apply_s, dim3(grid_for(nc)), dim3(kBlock), 0, s, b64_, b32, fp32 ? 1 : 0,
-                           (const double*)fp_c0_, (const double*)fp_s_, nc, P.dim);
+        hipLaunchKernelGGL(k_apply_s<LAY>, dim3(grid_for(nc)), dim3(kBlock), 0, s, b64_, b32, fp32 ? 1 : 0,
+                           (const double*)fp_c0_, (const double*)fp_s_, nc, P.dim, dt);
         solve_phases<LAY>(P, C, nc, prm, dt, s, info);
         info.outer_iterations += 1;
         hipLaunchKernelGGL(k_fixed_point_s, dim3(gp), dim3(kBlock), 0, s, b64_, (const int32_t*)C.i,
                            (const int32_t*)C.j, ni, nc, (const double*)u_, (const double*)w_, pcap_, dt, P.dim,
-                           fp_s_, fp_partial_);
+                           kind, fp_s_, fp_partial_);
         SCOPI_HIP_CHECK(
             hipMemcpyAsync(h_fp_partial_, fp_partial_, (2 * gp + 1) * sizeof(double), hipMemcpyDeviceToHost, s));
         SCOPI_HIP_CHECK(hipStreamSynchronize(s));
@@ -1362,9 +1541,11 @@ void Solver::solve_fixed_point(const Particles& P, const Contacts& C, int64_t nc
             ds2 += h_fp_partial_[2 * b];
             s2 += h_fp_partial_[2 * b + 1];
         }
-        // err = (1 / (mu_0 dt)) |s_new - s_old| / (1 + (1 / (mu_0 dt)) |s_new|)  (OptimGradient.hpp:157-158)
+        // FrictionFixedPoint: err = (1 / (mu_0 dt)) |s_new - s_old| / (1 + (1 / (mu_0 dt)) |s_new|)
+        // (OptimGradient.hpp:157-158); ViscousFriction: |s_new - s_old| / (1 + |s_new|) (:216)
         const double mudt = h_fp_partial_[2 * gp] * dt;
-        const double err = (1 / mudt) * std::sqrt(ds2) / (1 + (1 / mudt) * std::sqrt(s2));
+        const double err = visc ? std::sqrt(ds2) / (1 + std::sqrt(s2))
+                                : (1 / mudt) * std::sqrt(ds2) / (1 + (1 / mudt) * std::sqrt(s2));
         info.fixed_point_error = err;
         if (err < prm.fixed_point_tol || info.outer_iterations >= prm.fixed_point_max_iter) break;
     }
@@ -1389,25 +1570,49 @@ int Solver::set_tuning(int key, int value) {
     return SCOPI_HIP_ERR_ARG;
 }
 
+int64_t Solver::viscous_kinds(const double* gamma, int64_t nc, const scopi_hip_params& prm, hipStream_t s) {
+    if (nc > vkind_cap_) {
+        if (vkind_) SCOPI_HIP_CHECK(hipFree(vkind_));
+        vkind_cap_ = nc + nc / 4 + 1024;
+        SCOPI_HIP_CHECK(hipMalloc(&vkind_, vkind_cap_ * sizeof(int32_t)));
+    }
+    if (!vrows_) {
+        SCOPI_HIP_CHECK(hipMalloc(&vrows_, sizeof(unsigned long long)));
+        SCOPI_HIP_CHECK(hipHostMalloc(&h_vrows_, sizeof(unsigned long long)));
+    }
+    hipLaunchKernelGGL(k_zero<unsigned long long>, dim3(1), dim3(kBlock), 0, s, vrows_, (int64_t)1);
+    hipLaunchKernelGGL(k_visc_kinds, dim3(grid_for(nc)), dim3(kBlock), 0, s, gamma, nc, prm.gamma_min, prm.gamma_tol,
+                       prm.problem == SCOPI_PROBLEM_VISCOUS_FRICTION ? 1 : 0, vkind_, vrows_);
+    SCOPI_HIP_CHECK(hipMemcpyAsync(h_vrows_, vrows_, sizeof(unsigned long long), hipMemcpyDeviceToHost, s));
+    SCOPI_HIP_CHECK(hipStreamSynchronize(s));
+    b64_.kind = vkind_;
+    b32_.kind = vkind_;
+    return (int64_t)*h_vrows_;
+}
+
 int Solver::solve(const Particles& P, const Contacts& C, int64_t nc, const scopi_hip_params& prm, double dt,
-                  bool rot_free, hipStream_t s, scopi_hip_solve_info* info) {
+                  bool rot_free, hipStream_t s, scopi_hip_solve_info* info, double* gamma) {
     const int64_t na = P.n - P.n_inactive;
     const bool fixed_point = prm.problem == SCOPI_PROBLEM_FRICTION_FIXED_POINT;
     const bool friction = prm.problem == SCOPI_PROBLEM_FRICTION || fixed_point;
+    const bool viscous = prm.problem == SCOPI_PROBLEM_VISCOUS || prm.problem == SCOPI_PROBLEM_VISCOUS_FRICTION;
     const bool fp32 = prm.precision == SCOPI_PRECISION_FP32_REFINE;
-    if (prm.problem != SCOPI_PROBLEM_NOFRICTION && !friction) return SCOPI_HIP_ERR_UNSUPPORTED;
+    if (prm.problem != SCOPI_PROBLEM_NOFRICTION && !friction && !viscous) return SCOPI_HIP_ERR_UNSUPPORTED;
     if (prm.method != SCOPI_METHOD_APGD && prm.method != SCOPI_METHOD_PGD) return SCOPI_HIP_ERR_UNSUPPORTED;
     if (!(prm.alpha > 0)) return SCOPI_HIP_ERR_ARG;
     if (fixed_point && !(dt > 0)) return SCOPI_HIP_ERR_ARG;
+    if (viscous && nc > 0 && !gamma) return SCOPI_HIP_ERR_ARG;
     reserve(na, nc, fp32);
-    layout_ = friction ? kCone : ((rot_free && !prm.general_jacobian) ? kRotFree : kRigid);
-    nr_ = friction ? 3 : 1;
+    layout_ = viscous ? kVisc : friction ? kCone : ((rot_free && !prm.general_jacobian) ? kRotFree : kRigid);
+    nr_ = viscous ? 4 : friction ? 3 : 1;
+    last_problem_ = prm.problem;
     nc_ = nc;
     rows_ = nr_ * nc;
     scopi_hip_solve_info loc{};
     loc.n_contacts = nc;
-    loc.rows = rows_;
     loc.converged = 1;
+    if (viscous && nc > 0) rows_ = viscous_kinds(gamma, nc, prm, s);
+    loc.rows = (nc == 0) ? 0 : rows_;
     if (nc == 0 || na == 0) {
         hipLaunchKernelGGL(k_velocities<6>, dim3(grid_for(na)), dim3(kBlock), 0, s, P, (const double*)nullptr, pcap_, dt,
                            0, u_, w_);
@@ -1418,12 +1623,22 @@ int Solver::solve(const Particles& P, const Contacts& C, int64_t nc, const scopi
     }
     if (fixed_point)
         solve_fixed_point<kCone>(P, C, nc, prm, dt, s, loc);
+    else if (prm.problem == SCOPI_PROBLEM_VISCOUS_FRICTION)
+        solve_fixed_point<kVisc>(P, C, nc, prm, dt, s, loc);
+    else if (viscous)
+        solve_layout<kVisc>(P, C, nc, prm, dt, s, loc);
     else if (layout_ == kCone)
         solve_layout<kCone>(P, C, nc, prm, dt, s, loc);
     else if (layout_ == kRigid)
         solve_layout<kRigid>(P, C, nc, prm, dt, s, loc);
     else
         solve_layout<kRotFree>(P, C, nc, prm, dt, s, loc);
+    if (viscous) {
+        const double* lam = b64_.lam + (int64_t)final_slot_ * nr_ * b64_.cap;
+        hipLaunchKernelGGL(k_visc_gamma, dim3(grid_for(nc)), dim3(kBlock), 0, s, b64_, lam, nc, P.dim, dt,
+                           prm.gamma_min, gamma);
+        SCOPI_HIP_CHECK(hipGetLastError());
+    }
     has_solution_ = true;
     if (info) *info = loc;
     return SCOPI_HIP_OK;
@@ -1694,9 +1909,31 @@ void Solver::download_lambda_range(double* host, int64_t lo, int64_t hi, hipStre
 void Solver::download_lambda(double* host, hipStream_t s) const {
     if (!has_solution_ || nc_ == 0) return;
     const double* lam = b64_.lam + (int64_t)final_slot_ * nr_ * b64_.cap;
-    // rows are stored per contact (c * NR + d): the reference's order
-    SCOPI_HIP_CHECK(hipMemcpyAsync(host, lam, nr_ * nc_ * sizeof(double), hipMemcpyDeviceToHost, s));
+    if (layout_ != kVisc) {
+        // rows are stored per contact (c * NR + d): the reference's order
+        SCOPI_HIP_CHECK(hipMemcpyAsync(host, lam, nr_ * nc_ * sizeof(double), hipMemcpyDeviceToHost, s));
+        SCOPI_HIP_CHECK(hipStreamSynchronize(s));
+        return;
+    }
+    // kVisc: 4 slots per contact; the reference's local order is Viscous: every contact's first
+    // row, then the second rows in contact order; ViscousFriction: each contact's 1, 2 or 4 rows
+    std::vector<double> x(4 * nc_);
+    std::vector<int32_t> kind(nc_);
+    SCOPI_HIP_CHECK(hipMemcpyAsync(x.data(), lam, 4 * nc_ * sizeof(double), hipMemcpyDeviceToHost, s));
+    SCOPI_HIP_CHECK(hipMemcpyAsync(kind.data(), vkind_, nc_ * sizeof(int32_t), hipMemcpyDeviceToHost, s));
     SCOPI_HIP_CHECK(hipStreamSynchronize(s));
+    int64_t r = 0;
+    if (last_problem_ == SCOPI_PROBLEM_VISCOUS) {
+        for (int64_t c = 0; c < nc_; ++c) host[r++] = x[4 * c];
+        for (int64_t c = 0; c < nc_; ++c)
+            if ((kind[c] & kKindMask) == 1) host[r++] = x[4 * c + 1];
+    } else {
+        for (int64_t c = 0; c < nc_; ++c) {
+            const int kd = kind[c] & kKindMask;
+            const int nrow = kd == 0 ? 1 : kd == 1 ? 2 : 4;
+            for (int d = 0; d < nrow; ++d) host[r++] = x[4 * c + d];
+        }
+    }
 }
 
 }  // namespace scopi_hip

@@ -16,7 +16,13 @@ namespace scopi_hip {
 //              a_k = R_k^T (r_k x n), r_k = p_k - x_k (9 values per contact).
 //   kCone    : Friction: rows (c, d), d = 0..2, = [ e_d | K_i[d] ] on i, -[ e_d | K_j[d] ] on j,
 //              K_k[d] = R_k^T (r_k x e_d); n is kept for the cone projection (21 values).
-enum Layout : int { kRotFree = 0, kRigid = 1, kCone = 2 };
+//   kVisc    : Viscous / ViscousFriction: B as kCone, and 4 multiplier slots per contact that
+//              hold the contact's 1, 2 or 4 rows of the reference's local vector (unused slots
+//              stay 0).  The per-contact kind says how they map to the 3 global rows:
+//                kind 0: lambda = x0 n (x0 >= 0);
+//                kind 1: lambda = (x0 - x1) n (x0, x1 >= 0);
+//                kind 2: lambda = -x0 n + (x1, x2, x3) (the nearer of x0 >= 0 and the cone).
+enum Layout : int { kRotFree = 0, kRigid = 1, kCone = 2, kVisc = 3 };
 
 template <int LAY>
 struct lay;
@@ -32,6 +38,12 @@ template <>
 struct lay<kCone> {
     static constexpr int NR = 3, NC = 21, NT = 9, NW = 6;
 };
+template <>
+struct lay<kVisc> {
+    static constexpr int NR = 4, NC = 21, NT = 9, NW = 6;
+};
+// kVisc kind word: the kind in bits 0-1, bit 2 set when gamma == gamma_min
+constexpr int kKindMask = 3, kAtGammaMin = 4;
 
 // Scalar state of the projected-gradient loop, owned by the device.  Every block of
 // the next round's trial kernel runs the reference's control logic
@@ -92,6 +104,7 @@ struct BMatrix {
     T* lam = nullptr;    // 3 slots of NR*cap
     T* qlam = nullptr;   // 3 slots of NR*cap
     T* w = nullptr;      // M^-1 B^T lambda, NW per particle
+    const int32_t* kind = nullptr;  // kVisc: kind word per contact (shared by both precisions)
 };
 
 class Solver {
@@ -100,13 +113,15 @@ class Solver {
     // Assembles B and c from the device contacts and the a-priori velocities,
     // solves, and leaves u = v + dt M^-1 B^T lambda in u() / w() (3 x pcap each).
     // rot_free: every active particle is a sphere (enables the kRotFree layout).
+    // Viscous / ViscousFriction: `gamma` (device, nc) is each contact's adhesion potential; the
+    // solve reads it for the multiplier layout and leaves the updated values in it.
     int solve(const Particles& P, const Contacts& C, int64_t nc, const scopi_hip_params& prm, double dt,
-              bool rot_free, hipStream_t s, scopi_hip_solve_info* info);
+              bool rot_free, hipStream_t s, scopi_hip_solve_info* info, double* gamma = nullptr);
     // The same for a set of independent scenes packed in P / C (scene ranges in `S`);
     // one solve info per scene.  Afterwards every scene's final lambda is in slot 0.
     int solve_scenes(const Particles& P, const Contacts& C, int64_t nc, const scopi_hip_params& prm, double dt,
                      bool rot_free, hipStream_t s, const SceneSet& S, std::vector<scopi_hip_solve_info>& info);
-    // lambda of the last solve (FP64) in the reference's order (nc, or 3 nc with index 3c + d).
+    // lambda of the last solve (FP64) in the reference's order (last_rows() values).
     void download_lambda(double* host, hipStream_t s) const;
     // rows [NR lo, NR hi) of the last solve's lambda (FP64)
     void download_lambda_range(double* host, int64_t lo, int64_t hi, hipStream_t s) const;
@@ -144,6 +159,8 @@ class Solver {
     template <int LAY>
     void solve_fixed_point(const Particles& P, const Contacts& C, int64_t nc, const scopi_hip_params& prm, double dt,
                            hipStream_t s, scopi_hip_solve_info& info);
+    // Viscous problems: kind words of the contacts and the multiplier rows they make
+    int64_t viscous_kinds(const double* gamma, int64_t nc, const scopi_hip_params& prm, hipStream_t s);
     // multi-scene loop: tables of the scenes still running, phase loop, per-scene results
     void scene_tables(const std::vector<int>& scenes, hipStream_t s);
     template <class T, int LAY>
@@ -182,6 +199,7 @@ class Solver {
     int final_slot_ = 0;
     int ctrl_cur_ = 0;  // copy of ApgdCtrl with the last phase's final state
     int layout_ = kRotFree;
+    int last_problem_ = SCOPI_PROBLEM_NOFRICTION;
     int reduce_blocks_ = 1024;  // grid of the per-contact kernels (grid-stride)
     int bt_group_ = 8;          // lanes per particle in k_bt
     int graph_rounds_ = 0;      // > 0: the one-scene loop replays a hipGraph of this many rounds
@@ -192,6 +210,11 @@ class Solver {
     double* fp_partial_ = nullptr;
     double* h_fp_partial_ = nullptr;
     int64_t fp_cap_ = 0;
+    // Viscous problems: kind words, the row count of a solve (device + pinned)
+    int32_t* vkind_ = nullptr;
+    int64_t vkind_cap_ = 0;
+    unsigned long long* vrows_ = nullptr;
+    unsigned long long* h_vrows_ = nullptr;
     // multi-scene state
     const SceneSet* scenes_ = nullptr;
     int nscene_cap_ = 0;

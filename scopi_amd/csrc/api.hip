@@ -44,6 +44,14 @@ struct scopi_hip_ctx {
     int64_t cap_flags = 0, cap_offs = 0, cap_orig = 0, cap_keys = 0, cap_vals = 0, cap_iof = 0;
     RadixSorter psort;
     Scanner pscan;
+    // Viscous problems: gamma of the current contacts (valid when gamma_valid), and the contact
+    // history of the last step for transfer: its (i, j) keys (sorted) and gamma
+    double* gamma = nullptr;
+    int64_t gamma_cap = 0;
+    bool gamma_valid = false;
+    uint64_t* hist_keys = nullptr;
+    double* hist_gamma = nullptr;
+    int64_t hist_n = 0, hist_cap = 0;
 };
 
 namespace {
@@ -360,10 +368,88 @@ int guarded(scopi_hip_ctx* ctx, F&& f) {
     }
 }
 
+bool viscous_problem(int problem) {
+    return problem == SCOPI_PROBLEM_VISCOUS || problem == SCOPI_PROBLEM_VISCOUS_FRICTION;
+}
+
+__global__ void k_fill_d(double* __restrict__ a, int64_t n, double v) {
+    for (int64_t t = (int64_t)blockIdx.x * kBlock + threadIdx.x; t < n; t += (int64_t)gridDim.x * kBlock) a[t] = v;
+}
+
+// ScopiSolver::transfer (solver.hpp:241-259): a contact whose (i, j) was a contact of the last step
+// takes that contact's gamma (the last of its kind when a pair occurs twice, as the reference's
+// map keeps the last index), any other contact the default gamma.  keys: (i << 32 | j), sorted.
+__global__ void __launch_bounds__(kBlock) k_transfer_gamma(const int32_t* __restrict__ ci, const int32_t* __restrict__ cj,
+                                                          int64_t nc, const uint64_t* __restrict__ keys,
+                                                          const double* __restrict__ hg, int64_t hn, double g0,
+                                                          double* __restrict__ gamma) {
+    const int64_t c = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (c >= nc) return;
+    const uint64_t key = ((uint64_t)(uint32_t)ci[c] << 32) | (uint32_t)cj[c];
+    int64_t lo = 0, hi = hn;  // upper bound of key
+    while (lo < hi) {
+        const int64_t mid = (lo + hi) >> 1;
+        if (keys[mid] <= key)
+            lo = mid + 1;
+        else
+            hi = mid;
+    }
+    gamma[c] = (lo > 0 && keys[lo - 1] == key) ? hg[lo - 1] : g0;
+}
+
+__global__ void __launch_bounds__(kBlock) k_record_history(const int32_t* __restrict__ ci, const int32_t* __restrict__ cj,
+                                                          int64_t nc, const double* __restrict__ gamma,
+                                                          uint64_t* __restrict__ keys, double* __restrict__ hg) {
+    const int64_t c = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (c >= nc) return;
+    keys[c] = ((uint64_t)(uint32_t)ci[c] << 32) | (uint32_t)cj[c];
+    hg[c] = gamma[c];
+}
+
+void gamma_reserve(scopi_hip_ctx* ctx, int64_t nc) {
+    if (nc <= ctx->gamma_cap) return;
+    if (ctx->gamma) SCOPI_HIP_CHECK(hipFree(ctx->gamma));
+    ctx->gamma_cap = nc + nc / 4 + 1024;
+    SCOPI_HIP_CHECK(hipMalloc(&ctx->gamma, ctx->gamma_cap * sizeof(double)));
+}
+
+// the current contacts' gamma before a solve: kept when set for these contacts, else the
+// default (from_history: the transfer of the last step's values)
+void prepare_gamma(scopi_hip_ctx* ctx, const scopi_hip_params& prm, bool from_history) {
+    const int64_t nc = ctx->nc;
+    gamma_reserve(ctx, std::max<int64_t>(nc, 1));
+    if (nc == 0 || (ctx->gamma_valid && !from_history)) return;
+    if (from_history && ctx->hist_n > 0)
+        hipLaunchKernelGGL(k_transfer_gamma, dim3(grid_for(nc)), dim3(kBlock), 0, ctx->stream,
+                           (const int32_t*)ctx->C.i, (const int32_t*)ctx->C.j, nc, (const uint64_t*)ctx->hist_keys,
+                           (const double*)ctx->hist_gamma, ctx->hist_n, prm.gamma, ctx->gamma);
+    else
+        hipLaunchKernelGGL(k_fill_d, dim3(256), dim3(kBlock), 0, ctx->stream, ctx->gamma, nc, prm.gamma);
+    SCOPI_HIP_CHECK(hipGetLastError());
+    ctx->gamma_valid = true;
+}
+
+// m_old_contacts = this step's contacts (solver.hpp:300)
+void record_history(scopi_hip_ctx* ctx) {
+    const int64_t nc = ctx->nc;
+    if (nc > ctx->hist_cap) {
+        for (void* p : {(void*)ctx->hist_keys, (void*)ctx->hist_gamma})
+            if (p) SCOPI_HIP_CHECK(hipFree(p));
+        ctx->hist_cap = nc + nc / 4 + 1024;
+        SCOPI_HIP_CHECK(hipMalloc(&ctx->hist_keys, ctx->hist_cap * sizeof(uint64_t)));
+        SCOPI_HIP_CHECK(hipMalloc(&ctx->hist_gamma, ctx->hist_cap * sizeof(double)));
+    }
+    if (nc > 0)
+        hipLaunchKernelGGL(k_record_history, dim3(grid_for(nc)), dim3(kBlock), 0, ctx->stream, (const int32_t*)ctx->C.i,
+                           (const int32_t*)ctx->C.j, nc, (const double*)ctx->gamma, ctx->hist_keys, ctx->hist_gamma);
+    SCOPI_HIP_CHECK(hipGetLastError());
+    ctx->hist_n = nc;
+}
+
 bool params_ok(const scopi_hip_params* p) {
     return p &&
            (p->problem == SCOPI_PROBLEM_NOFRICTION || p->problem == SCOPI_PROBLEM_FRICTION ||
-            p->problem == SCOPI_PROBLEM_FRICTION_FIXED_POINT) &&
+            p->problem == SCOPI_PROBLEM_FRICTION_FIXED_POINT || viscous_problem(p->problem)) &&
            (p->contact_method == SCOPI_CONTACT_KDTREE || p->contact_method == SCOPI_CONTACT_BRUTE_FORCE) &&
            (p->vap == SCOPI_VAP_FIXED || p->vap == SCOPI_VAP_FPD || p->vap == SCOPI_VAP_PROJECTION) &&
            (p->method == SCOPI_METHOD_APGD || p->method == SCOPI_METHOD_PGD) &&
@@ -491,6 +577,9 @@ int scopi_hip_default_params(scopi_hip_params* p) {
     p->general_jacobian = 0;
     p->fixed_point_tol = 1e-6;
     p->fixed_point_max_iter = 1000;
+    p->gamma = 0.0;  // contact_property<Viscous> (contact/property.hpp:125-127)
+    p->gamma_min = -2.;
+    p->gamma_tol = 1e-6;
     return SCOPI_HIP_OK;
 }
 
@@ -522,6 +611,8 @@ int scopi_hip_destroy(scopi_hip_ctx* ctx) {
     contacts_free(ctx->C);
     contacts_free(ctx->C2);
     if (ctx->proj) (void)hipFree(ctx->proj);
+    for (void* p : {(void*)ctx->gamma, (void*)ctx->hist_keys, (void*)ctx->hist_gamma})
+        if (p) (void)hipFree(p);
     for (void* p : {(void*)ctx->pflags, (void*)ctx->poffs, (void*)ctx->porig, (void*)ctx->ptotal, (void*)ctx->pkeys,
                     (void*)ctx->pvals, (void*)ctx->pi_of})
         if (p) (void)hipFree(p);
@@ -560,6 +651,7 @@ int scopi_hip_upload_state(scopi_hip_ctx* ctx, int64_t n, int64_t n_inactive, co
         if (has_se && ctx->any_periodic())
             return fail(ctx, SCOPI_HIP_ERR_UNSUPPORTED, "upload_state: periodic axes with superellipsoids are not supported");
         ctx->has_se = has_se;
+        if (n != ctx->P.n || n_inactive != ctx->P.n_inactive) ctx->hist_n = 0;  // indices change meaning
         particles_reserve(ctx->P, n);
         Particles& P = ctx->P;
         P.n = n;
@@ -745,6 +837,7 @@ int scopi_hip_compute_contacts(scopi_hip_ctx* ctx, const scopi_hip_params* param
         if (!params_ok(params)) return fail(ctx, SCOPI_HIP_ERR_ARG, "compute_contacts: bad params");
         ctx->nc = contacts_step(ctx, *params);
         ctx->contacts_valid = true;
+        ctx->gamma_valid = false;
         if (n_contacts) *n_contacts = ctx->nc;
         return SCOPI_HIP_OK;
     });
@@ -809,6 +902,7 @@ int scopi_hip_upload_contacts(scopi_hip_ctx* ctx, int64_t nc, const int64_t* i, 
         }
         ctx->nc = nc;
         ctx->contacts_valid = true;
+        ctx->gamma_valid = false;
         return SCOPI_HIP_OK;
     });
 }
@@ -827,6 +921,35 @@ int scopi_hip_set_projection_velocity(scopi_hip_ctx* ctx, const double* u, const
         }
         SCOPI_HIP_CHECK(hipStreamSynchronize(ctx->stream));
         ctx->proj_n = na;
+        return SCOPI_HIP_OK;
+    });
+}
+
+int scopi_hip_set_contact_gamma(scopi_hip_ctx* ctx, const double* gamma) {
+    return guarded(ctx, [&]() -> int {
+        if (!ctx->contacts_valid) return fail(ctx, SCOPI_HIP_ERR_STATE, "set_contact_gamma: no contact list");
+        if (!gamma && ctx->nc > 0) return fail(ctx, SCOPI_HIP_ERR_ARG, "set_contact_gamma: bad argument");
+        gamma_reserve(ctx, std::max<int64_t>(ctx->nc, 1));
+        if (ctx->nc > 0) {
+            SCOPI_HIP_CHECK(
+                hipMemcpyAsync(ctx->gamma, gamma, ctx->nc * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
+            SCOPI_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+        }
+        ctx->gamma_valid = true;
+        return SCOPI_HIP_OK;
+    });
+}
+
+int scopi_hip_download_contact_gamma(scopi_hip_ctx* ctx, double* gamma) {
+    return guarded(ctx, [&]() -> int {
+        if (!ctx->contacts_valid || !ctx->gamma_valid)
+            return fail(ctx, SCOPI_HIP_ERR_STATE, "download_contact_gamma: no gamma for the current contacts");
+        if (!gamma && ctx->nc > 0) return fail(ctx, SCOPI_HIP_ERR_ARG, "download_contact_gamma: bad argument");
+        if (ctx->nc > 0) {
+            SCOPI_HIP_CHECK(
+                hipMemcpyAsync(gamma, ctx->gamma, ctx->nc * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+            SCOPI_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+        }
         return SCOPI_HIP_OK;
     });
 }
@@ -857,7 +980,10 @@ int scopi_hip_solve(scopi_hip_ctx* ctx, const scopi_hip_params* params, double d
         if (ctx->nc > 0)
             hipLaunchKernelGGL(k_fill_mu, dim3(grid_for(ctx->nc)), dim3(kBlock), 0, ctx->stream, ctx->C.mu, ctx->nc,
                                params->mu);
-        int rc = ctx->solver.solve(ctx->P, ctx->C, ctx->nc, *params, dt, ctx->all_spheres && !ctx->any_periodic(), ctx->stream, info);
+        const bool visc = viscous_problem(params->problem);
+        if (visc) prepare_gamma(ctx, *params, false);
+        int rc = ctx->solver.solve(ctx->P, ctx->C, ctx->nc, *params, dt, ctx->all_spheres && !ctx->any_periodic(),
+                                   ctx->stream, info, visc ? ctx->gamma : nullptr);
         if (rc != SCOPI_HIP_OK) return fail(ctx, rc, "solve: unsupported option");
         SCOPI_HIP_CHECK(hipStreamSynchronize(ctx->stream));
         return SCOPI_HIP_OK;
@@ -909,17 +1035,22 @@ int scopi_hip_run(scopi_hip_ctx* ctx, const scopi_hip_params* params, double dt,
         hipStream_t s = ctx->stream;
         Particles& P = ctx->P;
         const int64_t na = P.n - P.n_inactive;
+        const bool visc = viscous_problem(params->problem);
         for (int64_t nite = initial_iter; nite < total_it; ++nite) {
             if (P.n_inactive > 0)
                 hipLaunchKernelGGL(k_obstacles, dim3(grid_for(P.n_inactive)), dim3(kBlock), 0, s, P, dt);
             ctx->nc = contacts_step(ctx, *params);
             ctx->contacts_valid = true;
+            ctx->gamma_valid = false;
+            if (visc) prepare_gamma(ctx, *params, true);
             if (na > 0)
                 hipLaunchKernelGGL(k_vap, dim3(grid_for(na)), dim3(kBlock), 0, s, P, params->vap, dt,
                                    (const double*)ctx->proj, (const double*)(ctx->proj ? ctx->proj + 3 * na : nullptr));
             scopi_hip_solve_info one{};
-            int rc = ctx->solver.solve(P, ctx->C, ctx->nc, *params, dt, ctx->all_spheres && !ctx->any_periodic(), s, &one);
+            int rc = ctx->solver.solve(P, ctx->C, ctx->nc, *params, dt, ctx->all_spheres && !ctx->any_periodic(), s, &one,
+                                       visc ? ctx->gamma : nullptr);
             if (rc != SCOPI_HIP_OK) return fail(ctx, rc, "run: solve failed");
+            if (visc) record_history(ctx);
             if (na > 0)
                 hipLaunchKernelGGL(k_update_move, dim3(grid_for(na)), dim3(kBlock), 0, s, P, ctx->solver.u(),
                                    ctx->solver.w(), ctx->solver.pcap(), dt);
@@ -1184,7 +1315,8 @@ extern "C" {
 
 int scopi_hip_batch_create(const scopi_hip_params* params, double dt, scopi_hip_batch** out) {
     if (!out || !params_ok(params) || !(dt > 0) || params->vap == SCOPI_VAP_PROJECTION) return SCOPI_HIP_ERR_ARG;
-    if (params->problem == SCOPI_PROBLEM_FRICTION_FIXED_POINT) return SCOPI_HIP_ERR_UNSUPPORTED;
+    if (params->problem == SCOPI_PROBLEM_FRICTION_FIXED_POINT || viscous_problem(params->problem))
+        return SCOPI_HIP_ERR_UNSUPPORTED;
     *out = nullptr;
     auto* b = new scopi_hip_batch();
     b->prm = *params;

@@ -14,7 +14,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libscopi_hip.so")
 
 SHAPE_SPHERE, SHAPE_PLANE, SHAPE_SEGMENT, SHAPE_SUPERELLIPSOID = 0, 1, 2, 3
-NOFRICTION, FRICTION, FRICTION_FIXED_POINT = 0, 1, 2
+NOFRICTION, FRICTION, FRICTION_FIXED_POINT, VISCOUS, VISCOUS_FRICTION = 0, 1, 2, 3, 4
 KDTREE, BRUTE_FORCE = 0, 1
 VAP_FIXED, VAP_FPD, VAP_PROJECTION = 0, 1, 2
 APGD, PGD = 0, 1
@@ -33,6 +33,7 @@ EXPORTS = [
     "scopi_hip_batch_remove_scene", "scopi_hip_batch_step", "scopi_hip_batch_size", "scopi_hip_batch_download_scene",
     "scopi_hip_batch_scene_info", "scopi_hip_batch_download_contacts", "scopi_hip_batch_download_solution",
     "scopi_hip_scene_footprint", "scopi_hip_set_box", "scopi_hip_set_projection_velocity",
+    "scopi_hip_set_contact_gamma", "scopi_hip_download_contact_gamma",
 ]
 TUNE_REDUCE_BLOCKS, TUNE_BT_GROUP, TUNE_GRAPH_ROUNDS = 1, 2, 3
 
@@ -44,6 +45,7 @@ class Params(C.Structure):
         ("kd_tree_radius", C.c_double), ("alpha", C.c_double), ("tolerance", C.c_double), ("mu", C.c_double),
         ("max_ite", C.c_int64), ("fp32_tolerance", C.c_double), ("warm_start", C.c_int32),
         ("general_jacobian", C.c_int32), ("fixed_point_tol", C.c_double), ("fixed_point_max_iter", C.c_int64),
+        ("gamma", C.c_double), ("gamma_min", C.c_double), ("gamma_tol", C.c_double),
     ]
 
 
@@ -105,6 +107,8 @@ def lib():
                                            [D] * 4 + [C.c_void_p] * 4)
     L.scopi_hip_set_box.argtypes = [P, D, D, I32]
     L.scopi_hip_set_projection_velocity.argtypes = [P, D, D]
+    L.scopi_hip_set_contact_gamma.argtypes = [P, Dn]
+    L.scopi_hip_download_contact_gamma.argtypes = [P, Dn]
     L.scopi_hip_batch_create.argtypes = [C.POINTER(Params), C.c_double, C.POINTER(P)]
     L.scopi_hip_batch_destroy.argtypes = [P]
     L.scopi_hip_batch_last_error.argtypes = [P]
@@ -153,6 +157,7 @@ class HipScene:
         self.dim = scene.dim
         self.n = scene.n
         self.n_inactive = scene.n_inactive
+        self.nc = 0
         self.upload(scene)
 
     def close(self):
@@ -223,6 +228,16 @@ class HipScene:
         self._check(self._L.scopi_hip_solve(self._h, C.byref(prm), float(dt), C.byref(info)), "solve")
         self.last_rows = info.rows
         return info.as_dict()
+
+    def set_contact_gamma(self, gamma):
+        """gamma of every current contact (Viscous problems), in the contact list's order."""
+        g = np.ascontiguousarray(np.broadcast_to(np.asarray(gamma, np.float64), (max(self.nc, 1),)))
+        self._check(self._L.scopi_hip_set_contact_gamma(self._h, _ptr(g)), "set_contact_gamma")
+
+    def contact_gamma(self):
+        g = np.zeros(max(self.nc, 1))
+        self._check(self._L.scopi_hip_download_contact_gamma(self._h, _ptr(g)), "download_contact_gamma")
+        return g[: self.nc]
 
     def solution(self):
         na = self.n - self.n_inactive

@@ -12,8 +12,9 @@
 //   OptimGradient<apgd | pgd> (apgd_params: alpha, max_ite, tolerance, dynamic_descent)
 //                                                                          include/scopi/solvers/apgd.hpp:20-124
 //   vap_fixed / vap_fpd                                                    include/scopi/vap/*.hpp
-//   NoFriction / Friction / FrictionFixedPoint, contact_property<...>       include/scopi/contact/property.hpp
-//     (mu; fixed_point_tol, fixed_point_max_iter)
+//   NoFriction / Friction / FrictionFixedPoint / Viscous / ViscousFriction, contact_property<...>
+//     (mu; fixed_point_tol, fixed_point_max_iter; gamma, gamma_min, gamma_tol)
+//                                                                          include/scopi/contact/property.hpp
 //   neighbor<dim, problem_t>                                               include/scopi/objects/neighbor.hpp:23-70
 // Every slot runs on the GPU through the C ABI (include/scopi_hip.h).  When the
 // contact, optim and vap slots are the ones defined here, ScopiSolver::run keeps
@@ -83,6 +84,8 @@ inline type::quaternion_t quaternion(double angle = 0.0) { return quaternion(ang
 struct NoFriction {};
 struct Friction {};
 struct FrictionFixedPoint {};
+struct Viscous {};
+struct ViscousFriction {};
 
 template <class problem_t>
 struct contact_property {};
@@ -98,11 +101,40 @@ struct contact_property<FrictionFixedPoint> {
     double fixed_point_tol = 1e-6;
     double fixed_point_max_iter = 1000;
 };
+// contact_property<Viscous> (contact/property.hpp:100-128): the adhesion potential gamma of the
+// contact, its lower bound and the tolerance of gamma < -gamma_tol
+template <>
+struct contact_property<Viscous> {
+    double gamma = 0;
+    double gamma_min = -2.;
+    double gamma_tol = 1e-6;
+};
+// contact_property<ViscousFriction> (contact/property.hpp:151-175): Viscous, Friction and FixedPoint
+template <>
+struct contact_property<ViscousFriction> {
+    double gamma = 0;
+    double gamma_min = -2.;
+    double gamma_tol = 1e-6;
+    double mu = 0.1;
+    double fixed_point_tol = 1e-6;
+    double fixed_point_max_iter = 1000;
+};
 
 // problems whose contacts carry a friction coefficient
 template <class problem_t>
 constexpr bool has_mu() {
-    return std::is_same<problem_t, Friction>::value || std::is_same<problem_t, FrictionFixedPoint>::value;
+    return std::is_same<problem_t, Friction>::value || std::is_same<problem_t, FrictionFixedPoint>::value ||
+           std::is_same<problem_t, ViscousFriction>::value;
+}
+// problems whose contacts carry an adhesion potential gamma
+template <class problem_t>
+constexpr bool has_gamma() {
+    return std::is_same<problem_t, Viscous>::value || std::is_same<problem_t, ViscousFriction>::value;
+}
+// problems with an outer fixed point (fixed_point_tol, fixed_point_max_iter)
+template <class problem_t>
+constexpr bool has_fixed_point() {
+    return std::is_same<problem_t, FrictionFixedPoint>::value || std::is_same<problem_t, ViscousFriction>::value;
 }
 
 template <std::size_t dim, class problem_t_>
@@ -375,10 +407,36 @@ inline output::value property_json(const contact_property<Friction>& c) {  // co
 }
 // The reference stops the outer loop when its count (1, 2, ...) reaches the double
 // fixed_point_max_iter; the device takes the same bound as the smallest such count.
-inline void set_fixed_point(scopi_hip_params& prm, const contact_property<FrictionFixedPoint>& c) {
+template <class P>
+inline void set_fixed_point(scopi_hip_params& prm, const P& c) {
     prm.fixed_point_tol = c.fixed_point_tol;
     const double m = std::ceil(c.fixed_point_max_iter);
     prm.fixed_point_max_iter = !(m > 1.0) ? 1 : m > 1e15 ? (int64_t)1e15 : (int64_t)m;
+}
+template <class P>
+inline void set_gamma(scopi_hip_params& prm, const P& c) {
+    prm.gamma = c.gamma;
+    prm.gamma_min = c.gamma_min;
+    prm.gamma_tol = c.gamma_tol;
+}
+inline output::value property_json(const contact_property<Viscous>& c) {  // contact/property.hpp:114-123
+    output::value p;
+    p["type"] = "viscous";
+    p["gamma"] = c.gamma;
+    p["gamma_min"] = c.gamma_min;
+    p["gamma_tol"] = c.gamma_tol;
+    return p;
+}
+inline output::value property_json(const contact_property<ViscousFriction>& c) {  // contact/property.hpp:162-174
+    output::value p;
+    p["type"] = "viscous friction";
+    p["gamma"] = c.gamma;
+    p["gamma_min"] = c.gamma_min;
+    p["gamma_tol"] = c.gamma_tol;
+    p["mu"] = c.mu;
+    p["fixed_point_tol"] = c.fixed_point_tol;
+    p["fixed_point_max_iter"] = c.fixed_point_max_iter;
+    return p;
 }
 inline output::value property_json(const contact_property<FrictionFixedPoint>& c) {  // contact/property.hpp:140-148
     output::value p;
@@ -640,6 +698,8 @@ template <class problem_t>
 constexpr int problem_tag() {
     return std::is_same<problem_t, Friction>::value             ? SCOPI_PROBLEM_FRICTION
            : std::is_same<problem_t, FrictionFixedPoint>::value ? SCOPI_PROBLEM_FRICTION_FIXED_POINT
+           : std::is_same<problem_t, Viscous>::value            ? SCOPI_PROBLEM_VISCOUS
+           : std::is_same<problem_t, ViscousFriction>::value    ? SCOPI_PROBLEM_VISCOUS_FRICTION
                                                                  : SCOPI_PROBLEM_NOFRICTION;
 }
 
@@ -656,8 +716,19 @@ class OptimGradient {
     bool should_solve() const { return m_should_solve; }
     template <class C, std::size_t dim>
     void extra_steps_after_solve(C&, const scopi_container<dim>&) { m_should_solve = false; }
+    // update_contact_properties (OptimGradient.hpp:19-54, 309-315): the solve already updated the
+    // gammas on the device; they come back into the contacts' properties
     template <class C>
-    void update_contact_properties(C&) {}
+    void update_contact_properties(C& contacts) {
+        using problem_t = typename C::value_type::problem_t;
+        if constexpr (has_gamma<problem_t>()) {
+            if (contacts.empty()) return;
+            std::vector<double> g(contacts.size());
+            detail::check(scopi_hip_download_contact_gamma(m_dev.ctx, g.data()), m_dev.ctx,
+                          "scopi_hip_download_contact_gamma");
+            for (std::size_t c = 0; c < contacts.size(); ++c) contacts[c].property.gamma = g[c];
+        }
+    }
 
     scopi_hip_params hip_params(int problem, double mu) const {
         scopi_hip_params prm;
@@ -696,13 +767,20 @@ class OptimGradient {
                       m_dev.ctx, "scopi_hip_upload_contacts");
         auto prm = hip_params(problem_tag<problem_t>(), 0.1);
         // the outer loop reads its controls from the first contact (OptimGradient.hpp:145-146)
-        if constexpr (std::is_same<problem_t, FrictionFixedPoint>::value)
+        if constexpr (has_fixed_point<problem_t>())
             if (nc > 0) detail::set_fixed_point(prm, contacts[0].property);
+        if constexpr (has_gamma<problem_t>()) {
+            // every contact's own gamma; gamma_min and gamma_tol of the first one (the defaults)
+            if (nc > 0) detail::set_gamma(prm, contacts[0].property);
+            std::vector<double> g(nc);
+            for (int64_t c = 0; c < nc; ++c) g[c] = contacts[c].property.gamma;
+            detail::check(scopi_hip_set_contact_gamma(m_dev.ctx, g.data()), m_dev.ctx, "scopi_hip_set_contact_gamma");
+        }
         detail::check(scopi_hip_solve(m_dev.ctx, &prm, m_dt, &m_info), m_dev.ctx, "scopi_hip_solve");
         const std::size_t na = particles.nb_active();
         m_u.assign(3 * na, 0.0);
         m_omega.assign(3 * na, 0.0);
-        m_lambda.assign(has_mu<problem_t>() ? 3 * nc : nc, 0.0);
+        m_lambda.assign(nc > 0 ? (std::size_t)m_info.rows : 0, 0.0);
         detail::check(scopi_hip_download_solution(m_dev.ctx, m_u.data(), m_omega.data(), m_lambda.data()), m_dev.ctx,
                       "scopi_hip_download_solution");
     }
@@ -909,8 +987,8 @@ class ScopiSolver {
     }
     scopi_hip_params hip_params() {
         auto prm = m_optim.hip_params(problem_tag<problem_t>(), mu());
-        if constexpr (std::is_same<problem_t, FrictionFixedPoint>::value)
-            detail::set_fixed_point(prm, m_contact.default_contact_property());
+        if constexpr (has_fixed_point<problem_t>()) detail::set_fixed_point(prm, m_contact.default_contact_property());
+        if constexpr (has_gamma<problem_t>()) detail::set_gamma(prm, m_contact.default_contact_property());
         prm.vap = vap_t::kind;
         prm.contact_method = contact_method_t::method;
         prm.dmax = m_contact.get_params().dmax;
@@ -960,7 +1038,12 @@ class ScopiSolver {
                 nb.pi[d] = pi[3 * c + d];
                 nb.pj[d] = pj[3 * c + d];
             }
-            if constexpr (has_mu<problem_t>()) nb.property = m_contact.default_contact_property();
+            if constexpr (has_mu<problem_t>() || has_gamma<problem_t>()) nb.property = m_contact.default_contact_property();
+        }
+        if constexpr (has_gamma<problem_t>()) {
+            std::vector<double> g(nc);
+            detail::check(scopi_hip_download_contact_gamma(m_ctx, g.data()), m_ctx, "scopi_hip_download_contact_gamma");
+            for (std::size_t c = 0; c < nc; ++c) out[c].property.gamma = g[c];
         }
         if (lambda) {
             lambda->assign((std::size_t)info.rows, 0.0);

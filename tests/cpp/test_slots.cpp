@@ -227,6 +227,108 @@ void fixed_point_slot() {
     std::fflush(stdout);
 }
 
+// test/test_gradient.cpp:441-503 / 574-640: the three spheres under Viscous (defaults) or
+// ViscousFriction (mu 0.5, fixed_point_tol 1e-3, gamma_min -1.4)
+template <class problem_t>
+void three_spheres_viscous() {
+    constexpr std::size_t dim = 2;
+    constexpr bool friction = std::is_same<problem_t, ViscousFriction>::value;
+    scopi_container<dim> particles;
+    sphere<dim> s1({{-1.7, 1.3}}, 0.5);
+    sphere<dim> s2({{0.5, 1.7}}, 0.5);
+    sphere<dim> s3({{4.5, 1.3}}, 0.5);
+    particles.push_back(s1, property<dim>().mass(1).moment_inertia(1).force({{1.7, -1.3}}));
+    particles.push_back(s2, property<dim>().mass(1).moment_inertia(1).force({{-0.5, -1.7}}));
+    particles.push_back(s3, property<dim>().mass(1).moment_inertia(1).force({{-4.5, -1.3}}));
+    double Tf = 7, dt = 0.1;
+    ScopiSolver<dim, problem_t, OptimGradient<apgd>, contact_kdtree, vap_fpd> solver(particles);
+    auto params = solver.get_params();
+    params.optim_params.tolerance = 1e-7;
+    params.optim_params.max_ite = 10000;
+    params.optim_params.alpha = 0.1;
+    params.optim_params.dynamic_descent = true;
+    if constexpr (friction) {
+        params.default_contact_property.mu = 0.5;
+        params.default_contact_property.fixed_point_tol = 1e-3;
+        params.default_contact_property.fixed_point_max_iter = 1000;
+        params.default_contact_property.gamma = 0;
+        params.default_contact_property.gamma_min = -1.4;
+        params.default_contact_property.gamma_tol = 1e-6;
+    }
+    std::size_t total_it = Tf / dt;
+    const char* name = friction ? "3spheres_viscous_friction" : "3spheres_viscous";
+    const auto file = reference_output(solver, "three_spheres_viscous", name, total_it);
+    solver.run(dt, total_it);
+    dump(name, particles, file);
+}
+
+// test/test_gradient.cpp:86-155 / 284-373: a sphere dropped on a plane under tilted gravity, the
+// gravity reversed between two runs of the same solver (the contacts' gamma carries over)
+template <class problem_t>
+void sphere_plane_reversed_gravity() {
+    constexpr std::size_t dim = 2;
+    constexpr bool friction = std::is_same<problem_t, ViscousFriction>::value;
+    scopi_container<dim> particles;
+    const double radius = 1., g = 1., dt = 0.01, h = 2. * radius, alpha = PI / 4;
+    plane<dim> pl({{0., 0.}}, PI / 2);
+    sphere<dim> s({{0, h}}, radius);
+    particles.push_back(pl, property<dim>().deactivate());
+    particles.push_back(s, property<dim>().mass(1).moment_inertia(1. * radius * radius / 2.).force(
+                               {{g * std::sin(alpha), -g * std::cos(alpha)}}));
+    ScopiSolver<dim, problem_t, OptimGradient<apgd>, contact_kdtree, vap_fpd> solver(particles);
+    auto params = solver.get_params();
+    params.optim_params.tolerance = 1e-7;
+    params.optim_params.max_ite = 10000;
+    params.optim_params.alpha = 0.1;
+    params.optim_params.dynamic_descent = true;
+    params.solver_params.output_frequency = std::size_t(-1);
+    if constexpr (friction) {
+        params.default_contact_property.mu = 0.5;
+        params.default_contact_property.fixed_point_tol = 1e-3;
+        params.default_contact_property.fixed_point_max_iter = 1000;
+        params.default_contact_property.gamma = 0;
+        params.default_contact_property.gamma_min = -1.4;
+        params.default_contact_property.gamma_tol = 1e-6;
+    }
+    solver.run(dt, 500);
+    particles.f()[1] = {{-g * std::sin(alpha), g * std::cos(alpha)}};
+    solver.run(dt, friction ? 100 : 500);
+    std::printf("{\"case\": \"%s\", \"objects\": [", friction ? "reversed_viscous_friction" : "reversed_viscous");
+    std::printf("{\"position\": [%.17g, %.17g], \"quaternion\": [%.17g, %.17g, %.17g, %.17g], "
+                "\"velocity\": [%.17g, %.17g], \"omega\": %.17g}]}\n",
+                particles.pos()[1][0], particles.pos()[1][1], particles.q()[1][0], particles.q()[1][1],
+                particles.q()[1][2], particles.q()[1][3], particles.v()[1][0], particles.v()[1][1], particles.omega()[1]);
+    std::fflush(stdout);
+}
+
+// The contact slot, the optim slot and update_contact_properties called one by one with Viscous:
+// a unit sphere touching a plane, pulled away at speed 1 (dt 0.1), its contact at gamma = -2 holds it
+// (u = 0) and gamma rises by the impulse to -1.
+void viscous_slot() {
+    constexpr std::size_t dim = 2;
+    scopi_container<dim> particles;
+    particles.push_back(plane<dim>({{0., 0.}}, PI / 2), property<dim>().deactivate());
+    particles.push_back(sphere<dim>({{0., 1.}}, 1.), property<dim>().mass(1).moment_inertia(0.5).velocity({{0., 1.}}));
+    ContactsParams<contact_kdtree<Viscous>> cp;
+    contact_kdtree<Viscous> cont(cp);
+    auto contacts = cont.run(particles);
+    for (auto& c : contacts) c.property.gamma = -2.;
+    OptimGradient<apgd> optim;
+    optim.set_timestep(0.1);
+    optim.get_params().tolerance = 1e-12;
+    optim.get_params().max_ite = 100000;
+    optim.extra_steps_before_solve(contacts);
+    while (optim.should_solve()) {
+        optim.run(particles, contacts, 0);
+        optim.extra_steps_after_solve(contacts, particles);
+    }
+    optim.update_contact_properties(contacts);
+    const auto& u = optim.get_uadapt();
+    std::printf("{\"case\": \"viscous_slot\", \"n\": %zu, \"u\": [%.17g, %.17g], \"gamma\": %.17g, \"rows\": %zu}\n",
+                contacts.size(), u[0], u[1], contacts[0].property.gamma, optim.lagrange_multiplier().size());
+    std::fflush(stdout);
+}
+
 void sphere_plane_friction() {
     constexpr std::size_t dim = 2;
     scopi_container<dim> particles;
@@ -345,6 +447,13 @@ int main(int argc, char** argv) {
             sphere_plane_fixed_point(0.5);
             sphere_plane_fixed_point(0.1);
             fixed_point_slot();
+        }
+        if (only.empty() || only == "viscous") {
+            three_spheres_viscous<Viscous>();
+            three_spheres_viscous<ViscousFriction>();
+            sphere_plane_reversed_gravity<Viscous>();
+            sphere_plane_reversed_gravity<ViscousFriction>();
+            viscous_slot();
         }
         if (only.empty() || only == "slots") slots_one_by_one();
         if (only.empty() || only == "superellipsoids") superellipsoids_3d();

@@ -312,3 +312,16 @@ def sphere_plane_reversed_gravity(problem, steps1, steps2, **prm):
     return dict(scene=c.arrays(), dt=0.01, steps=(steps1, steps2), vap=FPD, alpha=alpha, radius=radius, g=g, h=h,
                 reversed_force=(-g * math.sin(alpha), g * math.cos(alpha)),
                 params=dict(tolerance=1e-7, max_ite=10000, alpha=0.1, dynamic_descent=1, problem=problem, **prm))
+
+
+def pile_2d(seed=3):
+    """18 disks (r 0.049) in 3 rows over a floor plane, seeded horizontal velocities, force (0, -10)."""
+    rng = np.random.default_rng(seed)
+    c = Container(2)
+    c.plane((0.0, 0.0), PI / 2)
+    for i in range(6):
+        for j in range(3):
+            p = (0.1 * i + 0.05 + rng.uniform(-0.005, 0.005), 0.05 + 0.1 * j + rng.uniform(0, 0.005))
+            c.sphere(p, 0.049, Property(mass=1.0, moment_inertia=0.5 * 0.049 ** 2, velocity=(rng.uniform(-1, 1), 0.0),
+                                        force=(0.0, -10.0)))
+    return c.arrays()

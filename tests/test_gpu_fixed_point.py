@@ -26,21 +26,8 @@ def say(msg):
     print(msg, flush=True)
 
 
-def pile_2d(seed=3):
-    """18 disks (r 0.049) in 3 rows over a floor plane, seeded horizontal velocities, force (0, -10)."""
-    rng = np.random.default_rng(seed)
-    c = Container(2)
-    c.plane((0.0, 0.0), math.pi / 2)
-    for i in range(6):
-        for j in range(3):
-            p = (0.1 * i + 0.05 + rng.uniform(-0.005, 0.005), 0.05 + 0.1 * j + rng.uniform(0, 0.005))
-            c.sphere(p, 0.049, Property(mass=1.0, moment_inertia=0.5 * 0.049 ** 2, velocity=(rng.uniform(-1, 1), 0.0),
-                                        force=(0.0, -10.0)))
-    return c.arrays()
-
-
 CASES = {
-    "pile_2d": (pile_2d, dict(vap=hip.VAP_FPD, dmax=0.02, kd_tree_radius=0.2 ** 2), 0.01),
+    "pile_2d": (fs.pile_2d, dict(vap=hip.VAP_FPD, dmax=0.02, kd_tree_radius=0.2 ** 2), 0.01),
     "box_3d": (lambda: scenes.spheres_3d(3, 3, 2, velocity_scale=1.0),
                dict(vap=hip.VAP_FPD, dmax=0.02, kd_tree_radius=0.042 ** 2), 1e-3),
 }

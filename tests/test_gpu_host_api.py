@@ -21,7 +21,7 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 BIN = os.path.join(ROOT, "tests", "cpp", "test_slots")
 TOL = {"two_spheres_asymmetrical": 1e-6, "two_spheres_symmetrical": 1e-6, "2d_case_spheres": 1e-6,
        "obstacles_sphere_sphere_moving": 1e-6, "3spheres_nofriction": 1e-5, "sphere_plane_friction_mu01": 1e-5,
-       "3spheres_friction_fp": 1e-5}
+       "3spheres_friction_fp": 1e-5, "3spheres_viscous": 1e-5, "3spheres_viscous_friction": 1e-5}
 
 
 @pytest.fixture(scope="module")
@@ -163,3 +163,24 @@ def test_fixed_point_optim_slot_closed_form(output):
     assert d["n"] == 1 and d["outer_iterations"] > 1
     np.testing.assert_allclose(d["u"], [0.8, 0.0], atol=1e-7)
     assert abs(d["omega"] + 0.4) < 1e-7
+
+
+def test_viscous_reversed_gravity_through_solver(output):
+    """ScopiSolver<2, Viscous | ViscousFriction, ...> run, force changed, run again
+    (test_gradient.cpp:86-155, 284-373): the reference's bounds against its analytical solutions."""
+    o = output["reversed_viscous"]["objects"][0]
+    pa, va = fs.analytical_sphere_plane_viscous(math.pi / 4, 10.0, 1.0, 1.0, 2.0, -2.0, 5.0)
+    assert fs.doctest_le(np.linalg.norm(np.array(o["position"]) - pa) / np.linalg.norm(pa), 1e-3)
+    assert fs.doctest_le(np.linalg.norm(np.array(o["velocity"]) - va) / np.linalg.norm(va), 1e-3)
+    o = output["reversed_viscous_friction"]["objects"][0]
+    pa, th, va, wa = fs.analytical_sphere_plane_viscous_friction(math.pi / 4, 0.5, 6.0, 1.0, 1.0, 2.0, -1.4, 5.0)
+    assert fs.doctest_le(np.linalg.norm(np.array(o["position"]) - pa) / np.linalg.norm(pa), 1e-1)
+    assert fs.doctest_le(np.linalg.norm(np.array(o["velocity"]) - va) / np.linalg.norm(va), 1e-3)
+    assert fs.doctest_le((o["omega"] - wa) ** 2 / wa ** 2, 1e-3)
+
+
+def test_viscous_slots_closed_form(output):
+    d = output["viscous_slot"]
+    assert d["n"] == 1 and d["rows"] == 2
+    np.testing.assert_allclose(d["u"], [0.0, 0.0], atol=1e-7)
+    assert abs(d["gamma"] + 1.0) < 1e-7

@@ -204,6 +204,8 @@ FIXTURES = {
     "3spheres_nofriction": (lambda o: fs.three_spheres_nofriction(), 1e-5),
     "sphere_plane_friction_mu01": (lambda o: fs.sphere_plane_friction_mu01(), 1e-5),
     "3spheres_friction_fp": (lambda o: fs.three_spheres_friction_fp(), 1e-5),
+    "3spheres_viscous": (lambda o: fs.three_spheres_viscous(), 1e-5),
+    "3spheres_viscous_friction": (lambda o: fs.three_spheres_viscous_friction(), 1e-5),
 }
 
 
