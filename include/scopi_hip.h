@@ -120,9 +120,9 @@ const char* scopi_hip_last_error(const scopi_hip_ctx* ctx);
 
 /* Host -> device particle state, obstacles in [0, n_inactive).  Replaces the
  * container fields read by every slot (container.hpp:248-323, 475-597).
- * squareness may be NULL when no superellipsoid is present.  Active particles
- * are spheres or (3D) superellipsoids; obstacles are planes, spheres or (2D)
- * segments. */
+ * squareness may be NULL when no superellipsoid is present (a 2D superellipsoid
+ * uses its first value, e).  Active particles are spheres or superellipsoids;
+ * obstacles are planes, spheres, superellipsoids or (2D) segments. */
 int scopi_hip_upload_state(scopi_hip_ctx* ctx, int64_t n, int64_t n_inactive, const int32_t* shape, const double* pos,
                            const double* q, const double* size, const double* squareness, const double* v,
                            const double* vd, const double* omega, const double* domega, const double* f,
@@ -217,9 +217,10 @@ int scopi_hip_se_closest_points(scopi_hip_ctx* ctx, int64_t n, const double* pos
                                 double* pj, double* u0, double* u, int32_t* info, int32_t* nfev);
 
 /* closest_points_dispatcher::dispatch (closest_points.hpp:2037-2043) for n independent
- * 3D pairs of spheres, planes and superellipsoids, in the given (i, j) order: closed
- * forms, the superellipsoid pair's grid start + hybrd1, or the sphere / plane against
- * superellipsoid Newton solve (:944-1302, :1478-1768, utils.hpp:81-118).  Arrays as for
+ * pairs of spheres, planes and superellipsoids of the context's dimension, in the given
+ * (i, j) order: closed forms, the superellipsoid pair's grid start + hybrd1, or the
+ * sphere / plane against superellipsoid Newton solve (3D: :263-733, :944-1302,
+ * :1478-1768; 2D: :64-249, :1316-1464, :1782-1968; utils.hpp:81-118).  Arrays as for
  * scopi_hip_se_closest_points plus the shape tags; sq_i / sq_j may be NULL when no
  * superellipsoid is involved.  info[k]: hybrd1's info (superellipsoid pairs), Newton's
  * iteration count or -1 (sphere / plane pairs), -1 for closed forms; nfev[k]: hybrd1's

@@ -86,6 +86,7 @@ def lib():
         L.oracle_newton_closest_points.argtypes = [C.c_int32, D, D, D, D, C.c_int32, D, D, D, D, D, D, D, D, D, D, I32]
         L.oracle_newton_residual.argtypes = [C.c_int32] + [D] * 8 + [D, C.c_void_p]
         L.oracle_lu_solve4.argtypes = [D, D, D]
+        L.oracle_se2_closest_points.argtypes = [D, D, D, C.c_double, D, D, D, C.c_double, D, D, D, D, D, I32]
         L.oracle_math.argtypes = [C.c_int32, C.c_double, C.c_double, C.POINTER(C.c_double)]
         L.oracle_quaternion.argtypes = [C.c_double, D, D]
         L.oracle_gen_2d_case_spheres.argtypes = [C.c_int32, D, D, D, D]
@@ -298,3 +299,13 @@ def gen_2d_case_spheres(n=3):
     x = np.zeros(k); y = np.zeros(k); r = np.zeros(k); vel = np.zeros(k)
     L.oracle_gen_2d_case_spheres(n, x, y, r, vel)
     return x, y, r, vel
+
+
+def se2_closest_points(pos_i, q_i, size_i, e_i, pos_j, q_j, size_j, e_j):
+    """closest_points(superellipsoid<2>, superellipsoid<2>) with hybrd1's root and info."""
+    v = lambda x, n: np.ascontiguousarray(np.asarray(x, np.float64)[:n])
+    dij = np.zeros(1); nij = np.zeros(2); pi = np.zeros(2); pj = np.zeros(2); u = np.zeros(2)
+    info = np.zeros(1, np.int32)
+    lib().oracle_se2_closest_points(v(pos_i, 2), v(q_i, 4), v(size_i, 2), float(e_i), v(pos_j, 2), v(q_j, 4),
+                                    v(size_j, 2), float(e_j), dij, nij, pi, pj, u, info)
+    return dict(dij=float(dij[0]), nij=nij, pi=pi, pj=pj, u=u, info=int(info[0]))

@@ -253,6 +253,79 @@ se_oracle::Pln make_pln(const vec3& c, const quat& q) {
     return p;
 }
 
+// rotation_matrix<2> (quaternion.hpp:65-84) in the 2 x 2 block of a 3 x 3 matrix
+mat3 rotation2(const quat& q) {
+    const double x = q[0], w = q[3];
+    mat3 r{};
+    r[0] = {1 - 2 * w * w, -2 * x * w, 0.0};
+    r[1] = {2 * x * w, 1 - 2 * w * w, 0.0};
+    r[2] = {0.0, 0.0, 1.0};
+    return r;
+}
+
+// the 2D objects: superellipsoid<2>(pos, q, radius, e) keeps e = squareness[0]
+se_oracle::SE make_se2(const vec3& c, const quat& q, const vec3& r, const std::array<double, 2>& sq) {
+    se_oracle::SE s = make_se(c, q, r, sq);
+    s.R = rotation2(q);
+    return s;
+}
+se_oracle::Sph make_sph2(const vec3& c, const quat& q, const vec3& r) {
+    se_oracle::Sph s = make_sph(c, q, r);
+    s.R = rotation2(q);
+    return s;
+}
+se_oracle::Pln make_pln2(const vec3& c, const quat& q) {
+    se_oracle::Pln p = make_pln(c, q);
+    p.R = rotation2(q);
+    return p;
+}
+
+// the 2D superellipsoid pairs (closest_points.hpp:64-249, 1316-1464, 1782-1968), in the
+// reference's dispatch order: (sphere i, SE j) and (SE i, plane j) directly, (SE i, sphere j) and
+// (plane i, SE j) through the other order with nij negated and pi / pj swapped
+bool se2_pair(int si, const vec3& pi, const quat& qi, const vec3& szi, const std::array<double, 2>& sqi, int sj,
+              const vec3& pj, const quat& qj, const vec3& szj, const std::array<double, 2>& sqj, Contact& out) {
+    const bool port = g_se_math != 0;
+    bool flip = false;
+    se_oracle::NewtonResult r;
+    if (si == ORACLE_SUPERELLIPSOID && sj == ORACLE_SUPERELLIPSOID) {
+        const se_oracle::SE a = make_se2(pi, qi, szi, sqi), b = make_se2(pj, qj, szj, sqj);
+        const se_oracle::SEResult e = port ? se_oracle::se2_se2<se_oracle::PortableMath>(a, b)
+                                           : se_oracle::se2_se2<se_oracle::LibMath>(a, b);
+        out.pi = e.pi;
+        out.pj = e.pj;
+        out.nij = e.nij;
+        out.dij = e.dij;
+        return true;
+    }
+    if (si == ORACLE_SPHERE && sj == ORACLE_SUPERELLIPSOID) {
+        r = port ? se_oracle::sph2_se2<se_oracle::PortableMath>(make_sph2(pi, qi, szi), make_se2(pj, qj, szj, sqj))
+                 : se_oracle::sph2_se2<se_oracle::LibMath>(make_sph2(pi, qi, szi), make_se2(pj, qj, szj, sqj));
+    } else if (si == ORACLE_SUPERELLIPSOID && sj == ORACLE_SPHERE) {
+        r = port ? se_oracle::sph2_se2<se_oracle::PortableMath>(make_sph2(pj, qj, szj), make_se2(pi, qi, szi, sqi))
+                 : se_oracle::sph2_se2<se_oracle::LibMath>(make_sph2(pj, qj, szj), make_se2(pi, qi, szi, sqi));
+        flip = true;
+    } else if (si == ORACLE_SUPERELLIPSOID && sj == ORACLE_PLANE) {
+        r = port ? se_oracle::se2_pln2<se_oracle::PortableMath>(make_se2(pi, qi, szi, sqi), make_pln2(pj, qj))
+                 : se_oracle::se2_pln2<se_oracle::LibMath>(make_se2(pi, qi, szi, sqi), make_pln2(pj, qj));
+    } else if (si == ORACLE_PLANE && sj == ORACLE_SUPERELLIPSOID) {
+        r = port ? se_oracle::se2_pln2<se_oracle::PortableMath>(make_se2(pj, qj, szj, sqj), make_pln2(pi, qi))
+                 : se_oracle::se2_pln2<se_oracle::LibMath>(make_se2(pj, qj, szj, sqj), make_pln2(pi, qi));
+        flip = true;
+    } else {
+        return false;
+    }
+    if (flip) {
+        for (int d = 0; d < 3; ++d) r.nij[d] *= -1.;
+        std::swap(r.pi, r.pj);
+    }
+    out.pi = r.pi;
+    out.pj = r.pj;
+    out.nij = r.nij;
+    out.dij = r.dij;
+    return true;
+}
+
 // sphere / plane against a superellipsoid (3D), in the reference's dispatch order:
 // (sphere i, SE j) and (SE i, plane j) directly; (SE i, sphere j) and (plane i, SE j) through
 // the other order with nij negated and pi / pj swapped (closest_points.hpp:1295-1302, 1761-1768).
@@ -305,6 +378,8 @@ bool closest_points(int dim, int si, const vec3& pi, const quat& qi, const vec3&
         out.dij = r.dij;
         return true;
     }
+    if (dim == 2 && (si == ORACLE_SUPERELLIPSOID || sj == ORACLE_SUPERELLIPSOID))
+        return se2_pair(si, pi, qi, szi, sqi, sj, pj, qj, szj, sqj, out);
     if (si == ORACLE_SPHERE && sj == ORACLE_SPHERE) {
         out = cp_sphere_sphere(dim, pi, szi[0], pj, szj[0]);
         return true;
@@ -1514,6 +1589,27 @@ int oracle_newton_residual(int32_t kind, const double* pos_i, const double* q_i,
         else
             se_oracle::se_pln_residual<se_oracle::LibMath>(s1, p2, sp, u, F, J);
     }
+    return 0;
+}
+
+int oracle_se2_closest_points(const double* pos_i, const double* q_i, const double* size_i, double e_i,
+                              const double* pos_j, const double* q_j, const double* size_j, double e_j, double* dij,
+                              double* nij, double* pi, double* pj, double* u, int32_t* info) {
+    const std::array<double, 2> si{e_i, 1.0}, sj{e_j, 1.0};
+    const se_oracle::SE a = make_se2({pos_i[0], pos_i[1], 0.0}, {q_i[0], q_i[1], q_i[2], q_i[3]},
+                                     {size_i[0], size_i[1], 0.0}, si);
+    const se_oracle::SE b = make_se2({pos_j[0], pos_j[1], 0.0}, {q_j[0], q_j[1], q_j[2], q_j[3]},
+                                     {size_j[0], size_j[1], 0.0}, sj);
+    const se_oracle::SEResult r = g_se_math ? se_oracle::se2_se2<se_oracle::PortableMath>(a, b)
+                                            : se_oracle::se2_se2<se_oracle::LibMath>(a, b);
+    *dij = r.dij;
+    for (int d = 0; d < 2; ++d) {
+        nij[d] = r.nij[d];
+        pi[d] = r.pi[d];
+        pj[d] = r.pj[d];
+        u[d] = r.u[d];
+    }
+    *info = r.info;
     return 0;
 }
 

@@ -140,6 +140,11 @@ int oracle_newton_closest_points(int32_t shape_i, const double* pos_i, const dou
 int oracle_newton_residual(int32_t kind, const double* pos_i, const double* q_i, const double* size_i,
                            const double* sq_i, const double* pos_j, const double* q_j, const double* size_j,
                            const double* u, double* F, double* J);
+/* closest_points(superellipsoid<2>, superellipsoid<2>) with the root u (b1, b2) and hybrd1's
+ * info; 2D arrays (pos, size: 2 values; q: 4). */
+int oracle_se2_closest_points(const double* pos_i, const double* q_i, const double* size_i, double e_i,
+                              const double* pos_j, const double* q_j, const double* size_j, double e_j, double* dij,
+                              double* nij, double* pi, double* pj, double* u, int32_t* info);
 /* The 4x4 linear solve of newton_method (LU with partial pivoting), A row-major. */
 int oracle_lu_solve4(const double* A, const double* b, double* x);
 /* Residual F(u) (4) and Jacobian (4x4 row-major, may be NULL) of the SE3-SE3 system. */

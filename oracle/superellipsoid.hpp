@@ -257,9 +257,7 @@ inline void se_se_residual(const SEPair& P, const double* u, double* F, double* 
     }
 }
 
-// ------------------------------------------------------------------ MINPACK hybrd (n = 4)
-constexpr int N = 4;
-constexpr int LR = N * (N + 1) / 2;
+// ------------------------------------------------------------------ MINPACK hybrd (n = 4 in 3D, 2 in 2D)
 
 inline double enorm(int n, const double* x) {
     const double rdwarf = 3.834e-20, rgiant = 1.304e19;
@@ -293,8 +291,13 @@ inline double enorm(int n, const double* x) {
     return x3max * std::sqrt(s3);
 }
 
+// the routines below for one system size N
+template <int N>
+struct Minpack {
+static constexpr int LR = N * (N + 1) / 2;
+
 // a is column-major n x n (a[i + n*j])
-inline void qrfac(double* a, double* rdiag, double* acnorm) {
+static inline void qrfac(double* a, double* rdiag, double* acnorm) {
     for (int j = 0; j < N; ++j) {
         acnorm[j] = enorm(N, a + N * j);
         rdiag[j] = acnorm[j];
@@ -316,7 +319,7 @@ inline void qrfac(double* a, double* rdiag, double* acnorm) {
     }
 }
 
-inline void qform(double* q, double* wa) {
+static inline void qform(double* q, double* wa) {
     for (int j = 1; j < N; ++j)
         for (int i = 0; i < j; ++i) q[i + N * j] = 0.0;
     for (int l = 0; l < N; ++l) {
@@ -338,7 +341,7 @@ inline void qform(double* q, double* wa) {
 }
 
 // r: upper triangle packed by rows (r[l] with row-major order); 1-based index arithmetic kept.
-inline void dogleg(const double* r, const double* diag, const double* qtb, double delta, double* x, double* wa1,
+static inline void dogleg(const double* r, const double* diag, const double* qtb, double delta, double* x, double* wa1,
                    double* wa2) {
     const double epsmch = 2.220446049250313e-16;
     int jj = (N * (N + 1)) / 2 + 1;
@@ -410,7 +413,7 @@ inline void dogleg(const double* r, const double* diag, const double* qtb, doubl
 }
 
 // Rank-1 update of the packed triangular s (m = n = N); 1-based index arithmetic kept.
-inline void r1updt(double* s, const double* u, double* v, double* w, bool& sing) {
+static inline void r1updt(double* s, const double* u, double* v, double* w, bool& sing) {
     const double giant = 1.79769313486231570e308;
     const int m = N, n = N;
     int jj = (n * (2 * m - n + 1)) / 2 - (m - n);
@@ -486,7 +489,7 @@ inline void r1updt(double* s, const double* u, double* v, double* w, bool& sing)
 }
 
 // a is column-major m x n with leading dimension lda
-inline void r1mpyq(int m, int n, double* a, int lda, const double* v, const double* w) {
+static inline void r1mpyq(int m, int n, double* a, int lda, const double* v, const double* w) {
     for (int nmj = 1; nmj <= n - 1; ++nmj) {
         const int j = n - nmj;
         double cs, sn;
@@ -522,11 +525,12 @@ inline void r1mpyq(int m, int n, double* a, int lda, const double* v, const doub
 
 // hybrd1 (minpack.hpp:851-923) with the analytic Jacobian: returns info (1: converged).
 template <class Fn>
-inline int hybrd1(Fn fcn, double* x, int* nfev_out = nullptr, int* njev_out = nullptr) {
+static inline int hybrd1(Fn fcn, double* x, int* nfev_out = nullptr, int* njev_out = nullptr) {
     const double xtol = 1.0e-10, factor = 100.0, epsmch = 2.220446049250313e-16;
     const int maxfev = 200 * (N + 1);
     const double p1 = 0.1, p5 = 0.5, p001 = 0.001, p0001 = 0.0001;
-    double diag[N] = {1.0, 1.0, 1.0, 1.0};  // mode 2: wa[0..n) = 1 (minpack.hpp:881-884)
+    double diag[N];  // mode 2: wa[0..n) = 1 (minpack.hpp:881-884)
+    for (int k = 0; k < N; ++k) diag[k] = 1.0;
     double fvec[N], fjac[N * N], r[LR], qtf[N], wa1[N], wa2[N], wa3[N], wa4[N];
     int info = 0, nfev = 0, njev = 0;
     fcn(x, fvec, nullptr);
@@ -652,6 +656,7 @@ inline int hybrd1(Fn fcn, double* x, int* nfev_out = nullptr, int* njev_out = nu
     if (info == 5) info = 4;  // hybrd1 (minpack.hpp:918-921)
     return info;
 }
+};
 
 // closest_points(superellipsoid<3>, superellipsoid<3>): closest_points.hpp:263-733.
 // Returns the reference's (pi, pj, nij, dij); u holds the root and info hybrd's code.
@@ -708,7 +713,8 @@ inline SEResult se_se(const SE& s1, const SE& s2) {
     SEResult res;
     se_grid_start<M>(s1, s2, res.u);
     SEPair pair{s1, s2};
-    res.info = hybrd1([&](const double* u, double* F, double* J) { se_se_residual<M>(pair, u, F, J); }, res.u, &res.nfev, &res.njev);
+    res.info = Minpack<4>::hybrd1([&](const double* u, double* F, double* J) { se_se_residual<M>(pair, u, F, J); }, res.u,
+                                   &res.nfev, &res.njev);
     res.pi = se_point<M>(s1, res.u[0], res.u[1]);
     res.pj = se_point<M>(s2, res.u[2], res.u[3]);
     res.nij = se_normal<M>(s2, res.u[2], res.u[3]);
@@ -868,60 +874,67 @@ inline void se_pln_residual(const SE& s1, const Pln& p2, double sp, const double
 }
 
 // xt::linalg::solve (LAPACK gesv): LU with partial pivoting (first largest |pivot|),
-// forward and back substitution.  A is row-major 4 x 4, b is overwritten with x.
-inline void lu_solve4(double* A, double* b) {
-    int piv[4];
-    for (int k = 0; k < 4; ++k) {
+// forward and back substitution.  A is row-major N x N, b is overwritten with x.
+template <int N>
+inline void lu_solve(double* A, double* b) {
+    int piv[N];
+    for (int k = 0; k < N; ++k) {
         int p = k;
-        double best = std::abs(A[4 * k + k]);
-        for (int i = k + 1; i < 4; ++i)
-            if (std::abs(A[4 * i + k]) > best) {
-                best = std::abs(A[4 * i + k]);
+        double best = std::abs(A[N * k + k]);
+        for (int i = k + 1; i < N; ++i)
+            if (std::abs(A[N * i + k]) > best) {
+                best = std::abs(A[N * i + k]);
                 p = i;
             }
         piv[k] = p;
         if (p != k)
-            for (int j = 0; j < 4; ++j) std::swap(A[4 * k + j], A[4 * p + j]);
-        if (A[4 * k + k] != 0.0) {
-            const double inv = 1.0 / A[4 * k + k];
-            for (int i = k + 1; i < 4; ++i) A[4 * i + k] *= inv;
+            for (int j = 0; j < N; ++j) std::swap(A[N * k + j], A[N * p + j]);
+        if (A[N * k + k] != 0.0) {
+            const double inv = 1.0 / A[N * k + k];
+            for (int i = k + 1; i < N; ++i) A[N * i + k] *= inv;
         }
-        for (int i = k + 1; i < 4; ++i)
-            for (int j = k + 1; j < 4; ++j) A[4 * i + j] -= A[4 * i + k] * A[4 * k + j];
+        for (int i = k + 1; i < N; ++i)
+            for (int j = k + 1; j < N; ++j) A[N * i + j] -= A[N * i + k] * A[N * k + j];
     }
-    for (int k = 0; k < 4; ++k)
+    for (int k = 0; k < N; ++k)
         if (piv[k] != k) std::swap(b[k], b[piv[k]]);
-    for (int i = 1; i < 4; ++i)
-        for (int j = 0; j < i; ++j) b[i] -= A[4 * i + j] * b[j];
-    for (int i = 3; i >= 0; --i) {
-        for (int j = i + 1; j < 4; ++j) b[i] -= A[4 * i + j] * b[j];
-        b[i] /= A[4 * i + i];
+    for (int i = 1; i < N; ++i)
+        for (int j = 0; j < i; ++j) b[i] -= A[N * i + j] * b[j];
+    for (int i = N - 1; i >= 0; --i) {
+        for (int j = i + 1; j < N; ++j) b[i] -= A[N * i + j] * b[j];
+        b[i] /= A[N * i + i];
     }
 }
+inline void lu_solve4(double* A, double* b) { lu_solve<4>(A, b); }
 
-inline double norm4(const double* x) { return std::sqrt(((x[0] * x[0] + x[1] * x[1]) + x[2] * x[2]) + x[3] * x[3]); }
+template <int N>
+inline double normv(const double* x) {
+    double s = 0.0;
+    for (int k = 0; k < N; ++k) s += x[k] * x[k];
+    return std::sqrt(s);
+}
 
 // newton_method (utils.hpp:81-118); returns the iteration count, -1 when itermax is reached
-template <class Fn>
+template <int N, class Fn>
 inline int newton(Fn fcn, double* u, int itermax = 200, double ftol = 1.0e-10, double xtol = 1.0e-7) {
     int cc = 0;
     while (cc < itermax) {
-        double F[4], J[16], d[4];
+        double F[N], J[N * N], d[N];
         fcn(u, F, J);
-        for (int k = 0; k < 4; ++k) d[k] = -F[k];
-        lu_solve4(J, d);
-        if (norm4(d) < xtol) return cc;
+        for (int k = 0; k < N; ++k) d[k] = -F[k];
+        lu_solve<N>(J, d);
+        if (normv<N>(d) < xtol) return cc;
         double t = 1;
-        const double ferr = norm4(F);
+        const double ferr = normv<N>(F);
         if (ferr < ftol) return cc;
         for (;;) {
-            double ut[4], Ft[4];
-            for (int k = 0; k < 4; ++k) ut[k] = u[k] + t * d[k];
+            double ut[N], Ft[N];
+            for (int k = 0; k < N; ++k) ut[k] = u[k] + t * d[k];
             fcn(ut, Ft, nullptr);
-            if (!(norm4(Ft) > ferr && t > 0.01)) break;
+            if (!(normv<N>(Ft) > ferr && t > 0.01)) break;
             t -= 0.01;
         }
-        for (int k = 0; k < 4; ++k) u[k] += t * d[k];
+        for (int k = 0; k < N; ++k) u[k] += t * d[k];
         cc += 1;
     }
     return -1;
@@ -969,7 +982,7 @@ inline NewtonResult sph_se(const Sph& s2, const SE& s1) {
                     [&](double a, double b) { return sph_point<M>(s2, a, b); }, res.u0);
     for (int k = 0; k < 4; ++k) res.u[k] = res.u0[k];
     res.iterations =
-        newton([&](const double* u, double* F, double* J) { sph_se_residual<M>(s1, s2, u, F, J); }, res.u);
+        newton<4>([&](const double* u, double* F, double* J) { sph_se_residual<M>(s1, s2, u, F, J); }, res.u);
     res.pi = se_point<M>(s1, res.u[0], res.u[1]);
     res.pj = sph_point<M>(s2, res.u[2], res.u[3]);
     res.nij = sph_normal<M>(s2, res.u[2], res.u[3]);
@@ -989,12 +1002,233 @@ inline NewtonResult se_pln(const SE& s1, const Pln& p2) {
                     [&](double a, double b) { return pln_point(p2, a, b); }, res.u0);
     for (int k = 0; k < 4; ++k) res.u[k] = res.u0[k];
     res.iterations =
-        newton([&](const double* u, double* F, double* J) { se_pln_residual<M>(s1, p2, sp, u, F, J); }, res.u);
+        newton<4>([&](const double* u, double* F, double* J) { se_pln_residual<M>(s1, p2, sp, u, F, J); }, res.u);
     res.pi = se_point<M>(s1, res.u[0], res.u[1]);
     res.pj = pln_point(p2, res.u[2], res.u[3]);
     res.nij = np;
     res.dij = (res.pi[0] - res.pj[0]) * res.nij[0] + (res.pi[1] - res.pj[1]) * res.nij[1] +
               (res.pi[2] - res.pj[2]) * res.nij[2];
+    return res;
+}
+
+// ------------------------------------------------------------------ 2D: superellipsoid<2>
+// A 2D superellipsoid uses c[0..1], r[0..1], the squareness e and the 2 x 2 block of R
+// (rotation_matrix<2>); spheres and planes likewise.  The three pairs of closest_points.hpp
+// (:64-249 by hybrd1, :1316-1464 and :1782-1958 by newton_method) solve 2 x 2 systems; their
+// Jacobians (hand-written in the reference) are the exact derivatives of the residuals, which the
+// dual numbers give here.
+
+// superellipsoid<2>::point(b) / normal(b): superellipsoid.hpp:281-289, 303-312
+template <class M>
+inline v3 se2_point(const SE& s, double b) {
+    const double cb = M::cos(b), sb = M::sin(b);
+    const double x = s.r[0] * sgn(cb) * M::pow(std::abs(cb), s.e);
+    const double y = s.r[1] * sgn(sb) * M::pow(std::abs(sb), s.e);
+    return {(s.R[0][0] * x + s.R[0][1] * y) + s.c[0], (s.R[1][0] * x + s.R[1][1] * y) + s.c[1], 0.0};
+}
+template <class M>
+inline v3 se2_normal(const SE& s, double b) {
+    const double cb = M::cos(b), sb = M::sin(b);
+    const double x = s.r[1] * sgn(cb) * M::pow(std::abs(cb), 2 - s.e);
+    const double y = s.r[0] * sgn(sb) * M::pow(std::abs(sb), 2 - s.e);
+    const double n0 = s.R[0][0] * x + s.R[0][1] * y, n1 = s.R[1][0] * x + s.R[1][1] * y;
+    const double l = std::sqrt(n0 * n0 + n1 * n1);
+    return {n0 / l, n1 / l, 0.0};
+}
+// sphere<2>::point(b) / normal(b): sphere.hpp:194-221; plane<2>::point(b) = R (0, b) + c (plane.hpp:175-182)
+template <class M>
+inline v3 sph2_point(const Sph& s, double b) {
+    const double x = s.r * M::cos(b), y = s.r * M::sin(b);
+    return {(s.R[0][0] * x + s.R[0][1] * y) + s.c[0], (s.R[1][0] * x + s.R[1][1] * y) + s.c[1], 0.0};
+}
+template <class M>
+inline v3 sph2_normal(const Sph& s, double b) {
+    const double x = s.r * M::cos(b), y = s.r * M::sin(b);
+    const double n0 = s.R[0][0] * x + s.R[0][1] * y, n1 = s.R[1][0] * x + s.R[1][1] * y;
+    const double l = std::sqrt(n0 * n0 + n1 * n1);
+    return {n0 / l, n1 / l, 0.0};
+}
+inline v3 pln2_point(const Pln& p, double b) {
+    return {(p.R[0][0] * 0.0 + p.R[0][1] * b) + p.c[0], (p.R[1][0] * 0.0 + p.R[1][1] * b) + p.c[1], 0.0};
+}
+
+// one superellipsoid side in the body frame: the normal direction (r_y c^(2-e), r_x s^(2-e)) and
+// the point (r_x c^e, r_y s^e), c = cos b, s = sin b, powers keeping the sign
+template <class M>
+inline void se2_terms(const SE& s, const D4& b, D4& nx, D4& ny, D4& px, D4& py) {
+    const D4 cb = dcos<M>(b), sb = dsin<M>(b);
+    nx = s.r[1] * spow<M>(cb, 2 - s.e);
+    ny = s.r[0] * spow<M>(sb, 2 - s.e);
+    px = s.r[0] * spow<M>(cb, s.e);
+    py = s.r[1] * spow<M>(sb, s.e);
+}
+inline void put2(const D4* f, double* F, double* J) {
+    for (int i = 0; i < 2; ++i) {
+        F[i] = f[i].v;
+        if (J)
+            for (int k = 0; k < 2; ++k) J[2 * i + k] = f[i].d[k];
+    }
+}
+
+// closest_points(superellipsoid<2>, superellipsoid<2>) residual (closest_points.hpp:79-125): the
+// normal n1 of s1 at b1 is collinear with p2 - p1 (cross product 0) and anti-parallel to n2 at b2
+template <class M>
+inline void se2_se2_residual(const SE& s1, const SE& s2, const double* u, double* F, double* J) {
+    D4 n1x, n1y, p1x, p1y, n2x, n2y, p2x, p2y;
+    se2_terms<M>(s1, var(u[0], 0), n1x, n1y, p1x, p1y);
+    se2_terms<M>(s2, var(u[1], 1), n2x, n2y, p2x, p2y);
+    const auto& A = s1.R;
+    const auto& B = s2.R;
+    const D4 m1x = A[0][1] * n1y + A[0][0] * n1x, m1y = A[1][1] * n1y + A[1][0] * n1x;
+    const D4 m2x = B[0][1] * n2y + B[0][0] * n2x, m2y = B[1][1] * n2y + B[1][0] * n2x;
+    const D4 dy = (((((-(A[1][0] * p1x)) - A[1][1] * p1y) - cst(s1.c[1])) + B[1][0] * p2x) + B[1][1] * p2y) + cst(s2.c[1]);
+    const D4 dx = (((((-(A[0][0] * p1x)) - A[0][1] * p1y) - cst(s1.c[0])) + B[0][0] * p2x) + B[0][1] * p2y) + cst(s2.c[0]);
+    D4 f[2];
+    f[0] = (-m1x) * dy + m1y * dx;
+    f[1] = (m1x * m2x + m1y * m2y) + dsqrt(m1x * m1x + m1y * m1y) * dsqrt(m2x * m2x + m2y * m2y);
+    put2(f, F, J);
+}
+
+// closest_points(sphere<2> s2, superellipsoid<2> s1) residual (closest_points.hpp:1332-1366)
+template <class M>
+inline void sph2_se2_residual(const SE& s1, const Sph& s2, const double* u, double* F, double* J) {
+    D4 n1x, n1y, p1x, p1y;
+    se2_terms<M>(s1, var(u[0], 0), n1x, n1y, p1x, p1y);
+    const D4 b2 = var(u[1], 1);
+    const D4 q5 = s2.r * dsin<M>(b2), q6 = s2.r * dcos<M>(b2);
+    const auto& A = s1.R;
+    const auto& B = s2.R;
+    const D4 m1x = A[0][0] * n1x + A[0][1] * n1y, m1y = A[1][0] * n1x + A[1][1] * n1y;
+    const D4 m2x = B[0][0] * q6 + B[0][1] * q5, m2y = B[1][0] * q6 + B[1][1] * q5;
+    const D4 dy = ((((-(A[1][0] * p1x)) - A[1][1] * p1y) + B[1][0] * q6) + B[1][1] * q5 - cst(s1.c[1])) + cst(s2.c[1]);
+    const D4 dx = ((((-(A[0][0] * p1x)) - A[0][1] * p1y) + B[0][0] * q6) + B[0][1] * q5 - cst(s1.c[0])) + cst(s2.c[0]);
+    D4 f[2];
+    f[0] = (-m1x) * dy + m1y * dx;
+    f[1] = (m2x * m1x + m2y * m1y) + dsqrt(m2x * m2x + m2y * m2y) * dsqrt(m1x * m1x + m1y * m1y);
+    put2(f, F, J);
+}
+
+// closest_points(superellipsoid<2> s1, plane<2> p2) residual (closest_points.hpp:1800-1838); sp is
+// the value (c_s1 - c_p) . n_p (the reference's `sign_d2s`, not a sign: F[1] has a root only
+// when |sp| >= 1, the reason the reference's own tests of this pair are marked as failing)
+template <class M>
+inline void se2_pln2_residual(const SE& s1, const Pln& p2, double sp, const double* u, double* F, double* J) {
+    D4 n1x, n1y, p1x, p1y;
+    se2_terms<M>(s1, var(u[0], 0), n1x, n1y, p1x, p1y);
+    const D4 a2 = var(u[1], 1);
+    const auto& A = s1.R;
+    const auto& N = p2.R;
+    const D4 m1x = A[0][0] * n1x + A[0][1] * n1y, m1y = A[1][0] * n1x + A[1][1] * n1y;
+    const D4 dy = ((((-(A[1][0] * p1x)) - A[1][1] * p1y) + N[1][1] * a2) + cst(p2.c[1])) - cst(s1.c[1]);
+    const D4 dx = ((((-(A[0][0] * p1x)) - A[0][1] * p1y) + N[0][1] * a2) + cst(p2.c[0])) - cst(s1.c[0]);
+    D4 f[2];
+    f[0] = (-m1x) * dy + m1y * dx;
+    const double lp = std::sqrt(N[0][0] * N[0][0] + N[1][0] * N[1][0]);
+    f[1] = ((N[0][0] * sp) * m1x + (N[1][0] * sp) * m1y) + lp * dsqrt(m1x * m1x + m1y * m1y);
+    put2(f, F, J);
+}
+
+inline double dist2(const v3& a, const v3& b) {
+    const double dx = a[0] - b[0], dy = a[1] - b[1];
+    return std::sqrt(dx * dx + dy * dy);
+}
+
+// closest_points(superellipsoid<2>, superellipsoid<2>): closest_points.hpp:64-249.  Start: the first
+// minimum, row-major, of |p1 - p2| + 2 (1 + n1 . n2) over unique(binit_xy(4)) of each side.
+template <class M>
+inline SEResult se2_se2(const SE& s1, const SE& s2) {
+    SEResult res;
+    const std::vector<double> b1 = unique_sorted(binit_xy<M>(s1, 4)), b2 = unique_sorted(binit_xy<M>(s2, 4));
+    double dmin = INFINITY;
+    size_t imin = 0, jmin = 0;
+    for (size_t i = 0; i < b1.size(); ++i) {
+        const v3 p = se2_point<M>(s1, b1[i]), n = se2_normal<M>(s1, b1[i]);
+        for (size_t j = 0; j < b2.size(); ++j) {
+            const v3 q = se2_point<M>(s2, b2[j]), m = se2_normal<M>(s2, b2[j]);
+            const double dist = dist2(p, q) + 2 * (1 + (n[0] * m[0] + n[1] * m[1]));
+            if (dist < dmin) {
+                dmin = dist;
+                imin = i;
+                jmin = j;
+            }
+        }
+    }
+    res.u[0] = b1[imin];
+    res.u[1] = b2[jmin];
+    res.u[2] = res.u[3] = 0.0;
+    res.info = Minpack<2>::hybrd1([&](const double* u, double* F, double* J) { se2_se2_residual<M>(s1, s2, u, F, J); },
+                                  res.u, &res.nfev, &res.njev);
+    res.pi = se2_point<M>(s1, res.u[0]);
+    res.pj = se2_point<M>(s2, res.u[1]);
+    res.nij = se2_normal<M>(s2, res.u[1]);
+    const double dx = res.pi[0] - res.pj[0], dy = res.pi[1] - res.pj[1];
+    res.dij = sgn(dx * res.nij[0] + dy * res.nij[1]) * std::sqrt(dx * dx + dy * dy);
+    return res;
+}
+
+// closest_points(sphere<2> s2, superellipsoid<2> s1): closest_points.hpp:1316-1448.  Start: the
+// first minimum of |s1.point(b_i) - s2.point(b_j)| over b = {-pi, -pi/2, 0.001, pi/2}; pi on the
+// superellipsoid, pj on the sphere, nij the sphere's normal, dij = (pi - pj) . nij.
+template <class M>
+inline NewtonResult sph2_se2(const Sph& s2, const SE& s1) {
+    NewtonResult res;
+    const double pi = 4 * std::atan(1);
+    const double bi[4] = {-pi, -pi / 2, 0.001, pi / 2};
+    double dmin = INFINITY;
+    int imin = 0, jmin = 0;
+    for (int i = 0; i < 4; ++i)
+        for (int j = 0; j < 4; ++j) {
+            const double d = dist2(se2_point<M>(s1, bi[i]), sph2_point<M>(s2, bi[j]));
+            if (d < dmin) {
+                dmin = d;
+                imin = i;
+                jmin = j;
+            }
+        }
+    res.u0[0] = res.u[0] = bi[imin];
+    res.u0[1] = res.u[1] = bi[jmin];
+    res.u0[2] = res.u0[3] = res.u[2] = res.u[3] = 0.0;
+    res.iterations =
+        newton<2>([&](const double* u, double* F, double* J) { sph2_se2_residual<M>(s1, s2, u, F, J); }, res.u);
+    res.pi = se2_point<M>(s1, res.u[0]);
+    res.pj = sph2_point<M>(s2, res.u[1]);
+    res.nij = sph2_normal<M>(s2, res.u[1]);
+    res.dij = (res.pi[0] - res.pj[0]) * res.nij[0] + (res.pi[1] - res.pj[1]) * res.nij[1];
+    return res;
+}
+
+// closest_points(superellipsoid<2> s1, plane<2> p2): closest_points.hpp:1782-1949.  Start: the first
+// minimum of |s1.point(b_i) - p2.point(a_j)| over b = linspace(-pi, pi, 10), a = linspace(-10, 10, 10);
+// pi on the superellipsoid, pj on the plane, nij the plane's normal, dij = (pi - pj) . nij.
+template <class M>
+inline NewtonResult se2_pln2(const SE& s1, const Pln& p2) {
+    NewtonResult res;
+    const double pi = 4 * std::atan(1);
+    const int num = 10;
+    const v3 np = {p2.R[0][0], p2.R[1][0], 0.0};
+    const double sp = (s1.c[0] - p2.c[0]) * np[0] + (s1.c[1] - p2.c[1]) * np[1];
+    double dmin = INFINITY;
+    int imin = 0, jmin = 0;
+    for (int i = 0; i < num; ++i) {
+        const v3 p = se2_point<M>(s1, linspace(-pi, pi, num, i));
+        for (int j = 0; j < num; ++j) {
+            const double d = dist2(p, pln2_point(p2, linspace(-num, num, num, j)));
+            if (d < dmin) {
+                dmin = d;
+                imin = i;
+                jmin = j;
+            }
+        }
+    }
+    res.u0[0] = res.u[0] = linspace(-pi, pi, num, imin);
+    res.u0[1] = res.u[1] = linspace(-num, num, num, jmin);
+    res.u0[2] = res.u0[3] = res.u[2] = res.u[3] = 0.0;
+    res.iterations =
+        newton<2>([&](const double* u, double* F, double* J) { se2_pln2_residual<M>(s1, p2, sp, u, F, J); }, res.u);
+    res.pi = se2_point<M>(s1, res.u[0]);
+    res.pj = pln2_point(p2, res.u[1]);
+    res.nij = np;
+    res.dij = (res.pi[0] - res.pj[0]) * res.nij[0] + (res.pi[1] - res.pj[1]) * res.nij[1];
     return res;
 }
 

@@ -646,8 +646,6 @@ int scopi_hip_upload_state(scopi_hip_ctx* ctx, int64_t n, int64_t n_inactive, co
                 return fail(ctx, SCOPI_HIP_ERR_UNSUPPORTED, "upload_state: segments are 2D objects");
             has_se = has_se || s == SCOPI_SHAPE_SUPERELLIPSOID;
         }
-        if (has_se && ctx->dim != 3)
-            return fail(ctx, SCOPI_HIP_ERR_UNSUPPORTED, "upload_state: superellipsoids are 3D objects here");
         if (has_se && ctx->any_periodic())
             return fail(ctx, SCOPI_HIP_ERR_UNSUPPORTED, "upload_state: periodic axes with superellipsoids are not supported");
         ctx->has_se = has_se;
@@ -720,13 +718,13 @@ int pairs_batch(scopi_hip_ctx* ctx, int64_t n, const int32_t* shape_i, const dou
         for (int64_t m : {k, n + k})
             if (shape[m] != SCOPI_SHAPE_SPHERE && shape[m] != SCOPI_SHAPE_PLANE &&
                 shape[m] != SCOPI_SHAPE_SUPERELLIPSOID)
-                return fail(ctx, SCOPI_HIP_ERR_UNSUPPORTED, "closest_points: 3D spheres, planes, superellipsoids");
+                return fail(ctx, SCOPI_HIP_ERR_UNSUPPORTED, "closest_points: spheres, planes, superellipsoids");
         ci[k] = (int32_t)k;
         cj[k] = (int32_t)(n + k);
     }
     hipStream_t s = ctx->stream;
     Particles T{};
-    T.dim = 3;
+    T.dim = ctx->dim;
     SeCandidates S{};
     SeGrids G{};
     struct Cleanup {

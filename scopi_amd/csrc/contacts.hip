@@ -744,9 +744,21 @@ int64_t ContactSearch::run(const Particles& P, Contacts& C, const scopi_hip_para
     return nc;
 }
 
-// Scenes with superellipsoids (3D): candidates are every (obstacle, active) pair and the active
+// Scenes with superellipsoids (2D or 3D): candidates are every (obstacle, active) pair and the active
 // pairs passing the kd-tree predicate (every pair for brute force); closest points of every
 // candidate on the device, then the dmax filter in candidate order, which is (i, j) order.
+// the candidate-mode pair kernels for the scene's dimension and contact method
+#define SCOPI_CAND_LAUNCH(KERNEL, ...)                                                                         \
+    do {                                                                                                       \
+        if (P.dim == 3 && kd)                                                                                  \
+            hipLaunchKernelGGL((KERNEL<3, true, true>), dim3(grid_for(na)), dim3(kBlock), 0, s, __VA_ARGS__); \
+        else if (P.dim == 3)                                                                                   \
+            hipLaunchKernelGGL((KERNEL<3, false, true>), dim3(grid_for(na)), dim3(kBlock), 0, s, __VA_ARGS__);\
+        else if (kd)                                                                                           \
+            hipLaunchKernelGGL((KERNEL<2, true, true>), dim3(grid_for(na)), dim3(kBlock), 0, s, __VA_ARGS__); \
+        else                                                                                                   \
+            hipLaunchKernelGGL((KERNEL<2, false, true>), dim3(grid_for(na)), dim3(kBlock), 0, s, __VA_ARGS__);\
+    } while (0)
 int64_t ContactSearch::run_candidates(const Particles& P, Contacts& C, const scopi_hip_params& prm, hipStream_t s) {
     const int64_t na = P.n - P.n_inactive, ni = P.n_inactive;
     const bool kd = prm.contact_method == SCOPI_CONTACT_KDTREE;
@@ -759,18 +771,18 @@ int64_t ContactSearch::run_candidates(const Particles& P, Contacts& C, const sco
         hipLaunchKernelGGL(k_bbox_partial, dim3(nparts), dim3(kBlock), 0, s, P, d_partial_);
         hipLaunchKernelGGL(k_grid_setup, dim3(1), dim3(kBlock), 0, s, d_partial_, (int)nparts, P.dim, kd ? 1 : 0,
                            prm.kd_tree_radius, prm.dmax, 0, d_grid_);
-        hipLaunchKernelGGL(k_cell_keys<3>, dim3(grid_for(na)), dim3(kBlock), 0, s, P, d_grid_, keys_, vals_);
-        sorter_.sort(keys_, vals_, na, 3 * kMaxBits3, s, scanner_);
-        const int64_t ncells = int64_t(1) << (3 * kMaxBits3);
+        if (P.dim == 3)
+            hipLaunchKernelGGL(k_cell_keys<3>, dim3(grid_for(na)), dim3(kBlock), 0, s, P, d_grid_, keys_, vals_);
+        else
+            hipLaunchKernelGGL(k_cell_keys<2>, dim3(grid_for(na)), dim3(kBlock), 0, s, P, d_grid_, keys_, vals_);
+        const int key_bits = (P.dim == 3) ? 3 * kMaxBits3 : 2 * kMaxBits2;
+        sorter_.sort(keys_, vals_, na, key_bits, s, scanner_);
+        const int64_t ncells = int64_t(1) << key_bits;
         hipLaunchKernelGGL(k_fill_int, dim3(1024), dim3(kBlock), 0, s, cell_start_, ncells, 0);
         hipLaunchKernelGGL(k_fill_int, dim3(1024), dim3(kBlock), 0, s, cell_end_, ncells, 0);
         hipLaunchKernelGGL(k_cell_range, dim3(grid_for(na)), dim3(kBlock), 0, s, keys_, na, cell_start_, cell_end_);
-        if (kd)
-            hipLaunchKernelGGL((k_pair_count<3, true, true>), dim3(grid_for(na)), dim3(kBlock), 0, s, P, d_grid_,
-                               cell_start_, cell_end_, vals_, prm.kd_tree_radius, prm.dmax, counts_);
-        else
-            hipLaunchKernelGGL((k_pair_count<3, false, true>), dim3(grid_for(na)), dim3(kBlock), 0, s, P, d_grid_,
-                               cell_start_, cell_end_, vals_, prm.kd_tree_radius, prm.dmax, counts_);
+        SCOPI_CAND_LAUNCH(k_pair_count, P, d_grid_, cell_start_, cell_end_, vals_, prm.kd_tree_radius, prm.dmax,
+                          counts_);
         scanner_.exclusive(counts_, offsets_, na, d_totals_, s);
         SCOPI_HIP_CHECK(hipMemcpyAsync(h_totals_, d_totals_, sizeof(int), hipMemcpyDeviceToHost, s));
         SCOPI_HIP_CHECK(hipStreamSynchronize(s));
@@ -791,12 +803,8 @@ int64_t ContactSearch::run_candidates(const Particles& P, Contacts& C, const sco
     if (nobs > 0)
         hipLaunchKernelGGL(k_cand_obstacles, dim3(grid_for(nobs)), dim3(kBlock), 0, s, ni, na, se_cand_);
     if (nact > 0) {
-        if (kd)
-            hipLaunchKernelGGL((k_pair_emit<3, true, true>), dim3(grid_for(na)), dim3(kBlock), 0, s, P, d_grid_,
-                               cell_start_, cell_end_, vals_, prm.kd_tree_radius, prm.dmax, offsets_, pair_j_);
-        else
-            hipLaunchKernelGGL((k_pair_emit<3, false, true>), dim3(grid_for(na)), dim3(kBlock), 0, s, P, d_grid_,
-                               cell_start_, cell_end_, vals_, prm.kd_tree_radius, prm.dmax, offsets_, pair_j_);
+        SCOPI_CAND_LAUNCH(k_pair_emit, P, d_grid_, cell_start_, cell_end_, vals_, prm.kd_tree_radius, prm.dmax,
+                          offsets_, pair_j_);
         hipLaunchKernelGGL(k_cand_scatter, dim3(grid_for(na)), dim3(kBlock), 0, s, ni, na, counts_, offsets_,
                            pair_j_, nobs, se_cand_);
     }
@@ -815,6 +823,7 @@ int64_t ContactSearch::run_candidates(const Particles& P, Contacts& C, const sco
     SCOPI_HIP_CHECK(hipGetLastError());
     return nc;
 }
+#undef SCOPI_CAND_LAUNCH
 
 // ---------------------------------------------------------------- batches of scenes (host)
 BatchSearch::~BatchSearch() {

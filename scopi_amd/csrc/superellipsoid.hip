@@ -227,9 +227,9 @@ __device__ inline void se_normal(const SE& s, double a, double b, double* out) {
 
 // binit_xy(6) (superellipsoid.hpp:359-412) when xy, ainit_yz(6) (:415-466) otherwise;
 // sorted distinct values (xt::unique).  Returns the count.
-__device__ inline int init_angles(const SE& s, bool xy, double* out) {
-    constexpr int num = 6;
-    double v[kSeGridAngles];
+template <int num>
+__device__ inline int init_angles_n(const SE& s, bool xy, double* out) {
+    double v[4 * num];
     int m = 0;
     for (int i = 0; i < num; ++i) {
         const double t_ell = 0.0 + (0.5 * kPi - 0.0) / (num - 1) * i;
@@ -272,6 +272,7 @@ __device__ inline int init_angles(const SE& s, bool xy, double* out) {
         if (cnt == 0 || !(v[k] == out[cnt - 1])) out[cnt++] = v[k];
     return cnt;
 }
+__device__ inline int init_angles(const SE& s, bool xy, double* out) { return init_angles_n<6>(s, xy, out); }
 
 // ------------------------------------------------------------------ residual and Jacobian
 // F(a1, b1, a2, b2) of closest_points.hpp:282-382, values in the reference's order.  The
@@ -413,11 +414,9 @@ __device__ inline void se_residual(const SE& s1, const SE& s2, const double* u, 
     }
 }
 
-// ------------------------------------------------------------------ MINPACK hybrd1, n = 4
+// ------------------------------------------------------------------ MINPACK hybrd1, n = 4 (3D) or 2 (2D)
 // Powell's hybrid method (More, Garbow, Hillstrom, ANL-80-74): enorm, qrfac, qform,
 // dogleg, r1updt, r1mpyq with the analytic Jacobian, as minpack.hpp:337-923 runs it.
-constexpr int N = 4;
-constexpr int LR = N * (N + 1) / 2;
 
 __device__ inline double enorm(int n, const double* x) {
     const double rdwarf = 3.834e-20, rgiant = 1.304e19;
@@ -451,7 +450,12 @@ __device__ inline double enorm(int n, const double* x) {
     return x3max * sqrt(s3);
 }
 
-__device__ inline void qrfac(double* a, double* rdiag, double* acnorm) {
+// the routines for one system size N
+template <int N>
+struct Minpack {
+static constexpr int LR = N * (N + 1) / 2;
+
+static __device__ inline void qrfac(double* a, double* rdiag, double* acnorm) {
     for (int j = 0; j < N; ++j) {
         acnorm[j] = enorm(N, a + N * j);
         rdiag[j] = acnorm[j];
@@ -473,7 +477,7 @@ __device__ inline void qrfac(double* a, double* rdiag, double* acnorm) {
     }
 }
 
-__device__ inline void qform(double* q, double* wa) {
+static __device__ inline void qform(double* q, double* wa) {
     for (int j = 1; j < N; ++j)
         for (int i = 0; i < j; ++i) q[i + N * j] = 0.0;
     for (int l = 0; l < N; ++l) {
@@ -494,7 +498,7 @@ __device__ inline void qform(double* q, double* wa) {
     }
 }
 
-__device__ inline void dogleg(const double* r, const double* diag, const double* qtb, double delta, double* x,
+static __device__ inline void dogleg(const double* r, const double* diag, const double* qtb, double delta, double* x,
                               double* wa1, double* wa2) {
     const double epsmch = 2.220446049250313e-16;
     int jj = (N * (N + 1)) / 2 + 1;
@@ -565,7 +569,7 @@ __device__ inline void dogleg(const double* r, const double* diag, const double*
     for (int j = 0; j < N; ++j) x[j] = temp * wa1[j] + alpha * x[j];
 }
 
-__device__ inline void r1updt(double* s, const double* u, double* v, double* w, bool& sing) {
+static __device__ inline void r1updt(double* s, const double* u, double* v, double* w, bool& sing) {
     const double giant = 1.79769313486231570e308;
     const int m = N, n = N;
     int jj = (n * (2 * m - n + 1)) / 2 - (m - n);
@@ -640,7 +644,7 @@ __device__ inline void r1updt(double* s, const double* u, double* v, double* w, 
     if (s[jj - 1] == 0.0) sing = true;
 }
 
-__device__ inline void r1mpyq(int m, int n, double* a, int lda, const double* v, const double* w) {
+static __device__ inline void r1mpyq(int m, int n, double* a, int lda, const double* v, const double* w) {
     for (int nmj = 1; nmj <= n - 1; ++nmj) {
         const int j = n - nmj;
         double cs, sn;
@@ -680,14 +684,14 @@ __device__ inline void r1mpyq(int m, int n, double* a, int lda, const double* v,
 // evaluation each (phase 0: F at the start, 1: F and J at x, 2: F at the trial point), so a
 // lane can stop after any pass and take another pair; the arithmetic is the loops' own,
 // operation for operation.
-struct Hybrd {
+struct State {
     double x[N], fvec[N], fjac[N * N], r[LR], qtf[N], wa1[N], wa2[N], wa3[N], wa4[N], Jrow[N * N], ftmp[N];
     double delta, xnorm, fnorm, pnorm;
     int info, nfev, iter, ncsuc, ncfail, nslow1, nslow2, phase;
     bool jeval;
 };
 
-__device__ inline void hybrd_start(Hybrd& h, const double* u0) {
+static __device__ inline void start(State& h, const double* u0) {
     for (int k = 0; k < N; ++k) h.x[k] = u0[k];
     h.delta = h.xnorm = h.fnorm = h.pnorm = 0.0;
     h.info = h.nfev = 0;
@@ -698,14 +702,16 @@ __device__ inline void hybrd_start(Hybrd& h, const double* u0) {
 }
 
 // one pass; returns true when hybrd1 has ended (h.info, h.nfev, h.x hold its results)
-__device__ inline bool hybrd_pass(const SE& s1, const SE& s2, Hybrd& h) {
+template <class Res>
+static __device__ inline bool pass(Res residual, State& h) {
     const double xtol = 1.0e-10, factor = 100.0, epsmch = 2.220446049250313e-16;
     const int maxfev = 200 * (N + 1);
     const double p1 = 0.1, p5 = 0.5, p001 = 0.001, p0001 = 0.0001;
-    const double diag[N] = {1.0, 1.0, 1.0, 1.0};  // mode 2: wa[0..n) = 1 (minpack.hpp:881-884)
+    double diag[N];  // mode 2: wa[0..n) = 1 (minpack.hpp:881-884)
+    for (int k = 0; k < N; ++k) diag[k] = 1.0;
     const int msum = N;
-    se_residual(s1, s2, h.phase == 2 ? h.wa2 : h.x, h.phase == 0 ? h.fvec : (h.phase == 1 ? h.ftmp : h.wa4),
-                h.phase == 1 ? h.Jrow : nullptr);
+    residual(h.phase == 2 ? h.wa2 : h.x, h.phase == 0 ? h.fvec : (h.phase == 1 ? h.ftmp : h.wa4),
+             h.phase == 1 ? h.Jrow : nullptr);
     if (h.phase == 0) {
         h.nfev = 1;
         h.fnorm = enorm(N, h.fvec);
@@ -827,6 +833,7 @@ __device__ inline bool hybrd_pass(const SE& s1, const SE& s2, Hybrd& h) {
     h.phase = 2;
     return false;
 }
+};
 
 // ------------------------------------------------------------------ sphere / plane against a superellipsoid
 // closest_points(sphere<3>, superellipsoid<3>) (closest_points.hpp:944-1281) and
@@ -1159,6 +1166,354 @@ __device__ inline void newton_pair(const Particles& P, int64_t i, int64_t j, dou
 }
 
 // ------------------------------------------------------------------ kernels
+// ------------------------------------------------------------------ 2D: superellipsoid<2>
+// closest_points(superellipsoid<2>, superellipsoid<2>) (closest_points.hpp:64-249): hybrd1 on the
+// 2 x 2 system (normal of s1 collinear with p2 - p1, normals anti-parallel) from the first
+// minimum, row-major, of |p1 - p2| + 2 (1 + n1 . n2) over unique(binit_xy(4)) of each side;
+// closest_points(sphere<2>, superellipsoid<2>) (:1316-1448) and (superellipsoid<2>, plane<2>)
+// (:1782-1949): damped Newton from a distance table.  The objects use the 2 x 2 block of the
+// rotation (rotation3 of a 2D quaternion holds rotation_matrix<2> there, bit for bit), the
+// superellipsoid its squareness e.  Jacobians: forward mode, one direction at a time, with
+// the same operations as the full forward mode (the reference's hand-written ones are the
+// exact derivatives of its residuals).
+__device__ inline void se2_point(const SE& s, double b, double* out) {  // superellipsoid.hpp:281-289
+    const double cb = pm_cos(b), sb = pm_sin(b);
+    const double x = s.r[0] * sgn(cb) * pm_pow(fabs(cb), s.e);
+    const double y = s.r[1] * sgn(sb) * pm_pow(fabs(sb), s.e);
+    out[0] = (s.R[0][0] * x + s.R[0][1] * y) + s.c[0];
+    out[1] = (s.R[1][0] * x + s.R[1][1] * y) + s.c[1];
+    out[2] = 0.0;
+}
+__device__ inline void se2_normal(const SE& s, double b, double* out) {  // superellipsoid.hpp:303-312
+    const double cb = pm_cos(b), sb = pm_sin(b);
+    const double x = s.r[1] * sgn(cb) * pm_pow(fabs(cb), 2 - s.e);
+    const double y = s.r[0] * sgn(sb) * pm_pow(fabs(sb), 2 - s.e);
+    const double n0 = s.R[0][0] * x + s.R[0][1] * y, n1 = s.R[1][0] * x + s.R[1][1] * y;
+    const double l = sqrt(n0 * n0 + n1 * n1);
+    out[0] = n0 / l;
+    out[1] = n1 / l;
+    out[2] = 0.0;
+}
+__device__ inline void sph2_point(const Sph& s, double b, double* out) {  // sphere.hpp:194-201
+    const double x = s.r * pm_cos(b), y = s.r * pm_sin(b);
+    out[0] = (s.R[0][0] * x + s.R[0][1] * y) + s.c[0];
+    out[1] = (s.R[1][0] * x + s.R[1][1] * y) + s.c[1];
+    out[2] = 0.0;
+}
+__device__ inline void sph2_normal(const Sph& s, double b, double* out) {  // sphere.hpp:213-221
+    const double x = s.r * pm_cos(b), y = s.r * pm_sin(b);
+    const double n0 = s.R[0][0] * x + s.R[0][1] * y, n1 = s.R[1][0] * x + s.R[1][1] * y;
+    const double l = sqrt(n0 * n0 + n1 * n1);
+    out[0] = n0 / l;
+    out[1] = n1 / l;
+    out[2] = 0.0;
+}
+__device__ inline void pln2_point(const Pln& p, double b, double* out) {  // plane.hpp:175-182
+    out[0] = (p.R[0][0] * 0.0 + p.R[0][1] * b) + p.c[0];
+    out[1] = (p.R[1][0] * 0.0 + p.R[1][1] * b) + p.c[1];
+    out[2] = 0.0;
+}
+
+// cos / sin of an angle as a value (V = double) or with its derivative along a seed direction
+template <class V>
+__device__ inline V v_cos(double b, double seed) {
+    if constexpr (sizeof(V) == sizeof(D1))
+        return D1{pm_cos(b), -pm_sin(b) * seed};
+    else
+        return pm_cos(b);
+}
+template <class V>
+__device__ inline V v_sin(double b, double seed) {
+    if constexpr (sizeof(V) == sizeof(D1))
+        return D1{pm_sin(b), pm_cos(b) * seed};
+    else
+        return pm_sin(b);
+}
+template <class V>
+__device__ inline V v_var(double x, double seed) {
+    if constexpr (sizeof(V) == sizeof(D1))
+        return D1{x, seed};
+    else
+        return x;
+}
+// sign(x) |x|^p, derivative p |x|^(p-1) x'
+__device__ inline double v_spow(double a, double p) { return sgn(a) * pm_pow(fabs(a), p); }
+__device__ inline D1 v_spow(const D1& a, double p) {
+    const double ax = fabs(a.v);
+    return D1{sgn(a.v) * pm_pow(ax, p), (p * pm_pow(ax, p - 1)) * a.d};
+}
+__device__ inline double vval(double a) { return a; }
+__device__ inline double vval(const D1& a) { return a.v; }
+__device__ inline double vder(double) { return 0.0; }
+__device__ inline double vder(const D1& a) { return a.d; }
+
+// one superellipsoid side in the body frame: normal direction (r_y c^(2-e), r_x s^(2-e)), point
+// (r_x c^e, r_y s^e)
+template <class V>
+__device__ inline void se2_terms(const SE& s, double b, double seed, V& nx, V& ny, V& px, V& py) {
+    const V cb = v_cos<V>(b, seed), sb = v_sin<V>(b, seed);
+    nx = s.r[1] * v_spow(cb, 2 - s.e);
+    ny = s.r[0] * v_spow(sb, 2 - s.e);
+    px = s.r[0] * v_spow(cb, s.e);
+    py = s.r[1] * v_spow(sb, s.e);
+}
+
+// residual of closest_points.hpp:79-125 (kind 2), :1332-1366 (kind 0: sphere s2 with
+// superellipsoid s1) or :1800-1838 (kind 1: superellipsoid s1 with plane p2, sp = (c_s1 - c_p) . n_p)
+template <class V>
+__device__ inline void se2_algebra(int kind, const SE& s1, const SE& s2, const Sph& sp2, const Pln& p2, double sp,
+                                   const double* u, int k, V* F) {
+    V n1x, n1y, p1x, p1y;
+    se2_terms<V>(s1, u[0], k == 0 ? 1.0 : 0.0, n1x, n1y, p1x, p1y);
+    const double seed2 = (k == 1) ? 1.0 : 0.0;
+    const auto& A = s1.R;
+    if (kind == 2) {
+        V n2x, n2y, p2x, p2y;
+        se2_terms<V>(s2, u[1], seed2, n2x, n2y, p2x, p2y);
+        const auto& B = s2.R;
+        const V m1x = A[0][1] * n1y + A[0][0] * n1x, m1y = A[1][1] * n1y + A[1][0] * n1x;
+        const V m2x = B[0][1] * n2y + B[0][0] * n2x, m2y = B[1][1] * n2y + B[1][0] * n2x;
+        const V dy = (((((-(A[1][0] * p1x)) - A[1][1] * p1y) - cst<V>(s1.c[1])) + B[1][0] * p2x) + B[1][1] * p2y) +
+                     cst<V>(s2.c[1]);
+        const V dx = (((((-(A[0][0] * p1x)) - A[0][1] * p1y) - cst<V>(s1.c[0])) + B[0][0] * p2x) + B[0][1] * p2y) +
+                     cst<V>(s2.c[0]);
+        F[0] = (-m1x) * dy + m1y * dx;
+        F[1] = (m1x * m2x + m1y * m2y) + dsqrt(m1x * m1x + m1y * m1y) * dsqrt(m2x * m2x + m2y * m2y);
+    } else if (kind == 0) {
+        const V q5 = sp2.r * v_sin<V>(u[1], seed2), q6 = sp2.r * v_cos<V>(u[1], seed2);
+        const auto& B = sp2.R;
+        const V m1x = A[0][0] * n1x + A[0][1] * n1y, m1y = A[1][0] * n1x + A[1][1] * n1y;
+        const V m2x = B[0][0] * q6 + B[0][1] * q5, m2y = B[1][0] * q6 + B[1][1] * q5;
+        const V dy =
+            ((((-(A[1][0] * p1x)) - A[1][1] * p1y) + B[1][0] * q6) + B[1][1] * q5 - cst<V>(s1.c[1])) + cst<V>(sp2.c[1]);
+        const V dx =
+            ((((-(A[0][0] * p1x)) - A[0][1] * p1y) + B[0][0] * q6) + B[0][1] * q5 - cst<V>(s1.c[0])) + cst<V>(sp2.c[0]);
+        F[0] = (-m1x) * dy + m1y * dx;
+        F[1] = (m2x * m1x + m2y * m1y) + dsqrt(m2x * m2x + m2y * m2y) * dsqrt(m1x * m1x + m1y * m1y);
+    } else {
+        const V a2 = v_var<V>(u[1], seed2);
+        const auto& N = p2.R;
+        const V m1x = A[0][0] * n1x + A[0][1] * n1y, m1y = A[1][0] * n1x + A[1][1] * n1y;
+        const V dy = ((((-(A[1][0] * p1x)) - A[1][1] * p1y) + N[1][1] * a2) + cst<V>(p2.c[1])) - cst<V>(s1.c[1]);
+        const V dx = ((((-(A[0][0] * p1x)) - A[0][1] * p1y) + N[0][1] * a2) + cst<V>(p2.c[0])) - cst<V>(s1.c[0]);
+        F[0] = (-m1x) * dy + m1y * dx;
+        const double lp = sqrt(N[0][0] * N[0][0] + N[1][0] * N[1][0]);
+        F[1] = ((N[0][0] * sp) * m1x + (N[1][0] * sp) * m1y) + lp * dsqrt(m1x * m1x + m1y * m1y);
+    }
+}
+
+struct Se2Pair {
+    int kind;  // 0 sphere s2 / superellipsoid s1, 1 superellipsoid s1 / plane p2, 2 superellipsoid pair
+    SE s1, s2;
+    Sph sp2;
+    Pln p2;
+    double sp;
+};
+
+__device__ inline void se2_residual(const Se2Pair& q, const double* u, double* F, double* J) {
+    if (!J) {
+        se2_algebra<double>(q.kind, q.s1, q.s2, q.sp2, q.p2, q.sp, u, -1, F);
+        return;
+    }
+    for (int k = 0; k < 2; ++k) {
+        D1 f[2];
+        se2_algebra<D1>(q.kind, q.s1, q.s2, q.sp2, q.p2, q.sp, u, k, f);
+        for (int i = 0; i < 2; ++i) {
+            J[2 * i + k] = f[i].d;
+            if (k == 0) F[i] = f[i].v;
+        }
+    }
+}
+
+// newton_method (utils.hpp:81-118) on the 2 x 2 system: LU with partial pivoting (LAPACK gesv)
+__device__ inline void lu_solve2(double* A, double* b) {
+    constexpr int n = 2;
+    int piv[n];
+    for (int k = 0; k < n; ++k) {
+        int p = k;
+        double best = fabs(A[n * k + k]);
+        for (int i = k + 1; i < n; ++i)
+            if (fabs(A[n * i + k]) > best) {
+                best = fabs(A[n * i + k]);
+                p = i;
+            }
+        piv[k] = p;
+        if (p != k)
+            for (int j = 0; j < n; ++j) {
+                const double t = A[n * k + j];
+                A[n * k + j] = A[n * p + j];
+                A[n * p + j] = t;
+            }
+        if (A[n * k + k] != 0.0) {
+            const double inv = 1.0 / A[n * k + k];
+            for (int i = k + 1; i < n; ++i) A[n * i + k] *= inv;
+        }
+        for (int i = k + 1; i < n; ++i)
+            for (int j = k + 1; j < n; ++j) A[n * i + j] -= A[n * i + k] * A[n * k + j];
+    }
+    for (int k = 0; k < n; ++k)
+        if (piv[k] != k) {
+            const double t = b[k];
+            b[k] = b[piv[k]];
+            b[piv[k]] = t;
+        }
+    for (int i = 1; i < n; ++i)
+        for (int j = 0; j < i; ++j) b[i] -= A[n * i + j] * b[j];
+    for (int i = n - 1; i >= 0; --i) {
+        for (int j = i + 1; j < n; ++j) b[i] -= A[n * i + j] * b[j];
+        b[i] /= A[n * i + i];
+    }
+}
+__device__ inline double norm2v(const double* x) {
+    double s = 0.0;
+    for (int k = 0; k < 2; ++k) s += x[k] * x[k];
+    return sqrt(s);
+}
+__device__ inline int newton2(const Se2Pair& q, double* u) {
+    int cc = 0;
+    while (cc < 200) {
+        double F[2], J[4], d[2];
+        se2_residual(q, u, F, J);
+        for (int k = 0; k < 2; ++k) d[k] = -F[k];
+        lu_solve2(J, d);
+        if (norm2v(d) < 1.0e-7) return cc;
+        double t = 1;
+        const double ferr = norm2v(F);
+        if (ferr < 1.0e-10) return cc;
+        for (;;) {
+            double ut[2], Ft[2];
+            for (int k = 0; k < 2; ++k) ut[k] = u[k] + t * d[k];
+            se2_residual(q, ut, Ft, nullptr);
+            if (!(norm2v(Ft) > ferr && t > 0.01)) break;
+            t -= 0.01;
+        }
+        for (int k = 0; k < 2; ++k) u[k] += t * d[k];
+        cc += 1;
+    }
+    return -1;
+}
+
+__device__ inline double dist2d(const double* a, const double* b) {
+    const double dx = a[0] - b[0], dy = a[1] - b[1];
+    return sqrt(dx * dx + dy * dy);
+}
+
+// The closest points of one 2D pair with a superellipsoid, in the dispatcher's order (i, j);
+// returns false for the shape pairs the reference has no form for.
+__device__ inline bool se2_pair(const Particles& P, int64_t i, int64_t j, double* pi, double* pj, double* nij,
+                                double& dij, double* u0, double* u, int& status) {
+    const int si = P.shape[i], sj = P.shape[j];
+    const bool se_i = si == SCOPI_SHAPE_SUPERELLIPSOID, se_j = sj == SCOPI_SHAPE_SUPERELLIPSOID;
+    Se2Pair q{};
+    u0[2] = u0[3] = u[2] = u[3] = 0.0;
+    if (se_i && se_j) {
+        q.kind = 2;
+        q.s1 = load_se(P, i);
+        q.s2 = load_se(P, j);
+        double b1[16], b2[16];
+        const int n1 = init_angles_n<4>(q.s1, true, b1), n2 = init_angles_n<4>(q.s2, true, b2);
+        double dmin = INFINITY;
+        int im = 0, jm = 0;
+        for (int a = 0; a < n1; ++a) {
+            double p[3], n[3];
+            se2_point(q.s1, b1[a], p);
+            se2_normal(q.s1, b1[a], n);
+            for (int b = 0; b < n2; ++b) {
+                double pq[3], m[3];
+                se2_point(q.s2, b2[b], pq);
+                se2_normal(q.s2, b2[b], m);
+                const double dist = dist2d(p, pq) + 2 * (1 + (n[0] * m[0] + n[1] * m[1]));
+                if (dist < dmin) {
+                    dmin = dist;
+                    im = a;
+                    jm = b;
+                }
+            }
+        }
+        u0[0] = b1[im];
+        u0[1] = b2[jm];
+        typename Minpack<2>::State h;
+        Minpack<2>::start(h, u0);
+        while (!Minpack<2>::pass([&](const double* x, double* F, double* J) { se2_residual(q, x, F, J); }, h)) {
+        }
+        u[0] = h.x[0];
+        u[1] = h.x[1];
+        status = h.info;
+        se2_point(q.s1, u[0], pi);
+        se2_point(q.s2, u[1], pj);
+        se2_normal(q.s2, u[1], nij);
+        const double dx = pi[0] - pj[0], dy = pi[1] - pj[1];
+        dij = sgn(dx * nij[0] + dy * nij[1]) * sqrt(dx * dx + dy * dy);
+        return true;
+    }
+    const int64_t ks = se_i ? i : j, ko = se_i ? j : i;
+    const int other = se_i ? sj : si;
+    if (other != SCOPI_SHAPE_SPHERE && other != SCOPI_SHAPE_PLANE) return false;
+    q.kind = (other == SCOPI_SHAPE_SPHERE) ? 0 : 1;
+    const bool flip = (q.kind == 0) ? se_i : !se_i;
+    q.s1 = load_se(P, ks);
+    double a[3], b[3], n[3];
+    if (q.kind == 0) {
+        q.sp2 = load_sph(P, ko);
+        const double bi[4] = {-kPi, -kPi / 2, 0.001, kPi / 2};
+        double dmin = INFINITY;
+        int im = 0, jm = 0;
+        for (int x = 0; x < 4; ++x)
+            for (int y = 0; y < 4; ++y) {
+                se2_point(q.s1, bi[x], a);
+                sph2_point(q.sp2, bi[y], b);
+                const double d = dist2d(a, b);
+                if (d < dmin) {
+                    dmin = d;
+                    im = x;
+                    jm = y;
+                }
+            }
+        u0[0] = bi[im];
+        u0[1] = bi[jm];
+    } else {
+        q.p2 = load_pln(P, ko);
+        q.sp = (q.s1.c[0] - q.p2.c[0]) * q.p2.R[0][0] + (q.s1.c[1] - q.p2.c[1]) * q.p2.R[1][0];
+        constexpr int num = 10;
+        double dmin = INFINITY;
+        int im = 0, jm = 0;
+        for (int x = 0; x < num; ++x) {
+            se2_point(q.s1, lin(-kPi, kPi, num, x), a);
+            for (int y = 0; y < num; ++y) {
+                pln2_point(q.p2, lin(-num, num, num, y), b);
+                const double d = dist2d(a, b);
+                if (d < dmin) {
+                    dmin = d;
+                    im = x;
+                    jm = y;
+                }
+            }
+        }
+        u0[0] = lin(-kPi, kPi, num, im);
+        u0[1] = lin(-num, num, num, jm);
+    }
+    u[0] = u0[0];
+    u[1] = u0[1];
+    status = newton2(q, u);
+    se2_point(q.s1, u[0], a);
+    if (q.kind == 0) {
+        sph2_point(q.sp2, u[1], b);
+        sph2_normal(q.sp2, u[1], n);
+    } else {
+        pln2_point(q.p2, u[1], b);
+        n[0] = q.p2.R[0][0];
+        n[1] = q.p2.R[1][0];
+        n[2] = 0.0;
+    }
+    dij = (a[0] - b[0]) * n[0] + (a[1] - b[1]) * n[1];
+    for (int d = 0; d < 3; ++d) {
+        pi[d] = flip ? b[d] : a[d];
+        pj[d] = flip ? a[d] : b[d];
+        nij[d] = flip ? n[d] * -1. : n[d];
+    }
+    return true;
+}
+
 __device__ inline bool is_se(const Particles& P, int64_t k) { return P.shape[k] == SCOPI_SHAPE_SUPERELLIPSOID; }
 
 // initial angles: one lane per active particle
@@ -1306,7 +1661,7 @@ __device__ inline void write_record(SeCandidates& S, int64_t c, double dij, cons
 constexpr int kSolveBlock = 64;
 __global__ void __launch_bounds__(kSolveBlock) k_se_solve(Particles P, SeCandidates S, int64_t n) {
     SE s1, s2;
-    Hybrd h;
+    Minpack<4>::State h;
     int64_t c = atomicAdd(S.counter, 1);
     bool busy = false;
     for (;;) {
@@ -1317,7 +1672,7 @@ __global__ void __launch_bounds__(kSolveBlock) k_se_solve(Particles P, SeCandida
             if (se_i && se_j) {
                 s1 = load_se(P, i);
                 s2 = load_se(P, j);
-                hybrd_start(h, S.u0 + 4 * c);
+                Minpack<4>::start(h, S.u0 + 4 * c);
                 busy = true;
             } else {
                 if (se_i != se_j) {  // sphere / plane - superellipsoid: k_newton_pairs
@@ -1339,7 +1694,8 @@ __global__ void __launch_bounds__(kSolveBlock) k_se_solve(Particles P, SeCandida
                 continue;
             }
         }
-        if (!hybrd_pass(s1, s2, h)) continue;
+        if (!Minpack<4>::pass([&](const double* x, double* F, double* J) { se_residual(s1, s2, x, F, J); }, h))
+            continue;
         // hybrd1 ended: the contact record of closest_points.hpp:724-728
         double pi[3], pj[3], nij[3];
         se_point(s1, h.x[0], h.x[1], pi);
@@ -1377,6 +1733,34 @@ __global__ void __launch_bounds__(kSolveBlock) k_newton_pairs(Particles P, SeCan
 }
 
 // the portable elementary functions, elementwise (tests)
+// 2D scenes with superellipsoids: one lane per candidate pair, every shape pair
+__global__ void __launch_bounds__(kSolveBlock) k_se2_pairs(Particles P, SeCandidates S, int64_t n) {
+    const int64_t c = (int64_t)blockIdx.x * kSolveBlock + threadIdx.x;
+    if (c >= n) return;
+    const int64_t i = S.i[c], j = S.j[c];
+    double pi[3] = {0, 0, 0}, pj[3] = {0, 0, 0}, nij[3] = {0, 0, 0}, u0[4] = {0, 0, 0, 0}, u[4] = {0, 0, 0, 0};
+    double dij = INFINITY;
+    int status = -1;
+    if (is_se(P, i) || is_se(P, j)) {
+        if (!se2_pair(P, i, j, pi, pj, nij, dij, u0, u, status)) dij = INFINITY;
+    } else {
+        CP cp;
+        if (closest_points<2>(P, i, j, cp)) {
+            dij = cp.dij;
+            nij[0] = cp.nij.x, nij[1] = cp.nij.y, nij[2] = cp.nij.z;
+            pi[0] = cp.pi.x, pi[1] = cp.pi.y, pi[2] = cp.pi.z;
+            pj[0] = cp.pj.x, pj[1] = cp.pj.y, pj[2] = cp.pj.z;
+        }
+    }
+    write_record(S, c, dij, nij, pi, pj);
+    for (int k = 0; k < 4; ++k) {
+        S.u0[4 * c + k] = u0[k];
+        S.u[4 * c + k] = u[k];
+    }
+    S.info[c] = status;
+    S.nfev[c] = 0;
+}
+
 __global__ void __launch_bounds__(kBlock) k_portable_math(int fn, int64_t n, const double* __restrict__ x,
                                                          const double* __restrict__ y, double* __restrict__ out) {
     const int64_t k = (int64_t)blockIdx.x * kBlock + threadIdx.x;
@@ -1485,6 +1869,12 @@ void se_candidates_reserve(SeCandidates& S, int64_t n) {
 void se_closest_points(const Particles& P, SeGrids& G, SeCandidates& S, int64_t n, hipStream_t s) {
     const int64_t na = P.n - P.n_inactive;
     if (n <= 0 || na <= 0) return;
+    if (P.dim == 2) {
+        hipLaunchKernelGGL(k_se2_pairs, dim3((unsigned)((n + kSolveBlock - 1) / kSolveBlock)), dim3(kSolveBlock), 0, s,
+                           P, S, n);
+        SCOPI_HIP_CHECK(hipGetLastError());
+        return;
+    }
     se_grids_reserve(G, na);
     hipLaunchKernelGGL(k_se_angles, dim3(grid_for(na)), dim3(kBlock), 0, s, P, G);
     hipLaunchKernelGGL(k_se_points, dim3(grid_for(na * kSeGridPoints)), dim3(kBlock), 0, s, P, G);

@@ -298,7 +298,7 @@ class HipScene:
                     nfev=nfev[:n])
 
     def closest_points(self, shape_i, pos_i, q_i, size_i, sq_i, shape_j, pos_j, q_j, size_j, sq_j):
-        """closest_points_dispatcher::dispatch of n independent 3D pairs on the device."""
+        """closest_points_dispatcher::dispatch of n independent pairs (the context's dimension) on the device."""
         f = lambda a, w: np.ascontiguousarray(np.asarray(a, np.float64).reshape(-1, w))
         si = np.ascontiguousarray(shape_i, np.int32); sj = np.ascontiguousarray(shape_j, np.int32)
         n = len(si)

@@ -206,16 +206,25 @@ struct plane : object<dim> {
     plane(type::position_t<dim> pos, type::quaternion_t q) : object<dim>{SCOPI_SHAPE_PLANE, pos, q, {0, 0, 0}} {}
 };
 
-// superellipsoid<3>(pos, q, radius (semi-axes), squareness (e, n)):
-// include/scopi/objects/types/superellipsoid.hpp:111-139 (3D only here).
+// superellipsoid<dim>(pos, q, radius (semi-axes), squareness (e in 2D, (e, n) in 3D)):
+// include/scopi/objects/types/superellipsoid.hpp:111-139.
 template <std::size_t dim>
 struct superellipsoid : object<dim> {
-    static_assert(dim == 3, "superellipsoids are supported in 3D");
-    superellipsoid(type::position_t<dim> pos, type::quaternion_t q, fixed<3> radius, fixed<2> squareness)
-        : object<dim>{SCOPI_SHAPE_SUPERELLIPSOID, pos, q, {radius[0], radius[1], radius[2]},
-                      {squareness[0], squareness[1]}} {}
-    fixed<3> radius() const { return fixed<3>{{this->size[0], this->size[1], this->size[2]}}; }
-    fixed<2> squareness() const { return fixed<2>{{this->sq[0], this->sq[1]}}; }
+    // superellipsoid(pos, q, radius, squareness) (objects/types/superellipsoid.hpp:57-66): dim radii,
+    // dim - 1 squareness values (e in 2D; e, n in 3D)
+    superellipsoid(type::position_t<dim> pos, type::quaternion_t q, fixed<dim> radius, fixed<dim - 1> squareness)
+        : object<dim>{SCOPI_SHAPE_SUPERELLIPSOID, pos, q, {radius[0], radius[1], dim == 3 ? radius[dim - 1] : 0.0},
+                      {squareness[0], dim == 3 ? squareness[dim - 2] : 1.0}} {}
+    fixed<dim> radius() const {
+        fixed<dim> r;
+        for (std::size_t d = 0; d < dim; ++d) r[d] = this->size[d];
+        return r;
+    }
+    fixed<dim - 1> squareness() const {
+        fixed<dim - 1> e;
+        for (std::size_t d = 0; d + 1 < dim; ++d) e[d] = this->sq[d];
+        return e;
+    }
 };
 
 template <std::size_t dim>

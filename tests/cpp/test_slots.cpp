@@ -429,6 +429,35 @@ void superellipsoids_3d() {
     std::fflush(stdout);
 }
 
+// test_closest_points.cpp:1502-1562 (superellipsoid_superellipsoid_2d and its 30-degree variant)
+// through the contact slot, 2D.
+void superellipsoids_2d() {
+    constexpr std::size_t dim = 2;
+    const double PI = 3.14159265358979323846, c30 = std::sqrt(3.) / 2., s30 = 0.5, dist = 0.4;
+    for (int rot = 0; rot < 2; ++rot) {
+        scopi_container<dim> particles;
+        auto p = property<dim>().mass(1.).moment_inertia(0.1);
+        if (rot == 0) {
+            particles.push_back(superellipsoid<dim>({{-0.2, 0.0}}, {quaternion(0.)}, {{0.1, 0.2}}, {{1}}), p);
+            particles.push_back(superellipsoid<dim>({{0.2, 0.0}}, {quaternion(0.)}, {{0.1, 0.3}}, {{1}}), p);
+        } else {
+            particles.push_back(
+                superellipsoid<dim>({{-dist * c30, dist * s30}}, {quaternion(-PI / 6.)}, {{0.1, 0.4}}, {{1}}), p);
+            particles.push_back(
+                superellipsoid<dim>({{dist * c30, -dist * s30}}, {quaternion(PI - PI / 6.)}, {{0.2, 0.3}}, {{1}}), p);
+        }
+        ContactsParams<contact_kdtree<NoFriction>> params;
+        params.dmax = 1.0;
+        contact_kdtree<NoFriction> cont(params);
+        auto contacts = cont.run(particles);
+        std::printf("{\"case\": \"superellipsoids_2d_%d\", \"n\": %zu, \"dij\": %.17g, \"nij\": [%.17g, %.17g, 0], "
+                    "\"pi\": [%.17g, %.17g, 0], \"pj\": [%.17g, %.17g, 0]}\n",
+                    rot, contacts.size(), contacts[0].dij, contacts[0].nij[0], contacts[0].nij[1], contacts[0].pi[0],
+                    contacts[0].pi[1], contacts[0].pj[0], contacts[0].pj[1]);
+    }
+    std::fflush(stdout);
+}
+
 int main(int argc, char** argv) {
     std::string only = argc > 1 ? argv[1] : "";
     using kd_fixed = ScopiSolver<2, NoFriction, OptimGradient<apgd>, contact_kdtree, vap_fixed>;
@@ -456,7 +485,10 @@ int main(int argc, char** argv) {
             viscous_slot();
         }
         if (only.empty() || only == "slots") slots_one_by_one();
-        if (only.empty() || only == "superellipsoids") superellipsoids_3d();
+        if (only.empty() || only == "superellipsoids") {
+            superellipsoids_3d();
+            superellipsoids_2d();
+        }
         if (only.empty() || only == "periodic") periodic_pair();
     } catch (const std::exception& e) {
         std::fprintf(stderr, "test_slots: %s\n", e.what());

@@ -108,7 +108,10 @@ def test_superellipsoid_contact_slot(output):
     contact slot: superellipsoid<3> objects in a scopi_container, contact_kdtree<NoFriction>::run."""
     c30, s30 = math.sqrt(3.0) / 2.0, 0.5
     known = {"superellipsoids_3d_0": ((-0.1, 0, 0), (0.1, 0, 0), (-1.0, 0, 0), 0.2),
-             "superellipsoids_3d_1": ((-0.3 * c30, 0.3 * s30, 0), (0.2 * c30, -0.2 * s30, 0), (-c30, s30, 0), 0.5)}
+             "superellipsoids_3d_1": ((-0.3 * c30, 0.3 * s30, 0), (0.2 * c30, -0.2 * s30, 0), (-c30, s30, 0), 0.5),
+             # superellipsoid<2> (test_closest_points.cpp:1502-1562)
+             "superellipsoids_2d_0": ((-0.1, 0, 0), (0.1, 0, 0), (-1.0, 0, 0), 0.2),
+             "superellipsoids_2d_1": ((-0.3 * c30, 0.3 * s30, 0), (0.2 * c30, -0.2 * s30, 0), (-c30, s30, 0), 0.5)}
     for name, (pi, pj, nij, dij) in known.items():
         c = output[name]
         assert c["n"] == 1
