@@ -46,20 +46,18 @@ def kernel_bytes(nc, nc_obs, na, s, layout="rotation_free"):
 
     s = value bytes (8 FP64, 4 FP32); entries of the particle-major copy of B:
     2 per particle pair, 1 per obstacle contact.
-      k_trial : lambda_k, lambda_{k-1}, Q lambda_k, Q lambda_{k-1}, c read and lambda+
-                written per row (+ the normal and mu per contact for the cone);
+      k_trial : lambda_k, lambda_{k-1}, Q lambda_{k-1}, c read and lambda+, Q lambda_k
+                written per row; per contact i, j (int32) and NC coefficients (+ mu for
+                the cone); per particle w_k (NW), which the row product gathers;
       k_bt    : per entry its contact index (int32) and NT coefficients; lambda+ once
-                per row; per particle its entry offset (int32), M^-1 and w (NW each);
-      k_row   : per contact i, j (int32), NC coefficients, lambda+ read and Q lambda+
-                written per row; w (NW) per particle.
+                per row; per particle its entry offset (int32), M^-1 and w+ (NW each).
     """
     NR, NC, NT, NW = LAYOUTS[layout]
     rows = NR * nc
     entries = 2 * nc - nc_obs
-    trial = rows * 6 * s + (nc * 4 * s if layout == "cone" else 0)
+    trial = rows * 6 * s + nc * (8 + NC * s) + (nc * s if layout == "cone" else 0) + na * NW * s
     bt = entries * (4 + NT * s) + rows * s + na * (4 + 2 * NW * s)
-    row = nc * 8 + nc * NC * s + 2 * rows * s + na * NW * s
-    return dict(bt=bt, row=row, trial=trial)
+    return dict(bt=bt, trial=trial)
 
 
 def cpu_baseline(sc, prm_kw, per_step_counts, budget_s=20.0):
@@ -380,8 +378,7 @@ def main():
         g.set_kernel_timing(False)
         nc = info_k["n_contacts"]
         it32 = info_k["iterations_fp32"] / max(1, info_k["iterations"])
-        for name, ms_key, n_key in (("bt", "bt_ms", "bt_launches"), ("row", "row_ms", "row_launches"),
-                                    ("trial", "trial_ms", "trial_launches")):
+        for name, ms_key, n_key in (("bt", "bt_ms", "bt_launches"), ("trial", "trial_ms", "trial_launches")):
             n = kt[n_key]
             per[name] = dict(avg_us=(kt[ms_key] / n * 1e3) if n else None, launches=n, total_ms=kt[ms_key])
     ct = g.contacts() if nc else dict(ij=np.zeros((0, 2), np.int64))
@@ -399,10 +396,10 @@ def main():
             a = per[k]["avg_us"]
             per[k]["algorithmic_bytes"] = kb[k]
             per[k]["achieved_gbs"] = kb[k] / (a * 1e-6) / 1e9 if a else None
-        roofline = {"bound": "hbm", "kernel": {"bt": "k_bt", "row": "k_row", "trial": "k_trial"}[dom],
+        roofline = {"bound": "hbm", "kernel": {"bt": "k_bt", "trial": "k_trial"}[dom],
                     "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                     "frac": (achieved / HBM_PEAK_GBS) if achieved else None,
-                    "traffic": measured_traffic({"bt": "k_bt", "row": "k_row", "trial": "k_trial"}[dom],
+                    "traffic": measured_traffic({"bt": "k_bt", "trial": "k_trial"}[dom],
                                                 args.precision),
                     "algorithmic_bytes_per_launch": kb[dom], "avg_launch_us": per[dom]["avg_us"]}
     log(f"[bench] {steps_per_s:.3f} steps/s, nc={nc}, iterations/step={it_per_step:.1f}, "

@@ -28,6 +28,7 @@ class Params(C.Structure):
         ("tolerance", C.c_double), ("dynamic_descent", C.c_int32), ("pad", C.c_int32), ("mu", C.c_double),
         ("fixed_point_tol", C.c_double), ("fixed_point_max_iter", C.c_double),
         ("gamma", C.c_double), ("gamma_min", C.c_double), ("gamma_tol", C.c_double),
+        ("objective_perturbation", C.c_double),
     ]
 
 

@@ -888,6 +888,7 @@ SolveResult apgd(const MinProblem& mp, const oracle_params& prm) {
         if (prm.dynamic_descent) {
             while (true) {
                 double f_new = mp.value(lam_np1.data());
+                if (prm.objective_perturbation != 0.0) f_new *= 1.0 + prm.objective_perturbation;
                 double f_y = mp.value(y_n.data());
                 double dt_ = 0.0, nr = 0.0;
                 for (int64_t k = 0; k < n; ++k) {

@@ -49,6 +49,10 @@ typedef struct oracle_params {
     double gamma;            /* contact_property<Viscous>::gamma (default 0)   */
     double gamma_min;        /* (default -2)                                   */
     double gamma_tol;        /* (default 1e-6)                                 */
+    double objective_perturbation; /* test hook, default 0: the line search     */
+                             /* compares f(lambda+) (1 + this): a rounding-level */
+                             /* change that shows how far a result depends on    */
+                             /* the rounding of the acceptance test              */
 } oracle_params;
 
 typedef struct oracle_stats {

