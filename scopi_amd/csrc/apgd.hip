@@ -38,12 +38,14 @@ namespace scopi_hip {
 
 constexpr int kRedBlocksMax = 2048;
 constexpr int kPartialStride = 16;
-// Blocks of the per-contact kernels for nc contacts: one per kRowsPerBlock contacts, at most
-// `cap` (grid-stride beyond).  The same rule sizes a scene's share of a batch's grid, so a
-// scene solved in a batch runs the one-scene arithmetic exactly.
-constexpr int64_t kRowsPerBlock = 1024;
-inline int64_t row_blocks(int64_t nc, int cap) {
-    return std::min<int64_t>(std::max<int64_t>((nc + kRowsPerBlock - 1) / kRowsPerBlock, 1), cap);
+// Blocks of the per-contact kernel for nc contacts: one per contacts_per_block(layout) contacts,
+// at most `cap` (grid-stride beyond).  That is two loop iterations per lane: the rotation-free
+// layout takes two contacts per iteration, the others one.  The same rule sizes a scene's share
+// of a batch's grid, so a scene solved in a batch runs the one-scene arithmetic exactly.
+inline int64_t contacts_per_block(int layout) { return layout == kRotFree ? 2 * 2 * kBlock : 2 * kBlock; }
+inline int64_t row_blocks(int64_t nc, int cap, int layout) {
+    const int64_t per = contacts_per_block(layout);
+    return std::min<int64_t>(std::max<int64_t>((nc + per - 1) / per, 1), cap);
 }
 // Blocks of k_bt for na particles at G lanes per particle (one block sum each in the loop).
 inline unsigned bt_blocks(int64_t na, int G) { return grid_for(na * (int64_t)G); }
@@ -1066,7 +1068,7 @@ void Solver::run_phase(BMatrix<T>& B, const Particles& P, const Contacts& C, int
                        const scopi_hip_params& prm, double dt, double tol, double rtol, int64_t max_ite, bool warm,
                        bool keep_L, hipStream_t s, scopi_hip_solve_info& info) {
     constexpr int NR = lay<LAY>::NR;
-    const unsigned gred = (unsigned)row_blocks(nc, reduce_blocks_);
+    const unsigned gred = (unsigned)row_blocks(nc, reduce_blocks_, LAY);
     const int nbt = (int)bt_blocks(na, bt_group_);
     const int64_t ni = P.n_inactive;
     const int64_t stride = NR * B.cap, wstride = 6 * B.pcap;
@@ -1733,7 +1735,7 @@ int Solver::solve_scenes(const Particles& P, const Contacts& C, int64_t nc, cons
         ar[2 * k] = S.alo[k];
         ar[2 * k + 1] = S.ahi[k];
         const int64_t nck = hc[2 * k + 1] - hc[2 * k];
-        scene_rblocks_[k] = (int)row_blocks(nck, reduce_blocks_);
+        scene_rblocks_[k] = (int)row_blocks(nck, reduce_blocks_, layout_);
         scene_bblocks_[k] = (int)std::max<int64_t>(bt_blocks(S.ahi[k] - S.alo[k], bt_group_), 1);
         R += scene_rblocks_[k];
         Bn += scene_bblocks_[k];
