@@ -1561,7 +1561,8 @@ __global__ void __launch_bounds__(kBlock) k_se_points(Particles P, SeGrids G) {
 // (staged in LDS) in order; its first strict minimum per i, then the (value, index)
 // minimum over lanes, is the first minimum of the row-major scan.  A term whose
 // penalty 2 (1 + n1.n2) already reaches the lane's best cannot win (the distance
-// term is >= 0), so its square root is skipped.
+// term is >= 0), nor one whose squared distance exceeds (best - penalty)^2 by more
+// than the rounding margin, so their square roots are skipped.
 constexpr int kSeGridBlock = 192;  // 3 x 192 = 576
 __global__ void __launch_bounds__(kSeGridBlock) k_se_grid(Particles P, SeGrids G, SeCandidates S, int64_t n) {
     __shared__ double g2[6][kSeGridPoints];
@@ -1600,10 +1601,19 @@ __global__ void __launch_bounds__(kSeGridBlock) k_se_grid(Particles P, SeGrids G
             const double pen = 2 * (1 + (m1[r][0] * nx + m1[r][1] * ny + m1[r][2] * nz));
             if (ok[r] && pen < best[r]) {
                 const double dx = p1[r][0] - g2[0][jj], dy = p1[r][1] - g2[1][jj], dz = p1[r][2] - g2[2][jj];
-                const double dist = sqrt(dx * dx + dy * dy + dz * dz) + pen;
-                if (dist < best[r]) {
-                    best[r] = dist;
-                    bj[r] = jj;
+                const double d2 = dx * dx + dy * dy + dz * dz;
+                // The term fl(sqrt(d2)) + pen cannot beat best when sqrt(d2) (1 - eps) >= best - pen
+                // (rounding is monotone and best is a double).  With T = fl(best - pen) and
+                // TT = fl(T T) normal, d2 > TT (1 + 1e-14) implies it (the factor covers the four
+                // roundings involved), so the square root is taken only for terms that may win:
+                // the same minimum, the same index.
+                const double t = best[r] - pen, tt = t * t;
+                if (!(tt > 1e-300 && d2 > tt * (1.0 + 1e-14))) {
+                    const double dist = sqrt(d2) + pen;
+                    if (dist < best[r]) {
+                        best[r] = dist;
+                        bj[r] = jj;
+                    }
                 }
             }
         }
