@@ -253,6 +253,10 @@ int scopi_hip_newton_probe(scopi_hip_ctx* ctx, int32_t kind, const double* in, d
  *     once per solve phase and replays it until the loop is done (same kernels, same results). */
 enum { SCOPI_TUNE_REDUCE_BLOCKS = 1, SCOPI_TUNE_BT_GROUP = 2, SCOPI_TUNE_GRAPH_ROUNDS = 3 };
 int scopi_hip_set_tuning(scopi_hip_ctx* ctx, int32_t key, int32_t value);
+/* Kernel timing of the projected-gradient loop since scopi_hip_set_kernel_timing(ctx, 1): out[6] =
+ * {ms_bt, n_bt, ms_row, n_row, ms_trial, n_trial}, total milliseconds (HIP events of each dispatch)
+ * and launches of k_bt and k_trial; the row product runs inside k_trial, so ms_row and n_row are 0.
+ * No counterpart in the reference. */
 int scopi_hip_kernel_timing(scopi_hip_ctx* ctx, double* out);
 
 /* ---- Batches of independent scenes ------------------------------------------
