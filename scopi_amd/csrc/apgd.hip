@@ -47,8 +47,12 @@ inline int64_t row_blocks(int64_t nc, int cap, int layout) {
     const int64_t per = contacts_per_block(layout);
     return std::min<int64_t>(std::max<int64_t>((nc + per - 1) / per, 1), cap);
 }
-// Blocks of k_bt for na particles at G lanes per particle (one block sum each in the loop).
-inline unsigned bt_blocks(int64_t na, int G) { return grid_for(na * (int64_t)G); }
+// Blocks of k_bt for na particles at G lanes per particle, at most kBtBlocksMax (grid-stride
+// beyond): each block writes one block sum per round, which every block of the next k_trial reads.
+constexpr int kBtBlocksMax = 2048;
+inline unsigned bt_blocks(int64_t na, int G) {
+    return std::min<unsigned>(grid_for(na * (int64_t)G), (unsigned)kBtBlocksMax);
+}
 constexpr int kStatusStalled = 2;                // FP32 phase reached its precision floor
 constexpr int kMaxConsecutiveRejections32 = 40;  // 2^40 growth of L without an accepted step
 constexpr int64_t kMaxStagnation32 = 300;        // FP32 accepted iterations without halving the residual
@@ -405,14 +409,16 @@ __global__ void __launch_bounds__(kBlock) k_bt(ApgdCtrl* __restrict__ ctrl, int 
     const int sc = multi ? M.bblk[blockIdx.x] : 0;
     const int lb = multi ? (int)blockIdx.x - M.bfirst[sc] : (int)blockIdx.x;
     const int64_t p0 = multi ? M.arange[2 * sc] : 0, p1 = multi ? M.arange[2 * sc + 1] : na;
-    const int64_t p = p0 + ((int64_t)lb * kBlock + threadIdx.x) / G;
+    // grid-stride over the scene's particles: kBlock / G particles per block and pass
+    const int nbk = multi ? M.bcount[sc] : (int)gridDim.x;
+    const int64_t pstep = (int64_t)nbk * (kBlock / G);
+    int64_t p = p0 + ((int64_t)lb * kBlock + threadIdx.x) / G;
     const int sub = (int)(threadIdx.x % G);
-    const bool valid = p < p1;
-    // the state and the particle's entry range are loaded together: neither waits for the other
+    // the state and the first particle's entry range are loaded together: neither waits for the other
     if (ctrl) ctrl += sc;
     const int done = ctrl ? ctrl->done : 0;
     const int s = (use_slot >= 0) ? use_slot : (ctrl ? ctrl->trial : 0);
-    const int e0 = valid ? tstart[p] : 0, e1 = valid ? tstart[p + 1] : 0;
+    int e0 = p < p1 ? tstart[p] : 0, e1 = p < p1 ? tstart[p + 1] : 0;
     if (done) return;
     if (ctrl && partial_trial && lb == 0) {
         double tot[8];
@@ -428,68 +434,73 @@ __global__ void __launch_bounds__(kBlock) k_bt(ApgdCtrl* __restrict__ ctrl, int 
     }
     const T* lam = slot<T>(B.lam, s, NR * B.cap);
     T* w = slot<T>(B.w, s, 6 * B.pcap);
-    T acc[NW];
+    double q[1] = {0.0};
+    // a lane group shares p, so each group passes through the loop as a whole (the shuffles
+    // below stay within it)
+    for (; p < p1; p += pstep) {
+        T acc[NW];
 #pragma unroll
-    for (int k = 0; k < NW; ++k) acc[k] = (T)0;
-    // two entries per iteration, every load issued before the first use (the same
-    // accumulation order as one entry at a time)
-    for (int e = e0 + sub; e < e1; e += 2 * G) {
-        const int eb = e + G;
-        const bool hb = eb < e1;
-        const int32_t csa = tc[e];
-        const int32_t csb = hb ? tc[eb] : csa;
-        const int64_t ca = csa < 0 ? ~csa : csa, cb = csb < 0 ? ~csb : csb;
-        const T* tva = B.tcoef + (int64_t)e * NT;
-        const T* tvb = B.tcoef + (int64_t)(hb ? eb : e) * NT;
-        T ta[NT], tb[NT];
+        for (int k = 0; k < NW; ++k) acc[k] = (T)0;
+        // two entries per iteration, every load issued before the first use (the same
+        // accumulation order as one entry at a time)
+        for (int e = e0 + sub; e < e1; e += 2 * G) {
+            const int eb = e + G;
+            const bool hb = eb < e1;
+            const int32_t csa = tc[e];
+            const int32_t csb = hb ? tc[eb] : csa;
+            const int64_t ca = csa < 0 ? ~csa : csa, cb = csb < 0 ? ~csb : csb;
+            const T* tva = B.tcoef + (int64_t)e * NT;
+            const T* tvb = B.tcoef + (int64_t)(hb ? eb : e) * NT;
+            T ta[NT], tb[NT];
 #pragma unroll
-        for (int k = 0; k < NT; ++k) ta[k] = tva[k], tb[k] = tvb[k];
-        if (LAY == kRotFree || LAY == kRigid) {
-            const T la = lam[ca], lb = lam[cb];
+            for (int k = 0; k < NT; ++k) ta[k] = tva[k], tb[k] = tvb[k];
+            if (LAY == kRotFree || LAY == kRigid) {
+                const T la = lam[ca], lb = lam[cb];
 #pragma unroll
-            for (int k = 0; k < NT; ++k) acc[k] += la * ta[k];
-            if (hb)
+                for (int k = 0; k < NT; ++k) acc[k] += la * ta[k];
+                if (hb)
 #pragma unroll
-                for (int k = 0; k < NT; ++k) acc[k] += lb * tb[k];
-        } else {
-            T a0, a1, a2, b0, b1, b2;
-            if (LAY == kVisc) {
-                visc_global<T>(B, lam, ca, a0, a1, a2);
-                visc_global<T>(B, lam, cb, b0, b1, b2);
+                    for (int k = 0; k < NT; ++k) acc[k] += lb * tb[k];
             } else {
-                a0 = lam[ca * 3], a1 = lam[ca * 3 + 1], a2 = lam[ca * 3 + 2];
-                b0 = lam[cb * 3], b1 = lam[cb * 3 + 1], b2 = lam[cb * 3 + 2];
-            }
-            if (csa < 0) {
-                acc[0] -= a0, acc[1] -= a1, acc[2] -= a2;
-            } else {
-                acc[0] += a0, acc[1] += a1, acc[2] += a2;
-            }
-#pragma unroll
-            for (int k = 0; k < 3; ++k) acc[3 + k] += a0 * ta[k] + a1 * ta[3 + k] + a2 * ta[6 + k];
-            if (hb) {
-                if (csb < 0) {
-                    acc[0] -= b0, acc[1] -= b1, acc[2] -= b2;
+                T a0, a1, a2, b0, b1, b2;
+                if (LAY == kVisc) {
+                    visc_global<T>(B, lam, ca, a0, a1, a2);
+                    visc_global<T>(B, lam, cb, b0, b1, b2);
                 } else {
-                    acc[0] += b0, acc[1] += b1, acc[2] += b2;
+                    a0 = lam[ca * 3], a1 = lam[ca * 3 + 1], a2 = lam[ca * 3 + 2];
+                    b0 = lam[cb * 3], b1 = lam[cb * 3 + 1], b2 = lam[cb * 3 + 2];
+                }
+                if (csa < 0) {
+                    acc[0] -= a0, acc[1] -= a1, acc[2] -= a2;
+                } else {
+                    acc[0] += a0, acc[1] += a1, acc[2] += a2;
                 }
 #pragma unroll
-                for (int k = 0; k < 3; ++k) acc[3 + k] += b0 * tb[k] + b1 * tb[3 + k] + b2 * tb[6 + k];
+                for (int k = 0; k < 3; ++k) acc[3 + k] += a0 * ta[k] + a1 * ta[3 + k] + a2 * ta[6 + k];
+                if (hb) {
+                    if (csb < 0) {
+                        acc[0] -= b0, acc[1] -= b1, acc[2] -= b2;
+                    } else {
+                        acc[0] += b0, acc[1] += b1, acc[2] += b2;
+                    }
+#pragma unroll
+                    for (int k = 0; k < 3; ++k) acc[3 + k] += b0 * tb[k] + b1 * tb[3 + k] + b2 * tb[6 + k];
+                }
             }
         }
+#pragma unroll
+        for (int o = G / 2; o > 0; o >>= 1)
+#pragma unroll
+            for (int k = 0; k < NW; ++k) acc[k] += __shfl_xor(acc[k], o, G);
+        if (sub == 0)
+#pragma unroll
+            for (int k = 0; k < NW; ++k) {
+                const T wk = B.minv[p * NW + k] * acc[k];
+                w[p * NW + k] = wk;
+                q[0] += (double)acc[k] * (double)wk;
+            }
+        if (p + pstep < p1) e0 = tstart[p + pstep], e1 = tstart[p + pstep + 1];
     }
-#pragma unroll
-    for (int o = G / 2; o > 0; o >>= 1)
-#pragma unroll
-        for (int k = 0; k < NW; ++k) acc[k] += __shfl_xor(acc[k], o, G);
-    double q[1] = {0.0};
-    if (valid && sub == 0)
-#pragma unroll
-        for (int k = 0; k < NW; ++k) {
-            const T wk = B.minv[p * NW + k] * acc[k];
-            w[p * NW + k] = wk;
-            q[0] += (double)acc[k] * (double)wk;
-        }
     if (ctrl) {
         block_sum<1>(q, lds);
         if (threadIdx.x == 0) partial_bt[blockIdx.x] = q[0];
