@@ -530,41 +530,42 @@ static void launch_bt(int G, hipStream_t s, ApgdCtrl* ctrl, int use_slot, const 
     }
 }
 
-// w at the contact's two particles (NW each; j < 0 is an obstacle, only in periodic boxes)
 template <class T, int LAY>
-__device__ inline void gather_w(const T* __restrict__ w, int64_t i, int64_t j, T (&wi)[lay<LAY>::NW],
-                                T (&wj)[lay<LAY>::NW]) {
-    constexpr int NW = lay<LAY>::NW;
+__device__ inline void row_product(const BMatrix<T>& B, const T* __restrict__ w, int64_t c, int64_t i, int64_t j,
+                                   T dt2, T (&q)[3]) {
+    constexpr int NC = lay<LAY>::NC, NW = lay<LAY>::NW;
+    T wi[NW], wj[NW];
 #pragma unroll
     for (int k = 0; k < NW; ++k) wi[k] = (i >= 0) ? w[i * NW + k] : (T)0;
 #pragma unroll
-    for (int k = 0; k < NW; ++k) wj[k] = (j >= 0) ? w[j * NW + k] : (T)0;
-}
-
-// Q lambda_s at contact c (the 1..4 local rows NR) from w_s = M^-1 B^T lambda_s at its two
-// particles: dt^2 B w_s (AMatrix::mat_mult, velocities.hpp:67-144), then global2local for kVisc.
-template <class T, int LAY>
-__device__ inline void q_rows(const BMatrix<T>& B, int64_t c, const T (&wi)[lay<LAY>::NW],
-                              const T (&wj)[lay<LAY>::NW], T dt2, T (&q)[lay<LAY>::NR]) {
-    constexpr int NR = lay<LAY>::NR, NC = lay<LAY>::NC;
+    for (int k = 0; k < NW; ++k) wj[k] = (j >= 0) ? w[j * NW + k] : (T)0;  // j is an obstacle only in periodic boxes
     const T* co = B.coef + c * NC;
-    T g[3];
     if (LAY == kRotFree) {
-        g[0] = dt2 * (co[0] * (wi[0] - wj[0]) + co[1] * (wi[1] - wj[1]) + co[2] * (wi[2] - wj[2]));
+        q[0] = dt2 * (co[0] * (wi[0] - wj[0]) + co[1] * (wi[1] - wj[1]) + co[2] * (wi[2] - wj[2]));
     } else if (LAY == kRigid) {
         T acc = co[0] * (wi[0] - wj[0]) + co[1] * (wi[1] - wj[1]) + co[2] * (wi[2] - wj[2]);
         acc += co[3] * wi[3] + co[4] * wi[4] + co[5] * wi[5];
         acc -= co[6] * wj[3] + co[7] * wj[4] + co[8] * wj[5];
-        g[0] = dt2 * acc;
+        q[0] = dt2 * acc;
     } else {
 #pragma unroll
         for (int d = 0; d < 3; ++d) {
             T acc = wi[d] - wj[d];
             acc += co[3 + 3 * d] * wi[3] + co[4 + 3 * d] * wi[4] + co[5 + 3 * d] * wi[5];
             acc -= co[12 + 3 * d] * wj[3] + co[13 + 3 * d] * wj[4] + co[14 + 3 * d] * wj[5];
-            g[d] = dt2 * acc;
+            q[d] = dt2 * acc;
         }
     }
+}
+
+// Q lambda_s at contact c (the 1..4 local rows NR) from w_s = M^-1 B^T lambda_s: dt^2 B w_s,
+// then global2local for kVisc.
+template <class T, int LAY>
+__device__ inline void q_rows(const BMatrix<T>& B, const T* __restrict__ w, int64_t c, int64_t i, int64_t j, T dt2,
+                              T (&q)[lay<LAY>::NR]) {
+    constexpr int NR = lay<LAY>::NR;
+    T g[3];
+    row_product<T, LAY>(B, w, c, i, j, dt2, g);
     if constexpr (LAY == kVisc) {
         visc_local<T>(B, c, g, q);
     } else {
@@ -591,58 +592,28 @@ __global__ void __launch_bounds__(kBlock) k_trial(const ApgdCtrl* __restrict__ i
     __shared__ ApgdCtrl st;
     const SceneBlock sb = scene_block(M, nc);
     // U contacts c + u cstep per iteration (2 for the rotation-free layout, 1 for the others,
-    // whose products need more registers)
+    // whose products need more registers); the first contacts' particle indices are loaded
+    // together with the state and the block sums, each next group one iteration ahead
     constexpr int U = LAY == kRotFree ? 2 : 1;
-    constexpr int NW = lay<LAY>::NW;
     const int64_t cstep = (int64_t)sb.nb * kBlock;
-    const int64_t stride = NR * B.cap, wstride = 6 * B.pcap;
     int64_t c = sb.lo + (int64_t)sb.lb * kBlock + threadIdx.x;
-    // One group of U contacts per iteration: its rows, w at its particles and Q lambda_{k-1}
-    // (`load`), loaded before the group is computed.  The first group is loaded before the
-    // control step, for the slots the pending trial gives if it is accepted (most are), and
-    // loaded again after a rejection; each next group's particle indices one iteration ahead.
-    int32_t i0[U], j0[U], inx[U], jnx[U];
+    int32_t inx[U], jnx[U];
 #pragma unroll
     for (int u = 0; u < U; ++u) {
-        i0[u] = j0[u] = 0;
-        if (c + u * cstep < sb.hi) i0[u] = ci[c + u * cstep], j0[u] = cj[c + u * cstep];
+        inx[u] = jnx[u] = 0;
+        if (c + u * cstep < sb.hi) inx[u] = ci[c + u * cstep], jnx[u] = cj[c + u * cstep];
     }
-    int64_t cu[U];
-    bool hu[U];
-    T l0[U][NR], l1[U][NR], cv[U][NR], q1[U][NR], wi[U][NW], wj[U][NW];
-    auto load = [&](int cur_, int prev_, int64_t c0, const int32_t (&ii)[U], const int32_t (&jj)[U]) {
-        const T* lc_ = slot<T>(B.lam, cur_, stride);
-        const T* lp_ = slot<T>(B.lam, prev_, stride);
-        const T* qp_ = slot<T>(B.qlam, prev_, stride);
-        const T* wc_ = slot<T>(B.w, cur_, wstride);
-#pragma unroll
-        for (int u = 0; u < U; ++u) {
-            hu[u] = c0 + u * cstep < sb.hi;
-            cu[u] = hu[u] ? c0 + u * cstep : c0;  // past the end: a duplicate of c0, not stored
-#pragma unroll
-            for (int d = 0; d < NR; ++d) {
-                const int64_t r = cu[u] * NR + d;
-                l0[u][d] = lc_[r], l1[u][d] = lp_[r], cv[u][d] = B.c[r], q1[u][d] = qp_[r];
-            }
-            gather_w<T, LAY>(wc_, (int64_t)ii[u] - ni, (int64_t)jj[u] - ni, wi[u], wj[u]);
-        }
-    };
-    const ApgdCtrl* ip = in + sb.sc;
-    const bool spec = ip->pending && !ip->done;
-    const int scur = spec ? ip->trial : ip->cur, sprev = spec ? ip->cur : ip->prev;
-    if (c < sb.hi) load(scur, sprev, c, i0, j0);
-#pragma unroll
-    for (int u = 0; u < U; ++u) {
-        inx[u] = i0[u], jnx[u] = j0[u];
-        if (c + (U + u) * cstep < sb.hi) inx[u] = ci[c + (U + u) * cstep], jnx[u] = cj[c + (U + u) * cstep];
-    }
-    control_step(ip, out + sb.sc, partial_bt + (M.rblk ? M.bfirst[sb.sc] : 0), M.rblk ? M.bcount[sb.sc] : nbt,
+    control_step(in + sb.sc, out + sb.sc, partial_bt + (M.rblk ? M.bfirst[sb.sc] : 0), M.rblk ? M.bcount[sb.sc] : nbt,
                  st, lds, sb.lb == 0, M.done ? M.done + sb.sc : nullptr);
     if (st.done) return;
     const int cur = st.cur, prev = st.prev, trial = st.trial;
-    if (c < sb.hi && (cur != scur || prev != sprev)) load(cur, prev, c, i0, j0);
+    const int64_t stride = NR * B.cap, wstride = 6 * B.pcap;
     const T beta = (T)st.beta, alpha = (T)st.alpha, dt2 = (T)st.dt2;
+    const T* lc = slot<T>(B.lam, cur, stride);
+    const T* lp = slot<T>(B.lam, prev, stride);
+    const T* wc = slot<T>(B.w, cur, wstride);
     T* qc = slot<T>(B.qlam, cur, stride);
+    const T* qp = slot<T>(B.qlam, prev, stride);
     T* lt = slot<T>(B.lam, trial, stride);
     double s[8] = {0, 0, 0, 0, 0, 0, 0, 0};
     // contact c: rows from (lambda_k, lambda_{k-1}, c) and the two Q products
@@ -701,9 +672,27 @@ __global__ void __launch_bounds__(kBlock) k_trial(const ApgdCtrl* __restrict__ i
         }
     };
     for (; c < sb.hi; c += U * cstep) {
+        int64_t cu[U], iu[U], ju[U];
+        bool hu[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            hu[u] = c + u * cstep < sb.hi;
+            cu[u] = hu[u] ? c + u * cstep : c;  // past the end: a duplicate of c, not stored
+            iu[u] = (int64_t)inx[u] - ni, ju[u] = (int64_t)jnx[u] - ni;
+            const int64_t cn = c + (U + u) * cstep;
+            if (cn < sb.hi) inx[u] = ci[cn], jnx[u] = cj[cn];
+        }
+        T l0[U][NR], l1[U][NR], cv[U][NR], q1[U][NR];
+#pragma unroll
+        for (int u = 0; u < U; ++u)
+#pragma unroll
+            for (int d = 0; d < NR; ++d) {
+                const int64_t r = cu[u] * NR + d;
+                l0[u][d] = lc[r], l1[u][d] = lp[r], cv[u][d] = B.c[r], q1[u][d] = qp[r];
+            }
         T q0[U][NR];
 #pragma unroll
-        for (int u = 0; u < U; ++u) q_rows<T, LAY>(B, cu[u], wi[u], wj[u], dt2, q0[u]);
+        for (int u = 0; u < U; ++u) q_rows<T, LAY>(B, wc, cu[u], iu[u], ju[u], dt2, q0[u]);
 #pragma unroll
         for (int u = 0; u < U; ++u)
             if (hu[u]) {
@@ -711,13 +700,6 @@ __global__ void __launch_bounds__(kBlock) k_trial(const ApgdCtrl* __restrict__ i
                 for (int d = 0; d < NR; ++d) qc[cu[u] * NR + d] = q0[u][d];
                 one(cu[u], l0[u], l1[u], cv[u], q0[u], q1[u]);
             }
-        const int64_t cn = c + U * cstep;
-        if (cn < sb.hi) {
-            load(cur, prev, cn, inx, jnx);
-#pragma unroll
-            for (int u = 0; u < U; ++u)
-                if (cn + (U + u) * cstep < sb.hi) inx[u] = ci[cn + (U + u) * cstep], jnx[u] = cj[cn + (U + u) * cstep];
-        }
     }
     block_sum<8>(s, lds);
     if (threadIdx.x == 0)
