@@ -1553,9 +1553,11 @@ int64_t scopi_hip_scene_footprint(int64_t n, int64_t n_inactive, const scopi_hip
     const int64_t na = n - n_inactive;
     const bool fp32 = params && params->precision == SCOPI_PRECISION_FP32_REFINE;
     const int64_t nc = 12 * na;
-    // particles: state (31 doubles + tag), solver per particle, search (keys, units, cells)
+    // particles: state (31 doubles + tag), solver per particle (M^-1 and three slots of w, 6 values
+    // each, per precision), search (keys, units, cells)
     const int64_t per_particle = 31 * 8 + 4;
-    const int64_t per_active = (6 + 6 + 6) * 4 + 6 * 8 + 2 * 6 * 8 * (fp32 ? 2 : 1) + 3 * 4 + 8 * (n_inactive + 1) + 2 * 4 * 3;
+    const int64_t per_active =
+        (6 + 6 + 6) * 4 + 6 * 8 + (6 + 3 * 6) * 8 * (fp32 ? 2 : 1) + 3 * 4 + 8 * (n_inactive + 1) + 2 * 4 * 3;
     // contacts: record (96 B), B and its vectors (488 B FP64, 244 B more for FP32), transposes (24 B)
     const int64_t per_contact = 96 + 488 + (fp32 ? 244 : 0) + 24;
     const int64_t bytes = n * per_particle + na * per_active + nc * per_contact;
