@@ -84,10 +84,16 @@ def lib():
         L.oracle_se_residual.argtypes = [D] * 8 + [D, D, C.c_void_p]
         L.oracle_se_point_normal.argtypes = [D, D, D, D, C.c_double, C.c_double, D, D]
         L.oracle_set_se_math.argtypes = [C.c_int32]
+        L.oracle_set_minpack_variant.argtypes = [C.c_int32]
+        L.oracle_set_jacobian_ad.argtypes = [C.c_int32]
+        L.oracle_dogleg_fresh_wa4_count.restype = C.c_int64
+        L.oracle_gen_segment_demo.argtypes = [C.c_int32, C.c_double, D, D, D]
+        L.oracle_minpack_dogleg.argtypes = [C.c_int32, D, D, D, C.c_double, C.c_double, D, I32]
         L.oracle_newton_closest_points.argtypes = [C.c_int32, D, D, D, D, C.c_int32, D, D, D, D, D, D, D, D, D, D, I32]
         L.oracle_newton_residual.argtypes = [C.c_int32] + [D] * 8 + [D, C.c_void_p]
         L.oracle_lu_solve4.argtypes = [D, D, D]
         L.oracle_se2_closest_points.argtypes = [D, D, D, C.c_double, D, D, D, C.c_double, D, D, D, D, D, I32]
+        L.oracle_se2_residual.argtypes = [D, D, D, C.c_double, D, D, D, C.c_double, D, C.c_void_p, C.c_void_p]
         L.oracle_math.argtypes = [C.c_int32, C.c_double, C.c_double, C.POINTER(C.c_double)]
         L.oracle_quaternion.argtypes = [C.c_double, D, D]
         L.oracle_gen_2d_case_spheres.argtypes = [C.c_int32, D, D, D, D]
@@ -278,6 +284,36 @@ def set_se_math(portable: bool):
     lib().oracle_set_se_math(1 if portable else 0)
 
 
+MINPACK_ANL_ENORM, MINPACK_ANL_GIANT, MINPACK_ANL_DOGLEG, MINPACK_ANL_TERMINATION = 1, 2, 4, 8
+MINPACK_ANL = 15
+
+
+def set_minpack_variant(mask: int):
+    """Test-only: 0 = the reference's src/minpack.cpp (default); each bit takes one ANL-80-74 form
+    (MINPACK_ANL_* ; MINPACK_ANL = all of them)."""
+    lib().oracle_set_minpack_variant(int(mask))
+
+
+def set_jacobian_ad(ad: bool):
+    """Test-only: forward-mode Jacobian (True) or the reference's hand-written one (False, default)."""
+    lib().oracle_set_jacobian_ad(1 if ad else 0)
+
+
+def minpack_dogleg(r_packed, diag, qtb, delta, wa4_last):
+    """One dogleg step of the restated MINPACK; returns (x, took_gradient_branch)."""
+    n = len(diag)
+    x = np.zeros(n); g = np.zeros(1, np.int32)
+    lib().oracle_minpack_dogleg(n, np.ascontiguousarray(r_packed, np.float64), np.ascontiguousarray(diag, np.float64),
+                                np.ascontiguousarray(qtb, np.float64), float(delta), float(wa4_last), x, g)
+    return x, bool(g[0])
+
+
+def dogleg_fresh_wa4_count() -> int:
+    """Scaled-gradient dogleg steps before hybrd's first trial point (reads 0 where the reference reads
+    an uninitialised work element)."""
+    return int(lib().oracle_dogleg_fresh_wa4_count())
+
+
 MATH_FN = {"cos": 0, "sin": 1, "asin": 2, "log": 3, "exp": 4, "pow": 5}
 
 
@@ -300,6 +336,23 @@ def gen_2d_case_spheres(n=3):
     x = np.zeros(k); y = np.zeros(k); r = np.zeros(k); vel = np.zeros(k)
     L.oracle_gen_2d_case_spheres(n, x, y, r, vel)
     return x, y, r, vel
+
+
+def se2_residual(pos_i, q_i, size_i, e_i, pos_j, q_j, size_j, e_j, u, jacobian=False):
+    """Residual (and Jacobian) of the SE2-SE2 system at u = (b1, b2)."""
+    v = lambda x, n: np.ascontiguousarray(np.asarray(x, np.float64)[:n])
+    F = np.zeros(2); J = np.zeros(4)
+    lib().oracle_se2_residual(v(pos_i, 2), v(q_i, 4), v(size_i, 2), C.c_double(e_i), v(pos_j, 2), v(q_j, 4),
+                              v(size_j, 2), C.c_double(e_j), v(u, 2), F.ctypes.data_as(C.c_void_p),
+                              J.ctypes.data_as(C.c_void_p) if jacobian else None)
+    return (F, J.reshape(2, 2)) if jacobian else F
+
+
+def gen_segment_demo(n=50, max_radius=0.06):
+    """The random draws (x, y, radius) of demos/segment.cpp's spheres, std::default_random_engine."""
+    x = np.zeros(n); y = np.zeros(n); r = np.zeros(n)
+    lib().oracle_gen_segment_demo(n, max_radius, x, y, r)
+    return x, y, r
 
 
 def se2_closest_points(pos_i, q_i, size_i, e_i, pos_j, q_j, size_j, e_j):

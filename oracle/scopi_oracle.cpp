@@ -1614,6 +1614,21 @@ int oracle_se2_closest_points(const double* pos_i, const double* q_i, const doub
     return 0;
 }
 
+int oracle_se2_residual(const double* pos_i, const double* q_i, const double* size_i, double e_i,
+                        const double* pos_j, const double* q_j, const double* size_j, double e_j, const double* u,
+                        double* F, double* J) {
+    const std::array<double, 2> si{e_i, 1.0}, sj{e_j, 1.0};
+    const se_oracle::SE a = make_se2({pos_i[0], pos_i[1], 0.0}, {q_i[0], q_i[1], q_i[2], q_i[3]},
+                                     {size_i[0], size_i[1], 0.0}, si);
+    const se_oracle::SE b = make_se2({pos_j[0], pos_j[1], 0.0}, {q_j[0], q_j[1], q_j[2], q_j[3]},
+                                     {size_j[0], size_j[1], 0.0}, sj);
+    if (g_se_math)
+        se_oracle::se2_se2_residual<se_oracle::PortableMath>(a, b, u, F, J);
+    else
+        se_oracle::se2_se2_residual<se_oracle::LibMath>(a, b, u, F, J);
+    return 0;
+}
+
 int oracle_lu_solve4(const double* A, const double* b, double* x) {
     double M[16];
     for (int k = 0; k < 16; ++k) M[k] = A[k];
@@ -1635,6 +1650,32 @@ int oracle_se_residual(const double* pos_i, const double* q_i, const double* siz
 
 int oracle_set_se_math(int32_t portable) {
     g_se_math = portable ? 1 : 0;
+    return 0;
+}
+
+int oracle_set_minpack_variant(int32_t mask) {
+    se_oracle::minpack_variant() = mask & 15;
+    return 0;
+}
+
+int oracle_set_jacobian_ad(int32_t ad) {
+    se_oracle::jacobian_ad() = ad ? 1 : 0;
+    return 0;
+}
+
+int64_t oracle_dogleg_fresh_wa4_count(void) { return se_oracle::dogleg_fresh_wa4_count(); }
+
+int oracle_minpack_dogleg(int32_t n, const double* r, const double* diag, const double* qtb, double delta,
+                          double wa4_last, double* x, int32_t* gradient_branch) {
+    double wa1[4], wa2[4];
+    bool g = false;
+    if (n == 4)
+        g = se_oracle::Minpack<4>::dogleg(r, diag, qtb, delta, x, wa1, wa2, wa4_last);
+    else if (n == 2)
+        g = se_oracle::Minpack<2>::dogleg(r, diag, qtb, delta, x, wa1, wa2, wa4_last);
+    else
+        return -1;
+    *gradient_branch = g ? 1 : 0;
     return 0;
 }
 
@@ -1696,6 +1737,21 @@ int oracle_gen_2d_case_spheres(int32_t n, double* x, double* y, double* r, doubl
             x[k] = xx, y[k] = yy, r[k] = rr, velocity[k] = -vel;
             ++k;
         }
+    return 0;
+}
+
+/* Draws of demos/segment.cpp:30-44: std::default_random_engine (default seed), per particle
+ * x ~ U(0.1, 0.9), y ~ U(0, 0.7), radius ~ U(0.01, max_radius), in that order. */
+int oracle_gen_segment_demo(int32_t n, double max_radius, double* x, double* y, double* r) {
+    std::default_random_engine generator;
+    std::uniform_real_distribution<double> distrib_x(0.1, 0.9);
+    std::uniform_real_distribution<double> distrib_y(0, 0.7);
+    std::uniform_real_distribution<double> distrib_r(0.01, max_radius);
+    for (int i = 0; i < n; ++i) {
+        x[i] = distrib_x(generator);
+        y[i] = distrib_y(generator);
+        r[i] = distrib_r(generator);
+    }
     return 0;
 }
 

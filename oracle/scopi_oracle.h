@@ -149,6 +149,10 @@ int oracle_newton_residual(int32_t kind, const double* pos_i, const double* q_i,
 int oracle_se2_closest_points(const double* pos_i, const double* q_i, const double* size_i, double e_i,
                               const double* pos_j, const double* q_j, const double* size_j, double e_j, double* dij,
                               double* nij, double* pi, double* pj, double* u, int32_t* info);
+/* Residual F(u) (2) and Jacobian (2x2 row-major, may be NULL) of the SE2-SE2 system. */
+int oracle_se2_residual(const double* pos_i, const double* q_i, const double* size_i, double e_i,
+                        const double* pos_j, const double* q_j, const double* size_j, double e_j, const double* u,
+                        double* F, double* J);
 /* The 4x4 linear solve of newton_method (LU with partial pivoting), A row-major. */
 int oracle_lu_solve4(const double* A, const double* b, double* x);
 /* Residual F(u) (4) and Jacobian (4x4 row-major, may be NULL) of the SE3-SE3 system. */
@@ -158,6 +162,19 @@ int oracle_se_residual(const double* pos_i, const double* q_i, const double* siz
 /* Elementary functions of the superellipsoid solve: 0 = C library (default),
  * 1 = the portable functions of se_math.hpp, which the device evaluates identically. */
 int oracle_set_se_math(int32_t portable);
+/* Test-only forms of the superellipsoid solve.  MINPACK: 0 = the reference's src/minpack.cpp
+ * (default); each set bit takes one ANL-80-74 form instead: 1 enorm, 2 giant, 4 dogleg's
+ * scaled-gradient loop, 8 termination order and the delta start (15: ANL-80-74).  Jacobian of the
+ * SE3-SE3 and SE2-SE2 systems: 0 = the reference's hand-written one (default), 1 = forward-mode
+ * differentiation of F.  The counter reports dogleg scaled-gradient steps taken before hybrd's
+ * first trial point, where the reference reads an uninitialised work element (0 here). */
+int oracle_set_minpack_variant(int32_t mask);
+int oracle_set_jacobian_ad(int32_t ad);
+int64_t oracle_dogleg_fresh_wa4_count(void);
+/* One dogleg call (n = 2 or 4): r packed upper triangle by rows, wa4_last the value the
+ * reference's loop reads before r (hybrd's wa4[n-1]); x receives the step. */
+int oracle_minpack_dogleg(int32_t n, const double* r, const double* diag, const double* qtb, double delta,
+                          double wa4_last, double* x, int32_t* gradient_branch);
 /* One portable elementary function: fn 0 cos, 1 sin, 2 asin, 3 log, 4 exp, 5 pow(x, y). */
 int oracle_math(int32_t fn, double x, double y, double* out);
 /* superellipsoid::point / normal (a, b). */
@@ -167,6 +184,8 @@ int oracle_se_point_normal(const double* pos, const double* q, const double* siz
 /* quaternion(angle, axis) of src/quaternion.cpp:12-26. */
 int oracle_quaternion(double angle, const double* axis, double* q);
 
+/* Draws of the spheres of demos/segment.cpp:30-44 (std::default_random_engine). */
+int oracle_gen_segment_demo(int32_t n, double max_radius, double* x, double* y, double* r);
 /* Scene generator of test/test_sphere.cpp:423-470 (std::minstd_rand0(123)). */
 int oracle_gen_2d_case_spheres(int32_t n, double* x, double* y, double* r, double* velocity);
 

@@ -9,11 +9,18 @@
 //                   F(a1, b1, a2, b2) = [(p2 - p1) x n1 ; n1.n2 + |n1||n2|] (:284-382),
 //                   hybrd1 (minpack.hpp:851-923: mode 2, unit diagonal, factor 100,
 //                   xtol 1e-10, maxfev 200 (n + 1)), result (:724-728)
-//   Powell hybrid   MINPACK hybrd / enorm / qrfac / qform / dogleg / r1updt / r1mpyq,
-//                   restated from the published algorithm (More, Garbow, Hillstrom,
-//                   ANL-80-74, 1980).  The Jacobian is evaluated exactly by forward-mode
-//                   automatic differentiation of F (the reference hand-codes the same
-//                   derivative, closest_points.hpp:385-603).
+//   Powell hybrid   MINPACK hybrd / enorm / qrfac / qform / dogleg / r1updt / r1mpyq as the
+//                   reference's C++ translation runs them (src/minpack.cpp, minpack.hpp:337-923):
+//                   enorm a plain sqrt of the sum of squares, r1updt's giant = r8_huge() = 1e30,
+//                   the first termination test that holds sets info, and dogleg's scaled-gradient
+//                   loop with its index shift and its left-to-right (delta/qnorm)(sgnorm/delta)^2
+//                   (Minpack::dogleg).  A test-only switch
+//                   (minpack_variant(), a bit mask) selects forms of the published algorithm (More,
+//                   Garbow, Hillstrom, ANL-80-74, 1980) instead, which an independent MINPACK
+//                   (scipy's hybrj) reproduces.
+//   Jacobian        the reference's hand-written derivative (closest_points.hpp:385-603; 2D
+//                   :127-189), term by term in its evaluation order; forward-mode
+//                   differentiation of F is kept as a test-side cross-check (jacobian_ad() = 1).
 #pragma once
 
 #include <algorithm>
@@ -207,24 +214,57 @@ inline D4 spow(const D4& a, double p) {
     for (int k = 0; k < 4; ++k) r.d[k] = dv * a.d[k];
     return r;
 }
+// sign(x)|x|^(2-s) and its derivative (2-s)|x|^(1-s) x', the exponent 1 - s formed as the reference
+// writes it (closest_points.hpp:430, 1100)
+template <class M>
+inline D4 spow2(const D4& a, double s) {
+    const double ax = std::abs(a.v);
+    D4 r{sgn(a.v) * M::pow(ax, 2 - s), {}};
+    const double dv = (2 - s) * M::pow(ax, 1 - s);
+    for (int k = 0; k < 4; ++k) r.d[k] = dv * a.d[k];
+    return r;
+}
 
 // Residual of closest_points.hpp:284-382 with its Jacobian.
 struct SEPair {
     SE s1, s2;
 };
 
+// Test-only switches of the restatement.  MINPACK forms, a bit mask (0: the reference's
+// src/minpack.cpp throughout; each set bit takes one ANL-80-74 form instead): 1 = scaled enorm,
+// 2 = giant = DBL_MAX in r1updt, 4 = dogleg's scaled-gradient loop without the index shift and
+// (delta/qnorm)(sgnorm/delta)^2 with the square taken first,
+// 8 = every termination test runs and the last that holds sets info, delta = factor xnorm
+// replaced by factor when it is 0.  Jacobian: 0 = the reference's hand-written one,
+// 1 = forward-mode AD.
+inline int& minpack_variant() {
+    static int v = 0;
+    return v;
+}
+inline int& jacobian_ad() {
+    static int v = 0;
+    return v;
+}
+// Number of dogleg calls that took the scaled-gradient branch before hybrd had evaluated a trial
+// point: there the reference reads wa4[n-1] of a fresh, uninitialised work array (see dogleg);
+// the restatement reads 0.
+inline long& dogleg_fresh_wa4_count() {
+    static long v = 0;
+    return v;
+}
+
 template <class M>
-inline void se_se_residual(const SEPair& P, const double* u, double* F, double* J /* 4x4 row-major or null */) {
+inline void se_se_residual_ad(const SEPair& P, const double* u, double* F, double* J /* 4x4 row-major or null */) {
     // Values follow the reference's expression order (left-to-right products and
     // sums, closest_points.hpp:323-381); derivatives ride along by forward-mode AD.
     D4 a1 = var(u[0], 0), b1 = var(u[1], 1), a2 = var(u[2], 2), b2 = var(u[3], 3);
     const SE &s1 = P.s1, &s2 = P.s2;
     D4 ca1 = dcos<M>(a1), sa1 = dsin<M>(a1), cb1 = dcos<M>(b1), sb1 = dsin<M>(b1);
     D4 ca2 = dcos<M>(a2), sa2 = dsin<M>(a2), cb2 = dcos<M>(b2), sb2 = dsin<M>(b2);
-    D4 ca1n = spow<M>(ca1, s1.n), ca1n2 = spow<M>(ca1, 2 - s1.n), sa1n = spow<M>(sa1, s1.n), sa1n2 = spow<M>(sa1, 2 - s1.n);
-    D4 cb1e = spow<M>(cb1, s1.e), cb1e2 = spow<M>(cb1, 2 - s1.e), sb1e = spow<M>(sb1, s1.e), sb1e2 = spow<M>(sb1, 2 - s1.e);
-    D4 ca2n = spow<M>(ca2, s2.n), ca2n2 = spow<M>(ca2, 2 - s2.n), sa2n = spow<M>(sa2, s2.n), sa2n2 = spow<M>(sa2, 2 - s2.n);
-    D4 cb2e = spow<M>(cb2, s2.e), cb2e2 = spow<M>(cb2, 2 - s2.e), sb2e = spow<M>(sb2, s2.e), sb2e2 = spow<M>(sb2, 2 - s2.e);
+    D4 ca1n = spow<M>(ca1, s1.n), ca1n2 = spow2<M>(ca1, s1.n), sa1n = spow<M>(sa1, s1.n), sa1n2 = spow2<M>(sa1, s1.n);
+    D4 cb1e = spow<M>(cb1, s1.e), cb1e2 = spow2<M>(cb1, s1.e), sb1e = spow<M>(sb1, s1.e), sb1e2 = spow2<M>(sb1, s1.e);
+    D4 ca2n = spow<M>(ca2, s2.n), ca2n2 = spow2<M>(ca2, s2.n), sa2n = spow<M>(sa2, s2.n), sa2n2 = spow2<M>(sa2, s2.n);
+    D4 cb2e = spow<M>(cb2, s2.e), cb2e2 = spow2<M>(cb2, s2.e), sb2e = spow<M>(sb2, s2.e), sb2e2 = spow2<M>(sb2, s2.e);
     // body-frame normal (A1..A3) and point (A4..A6) of s1; point (B1..B3) and normal (B4..B6) of s2
     const D4 A1 = ((s1.r[1] * s1.r[2]) * ca1n2) * cb1e2, A2 = ((s1.r[0] * s1.r[2]) * sb1e2) * ca1n2,
              A3 = (s1.r[0] * s1.r[1]) * sa1n2;
@@ -257,9 +297,157 @@ inline void se_se_residual(const SEPair& P, const double* u, double* F, double* 
     }
 }
 
+// Terms of one body at the angles (a, b), closest_points.hpp:426-446: cosines and sines, the
+// signed powers of the residual (sign(x)|x|^p, p = n, 2 - n for a and e, 2 - e for b) and the
+// unsigned factors of its derivative (n|x|^(n-1), (2-n)|x|^(1-n), e|x|^(e-1), (2-e)|x|^(1-e)).
+struct BodyTerms {
+    double ca, sa, cb, sb;
+    double can, can2, san, san2, cbe, cbe2, sbe, sbe2;
+    double can1, can3, san1, san3, cbe1, cbe3, sbe1, sbe3;
+};
+template <class M>
+inline BodyTerms body_terms(const SE& s, double a, double b, bool deriv) {
+    BodyTerms t{};
+    t.ca = M::cos(a);
+    t.sa = M::sin(a);
+    t.cb = M::cos(b);
+    t.sb = M::sin(b);
+    const double aca = std::abs(t.ca), asa = std::abs(t.sa), acb = std::abs(t.cb), asb = std::abs(t.sb);
+    t.can = sgn(t.ca) * M::pow(aca, s.n);
+    t.can2 = sgn(t.ca) * M::pow(aca, 2 - s.n);
+    t.san = sgn(t.sa) * M::pow(asa, s.n);
+    t.san2 = sgn(t.sa) * M::pow(asa, 2 - s.n);
+    t.cbe = sgn(t.cb) * M::pow(acb, s.e);
+    t.cbe2 = sgn(t.cb) * M::pow(acb, 2 - s.e);
+    t.sbe = sgn(t.sb) * M::pow(asb, s.e);
+    t.sbe2 = sgn(t.sb) * M::pow(asb, 2 - s.e);
+    if (deriv) {
+        t.can1 = s.n * M::pow(aca, s.n - 1);
+        t.can3 = (2 - s.n) * M::pow(aca, 1 - s.n);
+        t.san1 = s.n * M::pow(asa, s.n - 1);
+        t.san3 = (2 - s.n) * M::pow(asa, 1 - s.n);
+        t.cbe1 = s.e * M::pow(acb, s.e - 1);
+        t.cbe3 = (2 - s.e) * M::pow(acb, 1 - s.e);
+        t.sbe1 = s.e * M::pow(asb, s.e - 1);
+        t.sbe3 = (2 - s.e) * M::pow(asb, 1 - s.e);
+    }
+    return t;
+}
+
+// Body-frame normal n (A1..A3) and point p (A4..A6) of closest_points.hpp:448-453 and the
+// derivative terms: along a, pa = A7..A9 and na = A10..A12; along b, pb = A13, A14 and
+// nb = A15, A16 (:454-463).  The second body's B1..B16 (:496-518) are the same forms:
+// B1..B3 = p, B4..B6 = n, B7..B9 = pa, B10, B11 = pb, B12..B14 = na, B15, B16 = nb.
+struct BodyForms {
+    double n[3], p[3];
+    double pa[3], na[3], pb[2], nb[2];
+};
+inline BodyForms body_forms(const SE& s, const BodyTerms& t, bool deriv) {
+    const double rx = s.r[0], ry = s.r[1], rz = s.r[2];
+    BodyForms f{};
+    f.n[0] = ((ry * rz) * t.can2) * t.cbe2;
+    f.n[1] = ((rx * rz) * t.sbe2) * t.can2;
+    f.n[2] = (rx * ry) * t.san2;
+    f.p[0] = (rx * t.can) * t.cbe;
+    f.p[1] = (ry * t.sbe) * t.can;
+    f.p[2] = rz * t.san;
+    if (deriv) {
+        f.pa[0] = ((rx * t.sa) * t.can1) * t.cbe;
+        f.pa[1] = ((ry * t.sa) * t.sbe) * t.can1;
+        f.pa[2] = (rz * t.ca) * t.san1;
+        f.na[0] = (((ry * rz) * t.sa) * t.can3) * t.cbe2;
+        f.na[1] = (((rx * rz) * t.sa) * t.sbe2) * t.can3;
+        f.na[2] = ((rx * ry) * t.ca) * t.san3;
+        f.pb[0] = ((rx * t.sb) * t.can) * t.cbe1;
+        f.pb[1] = ((ry * t.cb) * t.sbe1) * t.can;
+        f.nb[0] = (((ry * rz) * t.sb) * t.can2) * t.cbe3;
+        f.nb[1] = (((rx * rz) * t.cb) * t.sbe3) * t.can2;
+    }
+    return f;
+}
+
+// F (closest_points.hpp:284-382) and the reference's Jacobian (:385-603), row-major.  Every
+// factor of the reference's entries is a rotated sum of these forms or its negation, and
+// negating a sum is exact: -M00 A10 - M01 A11 + M02 A12 = -((M00 A10 + M01 A11) - M02 A12)
+// bit for bit.  With q the row of the rotation:
+//   pa1[q] = (Mq0 A7 + Mq1 A8) - Mq2 A9      na1[q] = (Mq0 A10 + Mq1 A11) - Mq2 A12
+//   pb1[q] = Mq0 A13 - Mq1 A14               nb1[q] = Mq0 A15 - Mq1 A16
+//   pa2[q] = (Nq0 B7 + Nq1 B8) - Nq2 B9      na2[q] = (Nq0 B12 + Nq1 B13) - Nq2 B14
+//   pb2[q] = Nq0 B10 - Nq1 B11               nb2[q] = Nq0 B15 - Nq1 B16
+// and every entry is summed left to right as the reference writes it.
+template <class M>
+inline void se_se_residual(const SEPair& P, const double* u, double* F, double* J /* 4x4 row-major or null */) {
+    if (J && jacobian_ad()) {
+        se_se_residual_ad<M>(P, u, F, J);
+        return;
+    }
+    const SE &s1 = P.s1, &s2 = P.s2;
+    const bool deriv = J != nullptr;
+    const BodyForms A = body_forms(s1, body_terms<M>(s1, u[0], u[1], deriv), deriv);
+    const BodyForms B = body_forms(s2, body_terms<M>(s2, u[2], u[3], deriv), deriv);
+    const m33 &Mr = s1.R, &Nr = s2.R;
+    double n1[3], n2[3], d[3];
+    for (int q = 0; q < 3; ++q) {
+        n1[q] = (Mr[q][0] * A.n[0] + Mr[q][1] * A.n[1]) + Mr[q][2] * A.n[2];
+        n2[q] = (Nr[q][0] * B.n[0] + Nr[q][1] * B.n[1]) + Nr[q][2] * B.n[2];
+        d[q] = (((((((-(Mr[q][0] * A.p[0])) - Mr[q][1] * A.p[1]) - Mr[q][2] * A.p[2]) + Nr[q][0] * B.p[0]) +
+                  Nr[q][1] * B.p[1]) +
+                 Nr[q][2] * B.p[2]) -
+                s1.c[q]) +
+               s2.c[q];
+    }
+    F[0] = (-n1[1]) * d[2] + n1[2] * d[1];
+    F[1] = n1[0] * d[2] - n1[2] * d[0];
+    F[2] = (-n1[0]) * d[1] + n1[1] * d[0];
+    const double l1 = std::sqrt((n1[0] * n1[0] + n1[1] * n1[1]) + n1[2] * n1[2]);
+    const double l2 = std::sqrt((n2[0] * n2[0] + n2[1] * n2[1]) + n2[2] * n2[2]);
+    F[3] = ((n1[0] * n2[0] + n1[1] * n2[1]) + n1[2] * n2[2]) + l1 * l2;
+    if (!J) return;
+    double pa1[3], na1[3], pb1[3], nb1[3], pa2[3], na2[3], pb2[3], nb2[3];
+    for (int q = 0; q < 3; ++q) {
+        pa1[q] = (Mr[q][0] * A.pa[0] + Mr[q][1] * A.pa[1]) - Mr[q][2] * A.pa[2];
+        na1[q] = (Mr[q][0] * A.na[0] + Mr[q][1] * A.na[1]) - Mr[q][2] * A.na[2];
+        pb1[q] = Mr[q][0] * A.pb[0] - Mr[q][1] * A.pb[1];
+        nb1[q] = Mr[q][0] * A.nb[0] - Mr[q][1] * A.nb[1];
+        pa2[q] = (Nr[q][0] * B.pa[0] + Nr[q][1] * B.pa[1]) - Nr[q][2] * B.pa[2];
+        na2[q] = (Nr[q][0] * B.na[0] + Nr[q][1] * B.na[1]) - Nr[q][2] * B.na[2];
+        pb2[q] = Nr[q][0] * B.pb[0] - Nr[q][1] * B.pb[1];
+        nb2[q] = Nr[q][0] * B.nb[0] - Nr[q][1] * B.nb[1];
+    }
+    // rows 0..2: the three components of n1 x (p2 - p1) (:528-571)
+    J[0] = (((-n1[1]) * pa1[2] + n1[2] * pa1[1]) + na1[1] * d[2]) + (-na1[2]) * d[1];
+    J[1] = (((-n1[1]) * pb1[2] + n1[2] * pb1[1]) + nb1[1] * d[2]) + (-nb1[2]) * d[1];
+    J[2] = (-n1[1]) * (-pa2[2]) + n1[2] * (-pa2[1]);
+    J[3] = (-n1[1]) * (-pb2[2]) + n1[2] * (-pb2[1]);
+    J[4] = ((n1[0] * pa1[2] + (-n1[2]) * pa1[0]) + (-na1[0]) * d[2]) + na1[2] * d[0];
+    J[5] = ((n1[0] * pb1[2] + (-n1[2]) * pb1[0]) + (-nb1[0]) * d[2]) + nb1[2] * d[0];
+    J[6] = n1[0] * (-pa2[2]) + (-n1[2]) * (-pa2[0]);
+    J[7] = n1[0] * (-pb2[2]) + (-n1[2]) * (-pb2[0]);
+    J[8] = (((-n1[0]) * pa1[1] + n1[1] * pa1[0]) + na1[0] * d[1]) + (-na1[1]) * d[0];
+    J[9] = (((-n1[0]) * pb1[1] + n1[1] * pb1[0]) + nb1[0] * d[1]) + (-nb1[1]) * d[0];
+    J[10] = (-n1[0]) * (-pa2[1]) + n1[1] * (-pa2[0]);
+    J[11] = (-n1[0]) * (-pb2[1]) + n1[1] * (-pb2[0]);
+    // row 3: n1 . n2 + |n1| |n2| (:573-601)
+    auto row3 = [](const double* a, const double* b, const double* g, double la, double lb) {
+        return (((((a[0] * (-g[0]) + a[1] * (-g[1])) + a[2] * (-g[2])) * lb) / la + b[0] * (-g[0])) + b[1] * (-g[1])) +
+               b[2] * (-g[2]);
+    };
+    J[12] = row3(n1, n2, na1, l1, l2);
+    J[13] = row3(n1, n2, nb1, l1, l2);
+    J[14] = row3(n2, n1, na2, l2, l1);
+    J[15] = row3(n2, n1, nb2, l2, l1);
+}
+
 // ------------------------------------------------------------------ MINPACK hybrd (n = 4 in 3D, 2 in 2D)
 
-inline double enorm(int n, const double* x) {
+// enorm: src/minpack.cpp:36-45, sqrt of the sum of squares in index order.
+inline double enorm_ref(int n, const double* x) {
+    double value = 0.0;
+    for (int i = 0; i < n; ++i) value = value + x[i] * x[i];
+    return std::sqrt(value);
+}
+// ANL-80-74's enorm (scaled sums for components below rdwarf or above rgiant / n); test-only.
+inline double enorm_anl(int n, const double* x) {
     const double rdwarf = 3.834e-20, rgiant = 1.304e19;
     double s1 = 0, s2 = 0, s3 = 0, x1max = 0, x3max = 0;
     const double agiant = rgiant / n;
@@ -290,13 +478,14 @@ inline double enorm(int n, const double* x) {
     }
     return x3max * std::sqrt(s3);
 }
+inline double enorm(int n, const double* x) { return (minpack_variant() & 1) ? enorm_anl(n, x) : enorm_ref(n, x); }
 
 // the routines below for one system size N
 template <int N>
 struct Minpack {
 static constexpr int LR = N * (N + 1) / 2;
 
-// a is column-major n x n (a[i + n*j])
+// qrfac without pivoting (src/minpack.cpp:546-665); a is column-major n x n (a[i + n*j])
 static inline void qrfac(double* a, double* rdiag, double* acnorm) {
     for (int j = 0; j < N; ++j) {
         acnorm[j] = enorm(N, a + N * j);
@@ -319,6 +508,7 @@ static inline void qrfac(double* a, double* rdiag, double* acnorm) {
     }
 }
 
+// qform (src/minpack.cpp:243-314), m = n
 static inline void qform(double* q, double* wa) {
     for (int j = 1; j < N; ++j)
         for (int i = 0; i < j; ++i) q[i + N * j] = 0.0;
@@ -340,9 +530,19 @@ static inline void qform(double* q, double* wa) {
     }
 }
 
-// r: upper triangle packed by rows (r[l] with row-major order); 1-based index arithmetic kept.
-static inline void dogleg(const double* r, const double* diag, const double* qtb, double delta, double* x, double* wa1,
-                   double* wa2) {
+// dogleg (src/minpack.cpp:101-240).  r: upper triangle packed by rows; 1-based index arithmetic
+// kept.  The reference's scaled-gradient loop (:173-182) starts with l = 0 and updates
+// wa1[i - 1] with r[l - 1]; it is reproduced here as written.  hybrd1 carves every work array
+// out of one block (minpack.hpp:880-915: diag, qtf, wa1..wa4, r, fjac in that order) and hybrd
+// passes its wa1, wa2, wa3 as dogleg's x, wa1, wa2 (minpack.hpp:628), so the element before
+// dogleg's wa1 is x[n - 1] and the element before r is hybrd's wa4[n - 1], the residual at the
+// last trial point (wa4_last).  The loop therefore adds wa4[n - 1] qtb[0] to the Gauss-Newton
+// step's last component, accumulates each wa1[k] (k < n - 1) fully, divides it by diag[k], and
+// then adds r(k, n - 1) qtb[k + 1]; wa1[n - 1] stays 0.  The ANL-80-74 loop (variant 1) is
+// l = 1, wa1[i - 1] += r[l - 1] temp (variant bit 4).
+// Returns true when the Gauss-Newton step was longer than delta (the scaled-gradient branch).
+static inline bool dogleg(const double* r, const double* diag, const double* qtb, double delta, double* x, double* wa1,
+                          double* wa2, double wa4_last) {
     const double epsmch = 2.220446049250313e-16;
     int jj = (N * (N + 1)) / 2 + 1;
     for (int k = 1; k <= N; ++k) {
@@ -352,7 +552,7 @@ static inline void dogleg(const double* r, const double* diag, const double* qtb
         int l = jj + 1;
         double sum = 0.0;
         for (int i = jp1; i <= N; ++i) {
-            sum += r[l - 1] * x[i - 1];
+            sum = sum + r[l - 1] * x[i - 1];
             ++l;
         }
         double temp = r[jj - 1];
@@ -372,29 +572,44 @@ static inline void dogleg(const double* r, const double* diag, const double* qtb
         wa2[j] = diag[j] * x[j];
     }
     const double qnorm = enorm(N, wa2);
-    if (qnorm <= delta) return;
-    int l = 1;
-    for (int j = 1; j <= N; ++j) {
-        const double temp = qtb[j - 1];
-        for (int i = j; i <= N; ++i) {
-            wa1[i - 1] += r[l - 1] * temp;
-            ++l;
+    if (qnorm <= delta) return false;
+    if (minpack_variant() & 4) {
+        int l = 1;
+        for (int j = 1; j <= N; ++j) {
+            const double temp = qtb[j - 1];
+            for (int i = j; i <= N; ++i) {
+                wa1[i - 1] += r[l - 1] * temp;
+                ++l;
+            }
+            wa1[j - 1] /= diag[j - 1];
         }
-        wa1[j - 1] /= diag[j - 1];
+    } else {
+        // the reference's loop; W(k) and Rk(l) are its wa1[k] and r[l] for k, l >= -1
+        auto W = [&](int k) -> double& { return k >= 0 ? wa1[k] : x[N - 1]; };
+        auto Rk = [&](int l) -> double { return l >= 0 ? r[l] : wa4_last; };
+        int l = 0;
+        for (int j = 0; j < N; ++j) {
+            const double temp = qtb[j];
+            for (int i = j; i < N; ++i) {
+                W(i - 1) = W(i - 1) + Rk(l - 1) * temp;
+                l = l + 1;
+            }
+            wa1[j] = wa1[j] / diag[j];
+        }
     }
     const double gnorm = enorm(N, wa1);
     double sgnorm = 0.0;
     double alpha = delta / qnorm;
     if (gnorm != 0.0) {
         for (int j = 0; j < N; ++j) wa1[j] = (wa1[j] / gnorm) / diag[j];
-        l = 1;
-        for (int j = 1; j <= N; ++j) {
+        int l = 0;
+        for (int j = 0; j < N; ++j) {
             double sum = 0.0;
-            for (int i = j; i <= N; ++i) {
-                sum += r[l - 1] * wa1[i - 1];
+            for (int i = j; i < N; ++i) {
+                sum = sum + r[l] * wa1[i];
                 ++l;
             }
-            wa2[j - 1] = sum;
+            wa2[j] = sum;
         }
         const double temp = enorm(N, wa2);
         sgnorm = (gnorm / temp) / temp;
@@ -402,7 +617,11 @@ static inline void dogleg(const double* r, const double* diag, const double* qtb
         if (sgnorm < delta) {
             const double bnorm = enorm(N, qtb);
             double t = (bnorm / gnorm) * (bnorm / qnorm) * (sgnorm / delta);
-            t = t - (delta / qnorm) * (sgnorm / delta) * (sgnorm / delta) +
+            // src/minpack.cpp:223 multiplies left to right, (delta/qnorm)(sgnorm/delta) first;
+            // ANL-80-74's (delta/qnorm)*(sgnorm/delta)**2 squares first (variant bit 4)
+            const double dq = delta / qnorm, sd = sgnorm / delta;
+            const double corr = (minpack_variant() & 4) ? dq * (sd * sd) : (dq * sd) * sd;
+            t = t - corr +
                 std::sqrt((t - (delta / qnorm)) * (t - (delta / qnorm)) +
                           (1.0 - (delta / qnorm) * (delta / qnorm)) * (1.0 - (sgnorm / delta) * (sgnorm / delta)));
             alpha = ((delta / qnorm) * (1.0 - (sgnorm / delta) * (sgnorm / delta))) / t;
@@ -410,11 +629,13 @@ static inline void dogleg(const double* r, const double* diag, const double* qtb
     }
     const double temp = (1.0 - alpha) * std::min(sgnorm, delta);
     for (int j = 0; j < N; ++j) x[j] = temp * wa1[j] + alpha * x[j];
+    return true;
 }
 
-// Rank-1 update of the packed triangular s (m = n = N); 1-based index arithmetic kept.
+// Rank-1 update of the packed triangular s (m = n = N), src/minpack.cpp:373-542; 1-based index
+// arithmetic kept.  giant = r8_huge() = 1e30 (:60-65, :397); DBL_MAX with variant bit 2.
 static inline void r1updt(double* s, const double* u, double* v, double* w, bool& sing) {
-    const double giant = 1.79769313486231570e308;
+    const double giant = (minpack_variant() & 2) ? 1.79769313486231570e308 : 1.0e30;
     const int m = N, n = N;
     int jj = (n * (2 * m - n + 1)) / 2 - (m - n);
     int l = jj;
@@ -433,7 +654,7 @@ static inline void r1updt(double* s, const double* u, double* v, double* w, bool
                 sn = 0.5 / std::sqrt(0.25 + 0.25 * cotan * cotan);
                 cs = sn * cotan;
                 tau = 1.0;
-                if (std::abs(cs) * giant > 1.0) tau = 1.0 / cs;
+                if (1.0 < std::abs(cs) * giant) tau = 1.0 / cs;
             } else {
                 const double tn = v[j - 1] / v[n - 1];
                 cs = 0.5 / std::sqrt(0.25 + 0.25 * tn * tn);
@@ -451,7 +672,7 @@ static inline void r1updt(double* s, const double* u, double* v, double* w, bool
             }
         }
     }
-    for (int i = 1; i <= m; ++i) w[i - 1] += v[n - 1] * u[i - 1];
+    for (int i = 1; i <= m; ++i) w[i - 1] = w[i - 1] + v[n - 1] * u[i - 1];
     sing = false;
     for (int j = 1; j <= n - 1; ++j) {
         if (w[j - 1] != 0.0) {
@@ -461,7 +682,7 @@ static inline void r1updt(double* s, const double* u, double* v, double* w, bool
                 sn = 0.5 / std::sqrt(0.25 + 0.25 * cotan * cotan);
                 cs = sn * cotan;
                 tau = 1.0;
-                if (std::abs(cs) * giant > 1.0) tau = 1.0 / cs;
+                if (1.0 < std::abs(cs) * giant) tau = 1.0 / cs;
             } else {
                 const double tn = w[j - 1] / s[jj - 1];
                 cs = 0.5 / std::sqrt(0.25 + 0.25 * tn * tn);
@@ -488,55 +709,60 @@ static inline void r1updt(double* s, const double* u, double* v, double* w, bool
     if (s[jj - 1] == 0.0) sing = true;
 }
 
-// a is column-major m x n with leading dimension lda
+// r1mpyq (src/minpack.cpp:316-371); a is column-major m x n with leading dimension lda
 static inline void r1mpyq(int m, int n, double* a, int lda, const double* v, const double* w) {
-    for (int nmj = 1; nmj <= n - 1; ++nmj) {
-        const int j = n - nmj;
+    for (int j = n - 2; 0 <= j; --j) {
         double cs, sn;
-        if (std::abs(v[j - 1]) > 1.0) {
-            cs = 1.0 / v[j - 1];
+        if (1.0 < std::abs(v[j])) {
+            cs = 1.0 / v[j];
             sn = std::sqrt(1.0 - cs * cs);
         } else {
-            sn = v[j - 1];
+            sn = v[j];
             cs = std::sqrt(1.0 - sn * sn);
         }
-        for (int i = 1; i <= m; ++i) {
-            const double temp = cs * a[(i - 1) + lda * (j - 1)] - sn * a[(i - 1) + lda * (n - 1)];
-            a[(i - 1) + lda * (n - 1)] = sn * a[(i - 1) + lda * (j - 1)] + cs * a[(i - 1) + lda * (n - 1)];
-            a[(i - 1) + lda * (j - 1)] = temp;
+        for (int i = 0; i < m; ++i) {
+            const double temp = cs * a[i + j * lda] - sn * a[i + (n - 1) * lda];
+            a[i + (n - 1) * lda] = sn * a[i + j * lda] + cs * a[i + (n - 1) * lda];
+            a[i + j * lda] = temp;
         }
     }
-    for (int j = 1; j <= n - 1; ++j) {
+    for (int j = 0; j < n - 1; ++j) {
         double cs, sn;
-        if (std::abs(w[j - 1]) > 1.0) {
-            cs = 1.0 / w[j - 1];
+        if (1.0 < std::abs(w[j])) {
+            cs = 1.0 / w[j];
             sn = std::sqrt(1.0 - cs * cs);
         } else {
-            sn = w[j - 1];
+            sn = w[j];
             cs = std::sqrt(1.0 - sn * sn);
         }
-        for (int i = 1; i <= m; ++i) {
-            const double temp = cs * a[(i - 1) + lda * (j - 1)] + sn * a[(i - 1) + lda * (n - 1)];
-            a[(i - 1) + lda * (n - 1)] = -sn * a[(i - 1) + lda * (j - 1)] + cs * a[(i - 1) + lda * (n - 1)];
-            a[(i - 1) + lda * (j - 1)] = temp;
+        for (int i = 0; i < m; ++i) {
+            const double temp = cs * a[i + j * lda] + sn * a[i + (n - 1) * lda];
+            a[i + (n - 1) * lda] = -sn * a[i + j * lda] + cs * a[i + (n - 1) * lda];
+            a[i + j * lda] = temp;
         }
     }
 }
 
-// hybrd1 (minpack.hpp:851-923) with the analytic Jacobian: returns info (1: converged).
+// hybrd1 (minpack.hpp:851-923) running hybrd (:337-831) with the analytic Jacobian: returns
+// info (1: converged).  The reference's work block is fresh, uninitialised memory; wa4 is read
+// by dogleg before its first write only on a scaled-gradient first step, which reads 0 here
+// (counted in dogleg_fresh_wa4_count()).
 template <class Fn>
 static inline int hybrd1(Fn fcn, double* x, int* nfev_out = nullptr, int* njev_out = nullptr) {
+    const bool anl = (minpack_variant() & 8) != 0, anl_dogleg = (minpack_variant() & 4) != 0;
     const double xtol = 1.0e-10, factor = 100.0, epsmch = 2.220446049250313e-16;
     const int maxfev = 200 * (N + 1);
     const double p1 = 0.1, p5 = 0.5, p001 = 0.001, p0001 = 0.0001;
     double diag[N];  // mode 2: wa[0..n) = 1 (minpack.hpp:881-884)
     for (int k = 0; k < N; ++k) diag[k] = 1.0;
     double fvec[N], fjac[N * N], r[LR], qtf[N], wa1[N], wa2[N], wa3[N], wa4[N];
+    for (int k = 0; k < N; ++k) wa4[k] = 0.0;
+    bool wa4_written = false;
     int info = 0, nfev = 0, njev = 0;
     fcn(x, fvec, nullptr);
     nfev = 1;
     double fnorm = enorm(N, fvec);
-    const int msum = N;
+    const int msum = N;  // min(ml + mu + 1, n) with ml = mu = n - 1
     int iter = 1, ncsuc = 0, ncfail = 0, nslow1 = 0, nslow2 = 0;
     double delta = 0.0, xnorm = 0.0;
     for (;;) {
@@ -552,19 +778,22 @@ static inline int hybrd1(Fn fcn, double* x, int* nfev_out = nullptr, int* njev_o
         if (iter == 1) {
             for (int j = 0; j < N; ++j) wa3[j] = diag[j] * x[j];
             xnorm = enorm(N, wa3);
-            delta = factor * xnorm;
-            if (delta == 0.0) delta = factor;
+            if (anl) {
+                delta = factor * xnorm;
+                if (delta == 0.0) delta = factor;
+            } else {
+                delta = (xnorm == 0.0) ? factor : factor * xnorm;
+            }
         }
         for (int i = 0; i < N; ++i) qtf[i] = fvec[i];
         for (int j = 0; j < N; ++j) {
             if (fjac[j + N * j] != 0.0) {
                 double sum = 0.0;
-                for (int i = j; i < N; ++i) sum += fjac[i + N * j] * qtf[i];
+                for (int i = j; i < N; ++i) sum = sum + fjac[i + N * j] * qtf[i];
                 const double temp = -sum / fjac[j + N * j];
-                for (int i = j; i < N; ++i) qtf[i] += fjac[i + N * j] * temp;
+                for (int i = j; i < N; ++i) qtf[i] = qtf[i] + fjac[i + N * j] * temp;
             }
         }
-        bool sing = false;
         for (int j = 0; j < N; ++j) {
             int l = j;
             for (int i = 0; i < j; ++i) {
@@ -572,12 +801,11 @@ static inline int hybrd1(Fn fcn, double* x, int* nfev_out = nullptr, int* njev_o
                 l += N - i - 1;
             }
             r[l] = wa1[j];
-            if (wa1[j] == 0.0) sing = true;
         }
-        (void)sing;
         qform(fjac, wa1);
         for (;;) {
-            dogleg(r, diag, qtf, delta, wa1, wa2, wa3);
+            if (dogleg(r, diag, qtf, delta, wa1, wa2, wa3, wa4[N - 1]) && !wa4_written && !anl_dogleg)
+                ++dogleg_fresh_wa4_count();
             for (int j = 0; j < N; ++j) {
                 wa1[j] = -wa1[j];
                 wa2[j] = x[j] + wa1[j];
@@ -586,6 +814,7 @@ static inline int hybrd1(Fn fcn, double* x, int* nfev_out = nullptr, int* njev_o
             const double pnorm = enorm(N, wa3);
             if (iter == 1) delta = std::min(delta, pnorm);
             fcn(wa2, wa4, nullptr);
+            wa4_written = true;
             ++nfev;
             const double fnorm1 = enorm(N, wa4);
             double actred = -1.0;
@@ -594,7 +823,7 @@ static inline int hybrd1(Fn fcn, double* x, int* nfev_out = nullptr, int* njev_o
             for (int i = 0; i < N; ++i) {
                 double sum = 0.0;
                 for (int j = i; j < N; ++j) {
-                    sum += r[l] * wa1[j];
+                    sum = sum + r[l] * wa1[j];
                     ++l;
                 }
                 wa3[i] = qtf[i] + sum;
@@ -603,7 +832,7 @@ static inline int hybrd1(Fn fcn, double* x, int* nfev_out = nullptr, int* njev_o
             double prered = 0.0;
             if (temp < fnorm) prered = 1.0 - (temp / fnorm) * (temp / fnorm);
             double ratio = 0.0;
-            if (prered > 0.0) ratio = actred / prered;
+            if (0.0 < prered) ratio = actred / prered;
             if (ratio < p1) {
                 ncsuc = 0;
                 ++ncfail;
@@ -611,10 +840,10 @@ static inline int hybrd1(Fn fcn, double* x, int* nfev_out = nullptr, int* njev_o
             } else {
                 ncfail = 0;
                 ++ncsuc;
-                if (ratio >= p5 || ncsuc > 1) delta = std::max(delta, pnorm / p5);
+                if (p5 <= ratio || 1 < ncsuc) delta = std::max(delta, pnorm / p5);
                 if (std::abs(ratio - 1.0) <= p1) delta = pnorm / p5;
             }
-            if (ratio >= p0001) {
+            if (p0001 <= ratio) {
                 for (int j = 0; j < N; ++j) {
                     x[j] = wa2[j];
                     wa2[j] = diag[j] * x[j];
@@ -625,23 +854,34 @@ static inline int hybrd1(Fn fcn, double* x, int* nfev_out = nullptr, int* njev_o
                 ++iter;
             }
             ++nslow1;
-            if (actred >= p001) nslow1 = 0;
+            if (p001 <= actred) nslow1 = 0;
             if (jeval) ++nslow2;
-            if (actred >= p1) nslow2 = 0;
+            if (p1 <= actred) nslow2 = 0;
             if (delta <= xtol * xnorm || fnorm == 0.0) info = 1;
             if (info != 0) break;
-            if (maxfev <= nfev) info = 2;
-            if (p1 * std::max(p1 * delta, pnorm) <= epsmch * xnorm) info = 3;
-            if (nslow2 == 5) info = 4;
-            if (nslow1 == 10) info = 5;
+            if (anl) {  // ANL-80-74: every test runs, the last that holds sets info
+                if (maxfev <= nfev) info = 2;
+                if (p1 * std::max(p1 * delta, pnorm) <= epsmch * xnorm) info = 3;
+                if (nslow2 == 5) info = 4;
+                if (nslow1 == 10) info = 5;
+            } else {  // the reference returns at the first test that holds (minpack.hpp:760-791)
+                if (maxfev <= nfev)
+                    info = 2;
+                else if (p1 * std::max(p1 * delta, pnorm) <= epsmch * xnorm)
+                    info = 3;
+                else if (nslow2 == 5)
+                    info = 4;
+                else if (nslow1 == 10)
+                    info = 5;
+            }
             if (info != 0) break;
             if (ncfail == 2) break;
             for (int j = 0; j < N; ++j) {
                 double sum = 0.0;
-                for (int i = 0; i < N; ++i) sum += fjac[i + N * j] * wa4[i];
+                for (int i = 0; i < N; ++i) sum = sum + fjac[i + N * j] * wa4[i];
                 wa2[j] = (sum - wa3[j]) / pnorm;
                 wa1[j] = diag[j] * ((diag[j] * wa1[j]) / pnorm);
-                if (ratio >= p0001) qtf[j] = sum;
+                if (p0001 <= ratio) qtf[j] = sum;
             }
             bool sg = false;
             r1updt(r, wa1, wa2, wa3, sg);
@@ -656,6 +896,7 @@ static inline int hybrd1(Fn fcn, double* x, int* nfev_out = nullptr, int* njev_o
     if (info == 5) info = 4;  // hybrd1 (minpack.hpp:918-921)
     return info;
 }
+
 };
 
 // closest_points(superellipsoid<3>, superellipsoid<3>): closest_points.hpp:263-733.
@@ -776,10 +1017,10 @@ inline v3 pln_normal(const Pln& p) { return {p.R[0][0], p.R[1][0], p.R[2][0]}; }
 // |x|^p (no sign) with derivative p |x|^(p-1) sign(x) x' (exact) or p |x|^(p-1) x' (the
 // sphere pair's Jacobian)
 template <class M>
-inline D4 upow(const D4& a, double p, bool exact) {
+inline D4 upow2(const D4& a, double s, bool exact) {
     const double ax = std::abs(a.v);
-    D4 r{M::pow(ax, p), {}};
-    const double dv = exact ? (p * sgn(a.v)) * M::pow(ax, p - 1) : p * M::pow(ax, p - 1);
+    D4 r{M::pow(ax, 2 - s), {}};
+    const double dv = exact ? ((2 - s) * sgn(a.v)) * M::pow(ax, 1 - s) : (2 - s) * M::pow(ax, 1 - s);
     for (int k = 0; k < 4; ++k) r.d[k] = dv * a.d[k];
     return r;
 }
@@ -794,10 +1035,10 @@ inline D4 dabs(const D4& a) {
 template <class M>
 inline void se_side_terms(const SE& s1, const D4& a1, const D4& b1, bool exact, D4* A) {
     const D4 ca1 = dcos<M>(a1), sa1 = dsin<M>(a1), cb1 = dcos<M>(b1), sb1 = dsin<M>(b1);
-    const D4 ca1n = spow<M>(ca1, s1.n), ca1n2 = upow<M>(ca1, 2 - s1.n, exact);
-    const D4 sa1n = spow<M>(sa1, s1.n), sa1n2 = spow<M>(sa1, 2 - s1.n);
-    const D4 cb1e = spow<M>(cb1, s1.e), cb1e2 = spow<M>(cb1, 2 - s1.e);
-    const D4 sb1e = spow<M>(sb1, s1.e), sb1e2 = spow<M>(sb1, 2 - s1.e);
+    const D4 ca1n = spow<M>(ca1, s1.n), ca1n2 = upow2<M>(ca1, s1.n, exact);
+    const D4 sa1n = spow<M>(sa1, s1.n), sa1n2 = spow2<M>(sa1, s1.n);
+    const D4 cb1e = spow<M>(cb1, s1.e), cb1e2 = spow2<M>(cb1, s1.e);
+    const D4 sb1e = spow<M>(sb1, s1.e), sb1e2 = spow2<M>(sb1, s1.e);
     A[0] = ((s1.r[1] * s1.r[2]) * ca1n2) * cb1e2;
     A[1] = ((s1.r[0] * s1.r[2]) * sb1e2) * ca1n2;
     A[2] = sgn(ca1.v) * ((s1.r[0] * s1.r[1]) * sa1n2);
@@ -1057,8 +1298,8 @@ inline v3 pln2_point(const Pln& p, double b) {
 template <class M>
 inline void se2_terms(const SE& s, const D4& b, D4& nx, D4& ny, D4& px, D4& py) {
     const D4 cb = dcos<M>(b), sb = dsin<M>(b);
-    nx = s.r[1] * spow<M>(cb, 2 - s.e);
-    ny = s.r[0] * spow<M>(sb, 2 - s.e);
+    nx = s.r[1] * spow2<M>(cb, s.e);
+    ny = s.r[0] * spow2<M>(sb, s.e);
     px = s.r[0] * spow<M>(cb, s.e);
     py = s.r[1] * spow<M>(sb, s.e);
 }
@@ -1070,10 +1311,11 @@ inline void put2(const D4* f, double* F, double* J) {
     }
 }
 
-// closest_points(superellipsoid<2>, superellipsoid<2>) residual (closest_points.hpp:79-125): the
-// normal n1 of s1 at b1 is collinear with p2 - p1 (cross product 0) and anti-parallel to n2 at b2
+// closest_points(superellipsoid<2>, superellipsoid<2>) residual (closest_points.hpp:79-125) with the
+// forward-mode Jacobian (test-side cross-check): the normal n1 of s1 at b1 is collinear with p2 - p1
+// (cross product 0) and anti-parallel to n2 at b2
 template <class M>
-inline void se2_se2_residual(const SE& s1, const SE& s2, const double* u, double* F, double* J) {
+inline void se2_se2_residual_ad(const SE& s1, const SE& s2, const double* u, double* F, double* J) {
     D4 n1x, n1y, p1x, p1y, n2x, n2y, p2x, p2y;
     se2_terms<M>(s1, var(u[0], 0), n1x, n1y, p1x, p1y);
     se2_terms<M>(s2, var(u[1], 1), n2x, n2y, p2x, p2y);
@@ -1087,6 +1329,57 @@ inline void se2_se2_residual(const SE& s1, const SE& s2, const double* u, double
     f[0] = (-m1x) * dy + m1y * dx;
     f[1] = (m1x * m2x + m1y * m2y) + dsqrt(m1x * m1x + m1y * m1y) * dsqrt(m2x * m2x + m2y * m2y);
     put2(f, F, J);
+}
+
+// The same residual with the reference's hand-written Jacobian (closest_points.hpp:127-189).  Body
+// terms F1..F4 (s1) and F5..F8 (s2): F1 = rx sb^(2-e), F2 = ry sb^e, F3 = ry cb^(2-e), F4 = rx cb^e
+// (signed powers); derivative terms F9..F12 (s1), F13..F16 (s2): F9 = e rx sb |cb|^(e-1),
+// F10 = e ry cb |sb|^(e-1), F11 = ry (2-e) sb |cb|^(1-e), F12 = rx (2-e) cb |sb|^(1-e).
+// With the rotated sums m = (M01 F1 + M00 F3, M11 F1 + M10 F3), g1 = (M00 F9 - M01 F10,
+// M10 F9 - M11 F10), h1 = (M01 F12 - M00 F11, M11 F12 - M10 F11) and the same for s2 with N
+// (g2 from F13, F14; h2 from F15, F16), the reference's factors are these sums or their exact
+// negations, and each entry is summed left to right as written.
+template <class M>
+inline void se2_se2_residual(const SE& s1, const SE& s2, const double* u, double* F, double* J) {
+    if (J && jacobian_ad()) {
+        se2_se2_residual_ad<M>(s1, s2, u, F, J);
+        return;
+    }
+    struct Side {
+        double f1, f2, f3, f4, f9, f10, f11, f12;
+    };
+    auto side = [&](const SE& s, double b) {
+        const double cb = M::cos(b), sb = M::sin(b), acb = std::abs(cb), asb = std::abs(sb);
+        const double cbe = sgn(cb) * M::pow(acb, s.e), cbe2 = sgn(cb) * M::pow(acb, 2 - s.e);
+        const double sbe = sgn(sb) * M::pow(asb, s.e), sbe2 = sgn(sb) * M::pow(asb, 2 - s.e);
+        Side t{s.r[0] * sbe2, s.r[1] * sbe, s.r[1] * cbe2, s.r[0] * cbe, 0, 0, 0, 0};
+        if (J) {
+            t.f9 = ((s.e * s.r[0]) * sb) * M::pow(acb, s.e - 1);
+            t.f10 = ((s.e * s.r[1]) * cb) * M::pow(asb, s.e - 1);
+            t.f11 = ((s.r[1] * (2 - s.e)) * sb) * M::pow(acb, 1 - s.e);
+            t.f12 = ((s.r[0] * (2 - s.e)) * cb) * M::pow(asb, 1 - s.e);
+        }
+        return t;
+    };
+    const Side a = side(s1, u[0]), b = side(s2, u[1]);
+    const auto& Mr = s1.R;
+    const auto& Nr = s2.R;
+    const double m1x = Mr[0][1] * a.f1 + Mr[0][0] * a.f3, m1y = Mr[1][1] * a.f1 + Mr[1][0] * a.f3;
+    const double m2x = Nr[0][1] * b.f1 + Nr[0][0] * b.f3, m2y = Nr[1][1] * b.f1 + Nr[1][0] * b.f3;
+    const double dy = (((((-(Mr[1][0] * a.f4)) - Mr[1][1] * a.f2) - s1.c[1]) + Nr[1][0] * b.f4) + Nr[1][1] * b.f2) + s2.c[1];
+    const double dx = (((((-(Mr[0][0] * a.f4)) - Mr[0][1] * a.f2) - s1.c[0]) + Nr[0][0] * b.f4) + Nr[0][1] * b.f2) + s2.c[0];
+    F[0] = (-m1x) * dy + m1y * dx;
+    const double l1 = std::sqrt(m1x * m1x + m1y * m1y), l2 = std::sqrt(m2x * m2x + m2y * m2y);
+    F[1] = (m1x * m2x + m1y * m2y) + l1 * l2;
+    if (!J) return;
+    const double g1x = Mr[0][0] * a.f9 - Mr[0][1] * a.f10, g1y = Mr[1][0] * a.f9 - Mr[1][1] * a.f10;
+    const double h1x = Mr[0][1] * a.f12 - Mr[0][0] * a.f11, h1y = Mr[1][1] * a.f12 - Mr[1][0] * a.f11;
+    const double g2x = Nr[0][0] * b.f9 - Nr[0][1] * b.f10, g2y = Nr[1][0] * b.f9 - Nr[1][1] * b.f10;
+    const double h2x = Nr[0][1] * b.f12 - Nr[0][0] * b.f11, h2y = Nr[1][1] * b.f12 - Nr[1][0] * b.f11;
+    J[0] = (((-m1x) * g1y + m1y * g1x) + (-h1x) * dy) + h1y * dx;
+    J[1] = (-m1x) * (-g2y) + m1y * (-g2x);
+    J[2] = (((m1x * h1x + m1y * h1y) * l2) / l1 + m2x * h1x) + m2y * h1y;
+    J[3] = (((m2x * h2x + m2y * h2y) * l1) / l2 + m1x * h2x) + m1y * h2y;
 }
 
 // closest_points(sphere<2> s2, superellipsoid<2> s1) residual (closest_points.hpp:1332-1366)

@@ -275,14 +275,11 @@ __device__ inline int init_angles_n(const SE& s, bool xy, double* out) {
 __device__ inline int init_angles(const SE& s, bool xy, double* out) { return init_angles_n<6>(s, xy, out); }
 
 // ------------------------------------------------------------------ residual and Jacobian
-// F(a1, b1, a2, b2) of closest_points.hpp:282-382, values in the reference's order.  The
-// Jacobian is the forward-mode derivative of the same expression, evaluated one
-// direction at a time (D1: value + one derivative): every derivative component is
-// computed by the same operations, in the same order, as in a 4-component forward
-// mode, so the result is identical to it, with a quarter of the live registers.
-// The transcendental part (8 cos/sin, 16 signed powers and their derivative
-// factors) is evaluated once per point; the two powers of each |cos| / |sin| share
-// one logarithm, which gives the same bits as two separate pow calls.
+// F(a1, b1, a2, b2) of closest_points.hpp:282-382 and the reference's hand-written Jacobian
+// (:385-603), both in the reference's evaluation order.  The transcendental part (8 cos/sin,
+// 16 signed powers and 16 derivative factors) is evaluated once per point; the four powers of
+// each |cos| / |sin| share one logarithm, which gives the same bits as separate pow calls.
+// D1 (value + one directional derivative) serves the 2D Newton pairs below.
 struct D1 {
     double v, d;
 };
@@ -303,27 +300,30 @@ __device__ inline double cst<double>(double x) { return x; }
 template <>
 __device__ inline D1 cst<D1>(double x) { return D1{x, 0.0}; }
 
-// |x|^p and p |x|^(p-1) for the exponent pair (p, 2 - p) of one angle function, sharing log|x|
+// |x|^p, |x|^(2-p) and the derivative factors p |x|^(p-1), (2-p) |x|^(1-p) of one angle function,
+// exponents formed as the reference writes them (closest_points.hpp:426-446), sharing log|x|
 __device__ inline void pow_pairs(double x, double p, double* v, double* dv) {
     const double ax = fabs(x);
     const double q[2] = {p, 2 - p};
+    const double qd[2] = {p - 1, 1 - p};
     if (ax != ax || ax == 0 || ax == INFINITY) {
         for (int k = 0; k < 2; ++k) {
             v[k] = pm_pow(ax, q[k]);
-            if (dv) dv[k] = q[k] * pm_pow(ax, q[k] - 1);
+            if (dv) dv[k] = q[k] * pm_pow(ax, qd[k]);
         }
         return;
     }
     const double L = pm_log(ax);
     for (int k = 0; k < 2; ++k) {
         v[k] = pm_exp(q[k] * L);
-        if (dv) dv[k] = q[k] * pm_exp((q[k] - 1) * L);
+        if (dv) dv[k] = q[k] * pm_exp(qd[k] * L);
     }
 }
 
 // Transcendental terms at u: cs[2m] = cos u_m, cs[2m+1] = sin u_m; powers pw[2f + k] of the
 // angle function f (0 cos a1, 1 sin a1, 2 cos b1, 3 sin b1, 4..7 the same for body 2) with
-// exponent k (0: the shape exponent n or e, 1: 2 minus it), signed by the function.
+// exponent k (0: the shape exponent n or e, 1: 2 minus it), signed by the function; dp the
+// unsigned derivative factors in the same slots.
 struct SeTerms {
     double cs[8];
     double pw[16], dp[16];
@@ -335,119 +335,129 @@ __device__ inline void se_terms(const SE& s1, const SE& s2, const double* u, SeT
     }
     // function f: (angle m, cos or sin, exponent): a -> n, b -> e
     const double ex[8] = {s1.n, s1.n, s1.e, s1.e, s2.n, s2.n, s2.e, s2.e};
-    const int src[8] = {0, 1, 2, 3, 4, 5, 6, 7};  // cs index of function f: cos a1, sin a1, cos b1, ...
     for (int f = 0; f < 8; ++f) {
-        const double x = t.cs[src[f]];
+        const double x = t.cs[f];
         pow_pairs(x, ex[f], &t.pw[2 * f], deriv ? &t.dp[2 * f] : nullptr);
         t.pw[2 * f] = sgn(x) * t.pw[2 * f];
         t.pw[2 * f + 1] = sgn(x) * t.pw[2 * f + 1];
     }
 }
 
-// the algebraic part; V = double (values) or D1 (direction k of the derivative)
-template <class V>
-__device__ inline void se_algebra(const SE& s1, const SE& s2, const SeTerms& t, int k, V* F) {
-    // angle functions with their derivative along direction k (cos' = -sin, sin' = cos)
-    V fn[8];
-    for (int m = 0; m < 4; ++m) {
-        if constexpr (sizeof(V) == sizeof(D1)) {
-            const double seed = (m == k) ? 1.0 : 0.0;
-            fn[2 * m] = D1{t.cs[2 * m], -t.cs[2 * m + 1] * seed};
-            fn[2 * m + 1] = D1{t.cs[2 * m + 1], t.cs[2 * m] * seed};
-        } else {
-            fn[2 * m] = cst<V>(t.cs[2 * m]);
-            fn[2 * m + 1] = cst<V>(t.cs[2 * m + 1]);
-        }
-    }
-    // signed powers: value sgn(x)|x|^p, derivative p|x|^(p-1) x'
-    auto P = [&](int f, int e) -> V {
-        if constexpr (sizeof(V) == sizeof(D1))
-            return D1{t.pw[2 * f + e], t.dp[2 * f + e] * fn[f].d};
-        else
-            return t.pw[2 * f + e];
-    };
-    const V ca1n = P(0, 0), ca1n2 = P(0, 1), sa1n = P(1, 0), sa1n2 = P(1, 1);
-    const V cb1e = P(2, 0), cb1e2 = P(2, 1), sb1e = P(3, 0), sb1e2 = P(3, 1);
-    const V ca2n = P(4, 0), ca2n2 = P(4, 1), sa2n = P(5, 0), sa2n2 = P(5, 1);
-    const V cb2e = P(6, 0), cb2e2 = P(6, 1), sb2e = P(7, 0), sb2e2 = P(7, 1);
-    const V A1 = ((s1.r[1] * s1.r[2]) * ca1n2) * cb1e2, A2 = ((s1.r[0] * s1.r[2]) * sb1e2) * ca1n2,
-            A3 = (s1.r[0] * s1.r[1]) * sa1n2;
-    const V A4 = (s1.r[0] * ca1n) * cb1e, A5 = (s1.r[1] * sb1e) * ca1n, A6 = s1.r[2] * sa1n;
-    const V B1 = (s2.r[0] * ca2n) * cb2e, B2 = (s2.r[1] * sb2e) * ca2n, B3 = s2.r[2] * sa2n;
-    const V B4 = ((s2.r[1] * s2.r[2]) * ca2n2) * cb2e2, B5 = ((s2.r[0] * s2.r[2]) * sb2e2) * ca2n2,
-            B6 = (s2.r[0] * s2.r[1]) * sa2n2;
-    V n1[3], n2[3], d[3];
-    for (int q = 0; q < 3; ++q) {
-        n1[q] = (s1.R[q][0] * A1 + s1.R[q][1] * A2) + s1.R[q][2] * A3;
-        n2[q] = (s2.R[q][0] * B4 + s2.R[q][1] * B5) + s2.R[q][2] * B6;
-        d[q] = (((((((-(s1.R[q][0] * A4)) - s1.R[q][1] * A5) - s1.R[q][2] * A6) + s2.R[q][0] * B1) +
-                  s2.R[q][1] * B2) +
-                 s2.R[q][2] * B3) -
-                cst<V>(s1.c[q])) +
-               cst<V>(s2.c[q]);
-    }
-    F[0] = (-(n1[1] * d[2])) + n1[2] * d[1];
-    F[1] = n1[0] * d[2] - n1[2] * d[0];
-    F[2] = (-(n1[0] * d[1])) + n1[1] * d[0];
-    const V dotn = (n1[0] * n2[0] + n1[1] * n2[1]) + n1[2] * n2[2];
-    const V l1 = dsqrt((n1[0] * n1[0] + n1[1] * n1[1]) + n1[2] * n1[2]);
-    const V l2 = dsqrt((n2[0] * n2[0] + n2[1] * n2[1]) + n2[2] * n2[2]);
-    F[3] = dotn + l1 * l2;
+// Body-frame normal n (A1..A3) and point p (A4..A6) of one body (closest_points.hpp:448-453) and
+// the derivative terms pa = A7..A9, na = A10..A12 (along a), pb = A13, A14, nb = A15, A16
+// (along b) (:454-463); the second body's B1..B16 (:496-518) are the same forms.
+struct SeForms {
+    double n[3], p[3], pa[3], na[3], pb[2], nb[2];
+};
+__device__ inline void se_forms(const SE& s, const SeTerms& t, int body, bool deriv, SeForms& f) {
+    const double rx = s.r[0], ry = s.r[1], rz = s.r[2];
+    const double* c = &t.cs[4 * body];  // cos a, sin a, cos b, sin b
+    const double* w = &t.pw[8 * body];  // can, can2, san, san2, cbe, cbe2, sbe, sbe2
+    f.n[0] = ((ry * rz) * w[1]) * w[5];
+    f.n[1] = ((rx * rz) * w[7]) * w[1];
+    f.n[2] = (rx * ry) * w[3];
+    f.p[0] = (rx * w[0]) * w[4];
+    f.p[1] = (ry * w[6]) * w[0];
+    f.p[2] = rz * w[2];
+    if (!deriv) return;
+    const double* g = &t.dp[8 * body];  // can1, can3, san1, san3, cbe1, cbe3, sbe1, sbe3
+    f.pa[0] = ((rx * c[1]) * g[0]) * w[4];
+    f.pa[1] = ((ry * c[1]) * w[6]) * g[0];
+    f.pa[2] = (rz * c[0]) * g[2];
+    f.na[0] = (((ry * rz) * c[1]) * g[1]) * w[5];
+    f.na[1] = (((rx * rz) * c[1]) * w[7]) * g[1];
+    f.na[2] = ((rx * ry) * c[0]) * g[3];
+    f.pb[0] = ((rx * c[3]) * w[0]) * g[4];
+    f.pb[1] = ((ry * c[2]) * g[6]) * w[0];
+    f.nb[0] = (((ry * rz) * c[3]) * w[1]) * g[5];
+    f.nb[1] = (((rx * rz) * c[2]) * g[7]) * w[1];
 }
 
-// F at u, and the Jacobian (row-major) when J is not null
+// F at u, and the Jacobian (row-major) when J is not null.  Each factor of the reference's
+// entries is a rotated sum of the forms or its exact negation (negating a sum is exact):
+//   pa1 = (Mq0 A7 + Mq1 A8) - Mq2 A9, na1 = (Mq0 A10 + Mq1 A11) - Mq2 A12, pb1 = Mq0 A13 - Mq1 A14,
+//   nb1 = Mq0 A15 - Mq1 A16, and pa2, na2, pb2, nb2 the same with N and the B terms;
+// every entry is summed left to right as the reference writes it.
 __device__ inline void se_residual(const SE& s1, const SE& s2, const double* u, double* F, double* J) {
+    const bool deriv = J != nullptr;
     SeTerms t;
-    se_terms(s1, s2, u, t, J != nullptr);
-    if (!J) {
-        se_algebra<double>(s1, s2, t, -1, F);
-        return;
+    se_terms(s1, s2, u, t, deriv);
+    SeForms A, B;
+    se_forms(s1, t, 0, deriv, A);
+    se_forms(s2, t, 1, deriv, B);
+    double n1[3], n2[3], d[3];
+    for (int q = 0; q < 3; ++q) {
+        n1[q] = (s1.R[q][0] * A.n[0] + s1.R[q][1] * A.n[1]) + s1.R[q][2] * A.n[2];
+        n2[q] = (s2.R[q][0] * B.n[0] + s2.R[q][1] * B.n[1]) + s2.R[q][2] * B.n[2];
+        d[q] = (((((((-(s1.R[q][0] * A.p[0])) - s1.R[q][1] * A.p[1]) - s1.R[q][2] * A.p[2]) + s2.R[q][0] * B.p[0]) +
+                  s2.R[q][1] * B.p[1]) +
+                 s2.R[q][2] * B.p[2]) -
+                s1.c[q]) +
+               s2.c[q];
     }
-#pragma unroll 1
-    for (int k = 0; k < 4; ++k) {
-        D1 f[4];
-        se_algebra<D1>(s1, s2, t, k, f);
-        for (int i = 0; i < 4; ++i) {
-            J[4 * i + k] = f[i].d;
-            if (k == 0) F[i] = f[i].v;
-        }
+    F[0] = (-n1[1]) * d[2] + n1[2] * d[1];
+    F[1] = n1[0] * d[2] - n1[2] * d[0];
+    F[2] = (-n1[0]) * d[1] + n1[1] * d[0];
+    const double l1 = sqrt((n1[0] * n1[0] + n1[1] * n1[1]) + n1[2] * n1[2]);
+    const double l2 = sqrt((n2[0] * n2[0] + n2[1] * n2[1]) + n2[2] * n2[2]);
+    F[3] = ((n1[0] * n2[0] + n1[1] * n2[1]) + n1[2] * n2[2]) + l1 * l2;
+    if (!deriv) return;
+    // row 3 entry: ((a . -g) lb / la + b . -g), summed left to right
+    auto row3 = [](const double* a, const double* b, const double* g, double la, double lb) {
+        return (((((a[0] * (-g[0]) + a[1] * (-g[1])) + a[2] * (-g[2])) * lb) / la + b[0] * (-g[0])) + b[1] * (-g[1])) +
+               b[2] * (-g[2]);
+    };
+    double x[3], y[3];
+    // column 0 (a1): x = pa1, y = na1
+    for (int q = 0; q < 3; ++q) {
+        x[q] = (s1.R[q][0] * A.pa[0] + s1.R[q][1] * A.pa[1]) - s1.R[q][2] * A.pa[2];
+        y[q] = (s1.R[q][0] * A.na[0] + s1.R[q][1] * A.na[1]) - s1.R[q][2] * A.na[2];
     }
+    J[0] = (((-n1[1]) * x[2] + n1[2] * x[1]) + y[1] * d[2]) + (-y[2]) * d[1];
+    J[4] = ((n1[0] * x[2] + (-n1[2]) * x[0]) + (-y[0]) * d[2]) + y[2] * d[0];
+    J[8] = (((-n1[0]) * x[1] + n1[1] * x[0]) + y[0] * d[1]) + (-y[1]) * d[0];
+    J[12] = row3(n1, n2, y, l1, l2);
+    // column 1 (b1): x = pb1, y = nb1
+    for (int q = 0; q < 3; ++q) {
+        x[q] = s1.R[q][0] * A.pb[0] - s1.R[q][1] * A.pb[1];
+        y[q] = s1.R[q][0] * A.nb[0] - s1.R[q][1] * A.nb[1];
+    }
+    J[1] = (((-n1[1]) * x[2] + n1[2] * x[1]) + y[1] * d[2]) + (-y[2]) * d[1];
+    J[5] = ((n1[0] * x[2] + (-n1[2]) * x[0]) + (-y[0]) * d[2]) + y[2] * d[0];
+    J[9] = (((-n1[0]) * x[1] + n1[1] * x[0]) + y[0] * d[1]) + (-y[1]) * d[0];
+    J[13] = row3(n1, n2, y, l1, l2);
+    // column 2 (a2): x = pa2, y = na2
+    for (int q = 0; q < 3; ++q) {
+        x[q] = (s2.R[q][0] * B.pa[0] + s2.R[q][1] * B.pa[1]) - s2.R[q][2] * B.pa[2];
+        y[q] = (s2.R[q][0] * B.na[0] + s2.R[q][1] * B.na[1]) - s2.R[q][2] * B.na[2];
+    }
+    J[2] = (-n1[1]) * (-x[2]) + n1[2] * (-x[1]);
+    J[6] = n1[0] * (-x[2]) + (-n1[2]) * (-x[0]);
+    J[10] = (-n1[0]) * (-x[1]) + n1[1] * (-x[0]);
+    J[14] = row3(n2, n1, y, l2, l1);
+    // column 3 (b2): x = pb2, y = nb2
+    for (int q = 0; q < 3; ++q) {
+        x[q] = s2.R[q][0] * B.pb[0] - s2.R[q][1] * B.pb[1];
+        y[q] = s2.R[q][0] * B.nb[0] - s2.R[q][1] * B.nb[1];
+    }
+    J[3] = (-n1[1]) * (-x[2]) + n1[2] * (-x[1]);
+    J[7] = n1[0] * (-x[2]) + (-n1[2]) * (-x[0]);
+    J[11] = (-n1[0]) * (-x[1]) + n1[1] * (-x[0]);
+    J[15] = row3(n2, n1, y, l2, l1);
 }
 
 // ------------------------------------------------------------------ MINPACK hybrd1, n = 4 (3D) or 2 (2D)
-// Powell's hybrid method (More, Garbow, Hillstrom, ANL-80-74): enorm, qrfac, qform,
-// dogleg, r1updt, r1mpyq with the analytic Jacobian, as minpack.hpp:337-923 runs it.
+// Powell's hybrid method with the analytic Jacobian, as the reference's C++ translation runs it
+// (src/minpack.cpp, minpack.hpp:337-923): enorm, qrfac, qform, dogleg, r1updt, r1mpyq.  Where
+// that translation departs from the published algorithm (ANL-80-74) it is followed: a plain
+// enorm, giant = r8_huge() = 1e30, the first termination test that holds sets info, and dogleg's
+// scaled-gradient loop and its (delta/qnorm)(sgnorm/delta)^2 as written (see dogleg).
 
+// enorm: src/minpack.cpp:36-45
 __device__ inline double enorm(int n, const double* x) {
-    const double rdwarf = 3.834e-20, rgiant = 1.304e19;
-    double s1 = 0, s2 = 0, s3 = 0, x1max = 0, x3max = 0;
-    const double agiant = rgiant / n;
-    for (int i = 0; i < n; ++i) {
-        const double xabs = fabs(x[i]);
-        if (xabs > rdwarf && xabs < agiant) {
-            s2 += xabs * xabs;
-        } else if (xabs <= rdwarf) {
-            if (xabs > x3max) {
-                s3 = 1.0 + s3 * (x3max / xabs) * (x3max / xabs);
-                x3max = xabs;
-            } else if (xabs != 0.0) {
-                s3 += (xabs / x3max) * (xabs / x3max);
-            }
-        } else {
-            if (xabs > x1max) {
-                s1 = 1.0 + s1 * (x1max / xabs) * (x1max / xabs);
-                x1max = xabs;
-            } else {
-                s1 += (xabs / x1max) * (xabs / x1max);
-            }
-        }
-    }
-    if (s1 != 0.0) return x1max * sqrt(s1 + (s2 / x1max) / x1max);
-    if (s2 != 0.0) {
-        if (s2 >= x3max) return sqrt(s2 * (1.0 + (x3max / s2) * (x3max * s3)));
-        return sqrt(x3max * ((s2 / x3max) + (x3max * s3)));
-    }
-    return x3max * sqrt(s3);
+    double value = 0.0;
+    for (int i = 0; i < n; ++i) value = value + x[i] * x[i];
+    return sqrt(value);
 }
 
 // the routines for one system size N
@@ -498,8 +508,14 @@ static __device__ inline void qform(double* q, double* wa) {
     }
 }
 
+// dogleg (src/minpack.cpp:101-240).  hybrd1 carves its work arrays out of one block
+// (minpack.hpp:880-915: diag, qtf, wa1..wa4, r, fjac) and hybrd passes its wa1, wa2, wa3 as
+// dogleg's x, wa1, wa2, so the reference's scaled-gradient loop (:173-182: l = 0, wa1[i - 1] +=
+// r[l - 1] temp) reads hybrd's wa4[n - 1] (wa4_last) as r[-1] and updates x[n - 1] as wa1[-1]:
+// x[n - 1] gains wa4[n - 1] qtb[0], each wa1[k] (k < n - 1) is accumulated, divided by diag[k] and
+// then gains r(k, n - 1) qtb[k + 1], and wa1[n - 1] stays 0.  That loop is kept as written.
 static __device__ inline void dogleg(const double* r, const double* diag, const double* qtb, double delta, double* x,
-                              double* wa1, double* wa2) {
+                              double* wa1, double* wa2, double wa4_last) {
     const double epsmch = 2.220446049250313e-16;
     int jj = (N * (N + 1)) / 2 + 1;
     for (int k = 1; k <= N; ++k) {
@@ -509,7 +525,7 @@ static __device__ inline void dogleg(const double* r, const double* diag, const 
         int l = jj + 1;
         double sum = 0.0;
         for (int i = jp1; i <= N; ++i) {
-            sum += r[l - 1] * x[i - 1];
+            sum = sum + r[l - 1] * x[i - 1];
             ++l;
         }
         double temp = r[jj - 1];
@@ -530,28 +546,35 @@ static __device__ inline void dogleg(const double* r, const double* diag, const 
     }
     const double qnorm = enorm(N, wa2);
     if (qnorm <= delta) return;
-    int l = 1;
-    for (int j = 1; j <= N; ++j) {
-        const double temp = qtb[j - 1];
-        for (int i = j; i <= N; ++i) {
-            wa1[i - 1] += r[l - 1] * temp;
-            ++l;
+    {
+        // the reference's loop with its element -1 of wa1 (x[N - 1]) and of r (wa4_last)
+        int l = 0;
+        for (int j = 0; j < N; ++j) {
+            const double temp = qtb[j];
+            for (int i = j; i < N; ++i) {
+                const double rl = (l >= 1) ? r[l - 1] : wa4_last;
+                if (i >= 1)
+                    wa1[i - 1] = wa1[i - 1] + rl * temp;
+                else
+                    x[N - 1] = x[N - 1] + rl * temp;
+                l = l + 1;
+            }
+            wa1[j] = wa1[j] / diag[j];
         }
-        wa1[j - 1] /= diag[j - 1];
     }
     const double gnorm = enorm(N, wa1);
     double sgnorm = 0.0;
     double alpha = delta / qnorm;
     if (gnorm != 0.0) {
         for (int j = 0; j < N; ++j) wa1[j] = (wa1[j] / gnorm) / diag[j];
-        l = 1;
-        for (int j = 1; j <= N; ++j) {
+        int l = 0;
+        for (int j = 0; j < N; ++j) {
             double sum = 0.0;
-            for (int i = j; i <= N; ++i) {
-                sum += r[l - 1] * wa1[i - 1];
+            for (int i = j; i < N; ++i) {
+                sum = sum + r[l] * wa1[i];
                 ++l;
             }
-            wa2[j - 1] = sum;
+            wa2[j] = sum;
         }
         const double temp = enorm(N, wa2);
         sgnorm = (gnorm / temp) / temp;
@@ -559,7 +582,7 @@ static __device__ inline void dogleg(const double* r, const double* diag, const 
         if (sgnorm < delta) {
             const double bnorm = enorm(N, qtb);
             double t = (bnorm / gnorm) * (bnorm / qnorm) * (sgnorm / delta);
-            t = t - (delta / qnorm) * (sgnorm / delta) * (sgnorm / delta) +
+            t = t - ((delta / qnorm) * (sgnorm / delta)) * (sgnorm / delta) +
                 sqrt((t - (delta / qnorm)) * (t - (delta / qnorm)) +
                      (1.0 - (delta / qnorm) * (delta / qnorm)) * (1.0 - (sgnorm / delta) * (sgnorm / delta)));
             alpha = ((delta / qnorm) * (1.0 - (sgnorm / delta) * (sgnorm / delta))) / t;
@@ -569,8 +592,9 @@ static __device__ inline void dogleg(const double* r, const double* diag, const 
     for (int j = 0; j < N; ++j) x[j] = temp * wa1[j] + alpha * x[j];
 }
 
+// r1updt (src/minpack.cpp:373-542); giant = r8_huge() = 1e30 (:60-65, :397)
 static __device__ inline void r1updt(double* s, const double* u, double* v, double* w, bool& sing) {
-    const double giant = 1.79769313486231570e308;
+    const double giant = 1.0e30;
     const int m = N, n = N;
     int jj = (n * (2 * m - n + 1)) / 2 - (m - n);
     int l = jj;
@@ -692,7 +716,10 @@ struct State {
 };
 
 static __device__ inline void start(State& h, const double* u0) {
-    for (int k = 0; k < N; ++k) h.x[k] = u0[k];
+    for (int k = 0; k < N; ++k) {
+        h.x[k] = u0[k];
+        h.wa4[k] = 0.0;  // fresh work block: read by dogleg only on a scaled-gradient first step
+    }
     h.delta = h.xnorm = h.fnorm = h.pnorm = 0.0;
     h.info = h.nfev = 0;
     h.iter = 1;
@@ -728,8 +755,7 @@ static __device__ inline bool pass(Res residual, State& h) {
         if (h.iter == 1) {
             for (int j = 0; j < N; ++j) h.wa3[j] = diag[j] * h.x[j];
             h.xnorm = enorm(N, h.wa3);
-            h.delta = factor * h.xnorm;
-            if (h.delta == 0.0) h.delta = factor;
+            h.delta = (h.xnorm == 0.0) ? factor : factor * h.xnorm;
         }
         for (int i = 0; i < N; ++i) h.qtf[i] = h.fvec[i];
         for (int j = 0; j < N; ++j) {
@@ -793,13 +819,17 @@ static __device__ inline bool pass(Res residual, State& h) {
         if (actred >= p001) h.nslow1 = 0;
         if (h.jeval) ++h.nslow2;
         if (actred >= p1) h.nslow2 = 0;
-        if (h.delta <= xtol * h.xnorm || h.fnorm == 0.0) h.info = 1;
-        if (h.info == 0) {
-            if (maxfev <= h.nfev) h.info = 2;
-            if (p1 * smax(p1 * h.delta, h.pnorm) <= epsmch * h.xnorm) h.info = 3;
-            if (h.nslow2 == 5) h.info = 4;
-            if (h.nslow1 == 10) h.info = 5;
-        }
+        // the first test that holds ends hybrd (minpack.hpp:760-791)
+        if (h.delta <= xtol * h.xnorm || h.fnorm == 0.0)
+            h.info = 1;
+        else if (maxfev <= h.nfev)
+            h.info = 2;
+        else if (p1 * smax(p1 * h.delta, h.pnorm) <= epsmch * h.xnorm)
+            h.info = 3;
+        else if (h.nslow2 == 5)
+            h.info = 4;
+        else if (h.nslow1 == 10)
+            h.info = 5;
         if (h.info != 0) {
             if (h.info == 5) h.info = 4;
             return true;
@@ -822,7 +852,7 @@ static __device__ inline bool pass(Res residual, State& h) {
         h.jeval = false;
     }
     // propose the next trial point (head of hybrd's inner loop)
-    dogleg(h.r, diag, h.qtf, h.delta, h.wa1, h.wa2, h.wa3);
+    dogleg(h.r, diag, h.qtf, h.delta, h.wa1, h.wa2, h.wa3, h.wa4[N - 1]);
     for (int j = 0; j < N; ++j) {
         h.wa1[j] = -h.wa1[j];
         h.wa2[j] = h.x[j] + h.wa1[j];
@@ -911,10 +941,10 @@ __device__ inline void nt_terms(const SE& s1, const double* u, bool exact, NtTer
         if (deriv) {
             t.dp[2 * f] = dv[0];
             t.dp[2 * f + 1] = dv[1];
-            if (f == 0 && exact) {  // (p sign(x)) |x|^(p-1) instead of p |x|^(p-1)
-                const double p = 2 - s1.n, ax = fabs(x);
-                t.dp[1] = (ax != ax || ax == 0 || ax == INFINITY) ? (p * sgn(x)) * pm_pow(ax, p - 1)
-                                                                 : (p * sgn(x)) * pm_exp((p - 1) * pm_log(ax));
+            if (f == 0 && exact) {  // ((2-n) sign(x)) |x|^(1-n) instead of (2-n) |x|^(1-n)
+                const double p = 2 - s1.n, pd = 1 - s1.n, ax = fabs(x);
+                t.dp[1] = (ax != ax || ax == 0 || ax == INFINITY) ? (p * sgn(x)) * pm_pow(ax, pd)
+                                                                 : (p * sgn(x)) * pm_exp(pd * pm_log(ax));
             }
         }
     }
@@ -1173,9 +1203,9 @@ __device__ inline void newton_pair(const Particles& P, int64_t i, int64_t j, dou
 // closest_points(sphere<2>, superellipsoid<2>) (:1316-1448) and (superellipsoid<2>, plane<2>)
 // (:1782-1949): damped Newton from a distance table.  The objects use the 2 x 2 block of the
 // rotation (rotation3 of a 2D quaternion holds rotation_matrix<2> there, bit for bit), the
-// superellipsoid its squareness e.  Jacobians: forward mode, one direction at a time, with
-// the same operations as the full forward mode (the reference's hand-written ones are the
-// exact derivatives of its residuals).
+// superellipsoid its squareness e.  Jacobians: the reference's hand-written one for the
+// superellipsoid pair (se2_se2_residual); forward mode, one direction at a time, for the Newton
+// pairs, whose solve (LAPACK gesv, BLAS dnrm2 in the reference) is restated, not reproduced.
 __device__ inline void se2_point(const SE& s, double b, double* out) {  // superellipsoid.hpp:281-289
     const double cb = pm_cos(b), sb = pm_sin(b);
     const double x = s.r[0] * sgn(cb) * pm_pow(fabs(cb), s.e);
@@ -1242,6 +1272,12 @@ __device__ inline D1 v_spow(const D1& a, double p) {
     const double ax = fabs(a.v);
     return D1{sgn(a.v) * pm_pow(ax, p), (p * pm_pow(ax, p - 1)) * a.d};
 }
+// sign(x)|x|^(2-s), derivative (2-s)|x|^(1-s) x' with the exponent 1 - s as the reference writes it
+__device__ inline double v_spow2(double a, double s) { return sgn(a) * pm_pow(fabs(a), 2 - s); }
+__device__ inline D1 v_spow2(const D1& a, double s) {
+    const double ax = fabs(a.v);
+    return D1{sgn(a.v) * pm_pow(ax, 2 - s), ((2 - s) * pm_pow(ax, 1 - s)) * a.d};
+}
 __device__ inline double vval(double a) { return a; }
 __device__ inline double vval(const D1& a) { return a.v; }
 __device__ inline double vder(double) { return 0.0; }
@@ -1252,14 +1288,14 @@ __device__ inline double vder(const D1& a) { return a.d; }
 template <class V>
 __device__ inline void se2_terms(const SE& s, double b, double seed, V& nx, V& ny, V& px, V& py) {
     const V cb = v_cos<V>(b, seed), sb = v_sin<V>(b, seed);
-    nx = s.r[1] * v_spow(cb, 2 - s.e);
-    ny = s.r[0] * v_spow(sb, 2 - s.e);
+    nx = s.r[1] * v_spow2(cb, s.e);
+    ny = s.r[0] * v_spow2(sb, s.e);
     px = s.r[0] * v_spow(cb, s.e);
     py = s.r[1] * v_spow(sb, s.e);
 }
 
-// residual of closest_points.hpp:79-125 (kind 2), :1332-1366 (kind 0: sphere s2 with
-// superellipsoid s1) or :1800-1838 (kind 1: superellipsoid s1 with plane p2, sp = (c_s1 - c_p) . n_p)
+// residual of closest_points.hpp:1332-1366 (kind 0: sphere s2 with superellipsoid s1) or
+// :1800-1838 (kind 1: superellipsoid s1 with plane p2, sp = (c_s1 - c_p) . n_p)
 template <class V>
 __device__ inline void se2_algebra(int kind, const SE& s1, const SE& s2, const Sph& sp2, const Pln& p2, double sp,
                                    const double* u, int k, V* F) {
@@ -1267,19 +1303,7 @@ __device__ inline void se2_algebra(int kind, const SE& s1, const SE& s2, const S
     se2_terms<V>(s1, u[0], k == 0 ? 1.0 : 0.0, n1x, n1y, p1x, p1y);
     const double seed2 = (k == 1) ? 1.0 : 0.0;
     const auto& A = s1.R;
-    if (kind == 2) {
-        V n2x, n2y, p2x, p2y;
-        se2_terms<V>(s2, u[1], seed2, n2x, n2y, p2x, p2y);
-        const auto& B = s2.R;
-        const V m1x = A[0][1] * n1y + A[0][0] * n1x, m1y = A[1][1] * n1y + A[1][0] * n1x;
-        const V m2x = B[0][1] * n2y + B[0][0] * n2x, m2y = B[1][1] * n2y + B[1][0] * n2x;
-        const V dy = (((((-(A[1][0] * p1x)) - A[1][1] * p1y) - cst<V>(s1.c[1])) + B[1][0] * p2x) + B[1][1] * p2y) +
-                     cst<V>(s2.c[1]);
-        const V dx = (((((-(A[0][0] * p1x)) - A[0][1] * p1y) - cst<V>(s1.c[0])) + B[0][0] * p2x) + B[0][1] * p2y) +
-                     cst<V>(s2.c[0]);
-        F[0] = (-m1x) * dy + m1y * dx;
-        F[1] = (m1x * m2x + m1y * m2y) + dsqrt(m1x * m1x + m1y * m1y) * dsqrt(m2x * m2x + m2y * m2y);
-    } else if (kind == 0) {
+    if (kind == 0) {
         const V q5 = sp2.r * v_sin<V>(u[1], seed2), q6 = sp2.r * v_cos<V>(u[1], seed2);
         const auto& B = sp2.R;
         const V m1x = A[0][0] * n1x + A[0][1] * n1y, m1y = A[1][0] * n1x + A[1][1] * n1y;
@@ -1310,7 +1334,55 @@ struct Se2Pair {
     double sp;
 };
 
+// SE2-SE2 residual (closest_points.hpp:79-125) with the reference's hand-written Jacobian
+// (:127-189).  Body terms F1 = rx sb^(2-e), F2 = ry sb^e, F3 = ry cb^(2-e), F4 = rx cb^e (signed
+// powers) and derivative terms F9 = e rx sb |cb|^(e-1), F10 = e ry cb |sb|^(e-1),
+// F11 = ry (2-e) sb |cb|^(1-e), F12 = rx (2-e) cb |sb|^(1-e) (F5..F8, F13..F16 for the second
+// body); the factors of the entries are rotated sums of these or their exact negations, summed
+// left to right as written.
+struct Se2Side {
+    double f1, f2, f3, f4, f9, f10, f11, f12;
+};
+__device__ inline Se2Side se2_side(const SE& s, double b, bool deriv) {
+    const double cb = pm_cos(b), sb = pm_sin(b), acb = fabs(cb), asb = fabs(sb);
+    const double cbe = sgn(cb) * pm_pow(acb, s.e), cbe2 = sgn(cb) * pm_pow(acb, 2 - s.e);
+    const double sbe = sgn(sb) * pm_pow(asb, s.e), sbe2 = sgn(sb) * pm_pow(asb, 2 - s.e);
+    Se2Side t{s.r[0] * sbe2, s.r[1] * sbe, s.r[1] * cbe2, s.r[0] * cbe, 0.0, 0.0, 0.0, 0.0};
+    if (deriv) {
+        t.f9 = ((s.e * s.r[0]) * sb) * pm_pow(acb, s.e - 1);
+        t.f10 = ((s.e * s.r[1]) * cb) * pm_pow(asb, s.e - 1);
+        t.f11 = ((s.r[1] * (2 - s.e)) * sb) * pm_pow(acb, 1 - s.e);
+        t.f12 = ((s.r[0] * (2 - s.e)) * cb) * pm_pow(asb, 1 - s.e);
+    }
+    return t;
+}
+__device__ inline void se2_se2_residual(const SE& s1, const SE& s2, const double* u, double* F, double* J) {
+    const Se2Side a = se2_side(s1, u[0], J != nullptr), b = se2_side(s2, u[1], J != nullptr);
+    const auto& M = s1.R;
+    const auto& N = s2.R;
+    const double m1x = M[0][1] * a.f1 + M[0][0] * a.f3, m1y = M[1][1] * a.f1 + M[1][0] * a.f3;
+    const double m2x = N[0][1] * b.f1 + N[0][0] * b.f3, m2y = N[1][1] * b.f1 + N[1][0] * b.f3;
+    const double dy = (((((-(M[1][0] * a.f4)) - M[1][1] * a.f2) - s1.c[1]) + N[1][0] * b.f4) + N[1][1] * b.f2) + s2.c[1];
+    const double dx = (((((-(M[0][0] * a.f4)) - M[0][1] * a.f2) - s1.c[0]) + N[0][0] * b.f4) + N[0][1] * b.f2) + s2.c[0];
+    F[0] = (-m1x) * dy + m1y * dx;
+    const double l1 = sqrt(m1x * m1x + m1y * m1y), l2 = sqrt(m2x * m2x + m2y * m2y);
+    F[1] = (m1x * m2x + m1y * m2y) + l1 * l2;
+    if (!J) return;
+    const double g1x = M[0][0] * a.f9 - M[0][1] * a.f10, g1y = M[1][0] * a.f9 - M[1][1] * a.f10;
+    const double h1x = M[0][1] * a.f12 - M[0][0] * a.f11, h1y = M[1][1] * a.f12 - M[1][0] * a.f11;
+    const double g2x = N[0][0] * b.f9 - N[0][1] * b.f10, g2y = N[1][0] * b.f9 - N[1][1] * b.f10;
+    const double h2x = N[0][1] * b.f12 - N[0][0] * b.f11, h2y = N[1][1] * b.f12 - N[1][0] * b.f11;
+    J[0] = (((-m1x) * g1y + m1y * g1x) + (-h1x) * dy) + h1y * dx;
+    J[1] = (-m1x) * (-g2y) + m1y * (-g2x);
+    J[2] = (((m1x * h1x + m1y * h1y) * l2) / l1 + m2x * h1x) + m2y * h1y;
+    J[3] = (((m2x * h2x + m2y * h2y) * l1) / l2 + m1x * h2x) + m1y * h2y;
+}
+
 __device__ inline void se2_residual(const Se2Pair& q, const double* u, double* F, double* J) {
+    if (q.kind == 2) {
+        se2_se2_residual(q.s1, q.s2, u, F, J);
+        return;
+    }
     if (!J) {
         se2_algebra<double>(q.kind, q.s1, q.s2, q.sp2, q.p2, q.sp, u, -1, F);
         return;

@@ -100,9 +100,17 @@ class Container:
                    prop or Property(active=False))
 
     def segment(self, pos, length, angle=0.0, prop=None, q=None):
-        """segment<2>(pos, q, length) (include/scopi/objects/types/segment.hpp:135-141)."""
+        """segment<2>(pos, q, length) (include/scopi/objects/types/segment.hpp:147-152)."""
         self._push(SEGMENT, pos, quaternion(angle) if q is None else q, (length, 0, 0), (1.0, 1.0),
                    prop or Property(active=False))
+
+    def segment_points(self, pos1, pos2, prop=None):
+        """segment<2>(pos1, pos2) (include/scopi/objects/types/segment.hpp:133-138): centre at the
+        midpoint, angle atan2(x1 - x2, y2 - y1), length |pos2 - pos1|."""
+        p1, p2 = np.asarray(pos1, np.float64), np.asarray(pos2, np.float64)
+        d = p2 - p1
+        self.segment(0.5 * (p1 + p2), math.sqrt(d[0] * d[0] + d[1] * d[1]), math.atan2(p1[0] - p2[0], p2[1] - p1[1]),
+                     prop=prop)
 
     def superellipsoid(self, pos, q, radius, squareness, prop=None):
         """superellipsoid<3>(pos, q, radius(3), squareness(e, n)) (superellipsoid.hpp)."""
@@ -278,6 +286,18 @@ def superellipsoids_3d(nx=32, ny=32, nz=32, spacing=0.035, jitter=0.003, amin=0.
     return Scene(dim=3, n_inactive=0, shape=np.full(n, SUPERELLIPSOID, np.int32), pos=pos, q=q, size=size,
                  squareness=sq, v=v, vd=np.zeros((n, 3)), omega=np.zeros((n, 3)), domega=np.zeros((n, 3)),
                  f=np.zeros((n, 3)), m=np.ones(n), J=J)
+
+
+def segment_demo(x, y, r):
+    """The scene of demos/segment.cpp:15-60: two segment obstacles, (0, 1)-(0.4, 1) and (0.6, 1)-(1, 1),
+    and one sphere per draw (x, y, r) with desired velocity (0.5 - x, 2 - y), mass 1, inertia 0.1
+    (the demo takes its draws from std::default_random_engine)."""
+    c = Container(2)
+    c.segment_points((0.0, 1.0), (0.4, 1.0))
+    c.segment_points((0.6, 1.0), (1.0, 1.0))
+    for xi, yi, ri in zip(x, y, r):
+        c.sphere((xi, yi), ri, Property(desired_velocity=(0.5 - xi, 2 - yi), mass=1.0, moment_inertia=0.1))
+    return c.arrays()
 
 
 # Default solver parameters of the workloads (BASELINE configurations).

@@ -7,7 +7,16 @@ is pinned by:
     commented out in the reference but with closed-form expectations);
   * scipy's MINPACK hybrj, an independent implementation of the same Powell hybrid
     method: from the same start, with the same F and Jacobian and the same evaluation
-    budget, the restated hybrd1 must end on the same bits;
+    budget, the restated hybrd1 ends on the same bits when it takes the published
+    algorithm's forms (ANL-80-74: scaled enorm, giant = DBL_MAX, dogleg's gradient loop,
+    termination order; a test-only switch);
+  * the places where the reference's C++ translation (src/minpack.cpp) differs from the
+    published algorithm, one by one: plain enorm, giant = 1e30, the termination order,
+    dogleg's (delta/qnorm)(sgnorm/delta)^2 taken left to right (the hybrj pin fails without
+    it), and dogleg's scaled-gradient loop, whose index shift is pinned against its closed
+    form written as dense matrix algebra; the forms change the results of real pairs;
+  * the reference's hand-written Jacobian (closest_points.hpp:385-603) against forward-mode
+    differentiation of F and against finite differences;
   * invariants of the end point: both points on their implicit surfaces, unit normal
     anti-parallel to the first body's normal, residual at rounding level;
   * special cases: e = n = 1 with equal semi-axes is a sphere (closed form), e = n = 1
@@ -71,8 +80,17 @@ def test_se_known_answers(se, case):
 
 
 # ------------------------------------------------------------------ hybrd1 vs MINPACK hybrj
-def test_hybrd_matches_minpack_hybrj(se):
+@pytest.fixture
+def anl(se):
+    se.set_minpack_variant(se.MINPACK_ANL)
+    yield se
+    se.set_minpack_variant(0)
+
+
+def test_hybrd_matches_minpack_hybrj(anl):
+    """With the published algorithm's forms, every step of the restated hybrd1 is MINPACK's."""
     from scipy.optimize import root
+    se = anl
     rng = np.random.default_rng(1)
     infos = set()
     for _ in range(16):
@@ -89,10 +107,148 @@ def test_hybrd_matches_minpack_hybrj(se):
     assert infos >= {1, 4}  # both exits are exercised
 
 
+def dogleg_model(R, diag, qtb, delta, wa4_last, shifted):
+    """Powell's dogleg step (ANL-80-74 dogleg) as dense algebra.  shifted: the scaled gradient the
+    reference's loop (src/minpack.cpp:173-182, l = 0, wa1[i - 1] += r[l - 1] temp) forms inside
+    hybrd1's single work block, where the element before dogleg's wa1 is the Gauss-Newton step's
+    last component and the element before r is hybrd's wa4[n - 1]: each g[k], k < n - 1, gains
+    R[k, n-1] qtb[k + 1] after its division by diag[k], g[n - 1] = 0, and the Gauss-Newton step's
+    last component gains wa4[n - 1] qtb[0]."""
+    n = len(diag)
+    x = np.linalg.solve(R, qtb)  # Gauss-Newton direction (R upper triangular, nonsingular)
+    qnorm = np.linalg.norm(diag * x)
+    if qnorm <= delta:
+        return x, False
+    g = (R.T @ qtb) / diag
+    if shifted:
+        g[: n - 1] += R[: n - 1, n - 1] * qtb[1:]
+        g[n - 1] = 0.0
+        x[n - 1] += wa4_last * qtb[0]
+    gnorm = np.linalg.norm(g)
+    sgnorm, alpha = 0.0, delta / qnorm
+    if gnorm != 0:
+        g = (g / gnorm) / diag
+        t = np.linalg.norm(R @ g)
+        sgnorm, alpha = (gnorm / t) / t, 0.0
+        if sgnorm < delta:
+            bnorm = np.linalg.norm(qtb)
+            t = (bnorm / gnorm) * (bnorm / qnorm) * (sgnorm / delta)
+            t = t - (delta / qnorm) * (sgnorm / delta) ** 2 + math.sqrt(
+                (t - delta / qnorm) ** 2 + (1 - (delta / qnorm) ** 2) * (1 - (sgnorm / delta) ** 2))
+            alpha = ((delta / qnorm) * (1 - (sgnorm / delta) ** 2)) / t
+    return (1 - alpha) * min(sgnorm, delta) * g + alpha * x, True
+
+
+@pytest.mark.parametrize("n", [2, 4])
+def test_dogleg_index_shift(oracle_lib, n):
+    """The restated dogleg against the closed form of the reference's loop (both branches of the
+    gradient step), and against the published loop under the test-only switch."""
+    rng = np.random.default_rng(20 + n)
+    both = set()
+    for _ in range(40):
+        R = np.triu(rng.normal(size=(n, n))) + np.diag(rng.uniform(0.5, 2.0, n))
+        packed = np.concatenate([R[k, k:] for k in range(n)])
+        diag = rng.uniform(0.5, 2.0, n)
+        qtb = rng.normal(size=n)
+        wa4_last = rng.normal()
+        qnorm = np.linalg.norm(diag * np.linalg.solve(R, qtb))
+        delta = qnorm * rng.uniform(0.02, 0.9)
+        x, grad = oracle_lib.minpack_dogleg(packed, diag, qtb, delta, wa4_last)
+        xm, gm = dogleg_model(R, diag, qtb, delta, wa4_last, shifted=True)
+        assert grad and gm
+        np.testing.assert_allclose(x, xm, rtol=1e-10, atol=1e-12 * np.abs(xm).max())
+        xa, _ = dogleg_model(R, diag, qtb, delta, wa4_last, shifted=False)
+        assert np.abs(x - xa).max() > 1e-6 * np.abs(xa).max()  # the shift is visible
+        oracle_lib.set_minpack_variant(oracle_lib.MINPACK_ANL_DOGLEG)
+        try:
+            xp, _ = oracle_lib.minpack_dogleg(packed, diag, qtb, delta, wa4_last)
+        finally:
+            oracle_lib.set_minpack_variant(0)
+        np.testing.assert_allclose(xp, xa, rtol=1e-10, atol=1e-12 * np.abs(xa).max())
+        # alpha > 0 (wa4[n-1] enters through the Gauss-Newton component) and alpha = 0 both occur
+        g = (R.T @ qtb) / diag
+        g[: n - 1] += R[: n - 1, n - 1] * qtb[1:]
+        g[n - 1] = 0
+        gs = (g / np.linalg.norm(g)) / diag
+        sg = (np.linalg.norm(g) / np.linalg.norm(R @ gs)) / np.linalg.norm(R @ gs)
+        both.add(sg < delta)
+    # a Gauss-Newton step inside the trust region is returned unchanged
+    R = np.eye(n); packed = np.concatenate([R[k, k:] for k in range(n)])
+    x, grad = oracle_lib.minpack_dogleg(packed, np.ones(n), np.full(n, 0.1), 10.0, 5.0)
+    assert not grad and np.array_equal(x, np.full(n, 0.1))
+    assert both == {True, False}
+
+
+def neighbour_pairs(n_side=4, limit=120):
+    """Neighbouring superellipsoid pairs of a small scene built like the 32,768-superellipsoid workload."""
+    from scipy.spatial import cKDTree
+    from scopi_amd import scenes
+    sc = scenes.superellipsoids_3d(n_side, n_side, n_side)
+    pairs = cKDTree(sc.pos).query_pairs(math.sqrt(scenes.CONFIG3["kd_tree_radius"]), output_type="ndarray")
+    pairs = pairs[np.lexsort((pairs[:, 1], pairs[:, 0]))][:limit]
+    return [(sc.pos[a], sc.q[a], sc.size[a], sc.squareness[a], sc.pos[b], sc.q[b], sc.size[b], sc.squareness[b])
+            for a, b in pairs]
+
+
+def test_minpack_forms_change_results(oracle_lib):
+    """The reference's forms are not rounding-neutral on real pairs: the plain enorm alone (against
+    ANL-80-74's scaled one, which treats components below 3.834e-20 apart, as F3 = n1.n2 + |n1||n2|
+    becomes near the root) and the whole set of forms each move the end points of some neighbouring
+    pairs; giant = 1e30 against DBL_MAX moves none of them.  The default is the reference's."""
+    se = oracle_lib
+    pairs = neighbour_pairs()
+    assert len(pairs) == 120
+
+    def run(mask):
+        se.set_minpack_variant(mask)
+        try:
+            return [se.se_closest_points(*p) for p in pairs]
+        finally:
+            se.set_minpack_variant(0)
+    ref = run(0)
+    assert all(np.array_equal(a["u"], b["u"]) for a, b in zip(ref, [se.se_closest_points(*p) for p in pairs]))
+    moved = {}
+    for name, mask in (("enorm", se.MINPACK_ANL_ENORM), ("giant", se.MINPACK_ANL_GIANT), ("anl", se.MINPACK_ANL)):
+        moved[name] = sum(not np.array_equal(a["u"], b["u"]) for a, b in zip(ref, run(mask)))
+    assert moved["enorm"] >= 5 and moved["anl"] >= moved["enorm"] and moved["giant"] == 0, moved
+
+
+def test_fresh_work_array_reads(oracle_lib):
+    """The reference's first dogleg reads hybrd's wa4[n - 1] from hybrd1's fresh, uninitialised work
+    block (minpack.hpp:880) when its first step takes the scaled-gradient branch, i.e. when the
+    Gauss-Newton step is longer than 100 |u0|.  Where that happens the reference's result depends on
+    leftover heap contents; the restatement (and the device) read 0, the value of fresh zero-filled
+    memory.  It happens on a few percent of neighbouring pairs: parity with the reference is
+    unpinned for exactly those pairs."""
+    before = oracle_lib.dogleg_fresh_wa4_count()
+    pairs = neighbour_pairs()
+    for p in pairs:
+        oracle_lib.se_closest_points(*p)
+    fresh = oracle_lib.dogleg_fresh_wa4_count() - before
+    assert 0 < fresh <= 0.1 * len(pairs), fresh
+
+
+def test_reference_jacobian_equals_forward_mode(se):
+    """The hand-written Jacobian (closest_points.hpp:385-603) and the forward-mode derivative of the
+    same F agree to rounding: no term or sign of the restatement is off."""
+    rng = np.random.default_rng(12)
+    for _ in range(40):
+        args = random_pair(rng)
+        u = rng.uniform(-3.0, 3.0, 4)
+        F, J = se.se_residual(*args, u, jacobian=True)
+        se.set_jacobian_ad(True)
+        try:
+            Fa, Ja = se.se_residual(*args, u, jacobian=True)
+        finally:
+            se.set_jacobian_ad(False)
+        assert np.array_equal(F, Fa)
+        assert np.abs(J - Ja).max() <= 1e-12 * np.abs(Ja).max()
+
+
 def test_hybrd_budget(se):
     rng = np.random.default_rng(1)
     seen = False
-    for _ in range(16):
+    for _ in range(64):
         r = se.se_closest_points(*random_pair(rng))
         assert r["nfev"] < 1000 + 5
         if r["info"] == 2:
@@ -132,10 +288,13 @@ def rot(q):
 
 
 def test_se_end_point_invariants(se):
-    rng = np.random.default_rng(7)
+    """On neighbouring pairs of the superellipsoid workload's recipe: every end point lies on both
+    surfaces; at converged exits the normals are anti-parallel and pi - pj is parallel to nij to the
+    accuracy hybrd1's step test leaves on this singular root (the reference's src/minpack.cpp forms
+    stop short more often than the published algorithm: 13 of these 120 pairs converge against 25)."""
     n_conv = 0
-    for _ in range(16):
-        ca, qa, ra, ea, cb, qb, rb, eb = args = random_pair(rng)
+    for args in neighbour_pairs():
+        ca, qa, ra, ea, cb, qb, rb, eb = args
         r = se.se_closest_points(*args)
         assert abs(implicit(r["pi"], ca, qa, ra, ea) - 1) < 1e-9
         assert abs(implicit(r["pj"], cb, qb, rb, eb) - 1) < 1e-9
@@ -147,8 +306,8 @@ def test_se_end_point_invariants(se):
             n_conv += 1
             _, n1 = se.se_point_normal(ca, qa, ra, ea, r["u"][0], r["u"][1])
             assert n1 @ r["nij"] < -1 + 1e-9            # anti-parallel normals
-            assert np.linalg.norm(np.cross(d, r["nij"])) < 1e-6 * np.linalg.norm(d)  # d parallel to nij
-    assert n_conv >= 3
+            assert np.linalg.norm(np.cross(d, r["nij"])) < 1e-5 * np.linalg.norm(d)  # d parallel to nij
+    assert n_conv >= 8
 
 
 def test_se_sphere_special_case(se):
@@ -200,7 +359,19 @@ def ellipsoid_distance(ca, Ra, ra, cb, Rb, rb):
     return res.fun
 
 
-def test_se_ellipsoid_against_direct_minimisation(se):
+@pytest.mark.parametrize("forms", ["reference", "anl"])
+def test_se_ellipsoid_against_direct_minimisation(se, forms):
+    """e = n = 1: ellipsoids.  With the published algorithm's forms hybrd1 reaches the true distance
+    to 1e-8; the reference's forms may stall short of it (info 4: 0.7% above it on one of these
+    pairs), never below it."""
+    se.set_minpack_variant(se.MINPACK_ANL if forms == "anl" else 0)
+    try:
+        _ellipsoids(se, forms == "anl")
+    finally:
+        se.set_minpack_variant(0)
+
+
+def _ellipsoids(se, exact):
     rng = np.random.default_rng(5)
     for _ in range(3):
         qa = rng.normal(size=4); qa /= np.linalg.norm(qa)
@@ -210,7 +381,11 @@ def test_se_ellipsoid_against_direct_minimisation(se):
         r = se.se_closest_points(ca, qa, ra, (1, 1), cb, qb, rb, (1, 1))
         ref = ellipsoid_distance(ca, rot(qa), ra, cb, rot(qb), rb)
         assert r["dij"] > 0
-        assert abs(r["dij"] - ref) < 1e-8, (r["dij"], ref, r["info"])
+        if exact or r["info"] == 1:
+            assert abs(r["dij"] - ref) < 1e-8, (r["dij"], ref, r["info"])
+        else:
+            # a stalled end point is a pair of surface points: never closer than the true distance
+            assert ref - 1e-12 < r["dij"] < 1.02 * ref, (r["dij"], ref, r["info"])
 
 
 # ------------------------------------------------------------------ portable elementary functions
@@ -387,8 +562,8 @@ def test_separated_pairs_can_be_reference_contacts(oracle_lib):
     that did not converge leaves the normal pointing the wrong way, even for bodies far apart.  On the
     32,768-superellipsoid workload about half of the 469,920 candidate pairs have boxes more than dmax
     apart; among 3,000 of them the restated closest points give some a distance below -0.04, all with
-    hybrd1 info 4.  The reference records those as contacts, so the device solves every candidate
-    instead of dropping box-separated pairs."""
+    hybrd1 info 4 or 2 (no progress, evaluation budget).  The reference records those as contacts, so
+    the device solves every candidate instead of dropping box-separated pairs."""
     from scipy.spatial import cKDTree
     from scopi_amd import scenes
     sc = scenes.superellipsoids_3d()
@@ -406,4 +581,4 @@ def test_separated_pairs_can_be_reference_contacts(oracle_lib):
         if r["dij"] < dmax:
             spurious.append((r["dij"], r["info"]))
     assert 5 <= len(spurious) <= 30
-    assert all(d < -0.04 and info == 4 for d, info in spurious)
+    assert all(d < -0.04 and info in (2, 4) for d, info in spurious)

@@ -13,7 +13,9 @@ Pins:
   argument order asks (closest_points.hpp:1443-1446), the reason the reference's own test of
   this pair is marked as failing;
 * superellipsoid / plane: the reference's residual uses (c_s - c_p) . n_p as a sign; with that value
-  equal to -1 the Newton root is the true closest point.
+  equal to -1 the Newton root is the true closest point;
+* the SE2-SE2 Jacobian, written term by term as the reference's (closest_points.hpp:127-189),
+  against forward-mode differentiation of F and against finite differences.
 """
 import math
 
@@ -154,3 +156,30 @@ def test_superellipsoid_plane_at_unit_offset(cp):
     pts = rot(a) @ np.array([0.1 * np.sign(np.cos(ts)) * np.abs(np.cos(ts)) ** e, 0.2 * np.sign(np.sin(ts)) * np.abs(np.sin(ts)) ** e])
     assert abs(r["pi"][0] - pts[0].max()) < 1e-6
     assert abs(r["dij"] - (pts[0].max() - 1.0)) < 1e-6
+
+
+def test_se2_reference_jacobian(oracle_lib):
+    rng = np.random.default_rng(31)
+    for portable in (0, 1):
+        oracle_lib.lib().oracle_set_se_math(portable)
+        try:
+            for _ in range(30):
+                a = ((0.0, 0.0), q(rng.uniform(-3, 3)), rng.uniform(0.05, 0.2, 2), rng.uniform(0.4, 1.6))
+                b = ((rng.uniform(0.3, 0.5), rng.uniform(-0.1, 0.1)), q(rng.uniform(-3, 3)), rng.uniform(0.05, 0.2, 2),
+                     rng.uniform(0.4, 1.6))
+                u = rng.uniform(-3, 3, 2)
+                F, J = oracle_lib.se2_residual(*a, *b, u, jacobian=True)
+                oracle_lib.set_jacobian_ad(True)
+                try:
+                    Fa, Ja = oracle_lib.se2_residual(*a, *b, u, jacobian=True)
+                finally:
+                    oracle_lib.set_jacobian_ad(False)
+                assert np.array_equal(F, Fa)
+                assert np.abs(J - Ja).max() <= 1e-12 * np.abs(Ja).max()
+                h = 1e-7
+                for k in range(2):
+                    e = np.zeros(2); e[k] = h
+                    fd = (oracle_lib.se2_residual(*a, *b, u + e) - oracle_lib.se2_residual(*a, *b, u - e)) / (2 * h)
+                    assert np.allclose(J[:, k], fd, rtol=1e-5, atol=1e-9 * np.abs(J).max())
+        finally:
+            oracle_lib.lib().oracle_set_se_math(0)
