@@ -56,6 +56,8 @@ def lib():
         I64 = np.ctypeslib.ndpointer(dtype=np.int64, flags="C_CONTIGUOUS")
         L.oracle_scene_new.restype = P
         L.oracle_scene_free.argtypes = [P]
+        L.oracle_scene_clone.argtypes = [P]
+        L.oracle_scene_clone.restype = P
         L.oracle_default_params.argtypes = [C.POINTER(Params)]
         L.oracle_set_params.argtypes = [P, C.POINTER(Params)]
         L.oracle_scene_set.argtypes = [P, C.c_int32, C.c_int64, C.c_int64, I32] + [D] * 11
@@ -131,6 +133,14 @@ class Oracle:
         if getattr(self, "_h", None):
             self._L.oracle_scene_free(self._h)
             self._h = None
+
+    def clone(self):
+        """A copy of this scene, contact history and parameters included."""
+        o = Oracle.__new__(Oracle)
+        o._L, o.dim, o.n, o.n_inactive, o.problem = self._L, self.dim, self.n, self.n_inactive, self.problem
+        o.params = default_params(**{k: getattr(self.params, k) for k, _ in Params._fields_})
+        o._h = self._L.oracle_scene_clone(self._h)
+        return o
 
     def set_params(self, **kw):
         self.params = default_params(**kw)

@@ -1264,6 +1264,7 @@ void* oracle_scene_new(void) {
     return s;
 }
 void oracle_scene_free(void* scene) { delete static_cast<Scene*>(scene); }
+void* oracle_scene_clone(const void* scene) { return scene ? new Scene(*static_cast<const Scene*>(scene)) : nullptr; }
 
 // Defaults: ContactsParams (contact_kdtree.hpp:52,60), apgd_params (apgd.hpp:120-123),
 // contact_property<Friction>::mu (contact/property.hpp:72).

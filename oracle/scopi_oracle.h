@@ -69,6 +69,8 @@ typedef struct oracle_stats {
 
 void* oracle_scene_new(void);
 void oracle_scene_free(void* scene);
+/* A copy of a scene with its contact history (the solver's m_old_contacts) and parameters. */
+void* oracle_scene_clone(const void* scene);
 
 int oracle_default_params(oracle_params* p);
 int oracle_set_params(void* scene, const oracle_params* p);

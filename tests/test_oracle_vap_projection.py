@@ -1,33 +1,85 @@
-"""vap_projection in the oracle (vap/vap_projection.hpp:34-49): the slot writes the desired
-velocity (u) and the rotation velocity (w(i, 2), the scalar assigned to all three components of
-a 3D rotation velocity), and leaves the velocity itself as it was; the solve starts from that
-velocity.  No reference test uses the slot; the pins are its own assignments."""
+"""vap_projection in the oracle (vap/vap_projection.hpp:66-79, src/vap/vap_projection.cpp:6-10).
+
+The slot sets each active particle's desired velocity to u and its rotation velocity to w(i, 2)
+(a scalar, assigned to all three components of a 3D rotation velocity), and leaves the velocity
+itself alone.  No reference test uses the slot, so it is pinned through what a whole step then
+does, in closed form:
+  * free particles move by dt v, not by dt u, keep v, end the step with the rotation velocity
+    w(i, 2) (in every component in 3D, in the single rotation slot in 2D) and turn by
+    q <- q (cos(|w| dt / 2), sin(|w| dt / 2) w / |w|);
+  * a sphere driven into a plane by its own velocity (not by u) loses exactly the normal velocity
+    that would close the gap in one step (the one-contact NoFriction solve), keeps its tangential
+    velocity, and keeps the rotation velocity w(i, 2) (a sphere's contact arm is parallel to the
+    normal, so the contact does not turn it).
+A slot that copied u into the velocity, wrote the desired rotation velocity instead of the rotation
+velocity, or took another column of w fails one of these.
+"""
+import math
+
 import numpy as np
+import pytest
 
-from scopi_amd.scenes import Container, Property
+from scopi_amd.scenes import Container, Property, quaternion
 
 
-def scene(dim):
+def free_scene(dim, rng):
     c = Container(dim)
-    rng = np.random.default_rng(4)
     for k in range(6):
-        c.sphere(rng.uniform(0, 1, dim), 0.05, Property(mass=1.0, velocity=rng.uniform(-1, 1, dim),
-                                                         moment_inertia=0.1 if dim == 2 else (0.1, 0.1, 0.1)))
+        c.sphere(np.array([0.3 * k, 0.0, 0.0][:dim]) + rng.uniform(0, 0.05, dim), 0.05,
+                 Property(mass=1.0, velocity=rng.uniform(-1, 1, dim),
+                          omega=rng.uniform(-1, 1) if dim == 2 else rng.uniform(-1, 1, 3),
+                          moment_inertia=0.1 if dim == 2 else (0.1, 0.1, 0.1)),
+                 q=quaternion(rng.uniform(0, 2 * math.pi)))
     return c.arrays()
 
 
-def test_projection_assignments(oracle_lib):
-    for dim in (2, 3):
-        sc = scene(dim)
-        o = oracle_lib.Oracle(sc, vap=oracle_lib.VAP_PROJECTION, dmax=0.01, kd_tree_radius=0.2 ** 2)
-        u = np.arange(18, dtype=float).reshape(6, 3) * 0.1
-        w = np.arange(18, dtype=float).reshape(6, 3) * -0.01
-        o.set_projection_velocity(u, w)
-        o.set_a_priori_velocity(0.01)
-        st = o.state()
-        np.testing.assert_array_equal(st["v"], sc.v)  # the velocity is not touched
-        om = st["omega"]
+def qmul(a, b):
+    w1, x1, y1, z1 = a
+    w2, x2, y2, z2 = b
+    return np.array([w1 * w2 - x1 * x2 - y1 * y2 - z1 * z2, w1 * x2 + x1 * w2 + y1 * z2 - z1 * y2,
+                     w1 * y2 - x1 * z2 + y1 * w2 + z1 * x2, w1 * z2 + x1 * y2 - y1 * x2 + z1 * w2])
+
+
+@pytest.mark.parametrize("dim", [2, 3])
+def test_projection_free_particles(oracle_lib, dim):
+    rng = np.random.default_rng(4 + dim)
+    sc = free_scene(dim, rng)
+    dt = 0.01
+    o = oracle_lib.Oracle(sc, vap=oracle_lib.VAP_PROJECTION, dmax=0.01, kd_tree_radius=0.2 ** 2)
+    u = rng.uniform(-3, 3, (6, 3))
+    w = rng.uniform(-3, 3, (6, 3))
+    o.set_projection_velocity(u, w)
+    o.run(dt, 1)
+    st = o.state()
+    np.testing.assert_allclose(st["pos"][:, :dim], sc.pos[:, :dim] + dt * sc.v[:, :dim], rtol=0, atol=1e-15)
+    np.testing.assert_array_equal(st["v"], sc.v)
+    for k in range(6):
+        om = np.array([0.0, 0.0, w[k, 2]]) if dim == 2 else np.full(3, w[k, 2])
         if dim == 2:
-            np.testing.assert_array_equal(om[:, 2], w[:, 2])
+            assert st["omega"][k, 2] == w[k, 2]
         else:
-            np.testing.assert_array_equal(om, np.repeat(w[:, 2:3], 3, axis=1))
+            np.testing.assert_array_equal(st["omega"][k], om)
+        a = np.linalg.norm(om)
+        dq = np.concatenate([[math.cos(0.5 * a * dt)], math.sin(0.5 * a * dt) * om / a])
+        qe = qmul(sc.q[k], dq)
+        np.testing.assert_allclose(st["q"][k], qe / np.linalg.norm(qe), atol=1e-15)
+
+
+def test_projection_sphere_on_plane(oracle_lib):
+    dt, r, gap = 0.01, 0.05, 0.002
+    c = Container(3)
+    c.plane((0.0, 0.0, 0.0), q=quaternion(-math.pi / 2, (0.0, 1.0, 0.0)))  # normal +z
+    v0 = np.array([0.3, -0.2, -1.0])  # closes the gap in one step: -1.0 dt < -gap
+    c.sphere((0.1, 0.2, r + gap), r, Property(mass=1.0, velocity=v0, moment_inertia=(0.1, 0.1, 0.1)))
+    sc = c.arrays()
+    o = oracle_lib.Oracle(sc, vap=oracle_lib.VAP_PROJECTION, dmax=0.01, kd_tree_radius=0.2 ** 2, tolerance=1e-12)
+    u = np.array([[0.0, 0.0, 5.0]])   # would lift the sphere off if the slot used it as the velocity
+    w = np.array([[0.0, 0.0, 0.7]])
+    o.set_projection_velocity(u, w)
+    o.run(dt, 1)
+    st = o.state()
+    # to the solver's reach: APGD's step test stops short of the exact optimum
+    np.testing.assert_allclose(st["v"][1], [v0[0], v0[1], -gap / dt], rtol=0, atol=1e-6)
+    np.testing.assert_allclose(st["v"][1, :2], v0[:2], rtol=0, atol=1e-12)  # the normal is +z to rounding
+    np.testing.assert_array_equal(st["omega"][1], [0.7, 0.7, 0.7])
+    assert abs(st["pos"][1, 2] - r) < 1e-8  # resting on the plane after the step
