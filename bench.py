@@ -13,11 +13,18 @@ integration).  W untimed steps, then K timed steps from the state they left.
 
 Prints one JSON line (rank 0) with the metric, a `roofline` object for the
 dominant solver kernel and a `cpu_baseline` object (the CPU restatement of the
-reference, timed on a bounded sample of the same workload).  The K timed steps run
-uninstrumented; right after them, --kernel-steps more steps run with HIP events on
-the library's stream around every solver kernel (events serialise the host loop,
-so that pass only yields the per-kernel average launch times): roofline.achieved
-= algorithmic bytes per launch / that average.
+reference, timed on a bounded sample of the same workload, at 1 thread and at every
+available host thread).  The K timed steps run uninstrumented; right after them,
+--kernel-steps more steps run with HIP events on the library's stream around every
+solver kernel (events serialise the host loop, so that pass only yields the
+per-kernel average launch times): roofline.achieved = algorithmic bytes per launch
+/ that average; roofline.traffic is the committed counter summary's figure for the
+default FP64 workload (named in roofline.traffic_source), not a counter read in
+this run.  The line also carries `other_precision` (the same workload with FP32
+APGD and FP64 refinement, W + K steps from the initial state, and the ratio of the
+two step rates) and `served` (a short served run: Poisson arrivals of mixed sphere /
+superellipsoid scenes, p50/p99 step latency, against the same requests stepped one
+at a time), the other two parts of the headline metric.
 """
 from __future__ import annotations
 
@@ -60,20 +67,44 @@ def kernel_bytes(nc, nc_obs, na, s, layout="rotation_free"):
     return dict(bt=bt, trial=trial)
 
 
-def cpu_baseline(sc, prm_kw, per_step_counts, budget_s=20.0):
-    """The CPU restatement (oracle/, single thread) on a bounded sample of the workload.
+def cpu_threads():
+    """Host threads this process may use: its CPU affinity, capped by OMP_NUM_THREADS when set."""
+    n = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
+    env = os.environ.get("OMP_NUM_THREADS", "")
+    return max(1, min(n, int(env))) if env.isdigit() else n
 
-    Sample: one full contact search on the 131k-sphere state plus as many APGD
-    iterations (reference semantics: 3 Q products per iteration) as fit in the
-    budget, on the real first-step problem.  Scaled to steps/s with the per-step
-    iteration and line-search counts measured on the GPU (the reference costs
-    3 Q products per iteration and 2 per rejected line-search trial).
+
+def cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return None
+
+
+def cpu_baseline(sc, prm_kw, per_step_counts, budget_s=20.0):
+    """The CPU restatement (oracle/, built -O3 -march=x86-64-v3 -fopenmp) on a bounded sample of the workload.
+
+    The reference runs its contact search, vap, update and move under OpenMP when built with it
+    (contact_kdtree.hpp:220, vap_fpd.hpp:74, solver.hpp:446, 535) and its APGD serially.  Sample: one
+    full contact search of the 131k-sphere state at 1 thread and at every available host thread,
+    plus as many APGD iterations (reference semantics: 3 Q products per iteration) as fit in the
+    budget on the real first-step problem (serial in both builds).  Scaled to steps/s with the
+    per-step iteration and line-search counts measured on the GPU (the reference costs 3 Q
+    products per iteration and 2 per rejected line-search trial).
     """
     import oracle
-    t0 = time.perf_counter()
-    o = oracle.Oracle(sc, **prm_kw)
-    nc = o.compute_contacts()
-    t_contacts = time.perf_counter() - t0
+    nthreads = cpu_threads()
+    t_contacts = {}
+    for t in sorted({1, nthreads}):
+        oracle.set_threads(t)
+        t0 = time.perf_counter()
+        o = oracle.Oracle(sc, **prm_kw)
+        nc = o.compute_contacts()
+        t_contacts[t] = time.perf_counter() - t0
+    oracle.set_threads(1)
     o.set_a_priori_velocity(1e-3)
     # time APGD iterations on the real problem with a capped iteration count
     k = 2
@@ -89,23 +120,30 @@ def cpu_baseline(sc, prm_kw, per_step_counts, budget_s=20.0):
     t_q = dt_solve / max(1, st["q_applies"])
     it, bt = per_step_counts
     q_per_step = 3 * it + 2 * bt
-    t_step = t_contacts + q_per_step * t_q
-    return dict(value=1.0 / t_step, unit="time-steps/s", cores=1, kind="port",
-                sample=(f"CPU restatement, 1 thread: full contact search on the 131k-sphere state "
-                        f"({nc} contacts, {t_contacts:.2f} s) + {st['iterations']} APGD iterations "
-                        f"({st['q_applies']} Q products, {dt_solve:.2f} s); scaled to one step with the "
-                        f"GPU-measured {it:.1f} iterations + {bt:.1f} line-search rejections per step "
-                        f"({q_per_step:.0f} Q products at {t_q * 1e3:.1f} ms)"),
-                seconds_per_step=t_step, q_product_ms=t_q * 1e3, contact_search_s=t_contacts)
+    step = {t: t_contacts[t] + q_per_step * t_q for t in t_contacts}
+
+    def sample(t):
+        return (f"CPU restatement of the reference, {t} thread(s) (OpenMP at the reference's pragma sites; APGD "
+                f"serial): full contact search on the 131k-sphere state ({nc} contacts, {t_contacts[t]:.2f} s) + "
+                f"{st['iterations']} APGD iterations ({st['q_applies']} Q products, {dt_solve:.2f} s); scaled to one "
+                f"step with the GPU-measured {it:.1f} iterations + {bt:.1f} line-search rejections per step "
+                f"({q_per_step:.0f} Q products at {t_q * 1e3:.1f} ms)")
+    return dict(value=1.0 / step[nthreads], unit="time-steps/s", cores=nthreads, kind="port", sample=sample(nthreads),
+                cpu_model=cpu_model(), build="g++ -O3 -march=x86-64-v3 -ffp-contract=off -fopenmp",
+                single_thread={"value": 1.0 / step[1], "cores": 1, "sample": sample(1)},
+                seconds_per_step=step[nthreads], q_product_ms=t_q * 1e3, contact_search_s=t_contacts)
 
 
-def measured_traffic(kernel, precision):
-    """HBM bytes per launch of `kernel` from the committed counter summary of this workload
-    (profiles/apgd_spheres131k_fp64_hbm_traffic.md: two rocprofv3 --pmc runs, FETCH_SIZE and
-    WRITE_SIZE, calibrated on a plain copy), or None."""
+TRAFFIC_PROFILE = "profiles/apgd_spheres131k_fp64_hbm_traffic.md"
+
+
+def measured_traffic(kernel, precision, default_scene):
+    """HBM bytes per launch of `kernel` from the committed counter summary of the default 131k-sphere
+    FP64 workload (two rocprofv3 --pmc runs, FETCH_SIZE and WRITE_SIZE, calibrated on a plain copy),
+    or None for another workload or precision (counters are not collected inside this run)."""
     import re
-    path = os.path.join(ROOT, "profiles", "apgd_spheres131k_fp64_hbm_traffic.md")
-    if precision != "fp64" or not os.path.exists(path):
+    path = os.path.join(ROOT, TRAFFIC_PROFILE)
+    if precision != "fp64" or not default_scene or not os.path.exists(path):
         return None
     for line in open(path):
         m = re.match(r"\|\s*`" + re.escape(kernel) + r"<[^`]*`\s*\|(?:[^|]*\|){5}\s*([0-9.]+)\s*\|", line)
@@ -235,28 +273,27 @@ def bench_batched(args):
     print(json.dumps(result), flush=True)
 
 
-def bench_served(args):
-    """Served scenes: Poisson arrivals (rate --rate per s) of --requests scene requests, 80% sphere
-    scenes (the 131k-sphere recipe scaled, NoFriction) and 20% superellipsoid scenes (the 32k recipe
-    scaled, Friction mu 0.3), log-uniform 512..65,536 particles, --request-steps steps each, admitted
-    by estimated HBM footprint into one resident batch per kind (scopi_amd.serve).  The one-at-a-time
-    line steps every request's scene alone (its own context) on a sample of steps after one warm-up
-    step and scales each to its step count."""
+def served_leg(requests, rate, request_steps, sample_steps, hbm_budget_gib, one_at_a_time):
+    """Served scenes: Poisson arrivals (`rate` per s) of `requests` scene requests, 80% sphere scenes (the
+    131k-sphere recipe scaled, NoFriction) and 20% superellipsoid scenes (the 32k recipe scaled, Friction
+    mu 0.3), log-uniform 512..65,536 particles, `request_steps` steps each, admitted by estimated HBM
+    footprint into one resident batch per kind (scopi_amd.serve).  The one-at-a-time figure steps every
+    request's scene alone (its own context) for `sample_steps` timed steps after one warm-up step and
+    scales each to its step count."""
     import torch
     from scopi_amd import hip, serve
     kinds, footprint, prm = serve.hip_kinds()
-    reqs = serve.poisson_requests(args.requests, args.rate, [("spheres", 0.8), ("superellipsoids", 0.2)],
-                                  (512, 65536), args.request_steps, seed=2025)
-    torch.cuda.init()
+    reqs = serve.poisson_requests(requests, rate, [("spheres", 0.8), ("superellipsoids", 0.2)], (512, 65536),
+                                  request_steps, seed=2025)
     log(f"[bench] serving {len(reqs)} requests ({sum(r.kind == 'superellipsoids' for r in reqs)} superellipsoid "
-        f"scenes), {args.request_steps} steps each, rate {args.rate}/s")
+        f"scenes), {request_steps} steps each, rate {rate}/s")
     done = lambda r: log(f"[bench] request {r.rid} ({r.kind}, {r.n} particles) done at {r.finished:.2f} s")
-    srv = serve.Server(kinds, serve.served_scene, footprint, hbm_budget=int(args.hbm_budget_gib * (1 << 30)),
+    srv = serve.Server(kinds, serve.served_scene, footprint, hbm_budget=int(hbm_budget_gib * (1 << 30)),
                        on_finish=done)
     st = srv.run(reqs)
     log(f"[bench] served: {json.dumps(st)}")
     one = None
-    if args.one_at_a_time:
+    if one_at_a_time:
         log("[bench] one-at-a-time sample")
         t_total = 0.0
         for r in reqs:
@@ -264,27 +301,40 @@ def bench_served(args):
             g.run(prm[r.kind], 1e-3, 1)
             torch.cuda.synchronize()
             t1 = time.perf_counter()
-            k = min(args.sample_steps, r.steps)
+            k = min(sample_steps, r.steps)
             g.run(prm[r.kind], 1e-3, k)
             torch.cuda.synchronize()
             t_total += (time.perf_counter() - t1) / k * r.steps
             g.close()
         one = {"scene_steps_per_s": st["scene_steps"] / t_total, "seconds_for_all_requests": t_total,
-               "sample": f"each request alone: {args.sample_steps} timed steps after 1 warm-up step, scaled to its "
-                         f"{args.request_steps} steps"}
+               "sample": f"each request alone: {sample_steps} timed steps after 1 warm-up step, scaled to its "
+                         f"{request_steps} steps"}
+    return {
+        "scene_steps_per_s": st["scene_steps_per_s"], "scene_steps": st["scene_steps"],
+        "step_latency_ms": st["step_latency_ms"], "wait_ms": st["wait_ms"], "completion_s": st["completion_s"],
+        "peak_resident_scenes": st["peak_resident_scenes"],
+        "workload": f"{len(reqs)} requests, Poisson rate {rate}/s, {request_steps} steps each, sizes log-uniform "
+                    f"512..65,536, spheres NoFriction / superellipsoids Friction mu 0.3, HBM budget {hbm_budget_gib} GiB",
+        "one_at_a_time": one,
+        "speedup_vs_one_at_a_time": (st["scene_steps_per_s"] / one["scene_steps_per_s"]) if one else None,
+    }
+
+
+def bench_served(args):
+    """The served form alone as the JSON line (see served_leg)."""
+    import torch
+    torch.cuda.init()
+    sv = served_leg(args.requests, args.rate, args.request_steps, args.sample_steps, args.hbm_budget_gib,
+                    args.one_at_a_time)
     result = {
         "metric": "served scene time-steps/s (Poisson arrivals, 80% sphere / 20% superellipsoid scenes, "
                   "512..65,536 particles) with p50/p99 step latency",
-        "value": st["scene_steps_per_s"], "unit": "scene-steps/s", "n_gpus": 1, "steps": st["scene_steps"],
-        "warmup": 0, "ms_per_step": st["step_latency_ms"]["p50"], "higher_is_better": True, "scaling": "weak",
-        "vs_baseline": None, "dtype": "f64", "data": "synthetic",
-        "config": {"workload": f"{len(reqs)} requests, Poisson rate {args.rate}/s, {args.request_steps} steps each, "
-                               "sizes log-uniform 512..65,536, spheres NoFriction / superellipsoids Friction mu 0.3, "
-                               f"HBM budget {args.hbm_budget_gib} GiB"},
-        "step_latency_ms": st["step_latency_ms"], "wait_ms": st["wait_ms"], "completion_s": st["completion_s"],
-        "peak_resident_scenes": st["peak_resident_scenes"],
-        "one_at_a_time": one,
-        "speedup_vs_one_at_a_time": (st["scene_steps_per_s"] / one["scene_steps_per_s"]) if one else None,
+        "value": sv["scene_steps_per_s"], "unit": "scene-steps/s", "n_gpus": 1, "steps": sv["scene_steps"],
+        "warmup": 0, "ms_per_step": sv["step_latency_ms"]["p50"], "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f64", "data": "synthetic", "config": {"workload": sv["workload"]},
+        "step_latency_ms": sv["step_latency_ms"], "wait_ms": sv["wait_ms"], "completion_s": sv["completion_s"],
+        "peak_resident_scenes": sv["peak_resident_scenes"], "one_at_a_time": sv["one_at_a_time"],
+        "speedup_vs_one_at_a_time": sv["speedup_vs_one_at_a_time"],
     }
     print(json.dumps(result), flush=True)
 
@@ -296,6 +346,11 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--precision", choices=["fp64", "fp32_refine"], default="fp64")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-fp32-leg", action="store_true",
+                    help="spheres: skip the FP32 + FP64-refinement pass at the same size (the ratio to FP64)")
+    ap.add_argument("--no-served-leg", action="store_true", help="spheres: skip the short served leg")
+    ap.add_argument("--served-requests", type=int, default=12, help="spheres: requests of the served leg")
+    ap.add_argument("--served-request-steps", type=int, default=40, help="spheres: steps per served request")
     ap.add_argument("--nx", type=int, default=64)
     ap.add_argument("--ny", type=int, default=64)
     ap.add_argument("--nz", type=int, default=32)
@@ -333,6 +388,7 @@ def main():
         return bench_served(args)
     log("[bench] building the 131k-sphere scene")
     sc = scenes.spheres_3d(args.nx, args.ny, args.nz)
+    default_scene = (args.nx, args.ny, args.nz) == (64, 64, 32)
     precision = hip.FP64 if args.precision == "fp64" else hip.FP32_REFINE
     prm = hip.params(vap=hip.VAP_FPD, dmax=0.02, kd_tree_radius=0.042 ** 2, tolerance=1e-6, precision=precision)
     dt = 1e-3
@@ -399,11 +455,42 @@ def main():
         roofline = {"bound": "hbm", "kernel": {"bt": "k_bt", "trial": "k_trial"}[dom],
                     "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                     "frac": (achieved / HBM_PEAK_GBS) if achieved else None,
-                    "traffic": measured_traffic({"bt": "k_bt", "trial": "k_trial"}[dom],
-                                                args.precision),
+                    "traffic": measured_traffic({"bt": "k_bt", "trial": "k_trial"}[dom], args.precision,
+                                                default_scene),
+                    "traffic_source": (TRAFFIC_PROFILE + " (rocprofv3 --pmc FETCH_SIZE, WRITE_SIZE; not measured in "
+                                       "this run)") if measured_traffic({"bt": "k_bt", "trial": "k_trial"}[dom],
+                                                                        args.precision, default_scene) else None,
                     "algorithmic_bytes_per_launch": kb[dom], "avg_launch_us": per[dom]["avg_us"]}
     log(f"[bench] {steps_per_s:.3f} steps/s, nc={nc}, iterations/step={it_per_step:.1f}, "
         f"backtracks/step={bt_per_step:.1f}, kernels={per}")
+    # the other precision on the same workload: FP32 APGD with FP64 refinement against the FP64 step
+    other = None
+    if not args.no_fp32_leg:
+        oprec = hip.FP32_REFINE if precision == hip.FP64 else hip.FP64
+        oname = "fp32_refine" if oprec == hip.FP32_REFINE else "fp64"
+        log(f"[bench] {oname} pass: warmup {args.warmup}, timing {args.steps} steps")
+        oprm = hip.params(vap=hip.VAP_FPD, dmax=0.02, kd_tree_radius=0.042 ** 2, tolerance=1e-6, precision=oprec)
+        g2 = hip.HipScene(sc, capacity_contacts=20 * sc.n)
+        if args.warmup:
+            g2.run(oprm, dt, args.warmup)
+        torch.cuda.synchronize()
+        t2 = time.perf_counter()
+        info2 = g2.run(oprm, dt, args.steps)
+        torch.cuda.synchronize()
+        el2 = time.perf_counter() - t2
+        g2.close()
+        sps2 = args.steps / el2
+        fp32_rate, fp64_rate = (sps2, steps_per_s) if oprec == hip.FP32_REFINE else (steps_per_s, sps2)
+        other = {"precision": oname, "steps_per_s": sps2, "ms_per_step": 1e3 * el2 / args.steps,
+                 "apgd_iterations_per_step": info2["iterations"] / args.steps,
+                 "fp32_iterations_per_step": info2["iterations_fp32"] / args.steps,
+                 "residual_last_solve": info2["residual"], "tolerance": 1e-6,
+                 "fp32_refine_over_fp64": fp32_rate / fp64_rate}
+        log(f"[bench] {oname}: {sps2:.3f} steps/s, FP32+FP64 / FP64 = {fp32_rate / fp64_rate:.3f}")
+    served = None
+    if not args.no_served_leg:
+        served = served_leg(args.served_requests, args.rate, args.served_request_steps, args.sample_steps,
+                            args.hbm_budget_gib, True)
     result = {
         "metric": "time-steps/s at 131k spheres, APGD tol 1e-6",
         "value": steps_per_s,
@@ -427,12 +514,15 @@ def main():
         "kernels": per,
         "instrumented_steps_per_s": kt_steps_per_s,
         "roofline": roofline,
+        "other_precision": other,
+        "served": served,
     }
     if not args.no_cpu:
         log("[bench] CPU baseline (bounded sample)")
         prm_kw = dict(vap=1, dmax=0.02, kd_tree_radius=0.042 ** 2, tolerance=1e-6)
         cb = cpu_baseline(sc, prm_kw, (it_per_step, bt_per_step))
-        result["cpu_baseline"] = {k: cb[k] for k in ("value", "unit", "cores", "kind", "sample")}
+        result["cpu_baseline"] = {k: cb[k] for k in ("value", "unit", "cores", "kind", "sample", "cpu_model", "build",
+                                                     "single_thread")}
     print(json.dumps(result), flush=True)
     g.close()
 

@@ -87,6 +87,7 @@ def lib():
         L.oracle_se_point_normal.argtypes = [D, D, D, D, C.c_double, C.c_double, D, D]
         L.oracle_set_se_math.argtypes = [C.c_int32]
         L.oracle_set_minpack_variant.argtypes = [C.c_int32]
+        L.oracle_set_threads.argtypes = [C.c_int32]
         L.oracle_set_jacobian_ad.argtypes = [C.c_int32]
         L.oracle_dogleg_fresh_wa4_count.restype = C.c_int64
         L.oracle_gen_segment_demo.argtypes = [C.c_int32, C.c_double, D, D, D]
@@ -292,6 +293,11 @@ def lu_solve4(A, b):
 def set_se_math(portable: bool):
     """Select the elementary functions of the superellipsoid solve (C library or portable)."""
     lib().oracle_set_se_math(1 if portable else 0)
+
+
+def set_threads(n: int) -> int:
+    """Threads of the reference's OpenMP loops in the restatement (search, vap, update and move)."""
+    return int(lib().oracle_set_threads(int(n)))
 
 
 MINPACK_ANL_ENORM, MINPACK_ANL_GIANT, MINPACK_ANL_DOGLEG, MINPACK_ANL_TERMINATION = 1, 2, 4, 8

@@ -34,6 +34,10 @@
 #include <random>
 #include <vector>
 
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
 namespace {
 
 using vec3 = std::array<double, 3>;
@@ -483,7 +487,22 @@ std::vector<Contact> contacts_kdtree(const Scene& s) {
         std::sort(key.begin(), key.end());
         std::vector<int64_t> keys(na);
         for (int64_t p = 0; p < na; ++p) keys[p] = key[p].first;
-        std::vector<int64_t> cand;
+        // the reference's parallel loop over the queries (contact_kdtree.hpp:220): static blocks of i,
+        // each thread's contacts kept apart and joined in thread order, i.e. in i order
+        int nthreads = 1;
+#ifdef _OPENMP
+        nthreads = omp_get_max_threads();
+#endif
+        std::vector<std::vector<Contact>> part((size_t)nthreads);
+#pragma omp parallel
+        {
+            int tid = 0;
+#ifdef _OPENMP
+            tid = omp_get_thread_num();
+#endif
+            std::vector<Contact>& mine = part[(size_t)tid];
+            std::vector<int64_t> cand;
+#pragma omp for schedule(static)
         for (int64_t i = a0; i < s.n - 1; ++i) {
             std::array<int64_t, 3> c{0, 0, 0};
             for (int d = 0; d < s.dim; ++d) c[d] = cell_of(i, d);
@@ -502,8 +521,10 @@ std::vector<Contact> contacts_kdtree(const Scene& s) {
                             if (i < j && kd_dist2(s, i, j) < s.prm.kd_tree_radius) cand.push_back(j);
                         }
                     }
-            for (int64_t j : cand) exact_distance(s, i, j, out);
+            for (int64_t j : cand) exact_distance(s, i, j, mine);
         }
+        }
+        for (auto& pt : part) out.insert(out.end(), pt.begin(), pt.end());
     }
     // obstacles: contact_kdtree.hpp:249-256
     for (int64_t i = 0; i < s.n_inactive; ++i)
@@ -1104,6 +1125,7 @@ void optim_solve(Scene& s, double dt, const std::vector<double>* sij) {
 
 // vap_fixed (vap/vap_fixed.hpp:46-55) / vap_fpd (vap/vap_fpd.hpp:70-82, src/vap/vap_fpd.cpp:6-26)
 void set_a_priori_velocity(Scene& s, double dt) {
+#pragma omp parallel for schedule(static)
     for (int64_t i = s.n_inactive; i < s.n; ++i) {
         if (s.prm.vap == ORACLE_VAP_PROJECTION) {
             // vap_projection::set_a_priori_velocity_impl (vap/vap_projection.hpp:34-49): the desired
@@ -1167,6 +1189,7 @@ void displacement_obstacles(Scene& s, double dt) {
 // update_velocity (solver.hpp:527-546) + move_active_particles (solver.hpp:440-480)
 void update_and_move(Scene& s, double dt) {
     const int64_t off = s.n_inactive;
+#pragma omp parallel for schedule(static)
     for (int64_t i = 0; i < s.na(); ++i) {
         for (int d = 0; d < s.dim; ++d) s.v[i + off][d] = s.u[3 * i + d];
         if (s.dim == 2)
@@ -1174,6 +1197,7 @@ void update_and_move(Scene& s, double dt) {
         else
             for (int d = 0; d < 3; ++d) s.omega[i + off][d] = s.w[3 * i + d];
     }
+#pragma omp parallel for schedule(static)
     for (int64_t i = off; i < s.n; ++i) {
         rotate(s.dim, s.q[i], s.omega[i], dt);
         for (int d = 0; d < s.dim; ++d) s.pos[i][d] += dt * s.v[i][d];
@@ -1652,6 +1676,21 @@ int oracle_se_residual(const double* pos_i, const double* q_i, const double* siz
 int oracle_set_se_math(int32_t portable) {
     g_se_math = portable ? 1 : 0;
     return 0;
+}
+
+#ifdef _OPENMP
+// one thread until oracle_set_threads asks for more (the reference's default build has OpenMP off)
+[[maybe_unused]] static const int g_one_thread = (omp_set_num_threads(1), 1);
+#endif
+
+int oracle_set_threads(int32_t n) {
+#ifdef _OPENMP
+    omp_set_num_threads(n > 0 ? n : 1);
+    return omp_get_max_threads();
+#else
+    (void)n;
+    return 1;
+#endif
 }
 
 int oracle_set_minpack_variant(int32_t mask) {

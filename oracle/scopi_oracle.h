@@ -164,6 +164,10 @@ int oracle_se_residual(const double* pos_i, const double* q_i, const double* siz
 /* Elementary functions of the superellipsoid solve: 0 = C library (default),
  * 1 = the portable functions of se_math.hpp, which the device evaluates identically. */
 int oracle_set_se_math(int32_t portable);
+/* Threads of the loops the reference runs under OpenMP (contact search over the active particles,
+ * vap, velocity update and move: contact_kdtree.hpp:220, vap_fpd.hpp:74, solver.hpp:446, 535); the
+ * projected-gradient solve stays serial as in the reference.  Default 1.  Returns the count set. */
+int oracle_set_threads(int32_t n);
 /* Test-only forms of the superellipsoid solve.  MINPACK: 0 = the reference's src/minpack.cpp
  * (default); each set bit takes one ANL-80-74 form instead: 1 enorm, 2 giant, 4 dogleg's
  * scaled-gradient loop, 8 termination order and the delta start (15: ANL-80-74).  Jacobian of the

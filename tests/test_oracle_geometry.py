@@ -204,3 +204,27 @@ def test_a_is_material_point_velocity(oracle_lib):
         fd = ((point(i, ct["pi"][c], h) - point(j, ct["pj"][c], h))
               - (point(i, ct["pi"][c], -h) - point(j, ct["pj"][c], -h))) / (2 * h)
         np.testing.assert_allclose(au[c], fd, rtol=1e-6, atol=1e-6)
+
+
+def test_threads_do_not_change_results(oracle_lib):
+    """The restatement's OpenMP loops (the reference's pragma sites: search, vap, update and move)
+    give the same contact list and the same step with 1 and 4 threads."""
+    from scopi_amd import scenes
+    sc = scenes.spheres_3d(16, 16, 8)
+    kw = dict(vap=1, dmax=0.02, kd_tree_radius=0.042 ** 2, tolerance=1e-6)
+    out = []
+    for t in (1, 4):
+        oracle_lib.set_threads(t)
+        try:
+            o = oracle_lib.Oracle(sc, **kw)
+            o.compute_contacts()
+            c = o.contacts()
+            o.run(1e-3, 2)
+            out.append((c, o.state()))
+        finally:
+            oracle_lib.set_threads(1)
+    (c1, s1), (c4, s4) = out
+    for k in c1:
+        assert np.array_equal(c1[k], c4[k])
+    for k in s1:
+        assert np.array_equal(s1[k], s4[k])
