@@ -4,7 +4,7 @@
  * The reference (hpc-maths/scopi v0.1.0) has no FFI: its per-step policies are
  * C++ template slots of ScopiSolver<dim, problem, optim, contact, vap>
  * (include/scopi/solver.hpp:91-107).  These entry points are what the
- * project's C++ slot policies (scopi_amd/host/scopi_hip_policies.hpp) bind; each
+ * project's C++ slot policies (scopi_amd/host/scopi/scopi_hip_host.hpp) bind; each
  * one names the reference member it replaces.  Plain pointers and sizes only:
  * every entry returns an int status (SCOPI_HIP_OK or a negative error code)
  * and never throws; device memory belongs to the context, host buffers to the
@@ -215,6 +215,24 @@ int scopi_hip_se_closest_points(scopi_hip_ctx* ctx, int64_t n, const double* pos
                                 const double* size_i, const double* sq_i, const double* pos_j, const double* q_j,
                                 const double* size_j, const double* sq_j, double* dij, double* nij, double* pi,
                                 double* pj, double* u0, double* u, int32_t* info, int32_t* nfev);
+
+/* No counterpart in the reference.  An allocation-free variant of scopi_hip_se_closest_points for
+ * callers that cannot allocate device memory (code under hipGraph capture, tight serving loops,
+ * harnesses that forbid hipMalloc): the same kernels and results, bit for bit, on device buffers of
+ * the caller (same layouts as scopi_hip_se_closest_points, all device pointers; u0, u, info, nfev may
+ * be NULL), with every scratch array taken from one static device array of about 14.5 MB that each
+ * GPU the library runs on holds until the process ends.  n <= SCOPI_HIP_NOALLOC_MAX_PAIRS, otherwise
+ * SCOPI_HIP_ERR_CAPACITY.  Asynchronous: the work is enqueued on `stream` (a hipStream_t of the
+ * context's device; NULL: the context's stream) and the caller synchronizes before reading the
+ * outputs.  No hipMalloc, hipFree or other allocation happens in the call.  Every call on one GPU
+ * uses that GPU's scratch array: calls that overlap on one GPU (different streams or threads) race;
+ * nothing of a call's scratch contents is read by the next one. */
+#define SCOPI_HIP_NOALLOC_MAX_PAIRS 256
+int scopi_hip_se_closest_points_noalloc(scopi_hip_ctx* ctx, void* stream, int64_t n, const double* pos_i,
+                                        const double* q_i, const double* size_i, const double* sq_i,
+                                        const double* pos_j, const double* q_j, const double* size_j,
+                                        const double* sq_j, double* dij, double* nij, double* pi, double* pj,
+                                        double* u0, double* u, int32_t* info, int32_t* nfev);
 
 /* closest_points_dispatcher::dispatch (closest_points.hpp:2037-2043) for n independent
  * pairs of spheres, planes and superellipsoids of the context's dimension, in the given

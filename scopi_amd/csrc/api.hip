@@ -5,6 +5,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -555,6 +556,142 @@ void periodic_wrap(scopi_hip_ctx* ctx) {
 
 }  // namespace
 
+// ------------------------------------------------------------------ allocation-free closest points
+// scopi_hip_se_closest_points_noalloc: the same kernels as scopi_hip_se_closest_points on device
+// buffers of the caller, with every scratch array (the 2n bodies' particle records, their grids,
+// the candidates and the work counter) carved out of one file-scope device array.  Each GPU holds
+// its own copy of the array from module load until the process ends; host code reaches it only
+// through the address hipGetSymbolAddress returns for the current device.
+namespace {
+constexpr int64_t kNoAllocPairs = SCOPI_HIP_NOALLOC_MAX_PAIRS;
+constexpr int64_t kNoAllocBodies = 2 * kNoAllocPairs;
+constexpr int64_t kAlign = 32;  // doubles: 256 bytes
+constexpr int64_t up(int64_t x) { return (x + kAlign - 1) / kAlign * kAlign; }
+// sizes in doubles (int arrays rounded up to whole doubles)
+constexpr int64_t kNaParticles = up(3 * kNoAllocBodies) + up(4 * kNoAllocBodies) + up(3 * kNoAllocBodies) +
+                                 up(2 * kNoAllocBodies) + up(kNoAllocBodies / 2);
+constexpr int64_t kNaGrids = up(2 * kSeGridAngles * kNoAllocBodies) + up(kNoAllocBodies) +
+                             up((int64_t)6 * kSeGridPoints * kNoAllocBodies);
+constexpr int64_t kNaCandidates = 2 * up(kNoAllocPairs / 2) + 2 * up(4 * kNoAllocPairs) + 2 * up(kNoAllocPairs / 2) +
+                                  up(kNoAllocPairs) + 3 * up(3 * kNoAllocPairs) + up(1);
+constexpr int64_t kNoAllocScratch = kNaParticles + kNaGrids + kNaCandidates;  // about 14.5 MB
+
+static __device__ __align__(256) double scratch[kNoAllocScratch];
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+double* g_scratch[kMaxDevices] = {};
+
+// The device address of `scratch` on the current device (looked up once per device).
+double* scratch_address(int device) {
+    if (device < 0 || device >= kMaxDevices) return nullptr;
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch[device]) {
+        void* p = nullptr;
+        if (hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)) != hipSuccess || !p) return nullptr;
+        g_scratch[device] = static_cast<double*>(p);
+    }
+    return g_scratch[device];
+}
+
+// Particle records, grids and candidates for n pairs over the scratch array at `base`; false if the
+// layout would not fit the array.
+bool carve(double* base, int64_t n, Particles& T, SeGrids& G, SeCandidates& S) {
+    double* at = base;
+    auto take = [&](int64_t doubles) {
+        double* p = at;
+        at += up(doubles);
+        return p;
+    };
+    T = Particles{};
+    T.dim = 3;
+    T.n = 2 * n;
+    T.n_inactive = 0;
+    T.cap = kNoAllocBodies;
+    T.pos = take(3 * kNoAllocBodies);
+    T.q = take(4 * kNoAllocBodies);
+    T.size = take(3 * kNoAllocBodies);
+    T.sq = take(2 * kNoAllocBodies);
+    T.shape = reinterpret_cast<int32_t*>(take(kNoAllocBodies / 2));
+    G = SeGrids{};
+    G.cap = kNoAllocBodies;
+    G.ang = take(2 * kSeGridAngles * kNoAllocBodies);
+    G.count = reinterpret_cast<int32_t*>(take(kNoAllocBodies));
+    G.pts = take((int64_t)6 * kSeGridPoints * kNoAllocBodies);
+    S = SeCandidates{};
+    S.cap = kNoAllocPairs;
+    S.i = reinterpret_cast<int32_t*>(take(kNoAllocPairs / 2));
+    S.j = reinterpret_cast<int32_t*>(take(kNoAllocPairs / 2));
+    S.u0 = take(4 * kNoAllocPairs);
+    S.u = take(4 * kNoAllocPairs);
+    S.info = reinterpret_cast<int32_t*>(take(kNoAllocPairs / 2));
+    S.nfev = reinterpret_cast<int32_t*>(take(kNoAllocPairs / 2));
+    S.dij = take(kNoAllocPairs);
+    S.nij = take(3 * kNoAllocPairs);
+    S.pi = take(3 * kNoAllocPairs);
+    S.pj = take(3 * kNoAllocPairs);
+    S.counter = reinterpret_cast<int*>(take(1));
+    return at - base <= kNoAllocScratch;
+}
+
+// the caller's row-major pair arrays -> body records 0..n-1 (side i) and n..2n-1 (side j)
+__global__ void __launch_bounds__(kBlock) k_noalloc_stage(Particles T, SeCandidates S, int64_t n,
+                                                          const double* __restrict__ pos_i,
+                                                          const double* __restrict__ q_i,
+                                                          const double* __restrict__ size_i,
+                                                          const double* __restrict__ sq_i,
+                                                          const double* __restrict__ pos_j,
+                                                          const double* __restrict__ q_j,
+                                                          const double* __restrict__ size_j,
+                                                          const double* __restrict__ sq_j) {
+    const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    const int64_t cap = T.cap;
+    for (int side = 0; side < 2; ++side) {
+        const int64_t b = side ? n + k : k;
+        const double* pos = side ? pos_j : pos_i;
+        const double* q = side ? q_j : q_i;
+        const double* size = side ? size_j : size_i;
+        const double* sq = side ? sq_j : sq_i;
+        for (int d = 0; d < 3; ++d) T.pos[d * cap + b] = pos[3 * k + d];
+        for (int d = 0; d < 4; ++d) T.q[d * cap + b] = q[4 * k + d];
+        for (int d = 0; d < 3; ++d) T.size[d * cap + b] = size[3 * k + d];
+        for (int d = 0; d < 2; ++d) T.sq[d * cap + b] = sq[2 * k + d];
+        T.shape[b] = SCOPI_SHAPE_SUPERELLIPSOID;
+    }
+    S.i[k] = (int32_t)k;
+    S.j[k] = (int32_t)(n + k);
+}
+
+// the candidates' records -> the caller's row-major outputs
+__global__ void __launch_bounds__(kBlock) k_noalloc_scatter(SeCandidates S, int64_t n, double* __restrict__ dij,
+                                                            double* __restrict__ nij, double* __restrict__ pi,
+                                                            double* __restrict__ pj, double* __restrict__ u0,
+                                                            double* __restrict__ u, int32_t* __restrict__ info,
+                                                            int32_t* __restrict__ nfev) {
+    const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    const int64_t cap = S.cap;
+    dij[k] = S.dij[k];
+    for (int d = 0; d < 3; ++d) {
+        nij[3 * k + d] = S.nij[d * cap + k];
+        pi[3 * k + d] = S.pi[d * cap + k];
+        pj[3 * k + d] = S.pj[d * cap + k];
+    }
+    if (u0)
+        for (int d = 0; d < 4; ++d) u0[4 * k + d] = S.u0[4 * k + d];
+    if (u)
+        for (int d = 0; d < 4; ++d) u[4 * k + d] = S.u[4 * k + d];
+    if (info) info[k] = S.info[k];
+    if (nfev) nfev[k] = S.nfev[k];
+}
+
+int noalloc_fail(scopi_hip_ctx* ctx, int code, const char* msg) {
+    ctx->err.assign(msg);  // within the capacity reserved at create: no allocation
+    return code;
+}
+}  // namespace
+
 extern "C" {
 
 int scopi_hip_default_params(scopi_hip_params* p) {
@@ -589,6 +726,7 @@ int scopi_hip_create(int32_t dim, int64_t capacity_particles, int64_t capacity_c
     auto* ctx = new scopi_hip_ctx();
     ctx->dim = dim;
     ctx->P.dim = dim;
+    ctx->err.reserve(256);  // error messages up to this length never allocate afterwards
     try {
         SCOPI_HIP_CHECK(hipGetDevice(&ctx->device));
         SCOPI_HIP_CHECK(hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking));
@@ -788,6 +926,46 @@ int scopi_hip_closest_points(scopi_hip_ctx* ctx, int64_t n, const int32_t* shape
         return pairs_batch(ctx, n, shape_i, pos_i, q_i, size_i, sq_i, shape_j, pos_j, q_j, size_j, sq_j, dij, nij, pi,
                            pj, u0, u, info, nfev);
     });
+}
+
+// allocation-free variant of scopi_hip_se_closest_points (the scratch array and its helpers are
+// defined above the extern "C" block)
+int scopi_hip_se_closest_points_noalloc(scopi_hip_ctx* ctx, void* stream, int64_t n, const double* pos_i,
+                                        const double* q_i, const double* size_i, const double* sq_i,
+                                        const double* pos_j, const double* q_j, const double* size_j,
+                                        const double* sq_j, double* dij, double* nij, double* pi, double* pj,
+                                        double* u0, double* u, int32_t* info, int32_t* nfev) {
+    if (!ctx) return SCOPI_HIP_ERR_ARG;
+    if (ctx->dim != 3) return noalloc_fail(ctx, SCOPI_HIP_ERR_UNSUPPORTED, "se_closest_points_noalloc: 3D only");
+    if (n < 0) return noalloc_fail(ctx, SCOPI_HIP_ERR_ARG, "se_closest_points_noalloc: n < 0");
+    if (n > kNoAllocPairs)
+        return noalloc_fail(ctx, SCOPI_HIP_ERR_CAPACITY,
+                            "se_closest_points_noalloc: more than SCOPI_HIP_NOALLOC_MAX_PAIRS (256) pairs");
+    if (n == 0) return SCOPI_HIP_OK;
+    if (!pos_i || !q_i || !size_i || !sq_i || !pos_j || !q_j || !size_j || !sq_j || !dij || !nij || !pi || !pj)
+        return noalloc_fail(ctx, SCOPI_HIP_ERR_ARG, "se_closest_points_noalloc: NULL input or output");
+    if (hipSetDevice(ctx->device) != hipSuccess)
+        return noalloc_fail(ctx, SCOPI_HIP_ERR_HIP, "se_closest_points_noalloc: hipSetDevice failed");
+    double* base = scratch_address(ctx->device);
+    if (!base) return noalloc_fail(ctx, SCOPI_HIP_ERR_HIP, "se_closest_points_noalloc: no scratch address");
+    hipStream_t s = stream ? static_cast<hipStream_t>(stream) : ctx->stream;
+    Particles T;
+    SeGrids G;
+    SeCandidates S;
+    if (!carve(base, n, T, G, S))
+        return noalloc_fail(ctx, SCOPI_HIP_ERR_CAPACITY, "se_closest_points_noalloc: scratch layout too large");
+    const unsigned grid = (unsigned)((n + kBlock - 1) / kBlock);
+    hipLaunchKernelGGL(k_noalloc_stage, dim3(grid), dim3(kBlock), 0, s, T, S, n, pos_i, q_i, size_i, sq_i, pos_j,
+                       q_j, size_j, sq_j);
+    try {
+        se_closest_points(T, G, S, n, s);  // G and S already hold 2n bodies / n pairs: no allocation
+    } catch (const hip_error&) {
+        return noalloc_fail(ctx, SCOPI_HIP_ERR_HIP, "se_closest_points_noalloc: kernel launch failed");
+    }
+    hipLaunchKernelGGL(k_noalloc_scatter, dim3(grid), dim3(kBlock), 0, s, S, n, dij, nij, pi, pj, u0, u, info, nfev);
+    if (hipGetLastError() != hipSuccess)
+        return noalloc_fail(ctx, SCOPI_HIP_ERR_HIP, "se_closest_points_noalloc: kernel launch failed");
+    return SCOPI_HIP_OK;
 }
 
 int scopi_hip_newton_probe(scopi_hip_ctx* ctx, int32_t kind, const double* in, double* out) {

@@ -33,8 +33,9 @@ EXPORTS = [
     "scopi_hip_batch_remove_scene", "scopi_hip_batch_step", "scopi_hip_batch_size", "scopi_hip_batch_download_scene",
     "scopi_hip_batch_scene_info", "scopi_hip_batch_download_contacts", "scopi_hip_batch_download_solution",
     "scopi_hip_scene_footprint", "scopi_hip_set_box", "scopi_hip_set_projection_velocity",
-    "scopi_hip_set_contact_gamma", "scopi_hip_download_contact_gamma",
+    "scopi_hip_set_contact_gamma", "scopi_hip_download_contact_gamma", "scopi_hip_se_closest_points_noalloc",
 ]
+NOALLOC_MAX_PAIRS = 256  # SCOPI_HIP_NOALLOC_MAX_PAIRS
 TUNE_REDUCE_BLOCKS, TUNE_BT_GROUP, TUNE_GRAPH_ROUNDS = 1, 2, 3
 
 
@@ -101,6 +102,7 @@ def lib():
     L.scopi_hip_kernel_timing.argtypes = [P, D]
     L.scopi_hip_set_tuning.argtypes = [P, C.c_int32, C.c_int32]
     L.scopi_hip_se_closest_points.argtypes = [P, C.c_int64] + [D] * 12 + [C.c_void_p] * 4
+    L.scopi_hip_se_closest_points_noalloc.argtypes = [P, C.c_void_p, C.c_int64] + [C.c_void_p] * 16
     L.scopi_hip_portable_math.argtypes = [P, C.c_int32, C.c_int64, D, C.c_void_p, D]
     L.scopi_hip_newton_probe.argtypes = [P, C.c_int32, D, D]
     L.scopi_hip_closest_points.argtypes = ([P, C.c_int64, I32, D, D, D, C.c_void_p, I32, D, D, D, C.c_void_p] +
@@ -296,6 +298,16 @@ class HipScene:
                                                         _ptr(info), _ptr(nfev)), "se_closest_points")
         return dict(dij=dij[:n], nij=nij[:n], pi=pi[:n], pj=pj[:n], u0=u0[:n], u=u[:n], info=info[:n],
                     nfev=nfev[:n])
+
+    def se_closest_points_noalloc(self, n, ptrs, stream=None):
+        """scopi_hip_se_closest_points_noalloc on device buffers: `ptrs` holds the 16 device addresses
+        (pos_i, q_i, size_i, sq_i, pos_j, q_j, size_j, sq_j, dij, nij, pi, pj, u0, u, info, nfev; the last
+        four may be 0), `stream` a hipStream_t handle (None: the context's stream).  Asynchronous;
+        returns the status code."""
+        return int(self._L.scopi_hip_se_closest_points_noalloc(self._h, stream, n, *[p or None for p in ptrs]))
+
+    def last_error(self):
+        return self._L.scopi_hip_last_error(self._h).decode()
 
     def closest_points(self, shape_i, pos_i, q_i, size_i, sq_i, shape_j, pos_j, q_j, size_j, sq_j):
         """closest_points_dispatcher::dispatch of n independent pairs (the context's dimension) on the device."""
