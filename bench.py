@@ -22,7 +22,9 @@ per-kernel average launch times): roofline.achieved = algorithmic bytes per laun
 default FP64 workload (named in roofline.traffic_source), not a counter read in
 this run.  The line also carries `other_precision` (the same workload with FP32
 APGD and FP64 refinement, W + K steps from the initial state, and the ratio of the
-two step rates) and `served` (a short served run: Poisson arrivals of mixed sphere /
+two step rates), `warm_start` (the same workload with each solve started from the
+last step's multipliers instead of the reference's lambda = 0) and `served` (a
+short served run: Poisson arrivals of mixed sphere /
 superellipsoid scenes, p50/p99 step latency, against the same requests stepped one
 at a time), the other two parts of the headline metric.
 """
@@ -349,6 +351,7 @@ def main():
     ap.add_argument("--no-fp32-leg", action="store_true",
                     help="spheres: skip the FP32 + FP64-refinement pass at the same size (the ratio to FP64)")
     ap.add_argument("--no-served-leg", action="store_true", help="spheres: skip the short served leg")
+    ap.add_argument("--no-warm-leg", action="store_true", help="spheres: skip the warm-start pass")
     ap.add_argument("--served-requests", type=int, default=12, help="spheres: requests of the served leg")
     ap.add_argument("--served-request-steps", type=int, default=40, help="spheres: steps per served request")
     ap.add_argument("--nx", type=int, default=64)
@@ -487,6 +490,26 @@ def main():
                  "residual_last_solve": info2["residual"], "tolerance": 1e-6,
                  "fp32_refine_over_fp64": fp32_rate / fp64_rate}
         log(f"[bench] {oname}: {sps2:.3f} steps/s, FP32+FP64 / FP64 = {fp32_rate / fp64_rate:.3f}")
+    # warm start (lambda of the last step as the first iterate; not the reference's lambda = 0)
+    warm = None
+    if not args.no_warm_leg:
+        log(f"[bench] warm-start pass ({args.precision}): warmup {args.warmup}, timing {args.steps} steps")
+        wprm = hip.params(vap=hip.VAP_FPD, dmax=0.02, kd_tree_radius=0.042 ** 2, tolerance=1e-6, precision=precision,
+                          warm_start=1)
+        g3 = hip.HipScene(sc, capacity_contacts=20 * sc.n)
+        if args.warmup:
+            g3.run(wprm, dt, args.warmup)
+        torch.cuda.synchronize()
+        t3 = time.perf_counter()
+        info3 = g3.run(wprm, dt, args.steps)
+        torch.cuda.synchronize()
+        el3 = time.perf_counter() - t3
+        g3.close()
+        warm = {"precision": args.precision, "steps_per_s": args.steps / el3, "ms_per_step": 1e3 * el3 / args.steps,
+                "apgd_iterations_per_step": info3["iterations"] / args.steps,
+                "line_search_rejections_per_step": info3["backtracks"] / args.steps,
+                "over_cold_start": (args.steps / el3) / steps_per_s}
+        log(f"[bench] warm start: {args.steps / el3:.3f} steps/s, {info3['iterations'] / args.steps:.1f} iterations/step")
     served = None
     if not args.no_served_leg:
         served = served_leg(args.served_requests, args.rate, args.served_request_steps, args.sample_steps,
@@ -515,6 +538,7 @@ def main():
         "instrumented_steps_per_s": kt_steps_per_s,
         "roofline": roofline,
         "other_precision": other,
+        "warm_start": warm,
         "served": served,
     }
     if not args.no_cpu:

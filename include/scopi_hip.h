@@ -79,7 +79,12 @@ typedef struct scopi_hip_params {
     double mu;
     int64_t max_ite;
     double fp32_tolerance; /* FP32 phase stop, precision == FP32_REFINE (0: automatic) */
-    int32_t warm_start;       /* reserved, must be 0 (lambda_0 = 0 every step, apgd.hpp:152-159) */
+    int32_t warm_start;       /* 0: lambda_0 = 0 every solve, as the reference (apgd.hpp:152-159).
+                                 1 (no counterpart in the reference; NoFriction and Friction only,
+                                 SCOPI_HIP_ERR_UNSUPPORTED otherwise): each solve starts from the
+                                 last warm-started solve's multipliers of the contacts found again,
+                                 matched by (i, j), new contacts at 0.  The solve then ends on another
+                                 iterate of the same problem, within the tolerance of the reference's. */
     int32_t general_jacobian; /* 1: keep the rotational blocks of B even when every active particle is a
                                  sphere (they vanish for spheres; this only changes speed) */
     double fixed_point_tol;       /* FrictionFixedPoint / ViscousFriction: contact_property<FixedPoint>

@@ -25,7 +25,7 @@ class Params(C.Structure):
     _fields_ = [
         ("problem", C.c_int32), ("contact_method", C.c_int32), ("vap", C.c_int32), ("method", C.c_int32),
         ("dmax", C.c_double), ("kd_tree_radius", C.c_double), ("alpha", C.c_double), ("max_ite", C.c_int64),
-        ("tolerance", C.c_double), ("dynamic_descent", C.c_int32), ("pad", C.c_int32), ("mu", C.c_double),
+        ("tolerance", C.c_double), ("dynamic_descent", C.c_int32), ("warm_start", C.c_int32), ("mu", C.c_double),
         ("fixed_point_tol", C.c_double), ("fixed_point_max_iter", C.c_double),
         ("gamma", C.c_double), ("gamma_min", C.c_double), ("gamma_tol", C.c_double),
         ("objective_perturbation", C.c_double),

@@ -21,6 +21,8 @@
 //                                             obstacles_sphere_sphere_moving, 3spheres_nofriction,
 //                                             sphere_plane_friction_mu01}.json, inclined-plane
 //                                             expected errors (test_obstacles.cpp:256-313)
+//   warm start (not in the reference)      -> first step = cold step bitwise, one iteration per
+//                                             solve on a resting stack, off by default
 #include "scopi_oracle.h"
 #include "superellipsoid.hpp"
 
@@ -74,6 +76,7 @@ struct Scene {
     std::vector<Contact> contacts;
     std::vector<Contact> old_contacts;  // ScopiSolver::m_old_contacts: the last step's contacts and properties
     std::vector<double> lambda;
+    bool lambda_is_history = false;  // lambda belongs to old_contacts and may start the next solve
     // optim slot outputs (OptimGradient::m_u / m_omega), Na x 3
     std::vector<double> u, w;
     oracle_stats st{};
@@ -891,11 +894,16 @@ double norm_diff(const std::vector<double>& a, const std::vector<double>& b) {
     return std::sqrt(r);
 }
 
-// apgd::operator(): solvers/apgd.hpp:147-222 (theta is uniform, kept scalar)
-SolveResult apgd(const MinProblem& mp, const oracle_params& prm) {
+// apgd::operator(): solvers/apgd.hpp:147-222 (theta is uniform, kept scalar).  lam0 (not in the
+// reference, which starts from 0 at apgd.hpp:152-159) replaces the first iterate lambda_0 = y_0.
+SolveResult apgd(const MinProblem& mp, const oracle_params& prm, const std::vector<double>* lam0 = nullptr) {
     const int64_t n = mp.size();
     SolveResult res;
     std::vector<double> lam_n(n, 0.0), lam_np1(n, 0.0), y_n(n, 0.0), y_np1(n, 0.0), dG(n), tmp(n);
+    if (lam0) {
+        lam_n = *lam0;
+        y_n = *lam0;
+    }
     double theta_n = 1.0, theta_np1 = 1.0;
     double alpha = prm.alpha;
     double lipsch = 1. / alpha;
@@ -1084,13 +1092,35 @@ void transfer(const std::vector<Contact>& old, std::vector<Contact>& cs) {
     }
 }
 
+// Warm start (not in the reference): lambda_0 of the current contacts from the last solve's
+// multipliers, matched by (i, j) as ScopiSolver::transfer matches contacts (solver.hpp:241-259, the
+// last occurrence of a pair wins), 0 for a new contact.  Only NoFriction and Friction (their rows
+// do not change between steps), and only when the last solve was warm-started as well.
+bool warm_start_lambda(const Scene& s, std::vector<double>& lam0) {
+    if (!s.prm.warm_start || s.prm.method != ORACLE_METHOD_APGD || !s.lambda_is_history) return false;
+    if (s.prm.problem != ORACLE_NOFRICTION && s.prm.problem != ORACLE_FRICTION) return false;
+    const size_t nr = s.prm.problem == ORACLE_FRICTION ? 3 : 1;
+    if (s.old_contacts.empty() || s.lambda.size() != nr * s.old_contacts.size()) return false;
+    std::map<std::pair<int64_t, int64_t>, size_t> index;
+    for (size_t k = 0; k < s.old_contacts.size(); ++k) index[{s.old_contacts[k].i, s.old_contacts[k].j}] = k;
+    lam0.assign(nr * s.contacts.size(), 0.0);
+    for (size_t k = 0; k < s.contacts.size(); ++k) {
+        auto it = index.find({s.contacts[k].i, s.contacts[k].j});
+        if (it == index.end()) continue;
+        for (size_t d = 0; d < nr; ++d) lam0[nr * k + d] = s.lambda[nr * it->second + d];
+    }
+    return true;
+}
+
 void optim_solve(Scene& s, double dt, const std::vector<double>* sij) {
     const int64_t na = s.na(), off = s.n_inactive;
     s.u.assign(3 * na, 0.0);
     s.w.assign(3 * na, 0.0);
     if (!s.contacts.empty()) {
         MinProblem mp(s, s.contacts, dt, sij);
-        SolveResult r = (s.prm.method == ORACLE_METHOD_PGD) ? pgd(mp, s.prm) : apgd(mp, s.prm);
+        std::vector<double> lam0;
+        const bool warm = warm_start_lambda(s, lam0);
+        SolveResult r = (s.prm.method == ORACLE_METHOD_PGD) ? pgd(mp, s.prm) : apgd(mp, s.prm, warm ? &lam0 : nullptr);
         s.lambda = r.lambda;
         std::vector<double> vel(6 * na);
         mp.velocities(s.lambda.data(), vel.data());
@@ -1408,6 +1438,8 @@ int oracle_run(void* scene, double dt, int64_t total_it, int64_t initial_iter) {
         optim_run(s, dt);
         update_contact_properties(s, dt);
         s.old_contacts = s.contacts;
+        s.lambda_is_history =
+            s.prm.warm_start && (s.prm.problem == ORACLE_NOFRICTION || s.prm.problem == ORACLE_FRICTION);
         s.st.t_solve += seconds_since(t0);
         t0 = std::chrono::steady_clock::now();
         update_and_move(s, dt);
@@ -1467,6 +1499,7 @@ int oracle_solve(void* scene, double dt, double* lambda, double* u, double* w) {
     if (!scene) return -1;
     Scene& s = *static_cast<Scene*>(scene);
     optim_run(s, dt);
+    s.lambda_is_history = false;  // these multipliers belong to s.contacts, not to old_contacts
     if (lambda) std::copy(s.lambda.begin(), s.lambda.end(), lambda);
     if (u) std::copy(s.u.begin(), s.u.end(), u);
     if (w) std::copy(s.w.begin(), s.w.end(), w);

@@ -42,7 +42,9 @@ typedef struct oracle_params {
     int64_t max_ite;         /* apgd_params::max_ite (default 10000)           */
     double tolerance;        /* apgd_params::tolerance (default 1e-7)          */
     int32_t dynamic_descent; /* apgd_params::dynamic_descent (default true)    */
-    int32_t pad;
+    int32_t warm_start;      /* not in the reference (default 0): NoFriction and */
+                             /* Friction solves start from the last solve's      */
+                             /* multipliers of the contacts found again          */
     double mu;               /* contact_property<Friction>::mu (default 0.1)   */
     double fixed_point_tol;  /* contact_property<FixedPoint> (default 1e-6)    */
     double fixed_point_max_iter; /* (default 1000)                            */

@@ -1247,7 +1247,10 @@ void Solver::solve_phases(const Particles& P, const Contacts& C, int64_t nc, con
         const double rtol32 = prm.fp32_tolerance > 0 ? 0.0 : 1e-5;
         const double tol32 = prm.fp32_tolerance > 0 ? prm.fp32_tolerance : prm.tolerance;
         const int64_t before = info.iterations;
-        run_phase<float, LAY>(b32_, P, C, na, nc, prm, dt, tol32, rtol32, prm.max_ite, false, false, s, info);
+        if (lam0_)
+            hipLaunchKernelGGL((k_convert<double, float>), dim3(1024), dim3(kBlock), 0, s, lam0_, b32_.lam, NR * nc);
+        run_phase<float, LAY>(b32_, P, C, na, nc, prm, dt, tol32, rtol32, prm.max_ite, lam0_ != nullptr, false, s,
+                              info);
         const int64_t it32 = info.iterations - before;
         info.iterations_fp32 += it32;
         const int64_t stride32 = NR * b32_.cap;
@@ -1256,7 +1259,10 @@ void Solver::solve_phases(const Particles& P, const Contacts& C, int64_t nc, con
         const int64_t left = std::max<int64_t>(prm.max_ite - it32, 1);
         run_phase<double, LAY>(b64_, P, C, na, nc, prm, dt, prm.tolerance, 0.0, left, true, true, s, info);
     } else {
-        run_phase<double, LAY>(b64_, P, C, na, nc, prm, dt, prm.tolerance, 0.0, prm.max_ite, false, false, s, info);
+        if (lam0_)
+            SCOPI_HIP_CHECK(hipMemcpyAsync(b64_.lam, lam0_, NR * nc * sizeof(double), hipMemcpyDeviceToDevice, s));
+        run_phase<double, LAY>(b64_, P, C, na, nc, prm, dt, prm.tolerance, 0.0, prm.max_ite, lam0_ != nullptr, false,
+                               s, info);
     }
     // velocities from the final lambda (FP64)
     launch_bt<double, LAY>(bt_group_, s, (ApgdCtrl*)nullptr, final_slot_, b64_, na, tstart_, tc_, nullptr, 0, nullptr);
@@ -1480,7 +1486,7 @@ int64_t Solver::viscous_kinds(const double* gamma, int64_t nc, const scopi_hip_p
 }
 
 int Solver::solve(const Particles& P, const Contacts& C, int64_t nc, const scopi_hip_params& prm, double dt,
-                  bool rot_free, hipStream_t s, scopi_hip_solve_info* info, double* gamma) {
+                  bool rot_free, hipStream_t s, scopi_hip_solve_info* info, double* gamma, const double* lam0) {
     const int64_t na = P.n - P.n_inactive;
     const bool fixed_point = prm.problem == SCOPI_PROBLEM_FRICTION_FIXED_POINT;
     const bool friction = prm.problem == SCOPI_PROBLEM_FRICTION || fixed_point;
@@ -1491,7 +1497,13 @@ int Solver::solve(const Particles& P, const Contacts& C, int64_t nc, const scopi
     if (!(prm.alpha > 0)) return SCOPI_HIP_ERR_ARG;
     if (fixed_point && !(dt > 0)) return SCOPI_HIP_ERR_ARG;
     if (viscous && nc > 0 && !gamma) return SCOPI_HIP_ERR_ARG;
+    if (lam0 && (viscous || fixed_point)) return SCOPI_HIP_ERR_UNSUPPORTED;
     reserve(na, nc, fp32);
+    lam0_ = lam0;
+    struct Clear {
+        const double*& p;
+        ~Clear() { p = nullptr; }
+    } clear_lam0{lam0_};
     layout_ = viscous ? kVisc : friction ? kCone : ((rot_free && !prm.general_jacobian) ? kRotFree : kRigid);
     nr_ = viscous ? 4 : friction ? 3 : 1;
     last_problem_ = prm.problem;

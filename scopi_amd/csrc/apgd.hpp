@@ -116,8 +116,14 @@ class Solver {
     // rot_free: every active particle is a sphere (enables the kRotFree layout).
     // Viscous / ViscousFriction: `gamma` (device, nc) is each contact's adhesion potential; the
     // solve reads it for the multiplier layout and leaves the updated values in it.
+    // lam0 (device, NR per contact, FP64; may be null): the first iterate instead of lambda = 0 (NoFriction
+    // and Friction).
     int solve(const Particles& P, const Contacts& C, int64_t nc, const scopi_hip_params& prm, double dt,
-              bool rot_free, hipStream_t s, scopi_hip_solve_info* info, double* gamma = nullptr);
+              bool rot_free, hipStream_t s, scopi_hip_solve_info* info, double* gamma = nullptr,
+              const double* lam0 = nullptr);
+    // the last solve's multipliers (device, FP64, rows per contact rows_per_contact())
+    const double* final_lambda() const { return b64_.lam + (int64_t)final_slot_ * nr_ * b64_.cap; }
+    int rows_per_contact() const { return nr_; }
     // The same for a set of independent scenes packed in P / C (scene ranges in `S`);
     // one solve info per scene.  Afterwards every scene's final lambda is in slot 0.
     int solve_scenes(const Particles& P, const Contacts& C, int64_t nc, const scopi_hip_params& prm, double dt,
@@ -200,6 +206,7 @@ class Solver {
     int64_t rows_ = 0, nc_ = 0;
     int nr_ = 1;
     int final_slot_ = 0;
+    const double* lam0_ = nullptr;  // first iterate of the current solve (warm start), or null
     int ctrl_cur_ = 0;  // copy of ApgdCtrl with the last phase's final state
     int layout_ = kRotFree;
     int last_problem_ = SCOPI_PROBLEM_NOFRICTION;

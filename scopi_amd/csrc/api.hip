@@ -53,6 +53,14 @@ struct scopi_hip_ctx {
     uint64_t* hist_keys = nullptr;
     double* hist_gamma = nullptr;
     int64_t hist_n = 0, hist_cap = 0;
+    // warm start: the last solve's multipliers (hist_nr rows per history contact) and the current
+    // contacts' first iterate
+    double* hist_lam = nullptr;
+    int64_t hist_lam_cap = 0;
+    int hist_nr = 0;
+    int hist_kind = 0;  // what hist_keys index: 0 nothing, 1 hist_lam (hist_nr rows each), 2 hist_gamma
+    double* lam0 = nullptr;
+    int64_t lam0_cap = 0;
 };
 
 namespace {
@@ -407,6 +415,37 @@ __global__ void __launch_bounds__(kBlock) k_record_history(const int32_t* __rest
     hg[c] = gamma[c];
 }
 
+// Warm start: a contact whose (i, j) was a contact of the last solve starts from that contact's
+// multipliers (nr rows), any other from 0 -- transfer (solver.hpp:241-259) applied to lambda.
+__global__ void __launch_bounds__(kBlock) k_transfer_lambda(const int32_t* __restrict__ ci,
+                                                           const int32_t* __restrict__ cj, int64_t nc,
+                                                           const uint64_t* __restrict__ keys,
+                                                           const double* __restrict__ hl, int64_t hn, int nr,
+                                                           double* __restrict__ lam0) {
+    const int64_t c = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (c >= nc) return;
+    const uint64_t key = ((uint64_t)(uint32_t)ci[c] << 32) | (uint32_t)cj[c];
+    int64_t lo = 0, hi = hn;  // upper bound of key
+    while (lo < hi) {
+        const int64_t mid = (lo + hi) >> 1;
+        if (keys[mid] <= key)
+            lo = mid + 1;
+        else
+            hi = mid;
+    }
+    const bool found = lo > 0 && keys[lo - 1] == key;
+    for (int d = 0; d < nr; ++d) lam0[c * nr + d] = found ? hl[(lo - 1) * nr + d] : 0.0;
+}
+
+__global__ void __launch_bounds__(kBlock) k_record_lambda(const int32_t* __restrict__ ci, const int32_t* __restrict__ cj,
+                                                         int64_t nc, int nr, const double* __restrict__ lam,
+                                                         uint64_t* __restrict__ keys, double* __restrict__ hl) {
+    const int64_t c = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (c >= nc) return;
+    keys[c] = ((uint64_t)(uint32_t)ci[c] << 32) | (uint32_t)cj[c];
+    for (int d = 0; d < nr; ++d) hl[c * nr + d] = lam[c * nr + d];
+}
+
 void gamma_reserve(scopi_hip_ctx* ctx, int64_t nc) {
     if (nc <= ctx->gamma_cap) return;
     if (ctx->gamma) SCOPI_HIP_CHECK(hipFree(ctx->gamma));
@@ -420,7 +459,7 @@ void prepare_gamma(scopi_hip_ctx* ctx, const scopi_hip_params& prm, bool from_hi
     const int64_t nc = ctx->nc;
     gamma_reserve(ctx, std::max<int64_t>(nc, 1));
     if (nc == 0 || (ctx->gamma_valid && !from_history)) return;
-    if (from_history && ctx->hist_n > 0)
+    if (from_history && ctx->hist_n > 0 && ctx->hist_kind == 2)
         hipLaunchKernelGGL(k_transfer_gamma, dim3(grid_for(nc)), dim3(kBlock), 0, ctx->stream,
                            (const int32_t*)ctx->C.i, (const int32_t*)ctx->C.j, nc, (const uint64_t*)ctx->hist_keys,
                            (const double*)ctx->hist_gamma, ctx->hist_n, prm.gamma, ctx->gamma);
@@ -431,6 +470,54 @@ void prepare_gamma(scopi_hip_ctx* ctx, const scopi_hip_params& prm, bool from_hi
 }
 
 // m_old_contacts = this step's contacts (solver.hpp:300)
+bool warm_problem(const scopi_hip_params& prm) {
+    return prm.problem == SCOPI_PROBLEM_NOFRICTION || prm.problem == SCOPI_PROBLEM_FRICTION;
+}
+
+// warm start: the current contacts' first iterate from the last solve's multipliers, or null when
+// there is nothing to start from (no history, another row count)
+const double* prepare_lambda(scopi_hip_ctx* ctx, const scopi_hip_params& prm) {
+    const int nr = prm.problem == SCOPI_PROBLEM_FRICTION ? 3 : 1;
+    const int64_t nc = ctx->nc;
+    if (!prm.warm_start || nc == 0 || ctx->hist_n == 0 || ctx->hist_kind != 1 || ctx->hist_nr != nr) return nullptr;
+    if (nr * nc > ctx->lam0_cap) {
+        if (ctx->lam0) SCOPI_HIP_CHECK(hipFree(ctx->lam0));
+        ctx->lam0_cap = nr * nc + nr * nc / 4 + 1024;
+        SCOPI_HIP_CHECK(hipMalloc(&ctx->lam0, ctx->lam0_cap * sizeof(double)));
+    }
+    hipLaunchKernelGGL(k_transfer_lambda, dim3(grid_for(nc)), dim3(kBlock), 0, ctx->stream, (const int32_t*)ctx->C.i,
+                       (const int32_t*)ctx->C.j, nc, (const uint64_t*)ctx->hist_keys, (const double*)ctx->hist_lam,
+                       ctx->hist_n, nr, ctx->lam0);
+    SCOPI_HIP_CHECK(hipGetLastError());
+    return ctx->lam0;
+}
+
+// warm start: keep this solve's contacts and multipliers for the next one
+void record_lambda(scopi_hip_ctx* ctx) {
+    const int64_t nc = ctx->nc;
+    const int nr = ctx->solver.rows_per_contact();
+    if (nc > ctx->hist_cap) {
+        for (void* p : {(void*)ctx->hist_keys, (void*)ctx->hist_gamma})
+            if (p) SCOPI_HIP_CHECK(hipFree(p));
+        ctx->hist_cap = nc + nc / 4 + 1024;
+        SCOPI_HIP_CHECK(hipMalloc(&ctx->hist_keys, ctx->hist_cap * sizeof(uint64_t)));
+        SCOPI_HIP_CHECK(hipMalloc(&ctx->hist_gamma, ctx->hist_cap * sizeof(double)));
+    }
+    if (nr * nc > ctx->hist_lam_cap) {
+        if (ctx->hist_lam) SCOPI_HIP_CHECK(hipFree(ctx->hist_lam));
+        ctx->hist_lam_cap = nr * nc + nr * nc / 4 + 1024;
+        SCOPI_HIP_CHECK(hipMalloc(&ctx->hist_lam, ctx->hist_lam_cap * sizeof(double)));
+    }
+    if (nc > 0)
+        hipLaunchKernelGGL(k_record_lambda, dim3(grid_for(nc)), dim3(kBlock), 0, ctx->stream, (const int32_t*)ctx->C.i,
+                           (const int32_t*)ctx->C.j, nc, nr, ctx->solver.final_lambda(), ctx->hist_keys,
+                           ctx->hist_lam);
+    SCOPI_HIP_CHECK(hipGetLastError());
+    ctx->hist_n = nc;
+    ctx->hist_nr = nr;
+    ctx->hist_kind = 1;
+}
+
 void record_history(scopi_hip_ctx* ctx) {
     const int64_t nc = ctx->nc;
     if (nc > ctx->hist_cap) {
@@ -445,6 +532,7 @@ void record_history(scopi_hip_ctx* ctx) {
                            (const int32_t*)ctx->C.j, nc, (const double*)ctx->gamma, ctx->hist_keys, ctx->hist_gamma);
     SCOPI_HIP_CHECK(hipGetLastError());
     ctx->hist_n = nc;
+    ctx->hist_kind = 2;
 }
 
 bool params_ok(const scopi_hip_params* p) {
@@ -455,7 +543,7 @@ bool params_ok(const scopi_hip_params* p) {
            (p->vap == SCOPI_VAP_FIXED || p->vap == SCOPI_VAP_FPD || p->vap == SCOPI_VAP_PROJECTION) &&
            (p->method == SCOPI_METHOD_APGD || p->method == SCOPI_METHOD_PGD) &&
            (p->precision == SCOPI_PRECISION_FP64 || p->precision == SCOPI_PRECISION_FP32_REFINE) &&
-           p->warm_start == 0;
+           (p->warm_start == 0 || p->warm_start == 1);
 }
 
 BoxDev box_dev(const scopi_hip_ctx* ctx) {
@@ -749,7 +837,8 @@ int scopi_hip_destroy(scopi_hip_ctx* ctx) {
     contacts_free(ctx->C);
     contacts_free(ctx->C2);
     if (ctx->proj) (void)hipFree(ctx->proj);
-    for (void* p : {(void*)ctx->gamma, (void*)ctx->hist_keys, (void*)ctx->hist_gamma})
+    for (void* p : {(void*)ctx->gamma, (void*)ctx->hist_keys, (void*)ctx->hist_gamma, (void*)ctx->hist_lam,
+                    (void*)ctx->lam0})
         if (p) (void)hipFree(p);
     for (void* p : {(void*)ctx->pflags, (void*)ctx->poffs, (void*)ctx->porig, (void*)ctx->ptotal, (void*)ctx->pkeys,
                     (void*)ctx->pvals, (void*)ctx->pi_of})
@@ -1157,10 +1246,16 @@ int scopi_hip_solve(scopi_hip_ctx* ctx, const scopi_hip_params* params, double d
             hipLaunchKernelGGL(k_fill_mu, dim3(grid_for(ctx->nc)), dim3(kBlock), 0, ctx->stream, ctx->C.mu, ctx->nc,
                                params->mu);
         const bool visc = viscous_problem(params->problem);
+        if (params->warm_start && !warm_problem(*params))
+            return fail(ctx, SCOPI_HIP_ERR_UNSUPPORTED, "solve: warm_start with NoFriction or Friction only");
         if (visc) prepare_gamma(ctx, *params, false);
         int rc = ctx->solver.solve(ctx->P, ctx->C, ctx->nc, *params, dt, ctx->all_spheres && !ctx->any_periodic(),
-                                   ctx->stream, info, visc ? ctx->gamma : nullptr);
+                                   ctx->stream, info, visc ? ctx->gamma : nullptr, prepare_lambda(ctx, *params));
         if (rc != SCOPI_HIP_OK) return fail(ctx, rc, "solve: unsupported option");
+        if (params->warm_start)
+            record_lambda(ctx);
+        else if (ctx->hist_kind == 1)
+            ctx->hist_kind = 0;  // no multipliers for these contacts: a later warm start begins at 0
         SCOPI_HIP_CHECK(hipStreamSynchronize(ctx->stream));
         return SCOPI_HIP_OK;
     });
@@ -1204,6 +1299,8 @@ int scopi_hip_run(scopi_hip_ctx* ctx, const scopi_hip_params* params, double dt,
                   scopi_hip_solve_info* info) {
     return guarded(ctx, [&]() -> int {
         if (!params_ok(params)) return fail(ctx, SCOPI_HIP_ERR_ARG, "run: bad params");
+        if (params->warm_start && !warm_problem(*params))
+            return fail(ctx, SCOPI_HIP_ERR_UNSUPPORTED, "run: warm_start with NoFriction or Friction only");
         if (params->vap == SCOPI_VAP_PROJECTION && ctx->proj_n != ctx->P.n - ctx->P.n_inactive)
             return fail(ctx, SCOPI_HIP_ERR_STATE, "run: no projection velocities for these particles");
         scopi_hip_solve_info acc{};
@@ -1224,9 +1321,13 @@ int scopi_hip_run(scopi_hip_ctx* ctx, const scopi_hip_params* params, double dt,
                                    (const double*)ctx->proj, (const double*)(ctx->proj ? ctx->proj + 3 * na : nullptr));
             scopi_hip_solve_info one{};
             int rc = ctx->solver.solve(P, ctx->C, ctx->nc, *params, dt, ctx->all_spheres && !ctx->any_periodic(), s, &one,
-                                       visc ? ctx->gamma : nullptr);
+                                       visc ? ctx->gamma : nullptr, prepare_lambda(ctx, *params));
             if (rc != SCOPI_HIP_OK) return fail(ctx, rc, "run: solve failed");
             if (visc) record_history(ctx);
+            if (params->warm_start)
+                record_lambda(ctx);
+            else if (ctx->hist_kind == 1)
+                ctx->hist_kind = 0;  // no multipliers for these contacts: a later warm start begins at 0
             if (na > 0)
                 hipLaunchKernelGGL(k_update_move, dim3(grid_for(na)), dim3(kBlock), 0, s, P, ctx->solver.u(),
                                    ctx->solver.w(), ctx->solver.pcap(), dt);
@@ -1491,7 +1592,8 @@ extern "C" {
 
 int scopi_hip_batch_create(const scopi_hip_params* params, double dt, scopi_hip_batch** out) {
     if (!out || !params_ok(params) || !(dt > 0) || params->vap == SCOPI_VAP_PROJECTION) return SCOPI_HIP_ERR_ARG;
-    if (params->problem == SCOPI_PROBLEM_FRICTION_FIXED_POINT || viscous_problem(params->problem))
+    if (params->problem == SCOPI_PROBLEM_FRICTION_FIXED_POINT || viscous_problem(params->problem) ||
+        params->warm_start)
         return SCOPI_HIP_ERR_UNSUPPORTED;
     *out = nullptr;
     auto* b = new scopi_hip_batch();

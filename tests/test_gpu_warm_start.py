@@ -1,0 +1,103 @@
+"""Warm start (scopi_hip_params.warm_start = 1): every NoFriction or Friction solve starts from the
+last solve's multipliers of the contacts found again (matched by (i, j), new contacts at 0) instead
+of lambda = 0 (apgd.hpp:152-159).  No counterpart in the reference; the oracle's warm_start option
+is its checker (same transfer, same first iterate), so the device follows the oracle's iterate
+path and is held to the bars of the cold trajectory tests:
+
+* a moving scene (8 x 8 x 4 spheres, |v| <= 1), 20 steps: NoFriction with fixed-step APGD (alpha
+  3.8e4 < 1/L), tol 1e-10, FP64 and FP32 + FP64 refinement: positions within 1e-9, velocities
+  within 1e-6 of the oracle; Friction mu 0.3 with the reference's adaptive APGD, tol 1e-6:
+  positions within 1e-8;
+* a resting stack (4 x 4 x 4 spheres at zero gap under gravity): after the first step every FP64
+  solve stops at its first iteration, as the oracle's does, while the cold solves take hundreds
+  (FP32 + FP64: under a quarter of the cold iterations);
+* the problems whose rows change between steps refuse it (FrictionFixedPoint, Viscous,
+  ViscousFriction), and so do batches.
+"""
+import sys
+
+import numpy as np
+import pytest
+
+from scopi_amd import hip, scenes
+
+pytestmark = pytest.mark.gpu
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+MOVING = {
+    "nofriction": dict(problem=hip.NOFRICTION, tolerance=1e-10, dynamic_descent=0, alpha=3.8e4, max_ite=200000),
+    "friction": dict(problem=hip.FRICTION, mu=0.3, tolerance=1e-6, max_ite=100000),
+}
+BARS = {"nofriction": (1e-9, 1e-6), "friction": (1e-8, None)}
+
+
+def oracle_kw(kw):
+    return dict(vap=1, dmax=0.02, kd_tree_radius=0.042 ** 2, warm_start=1,
+                **{k: v for k, v in kw.items() if k != "precision"})
+
+
+@pytest.mark.parametrize("precision", [hip.FP64, hip.FP32_REFINE], ids=["fp64", "fp32_refine"])
+@pytest.mark.parametrize("case", list(MOVING))
+def test_warm_start_trajectory(oracle_lib, case, precision):
+    sc = scenes.spheres_3d(8, 8, 4, velocity_scale=1.0)
+    kw = dict(MOVING[case])
+    say(f"[warm start] moving {case} precision={precision}")
+    g = hip.HipScene(sc)
+    info = g.run(hip.params(vap=hip.VAP_FPD, dmax=0.02, kd_tree_radius=0.042 ** 2, warm_start=1, precision=precision,
+                            **kw), 1e-3, 20)
+    sg = g.state()
+    g.close()
+    o = oracle_lib.Oracle(sc, **oracle_kw(kw))
+    o.run(1e-3, 20)
+    so = o.state()
+    say(f"  iterations gpu {info['iterations']} oracle {o.stats()['total_iterations']}; max|dpos| "
+        f"{np.abs(sg['pos'] - so['pos']).max():.3e} max|dv| {np.abs(sg['v'] - so['v']).max():.3e}")
+    bp, bv = BARS[case]
+    np.testing.assert_allclose(sg["pos"], so["pos"], rtol=0, atol=bp)
+    if bv is not None:
+        np.testing.assert_allclose(sg["v"], so["v"], rtol=0, atol=bv)
+
+
+@pytest.mark.parametrize("precision", [hip.FP64, hip.FP32_REFINE], ids=["fp64", "fp32_refine"])
+@pytest.mark.parametrize("case", ["nofriction", "friction"])
+def test_warm_start_resting_stack(oracle_lib, case, precision):
+    sc = scenes.spheres_3d(4, 4, 4, spacing=0.02, jitter=0.0, rmin=0.01, rmax=0.01)
+    kw = dict(dmax=0.005, kd_tree_radius=0.03 ** 2, tolerance=1e-8, max_ite=100000,
+              problem=hip.FRICTION if case == "friction" else hip.NOFRICTION, **({"mu": 0.3} if case == "friction" else {}))
+    say(f"[warm start] stack {case} precision={precision}")
+    its = {}
+    states = {}
+    for warm in (1, 0):
+        g = hip.HipScene(sc)
+        prm = hip.params(vap=hip.VAP_FPD, warm_start=warm, precision=precision, **kw)
+        its[warm] = [g.run(prm, 1e-3, k + 1, k)["iterations"] for k in range(8)]
+        states[warm] = g.state()
+        g.close()
+    o = oracle_lib.Oracle(sc, vap=1, warm_start=1, **kw)
+    o.run(1e-3, 8)
+    so = o.state()
+    say(f"  gpu warm {its[1]} cold {its[0]}; max|dpos| vs oracle {np.abs(states[1]['pos'] - so['pos']).max():.3e}")
+    assert min(its[0][1:]) > 100, its[0]
+    if precision == hip.FP64:
+        assert its[1][1:] == [1] * 7, its[1]
+    else:
+        # the FP32 phase starts from the FP32 rounding of the last solution and stops at once; the
+        # FP64 phase then recovers the digits the rounding took
+        assert max(its[1][1:]) < min(its[0][1:]) / 4, its
+    np.testing.assert_allclose(states[1]["pos"], so["pos"], rtol=0, atol=1e-9)
+    np.testing.assert_allclose(states[1]["v"], so["v"], rtol=0, atol=1e-6)
+
+
+@pytest.mark.parametrize("problem", [hip.FRICTION_FIXED_POINT, hip.VISCOUS, hip.VISCOUS_FRICTION])
+def test_warm_start_refused(problem):
+    g = hip.HipScene(scenes.spheres_3d(2, 2, 2))
+    with pytest.raises(hip.ScopiHipError):
+        g.run(hip.params(problem=problem, warm_start=1, dmax=0.02, kd_tree_radius=0.042 ** 2), 1e-3, 1)
+    g.close()
+    with pytest.raises(hip.ScopiHipError):
+        hip.HipBatch(hip.params(warm_start=1), 1e-3)
