@@ -37,21 +37,28 @@
 namespace scopi_hip {
 
 constexpr int kRedBlocksMax = 2048;
-constexpr int kPartialStride = 16;
-// Blocks of the per-contact kernel for nc contacts: one per contacts_per_block(layout) contacts,
-// at most `cap` (grid-stride beyond).  That is two loop iterations per lane: the rotation-free
-// layout takes two contacts per iteration, the others one.  The same rule sizes a scene's share
-// of a batch's grid, so a scene solved in a batch runs the one-scene arithmetic exactly.
-inline int64_t contacts_per_block(int layout) { return layout == kRotFree ? 2 * 2 * kBlock : 2 * kBlock; }
+constexpr int kPartialStride = 16;  // doubles of partial_trial_ per block of capacity
+__host__ __device__ inline int64_t trial_pstride(const SceneMap& M) { return M.pstride ? M.pstride : kRedBlocksMax; }
+// Blocks of the per-contact kernel for nc contacts.  A lane takes `per_iter` contacts per loop
+// iteration (two in the rotation-free layout, one in the others) and k iterations, k = 2 unless
+// `cap` blocks of two iterations cannot hold the contacts; the grid is then the fewest blocks that
+// hold nc in k iterations, so that every block runs the same number of iterations (a block with one
+// iteration more than the others would hold up the whole launch).  The same rule sizes a scene's
+// share of a batch's grid, so a scene solved in a batch runs the one-scene arithmetic exactly.
 inline int64_t row_blocks(int64_t nc, int cap, int layout) {
-    const int64_t per = contacts_per_block(layout);
-    return std::min<int64_t>(std::max<int64_t>((nc + per - 1) / per, 1), cap);
+    const int64_t per_iter = (layout == kRotFree ? 2 : 1) * (int64_t)kBlock;
+    const int64_t k = std::max<int64_t>(2, (nc + per_iter * cap - 1) / (per_iter * cap));
+    return std::min<int64_t>(std::max<int64_t>((nc + per_iter * k - 1) / (per_iter * k), 1), cap);
 }
 // Blocks of k_bt for na particles at G lanes per particle, at most kBtBlocksMax (grid-stride
 // beyond): each block writes one block sum per round, which every block of the next k_trial reads.
-constexpr int kBtBlocksMax = 2048;
+// Its blocks are 1024 lanes wide, so that there are 4x fewer of those sums than with 256-lane
+// blocks for the same lanes: every k_trial block loads them all, and with 2048 sums those loads
+// on the same few cache lines, from every block, cost about 2 us per round.
+constexpr int kBtBlock = 1024;
+constexpr int kBtBlocksMax = 512;
 inline unsigned bt_blocks(int64_t na, int G) {
-    return std::min<unsigned>(grid_for(na * (int64_t)G), (unsigned)kBtBlocksMax);
+    return std::min<unsigned>(grid_for(na * (int64_t)G, kBtBlock), (unsigned)kBtBlocksMax);
 }
 constexpr int kStatusStalled = 2;                // FP32 phase reached its precision floor
 constexpr int kMaxConsecutiveRejections32 = 40;  // 2^40 growth of L without an accepted step
@@ -59,38 +66,56 @@ constexpr int64_t kMaxStagnation32 = 300;        // FP32 accepted iterations wit
 
 // ------------------------------------------------------------------ reductions
 // Sum of NV values over the block; the totals land in thread 0.
-template <int NV>
-__device__ inline void block_sum(double (&v)[NV], double (*lds)[kBlock / 64]) {
+// Each level of the wavefront tree shuffles all NV values before adding any, so the NV dependency
+// chains overlap (one cross-lane latency per level, not NV); every value is summed in the same
+// order as one value alone.
+template <int NV, int NTH = kBlock>
+__device__ inline void block_sum(double (&v)[NV], double (*lds)[NTH / 64]) {
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    double x[NV];
 #pragma unroll
-    for (int k = 0; k < NV; ++k) {
-        double x = v[k];
+    for (int k = 0; k < NV; ++k) x[k] = v[k];
 #pragma unroll
-        for (int o = 32; o > 0; o >>= 1) x += __shfl_down(x, o, 64);
-        if (lane == 0) lds[k][wave] = x;
+    for (int o = 32; o > 0; o >>= 1) {
+        double y[NV];
+#pragma unroll
+        for (int k = 0; k < NV; ++k) y[k] = __shfl_down(x[k], o, 64);
+#pragma unroll
+        for (int k = 0; k < NV; ++k) x[k] += y[k];
     }
+    if (lane == 0)
+#pragma unroll
+        for (int k = 0; k < NV; ++k) lds[k][wave] = x[k];
     __syncthreads();
     if (threadIdx.x == 0) {
 #pragma unroll
         for (int k = 0; k < NV; ++k) {
             double s = 0.0;
-            for (int w = 0; w < kBlock / 64; ++w) s += lds[k][w];
+            for (int w = 0; w < NTH / 64; ++w) s += lds[k][w];
             v[k] = s;
         }
     }
     __syncthreads();
 }
 
-// Sum of the first NV partials of `nblocks` blocks, in block order (deterministic).
-template <int NV>
-__device__ inline void reduce_partials(const double* __restrict__ partial, int nblocks, double (*lds)[kBlock / 64],
-                                       double (&tot)[NV]) {
+// The thread's share of a fixed-order sum: p[b * stride] for b = threadIdx.x, threadIdx.x + NTH,
+// ... < n, accumulated in that order from 0.  The first R loads are issued together, so the sum
+// waits for one memory round trip instead of R (the partials were written by blocks on other XCDs
+// and come from the Infinity Cache or HBM); any further ones follow one at a time.  Adding 0.0 for
+// the missing terms leaves every sum bitwise unchanged (a sum started from +0.0 is never -0.0).
+template <int R, int NTH = kBlock>
+__device__ inline double thread_sum(const double* __restrict__ p, int n, int stride) {
+    double v[R];
 #pragma unroll
-    for (int k = 0; k < NV; ++k) tot[k] = 0.0;
-    for (int b = threadIdx.x; b < nblocks; b += kBlock)
+    for (int r = 0; r < R; ++r) {
+        const int b = (int)threadIdx.x + r * NTH;
+        v[r] = b < n ? p[(int64_t)b * stride] : 0.0;
+    }
+    double t = 0.0;
 #pragma unroll
-        for (int k = 0; k < NV; ++k) tot[k] += partial[b * kPartialStride + k];
-    block_sum<NV>(tot, lds);
+    for (int r = 0; r < R; ++r) t += v[r];
+    for (int b = (int)threadIdx.x + R * NTH; b < n; b += NTH) t += p[(int64_t)b * stride];
+    return t;
 }
 
 // ------------------------------------------------------------------ geometry helpers
@@ -399,20 +424,20 @@ __device__ inline SceneBlock scene_block(const SceneMap& M, int64_t nc) {
 // <lambda, Q lambda> = dt^2 a^T M^-1 a without a pass over the contacts; and block 0
 // reduces the trial kernel's block sums into ctrl->red.
 template <class T, int LAY, int G>
-__global__ void __launch_bounds__(kBlock) k_bt(ApgdCtrl* __restrict__ ctrl, int use_slot, BMatrix<T> B, int64_t na,
+__global__ void __launch_bounds__(kBtBlock) k_bt(ApgdCtrl* __restrict__ ctrl, int use_slot, BMatrix<T> B, int64_t na,
                                               const int* __restrict__ tstart, const int32_t* __restrict__ tc,
                                               const double* __restrict__ partial_trial, int ntrial,
                                               double* __restrict__ partial_bt, SceneMap M) {
     constexpr int NR = lay<LAY>::NR, NT = lay<LAY>::NT, NW = lay<LAY>::NW;
-    __shared__ double lds[8][kBlock / 64];
+    __shared__ double lds[1][kBtBlock / 64];
     const bool multi = M.bblk != nullptr;
     const int sc = multi ? M.bblk[blockIdx.x] : 0;
     const int lb = multi ? (int)blockIdx.x - M.bfirst[sc] : (int)blockIdx.x;
     const int64_t p0 = multi ? M.arange[2 * sc] : 0, p1 = multi ? M.arange[2 * sc + 1] : na;
-    // grid-stride over the scene's particles: kBlock / G particles per block and pass
+    // grid-stride over the scene's particles: kBtBlock / G particles per block and pass
     const int nbk = multi ? M.bcount[sc] : (int)gridDim.x;
-    const int64_t pstep = (int64_t)nbk * (kBlock / G);
-    int64_t p = p0 + ((int64_t)lb * kBlock + threadIdx.x) / G;
+    const int64_t pstep = (int64_t)nbk * (kBtBlock / G);
+    int64_t p = p0 + ((int64_t)lb * kBtBlock + threadIdx.x) / G;
     const int sub = (int)(threadIdx.x % G);
     // the state and the first particle's entry range are loaded together: neither waits for the other
     if (ctrl) ctrl += sc;
@@ -420,17 +445,17 @@ __global__ void __launch_bounds__(kBlock) k_bt(ApgdCtrl* __restrict__ ctrl, int 
     const int s = (use_slot >= 0) ? use_slot : (ctrl ? ctrl->trial : 0);
     int e0 = p < p1 ? tstart[p] : 0, e1 = p < p1 ? tstart[p + 1] : 0;
     if (done) return;
-    if (ctrl && partial_trial && lb == 0) {
-        double tot[8];
-        if (multi)
-            reduce_partials<8>(partial_trial + (int64_t)M.rfirst[sc] * kPartialStride, M.rcount[sc], lds, tot);
-        else
-            reduce_partials<8>(partial_trial, ntrial, lds, tot);
-        if (threadIdx.x == 0) {
-            for (int k = 0; k < 7; ++k) ctrl->red[k] = tot[k];
-            ctrl->red[8] = tot[7];
-            ctrl->pending = 1;
+    if (ctrl && partial_trial && lb < 8) {
+        // the trial kernel's 8 block sums, one quantity per block (blocks lb, lb + nbk, ... < 8),
+        // each in block order; a quantity's sums are contiguous
+        const double* pt = multi ? partial_trial + M.rfirst[sc] : partial_trial;
+        const int nt = multi ? M.rcount[sc] : ntrial;
+        for (int k = lb; k < 8; k += nbk) {
+            double tot[1] = {thread_sum<kRedBlocksMax / kBtBlock, kBtBlock>(pt + k * trial_pstride(M), nt, 1)};
+            block_sum<1, kBtBlock>(tot, lds);
+            if (threadIdx.x == 0) ctrl->red[k < 7 ? k : 8] = tot[0];
         }
+        if (lb == 0 && threadIdx.x == 0) ctrl->pending = 1;
     }
     const T* lam = slot<T>(B.lam, s, NR * B.cap);
     T* w = slot<T>(B.w, s, 6 * B.pcap);
@@ -502,7 +527,7 @@ __global__ void __launch_bounds__(kBlock) k_bt(ApgdCtrl* __restrict__ ctrl, int 
         if (p + pstep < p1) e0 = tstart[p + pstep], e1 = tstart[p + pstep + 1];
     }
     if (ctrl) {
-        block_sum<1>(q, lds);
+        block_sum<1, kBtBlock>(q, lds);
         if (threadIdx.x == 0) partial_bt[blockIdx.x] = q[0];
     }
 }
@@ -515,10 +540,10 @@ static void launch_bt(int G, hipStream_t s, ApgdCtrl* ctrl, int use_slot, const 
     const unsigned grid = M.bblk ? (unsigned)grid_blocks : bt_blocks(na, G);
     auto go = [&](auto kernel) {
         if (e0)  // start / stop events taken from the kernel's own dispatch
-            hipExtLaunchKernelGGL(kernel, dim3(grid), dim3(kBlock), 0, s, e0, e1, 0, ctrl, use_slot, B, na, tstart, tc,
-                                  partial, ntrial, partial_bt, M);
+            hipExtLaunchKernelGGL(kernel, dim3(grid), dim3(kBtBlock), 0, s, e0, e1, 0, ctrl, use_slot, B, na, tstart,
+                                  tc, partial, ntrial, partial_bt, M);
         else
-            hipLaunchKernelGGL(kernel, dim3(grid), dim3(kBlock), 0, s, ctrl, use_slot, B, na, tstart, tc, partial,
+            hipLaunchKernelGGL(kernel, dim3(grid), dim3(kBtBlock), 0, s, ctrl, use_slot, B, na, tstart, tc, partial,
                                ntrial, partial_bt, M);
     };
     switch (G) {
@@ -704,7 +729,7 @@ __global__ void __launch_bounds__(kBlock) k_trial(const ApgdCtrl* __restrict__ i
     block_sum<8>(s, lds);
     if (threadIdx.x == 0)
 #pragma unroll
-        for (int k = 0; k < 8; ++k) partial[blockIdx.x * kPartialStride + k] = s[k];
+        for (int k = 0; k < 8; ++k) partial[k * trial_pstride(M) + blockIdx.x] = s[k];
 }
 
 // Control logic of apgd.hpp:172-209 / pgd apgd.hpp:68-89 on the reduced sums.
@@ -807,25 +832,28 @@ __device__ inline void apgd_control(ApgdCtrl* ctrl) {
 __device__ inline void control_step(const ApgdCtrl* __restrict__ in, ApgdCtrl* __restrict__ out,
                                     const double* __restrict__ partial_bt, int nbt, ApgdCtrl& st,
                                     double (*lds)[kBlock / 64], bool leader, int32_t* done_flag) {
-    // k_bt's block sums are read before the state is known to need them, so their
-    // loads overlap the state's (a fixed summation order)
-    double tot[1] = {0.0};
-    for (int b = threadIdx.x; b < nbt; b += kBlock) tot[0] += partial_bt[b];
-    if (threadIdx.x == 0) st = *in;
-    __syncthreads();
-    if (st.pending && !st.done) {
-        block_sum<1>(tot, lds);
-        if (threadIdx.x == 0) {
-            st.red[7] = st.dt2 * tot[0];
-            apgd_control(&st);
+    // the state and k_bt's block sums are loaded together, before the state is known to need the
+    // sums (a fixed summation order)
+    ApgdCtrl c;
+    if (threadIdx.x == 0) c = *in;
+    double tot[1] = {thread_sum<kBtBlocksMax / kBlock>(partial_bt, nbt, 1)};
+    // summed whether or not the state needs it, so that no lane waits for the state to learn that
+    block_sum<1>(tot, lds);
+    if (threadIdx.x == 0) {
+        // the step runs on a register copy of the state (on the shared copy, every field access would
+        // be a round trip to LDS), which is then published to the block and, by the leader, to `out`
+        if (c.pending && !c.done) {
+            c.red[7] = c.dt2 * tot[0];
+            apgd_control(&c);
+        }
+        c.pending = 0;
+        st = c;
+        if (leader) {
+            *out = c;
+            if (done_flag) *done_flag = c.done;
         }
     }
-    if (threadIdx.x == 0) st.pending = 0;
     __syncthreads();
-    if (leader && threadIdx.x == 0) {
-        *out = st;
-        if (done_flag) *done_flag = st.done;
-    }
 }
 
 // The control step alone (end of a batch of rounds, before the host reads the state);
@@ -1775,6 +1803,7 @@ int Solver::solve_scenes(const Particles& P, const Contacts& C, int64_t nc, cons
     }
     SCOPI_HIP_CHECK(hipMemcpyAsync(d_arange_, ar.data(), 2 * K * sizeof(int64_t), hipMemcpyHostToDevice, s));
     map_ = SceneMap{};
+    map_.pstride = partial_cap_;
     map_.crange = S.d_crange;
     map_.arange = d_arange_;
     map_.done = d_done_;

@@ -82,6 +82,8 @@ struct SceneMap {
     const int64_t* crange = nullptr;   // per scene: contacts [lo, hi)
     const int64_t* arange = nullptr;   // per scene: active particles [lo, hi)
     int32_t* done = nullptr;           // per scene: done flag, written by the control steps
+    int64_t pstride = 0;               // trial kernel's block sums: quantity k of block b at k * pstride + b
+                                       // (0: kRedBlocksMax, the one-scene buffer)
 };
 
 // Host description of a scene set (scenes in order, consecutive ranges).
@@ -200,7 +202,7 @@ class Solver {
     double* w_ = nullptr;  // 3 x pcap
     ApgdCtrl* ctrl_ = nullptr;    // 2 alternating copies
     ApgdCtrl* h_ctrl_ = nullptr;  // pinned
-    double* partial_trial_ = nullptr;  // kPartialStride per block of k_trial
+    double* partial_trial_ = nullptr;  // 8 block sums per block of k_trial, quantity-major (SceneMap::pstride)
     double* partial_bt_ = nullptr;     // one per block of k_bt, contiguous
     int64_t partial_bt_cap_ = 0;
     int64_t rows_ = 0, nc_ = 0;
