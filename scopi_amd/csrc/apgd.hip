@@ -393,6 +393,16 @@ __device__ inline void control_step(const ApgdCtrl* __restrict__ in, ApgdCtrl* _
                                     const double* __restrict__ partial_bt, int nbt, ApgdCtrl& st,
                                     double (*lds)[kBlock / 64], bool leader, int32_t* done_flag);
 
+// Logical index of block b among nb: the hardware deals workgroups to the 8 XCDs in turn (XCD
+// b mod 8), so XCD x runs blocks x, x + 8, ...; they are given a contiguous run of logical blocks,
+// so that in every sweep of a grid-stride loop an XCD works on neighbouring contacts (or
+// particles), whose w and lambda gathers then hit its own L2 instead of fetching the same lines
+// on every XCD.  Measured on 1.17M contacts: k_trial 19.5 -> 18.3 us.
+__device__ inline int xcd_logical(int b, int nb) {
+    const int x = b & 7, r = b >> 3, q = nb >> 3, rem = nb & 7;
+    return x < rem ? x * (q + 1) + r : rem * (q + 1) + (x - rem) * q + r;
+}
+
 // The block's scene and its share of the per-contact grid.
 struct SceneBlock {
     int sc, lb, nb;      // scene, block index within the scene, blocks of the scene
@@ -434,16 +444,19 @@ __global__ void __launch_bounds__(kBtBlock) k_bt(ApgdCtrl* __restrict__ ctrl, in
     const int sc = multi ? M.bblk[blockIdx.x] : 0;
     const int lb = multi ? (int)blockIdx.x - M.bfirst[sc] : (int)blockIdx.x;
     const int64_t p0 = multi ? M.arange[2 * sc] : 0, p1 = multi ? M.arange[2 * sc + 1] : na;
-    // grid-stride over the scene's particles: kBtBlock / G particles per block and pass
+    // grid-stride over the scene's particles in logical block order (XCD-aware for one scene),
+    // kBtBlock / G particles per block and pass
     const int nbk = multi ? M.bcount[sc] : (int)gridDim.x;
+    const int lblk = multi ? lb : xcd_logical(lb, nbk);
     const int64_t pstep = (int64_t)nbk * (kBtBlock / G);
-    int64_t p = p0 + ((int64_t)lb * kBtBlock + threadIdx.x) / G;
+    const int64_t pend = p1;
+    int64_t p = p0 + ((int64_t)lblk * kBtBlock + threadIdx.x) / G;
     const int sub = (int)(threadIdx.x % G);
     // the state and the first particle's entry range are loaded together: neither waits for the other
     if (ctrl) ctrl += sc;
     const int done = ctrl ? ctrl->done : 0;
     const int s = (use_slot >= 0) ? use_slot : (ctrl ? ctrl->trial : 0);
-    int e0 = p < p1 ? tstart[p] : 0, e1 = p < p1 ? tstart[p + 1] : 0;
+    int e0 = p < pend ? tstart[p] : 0, e1 = p < pend ? tstart[p + 1] : 0;
     if (done) return;
     if (ctrl && partial_trial && lb < 8) {
         // the trial kernel's 8 block sums, one quantity per block (blocks lb, lb + nbk, ... < 8),
@@ -462,7 +475,7 @@ __global__ void __launch_bounds__(kBtBlock) k_bt(ApgdCtrl* __restrict__ ctrl, in
     double q[1] = {0.0};
     // a lane group shares p, so each group passes through the loop as a whole (the shuffles
     // below stay within it)
-    for (; p < p1; p += pstep) {
+    for (; p < pend; p += pstep) {
         T acc[NW];
 #pragma unroll
         for (int k = 0; k < NW; ++k) acc[k] = (T)0;
@@ -524,11 +537,11 @@ __global__ void __launch_bounds__(kBtBlock) k_bt(ApgdCtrl* __restrict__ ctrl, in
                 w[p * NW + k] = wk;
                 q[0] += (double)acc[k] * (double)wk;
             }
-        if (p + pstep < p1) e0 = tstart[p + pstep], e1 = tstart[p + pstep + 1];
+        if (p + pstep < pend) e0 = tstart[p + pstep], e1 = tstart[p + pstep + 1];
     }
     if (ctrl) {
         block_sum<1, kBtBlock>(q, lds);
-        if (threadIdx.x == 0) partial_bt[blockIdx.x] = q[0];
+        if (threadIdx.x == 0) partial_bt[(multi ? M.bfirst[sc] : 0) + lblk] = q[0];
     }
 }
 
@@ -620,13 +633,18 @@ __global__ void __launch_bounds__(kBlock) k_trial(const ApgdCtrl* __restrict__ i
     // whose products need more registers); the first contacts' particle indices are loaded
     // together with the state and the block sums, each next group one iteration ahead
     constexpr int U = LAY == kRotFree ? 2 : 1;
+    // grid-stride over the scene's contacts in logical block order (XCD-aware for one scene: the
+    // blocks of an XCD take neighbouring kBlock-contact groups of every sweep); the grid (row_blocks)
+    // gives every block the same iteration count
+    const int lblk = M.rblk ? sb.lb : xcd_logical(sb.lb, sb.nb);
     const int64_t cstep = (int64_t)sb.nb * kBlock;
-    int64_t c = sb.lo + (int64_t)sb.lb * kBlock + threadIdx.x;
+    const int64_t cend = sb.hi;
+    int64_t c = sb.lo + (int64_t)lblk * kBlock + threadIdx.x;
     int32_t inx[U], jnx[U];
 #pragma unroll
     for (int u = 0; u < U; ++u) {
         inx[u] = jnx[u] = 0;
-        if (c + u * cstep < sb.hi) inx[u] = ci[c + u * cstep], jnx[u] = cj[c + u * cstep];
+        if (c + u * cstep < cend) inx[u] = ci[c + u * cstep], jnx[u] = cj[c + u * cstep];
     }
     control_step(in + sb.sc, out + sb.sc, partial_bt + (M.rblk ? M.bfirst[sb.sc] : 0), M.rblk ? M.bcount[sb.sc] : nbt,
                  st, lds, sb.lb == 0, M.done ? M.done + sb.sc : nullptr);
@@ -696,16 +714,16 @@ __global__ void __launch_bounds__(kBlock) k_trial(const ApgdCtrl* __restrict__ i
             s[7] += (double)z[d] * (double)z[d];
         }
     };
-    for (; c < sb.hi; c += U * cstep) {
+    for (; c < cend; c += U * cstep) {
         int64_t cu[U], iu[U], ju[U];
         bool hu[U];
 #pragma unroll
         for (int u = 0; u < U; ++u) {
-            hu[u] = c + u * cstep < sb.hi;
+            hu[u] = c + u * cstep < cend;
             cu[u] = hu[u] ? c + u * cstep : c;  // past the end: a duplicate of c, not stored
             iu[u] = (int64_t)inx[u] - ni, ju[u] = (int64_t)jnx[u] - ni;
             const int64_t cn = c + (U + u) * cstep;
-            if (cn < sb.hi) inx[u] = ci[cn], jnx[u] = cj[cn];
+            if (cn < cend) inx[u] = ci[cn], jnx[u] = cj[cn];
         }
         T l0[U][NR], l1[U][NR], cv[U][NR], q1[U][NR];
 #pragma unroll
@@ -729,7 +747,7 @@ __global__ void __launch_bounds__(kBlock) k_trial(const ApgdCtrl* __restrict__ i
     block_sum<8>(s, lds);
     if (threadIdx.x == 0)
 #pragma unroll
-        for (int k = 0; k < 8; ++k) partial[k * trial_pstride(M) + blockIdx.x] = s[k];
+        for (int k = 0; k < 8; ++k) partial[k * trial_pstride(M) + sb.pbase + lblk] = s[k];
 }
 
 // Control logic of apgd.hpp:172-209 / pgd apgd.hpp:68-89 on the reduced sums.
