@@ -275,16 +275,17 @@ def bench_batched(args):
     print(json.dumps(result), flush=True)
 
 
-def served_leg(requests, rate, request_steps, sample_steps, hbm_budget_gib, one_at_a_time):
+def served_leg(requests, rate, request_steps, sample_steps, hbm_budget_gib, one_at_a_time, warm_start=True):
     """Served scenes: Poisson arrivals (`rate` per s) of `requests` scene requests, 80% sphere scenes (the
     131k-sphere recipe scaled, NoFriction) and 20% superellipsoid scenes (the 32k recipe scaled, Friction
     mu 0.3), log-uniform 512..65,536 particles, `request_steps` steps each, admitted by estimated HBM
     footprint into one resident batch per kind (scopi_amd.serve).  The one-at-a-time figure steps every
     request's scene alone (its own context) for `sample_steps` timed steps after one warm-up step and
-    scales each to its step count."""
+    scales each to its step count.  With warm_start (the default), every scene's solves, served and
+    alone, start from its own last step's multipliers, kept on the device."""
     import torch
     from scopi_amd import hip, serve
-    kinds, footprint, prm = serve.hip_kinds()
+    kinds, footprint, prm = serve.hip_kinds(warm_start=warm_start)
     reqs = serve.poisson_requests(requests, rate, [("spheres", 0.8), ("superellipsoids", 0.2)], (512, 65536),
                                   request_steps, seed=2025)
     log(f"[bench] serving {len(reqs)} requests ({sum(r.kind == 'superellipsoids' for r in reqs)} superellipsoid "
@@ -314,7 +315,7 @@ def served_leg(requests, rate, request_steps, sample_steps, hbm_budget_gib, one_
     return {
         "scene_steps_per_s": st["scene_steps_per_s"], "scene_steps": st["scene_steps"],
         "step_latency_ms": st["step_latency_ms"], "wait_ms": st["wait_ms"], "completion_s": st["completion_s"],
-        "peak_resident_scenes": st["peak_resident_scenes"],
+        "peak_resident_scenes": st["peak_resident_scenes"], "warm_start": bool(warm_start),
         "workload": f"{len(reqs)} requests, Poisson rate {rate}/s, {request_steps} steps each, sizes log-uniform "
                     f"512..65,536, spheres NoFriction / superellipsoids Friction mu 0.3, HBM budget {hbm_budget_gib} GiB",
         "one_at_a_time": one,
@@ -327,7 +328,7 @@ def bench_served(args):
     import torch
     torch.cuda.init()
     sv = served_leg(args.requests, args.rate, args.request_steps, args.sample_steps, args.hbm_budget_gib,
-                    args.one_at_a_time)
+                    args.one_at_a_time, warm_start=not args.served_cold)
     result = {
         "metric": "served scene time-steps/s (Poisson arrivals, 80% sphere / 20% superellipsoid scenes, "
                   "512..65,536 particles) with p50/p99 step latency",
@@ -369,6 +370,8 @@ def main():
                          "Friction mu 0.3; batched: independent sphere scenes advanced together (secondary lines)")
     ap.add_argument("--scenes", type=int, default=512, help="batched: number of scenes")
     ap.add_argument("--one-at-a-time", action="store_true", help="batched / served: also step every scene alone")
+    ap.add_argument("--served-cold", action="store_true",
+                    help="served: start every solve from lambda = 0 instead of the scene's last multipliers")
     ap.add_argument("--requests", type=int, default=24, help="served: number of requests")
     ap.add_argument("--rate", type=float, default=4.0, help="served: mean arrivals per second")
     ap.add_argument("--request-steps", type=int, default=100, help="served: steps asked by each request")
@@ -513,7 +516,7 @@ def main():
     served = None
     if not args.no_served_leg:
         served = served_leg(args.served_requests, args.rate, args.served_request_steps, args.sample_steps,
-                            args.hbm_budget_gib, True)
+                            args.hbm_budget_gib, True, warm_start=not args.served_cold)
     result = {
         "metric": "time-steps/s at 131k spheres, APGD tol 1e-6",
         "value": steps_per_s,

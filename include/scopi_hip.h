@@ -109,6 +109,9 @@ typedef struct scopi_hip_solve_info {
     double residual;       /* last ||lambda_{k+1} - lambda_k||_2 */
     int64_t outer_iterations;  /* FrictionFixedPoint / ViscousFriction: solves of the fixed point (0 otherwise) */
     double fixed_point_error;  /* the fixed point's last relative change of s */
+    double step;               /* the step alpha = 1/L of the last accepted projected-gradient iteration:
+                                  at convergence |lambda - Proj(lambda - step grad f(lambda))| is about
+                                  the tolerance (the KKT residual the stopping test bounds) */
 } scopi_hip_solve_info;
 
 typedef struct scopi_hip_ctx scopi_hip_ctx;
@@ -293,10 +296,12 @@ int scopi_hip_kernel_timing(scopi_hip_ctx* ctx, double* out);
  * results do not depend on the other scenes of the batch (bitwise: the same alone
  * and in any mix).  Scene ids are returned by add_scene and stay valid until
  * remove_scene; scenes added or removed take effect at the next call that reads or
- * steps the batch.  The params must not ask for warm_start or vap_projection
- * (SCOPI_HIP_ERR_ARG); FrictionFixedPoint, Viscous and ViscousFriction are refused with
- * SCOPI_HIP_ERR_UNSUPPORTED (their outer loop and contact history are per scene; step such
- * scenes through scopi_hip_run). */
+ * steps the batch.  With warm_start, every scene's solve starts from its own last step's
+ * multipliers (matched by the scene's (i, j)), as the same scene run alone through scopi_hip_run
+ * would; that needs scenes of fewer than 2^20 particles (SCOPI_HIP_ERR_UNSUPPORTED at the step
+ * otherwise).  The params must not ask for vap_projection (SCOPI_HIP_ERR_ARG); FrictionFixedPoint,
+ * Viscous and ViscousFriction are refused with SCOPI_HIP_ERR_UNSUPPORTED (their outer loop and
+ * contact history are per scene; step such scenes through scopi_hip_run). */
 typedef struct scopi_hip_batch scopi_hip_batch;
 int scopi_hip_batch_create(const scopi_hip_params* params, double dt, scopi_hip_batch** out);
 int scopi_hip_batch_destroy(scopi_hip_batch* b);

@@ -73,6 +73,8 @@ def lib():
         L.oracle_amat.argtypes = [P, D, D]
         L.oracle_atmat.argtypes = [P, D, D]
         L.oracle_solve.argtypes = [P, C.c_double, D, D, D]
+        L.oracle_gradient.argtypes = [P, C.c_double, D, D]
+        L.oracle_set_contacts.argtypes = [P, C.c_int64, I64, D, D, D, D]
         L.oracle_set_a_priori_velocity.argtypes = [P, C.c_double]
         L.oracle_set_forces.argtypes = [P, D]
         L.oracle_update_contact_properties.argtypes = [P, C.c_double]
@@ -180,6 +182,14 @@ class Oracle:
             self._L.oracle_get_contacts(self._h, ij, dij, nij, pi, pj)
         return dict(ij=ij, dij=dij, nij=nij, pi=pi, pj=pj)
 
+    def set_contacts(self, ct):
+        """Replace the contact list with records {ij, dij, nij, pi, pj} (e.g. the device's)."""
+        nc = len(ct["dij"])
+        f = lambda a, w: np.ascontiguousarray(np.reshape(a, (nc, w)) if nc else np.zeros((1, w)), np.float64)
+        ij = np.ascontiguousarray(np.reshape(ct["ij"], (nc, 2)) if nc else np.zeros((1, 2)), np.int64)
+        dij = np.ascontiguousarray(ct["dij"] if nc else np.zeros(1), np.float64)
+        assert self._L.oracle_set_contacts(self._h, nc, ij, dij, f(ct["nij"], 3), f(ct["pi"], 3), f(ct["pj"], 3)) == 0
+
     def amat(self, u):
         nc = int(self._L.oracle_n_contacts(self._h))
         out = np.zeros(3 * nc)
@@ -202,6 +212,16 @@ class Oracle:
         self._L.oracle_solve(self._h, float(dt), lam, u, w)
         nl = int(self._L.oracle_lambda_size(self._h))
         return dict(lam=lam[:nl], u=u[: 3 * na].reshape(na, 3), w=w[: 3 * na].reshape(na, 3))
+
+    def gradient(self, dt, lam):
+        """Q lambda + c of the optim slot's problem (current contacts and a-priori velocities)."""
+        assert self.problem in (NOFRICTION, FRICTION)
+        nl = int(self._L.oracle_n_contacts(self._h)) * (3 if self.problem == FRICTION else 1)
+        lam = np.ascontiguousarray(lam, np.float64)
+        assert lam.shape == (nl,), (lam.shape, nl)
+        out = np.zeros(max(1, nl))
+        assert self._L.oracle_gradient(self._h, float(dt), lam if nl else np.zeros(1), out) == 0
+        return out[:nl]
 
     def update_contact_properties(self, dt):
         """The contacts' gamma from the last solve's multipliers (Viscous problems)."""

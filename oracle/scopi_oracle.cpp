@@ -1457,6 +1457,33 @@ int64_t oracle_compute_contacts(void* scene) {
     return (int64_t)s.contacts.size();
 }
 
+// Replace the contact list with given records (contact properties from the parameters, as a new
+// contact's): the optim slot's problem on another code's contacts, for checking its solution.
+int oracle_set_contacts(void* scene, int64_t nc, const int64_t* ij, const double* dij, const double* nij,
+                        const double* pi, const double* pj) {
+    if (!scene || nc < 0 || (nc > 0 && (!ij || !dij || !nij || !pi || !pj))) return -1;
+    Scene& s = *static_cast<Scene*>(scene);
+    s.contacts.assign((size_t)nc, Contact{});
+    for (int64_t c = 0; c < nc; ++c) {
+        Contact& ct = s.contacts[c];
+        ct.i = ij[2 * c];
+        ct.j = ij[2 * c + 1];
+        if (ct.i < 0 || ct.i >= s.n || ct.j < 0 || ct.j >= s.n) return -1;
+        ct.dij = dij[c];
+        for (int d = 0; d < 3; ++d) {
+            ct.nij[d] = nij[3 * c + d];
+            ct.pi[d] = pi[3 * c + d];
+            ct.pj[d] = pj[3 * c + d];
+        }
+        ct.mu = s.prm.mu;
+        ct.gamma = s.prm.gamma;
+        ct.gamma_min = s.prm.gamma_min;
+        ct.gamma_tol = s.prm.gamma_tol;
+    }
+    s.st.n_contacts = nc;
+    return 0;
+}
+
 int64_t oracle_n_contacts(void* scene) {
     if (!scene) return -1;
     return (int64_t) static_cast<Scene*>(scene)->contacts.size();
@@ -1503,6 +1530,18 @@ int oracle_solve(void* scene, double dt, double* lambda, double* u, double* w) {
     if (lambda) std::copy(s.lambda.begin(), s.lambda.end(), lambda);
     if (u) std::copy(s.u.begin(), s.u.end(), u);
     if (w) std::copy(s.w.begin(), s.w.end(), w);
+    return 0;
+}
+
+// MinProblem::gradient (minimization_problem.hpp:136-140) at a given lambda: the test of the KKT
+// conditions of another solver's multipliers on the same problem
+int oracle_gradient(void* scene, double dt, const double* lambda, double* out) {
+    if (!scene || !lambda || !out) return -1;
+    Scene& s = *static_cast<Scene*>(scene);
+    if (s.prm.problem != ORACLE_NOFRICTION && s.prm.problem != ORACLE_FRICTION) return -2;
+    if (s.contacts.empty()) return 0;
+    MinProblem mp(s, s.contacts, dt, nullptr);
+    mp.gradient(lambda, out);
     return 0;
 }
 

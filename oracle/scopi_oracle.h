@@ -106,6 +106,13 @@ int oracle_atmat(void* scene, const double* f, double* out); /* 3*nc -> 6*Na */
  * velocities taken as a-priori velocities.  lambda (size nc or 3nc), u (Na*3),
  * w (Na*3) may be NULL. */
 int oracle_solve(void* scene, double dt, double* lambda, double* u, double* w);
+/* The gradient Q lambda + c (+ S) of the optim slot's problem on the current contacts and a-priori
+ * velocities (NoFriction and Friction), at the given multipliers; both of oracle_lambda_size. */
+int oracle_gradient(void* scene, double dt, const double* lambda, double* out);
+/* Replace the contact list (nc records: ij as 2 per contact, dij, nij, pi, pj as 3 per contact);
+ * contact properties from the parameters. */
+int oracle_set_contacts(void* scene, int64_t nc, const int64_t* ij, const double* dij, const double* nij,
+                        const double* pi, const double* pj);
 /* OptimGradient::update_contact_properties after oracle_solve (Viscous problems: the contacts'
  * gamma from the last solve's multipliers). */
 int oracle_update_contact_properties(void* scene, double dt);

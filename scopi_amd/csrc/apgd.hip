@@ -1235,6 +1235,7 @@ void Solver::run_phase(BMatrix<T>& B, const Particles& P, const Contacts& C, int
     info.status = h_ctrl_->status == kStatusStalled ? 0 : h_ctrl_->status;
     info.converged = info.status == 0 ? 1 : 0;
     info.residual = h_ctrl_->res;
+    info.step = 1.0 / h_ctrl_->L_acc;
     final_slot_ = h_ctrl_->cur;
 }
 
@@ -1715,6 +1716,7 @@ void Solver::run_phase_scenes(BMatrix<T>& B, const Contacts& C, int64_t na, int6
         in.status = c.status == kStatusStalled ? 0 : c.status;
         in.converged = in.status == 0 ? 1 : 0;
         in.residual = c.res;
+        in.step = 1.0 / c.L_acc;
     }
 }
 
@@ -1739,7 +1741,10 @@ void Solver::solve_layout_scenes(const Particles& P, const Contacts& C, int64_t 
     if (fp32) {
         const double rtol32 = prm.fp32_tolerance > 0 ? 0.0 : 1e-5;
         const double tol32 = prm.fp32_tolerance > 0 ? prm.fp32_tolerance : prm.tolerance;
-        run_phase_scenes<float, LAY>(b32_, C, na, nc, prm, dt, tol32, rtol32, prm.max_ite, false, false, s, info);
+        if (lam0_)
+            hipLaunchKernelGGL((k_convert<double, float>), dim3(1024), dim3(kBlock), 0, s, lam0_, b32_.lam, NR * nc);
+        run_phase_scenes<float, LAY>(b32_, C, na, nc, prm, dt, tol32, rtol32, prm.max_ite, lam0_ != nullptr, false, s,
+                                     info);
         for (auto& in : info) in.iterations_fp32 = in.iterations;
         slots_to_device(d_slot);
         scene_tables(all, s);
@@ -1747,8 +1752,10 @@ void Solver::solve_layout_scenes(const Particles& P, const Contacts& C, int64_t 
                            (const float*)b32_.lam, NR * b32_.cap, b64_.lam, (const int32_t*)d_slot, map_);
         run_phase_scenes<double, LAY>(b64_, C, na, nc, prm, dt, prm.tolerance, 0.0, prm.max_ite, true, true, s, info);
     } else {
-        run_phase_scenes<double, LAY>(b64_, C, na, nc, prm, dt, prm.tolerance, 0.0, prm.max_ite, false, false, s,
-                                      info);
+        if (lam0_)
+            SCOPI_HIP_CHECK(hipMemcpyAsync(b64_.lam, lam0_, NR * nc * sizeof(double), hipMemcpyDeviceToDevice, s));
+        run_phase_scenes<double, LAY>(b64_, C, na, nc, prm, dt, prm.tolerance, 0.0, prm.max_ite, lam0_ != nullptr, false,
+                                      s, info);
     }
     // every scene's final lambda to slot 0, then the velocities (FP64)
     slots_to_device(d_slot);
@@ -1763,7 +1770,8 @@ void Solver::solve_layout_scenes(const Particles& P, const Contacts& C, int64_t 
 }
 
 int Solver::solve_scenes(const Particles& P, const Contacts& C, int64_t nc, const scopi_hip_params& prm, double dt,
-                         bool rot_free, hipStream_t s, const SceneSet& S, std::vector<scopi_hip_solve_info>& info) {
+                         bool rot_free, hipStream_t s, const SceneSet& S, std::vector<scopi_hip_solve_info>& info,
+                         const double* lam0) {
     const int64_t na = P.n - P.n_inactive;
     const int K = S.nscene;
     const bool friction = prm.problem == SCOPI_PROBLEM_FRICTION;
@@ -1772,6 +1780,11 @@ int Solver::solve_scenes(const Particles& P, const Contacts& C, int64_t nc, cons
     if (prm.method != SCOPI_METHOD_APGD && prm.method != SCOPI_METHOD_PGD) return SCOPI_HIP_ERR_UNSUPPORTED;
     if (!(prm.alpha > 0) || K < 1 || (int)S.alo.size() != K || (int)S.ahi.size() != K) return SCOPI_HIP_ERR_ARG;
     reserve(na, nc, fp32);
+    lam0_ = lam0;
+    struct Clear {
+        const double*& p;
+        ~Clear() { p = nullptr; }
+    } clear_lam0{lam0_};
     layout_ = friction ? kCone : ((rot_free && !prm.general_jacobian) ? kRotFree : kRigid);
     nr_ = friction ? 3 : 1;
     nc_ = nc;

@@ -128,8 +128,10 @@ class Solver {
     int rows_per_contact() const { return nr_; }
     // The same for a set of independent scenes packed in P / C (scene ranges in `S`);
     // one solve info per scene.  Afterwards every scene's final lambda is in slot 0.
+    // lam0 (NR rows per contact, FP64): every scene's first iterate (warm start), or null for 0.
     int solve_scenes(const Particles& P, const Contacts& C, int64_t nc, const scopi_hip_params& prm, double dt,
-                     bool rot_free, hipStream_t s, const SceneSet& S, std::vector<scopi_hip_solve_info>& info);
+                     bool rot_free, hipStream_t s, const SceneSet& S, std::vector<scopi_hip_solve_info>& info,
+                     const double* lam0 = nullptr);
     // lambda of the last solve (FP64) in the reference's order (last_rows() values).
     void download_lambda(double* host, hipStream_t s) const;
     // rows [NR lo, NR hi) of the last solve's lambda (FP64)

@@ -1342,6 +1342,7 @@ int scopi_hip_run(scopi_hip_ctx* ctx, const scopi_hip_params* params, double dt,
             acc.converged = acc.converged && one.converged;
             if (one.status != 0) acc.status = one.status;
             acc.residual = one.residual;
+            acc.step = one.step;
             acc.outer_iterations += one.outer_iterations;
             acc.fixed_point_error = one.fixed_point_error;
         }
@@ -1422,6 +1423,18 @@ struct scopi_hip_batch {
     std::vector<int64_t> removed;
     int64_t next_id = 1;
     bool solved = false;
+    // warm start: the last step's contacts, keyed (scene id, local i, local j), and their multipliers
+    // (hist_nr rows each); the current contacts' keys and first iterate; per scene (id, first
+    // obstacle, first active particle, obstacles) in packed order
+    uint64_t* hist_keys = nullptr;
+    uint64_t* cur_keys = nullptr;
+    int64_t key_cap = 0, hist_n = 0;
+    double* hist_lam = nullptr;
+    double* lam0 = nullptr;
+    int64_t lam_cap = 0;
+    int hist_nr = 0;
+    int64_t* d_tab = nullptr;
+    int64_t tab_cap = 0;
     bool has_se = false;       // any scene with superellipsoids: candidate search for the batch
     bool all_spheres = true;   // every active particle of the batch is a sphere
     std::string err;
@@ -1580,6 +1593,77 @@ void batch_pack(scopi_hip_batch* b) {
     }
 }
 
+// Warm start in a batch: contact c's key (scene id, local i, local j), 24 + 20 + 20 bits.  The packed
+// contacts are in scene order (ids increase along it) and each scene's in (i, j) order, with its
+// obstacles before its active particles as in the scene's own numbering, so the keys are sorted.
+constexpr int kKeyIdBits = 24, kKeyLocalBits = 20;
+__global__ void __launch_bounds__(kBlock) k_batch_keys(const int32_t* __restrict__ ci, const int32_t* __restrict__ cj,
+                                                      int64_t nc, const int64_t* __restrict__ crange, int K,
+                                                      const int64_t* __restrict__ tab, int64_t NI,
+                                                      uint64_t* __restrict__ keys) {
+    const int64_t c = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (c >= nc) return;
+    int lo = 0, hi = K;  // the last scene whose contacts start at or before c
+    while (hi - lo > 1) {
+        const int mid = (lo + hi) >> 1;
+        if (crange[2 * mid] <= c)
+            lo = mid;
+        else
+            hi = mid;
+    }
+    const int64_t* t = tab + 4 * lo;
+    auto local = [&](int64_t g) { return (uint64_t)(g < NI ? g - t[1] : t[3] + (g - t[2])); };
+    keys[c] = ((uint64_t)t[0] << (2 * kKeyLocalBits)) | (local(ci[c]) << kKeyLocalBits) | local(cj[c]);
+}
+
+// lam0 of the current contacts: the rows of the history's last equal key, 0 for a new contact
+__global__ void __launch_bounds__(kBlock) k_transfer_keys(const uint64_t* __restrict__ cur, int64_t nc,
+                                                         const uint64_t* __restrict__ keys,
+                                                         const double* __restrict__ hl, int64_t hn, int nr,
+                                                         double* __restrict__ lam0) {
+    const int64_t c = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (c >= nc) return;
+    const uint64_t key = cur[c];
+    int64_t lo = 0, hi = hn;  // upper bound of key
+    while (lo < hi) {
+        const int64_t mid = (lo + hi) >> 1;
+        if (keys[mid] <= key)
+            lo = mid + 1;
+        else
+            hi = mid;
+    }
+    const bool found = lo > 0 && keys[lo - 1] == key;
+    for (int d = 0; d < nr; ++d) lam0[c * nr + d] = found ? hl[(lo - 1) * nr + d] : 0.0;
+}
+
+void batch_warm_reserve(scopi_hip_batch* b, int64_t nc, int nr) {
+    if (nc > b->key_cap) {
+        for (uint64_t** p : {&b->hist_keys, &b->cur_keys}) {
+            uint64_t* q = nullptr;
+            SCOPI_HIP_CHECK(hipMalloc(&q, (nc + nc / 4 + 1024) * sizeof(uint64_t)));
+            if (*p && b->hist_n > 0 && p == &b->hist_keys)
+                SCOPI_HIP_CHECK(hipMemcpyAsync(q, *p, b->hist_n * sizeof(uint64_t), hipMemcpyDeviceToDevice, b->stream));
+            SCOPI_HIP_CHECK(hipStreamSynchronize(b->stream));
+            if (*p) SCOPI_HIP_CHECK(hipFree(*p));
+            *p = q;
+        }
+        b->key_cap = nc + nc / 4 + 1024;
+    }
+    if (nr * nc > b->lam_cap) {
+        for (double** p : {&b->hist_lam, &b->lam0}) {
+            double* q = nullptr;
+            SCOPI_HIP_CHECK(hipMalloc(&q, (nr * nc + nr * nc / 4 + 1024) * sizeof(double)));
+            if (*p && b->hist_n > 0 && p == &b->hist_lam)
+                SCOPI_HIP_CHECK(hipMemcpyAsync(q, *p, b->hist_nr * b->hist_n * sizeof(double), hipMemcpyDeviceToDevice,
+                                               b->stream));
+            SCOPI_HIP_CHECK(hipStreamSynchronize(b->stream));
+            if (*p) SCOPI_HIP_CHECK(hipFree(*p));
+            *p = q;
+        }
+        b->lam_cap = nr * nc + nr * nc / 4 + 1024;
+    }
+}
+
 scopi_hip_batch::Scene* batch_find(scopi_hip_batch* b, int64_t id) {
     for (auto& sc : b->scenes)
         if (sc.id == id) return &sc;
@@ -1592,8 +1676,7 @@ extern "C" {
 
 int scopi_hip_batch_create(const scopi_hip_params* params, double dt, scopi_hip_batch** out) {
     if (!out || !params_ok(params) || !(dt > 0) || params->vap == SCOPI_VAP_PROJECTION) return SCOPI_HIP_ERR_ARG;
-    if (params->problem == SCOPI_PROBLEM_FRICTION_FIXED_POINT || viscous_problem(params->problem) ||
-        params->warm_start)
+    if (params->problem == SCOPI_PROBLEM_FRICTION_FIXED_POINT || viscous_problem(params->problem))
         return SCOPI_HIP_ERR_UNSUPPORTED;
     *out = nullptr;
     auto* b = new scopi_hip_batch();
@@ -1620,6 +1703,8 @@ int scopi_hip_batch_destroy(scopi_hip_batch* b) {
     if (b->stream) (void)hipStreamSynchronize(b->stream);
     particles_free(b->P);
     contacts_free(b->C);
+    for (void* p : {(void*)b->hist_keys, (void*)b->cur_keys, (void*)b->hist_lam, (void*)b->lam0, (void*)b->d_tab})
+        if (p) (void)hipFree(p);
     if (b->stream) (void)hipStreamDestroy(b->stream);
     delete b;
     return SCOPI_HIP_OK;
@@ -1707,6 +1792,28 @@ int scopi_hip_batch_step(scopi_hip_batch* b, int64_t n_steps) {
             S.alo[k] = b->scenes[k].a_lo - P.n_inactive;
             S.ahi[k] = S.alo[k] + (b->scenes[k].n - b->scenes[k].n_inactive);
         }
+        const bool warm = b->prm.warm_start != 0;
+        const int nr = b->prm.problem == SCOPI_PROBLEM_FRICTION ? 3 : 1;
+        if (warm) {
+            // the scenes' table for the contact keys (packed order)
+            for (const auto& sc : b->scenes)
+                if (sc.n >= ((int64_t)1 << kKeyLocalBits) || sc.id >= ((int64_t)1 << kKeyIdBits))
+                    return bfail(b, SCOPI_HIP_ERR_UNSUPPORTED,
+                                 "batch_step: warm start needs scenes of fewer than 2^20 particles (ids below 2^24)");
+            if (K > b->tab_cap) {
+                if (b->d_tab) SCOPI_HIP_CHECK(hipFree(b->d_tab));
+                b->tab_cap = K + K / 2 + 16;
+                SCOPI_HIP_CHECK(hipMalloc(&b->d_tab, 4 * b->tab_cap * sizeof(int64_t)));
+            }
+            std::vector<int64_t> tab(4 * (size_t)K);
+            for (int k = 0; k < K; ++k) {
+                const auto& sc = b->scenes[k];
+                tab[4 * k] = sc.id, tab[4 * k + 1] = sc.ob_lo, tab[4 * k + 2] = sc.a_lo, tab[4 * k + 3] = sc.n_inactive;
+            }
+            SCOPI_HIP_CHECK(hipMemcpyAsync(b->d_tab, tab.data(), tab.size() * sizeof(int64_t), hipMemcpyHostToDevice, s));
+            SCOPI_HIP_CHECK(hipStreamSynchronize(s));
+            if (b->hist_nr != nr) b->hist_n = 0;
+        }
         std::vector<scopi_hip_solve_info> infos;
         for (int64_t it = 0; it < n_steps; ++it) {
             if (P.n_inactive > 0)
@@ -1716,8 +1823,32 @@ int scopi_hip_batch_step(scopi_hip_batch* b, int64_t n_steps) {
                 hipLaunchKernelGGL(k_vap, dim3(grid_for(na)), dim3(kBlock), 0, s, P, b->prm.vap, b->dt,
                                    (const double*)nullptr, (const double*)nullptr);
             S.d_crange = b->search.d_crange();
-            int rc = b->solver.solve_scenes(P, b->C, b->nc, b->prm, b->dt, b->all_spheres, s, S, infos);
+            const double* lam0 = nullptr;
+            if (warm && b->nc > 0) {
+                batch_warm_reserve(b, b->nc, nr);
+                hipLaunchKernelGGL(k_batch_keys, dim3(grid_for(b->nc)), dim3(kBlock), 0, s, (const int32_t*)b->C.i,
+                                   (const int32_t*)b->C.j, b->nc, S.d_crange, K, (const int64_t*)b->d_tab,
+                                   P.n_inactive, b->cur_keys);
+                if (b->hist_n > 0) {
+                    hipLaunchKernelGGL(k_transfer_keys, dim3(grid_for(b->nc)), dim3(kBlock), 0, s,
+                                       (const uint64_t*)b->cur_keys, b->nc, (const uint64_t*)b->hist_keys,
+                                       (const double*)b->hist_lam, b->hist_n, nr, b->lam0);
+                    lam0 = b->lam0;
+                }
+                SCOPI_HIP_CHECK(hipGetLastError());
+            }
+            int rc = b->solver.solve_scenes(P, b->C, b->nc, b->prm, b->dt, b->all_spheres, s, S, infos, lam0);
             if (rc != SCOPI_HIP_OK) return bfail(b, rc, "batch_step: solve failed");
+            if (warm) {
+                // this step's contacts and multipliers become the history
+                if (b->nc > 0) {
+                    std::swap(b->hist_keys, b->cur_keys);
+                    SCOPI_HIP_CHECK(hipMemcpyAsync(b->hist_lam, b->solver.final_lambda(), nr * b->nc * sizeof(double),
+                                                   hipMemcpyDeviceToDevice, s));
+                }
+                b->hist_n = b->nc;
+                b->hist_nr = nr;
+            }
             if (na > 0)
                 hipLaunchKernelGGL(k_update_move, dim3(grid_for(na)), dim3(kBlock), 0, s, P, b->solver.u(),
                                    b->solver.w(), b->solver.pcap(), b->dt);

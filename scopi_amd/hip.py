@@ -55,6 +55,7 @@ class SolveInfo(C.Structure):
         ("n_contacts", C.c_int64), ("rows", C.c_int64), ("iterations", C.c_int64), ("iterations_fp32", C.c_int64),
         ("backtracks", C.c_int64), ("q_applies", C.c_int64), ("converged", C.c_int32), ("status", C.c_int32),
         ("residual", C.c_double), ("outer_iterations", C.c_int64), ("fixed_point_error", C.c_double),
+        ("step", C.c_double),
     ]
 
     def as_dict(self):

@@ -223,15 +223,18 @@ def served_scene(r: Request):
     return scenes.superellipsoids_3d(nx=side, ny=side, nz=max(1, int(round(r.n / (side * side)))), seed=r.seed)
 
 
-def hip_kinds(dt=1e-3, sphere_precision=None):
+def hip_kinds(dt=1e-3, sphere_precision=None, warm_start=True):
     """Batch factories of the two request kinds of the served workload: sphere scenes
     (NoFriction, vap_fpd, dmax 0.02, kd_tree_radius 0.042^2) and superellipsoid scenes
-    (Friction mu 0.3, dmax 0.02, kd_tree_radius (2 sqrt(3) 0.015 + 0.02)^2), APGD tol 1e-6."""
+    (Friction mu 0.3, dmax 0.02, kd_tree_radius (2 sqrt(3) 0.015 + 0.02)^2), APGD tol 1e-6.
+    With warm_start, each scene's solves start from its own last step's multipliers, kept on the
+    device with its state."""
     from scopi_amd import hip, scenes
     c3 = scenes.CONFIG3
-    sp = hip.params(vap=hip.VAP_FPD, dmax=0.02, kd_tree_radius=0.042 ** 2, tolerance=1e-6,
+    ws = 1 if warm_start else 0
+    sp = hip.params(vap=hip.VAP_FPD, dmax=0.02, kd_tree_radius=0.042 ** 2, tolerance=1e-6, warm_start=ws,
                     precision=hip.FP64 if sphere_precision is None else sphere_precision)
-    se = hip.params(problem=hip.FRICTION, mu=c3["mu"], vap=hip.VAP_FPD, dmax=c3["dmax"],
+    se = hip.params(problem=hip.FRICTION, mu=c3["mu"], vap=hip.VAP_FPD, dmax=c3["dmax"], warm_start=ws,
                     kd_tree_radius=c3["kd_tree_radius"], tolerance=c3["tolerance"])
     prm = {"spheres": sp, "superellipsoids": se}
     kinds = {k: (lambda p=p: hip.HipBatch(p, dt)) for k, p in prm.items()}

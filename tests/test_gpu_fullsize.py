@@ -1,0 +1,211 @@
+"""The workloads at their full sizes on the device, each held to a property that does not need the
+oracle to redo the whole workload:
+
+* 131,072 spheres (64 x 64 x 32 lattice, 5 planes; NoFriction, tol 1e-6), one step after three: the
+  multipliers of the FP64 solve and of the FP32 + FP64-refinement solve satisfy the KKT conditions of
+  the step's problem, with the gradient g = Q lambda + c formed by the oracle on the device's
+  contacts and a-priori velocities: lambda >= 0 exactly, g >= -1e-9, |g| <= 1e-8 where lambda > 0,
+  |lambda g| <= 1e-6 (the oracle's own solves of this recipe at 8 x 8 x 4 to 32 x 32 x 8 give
+  -2.3e-12, 6.1e-11 and 2.3e-9), and the residual that the stopping test bounds,
+  |lambda - max(lambda - a g, 0)|_2 with a = 1/L of the last accepted iteration, within 10 tol;
+  and the two solves' velocities agree with a tight FP64 solve (tol 1e-9) to max(10 x the FP64
+  solve's own distance to it, 1e-6);
+* 32,768 superellipsoids (Friction mu 0.3, tol 1e-6), one step after one: contact pairs sorted and
+  unique, every dij < dmax, lambda in the Coulomb cone (violation <= 1e-9 max |lambda|), g in the
+  dual cone (<= 1e-5 max |g|), and the residual that APGD's stopping test bounds,
+  |lambda - P_cone(lambda - a g)|_2 with a = 1/L of the last accepted iteration, within 10 tol;
+* the 512-scene batch (log-uniform 256 .. 32,768 spheres, FP32 + FP64 refinement), 5 steps: 8 of its
+  scenes, the smallest and the largest among them, bitwise equal to the same scenes stepped alone;
+* 1,024 disks, all 200 steps, against the oracle's trajectory (tests/golden/c1_trajectory.npz), whose
+  own spread under a 4e-16 change of the line search's acceptance test sets the bars: single disks
+  while that spread is below r_min / 10, the centre of mass at every 40th step, and no overlap or
+  penetration beyond 1e-8.
+"""
+import dataclasses
+import os
+import sys
+
+import numpy as np
+import pytest
+
+from scopi_amd import hip, scenes
+
+pytestmark = pytest.mark.gpu
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def at_state(sc, st):
+    return dataclasses.replace(sc, pos=st["pos"], q=st["q"], v=st["v"], omega=st["omega"])
+
+
+def test_c2_one_step_kkt_and_precisions(oracle_lib):
+    cfg = scenes.CONFIG2
+    sc = scenes.spheres_3d()
+    dt = cfg["dt"]
+    base = dict(vap=hip.VAP_FPD, dmax=cfg["dmax"], kd_tree_radius=cfg["kd_tree_radius"], max_ite=100000)
+    say("[fullsize C2] 131072 spheres: 3 steps, then one step's solve three ways")
+    g = hip.HipScene(sc, capacity_contacts=20 * sc.n)
+    g.run(hip.params(tolerance=cfg["tolerance"], **base), dt, 3)
+    st = g.state()
+    sol = {}
+    for name, tol, prec in (("fp64", 1e-6, hip.FP64), ("tight", 1e-9, hip.FP64), ("fp32", 1e-6, hip.FP32_REFINE)):
+        prm = hip.params(tolerance=tol, precision=prec, **base)
+        g.upload(at_state(sc, st))
+        nc = g.compute_contacts(prm)
+        g.set_a_priori_velocity(prm, dt)
+        info = g.solve(prm, dt)
+        sol[name] = g.solution()
+        sol[name]["info"] = info
+        say(f"  {name}: nc {nc}, iterations {info['iterations']} (fp32 {info['iterations_fp32']}), "
+            f"residual {info['residual']:.2e}")
+    ct = g.contacts()
+    g.close()
+    # the step's problem on the oracle, on the device's contacts (the contact set is bitwise the
+    # oracle's: test_gpu_parity.py::test_contact_set_full_config2)
+    o = oracle_lib.Oracle(at_state(sc, st), vap=1, dmax=cfg["dmax"], kd_tree_radius=cfg["kd_tree_radius"],
+                          tolerance=1e-6)
+    o.set_contacts(ct)
+    o.set_a_priori_velocity(dt)
+    for name in ("fp64", "fp32"):
+        lam = sol[name]["lam"]
+        assert lam.size == nc and np.all(lam >= 0.0)
+        gr = o.gradient(dt, lam)
+        act = lam > 0
+        say(f"  {name}: active {int(act.sum())}, min g {gr.min():.3e}, max |g| active "
+            f"{np.abs(gr[act]).max() if act.any() else 0:.3e}, max |lam g| {np.abs(lam * gr).max():.3e}")
+        nat = float(np.linalg.norm(lam - np.maximum(lam - sol[name]["info"]["step"] * gr, 0.0)))
+        say(f"    step {sol[name]['info']['step']:.3e}: |lam - max(lam - step g, 0)| {nat:.3e}")
+        assert gr.min() >= -1e-9
+        assert (not act.any()) or np.abs(gr[act]).max() <= 1e-8
+        assert np.abs(lam * gr).max() <= 1e-6
+        assert nat <= 10 * 1e-6
+    reach = float(np.abs(sol["fp64"]["u"] - sol["tight"]["u"]).max())
+    d32 = float(np.abs(sol["fp32"]["u"] - sol["tight"]["u"]).max())
+    say(f"  max|u_fp64 - u_tight| {reach:.3e}, max|u_fp32 - u_tight| {d32:.3e}")
+    assert d32 <= max(10 * reach, 1e-6)
+
+
+def cone_projection(v, n, mu):
+    """Projection of 3-vectors v on the Coulomb cones {|x_t| <= mu x_n} of normals n."""
+    vn = (v * n).sum(1)
+    vt = v - vn[:, None] * n
+    nt = np.linalg.norm(vt, axis=1)
+    out = v.copy()
+    zero = mu * nt <= -vn
+    bd = ~(zero | (nt <= mu * vn))
+    a = (vn[bd] + mu * nt[bd]) / (1 + mu * mu)
+    out[bd] = a[:, None] * (n[bd] + mu * vt[bd] / nt[bd][:, None])
+    out[zero] = 0.0
+    return out
+
+
+def test_c3_one_step_invariants(oracle_lib):
+    cfg = scenes.CONFIG3
+    sc = scenes.superellipsoids_3d()
+    dt, mu = cfg["dt"], cfg["mu"]
+    prm = hip.params(vap=hip.VAP_FPD, problem=hip.FRICTION, mu=mu, dmax=cfg["dmax"],
+                     kd_tree_radius=cfg["kd_tree_radius"], tolerance=cfg["tolerance"], max_ite=100000)
+    say("[fullsize C3] 32768 superellipsoids: one step, then the next step's solve")
+    g = hip.HipScene(sc)
+    g.run(prm, dt, 1)
+    st = g.state()
+    nc = g.compute_contacts(prm)
+    g.set_a_priori_velocity(prm, dt)
+    info = g.solve(prm, dt)
+    sol = g.solution()
+    ct = g.contacts()
+    g.close()
+    say(f"  nc {nc}, iterations {info['iterations']}")
+    ij = ct["ij"]
+    key = ij[:, 0] * (sc.n + 1) + ij[:, 1]
+    assert np.all(ij[:, 0] < ij[:, 1]) and np.all(np.diff(key) > 0)
+    assert np.all(ct["dij"] < cfg["dmax"])
+    lam = sol["lam"].reshape(nc, 3)
+    n = ct["nij"]
+    ln = (lam * n).sum(1)
+    lt = np.linalg.norm(lam - ln[:, None] * n, axis=1)
+    o = oracle_lib.Oracle(at_state(sc, st), vap=1, problem=oracle_lib.FRICTION, mu=mu, dmax=cfg["dmax"],
+                          kd_tree_radius=cfg["kd_tree_radius"], tolerance=cfg["tolerance"])
+    o.set_contacts(ct)
+    o.set_a_priori_velocity(dt)
+    gr = o.gradient(dt, sol["lam"]).reshape(nc, 3)
+    gn = (gr * n).sum(1)
+    gt = np.linalg.norm(gr - gn[:, None] * n, axis=1)
+    ls, gs = float(np.abs(lam).max()), float(np.abs(gr).max())
+    cone = float(np.max(lt - mu * ln))
+    dual = float(np.max(mu * gt - gn))
+    nat = float(np.linalg.norm(lam - cone_projection(lam - info["step"] * gr, n, mu)))
+    say(f"  active {int((ln > 0).sum())}: scales lam {ls:.3e} g {gs:.3e}; cone {cone:.3e}, dual cone {dual:.3e}, "
+        f"step {info['step']:.3e}, |lam - P(lam - step g)| {nat:.3e}")
+    assert cone <= 1e-9 * ls
+    assert dual <= 1e-5 * gs
+    assert nat <= 10 * cfg["tolerance"]
+
+
+def test_c4_batch_scenes_bitwise_alone():
+    cfg = scenes.CONFIG4
+    scs = scenes.batched_sphere_scenes(cfg["n_scenes"], cfg["nmin"], cfg["nmax"])
+    prm = hip.params(vap=hip.VAP_FPD, dmax=cfg["dmax"], kd_tree_radius=cfg["kd_tree_radius"],
+                     tolerance=cfg["tolerance"], precision=hip.FP32_REFINE)
+    dt, steps = cfg["dt"], 5
+    say(f"[fullsize C4] {len(scs)} scenes, {sum(s.n for s in scs)} particles, {steps} steps")
+    b = hip.HipBatch(prm, dt)
+    ids = [b.add(s) for s in scs]
+    b.step(steps)
+    sizes = np.array([s.n for s in scs])
+    rng = np.random.default_rng(5)
+    pick = sorted(set([int(np.argmin(sizes)), int(np.argmax(sizes))] + list(rng.choice(len(scs), 6, replace=False))))
+    got = {k: b.state(ids[k]) for k in pick}
+    b.close()
+    for k in pick:
+        g = hip.HipScene(scs[k])
+        g.run(prm, dt, steps)
+        alone = g.state()
+        g.close()
+        say(f"  scene {k} ({scs[k].n} particles)")
+        for q in ("pos", "q", "v", "omega"):
+            np.testing.assert_array_equal(got[k][q], alone[q])
+
+
+def test_c1_200_steps_against_the_oracle():
+    """The oracle's own 200-step trajectory is chaotic: a 4e-16 change of the line search's acceptance
+    test moves its disks by 2e-2 at step 80 and 0.15 at step 200 (the radii are 0.008 .. 0.012), so
+    single disks are compared only while that spread stays below r_min / 10 (step 40: 4e-7).  At
+    every 40th step: the centre of mass within 3 x the oracle's own spread of it (3.5e-4 at most)
+    plus 1e-6; no pair of disks overlapping, and no disk below the plane, by more than 1e-8 (the
+    oracle's trajectory: 3.4e-10)."""
+    cfg = scenes.CONFIG1
+    ref = np.load(os.path.join(HERE, "golden", "c1_trajectory.npz"))
+    sc = scenes.disks_2d()
+    prm = hip.params(vap=hip.VAP_FPD, dmax=cfg["dmax"], kd_tree_radius=cfg["kd_tree_radius"],
+                     tolerance=cfg["tolerance"])
+    say("[fullsize C1] 1024 disks, 200 steps")
+    g = hip.HipScene(sc)
+    ni = sc.n_inactive
+    rad = sc.size[ni:, 0]
+    rmin = float(rad.min())
+    for s, step in enumerate(ref["steps"]):
+        g.run(prm, cfg["dt"], int(step), int(step) - 40)
+        pos = g.state()["pos"][:, :2]
+        spread = max(float(np.abs(ref[k][s] - ref["ref"][s]).max()) for k in ("plus", "minus"))
+        com = pos[ni:].mean(0)
+        com_ref = ref["ref"][s][ni:].mean(0)
+        com_spread = max(float(np.abs(ref[k][s][ni:].mean(0) - com_ref).max()) for k in ("plus", "minus"))
+        p = pos[ni:]
+        gap = np.sqrt(((p[:, None, :] - p[None, :, :]) ** 2).sum(-1)) - rad[:, None] - rad[None, :]
+        np.fill_diagonal(gap, np.inf)
+        d = float(np.abs(pos - ref["ref"][s]).max())
+        dcom = float(np.abs(com - com_ref).max())
+        say(f"  step {int(step)}: max|dpos| {d:.3e} (oracle spread {spread:.3e}), |dcom| {dcom:.3e} (oracle spread "
+            f"{com_spread:.3e}), min gap {gap.min():.3e}, min y - r {(p[:, 1] - rad).min():.3e}")
+        if spread < 0.1 * rmin:
+            assert d <= max(0.4 * rmin, 3 * spread)
+        assert dcom <= 3 * com_spread + 1e-6
+        assert gap.min() >= -1e-8 and (p[:, 1] - rad).min() >= -1e-8
+    g.close()

@@ -12,7 +12,9 @@ path and is held to the bars of the cold trajectory tests:
   solve stops at its first iteration, as the oracle's does, while the cold solves take hundreds
   (FP32 + FP64: under a quarter of the cold iterations);
 * the problems whose rows change between steps refuse it (FrictionFixedPoint, Viscous,
-  ViscousFriction), and so do batches.
+  ViscousFriction);
+* in a batch, every scene starts from its own history: its trajectory is bitwise the one it has
+  alone with warm start, across scenes added and removed between steps.
 """
 import sys
 
@@ -99,5 +101,44 @@ def test_warm_start_refused(problem):
     with pytest.raises(hip.ScopiHipError):
         g.run(hip.params(problem=problem, warm_start=1, dmax=0.02, kd_tree_radius=0.042 ** 2), 1e-3, 1)
     g.close()
-    with pytest.raises(hip.ScopiHipError):
-        hip.HipBatch(hip.params(warm_start=1), 1e-3)
+
+
+@pytest.mark.parametrize("precision", [hip.FP64, hip.FP32_REFINE], ids=["fp64", "fp32_refine"])
+@pytest.mark.parametrize("case", ["nofriction", "friction"])
+def test_warm_start_batch_bitwise_alone(case, precision):
+    """A batch with warm start: every scene's trajectory bitwise equal to the same scene run alone
+    with warm start (its history is its own, matched by its own (i, j)), across a scene added and one
+    removed between steps; and the resting stack's solves after its first take one iteration
+    (FP64) in the batch as alone."""
+    stack = scenes.spheres_3d(4, 4, 4, spacing=0.02, jitter=0.0, rmin=0.01, rmax=0.01)
+    moving = scenes.spheres_3d(8, 8, 4, velocity_scale=1.0)
+    late = scenes.spheres_3d(6, 6, 3, velocity_scale=0.5, seed=11)
+    kw = dict(vap=hip.VAP_FPD, dmax=0.005 if case == "nofriction" else 0.006, kd_tree_radius=0.042 ** 2,
+              tolerance=1e-8, max_ite=100000, warm_start=1, precision=precision,
+              problem=hip.FRICTION if case == "friction" else hip.NOFRICTION,
+              **({"mu": 0.3} if case == "friction" else {}))
+    prm = hip.params(**kw)
+    say(f"[warm start] batch {case} precision={precision}")
+    b = hip.HipBatch(prm, 1e-3)
+    ids = {"stack": b.add(stack), "moving": b.add(moving)}
+    b.step(3)
+    its = [b.info(ids["stack"])["iterations"]]
+    ids["late"] = b.add(late)
+    b.step(2)
+    its.append(b.info(ids["stack"])["iterations"])
+    got_moving = b.state(ids["moving"])
+    b.remove(ids["moving"])
+    b.step(2)
+    its.append(b.info(ids["stack"])["iterations"])
+    got = {"stack": b.state(ids["stack"]), "late": b.state(ids["late"]), "moving": got_moving}
+    b.close()
+    for name, sc, steps in (("stack", stack, 7), ("late", late, 4), ("moving", moving, 5)):
+        g = hip.HipScene(sc)
+        g.run(prm, 1e-3, steps)
+        alone = g.state()
+        g.close()
+        for q in ("pos", "q", "v", "omega"):
+            np.testing.assert_array_equal(got[name][q], alone[q], err_msg=f"{name} {q}")
+    say(f"  stack iterations of the last step of each batch call: {its}")
+    if precision == hip.FP64:
+        assert its == [1, 1, 1], its
