@@ -216,13 +216,39 @@ def bench_batched(args):
     from scopi_amd import hip, scenes
     cfg = scenes.CONFIG4
     precision = hip.FP64 if args.precision == "fp64" else hip.FP32_REFINE
+    nmin = args.nmin if args.nmin else cfg["nmin"]
+    nmax = args.nmax if args.nmax else cfg["nmax"]
     log(f"[bench] building {args.scenes} scenes")
-    scs = scenes.batched_sphere_scenes(args.scenes, cfg["nmin"], cfg["nmax"])
+    scs = scenes.batched_sphere_scenes(args.scenes, nmin, nmax)
     prm = hip.params(vap=hip.VAP_FPD, dmax=cfg["dmax"], kd_tree_radius=cfg["kd_tree_radius"],
                      tolerance=cfg["tolerance"], precision=precision)
     dt = cfg["dt"]
     torch.cuda.init()
+    tuned = None
+    if not args.no_tune:
+        from scopi_amd import tune
+
+        def make_batch():
+            nb = hip.HipBatch(prm, dt)
+            for sc in scs:
+                nb.add(sc)
+            return nb
+        log("[bench] autotuning the batch schedule")
+        tuned = tune.tune_batch(make_batch, steps=2, log=log)
+        log(f"[bench] schedule: {tuned['schedule']}")
     b = hip.HipBatch(prm, dt)
+    if tuned:
+        tune.apply_batch_schedule(b, tuned["schedule"])
+    if args.queue_max_contacts is not None:
+        b.set_tuning(hip.TUNE_QUEUE_MAX_CONTACTS, args.queue_max_contacts)
+    if args.queue_threads is not None:
+        b.set_tuning(hip.TUNE_QUEUE_THREADS, args.queue_threads)
+    if args.queue_blocks is not None:
+        b.set_tuning(hip.TUNE_QUEUE_BLOCKS, args.queue_blocks)
+    if args.row_iters is not None:
+        b.set_tuning(hip.TUNE_ROW_ITERS, args.row_iters)
+    if args.bt_group is not None:
+        b.set_tuning(hip.TUNE_BT_GROUP, args.bt_group)
     ids = [b.add(sc) for sc in scs]
     log(f"[bench] warmup {args.warmup} steps")
     b.step(args.warmup)
@@ -255,15 +281,16 @@ def bench_batched(args):
                "sample": f"all {len(scs)} scenes, one step each after {args.warmup} warm-up steps"}
     value = len(scs) * args.steps / elapsed
     result = {
-        "metric": f"scene time-steps/s, {len(scs)} independent sphere scenes (log-uniform {cfg['nmin']}.."
-                  f"{cfg['nmax']} particles), APGD tol 1e-6",
+        "metric": f"scene time-steps/s, {len(scs)} independent sphere scenes (log-uniform {nmin}.."
+                  f"{nmax} particles), APGD tol 1e-6",
         "value": value, "unit": "scene-steps/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
         "ms_per_step": 1e3 * elapsed / args.steps, "higher_is_better": True, "scaling": "weak",
         "vs_baseline": None, "dtype": "f64" if precision == hip.FP64 else "f32+f64", "data": "synthetic",
-        "config": {"workload": f"{len(scs)} independent sphere scenes, log-uniform {cfg['nmin']}..{cfg['nmax']} "
+        "config": {"workload": f"{len(scs)} independent sphere scenes, log-uniform {nmin}..{nmax} "
                                "particles each (131k-sphere recipe scaled), 5 planes each, NoFriction, dt 1e-3, "
                                "dmax 0.02, kd_tree_radius 0.042^2", "precision": args.precision,
-                   "particles": size["particles"]},
+                   "particles": size["particles"], "queue_max_contacts": args.queue_max_contacts,
+                   "queue_threads": args.queue_threads},
         "contacts_last_step": contacts,
         "apgd_iterations_last_step": {"mean": sum(i["iterations"] for i in last) / len(last),
                                       "max": max(i["iterations"] for i in last)},
@@ -271,6 +298,8 @@ def bench_batched(args):
         "all_converged": all(i["converged"] == 1 for i in last),
         "one_at_a_time": one,
         "speedup_vs_one_at_a_time": (value / one["scene_steps_per_s"]) if one else None,
+        "schedule": tuned["schedule"] if tuned else None,
+        "autotuner": ({"candidates": tuned["table"], "timed_steps_per_candidate": 2} if tuned else None),
     }
     print(json.dumps(result), flush=True)
 
@@ -370,6 +399,16 @@ def main():
                          "Friction mu 0.3; batched: independent sphere scenes advanced together (secondary lines)")
     ap.add_argument("--scenes", type=int, default=512, help="batched: number of scenes")
     ap.add_argument("--one-at-a-time", action="store_true", help="batched / served: also step every scene alone")
+    ap.add_argument("--queue-max-contacts", type=int, default=None,
+                    help="batched: scenes of at most this many contacts go to the per-scene work-queue kernel")
+    ap.add_argument("--queue-threads", type=int, default=None, help="batched: work-queue workgroup size")
+    ap.add_argument("--queue-blocks", type=int, default=None, help="batched: work-queue workgroups")
+    ap.add_argument("--row-iters", type=int, default=None, help="batched: loop iterations per lane of k_trial")
+    ap.add_argument("--bt-group", type=int, default=None, help="batched: lanes per particle of k_bt")
+    ap.add_argument("--nmin", type=int, default=0, help="batched: smallest scene (default 256 particles)")
+    ap.add_argument("--nmax", type=int, default=0, help="batched: largest scene (default 32,768 particles)")
+    ap.add_argument("--no-tune", action="store_true",
+                    help="batched: skip the autotuner (scopi_amd.tune.tune_batch) and run the library's defaults")
     ap.add_argument("--served-cold", action="store_true",
                     help="served: start every solve from lambda = 0 instead of the scene's last multipliers")
     ap.add_argument("--requests", type=int, default=24, help="served: number of requests")

@@ -276,8 +276,27 @@ int scopi_hip_newton_probe(scopi_hip_ctx* ctx, int32_t kind, const double* in, d
  *   SCOPI_TUNE_BT_GROUP: lanes sharing one particle in the B^T gather (1, 2, 4, 8 or 16, default 8);
  *   SCOPI_TUNE_GRAPH_ROUNDS: 0 (default) launches the projected-gradient rounds one by one; an odd
  *     value R captures R rounds, the control step and the state's copy to the host in a hipGraph
- *     once per solve phase and replays it until the loop is done (same kernels, same results). */
-enum { SCOPI_TUNE_REDUCE_BLOCKS = 1, SCOPI_TUNE_BT_GROUP = 2, SCOPI_TUNE_GRAPH_ROUNDS = 3 };
+ *     once per solve phase and replays it until the loop is done (same kernels, same results);
+ *   SCOPI_TUNE_QUEUE_MAX_CONTACTS (batches): scenes of at most this many contacts are solved by the
+ *     per-scene work-queue kernel, one workgroup running a whole scene's solve in one launch
+ *     (0: none, every scene takes the two-launch rounds); a scene's results then depend only on
+ *     the scene and SCOPI_TUNE_QUEUE_THREADS;
+ *   SCOPI_TUNE_QUEUE_BLOCKS: workgroups of the work-queue kernel (1..65536, default 1024);
+ *   SCOPI_TUNE_QUEUE_THREADS: threads per workgroup of the work-queue kernel (256, 512 or 1024,
+ *     default 256);
+ *   SCOPI_TUNE_ROW_ITERS: loop iterations per lane of the per-contact kernels below the block cap
+ *     (1..64, default 2): a scene of nc contacts gets about nc / (ROW_ITERS x contacts per block
+ *     iteration) blocks, so larger values mean fewer, fuller blocks (and fewer per-block control
+ *     steps, which matters for many small scenes). */
+enum {
+    SCOPI_TUNE_REDUCE_BLOCKS = 1,
+    SCOPI_TUNE_BT_GROUP = 2,
+    SCOPI_TUNE_GRAPH_ROUNDS = 3,
+    SCOPI_TUNE_QUEUE_MAX_CONTACTS = 4,
+    SCOPI_TUNE_QUEUE_BLOCKS = 5,
+    SCOPI_TUNE_QUEUE_THREADS = 6,
+    SCOPI_TUNE_ROW_ITERS = 7
+};
 int scopi_hip_set_tuning(scopi_hip_ctx* ctx, int32_t key, int32_t value);
 /* Kernel timing of the projected-gradient loop since scopi_hip_set_kernel_timing(ctx, 1): out[6] =
  * {ms_bt, n_bt, ms_row, n_row, ms_trial, n_trial}, total milliseconds (HIP events of each dispatch)
@@ -314,6 +333,10 @@ int scopi_hip_batch_add_scene(scopi_hip_batch* b, int64_t n, int64_t n_inactive,
 int scopi_hip_batch_remove_scene(scopi_hip_batch* b, int64_t scene_id);
 /* n_steps time steps of every scene of the batch. */
 int scopi_hip_batch_step(scopi_hip_batch* b, int64_t n_steps);
+/* Launch shapes of the batch's projected-gradient kernels (keys as scopi_hip_set_tuning, except
+ * SCOPI_TUNE_GRAPH_ROUNDS), taking effect at the next step: the hook through which the autotuner
+ * (scopi_amd.tune) sets a batch's schedule.  No counterpart in the reference. */
+int scopi_hip_batch_set_tuning(scopi_hip_batch* b, int32_t key, int32_t value);
 /* Scenes, particles and contacts (last step) of the batch (any pointer may be NULL). */
 int scopi_hip_batch_size(scopi_hip_batch* b, int64_t* n_scenes, int64_t* n_particles, int64_t* n_contacts);
 /* One scene's state, in its own particle order (as scopi_hip_download_state). */

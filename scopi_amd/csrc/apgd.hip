@@ -40,14 +40,15 @@ constexpr int kRedBlocksMax = 2048;
 constexpr int kPartialStride = 16;  // doubles of partial_trial_ per block of capacity
 __host__ __device__ inline int64_t trial_pstride(const SceneMap& M) { return M.pstride ? M.pstride : kRedBlocksMax; }
 // Blocks of the per-contact kernel for nc contacts.  A lane takes `per_iter` contacts per loop
-// iteration (two in the rotation-free layout, one in the others) and k iterations, k = 2 unless
-// `cap` blocks of two iterations cannot hold the contacts; the grid is then the fewest blocks that
-// hold nc in k iterations, so that every block runs the same number of iterations (a block with one
-// iteration more than the others would hold up the whole launch).  The same rule sizes a scene's
-// share of a batch's grid, so a scene solved in a batch runs the one-scene arithmetic exactly.
-inline int64_t row_blocks(int64_t nc, int cap, int layout) {
+// iteration (two in the rotation-free layout, one in the others) and k iterations, k = kmin (2 by
+// default) unless `cap` blocks of kmin iterations cannot hold the contacts; the grid is then the
+// fewest blocks that hold nc in k iterations, so that every block runs the same number of iterations
+// (a block with one iteration more than the others would hold up the whole launch).  The same rule
+// sizes a scene's share of a batch's grid, so a scene solved in a batch runs the one-scene
+// arithmetic exactly (with the same kmin).
+inline int64_t row_blocks(int64_t nc, int cap, int layout, int kmin = 2) {
     const int64_t per_iter = (layout == kRotFree ? 2 : 1) * (int64_t)kBlock;
-    const int64_t k = std::max<int64_t>(2, (nc + per_iter * cap - 1) / (per_iter * cap));
+    const int64_t k = std::max<int64_t>(kmin, (nc + per_iter * cap - 1) / (per_iter * cap));
     return std::min<int64_t>(std::max<int64_t>((nc + per_iter * k - 1) / (per_iter * k), 1), cap);
 }
 // Blocks of k_bt for na particles at G lanes per particle, at most kBtBlocksMax (grid-stride
@@ -425,6 +426,61 @@ __device__ inline SceneBlock scene_block(const SceneMap& M, int64_t nc) {
     return b;
 }
 
+// B^T lambda over the entries [e0, e1) of one particle of the particle-major copy, lane `sub` of a
+// group of G taking entries sub, sub + G, ... (acc holds the lane's share; the group sums the shares).
+template <class T, int LAY, int G>
+__device__ inline void bt_gather(const BMatrix<T>& B, const T* __restrict__ lam, const int32_t* __restrict__ tc,
+                                 int e0, int e1, int sub, T (&acc)[lay<LAY>::NW]) {
+    constexpr int NT = lay<LAY>::NT;
+    // two entries per iteration, every load issued before the first use (the same
+    // accumulation order as one entry at a time)
+    for (int e = e0 + sub; e < e1; e += 2 * G) {
+        const int eb = e + G;
+        const bool hb = eb < e1;
+        const int32_t csa = tc[e];
+        const int32_t csb = hb ? tc[eb] : csa;
+        const int64_t ca = csa < 0 ? ~csa : csa, cb = csb < 0 ? ~csb : csb;
+        const T* tva = B.tcoef + (int64_t)e * NT;
+        const T* tvb = B.tcoef + (int64_t)(hb ? eb : e) * NT;
+        T ta[NT], tb[NT];
+#pragma unroll
+        for (int k = 0; k < NT; ++k) ta[k] = tva[k], tb[k] = tvb[k];
+        if (LAY == kRotFree || LAY == kRigid) {
+            const T la = lam[ca], lb = lam[cb];
+#pragma unroll
+            for (int k = 0; k < NT; ++k) acc[k] += la * ta[k];
+            if (hb)
+#pragma unroll
+                for (int k = 0; k < NT; ++k) acc[k] += lb * tb[k];
+        } else {
+            T a0, a1, a2, b0, b1, b2;
+            if (LAY == kVisc) {
+                visc_global<T>(B, lam, ca, a0, a1, a2);
+                visc_global<T>(B, lam, cb, b0, b1, b2);
+            } else {
+                a0 = lam[ca * 3], a1 = lam[ca * 3 + 1], a2 = lam[ca * 3 + 2];
+                b0 = lam[cb * 3], b1 = lam[cb * 3 + 1], b2 = lam[cb * 3 + 2];
+            }
+            if (csa < 0) {
+                acc[0] -= a0, acc[1] -= a1, acc[2] -= a2;
+            } else {
+                acc[0] += a0, acc[1] += a1, acc[2] += a2;
+            }
+#pragma unroll
+            for (int k = 0; k < 3; ++k) acc[3 + k] += a0 * ta[k] + a1 * ta[3 + k] + a2 * ta[6 + k];
+            if (hb) {
+                if (csb < 0) {
+                    acc[0] -= b0, acc[1] -= b1, acc[2] -= b2;
+                } else {
+                    acc[0] += b0, acc[1] += b1, acc[2] += b2;
+                }
+#pragma unroll
+                for (int k = 0; k < 3; ++k) acc[3 + k] += b0 * tb[k] + b1 * tb[3 + k] + b2 * tb[6 + k];
+            }
+        }
+    }
+}
+
 // w_s = M^-1 B^T lambda_s (ATMatrix::mat_mult, velocities.hpp:167-254, then M^-1)
 // into the iterate's slot s of w.  G consecutive lanes share one active particle:
 // lane k of the group takes its entries k, k + G, ... of the particle-major copy (so
@@ -438,7 +494,7 @@ __global__ void __launch_bounds__(kBtBlock) k_bt(ApgdCtrl* __restrict__ ctrl, in
                                               const int* __restrict__ tstart, const int32_t* __restrict__ tc,
                                               const double* __restrict__ partial_trial, int ntrial,
                                               double* __restrict__ partial_bt, SceneMap M) {
-    constexpr int NR = lay<LAY>::NR, NT = lay<LAY>::NT, NW = lay<LAY>::NW;
+    constexpr int NR = lay<LAY>::NR, NW = lay<LAY>::NW;
     __shared__ double lds[1][kBtBlock / 64];
     const bool multi = M.bblk != nullptr;
     const int sc = multi ? M.bblk[blockIdx.x] : 0;
@@ -479,53 +535,7 @@ __global__ void __launch_bounds__(kBtBlock) k_bt(ApgdCtrl* __restrict__ ctrl, in
         T acc[NW];
 #pragma unroll
         for (int k = 0; k < NW; ++k) acc[k] = (T)0;
-        // two entries per iteration, every load issued before the first use (the same
-        // accumulation order as one entry at a time)
-        for (int e = e0 + sub; e < e1; e += 2 * G) {
-            const int eb = e + G;
-            const bool hb = eb < e1;
-            const int32_t csa = tc[e];
-            const int32_t csb = hb ? tc[eb] : csa;
-            const int64_t ca = csa < 0 ? ~csa : csa, cb = csb < 0 ? ~csb : csb;
-            const T* tva = B.tcoef + (int64_t)e * NT;
-            const T* tvb = B.tcoef + (int64_t)(hb ? eb : e) * NT;
-            T ta[NT], tb[NT];
-#pragma unroll
-            for (int k = 0; k < NT; ++k) ta[k] = tva[k], tb[k] = tvb[k];
-            if (LAY == kRotFree || LAY == kRigid) {
-                const T la = lam[ca], lb = lam[cb];
-#pragma unroll
-                for (int k = 0; k < NT; ++k) acc[k] += la * ta[k];
-                if (hb)
-#pragma unroll
-                    for (int k = 0; k < NT; ++k) acc[k] += lb * tb[k];
-            } else {
-                T a0, a1, a2, b0, b1, b2;
-                if (LAY == kVisc) {
-                    visc_global<T>(B, lam, ca, a0, a1, a2);
-                    visc_global<T>(B, lam, cb, b0, b1, b2);
-                } else {
-                    a0 = lam[ca * 3], a1 = lam[ca * 3 + 1], a2 = lam[ca * 3 + 2];
-                    b0 = lam[cb * 3], b1 = lam[cb * 3 + 1], b2 = lam[cb * 3 + 2];
-                }
-                if (csa < 0) {
-                    acc[0] -= a0, acc[1] -= a1, acc[2] -= a2;
-                } else {
-                    acc[0] += a0, acc[1] += a1, acc[2] += a2;
-                }
-#pragma unroll
-                for (int k = 0; k < 3; ++k) acc[3 + k] += a0 * ta[k] + a1 * ta[3 + k] + a2 * ta[6 + k];
-                if (hb) {
-                    if (csb < 0) {
-                        acc[0] -= b0, acc[1] -= b1, acc[2] -= b2;
-                    } else {
-                        acc[0] += b0, acc[1] += b1, acc[2] += b2;
-                    }
-#pragma unroll
-                    for (int k = 0; k < 3; ++k) acc[3 + k] += b0 * tb[k] + b1 * tb[3 + k] + b2 * tb[6 + k];
-                }
-            }
-        }
+        bt_gather<T, LAY, G>(B, lam, tc, e0, e1, sub, acc);
 #pragma unroll
         for (int o = G / 2; o > 0; o >>= 1)
 #pragma unroll
@@ -612,6 +622,68 @@ __device__ inline void q_rows(const BMatrix<T>& B, const T* __restrict__ w, int6
     }
 }
 
+// One contact of a trial point: y = lambda_k + beta (lambda_k - lambda_{k-1}), Q y from the two Q
+// products, the gradient g = Q y + c, lambda+ = Proj(y - alpha g) stored in `lt`, and the contact's
+// terms of the 8 line-search sums (see k_trial).
+template <class T, int LAY>
+__device__ inline void trial_contact(const BMatrix<T>& B, T* __restrict__ lt, int64_t cc_, T beta, T alpha,
+                                     const T (&l0)[lay<LAY>::NR], const T (&l1)[lay<LAY>::NR],
+                                     const T (&cv)[lay<LAY>::NR], const T (&q0)[lay<LAY>::NR],
+                                     const T (&q1)[lay<LAY>::NR], double (&s)[8]) {
+    constexpr int NR = lay<LAY>::NR, NC = lay<LAY>::NC;
+    T y[NR], g[NR], z[NR], qy[NR];
+#pragma unroll
+    for (int d = 0; d < NR; ++d) {
+        y[d] = l0[d] + beta * (l0[d] - l1[d]);
+        qy[d] = q0[d] + beta * (q0[d] - q1[d]);
+        g[d] = qy[d] + cv[d];
+        z[d] = y[d] - alpha * g[d];
+    }
+    if constexpr (LAY == kVisc) {
+        const int kd = B.kind[cc_] & kKindMask;
+        if (kd == 2) {
+            // ViscousFriction at gamma_min (lagrange_multiplier.hpp:495-539): the nearer of
+            // (0, Proj_cone(f)) and (max(x-, 0), 0)
+            T f[3] = {z[1], z[2], z[3]};
+            cone_project<T>(f, B.coef[cc_ * NC], B.coef[cc_ * NC + 1], B.coef[cc_ * NC + 2], B.mu[cc_]);
+            const T zm = z[0] > (T)0 ? z[0] : (T)0;
+            T df = z[0] * z[0], dm = (zm - z[0]) * (zm - z[0]);
+            for (int d = 0; d < 3; ++d) {
+                df += (f[d] - z[1 + d]) * (f[d] - z[1 + d]);
+                dm += z[1 + d] * z[1 + d];
+            }
+            if (sqrt(df) < sqrt(dm)) {
+                z[0] = (T)0, z[1] = f[0], z[2] = f[1], z[3] = f[2];
+            } else {
+                z[0] = zm, z[1] = z[2] = z[3] = (T)0;
+            }
+        } else {
+            // max(lambda, 0) on the contact's 1 or 2 rows (lagrange_multiplier.hpp:197-201, 487-493)
+            z[0] = z[0] > (T)0 ? z[0] : (T)0;
+            z[1] = (kd == 1 && z[1] > (T)0) ? z[1] : (T)0;
+            z[2] = z[3] = (T)0;
+        }
+    } else if constexpr (LAY != kCone) {
+        z[0] = z[0] > (T)0 ? z[0] : (T)0;  // max(lambda, 0): lagrange_multiplier.hpp:104-107
+    } else {
+        // Coulomb-cone projection: lagrange_multiplier.hpp:243-268
+        cone_project<T>(z, B.coef[cc_ * NC + 0], B.coef[cc_ * NC + 1], B.coef[cc_ * NC + 2], B.mu[cc_]);
+    }
+#pragma unroll
+    for (int d = 0; d < NR; ++d) {
+        lt[cc_ * NR + d] = z[d];
+        const double dy = (double)z[d] - (double)y[d], dl = (double)z[d] - (double)l0[d];
+        s[0] += (double)g[d] * dy;
+        s[1] += dy * dy;
+        s[2] += (double)g[d] * dl;
+        s[3] += dl * dl;
+        s[4] += (double)y[d] * (double)qy[d];
+        s[5] += (double)cv[d] * (double)y[d];
+        s[6] += (double)cv[d] * (double)z[d];
+        s[7] += (double)z[d] * (double)z[d];
+    }
+}
+
 // One trial point of the line search over the block's contacts:
 // y = lambda_k + beta (lambda_k - lambda_{k-1}) and Q y = (1 + beta) Q lambda_k - beta Q lambda_{k-1},
 // with Q lambda_k formed on the fly from w_k (dt^2 B w_k, the row product) and stored in lambda_k's
@@ -625,7 +697,7 @@ __global__ void __launch_bounds__(kBlock) k_trial(const ApgdCtrl* __restrict__ i
                                                  const int32_t* __restrict__ cj, int64_t ni,
                                                  double* __restrict__ partial, const double* __restrict__ partial_bt,
                                                  int nbt, SceneMap M) {
-    constexpr int NR = lay<LAY>::NR, NC = lay<LAY>::NC;
+    constexpr int NR = lay<LAY>::NR;
     __shared__ double lds[8][kBlock / 64];
     __shared__ ApgdCtrl st;
     const SceneBlock sb = scene_block(M, nc);
@@ -659,61 +731,6 @@ __global__ void __launch_bounds__(kBlock) k_trial(const ApgdCtrl* __restrict__ i
     const T* qp = slot<T>(B.qlam, prev, stride);
     T* lt = slot<T>(B.lam, trial, stride);
     double s[8] = {0, 0, 0, 0, 0, 0, 0, 0};
-    // contact c: rows from (lambda_k, lambda_{k-1}, c) and the two Q products
-    auto one = [&](int64_t cc_, const T (&l0)[NR], const T (&l1)[NR], const T (&cv)[NR], const T (&q0)[NR],
-                   const T (&q1)[NR]) {
-        T y[NR], g[NR], z[NR], qy[NR];
-#pragma unroll
-        for (int d = 0; d < NR; ++d) {
-            y[d] = l0[d] + beta * (l0[d] - l1[d]);
-            qy[d] = q0[d] + beta * (q0[d] - q1[d]);
-            g[d] = qy[d] + cv[d];
-            z[d] = y[d] - alpha * g[d];
-        }
-        if constexpr (LAY == kVisc) {
-            const int kd = B.kind[cc_] & kKindMask;
-            if (kd == 2) {
-                // ViscousFriction at gamma_min (lagrange_multiplier.hpp:495-539): the nearer of
-                // (0, Proj_cone(f)) and (max(x-, 0), 0)
-                T f[3] = {z[1], z[2], z[3]};
-                cone_project<T>(f, B.coef[cc_ * NC], B.coef[cc_ * NC + 1], B.coef[cc_ * NC + 2], B.mu[cc_]);
-                const T zm = z[0] > (T)0 ? z[0] : (T)0;
-                T df = z[0] * z[0], dm = (zm - z[0]) * (zm - z[0]);
-                for (int d = 0; d < 3; ++d) {
-                    df += (f[d] - z[1 + d]) * (f[d] - z[1 + d]);
-                    dm += z[1 + d] * z[1 + d];
-                }
-                if (sqrt(df) < sqrt(dm)) {
-                    z[0] = (T)0, z[1] = f[0], z[2] = f[1], z[3] = f[2];
-                } else {
-                    z[0] = zm, z[1] = z[2] = z[3] = (T)0;
-                }
-            } else {
-                // max(lambda, 0) on the contact's 1 or 2 rows (lagrange_multiplier.hpp:197-201, 487-493)
-                z[0] = z[0] > (T)0 ? z[0] : (T)0;
-                z[1] = (kd == 1 && z[1] > (T)0) ? z[1] : (T)0;
-                z[2] = z[3] = (T)0;
-            }
-        } else if constexpr (LAY != kCone) {
-            z[0] = z[0] > (T)0 ? z[0] : (T)0;  // max(lambda, 0): lagrange_multiplier.hpp:104-107
-        } else {
-            // Coulomb-cone projection: lagrange_multiplier.hpp:243-268
-            cone_project<T>(z, B.coef[cc_ * NC + 0], B.coef[cc_ * NC + 1], B.coef[cc_ * NC + 2], B.mu[cc_]);
-        }
-#pragma unroll
-        for (int d = 0; d < NR; ++d) {
-            lt[cc_ * NR + d] = z[d];
-            const double dy = (double)z[d] - (double)y[d], dl = (double)z[d] - (double)l0[d];
-            s[0] += (double)g[d] * dy;
-            s[1] += dy * dy;
-            s[2] += (double)g[d] * dl;
-            s[3] += dl * dl;
-            s[4] += (double)y[d] * (double)qy[d];
-            s[5] += (double)cv[d] * (double)y[d];
-            s[6] += (double)cv[d] * (double)z[d];
-            s[7] += (double)z[d] * (double)z[d];
-        }
-    };
     for (; c < cend; c += U * cstep) {
         int64_t cu[U], iu[U], ju[U];
         bool hu[U];
@@ -741,7 +758,7 @@ __global__ void __launch_bounds__(kBlock) k_trial(const ApgdCtrl* __restrict__ i
             if (hu[u]) {
 #pragma unroll
                 for (int d = 0; d < NR; ++d) qc[cu[u] * NR + d] = q0[u][d];
-                one(cu[u], l0[u], l1[u], cv[u], q0[u], q1[u]);
+                trial_contact<T, LAY>(B, lt, cu[u], beta, alpha, l0[u], l1[u], cv[u], q0[u], q1[u], s);
             }
     }
     block_sum<8>(s, lds);
@@ -888,6 +905,233 @@ __global__ void __launch_bounds__(kBlock) k_ctrl(const ApgdCtrl* __restrict__ in
     control_step(in + sc, out + sc, partial_bt + M.bfirst[sc], M.bcount[sc], st, lds, true, M.done + sc);
 }
 
+// ------------------------------------------------------------------ per-scene work queue
+// Small scenes of a batch are solved whole by one workgroup each: a persistent grid takes scenes
+// from an atomic counter over a list in decreasing contact count, and the workgroup runs every
+// round of its scene (the trial sweep over the scene's contacts, the B^T sweep over its particles,
+// then the control step of apgd.hpp:172-209 by thread 0) with barriers between the sweeps, so a
+// scene's whole solve (both phases of FP32 + FP64 refinement) is one launch with no host polling.
+// Workgroup-scope barriers order the sweeps' global stores and loads: a workgroup runs on one CU,
+// whose waves share its L1.  Sums are taken in a fixed order within the workgroup, so a scene's
+// iterates depend only on the scene and the workgroup size, the same in any mix.
+
+// Fields of the state every thread reads in a round, published by thread 0.
+struct QueueRound {
+    int32_t cur, prev, trial, done;
+    double beta, alpha, dt2;
+};
+
+// w_s = M^-1 B^T lambda_s over the particles [plo, phi), G lanes per particle; returns the thread's
+// share of sum a . w_s, a = B^T lambda_s.
+template <class T, int LAY, int NTH, int G>
+__device__ inline double queue_bt(const BMatrix<T>& B, const QueueArgs& A, int s, int64_t plo, int64_t phi) {
+    constexpr int NR = lay<LAY>::NR, NW = lay<LAY>::NW;
+    const T* lam = slot<T>(B.lam, s, NR * B.cap);
+    T* w = slot<T>(B.w, s, 6 * B.pcap);
+    const int sub = (int)(threadIdx.x % G);
+    double q = 0.0;
+    int64_t p = plo + threadIdx.x / G;
+    int e0 = p < phi ? A.tstart[p] : 0, e1 = p < phi ? A.tstart[p + 1] : 0;
+    // a lane group shares p, so each group passes through the loop as a whole; the next particle's
+    // entry range is loaded one pass ahead
+    for (; p < phi; p += NTH / G) {
+        const int f0 = e0, f1 = e1;
+        if (p + NTH / G < phi) e0 = A.tstart[p + NTH / G], e1 = A.tstart[p + NTH / G + 1];
+        T acc[NW];
+#pragma unroll
+        for (int k = 0; k < NW; ++k) acc[k] = (T)0;
+        bt_gather<T, LAY, G>(B, lam, A.tc, f0, f1, sub, acc);
+#pragma unroll
+        for (int o = G / 2; o > 0; o >>= 1)
+#pragma unroll
+            for (int k = 0; k < NW; ++k) acc[k] += __shfl_xor(acc[k], o, G);
+        if (sub == 0)
+#pragma unroll
+            for (int k = 0; k < NW; ++k) {
+                const T wk = B.minv[p * NW + k] * acc[k];
+                w[p * NW + k] = wk;
+                q += (double)acc[k] * (double)wk;
+            }
+    }
+    return q;
+}
+
+// One phase of the loop (one precision) on the scene's contacts [clo, chi) and particles [plo, phi):
+// lambda_0 = 0, or the warm start in slot 0; L and alpha the first step's.  The final state is left
+// in `st` (all threads may read it after the phase).
+template <class T, int LAY, int NTH, int G>
+__device__ void queue_phase(const BMatrix<T>& B, const QueueArgs& A, int64_t clo, int64_t chi, int64_t plo,
+                            int64_t phi, bool warm, double tol, double rtol, int64_t max_ite, double L, double alpha,
+                            int low, ApgdCtrl& st, QueueRound& sh, double (*lds)[NTH / 64]) {
+    constexpr int NR = lay<LAY>::NR, NW = lay<LAY>::NW;
+    const int64_t stride = NR * B.cap, wstride = 6 * B.pcap;
+    // lambda_0 in slot 0, lambda_{-1} = lambda_0 in slot 1 with a finite Q lambda_{-1} (beta = 0 in
+    // the first round), w_0 in slot 0
+    for (int64_t r = NR * clo + threadIdx.x; r < NR * chi; r += NTH) {
+        const T l0 = warm ? B.lam[r] : (T)0;
+        if (!warm) B.lam[r] = (T)0;
+        B.lam[stride + r] = l0;
+        B.qlam[stride + r] = (T)0;
+    }
+    if (!warm)
+        for (int64_t k = NW * plo + threadIdx.x; k < NW * phi; k += NTH) B.w[k] = (T)0;
+    __syncthreads();
+    if (warm) {
+        (void)queue_bt<T, LAY, NTH, G>(B, A, 0, plo, phi);
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        ApgdCtrl c{};
+        const bool empty = chi <= clo;
+        c.L = L;
+        c.L_acc = L;
+        c.alpha = alpha;
+        c.theta = 1.0;
+        c.beta = 0.0;
+        c.tol = tol;
+        c.rtol = rtol;
+        c.dt2 = A.dt2;
+        c.best_res = INFINITY;
+        c.max_ite = max_ite;
+        c.q_applies = warm ? 1 : 0;  // the product of the warm start's w_0
+        c.cur = 0;
+        c.prev = 1;
+        c.trial = 2;
+        c.done = (empty || max_ite <= 0) ? 1 : 0;
+        c.status = (!empty && max_ite <= 0) ? SCOPI_HIP_NOT_CONVERGED : 0;
+        c.res = c.done ? 0.0 : INFINITY;
+        c.method = A.method;
+        c.dynamic = A.dynamic;
+        c.low_precision = low;
+        st = c;
+        sh = QueueRound{c.cur, c.prev, c.trial, c.done, c.beta, c.alpha, c.dt2};
+    }
+    __syncthreads();
+    for (;;) {
+        const QueueRound rd = sh;
+        if (rd.done) break;
+        const T beta = (T)rd.beta, alpha_t = (T)rd.alpha, dt2 = (T)rd.dt2;
+        const T* lc = slot<T>(B.lam, rd.cur, stride);
+        const T* lp = slot<T>(B.lam, rd.prev, stride);
+        const T* wc = slot<T>(B.w, rd.cur, wstride);
+        T* qc = slot<T>(B.qlam, rd.cur, stride);
+        const T* qp = slot<T>(B.qlam, rd.prev, stride);
+        T* lt = slot<T>(B.lam, rd.trial, stride);
+        double s[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+        // U contacts c + u NTH per iteration, as in k_trial, every load of an iteration issued before
+        // the first use and the next contacts' particle indices one iteration ahead
+        constexpr int U = LAY == kRotFree ? 2 : 1;
+        int64_t c = clo + threadIdx.x;
+        int32_t inx[U], jnx[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            inx[u] = jnx[u] = 0;
+            if (c + u * NTH < chi) inx[u] = A.ci[c + u * NTH], jnx[u] = A.cj[c + u * NTH];
+        }
+        for (; c < chi; c += U * NTH) {
+            int64_t cu[U], iu[U], ju[U];
+            bool hu[U];
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                hu[u] = c + u * NTH < chi;
+                cu[u] = hu[u] ? c + u * NTH : c;  // past the end: a duplicate of c, not stored
+                iu[u] = (int64_t)inx[u] - A.ni, ju[u] = (int64_t)jnx[u] - A.ni;
+                const int64_t cn = c + (U + u) * NTH;
+                if (cn < chi) inx[u] = A.ci[cn], jnx[u] = A.cj[cn];
+            }
+            T l0[U][NR], l1[U][NR], cv[U][NR], q1[U][NR], q0[U][NR];
+#pragma unroll
+            for (int u = 0; u < U; ++u)
+#pragma unroll
+                for (int d = 0; d < NR; ++d) {
+                    const int64_t r = cu[u] * NR + d;
+                    l0[u][d] = lc[r], l1[u][d] = lp[r], cv[u][d] = B.c[r], q1[u][d] = qp[r];
+                }
+#pragma unroll
+            for (int u = 0; u < U; ++u) q_rows<T, LAY>(B, wc, cu[u], iu[u], ju[u], dt2, q0[u]);
+#pragma unroll
+            for (int u = 0; u < U; ++u)
+                if (hu[u]) {
+#pragma unroll
+                    for (int d = 0; d < NR; ++d) qc[cu[u] * NR + d] = q0[u][d];
+                    trial_contact<T, LAY>(B, lt, cu[u], beta, alpha_t, l0[u], l1[u], cv[u], q0[u], q1[u], s);
+                }
+        }
+        // the barriers of the sum also order the block's stores of lambda+ before the B^T sweep
+        block_sum<8, NTH>(s, lds);
+        double q[1] = {queue_bt<T, LAY, NTH, G>(B, A, rd.trial, plo, phi)};
+        block_sum<1, NTH>(q, lds);
+        if (threadIdx.x == 0) {
+            ApgdCtrl c = st;
+#pragma unroll
+            for (int k = 0; k < 7; ++k) c.red[k] = s[k];
+            c.red[8] = s[7];
+            c.red[7] = c.dt2 * q[0];
+            apgd_control(&c);
+            st = c;
+            sh = QueueRound{c.cur, c.prev, c.trial, c.done, c.beta, c.alpha, c.dt2};
+        }
+        __syncthreads();
+    }
+}
+
+// The persistent grid: every workgroup takes scenes until the list is exhausted, so each one
+// reaches the exit.  A scene's final multipliers (FP64) end in slot 0; its final state in A.result.
+template <int LAY, int NTH, int G>
+__global__ void __launch_bounds__(NTH) k_apgd_queue(QueueArgs A, BMatrix<double> B64, BMatrix<float> B32) {
+    constexpr int NR = lay<LAY>::NR;
+    __shared__ double lds[8][NTH / 64];
+    __shared__ ApgdCtrl st;
+    __shared__ QueueRound sh;
+    __shared__ int32_t s_item;
+    for (;;) {
+        if (threadIdx.x == 0) s_item = atomicAdd(A.next, 1);
+        __syncthreads();
+        const int32_t item = s_item;
+        if (item >= A.nqueue) return;
+        const int sc = A.order[item];
+        const int64_t clo = A.crange[2 * sc], chi = A.crange[2 * sc + 1];
+        const int64_t plo = A.arange[2 * sc], phi = A.arange[2 * sc + 1];
+        int64_t it32 = 0, bt32 = 0, qa32 = 0, max_ite = A.max_ite;
+        double L = 1.0 / A.alpha, alpha = A.alpha;
+        bool warm = A.warm != 0;
+        if (A.fp32) {
+            // FP32 phase to its relative stop or precision floor, then FP64 refinement from its
+            // last iterate with its last accepted L and the iterations left
+            queue_phase<float, LAY, NTH, G>(B32, A, clo, chi, plo, phi, warm, A.tol32, A.rtol32, max_ite, L, alpha, 1,
+                                            st, sh, lds);
+            const float* lf = B32.lam + (int64_t)st.cur * NR * B32.cap;
+            for (int64_t r = NR * clo + threadIdx.x; r < NR * chi; r += NTH) B64.lam[r] = (double)lf[r];
+            it32 = st.iter, bt32 = st.backtracks, qa32 = st.q_applies;
+            const double La = st.L_acc;
+            if (isfinite(La) && La > 0) L = La, alpha = 1. / La;
+            max_ite = A.max_ite - it32 < 1 ? 1 : A.max_ite - it32;
+            warm = true;
+            __syncthreads();
+        }
+        queue_phase<double, LAY, NTH, G>(B64, A, clo, chi, plo, phi, warm, A.tol, 0.0, max_ite, L, alpha, 0, st, sh,
+                                         lds);
+        const int fin = st.cur;
+        if (fin != 0) {
+            const double* lf = B64.lam + (int64_t)fin * NR * B64.cap;
+            for (int64_t r = NR * clo + threadIdx.x; r < NR * chi; r += NTH) B64.lam[r] = lf[r];
+        }
+        if (threadIdx.x == 0) {
+            QueueResult q;
+            q.iter = st.iter + it32;
+            q.iter32 = it32;
+            q.backtracks = st.backtracks + bt32;
+            q.q_applies = st.q_applies + qa32;
+            q.res = st.res;
+            q.L_acc = st.L_acc;
+            q.status = st.status == kStatusStalled ? 0 : st.status;
+            q.pad = 0;
+            A.result[sc] = q;
+        }
+        __syncthreads();  // st, sh and s_item are reused by the next scene
+    }
+}
+
 template <class T>
 __global__ void __launch_bounds__(kBlock) k_zero(T* __restrict__ a, int64_t n) {
     for (int64_t t = (int64_t)blockIdx.x * kBlock + threadIdx.x; t < n; t += (int64_t)gridDim.x * kBlock) a[t] = (T)0;
@@ -1004,6 +1248,30 @@ __global__ void __launch_bounds__(kBlock) k_scene_slot(const TA* __restrict__ sr
         dst[r] = (TB)a[r];
 }
 
+// Set-up of the two-launch rounds for the scenes of the grid only, so that rows and particles of
+// other scenes (a concurrent work-queue launch may own them) are not touched: cold, lambda_0 =
+// lambda_{-1} = 0 and w_0 = 0; warm, lambda_{-1} = lambda_0 (the warm start in slot 0); and a finite
+// Q lambda_{-1} = 0 (per-contact grid of the scene map).
+template <class T, int NR, int NW>
+__global__ void __launch_bounds__(kBlock) k_scene_setup(BMatrix<T> B, int warm, SceneMap M) {
+    const SceneBlock sb = scene_block(M, 0);
+    const int64_t stride = NR * B.cap;
+    const int64_t step = (int64_t)sb.nb * kBlock;
+    for (int64_t r = NR * sb.lo + (int64_t)sb.lb * kBlock + threadIdx.x; r < NR * sb.hi; r += step) {
+        if (warm) {
+            B.lam[stride + r] = B.lam[r];
+        } else {
+            B.lam[r] = (T)0;
+            B.lam[stride + r] = (T)0;
+        }
+        B.qlam[stride + r] = (T)0;
+    }
+    if (!warm) {
+        const int64_t p0 = M.arange[2 * sb.sc], p1 = M.arange[2 * sb.sc + 1];
+        for (int64_t k = NW * p0 + (int64_t)sb.lb * kBlock + threadIdx.x; k < NW * p1; k += step) B.w[k] = (T)0;
+    }
+}
+
 // u = v + dt M^-1 B^T lambda, omega likewise (OptimGradient.hpp:267-284); with no
 // contact, or no rotational block (kRotFree), the corresponding part is v / omega.
 template <int NW>
@@ -1060,8 +1328,16 @@ Solver::~Solver() {
         if (p) (void)hipFree(p);
     if (h_fp_partial_) (void)hipHostFree(h_fp_partial_);
     if (h_vrows_) (void)hipHostFree(h_vrows_);
-    for (void* p : {(void*)h_done_, (void*)h_tab_, (void*)h_ctrls_})
+    for (void* p : {(void*)h_done_, (void*)h_tab_, (void*)h_ctrls_, (void*)h_qorder_, (void*)h_qres_})
         if (p) (void)hipHostFree(p);
+    for (void* p : {(void*)d_qorder_, (void*)d_qnext_, (void*)d_qres_})
+        if (p) (void)hipFree(p);
+    if (qstream_) {
+        (void)hipStreamSynchronize(qstream_);
+        (void)hipStreamDestroy(qstream_);
+        (void)hipEventDestroy(qfork_);
+        (void)hipEventDestroy(qjoin_);
+    }
 }
 
 void Solver::reserve(int64_t na, int64_t nc, bool fp32) {
@@ -1125,7 +1401,7 @@ void Solver::run_phase(BMatrix<T>& B, const Particles& P, const Contacts& C, int
                        const scopi_hip_params& prm, double dt, double tol, double rtol, int64_t max_ite, bool warm,
                        bool keep_L, hipStream_t s, scopi_hip_solve_info& info) {
     constexpr int NR = lay<LAY>::NR;
-    const unsigned gred = (unsigned)row_blocks(nc, reduce_blocks_, LAY);
+    const unsigned gred = (unsigned)row_blocks(nc, reduce_blocks_, LAY, row_iters_);
     const int nbt = (int)bt_blocks(na, bt_group_);
     const int64_t ni = P.n_inactive;
     const int64_t stride = NR * B.cap, wstride = 6 * B.pcap;
@@ -1274,8 +1550,6 @@ void Solver::assemble(const Particles& P, const Contacts& C, int64_t nc, double 
 template <int LAY>
 void Solver::solve_layout(const Particles& P, const Contacts& C, int64_t nc, const scopi_hip_params& prm, double dt,
                           hipStream_t s, scopi_hip_solve_info& info) {
-    constexpr int NR = lay<LAY>::NR, NW = lay<LAY>::NW;
-    const int64_t na = P.n - P.n_inactive;
     const bool fp32 = prm.precision == SCOPI_PRECISION_FP32_REFINE;
     assemble<LAY>(P, C, nc, dt, fp32, s);
     solve_phases<LAY>(P, C, nc, prm, dt, s, info);
@@ -1509,6 +1783,26 @@ int Solver::set_tuning(int key, int value) {
         bt_group_ = value;
         return SCOPI_HIP_OK;
     }
+    if (key == SCOPI_TUNE_ROW_ITERS) {
+        if (value < 1 || value > 64) return SCOPI_HIP_ERR_ARG;
+        row_iters_ = value;
+        return SCOPI_HIP_OK;
+    }
+    if (key == SCOPI_TUNE_QUEUE_MAX_CONTACTS) {
+        if (value < 0) return SCOPI_HIP_ERR_ARG;
+        queue_max_contacts_ = value;
+        return SCOPI_HIP_OK;
+    }
+    if (key == SCOPI_TUNE_QUEUE_BLOCKS) {
+        if (value < 1 || value > 65536) return SCOPI_HIP_ERR_ARG;
+        queue_blocks_ = value;
+        return SCOPI_HIP_OK;
+    }
+    if (key == SCOPI_TUNE_QUEUE_THREADS) {
+        if (value != 256 && value != 512 && value != 1024) return SCOPI_HIP_ERR_ARG;
+        queue_threads_ = value;
+        return SCOPI_HIP_OK;
+    }
     return SCOPI_HIP_ERR_ARG;
 }
 
@@ -1644,32 +1938,27 @@ void Solver::scene_tables(const std::vector<int>& scenes, hipStream_t s) {
 template <class T, int LAY>
 void Solver::run_phase_scenes(BMatrix<T>& B, const Contacts& C, int64_t na, int64_t nc, const scopi_hip_params& prm,
                               double dt, double tol, double rtol, int64_t max_ite, bool warm, bool keep_L,
-                              hipStream_t s, std::vector<scopi_hip_solve_info>& info) {
+                              hipStream_t s, const std::vector<int>& scenes, std::vector<scopi_hip_solve_info>& info) {
     constexpr int NR = lay<LAY>::NR;
     const int K = scenes_->nscene;
-    const int64_t stride = NR * B.cap, wstride = 6 * B.pcap;
     const double dt2 = dt * dt;
     const int32_t* ci = (const int32_t*)C.i;
     const int32_t* cj = (const int32_t*)C.j;
-    // slots as in the one-scene loop: lambda_0 and w_0 in slot 0, lambda_{-1} and a finite
-    // Q lambda_{-1} in slot 1 (beta = 0 in every scene's first round)
-    if (!warm) {
-        hipLaunchKernelGGL(k_zero<T>, dim3(1024), dim3(kBlock), 0, s, B.lam, 3 * stride);
-        hipLaunchKernelGGL(k_zero<T>, dim3(1024), dim3(kBlock), 0, s, B.w, wstride);
-    } else {
-        // every scene's lambda_0 is in slot 0: lambda_{-1} = lambda_0, and w_0
-        SCOPI_HIP_CHECK(hipMemcpyAsync(B.lam + stride, B.lam, stride * sizeof(T), hipMemcpyDeviceToDevice, s));
-        launch_bt<T, LAY>(bt_group_, s, (ApgdCtrl*)nullptr, 0, B, na, tstart_, tc_, nullptr, 0, nullptr);
-        for (auto& in : info) in.q_applies += 1;
-    }
-    hipLaunchKernelGGL(k_zero<T>, dim3(1024), dim3(kBlock), 0, s, B.qlam + stride, stride);
     hipLaunchKernelGGL(k_ctrl_init_scenes, dim3(grid_for(K, 64)), dim3(64), 0, s, mctrl_, (const int32_t*)d_buf_, K,
                        prm.alpha, 1. / prm.alpha, tol, rtol, dt2, max_ite, prm.method, prm.dynamic_descent ? 1 : 0,
                        keep_L ? 1 : 0, sizeof(T) == 4 ? 1 : 0, scenes_->d_crange, d_done_);
     SCOPI_HIP_CHECK(hipGetLastError());
-    std::vector<int> live(K), buf(K, 0);
-    for (int k = 0; k < K; ++k) live[k] = k;
+    std::vector<int> live(scenes), buf(K, 0);
     scene_tables(live, s);
+    // slots as in the one-scene loop, for these scenes only: lambda_0 and w_0 in slot 0, lambda_{-1}
+    // and a finite Q lambda_{-1} in slot 1 (beta = 0 in every scene's first round); warm, every
+    // scene's lambda_0 is in slot 0 and w_0 comes from it
+    hipLaunchKernelGGL((k_scene_setup<T, NR, lay<LAY>::NW>), dim3(rgrid_), dim3(kBlock), 0, s, B, warm ? 1 : 0, map_);
+    if (warm) {
+        launch_bt<T, LAY>(bt_group_, s, (ApgdCtrl*)nullptr, 0, B, na, tstart_, tc_, nullptr, 0, nullptr, nullptr,
+                          nullptr, map_, bgrid_);
+        for (int sc : scenes) info[sc].q_applies += 1;
+    }
     int batch = 2;
     int cur = 0;  // ctrl copy (K scenes) holding the newest state of the scenes in the grid
     for (;;) {
@@ -1707,7 +1996,7 @@ void Solver::run_phase_scenes(BMatrix<T>& B, const Contacts& C, int64_t na, int6
     SCOPI_HIP_CHECK(hipMemcpyAsync(d_buf_, scene_buf_.data(), K * sizeof(int32_t), hipMemcpyHostToDevice, s));
     SCOPI_HIP_CHECK(hipMemcpyAsync(h_ctrls_, mctrl_, 2 * (int64_t)K * sizeof(ApgdCtrl), hipMemcpyDeviceToHost, s));
     SCOPI_HIP_CHECK(hipStreamSynchronize(s));
-    for (int sc = 0; sc < K; ++sc) {
+    for (int sc : scenes) {
         const ApgdCtrl& c = h_ctrls_[(int64_t)buf[sc] * K + sc];
         scopi_hip_solve_info& in = info[sc];
         in.iterations += c.iter;
@@ -1728,35 +2017,51 @@ void Solver::solve_layout_scenes(const Particles& P, const Contacts& C, int64_t 
     const int64_t na = P.n - P.n_inactive;
     const bool fp32 = prm.precision == SCOPI_PRECISION_FP32_REFINE;
     assemble<LAY>(P, C, nc, dt, fp32, s);
-    std::vector<int> all(K);
-    for (int k = 0; k < K; ++k) all[k] = k;
-    // per scene, the slot holding the final lambda of the last phase
+    // scenes of at most queue_max_contacts_ contacts go to the work-queue kernel, the others take
+    // the two-launch rounds
+    std::vector<int> all(K), big, small;
+    std::vector<char> queued(K, 0);
+    for (int k = 0; k < K; ++k) {
+        all[k] = k;
+        queued[k] = queue_max_contacts_ > 0 && scene_nc_[k] <= queue_max_contacts_;
+        (queued[k] ? small : big).push_back(k);
+    }
+    // per scene, the slot holding the final lambda of the last phase (a queued scene's is slot 0)
     std::vector<int32_t> sl(K);
     auto slots_to_device = [&](int32_t* dev) {
-        for (int k = 0; k < K; ++k) sl[k] = h_ctrls_[(int64_t)scene_buf_[k] * K + k].cur;
+        for (int k = 0; k < K; ++k) sl[k] = queued[k] ? 0 : h_ctrls_[(int64_t)scene_buf_[k] * K + k].cur;
         SCOPI_HIP_CHECK(hipMemcpyAsync(dev, sl.data(), K * sizeof(int32_t), hipMemcpyHostToDevice, s));
         SCOPI_HIP_CHECK(hipStreamSynchronize(s));
     };
     int32_t* d_slot = d_done_;  // reused as the per-scene slot list between phases
+    if (fp32 && lam0_)
+        hipLaunchKernelGGL((k_convert<double, float>), dim3(1024), dim3(kBlock), 0, s, lam0_, b32_.lam, NR * nc);
+    if (!fp32 && lam0_)
+        SCOPI_HIP_CHECK(hipMemcpyAsync(b64_.lam, lam0_, NR * nc * sizeof(double), hipMemcpyDeviceToDevice, s));
+    // The queued scenes run on a second stream, concurrently with the rounds of the others: the
+    // latency-bound work-queue launch overlaps the bandwidth-bound rounds.  From here on each side
+    // writes only its own scenes' rows and particles (the rounds' set-up is per scene), and both
+    // only read B, c, M^-1 and the particle-major copy.
+    if (!small.empty()) queue_launch<LAY>(C, prm, dt, s, small);
     if (fp32) {
         const double rtol32 = prm.fp32_tolerance > 0 ? 0.0 : 1e-5;
         const double tol32 = prm.fp32_tolerance > 0 ? prm.fp32_tolerance : prm.tolerance;
-        if (lam0_)
-            hipLaunchKernelGGL((k_convert<double, float>), dim3(1024), dim3(kBlock), 0, s, lam0_, b32_.lam, NR * nc);
-        run_phase_scenes<float, LAY>(b32_, C, na, nc, prm, dt, tol32, rtol32, prm.max_ite, lam0_ != nullptr, false, s,
-                                     info);
-        for (auto& in : info) in.iterations_fp32 = in.iterations;
-        slots_to_device(d_slot);
-        scene_tables(all, s);
-        hipLaunchKernelGGL((k_scene_slot<float, double, NR>), dim3(rgrid_), dim3(kBlock), 0, s,
-                           (const float*)b32_.lam, NR * b32_.cap, b64_.lam, (const int32_t*)d_slot, map_);
-        run_phase_scenes<double, LAY>(b64_, C, na, nc, prm, dt, prm.tolerance, 0.0, prm.max_ite, true, true, s, info);
-    } else {
-        if (lam0_)
-            SCOPI_HIP_CHECK(hipMemcpyAsync(b64_.lam, lam0_, NR * nc * sizeof(double), hipMemcpyDeviceToDevice, s));
-        run_phase_scenes<double, LAY>(b64_, C, na, nc, prm, dt, prm.tolerance, 0.0, prm.max_ite, lam0_ != nullptr, false,
-                                      s, info);
+        if (!big.empty()) {
+            run_phase_scenes<float, LAY>(b32_, C, na, nc, prm, dt, tol32, rtol32, prm.max_ite, lam0_ != nullptr, false,
+                                         s, big, info);
+            for (int k : big) info[k].iterations_fp32 = info[k].iterations;
+            slots_to_device(d_slot);
+            scene_tables(big, s);
+            hipLaunchKernelGGL((k_scene_slot<float, double, NR>), dim3(rgrid_), dim3(kBlock), 0, s,
+                               (const float*)b32_.lam, NR * b32_.cap, b64_.lam, (const int32_t*)d_slot, map_);
+            run_phase_scenes<double, LAY>(b64_, C, na, nc, prm, dt, prm.tolerance, 0.0, prm.max_ite, true, true, s,
+                                          big, info);
+        }
+    } else if (!big.empty()) {
+        run_phase_scenes<double, LAY>(b64_, C, na, nc, prm, dt, prm.tolerance, 0.0, prm.max_ite, lam0_ != nullptr,
+                                      false, s, big, info);
     }
+    if (!small.empty()) queue_join(s, small, info);
     // every scene's final lambda to slot 0, then the velocities (FP64)
     slots_to_device(d_slot);
     scene_tables(all, s);
@@ -1767,6 +2072,91 @@ void Solver::solve_layout_scenes(const Particles& P, const Contacts& C, int64_t 
     hipLaunchKernelGGL(k_velocities<NW>, dim3(grid_for(na)), dim3(kBlock), 0, s, P, (const double*)b64_.w, pcap_, dt, 1,
                        u_, w_);
     SCOPI_HIP_CHECK(hipGetLastError());
+}
+
+template <int LAY>
+void Solver::queue_launch(const Contacts& C, const scopi_hip_params& prm, double dt, hipStream_t s,
+                          const std::vector<int>& scenes) {
+    const int K = scenes_->nscene;
+    const int nq = (int)scenes.size();
+    if (!qstream_) {
+        SCOPI_HIP_CHECK(hipStreamCreateWithFlags(&qstream_, hipStreamNonBlocking));  // default priority
+        SCOPI_HIP_CHECK(hipEventCreateWithFlags(&qfork_, hipEventDisableTiming));
+        SCOPI_HIP_CHECK(hipEventCreateWithFlags(&qjoin_, hipEventDisableTiming));
+        SCOPI_HIP_CHECK(hipMalloc(&d_qnext_, sizeof(int32_t)));
+    }
+    if (std::max(nq, K) > qcap_) {
+        for (void* p : {(void*)d_qorder_, (void*)d_qres_})
+            if (p) SCOPI_HIP_CHECK(hipFree(p));
+        for (void* p : {(void*)h_qorder_, (void*)h_qres_})
+            if (p) SCOPI_HIP_CHECK(hipHostFree(p));
+        qcap_ = std::max(nq, K) + 64;
+        SCOPI_HIP_CHECK(hipMalloc(&d_qorder_, qcap_ * sizeof(int32_t)));
+        SCOPI_HIP_CHECK(hipHostMalloc(&h_qorder_, qcap_ * sizeof(int32_t)));
+        SCOPI_HIP_CHECK(hipMalloc(&d_qres_, qcap_ * sizeof(QueueResult)));
+        SCOPI_HIP_CHECK(hipHostMalloc(&h_qres_, qcap_ * sizeof(QueueResult)));
+    }
+    // largest scenes first, so that the last workgroups to finish hold small scenes
+    std::vector<int> order(scenes);
+    std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return scene_nc_[a] > scene_nc_[b]; });
+    SCOPI_HIP_CHECK(hipStreamSynchronize(qstream_));  // the staging buffers may still feed the last copies
+    for (int k = 0; k < nq; ++k) h_qorder_[k] = order[k];
+    // the queue starts once B, the particle-major copy and the warm start are in place on `s`
+    SCOPI_HIP_CHECK(hipEventRecord(qfork_, s));
+    SCOPI_HIP_CHECK(hipStreamWaitEvent(qstream_, qfork_, 0));
+    SCOPI_HIP_CHECK(hipMemcpyAsync(d_qorder_, h_qorder_, nq * sizeof(int32_t), hipMemcpyHostToDevice, qstream_));
+    hipLaunchKernelGGL(k_fill_i, dim3(1), dim3(kBlock), 0, qstream_, d_qnext_, (int64_t)1, 0);
+    QueueArgs A;
+    A.order = d_qorder_;
+    A.nqueue = nq;
+    A.next = d_qnext_;
+    A.crange = scenes_->d_crange;
+    A.arange = d_arange_;
+    A.result = d_qres_;
+    A.tstart = tstart_;
+    A.tc = tc_;
+    A.ci = (const int32_t*)C.i;
+    A.cj = (const int32_t*)C.j;
+    A.ni = p_inactive_;
+    A.alpha = prm.alpha;
+    A.tol = prm.tolerance;
+    A.tol32 = prm.fp32_tolerance > 0 ? prm.fp32_tolerance : prm.tolerance;
+    A.rtol32 = prm.fp32_tolerance > 0 ? 0.0 : 1e-5;
+    A.dt2 = dt * dt;
+    A.max_ite = prm.max_ite;
+    A.method = prm.method;
+    A.dynamic = prm.dynamic_descent ? 1 : 0;
+    A.warm = lam0_ != nullptr ? 1 : 0;
+    A.fp32 = prm.precision == SCOPI_PRECISION_FP32_REFINE ? 1 : 0;
+    const unsigned grid = (unsigned)std::min(nq, queue_blocks_);
+    BMatrix<float> b32 = A.fp32 ? b32_ : BMatrix<float>{};
+    if (queue_threads_ == 1024)
+        hipLaunchKernelGGL((k_apgd_queue<LAY, 1024, 8>), dim3(grid), dim3(1024), 0, qstream_, A, b64_, b32);
+    else if (queue_threads_ == 512)
+        hipLaunchKernelGGL((k_apgd_queue<LAY, 512, 8>), dim3(grid), dim3(512), 0, qstream_, A, b64_, b32);
+    else
+        hipLaunchKernelGGL((k_apgd_queue<LAY, 256, 8>), dim3(grid), dim3(256), 0, qstream_, A, b64_, b32);
+    SCOPI_HIP_CHECK(hipGetLastError());
+    SCOPI_HIP_CHECK(hipMemcpyAsync(h_qres_, d_qres_, K * sizeof(QueueResult), hipMemcpyDeviceToHost, qstream_));
+    SCOPI_HIP_CHECK(hipEventRecord(qjoin_, qstream_));
+}
+
+void Solver::queue_join(hipStream_t s, const std::vector<int>& scenes, std::vector<scopi_hip_solve_info>& info) {
+    // `s` continues (final slot copy, velocities) after the queue's results; the host reads them
+    SCOPI_HIP_CHECK(hipStreamWaitEvent(s, qjoin_, 0));
+    SCOPI_HIP_CHECK(hipEventSynchronize(qjoin_));
+    for (int sc : scenes) {
+        const QueueResult& r = h_qres_[sc];
+        scopi_hip_solve_info& in = info[sc];
+        in.iterations += r.iter;
+        in.iterations_fp32 += r.iter32;
+        in.backtracks += r.backtracks;
+        in.q_applies += r.q_applies;
+        in.status = r.status;
+        in.converged = in.status == 0 ? 1 : 0;
+        in.residual = r.res;
+        in.step = 1.0 / r.L_acc;
+    }
 }
 
 int Solver::solve_scenes(const Particles& P, const Contacts& C, int64_t nc, const scopi_hip_params& prm, double dt,
@@ -1812,12 +2202,14 @@ int Solver::solve_scenes(const Particles& P, const Contacts& C, int64_t nc, cons
     SCOPI_HIP_CHECK(hipStreamSynchronize(s));
     scene_rblocks_.assign(K, 1);
     scene_bblocks_.assign(K, 1);
+    scene_nc_.assign(K, 0);
     int64_t R = 0, Bn = 0;
     for (int k = 0; k < K; ++k) {
         ar[2 * k] = S.alo[k];
         ar[2 * k + 1] = S.ahi[k];
         const int64_t nck = hc[2 * k + 1] - hc[2 * k];
-        scene_rblocks_[k] = (int)row_blocks(nck, reduce_blocks_, layout_);
+        scene_nc_[k] = nck;
+        scene_rblocks_[k] = (int)row_blocks(nck, reduce_blocks_, layout_, row_iters_);
         scene_bblocks_[k] = (int)std::max<int64_t>(bt_blocks(S.ahi[k] - S.alo[k], bt_group_), 1);
         R += scene_rblocks_[k];
         Bn += scene_bblocks_[k];

@@ -86,6 +86,32 @@ struct SceneMap {
                                        // (0: kRedBlocksMax, the one-scene buffer)
 };
 
+// Final state of one scene solved by the per-scene work-queue kernel (k_apgd_queue).
+struct QueueResult {
+    int64_t iter, iter32, backtracks, q_applies;
+    double res, L_acc;
+    int32_t status, pad;
+};
+
+// Arguments of the work-queue kernel: the scenes it solves, largest first, and the solve's
+// parameters (the FP32 phase's stop for FP32 + FP64 refinement).
+struct QueueArgs {
+    const int32_t* order = nullptr;   // scenes to solve, in the order they are taken
+    int32_t nqueue = 0;
+    int32_t* next = nullptr;          // work counter, 0 at launch
+    const int64_t* crange = nullptr;  // per scene: contacts [lo, hi)
+    const int64_t* arange = nullptr;  // per scene: active particles [lo, hi)
+    QueueResult* result = nullptr;    // per scene
+    const int* tstart = nullptr;      // particle-major copy of B: entries of each active particle
+    const int32_t* tc = nullptr;
+    const int32_t* ci = nullptr;
+    const int32_t* cj = nullptr;
+    int64_t ni = 0;                   // obstacles before the active particles
+    double alpha = 0, tol = 0, tol32 = 0, rtol32 = 0, dt2 = 0;
+    int64_t max_ite = 0;
+    int32_t method = 0, dynamic = 0, warm = 0, fp32 = 0;
+};
+
 // Host description of a scene set (scenes in order, consecutive ranges).
 struct SceneSet {
     int nscene = 0;
@@ -178,7 +204,14 @@ class Solver {
     template <class T, int LAY>
     void run_phase_scenes(BMatrix<T>& B, const Contacts& C, int64_t na, int64_t nc, const scopi_hip_params& prm,
                           double dt, double tol, double rtol, int64_t max_ite, bool warm, bool keep_L, hipStream_t s,
-                          std::vector<scopi_hip_solve_info>& info);
+                          const std::vector<int>& scenes, std::vector<scopi_hip_solve_info>& info);
+    // the scenes of `scenes` solved whole by the per-scene work-queue kernel (both phases), on the
+    // queue stream after the work enqueued so far on `s`; queue_join makes `s` wait for it and
+    // reads back the scenes' results
+    template <int LAY>
+    void queue_launch(const Contacts& C, const scopi_hip_params& prm, double dt, hipStream_t s,
+                      const std::vector<int>& scenes);
+    void queue_join(hipStream_t s, const std::vector<int>& scenes, std::vector<scopi_hip_solve_info>& info);
     template <int LAY>
     void solve_layout_scenes(const Particles& P, const Contacts& C, int64_t nc, const scopi_hip_params& prm, double dt,
                              hipStream_t s, std::vector<scopi_hip_solve_info>& info);
@@ -216,6 +249,7 @@ class Solver {
     int last_problem_ = SCOPI_PROBLEM_NOFRICTION;
     int reduce_blocks_ = 1024;  // grid of the per-contact kernels (grid-stride)
     int bt_group_ = 8;          // lanes per particle in k_bt
+    int row_iters_ = 2;         // loop iterations per lane of the per-contact grid below its cap
     int graph_rounds_ = 0;      // > 0: the one-scene loop replays a hipGraph of this many rounds
     bool has_solution_ = false;
     // FrictionFixedPoint: c without S, s_ij, block sums of the error
@@ -247,6 +281,20 @@ class Solver {
     int rgrid_ = 0, bgrid_ = 0, nlist_ = 0;
     std::vector<int> scene_rblocks_, scene_bblocks_;
     std::vector<int32_t> scene_buf_;  // per scene: ctrl copy holding its final state
+    std::vector<int64_t> scene_nc_;   // per scene: contacts of the current solve
+    // per-scene work queue: scenes of at most queue_max_contacts_ contacts (0: none) are solved by
+    // k_apgd_queue, queue_blocks_ workgroups of queue_threads_ threads at most
+    int64_t queue_max_contacts_ = 0;
+    int queue_blocks_ = 1024;
+    int queue_threads_ = 256;
+    hipStream_t qstream_ = nullptr;   // the work-queue launches (default priority)
+    hipEvent_t qfork_ = nullptr, qjoin_ = nullptr;
+    int32_t* d_qorder_ = nullptr;
+    int32_t* h_qorder_ = nullptr;     // pinned
+    int32_t* d_qnext_ = nullptr;
+    QueueResult* d_qres_ = nullptr;
+    QueueResult* h_qres_ = nullptr;   // pinned
+    int qcap_ = 0;
     // kernel timing (HIP events on the solve stream)
     bool timing_ = false;
     double t_bt_ms_ = 0, t_row_ms_ = 0, t_trial_ms_ = 0;

@@ -1775,6 +1775,15 @@ int scopi_hip_batch_remove_scene(scopi_hip_batch* b, int64_t scene_id) {
     });
 }
 
+int scopi_hip_batch_set_tuning(scopi_hip_batch* b, int32_t key, int32_t value) {
+    return bguarded(b, [&]() -> int {
+        if (key == SCOPI_TUNE_GRAPH_ROUNDS) return bfail(b, SCOPI_HIP_ERR_ARG, "batch_set_tuning: no graph rounds");
+        const int rc = b->solver.set_tuning(key, value);
+        if (rc != SCOPI_HIP_OK) return bfail(b, rc, "batch_set_tuning: unknown key or value out of range");
+        return SCOPI_HIP_OK;
+    });
+}
+
 int scopi_hip_batch_step(scopi_hip_batch* b, int64_t n_steps) {
     return bguarded(b, [&]() -> int {
         if (n_steps < 0) return bfail(b, SCOPI_HIP_ERR_ARG, "batch_step: n_steps < 0");

@@ -34,9 +34,11 @@ EXPORTS = [
     "scopi_hip_batch_scene_info", "scopi_hip_batch_download_contacts", "scopi_hip_batch_download_solution",
     "scopi_hip_scene_footprint", "scopi_hip_set_box", "scopi_hip_set_projection_velocity",
     "scopi_hip_set_contact_gamma", "scopi_hip_download_contact_gamma", "scopi_hip_se_closest_points_noalloc",
+    "scopi_hip_batch_set_tuning",
 ]
 NOALLOC_MAX_PAIRS = 256  # SCOPI_HIP_NOALLOC_MAX_PAIRS
 TUNE_REDUCE_BLOCKS, TUNE_BT_GROUP, TUNE_GRAPH_ROUNDS = 1, 2, 3
+TUNE_QUEUE_MAX_CONTACTS, TUNE_QUEUE_BLOCKS, TUNE_QUEUE_THREADS, TUNE_ROW_ITERS = 4, 5, 6, 7
 
 
 class Params(C.Structure):
@@ -120,6 +122,7 @@ def lib():
                                             C.POINTER(C.c_int64)]
     L.scopi_hip_batch_remove_scene.argtypes = [P, C.c_int64]
     L.scopi_hip_batch_step.argtypes = [P, C.c_int64]
+    L.scopi_hip_batch_set_tuning.argtypes = [P, C.c_int32, C.c_int32]
     L.scopi_hip_batch_size.argtypes = [P, C.c_void_p, C.c_void_p, C.c_void_p]
     L.scopi_hip_batch_download_scene.argtypes = [P, C.c_int64, Dn, Dn, Dn, Dn]
     L.scopi_hip_batch_scene_info.argtypes = [P, C.c_int64, C.c_void_p, C.c_void_p]
@@ -403,6 +406,9 @@ class HipBatch:
 
     def step(self, n_steps=1):
         self._check(self._L.scopi_hip_batch_step(self._h, int(n_steps)), "batch_step")
+
+    def set_tuning(self, key, value):
+        self._check(self._L.scopi_hip_batch_set_tuning(self._h, int(key), int(value)), "batch_set_tuning")
 
     def size(self):
         a, b, c = C.c_int64(), C.c_int64(), C.c_int64()
