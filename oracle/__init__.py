@@ -93,6 +93,7 @@ def lib():
         L.oracle_set_jacobian_ad.argtypes = [C.c_int32]
         L.oracle_dogleg_fresh_wa4_count.restype = C.c_int64
         L.oracle_gen_segment_demo.argtypes = [C.c_int32, C.c_double, D, D, D]
+        L.oracle_gen_box_spheres_demo.argtypes = [C.c_int32, C.c_double, D, D, D, D, D]
         L.oracle_minpack_dogleg.argtypes = [C.c_int32, D, D, D, C.c_double, C.c_double, D, I32]
         L.oracle_newton_closest_points.argtypes = [C.c_int32, D, D, D, D, C.c_int32, D, D, D, D, D, D, D, D, D, D, I32]
         L.oracle_newton_residual.argtypes = [C.c_int32] + [D] * 8 + [D, C.c_void_p]
@@ -389,6 +390,15 @@ def gen_segment_demo(n=50, max_radius=0.06):
     x = np.zeros(n); y = np.zeros(n); r = np.zeros(n)
     lib().oracle_gen_segment_demo(n, max_radius, x, y, r)
     return x, y, r
+
+
+def gen_box_spheres_demo(n_side=10, r0=0.5):
+    """The random draws of demos/box_spheres_3d.cpp (radius, mass, dx, dy, dz per sphere, in the demo's
+    i, j, k loop order), std::default_random_engine."""
+    n = n_side ** 3
+    out = [np.zeros(n) for _ in range(5)]
+    lib().oracle_gen_box_spheres_demo(n_side, r0, *out)
+    return tuple(out)
 
 
 def se2_closest_points(pos_i, q_i, size_i, e_i, pos_j, q_j, size_j, e_j):

@@ -1867,4 +1867,23 @@ int oracle_gen_segment_demo(int32_t n, double max_radius, double* x, double* y, 
     return 0;
 }
 
+// The random draws of demos/box_spheres_3d.cpp:73-95, in the demo's order: for each sphere (i, j, k
+// loops) its radius U(0.8 r0, 0.9 r0), mass U(1, 2) and centre offsets dx, dy, dz U(-0.05, 0.05), all
+// from one std::default_random_engine.
+int oracle_gen_box_spheres_demo(int32_t n_side, double r0, double* r, double* m, double* dx, double* dy, double* dz) {
+    std::default_random_engine generator;
+    std::uniform_real_distribution<double> distrib_r(0.8 * r0, 0.9 * r0);
+    std::uniform_real_distribution<double> distrib_m(1., 2.);
+    std::uniform_real_distribution<double> distrib_mouve_center(-0.05, 0.05);
+    const int n = n_side * n_side * n_side;
+    for (int k = 0; k < n; ++k) {
+        r[k] = distrib_r(generator);
+        m[k] = distrib_m(generator);
+        dx[k] = distrib_mouve_center(generator);
+        dy[k] = distrib_mouve_center(generator);
+        dz[k] = distrib_mouve_center(generator);
+    }
+    return 0;
+}
+
 }  // extern "C"

@@ -231,6 +231,23 @@ template <std::size_t dim>
 struct segment : object<dim> {
     segment(type::position_t<dim> pos, type::quaternion_t q, double length)
         : object<dim>{SCOPI_SHAPE_SEGMENT, pos, q, {length, 0, 0}} {}
+    // segment(pos1, pos2) (objects/types/segment.hpp:133-138): centre at the midpoint, angle
+    // atan2(x1 - x2, y2 - y1) about z, length |pos2 - pos1|
+    segment(const type::position_t<dim>& pos1, const type::position_t<dim>& pos2)
+        : segment(midpoint(pos1, pos2), quaternion(std::atan2(pos1[0] - pos2[0], pos2[1] - pos1[1])),
+                  distance(pos1, pos2)) {}
+
+  private:
+    static type::position_t<dim> midpoint(const type::position_t<dim>& a, const type::position_t<dim>& b) {
+        type::position_t<dim> c;
+        for (std::size_t d = 0; d < dim; ++d) c[d] = 0.5 * (a[d] + b[d]);
+        return c;
+    }
+    static double distance(const type::position_t<dim>& a, const type::position_t<dim>& b) {
+        double s = 0.0;
+        for (std::size_t d = 0; d < dim; ++d) s += (b[d] - a[d]) * (b[d] - a[d]);
+        return std::sqrt(s);
+    }
 };
 
 // ------------------------------------------------------------------ container

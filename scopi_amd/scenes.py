@@ -300,6 +300,31 @@ def segment_demo(x, y, r):
     return c.arrays()
 
 
+def box_spheres_demo(n_side, r0, g, r, m, dx, dy, dz):
+    """The scene of demos/box_spheres_3d.cpp:12-95: five obstacle planes (x = 0 and x = W + 2 r0 with
+    normal x, y = 0 with normal y, z = 0 and z = W + 2 r0 with the y-rotated normal), then n_side^3
+    spheres at (r0 + 0.1 + 2 i r0 + dx, 1.1 r0 + 2.1 r0 j + dy, r0 + 0.1 + 2 k r0 + dz), loops i, j, k
+    outermost first, with force (0, -g, 0), mass m and moment of inertia m r^2 / 2 about each axis;
+    W = 2 n_side r0.  (r, m, dx, dy, dz) are the demo's draws in its loop order."""
+    w = 2.0 * n_side * r0
+    c = Container(3)
+    c.plane((0.0, 0.0, 0.0), q=quaternion(0.0, (0.0, 0.0, 1.0)))
+    c.plane((w + 2 * r0, 0.0, 0.0), q=quaternion(0.0, (0.0, 0.0, 1.0)))
+    c.plane((0.0, 0.0, 0.0), q=quaternion(math.pi / 2.0, (0.0, 0.0, 1.0)))
+    c.plane((0.0, 0.0, 0.0), q=quaternion(math.pi / 2.0, (0.0, 1.0, 0.0)))
+    c.plane((0.0, 0.0, w + 2 * r0), q=quaternion(math.pi / 2.0, (0.0, 1.0, 0.0)))
+    k = 0
+    for i in range(n_side):
+        for j in range(n_side):
+            for l in range(n_side):
+                ri, mi = r[k], m[k]
+                c.sphere((r0 + 0.1 + i * 2.0 * r0 + dx[k], 1.1 * r0 + 2.1 * r0 * j + dy[k], r0 + 0.1 + l * 2.0 * r0 + dz[k]),
+                         ri, Property(force=(0.0, -g, 0.0), mass=mi,
+                                      moment_inertia=(mi * ri * ri / 2.0, mi * ri * ri / 2.0, mi * ri * ri / 2.0)))
+                k += 1
+    return c.arrays()
+
+
 # Default solver parameters of the workloads (BASELINE configurations).
 CONFIG1 = dict(dim=2, dt=0.005, dmax=0.025, kd_tree_radius=(0.024 + 0.025) ** 2, tolerance=1e-6, steps=200)
 CONFIG2 = dict(dim=3, dt=1e-3, dmax=0.02, kd_tree_radius=0.042 ** 2, tolerance=1e-6)

@@ -458,6 +458,104 @@ void superellipsoids_2d() {
     std::fflush(stdout);
 }
 
+// Object states after a run: position, velocity (and the 3D rotation velocity) of every object.
+template <std::size_t dim>
+void dump_state(const char* name, scopi_container<dim>& p, std::size_t steps) {
+    std::printf("{\"case\": \"%s\", \"steps\": %zu, \"pos\": [", name, steps);
+    for (std::size_t k = 0; k < p.size(); ++k)
+        for (std::size_t d = 0; d < dim; ++d) std::printf("%s%.17g", (k || d) ? ", " : "", p.pos()[k][d]);
+    std::printf("], \"v\": [");
+    for (std::size_t k = 0; k < p.size(); ++k)
+        for (std::size_t d = 0; d < dim; ++d) std::printf("%s%.17g", (k || d) ? ", " : "", p.v()[k][d]);
+    std::printf("]}\n");
+    std::fflush(stdout);
+}
+
+// demos/box_spheres_3d.cpp:10-112 written against the slot mirror, run for total_it steps (the demo
+// runs 1000): the demo's scene, draws, solver slots and parameters; its xtensor vectors are the
+// mirror's fixed arrays.  The solver writes its output files every step, as the demo's does.
+void box_spheres_3d_demo(std::size_t n, std::size_t total_it) {
+    constexpr std::size_t dim = 3;
+    const double PI = 3.14159265358979323846;
+    double width_box = 10.;
+    double g = 1.;
+    double r0 = width_box / n / 2.;
+    double dt = 0.2 * 0.9 * r0 / (2. * g);
+    double rho = 0.2 / (dt * dt);
+    scopi_container<dim> particles;
+    auto prop = property<dim>().force({{0., -g, 0.}});
+    const fixed<3> axes_y{{0., 1., 0.}};
+    const fixed<3> axes_z{{0., 0., 1.}};
+    plane<dim> p_left({{0., 0., 0.}}, {quaternion(0., axes_z)});
+    particles.push_back(p_left, property<dim>().deactivate());
+    plane<dim> p_right({{width_box + 2 * r0, 0., 0.}}, {quaternion(0., axes_z)});
+    particles.push_back(p_right, property<dim>().deactivate());
+    plane<dim> p_horizontal({{0., 0., 0.}}, {quaternion(PI / 2., axes_z)});
+    particles.push_back(p_horizontal, property<dim>().deactivate());
+    plane<dim> p_front({{0., 0., 0.}}, {quaternion(PI / 2., axes_y)});
+    particles.push_back(p_front, property<dim>().deactivate());
+    plane<dim> p_back({{0., 0., width_box + 2 * r0}}, {quaternion(PI / 2., axes_y)});
+    particles.push_back(p_back, property<dim>().deactivate());
+    std::default_random_engine generator;
+    std::uniform_real_distribution<double> distrib_r(0.8 * r0, 0.9 * r0);
+    std::uniform_real_distribution<double> distrib_m(1., 2.);
+    std::uniform_real_distribution<double> distrib_mouve_center(-0.05, 0.05);
+    for (std::size_t i = 0; i < n; ++i)
+        for (std::size_t j = 0; j < n; ++j)
+            for (std::size_t k = 0; k < n; ++k) {
+                double r = distrib_r(generator);
+                double m = distrib_m(generator);
+                double dx = distrib_mouve_center(generator);
+                double dy = distrib_mouve_center(generator);
+                double dz = distrib_mouve_center(generator);
+                sphere<dim> s({{r0 + 0.1 + i * 2. * r0 + dx, 1.1 * r0 + 2.1 * r0 * j + dy, r0 + 0.1 + k * 2. * r0 + dz}},
+                              r);
+                particles.push_back(s, prop.mass(m).moment_inertia({{m * r * r / 2., m * r * r / 2., m * r * r / 2.}}));
+            }
+    dump_state("box_spheres_3d_initial", particles, 0);
+    ScopiSolver<dim, NoFriction, OptimGradient<apgd>, contact_kdtree, vap_fpd> solver(particles);
+    auto params = solver.get_params();
+    params.optim_params.tolerance = 1e-3;
+    params.optim_params.alpha = rho;
+    params.contact_method_params.dmax = 0.9 * r0;
+    params.contact_method_params.kd_tree_radius = params.contact_method_params.dmax + 2. * 0.9 * r0;
+    solver.run(dt, total_it);
+    dump_state("box_spheres_3d", particles, total_it);
+}
+
+// demos/segment.cpp:8-60 written against the slot mirror (default slots: NoFriction, APGD,
+// kd-tree, vap_fixed), run for total_it steps (the demo runs 1000): two segment obstacles built from
+// their end points, n_parts spheres from std::default_random_engine, dmax 2 dt, kd_tree_radius
+// 4 max_radius.  The demo's command-line options keep their defaults.
+void segment_demo(std::size_t n_parts, std::size_t total_it) {
+    constexpr std::size_t dim = 2;
+    double dt = .001;
+    const double max_radius = 0.06;
+    scopi_container<dim> particles;
+    ScopiSolver<dim> solver(particles);
+    segment<dim> seg1(type::position_t<dim>{0., 1.}, type::position_t<dim>{0.4, 1.});
+    segment<dim> seg2(type::position_t<dim>{0.6, 1.}, type::position_t<dim>{1., 1.});
+    particles.push_back(seg1, property<dim>().deactivate());
+    particles.push_back(seg2, property<dim>().deactivate());
+    std::default_random_engine generator;
+    std::uniform_real_distribution<double> distrib_x(0.1, 0.9);
+    std::uniform_real_distribution<double> distrib_y(0, 0.7);
+    std::uniform_real_distribution<double> distrib_r(0.01, max_radius);
+    for (std::size_t i = 0; i < n_parts; ++i) {
+        auto x = distrib_x(generator);
+        auto y = distrib_y(generator);
+        auto radius = distrib_r(generator);
+        auto prop = property<dim>().desired_velocity({{0.5 - x, 2 - y}}).mass(1.).moment_inertia(0.1);
+        particles.push_back(sphere<dim>({{x, y}}, radius), prop);
+    }
+    dump_state("segment_initial", particles, 0);
+    auto params = solver.get_params();
+    params.contact_method_params.dmax = 2 * dt;
+    params.contact_method_params.kd_tree_radius = 4 * max_radius;
+    solver.run(dt, total_it);
+    dump_state("segment", particles, total_it);
+}
+
 int main(int argc, char** argv) {
     std::string only = argc > 1 ? argv[1] : "";
     using kd_fixed = ScopiSolver<2, NoFriction, OptimGradient<apgd>, contact_kdtree, vap_fixed>;
@@ -490,6 +588,13 @@ int main(int argc, char** argv) {
             superellipsoids_2d();
         }
         if (only.empty() || only == "periodic") periodic_pair();
+        // the demos run only when asked for, with their step counts: demos <box steps> <segment steps>
+        if (only == "demos") {
+            const std::size_t box_steps = argc > 2 ? std::stoul(argv[2]) : 30;
+            const std::size_t seg_steps = argc > 3 ? std::stoul(argv[3]) : 200;
+            box_spheres_3d_demo(10, box_steps);
+            segment_demo(50, seg_steps);
+        }
     } catch (const std::exception& e) {
         std::fprintf(stderr, "test_slots: %s\n", e.what());
         return 1;
