@@ -182,6 +182,8 @@ def bench_superellipsoids(args):
     dt = cfg["dt"]
     torch.cuda.init()
     g = hip.HipScene(sc, capacity_contacts=16 * sc.n)
+    if args.row_iters is not None:
+        g.set_tuning(hip.TUNE_ROW_ITERS, args.row_iters)
     log(f"[bench] warmup {args.warmup} steps")
     if args.warmup:
         g.run(prm, dt, args.warmup)
@@ -403,7 +405,8 @@ def main():
                     help="batched: scenes of at most this many contacts go to the per-scene work-queue kernel")
     ap.add_argument("--queue-threads", type=int, default=None, help="batched: work-queue workgroup size")
     ap.add_argument("--queue-blocks", type=int, default=None, help="batched: work-queue workgroups")
-    ap.add_argument("--row-iters", type=int, default=None, help="batched: loop iterations per lane of k_trial")
+    ap.add_argument("--row-iters", type=int, default=None,
+                    help="batched / superellipsoids: loop iterations per lane of k_trial")
     ap.add_argument("--bt-group", type=int, default=None, help="batched: lanes per particle of k_bt")
     ap.add_argument("--nmin", type=int, default=0, help="batched: smallest scene (default 256 particles)")
     ap.add_argument("--nmax", type=int, default=0, help="batched: largest scene (default 32,768 particles)")

@@ -1435,6 +1435,8 @@ struct scopi_hip_batch {
     int hist_nr = 0;
     int64_t* d_tab = nullptr;
     int64_t tab_cap = 0;
+    unsigned char* h_stage = nullptr;  // pinned staging of the fields of scenes being added
+    int64_t stage_cap = 0;             // bytes
     bool has_se = false;       // any scene with superellipsoids: candidate search for the batch
     bool all_spheres = true;   // every active particle of the batch is a sphere
     std::string err;
@@ -1482,15 +1484,70 @@ __global__ void __launch_bounds__(kBlock) k_gather_particles(Particles D, Partic
     for (int d = 0; d < 2; ++d) D.sq[d * D.cap + k] = S.sq[d * S.cap + o];
 }
 
-// host rows [r0, r0 + n) of an (rows x w) array -> device component-major at k0
-void upload_rows_at(double* dev, int64_t cap, int64_t k0, const double* host, int64_t r0, int64_t n, int w,
-                    hipStream_t s) {
-    std::vector<double> t((size_t)w * n);
-    for (int64_t i = 0; i < n; ++i)
-        for (int d = 0; d < w; ++d) t[(size_t)d * n + i] = host[(r0 + i) * w + d];
-    for (int d = 0; d < w; ++d)
-        SCOPI_HIP_CHECK(hipMemcpyAsync(dev + d * cap + k0, t.data() + (size_t)d * n, n * sizeof(double),
-                                       hipMemcpyHostToDevice, s));
+// The fields of the staged scenes (next[kept..]) into Q.  They come after the kept scenes, so their
+// obstacles form one contiguous range of Q and their active particles another: every field component
+// goes up in two copies (obstacles, active particles) from one pinned staging buffer, in which each
+// component is a column of the staged rows (obstacles of every staged scene, then their active
+// particles, scenes in order).
+void upload_staged(scopi_hip_batch* b, Particles& Q, const std::vector<scopi_hip_batch::Scene>& next, size_t kept) {
+    int64_t nob = 0, nac = 0;
+    for (size_t k = kept; k < next.size(); ++k) {
+        nob += next[k].n_inactive;
+        nac += next[k].n - next[k].n_inactive;
+    }
+    const int64_t rows = nob + nac;
+    if (rows == 0) return;
+    using Staged = scopi_hip_batch::Staged;
+    struct Field {
+        double* dev;
+        int w;
+        const std::vector<double> Staged::*host;
+    };
+    const Field fields[] = {{Q.pos, 3, &Staged::pos},       {Q.q, 4, &Staged::q},         {Q.size, 3, &Staged::size},
+                            {Q.sq, 2, &Staged::sq},         {Q.v, 3, &Staged::v},         {Q.vd, 3, &Staged::vd},
+                            {Q.omega, 3, &Staged::omega},   {Q.domega, 3, &Staged::domega}, {Q.f, 3, &Staged::f},
+                            {Q.m, 1, &Staged::m},           {Q.J, 3, &Staged::J}};
+    int cols = 0;
+    for (const auto& f : fields) cols += f.w;
+    const int64_t bytes = (int64_t)cols * rows * (int64_t)sizeof(double) + rows * (int64_t)sizeof(int32_t);
+    hipStream_t s = b->stream;
+    SCOPI_HIP_CHECK(hipStreamSynchronize(s));  // the staging buffer may still feed earlier copies
+    if (bytes > b->stage_cap) {
+        if (b->h_stage) SCOPI_HIP_CHECK(hipHostFree(b->h_stage));
+        b->stage_cap = bytes + bytes / 2;
+        SCOPI_HIP_CHECK(hipHostMalloc(&b->h_stage, b->stage_cap));
+    }
+    double* col = (double*)b->h_stage;
+    int32_t* shp = (int32_t*)(col + (int64_t)cols * rows);
+    // row of staged scene k's obstacle / active particle t within the staging columns
+    int64_t ob = 0, ac = nob;
+    for (size_t k = kept; k < next.size(); ++k) {
+        const Staged& st = b->staged[k - kept];
+        const int64_t ni = next[k].n_inactive, n = next[k].n;
+        for (int64_t t = 0; t < n; ++t) {
+            const int64_t r = t < ni ? ob + t : ac + (t - ni);
+            shp[r] = st.shape[t];
+            int64_t c = 0;
+            for (const auto& f : fields) {
+                const std::vector<double>& h = st.*(f.host);
+                for (int d = 0; d < f.w; ++d, ++c) col[c * rows + r] = h[(size_t)t * f.w + d];
+            }
+        }
+        ob += ni;
+        ac += n - ni;
+    }
+    const int64_t ob0 = next[kept].ob_lo, ac0 = next[kept].a_lo;
+    auto up = [&](void* dst, const void* src, int64_t nbytes) {
+        if (nbytes > 0) SCOPI_HIP_CHECK(hipMemcpyAsync(dst, src, nbytes, hipMemcpyHostToDevice, s));
+    };
+    up(Q.shape + ob0, shp, nob * (int64_t)sizeof(int32_t));
+    up(Q.shape + ac0, shp + nob, nac * (int64_t)sizeof(int32_t));
+    int64_t c = 0;
+    for (const auto& f : fields)
+        for (int d = 0; d < f.w; ++d, ++c) {
+            up(f.dev + d * Q.cap + ob0, col + c * rows, nob * (int64_t)sizeof(double));
+            up(f.dev + d * Q.cap + ac0, col + c * rows + nob, nac * (int64_t)sizeof(double));
+        }
     SCOPI_HIP_CHECK(hipStreamSynchronize(s));
 }
 
@@ -1538,28 +1595,7 @@ void batch_pack(scopi_hip_batch* b) {
         SCOPI_HIP_CHECK(hipStreamSynchronize(s));
         SCOPI_HIP_CHECK(hipFree(dmap));
     }
-    for (size_t k = kept; k < next.size(); ++k) {
-        const auto& st = b->staged[k - kept];
-        const auto& sc = next[k];
-        const int64_t ni = sc.n_inactive, na = sc.n - ni;
-        for (int part = 0; part < 2; ++part) {
-            const int64_t r0 = part ? ni : 0, cnt = part ? na : ni, k0 = part ? sc.a_lo : sc.ob_lo;
-            if (cnt == 0) continue;
-            SCOPI_HIP_CHECK(hipMemcpyAsync(Q.shape + k0, st.shape.data() + r0, cnt * sizeof(int32_t),
-                                           hipMemcpyHostToDevice, s));
-            SCOPI_HIP_CHECK(hipMemcpyAsync(Q.m + k0, st.m.data() + r0, cnt * sizeof(double), hipMemcpyHostToDevice, s));
-            upload_rows_at(Q.pos, Q.cap, k0, st.pos.data(), r0, cnt, 3, s);
-            upload_rows_at(Q.q, Q.cap, k0, st.q.data(), r0, cnt, 4, s);
-            upload_rows_at(Q.size, Q.cap, k0, st.size.data(), r0, cnt, 3, s);
-            upload_rows_at(Q.sq, Q.cap, k0, st.sq.data(), r0, cnt, 2, s);
-            upload_rows_at(Q.v, Q.cap, k0, st.v.data(), r0, cnt, 3, s);
-            upload_rows_at(Q.vd, Q.cap, k0, st.vd.data(), r0, cnt, 3, s);
-            upload_rows_at(Q.omega, Q.cap, k0, st.omega.data(), r0, cnt, 3, s);
-            upload_rows_at(Q.domega, Q.cap, k0, st.domega.data(), r0, cnt, 3, s);
-            upload_rows_at(Q.f, Q.cap, k0, st.f.data(), r0, cnt, 3, s);
-            upload_rows_at(Q.J, Q.cap, k0, st.J.data(), r0, cnt, 3, s);
-        }
-    }
+    upload_staged(b, Q, next, kept);
     particles_free(b->P);
     b->P = Q;
     b->scenes.swap(next);
@@ -1705,6 +1741,7 @@ int scopi_hip_batch_destroy(scopi_hip_batch* b) {
     contacts_free(b->C);
     for (void* p : {(void*)b->hist_keys, (void*)b->cur_keys, (void*)b->hist_lam, (void*)b->lam0, (void*)b->d_tab})
         if (p) (void)hipFree(p);
+    if (b->h_stage) (void)hipHostFree(b->h_stage);
     if (b->stream) (void)hipStreamDestroy(b->stream);
     delete b;
     return SCOPI_HIP_OK;
