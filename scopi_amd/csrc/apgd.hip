@@ -1332,6 +1332,8 @@ Solver::~Solver() {
         if (p) (void)hipHostFree(p);
     for (void* p : {(void*)d_qorder_, (void*)d_qnext_, (void*)d_qres_})
         if (p) (void)hipFree(p);
+    for (hipGraphExec_t e : {graph_exec32_, graph_exec64_})
+        if (e) (void)hipGraphExecDestroy(e);
     if (qstream_) {
         (void)hipStreamSynchronize(qstream_);
         (void)hipStreamDestroy(qstream_);
@@ -1437,9 +1439,11 @@ void Solver::run_phase(BMatrix<T>& B, const Particles& P, const Contacts& C, int
         // hipGraph of graph_rounds_ rounds, the control step and the state's copy to the host,
         // replayed until the loop is done (rounds after `done` exit at their first instruction).
         // graph_rounds_ is odd: the rounds and the control step swap the state copies an even
-        // number of times, so every replay starts from copy 0 like the first.
+        // number of times, so every replay starts from copy 0 like the first.  The executable graph
+        // of each precision is kept across solves: a new capture (the launch shapes and arguments of
+        // this solve) updates it in place when only parameters changed, and is instantiated otherwise.
         hipGraph_t graph = nullptr;
-        hipGraphExec_t exec = nullptr;
+        hipGraphExec_t& exec = sizeof(T) == 4 ? graph_exec32_ : graph_exec64_;
         SCOPI_HIP_CHECK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
         int c = 0;
         for (int r = 0; r < graph_rounds_; ++r) {
@@ -1454,14 +1458,22 @@ void Solver::run_phase(BMatrix<T>& B, const Particles& P, const Contacts& C, int
         c = 1 - c;
         SCOPI_HIP_CHECK(hipMemcpyAsync(h_ctrl_, ctrl_ + c, sizeof(ApgdCtrl), hipMemcpyDeviceToHost, s));
         SCOPI_HIP_CHECK(hipStreamEndCapture(s, &graph));
-        SCOPI_HIP_CHECK(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
+        if (exec) {
+            hipGraphNode_t bad = nullptr;
+            hipGraphExecUpdateResult res = hipGraphExecUpdateError;
+            if (hipGraphExecUpdate(exec, graph, &bad, &res) != hipSuccess || res != hipGraphExecUpdateSuccess) {
+                (void)hipGetLastError();
+                SCOPI_HIP_CHECK(hipGraphExecDestroy(exec));
+                exec = nullptr;
+            }
+        }
+        if (!exec) SCOPI_HIP_CHECK(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
+        SCOPI_HIP_CHECK(hipGraphDestroy(graph));
         for (;;) {
             SCOPI_HIP_CHECK(hipGraphLaunch(exec, s));
             SCOPI_HIP_CHECK(hipStreamSynchronize(s));
             if (h_ctrl_->done) break;
         }
-        SCOPI_HIP_CHECK(hipGraphExecDestroy(exec));
-        SCOPI_HIP_CHECK(hipGraphDestroy(graph));
         cur = c;
     }
     for (; !(graph_rounds_ > 0 && !timing_);) {

@@ -12,10 +12,11 @@ from __future__ import annotations
 
 import time
 
-from .hip import TUNE_BT_GROUP, TUNE_REDUCE_BLOCKS, HipScene, Params
+from .hip import TUNE_BT_GROUP, TUNE_GRAPH_ROUNDS, TUNE_REDUCE_BLOCKS, HipScene, Params
 
 BT_GROUPS = (1, 2, 4, 8, 16)
 REDUCE_BLOCKS = (256, 512, 1024, 2048)
+GRAPH_ROUNDS = (15, 63)
 
 
 def _time_solve(hs: HipScene, prm: Params, dt: float, repeats: int) -> float:
@@ -27,11 +28,13 @@ def _time_solve(hs: HipScene, prm: Params, dt: float, repeats: int) -> float:
     return best
 
 
-def tune(hs: HipScene, prm: Params, dt: float, repeats: int = 2, groups=BT_GROUPS, blocks=REDUCE_BLOCKS):
-    """Sweep the launch shapes on the contacts currently held by `hs`.
+def tune(hs: HipScene, prm: Params, dt: float, repeats: int = 2, groups=BT_GROUPS, blocks=REDUCE_BLOCKS,
+         graph_rounds=GRAPH_ROUNDS):
+    """Sweep the launch shapes on the contacts currently held by `hs`, then whether a hipGraph of
+    rounds (bitwise the eager loop) is faster on them.
 
     The caller has run compute_contacts and set_a_priori_velocity.  Returns
-    {"bt_group", "reduce_blocks", "seconds", "table"} and leaves the best shape
+    {"bt_group", "reduce_blocks", "graph_rounds", "seconds", "table"} and leaves the best shape
     set on `hs`.
     """
     hs.solve(prm, dt)  # first solve allocates the solver's buffers
@@ -48,8 +51,17 @@ def tune(hs: HipScene, prm: Params, dt: float, repeats: int = 2, groups=BT_GROUP
     g, b = min(table, key=table.get)
     hs.set_tuning(TUNE_BT_GROUP, g)
     hs.set_tuning(TUNE_REDUCE_BLOCKS, b)
-    return {"bt_group": g, "reduce_blocks": b, "seconds": table[(g, b)],
-            "table": {f"{k[0]}x{k[1]}": v for k, v in table.items()}}
+    best_r, best_t = 0, table[(g, b)]
+    graphs = {}
+    for r in graph_rounds:
+        hs.set_tuning(TUNE_GRAPH_ROUNDS, r)
+        graphs[r] = _time_solve(hs, prm, dt, repeats)
+        if graphs[r] < best_t:
+            best_r, best_t = r, graphs[r]
+    hs.set_tuning(TUNE_GRAPH_ROUNDS, best_r)
+    table_out = {f"{k[0]}x{k[1]}": v for k, v in table.items()}
+    table_out.update({f"graph{r}": v for r, v in graphs.items()})
+    return {"bt_group": g, "reduce_blocks": b, "graph_rounds": best_r, "seconds": best_t, "table": table_out}
 
 
 # Batch schedules: loop iterations per lane of the per-contact grid, lanes per particle of the

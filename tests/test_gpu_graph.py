@@ -35,3 +35,21 @@ def test_graph_rounds_must_be_odd():
     g = hip.HipScene(scenes.spheres_3d_count(64, seed=1, exact=True))
     with pytest.raises(hip.ScopiHipError):
         g.set_tuning(hip.TUNE_GRAPH_ROUNDS, 16)
+
+
+def test_graph_kept_across_solves_and_shapes():
+    """The executable graph is kept across solves and updated in place; a change of the round count (another
+    topology) instantiates a new one.  Steps with 15-round graphs, then 3-round graphs, then eager, against
+    the eager loop throughout."""
+    sc = scenes.spheres_3d_count(2000, seed=9, exact=True)
+    prm = hip.params(vap=hip.VAP_FPD, dmax=0.02, kd_tree_radius=0.042 ** 2, tolerance=1e-6,
+                     precision=hip.FP32_REFINE)
+    a = hip.HipScene(sc)
+    a.run(prm, 1e-3, 7)
+    b = hip.HipScene(sc)
+    for rounds, steps in ((15, 3), (3, 2), (0, 2)):
+        b.set_tuning(hip.TUNE_GRAPH_ROUNDS, rounds)
+        b.run(prm, 1e-3, steps)
+    sa, sb = a.state(), b.state()
+    for k in sa:
+        assert np.array_equal(sa[k], sb[k]), k
