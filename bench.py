@@ -406,8 +406,9 @@ def main():
     ap.add_argument("--queue-threads", type=int, default=None, help="batched: work-queue workgroup size")
     ap.add_argument("--queue-blocks", type=int, default=None, help="batched: work-queue workgroups")
     ap.add_argument("--row-iters", type=int, default=None,
-                    help="batched / superellipsoids: loop iterations per lane of k_trial")
+                    help="spheres / batched / superellipsoids: loop iterations per lane of k_trial")
     ap.add_argument("--bt-group", type=int, default=None, help="spheres / batched: lanes per particle of k_bt")
+    ap.add_argument("--reduce-blocks", type=int, default=None, help="spheres: block cap of the per-contact grid")
     ap.add_argument("--nmin", type=int, default=0, help="batched: smallest scene (default 256 particles)")
     ap.add_argument("--nmax", type=int, default=0, help="batched: largest scene (default 32,768 particles)")
     ap.add_argument("--no-tune", action="store_true",
@@ -448,6 +449,10 @@ def main():
         g.set_tuning(hip.TUNE_GRAPH_ROUNDS, args.graph_rounds)
     if args.bt_group is not None:
         g.set_tuning(hip.TUNE_BT_GROUP, args.bt_group)
+    if args.reduce_blocks is not None:
+        g.set_tuning(hip.TUNE_REDUCE_BLOCKS, args.reduce_blocks)
+    if args.row_iters is not None:
+        g.set_tuning(hip.TUNE_ROW_ITERS, args.row_iters)
     log(f"[bench] warmup {args.warmup} steps")
     if args.warmup:
         g.run(prm, dt, args.warmup)
