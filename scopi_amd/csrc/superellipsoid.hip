@@ -1628,18 +1628,24 @@ __global__ void __launch_bounds__(kBlock) k_se_points(Particles P, SeGrids G) {
     }
 }
 
-// Penalised grid minimum: one workgroup of kSeGridBlock lanes per candidate pair.
-// Lane l owns grid points i = l, l + B, l + 2B of body 1 and scans body 2's grid
-// (staged in LDS) in order; its first strict minimum per i, then the (value, index)
-// minimum over lanes, is the first minimum of the row-major scan.  A term whose
-// penalty 2 (1 + n1.n2) already reaches the lane's best cannot win (the distance
-// term is >= 0), nor one whose squared distance exceeds (best - penalty)^2 by more
-// than the rounding margin, so their square roots are skipped.
+// Penalised grid minimum: one workgroup of kSeGridBlock lanes per candidate pair, the first
+// minimum of fl(sqrt(d2)) + 2 (1 + n1.n2) over the row-major scan of (i, j) (closest_points.hpp:
+// 604-690): the minimum value, and among equal values the smallest index i n2 + j.  Lane l owns grid
+// points i = l, l + B, l + 2B of body 1 and scans body 2's grid (staged in LDS) in order, keeping its
+// own (value, index) minimum; the workgroup's minimum is then the (value, index) minimum over lanes.
+// Only that minimum matters, so a term may be skipped when it is provably above an upper bound g of
+// it: g is the lowest term value any lane has found, shared through LDS every 32 points of body 2
+// (a value read there is some lane's found value, old or new, so any read is a valid bound).  A term
+// whose penalty exceeds g is above g (the distance term is >= 0), and so is one whose squared
+// distance exceeds (g - penalty + 1e-15 g)^2 (1 + 1e-14), the margins covering the roundings of the
+// term; their square roots are skipped.  Terms equal to g are never skipped, so the minimum and its
+// index are those of the full scan, bit for bit.
 constexpr int kSeGridBlock = 192;  // 3 x 192 = 576
 __global__ void __launch_bounds__(kSeGridBlock) k_se_grid(Particles P, SeGrids G, SeCandidates S, int64_t n) {
     __shared__ double g2[6][kSeGridPoints];
     __shared__ double red_v[kSeGridBlock];
     __shared__ int red_i[kSeGridBlock];
+    __shared__ double wbest[kSeGridBlock / 64];
     const int64_t c = blockIdx.x;
     if (c >= n) return;
     const int64_t i = S.i[c], j = S.j[c];
@@ -1651,54 +1657,51 @@ __global__ void __launch_bounds__(kSeGridBlock) k_se_grid(Particles P, SeGrids G
     const double* src2 = G.pts + 6 * kSeGridPoints * t2;
     for (int k = threadIdx.x; k < n2; k += kSeGridBlock)
         for (int d = 0; d < 6; ++d) g2[d][k] = src2[d * kSeGridPoints + k];
+    if (threadIdx.x < kSeGridBlock / 64) wbest[threadIdx.x] = INFINITY;
     const double* src1 = G.pts + 6 * kSeGridPoints * t1;
-    double p1[3][3], m1[3][3], best[3];
-    int bj[3];
+    double p1[3][3], m1[3][3];
     bool ok[3];
     for (int r = 0; r < 3; ++r) {
         const int ii = threadIdx.x + r * kSeGridBlock;
         ok[r] = ii < n1;
-        best[r] = INFINITY;
-        bj[r] = -1;
         for (int d = 0; d < 3; ++d) {
             p1[r][d] = ok[r] ? src1[d * kSeGridPoints + ii] : 0.0;
             m1[r][d] = ok[r] ? src1[(3 + d) * kSeGridPoints + ii] : 0.0;
         }
     }
     __syncthreads();
+    volatile double* vb = wbest;
+    const int wave = threadIdx.x >> 6;
+    double bv = INFINITY;   // the lane's minimum ...
+    int bidx = 0x7fffffff;  // ... and its row-major index
+    double g = INFINITY;    // an upper bound of the workgroup's minimum
     for (int jj = 0; jj < n2; ++jj) {
+        if ((jj & 31) == 0 && jj > 0) {
+            double w = bv;
+#pragma unroll
+            for (int o = 32; o > 0; o >>= 1) w = fmin(w, __shfl_xor(w, o, 64));
+            if ((threadIdx.x & 63) == 0 && w < vb[wave]) vb[wave] = w;
+#pragma unroll
+            for (int k = 0; k < kSeGridBlock / 64; ++k) g = fmin(g, vb[k]);
+        }
         const double nx = g2[3][jj], ny = g2[4][jj], nz = g2[5][jj];
 #pragma unroll
         for (int r = 0; r < 3; ++r) {
             const double pen = 2 * (1 + (m1[r][0] * nx + m1[r][1] * ny + m1[r][2] * nz));
-            if (ok[r] && pen < best[r]) {
+            if (ok[r] && pen <= g) {
                 const double dx = p1[r][0] - g2[0][jj], dy = p1[r][1] - g2[1][jj], dz = p1[r][2] - g2[2][jj];
                 const double d2 = dx * dx + dy * dy + dz * dz;
-                // The term fl(sqrt(d2)) + pen cannot beat best when sqrt(d2) (1 - eps) >= best - pen
-                // (rounding is monotone and best is a double).  With T = fl(best - pen) and
-                // TT = fl(T T) normal, d2 > TT (1 + 1e-14) implies it (the factor covers the four
-                // roundings involved), so the square root is taken only for terms that may win:
-                // the same minimum, the same index.
-                const double t = best[r] - pen, tt = t * t;
-                if (!(tt > 1e-300 && d2 > tt * (1.0 + 1e-14))) {
+                const double u = (g - pen) + 1e-15 * g;
+                if (!(u > 0 && d2 > u * u * (1.0 + 1e-14))) {
                     const double dist = sqrt(d2) + pen;
-                    if (dist < best[r]) {
-                        best[r] = dist;
-                        bj[r] = jj;
+                    const int idx = (threadIdx.x + r * kSeGridBlock) * n2 + jj;
+                    if (dist < bv || (dist == bv && idx < bidx)) {
+                        bv = dist;
+                        bidx = idx;
+                        if (dist < g) g = dist;
                     }
                 }
             }
-        }
-    }
-    // lane minimum in row-major order (i ascending over r), then across lanes
-    double bv = INFINITY;
-    int bidx = 0x7fffffff;
-    for (int r = 0; r < 3; ++r) {
-        if (bj[r] < 0) continue;
-        const int idx = (threadIdx.x + r * kSeGridBlock) * n2 + bj[r];
-        if (best[r] < bv || (best[r] == bv && idx < bidx)) {
-            bv = best[r];
-            bidx = idx;
         }
     }
     red_v[threadIdx.x] = bv;
@@ -1761,7 +1764,6 @@ __global__ void __launch_bounds__(kSolveBlock) k_se_solve(Particles P, SeCandida
                     c = atomicAdd(S.counter, 1);
                     continue;
                 }
-                double pi[3], pj[3], nij[3];
                 {
                     CP cp;
                     const bool ok = closest_points<3>(P, i, j, cp);
