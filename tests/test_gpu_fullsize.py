@@ -30,6 +30,8 @@ import pytest
 
 from scopi_amd import hip, scenes
 
+from tests.cones import cone_projection
+
 pytestmark = pytest.mark.gpu
 
 HERE = os.path.dirname(os.path.abspath(__file__))
@@ -89,20 +91,6 @@ def test_c2_one_step_kkt_and_precisions(oracle_lib):
     d32 = float(np.abs(sol["fp32"]["u"] - sol["tight"]["u"]).max())
     say(f"  max|u_fp64 - u_tight| {reach:.3e}, max|u_fp32 - u_tight| {d32:.3e}")
     assert d32 <= max(10 * reach, 1e-6)
-
-
-def cone_projection(v, n, mu):
-    """Projection of 3-vectors v on the Coulomb cones {|x_t| <= mu x_n} of normals n."""
-    vn = (v * n).sum(1)
-    vt = v - vn[:, None] * n
-    nt = np.linalg.norm(vt, axis=1)
-    out = v.copy()
-    zero = mu * nt <= -vn
-    bd = ~(zero | (nt <= mu * vn))
-    a = (vn[bd] + mu * nt[bd]) / (1 + mu * mu)
-    out[bd] = a[:, None] * (n[bd] + mu * vt[bd] / nt[bd][:, None])
-    out[zero] = 0.0
-    return out
 
 
 def test_c3_one_step_invariants(oracle_lib):

@@ -188,3 +188,72 @@ def test_queue_split_batch(precision, warm):
             same_state(g.state(), b.state(sid), f"rounds n={sc.n}")
             g.close()
         assert b.info(sid)["converged"] == 1
+
+
+@pytest.mark.parametrize("name", ["lattice_3d_nofriction", "random_3d_nofriction"])
+def test_queue_general_jacobian_layout(name):
+    """The rigid layout (rotational blocks kept, general_jacobian=1) through the queue: the fixed-step
+    solution within 1e-8 of the exact optimum, as test_gpu_parity.test_general_jacobian_layout holds
+    the two-launch rounds."""
+    case = solve_cases.CASES[name]
+    alpha = 1.0 / float(GOLDEN[name + "/L"])
+    prm = case_params(case, dynamic_descent=0, alpha=alpha, tolerance=1e-10, general_jacobian=1)
+    say(f"[queue-rigid] {name}")
+    u, info = first_step_u(case, prm, 512)
+    assert info["converged"] == 1
+    np.testing.assert_allclose(u, GOLDEN[name + "/u_exact"], atol=1e-8)
+
+
+def test_queue_superellipsoid_scenes(oracle_lib):
+    """Superellipsoid scenes (Friction mu 0.3, the cone layout over rotating bodies) in the queue: each
+    scene bitwise the same alone and in a mix; its contacts bitwise those of the two-launch rounds
+    (the same state); its multipliers meet the bars test_gpu_fullsize holds the 32k-superellipsoid
+    step to: in the Coulomb cone (1e-9 of their scale), the gradient (formed by the oracle on the
+    device's contacts) in the dual cone (1e-5 of its scale), |lam - P(lam - step g)| <= 10 tol."""
+    from scopi_amd import scenes, serve
+    from tests.cones import cone_projection
+    kinds, _, prm = serve.hip_kinds(warm_start=False)
+    p = prm["superellipsoids"]
+    reqs = [serve.Request(rid=k, kind="superellipsoids", n=n, seed=30 + k, steps=1, arrival=0.0)
+            for k, n in enumerate([125, 216, 343])]
+    say("[queue] superellipsoid scenes")
+    q = queued_batch(p, 1e-3, threads=512)
+    ids = [q.add(serve.served_scene(r)) for r in reqs]
+    q.step(1)
+    alone = queued_batch(p, 1e-3, threads=512)
+    a = alone.add(serve.served_scene(reqs[1]))
+    alone.step(1)
+    same_state(alone.state(a), q.state(ids[1]), "superellipsoids alone vs mix")
+    rounds = hip.HipBatch(p, 1e-3)
+    rids = [rounds.add(serve.served_scene(r)) for r in reqs]
+    rounds.step(1)
+    cfg = scenes.CONFIG3
+    for sid, rid, r in zip(ids, rids, reqs):
+        sc = serve.served_scene(r)
+        ct, cr = q.contacts(sid), rounds.contacts(rid)
+        for k in ct:
+            assert np.array_equal(ct[k], cr[k]), f"n={r.n}: contacts {k}"
+        info = q.info(sid)
+        assert info["converged"] == 1, info
+        nc = info["n_contacts"]
+        lam = q.solution(sid)["lam"]
+        o = oracle_lib.Oracle(sc, vap=1, problem=oracle_lib.FRICTION, mu=cfg["mu"], dmax=cfg["dmax"],
+                              kd_tree_radius=cfg["kd_tree_radius"], tolerance=cfg["tolerance"])
+        o.set_contacts(ct)
+        o.set_a_priori_velocity(1e-3)
+        gr = o.gradient(1e-3, lam).reshape(nc, 3)
+        lam = lam.reshape(nc, 3)
+        n = ct["nij"]
+        ln = (lam * n).sum(1)
+        lt = np.linalg.norm(lam - ln[:, None] * n, axis=1)
+        gn = (gr * n).sum(1)
+        gt = np.linalg.norm(gr - gn[:, None] * n, axis=1)
+        ls, gs = float(np.abs(lam).max()), float(np.abs(gr).max())
+        cone = float(np.max(lt - cfg["mu"] * ln))
+        dual = float(np.max(cfg["mu"] * gt - gn))
+        nat = float(np.linalg.norm(lam - cone_projection(lam - info["step"] * gr, n, cfg["mu"])))
+        say(f"  n={r.n}: contacts {nc}, iterations {info['iterations']}; cone {cone:.3e}, dual {dual:.3e}, "
+            f"|lam - P(lam - step g)| {nat:.3e}")
+        assert cone <= 1e-9 * max(ls, 1e-300)
+        assert dual <= 1e-5 * max(gs, 1e-300)
+        assert nat <= 10 * cfg["tolerance"]
