@@ -209,6 +209,64 @@ def bench_superellipsoids(args):
     g.close()
 
 
+def bench_disks(args):
+    """The reference's own CPU-runnable case: 1,024 disks (2D) on a jittered 32x32 lattice above a plane,
+    gravity through vap_fpd, NoFriction APGD tol 1e-6, dt 0.005, dmax 0.025, kd_tree_radius (0.024 +
+    0.025)^2, 200 steps.  The GPU runs all 200 steps (after --warmup steps of a separate context), with
+    the round graph when --graph-rounds is given; beside it, unless --no-cpu, the CPU restatement of the
+    reference runs the first 25 of those steps on one host thread (about 0.8 s per step, so a bounded
+    sample), and the GPU's time for the same first 25 steps is reported beside it."""
+    import torch
+    from scopi_amd import hip, scenes
+    cfg = scenes.CONFIG1
+    sc = scenes.disks_2d()
+    prm = hip.params(vap=hip.VAP_FPD, dmax=cfg["dmax"], kd_tree_radius=cfg["kd_tree_radius"],
+                     tolerance=cfg["tolerance"])
+    torch.cuda.init()
+    if args.warmup:
+        w = hip.HipScene(sc)
+        w.run(prm, cfg["dt"], args.warmup)
+        w.close()
+    g = hip.HipScene(sc)
+    if args.graph_rounds:
+        g.set_tuning(hip.TUNE_GRAPH_ROUNDS, args.graph_rounds)
+    torch.cuda.synchronize()
+    log(f"[bench] 1,024 disks: {cfg['steps']} steps on the GPU")
+    sample = 25
+    t0 = time.perf_counter()
+    g.run(prm, cfg["dt"], sample)
+    torch.cuda.synchronize()
+    t_sample = time.perf_counter() - t0
+    info = g.run(prm, cfg["dt"], cfg["steps"], sample)
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t0
+    g.close()
+    result = {
+        "metric": "time-steps/s at 1,024 disks (2D), APGD tol 1e-6, 200 steps",
+        "value": cfg["steps"] / elapsed, "unit": "time-steps/s", "n_gpus": 1, "steps": cfg["steps"],
+        "warmup": args.warmup, "ms_per_step": 1e3 * elapsed / cfg["steps"], "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+        "config": {"workload": "1,024 disks (32x32 jittered lattice, spacing 0.025) above a plane, NoFriction APGD "
+                               "tol 1e-6, dt 0.005, dmax 0.025, kd_tree_radius (0.024+0.025)^2, 200 steps",
+                   "graph_rounds": args.graph_rounds},
+        "first_steps": {"steps": sample, "steps_per_s": sample / t_sample},
+    }
+    if not args.no_cpu:
+        import oracle
+        log(f"[bench] the CPU restatement: the first {sample} steps, one thread")
+        oracle.set_threads(1)
+        o = oracle.Oracle(sc, vap=1, dmax=cfg["dmax"], kd_tree_radius=cfg["kd_tree_radius"],
+                          tolerance=cfg["tolerance"])
+        t1 = time.perf_counter()
+        o.run(cfg["dt"], sample)
+        tc = time.perf_counter() - t1
+        result["cpu_baseline"] = {"value": sample / tc, "unit": "time-steps/s", "cores": 1, "kind": "port",
+                                  "sample": f"the first {sample} of the 200 steps (compare first_steps)",
+                                  "cpu_model": cpu_model(),
+                                  "apgd_iterations_per_step": o.stats()["total_iterations"] / sample}
+    print(json.dumps(result), flush=True)
+
+
 def bench_batched(args):
     """Independent sphere scenes (default 512, log-uniform 256..32,768 particles, each the 131k-sphere
     recipe scaled to its count: box of 5 planes, gravity, jittered lattice), NoFriction APGD tol 1e-6,
@@ -396,8 +454,10 @@ def main():
                     help="steps of the instrumented pass (HIP events around every solver kernel)")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="run one plain device copy of MIB MiB first (traffic calibration for counter runs)")
-    ap.add_argument("--workload", choices=["spheres", "superellipsoids", "batched", "served"], default="spheres",
-                    help="spheres: the headline 131k-sphere step; superellipsoids: 32,768 superellipsoids, "
+    ap.add_argument("--workload", choices=["spheres", "superellipsoids", "batched", "served", "disks"],
+                    default="spheres",
+                    help="spheres: the headline 131k-sphere step; disks: 1,024 disks, 200 steps; "
+                         "superellipsoids: 32,768 superellipsoids, "
                          "Friction mu 0.3; batched: independent sphere scenes advanced together (secondary lines)")
     ap.add_argument("--scenes", type=int, default=512, help="batched: number of scenes")
     ap.add_argument("--one-at-a-time", action="store_true", help="batched / served: also step every scene alone")
@@ -435,6 +495,8 @@ def main():
         return bench_batched(args)
     if args.workload == "served":
         return bench_served(args)
+    if args.workload == "disks":
+        return bench_disks(args)
     log("[bench] building the 131k-sphere scene")
     sc = scenes.spheres_3d(args.nx, args.ny, args.nz)
     default_scene = (args.nx, args.ny, args.nz) == (64, 64, 32)
