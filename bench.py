@@ -364,7 +364,8 @@ def bench_batched(args):
     print(json.dumps(result), flush=True)
 
 
-def served_leg(requests, rate, request_steps, sample_steps, hbm_budget_gib, one_at_a_time, warm_start=True):
+def served_leg(requests, rate, request_steps, sample_steps, hbm_budget_gib, one_at_a_time, warm_start=True,
+               schedules=None):
     """Served scenes: Poisson arrivals (`rate` per s) of `requests` scene requests, 80% sphere scenes (the
     131k-sphere recipe scaled, NoFriction) and 20% superellipsoid scenes (the 32k recipe scaled, Friction
     mu 0.3), log-uniform 512..65,536 particles, `request_steps` steps each, admitted by estimated HBM
@@ -374,7 +375,7 @@ def served_leg(requests, rate, request_steps, sample_steps, hbm_budget_gib, one_
     alone, start from its own last step's multipliers, kept on the device."""
     import torch
     from scopi_amd import hip, serve
-    kinds, footprint, prm = serve.hip_kinds(warm_start=warm_start)
+    kinds, footprint, prm = serve.hip_kinds(warm_start=warm_start, schedules=schedules)
     reqs = serve.poisson_requests(requests, rate, [("spheres", 0.8), ("superellipsoids", 0.2)], (512, 65536),
                                   request_steps, seed=2025)
     log(f"[bench] serving {len(reqs)} requests ({sum(r.kind == 'superellipsoids' for r in reqs)} superellipsoid "
@@ -416,8 +417,11 @@ def bench_served(args):
     """The served form alone as the JSON line (see served_leg)."""
     import torch
     torch.cuda.init()
+    schedules = None
+    if args.served_schedule:
+        schedules = {k: json.loads(v) for k, v in (item.split("=", 1) for item in args.served_schedule)}
     sv = served_leg(args.requests, args.rate, args.request_steps, args.sample_steps, args.hbm_budget_gib,
-                    args.one_at_a_time, warm_start=not args.served_cold)
+                    args.one_at_a_time, warm_start=not args.served_cold, schedules=schedules)
     result = {
         "metric": "served scene time-steps/s (Poisson arrivals, 80% sphere / 20% superellipsoid scenes, "
                   "512..65,536 particles) with p50/p99 step latency",
@@ -476,6 +480,9 @@ def main():
     ap.add_argument("--served-cold", action="store_true",
                     help="served: start every solve from lambda = 0 instead of the scene's last multipliers")
     ap.add_argument("--requests", type=int, default=24, help="served: number of requests")
+    ap.add_argument("--served-schedule", action="append", default=[], metavar="KIND=JSON",
+                    help="served: batch schedule of one request kind, e.g. spheres='{\"row_iters\": 4, ...}' "
+                         "(keys of scopi_amd.tune.BATCH_DEFAULT)")
     ap.add_argument("--rate", type=float, default=4.0, help="served: mean arrivals per second")
     ap.add_argument("--request-steps", type=int, default=100, help="served: steps asked by each request")
     ap.add_argument("--sample-steps", type=int, default=2, help="served: timed steps per request alone")

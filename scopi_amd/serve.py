@@ -223,12 +223,13 @@ def served_scene(r: Request):
     return scenes.superellipsoids_3d(nx=side, ny=side, nz=max(1, int(round(r.n / (side * side)))), seed=r.seed)
 
 
-def hip_kinds(dt=1e-3, sphere_precision=None, warm_start=True):
+def hip_kinds(dt=1e-3, sphere_precision=None, warm_start=True, schedules=None):
     """Batch factories of the two request kinds of the served workload: sphere scenes
     (NoFriction, vap_fpd, dmax 0.02, kd_tree_radius 0.042^2) and superellipsoid scenes
     (Friction mu 0.3, dmax 0.02, kd_tree_radius (2 sqrt(3) 0.015 + 0.02)^2), APGD tol 1e-6.
     With warm_start, each scene's solves start from its own last step's multipliers, kept on the
-    device with its state."""
+    device with its state.  schedules: optional {kind: batch schedule} (scopi_amd.tune's keys) set
+    on every batch of that kind."""
     from scopi_amd import hip, scenes
     c3 = scenes.CONFIG3
     ws = 1 if warm_start else 0
@@ -237,6 +238,14 @@ def hip_kinds(dt=1e-3, sphere_precision=None, warm_start=True):
     se = hip.params(problem=hip.FRICTION, mu=c3["mu"], vap=hip.VAP_FPD, dmax=c3["dmax"], warm_start=ws,
                     kd_tree_radius=c3["kd_tree_radius"], tolerance=c3["tolerance"])
     prm = {"spheres": sp, "superellipsoids": se}
-    kinds = {k: (lambda p=p: hip.HipBatch(p, dt)) for k, p in prm.items()}
+    def factory(kind, p):
+        def make():
+            b = hip.HipBatch(p, dt)
+            if schedules and kind in schedules:
+                from scopi_amd import tune
+                tune.apply_batch_schedule(b, schedules[kind])
+            return b
+        return make
+    kinds = {k: factory(k, p) for k, p in prm.items()}
     footprint = lambda r, sc: hip.scene_footprint(sc.n, sc.n_inactive, prm[r.kind])
     return kinds, footprint, prm
