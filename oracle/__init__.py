@@ -94,6 +94,7 @@ def lib():
         L.oracle_dogleg_fresh_wa4_count.restype = C.c_int64
         L.oracle_gen_segment_demo.argtypes = [C.c_int32, C.c_double, D, D, D]
         L.oracle_gen_box_spheres_demo.argtypes = [C.c_int32, C.c_double, D, D, D, D, D]
+        L.oracle_gen_uniform.argtypes = [C.c_int32, C.c_double, C.c_double, D]
         L.oracle_minpack_dogleg.argtypes = [C.c_int32, D, D, D, C.c_double, C.c_double, D, I32]
         L.oracle_newton_closest_points.argtypes = [C.c_int32, D, D, D, D, C.c_int32, D, D, D, D, D, D, D, D, D, D, I32]
         L.oracle_newton_residual.argtypes = [C.c_int32] + [D] * 8 + [D, C.c_void_p]
@@ -390,6 +391,13 @@ def gen_segment_demo(n=50, max_radius=0.06):
     x = np.zeros(n); y = np.zeros(n); r = np.zeros(n)
     lib().oracle_gen_segment_demo(n, max_radius, x, y, r)
     return x, y, r
+
+
+def gen_uniform(n, a, b):
+    """n draws of uniform_real_distribution<double>(a, b) from a default std::default_random_engine."""
+    out = np.zeros(n)
+    lib().oracle_gen_uniform(n, a, b, out)
+    return out
 
 
 def gen_box_spheres_demo(n_side=10, r0=0.5):

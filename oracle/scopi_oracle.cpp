@@ -1867,6 +1867,15 @@ int oracle_gen_segment_demo(int32_t n, double max_radius, double* x, double* y, 
     return 0;
 }
 
+// n draws of uniform_real_distribution<double>(a, b) from a default-constructed
+// std::default_random_engine (demos/pile_of_sand_spheres.cpp:30-71 draws its masses so).
+int oracle_gen_uniform(int32_t n, double a, double b, double* out) {
+    std::default_random_engine generator;
+    std::uniform_real_distribution<double> distrib(a, b);
+    for (int k = 0; k < n; ++k) out[k] = distrib(generator);
+    return 0;
+}
+
 // The random draws of demos/box_spheres_3d.cpp:73-95, in the demo's order: for each sphere (i, j, k
 // loops) its radius U(0.8 r0, 0.9 r0), mass U(1, 2) and centre offsets dx, dy, dz U(-0.05, 0.05), all
 // from one std::default_random_engine.

@@ -202,6 +202,8 @@ int oracle_quaternion(double angle, const double* axis, double* q);
 /* Draws of the spheres of demos/segment.cpp:30-44 (std::default_random_engine). */
 int oracle_gen_segment_demo(int32_t n, double max_radius, double* x, double* y, double* r);
 /* demos/box_spheres_3d.cpp's draws (radius, mass, centre offsets per sphere, in the demo's loop order) */
+/* n draws of uniform_real_distribution<double>(a, b) from a default std::default_random_engine */
+int oracle_gen_uniform(int32_t n, double a, double b, double* out);
 int oracle_gen_box_spheres_demo(int32_t n_side, double r0, double* r, double* m, double* dx, double* dy, double* dz);
 /* Scene generator of test/test_sphere.cpp:423-470 (std::minstd_rand0(123)). */
 int oracle_gen_2d_case_spheres(int32_t n, double* x, double* y, double* r, double* velocity);

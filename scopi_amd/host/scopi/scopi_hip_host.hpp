@@ -215,6 +215,10 @@ struct superellipsoid : object<dim> {
     superellipsoid(type::position_t<dim> pos, type::quaternion_t q, fixed<dim> radius, fixed<dim - 1> squareness)
         : object<dim>{SCOPI_SHAPE_SUPERELLIPSOID, pos, q, {radius[0], radius[1], dim == 3 ? radius[dim - 1] : 0.0},
                       {squareness[0], dim == 3 ? squareness[dim - 2] : 1.0}} {}
+    // 2D: the squareness e as a number, as the reference's one-element squareness takes it
+    template <std::size_t D = dim, typename = std::enable_if_t<D == 2>>
+    superellipsoid(type::position_t<dim> pos, type::quaternion_t q, fixed<dim> radius, double e)
+        : superellipsoid(pos, q, radius, fixed<1>{{e}}) {}
     fixed<dim> radius() const {
         fixed<dim> r;
         for (std::size_t d = 0; d < dim; ++d) r[d] = this->size[d];

@@ -325,6 +325,56 @@ def box_spheres_demo(n_side, r0, g, r, m, dx, dy, dz):
     return c.arrays()
 
 
+def sphere_plane_demo():
+    """The scene of demos/sphere_plane.cpp:14-40 (2D): a plane through the origin at angle pi/2 - pi/6
+    (an incline of 30 degrees), a sphere of radius 1 at 2 (sin(pi/6), cos(pi/6)) with mass 1, moment of
+    inertia 1/2 and force (0, -1)."""
+    alpha = math.pi / 6.0
+    c = Container(2)
+    c.plane((0.0, 0.0), math.pi / 2.0 - alpha)
+    c.sphere((2.0 * math.sin(alpha), 2.0 * math.cos(alpha)), 1.0,
+             Property(mass=1.0, moment_inertia=0.5, force=(0.0, -1.0)))
+    return c.arrays()
+
+
+def two_ellipsoids_demo():
+    """The scene of demos/two_ellipsoids.cpp:13-46 (2D): two ellipses of semi-axes (0.1, 0.05) at
+    (-0.2, 0) and (0.2, 0), turned by pi/4 and -pi/4, desired velocities (0.25, 0) and (-0.25, 0),
+    mass 1, moment of inertia 0.1."""
+    c = Container(2)
+    for x, ang, vx in ((-0.2, math.pi / 4.0, 0.25), (0.2, -math.pi / 4.0, -0.25)):
+        c.superellipsoid((x, 0.0), quaternion(ang), (0.1, 0.05, 0.0), (1.0, 1.0),
+                         Property(desired_velocity=(vx, 0.0), mass=1.0, moment_inertia=0.1))
+    return c.arrays()
+
+
+def pile_of_sand_demo(masses, n=3, width_box=10.0, g=1.0):
+    """The scene of demos/pile_of_sand_spheres.cpp:14-71 (3D): the plane y = 0 (angle pi/2), then spheres
+    of radius r = width_box / 2 / (n + 1) at (2 i r, r + 2 j r, 2 k r) for j >= 1 (loops i, j, k), then the
+    layer j = 0 shifted by r / 2 in x, force (0, -g, 0), mass from `masses` in that order and moment of
+    inertia m r^2 / 2 about each axis."""
+    r = width_box / 2.0 / (n + 1)
+    c = Container(3)
+    c.plane((0.0, 0.0, 0.0), math.pi / 2.0)
+    k = 0
+
+    def add(pos):
+        nonlocal k
+        m = masses[k]
+        k += 1
+        c.sphere(pos, r, Property(force=(0.0, -g, 0.0), mass=m,
+                                  moment_inertia=(m * r * r / 2.0, m * r * r / 2.0, m * r * r / 2.0)))
+    for i in range(n):
+        for j in range(1, n):
+            for l in range(n):
+                add((i * 2.0 * r, r + j * 2.0 * r, l * 2.0 * r))
+    dec_x = 0.5 * r
+    for i in range(n):
+        for l in range(n):
+            add((i * 2.0 * r + dec_x, r, l * 2.0 * r))
+    return c.arrays()
+
+
 # Default solver parameters of the workloads (BASELINE configurations).
 CONFIG1 = dict(dim=2, dt=0.005, dmax=0.025, kd_tree_radius=(0.024 + 0.025) ** 2, tolerance=1e-6, steps=200)
 CONFIG2 = dict(dim=3, dt=1e-3, dmax=0.02, kd_tree_radius=0.042 ** 2, tolerance=1e-6)

@@ -556,6 +556,85 @@ void segment_demo(std::size_t n_parts, std::size_t total_it) {
     dump_state("segment", particles, total_it);
 }
 
+// demos/sphere_plane.cpp:8-50 written against the slot mirror (2D): a sphere on a 30-degree incline,
+// FrictionFixedPoint mu 0.1, vap_fpd, dt 0.05, total_it steps (the demo runs 200).
+void sphere_plane_demo(std::size_t total_it) {
+    constexpr std::size_t dim = 2;
+    double radius = 1.;
+    double g = 1.;
+    double mass = 1.;
+    double h = 2. * radius;
+    auto prop = property<dim>().mass(mass).moment_inertia(mass * radius * radius / 2.);
+    double dt = 0.05;
+    double alpha = PI / 6.;
+    scopi_container<dim> particles;
+    plane<dim> p({{0., 0.}}, PI / 2. - alpha);
+    sphere<dim> s({{h * std::sin(alpha), h * std::cos(alpha)}}, radius);
+    particles.push_back(p, property<dim>().deactivate());
+    particles.push_back(s, prop.force({{0., -g}}));
+    dump_state("sphere_plane_initial", particles, 0);
+    ScopiSolver<dim, FrictionFixedPoint, OptimGradient<apgd>, contact_kdtree, vap_fpd> solver(particles);
+    auto params = solver.get_params();
+    params.default_contact_property.mu = 0.1;
+    solver.run(dt, total_it);
+    dump_state("sphere_plane", particles, total_it);
+}
+
+// demos/two_ellipsoids.cpp:7-50 written against the slot mirror (2D superellipsoids, default slots),
+// dt 0.005, total_it steps (the demo runs 1000).
+void two_ellipsoids_demo(std::size_t total_it) {
+    constexpr std::size_t dim = 2;
+    double dt = .005;
+    scopi_container<dim> particles;
+    superellipsoid<dim> s1({{-0.2, 0.}}, {quaternion(PI / 4)}, {{.1, .05}}, 1);
+    superellipsoid<dim> s2({{0.2, 0.}}, {quaternion(-PI / 4)}, {{.1, .05}}, 1);
+    auto prop1 = property<dim>().desired_velocity({{0.25, 0}}).mass(1.).moment_inertia(0.1);
+    auto prop2 = property<dim>().desired_velocity({{-0.25, 0}}).mass(1.).moment_inertia(0.1);
+    particles.push_back(s1, prop1);
+    particles.push_back(s2, prop2);
+    dump_state("two_ellipsoids_initial", particles, 0);
+    ScopiSolver<dim> solver(particles);
+    solver.run(dt, total_it);
+    dump_state("two_ellipsoids", particles, total_it);
+}
+
+// demos/pile_of_sand_spheres.cpp:9-80 written against the slot mirror (3D, FrictionFixedPoint mu 0.1,
+// vap_fpd): n = 3, masses from std::default_random_engine, dt 0.01, total_it steps (the demo runs 1000).
+void pile_of_sand_demo(std::size_t total_it) {
+    constexpr std::size_t dim = 3;
+    double dt = 0.01;
+    double width_box = 10.;
+    std::size_t n = 3;
+    double g = 1.;
+    scopi_container<dim> particles;
+    auto prop = property<dim>().force({{0., -g, 0.}});
+    plane<dim> p_horizontal({{0., 0., 0.}}, PI / 2.);
+    particles.push_back(p_horizontal, property<dim>().deactivate());
+    std::default_random_engine generator;
+    std::uniform_real_distribution<double> distrib_m(1., 2.);
+    double r = width_box / 2. / (n + 1);
+    for (std::size_t i = 0; i < n; ++i)
+        for (std::size_t j = 1; j < n; ++j)
+            for (std::size_t k = 0; k < n; ++k) {
+                double m = distrib_m(generator);
+                sphere<dim> s({{i * 2. * r, r + j * 2. * r, k * 2. * r}}, r);
+                particles.push_back(s, prop.mass(m).moment_inertia({{m * r * r / 2., m * r * r / 2., m * r * r / 2.}}));
+            }
+    double dec_x = 0.5 * r;
+    for (std::size_t i = 0; i < n; ++i)
+        for (std::size_t k = 0; k < n; ++k) {
+            double m = distrib_m(generator);
+            sphere<dim> s({{i * 2. * r + dec_x, r, k * 2. * r}}, r);
+            particles.push_back(s, prop.mass(m).moment_inertia({{m * r * r / 2., m * r * r / 2., m * r * r / 2.}}));
+        }
+    dump_state("pile_of_sand_initial", particles, 0);
+    ScopiSolver<dim, FrictionFixedPoint, OptimGradient<apgd>, contact_kdtree, vap_fpd> solver(particles);
+    auto params = solver.get_params();
+    params.default_contact_property.mu = 0.1;
+    solver.run(dt, total_it);
+    dump_state("pile_of_sand", particles, total_it);
+}
+
 int main(int argc, char** argv) {
     std::string only = argc > 1 ? argv[1] : "";
     using kd_fixed = ScopiSolver<2, NoFriction, OptimGradient<apgd>, contact_kdtree, vap_fixed>;
@@ -588,12 +667,17 @@ int main(int argc, char** argv) {
             superellipsoids_2d();
         }
         if (only.empty() || only == "periodic") periodic_pair();
-        // the demos run only when asked for, with their step counts: demos <box steps> <segment steps>
+        // the demos run only when asked for, with their step counts:
+        // demos <box steps> <segment steps> <steps of sphere_plane, two_ellipsoids and pile_of_sand>
         if (only == "demos") {
             const std::size_t box_steps = argc > 2 ? std::stoul(argv[2]) : 30;
             const std::size_t seg_steps = argc > 3 ? std::stoul(argv[3]) : 200;
             box_spheres_3d_demo(10, box_steps);
             segment_demo(50, seg_steps);
+            const std::size_t more = argc > 4 ? std::stoul(argv[4]) : 200;
+            sphere_plane_demo(more);
+            two_ellipsoids_demo(more);
+            pile_of_sand_demo(more);
         }
     } catch (const std::exception& e) {
         std::fprintf(stderr, "test_slots: %s\n", e.what());

@@ -7,7 +7,11 @@ run on the GPU: the demo code's scene set-up, slots and parameters, as a user of
     (the demo runs 1000; the spheres land after about 10 steps and are piled up by step 30);
   * demos/segment.cpp: 50 spheres driven through the gap between two segment obstacles at their desired
     velocity (default slots: vap_fixed), dmax 2 dt, kd_tree_radius 4 max_radius; 300 steps of dt = 0.001
-    (the spheres reach the segments after about 220).
+    (the spheres reach the segments after about 220);
+  * demos/sphere_plane.cpp (2D): a sphere on a 30-degree incline, FrictionFixedPoint mu 0.1, vap_fpd,
+    dt 0.05; demos/two_ellipsoids.cpp (2D superellipsoids, default slots), dt 0.005;
+    demos/pile_of_sand_spheres.cpp (27 spheres on a plane, FrictionFixedPoint mu 0.1, vap_fpd, masses from
+    std::default_random_engine), dt 0.01; 200 steps each (the demos run 200, 1000 and 1000).
 
 Bars:
   * the scene each program builds is bitwise the one scopi_amd.scenes builds from the oracle's copy of
@@ -16,7 +20,8 @@ Bars:
     test_gpu_parity.test_trajectory_3d_small); the device and the oracle run the same iterations of the
     same line search, apart from the order of their sums; every sphere stays inside the box;
   * segment, 300 steps against the oracle (reference default tol 1e-6): positions within 1e-8 (the bar
-    of test_gpu_segment's trajectory test).
+    of test_gpu_segment's trajectory test);
+  * sphere_plane, two_ellipsoids and pile_of_sand, 200 steps against the oracle: positions within 1e-8.
 """
 import json
 import os
@@ -31,7 +36,7 @@ from scopi_amd import scenes
 pytestmark = pytest.mark.gpu
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 BIN = os.path.join(ROOT, "tests", "cpp", "test_slots")
-BOX_STEPS, SEG_STEPS = 30, 300
+BOX_STEPS, SEG_STEPS, MORE_STEPS = 30, 300, 200
 N_SIDE, R0, G = 10, 0.5, 1.0
 
 
@@ -45,9 +50,9 @@ def demo_runs(tmp_path_factory):
     if not os.path.exists(BIN):
         pytest.fail(f"{BIN} not built (run __graft_entry__.build())")
     cwd = tmp_path_factory.mktemp("demos")  # the solver's output files land in ./Results here
-    say(f"[demos] {BIN} demos {BOX_STEPS} {SEG_STEPS}")
-    res = subprocess.run([BIN, "demos", str(BOX_STEPS), str(SEG_STEPS)], capture_output=True, text=True, timeout=240,
-                         cwd=str(cwd))
+    say(f"[demos] {BIN} demos {BOX_STEPS} {SEG_STEPS} {MORE_STEPS}")
+    res = subprocess.run([BIN, "demos", str(BOX_STEPS), str(SEG_STEPS), str(MORE_STEPS)], capture_output=True,
+                         text=True, timeout=240, cwd=str(cwd))
     assert res.returncode == 0, res.stderr
     out = {d["case"]: d for d in (json.loads(l) for l in res.stdout.splitlines() if l.startswith("{"))}
     out["_results"] = os.path.join(str(cwd), "Results")
@@ -101,3 +106,30 @@ def test_segment_demo(oracle_lib, demo_runs):
     d = float(np.abs(got - ref).max())
     say(f"[demos] segment {SEG_STEPS} steps: max|pos_gpu - pos_oracle| = {d:.3e}")
     assert d <= 1e-8
+
+
+def demo_against_oracle(oracle_lib, demo_runs, name, sc, dt, **prm):
+    dim = sc.dim
+    init = positions(demo_runs[name + "_initial"], dim)
+    assert np.array_equal(init, sc.pos[:, :dim]), "the program's scene differs from the builder's"
+    o = oracle_lib.Oracle(sc, **prm)
+    o.run(dt, MORE_STEPS)
+    got = positions(demo_runs[name], dim)
+    d = float(np.abs(got - o.state()["pos"][:, :dim]).max())
+    say(f"[demos] {name} {MORE_STEPS} steps: max|pos_gpu - pos_oracle| = {d:.3e}")
+    assert d <= 1e-8
+
+
+def test_sphere_plane_demo(oracle_lib, demo_runs):
+    demo_against_oracle(oracle_lib, demo_runs, "sphere_plane", scenes.sphere_plane_demo(), 0.05,
+                        problem=oracle_lib.FRICTION_FIXED_POINT, vap=1, mu=0.1)
+
+
+def test_two_ellipsoids_demo(oracle_lib, demo_runs):
+    demo_against_oracle(oracle_lib, demo_runs, "two_ellipsoids", scenes.two_ellipsoids_demo(), 0.005)
+
+
+def test_pile_of_sand_demo(oracle_lib, demo_runs):
+    masses = oracle_lib.gen_uniform(27, 1.0, 2.0)
+    demo_against_oracle(oracle_lib, demo_runs, "pile_of_sand", scenes.pile_of_sand_demo(masses), 0.01,
+                        problem=oracle_lib.FRICTION_FIXED_POINT, vap=1, mu=0.1)

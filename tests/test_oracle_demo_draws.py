@@ -78,3 +78,27 @@ def test_box_spheres_demo_scene_layout():
         np.testing.assert_allclose(normal(sc.q[p]), n, atol=1e-15)
     np.testing.assert_allclose(sc.pos[1], [2 * 2 * r0 + 2 * r0, 0, 0])
     np.testing.assert_allclose(sc.pos[4], [0, 0, 2 * 2 * r0 + 2 * r0])
+
+
+def test_uniform_draws():
+    """oracle.gen_uniform (the pile-of-sand demo's masses) is the same sequence."""
+    np.testing.assert_array_equal(oracle.gen_uniform(9, 1.0, 2.0), uniforms([(1.0, 2.0)], 9)[:, 0])
+
+
+def test_demo_scene_builders():
+    """sphere_plane: the sphere touches the 30-degree incline (its centre 2 r from the plane's origin
+    along the plane's normal); pile_of_sand: 27 spheres of radius 1.25, the layer j = 0 shifted by r / 2;
+    two_ellipsoids: semi-axes (0.1, 0.05), turned by +-pi/4."""
+    import math
+    from scopi_amd import scenes
+    sp = scenes.sphere_plane_demo()
+    assert sp.n == 2 and sp.n_inactive == 1
+    w, z = sp.q[0, 0], sp.q[0, 3]
+    normal = np.array([1 - 2 * z * z, 2 * w * z])  # first column of the plane's 2D rotation
+    assert abs(float(normal @ sp.pos[1, :2]) - 2.0) < 1e-12 and abs(float(normal[1]) - math.cos(math.pi / 6)) < 1e-12
+    pile = scenes.pile_of_sand_demo(oracle.gen_uniform(27, 1.0, 2.0))
+    assert pile.n == 28 and np.all(pile.size[1:, 0] == 1.25)
+    assert np.allclose(pile.pos[19:, 1], 1.25) and np.allclose(pile.pos[19, 0], 0.625)
+    te = scenes.two_ellipsoids_demo()
+    assert np.allclose(te.size[:, :2], [[0.1, 0.05], [0.1, 0.05]])
+    assert np.allclose(te.q[:, 3], [math.sin(math.pi / 8), -math.sin(math.pi / 8)])
