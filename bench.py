@@ -184,6 +184,8 @@ def bench_superellipsoids(args):
     g = hip.HipScene(sc, capacity_contacts=16 * sc.n)
     if args.row_iters is not None:
         g.set_tuning(hip.TUNE_ROW_ITERS, args.row_iters)
+    if args.graph_rounds:
+        g.set_tuning(hip.TUNE_GRAPH_ROUNDS, args.graph_rounds)
     log(f"[bench] warmup {args.warmup} steps")
     if args.warmup:
         g.run(prm, dt, args.warmup)
@@ -200,7 +202,8 @@ def bench_superellipsoids(args):
         "scaling": "weak", "vs_baseline": None, "dtype": "f64" if precision == hip.FP64 else "f32+f64",
         "data": "synthetic",
         "config": {"workload": "32,768 superellipsoids (32^3 jittered lattice, spacing 0.035), Friction mu 0.3, "
-                               "APGD tol 1e-6, dt 1e-3, dmax 0.02", "precision": args.precision},
+                               "APGD tol 1e-6, dt 1e-3, dmax 0.02", "precision": args.precision,
+                   "graph_rounds": args.graph_rounds},
         "n_contacts_last_step": info["n_contacts"],
         "apgd_iterations_per_step": info["iterations"] / args.steps,
         "line_search_rejections_per_step": info["backtracks"] / args.steps,

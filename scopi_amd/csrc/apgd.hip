@@ -34,6 +34,13 @@
 #include <type_traits>
 #include <vector>
 
+// Multiply-adds are fused only within one source expression (the functions below that follow the
+// CPU's rounding turn fusion off altogether).  Under the compiler's default, the backend fuses
+// across statements within a basic block, so which products end in an FMA, and with them the
+// rounding of the solve, would follow the code's block structure: a change of load order alone
+// could move the iterates.
+#pragma clang fp contract(on)
+
 namespace scopi_hip {
 
 constexpr int kRedBlocksMax = 2048;
@@ -58,6 +65,7 @@ inline int64_t row_blocks(int64_t nc, int cap, int layout, int kmin = 2) {
 // on the same few cache lines, from every block, cost about 2 us per round.
 constexpr int kBtBlock = 1024;
 constexpr int kBtBlocksMax = 512;
+constexpr int kCtrlR = kBtBlocksMax / kBlock;  // k_bt block sums per thread of a control step
 inline unsigned bt_blocks(int64_t na, int G) {
     return std::min<unsigned>(grid_for(na * (int64_t)G, kBtBlock), (unsigned)kBtBlocksMax);
 }
@@ -348,27 +356,6 @@ __device__ inline void visc_global(const BMatrix<T>& B, const T* __restrict__ la
     }
 }
 
-// kVisc: the 4 local rows of a global vector q at contact c (global2local,
-// lagrange_multiplier.hpp:143-163, 403-434)
-template <class T>
-__device__ inline void visc_local(const BMatrix<T>& B, int64_t c, const T (&q)[3], T (&out)[4]) {
-    constexpr int NC = lay<kVisc>::NC;
-    const int kd = B.kind[c] & kKindMask;
-    T dn = q[0] * B.coef[c * NC];
-    dn += q[1] * B.coef[c * NC + 1];
-    dn += q[2] * B.coef[c * NC + 2];
-    out[0] = out[1] = out[2] = out[3] = (T)0;
-    if (kd == 0) {
-        out[0] = dn;
-    } else if (kd == 1) {
-        out[0] = dn;
-        out[1] = -dn;
-    } else {
-        out[0] = -dn;
-        out[1] = q[0], out[2] = q[1], out[3] = q[2];
-    }
-}
-
 // Coulomb-cone projection of f (lagrange_multiplier.hpp:243-268, 500-519)
 template <class T>
 __device__ inline void cone_project(T (&f)[3], T nx, T ny, T nz, T mu) {
@@ -393,6 +380,10 @@ __device__ inline void cone_project(T (&f)[3], T nx, T ny, T nz, T mu) {
 __device__ inline void control_step(const ApgdCtrl* __restrict__ in, ApgdCtrl* __restrict__ out,
                                     const double* __restrict__ partial_bt, int nbt, ApgdCtrl& st,
                                     double (*lds)[kBlock / 64], bool leader, int32_t* done_flag);
+__device__ inline void control_load(const double* __restrict__ partial_bt, int nbt, double (&pv)[kCtrlR]);
+__device__ inline void control_finish(const ApgdCtrl* __restrict__ in, ApgdCtrl* __restrict__ out,
+                                      const double* __restrict__ partial_bt, int nbt, const double (&pv)[kCtrlR],
+                                      ApgdCtrl& st, double (*lds)[kBlock / 64], bool leader, int32_t* done_flag);
 
 // Logical index of block b among nb: the hardware deals workgroups to the 8 XCDs in turn (XCD
 // b mod 8), so XCD x runs blocks x, x + 8, ...; they are given a contiguous run of logical blocks,
@@ -437,8 +428,10 @@ __device__ inline void bt_gather(const BMatrix<T>& B, const T* __restrict__ lam,
     for (int e = e0 + sub; e < e1; e += 2 * G) {
         const int eb = e + G;
         const bool hb = eb < e1;
+        // the second index is read from a selected address (tc[e] again past the end), not
+        // selected after the load, so both loads are in flight together
         const int32_t csa = tc[e];
-        const int32_t csb = hb ? tc[eb] : csa;
+        const int32_t csb = tc[hb ? eb : e];
         const int64_t ca = csa < 0 ? ~csa : csa, cb = csb < 0 ? ~csb : csb;
         const T* tva = B.tcoef + (int64_t)e * NT;
         const T* tvb = B.tcoef + (int64_t)(hb ? eb : e) * NT;
@@ -449,9 +442,13 @@ __device__ inline void bt_gather(const BMatrix<T>& B, const T* __restrict__ lam,
             const T la = lam[ca], lb = lam[cb];
 #pragma unroll
             for (int k = 0; k < NT; ++k) acc[k] += la * ta[k];
-            if (hb)
+            // the second entry's term is selected rather than branched over, so that its loads are
+            // not sunk behind the first entry's
 #pragma unroll
-                for (int k = 0; k < NT; ++k) acc[k] += lb * tb[k];
+            for (int k = 0; k < NT; ++k) {
+                const T nb = acc[k] + lb * tb[k];
+                acc[k] = hb ? nb : acc[k];
+            }
         } else {
             T a0, a1, a2, b0, b1, b2;
             if (LAY == kVisc) {
@@ -532,6 +529,11 @@ __global__ void __launch_bounds__(kBtBlock) k_bt(ApgdCtrl* __restrict__ ctrl, in
     // a lane group shares p, so each group passes through the loop as a whole (the shuffles
     // below stay within it)
     for (; p < pend; p += pstep) {
+        // the particle's M^-1 is loaded with its first entries: after the stores of w below, the
+        // compiler could not tell these loads apart from them and would wait for each in turn
+        T mv[NW];
+#pragma unroll
+        for (int k = 0; k < NW; ++k) mv[k] = B.minv[p * NW + k];
         T acc[NW];
 #pragma unroll
         for (int k = 0; k < NW; ++k) acc[k] = (T)0;
@@ -543,7 +545,7 @@ __global__ void __launch_bounds__(kBtBlock) k_bt(ApgdCtrl* __restrict__ ctrl, in
         if (sub == 0)
 #pragma unroll
             for (int k = 0; k < NW; ++k) {
-                const T wk = B.minv[p * NW + k] * acc[k];
+                const T wk = mv[k] * acc[k];
                 w[p * NW + k] = wk;
                 q[0] += (double)acc[k] * (double)wk;
             }
@@ -578,16 +580,43 @@ static void launch_bt(int G, hipStream_t s, ApgdCtrl* ctrl, int use_slot, const 
     }
 }
 
+// Contact c's inputs to its Q rows and its projection: its coefficients (for the cone and
+// viscous layouts the first three are the normal), the rows of w_s of its two particles, mu and,
+// for kVisc, its kind.  A trial sweep loads them for all its contacts of an iteration before any
+// arithmetic or store: a load the compiler finds after a store it cannot tell apart from it is
+// issued only then, one more memory latency.  A row of an obstacle (a negative index) is zero; it
+// is read at index 0 and replaced where it is used, so neither the load nor its arrival waits on
+// a branch.
 template <class T, int LAY>
-__device__ inline void row_product(const BMatrix<T>& B, const T* __restrict__ w, int64_t c, int64_t i, int64_t j,
-                                   T dt2, T (&q)[3]) {
+struct RowIn {
+    T co[lay<LAY>::NC], wi[lay<LAY>::NW], wj[lay<LAY>::NW];
+    T mu;
+    int kd;
+    bool zi, zj;  // particle i / j is an obstacle: its row of w is zero
+};
+template <class T, int LAY>
+__device__ inline void load_row(const BMatrix<T>& B, const T* __restrict__ w, int64_t c, int64_t i, int64_t j,
+                                RowIn<T, LAY>& r) {
     constexpr int NC = lay<LAY>::NC, NW = lay<LAY>::NW;
+#pragma unroll
+    for (int k = 0; k < NC; ++k) r.co[k] = B.coef[c * NC + k];
+    const T* wi = w + (i >= 0 ? i : 0) * NW;
+    const T* wj = w + (j >= 0 ? j : 0) * NW;  // j is an obstacle only in periodic boxes
+#pragma unroll
+    for (int k = 0; k < NW; ++k) r.wi[k] = wi[k], r.wj[k] = wj[k];
+    r.zi = i < 0, r.zj = j < 0;
+    r.mu = (LAY == kCone || LAY == kVisc) ? B.mu[c] : (T)0;
+    r.kd = LAY == kVisc ? (B.kind[c] & kKindMask) : 0;
+}
+
+// dt^2 (B w_s) at contact c, its 1 (rotation-free, rigid) or 3 global rows.
+template <class T, int LAY>
+__device__ inline void row_product(const RowIn<T, LAY>& r, T dt2, T (&q)[3]) {
+    constexpr int NW = lay<LAY>::NW;
+    const T* co = r.co;
     T wi[NW], wj[NW];
 #pragma unroll
-    for (int k = 0; k < NW; ++k) wi[k] = (i >= 0) ? w[i * NW + k] : (T)0;
-#pragma unroll
-    for (int k = 0; k < NW; ++k) wj[k] = (j >= 0) ? w[j * NW + k] : (T)0;  // j is an obstacle only in periodic boxes
-    const T* co = B.coef + c * NC;
+    for (int k = 0; k < NW; ++k) wi[k] = r.zi ? (T)0 : r.wi[k], wj[k] = r.zj ? (T)0 : r.wj[k];
     if (LAY == kRotFree) {
         q[0] = dt2 * (co[0] * (wi[0] - wj[0]) + co[1] * (wi[1] - wj[1]) + co[2] * (wi[2] - wj[2]));
     } else if (LAY == kRigid) {
@@ -607,15 +636,26 @@ __device__ inline void row_product(const BMatrix<T>& B, const T* __restrict__ w,
 }
 
 // Q lambda_s at contact c (the 1..4 local rows NR) from w_s = M^-1 B^T lambda_s: dt^2 B w_s,
-// then global2local for kVisc.
+// then global2local for kVisc on the loaded normal and kind (lagrange_multiplier.hpp:143-163, 403-434).
 template <class T, int LAY>
-__device__ inline void q_rows(const BMatrix<T>& B, const T* __restrict__ w, int64_t c, int64_t i, int64_t j, T dt2,
-                              T (&q)[lay<LAY>::NR]) {
+__device__ inline void q_rows(const RowIn<T, LAY>& r, T dt2, T (&q)[lay<LAY>::NR]) {
     constexpr int NR = lay<LAY>::NR;
     T g[3];
-    row_product<T, LAY>(B, w, c, i, j, dt2, g);
+    row_product<T, LAY>(r, dt2, g);
     if constexpr (LAY == kVisc) {
-        visc_local<T>(B, c, g, q);
+        T dn = g[0] * r.co[0];
+        dn += g[1] * r.co[1];
+        dn += g[2] * r.co[2];
+        q[0] = q[1] = q[2] = q[3] = (T)0;
+        if (r.kd == 0) {
+            q[0] = dn;
+        } else if (r.kd == 1) {
+            q[0] = dn;
+            q[1] = -dn;
+        } else {
+            q[0] = -dn;
+            q[1] = g[0], q[2] = g[1], q[3] = g[2];
+        }
     } else {
 #pragma unroll
         for (int d = 0; d < NR; ++d) q[d] = g[d];
@@ -626,11 +666,11 @@ __device__ inline void q_rows(const BMatrix<T>& B, const T* __restrict__ w, int6
 // products, the gradient g = Q y + c, lambda+ = Proj(y - alpha g) stored in `lt`, and the contact's
 // terms of the 8 line-search sums (see k_trial).
 template <class T, int LAY>
-__device__ inline void trial_contact(const BMatrix<T>& B, T* __restrict__ lt, int64_t cc_, T beta, T alpha,
+__device__ inline void trial_contact(T* __restrict__ lt, int64_t cc_, T beta, T alpha, const RowIn<T, LAY>& rin,
                                      const T (&l0)[lay<LAY>::NR], const T (&l1)[lay<LAY>::NR],
                                      const T (&cv)[lay<LAY>::NR], const T (&q0)[lay<LAY>::NR],
                                      const T (&q1)[lay<LAY>::NR], double (&s)[8]) {
-    constexpr int NR = lay<LAY>::NR, NC = lay<LAY>::NC;
+    constexpr int NR = lay<LAY>::NR;
     T y[NR], g[NR], z[NR], qy[NR];
 #pragma unroll
     for (int d = 0; d < NR; ++d) {
@@ -640,12 +680,12 @@ __device__ inline void trial_contact(const BMatrix<T>& B, T* __restrict__ lt, in
         z[d] = y[d] - alpha * g[d];
     }
     if constexpr (LAY == kVisc) {
-        const int kd = B.kind[cc_] & kKindMask;
+        const int kd = rin.kd;
         if (kd == 2) {
             // ViscousFriction at gamma_min (lagrange_multiplier.hpp:495-539): the nearer of
             // (0, Proj_cone(f)) and (max(x-, 0), 0)
             T f[3] = {z[1], z[2], z[3]};
-            cone_project<T>(f, B.coef[cc_ * NC], B.coef[cc_ * NC + 1], B.coef[cc_ * NC + 2], B.mu[cc_]);
+            cone_project<T>(f, rin.co[0], rin.co[1], rin.co[2], rin.mu);
             const T zm = z[0] > (T)0 ? z[0] : (T)0;
             T df = z[0] * z[0], dm = (zm - z[0]) * (zm - z[0]);
             for (int d = 0; d < 3; ++d) {
@@ -667,7 +707,7 @@ __device__ inline void trial_contact(const BMatrix<T>& B, T* __restrict__ lt, in
         z[0] = z[0] > (T)0 ? z[0] : (T)0;  // max(lambda, 0): lagrange_multiplier.hpp:104-107
     } else {
         // Coulomb-cone projection: lagrange_multiplier.hpp:243-268
-        cone_project<T>(z, B.coef[cc_ * NC + 0], B.coef[cc_ * NC + 1], B.coef[cc_ * NC + 2], B.mu[cc_]);
+        cone_project<T>(z, rin.co[0], rin.co[1], rin.co[2], rin.mu);
     }
 #pragma unroll
     for (int d = 0; d < NR; ++d) {
@@ -712,54 +752,97 @@ __global__ void __launch_bounds__(kBlock) k_trial(const ApgdCtrl* __restrict__ i
     const int64_t cstep = (int64_t)sb.nb * kBlock;
     const int64_t cend = sb.hi;
     int64_t c = sb.lo + (int64_t)lblk * kBlock + threadIdx.x;
+    // particle indices are read at every position, those past the end at the last contact (or 0;
+    // the arrays hold at least 1024 entries), so that no branch waits for them: a contact past the
+    // end is never stored or summed, and reads only valid rows
+    auto clampc = [&](int64_t x) { return x < cend ? x : (cend > 0 ? cend - 1 : (int64_t)0); };
     int32_t inx[U], jnx[U];
 #pragma unroll
     for (int u = 0; u < U; ++u) {
-        inx[u] = jnx[u] = 0;
-        if (c + u * cstep < cend) inx[u] = ci[c + u * cstep], jnx[u] = cj[c + u * cstep];
+        const int64_t c0 = clampc(c + u * cstep);
+        inx[u] = ci[c0], jnx[u] = cj[c0];
     }
-    control_step(in + sb.sc, out + sb.sc, partial_bt + (M.rblk ? M.bfirst[sb.sc] : 0), M.rblk ? M.bcount[sb.sc] : nbt,
-                 st, lds, sb.lb == 0, M.done ? M.done + sb.sc : nullptr);
-    if (st.done) return;
-    const int cur = st.cur, prev = st.prev, trial = st.trial;
+    // The first contacts' inputs are loaded before the control step has chosen the iterates' slots,
+    // from the slots it will choose when it accepts the pending trial point (lambda_k := the trial
+    // point, lambda_{k-1} := lambda_k) or, with no step pending, the current ones; every thread
+    // reads the three slot numbers of the state, so the loads wait for one memory round trip and the
+    // step's reduction and arithmetic overlap them.  After a rejection (the slots stay) the block
+    // loads them again.  The values are the same either way: this only moves the loads earlier.
+    const ApgdCtrl* cin = in + sb.sc;
+    const bool opend = cin->pending != 0;
+    const int pcur = opend ? cin->trial : cin->cur, pprev = opend ? cin->cur : cin->prev;
     const int64_t stride = NR * B.cap, wstride = 6 * B.pcap;
-    const T beta = (T)st.beta, alpha = (T)st.alpha, dt2 = (T)st.dt2;
-    const T* lc = slot<T>(B.lam, cur, stride);
-    const T* lp = slot<T>(B.lam, prev, stride);
-    const T* wc = slot<T>(B.w, cur, wstride);
-    T* qc = slot<T>(B.qlam, cur, stride);
-    const T* qp = slot<T>(B.qlam, prev, stride);
-    T* lt = slot<T>(B.lam, trial, stride);
-    double s[8] = {0, 0, 0, 0, 0, 0, 0, 0};
-    for (; c < cend; c += U * cstep) {
-        int64_t cu[U], iu[U], ju[U];
-        bool hu[U];
+    int64_t cu[U], iu[U], ju[U];
+    bool hu[U];
+    T l0[U][NR], l1[U][NR], cv[U][NR], q1[U][NR];
+    RowIn<T, LAY> rin[U];
+    // the group of contacts at c (its particle indices loaded one group ahead)
+    auto index = [&]() {
 #pragma unroll
         for (int u = 0; u < U; ++u) {
             hu[u] = c + u * cstep < cend;
             cu[u] = hu[u] ? c + u * cstep : c;  // past the end: a duplicate of c, not stored
             iu[u] = (int64_t)inx[u] - ni, ju[u] = (int64_t)jnx[u] - ni;
-            const int64_t cn = c + (U + u) * cstep;
-            if (cn < cend) inx[u] = ci[cn], jnx[u] = cj[cn];
+            const int64_t cn = clampc(c + (U + u) * cstep);
+            inx[u] = ci[cn], jnx[u] = cj[cn];
         }
-        T l0[U][NR], l1[U][NR], cv[U][NR], q1[U][NR];
+    };
+    // its inputs, lambda_k and Q lambda_k from slot scur and lambda_{k-1}, Q lambda_{k-1} from sprev
+    auto fetch = [&](int scur, int sprev) {
+        const T* lc = slot<T>(B.lam, scur, stride);
+        const T* lp = slot<T>(B.lam, sprev, stride);
+        const T* qp = slot<T>(B.qlam, sprev, stride);
+        const T* wc = slot<T>(B.w, scur, wstride);
 #pragma unroll
-        for (int u = 0; u < U; ++u)
+        for (int u = 0; u < U; ++u) {
 #pragma unroll
             for (int d = 0; d < NR; ++d) {
                 const int64_t r = cu[u] * NR + d;
                 l0[u][d] = lc[r], l1[u][d] = lp[r], cv[u][d] = B.c[r], q1[u][d] = qp[r];
             }
+            load_row<T, LAY>(B, wc, cu[u], iu[u], ju[u], rin[u]);
+        }
+    };
+    const double* pbt = partial_bt + (M.rblk ? M.bfirst[sb.sc] : 0);
+    const int npbt = M.rblk ? M.bcount[sb.sc] : nbt;
+    double pv[kCtrlR];
+    control_load(pbt, npbt, pv);
+    {
+        // every lane issues the first group's loads, a lane with no contact at a valid index whose
+        // values it leaves unused (contact and particle arrays hold at least 1024 entries)
+        const int64_t c0 = c;
+        if (c >= cend) c = clampc(c);
+        index();
+        if (c0 >= cend) hu[0] = false;
+        fetch(pcur, pprev);
+        c = c0;
+    }
+    // keep the block sums' use after these loads are issued
+    __builtin_amdgcn_sched_barrier(0);
+    control_finish(cin, out + sb.sc, pbt, npbt, pv, st, lds, sb.lb == 0, M.done ? M.done + sb.sc : nullptr);
+    if (st.done) return;
+    const int cur = st.cur, prev = st.prev, trial = st.trial;
+    if (c < cend && (cur != pcur || prev != pprev)) fetch(cur, prev);  // the same in the whole block
+    const T beta = (T)st.beta, alpha = (T)st.alpha, dt2 = (T)st.dt2;
+    T* qc = slot<T>(B.qlam, cur, stride);
+    T* lt = slot<T>(B.lam, trial, stride);
+    double s[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    while (c < cend) {
         T q0[U][NR];
 #pragma unroll
-        for (int u = 0; u < U; ++u) q_rows<T, LAY>(B, wc, cu[u], iu[u], ju[u], dt2, q0[u]);
+        for (int u = 0; u < U; ++u) q_rows<T, LAY>(rin[u], dt2, q0[u]);
 #pragma unroll
         for (int u = 0; u < U; ++u)
             if (hu[u]) {
 #pragma unroll
                 for (int d = 0; d < NR; ++d) qc[cu[u] * NR + d] = q0[u][d];
-                trial_contact<T, LAY>(B, lt, cu[u], beta, alpha, l0[u], l1[u], cv[u], q0[u], q1[u], s);
+                trial_contact<T, LAY>(lt, cu[u], beta, alpha, rin[u], l0[u], l1[u], cv[u], q0[u], q1[u], s);
             }
+        c += U * cstep;
+        if (c < cend) {
+            index();
+            fetch(cur, prev);
+        }
     }
     block_sum<8>(s, lds);
     if (threadIdx.x == 0)
@@ -863,15 +946,30 @@ __device__ inline void apgd_control(ApgdCtrl* ctrl) {
 // One block: <lambda+, Q lambda+> = dt^2 (sum of k_bt's block sums), then the control step.
 // Publishes the pending control step of the previous product to the block's copy
 // `st` (every block computes the same state: same inputs, same reduction order)
-// and block 0 writes it to `out`.
-__device__ inline void control_step(const ApgdCtrl* __restrict__ in, ApgdCtrl* __restrict__ out,
-                                    const double* __restrict__ partial_bt, int nbt, ApgdCtrl& st,
-                                    double (*lds)[kBlock / 64], bool leader, int32_t* done_flag) {
-    // the state and k_bt's block sums are loaded together, before the state is known to need the
-    // sums (a fixed summation order)
-    ApgdCtrl c;
-    if (threadIdx.x == 0) c = *in;
-    double tot[1] = {thread_sum<kBtBlocksMax / kBlock>(partial_bt, nbt, 1)};
+// and block 0 writes it to `out`.  In two parts, so that a kernel can issue other loads between
+// the block sums' and their use: control_load loads the thread's share of the sums (the first
+// kBtBlocksMax of them, all of them for every grid of k_bt), control_finish sums them in the order
+// of thread_sum and runs the step.
+__device__ inline void control_load(const double* __restrict__ partial_bt, int nbt, double (&pv)[kCtrlR]) {
+    // a missing term reads entry 0 (the buffer holds at least 1024) and is replaced by 0.0 where it
+    // is summed, so that no branch waits for the load
+#pragma unroll
+    for (int r = 0; r < kCtrlR; ++r) {
+        const int b = (int)threadIdx.x + r * kBlock;
+        pv[r] = partial_bt[b < nbt ? b : 0];
+    }
+}
+__device__ inline void control_finish(const ApgdCtrl* __restrict__ in, ApgdCtrl* __restrict__ out,
+                                      const double* __restrict__ partial_bt, int nbt, const double (&pv)[kCtrlR],
+                                      ApgdCtrl& st, double (*lds)[kBlock / 64], bool leader, int32_t* done_flag) {
+    // the state is read by every thread from one address (scalar loads, whose wait does not include
+    // the vector loads a kernel issued before this call)
+    ApgdCtrl c = *in;
+    double t = 0.0;
+#pragma unroll
+    for (int r = 0; r < kCtrlR; ++r) t += ((int)threadIdx.x + r * kBlock < nbt) ? pv[r] : 0.0;
+    for (int b = (int)threadIdx.x + kCtrlR * kBlock; b < nbt; b += kBlock) t += partial_bt[b];
+    double tot[1] = {t};
     // summed whether or not the state needs it, so that no lane waits for the state to learn that
     block_sum<1>(tot, lds);
     if (threadIdx.x == 0) {
@@ -889,6 +987,15 @@ __device__ inline void control_step(const ApgdCtrl* __restrict__ in, ApgdCtrl* _
         }
     }
     __syncthreads();
+}
+__device__ inline void control_step(const ApgdCtrl* __restrict__ in, ApgdCtrl* __restrict__ out,
+                                    const double* __restrict__ partial_bt, int nbt, ApgdCtrl& st,
+                                    double (*lds)[kBlock / 64], bool leader, int32_t* done_flag) {
+    // the state and k_bt's block sums are loaded together, before the state is known to need the
+    // sums (a fixed summation order)
+    double pv[kCtrlR];
+    control_load(partial_bt, nbt, pv);
+    control_finish(in, out, partial_bt, nbt, pv, st, lds, leader, done_flag);
 }
 
 // The control step alone (end of a batch of rounds, before the host reads the state);
@@ -937,6 +1044,9 @@ __device__ inline double queue_bt(const BMatrix<T>& B, const QueueArgs& A, int s
     for (; p < phi; p += NTH / G) {
         const int f0 = e0, f1 = e1;
         if (p + NTH / G < phi) e0 = A.tstart[p + NTH / G], e1 = A.tstart[p + NTH / G + 1];
+        T mv[NW];  // M^-1 ahead of the stores of w, as in k_bt
+#pragma unroll
+        for (int k = 0; k < NW; ++k) mv[k] = B.minv[p * NW + k];
         T acc[NW];
 #pragma unroll
         for (int k = 0; k < NW; ++k) acc[k] = (T)0;
@@ -948,7 +1058,7 @@ __device__ inline double queue_bt(const BMatrix<T>& B, const QueueArgs& A, int s
         if (sub == 0)
 #pragma unroll
             for (int k = 0; k < NW; ++k) {
-                const T wk = B.minv[p * NW + k] * acc[k];
+                const T wk = mv[k] * acc[k];
                 w[p * NW + k] = wk;
                 q += (double)acc[k] * (double)wk;
             }
@@ -1040,21 +1150,24 @@ __device__ void queue_phase(const BMatrix<T>& B, const QueueArgs& A, int64_t clo
                 if (cn < chi) inx[u] = A.ci[cn], jnx[u] = A.cj[cn];
             }
             T l0[U][NR], l1[U][NR], cv[U][NR], q1[U][NR], q0[U][NR];
+            RowIn<T, LAY> rin[U];
 #pragma unroll
-            for (int u = 0; u < U; ++u)
+            for (int u = 0; u < U; ++u) {
 #pragma unroll
                 for (int d = 0; d < NR; ++d) {
                     const int64_t r = cu[u] * NR + d;
                     l0[u][d] = lc[r], l1[u][d] = lp[r], cv[u][d] = B.c[r], q1[u][d] = qp[r];
                 }
+                load_row<T, LAY>(B, wc, cu[u], iu[u], ju[u], rin[u]);
+            }
 #pragma unroll
-            for (int u = 0; u < U; ++u) q_rows<T, LAY>(B, wc, cu[u], iu[u], ju[u], dt2, q0[u]);
+            for (int u = 0; u < U; ++u) q_rows<T, LAY>(rin[u], dt2, q0[u]);
 #pragma unroll
             for (int u = 0; u < U; ++u)
                 if (hu[u]) {
 #pragma unroll
                     for (int d = 0; d < NR; ++d) qc[cu[u] * NR + d] = q0[u][d];
-                    trial_contact<T, LAY>(B, lt, cu[u], beta, alpha_t, l0[u], l1[u], cv[u], q0[u], q1[u], s);
+                    trial_contact<T, LAY>(lt, cu[u], beta, alpha_t, rin[u], l0[u], l1[u], cv[u], q0[u], q1[u], s);
                 }
         }
         // the barriers of the sum also order the block's stores of lambda+ before the B^T sweep
