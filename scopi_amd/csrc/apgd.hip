@@ -493,6 +493,7 @@ __global__ void __launch_bounds__(kBtBlock) k_bt(ApgdCtrl* __restrict__ ctrl, in
                                               double* __restrict__ partial_bt, SceneMap M) {
     constexpr int NR = lay<LAY>::NR, NW = lay<LAY>::NW;
     __shared__ double lds[1][kBtBlock / 64];
+    __shared__ double lds2[2][kBtBlock / 64];
     const bool multi = M.bblk != nullptr;
     const int sc = multi ? M.bblk[blockIdx.x] : 0;
     const int lb = multi ? (int)blockIdx.x - M.bfirst[sc] : (int)blockIdx.x;
@@ -511,17 +512,23 @@ __global__ void __launch_bounds__(kBtBlock) k_bt(ApgdCtrl* __restrict__ ctrl, in
     const int s = (use_slot >= 0) ? use_slot : (ctrl ? ctrl->trial : 0);
     int e0 = p < pend ? tstart[p] : 0, e1 = p < pend ? tstart[p + 1] : 0;
     if (done) return;
-    if (ctrl && partial_trial && lb < 8) {
-        // the trial kernel's 8 block sums, one quantity per block (blocks lb, lb + nbk, ... < 8),
-        // each in block order; a quantity's sums are contiguous
-        const double* pt = multi ? partial_trial + M.rfirst[sc] : partial_trial;
-        const int nt = multi ? M.rcount[sc] : ntrial;
-        for (int k = lb; k < 8; k += nbk) {
-            double tot[1] = {thread_sum<kRedBlocksMax / kBtBlock, kBtBlock>(pt + k * trial_pstride(M), nt, 1)};
-            block_sum<1, kBtBlock>(tot, lds);
-            if (threadIdx.x == 0) ctrl->red[k < 7 ? k : 8] = tot[0];
+    // Blocks 0-7 also reduce the trial kernel's 8 block sums, one quantity per block (blocks lb, lb + nbk,
+    // ... < 8), each in block order; a quantity's sums are contiguous.  The first quantity's loads are
+    // issued here, ahead of the gather (a missing term reads the quantity's entry 0 and is replaced by
+    // 0.0 where it is summed), and summed at the end with the block's own sum, in the order of
+    // thread_sum; so a reducing block waits for no extra memory round trip or barrier.
+    constexpr int RT = kRedBlocksMax / kBtBlock;
+    const bool red = ctrl && partial_trial && lb < 8;
+    const double* pt = multi ? partial_trial + M.rfirst[sc] : partial_trial;
+    const int nt = multi ? M.rcount[sc] : ntrial;
+    double tv[RT];
+    if (red) {
+        const double* p = pt + lb * trial_pstride(M);
+#pragma unroll
+        for (int r = 0; r < RT; ++r) {
+            const int b = (int)threadIdx.x + r * kBtBlock;
+            tv[r] = p[b < nt ? b : 0];
         }
-        if (lb == 0 && threadIdx.x == 0) ctrl->pending = 1;
     }
     const T* lam = slot<T>(B.lam, s, NR * B.cap);
     T* w = slot<T>(B.w, s, 6 * B.pcap);
@@ -551,7 +558,26 @@ __global__ void __launch_bounds__(kBtBlock) k_bt(ApgdCtrl* __restrict__ ctrl, in
             }
         if (p + pstep < pend) e0 = tstart[p + pstep], e1 = tstart[p + pstep + 1];
     }
-    if (ctrl) {
+    if (red) {
+        double t = 0.0;
+#pragma unroll
+        for (int r = 0; r < RT; ++r) t += ((int)threadIdx.x + r * kBtBlock < nt) ? tv[r] : 0.0;
+        const double* p = pt + lb * trial_pstride(M);
+        for (int b = (int)threadIdx.x + RT * kBtBlock; b < nt; b += kBtBlock) t += p[b];
+        double v[2] = {q[0], t};
+        block_sum<2, kBtBlock>(v, lds2);
+        if (threadIdx.x == 0) {
+            partial_bt[(multi ? M.bfirst[sc] : 0) + lblk] = v[0];
+            ctrl->red[lb < 7 ? lb : 8] = v[1];
+        }
+        // further quantities when the scene has fewer than 8 blocks
+        for (int k = lb + nbk; k < 8; k += nbk) {
+            double tot[1] = {thread_sum<RT, kBtBlock>(pt + k * trial_pstride(M), nt, 1)};
+            block_sum<1, kBtBlock>(tot, lds);
+            if (threadIdx.x == 0) ctrl->red[k < 7 ? k : 8] = tot[0];
+        }
+        if (lb == 0 && threadIdx.x == 0) ctrl->pending = 1;
+    } else if (ctrl) {
         block_sum<1, kBtBlock>(q, lds);
         if (threadIdx.x == 0) partial_bt[(multi ? M.bfirst[sc] : 0) + lblk] = q[0];
     }
