@@ -536,15 +536,21 @@ __global__ void __launch_bounds__(kBtBlock) k_bt(ApgdCtrl* __restrict__ ctrl, in
     // a lane group shares p, so each group passes through the loop as a whole (the shuffles
     // below stay within it)
     for (; p < pend; p += pstep) {
-        // the particle's M^-1 is loaded with its first entries: after the stores of w below, the
-        // compiler could not tell these loads apart from them and would wait for each in turn
+        // the next pass's entry range is loaded now, ahead of this pass's gather (after the stores
+        // of w it would wait for them), and the particle's M^-1 with its first entries: after the
+        // stores, the compiler could not tell these loads apart from them and would wait for each
+        const int f0 = e0, f1 = e1;
+        {
+            const int64_t pn = p + pstep < pend ? p + pstep : p;
+            e0 = tstart[pn], e1 = tstart[pn + 1];
+        }
         T mv[NW];
 #pragma unroll
         for (int k = 0; k < NW; ++k) mv[k] = B.minv[p * NW + k];
         T acc[NW];
 #pragma unroll
         for (int k = 0; k < NW; ++k) acc[k] = (T)0;
-        bt_gather<T, LAY, G>(B, lam, tc, e0, e1, sub, acc);
+        bt_gather<T, LAY, G>(B, lam, tc, f0, f1, sub, acc);
 #pragma unroll
         for (int o = G / 2; o > 0; o >>= 1)
 #pragma unroll
@@ -556,14 +562,13 @@ __global__ void __launch_bounds__(kBtBlock) k_bt(ApgdCtrl* __restrict__ ctrl, in
                 w[p * NW + k] = wk;
                 q[0] += (double)acc[k] * (double)wk;
             }
-        if (p + pstep < pend) e0 = tstart[p + pstep], e1 = tstart[p + pstep + 1];
     }
     if (red) {
         double t = 0.0;
 #pragma unroll
         for (int r = 0; r < RT; ++r) t += ((int)threadIdx.x + r * kBtBlock < nt) ? tv[r] : 0.0;
-        const double* p = pt + lb * trial_pstride(M);
-        for (int b = (int)threadIdx.x + RT * kBtBlock; b < nt; b += kBtBlock) t += p[b];
+        const double* ptk = pt + lb * trial_pstride(M);
+        for (int b = (int)threadIdx.x + RT * kBtBlock; b < nt; b += kBtBlock) t += ptk[b];
         double v[2] = {q[0], t};
         block_sum<2, kBtBlock>(v, lds2);
         if (threadIdx.x == 0) {

@@ -1639,13 +1639,16 @@ __global__ void __launch_bounds__(kBlock) k_se_points(Particles P, SeGrids G) {
 // whose penalty exceeds g is above g (the distance term is >= 0), and so is one whose squared
 // distance exceeds (g - penalty + 1e-15 g)^2 (1 + 1e-14), the margins covering the roundings of the
 // term; their square roots are skipped.  Terms equal to g are never skipped, so the minimum and its
-// index are those of the full scan, bit for bit.
+// index are those of the full scan, bit for bit.  The penalty test itself is screened first with
+// the dot product n1.n2 in three FMAs against 0.5 g - 1 + 1e-13: it differs from the term's dot
+// product (three products and two sums, rounded each) by less than 1e-15 for unit normals, so a term
+// the screen rejects has 2 (1 + n1.n2) > g as computed; the others take the exact test.
 constexpr int kSeGridBlock = 192;  // 3 x 192 = 576
 __global__ void __launch_bounds__(kSeGridBlock) k_se_grid(Particles P, SeGrids G, SeCandidates S, int64_t n) {
     __shared__ double g2[6][kSeGridPoints];
     __shared__ double red_v[kSeGridBlock];
     __shared__ int red_i[kSeGridBlock];
-    __shared__ double wbest[kSeGridBlock / 64];
+    __shared__ double wbest[kSeGridBlock / 64];  // read while other waves write: relaxed atomics (LDS reads, writes)
     const int64_t c = blockIdx.x;
     if (c >= n) return;
     const int64_t i = S.i[c], j = S.j[c];
@@ -1670,7 +1673,6 @@ __global__ void __launch_bounds__(kSeGridBlock) k_se_grid(Particles P, SeGrids G
         }
     }
     __syncthreads();
-    volatile double* vb = wbest;
     const int wave = threadIdx.x >> 6;
     double bv = INFINITY;   // the lane's minimum ...
     int bidx = 0x7fffffff;  // ... and its row-major index
@@ -1680,13 +1682,18 @@ __global__ void __launch_bounds__(kSeGridBlock) k_se_grid(Particles P, SeGrids G
             double w = bv;
 #pragma unroll
             for (int o = 32; o > 0; o >>= 1) w = fmin(w, __shfl_xor(w, o, 64));
-            if ((threadIdx.x & 63) == 0 && w < vb[wave]) vb[wave] = w;
+            if ((threadIdx.x & 63) == 0 &&
+                w < __hip_atomic_load(&wbest[wave], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP))
+                __hip_atomic_store(&wbest[wave], w, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
 #pragma unroll
-            for (int k = 0; k < kSeGridBlock / 64; ++k) g = fmin(g, vb[k]);
+            for (int k = 0; k < kSeGridBlock / 64; ++k)
+                g = fmin(g, __hip_atomic_load(&wbest[k], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP));
         }
         const double nx = g2[3][jj], ny = g2[4][jj], nz = g2[5][jj];
+        const double screen = 0.5 * g - 1.0 + 1e-13;  // +inf while g is
 #pragma unroll
         for (int r = 0; r < 3; ++r) {
+            if (fma(m1[r][2], nz, fma(m1[r][1], ny, m1[r][0] * nx)) > screen) continue;
             const double pen = 2 * (1 + (m1[r][0] * nx + m1[r][1] * ny + m1[r][2] * nz));
             if (ok[r] && pen <= g) {
                 const double dx = p1[r][0] - g2[0][jj], dy = p1[r][1] - g2[1][jj], dz = p1[r][2] - g2[2][jj];
