@@ -64,6 +64,7 @@ inline int64_t row_blocks(int64_t nc, int cap, int layout, int kmin = 2) {
 // blocks for the same lanes: every k_trial block loads them all, and with 2048 sums those loads
 // on the same few cache lines, from every block, cost about 2 us per round.
 constexpr int kBtBlock = 1024;
+constexpr int kTransposeGroup = 8;  // lanes per particle of k_transpose
 constexpr int kBtBlocksMax = 512;
 constexpr int kCtrlR = kBtBlocksMax / kBlock;  // k_bt block sums per thread of a control step
 inline unsigned bt_blocks(int64_t na, int G) {
@@ -302,30 +303,39 @@ __global__ void __launch_bounds__(kBlock) k_transpose(BMatrix<T> B, int64_t na, 
                                                      const int* __restrict__ ie, const int* __restrict__ js,
                                                      const int* __restrict__ je, const int32_t* __restrict__ jlist,
                                                      const int* __restrict__ tstart, int32_t* __restrict__ tc) {
-    constexpr int NC = lay<LAY>::NC, NT = lay<LAY>::NT;
-    const int64_t p = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    // kTransposeGroup lanes per particle, lane k writing the particle's entries k, k + group, ...:
+    // a lane's chain of dependent loads (jlist, then the contact's coefficients) is a few entries
+    // long instead of the particle's whole list, and a wavefront's stores fall on neighbouring entries
+    constexpr int NC = lay<LAY>::NC, NT = lay<LAY>::NT, G = kTransposeGroup;
+    const int64_t g = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    const int64_t p = g / G;
+    const int sub = (int)(g % G);
     if (p >= na) return;
-    int64_t e = tstart[p];
-    for (int c = is[p]; c < ie[p]; ++c, ++e) {
-        tc[e] = c;
-        if (LAY == kRotFree)
-            for (int k = 0; k < 3; ++k) B.tcoef[e * NT + k] = B.coef[(int64_t)c * NC + k];
-        if (LAY == kRigid)
-            for (int k = 0; k < 6; ++k) B.tcoef[e * NT + k] = B.coef[(int64_t)c * NC + k];
-        if (LAY == kCone || LAY == kVisc)
-            for (int k = 0; k < 9; ++k) B.tcoef[e * NT + k] = B.coef[(int64_t)c * NC + 3 + k];
-    }
-    for (int m = js[p]; m < je[p]; ++m, ++e) {
-        const int c = jlist[m];
-        tc[e] = ~c;
-        if (LAY == kRotFree)
-            for (int k = 0; k < 3; ++k) B.tcoef[e * NT + k] = -B.coef[(int64_t)c * NC + k];
-        if (LAY == kRigid) {
-            for (int k = 0; k < 3; ++k) B.tcoef[e * NT + k] = -B.coef[(int64_t)c * NC + k];
-            for (int k = 0; k < 3; ++k) B.tcoef[e * NT + 3 + k] = -B.coef[(int64_t)c * NC + 6 + k];
+    const int i0 = is[p], ni = ie[p] - i0, j0 = js[p], nt = ni + (je[p] - j0);
+    const int64_t e0 = tstart[p];
+    for (int k = sub; k < nt; k += G) {
+        const int64_t e = e0 + k;
+        if (k < ni) {
+            const int c = i0 + k;
+            tc[e] = c;
+            if (LAY == kRotFree)
+                for (int q = 0; q < 3; ++q) B.tcoef[e * NT + q] = B.coef[(int64_t)c * NC + q];
+            if (LAY == kRigid)
+                for (int q = 0; q < 6; ++q) B.tcoef[e * NT + q] = B.coef[(int64_t)c * NC + q];
+            if (LAY == kCone || LAY == kVisc)
+                for (int q = 0; q < 9; ++q) B.tcoef[e * NT + q] = B.coef[(int64_t)c * NC + 3 + q];
+        } else {
+            const int c = jlist[j0 + (k - ni)];
+            tc[e] = ~c;
+            if (LAY == kRotFree)
+                for (int q = 0; q < 3; ++q) B.tcoef[e * NT + q] = -B.coef[(int64_t)c * NC + q];
+            if (LAY == kRigid) {
+                for (int q = 0; q < 3; ++q) B.tcoef[e * NT + q] = -B.coef[(int64_t)c * NC + q];
+                for (int q = 0; q < 3; ++q) B.tcoef[e * NT + 3 + q] = -B.coef[(int64_t)c * NC + 6 + q];
+            }
+            if (LAY == kCone || LAY == kVisc)
+                for (int q = 0; q < 9; ++q) B.tcoef[e * NT + q] = -B.coef[(int64_t)c * NC + 12 + q];
         }
-        if (LAY == kCone || LAY == kVisc)
-            for (int k = 0; k < 9; ++k) B.tcoef[e * NT + k] = -B.coef[(int64_t)c * NC + 12 + k];
     }
 }
 
@@ -1695,11 +1705,11 @@ void Solver::assemble(const Particles& P, const Contacts& C, int64_t nc, double 
     hipLaunchKernelGGL(k_tcount, dim3(grid_for(na + 1)), dim3(kBlock), 0, s, istart_, iend_, jstart_, jend_, na,
                        tcount_);
     scanner_.exclusive(tcount_, tstart_, na + 1, nullptr, s);
-    hipLaunchKernelGGL((k_transpose<double, LAY>), dim3(grid_for(na)), dim3(kBlock), 0, s, b64_, na, istart_, iend_,
-                       jstart_, jend_, jlist_, tstart_, tc_);
+    hipLaunchKernelGGL((k_transpose<double, LAY>), dim3(grid_for(na * kTransposeGroup)), dim3(kBlock), 0, s, b64_, na,
+                       istart_, iend_, jstart_, jend_, jlist_, tstart_, tc_);
     if (fp32)
-        hipLaunchKernelGGL((k_transpose<float, LAY>), dim3(grid_for(na)), dim3(kBlock), 0, s, b32_, na, istart_, iend_,
-                           jstart_, jend_, jlist_, tstart_, tc_);
+        hipLaunchKernelGGL((k_transpose<float, LAY>), dim3(grid_for(na * kTransposeGroup)), dim3(kBlock), 0, s, b32_, na,
+                           istart_, iend_, jstart_, jend_, jlist_, tstart_, tc_);
     SCOPI_HIP_CHECK(hipGetLastError());
 }
 
