@@ -1642,7 +1642,8 @@ __global__ void __launch_bounds__(kBlock) k_se_points(Particles P, SeGrids G) {
 // index are those of the full scan, bit for bit.  The penalty test itself is screened first with
 // the dot product n1.n2 in three FMAs against 0.5 g - 1 + 1e-13: it differs from the term's dot
 // product (three products and two sums, rounded each) by less than 1e-15 for unit normals, so a term
-// the screen rejects has 2 (1 + n1.n2) > g as computed; the others take the exact test.
+// the screen rejects has 2 (1 + n1.n2) > g as computed; a second screen does the same for the
+// distance test, and the terms that pass both take the exact test.
 constexpr int kSeGridBlock = 192;  // 3 x 192 = 576
 __global__ void __launch_bounds__(kSeGridBlock) k_se_grid(Particles P, SeGrids G, SeCandidates S, int64_t n) {
     __shared__ double g2[6][kSeGridPoints];
@@ -1693,10 +1694,20 @@ __global__ void __launch_bounds__(kSeGridBlock) k_se_grid(Particles P, SeGrids G
         const double screen = 0.5 * g - 1.0 + 1e-13;  // +inf while g is
 #pragma unroll
         for (int r = 0; r < 3; ++r) {
-            if (fma(m1[r][2], nz, fma(m1[r][1], ny, m1[r][0] * nx)) > screen) continue;
+            const double dotf = fma(m1[r][2], nz, fma(m1[r][1], ny, m1[r][0] * nx));
+            if (dotf > screen) continue;
+            const double dx = p1[r][0] - g2[0][jj], dy = p1[r][1] - g2[1][jj], dz = p1[r][2] - g2[2][jj];
+            {
+                // the distance test in FMAs: penf is within 2e-15 of the exact penalty and d2f within
+                // 5e-16 d2 of the exact squared distance, so a term rejected here with these margins
+                // fails the exact test below too
+                const double penf = fma(2.0, dotf, 2.0);
+                const double uf = (g - penf) + 1e-14 * (g + 1.0);
+                const double d2f = fma(dz, dz, fma(dy, dy, dx * dx));
+                if (uf > 0 && d2f > uf * uf * (1.0 + 1e-13)) continue;
+            }
             const double pen = 2 * (1 + (m1[r][0] * nx + m1[r][1] * ny + m1[r][2] * nz));
             if (ok[r] && pen <= g) {
-                const double dx = p1[r][0] - g2[0][jj], dy = p1[r][1] - g2[1][jj], dz = p1[r][2] - g2[2][jj];
                 const double d2 = dx * dx + dy * dy + dz * dz;
                 const double u = (g - pen) + 1e-15 * g;
                 if (!(u > 0 && d2 > u * u * (1.0 + 1e-14))) {
