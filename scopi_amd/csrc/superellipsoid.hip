@@ -65,7 +65,7 @@ __device__ inline double reduce_pio2(double x, int& quadrant) {
     quadrant = (int)(((int64_t)k) & 3);
     return r;
 }
-__device__ __attribute__((noinline)) double pm_cos(double x) {
+__device__ inline double pm_cos(double x) {
     if (!(fabs(x) < 1e15)) return x - x;
     int q;
     const double r = reduce_pio2(x, q);
@@ -74,7 +74,7 @@ __device__ __attribute__((noinline)) double pm_cos(double x) {
     if (q == 2) return -cos_poly(r);
     return sin_poly(r);
 }
-__device__ __attribute__((noinline)) double pm_sin(double x) {
+__device__ inline double pm_sin(double x) {
     if (!(fabs(x) < 1e15)) return x - x;
     int q;
     const double r = reduce_pio2(x, q);
@@ -106,7 +106,7 @@ __device__ inline double pm_asin(double x) {
         r = kPio2 - 2.0 * asin_small(sqrt((1.0 - a) * 0.5));
     return x < 0 ? -r : r;
 }
-__device__ __attribute__((noinline)) double pm_log(double x) {
+__device__ inline double pm_log(double x) {
     if (x != x || x < 0) return (x - x) / (x - x);
     if (x == 0) return -INFINITY;
     if (x == INFINITY) return x;
@@ -140,7 +140,7 @@ __device__ __attribute__((noinline)) double pm_log(double x) {
     const double dk = (double)k;
     return dk * kLn2Hi + (lm + dk * kLn2Lo);
 }
-__device__ __attribute__((noinline)) double pm_exp(double y) {
+__device__ inline double pm_exp(double y) {
     if (y != y) return y;
     if (y > 709.782712893384) return INFINITY;
     if (y < -745.2) return 0.0;
