@@ -277,6 +277,51 @@ __global__ void __launch_bounds__(kBlock) k_pair_emit(Particles P, const GridInf
     }
 }
 
+// The count pass of the one-scene search also keeps a particle's first kKeepPartners partners (in the
+// order it visits them), so that the emit needs no second search: it copies them and sorts them by j,
+// which gives the same pairs as k_pair_emit (the partners are distinct).  A particle with more
+// partners sets `overflow`, and the search then emits with k_pair_emit.
+constexpr int kKeepPartners = 16;
+template <int DIM, bool KD>
+__global__ void __launch_bounds__(kBlock) k_pair_count_keep(Particles P, const GridInfo* __restrict__ gp,
+                                                           const int* __restrict__ start, const int* __restrict__ end,
+                                                           const int32_t* __restrict__ sorted, double kd_radius,
+                                                           double dmax, int* __restrict__ counts,
+                                                           int32_t* __restrict__ keep, int* __restrict__ overflow) {
+    const int64_t na = P.n - P.n_inactive;
+    const int64_t t = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (t >= na) return;
+    const GridInfo g = *gp;
+    int32_t* kp = keep + t * kKeepPartners;
+    int cnt = 0;
+    for_partners<DIM, KD, false>(P, g, start, end, sorted, P.n_inactive + t, kd_radius, dmax, [&](int64_t j) {
+        if (cnt < kKeepPartners) kp[cnt] = (int32_t)j;
+        ++cnt;
+    });
+    counts[t] = cnt;
+    if (cnt > kKeepPartners) atomicOr(overflow, 1);
+}
+
+__global__ void __launch_bounds__(kBlock) k_pair_emit_kept(int64_t na, const int* __restrict__ counts,
+                                                          const int* __restrict__ offsets,
+                                                          const int32_t* __restrict__ keep, int32_t* __restrict__ pair_j) {
+    const int64_t t = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (t >= na) return;
+    const int64_t cnt = counts[t];
+    const int64_t base = offsets[t];
+    const int32_t* kp = keep + t * kKeepPartners;
+    for (int64_t m = 0; m < cnt; ++m) pair_j[base + m] = kp[m];
+    for (int64_t a = 1; a < cnt; ++a) {  // insertion sort by j, as k_pair_emit
+        const int32_t key = pair_j[base + a];
+        int64_t b = a - 1;
+        while (b >= 0 && pair_j[base + b] > key) {
+            pair_j[base + b + 1] = pair_j[base + b];
+            --b;
+        }
+        pair_j[base + b + 1] = key;
+    }
+}
+
 template <int DIM>
 __global__ void __launch_bounds__(kBlock) k_obstacle_flags(Particles P, double dmax, int* __restrict__ flags) {
     const int64_t na = P.n - P.n_inactive;
@@ -619,8 +664,8 @@ void contacts_reserve(Contacts& C, int64_t n) {
 
 ContactSearch::~ContactSearch() {
     for (void* p : {(void*)d_grid_, (void*)d_partial_, (void*)keys_, (void*)vals_, (void*)counts_, (void*)offsets_,
-                    (void*)obst_flags_, (void*)obst_offsets_, (void*)cell_start_, (void*)cell_end_, (void*)pair_j_,
-                    (void*)d_totals_, (void*)cand_flags_, (void*)cand_offsets_})
+                    (void*)keep_, (void*)obst_flags_, (void*)obst_offsets_, (void*)cell_start_, (void*)cell_end_,
+                    (void*)pair_j_, (void*)d_totals_, (void*)cand_flags_, (void*)cand_offsets_})
         if (p) (void)hipFree(p);
     if (h_totals_) (void)hipHostFree(h_totals_);
     se_grids_free(se_grids_);
@@ -630,20 +675,21 @@ ContactSearch::~ContactSearch() {
 void ContactSearch::reserve_particles(int64_t na, int64_t n_inactive) {
     if (!d_grid_) {
         SCOPI_HIP_CHECK(hipMalloc(&d_grid_, sizeof(GridInfo)));
-        SCOPI_HIP_CHECK(hipMalloc(&d_totals_, 2 * sizeof(int)));
-        SCOPI_HIP_CHECK(hipHostMalloc(&h_totals_, 2 * sizeof(int)));
+        SCOPI_HIP_CHECK(hipMalloc(&d_totals_, 3 * sizeof(int)));  // pairs, obstacle pairs, kept-partner overflow
+        SCOPI_HIP_CHECK(hipHostMalloc(&h_totals_, 3 * sizeof(int)));
         SCOPI_HIP_CHECK(hipMalloc(&cell_start_, kMaxCells * sizeof(int)));
         SCOPI_HIP_CHECK(hipMalloc(&cell_end_, kMaxCells * sizeof(int)));
         cells_cap_ = kMaxCells;
     }
     if (na + 1 > pcap_) {
-        for (void* p : {(void*)keys_, (void*)vals_, (void*)counts_, (void*)offsets_})
+        for (void* p : {(void*)keys_, (void*)vals_, (void*)counts_, (void*)offsets_, (void*)keep_})
             if (p) SCOPI_HIP_CHECK(hipFree(p));
         pcap_ = na + na / 4 + 1024;
         SCOPI_HIP_CHECK(hipMalloc(&keys_, pcap_ * sizeof(uint32_t)));
         SCOPI_HIP_CHECK(hipMalloc(&vals_, pcap_ * sizeof(int32_t)));
         SCOPI_HIP_CHECK(hipMalloc(&counts_, pcap_ * sizeof(int)));
         SCOPI_HIP_CHECK(hipMalloc(&offsets_, pcap_ * sizeof(int)));
+        SCOPI_HIP_CHECK(hipMalloc(&keep_, pcap_ * kKeepPartners * sizeof(int32_t)));
         const int64_t nparts = std::min<int64_t>(grid_for(pcap_), 1024);
         if (nparts * 7 > partial_cap_) {
             if (d_partial_) SCOPI_HIP_CHECK(hipFree(d_partial_));
@@ -701,11 +747,13 @@ int64_t ContactSearch::run(const Particles& P, Contacts& C, const scopi_hip_para
         else                                                                                                   \
             hipLaunchKernelGGL((KERNEL<2, false>), dim3(grid_for(na)), dim3(kBlock), 0, s, __VA_ARGS__);      \
     } while (0)
-        SCOPI_PAIR_LAUNCH(k_pair_count, P, d_grid_, cell_start_, cell_end_, vals_, prm.kd_tree_radius, prm.dmax,
-                          counts_);
+        hipLaunchKernelGGL(k_fill_int, dim3(1), dim3(64), 0, s, d_totals_ + 2, 1, 0);
+        SCOPI_PAIR_LAUNCH(k_pair_count_keep, P, d_grid_, cell_start_, cell_end_, vals_, prm.kd_tree_radius, prm.dmax,
+                          counts_, keep_, d_totals_ + 2);
         scanner_.exclusive(counts_, offsets_, na, d_totals_, s);
     } else {
         hipLaunchKernelGGL(k_fill_int, dim3(1), dim3(64), 0, s, d_totals_, 1, 0);
+        hipLaunchKernelGGL(k_fill_int, dim3(1), dim3(64), 0, s, d_totals_ + 2, 1, 0);
     }
     const int64_t no = na * ni;
     if (no > 0) {
@@ -717,9 +765,10 @@ int64_t ContactSearch::run(const Particles& P, Contacts& C, const scopi_hip_para
     } else {
         hipLaunchKernelGGL(k_fill_int, dim3(1), dim3(64), 0, s, d_totals_ + 1, 1, 0);
     }
-    SCOPI_HIP_CHECK(hipMemcpyAsync(h_totals_, d_totals_, 2 * sizeof(int), hipMemcpyDeviceToHost, s));
+    SCOPI_HIP_CHECK(hipMemcpyAsync(h_totals_, d_totals_, 3 * sizeof(int), hipMemcpyDeviceToHost, s));
     SCOPI_HIP_CHECK(hipStreamSynchronize(s));
     nc_aa = (na > 1) ? h_totals_[0] : 0;
+    const bool kept = h_totals_[2] == 0;  // every particle's partners are in keep_
     nc_obs_ = (no > 0) ? h_totals_[1] : 0;
     const int64_t nc = nc_aa + nc_obs_;
     contacts_reserve(C, nc);
@@ -728,8 +777,12 @@ int64_t ContactSearch::run(const Particles& P, Contacts& C, const scopi_hip_para
                            obst_offsets_, C);
     if (nc_aa > 0) {
         reserve_pairs(nc_aa);
-        SCOPI_PAIR_LAUNCH(k_pair_emit, P, d_grid_, cell_start_, cell_end_, vals_, prm.kd_tree_radius, prm.dmax,
-                          offsets_, pair_j_);
+        if (kept)
+            hipLaunchKernelGGL(k_pair_emit_kept, dim3(grid_for(na)), dim3(kBlock), 0, s, na, counts_, offsets_, keep_,
+                               pair_j_);
+        else
+            SCOPI_PAIR_LAUNCH(k_pair_emit, P, d_grid_, cell_start_, cell_end_, vals_, prm.kd_tree_radius, prm.dmax,
+                              offsets_, pair_j_);
         hipLaunchKernelGGL(k_pair_scatter_i, dim3(grid_for(na)), dim3(kBlock), 0, s, ni, na, nc_obs_, counts_,
                            offsets_, pair_j_, C);
     }

@@ -55,8 +55,9 @@ class ContactSearch {
     int* cell_end_ = nullptr;
     int64_t cells_cap_ = 0;
     int32_t* pair_j_ = nullptr;
+    int32_t* keep_ = nullptr;  // the count pass's first partners of each particle (kKeepPartners each)
     int64_t pair_cap_ = 0;
-    int* d_totals_ = nullptr;   // [0] active pairs, [1] obstacle pairs
+    int* d_totals_ = nullptr;   // [0] active pairs, [1] obstacle pairs, [2] a particle had more partners than it kept
     int* h_totals_ = nullptr;   // pinned
     int64_t nc_obs_ = 0;
     SeGrids se_grids_;

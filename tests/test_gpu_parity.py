@@ -55,6 +55,27 @@ def random_scene(dim, n, seed, box=1.0, rmin=0.02, rmax=0.06):
     return c.arrays()
 
 
+def crowded_scene(dim, n_small=40, R=0.5, r=0.05, gap=0.002):
+    """A large sphere followed by n_small small ones on a shell around it, each `gap` from its surface:
+    the large sphere has n_small partners of higher index within dmax, more than the count pass of the
+    one-scene search keeps per particle, so its pairs come from the second search."""
+    c = Container(dim)
+    centre = np.full(dim, 0.5 + R + r)
+    prop = Property(mass=1.0, moment_inertia=0.1 if dim == 2 else (0.1, 0.1, 0.1))
+    c.sphere(centre, R, prop)
+    k = np.arange(n_small) + 0.5
+    if dim == 3:
+        th = np.arccos(1 - 2 * k / n_small)
+        ph = math.pi * (1 + 5 ** 0.5) * k
+        dirs = np.stack([np.sin(th) * np.cos(ph), np.sin(th) * np.sin(ph), np.cos(th)], axis=1)
+    else:
+        a = 2 * math.pi * k / n_small
+        dirs = np.stack([np.cos(a), np.sin(a)], axis=1)
+    for d in dirs:
+        c.sphere(centre + (R + r + gap) * d, r, prop)
+    return c.arrays()
+
+
 def compare_contacts(g, o, tag):
     assert g["ij"].shape == o["ij"].shape, f"{tag}: {len(g['ij'])} vs {len(o['ij'])} contacts"
     assert np.array_equal(g["ij"], o["ij"]), f"{tag}: pair sets differ"
@@ -70,6 +91,8 @@ CONTACT_CASES = {
     "random_2d_default_radius": lambda: (random_scene(2, 400, 1), dict(dmax=0.05)),
     "random_3d_kd": lambda: (random_scene(3, 2000, 2), dict(dmax=0.03, kd_tree_radius=0.09 ** 2)),
     "random_3d_binding_kd": lambda: (random_scene(3, 2000, 3), dict(dmax=0.05, kd_tree_radius=0.07 ** 2)),
+    "crowded_3d": lambda: (crowded_scene(3), dict(dmax=0.01, kd_tree_radius=0.7 ** 2)),
+    "crowded_2d": lambda: (crowded_scene(2, 24), dict(dmax=0.01, kd_tree_radius=0.7 ** 2)),
 }
 
 
