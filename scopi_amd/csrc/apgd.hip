@@ -1587,7 +1587,17 @@ void Solver::run_phase(BMatrix<T>& B, const Particles& P, const Contacts& C, int
         for (auto& e : ev) SCOPI_HIP_CHECK(hipEventCreate(&e));
     }
     int64_t before = 0;
-    int batch = 2;
+    // Rounds enqueued before each poll of the control state.  Without a prediction: 2, 4, 8, ... up
+    // to 64.  A solve phase takes about as many rounds as the same phase of the last time step, so
+    // with that count the first poll comes after three quarters of it, the next ones after half of
+    // what the prediction leaves (each poll idles the device while the host reads the state and
+    // enqueues the next batch), and past the prediction the batches double again from 4.  Rounds
+    // enqueued after convergence exit at their first instruction: the batches change the number of
+    // polls and of such empty rounds, not the solve.  At most 1024 rounds go between two polls.
+    const int64_t predicted = timing_ ? 0 : last_rounds_[sizeof(T) == 4 ? 1 : 0];
+    const bool predict = predicted > 32;
+    bool past = false;
+    int batch = predict ? (int)std::min<int64_t>(predicted * 3 / 4, 1024) : 2;
     int cur = 0;  // copy of ApgdCtrl holding the newest state
     if (graph_rounds_ > 0 && !timing_) {
         // hipGraph of graph_rounds_ rounds, the control step and the state's copy to the host,
@@ -1667,8 +1677,17 @@ void Solver::run_phase(BMatrix<T>& B, const Particles& P, const Contacts& C, int
         }
         before = rounds;
         if (h_ctrl_->done) break;
-        batch = std::min(batch * 2, 64);
+        const int64_t left = predicted - rounds;
+        if (predict && !past && left >= 8) {
+            batch = (int)std::min<int64_t>(left / 2, 1024);
+        } else if (predict && !past) {
+            past = true;
+            batch = 4;
+        } else {
+            batch = std::min(batch * 2, 64);
+        }
     }
+    if (!(graph_rounds_ > 0 && !timing_)) last_rounds_[sizeof(T) == 4 ? 1 : 0] = h_ctrl_->q_applies;
     ctrl_cur_ = cur;
     for (auto& e : ev) (void)hipEventDestroy(e);
     info.iterations += h_ctrl_->iter;

@@ -252,6 +252,7 @@ class Solver {
     int row_iters_ = 2;         // loop iterations per lane of the per-contact grid below its cap
     int graph_rounds_ = 0;      // > 0: the one-scene loop replays a hipGraph of this many rounds
     hipGraphExec_t graph_exec32_ = nullptr, graph_exec64_ = nullptr;  // kept across solves (updated in place)
+    int64_t last_rounds_[2] = {0, 0};  // rounds of the last one-scene solve phase, FP64 [0] and FP32 [1]
     bool has_solution_ = false;
     // FrictionFixedPoint: c without S, s_ij, block sums of the error
     double* fp_c0_ = nullptr;
