@@ -1633,10 +1633,13 @@ void Solver::run_phase(BMatrix<T>& B, const Particles& P, const Contacts& C, int
         }
         if (!exec) SCOPI_HIP_CHECK(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
         SCOPI_HIP_CHECK(hipGraphDestroy(graph));
+        // With a prediction, the replays that cover three quarters of it go out before the first poll.
+        int64_t replays = predict ? std::max<int64_t>(1, predicted * 3 / 4 / graph_rounds_) : 1;
         for (;;) {
-            SCOPI_HIP_CHECK(hipGraphLaunch(exec, s));
+            for (int64_t r = 0; r < replays; ++r) SCOPI_HIP_CHECK(hipGraphLaunch(exec, s));
             SCOPI_HIP_CHECK(hipStreamSynchronize(s));
             if (h_ctrl_->done) break;
+            replays = 1;
         }
         cur = c;
     }
@@ -1687,7 +1690,7 @@ void Solver::run_phase(BMatrix<T>& B, const Particles& P, const Contacts& C, int
             batch = std::min(batch * 2, 64);
         }
     }
-    if (!(graph_rounds_ > 0 && !timing_)) last_rounds_[sizeof(T) == 4 ? 1 : 0] = h_ctrl_->q_applies;
+    last_rounds_[sizeof(T) == 4 ? 1 : 0] = h_ctrl_->q_applies;
     ctrl_cur_ = cur;
     for (auto& e : ev) (void)hipEventDestroy(e);
     info.iterations += h_ctrl_->iter;
