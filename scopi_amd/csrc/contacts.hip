@@ -307,7 +307,8 @@ __global__ void __launch_bounds__(kBlock) k_pair_emit_kept(int64_t na, const int
                                                           const int32_t* __restrict__ keep, int32_t* __restrict__ pair_j) {
     const int64_t t = (int64_t)blockIdx.x * kBlock + threadIdx.x;
     if (t >= na) return;
-    const int64_t cnt = counts[t];
+    // counts[t] <= kKeepPartners when no particle overflowed, the only case this kernel runs in
+    const int64_t cnt = counts[t] < kKeepPartners ? counts[t] : kKeepPartners;
     const int64_t base = offsets[t];
     const int32_t* kp = keep + t * kKeepPartners;
     for (int64_t m = 0; m < cnt; ++m) pair_j[base + m] = kp[m];
