@@ -197,8 +197,31 @@ __global__ void __launch_bounds__(kBlock) k_cell_range(const uint32_t* __restric
     if (s == na - 1 || keys[s + 1] != k) end[k] = (int)(s + 1);
 }
 
-// Visits every j > i in the neighbouring cells that satisfies both predicates
-// (CAND: the kd-tree predicate only; the closest points come later).
+// Visits every j > i of one cell that satisfies both predicates (CAND: the kd-tree predicate only;
+// the closest points come later).
+template <int DIM, bool KD, bool CAND, class F>
+__device__ inline void visit_cell(const Particles& P, const int* __restrict__ start, const int* __restrict__ end,
+                                  const int32_t* __restrict__ sorted, int64_t i, const d3& ci, uint32_t key,
+                                  double kd_radius, double dmax, F& visit) {
+    const int s1 = end[key];
+    for (int s = start[key]; s < s1; ++s) {
+        const int64_t j = sorted[s];
+        if (j <= i) continue;
+        if (KD) {
+            const d3 cj = load3(P.pos, P.cap, j);
+            if (!(kd_dist2<DIM>(ci, cj) < kd_radius)) continue;
+        }
+        if (CAND) {
+            visit(j);
+            continue;
+        }
+        CP cp;
+        if (!closest_points<DIM>(P, i, j, cp)) continue;
+        if (cp.dij < dmax) visit(j);
+    }
+}
+
+// Visits every j > i in the neighbouring cells that satisfies both predicates.
 template <int DIM, bool KD, bool CAND, class F>
 __device__ inline void for_partners(const Particles& P, const GridInfo& g, const int* __restrict__ start,
                                     const int* __restrict__ end, const int32_t* __restrict__ sorted, int64_t i,
@@ -216,25 +239,29 @@ __device__ inline void for_partners(const Particles& P, const GridInfo& g, const
             for (int dx = -1; dx <= 1; ++dx) {
                 const int x = cx + dx;
                 if (x < 0 || x >= g.nc[0]) continue;
-                const uint32_t key = morton<DIM>(x, y, z);
-                const int s1 = end[key];
-                for (int s = start[key]; s < s1; ++s) {
-                    const int64_t j = sorted[s];
-                    if (j <= i) continue;
-                    if (KD) {
-                        const d3 cj = load3(P.pos, P.cap, j);
-                        if (!(kd_dist2<DIM>(ci, cj) < kd_radius)) continue;
-                    }
-                    if (CAND) {
-                        visit(j);
-                        continue;
-                    }
-                    CP cp;
-                    if (!closest_points<DIM>(P, i, j, cp)) continue;
-                    if (cp.dij < dmax) visit(j);
-                }
+                visit_cell<DIM, KD, CAND>(P, start, end, sorted, i, ci, morton<DIM>(x, y, z), kd_radius, dmax, visit);
             }
         }
+    }
+}
+
+// The part `part` (0..8) of for_partners' neighbourhood: in 3D the column of 3 cells along x at
+// (dz, dy) = (part / 3 - 1, part % 3 - 1), in 2D the cell (dy, dx) = (part / 3 - 1, part % 3 - 1).
+template <int DIM, bool KD, class F>
+__device__ inline void for_partners_part(const Particles& P, const GridInfo& g, const int* __restrict__ start,
+                                         const int* __restrict__ end, const int32_t* __restrict__ sorted, int64_t i,
+                                         int part, double kd_radius, double dmax, F&& visit) {
+    const d3 ci = load3(P.pos, P.cap, i);
+    const int cx = cell_coord<DIM>(g, ci.x, 0), cy = cell_coord<DIM>(g, ci.y, 1);
+    const int cz = (DIM == 3) ? cell_coord<DIM>(g, ci.z, 2) : 0;
+    const int a = part / 3 - 1, b = part % 3 - 1;
+    const int z = (DIM == 3) ? cz + a : 0;
+    const int y = (DIM == 3) ? cy + b : cy + a;
+    if (z < 0 || z >= g.nc[2] || y < 0 || y >= g.nc[1]) return;
+    const int x0 = (DIM == 3) ? cx - 1 : cx + b, x1 = (DIM == 3) ? cx + 1 : cx + b;
+    for (int x = x0; x <= x1; ++x) {
+        if (x < 0 || x >= g.nc[0]) continue;
+        visit_cell<DIM, KD, false>(P, start, end, sorted, i, ci, morton<DIM>(x, y, z), kd_radius, dmax, visit);
     }
 }
 
@@ -277,29 +304,42 @@ __global__ void __launch_bounds__(kBlock) k_pair_emit(Particles P, const GridInf
     }
 }
 
-// The count pass of the one-scene search also keeps a particle's first kKeepPartners partners (in the
-// order it visits them), so that the emit needs no second search: it copies them and sorts them by j,
-// which gives the same pairs as k_pair_emit (the partners are distinct).  A particle with more
-// partners sets `overflow`, and the search then emits with k_pair_emit.
+// The count pass of the one-scene search also keeps up to kKeepPartners partners of each particle, so
+// that the emit needs no second search: it copies them and sorts them by j, which gives the same pairs
+// as k_pair_emit (the partners are distinct, so their order before the sort does not matter).  A
+// particle with more partners sets `overflow`, and the search then emits with k_pair_emit.
+// kCountGroup lanes share a particle, one part of its neighbourhood each (for_partners_part), and
+// take the slots of its row from a counter in LDS.
 constexpr int kKeepPartners = 16;
+constexpr int kCountGroup = 9;
+constexpr int kCountPerBlock = kBlock / kCountGroup;  // particles per block; the last lanes idle
 template <int DIM, bool KD>
 __global__ void __launch_bounds__(kBlock) k_pair_count_keep(Particles P, const GridInfo* __restrict__ gp,
                                                            const int* __restrict__ start, const int* __restrict__ end,
                                                            const int32_t* __restrict__ sorted, double kd_radius,
                                                            double dmax, int* __restrict__ counts,
                                                            int32_t* __restrict__ keep, int* __restrict__ overflow) {
+    __shared__ int found[kCountPerBlock];
     const int64_t na = P.n - P.n_inactive;
-    const int64_t t = (int64_t)blockIdx.x * kBlock + threadIdx.x;
-    if (t >= na) return;
-    const GridInfo g = *gp;
-    int32_t* kp = keep + t * kKeepPartners;
-    int cnt = 0;
-    for_partners<DIM, KD, false>(P, g, start, end, sorted, P.n_inactive + t, kd_radius, dmax, [&](int64_t j) {
-        if (cnt < kKeepPartners) kp[cnt] = (int32_t)j;
-        ++cnt;
-    });
-    counts[t] = cnt;
-    if (cnt > kKeepPartners) atomicOr(overflow, 1);
+    const int grp = threadIdx.x / kCountGroup, part = threadIdx.x % kCountGroup;
+    const int64_t t = (int64_t)blockIdx.x * kCountPerBlock + grp;
+    const bool active = grp < kCountPerBlock && t < na;
+    if (threadIdx.x < kCountPerBlock) found[threadIdx.x] = 0;
+    __syncthreads();
+    if (active) {
+        const GridInfo g = *gp;
+        int32_t* kp = keep + t * kKeepPartners;
+        for_partners_part<DIM, KD>(P, g, start, end, sorted, P.n_inactive + t, part, kd_radius, dmax, [&](int64_t j) {
+            const int slot = atomicAdd(&found[grp], 1);
+            if (slot < kKeepPartners) kp[slot] = (int32_t)j;
+        });
+    }
+    __syncthreads();
+    if (active && part == 0) {
+        const int cnt = found[grp];
+        counts[t] = cnt;
+        if (cnt > kKeepPartners) atomicOr(overflow, 1);
+    }
 }
 
 __global__ void __launch_bounds__(kBlock) k_pair_emit_kept(int64_t na, const int* __restrict__ counts,
@@ -749,8 +789,21 @@ int64_t ContactSearch::run(const Particles& P, Contacts& C, const scopi_hip_para
             hipLaunchKernelGGL((KERNEL<2, false>), dim3(grid_for(na)), dim3(kBlock), 0, s, __VA_ARGS__);      \
     } while (0)
         hipLaunchKernelGGL(k_fill_int, dim3(1), dim3(64), 0, s, d_totals_ + 2, 1, 0);
-        SCOPI_PAIR_LAUNCH(k_pair_count_keep, P, d_grid_, cell_start_, cell_end_, vals_, prm.kd_tree_radius, prm.dmax,
-                          counts_, keep_, d_totals_ + 2);
+        {
+            const dim3 gk(grid_for(na, kCountPerBlock));
+            if (P.dim == 3 && kd)
+                hipLaunchKernelGGL((k_pair_count_keep<3, true>), gk, dim3(kBlock), 0, s, P, d_grid_, cell_start_,
+                                   cell_end_, vals_, prm.kd_tree_radius, prm.dmax, counts_, keep_, d_totals_ + 2);
+            else if (P.dim == 3)
+                hipLaunchKernelGGL((k_pair_count_keep<3, false>), gk, dim3(kBlock), 0, s, P, d_grid_, cell_start_,
+                                   cell_end_, vals_, prm.kd_tree_radius, prm.dmax, counts_, keep_, d_totals_ + 2);
+            else if (kd)
+                hipLaunchKernelGGL((k_pair_count_keep<2, true>), gk, dim3(kBlock), 0, s, P, d_grid_, cell_start_,
+                                   cell_end_, vals_, prm.kd_tree_radius, prm.dmax, counts_, keep_, d_totals_ + 2);
+            else
+                hipLaunchKernelGGL((k_pair_count_keep<2, false>), gk, dim3(kBlock), 0, s, P, d_grid_, cell_start_,
+                                   cell_end_, vals_, prm.kd_tree_radius, prm.dmax, counts_, keep_, d_totals_ + 2);
+        }
         scanner_.exclusive(counts_, offsets_, na, d_totals_, s);
     } else {
         hipLaunchKernelGGL(k_fill_int, dim3(1), dim3(64), 0, s, d_totals_, 1, 0);
